@@ -1,0 +1,379 @@
+/*
+ * kmeans_oracle.c -- CPU restatement of the Lloyd K-means hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ * The product library (kmeans_amd/) never links, loads or calls it.
+ *
+ * What it restates (reference paths relative to the reference repo):
+ *   orc_col_norms    main/cuda/Normalize.cu:33-47 (_dev_colNormalize), with
+ *                    the square-accumulate written as an explicit fma chain.
+ *   orc_assign       main/cuda/ClosestCenters.cu:55-69 (score starts at
+ *                    -0.5f*||c||^2), multiply.hcu:27-53 + dotproducts.hcu:11-25
+ *                    + fma.hcu:13 (fmaf over d in ascending order),
+ *                    vectorMin.hcu:19-38 + reduction.hcu:60-65 +
+ *                    ClosestCenters.cu:111-121 (lowest index among maxima).
+ *   orc_update       main/cuda/ClusterCenters.cu:3-87 (count and mean of the
+ *                    members of each centre) with the summation order fixed
+ *                    to the blocked canonical order below and carried in FP64.
+ *   orc_compactness  main/cuda/Compactness.cu:3-59 and main/cpp/kmeans.cpp:
+ *                    1045-1104 (sum of squared distances to the NEW centres).
+ *   orc_lloyd        main/cpp/kmeans.cpp:1186-1247 (loop, relErr criterion).
+ *
+ * Canonical summation (the contract the GPU path reproduces bit for bit):
+ *   CSUM(v[0..n)) = left-to-right FP64 sum from +0.0 if n <= KM_CHUNK,
+ *                   else CSUM of the left-to-right sums of consecutive
+ *                   KM_CHUNK-sized pieces (the last piece may be short).
+ *   Points are cut into blocks of KM_BLOCK consecutive indices.
+ *   Centre sum   S[j][d] = (((0 + B_0[j][d]) + B_1[j][d]) + ...) where
+ *                B_b[j][d] = CSUM(x[i][d] : i in block b, label i == j, i asc).
+ *   Compactness  comp = (((0 + E_0) + E_1) + ...), E_b = CSUM(e_i : i in b),
+ *                e_i = fma chain over d of v*v with v = x[i][d] - c'[l_i][d].
+ *   New centre   c'[j][d] = (float)(S[j][d] / (double)n_j) if n_j > 0, else
+ *                the previous centre (the reference divides 0/0 -> NaN,
+ *                ClusterCenters.cu:81; this build keeps the old centre).
+ *
+ * Build: see oracle/Makefile (-O3 -march=x86-64-v3 -ffp-contract=off -fopenmp).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <omp.h>
+
+#define KM_CHUNK 256
+#define KM_BLOCK (1L << 23)
+
+/* ------------------------------------------------------------------ */
+/* Counter-based random numbers (SplitMix64 finaliser, Steele et al.,
+ * "Fast splittable pseudorandom number generators", OOPSLA 2014).
+ * u(seed, stream, i) is a pure function, so any element of any config
+ * can be generated independently on the CPU or the GPU.               */
+static inline uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+static inline uint64_t rng_u64(uint64_t seed, uint64_t stream, uint64_t i) {
+  uint64_t k = mix64(seed * 0x9E3779B97F4A7C15ULL + stream * 0xD1B54A32D192ED03ULL + 0x632BE59BD9B4E019ULL);
+  return mix64(k + i * 0x9E3779B97F4A7C15ULL);
+}
+/* uniform in (0,1): 53-bit mantissa, never 0 */
+static inline double rng_unif(uint64_t seed, uint64_t stream, uint64_t i) {
+  return ((double)(rng_u64(seed, stream, i) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+/* standard normal by Box-Muller (cosine branch) */
+static inline double rng_normal(uint64_t seed, uint64_t stream, uint64_t i) {
+  double u1 = rng_unif(seed, stream, 2 * i), u2 = rng_unif(seed, stream, 2 * i + 1);
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+uint64_t orc_rng_u64(uint64_t seed, uint64_t stream, uint64_t i) { return rng_u64(seed, stream, i); }
+double orc_rng_unif(uint64_t seed, uint64_t stream, uint64_t i) { return rng_unif(seed, stream, i); }
+
+/* Gaussian mixture: means U[mlo,mhi]^d, per-component sigma U[slo,shi],
+ * component weights equal (zipf_s == 0) or Zipf(zipf_s) over ranks 1..k.
+ * Streams: 0 means, 1 sigmas, 2 component choice, 3 noise.           */
+void orc_gen_means(uint64_t seed, int d, int k, float mlo, float mhi, float slo, float shi,
+                   float *means, float *sigmas) {
+  for (long j = 0; j < (long)k * d; ++j)
+    means[j] = (float)(mlo + (mhi - mlo) * rng_unif(seed, 0, (uint64_t)j));
+  for (int j = 0; j < k; ++j)
+    sigmas[j] = (float)(slo + (shi - slo) * rng_unif(seed, 1, (uint64_t)j));
+}
+
+/* cumulative weights for component choice (k entries, last == 1) */
+void orc_gen_cdf(int k, double zipf_s, double *cdf) {
+  double tot = 0.0;
+  for (int j = 0; j < k; ++j) tot += (zipf_s > 0.0) ? pow((double)(j + 1), -zipf_s) : 1.0;
+  double acc = 0.0;
+  for (int j = 0; j < k; ++j) {
+    acc += ((zipf_s > 0.0) ? pow((double)(j + 1), -zipf_s) : 1.0) / tot;
+    cdf[j] = acc;
+  }
+  cdf[k - 1] = 1.0;
+}
+
+static inline int pick_component(const double *cdf, int k, double u) {
+  int lo = 0, hi = k - 1;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (u <= cdf[mid]) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
+
+/* rows [i0, i0+n) of the mixture into X (n x d row-major) */
+void orc_gen_points(uint64_t seed, long i0, long n, int d, int k, const float *means,
+                    const float *sigmas, const double *cdf, float *X, int *z) {
+#pragma omp parallel for schedule(static)
+  for (long r = 0; r < n; ++r) {
+    long i = i0 + r;
+    int c = pick_component(cdf, k, rng_unif(seed, 2, (uint64_t)i));
+    if (z) z[r] = c;
+    for (int q = 0; q < d; ++q) {
+      double g = rng_normal(seed, 3, (uint64_t)i * (uint64_t)d + (uint64_t)q);
+      X[r * d + q] = (float)((double)means[(long)c * d + q] + (double)sigmas[c] * g);
+    }
+  }
+}
+
+/* k distinct indices in [0,n) by Floyd's algorithm, returned in draw order */
+int orc_sample_init(uint64_t seed, long n, int k, int *idx) {
+  if (k > n) return -1;
+  long t = 0;
+  for (long j = n - k; j < n; ++j) {
+    long r = (long)(rng_u64(seed, 4, (uint64_t)t) % (uint64_t)(j + 1));
+    int dup = 0;
+    for (long q = 0; q < t; ++q) if (idx[q] == r) { dup = 1; break; }
+    idx[t++] = dup ? (int)j : (int)r;
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Centre norms: nc_j = fma chain over d of c_jd*c_jd (Normalize.cu:41-44). */
+void orc_col_norms(const float *C, int k, int d, float *nc) {
+  for (int j = 0; j < k; ++j) {
+    float s = 0.0f;
+    for (int q = 0; q < d; ++q) s = fmaf(C[(long)j * d + q], C[(long)j * d + q], s);
+    nc[j] = s;
+  }
+}
+
+/* Score and label.  s_ij = fma chain over d from -0.5f*nc_j; label = lowest
+ * j among the maxima (strict '>' while scanning j ascending).  best/second
+ * (optional) receive the largest and second-largest score over j.
+ * SIMD runs across centres; each s_ij is still one sequential chain.  */
+#define PB 8
+#define JB 256
+void orc_assign(const float *X, long n, int d, const float *C, int k, const float *nc,
+                int *labels, float *best_out, float *second_out) {
+  float *Ct = (float *)malloc(sizeof(float) * (size_t)d * (size_t)k);
+  for (int j = 0; j < k; ++j)
+    for (int q = 0; q < d; ++q) Ct[(long)q * k + j] = C[(long)j * d + q];
+  long nb = (n + PB - 1) / PB;
+#pragma omp parallel
+  {
+    float s[PB][JB];
+    float best[PB], second[PB];
+    int lab[PB];
+#pragma omp for schedule(dynamic, 16)
+    for (long b = 0; b < nb; ++b) {
+      long i0 = b * PB;
+      int np = (int)((n - i0) < PB ? (n - i0) : PB);
+      for (int p = 0; p < PB; ++p) { best[p] = -INFINITY; second[p] = -INFINITY; lab[p] = 0; }
+      for (int j0 = 0; j0 < k; j0 += JB) {
+        int nj = (k - j0) < JB ? (k - j0) : JB;
+        for (int p = 0; p < np; ++p)
+          for (int j = 0; j < nj; ++j) s[p][j] = -0.5f * nc[j0 + j];
+        for (int q = 0; q < d; ++q) {
+          const float *crow = Ct + (long)q * k + j0;
+          for (int p = 0; p < np; ++p) {
+            float xv = X[(i0 + p) * d + q];
+            float *sp = s[p];
+#pragma omp simd
+            for (int j = 0; j < nj; ++j) sp[j] = fmaf(xv, crow[j], sp[j]);
+          }
+        }
+        for (int p = 0; p < np; ++p)
+          for (int j = 0; j < nj; ++j) {
+            float v = s[p][j];
+            if (v > best[p]) { second[p] = best[p]; best[p] = v; lab[p] = j0 + j; }
+            else if (v > second[p]) second[p] = v;
+          }
+      }
+      for (int p = 0; p < np; ++p) {
+        labels[i0 + p] = lab[p];
+        if (best_out) best_out[i0 + p] = best[p];
+        if (second_out) second_out[i0 + p] = second[p];
+      }
+    }
+  }
+  free(Ct);
+}
+
+/* Score of one pair, same chain as orc_assign (for tests of single pairs). */
+float orc_score(const float *x, const float *c, int d, float ncj) {
+  float s = -0.5f * ncj;
+  for (int q = 0; q < d; ++q) s = fmaf(x[q], c[q], s);
+  return s;
+}
+
+/* ------------------------------------------------------------------ */
+/* Canonical FP64 sum (see header). */
+static double seq_sum(const double *v, long n) {
+  double s = 0.0;
+  for (long i = 0; i < n; ++i) s += v[i];
+  return s;
+}
+double orc_csum(const double *v, long n) {
+  if (n <= KM_CHUNK) return seq_sum(v, n);
+  long m = (n + KM_CHUNK - 1) / KM_CHUNK;
+  double *p = (double *)malloc(sizeof(double) * (size_t)m);
+  for (long c = 0; c < m; ++c) {
+    long a = c * KM_CHUNK, e = a + KM_CHUNK < n ? a + KM_CHUNK : n;
+    p[c] = seq_sum(v + a, e - a);
+  }
+  double r = orc_csum(p, m);
+  free(p);
+  return r;
+}
+
+/* Per-block canonical centre sums.  For block b and each centre j, collect
+ * the members in ascending i (stable counting sort by label), then CSUM each
+ * dimension.  Adds B_b[j][d] into S (FP64, K x d) and n_b into counts. */
+static void block_update(const float *X, long i0, long n, int d, const int *labels, int k,
+                         double *S, long long *counts) {
+  long *cnt = (long *)calloc((size_t)k + 1, sizeof(long));
+  for (long r = 0; r < n; ++r) cnt[labels[i0 + r] + 1]++;
+  for (int j = 0; j < k; ++j) cnt[j + 1] += cnt[j];
+  long *off = (long *)malloc(sizeof(long) * ((size_t)k + 1));
+  memcpy(off, cnt, sizeof(long) * ((size_t)k + 1));
+  long *perm = (long *)malloc(sizeof(long) * (size_t)(n > 0 ? n : 1));
+  for (long r = 0; r < n; ++r) perm[off[labels[i0 + r]]++] = i0 + r;
+#pragma omp parallel
+  {
+    double *v = NULL;
+    long vcap = 0;
+#pragma omp for schedule(dynamic, 1)
+    for (int j = 0; j < k; ++j) {
+      long a = cnt[j], m = cnt[j + 1] - cnt[j];
+      counts[j] += m;
+      if (m > vcap) { free(v); vcap = m; v = (double *)malloc(sizeof(double) * (size_t)m); }
+      for (int q = 0; q < d; ++q) {
+        for (long t = 0; t < m; ++t) v[t] = (double)X[perm[a + t] * d + q];
+        double bsum = orc_csum(v, m);
+        S[(long)j * d + q] += bsum;
+      }
+    }
+    free(v);
+  }
+  free(perm); free(off); free(cnt);
+}
+
+/* Update: new centres from labels (canonical order).  C_old/C_new are K x d
+ * row-major; C_new may alias C_old.  counts (K) and S (K x d, FP64) optional. */
+int orc_update(const float *X, long n, int d, const int *labels, int k, const float *C_old,
+               float *C_new, long long *counts_out, double *S_out) {
+  double *S = (double *)calloc((size_t)k * d, sizeof(double));
+  long long *counts = (long long *)calloc((size_t)k, sizeof(long long));
+  for (long i0 = 0; i0 < n; i0 += KM_BLOCK) {
+    long nb = (n - i0) < KM_BLOCK ? (n - i0) : KM_BLOCK;
+    block_update(X, i0, nb, d, labels, k, S, counts);
+  }
+  int empty = 0;
+  for (int j = 0; j < k; ++j) {
+    if (counts[j] > 0) {
+      for (int q = 0; q < d; ++q)
+        C_new[(long)j * d + q] = (float)(S[(long)j * d + q] / (double)counts[j]);
+    } else {
+      ++empty;
+      if (C_new != C_old)
+        for (int q = 0; q < d; ++q) C_new[(long)j * d + q] = C_old[(long)j * d + q];
+    }
+  }
+  if (counts_out) memcpy(counts_out, counts, sizeof(long long) * (size_t)k);
+  if (S_out) memcpy(S_out, S, sizeof(double) * (size_t)k * d);
+  free(S); free(counts);
+  return empty;
+}
+
+/* Per-point compactness term e_i (FP32 fma chain of squared differences). */
+static inline float point_err(const float *x, const float *c, int d) {
+  float e = 0.0f;
+  for (int q = 0; q < d; ++q) {
+    float v = x[q] - c[q];
+    e = fmaf(v, v, e);
+  }
+  return e;
+}
+
+double orc_compactness(const float *X, long n, int d, const int *labels, const float *C, int k,
+                       float *e_out) {
+  (void)k;
+  double total = 0.0;
+  double *e = (double *)malloc(sizeof(double) * (size_t)(n < KM_BLOCK ? (n > 0 ? n : 1) : KM_BLOCK));
+  for (long i0 = 0; i0 < n; i0 += KM_BLOCK) {
+    long nb = (n - i0) < KM_BLOCK ? (n - i0) : KM_BLOCK;
+#pragma omp parallel for schedule(static)
+    for (long r = 0; r < nb; ++r) {
+      long i = i0 + r;
+      float ei = point_err(X + i * d, C + (long)labels[i] * d, d);
+      e[r] = (double)ei;
+      if (e_out) e_out[i] = ei;
+    }
+    total += orc_csum(e, nb);
+  }
+  free(e);
+  return total;
+}
+
+/* ------------------------------------------------------------------ */
+/* Full Lloyd loop (kmeans.cpp:1186-1247): while (it < maxIters && relErr >
+ * criterion): assign, update, compactness against the new centres,
+ * relErr = |comp - prev| / |comp| with prev = 0 before the first iteration.
+ * C (K x d) holds C0 on entry and the final centres on exit.
+ * Returns 0 on convergence (relErr <= criterion), 13 otherwise.        */
+int orc_lloyd(const float *X, long n, int d, int k, float *C, int maxIters, double criterion,
+              int *labels, double *comp_hist, int *iters_out, double *relerr_out) {
+  float *nc = (float *)malloc(sizeof(float) * (size_t)k);
+  double prev = 0.0, relErr = 1.0;
+  int it = 0;
+  while (it < maxIters && relErr > criterion) {
+    orc_col_norms(C, k, d, nc);
+    orc_assign(X, n, d, C, k, nc, labels, NULL, NULL);
+    orc_update(X, n, d, labels, k, C, C, NULL, NULL);
+    double comp = orc_compactness(X, n, d, labels, C, k, NULL);
+    relErr = fabs(comp - prev) / fabs(comp);
+    prev = comp;
+    if (comp_hist) comp_hist[it] = comp;
+    ++it;
+  }
+  free(nc);
+  if (iters_out) *iters_out = it;
+  if (relerr_out) *relerr_out = relErr;
+  return relErr <= criterion ? 0 : 13;
+}
+
+/* ------------------------------------------------------------------ */
+/* FP64 oracle: exact-as-possible reference values for error reports.
+ * Means by long-double sums, compactness in FP64 against given centres. */
+void orc_means_fp64(const float *X, long n, int d, const int *labels, int k, double *M,
+                    long long *counts) {
+  long double *acc = (long double *)calloc((size_t)k * d, sizeof(long double));
+  memset(counts, 0, sizeof(long long) * (size_t)k);
+  for (long i = 0; i < n; ++i) {
+    int j = labels[i];
+    counts[j]++;
+    for (int q = 0; q < d; ++q) acc[(long)j * d + q] += (long double)X[i * d + q];
+  }
+  for (int j = 0; j < k; ++j)
+    for (int q = 0; q < d; ++q)
+      M[(long)j * d + q] = counts[j] ? (double)(acc[(long)j * d + q] / (long double)counts[j]) : 0.0;
+  free(acc);
+}
+
+double orc_compactness_fp64(const float *X, long n, int d, const int *labels, const float *C) {
+  long double tot = 0.0L;
+#pragma omp parallel
+  {
+    long double loc = 0.0L;
+#pragma omp for schedule(static)
+    for (long i = 0; i < n; ++i) {
+      const float *c = C + (long)labels[i] * d;
+      double e = 0.0;
+      for (int q = 0; q < d; ++q) {
+        double v = (double)X[i * d + q] - (double)c[q];
+        e += v * v;
+      }
+      loc += e;
+    }
+#pragma omp critical
+    tot += loc;
+  }
+  return (double)tot;
+}
+
+int orc_num_threads(void) { return omp_get_max_threads(); }
+void orc_set_threads(int t) { omp_set_num_threads(t); }

@@ -1,0 +1,267 @@
+"""Pins for the CPU oracle (no GPU).  The reference ships no golden vectors
+or known-answer tests for this path, so each oracle function is pinned here by
+exact rational arithmetic, brute force in FP64, closed forms, hand-computed
+known answers or an independent library (scikit-learn).  Each check is chosen
+so that a plausible slip (wrong sign on the norm term, wrong tie rule, a
+dropped chunk level, old instead of new centres) fails it."""
+from fractions import Fraction
+import math
+
+import numpy as np
+import pytest
+
+
+# ------------------------------------------------------------------ helpers
+def rnd32(q):
+    """Round an exact rational to the nearest float32 (ties to even)."""
+    q = Fraction(q)
+    if q == 0:
+        return 0.0
+    s = -1 if q < 0 else 1
+    a = abs(q)
+    e = a.numerator.bit_length() - a.denominator.bit_length()
+    if Fraction(2) ** e > a:
+        e -= 1
+    m = a / Fraction(2) ** (e - 23)  # in [2^23, 2^24)
+    r = round(m)  # Fraction.__round__ rounds half to even
+    return float(s * r * Fraction(2) ** (e - 23))
+
+
+def exact_chain(x, c, ncj):
+    s = rnd32(Fraction(-0.5) * Fraction(float(np.float32(ncj))))
+    for a, b in zip(x, c):
+        s = rnd32(Fraction(float(a)) * Fraction(float(b)) + Fraction(s))
+    return s
+
+
+def exact_norm(c):
+    s = 0.0
+    for a in c:
+        s = rnd32(Fraction(float(a)) ** 2 + Fraction(s))
+    return s
+
+
+# ------------------------------------------------------------------ scores
+def test_norm_and_score_are_single_rounding_fma_chains(orc):
+    rng = np.random.default_rng(7)
+    for d in (1, 3, 17, 40):
+        for _ in range(20):
+            x = rng.standard_normal(d).astype(np.float32) * 3
+            c = rng.standard_normal(d).astype(np.float32) * 3
+            nc = orc.col_norms(c[None, :])[0]
+            assert float(nc) == exact_norm(c)
+            assert float(orc.score(x, c, nc)) == exact_chain(x, c, nc)
+
+
+def test_fma_chain_differs_from_mul_add(orc):
+    # a case where rounding the product first changes the result: the chain
+    # must keep the product unrounded (fma), as fma.hcu:13 does.  With
+    # ncj = 2 the chain starts at -1 and adds (1+2^-12)^2 = 1 + 2^-11 + 2^-24.
+    x = np.array([1.0 + 2.0 ** -12], np.float32)
+    s = orc.score(x, x, 2.0)
+    assert s == exact_chain(x, x, 2.0) == 2.0 ** -11 + 2.0 ** -24
+    twostep = float(np.float32(x[0] * x[0])) - 1.0
+    assert twostep == 2.0 ** -11 and s != twostep
+
+
+def test_assign_matches_fp64_bruteforce(orc):
+    X, z, means, _ = orc.mixture(11, 3000, 24, 40, -10, 10, 0.5, 0.5)
+    C = means + 0.01
+    lab, best, second = orc.assign(X, C, with_scores=True)
+    D2 = ((X.astype(np.float64)[:, None, :] - C.astype(np.float64)[None]) ** 2).sum(-1)
+    ref = D2.argmin(1)
+    srt = np.sort(D2, 1)
+    clear = (srt[:, 1] - srt[:, 0]) > 1e-2  # far from any FP32 near-tie
+    assert clear.mean() > 0.99
+    assert np.array_equal(lab[clear], ref[clear])
+    assert np.all(best >= second)
+
+
+def test_assign_uses_norm_term_with_correct_sign(orc):
+    # x.c alone would pick the long centre; the distance picks the short one.
+    X = np.array([[1.0, 0.0]], np.float32)
+    C = np.array([[10.0, 0.0], [1.0, 0.1]], np.float32)
+    assert orc.assign(X, C)[0] == 1
+
+
+def test_assign_tie_goes_to_lowest_index(orc):
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((500, 8)).astype(np.float32)
+    base = rng.standard_normal((6, 8)).astype(np.float32)
+    C = np.concatenate([base, base[[2, 0, 5]]])  # centres 6,7,8 duplicate 2,0,5
+    lab = orc.assign(X, C)
+    assert lab.max() < 6
+    lab2 = orc.assign(X, C[::-1].copy())  # duplicates now come first at 0,1,2
+    inv = {8 - j: j for j in range(9)}
+    expect = np.array([min(inv[l], {2: 8 - 6, 0: 8 - 7, 5: 8 - 8}.get(l, 99)) for l in lab])
+    assert np.array_equal(lab2, expect)
+
+
+def test_assign_k_not_multiple_of_block(orc):
+    X, _, means, _ = orc.mixture(5, 700, 9, 300, -20, 20, 0.3, 0.3)
+    lab = orc.assign(X, means)
+    D2 = ((X.astype(np.float64)[:, None, :] - means.astype(np.float64)[None]) ** 2).sum(-1)
+    assert (lab == D2.argmin(1)).mean() > 0.999
+
+
+# ------------------------------------------------------------------ canonical sum
+def test_csum_short_is_left_to_right(orc):
+    rng = np.random.default_rng(1)
+    for n in (0, 1, 2, 255, 256):
+        v = rng.standard_normal(n) * 10.0 ** rng.integers(-8, 8, n)
+        s = 0.0
+        for t in v:
+            s += float(t)
+        assert orc.csum(v) == s
+
+
+def test_csum_known_answer_pins_chunking(orc):
+    # 2^53 followed by ones: a left-to-right sum loses every +1 (ties to even),
+    # fsum keeps all, CSUM keeps exactly the ones outside the first chunk.
+    v = np.ones(512)
+    v[0] = 2.0 ** 53
+    assert orc.csum(v) == 2.0 ** 53 + 256
+    assert math.fsum(v) == 2.0 ** 53 + 511
+    v = np.ones(256 * 256 + 256)
+    v[0] = 2.0 ** 53
+    # level 1: chunk0 = 2^53, 256 other chunks of 256; level 2: group0 =
+    # 2^53 + 255*256, group1 = 256; total 2^53 + 65536.
+    assert orc.csum(v) == 2.0 ** 53 + 65536
+
+
+def test_csum_accuracy(orc):
+    rng = np.random.default_rng(2)
+    v = rng.standard_normal(100_000)
+    assert abs(orc.csum(v) - math.fsum(v)) < 1e-11 * np.abs(v).sum()
+
+
+# ------------------------------------------------------------------ update
+def test_update_matches_fp64_means(orc):
+    X, z, means, _ = orc.mixture(4, 20_000, 16, 32, -5, 5, 1.0, 1.0)
+    C0 = X[:32].copy()
+    lab = orc.assign(X, C0)
+    C1, cnt, S, empty = orc.update(X, lab, C0)
+    M, cnt64 = orc.means_fp64(X, lab, 32)
+    assert np.array_equal(cnt, np.bincount(lab, minlength=32))
+    assert np.array_equal(cnt, cnt64)
+    ok = cnt > 0
+    assert np.allclose(C1[ok], M[ok], rtol=1e-6, atol=1e-6)
+    assert np.array_equal(C1[ok], M[ok].astype(np.float32)) or np.abs(C1[ok] - M[ok]).max() < 1e-5
+    assert np.array_equal(C1[~ok], C0[~ok])
+    assert empty == (~ok).sum()
+
+
+def test_update_empty_cluster_keeps_previous_centre(orc):
+    X = np.array([[0.0, 0.0], [0.1, 0.0], [5.0, 5.0]], np.float32)
+    C0 = np.array([[0.0, 0.0], [5.0, 5.0], [100.0, 100.0]], np.float32)
+    lab = orc.assign(X, C0)
+    assert list(lab) == [0, 0, 1]
+    C1, cnt, S, empty = orc.update(X, lab, C0)
+    assert empty == 1 and list(cnt) == [2, 1, 0]
+    assert np.array_equal(C1[2], C0[2])
+    assert C1[0, 0] == np.float32(0.05) and C1[1, 0] == 5.0
+
+
+def test_update_block_structure_known_answer(orc):
+    # one centre, d=1: x0 = 2^53 then ones across two KM_BLOCK blocks.
+    n = orc.KM_BLOCK + 1000
+    X = np.ones((n, 1), np.float32)
+    X[0, 0] = 2.0 ** 53
+    lab = np.zeros(n, np.int32)
+    C1, cnt, S, _ = orc.update(X, lab, np.zeros((1, 1), np.float32))
+    # block 0: chunk0 loses its 255 ones, every other one is kept; block 1 exact
+    assert S[0, 0] == 2.0 ** 53 + (orc.KM_BLOCK - 256) + 1000
+    assert cnt[0] == n
+
+
+# ------------------------------------------------------------------ compactness
+def test_compactness_fp64_and_identity(orc):
+    X, z, means, _ = orc.mixture(9, 30_000, 20, 16, -6, 6, 0.8, 1.2)
+    C0 = X[:16].copy()
+    lab = orc.assign(X, C0)
+    C1, cnt, S, _ = orc.update(X, lab, C0)
+    comp, e = orc.compactness(X, lab, C1, with_terms=True)
+    ref = orc.compactness_fp64(X, lab, C1)
+    assert abs(comp - ref) <= 1e-5 * ref
+    # sum ||x - mu||^2 = sum ||x||^2 - sum_j n_j ||mu_j||^2 at the exact means
+    M, cnt64 = orc.means_fp64(X, lab, 16)
+    X64 = X.astype(np.float64)
+    lhs = ((X64 - M[lab]) ** 2).sum()
+    rhs = (X64 ** 2).sum() - (cnt64[:, None] * M ** 2).sum()
+    assert abs(lhs - rhs) < 1e-9 * lhs
+    assert abs(comp - lhs) <= 1e-5 * lhs
+    assert np.all(e >= 0)
+
+
+def test_compactness_uses_given_centres(orc):
+    X = np.array([[0.0], [2.0]], np.float32)
+    lab = np.array([0, 0], np.int32)
+    assert orc.compactness(X, lab, np.array([[1.0]], np.float32)) == 2.0
+    assert orc.compactness(X, lab, np.array([[0.0]], np.float32)) == 4.0
+
+
+# ------------------------------------------------------------------ Lloyd loop
+def test_lloyd_1d_two_clusters_by_hand(orc):
+    X = np.array([[0.0], [1.0], [2.0], [10.0], [11.0]], np.float32)
+    r = orc.lloyd(X, np.array([[0.0], [1.0]], np.float32), 10, criterion=0.0)
+    # it1: {0} {1,2,10,11} -> c=(0, 6); comp = 0 + 25+16+16+25 = 82
+    # it2: {0,1,2} {10,11} -> c=(1, 10.5); comp = 1+0+1+0.25+0.25 = 2.5
+    # it3: unchanged -> comp 2.5, relErr 0 -> stop
+    assert list(r["comp_hist"]) == [82.0, 2.5, 2.5]
+    assert r["iters"] == 3 and r["status"] == 0
+    assert np.array_equal(r["centers"], np.array([[1.0], [10.5]], np.float32))
+    assert list(r["labels"]) == [0, 0, 0, 1, 1]
+
+
+def test_lloyd_nonconvergence_status(orc):
+    X, _, _, _ = orc.mixture(1, 5000, 4, 8, -3, 3, 1.0, 1.0)
+    r = orc.lloyd(X, X[:8], 2, criterion=1e-12)
+    assert r["iters"] == 2 and r["status"] == 13
+
+
+def test_lloyd_closed_form_tiny_sigma(orc):
+    # sigma -> 0: one step from one point per component lands on the means.
+    X, z, means, _ = orc.mixture(21, 4000, 6, 5, -10, 10, 1e-6, 1e-6)
+    first = [int(np.nonzero(z == j)[0][0]) for j in range(5)]
+    r = orc.lloyd(X, X[first], 1)
+    assert np.array_equal(r["labels"], z)
+    assert np.allclose(r["centers"], means, atol=1e-5)
+
+
+def test_lloyd_matches_sklearn(orc):
+    from sklearn.cluster import KMeans
+    X, z, means, _ = orc.mixture(0, 20_000, 8, 12, -10, 10, 1.0, 1.0)
+    C0 = X[:12].copy()
+    r = orc.lloyd(X, C0, 15)
+    km = KMeans(n_clusters=12, init=C0.astype(np.float64), n_init=1, max_iter=15, tol=0.0,
+                algorithm="lloyd").fit(X.astype(np.float64))
+    # sklearn's labels_ come from a final assignment against the final centres
+    assert np.array_equal(orc.assign(X, r["centers"]), km.labels_)
+    assert np.abs(r["centers"] - km.cluster_centers_).max() < 1e-5
+
+
+# ------------------------------------------------------------------ generators
+def test_generator_is_order_free_and_matches_spec(orc):
+    X, z, means, sig = orc.mixture(1, 50_000, 4, 8, -5, 5, 0.5, 2.0)
+    X2, z2, _, _ = orc.mixture(1, 1000, 4, 8, -5, 5, 0.5, 2.0, i0=20_000)
+    assert np.array_equal(X[20_000:21_000], X2) and np.array_equal(z[20_000:21_000], z2)
+    assert means.min() >= -5 and means.max() <= 5 and sig.min() >= 0.5 and sig.max() <= 2.0
+    for j in range(8):
+        r = X[z == j] - means[j]
+        assert abs(r.std() - sig[j]) < 0.05 * sig[j]
+        assert np.abs(r.mean(0)).max() < 0.1
+    assert abs(np.bincount(z).std() / np.bincount(z).mean()) < 0.05
+
+
+def test_generator_zipf_weights(orc):
+    X, z, _, _ = orc.mixture(2, 200_000, 2, 64, -8, 8, 1.0, 1.0, zipf_s=1.1)
+    cnt = np.bincount(z, minlength=64).astype(float)
+    w = np.arange(1, 65) ** -1.1
+    w /= w.sum()
+    assert np.allclose(cnt / cnt.sum(), w, atol=3e-3)
+
+
+def test_sample_init_distinct(orc):
+    idx = orc.sample_init(1, 10_000_000, 1024)
+    assert len(set(idx.tolist())) == 1024 and idx.min() >= 0 and idx.max() < 10_000_000
+    assert np.array_equal(idx, orc.sample_init(1, 10_000_000, 1024))
