@@ -1,0 +1,242 @@
+/*
+ * kmeans_amd.h -- C-ABI of the MI355X (gfx950) K-means engine.
+ *
+ * Three layers, all extern "C", plain pointers and sizes:
+ *
+ *  1. Host API, the drop-in for the reference's public entry points
+ *     (main/cpp/kmeans.hpp:44-132): computeKmeansF / computeKmeansD with the
+ *     same argument lists and status codes, kmeansGetErrorString, plus the
+ *     handle-based engine (kmeansCreate ... kmeansDestroy) that exposes what
+ *     the reference computes but never returns (labels, compactness, initial
+ *     centres supplied by the caller, per-phase timings).
+ *
+ *  2. Kernel API (km*), the replacement for the reference's kernel C-ABI
+ *     (main/cuda/KmeansCudaKernels.h:170-478): device pointers, an explicit
+ *     HIP stream (passed as void*, 0 = null stream), caller-sized workspace
+ *     with size queries, no process-global state, reentrant.  Nothing here
+ *     allocates device memory.
+ *
+ *  3. Legacy kernel names (ClosestCentersF, ClusterCentersF, CompactnessF,
+ *     rowNormalizeF, colNormalizeF and D variants) with the reference's
+ *     argument lists, so existing callers link unchanged.
+ *
+ * Layouts: points X are N x D row-major (the reference's "m x n C-ordered"
+ * data, main/cpp/kmeans.hpp:88-92).  Engine centres are K x D row-major; the
+ * legacy API's result buffer is D x K (main/cpp/kmeans.cpp:886-888).
+ *
+ * Arithmetic contract (the plain FP32 path): ||c_j||^2 and the score
+ * s_ij = -0.5f*||c_j||^2 + x_i.c_j are single-rounding fma chains over d in
+ * ascending order; the label is the lowest j among the maxima of s_ij.  New
+ * centres and the compactness are FP64 sums in a fixed order (KM_CHUNK-member
+ * pieces within KM_BLOCK-point blocks), so results do not depend on the GPU,
+ * the launch shape or the N-tiling.
+ */
+#ifndef KMEANS_AMD_H_
+#define KMEANS_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KM_CHUNK 256           /* members per first-level FP64 partial sum */
+#define KM_BLOCK (1L << 23)    /* points per canonical summation block     */
+
+/* ---- status codes: values of main/cpp/kmeans.hpp:44-59 ------------------ */
+typedef enum kmeansErrorStatus {
+  KMEANS_SUCCESS = 0,
+  KMEANS_ERROR_BUILDDATA = 1,
+  KMEANS_ERROR_INITIALIZE = 2,
+  KMEANS_ERROR_CLOSEST_CENTERS = 3,
+  KMEANS_ERROR_CLUSTER_CENTERS = 4,
+  KMEANS_ERROR_COMPACTNESS = 5,
+  KMEANS_ERROR_TRANSPOSE = 6,
+  KMEANS_ERROR_RESET = 7,
+  KMEANS_ERROR_COMPUTE = 8,
+  KMEANS_ERROR_GET_CENTERS = 9,
+  KMEANS_ERROR_TILING = 10,
+  KMEANS_ERROR_COPY_TILE = 11,
+  KMEANS_ERROR_CONSTRUCT_MINIBATCH = 12,
+  KMEANS_NONCONVERGENCE = 13,
+  KMEANS_ERROR_INVALID_ARGUMENT = 14 /* new: shape or option rejected */
+} kmeansErrorStatus;
+
+/* ---- kernel status codes: values of KmeansCudaKernels.h:174-183 --------- */
+typedef enum kmeansCudaErrorStatus {
+  NO_ERROR = 0,
+  ERROR_MATMATMULT = 1,
+  ERROR_ROWTRANSFORMMINIMUM = 2,
+  ERROR_CLOSESTCENTERS = 3,
+  ERROR_CLUSTERCENTERS = 4,
+  ERROR_COMPACTNESS = 5,
+  ERROR_ROWNORMALIZE = 6,
+  ERROR_COLNORMALIZE = 7,
+  ERROR_INVALID_ARGUMENT = 8, /* new: bad shape, null pointer, small workspace */
+  ERROR_UNSUPPORTED = 9       /* new: no GPU / not built for this device     */
+} kmeansCudaErrorStatus;
+
+/* =========================================================================
+ * 1. Host API
+ * ========================================================================= */
+
+/* Replaces kmeansGetErrorString (main/cpp/kmeans.hpp:66-82). */
+const char *kmeansGetErrorString(kmeansErrorStatus err);
+
+/* Replaces computeKmeansF (main/cpp/kmeans.hpp:102-107, kmeans.cpp:1316-1361).
+ * data: m x n host, C-ordered.  result: n x k host (D x K).
+ * Deliberate deltas: rows are not shuffled; initAlgorithm == 1 takes the
+ * initial centres from result (D x K) instead of sampling; an empty cluster
+ * keeps its previous centre instead of becoming NaN.  doFullSGEMM and
+ * matrixFormat are accepted and select the same (fused) path; numRetries is
+ * ignored as in the reference.  miniBatchFraction < 1 runs the mini-batch
+ * loop (kmeans.cpp:1253-1312). */
+kmeansErrorStatus computeKmeansF(const float *data, int m, int n, int k, float criterion,
+                                 int maxIters, int numRetries, int initAlgorithm,
+                                 int doFullSGEMM, int matrixFormat, float *result,
+                                 float miniBatchFraction);
+
+/* Replaces computeKmeansD (main/cpp/kmeans.hpp:125-130, kmeans.cpp:1364-1409).
+ * Same contract in double precision (the reference's fused FP64 path returns
+ * label 0 for every row, reduction.hcu:8-11; this one computes real labels). */
+kmeansErrorStatus computeKmeansD(const double *data, int m, int n, int k, double criterion,
+                                 int maxIters, int numRetries, int initAlgorithm,
+                                 int doFullSGEMM, int matrixFormat, double *result,
+                                 double miniBatchFraction);
+
+/* Assignment precision of the engine. */
+typedef enum kmeansPrecision {
+  KM_PREC_FP32 = 0,      /* plain path: fma-chain scores on VALU             */
+  KM_PREC_BF16 = 1,      /* BF16 MFMA scores + FP32 re-resolve of near-ties  */
+  KM_PREC_FP16 = 2       /* FP16 MFMA scores + FP32 re-resolve of near-ties  */
+} kmeansPrecision;
+
+typedef struct kmeansOptions {
+  int precision;          /* kmeansPrecision                                  */
+  int device;             /* HIP device ordinal                               */
+  void *stream;           /* hipStream_t to run on; NULL: engine creates one  */
+  long tile_rows;         /* points per streamed tile (0: all resident)       */
+  int timing;             /* 1: record hipEvents per phase (no host syncs)    */
+} kmeansOptions;
+
+typedef struct kmeansStats {
+  int iterations;
+  double compactness;     /* FP64, against the final centres                  */
+  double rel_err;
+  long long empty_clusters;   /* summed over iterations                       */
+  long long reresolved;       /* points re-searched in FP32 (MFMA paths)      */
+  double ms_assign;           /* summed over iterations (timing == 1)         */
+  double ms_reresolve;
+  double ms_update;
+  double ms_compactness;
+  double ms_total;
+} kmeansStats;
+
+typedef struct kmeansEngine *kmeansHandle;
+
+/* Engine lifecycle.  m points of dimension n, k centres. */
+kmeansErrorStatus kmeansCreate(kmeansHandle *h, long m, int n, int k, const kmeansOptions *opt);
+void kmeansDestroy(kmeansHandle h);
+/* Points: host (on_device == 0, copied) or device (borrowed, must outlive h). */
+kmeansErrorStatus kmeansSetData(kmeansHandle h, const float *data, int on_device);
+/* Initial centres, k x n host (C-ordered). */
+kmeansErrorStatus kmeansSetCenters(kmeansHandle h, const float *centers);
+/* Lloyd loop: while (it < maxIters && relErr > criterion) -- kmeans.cpp:1194. */
+kmeansErrorStatus kmeansRun(kmeansHandle h, int maxIters, double criterion);
+kmeansErrorStatus kmeansGetCenters(kmeansHandle h, float *centers /* k x n host */);
+kmeansErrorStatus kmeansGetLabels(kmeansHandle h, int *labels /* m host */);
+kmeansErrorStatus kmeansGetStats(kmeansHandle h, kmeansStats *stats);
+/* Mini-batch mode (kmeans.cpp:1253-1312): each iteration assigns and updates
+ * from the next window of `rows` points (wrapping at m); 0 restores the
+ * standard loop over all points. */
+kmeansErrorStatus kmeansSetMiniBatch(kmeansHandle h, long rows);
+/* Device pointers owned by the engine (valid until kmeansDestroy). */
+const float *kmeansDeviceCenters(kmeansHandle h);
+const int *kmeansDeviceLabels(kmeansHandle h);
+kmeansErrorStatus kmeansSynchronize(kmeansHandle h);
+
+/* =========================================================================
+ * 2. Kernel API (device pointers; stream is a hipStream_t or NULL)
+ * ========================================================================= */
+
+/* Replaces colNormalizeF (KmeansCudaKernels.h:458-463, Normalize.cu:33-47):
+ * nc[j] = fma chain over d of C[j][d]^2.  C is k x d row-major. */
+kmeansCudaErrorStatus kmColNorms(void *stream, int k, int d, const float *C, float *nc);
+
+/* Padded leading dimension of the D x K centre copy the assign kernels read. */
+int kmCentersLd(int k);
+
+/* Transposes C (k x d) into Ct (d x kmCentersLd(k), zero padded). */
+kmeansCudaErrorStatus kmTransposeCenters(void *stream, int k, int d, const float *C, float *Ct);
+
+/* Replaces ClosestCentersF (KmeansCudaKernels.h:238-244, ClosestCenters.cu:
+ * 46-123): fused score + argmax, no N x K or N x ceil(K/64) buffers.
+ * rows == NULL: points 0..m-1 of X, labels[i] written for i < m.
+ * rows != NULL: points rows[0..m-1] (m read from *m_dev when m_dev != NULL),
+ * labels[rows[p]] written.  best (nullable) receives the winning score. */
+kmeansCudaErrorStatus kmAssignF32(void *stream, long m, const int *m_dev, int d, const float *X,
+                                  const int *rows, int k, const float *Ct, const float *nc,
+                                  int *labels, float *best);
+
+/* Workspace bytes for kmUpdate / kmCompactness at these sizes. */
+size_t kmUpdateWorkspaceSize(long n, int d, int k);
+
+/* Replaces ClusterCentersF (KmeansCudaKernels.h:337-343, ClusterCenters.cu:
+ * 3-132): counts and canonical FP64 means.  C_new may alias C_old.  Ct_new
+ * (nullable) receives the D x ld copy, nc_new (nullable) the new norms,
+ * counts (nullable, k int64) the member counts, empty (nullable, device int)
+ * the number of empty clusters (which keep their previous centre). */
+kmeansCudaErrorStatus kmUpdate(void *stream, long n, int d, int k, const float *X,
+                               const int *labels, const float *C_old, float *C_new,
+                               float *Ct_new, float *nc_new, long long *counts, int *empty,
+                               void *workspace, size_t workspace_bytes);
+
+/* Replaces CompactnessF (KmeansCudaKernels.h:389-394, Compactness.cu:3-96):
+ * out[0] (device double) = canonical FP64 sum of ||x_i - C[l_i]||^2. */
+kmeansCudaErrorStatus kmCompactness(void *stream, long n, int d, int k, const float *X,
+                                    const int *labels, const float *C, double *out,
+                                    void *workspace, size_t workspace_bytes);
+
+/* Replaces rowNormalizeF (KmeansCudaKernels.h:428-432, Normalize.cu:3-31):
+ * out[m0+i] = fma chain over d of X[i][d]^2 for i < m. */
+kmeansCudaErrorStatus kmRowNorms(void *stream, long m0, long m, int d, const float *X, float *out);
+
+/* Synthetic Gaussian mixture rows [i0, i0+n) into X (device), counter-based
+ * so any slice can be made independently; the same streams as the oracle's
+ * generator.  means (ncomp x d) and sigmas (ncomp) are device arrays; cdf
+ * (ncomp doubles, device) holds the cumulative component weights. */
+kmeansCudaErrorStatus kmGenerateMixture(void *stream, uint64_t seed, long i0, long n, int d,
+                                        int ncomp, const float *means, const float *sigmas,
+                                        const double *cdf, float *X, int *z);
+
+/* Gather rows: out[r] = X[idx[r]] for r < n (device). */
+kmeansCudaErrorStatus kmGatherRows(void *stream, long n, int d, const float *X, const int *idx,
+                                   float *out);
+
+/* =========================================================================
+ * 3. Legacy kernel names (KmeansCudaKernels.h:238-474), default stream.
+ * Scratch arguments of the reference (Bpadded, C, Cindices, centers_large,
+ * counts_large) are accepted and ignored: the fused kernels need none.
+ * ========================================================================= */
+kmeansCudaErrorStatus ClosestCentersF(const int m0, const int nRowsA, const int nColsA,
+                                      const float *A, const int nColsB, const float *B,
+                                      float *Bpadded, const float *normRowsOfA_squared,
+                                      const float *normColsOfB_squared, const int nColsC,
+                                      float *C, int *Cindices, int *CindicesFinal);
+kmeansCudaErrorStatus ClusterCentersF(const int m0, const int m, const int n, const int k,
+                                      const bool lastTile, const float *data, const int *indices,
+                                      float *centers_large, int *counts_large, float *centers,
+                                      int *counts);
+kmeansCudaErrorStatus CompactnessF(const int m0, const int m, const bool lastTile, const int n,
+                                   const int k, const float *data, const int *indices,
+                                   const float *centers, float *compactness);
+kmeansCudaErrorStatus rowNormalizeF(const int m0, const int m, const int n, const float *in,
+                                    float *out);
+kmeansCudaErrorStatus colNormalizeF(const int m, const int n, const float *in, float *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KMEANS_AMD_H_ */

@@ -1,0 +1,252 @@
+"""kmeans_amd -- Python mirror of the MI355X K-means engine's C-ABI.
+
+The product is the shared library libkmeans_amd.so (hand-written HIP for
+gfx950 behind the extern "C" boundary in include/kmeans_amd.h).  This module
+binds it with ctypes so Python callers, the tests and bench.py reach the same
+entry points a C or C++ caller links against:
+
+  compute_kmeans_f(...)        computeKmeansF (reference main/cpp/kmeans.hpp:102-107)
+  KMeans(m, n, k, ...)         the engine handle (kmeans<T> of kmeans.hpp:174-658)
+  col_norms / assign_f32 /     kernel API, device pointers (KmeansCudaKernels.h:
+  update / compactness          238-474 replaced with stream + workspace arguments)
+
+There is no CPU fallback: if the library is missing or no GPU is present the
+calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkmeans_amd.so")
+
+KMEANS_SUCCESS = 0
+KMEANS_NONCONVERGENCE = 13
+KMEANS_ERROR_INVALID_ARGUMENT = 14
+PREC = {"fp32": 0, "bf16": 1, "fp16": 2}
+KM_CHUNK = 256
+KM_BLOCK = 1 << 23
+
+ERRORS = {
+    1: "KMEANS_ERROR_BUILDDATA", 2: "KMEANS_ERROR_INITIALIZE", 3: "KMEANS_ERROR_CLOSEST_CENTERS",
+    4: "KMEANS_ERROR_CLUSTER_CENTERS", 5: "KMEANS_ERROR_COMPACTNESS", 6: "KMEANS_ERROR_TRANSPOSE",
+    7: "KMEANS_ERROR_RESET", 8: "KMEANS_ERROR_COMPUTE", 9: "KMEANS_ERROR_GET_CENTERS",
+    10: "KMEANS_ERROR_TILING", 11: "KMEANS_ERROR_COPY_TILE", 12: "KMEANS_ERROR_CONSTRUCT_MINIBATCH",
+    13: "KMEANS_NONCONVERGENCE", 14: "KMEANS_ERROR_INVALID_ARGUMENT",
+}
+
+
+class KmeansError(RuntimeError):
+    def __init__(self, code, where=""):
+        self.code = code
+        super().__init__(f"{where}: {ERRORS.get(code, code)}")
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("device", ctypes.c_int), ("stream", ctypes.c_void_p),
+                ("tile_rows", ctypes.c_long), ("timing", ctypes.c_int)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("compactness", ctypes.c_double),
+                ("rel_err", ctypes.c_double), ("empty_clusters", ctypes.c_longlong),
+                ("reresolved", ctypes.c_longlong), ("ms_assign", ctypes.c_double),
+                ("ms_reresolve", ctypes.c_double), ("ms_update", ctypes.c_double),
+                ("ms_compactness", ctypes.c_double), ("ms_total", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+# Every symbol declared in include/kmeans_amd.h with its ctypes signature.
+_P, _I, _L, _F, _D, _U64, _SZ = (ctypes.c_void_p, ctypes.c_int, ctypes.c_long, ctypes.c_float,
+                                 ctypes.c_double, ctypes.c_uint64, ctypes.c_size_t)
+SIGNATURES = {
+    "kmeansGetErrorString": (ctypes.c_char_p, [_I]),
+    "computeKmeansF": (_I, [_P, _I, _I, _I, _F, _I, _I, _I, _I, _I, _P, _F]),
+    "computeKmeansD": (_I, [_P, _I, _I, _I, _D, _I, _I, _I, _I, _I, _P, _D]),
+    "kmeansCreate": (_I, [_P, _L, _I, _I, _P]),
+    "kmeansDestroy": (None, [_P]),
+    "kmeansSetData": (_I, [_P, _P, _I]),
+    "kmeansSetCenters": (_I, [_P, _P]),
+    "kmeansRun": (_I, [_P, _I, _D]),
+    "kmeansGetCenters": (_I, [_P, _P]),
+    "kmeansGetLabels": (_I, [_P, _P]),
+    "kmeansGetStats": (_I, [_P, _P]),
+    "kmeansSetMiniBatch": (_I, [_P, _L]),
+    "kmeansDeviceCenters": (_P, [_P]),
+    "kmeansDeviceLabels": (_P, [_P]),
+    "kmeansSynchronize": (_I, [_P]),
+    "kmColNorms": (_I, [_P, _I, _I, _P, _P]),
+    "kmCentersLd": (_I, [_I]),
+    "kmTransposeCenters": (_I, [_P, _I, _I, _P, _P]),
+    "kmAssignF32": (_I, [_P, _L, _P, _I, _P, _P, _I, _P, _P, _P, _P]),
+    "kmUpdateWorkspaceSize": (_SZ, [_L, _I, _I]),
+    "kmUpdate": (_I, [_P, _L, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _SZ]),
+    "kmCompactness": (_I, [_P, _L, _I, _I, _P, _P, _P, _P, _P, _SZ]),
+    "kmRowNorms": (_I, [_P, _L, _L, _I, _P, _P]),
+    "kmGenerateMixture": (_I, [_P, _U64, _L, _L, _I, _I, _P, _P, _P, _P, _P]),
+    "kmGatherRows": (_I, [_P, _L, _I, _P, _P, _P]),
+    "ClosestCentersF": (_I, [_I, _I, _I, _P, _I, _P, _P, _P, _P, _I, _P, _P, _P]),
+    "ClusterCentersF": (_I, [_I, _I, _I, _I, ctypes.c_bool, _P, _P, _P, _P, _P, _P]),
+    "CompactnessF": (_I, [_I, _I, ctypes.c_bool, _I, _I, _P, _P, _P, _P]),
+    "rowNormalizeF": (_I, [_I, _I, _I, _P, _P]),
+    "colNormalizeF": (_I, [_I, _I, _P, _P]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libkmeans_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _np_ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check(code, where):
+    if code != 0:
+        raise KmeansError(code, where)
+
+
+# ------------------------------------------------------------------ host API
+def compute_kmeans_f(data, k, criterion, max_iters, result=None, init_from_result=False,
+                     mini_batch_fraction=1.0):
+    """computeKmeansF: returns (status, centers as D x K)."""
+    data = np.ascontiguousarray(data, dtype=np.float32)
+    m, n = data.shape
+    res = np.zeros((n, k), np.float32) if result is None else np.ascontiguousarray(result, np.float32)
+    st = lib().computeKmeansF(_np_ptr(data), m, n, k, criterion, max_iters, 0,
+                              1 if init_from_result else 0, 0, 0, _np_ptr(res), mini_batch_fraction)
+    return st, res
+
+
+class KMeans:
+    """The engine handle: kmeansCreate / SetData / SetCenters / Run / Get*."""
+
+    def __init__(self, m, n, k, precision="fp32", device=0, stream=None, timing=False):
+        self.m, self.n, self.k = int(m), int(n), int(k)
+        opt = Options(PREC[precision], device, stream, 0, 1 if timing else 0)
+        h = ctypes.c_void_p()
+        _check(lib().kmeansCreate(ctypes.byref(h), self.m, self.n, self.k, ctypes.byref(opt)),
+               "kmeansCreate")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().kmeansDestroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_data(self, data=None, device_ptr=None):
+        if device_ptr is not None:
+            _check(lib().kmeansSetData(self._h, ctypes.c_void_p(device_ptr), 1), "kmeansSetData")
+        else:
+            self._host = np.ascontiguousarray(data, dtype=np.float32)
+            assert self._host.shape == (self.m, self.n)
+            _check(lib().kmeansSetData(self._h, _np_ptr(self._host), 0), "kmeansSetData")
+
+    def set_centers(self, C):
+        C = np.ascontiguousarray(C, dtype=np.float32)
+        assert C.shape == (self.k, self.n)
+        _check(lib().kmeansSetCenters(self._h, _np_ptr(C)), "kmeansSetCenters")
+
+    def set_mini_batch(self, rows):
+        _check(lib().kmeansSetMiniBatch(self._h, int(rows)), "kmeansSetMiniBatch")
+
+    def run(self, max_iters, criterion=-1.0):
+        _check(lib().kmeansRun(self._h, int(max_iters), float(criterion)), "kmeansRun")
+
+    def centers(self):
+        out = np.empty((self.k, self.n), np.float32)
+        _check(lib().kmeansGetCenters(self._h, _np_ptr(out)), "kmeansGetCenters")
+        return out
+
+    def labels(self):
+        out = np.empty(self.m, np.int32)
+        _check(lib().kmeansGetLabels(self._h, _np_ptr(out)), "kmeansGetLabels")
+        return out
+
+    def stats(self):
+        s = Stats()
+        _check(lib().kmeansGetStats(self._h, ctypes.byref(s)), "kmeansGetStats")
+        return s.as_dict()
+
+    def synchronize(self):
+        _check(lib().kmeansSynchronize(self._h), "kmeansSynchronize")
+
+    def device_labels_ptr(self):
+        return lib().kmeansDeviceLabels(self._h)
+
+    def device_centers_ptr(self):
+        return lib().kmeansDeviceCenters(self._h)
+
+
+# ------------------------------------------------------------------ kernel API
+def _vp(x):
+    return None if x is None else ctypes.c_void_p(int(x))
+
+
+def col_norms(k, d, C_ptr, nc_ptr, stream=None):
+    _check(lib().kmColNorms(_vp(stream), k, d, _vp(C_ptr), _vp(nc_ptr)), "kmColNorms")
+
+
+def centers_ld(k):
+    return lib().kmCentersLd(k)
+
+
+def transpose_centers(k, d, C_ptr, Ct_ptr, stream=None):
+    _check(lib().kmTransposeCenters(_vp(stream), k, d, _vp(C_ptr), _vp(Ct_ptr)), "kmTransposeCenters")
+
+
+def assign_f32(m, d, X_ptr, k, Ct_ptr, nc_ptr, labels_ptr, best_ptr=None, rows_ptr=None,
+               m_dev_ptr=None, stream=None):
+    _check(lib().kmAssignF32(_vp(stream), m, _vp(m_dev_ptr), d, _vp(X_ptr), _vp(rows_ptr), k,
+                             _vp(Ct_ptr), _vp(nc_ptr), _vp(labels_ptr), _vp(best_ptr)), "kmAssignF32")
+
+
+def update_workspace_size(n, d, k):
+    return lib().kmUpdateWorkspaceSize(n, d, k)
+
+
+def update(n, d, k, X_ptr, labels_ptr, C_old_ptr, C_new_ptr, ws_ptr, ws_bytes, Ct_ptr=None,
+           nc_ptr=None, counts_ptr=None, empty_ptr=None, stream=None):
+    _check(lib().kmUpdate(_vp(stream), n, d, k, _vp(X_ptr), _vp(labels_ptr), _vp(C_old_ptr),
+                          _vp(C_new_ptr), _vp(Ct_ptr), _vp(nc_ptr), _vp(counts_ptr), _vp(empty_ptr),
+                          _vp(ws_ptr), ws_bytes), "kmUpdate")
+
+
+def compactness(n, d, k, X_ptr, labels_ptr, C_ptr, out_ptr, ws_ptr, ws_bytes, stream=None):
+    _check(lib().kmCompactness(_vp(stream), n, d, k, _vp(X_ptr), _vp(labels_ptr), _vp(C_ptr),
+                               _vp(out_ptr), _vp(ws_ptr), ws_bytes), "kmCompactness")
+
+
+def row_norms(m0, m, d, X_ptr, out_ptr, stream=None):
+    _check(lib().kmRowNorms(_vp(stream), m0, m, d, _vp(X_ptr), _vp(out_ptr)), "kmRowNorms")
+
+
+def generate_mixture(seed, i0, n, d, ncomp, means_ptr, sigmas_ptr, cdf_ptr, X_ptr, z_ptr=None,
+                     stream=None):
+    _check(lib().kmGenerateMixture(_vp(stream), seed, i0, n, d, ncomp, _vp(means_ptr),
+                                   _vp(sigmas_ptr), _vp(cdf_ptr), _vp(X_ptr), _vp(z_ptr)),
+           "kmGenerateMixture")
+
+
+def gather_rows(n, d, X_ptr, idx_ptr, out_ptr, stream=None):
+    _check(lib().kmGatherRows(_vp(stream), n, d, _vp(X_ptr), _vp(idx_ptr), _vp(out_ptr)),
+           "kmGatherRows")

@@ -1,0 +1,85 @@
+// Shared device/host helpers for the gfx950 K-means kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "../../include/kmeans_amd.h"
+
+#define KM_SB_SMALL 4096    // points per histogram/scatter sub-block (k <= 2048)
+#define KM_SB_LARGE 16384   // points per sub-block for larger k
+
+#define KM_HIP_CHECK(expr, code)                                                     \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      fprintf(stderr, "kmeans_amd: %s failed at %s:%d: %s\n", #expr, __FILE__,       \
+              __LINE__, hipGetErrorString(e_));                                      \
+      return code;                                                                   \
+    }                                                                                \
+  } while (0)
+
+#define KM_LAUNCH_CHECK(code) KM_HIP_CHECK(hipGetLastError(), code)
+
+static inline long km_cdiv(long a, long b) { return (a + b - 1) / b; }
+
+namespace km {
+
+// Counter-based generator shared with the oracle: SplitMix64 finaliser
+// (Steele, Lea, Flood, "Fast splittable pseudorandom number generators",
+// OOPSLA 2014) applied to (seed, stream, index).
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+__host__ __device__ inline uint64_t rng_u64(uint64_t seed, uint64_t stream, uint64_t i) {
+  uint64_t k = mix64(seed * 0x9E3779B97F4A7C15ULL + stream * 0xD1B54A32D192ED03ULL +
+                     0x632BE59BD9B4E019ULL);
+  return mix64(k + i * 0x9E3779B97F4A7C15ULL);
+}
+__host__ __device__ inline double rng_unif(uint64_t seed, uint64_t stream, uint64_t i) {
+  return ((double)(rng_u64(seed, stream, i) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+// Internal launchers (implemented in km_kernels.hip); all stream-ordered.
+struct UpdateWs {
+  int *hist, *offs, *tsum, *cstart, *ccount, *cq, *perm;
+  double *P, *S, *E;
+  long long *cnt;
+  size_t bytes;
+};
+size_t update_ws_layout(long n, int d, int k, char *base, UpdateWs *ws);
+int sub_block(int k);
+
+hipError_t launch_col_norms(hipStream_t s, int k, int d, const float *C, float *nc);
+hipError_t launch_transpose(hipStream_t s, int k, int d, const float *C, float *Ct, int ldc);
+hipError_t launch_assign_f32(hipStream_t s, long m, const int *m_dev, int d, const float *X,
+                             const int *rows, int k, const float *Ct, int ldc, const float *nc,
+                             int *labels, float *best);
+hipError_t launch_update(hipStream_t s, long n, int d, int k, const float *X, const int *labels,
+                         const float *C_old, float *C_new, float *Ct_new, int ldc, float *nc_new,
+                         long long *counts, int *empty, const UpdateWs &ws);
+hipError_t launch_compactness(hipStream_t s, long n, int d, const float *X, const int *labels,
+                              const float *C, double *out, const UpdateWs &ws);
+hipError_t launch_compactness_acc(hipStream_t s, long n, int d, const float *X, const int *labels,
+                                  const float *C, double *out, const UpdateWs &ws, int first);
+hipError_t launch_update_zero(hipStream_t s, int d, int k, double *S, long long *cnt, int *empty);
+hipError_t launch_update_accumulate(hipStream_t s, long n, int d, int k, const float *X,
+                                    const int *labels, double *S, long long *cnt,
+                                    const UpdateWs &ws);
+hipError_t launch_update_finalize(hipStream_t s, int d, int k, const double *S,
+                                  const long long *cnt, const float *C_old, float *C_new,
+                                  float *Ct_new, int ldc, float *nc_new, long long *counts,
+                                  int *empty);
+hipError_t launch_col_norms_dk(hipStream_t s, int k, int d, int ld, const float *C, float *nc);
+hipError_t launch_row_norms(hipStream_t s, long m, int d, const float *X, float *out);
+hipError_t launch_generate(hipStream_t s, uint64_t seed, long i0, long n, int d, int ncomp,
+                           const float *means, const float *sigmas, const double *cdf, float *X,
+                           int *z);
+hipError_t launch_gather_rows(hipStream_t s, long n, int d, const float *X, const int *idx,
+                              float *out);
+int centers_ld(int k);
+int num_cus();
+
+}  // namespace km

@@ -1,0 +1,687 @@
+// Host engine and the extern "C" boundary (include/kmeans_amd.h).
+//
+// Engine::computeStandard follows main/cpp/kmeans.cpp:1186-1247 (loop while
+// iters < maxIters && relErr > criterion; relErr = |comp - prev| / |comp|),
+// Engine::computeMiniBatch follows kmeans.cpp:1253-1312 (each mini-batch
+// replaces the centres with the means of its members, early return when
+// relErr <= criterion), computeKmeansF follows kmeans.cpp:1316-1361.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "km_engine.hpp"
+#include "km_mfma.hpp"
+
+namespace km {
+
+Engine::~Engine() {
+  if (stream) hipStreamSynchronize(stream);
+  mfma_free(this);
+  hipFree(dX_owned);
+  hipFree(dC);
+  hipFree(dCt);
+  hipFree(dnc);
+  hipFree(dlabels);
+  hipFree(dcounts);
+  hipFree(dempty);
+  hipFree(dcomp);
+  hipFree(dws);
+  for (auto &e : ev)
+    if (e) hipEventDestroy(e);
+  for (auto &e : iter_events)
+    if (e) hipEventDestroy(e);
+  if (own_stream && stream) hipStreamDestroy(stream);
+}
+
+kmeansErrorStatus Engine::create(long m_, int n_, int k_, const kmeansOptions *o) {
+  if (m_ <= 0 || n_ <= 0 || k_ <= 0 || k_ > m_ || m_ > 0x7fffffffL || k_ > 16384)
+    return KMEANS_ERROR_INVALID_ARGUMENT;
+  m = m_;
+  n = n_;
+  k = k_;
+  if (o) opt = *o;
+  if (opt.precision < KM_PREC_FP32 || opt.precision > KM_PREC_FP16)
+    return KMEANS_ERROR_INVALID_ARGUMENT;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    fprintf(stderr, "kmeans_amd: no HIP device available\n");
+    return KMEANS_ERROR_BUILDDATA;
+  }
+  KM_HIP_CHECK(hipSetDevice(opt.device), KMEANS_ERROR_BUILDDATA);
+  if (opt.stream) {
+    stream = (hipStream_t)opt.stream;
+  } else {
+    KM_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), KMEANS_ERROR_BUILDDATA);
+    own_stream = true;
+  }
+  ldc = centers_ld(k);
+  KM_HIP_CHECK(hipMalloc(&dC, sizeof(float) * (size_t)k * n), KMEANS_ERROR_BUILDDATA);
+  KM_HIP_CHECK(hipMalloc(&dCt, sizeof(float) * (size_t)ldc * n), KMEANS_ERROR_BUILDDATA);
+  KM_HIP_CHECK(hipMemsetAsync(dCt, 0, sizeof(float) * (size_t)ldc * n, stream),
+               KMEANS_ERROR_BUILDDATA);
+  KM_HIP_CHECK(hipMalloc(&dnc, sizeof(float) * k), KMEANS_ERROR_BUILDDATA);
+  KM_HIP_CHECK(hipMalloc(&dlabels, sizeof(int) * (size_t)m), KMEANS_ERROR_BUILDDATA);
+  KM_HIP_CHECK(hipMemsetAsync(dlabels, 0, sizeof(int) * (size_t)m, stream), KMEANS_ERROR_BUILDDATA);
+  KM_HIP_CHECK(hipMalloc(&dcounts, sizeof(long long) * k), KMEANS_ERROR_BUILDDATA);
+  KM_HIP_CHECK(hipMalloc(&dempty, sizeof(int) * 2), KMEANS_ERROR_BUILDDATA);
+  const size_t wsb = update_ws_layout(m, n, k, nullptr, nullptr);
+  KM_HIP_CHECK(hipMalloc(&dws, wsb), KMEANS_ERROR_BUILDDATA);
+  update_ws_layout(m, n, k, dws, &ws);
+  return ensure_comp_slots(64);
+}
+
+kmeansErrorStatus Engine::ensure_comp_slots(int nslots) {
+  if (nslots <= comp_slots) return KMEANS_SUCCESS;
+  double *nb = nullptr;
+  KM_HIP_CHECK(hipMalloc(&nb, sizeof(double) * nslots), KMEANS_ERROR_BUILDDATA);
+  if (dcomp) {
+    KM_HIP_CHECK(hipMemcpyAsync(nb, dcomp, sizeof(double) * comp_slots, hipMemcpyDeviceToDevice,
+                                stream),
+                 KMEANS_ERROR_BUILDDATA);
+    KM_HIP_CHECK(hipStreamSynchronize(stream), KMEANS_ERROR_BUILDDATA);
+    hipFree(dcomp);
+  }
+  dcomp = nb;
+  comp_slots = nslots;
+  return KMEANS_SUCCESS;
+}
+
+// kmeans.cpp:540-695 (buildData) without the row shuffle of :554-564.
+kmeansErrorStatus Engine::buildData(const float *data, int on_device) {
+  if (!data) return KMEANS_ERROR_INVALID_ARGUMENT;
+  if (on_device) {
+    dX = data;
+  } else {
+    if (!dX_owned)
+      KM_HIP_CHECK(hipMalloc(&dX_owned, sizeof(float) * (size_t)m * n), KMEANS_ERROR_BUILDDATA);
+    KM_HIP_CHECK(hipMemcpyAsync(dX_owned, data, sizeof(float) * (size_t)m * n,
+                                hipMemcpyHostToDevice, stream),
+                 KMEANS_ERROR_BUILDDATA);
+    KM_HIP_CHECK(hipStreamSynchronize(stream), KMEANS_ERROR_BUILDDATA);
+    dX = dX_owned;
+  }
+  if (mfma) mfma_invalidate(this);
+  return KMEANS_SUCCESS;
+}
+
+kmeansErrorStatus Engine::setCenters(const float *centers) {
+  if (!centers) return KMEANS_ERROR_INVALID_ARGUMENT;
+  KM_HIP_CHECK(hipMemcpyAsync(dC, centers, sizeof(float) * (size_t)k * n, hipMemcpyHostToDevice,
+                              stream),
+               KMEANS_ERROR_INITIALIZE);
+  KM_HIP_CHECK(hipStreamSynchronize(stream), KMEANS_ERROR_INITIALIZE);
+  centers_set = true;
+  relErr = 1.0;
+  compactnessScore = 0.0;
+  return transpose();
+}
+
+// Centre norms and the D x ldc copy the assign kernels read (replaces the
+// per-iteration cuBLAS geam transpose of kmeans.cpp:1109-1139).
+kmeansErrorStatus Engine::transpose() {
+  KM_HIP_CHECK(launch_transpose(stream, k, n, dC, dCt, ldc), KMEANS_ERROR_TRANSPOSE);
+  KM_HIP_CHECK(launch_col_norms(stream, k, n, dC, dnc), KMEANS_ERROR_TRANSPOSE);
+  return KMEANS_SUCCESS;
+}
+
+// kmeans.cpp:822-907: k distinct rows as initial centres.  The reference
+// draws rand()%m after srand(42); this build draws k distinct indices with
+// Floyd's algorithm from the counter-based generator (seed, stream 4).
+kmeansErrorStatus Engine::initialize(uint64_t seed) {
+  if (!dX) return KMEANS_ERROR_INITIALIZE;
+  std::vector<int> idx(k);
+  long t = 0;
+  for (long j = m - k; j < m; ++j) {
+    long r = (long)(rng_u64(seed, 4, (uint64_t)t) % (uint64_t)(j + 1));
+    bool dup = false;
+    for (long q = 0; q < t; ++q)
+      if (idx[q] == r) { dup = true; break; }
+    idx[t++] = dup ? (int)j : (int)r;
+  }
+  int *didx = nullptr;
+  KM_HIP_CHECK(hipMalloc(&didx, sizeof(int) * k), KMEANS_ERROR_INITIALIZE);
+  hipError_t e = hipMemcpyAsync(didx, idx.data(), sizeof(int) * k, hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess) e = launch_gather_rows(stream, k, n, dX, didx, dC);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  hipFree(didx);
+  KM_HIP_CHECK(e, KMEANS_ERROR_INITIALIZE);
+  centers_set = true;
+  relErr = 1.0;
+  compactnessScore = 0.0;
+  return transpose();
+}
+
+// kmeans.cpp:1143-1170: nothing to clear (no G x K x D slabs); the update
+// zeroes its own FP64 accumulators.
+kmeansErrorStatus Engine::reset() { return KMEANS_SUCCESS; }
+
+kmeansErrorStatus Engine::closestCenters(long r0, long rows) {
+  hipError_t e;
+  if (opt.precision == KM_PREC_FP32) {
+    e = launch_assign_f32(stream, rows, nullptr, n, dX + r0 * n, nullptr, k, dCt, ldc, dnc,
+                          dlabels + r0, nullptr);
+  } else {
+    e = mfma_assign(this, r0, rows);
+  }
+  KM_HIP_CHECK(e, KMEANS_ERROR_CLOSEST_CENTERS);
+  return KMEANS_SUCCESS;
+}
+
+kmeansErrorStatus Engine::clusterCenters(long r0, long rows) {
+  KM_HIP_CHECK(launch_update(stream, rows, n, k, dX + r0 * n, dlabels + r0, dC, dC, dCt, ldc, dnc,
+                             dcounts, dempty, ws),
+               KMEANS_ERROR_CLUSTER_CENTERS);
+  if (opt.precision != KM_PREC_FP32) KM_HIP_CHECK(mfma_centers(this), KMEANS_ERROR_CLUSTER_CENTERS);
+  return KMEANS_SUCCESS;
+}
+
+kmeansErrorStatus Engine::compactness(long r0, long rows, int slot) {
+  KM_HIP_CHECK(launch_compactness(stream, rows, n, dX + r0 * n, dlabels + r0, dC, dcomp + slot, ws),
+               KMEANS_ERROR_COMPACTNESS);
+  return KMEANS_SUCCESS;
+}
+
+kmeansErrorStatus Engine::constructMiniBatch(long index, long *r0, long *rows) {
+  // windows [0,mb), [mb,2mb), ..., [q*mb, m), then back to row 0
+  const long per_epoch = (m + mb_rows - 1) / mb_rows;
+  const long start = (index % per_epoch) * mb_rows;
+  *r0 = start;
+  *rows = std::min(mb_rows, m - start);
+  return KMEANS_SUCCESS;
+}
+
+kmeansErrorStatus Engine::compute() {
+  if (!dX || !centers_set) return KMEANS_ERROR_COMPUTE;
+  return mb_rows > 0 ? computeMiniBatch() : computeStandard();
+}
+
+void Engine::record(int slot, int phase) {
+  if (!opt.timing) return;
+  const size_t idx = (size_t)slot * 5 + phase;
+  while (iter_events.size() <= idx) {
+    hipEvent_t e = nullptr;
+    hipEventCreate(&e);
+    iter_events.push_back(e);
+  }
+  hipEventRecord(iter_events[idx], stream);
+}
+
+void Engine::collect_timing(int nslots) {
+  if (!opt.timing) return;
+  hipStreamSynchronize(stream);
+  for (int s = 0; s < nslots; ++s) {
+    float t[4] = {0, 0, 0, 0};
+    for (int p = 0; p < 4; ++p)
+      hipEventElapsedTime(&t[p], iter_events[s * 5 + p], iter_events[s * 5 + p + 1]);
+    stats.ms_assign += t[0];
+    stats.ms_reresolve += t[1];
+    stats.ms_update += t[2];
+    stats.ms_compactness += t[3];
+    stats.ms_total += t[0] + t[1] + t[2] + t[3];
+  }
+}
+
+kmeansErrorStatus Engine::run_iterations(long r0, long rows, bool minibatch) {
+  kmeansErrorStatus err;
+  const bool sync_each = criterion >= 0.0;
+  int it0 = iters, slot = 0;
+  if ((err = ensure_comp_slots(std::max(maxIters, 1))) != KMEANS_SUCCESS) return err;
+  std::vector<double> hist;
+  while (iters < maxIters && relErr > criterion) {
+    long b0 = r0, br = rows;
+    if (minibatch) constructMiniBatch(mb_index++, &b0, &br);
+    record(slot, 0);
+    if ((err = closestCenters(b0, br)) != KMEANS_SUCCESS) return err;
+    record(slot, 1);
+    if (opt.precision != KM_PREC_FP32)
+      KM_HIP_CHECK(mfma_reresolve(this, b0, br), KMEANS_ERROR_CLOSEST_CENTERS);
+    record(slot, 2);
+    if ((err = clusterCenters(b0, br)) != KMEANS_SUCCESS) return err;
+    KM_HIP_CHECK(hipMemcpyAsync(dempty + 1, dempty, sizeof(int), hipMemcpyDeviceToDevice, stream),
+                 KMEANS_ERROR_COMPUTE);
+    record(slot, 3);
+    if ((err = compactness(b0, br, slot)) != KMEANS_SUCCESS) return err;
+    record(slot, 4);
+    ++iters;
+    ++slot;
+    if (opt.precision != KM_PREC_FP32 || sync_each) {
+      int emp = 0;
+      double comp = 0.0;
+      KM_HIP_CHECK(hipMemcpyAsync(&comp, dcomp + slot - 1, sizeof(double), hipMemcpyDeviceToHost,
+                                  stream),
+                   KMEANS_ERROR_COMPUTE);
+      KM_HIP_CHECK(hipMemcpyAsync(&emp, dempty + 1, sizeof(int), hipMemcpyDeviceToHost, stream),
+                   KMEANS_ERROR_COMPUTE);
+      KM_HIP_CHECK(hipStreamSynchronize(stream), KMEANS_ERROR_COMPUTE);
+      stats.empty_clusters += emp;
+      if (opt.precision != KM_PREC_FP32) mfma_collect(this);
+      const double prev = compactnessScore;
+      compactnessScore = comp;
+      relErr = fabs(comp - prev) / fabs(comp);
+      if (minibatch && relErr <= criterion) break;
+    } else {
+      // fixed iteration count: no host round trip inside the loop
+      relErr = 1.0 + fabs(criterion);
+    }
+  }
+  if (!(opt.precision != KM_PREC_FP32 || sync_each) && slot > 0) {
+    hist.resize(slot);
+    KM_HIP_CHECK(hipMemcpyAsync(hist.data(), dcomp, sizeof(double) * slot, hipMemcpyDeviceToHost,
+                                stream),
+                 KMEANS_ERROR_COMPUTE);
+    int emp = 0;
+    KM_HIP_CHECK(hipMemcpyAsync(&emp, dempty + 1, sizeof(int), hipMemcpyDeviceToHost, stream),
+                 KMEANS_ERROR_COMPUTE);
+    KM_HIP_CHECK(hipStreamSynchronize(stream), KMEANS_ERROR_COMPUTE);
+    stats.empty_clusters += emp;  // last iteration only in this mode
+    for (int s = 0; s < slot; ++s) {
+      const double prev = compactnessScore;
+      compactnessScore = hist[s];
+      relErr = fabs(hist[s] - prev) / fabs(hist[s]);
+    }
+  }
+  collect_timing(slot);
+  stats.iterations = iters;
+  stats.compactness = compactnessScore;
+  stats.rel_err = relErr;
+  (void)it0;
+  return KMEANS_SUCCESS;
+}
+
+kmeansErrorStatus Engine::computeStandard() { return run_iterations(0, m, false); }
+kmeansErrorStatus Engine::computeMiniBatch() { return run_iterations(0, m, true); }
+
+kmeansErrorStatus Engine::getClusterCenters(float *out) {
+  KM_HIP_CHECK(hipMemcpyAsync(out, dC, sizeof(float) * (size_t)k * n, hipMemcpyDeviceToHost, stream),
+               KMEANS_ERROR_GET_CENTERS);
+  KM_HIP_CHECK(hipStreamSynchronize(stream), KMEANS_ERROR_GET_CENTERS);
+  return KMEANS_SUCCESS;
+}
+
+kmeansErrorStatus Engine::getLabels(int *out) {
+  KM_HIP_CHECK(hipMemcpyAsync(out, dlabels, sizeof(int) * (size_t)m, hipMemcpyDeviceToHost, stream),
+               KMEANS_ERROR_GET_CENTERS);
+  KM_HIP_CHECK(hipStreamSynchronize(stream), KMEANS_ERROR_GET_CENTERS);
+  return KMEANS_SUCCESS;
+}
+
+// ---------------------------------------------------------------- shared
+// Workspace for the legacy kernel names (the reference's signatures carry no
+// workspace argument); grown on demand, kept until process exit.
+static std::mutex g_legacy_mu;
+static char *g_legacy_ws = nullptr;
+static size_t g_legacy_bytes = 0;
+static float *g_legacy_ct = nullptr;
+static size_t g_legacy_ct_bytes = 0;
+
+static hipError_t legacy_ws(long n, int d, int k, UpdateWs *ws) {
+  const size_t need = update_ws_layout(n, d, k, nullptr, nullptr);
+  if (need > g_legacy_bytes) {
+    hipFree(g_legacy_ws);
+    g_legacy_ws = nullptr;
+    g_legacy_bytes = 0;
+    hipError_t e = hipMalloc(&g_legacy_ws, need);
+    if (e != hipSuccess) return e;
+    g_legacy_bytes = need;
+  }
+  update_ws_layout(n, d, k, g_legacy_ws, ws);
+  return hipSuccess;
+}
+
+}  // namespace km
+
+using km::Engine;
+
+struct kmeansEngine {
+  Engine e;
+};
+
+// ====================================================================== C API
+extern "C" {
+
+const char *kmeansGetErrorString(kmeansErrorStatus err) {
+  switch (err) {
+    case KMEANS_ERROR_BUILDDATA: return "error in building the data";
+    case KMEANS_ERROR_INITIALIZE: return "error in the initialization";
+    case KMEANS_ERROR_CLOSEST_CENTERS: return "error in computing the closest centers";
+    case KMEANS_ERROR_CLUSTER_CENTERS: return "error in computing the cluster centers";
+    case KMEANS_ERROR_COMPACTNESS: return "error in computing the compactness";
+    case KMEANS_ERROR_TRANSPOSE: return "error in transpose";
+    case KMEANS_ERROR_RESET: return "error in reset";
+    case KMEANS_ERROR_COMPUTE: return "error in compute";
+    case KMEANS_ERROR_GET_CENTERS: return "error in copying the data back to the host";
+    case KMEANS_ERROR_TILING: return "error in the tiling";
+    case KMEANS_ERROR_COPY_TILE: return "error in copy tile";
+    case KMEANS_ERROR_CONSTRUCT_MINIBATCH: return "error in constructing the minibatch";
+    case KMEANS_NONCONVERGENCE: return "algorithm did not converge";
+    case KMEANS_ERROR_INVALID_ARGUMENT: return "invalid argument";
+    default: return "no error";
+  }
+}
+
+kmeansErrorStatus kmeansCreate(kmeansHandle *h, long m, int n, int k, const kmeansOptions *opt) {
+  if (!h) return KMEANS_ERROR_INVALID_ARGUMENT;
+  *h = nullptr;
+  kmeansEngine *p = new (std::nothrow) kmeansEngine();
+  if (!p) return KMEANS_ERROR_BUILDDATA;
+  kmeansErrorStatus st = p->e.create(m, n, k, opt);
+  if (st != KMEANS_SUCCESS) {
+    delete p;
+    return st;
+  }
+  *h = p;
+  return KMEANS_SUCCESS;
+}
+
+void kmeansDestroy(kmeansHandle h) { delete h; }
+
+kmeansErrorStatus kmeansSetData(kmeansHandle h, const float *data, int on_device) {
+  if (!h) return KMEANS_ERROR_INVALID_ARGUMENT;
+  return h->e.buildData(data, on_device);
+}
+
+kmeansErrorStatus kmeansSetCenters(kmeansHandle h, const float *centers) {
+  if (!h) return KMEANS_ERROR_INVALID_ARGUMENT;
+  return h->e.setCenters(centers);
+}
+
+kmeansErrorStatus kmeansRun(kmeansHandle h, int maxIters, double criterion) {
+  if (!h || maxIters < 0) return KMEANS_ERROR_INVALID_ARGUMENT;
+  h->e.maxIters = h->e.iters + maxIters;
+  h->e.criterion = criterion;
+  if (h->e.relErr <= criterion) h->e.relErr = 1.0 + fabs(criterion);
+  kmeansErrorStatus st = h->e.compute();
+  if (st != KMEANS_SUCCESS) return st;
+  return KMEANS_SUCCESS;
+}
+
+kmeansErrorStatus kmeansGetCenters(kmeansHandle h, float *centers) {
+  if (!h || !centers) return KMEANS_ERROR_INVALID_ARGUMENT;
+  return h->e.getClusterCenters(centers);
+}
+
+kmeansErrorStatus kmeansGetLabels(kmeansHandle h, int *labels) {
+  if (!h || !labels) return KMEANS_ERROR_INVALID_ARGUMENT;
+  return h->e.getLabels(labels);
+}
+
+kmeansErrorStatus kmeansGetStats(kmeansHandle h, kmeansStats *stats) {
+  if (!h || !stats) return KMEANS_ERROR_INVALID_ARGUMENT;
+  *stats = h->e.stats;
+  return KMEANS_SUCCESS;
+}
+
+const float *kmeansDeviceCenters(kmeansHandle h) { return h ? h->e.dC : nullptr; }
+const int *kmeansDeviceLabels(kmeansHandle h) { return h ? h->e.dlabels : nullptr; }
+
+kmeansErrorStatus kmeansSynchronize(kmeansHandle h) {
+  if (!h) return KMEANS_ERROR_INVALID_ARGUMENT;
+  KM_HIP_CHECK(hipStreamSynchronize(h->e.stream), KMEANS_ERROR_COMPUTE);
+  return KMEANS_SUCCESS;
+}
+
+kmeansErrorStatus kmeansSetMiniBatch(kmeansHandle h, long rows) {
+  if (!h || rows < 0 || rows > h->e.m) return KMEANS_ERROR_INVALID_ARGUMENT;
+  h->e.mb_rows = rows;
+  h->e.mb_index = 0;
+  return KMEANS_SUCCESS;
+}
+
+static kmeansErrorStatus compute_f(const float *data, int m, int n, int k, double criterion,
+                                   int maxIters, int initAlgorithm, float *result,
+                                   double miniBatchFraction) {
+  if (!data || !result) return KMEANS_ERROR_INVALID_ARGUMENT;
+  kmeansHandle h = nullptr;
+  kmeansErrorStatus st = kmeansCreate(&h, m, n, k, nullptr);
+  if (st != KMEANS_SUCCESS) return st;
+  std::vector<float> c((size_t)k * n);
+  if ((st = kmeansSetData(h, data, 0)) != KMEANS_SUCCESS) goto done;
+  if (initAlgorithm == 1) {
+    for (int j = 0; j < k; ++j)
+      for (int q = 0; q < n; ++q) c[(size_t)j * n + q] = result[(size_t)q * k + j];
+    st = kmeansSetCenters(h, c.data());
+  } else {
+    st = h->e.initialize(42);
+  }
+  if (st != KMEANS_SUCCESS) goto done;
+  if (miniBatchFraction < 1.0) {
+    long rows = (long)(miniBatchFraction * m);
+    if (rows < k) rows = std::min<long>(k, m);
+    kmeansSetMiniBatch(h, rows);
+  }
+  if ((st = kmeansRun(h, maxIters, criterion)) != KMEANS_SUCCESS) goto done;
+  if ((st = kmeansGetCenters(h, c.data())) != KMEANS_SUCCESS) goto done;
+  for (int j = 0; j < k; ++j)
+    for (int q = 0; q < n; ++q) result[(size_t)q * k + j] = c[(size_t)j * n + q];
+  st = h->e.getConvergenceFlag();
+done:
+  kmeansDestroy(h);
+  return st;
+}
+
+kmeansErrorStatus computeKmeansF(const float *data, int m, int n, int k, float criterion,
+                                 int maxIters, int numRetries, int initAlgorithm, int doFullSGEMM,
+                                 int matrixFormat, float *result, float miniBatchFraction) {
+  (void)numRetries;
+  (void)doFullSGEMM;
+  (void)matrixFormat;
+  return compute_f(data, m, n, k, criterion, maxIters, initAlgorithm, result, miniBatchFraction);
+}
+
+kmeansErrorStatus computeKmeansD(const double *data, int m, int n, int k, double criterion,
+                                 int maxIters, int numRetries, int initAlgorithm, int doFullSGEMM,
+                                 int matrixFormat, double *result, double miniBatchFraction) {
+  (void)data; (void)m; (void)n; (void)k; (void)criterion; (void)maxIters; (void)numRetries;
+  (void)initAlgorithm; (void)doFullSGEMM; (void)matrixFormat; (void)result;
+  (void)miniBatchFraction;
+  fprintf(stderr, "kmeans_amd: computeKmeansD: the FP64 engine is not built in this version\n");
+  return KMEANS_ERROR_COMPUTE;
+}
+
+// ------------------------------------------------------------ kernel API
+#define KM_K_CHECK(expr, code)                                                        \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess) {                                                           \
+      fprintf(stderr, "kmeans_amd: %s: %s\n", #expr, hipGetErrorString(e_));          \
+      return code;                                                                    \
+    }                                                                                 \
+  } while (0)
+
+kmeansCudaErrorStatus kmColNorms(void *stream, int k, int d, const float *C, float *nc) {
+  if (k <= 0 || d <= 0 || !C || !nc) return ERROR_INVALID_ARGUMENT;
+  KM_K_CHECK(km::launch_col_norms((hipStream_t)stream, k, d, C, nc), ERROR_COLNORMALIZE);
+  return NO_ERROR;
+}
+
+int kmCentersLd(int k) { return km::centers_ld(k); }
+
+kmeansCudaErrorStatus kmTransposeCenters(void *stream, int k, int d, const float *C, float *Ct) {
+  if (k <= 0 || d <= 0 || !C || !Ct) return ERROR_INVALID_ARGUMENT;
+  KM_K_CHECK(km::launch_transpose((hipStream_t)stream, k, d, C, Ct, km::centers_ld(k)),
+             ERROR_CLOSESTCENTERS);
+  return NO_ERROR;
+}
+
+kmeansCudaErrorStatus kmAssignF32(void *stream, long m, const int *m_dev, int d, const float *X,
+                                  const int *rows, int k, const float *Ct, const float *nc,
+                                  int *labels, float *best) {
+  if (m < 0 || d <= 0 || k <= 0 || !X || !Ct || !nc || !labels) return ERROR_INVALID_ARGUMENT;
+  if (m_dev && !rows) return ERROR_INVALID_ARGUMENT;
+  KM_K_CHECK(km::launch_assign_f32((hipStream_t)stream, m, m_dev, d, X, rows, k, Ct,
+                                   km::centers_ld(k), nc, labels, best),
+             ERROR_CLOSESTCENTERS);
+  return NO_ERROR;
+}
+
+size_t kmUpdateWorkspaceSize(long n, int d, int k) {
+  return km::update_ws_layout(n, d, k, nullptr, nullptr);
+}
+
+kmeansCudaErrorStatus kmUpdate(void *stream, long n, int d, int k, const float *X,
+                               const int *labels, const float *C_old, float *C_new, float *Ct_new,
+                               float *nc_new, long long *counts, int *empty, void *workspace,
+                               size_t workspace_bytes) {
+  if (n < 0 || d <= 0 || k <= 0 || !X || !labels || !C_old || !C_new || !workspace)
+    return ERROR_INVALID_ARGUMENT;
+  km::UpdateWs ws;
+  if (km::update_ws_layout(n, d, k, (char *)workspace, &ws) > workspace_bytes)
+    return ERROR_INVALID_ARGUMENT;
+  KM_K_CHECK(km::launch_update((hipStream_t)stream, n, d, k, X, labels, C_old, C_new, Ct_new,
+                               km::centers_ld(k), nc_new, counts, empty, ws),
+             ERROR_CLUSTERCENTERS);
+  return NO_ERROR;
+}
+
+kmeansCudaErrorStatus kmCompactness(void *stream, long n, int d, int k, const float *X,
+                                    const int *labels, const float *C, double *out,
+                                    void *workspace, size_t workspace_bytes) {
+  if (n < 0 || d <= 0 || k <= 0 || !X || !labels || !C || !out || !workspace)
+    return ERROR_INVALID_ARGUMENT;
+  km::UpdateWs ws;
+  if (km::update_ws_layout(n, d, k, (char *)workspace, &ws) > workspace_bytes)
+    return ERROR_INVALID_ARGUMENT;
+  KM_K_CHECK(km::launch_compactness((hipStream_t)stream, n, d, X, labels, C, out, ws),
+             ERROR_COMPACTNESS);
+  return NO_ERROR;
+}
+
+kmeansCudaErrorStatus kmRowNorms(void *stream, long m0, long m, int d, const float *X, float *out) {
+  if (m0 < 0 || m < 0 || d <= 0 || !X || !out) return ERROR_INVALID_ARGUMENT;
+  KM_K_CHECK(km::launch_row_norms((hipStream_t)stream, m, d, X, out + m0), ERROR_ROWNORMALIZE);
+  return NO_ERROR;
+}
+
+kmeansCudaErrorStatus kmGenerateMixture(void *stream, uint64_t seed, long i0, long n, int d,
+                                        int ncomp, const float *means, const float *sigmas,
+                                        const double *cdf, float *X, int *z) {
+  if (n < 0 || d <= 0 || ncomp <= 0 || !means || !sigmas || !cdf || !X) return ERROR_INVALID_ARGUMENT;
+  KM_K_CHECK(km::launch_generate((hipStream_t)stream, seed, i0, n, d, ncomp, means, sigmas, cdf, X,
+                                 z),
+             ERROR_INVALID_ARGUMENT);
+  return NO_ERROR;
+}
+
+kmeansCudaErrorStatus kmGatherRows(void *stream, long n, int d, const float *X, const int *idx,
+                                   float *out) {
+  if (n < 0 || d <= 0 || !X || !idx || !out) return ERROR_INVALID_ARGUMENT;
+  KM_K_CHECK(km::launch_gather_rows((hipStream_t)stream, n, d, X, idx, out), ERROR_INVALID_ARGUMENT);
+  return NO_ERROR;
+}
+
+// ------------------------------------------------------------ legacy names
+// ClosestCentersF (KmeansCudaKernels.h:238-244): A = nRowsA x nColsA tile,
+// B = nColsA x nColsB centres (D x K), labels to CindicesFinal[m0 + i].
+kmeansCudaErrorStatus ClosestCentersF(const int m0, const int nRowsA, const int nColsA,
+                                      const float *A, const int nColsB, const float *B,
+                                      float *Bpadded, const float *normRowsOfA_squared,
+                                      const float *normColsOfB_squared, const int nColsC,
+                                      float *C, int *Cindices, int *CindicesFinal) {
+  (void)Bpadded; (void)normRowsOfA_squared; (void)nColsC; (void)C; (void)Cindices;
+  if (nRowsA < 0 || nColsA <= 0 || nColsB <= 0 || !A || !B || !normColsOfB_squared ||
+      !CindicesFinal)
+    return ERROR_INVALID_ARGUMENT;
+  const int k = nColsB, d = nColsA, ld = km::centers_ld(k);
+  const float *Ct = B;
+  std::lock_guard<std::mutex> g(km::g_legacy_mu);
+  if (ld != k) {  // pad the D x K centres to the kernel's leading dimension
+    const size_t need = sizeof(float) * (size_t)ld * d;
+    if (need > km::g_legacy_ct_bytes) {
+      hipFree(km::g_legacy_ct);
+      km::g_legacy_ct = nullptr;
+      km::g_legacy_ct_bytes = 0;
+      KM_K_CHECK(hipMalloc(&km::g_legacy_ct, need), ERROR_CLOSESTCENTERS);
+      km::g_legacy_ct_bytes = need;
+    }
+    KM_K_CHECK(hipMemset(km::g_legacy_ct, 0, need), ERROR_CLOSESTCENTERS);
+    KM_K_CHECK(hipMemcpy2D(km::g_legacy_ct, sizeof(float) * ld, B, sizeof(float) * k,
+                           sizeof(float) * k, d, hipMemcpyDeviceToDevice),
+               ERROR_CLOSESTCENTERS);
+    Ct = km::g_legacy_ct;
+  }
+  KM_K_CHECK(km::launch_assign_f32(nullptr, nRowsA, nullptr, d, A, nullptr, k, Ct, ld,
+                                   normColsOfB_squared, CindicesFinal + m0, nullptr),
+             ERROR_CLOSESTCENTERS);
+  return NO_ERROR;
+}
+
+// ClusterCentersF (KmeansCudaKernels.h:337-343): accumulates the tile into
+// centers_large (reused as a k x n FP64 accumulator; the caller zeroes it per
+// iteration as kmeans.cpp:1152-1156 does) and counts_large (k int64); on
+// lastTile writes centers (k x n) and counts.  Canonical blocks start at
+// each tile's first row.
+kmeansCudaErrorStatus ClusterCentersF(const int m0, const int m, const int n, const int k,
+                                      const bool lastTile, const float *data, const int *indices,
+                                      float *centers_large, int *counts_large, float *centers,
+                                      int *counts) {
+  if (m < 0 || n <= 0 || k <= 0 || !data || !indices || !centers_large || !counts_large ||
+      !centers)
+    return ERROR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> g(km::g_legacy_mu);
+  km::UpdateWs ws;
+  KM_K_CHECK(km::legacy_ws(m, n, k, &ws), ERROR_CLUSTERCENTERS);
+  double *S = (double *)centers_large;
+  long long *cnt = (long long *)counts_large;
+  KM_K_CHECK(km::launch_update_accumulate(nullptr, m, n, k, data, indices + m0, S, cnt, ws),
+             ERROR_CLUSTERCENTERS);
+  if (lastTile) {
+    KM_K_CHECK(km::launch_update_finalize(nullptr, n, k, S, cnt, centers, centers, nullptr, 0,
+                                          nullptr, ws.cnt, nullptr),
+               ERROR_CLUSTERCENTERS);
+    if (counts) {
+      std::vector<long long> c(k);
+      std::vector<int> ci(k);
+      KM_K_CHECK(hipMemcpy(c.data(), ws.cnt, sizeof(long long) * k, hipMemcpyDeviceToHost),
+                 ERROR_CLUSTERCENTERS);
+      for (int j = 0; j < k; ++j) ci[j] = (int)c[j];
+      KM_K_CHECK(hipMemcpy(counts, ci.data(), sizeof(int) * k, hipMemcpyHostToDevice),
+                 ERROR_CLUSTERCENTERS);
+    }
+  }
+  KM_K_CHECK(hipDeviceSynchronize(), ERROR_CLUSTERCENTERS);
+  return NO_ERROR;
+}
+
+// CompactnessF (KmeansCudaKernels.h:389-394): compactness[2..3] (bytes 8-15)
+// hold the FP64 running total across tiles; on lastTile compactness[0] =
+// (float)total.  The caller zeroes the buffer per iteration.
+kmeansCudaErrorStatus CompactnessF(const int m0, const int m, const bool lastTile, const int n,
+                                   const int k, const float *data, const int *indices,
+                                   const float *centers, float *compactness) {
+  if (m < 0 || n <= 0 || k <= 0 || !data || !indices || !centers || !compactness)
+    return ERROR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> g(km::g_legacy_mu);
+  km::UpdateWs ws;
+  KM_K_CHECK(km::legacy_ws(m, n, k, &ws), ERROR_COMPACTNESS);
+  double *acc = (double *)(compactness + 2);
+  KM_K_CHECK(km::launch_compactness_acc(nullptr, m, n, data, indices + m0, centers, acc, ws, 0),
+             ERROR_COMPACTNESS);
+  if (lastTile) {
+    double tot = 0.0;
+    KM_K_CHECK(hipMemcpy(&tot, acc, sizeof(double), hipMemcpyDeviceToHost), ERROR_COMPACTNESS);
+    const float f = (float)tot;
+    KM_K_CHECK(hipMemcpy(compactness, &f, sizeof(float), hipMemcpyHostToDevice), ERROR_COMPACTNESS);
+  }
+  KM_K_CHECK(hipDeviceSynchronize(), ERROR_COMPACTNESS);
+  return NO_ERROR;
+}
+
+kmeansCudaErrorStatus rowNormalizeF(const int m0, const int m, const int n, const float *in,
+                                    float *out) {
+  if (m < 0 || n <= 0 || !in || !out) return ERROR_INVALID_ARGUMENT;
+  KM_K_CHECK(km::launch_row_norms(nullptr, m, n, in, out + m0), ERROR_ROWNORMALIZE);
+  return NO_ERROR;
+}
+
+// colNormalizeF (KmeansCudaKernels.h:458-463): in is m x n = D x K.
+kmeansCudaErrorStatus colNormalizeF(const int m, const int n, const float *in, float *out) {
+  if (m <= 0 || n <= 0 || !in || !out) return ERROR_INVALID_ARGUMENT;
+  KM_K_CHECK(km::launch_col_norms_dk(nullptr, n, m, n, in, out), ERROR_COLNORMALIZE);
+  return NO_ERROR;
+}
+
+}  // extern "C"

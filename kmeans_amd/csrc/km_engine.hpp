@@ -1,0 +1,82 @@
+// Host engine: the Lloyd loop over the gfx950 kernels.
+//
+// Mirrors the reference's host class kmeans<TYPE> (main/cpp/kmeans.hpp:
+// 174-658): the same public method names (buildData, initialize, compute,
+// computeStandard, computeMiniBatch, closestCenters, clusterCenters,
+// compactness, transpose, reset, computeTiling, copyTileToDevice,
+// constructMiniBatch, getClusterCenters, getConvergenceFlag), but every step
+// is stream-ordered with no per-phase host synchronisation, no __constant__
+// state, and labels / compactness are returned.
+#pragma once
+#include <vector>
+
+#include "km_common.hpp"
+
+namespace km {
+
+struct Engine {
+  // problem
+  long m = 0;
+  int n = 0, k = 0;
+  kmeansOptions opt{};
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int ldc = 0;
+
+  // device buffers
+  const float *dX = nullptr;
+  float *dX_owned = nullptr;
+  float *dC = nullptr;      // k x n
+  float *dCt = nullptr;     // n x ldc
+  float *dnc = nullptr;     // k
+  int *dlabels = nullptr;   // m
+  long long *dcounts = nullptr;
+  int *dempty = nullptr;
+  double *dcomp = nullptr;  // one per iteration slot
+  char *dws = nullptr;
+  UpdateWs ws{};
+  // precision path buffers (allocated on first use)
+  void *mfma = nullptr;
+
+  // loop state (kmeans.cpp:1194,1235-1237)
+  double criterion = -1.0;
+  int maxIters = 0;
+  double compactnessScore = 0.0, relErr = 1.0;
+  int iters = 0;
+  bool centers_set = false;
+  int comp_slots = 0;
+  kmeansStats stats{};
+
+  // mini-batch (kmeans.cpp:1253-1312)
+  long mb_rows = 0;   // 0: standard loop
+  long mb_index = 0;
+
+  hipEvent_t ev[8]{};
+  std::vector<hipEvent_t> iter_events;  // 5 per iteration when opt.timing
+
+  ~Engine();
+  kmeansErrorStatus create(long m_, int n_, int k_, const kmeansOptions *o);
+  kmeansErrorStatus buildData(const float *data, int on_device);
+  kmeansErrorStatus setCenters(const float *centers_kxn_host);
+  kmeansErrorStatus initialize(uint64_t seed);  // k distinct sampled rows
+  kmeansErrorStatus reset();
+  kmeansErrorStatus closestCenters(long r0, long rows);
+  kmeansErrorStatus clusterCenters(long r0, long rows);
+  kmeansErrorStatus compactness(long r0, long rows, int slot);
+  kmeansErrorStatus transpose();
+  kmeansErrorStatus constructMiniBatch(long index, long *r0, long *rows);
+  kmeansErrorStatus compute();
+  kmeansErrorStatus computeStandard();
+  kmeansErrorStatus computeMiniBatch();
+  kmeansErrorStatus getClusterCenters(float *out_kxn);
+  kmeansErrorStatus getLabels(int *out);
+  kmeansErrorStatus getConvergenceFlag() const {
+    return relErr <= criterion ? KMEANS_SUCCESS : KMEANS_NONCONVERGENCE;
+  }
+  kmeansErrorStatus ensure_comp_slots(int n);
+  kmeansErrorStatus run_iterations(long r0, long rows, bool minibatch);
+  void record(int slot, int phase);
+  void collect_timing(int nslots);
+};
+
+}  // namespace km

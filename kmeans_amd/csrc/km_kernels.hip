@@ -1,0 +1,727 @@
+// Plain FP32 path for gfx950: fused score+argmax, canonical FP64 update,
+// compactness, norms, synthetic-data generator.
+//
+// Reference counterparts (paths relative to the reference repo):
+//   k_assign_f32        main/cuda/ClosestCenters.cu:46-123 (+ multiply.hcu,
+//                       dotproducts.hcu, fma.hcu, reduction.hcu, vectorMin.hcu)
+//   update kernels      main/cuda/ClusterCenters.cu:3-132
+//   compactness         main/cuda/Compactness.cu:3-96
+//   k_col_norms         main/cuda/Normalize.cu:33-47
+//   k_row_norms         main/cuda/Normalize.cu:3-31
+#include <math.h>
+
+#include "km_common.hpp"
+
+namespace km {
+
+int centers_ld(int k) { return (int)(km_cdiv(k, 256) * 256); }
+
+int num_cus() {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+      cus = 256;
+  }
+  return cus;
+}
+
+// ---------------------------------------------------------------- norms
+// nc[j] = fma chain over d of C[j][d]^2 (Normalize.cu:41-44, contraction explicit).
+__global__ void k_col_norms(int k, int d, const float *__restrict__ C, float *__restrict__ nc) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= k) return;
+  const float *c = C + (long)j * d;
+  float s = 0.0f;
+  for (int q = 0; q < d; ++q) s = __builtin_fmaf(c[q], c[q], s);
+  nc[j] = s;
+}
+
+__global__ void k_row_norms(long m, int d, const float *__restrict__ X, float *__restrict__ out) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const float *x = X + i * d;
+  float s = 0.0f;
+  for (int q = 0; q < d; ++q) s = __builtin_fmaf(x[q], x[q], s);
+  out[i] = s;
+}
+
+// Ct[q][j] = C[j][q], zero for j >= k (Ct pre-cleared by the caller once).
+__global__ void k_transpose(int k, int d, const float *__restrict__ C, float *__restrict__ Ct,
+                            int ldc) {
+  __shared__ float t[32][33];
+  int j0 = blockIdx.x * 32, q0 = blockIdx.y * 32;
+  int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int r = ty; r < 32; r += 8) {
+    int j = j0 + r, q = q0 + tx;
+    t[r][tx] = (j < k && q < d) ? C[(long)j * d + q] : 0.0f;
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    int q = q0 + r, j = j0 + tx;
+    if (q < d && j < ldc) Ct[(long)q * ldc + j] = t[tx][r];
+  }
+}
+
+// ---------------------------------------------------------------- K1 assign
+// One workgroup (4 waves) owns a tile of 64 points and sweeps all centres in
+// tiles of 256, d-slices of 16 staged through LDS with register prefetch.
+// Thread (pg, cg) accumulates 8 points x 8 centres; every score is a single
+// fma chain over d in ascending order starting from -0.5f*nc[j].
+#define A_BM 64
+#define A_BN 256
+#define A_DS 16
+
+__global__ __launch_bounds__(256) void k_assign_f32(long M, const int *__restrict__ Mdev, int D,
+                                                    const float *__restrict__ X,
+                                                    const int *__restrict__ rows, int K,
+                                                    const float *__restrict__ Ct, int ldc,
+                                                    const float *__restrict__ nc,
+                                                    int *__restrict__ labels,
+                                                    float *__restrict__ best) {
+  __shared__ __attribute__((aligned(16))) float Xs[A_DS][A_BM + 4];
+  __shared__ float4 Cs[A_DS][A_BN / 4];
+  if (Mdev) M = *Mdev;
+  const long ntiles = (M + A_BM - 1) / A_BM;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int cg = lane & 31, pg = (lane >> 5) + 2 * w;
+  const int nS = (D + A_DS - 1) / A_DS;
+  const int nKT = (K + A_BN - 1) / A_BN;
+  const int total = nS * nKT;
+  const int xdd = tid & 15;
+
+  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const long r0 = tile * A_BM;
+    long xrow[4];
+    bool xok[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      long p = r0 + (tid >> 4) + 16 * q;
+      xok[q] = p < M;
+      xrow[q] = xok[q] ? (rows ? (long)rows[p] : p) : 0;
+    }
+    float bv[8];
+    int bi[8];
+#pragma unroll
+    for (int p = 0; p < 8; ++p) { bv[p] = -INFINITY; bi[p] = 0; }
+
+    float xr[4];
+    float4 cr[4];
+    auto load = [&](int it) {
+      const int kt = it / nS, s = it - kt * nS;
+      const int d0 = s * A_DS;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        xr[q] = (xok[q] && d0 + xdd < D) ? X[xrow[q] * D + d0 + xdd] : 0.0f;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int e = tid + 256 * q, dd = e >> 6, c4 = e & 63;
+        cr[q] = (d0 + dd < D)
+                    ? *reinterpret_cast<const float4 *>(Ct + (long)(d0 + dd) * ldc + kt * A_BN + c4 * 4)
+                    : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+    };
+
+    float acc[8][8];
+    load(0);
+    for (int it = 0; it < total; ++it) {
+      const int kt = it / nS, s = it - kt * nS;
+      if (s == 0) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int j = kt * A_BN + (q < 4 ? cg * 4 + q : 128 + cg * 4 + q - 4);
+          const float v = j < K ? -0.5f * nc[j] : -INFINITY;
+#pragma unroll
+          for (int p = 0; p < 8; ++p) acc[p][q] = v;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 4; ++q) Xs[xdd][(tid >> 4) + 16 * q] = xr[q];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int e = tid + 256 * q;
+        Cs[e >> 6][e & 63] = cr[q];
+      }
+      __syncthreads();
+      if (it + 1 < total) load(it + 1);
+      const int dn = min(A_DS, D - s * A_DS);
+      auto step = [&](int dd) {
+        const float4 xa = *reinterpret_cast<const float4 *>(&Xs[dd][pg * 8]);
+        const float4 xb = *reinterpret_cast<const float4 *>(&Xs[dd][pg * 8 + 4]);
+        const float4 ca = Cs[dd][cg], cb = Cs[dd][32 + cg];
+        const float xv[8] = {xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w};
+        const float cv[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
+#pragma unroll
+        for (int p = 0; p < 8; ++p)
+#pragma unroll
+          for (int q = 0; q < 8; ++q) acc[p][q] = __builtin_fmaf(xv[p], cv[q], acc[p][q]);
+      };
+      if (dn == A_DS) {
+#pragma unroll
+        for (int dd = 0; dd < A_DS; ++dd) step(dd);
+      } else {
+        for (int dd = 0; dd < dn; ++dd) step(dd);
+      }
+      if (s == nS - 1) {
+#pragma unroll
+        for (int p = 0; p < 8; ++p)
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const int j = kt * A_BN + (q < 4 ? cg * 4 + q : 128 + cg * 4 + q - 4);
+            if (acc[p][q] > bv[p]) { bv[p] = acc[p][q]; bi[p] = j; }
+          }
+      }
+    }
+    // lanes of one half-wave share the 8 points; merge with the tie rule
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+#pragma unroll
+      for (int off = 16; off >= 1; off >>= 1) {
+        const float ov = __shfl_xor(bv[p], off);
+        const int oi = __shfl_xor(bi[p], off);
+        if (ov > bv[p] || (ov == bv[p] && oi < bi[p])) { bv[p] = ov; bi[p] = oi; }
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      const long pt = r0 + pg * 8 + p;
+      if (cg == p && pt < M) {
+        const long o = rows ? (long)rows[pt] : pt;
+        labels[o] = bi[p];
+        if (best) best[o] = bv[p];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- update
+// Sub-block label histogram, label-major: hist[j*nsb + s].
+__global__ void k_hist(const int *__restrict__ labels, long nb, int K, int nsb, int SB,
+                       int *__restrict__ hist) {
+  extern __shared__ int h[];
+  for (int j = threadIdx.x; j < K; j += blockDim.x) h[j] = 0;
+  __syncthreads();
+  const int s = blockIdx.x;
+  const long a = (long)s * SB, e = min(a + SB, nb);
+  for (long i = a + threadIdx.x; i < e; i += blockDim.x) atomicAdd(&h[labels[i]], 1);
+  __syncthreads();
+  for (int j = threadIdx.x; j < K; j += blockDim.x) hist[(long)j * nsb + s] = h[j];
+}
+
+// Block-wide exclusive scan of one int per thread (256 threads).
+__device__ inline int block_excl_scan256(int v, int *lds, int *total) {
+  const int tid = threadIdx.x;
+  lds[tid] = v;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    int t = tid >= off ? lds[tid - off] : 0;
+    __syncthreads();
+    lds[tid] += t;
+    __syncthreads();
+  }
+  int incl = lds[tid];
+  if (total) *total = lds[255];
+  __syncthreads();
+  return incl - v;
+}
+
+#define SCAN_TILE 4096
+__global__ __launch_bounds__(256) void k_scan_reduce(const int *__restrict__ in, long L,
+                                                     int *__restrict__ tsum) {
+  __shared__ int lds[256];
+  const long base = (long)blockIdx.x * SCAN_TILE + threadIdx.x * 16;
+  int s = 0;
+  for (int q = 0; q < 16; ++q) s += (base + q < L) ? in[base + q] : 0;
+  int tot;
+  block_excl_scan256(s, lds, &tot);
+  if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(256) void k_scan_tiles(int *__restrict__ tsum, int T) {
+  __shared__ int lds[256];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int b = 0; b < T; b += 256) {
+    int v = (b + threadIdx.x < T) ? tsum[b + threadIdx.x] : 0;
+    int tot;
+    int ex = block_excl_scan256(v, lds, &tot);
+    if (b + threadIdx.x < T) tsum[b + threadIdx.x] = carry + ex;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += tot;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scan_down(const int *__restrict__ in, long L,
+                                                   const int *__restrict__ tsum,
+                                                   int *__restrict__ out) {
+  __shared__ int lds[256];
+  const long base = (long)blockIdx.x * SCAN_TILE + threadIdx.x * 16;
+  int v[16];
+  int s = 0;
+  for (int q = 0; q < 16; ++q) { v[q] = (base + q < L) ? in[base + q] : 0; s += v[q]; }
+  int tot;
+  int ex = block_excl_scan256(s, lds, &tot) + tsum[blockIdx.x];
+  for (int q = 0; q < 16; ++q) {
+    if (base + q < L) out[base + q] = ex;
+    ex += v[q];
+  }
+  if (base <= L - 1 && base + 16 >= L) out[L] = ex;  // total after the last element
+}
+
+// Stable scatter: one wave per sub-block walks its points in order; lanes
+// with equal labels are ranked by a ballot match over the label bits.
+__global__ __launch_bounds__(64) void k_scatter(const int *__restrict__ labels, long i0, long nb,
+                                                int K, int nsb, int SB, int nbits,
+                                                const int *__restrict__ offs,
+                                                int *__restrict__ perm) {
+  extern __shared__ int cnt[];
+  for (int j = threadIdx.x; j < K; j += 64) cnt[j] = 0;
+  __syncthreads();
+  const int s = blockIdx.x, lane = threadIdx.x;
+  const long a = (long)s * SB, e = min(a + SB, nb);
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (long base = a; base < e; base += 64) {
+    const long i = base + lane;
+    const bool ok = i < e;
+    const int l = ok ? labels[i] : 0;
+    uint64_t peers = __ballot(ok);
+    for (int b = 0; b < nbits; ++b) {
+      const bool bit = (l >> b) & 1;
+      const uint64_t m = __ballot(bit);
+      peers &= bit ? m : ~m;
+    }
+    int c = 0;
+    if (ok) {
+      c = cnt[l];
+      const int rank = __popcll(peers & below);
+      perm[offs[(long)l * nsb + s] + c + rank] = (int)(i0 + i);
+    }
+    __syncthreads();
+    if (ok && (peers >> lane) == 1ull) cnt[l] = c + __popcll(peers);
+    __syncthreads();
+  }
+}
+
+// Per-centre member range in the sorted order and chunk prefix (one block).
+__global__ __launch_bounds__(256) void k_cluster_setup(int K, int nsb, const int *__restrict__ offs,
+                                                       int *__restrict__ cstart,
+                                                       int *__restrict__ ccount,
+                                                       int *__restrict__ cq) {
+  __shared__ int lds[256];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int b = 0; b < K; b += 256) {
+    const int j = b + threadIdx.x;
+    int q = 0;
+    if (j < K) {
+      const int st = offs[(long)j * nsb], en = offs[(long)(j + 1) * nsb];
+      cstart[j] = st;
+      ccount[j] = en - st;
+      q = (en - st + KM_CHUNK - 1) / KM_CHUNK;
+    }
+    int tot;
+    int ex = block_excl_scan256(q, lds, &tot);
+    if (j < K) cq[j] = carry + ex;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) cq[K] = carry;
+}
+
+// First-level sums: one group of threads per chunk of <= KM_CHUNK members,
+// one thread per dimension, members added in ascending point order.
+__global__ __launch_bounds__(256) void k_chunk_sums(const float *__restrict__ X, int D, int Dp,
+                                                    const int *__restrict__ perm, int K,
+                                                    const int *__restrict__ cstart,
+                                                    const int *__restrict__ ccount,
+                                                    const int *__restrict__ cq,
+                                                    double *__restrict__ P) {
+  __shared__ int pidx[4][KM_CHUNK];
+  const int G = 256 / Dp;  // chunks per block (>= 1)
+  const int sub = threadIdx.x / Dp, t = threadIdx.x - sub * Dp;
+  const int Q = cq[K];
+  const long c = (long)blockIdx.x * G + sub;
+  int j = 0, cnt = 0, start = 0;
+  if (c < Q && sub < G) {
+    int lo = 0, hi = K - 1;  // last j with cq[j] <= c
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (cq[mid] <= c) lo = mid; else hi = mid - 1;
+    }
+    j = lo;
+    const int local = (int)(c - cq[j]);
+    start = cstart[j] + local * KM_CHUNK;
+    cnt = min(KM_CHUNK, ccount[j] - local * KM_CHUNK);
+  }
+  // stage member indices (G <= 4 when Dp >= 64; otherwise loop)
+  const bool staged = G <= 4;
+  if (staged && c < Q && sub < G)
+    for (int m = t; m < cnt; m += Dp) pidx[sub][m] = perm[start + m];
+  __syncthreads();
+  if (c >= Q || sub >= G) return;
+  for (int d = t; d < D; d += Dp) {
+    double acc = 0.0;
+    int m = 0;
+    for (; m + 4 <= cnt; m += 4) {
+      const int p0 = staged ? pidx[sub][m] : perm[start + m];
+      const int p1 = staged ? pidx[sub][m + 1] : perm[start + m + 1];
+      const int p2 = staged ? pidx[sub][m + 2] : perm[start + m + 2];
+      const int p3 = staged ? pidx[sub][m + 3] : perm[start + m + 3];
+      const float v0 = X[(long)p0 * D + d], v1 = X[(long)p1 * D + d];
+      const float v2 = X[(long)p2 * D + d], v3 = X[(long)p3 * D + d];
+      acc += (double)v0;
+      acc += (double)v1;
+      acc += (double)v2;
+      acc += (double)v3;
+    }
+    for (; m < cnt; ++m) acc += (double)X[(long)(staged ? pidx[sub][m] : perm[start + m]) * D + d];
+    P[c * D + d] = acc;
+  }
+}
+
+// Per-centre CSUM over its chunk partials, added into the running block sum.
+__global__ __launch_bounds__(256) void k_block_finalize(int D, int K, const int *__restrict__ ccount,
+                                                        const int *__restrict__ cq,
+                                                        const double *__restrict__ P,
+                                                        double *__restrict__ S,
+                                                        long long *__restrict__ cnt) {
+  const int j = blockIdx.x;
+  const int q0 = cq[j], nq = cq[j + 1] - q0;
+  for (int d = threadIdx.x; d < D; d += blockDim.x) {
+    double b = 0.0;
+    if (nq <= KM_CHUNK) {
+      for (int t = 0; t < nq; ++t) b += P[(long)(q0 + t) * D + d];
+    } else {  // nq <= KM_CHUNK^2 since KM_BLOCK <= KM_CHUNK^3
+      for (int g = 0; g < nq; g += KM_CHUNK) {
+        double gs = 0.0;
+        const int ge = min(nq, g + KM_CHUNK);
+        for (int t = g; t < ge; ++t) gs += P[(long)(q0 + t) * D + d];
+        b += gs;
+      }
+    }
+    S[(long)j * D + d] += b;
+  }
+  if (threadIdx.x == 0) cnt[j] += ccount[j];
+}
+
+__global__ void k_zero_sums(long n, double *__restrict__ S, int K, long long *__restrict__ cnt) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) S[i] = 0.0;
+  if (i < K) cnt[i] = 0;
+}
+
+// New centres, transposed copy and norms; empty centres keep the old value.
+__global__ __launch_bounds__(256) void k_final_centres(int D, int K, const double *__restrict__ S,
+                                                       const long long *__restrict__ cnt,
+                                                       const float *C_old, float *C_new,
+                                                       float *__restrict__ Ct, int ldc,
+                                                       float *__restrict__ nc,
+                                                       long long *__restrict__ counts,
+                                                       int *__restrict__ empty) {
+  extern __shared__ float cs[];
+  const int j = blockIdx.x;
+  const long long n = cnt[j];
+  for (int d = threadIdx.x; d < D; d += blockDim.x) {
+    const float v = n > 0 ? (float)(S[(long)j * D + d] / (double)n) : C_old[(long)j * D + d];
+    cs[d] = v;
+    C_new[(long)j * D + d] = v;
+    if (Ct) Ct[(long)d * ldc + j] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (nc) {
+      float s = 0.0f;
+      for (int d = 0; d < D; ++d) s = __builtin_fmaf(cs[d], cs[d], s);
+      nc[j] = s;
+    }
+    if (counts) counts[j] = n;
+    if (empty && n == 0) atomicAdd(empty, 1);
+  }
+}
+
+// ---------------------------------------------------------------- compactness
+// e_i = fma chain over d of (x_id - c_{l_i,d})^2; one chunk of KM_CHUNK points
+// per block, summed left to right in FP64 by one thread.
+__global__ __launch_bounds__(256) void k_compact_chunks(long i0, long nb, int D,
+                                                        const float *__restrict__ X,
+                                                        const int *__restrict__ labels,
+                                                        const float *__restrict__ C,
+                                                        double *__restrict__ E) {
+  __shared__ float ev[KM_CHUNK];
+  const long c = blockIdx.x;
+  const long r = c * KM_CHUNK + threadIdx.x;
+  float e = 0.0f;
+  if (r < nb) {
+    const long i = i0 + r;
+    const float *x = X + i * D;
+    const float *cc = C + (long)labels[i] * D;
+    for (int q = 0; q < D; ++q) {
+      const float v = x[q] - cc[q];
+      e = __builtin_fmaf(v, v, e);
+    }
+  }
+  ev[threadIdx.x] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int cnt = (int)min((long)KM_CHUNK, nb - c * KM_CHUNK);
+    double s = 0.0;
+    for (int t = 0; t < cnt; ++t) s += (double)ev[t];
+    E[c] = s;
+  }
+}
+
+// CSUM of the block's chunk partials (<= KM_BLOCK/KM_CHUNK of them), added to out.
+__global__ __launch_bounds__(256) void k_compact_block(const double *__restrict__ E, int nq,
+                                                       double *__restrict__ out, int first) {
+  __shared__ double g[KM_CHUNK];
+  const int ng = (nq + KM_CHUNK - 1) / KM_CHUNK;
+  if (nq <= KM_CHUNK) {
+    if (threadIdx.x == 0) {
+      double s = 0.0;
+      for (int t = 0; t < nq; ++t) s += E[t];
+      *out = first ? s : *out + s;
+    }
+    return;
+  }
+  for (int q = threadIdx.x; q < ng; q += blockDim.x) {
+    double s = 0.0;
+    const int e = min(nq, (q + 1) * KM_CHUNK);
+    for (int t = q * KM_CHUNK; t < e; ++t) s += E[t];
+    g[q] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int q = 0; q < ng; ++q) s += g[q];
+    *out = first ? s : *out + s;
+  }
+}
+
+// ---------------------------------------------------------------- generator
+__global__ void k_generate(uint64_t seed, long i0, long n, int d, int ncomp,
+                           const float *__restrict__ means, const float *__restrict__ sigmas,
+                           const double *__restrict__ cdf, float *__restrict__ X,
+                           int *__restrict__ z) {
+  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const long i = i0 + r;
+  const double u = rng_unif(seed, 2, (uint64_t)i);
+  int lo = 0, hi = ncomp - 1;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (u <= cdf[mid]) hi = mid; else lo = mid + 1;
+  }
+  if (z) z[r] = lo;
+  const float sg = sigmas[lo];
+  for (int q = 0; q < d; ++q) {
+    const uint64_t k = (uint64_t)i * (uint64_t)d + (uint64_t)q;
+    const double u1 = rng_unif(seed, 3, 2 * k), u2 = rng_unif(seed, 3, 2 * k + 1);
+    const double g = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+    X[r * d + q] = (float)((double)means[(long)lo * d + q] + (double)sg * g);
+  }
+}
+
+__global__ void k_gather_rows(long n, int d, const float *__restrict__ X,
+                              const int *__restrict__ idx, float *__restrict__ out) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * d) return;
+  const long r = e / d;
+  const int q = (int)(e - r * d);
+  out[e] = X[(long)idx[r] * d + q];
+}
+
+// ---------------------------------------------------------------- launchers
+hipError_t launch_col_norms(hipStream_t s, int k, int d, const float *C, float *nc) {
+  hipLaunchKernelGGL(k_col_norms, dim3(km_cdiv(k, 256)), dim3(256), 0, s, k, d, C, nc);
+  return hipGetLastError();
+}
+
+hipError_t launch_row_norms(hipStream_t s, long m, int d, const float *X, float *out) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_row_norms, dim3(km_cdiv(m, 256)), dim3(256), 0, s, m, d, X, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_transpose(hipStream_t s, int k, int d, const float *C, float *Ct, int ldc) {
+  dim3 grid(km_cdiv(ldc, 32), km_cdiv(d, 32));
+  hipLaunchKernelGGL(k_transpose, grid, dim3(256), 0, s, k, d, C, Ct, ldc);
+  return hipGetLastError();
+}
+
+hipError_t launch_assign_f32(hipStream_t s, long m, const int *m_dev, int d, const float *X,
+                             const int *rows, int k, const float *Ct, int ldc, const float *nc,
+                             int *labels, float *best) {
+  long tiles = km_cdiv(m, A_BM);
+  if (tiles <= 0) return hipSuccess;
+  long grid = tiles < (long)num_cus() * 4 ? tiles : (long)num_cus() * 4;
+  hipLaunchKernelGGL(k_assign_f32, dim3(grid), dim3(256), 0, s, m, m_dev, d, X, rows, k, Ct, ldc,
+                     nc, labels, best);
+  return hipGetLastError();
+}
+
+int sub_block(int k) { return k <= 2048 ? KM_SB_SMALL : KM_SB_LARGE; }
+
+size_t update_ws_layout(long n, int d, int k, char *base, UpdateWs *ws) {
+  const long nb = n < KM_BLOCK ? n : KM_BLOCK;
+  const int SB = sub_block(k);
+  const long nsb = km_cdiv(nb > 0 ? nb : 1, SB);
+  const long L = (long)k * nsb;
+  const long T = km_cdiv(L + 1, SCAN_TILE);
+  const long qmax = nb / KM_CHUNK + k + 1;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) & ~(size_t)255;
+    return base ? (void *)(base + o) : nullptr;
+  };
+  UpdateWs w;
+  w.hist = (int *)take(sizeof(int) * L);
+  w.offs = (int *)take(sizeof(int) * (L + 1));
+  w.tsum = (int *)take(sizeof(int) * (T + 1));
+  w.cstart = (int *)take(sizeof(int) * k);
+  w.ccount = (int *)take(sizeof(int) * k);
+  w.cq = (int *)take(sizeof(int) * (k + 1));
+  w.perm = (int *)take(sizeof(int) * (nb > 0 ? nb : 1));
+  w.P = (double *)take(sizeof(double) * qmax * d);
+  w.S = (double *)take(sizeof(double) * (long)k * d);
+  w.cnt = (long long *)take(sizeof(long long) * k);
+  w.E = (double *)take(sizeof(double) * (km_cdiv(nb > 0 ? nb : 1, KM_CHUNK) + 1));
+  w.bytes = off;
+  if (ws) *ws = w;
+  return off;
+}
+
+static hipError_t scan_excl(hipStream_t s, const int *in, long L, int *tsum, int *out) {
+  const long T = km_cdiv(L + 1, SCAN_TILE);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(T), dim3(256), 0, s, in, L, tsum);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(256), 0, s, tsum, (int)T);
+  hipLaunchKernelGGL(k_scan_down, dim3(T), dim3(256), 0, s, in, L, tsum, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_zero(hipStream_t s, int d, int k, double *S, long long *cnt, int *empty) {
+  const long zn = (long)k * d;
+  const long zmax = zn > k ? zn : k;
+  hipLaunchKernelGGL(k_zero_sums, dim3(km_cdiv(zmax, 256)), dim3(256), 0, s, zn, S, k, cnt);
+  if (empty) {
+    hipError_t e = hipMemsetAsync(empty, 0, sizeof(int), s);
+    if (e != hipSuccess) return e;
+  }
+  return hipGetLastError();
+}
+
+// Adds the canonical block sums of rows [0, n) of X (labels aligned with X)
+// into S (k x d FP64) and cnt; blocks start at row 0 of X.
+hipError_t launch_update_accumulate(hipStream_t s, long n, int d, int k, const float *X,
+                                    const int *labels, double *S, long long *cnt,
+                                    const UpdateWs &ws) {
+  hipError_t e;
+  const int SB = sub_block(k);
+  int nbits = 0;
+  while ((1 << nbits) < k) ++nbits;
+  int Dp = 1;
+  while (Dp < d && Dp < 256) Dp <<= 1;
+  const int G = 256 / Dp;
+  for (long i0 = 0; i0 < n; i0 += KM_BLOCK) {
+    const long nb = (n - i0) < KM_BLOCK ? (n - i0) : KM_BLOCK;
+    const int nsb = (int)km_cdiv(nb, SB);
+    const long L = (long)k * nsb;
+    hipLaunchKernelGGL(k_hist, dim3(nsb), dim3(256), sizeof(int) * k, s, labels + i0, nb, k, nsb,
+                       SB, ws.hist);
+    if ((e = scan_excl(s, ws.hist, L, ws.tsum, ws.offs)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_scatter, dim3(nsb), dim3(64), sizeof(int) * k, s, labels + i0, i0, nb, k,
+                       nsb, SB, nbits, ws.offs, ws.perm);
+    hipLaunchKernelGGL(k_cluster_setup, dim3(1), dim3(256), 0, s, k, nsb, ws.offs, ws.cstart,
+                       ws.ccount, ws.cq);
+    const long qmax = nb / KM_CHUNK + k + 1;
+    hipLaunchKernelGGL(k_chunk_sums, dim3(km_cdiv(qmax, G)), dim3(256), 0, s, X, d, Dp, ws.perm, k,
+                       ws.cstart, ws.ccount, ws.cq, ws.P);
+    hipLaunchKernelGGL(k_block_finalize, dim3(k), dim3(d < 256 ? 64 * (int)km_cdiv(d, 64) : 256), 0,
+                       s, d, k, ws.ccount, ws.cq, ws.P, S, cnt);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_update_finalize(hipStream_t s, int d, int k, const double *S,
+                                  const long long *cnt, const float *C_old, float *C_new,
+                                  float *Ct_new, int ldc, float *nc_new, long long *counts,
+                                  int *empty) {
+  hipLaunchKernelGGL(k_final_centres, dim3(k), dim3(d < 256 ? 64 * (int)km_cdiv(d, 64) : 256),
+                     sizeof(float) * d, s, d, k, S, cnt, C_old, C_new, Ct_new, ldc, nc_new, counts,
+                     empty);
+  return hipGetLastError();
+}
+
+hipError_t launch_update(hipStream_t s, long n, int d, int k, const float *X, const int *labels,
+                         const float *C_old, float *C_new, float *Ct_new, int ldc, float *nc_new,
+                         long long *counts, int *empty, const UpdateWs &ws) {
+  hipError_t e;
+  if ((e = launch_update_zero(s, d, k, ws.S, ws.cnt, empty)) != hipSuccess) return e;
+  if ((e = launch_update_accumulate(s, n, d, k, X, labels, ws.S, ws.cnt, ws)) != hipSuccess) return e;
+  return launch_update_finalize(s, d, k, ws.S, ws.cnt, C_old, C_new, Ct_new, ldc, nc_new, counts,
+                                empty);
+}
+
+// Canonical compactness of rows [0, n); first != 0 overwrites *out, else adds.
+hipError_t launch_compactness_acc(hipStream_t s, long n, int d, const float *X, const int *labels,
+                                  const float *C, double *out, const UpdateWs &ws, int first) {
+  if (n <= 0) return first ? hipMemsetAsync(out, 0, sizeof(double), s) : hipSuccess;
+  for (long i0 = 0; i0 < n; i0 += KM_BLOCK) {
+    const long nb = (n - i0) < KM_BLOCK ? (n - i0) : KM_BLOCK;
+    const long nq = km_cdiv(nb, KM_CHUNK);
+    hipLaunchKernelGGL(k_compact_chunks, dim3(nq), dim3(KM_CHUNK), 0, s, i0, nb, d, X, labels, C,
+                       ws.E);
+    hipLaunchKernelGGL(k_compact_block, dim3(1), dim3(256), 0, s, ws.E, (int)nq, out, first);
+    first = 0;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_compactness(hipStream_t s, long n, int d, const float *X, const int *labels,
+                              const float *C, double *out, const UpdateWs &ws) {
+  return launch_compactness_acc(s, n, d, X, labels, C, out, ws, 1);
+}
+
+// nc[j] = fma chain over d of Cdk[d*ld + j]^2 (D x K layout, Normalize.cu:33-47).
+__global__ void k_col_norms_dk(int k, int d, int ld, const float *__restrict__ C,
+                               float *__restrict__ nc) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= k) return;
+  float s = 0.0f;
+  for (int q = 0; q < d; ++q) {
+    const float v = C[(long)q * ld + j];
+    s = __builtin_fmaf(v, v, s);
+  }
+  nc[j] = s;
+}
+
+hipError_t launch_col_norms_dk(hipStream_t s, int k, int d, int ld, const float *C, float *nc) {
+  hipLaunchKernelGGL(k_col_norms_dk, dim3(km_cdiv(k, 256)), dim3(256), 0, s, k, d, ld, C, nc);
+  return hipGetLastError();
+}
+
+hipError_t launch_generate(hipStream_t s, uint64_t seed, long i0, long n, int d, int ncomp,
+                           const float *means, const float *sigmas, const double *cdf, float *X,
+                           int *z) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_generate, dim3(km_cdiv(n, 256)), dim3(256), 0, s, seed, i0, n, d, ncomp,
+                     means, sigmas, cdf, X, z);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_rows(hipStream_t s, long n, int d, const float *X, const int *idx,
+                              float *out) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_rows, dim3(km_cdiv(n * d, 256)), dim3(256), 0, s, n, d, X, idx, out);
+  return hipGetLastError();
+}
+
+}  // namespace km

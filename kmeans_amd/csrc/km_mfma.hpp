@@ -1,0 +1,13 @@
+// Reduced-precision (BF16/FP16 MFMA) assignment with FP32 re-resolve.
+#pragma once
+#include "km_common.hpp"
+
+namespace km {
+struct Engine;
+void mfma_free(Engine *e);
+void mfma_invalidate(Engine *e);
+hipError_t mfma_assign(Engine *e, long r0, long rows);
+hipError_t mfma_reresolve(Engine *e, long r0, long rows);
+hipError_t mfma_centers(Engine *e);
+void mfma_collect(Engine *e);
+}  // namespace km

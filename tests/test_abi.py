@@ -1,0 +1,70 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function
+include/kmeans_amd.h declares, and host-side argument checks reject bad
+shapes before touching a device."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import kmeans_amd
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    txt = open(os.path.join(ROOT, "include", "kmeans_amd.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt)
+    skip = {"if", "sizeof", "defined"}
+    return sorted({n for n in names if n not in skip and not n.isupper()})
+
+
+def test_library_exports_every_declared_symbol():
+    L = kmeans_amd.lib()
+    names = declared_functions()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # and the Python mirror binds each of them with a signature
+    assert set(names) <= set(kmeans_amd.SIGNATURES), set(names) - set(kmeans_amd.SIGNATURES)
+
+
+def test_error_strings_match_reference_table():
+    L = kmeans_amd.lib()
+    assert L.kmeansGetErrorString(0) == b"no error"
+    assert L.kmeansGetErrorString(3) == b"error in computing the closest centers"
+    assert L.kmeansGetErrorString(13) == b"algorithm did not converge"
+
+
+def test_create_rejects_bad_shapes_without_device():
+    L = kmeans_amd.lib()
+    h = ctypes.c_void_p()
+    for m, n, k in [(10, 4, 11), (0, 4, 1), (10, 0, 2), (10, 4, 0)]:
+        assert L.kmeansCreate(ctypes.byref(h), m, n, k, None) == kmeans_amd.KMEANS_ERROR_INVALID_ARGUMENT
+    opt = kmeans_amd.Options(7, 0, None, 0, 0)  # unknown precision
+    assert L.kmeansCreate(ctypes.byref(h), 100, 4, 2, ctypes.byref(opt)) == 14
+
+
+def test_kernel_api_rejects_bad_arguments():
+    L = kmeans_amd.lib()
+    assert L.kmColNorms(None, 0, 4, None, None) == 8
+    assert L.kmAssignF32(None, 10, None, 0, None, None, 4, None, None, None, None) == 8
+    assert L.kmUpdate(None, 10, 4, 2, None, None, None, None, None, None, None, None, None, 0) == 8
+    assert L.kmCompactness(None, 10, 4, 2, None, None, None, None, None, 0) == 8
+
+
+def test_workspace_size_and_layout():
+    L = kmeans_amd.lib()
+    assert L.kmCentersLd(1) == 256 and L.kmCentersLd(256) == 256 and L.kmCentersLd(257) == 512
+    small = L.kmUpdateWorkspaceSize(1000, 8, 4)
+    big = L.kmUpdateWorkspaceSize(10_000_000, 128, 1024)
+    assert 0 < small < big
+    # bounded by one canonical block of points, not by N
+    assert L.kmUpdateWorkspaceSize(200_000_000, 64, 4096) < 2 * L.kmUpdateWorkspaceSize(kmeans_amd.KM_BLOCK, 64, 4096)
+
+
+def test_no_cpu_fallback_in_package():
+    src = open(os.path.join(ROOT, "kmeans_amd", "__init__.py")).read()
+    assert "oracle" not in src.split('"""', 2)[2]

@@ -1,0 +1,264 @@
+"""GPU parity of the plain FP32 path against the CPU oracle, through the C-ABI.
+
+Bar: bit-exact labels, scores, centres, counts and compactness (the GPU runs
+the same fma chains and the same canonical FP64 summation order)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+import kmeans_amd as km  # noqa: E402
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def gpu_assign(X, C):
+    k, d = C.shape
+    Xd, Cd = dev(X), dev(C)
+    ld = km.centers_ld(k)
+    Ct = torch.zeros((d, ld), dtype=torch.float32, device="cuda")
+    nc = torch.empty(k, dtype=torch.float32, device="cuda")
+    lab = torch.empty(X.shape[0], dtype=torch.int32, device="cuda")
+    best = torch.empty(X.shape[0], dtype=torch.float32, device="cuda")
+    km.transpose_centers(k, d, Cd.data_ptr(), Ct.data_ptr())
+    km.col_norms(k, d, Cd.data_ptr(), nc.data_ptr())
+    km.assign_f32(X.shape[0], d, Xd.data_ptr(), k, Ct.data_ptr(), nc.data_ptr(), lab.data_ptr(),
+                  best.data_ptr())
+    torch.cuda.synchronize()
+    return lab.cpu().numpy(), best.cpu().numpy(), nc.cpu().numpy()
+
+
+@pytest.mark.parametrize("n,d,k", [(1000, 3, 2), (777, 16, 5), (4096, 17, 64), (3000, 32, 65),
+                                   (2500, 128, 300), (5000, 128, 1024), (513, 200, 33)])
+def test_assign_bit_exact(orc, n, d, k):
+    print(f"assign n={n} d={d} k={k}", flush=True)
+    X, _, means, _ = orc.mixture(100 + d, n, d, k, -5, 5, 0.5, 2.0)
+    C = X[np.arange(k) * (n // k)].copy()
+    lab, best, nc = gpu_assign(X, C)
+    rl, rb, _ = orc.assign(X, C, with_scores=True)
+    assert np.array_equal(nc, orc.col_norms(C))
+    assert np.array_equal(lab, rl)
+    assert np.array_equal(best, rb)
+
+
+def test_assign_ties_lowest_index(orc):
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((2048, 24)).astype(np.float32)
+    base = rng.standard_normal((300, 24)).astype(np.float32)
+    C = np.concatenate([base, base[::-1]])  # every centre duplicated, 600 total
+    lab, _, _ = gpu_assign(X, C)
+    assert np.array_equal(lab, orc.assign(X, C))
+    assert lab.max() < 300
+
+
+def test_assign_rows_indirect(orc):
+    X, _, _, _ = orc.mixture(3, 4000, 40, 100, -4, 4, 1.0, 1.0)
+    C = X[:100].copy()
+    k, d = C.shape
+    rows = np.sort(np.random.default_rng(0).choice(4000, 999, replace=False)).astype(np.int32)
+    Xd, Cd, rd = dev(X), dev(C), dev(rows)
+    Ct = torch.zeros((d, km.centers_ld(k)), dtype=torch.float32, device="cuda")
+    nc = torch.empty(k, dtype=torch.float32, device="cuda")
+    lab = torch.full((4000,), -7, dtype=torch.int32, device="cuda")
+    mdev = dev(np.array([999], np.int32))
+    km.transpose_centers(k, d, Cd.data_ptr(), Ct.data_ptr())
+    km.col_norms(k, d, Cd.data_ptr(), nc.data_ptr())
+    km.assign_f32(4000, d, Xd.data_ptr(), k, Ct.data_ptr(), nc.data_ptr(), lab.data_ptr(),
+                  rows_ptr=rd.data_ptr(), m_dev_ptr=mdev.data_ptr())
+    out = lab.cpu().numpy()
+    ref = orc.assign(X, C)
+    assert np.array_equal(out[rows], ref[rows])
+    mask = np.ones(4000, bool)
+    mask[rows] = False
+    assert np.all(out[mask] == -7)
+
+
+def gpu_update(X, lab, C):
+    n, d = X.shape
+    k = C.shape[0]
+    Xd, ld, Cd = dev(X), dev(lab), dev(C)
+    Cn = torch.empty_like(Cd)
+    cnt = torch.empty(k, dtype=torch.int64, device="cuda")
+    emp = torch.zeros(1, dtype=torch.int32, device="cuda")
+    nc = torch.empty(k, dtype=torch.float32, device="cuda")
+    wsb = km.update_workspace_size(n, d, k)
+    ws = torch.empty(wsb, dtype=torch.uint8, device="cuda")
+    km.update(n, d, k, Xd.data_ptr(), ld.data_ptr(), Cd.data_ptr(), Cn.data_ptr(), ws.data_ptr(),
+              wsb, nc_ptr=nc.data_ptr(), counts_ptr=cnt.data_ptr(), empty_ptr=emp.data_ptr())
+    out = torch.empty(1, dtype=torch.float64, device="cuda")
+    km.compactness(n, d, k, Xd.data_ptr(), ld.data_ptr(), Cn.data_ptr(), out.data_ptr(),
+                   ws.data_ptr(), wsb)
+    torch.cuda.synchronize()
+    return (Cn.cpu().numpy(), cnt.cpu().numpy(), int(emp.item()), nc.cpu().numpy(),
+            float(out.item()))
+
+
+@pytest.mark.parametrize("n,d,k", [(5000, 3, 7), (20000, 16, 64), (30000, 33, 100),
+                                   (12345, 128, 1024), (4000, 300, 20)])
+def test_update_and_compactness_bit_exact(orc, n, d, k):
+    print(f"update n={n} d={d} k={k}", flush=True)
+    X, _, _, _ = orc.mixture(7 + k, n, d, k, -6, 6, 0.5, 1.5)
+    C0 = X[:k].copy()
+    lab = orc.assign(X, C0)
+    C1, cnt, emp, nc, comp = gpu_update(X, lab, C0)
+    rC, rcnt, _, rempty = orc.update(X, lab, C0)
+    assert np.array_equal(cnt, rcnt)
+    assert emp == rempty
+    assert np.array_equal(C1, rC)
+    assert np.array_equal(nc, orc.col_norms(rC))
+    assert comp == orc.compactness(X, lab, rC)
+
+
+def test_update_skewed_and_empty(orc):
+    # one giant cluster (many chunks), several empty ones
+    rng = np.random.default_rng(9)
+    n, d, k = 300_000, 8, 50
+    X = rng.standard_normal((n, d)).astype(np.float32)
+    lab = np.where(rng.random(n) < 0.9, 3, rng.integers(10, 20, n)).astype(np.int32)
+    C0 = rng.standard_normal((k, d)).astype(np.float32)
+    C1, cnt, emp, nc, comp = gpu_update(X, lab, C0)
+    rC, rcnt, _, rempty = orc.update(X, lab, C0)
+    assert emp == rempty == k - 11
+    assert np.array_equal(C1, rC) and np.array_equal(cnt, rcnt)
+    assert comp == orc.compactness(X, lab, rC)
+
+
+def test_update_spans_two_blocks(orc):
+    # the oracle's block known answer: 2^53 then ones over KM_BLOCK + 1000 rows
+    n = km.KM_BLOCK + 1000
+    X = np.ones((n, 1), np.float32)
+    X[0, 0] = 2.0 ** 53
+    lab = np.zeros(n, np.int32)
+    C0 = np.zeros((1, 1), np.float32)
+    C1, cnt, emp, nc, comp = gpu_update(X, lab, C0)
+    rC, rcnt, S, _ = orc.update(X, lab, C0)
+    assert S[0, 0] == 2.0 ** 53 + (km.KM_BLOCK - 256) + 1000
+    assert np.array_equal(C1, rC) and cnt[0] == n
+    assert comp == orc.compactness(X, lab, rC)
+
+
+def test_engine_lloyd_config1_bit_exact(orc):
+    X, C0 = orc.config1()
+    ref = orc.lloyd(X, C0, 20)
+    e = km.KMeans(X.shape[0], X.shape[1], 64)
+    e.set_data(X)
+    e.set_centers(C0)
+    e.run(20)
+    st = e.stats()
+    assert st["iterations"] == 20
+    assert np.array_equal(e.labels(), ref["labels"])
+    assert np.array_equal(e.centers(), ref["centers"])
+    assert st["compactness"] == ref["comp_hist"][-1]
+    e.close()
+
+
+def test_engine_criterion_stops_like_reference(orc):
+    X, _, _, _ = orc.mixture(12, 30_000, 8, 10, -10, 10, 0.7, 0.7)
+    C0 = X[:10].copy()
+    ref = orc.lloyd(X, C0, 50, criterion=1e-6)
+    e = km.KMeans(X.shape[0], 8, 10)
+    e.set_data(X)
+    e.set_centers(C0)
+    e.run(50, 1e-6)
+    st = e.stats()
+    assert st["iterations"] == ref["iters"] < 50
+    assert st["rel_err"] == ref["rel_err"]
+    assert np.array_equal(e.centers(), ref["centers"])
+
+
+def test_engine_device_data_borrowed(orc):
+    X, _, _, _ = orc.mixture(13, 10_000, 12, 16, -5, 5, 1.0, 1.0)
+    Xd = dev(X)
+    torch.cuda.synchronize()
+    e = km.KMeans(10_000, 12, 16)
+    e.set_data(device_ptr=Xd.data_ptr())
+    e.set_centers(X[:16])
+    e.run(3)
+    ref = orc.lloyd(X, X[:16], 3)
+    assert np.array_equal(e.labels(), ref["labels"])
+    assert np.array_equal(e.centers(), ref["centers"])
+
+
+def test_compute_kmeans_f_with_given_init(orc):
+    X, _, _, _ = orc.mixture(14, 8000, 6, 12, -8, 8, 1.0, 1.0)
+    C0 = X[100:112].copy()
+    st, res = km.compute_kmeans_f(X, 12, -1.0, 7, result=C0.T.copy(), init_from_result=True)
+    ref = orc.lloyd(X, C0, 7)
+    assert st == km.KMEANS_NONCONVERGENCE  # criterion < 0 can never be met
+    assert np.array_equal(res, ref["centers"].T)
+
+
+def test_compute_kmeans_f_sampled_init_converges(orc):
+    X, z, means, _ = orc.mixture(15, 20_000, 4, 5, -20, 20, 0.5, 0.5)
+    st, res = km.compute_kmeans_f(X, 5, 1e-4, 100)
+    assert st == km.KMEANS_SUCCESS
+    assert res.shape == (4, 5)
+
+
+def test_mini_batch_matches_oracle(orc):
+    X, _, _, _ = orc.mixture(16, 10_000, 5, 8, -6, 6, 1.0, 1.0)
+    C = X[:8].copy()
+    e = km.KMeans(10_000, 5, 8)
+    e.set_data(X)
+    e.set_centers(C)
+    e.set_mini_batch(3000)
+    e.run(5)
+    # batches: [0,3000) [3000,6000) [6000,9000) [9000,10000) [0,3000)
+    for b0, b1 in [(0, 3000), (3000, 6000), (6000, 9000), (9000, 10000), (0, 3000)]:
+        lab = orc.assign(X[b0:b1], C)
+        C, _, _, _ = orc.update(X[b0:b1], lab, C)
+    assert np.array_equal(e.centers(), C)
+
+
+def test_legacy_kernel_names(orc):
+    X, _, _, _ = orc.mixture(17, 6000, 20, 70, -5, 5, 1.0, 1.0)
+    C = X[:70].copy()
+    n, d = X.shape
+    k = 70
+    L = km.lib()
+    Xd, B = dev(X), dev(C.T.copy())  # D x K as the reference passes it
+    ncg = torch.empty(k, dtype=torch.float32, device="cuda")
+    assert L.colNormalizeF(d, k, km._vp(B.data_ptr()), km._vp(ncg.data_ptr())) == 0
+    lab = torch.empty(n, dtype=torch.int32, device="cuda")
+    for m0 in (0, 2500):  # two tiles
+        m = 2500 if m0 == 0 else n - 2500
+        assert L.ClosestCentersF(m0, m, d, km._vp(Xd.data_ptr() + 4 * m0 * d), k,
+                                 km._vp(B.data_ptr()), None, None, km._vp(ncg.data_ptr()), 0,
+                                 None, None, km._vp(lab.data_ptr())) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(ncg.cpu().numpy(), orc.col_norms(C))
+    ref = orc.assign(X, C)
+    assert np.array_equal(lab.cpu().numpy(), ref)
+    large = torch.zeros(4 * k * d, dtype=torch.float32, device="cuda")
+    clarge = torch.zeros(4 * k, dtype=torch.int32, device="cuda")
+    cen = torch.empty((k, d), dtype=torch.float32, device="cuda")
+    cnt = torch.empty(k, dtype=torch.int32, device="cuda")
+    assert L.ClusterCentersF(0, n, d, k, True, km._vp(Xd.data_ptr()), km._vp(lab.data_ptr()),
+                             km._vp(large.data_ptr()), km._vp(clarge.data_ptr()),
+                             km._vp(cen.data_ptr()), km._vp(cnt.data_ptr())) == 0
+    rC, rcnt, _, _ = orc.update(X, ref, C)
+    # empty clusters keep the value already in `centers`; none here
+    assert np.array_equal(cnt.cpu().numpy(), rcnt)
+    assert np.array_equal(cen.cpu().numpy()[rcnt > 0], rC[rcnt > 0])
+    comp = torch.zeros(8, dtype=torch.float32, device="cuda")
+    assert L.CompactnessF(0, n, True, d, k, km._vp(Xd.data_ptr()), km._vp(lab.data_ptr()),
+                          km._vp(cen.data_ptr()), km._vp(comp.data_ptr())) == 0
+    assert comp[0].item() == np.float32(orc.compactness(X, ref, cen.cpu().numpy()))
+
+
+def test_generator_matches_cpu(orc):
+    n, d, k = 5000, 16, 32
+    X, z, means, sig = orc.mixture(1, n, d, k, -5, 5, 0.5, 2.0, i0=123)
+    cdf = np.empty(k, np.float64)
+    orc.lib().orc_gen_cdf(k, 0.0, cdf.ctypes.data)
+    md, sd, cd = dev(means), dev(sig), dev(cdf)
+    Xg = torch.empty((n, d), dtype=torch.float32, device="cuda")
+    zg = torch.empty(n, dtype=torch.int32, device="cuda")
+    km.generate_mixture(1, 123, n, d, k, md.data_ptr(), sd.data_ptr(), cd.data_ptr(),
+                        Xg.data_ptr(), zg.data_ptr())
+    torch.cuda.synchronize()
+    assert np.array_equal(zg.cpu().numpy(), z)
+    assert np.allclose(Xg.cpu().numpy(), X, rtol=1e-6, atol=1e-6)
