@@ -125,6 +125,7 @@ kmeansErrorStatus Engine::setCenters(const float *centers) {
 kmeansErrorStatus Engine::transpose() {
   KM_HIP_CHECK(launch_transpose(stream, k, n, dC, dCt, ldc), KMEANS_ERROR_TRANSPOSE);
   KM_HIP_CHECK(launch_col_norms(stream, k, n, dC, dnc), KMEANS_ERROR_TRANSPOSE);
+  mfma_centers_changed(this, true);
   return KMEANS_SUCCESS;
 }
 
@@ -234,6 +235,7 @@ kmeansErrorStatus Engine::run_iterations(long r0, long rows, bool minibatch) {
   while (iters < maxIters && relErr > criterion) {
     long b0 = r0, br = rows;
     if (minibatch) constructMiniBatch(mb_index++, &b0, &br);
+    cur_slot = slot;
     record(slot, 0);
     if ((err = closestCenters(b0, br)) != KMEANS_SUCCESS) return err;
     record(slot, 1);
@@ -248,7 +250,7 @@ kmeansErrorStatus Engine::run_iterations(long r0, long rows, bool minibatch) {
     record(slot, 4);
     ++iters;
     ++slot;
-    if (opt.precision != KM_PREC_FP32 || sync_each) {
+    if (sync_each) {
       int emp = 0;
       double comp = 0.0;
       KM_HIP_CHECK(hipMemcpyAsync(&comp, dcomp + slot - 1, sizeof(double), hipMemcpyDeviceToHost,
@@ -258,7 +260,6 @@ kmeansErrorStatus Engine::run_iterations(long r0, long rows, bool minibatch) {
                    KMEANS_ERROR_COMPUTE);
       KM_HIP_CHECK(hipStreamSynchronize(stream), KMEANS_ERROR_COMPUTE);
       stats.empty_clusters += emp;
-      if (opt.precision != KM_PREC_FP32) mfma_collect(this);
       const double prev = compactnessScore;
       compactnessScore = comp;
       relErr = fabs(comp - prev) / fabs(comp);
@@ -268,7 +269,7 @@ kmeansErrorStatus Engine::run_iterations(long r0, long rows, bool minibatch) {
       relErr = 1.0 + fabs(criterion);
     }
   }
-  if (!(opt.precision != KM_PREC_FP32 || sync_each) && slot > 0) {
+  if (!sync_each && slot > 0) {
     hist.resize(slot);
     KM_HIP_CHECK(hipMemcpyAsync(hist.data(), dcomp, sizeof(double) * slot, hipMemcpyDeviceToHost,
                                 stream),
@@ -284,6 +285,7 @@ kmeansErrorStatus Engine::run_iterations(long r0, long rows, bool minibatch) {
       relErr = fabs(hist[s] - prev) / fabs(hist[s]);
     }
   }
+  if (opt.precision != KM_PREC_FP32) mfma_collect(this, slot);
   collect_timing(slot);
   stats.iterations = iters;
   stats.compactness = compactnessScore;

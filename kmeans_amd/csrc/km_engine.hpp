@@ -45,6 +45,8 @@ struct Engine {
   int iters = 0;
   bool centers_set = false;
   int comp_slots = 0;
+  int cur_slot = 0;               // iteration slot of the running kmeansRun
+  long long last_reresolved = 0;
   kmeansStats stats{};
 
   // mini-batch (kmeans.cpp:1253-1312)

@@ -9,5 +9,7 @@ void mfma_invalidate(Engine *e);
 hipError_t mfma_assign(Engine *e, long r0, long rows);
 hipError_t mfma_reresolve(Engine *e, long r0, long rows);
 hipError_t mfma_centers(Engine *e);
-void mfma_collect(Engine *e);
+void mfma_collect(Engine *e, int nslots);
+void mfma_centers_changed(Engine *e, bool new_range);
+hipError_t mfma_ensure_slots(Engine *e, int n);
 }  // namespace km

@@ -1,0 +1,95 @@
+"""BF16/FP16 MFMA assignment: labels must equal the plain FP32 labels (and so
+the oracle's) on every input, including adversarial near-ties; the certified
+fraction must be high on clustered data (otherwise the speed-up is fake)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+import kmeans_amd as km  # noqa: E402
+
+
+def run_engine(X, C0, iters, precision):
+    e = km.KMeans(X.shape[0], X.shape[1], C0.shape[0], precision=precision)
+    e.set_data(X)
+    e.set_centers(C0)
+    e.run(iters)
+    out = (e.labels(), e.centers(), e.stats())
+    e.close()
+    return out
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+@pytest.mark.parametrize("n,d,k", [(3000, 3, 1), (3000, 3, 2), (3000, 5, 3), (5000, 16, 64),
+                                   (4000, 17, 65), (6000, 32, 100), (5000, 64, 256),
+                                   (5000, 96, 300), (20000, 128, 1024), (3000, 200, 40)])
+def test_mfma_labels_equal_oracle(orc, prec, n, d, k):
+    print(f"mfma {prec} n={n} d={d} k={k}", flush=True)
+    X, _, _, _ = orc.mixture(200 + d + k, n, d, max(k, 2), -5, 5, 0.5, 2.0)
+    C0 = X[np.arange(k) * (n // k)].copy()
+    lab, C, st = run_engine(X, C0, 3, prec)
+    ref = orc.lloyd(X, C0, 3)
+    assert np.array_equal(lab, ref["labels"])
+    assert np.array_equal(C, ref["centers"])
+    assert st["compactness"] == ref["comp_hist"][-1]
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_mfma_certifies_most_points(orc, prec):
+    X, z, means, _ = orc.mixture(31, 50_000, 128, 256, -5, 5, 0.5, 2.0)
+    C0 = X[:256].copy()
+    lab, C, st = run_engine(X, C0, 4, prec)
+    ref = orc.lloyd(X, C0, 4)
+    assert np.array_equal(lab, ref["labels"])
+    frac = st["reresolved"] / (4 * 50_000)
+    print(f"{prec}: re-resolved fraction {frac:.4f}")
+    assert frac < 0.2
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_mfma_adversarial_near_ties(orc, prec):
+    rng = np.random.default_rng(77)
+    d, k = 48, 40
+    base = (rng.standard_normal((k // 2, d)) * 4).astype(np.float32)
+    # pairs of centres closer than the BF16 resolution, and exact duplicates
+    C0 = np.concatenate([base, base + (rng.standard_normal((k // 2, d)) * 1e-3).astype(np.float32)])
+    C0[5] = C0[25]
+    C0[7] = C0[30]
+    # points on (or next to) bisectors of the close pairs
+    mids = 0.5 * (C0[: k // 2] + C0[k // 2:])
+    X = np.concatenate([
+        np.repeat(mids, 100, axis=0) + (rng.standard_normal((100 * (k // 2), d)) * 1e-4).astype(np.float32),
+        (rng.standard_normal((3000, d)) * 4).astype(np.float32),
+    ]).astype(np.float32)
+    nc = orc.col_norms(C0)
+    ref = orc.assign(X, C0, nc)
+    e = km.KMeans(X.shape[0], d, k, precision=prec)
+    e.set_data(X)
+    e.set_centers(C0)
+    e.run(1)
+    # after one iteration labels are those of the first assignment
+    assert np.array_equal(e.labels(), ref)
+    st = e.stats()
+    assert st["reresolved"] > 0  # the near-ties must have gone to the FP32 path
+    e.close()
+
+
+def test_mfma_large_magnitudes_fp16_scaling(orc):
+    # values far outside FP16 range must be scaled, not overflow
+    X, _, _, _ = orc.mixture(41, 4000, 24, 20, -5, 5, 1.0, 1.0)
+    X = X * np.float32(1e6)
+    C0 = X[:20].copy()
+    lab, C, st = run_engine(X, C0, 2, "fp16")
+    ref = orc.lloyd(X, C0, 2)
+    assert np.array_equal(lab, ref["labels"]) and np.array_equal(C, ref["centers"])
+
+
+def test_mfma_tiny_magnitudes(orc):
+    X, _, _, _ = orc.mixture(42, 4000, 24, 20, -5, 5, 1.0, 1.0)
+    X = X * np.float32(1e-15)
+    C0 = X[:20].copy()
+    for prec in ("bf16", "fp16"):
+        lab, C, st = run_engine(X, C0, 2, prec)
+        ref = orc.lloyd(X, C0, 2)
+        assert np.array_equal(lab, ref["labels"]) and np.array_equal(C, ref["centers"])
