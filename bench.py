@@ -33,6 +33,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=0, help="points in the CPU baseline sample (0: all)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32-ref", action="store_true")
+    ap.add_argument("--calib", action="store_true",
+                    help="also run a plain 1 GiB device-to-device copy (PMC byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus > 1 is not supported "
@@ -145,6 +147,13 @@ def main():
             "seconds": t_cpu,
             "gpu_fp32_iteration_bit_exact": parity,
         }
+    if args.calib:
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)  # reads 1 GiB, writes 1 GiB
+        torch.cuda.synchronize()
+        out["calibration_copy_bytes"] = 1 << 30
     print(json.dumps(out))
 
 

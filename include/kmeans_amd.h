@@ -126,6 +126,7 @@ typedef struct kmeansStats {
   double rel_err;
   long long empty_clusters;   /* summed over iterations                       */
   long long reresolved;       /* points re-searched in FP32 (MFMA paths)      */
+  long long rescored;         /* points certified by exact candidate scores   */
   double ms_assign;           /* summed over iterations (timing == 1)         */
   double ms_reresolve;
   double ms_update;

@@ -51,7 +51,8 @@ class Options(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("compactness", ctypes.c_double),
                 ("rel_err", ctypes.c_double), ("empty_clusters", ctypes.c_longlong),
-                ("reresolved", ctypes.c_longlong), ("ms_assign", ctypes.c_double),
+                ("reresolved", ctypes.c_longlong), ("rescored", ctypes.c_longlong),
+                ("ms_assign", ctypes.c_double),
                 ("ms_reresolve", ctypes.c_double), ("ms_update", ctypes.c_double),
                 ("ms_compactness", ctypes.c_double), ("ms_total", ctypes.c_double)]
 

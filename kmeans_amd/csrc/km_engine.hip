@@ -105,6 +105,7 @@ kmeansErrorStatus Engine::buildData(const float *data, int on_device) {
     dX = dX_owned;
   }
   if (mfma) mfma_invalidate(this);
+  if (opt.precision != KM_PREC_FP32) KM_HIP_CHECK(mfma_prepare(this), KMEANS_ERROR_BUILDDATA);
   return KMEANS_SUCCESS;
 }
 
@@ -126,6 +127,7 @@ kmeansErrorStatus Engine::transpose() {
   KM_HIP_CHECK(launch_transpose(stream, k, n, dC, dCt, ldc), KMEANS_ERROR_TRANSPOSE);
   KM_HIP_CHECK(launch_col_norms(stream, k, n, dC, dnc), KMEANS_ERROR_TRANSPOSE);
   mfma_centers_changed(this, true);
+  if (opt.precision != KM_PREC_FP32) KM_HIP_CHECK(mfma_prepare(this), KMEANS_ERROR_TRANSPOSE);
   return KMEANS_SUCCESS;
 }
 

@@ -448,48 +448,47 @@ __global__ __launch_bounds__(256) void k_final_centres(int D, int K, const doubl
 
 // ---------------------------------------------------------------- compactness
 // e_i = fma chain over d of (x_id - c_{l_i,d})^2; one chunk of KM_CHUNK points
-// per block, summed left to right in FP64 by one thread.  Each wave owns 64
-// points; rows of X and of the matching centres are staged through LDS in
-// d-slices of 32 with coalesced loads, then every lane runs its own chain.
-#define CP_DS 32
+// per block (one point per thread), summed left to right in FP64 by one
+// thread.  Rows are read in 16-byte pieces, eight in flight per lane, so each
+// lane consumes whole cache lines of its point and centre rows.
 __global__ __launch_bounds__(256) void k_compact_chunks(long i0, long nb, int D,
                                                         const float *__restrict__ X,
                                                         const int *__restrict__ labels,
                                                         const float *__restrict__ C,
                                                         double *__restrict__ E) {
-  __shared__ float xs[4][64][CP_DS + 1];
-  __shared__ float cs[4][64][CP_DS + 1];
   __shared__ float ev[KM_CHUNK];
   const long c = blockIdx.x;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const long base = c * KM_CHUNK + w * 64;  // first point of this wave (block-relative)
-  // loader mapping: 64 lanes cover 2 rows x 32 columns per step
-  const int lr = lane >> 5, lc = lane & 31;
+  const long r = c * KM_CHUNK + threadIdx.x;
   float e = 0.0f;
-  for (int d0 = 0; d0 < D; d0 += CP_DS) {
-    const int dn = min(CP_DS, D - d0);
-#pragma unroll 4
-    for (int r = 0; r < 64; r += 2) {
-      const long pr = base + r + lr;
-      float xv = 0.0f, cv = 0.0f;
-      if (pr < nb && lc < dn) {
-        const long i = i0 + pr;
-        xv = X[i * D + d0 + lc];
-        cv = C[(long)labels[i] * D + d0 + lc];
+  if (r < nb) {
+    const long i = i0 + r;
+    const float *x = X + i * D;
+    const float *cc = C + (long)labels[i] * D;
+    if ((D & 3) == 0) {
+      const float4 *x4 = reinterpret_cast<const float4 *>(x);
+      const float4 *c4 = reinterpret_cast<const float4 *>(cc);
+#pragma unroll 8
+      for (int q = 0; q < D / 4; ++q) {
+        const float4 a = x4[q], b = c4[q];
+        float v = a.x - b.x;
+        e = __builtin_fmaf(v, v, e);
+        v = a.y - b.y;
+        e = __builtin_fmaf(v, v, e);
+        v = a.z - b.z;
+        e = __builtin_fmaf(v, v, e);
+        v = a.w - b.w;
+        e = __builtin_fmaf(v, v, e);
       }
-      xs[w][r + lr][lc] = xv;
-      cs[w][r + lr][lc] = cv;
+    } else {
+      for (int q = 0; q < D; ++q) {
+        const float v = x[q] - cc[q];
+        e = __builtin_fmaf(v, v, e);
+      }
     }
-    __syncthreads();
-    for (int dd = 0; dd < dn; ++dd) {
-      const float v = xs[w][lane][dd] - cs[w][lane][dd];
-      e = __builtin_fmaf(v, v, e);
-    }
-    __syncthreads();
   }
-  ev[tid] = e;
+  ev[threadIdx.x] = e;
   __syncthreads();
-  if (tid == 0) {
+  if (threadIdx.x == 0) {
     const int cnt = (int)min((long)KM_CHUNK, nb - c * KM_CHUNK);
     double s = 0.0;
     for (int t = 0; t < cnt; ++t) s += (double)ev[t];

@@ -35,6 +35,7 @@ typedef __attribute__((ext_vector_type(4))) unsigned int u32x4_t;
 
 #define MF_TILE 32     // centres per MFMA tile (rows of the 32x32 output)
 #define MF_PTS 256     // points per workgroup (4 waves x 2 groups x 32)
+#define MF_STAGE 4     // centre tiles per LDS stage (128 centres)
 
 // Round-to-nearest-even conversions written out so the rounding is known.
 __device__ inline uint16_t f32_to_bf16_rne(float f) {
@@ -58,7 +59,10 @@ struct MfmaState {
   uint16_t *Ch = nullptr;   // Kp x Dp
   float *bias = nullptr;    // Kp: -0.5*scale^2*nc (or -FLT_MAX past k)
   float *bnd = nullptr;     // [cmax, ncmax]
+  float *xnorm = nullptr;   // m: upper bound of |x_i|
   int *flags = nullptr;     // m
+  int4 *cands = nullptr;    // m entries (point, k1, k2, threshold bits)
+  int *ncand = nullptr;     // per iteration slot
   int *nflag = nullptr;     // per iteration slot
   int nflag_slots = 0;
   bool x_ready = false;
@@ -112,13 +116,20 @@ __global__ __launch_bounds__(256) void k_prepare_centres(int k, int d, int Dp, i
 }
 
 // top-3 of packed values: b1 >= b2 >= b3
-__device__ inline void top3_insert(float p, float &b1, float &b2, float &b3) {
+// (max written as med3 with a run-time +FLT_MAX the compiler cannot fold:
+// one instruction, no NaN canonicalisation of the bit-packed input)
+__device__ inline void top3_insert(float p, float &b1, float &b2, float &b3, float big) {
   const float n3 = __builtin_amdgcn_fmed3f(b2, b3, p);
   const float n2 = __builtin_amdgcn_fmed3f(b1, b2, p);
-  b1 = fmaxf(b1, p);
+  b1 = __builtin_amdgcn_fmed3f(b1, p, big);
   b2 = n2;
   b3 = n3;
 }
+
+struct Top3 {
+  float b1 = -FLT_MAX, b2 = -FLT_MAX, b3 = -FLT_MAX;
+  int t1 = 0, t2 = 0;
+};
 
 // row of output register r (0..15) of a 32x32 MFMA tile for lane half h
 __device__ inline int mf_row(int r, int h) { return 8 * (r >> 2) + 4 * h + (r & 3); }
@@ -127,17 +138,26 @@ template <int KS, bool FP16>
 __global__ __launch_bounds__(256) void k_assign_mfma(
     long M, const uint16_t *__restrict__ Xh, const float *__restrict__ X, int D, int K, int Kp,
     const uint16_t *__restrict__ Ch, const float *__restrict__ bias, const float *__restrict__ C,
-    const float *__restrict__ nc, const float *__restrict__ bnd, float inv_s2, float ub,
-    float eta, int *__restrict__ labels, int *__restrict__ flags, int *__restrict__ nflag) {
+    const float *__restrict__ nc, const float *__restrict__ bnd, const float *__restrict__ xnorm,
+    float inv_s2, float ub, float eta, float big, int *__restrict__ labels,
+    int *__restrict__ flags, int *__restrict__ nflag, int4 *__restrict__ cands,
+    int *__restrict__ ncand) {
   constexpr int Dp = KS * 16;
-  constexpr int LDA = Dp + 8;  // bf16 elements per LDS row (16-byte pad)
-  __shared__ __attribute__((aligned(16))) uint16_t As[2][MF_TILE * LDA];
-  __shared__ __attribute__((aligned(16))) float Bs[2][MF_TILE];
+  constexpr int CPR = Dp / 8;               // 16-byte chunks per centre row
+  constexpr int SROWS = MF_STAGE * MF_TILE; // centre rows per LDS stage
+  constexpr int SCH = SROWS * CPR;          // chunks per stage
+  // LDS: two stages of centre rows (chunk q of row r stored at position
+  // (q + r) % CPR, so the 32 rows one ds_read_b128 touches fall in distinct
+  // banks) and two stages of biases; filled by global->LDS DMA.
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  u32x4_t *As = reinterpret_cast<u32x4_t *>(smem);                         // [2][SCH]
+  float *Bs = reinterpret_cast<float *>(smem + 2 * SCH * sizeof(u32x4_t));  // [2][SROWS]
   typedef typename std::conditional<FP16, f16x8_t, bf16x8_t>::type vec8;
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, h = lane >> 5, col = lane & 31;
   const long p0 = (long)blockIdx.x * MF_PTS + w * 64;
-  const int nKT = Kp / MF_TILE;
+  const int nKT = Kp / MF_TILE;      // multiple of MF_STAGE
+  const int nS = nKT / MF_STAGE;
 
   // point fragments: lane holds point (p0 + g*32 + col), k = ks*16 + 8h + 0..7
   u32x4_t bfrag[2][KS];
@@ -153,172 +173,263 @@ __global__ __launch_bounds__(256) void k_assign_mfma(
     }
   }
 
-  // centre tiles: global -> registers (issued before the MFMAs of the
-  // current tile) -> LDS (after them), double-buffered
-  constexpr int CHUNKS = MF_TILE * Dp / 8;  // 16-byte chunks per tile
-  constexpr int PER = (CHUNKS + 255) / 256;
-  u32x4_t pre[PER];
-  float prebias = 0.0f;
-  auto fetch_tile = [&](int kt) {
+  // asynchronous copy of centre stage st into LDS buffer b: each DMA
+  // instruction writes 64 consecutive chunks (one per lane, in lane order).
+  // Chunk q of row r lives at position (q + rot(r)) % CPR with rot(r) = r & 7
+  // (for CPR >= 8), so the 8 rows one ds_read_b128 phase touches hit
+  // distinct banks.
+  auto issue_stage = [&](int st, int b) {
+    u32x4_t *dst = As + b * SCH;
+    const uint16_t *cs = Ch + (long)st * SROWS * Dp;
+    if constexpr (64 % CPR == 0) {
+      constexpr int RPI = 64 / CPR;  // rows per DMA instruction
+      const int r0 = w * RPI + lane / CPR;
+      const int rot = CPR >= 8 ? (r0 & 7) : 0;  // same for every instruction below
+      const int q = (lane % CPR - rot + CPR) % CPR;
+      const uint16_t *src = cs + (long)r0 * Dp + q * 8;
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int c = tid + 256 * i;
-      if (c < CHUNKS) {
-        const int r = c / (Dp / 8), q = c - r * (Dp / 8);
-        pre[i] = *reinterpret_cast<const u32x4_t *>(Ch + ((long)kt * MF_TILE + r) * Dp + q * 8);
+      for (int i = 0; i < SCH / 256; ++i)
+        __builtin_amdgcn_global_load_lds(src + (long)i * 4 * RPI * Dp, dst + (i * 4 + w) * 64, 16,
+                                         0, 0);
+    } else {
+#pragma unroll
+      for (int i = 0; i < SCH / 256; ++i) {
+        const int c0 = (i * 4 + w) * 64;
+        const int c = c0 + lane;
+        const int r = c / CPR, pos = c - r * CPR;
+        const int rot = CPR >= 8 ? (r & 7) : 0;
+        const int q = (pos - rot + CPR) % CPR;
+        __builtin_amdgcn_global_load_lds(cs + (long)r * Dp + q * 8, dst + c0, 16, 0, 0);
       }
     }
-    if (tid < MF_TILE) prebias = bias[kt * MF_TILE + tid];
-  };
-  auto store_tile = [&](int buf) {
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int c = tid + 256 * i;
-      if (c < CHUNKS) {
-        const int r = c / (Dp / 8), q = c - r * (Dp / 8);
-        *reinterpret_cast<u32x4_t *>(&As[buf][r * LDA + q * 8]) = pre[i];
-      }
-    }
-    if (tid < MF_TILE) Bs[buf][tid] = prebias;
+    if (tid < SROWS / 4)  // biases: 4 floats per lane
+      __builtin_amdgcn_global_load_lds(bias + (long)st * SROWS + tid * 4, Bs + b * SROWS, 16, 0, 0);
   };
 
-  float b1[2], b2[2], b3[2];
-  int t1[2], t2[2];
-#pragma unroll
-  for (int g = 0; g < 2; ++g) {
-    b1[g] = b2[g] = b3[g] = -FLT_MAX;
-    t1[g] = t2[g] = 0;
-  }
+  // per point group: the three largest packed scores and the tiles of the
+  // first two (named structs, not arrays: selects between array elements
+  // would force the state into scratch memory)
+  Top3 S0, S1;
 
-  fetch_tile(0);
-  store_tile(0);
-  __syncthreads();
-  for (int kt = 0; kt < nKT; ++kt) {
-    const int buf = kt & 1;
-    if (kt + 1 < nKT) fetch_tile(kt + 1);
-    // bias as the accumulator input: rows 8*(r/4) + 4h + r%4
+  // scores of one centre tile for both point groups; bias is the
+  // accumulator input (rows 8*(r/4) + 4h + r%4)
+  auto mfma_tile = [&](int kt, f32x16_t &a0, f32x16_t &a1) {
+    const int b = (kt / MF_STAGE) & 1, j = kt % MF_STAGE;
+    const u32x4_t *A = As + b * SCH;
+    const float *Bb = Bs + b * SROWS + j * MF_TILE;
     f32x16_t cinit;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) cinit[r] = Bs[buf][mf_row(r, h)];
-    f32x16_t acc0 = cinit, acc1 = cinit;
+    for (int r = 0; r < 16; ++r) cinit[r] = Bb[mf_row(r, h)];
+    a0 = cinit;
+    a1 = cinit;
+    const int row = j * MF_TILE + col;
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
-      const u32x4_t a = *reinterpret_cast<const u32x4_t *>(&As[buf][col * LDA + ks * 16 + 8 * h]);
+      const int q = ks * 2 + h;
+      const u32x4_t a = A[row * CPR + (q + (CPR >= 8 ? (row & 7) : 0)) % CPR];
       if constexpr (FP16) {
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(vec8, a),
-                                                      __builtin_bit_cast(vec8, bfrag[0][ks]), acc0,
-                                                      0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(vec8, a),
-                                                      __builtin_bit_cast(vec8, bfrag[1][ks]), acc1,
-                                                      0, 0, 0);
+        a0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(vec8, a),
+                                                    __builtin_bit_cast(vec8, bfrag[0][ks]), a0, 0,
+                                                    0, 0);
+        a1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(vec8, a),
+                                                    __builtin_bit_cast(vec8, bfrag[1][ks]), a1, 0,
+                                                    0, 0);
       } else {
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(vec8, a),
-                                                       __builtin_bit_cast(vec8, bfrag[0][ks]),
-                                                       acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(vec8, a),
-                                                       __builtin_bit_cast(vec8, bfrag[1][ks]),
-                                                       acc1, 0, 0, 0);
+        a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(vec8, a),
+                                                     __builtin_bit_cast(vec8, bfrag[0][ks]), a0, 0,
+                                                     0, 0);
+        a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(vec8, a),
+                                                     __builtin_bit_cast(vec8, bfrag[1][ks]), a1, 0,
+                                                     0, 0);
       }
     }
-    // epilogue: pack the register id into the low 4 bits, keep the top 3
+  };
+  // pack the register id into the low 4 bits, keep the top 3 and the tiles
+  // of the first two
+  auto epi_group = [&](int kt, const f32x16_t &acc, Top3 &T) {
+    const float o1 = T.b1, o2 = T.b2;
 #pragma unroll
-    for (int g = 0; g < 2; ++g) {
-      const f32x16_t &acc = g == 0 ? acc0 : acc1;
-      const float o1 = b1[g], o2 = b2[g];
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const float p = __uint_as_float((__float_as_uint(acc[r]) & 0xFFFFFFF0u) | (unsigned)r);
-        top3_insert(p, b1[g], b2[g], b3[g]);
-      }
-      const unsigned u1 = __float_as_uint(b1[g]), u2 = __float_as_uint(b2[g]);
-      const int nt2 = (u2 == __float_as_uint(o1)) ? t1[g] : (u2 == __float_as_uint(o2)) ? t2[g] : kt;
-      t1[g] = (u1 != __float_as_uint(o1)) ? kt : t1[g];
-      t2[g] = nt2;
+    for (int r = 0; r < 16; ++r) {
+      const float p = __uint_as_float((__float_as_uint(acc[r]) & 0xFFFFFFF0u) | (unsigned)r);
+      top3_insert(p, T.b1, T.b2, T.b3, big);
     }
-    if (kt + 1 < nKT) store_tile(buf ^ 1);
-    __syncthreads();
+    const unsigned u1 = __float_as_uint(T.b1), u2 = __float_as_uint(T.b2);
+    const int ot1 = T.t1, ot2 = T.t2;
+    const int nt2 = (u2 == __float_as_uint(o1)) ? ot1 : (u2 == __float_as_uint(o2)) ? ot2 : kt;
+    T.t1 = (u1 != __float_as_uint(o1)) ? kt : ot1;
+    T.t2 = nt2;
+  };
+  auto epi_tile = [&](int kt, const f32x16_t &a0, const f32x16_t &a1) {
+    epi_group(kt, a0, S0);
+    epi_group(kt, a1, S1);
+  };
+
+  // Software pipeline: while the VALU runs the epilogue of tile t, the
+  // matrix core computes tile t+1 (same basic block); stage s+1 streams into
+  // the other LDS buffer while stage s is consumed.  Tiles past the end are
+  // clamped to the last one and their scores are never read.
+  issue_stage(0, 0);
+  __builtin_amdgcn_s_waitcnt(0);  // point fragments and stage 0 have arrived
+  __syncthreads();
+  f32x16_t A0, A1, B0, B1;
+  mfma_tile(0, A0, A1);
+  for (int kt = 0; kt < nKT; kt += 2) {
+    if (kt % MF_STAGE == 0 && kt / MF_STAGE + 1 < nS) issue_stage(kt / MF_STAGE + 1, (kt / MF_STAGE + 1) & 1);
+    mfma_tile(min(kt + 1, nKT - 1), B0, B1);
+    epi_tile(kt, A0, A1);
+    if (kt + 1 >= nKT) break;
+    if ((kt + 2) % MF_STAGE == 0) {  // next tile opens a new stage
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+    }
+    mfma_tile(min(kt + 2, nKT - 1), A0, A1);
+    epi_tile(kt + 1, B0, B1);
   }
 
   // merge the two lane halves (same points, different centre rows)
   const float cmax = bnd[0], ncmax = bnd[1];
-#pragma unroll
-  for (int g = 0; g < 2; ++g) {
+  auto finish = [&](int g, const Top3 &T) {
     // candidate indices of this half
-    const unsigned u1 = __float_as_uint(b1[g]), u2 = __float_as_uint(b2[g]);
-    int i1 = t1[g] * MF_TILE + mf_row((int)(u1 & 15u), h);
-    int i2 = t2[g] * MF_TILE + mf_row((int)(u2 & 15u), h);
-    bool tie = (u1 == u2) || (u2 == __float_as_uint(b3[g]));
-    float v1 = b1[g], v2 = b2[g], v3 = b3[g];
+    const unsigned u1 = __float_as_uint(T.b1), u2 = __float_as_uint(T.b2);
+    const int i1 = T.t1 * MF_TILE + mf_row((int)(u1 & 15u), h);
+    const int i2 = T.t2 * MF_TILE + mf_row((int)(u2 & 15u), h);
+    bool tie = (u1 == u2) || (u2 == __float_as_uint(T.b3));
+    const float v1 = T.b1, v2 = T.b2, v3 = T.b3;
     // the other half's triple
     const float w1 = __shfl_xor(v1, 32), w2 = __shfl_xor(v2, 32), w3 = __shfl_xor(v3, 32);
     const int j1 = __shfl_xor(i1, 32), j2 = __shfl_xor(i2, 32);
     const bool otie = __shfl_xor((int)tie, 32) != 0;
-    // merge two sorted triples (exact bitwise duplicates are flagged)
+    // merge two sorted triples (exact bitwise duplicates are flagged); the
+    // two lists are disjoint sets of centres, each sorted descending
     float m1, m2, m3;
     int k1, k2;
     {
-      // take in order; both halves are sorted descending
-      float a[3] = {v1, v2, v3}, b[3] = {w1, w2, w3};
-      int ai[2] = {i1, i2}, bi[2] = {j1, j2};
-      int ia = 0, ib = 0;
-      float out[3];
-      int oi[3];
-#pragma unroll
-      for (int t = 0; t < 3; ++t) {
-        const bool takeA = a[ia] >= b[ib];
-        out[t] = takeA ? a[ia] : b[ib];
-        oi[t] = takeA ? (ia < 2 ? ai[ia] : -1) : (ib < 2 ? bi[ib] : -1);
-        if (takeA) ++ia; else ++ib;
-      }
-      m1 = out[0]; m2 = out[1]; m3 = out[2];
-      k1 = oi[0]; k2 = oi[1];
+      const bool A1 = v1 >= w1;                  // first from this half?
+      m1 = A1 ? v1 : w1;
+      k1 = A1 ? i1 : j1;
+      const float x2 = A1 ? v2 : v1, y2 = A1 ? w1 : w2;  // heads after taking one
+      const int xi2 = A1 ? i2 : i1, yi2 = A1 ? j1 : j2;
+      const bool A2 = x2 >= y2;
+      m2 = A2 ? x2 : y2;
+      k2 = A2 ? xi2 : yi2;
+      // third: the larger of the two remaining heads
+      const float xa = A1 ? (A2 ? v3 : v2) : (A2 ? v2 : v1);
+      const float ya = A1 ? (A2 ? w1 : w2) : (A2 ? w2 : w3);
+      m3 = fmaxf(xa, ya);
       tie = tie || otie || (__float_as_uint(m1) == __float_as_uint(m2)) ||
             (__float_as_uint(m2) == __float_as_uint(m3));
-      // a candidate index of -1 can only appear if a half's third entry
-      // outranked the other half's first two, which then was a tie or K<3
-      if (k1 < 0 || k2 < 0) tie = true;
     }
-    // exact FP32 chains: half 0 rescores k1, half 1 rescores k2
     const long p = p0 + g * 32 + col;
-    const int cand = h == 0 ? k1 : k2;
-    float s = -INFINITY, xx = 0.0f;
-    if (p < M && cand >= 0 && cand < K) {
-      const float *x = X + p * D;
+    // forward-error bound E for every centre's approximate score (FP32 units)
+    const float xn = p < M ? xnorm[p] : 0.0f;
+    const float xc = xn * cmax;
+    const float Df = (float)D;
+    const float g32 = Df * 5.9604645e-8f * 1.0001f;        // gamma_D, u = 2^-24
+    const float gmf = (Df + 1.0f) * 2.3841858e-7f * 1.001f; // gamma'_{D+1}, u = 2^-22
+    const float rnd = (2.0f * ub + ub * ub) * xc;
+    const float etat = eta * sqrtf(Df) * (xn + cmax) + eta * eta * Df;
+    const float mfe = gmf * (0.5f * ncmax + (1.0f + ub) * (1.0f + ub) * xc + etat);
+    const float fpe = g32 * (0.5f * ncmax + xc);
+    const float E = (rnd + etat + mfe + fpe) * 1.001f + 1e-30f;
+    const float a1 = m1 * inv_s2, a2 = m2 * inv_s2, a3 = m3 * inv_s2;  // FP32 score units
+    // packed values carry 4 replaced low bits (< 16 ulp); FP32 subtraction
+    // and the scaling add a few more ulps
+    const float slack = (fabsf(a1) + fabsf(a2) + fabsf(a3)) * 1.6e-5f;
+    const bool live = h == 0 && p < M;
+    // certificate 1: the best approximate score beats every other by > 2E
+    const bool quick = !tie && (a1 - a2 > 2.0f * E + slack) && k1 >= 0 && k1 < K;
+    // certificate 2 (separate pass): the exact FP32 chains of the two
+    // candidates must beat every other centre's upper bound a3 + E; with
+    // k <= 2 there is no other centre
+    const bool has_third = m3 > -FLT_MAX * 0.5f;
+    const float thr = has_third ? a3 + E + slack : -INFINITY;
+    const bool two = live && !quick && !tie && k1 >= 0 && k1 < K;
+    const bool fall = live && !quick && !two;
+    if (live && quick) labels[p] = k1;
+    // wave-aggregated appends (one atomic per wave and list)
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    {
+      const uint64_t mk = __ballot(two);
+      if (mk) {
+        const int lead = __ffsll((long long)mk) - 1;
+        int base = 0;
+        if (lane == lead) base = atomicAdd(ncand, __popcll(mk));
+        base = __shfl(base, lead);
+        if (two) cands[base + __popcll(mk & below)] = make_int4((int)p, k1, k2, __float_as_int(thr));
+      }
+    }
+    {
+      const uint64_t mk = __ballot(fall);
+      if (mk) {
+        const int lead = __ffsll((long long)mk) - 1;
+        int base = 0;
+        if (lane == lead) base = atomicAdd(nflag, __popcll(mk));
+        base = __shfl(base, lead);
+        if (fall) flags[base + __popcll(mk & below)] = (int)p;
+      }
+    }
+  };
+  finish(0, S0);
+  finish(1, S1);
+}
+
+// Certificate 2: exact FP32 chains (the plain path's arithmetic) of the two
+// candidates of each listed point, one lane per (point, candidate); the
+// better one (lowest index on equal scores) is the label if it exceeds the
+// point's threshold, otherwise the point goes to the full FP32 search.
+__global__ __launch_bounds__(256) void k_rescore2(const int *__restrict__ ncand,
+                                                  const int4 *__restrict__ cands, int D, int K,
+                                                  const float *__restrict__ X,
+                                                  const float *__restrict__ C,
+                                                  const float *__restrict__ nc,
+                                                  int *__restrict__ labels, int *__restrict__ flags,
+                                                  int *__restrict__ nflag) {
+  const long total = 2L * *ncand;
+  const int lane = threadIdx.x & 63;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (long t0 = (long)blockIdx.x * blockDim.x; t0 < total; t0 += (long)gridDim.x * blockDim.x) {
+    const long t = t0 + threadIdx.x;
+    const bool ok = t < total;
+    int4 ce = make_int4(0, 0, -1, 0);
+    if (ok) ce = cands[t >> 1];
+    const int cand = (t & 1) ? ce.z : ce.y;
+    float s = -INFINITY;
+    if (ok && cand >= 0 && cand < K) {
+      const float *x = X + (long)ce.x * D;
       const float *c = C + (long)cand * D;
       s = -0.5f * nc[cand];
-      for (int q = 0; q < D; ++q) {
-        const float xv = x[q];
-        s = __builtin_fmaf(xv, c[q], s);
-        xx = __builtin_fmaf(xv, xv, xx);
+      if ((D & 3) == 0) {
+        const float4 *x4 = reinterpret_cast<const float4 *>(x);
+        const float4 *c4 = reinterpret_cast<const float4 *>(c);
+#pragma unroll 8
+        for (int q = 0; q < D / 4; ++q) {
+          const float4 xv = x4[q], cv = c4[q];
+          s = __builtin_fmaf(xv.x, cv.x, s);
+          s = __builtin_fmaf(xv.y, cv.y, s);
+          s = __builtin_fmaf(xv.z, cv.z, s);
+          s = __builtin_fmaf(xv.w, cv.w, s);
+        }
+      } else {
+        for (int q = 0; q < D; ++q) s = __builtin_fmaf(x[q], c[q], s);
       }
     }
-    const float so = __shfl_xor(s, 32);
-    if (h == 0 && p < M) {
-      const float s1 = s, s2 = so;  // exact scores of k1, k2
-      int lab = k1;
+    const float so = __shfl_xor(s, 1);
+    bool fall = false;
+    if (ok && (t & 1) == 0) {
+      const float s1 = s, s2 = so;
+      int lab = ce.y;
       float sb = s1;
-      if (k2 >= 0 && k2 < K && (s2 > s1 || (s2 == s1 && k2 < k1))) { lab = k2; sb = s2; }
-      // forward-error bound for every other centre (scores in FP32 units)
-      const float xn = sqrtf(xx * (1.0f + 1e-5f)) * (1.0f + 1e-6f) + 1e-30f;
-      const float xc = xn * cmax;
-      const float Df = (float)D;
-      const float g32 = Df * 5.9604645e-8f * 1.0001f;        // gamma_D, u = 2^-24
-      const float gmf = (Df + 1.0f) * 2.3841858e-7f * 1.001f; // gamma'_{D+1}, u = 2^-22
-      const float rnd = (2.0f * ub + ub * ub) * xc;
-      const float etat = eta * sqrtf(Df) * (xn + cmax) + eta * eta * Df;
-      const float mfe = gmf * (0.5f * ncmax + (1.0f + ub) * (1.0f + ub) * xc + etat);
-      const float fpe = g32 * (0.5f * ncmax + xc);
-      const float third = m3 * inv_s2;  // back to FP32 score units
-      const float E = (rnd + etat + mfe + fpe) * 1.001f + fabsf(third) * 8e-6f + 1e-30f;
-      // no third centre (k <= 2): the two candidates are all there is
-      const bool has_third = m3 > -FLT_MAX * 0.5f;
-      const bool ok = !tie && (!has_third || sb > third + E);
-      if (ok && lab >= 0 && lab < K && sb > -INFINITY) {
-        labels[p] = lab;
-      } else {
-        const int slot = atomicAdd(nflag, 1);
-        flags[slot] = (int)p;
-      }
+      if (ce.z >= 0 && ce.z < K && (s2 > s1 || (s2 == s1 && ce.z < ce.y))) { lab = ce.z; sb = s2; }
+      if (sb > __int_as_float(ce.w) && sb > -INFINITY) labels[ce.x] = lab;
+      else fall = true;
+    }
+    const uint64_t mk = __ballot(fall);
+    if (mk) {
+      const int lead = __ffsll((long long)mk) - 1;
+      int base = 0;
+      if (lane == lead) base = atomicAdd(nflag, __popcll(mk));
+      base = __shfl(base, lead);
+      if (fall) flags[base + __popcll(mk & below)] = ce.x;
     }
   }
 }
@@ -331,7 +442,10 @@ void mfma_free(Engine *e) {
   hipFree(s->Ch);
   hipFree(s->bias);
   hipFree(s->bnd);
+  hipFree(s->xnorm);
   hipFree(s->flags);
+  hipFree(s->cands);
+  hipFree(s->ncand);
   hipFree(s->nflag);
   delete s;
   e->mfma = nullptr;
@@ -359,16 +473,45 @@ static hipError_t ensure_state(Engine *e) {
   else if (s->KS > 4) s->KS = 6;
   else if (s->KS > 2) s->KS = 4;
   s->Dp = s->KS * 16;
-  s->Kp = (int)km_cdiv(e->k, MF_TILE) * MF_TILE;
+  s->Kp = (int)km_cdiv(e->k, MF_TILE * MF_STAGE) * MF_TILE * MF_STAGE;
   hipError_t r;
   if ((r = hipMalloc(&s->Xh, sizeof(uint16_t) * (size_t)e->m * s->Dp)) != hipSuccess) return r;
   if ((r = hipMalloc(&s->Ch, sizeof(uint16_t) * (size_t)s->Kp * s->Dp)) != hipSuccess) return r;
   if ((r = hipMalloc(&s->bias, sizeof(float) * s->Kp)) != hipSuccess) return r;
   if ((r = hipMalloc(&s->bnd, sizeof(float) * 4)) != hipSuccess) return r;
+  if ((r = hipMalloc(&s->xnorm, sizeof(float) * (size_t)e->m)) != hipSuccess) return r;
   if ((r = hipMalloc(&s->flags, sizeof(int) * (size_t)e->m)) != hipSuccess) return r;
+  if ((r = hipMalloc(&s->cands, sizeof(int4) * (size_t)e->m)) != hipSuccess) return r;
   s->nflag_slots = 64;
   if ((r = hipMalloc(&s->nflag, sizeof(int) * s->nflag_slots)) != hipSuccess) return r;
+  if ((r = hipMalloc(&s->ncand, sizeof(int) * s->nflag_slots)) != hipSuccess) return r;
+  if ((r = hipMemsetAsync(s->ncand, 0, sizeof(int) * s->nflag_slots, e->stream)) != hipSuccess)
+    return r;
   return hipMemsetAsync(s->nflag, 0, sizeof(int) * s->nflag_slots, e->stream);
+}
+
+// upper bound of |x_i|: the FP32 chain of squares is within gamma_D of the
+// exact sum, inflated by 1e-5 (covers D <= 160,000) and one more ulp for sqrt.
+// One point per lane, rows read in 16-byte pieces (whole cache lines).
+__global__ void k_xnorm(long m, int d, const float *__restrict__ X, float *__restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const float *x = X + i * d;
+  float s = 0.0f;
+  if ((d & 3) == 0) {
+    const float4 *x4 = reinterpret_cast<const float4 *>(x);
+#pragma unroll 8
+    for (int q = 0; q < d / 4; ++q) {
+      const float4 v = x4[q];
+      s = __builtin_fmaf(v.x, v.x, s);
+      s = __builtin_fmaf(v.y, v.y, s);
+      s = __builtin_fmaf(v.z, v.z, s);
+      s = __builtin_fmaf(v.w, v.w, s);
+    }
+  } else {
+    for (int q = 0; q < d; ++q) s = __builtin_fmaf(x[q], x[q], s);
+  }
+  out[i] = sqrtf(s * (1.0f + 1e-5f)) * (1.0f + 1e-6f) + 1e-30f;
 }
 
 // max |v| over a device array (FP16 scaling only; one host read per run)
@@ -403,6 +546,8 @@ static hipError_t prepare_x(Engine *e) {
   const long tot = (long)e->m * s->Dp;
   hipLaunchKernelGGL(k_convert_rows, dim3(km_cdiv(tot, 256)), dim3(256), 0, e->stream, e->m, e->n,
                      s->Dp, e->dX, s->scale, (int)s->fp16, s->Xh);
+  hipLaunchKernelGGL(k_xnorm, dim3(km_cdiv(e->m, 256)), dim3(256), 0, e->stream, e->m, e->n, e->dX,
+                     s->xnorm);
   if ((r = hipGetLastError()) != hipSuccess) return r;
   s->x_ready = true;
   return hipSuccess;
@@ -432,13 +577,25 @@ template <int KS>
 static void launch_mfma_ks(hipStream_t strm, bool fp16, dim3 g, long M, const uint16_t *Xh,
                            const float *X, int D, int K, int Kp, const uint16_t *Ch,
                            const float *bias, const float *C, const float *nc, const float *bnd,
-                           float inv_s2, float ub, float eta, int *labels, int *flags, int *nflag) {
+                           const float *xnorm, float inv_s2, float ub, float eta, int *labels,
+                           int *flags, int *nflag, int4 *cands, int *ncand) {
+  const size_t lds = 2 * (size_t)MF_STAGE * MF_TILE * (KS * 2) * 16 + 2 * MF_STAGE * MF_TILE * 4;
+  static bool attr_set[2] = {false, false};
+  if (!attr_set[fp16]) {
+    if (fp16)
+      hipFuncSetAttribute((const void *)k_assign_mfma<KS, true>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    else
+      hipFuncSetAttribute((const void *)k_assign_mfma<KS, false>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr_set[fp16] = true;
+  }
   if (fp16)
-    hipLaunchKernelGGL((k_assign_mfma<KS, true>), g, dim3(256), 0, strm, M, Xh, X, D, K, Kp, Ch,
-                       bias, C, nc, bnd, inv_s2, ub, eta, labels, flags, nflag);
+    hipLaunchKernelGGL((k_assign_mfma<KS, true>), g, dim3(256), lds, strm, M, Xh, X, D, K, Kp, Ch,
+                       bias, C, nc, bnd, xnorm, inv_s2, ub, eta, FLT_MAX, labels, flags, nflag, cands, ncand);
   else
-    hipLaunchKernelGGL((k_assign_mfma<KS, false>), g, dim3(256), 0, strm, M, Xh, X, D, K, Kp, Ch,
-                       bias, C, nc, bnd, inv_s2, ub, eta, labels, flags, nflag);
+    hipLaunchKernelGGL((k_assign_mfma<KS, false>), g, dim3(256), lds, strm, M, Xh, X, D, K, Kp, Ch,
+                       bias, C, nc, bnd, xnorm, inv_s2, ub, eta, FLT_MAX, labels, flags, nflag, cands, ncand);
 }
 
 hipError_t mfma_assign(Engine *e, long r0, long rows) {
@@ -454,7 +611,9 @@ hipError_t mfma_assign(Engine *e, long r0, long rows) {
     if ((r = prepare_centres(e)) != hipSuccess) return r;
   if ((r = mfma_ensure_slots(e, e->cur_slot + 1)) != hipSuccess) return r;
   int *nf = s->nflag + e->cur_slot;
+  int *nc2 = s->ncand + e->cur_slot;
   if ((r = hipMemsetAsync(nf, 0, sizeof(int), e->stream)) != hipSuccess) return r;
+  if ((r = hipMemsetAsync(nc2, 0, sizeof(int), e->stream)) != hipSuccess) return r;
   if (rows <= 0) return hipSuccess;
   const float ub = s->fp16 ? 4.8828125e-4f : 3.90625e-3f;       // 2^-11, 2^-8
   const float eta = s->fp16 ? 2.9802322e-8f / s->scale : 0.0f;  // half FP16 subnormal step
@@ -464,13 +623,17 @@ hipError_t mfma_assign(Engine *e, long r0, long rows) {
   const float *X = e->dX + r0 * e->n;
   int *lab = e->dlabels + r0;
   switch (s->KS) {
-    case 1: launch_mfma_ks<1>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, inv_s2, ub, eta, lab, s->flags, nf); break;
-    case 2: launch_mfma_ks<2>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, inv_s2, ub, eta, lab, s->flags, nf); break;
-    case 4: launch_mfma_ks<4>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, inv_s2, ub, eta, lab, s->flags, nf); break;
-    case 6: launch_mfma_ks<6>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, inv_s2, ub, eta, lab, s->flags, nf); break;
-    case 8: launch_mfma_ks<8>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, inv_s2, ub, eta, lab, s->flags, nf); break;
-    default: launch_mfma_ks<16>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, inv_s2, ub, eta, lab, s->flags, nf); break;
+    case 1: launch_mfma_ks<1>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, s->xnorm + r0, inv_s2, ub, eta, lab, s->flags, nf, s->cands, nc2); break;
+    case 2: launch_mfma_ks<2>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, s->xnorm + r0, inv_s2, ub, eta, lab, s->flags, nf, s->cands, nc2); break;
+    case 4: launch_mfma_ks<4>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, s->xnorm + r0, inv_s2, ub, eta, lab, s->flags, nf, s->cands, nc2); break;
+    case 6: launch_mfma_ks<6>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, s->xnorm + r0, inv_s2, ub, eta, lab, s->flags, nf, s->cands, nc2); break;
+    case 8: launch_mfma_ks<8>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, s->xnorm + r0, inv_s2, ub, eta, lab, s->flags, nf, s->cands, nc2); break;
+    default: launch_mfma_ks<16>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, s->xnorm + r0, inv_s2, ub, eta, lab, s->flags, nf, s->cands, nc2); break;
   }
+  if ((r = hipGetLastError()) != hipSuccess) return r;
+  const long rg = km_cdiv(2 * rows, 256);
+  hipLaunchKernelGGL(k_rescore2, dim3(rg < num_cus() * 8 ? rg : num_cus() * 8), dim3(256), 0,
+                     e->stream, nc2, s->cands, e->n, e->k, X, e->dC, e->dnc, lab, s->flags, nf);
   return hipGetLastError();
 }
 
@@ -479,17 +642,38 @@ hipError_t mfma_ensure_slots(Engine *e, int n) {
   if ((r = ensure_state(e)) != hipSuccess) return r;
   MfmaState *s = st(e);
   if (n <= s->nflag_slots) return hipSuccess;
-  int *nb = nullptr;
-  if ((r = hipMalloc(&nb, sizeof(int) * n)) != hipSuccess) return r;
-  if ((r = hipMemsetAsync(nb, 0, sizeof(int) * n, e->stream)) != hipSuccess) return r;
-  if ((r = hipMemcpyAsync(nb, s->nflag, sizeof(int) * s->nflag_slots, hipMemcpyDeviceToDevice,
-                          e->stream)) != hipSuccess)
-    return r;
-  if ((r = hipStreamSynchronize(e->stream)) != hipSuccess) return r;
-  hipFree(s->nflag);
-  s->nflag = nb;
+  int *arrs[2] = {s->nflag, s->ncand};
+  for (int a = 0; a < 2; ++a) {
+    int *nb = nullptr;
+    if ((r = hipMalloc(&nb, sizeof(int) * n)) != hipSuccess) return r;
+    if ((r = hipMemsetAsync(nb, 0, sizeof(int) * n, e->stream)) != hipSuccess) return r;
+    if ((r = hipMemcpyAsync(nb, arrs[a], sizeof(int) * s->nflag_slots, hipMemcpyDeviceToDevice,
+                            e->stream)) != hipSuccess)
+      return r;
+    if ((r = hipStreamSynchronize(e->stream)) != hipSuccess) return r;
+    hipFree(arrs[a]);
+    arrs[a] = nb;
+  }
+  s->nflag = arrs[0];
+  s->ncand = arrs[1];
   s->nflag_slots = n;
   return hipSuccess;
+}
+
+// One-time data preparation (reduced-precision copy of X, row norms) so the
+// iteration loop does no per-dataset work; called when data and centres are set.
+hipError_t mfma_prepare(Engine *e) {
+  hipError_t r;
+  if (!e->dX || !e->centers_set || e->n > 256) return hipSuccess;
+  if ((r = ensure_state(e)) != hipSuccess) return r;
+  MfmaState *s = st(e);
+  if (!s->x_ready) {
+    if ((r = prepare_x(e)) != hipSuccess) return r;
+    s->c_ready = false;
+  }
+  if (!s->c_ready)
+    if ((r = prepare_centres(e)) != hipSuccess) return r;
+  return hipStreamSynchronize(e->stream);
 }
 
 // FP32 full search for the uncertified points (device-side count).
@@ -512,6 +696,11 @@ void mfma_collect(Engine *e, int nslots) {
     e->stats.reresolved += v[i];
     e->last_reresolved = v[i];
   }
+  if (hipMemcpyAsync(v.data(), s->ncand, sizeof(int) * n, hipMemcpyDeviceToHost, e->stream) !=
+      hipSuccess)
+    return;
+  if (hipStreamSynchronize(e->stream) != hipSuccess) return;
+  for (int i = 0; i < n; ++i) e->stats.rescored += v[i];
 }
 
 }  // namespace km

@@ -12,4 +12,5 @@ hipError_t mfma_centers(Engine *e);
 void mfma_collect(Engine *e, int nslots);
 void mfma_centers_changed(Engine *e, bool new_range);
 hipError_t mfma_ensure_slots(Engine *e, int n);
+hipError_t mfma_prepare(Engine *e);
 }  // namespace km
