@@ -45,7 +45,7 @@ __host__ __device__ inline double rng_unif(uint64_t seed, uint64_t stream, uint6
 // Internal launchers (implemented in km_kernels.hip); all stream-ordered.
 struct UpdateWs {
   int *hist, *offs, *tsum, *cstart, *ccount, *cq, *perm;
-  double *P, *S, *E;
+  double *P, *S, *compj, *qb;  // S: k x (d+1), column d = sum of q_i
   long long *cnt;
   size_t bytes;
 };
@@ -58,20 +58,21 @@ hipError_t launch_assign_f32(hipStream_t s, long m, const int *m_dev, int d, con
                              const int *rows, int k, const float *Ct, int ldc, const float *nc,
                              int *labels, float *best);
 hipError_t launch_update(hipStream_t s, long n, int d, int k, const float *X, const int *labels,
-                         const float *C_old, float *C_new, float *Ct_new, int ldc, float *nc_new,
-                         long long *counts, int *empty, const UpdateWs &ws);
-hipError_t launch_compactness(hipStream_t s, long n, int d, const float *X, const int *labels,
-                              const float *C, double *out, const UpdateWs &ws);
-hipError_t launch_compactness_acc(hipStream_t s, long n, int d, const float *X, const int *labels,
-                                  const float *C, double *out, const UpdateWs &ws, int first);
+                         const double *q, const float *C_old, float *C_new, float *Ct_new, int ldc,
+                         float *nc_new, long long *counts, int *empty, const UpdateWs &ws);
+hipError_t launch_compactness(hipStream_t s, long n, int d, int k, const float *X,
+                              const int *labels, const double *q, const float *C, double *out,
+                              const UpdateWs &ws);
+hipError_t launch_comp_reduce(hipStream_t s, int k, const double *compj, double *out);
+hipError_t launch_point_sq(hipStream_t s, long m, int d, const float *X, double *q);
 hipError_t launch_update_zero(hipStream_t s, int d, int k, double *S, long long *cnt, int *empty);
 hipError_t launch_update_accumulate(hipStream_t s, long n, int d, int k, const float *X,
-                                    const int *labels, double *S, long long *cnt,
+                                    const int *labels, const double *q, double *S, long long *cnt,
                                     const UpdateWs &ws);
 hipError_t launch_update_finalize(hipStream_t s, int d, int k, const double *S,
                                   const long long *cnt, const float *C_old, float *C_new,
                                   float *Ct_new, int ldc, float *nc_new, long long *counts,
-                                  int *empty);
+                                  int *empty, double *compj);
 hipError_t launch_col_norms_dk(hipStream_t s, int k, int d, int ld, const float *C, float *nc);
 hipError_t launch_row_norms(hipStream_t s, long m, int d, const float *X, float *out);
 hipError_t launch_generate(hipStream_t s, uint64_t seed, long i0, long n, int d, int ncomp,

@@ -29,6 +29,7 @@ Engine::~Engine() {
   hipFree(dcounts);
   hipFree(dempty);
   hipFree(dcomp);
+  hipFree(dq);
   hipFree(dws);
   for (auto &e : ev)
     if (e) hipEventDestroy(e);
@@ -104,6 +105,10 @@ kmeansErrorStatus Engine::buildData(const float *data, int on_device) {
     KM_HIP_CHECK(hipStreamSynchronize(stream), KMEANS_ERROR_BUILDDATA);
     dX = dX_owned;
   }
+  // per-point FP64 squared norms, used by the compactness moments
+  if (!dq) KM_HIP_CHECK(hipMalloc(&dq, sizeof(double) * (size_t)m), KMEANS_ERROR_BUILDDATA);
+  KM_HIP_CHECK(launch_point_sq(stream, m, n, dX, dq), KMEANS_ERROR_BUILDDATA);
+  KM_HIP_CHECK(hipStreamSynchronize(stream), KMEANS_ERROR_BUILDDATA);
   if (mfma) mfma_invalidate(this);
   if (opt.precision != KM_PREC_FP32) KM_HIP_CHECK(mfma_prepare(this), KMEANS_ERROR_BUILDDATA);
   return KMEANS_SUCCESS;
@@ -175,16 +180,20 @@ kmeansErrorStatus Engine::closestCenters(long r0, long rows) {
 }
 
 kmeansErrorStatus Engine::clusterCenters(long r0, long rows) {
-  KM_HIP_CHECK(launch_update(stream, rows, n, k, dX + r0 * n, dlabels + r0, dC, dC, dCt, ldc, dnc,
-                             dcounts, dempty, ws),
+  KM_HIP_CHECK(launch_update(stream, rows, n, k, dX + r0 * n, dlabels + r0, dq + r0, dC, dC, dCt,
+                             ldc, dnc, dcounts, dempty, ws),
                KMEANS_ERROR_CLUSTER_CENTERS);
   if (opt.precision != KM_PREC_FP32) KM_HIP_CHECK(mfma_centers(this), KMEANS_ERROR_CLUSTER_CENTERS);
   return KMEANS_SUCCESS;
 }
 
+// kmeans.cpp:1045-1104: compactness against the new centres.  The update
+// already produced the per-centre terms from its FP64 moments (no second
+// pass over the points); this reduces them in the canonical order.
 kmeansErrorStatus Engine::compactness(long r0, long rows, int slot) {
-  KM_HIP_CHECK(launch_compactness(stream, rows, n, dX + r0 * n, dlabels + r0, dC, dcomp + slot, ws),
-               KMEANS_ERROR_COMPACTNESS);
+  (void)r0;
+  (void)rows;
+  KM_HIP_CHECK(launch_comp_reduce(stream, k, ws.compj, dcomp + slot), KMEANS_ERROR_COMPACTNESS);
   return KMEANS_SUCCESS;
 }
 
@@ -535,8 +544,8 @@ kmeansCudaErrorStatus kmUpdate(void *stream, long n, int d, int k, const float *
   km::UpdateWs ws;
   if (km::update_ws_layout(n, d, k, (char *)workspace, &ws) > workspace_bytes)
     return ERROR_INVALID_ARGUMENT;
-  KM_K_CHECK(km::launch_update((hipStream_t)stream, n, d, k, X, labels, C_old, C_new, Ct_new,
-                               km::centers_ld(k), nc_new, counts, empty, ws),
+  KM_K_CHECK(km::launch_update((hipStream_t)stream, n, d, k, X, labels, nullptr, C_old, C_new,
+                               Ct_new, km::centers_ld(k), nc_new, counts, empty, ws),
              ERROR_CLUSTERCENTERS);
   return NO_ERROR;
 }
@@ -549,7 +558,7 @@ kmeansCudaErrorStatus kmCompactness(void *stream, long n, int d, int k, const fl
   km::UpdateWs ws;
   if (km::update_ws_layout(n, d, k, (char *)workspace, &ws) > workspace_bytes)
     return ERROR_INVALID_ARGUMENT;
-  KM_K_CHECK(km::launch_compactness((hipStream_t)stream, n, d, X, labels, C, out, ws),
+  KM_K_CHECK(km::launch_compactness((hipStream_t)stream, n, d, k, X, labels, nullptr, C, out, ws),
              ERROR_COMPACTNESS);
   return NO_ERROR;
 }
@@ -630,11 +639,11 @@ kmeansCudaErrorStatus ClusterCentersF(const int m0, const int m, const int n, co
   KM_K_CHECK(km::legacy_ws(m, n, k, &ws), ERROR_CLUSTERCENTERS);
   double *S = (double *)centers_large;
   long long *cnt = (long long *)counts_large;
-  KM_K_CHECK(km::launch_update_accumulate(nullptr, m, n, k, data, indices + m0, S, cnt, ws),
+  KM_K_CHECK(km::launch_update_accumulate(nullptr, m, n, k, data, indices + m0, nullptr, S, cnt, ws),
              ERROR_CLUSTERCENTERS);
   if (lastTile) {
     KM_K_CHECK(km::launch_update_finalize(nullptr, n, k, S, cnt, centers, centers, nullptr, 0,
-                                          nullptr, ws.cnt, nullptr),
+                                          nullptr, ws.cnt, nullptr, nullptr),
                ERROR_CLUSTERCENTERS);
     if (counts) {
       std::vector<long long> c(k);
@@ -662,11 +671,15 @@ kmeansCudaErrorStatus CompactnessF(const int m0, const int m, const bool lastTil
   km::UpdateWs ws;
   KM_K_CHECK(km::legacy_ws(m, n, k, &ws), ERROR_COMPACTNESS);
   double *acc = (double *)(compactness + 2);
-  KM_K_CHECK(km::launch_compactness_acc(nullptr, m, n, data, indices + m0, centers, acc, ws, 0),
+  double *tile = ws.compj + k;  // one spare slot past the per-centre terms
+  KM_K_CHECK(km::launch_compactness(nullptr, m, n, k, data, indices + m0, nullptr, centers, tile, ws),
              ERROR_COMPACTNESS);
+  double tv = 0.0, tot = 0.0;
+  KM_K_CHECK(hipMemcpy(&tv, tile, sizeof(double), hipMemcpyDeviceToHost), ERROR_COMPACTNESS);
+  KM_K_CHECK(hipMemcpy(&tot, acc, sizeof(double), hipMemcpyDeviceToHost), ERROR_COMPACTNESS);
+  tot += tv;
+  KM_K_CHECK(hipMemcpy(acc, &tot, sizeof(double), hipMemcpyHostToDevice), ERROR_COMPACTNESS);
   if (lastTile) {
-    double tot = 0.0;
-    KM_K_CHECK(hipMemcpy(&tot, acc, sizeof(double), hipMemcpyDeviceToHost), ERROR_COMPACTNESS);
     const float f = (float)tot;
     KM_K_CHECK(hipMemcpy(compactness, &f, sizeof(float), hipMemcpyHostToDevice), ERROR_COMPACTNESS);
   }

@@ -33,6 +33,7 @@ struct Engine {
   long long *dcounts = nullptr;
   int *dempty = nullptr;
   double *dcomp = nullptr;  // one per iteration slot
+  double *dq = nullptr;     // m: FP64 squared norms of the points
   char *dws = nullptr;
   UpdateWs ws{};
   // precision path buffers (allocated on first use)

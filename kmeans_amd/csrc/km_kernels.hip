@@ -336,18 +336,23 @@ __global__ __launch_bounds__(256) void k_cluster_setup(int K, int nsb, const int
 }
 
 // First-level sums: one group of threads per chunk of <= KM_CHUNK members,
-// one thread per dimension, members added in ascending point order.
-__global__ __launch_bounds__(256) void k_chunk_sums(const float *__restrict__ X, int D, int Dp,
+// one thread per dimension, members added in ascending point order.  Column
+// D of each partial holds the sum of the members' squared norms q_i.
+__global__ __launch_bounds__(320) void k_chunk_sums(const float *__restrict__ X, int D, int Dp,
+                                                    const double *__restrict__ q,
                                                     const int *__restrict__ perm, int K,
                                                     const int *__restrict__ cstart,
                                                     const int *__restrict__ ccount,
                                                     const int *__restrict__ cq,
                                                     double *__restrict__ P) {
   __shared__ int pidx[4][KM_CHUNK];
-  const int G = 256 / Dp;  // chunks per block (>= 1)
-  const int sub = threadIdx.x / Dp, t = threadIdx.x - sub * Dp;
+  const int G = min(256 / Dp, 64);  // chunks per block; threads >= 256 sum q (one each)
+  const bool qwave = threadIdx.x >= 256;
+  const int sub = qwave ? (int)threadIdx.x - 256 : threadIdx.x / Dp;
+  const int t = qwave ? 0 : threadIdx.x - sub * Dp;
   const int Q = cq[K];
   const long c = (long)blockIdx.x * G + sub;
+  const int ld = D + 1;
   int j = 0, cnt = 0, start = 0;
   if (c < Q && sub < G) {
     int lo = 0, hi = K - 1;  // last j with cq[j] <= c
@@ -360,12 +365,26 @@ __global__ __launch_bounds__(256) void k_chunk_sums(const float *__restrict__ X,
     start = cstart[j] + local * KM_CHUNK;
     cnt = min(KM_CHUNK, ccount[j] - local * KM_CHUNK);
   }
-  // stage member indices (G <= 4 when Dp >= 64; otherwise loop)
+  // stage member indices (G <= 4 when Dp >= 64; otherwise read perm directly)
   const bool staged = G <= 4;
-  if (staged && c < Q && sub < G)
+  if (!qwave && staged && c < Q && sub < G)
     for (int m = t; m < cnt; m += Dp) pidx[sub][m] = perm[start + m];
   __syncthreads();
   if (c >= Q || sub >= G) return;
+  if (qwave) {
+    double acc = 0.0;
+    int m = 0;
+    for (; m + 8 <= cnt; m += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = q[staged ? pidx[sub][m + u] : perm[start + m + u]];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc += v[u];
+    }
+    for (; m < cnt; ++m) acc += q[staged ? pidx[sub][m] : perm[start + m]];
+    P[c * ld + D] = acc;
+    return;
+  }
   for (int d = t; d < D; d += Dp) {
     double acc = 0.0;
     int m = 0;
@@ -382,11 +401,12 @@ __global__ __launch_bounds__(256) void k_chunk_sums(const float *__restrict__ X,
       acc += (double)v3;
     }
     for (; m < cnt; ++m) acc += (double)X[(long)(staged ? pidx[sub][m] : perm[start + m]) * D + d];
-    P[c * D + d] = acc;
+    P[c * ld + d] = acc;
   }
 }
 
-// Per-centre CSUM over its chunk partials, added into the running block sum.
+// Per-centre CSUM over its chunk partials (D sums and the q column),
+// added into the running block sums.
 __global__ __launch_bounds__(256) void k_block_finalize(int D, int K, const int *__restrict__ ccount,
                                                         const int *__restrict__ cq,
                                                         const double *__restrict__ P,
@@ -394,19 +414,20 @@ __global__ __launch_bounds__(256) void k_block_finalize(int D, int K, const int 
                                                         long long *__restrict__ cnt) {
   const int j = blockIdx.x;
   const int q0 = cq[j], nq = cq[j + 1] - q0;
-  for (int d = threadIdx.x; d < D; d += blockDim.x) {
+  const int ld = D + 1;
+  for (int d = threadIdx.x; d <= D; d += blockDim.x) {
     double b = 0.0;
     if (nq <= KM_CHUNK) {
-      for (int t = 0; t < nq; ++t) b += P[(long)(q0 + t) * D + d];
+      for (int t = 0; t < nq; ++t) b += P[(long)(q0 + t) * ld + d];
     } else {  // nq <= KM_CHUNK^2 since KM_BLOCK <= KM_CHUNK^3
       for (int g = 0; g < nq; g += KM_CHUNK) {
         double gs = 0.0;
         const int ge = min(nq, g + KM_CHUNK);
-        for (int t = g; t < ge; ++t) gs += P[(long)(q0 + t) * D + d];
+        for (int t = g; t < ge; ++t) gs += P[(long)(q0 + t) * ld + d];
         b += gs;
       }
     }
-    S[(long)j * D + d] += b;
+    S[(long)j * ld + d] += b;
   }
   if (threadIdx.x == 0) cnt[j] += ccount[j];
 }
@@ -418,18 +439,22 @@ __global__ void k_zero_sums(long n, double *__restrict__ S, int K, long long *__
 }
 
 // New centres, transposed copy and norms; empty centres keep the old value.
+// compj[j] = (Q_j - 2 sum_d c_jd S_jd) + n_j sum_d c_jd^2, the compactness
+// contribution of centre j against the new centre (sums left to right).
 __global__ __launch_bounds__(256) void k_final_centres(int D, int K, const double *__restrict__ S,
                                                        const long long *__restrict__ cnt,
                                                        const float *C_old, float *C_new,
                                                        float *__restrict__ Ct, int ldc,
                                                        float *__restrict__ nc,
                                                        long long *__restrict__ counts,
-                                                       int *__restrict__ empty) {
+                                                       int *__restrict__ empty,
+                                                       double *__restrict__ compj) {
   extern __shared__ float cs[];
   const int j = blockIdx.x;
   const long long n = cnt[j];
+  const int ld = D + 1;
   for (int d = threadIdx.x; d < D; d += blockDim.x) {
-    const float v = n > 0 ? (float)(S[(long)j * D + d] / (double)n) : C_old[(long)j * D + d];
+    const float v = n > 0 ? (float)(S[(long)j * ld + d] / (double)n) : C_old[(long)j * D + d];
     cs[d] = v;
     C_new[(long)j * D + d] = v;
     if (Ct) Ct[(long)d * ldc + j] = v;
@@ -441,86 +466,83 @@ __global__ __launch_bounds__(256) void k_final_centres(int D, int K, const doubl
       for (int d = 0; d < D; ++d) s = __builtin_fmaf(cs[d], cs[d], s);
       nc[j] = s;
     }
+    if (compj) {
+      double a = 0.0, b = 0.0;
+      for (int d = 0; d < D; ++d) {
+        const double c = (double)cs[d];
+        a += c * S[(long)j * ld + d];
+        b += c * c;
+      }
+      compj[j] = (S[(long)j * ld + D] - 2.0 * a) + (double)n * b;
+    }
     if (counts) counts[j] = n;
     if (empty && n == 0) atomicAdd(empty, 1);
   }
 }
 
-// ---------------------------------------------------------------- compactness
-// e_i = fma chain over d of (x_id - c_{l_i,d})^2; one chunk of KM_CHUNK points
-// per block (one point per thread), summed left to right in FP64 by one
-// thread.  Rows are read in 16-byte pieces, eight in flight per lane, so each
-// lane consumes whole cache lines of its point and centre rows.
-__global__ __launch_bounds__(256) void k_compact_chunks(long i0, long nb, int D,
-                                                        const float *__restrict__ X,
-                                                        const int *__restrict__ labels,
-                                                        const float *__restrict__ C,
-                                                        double *__restrict__ E) {
-  __shared__ float ev[KM_CHUNK];
-  const long c = blockIdx.x;
-  const long r = c * KM_CHUNK + threadIdx.x;
-  float e = 0.0f;
-  if (r < nb) {
-    const long i = i0 + r;
-    const float *x = X + i * D;
-    const float *cc = C + (long)labels[i] * D;
-    if ((D & 3) == 0) {
-      const float4 *x4 = reinterpret_cast<const float4 *>(x);
-      const float4 *c4 = reinterpret_cast<const float4 *>(cc);
-#pragma unroll 8
-      for (int q = 0; q < D / 4; ++q) {
-        const float4 a = x4[q], b = c4[q];
-        float v = a.x - b.x;
-        e = __builtin_fmaf(v, v, e);
-        v = a.y - b.y;
-        e = __builtin_fmaf(v, v, e);
-        v = a.z - b.z;
-        e = __builtin_fmaf(v, v, e);
-        v = a.w - b.w;
-        e = __builtin_fmaf(v, v, e);
-      }
-    } else {
-      for (int q = 0; q < D; ++q) {
-        const float v = x[q] - cc[q];
-        e = __builtin_fmaf(v, v, e);
-      }
-    }
+// comp_j terms for arbitrary centres C (k x D) from the moments.
+__global__ void k_comp_terms(int D, int K, const double *__restrict__ S,
+                             const long long *__restrict__ cnt, const float *__restrict__ C,
+                             double *__restrict__ compj) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= K) return;
+  const int ld = D + 1;
+  double a = 0.0, b = 0.0;
+  for (int d = 0; d < D; ++d) {
+    const double c = (double)C[(long)j * D + d];
+    a += c * S[(long)j * ld + d];
+    b += c * c;
   }
-  ev[threadIdx.x] = e;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int cnt = (int)min((long)KM_CHUNK, nb - c * KM_CHUNK);
-    double s = 0.0;
-    for (int t = 0; t < cnt; ++t) s += (double)ev[t];
-    E[c] = s;
-  }
+  compj[j] = (S[(long)j * ld + D] - 2.0 * a) + (double)cnt[j] * b;
 }
 
-// CSUM of the block's chunk partials (<= KM_BLOCK/KM_CHUNK of them), added to out.
-__global__ __launch_bounds__(256) void k_compact_block(const double *__restrict__ E, int nq,
-                                                       double *__restrict__ out, int first) {
+// out = CSUM of compj[0..K) (K <= KM_CHUNK^2).
+__global__ __launch_bounds__(256) void k_comp_reduce(const double *__restrict__ compj, int K,
+                                                     double *__restrict__ out) {
   __shared__ double g[KM_CHUNK];
-  const int ng = (nq + KM_CHUNK - 1) / KM_CHUNK;
-  if (nq <= KM_CHUNK) {
+  const int ng = (K + KM_CHUNK - 1) / KM_CHUNK;
+  if (K <= KM_CHUNK) {
     if (threadIdx.x == 0) {
       double s = 0.0;
-      for (int t = 0; t < nq; ++t) s += E[t];
-      *out = first ? s : *out + s;
+      for (int t = 0; t < K; ++t) s += compj[t];
+      *out = s;
     }
     return;
   }
   for (int q = threadIdx.x; q < ng; q += blockDim.x) {
     double s = 0.0;
-    const int e = min(nq, (q + 1) * KM_CHUNK);
-    for (int t = q * KM_CHUNK; t < e; ++t) s += E[t];
+    const int e = min(K, (q + 1) * KM_CHUNK);
+    for (int t = q * KM_CHUNK; t < e; ++t) s += compj[t];
     g[q] = s;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     double s = 0.0;
     for (int q = 0; q < ng; ++q) s += g[q];
-    *out = first ? s : *out + s;
+    *out = s;
   }
+}
+
+// q_i = left-to-right FP64 sum over d of (double)x_id^2 (squares exact).
+__global__ void k_point_sq(long m, int d, const float *__restrict__ X, double *__restrict__ q) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const float *x = X + i * d;
+  double s = 0.0;
+  if ((d & 3) == 0) {
+    const float4 *x4 = reinterpret_cast<const float4 *>(x);
+#pragma unroll 8
+    for (int t = 0; t < d / 4; ++t) {
+      const float4 v = x4[t];
+      s += (double)v.x * (double)v.x;
+      s += (double)v.y * (double)v.y;
+      s += (double)v.z * (double)v.z;
+      s += (double)v.w * (double)v.w;
+    }
+  } else {
+    for (int t = 0; t < d; ++t) s += (double)x[t] * (double)x[t];
+  }
+  q[i] = s;
 }
 
 // ---------------------------------------------------------------- generator
@@ -608,10 +630,11 @@ size_t update_ws_layout(long n, int d, int k, char *base, UpdateWs *ws) {
   w.ccount = (int *)take(sizeof(int) * k);
   w.cq = (int *)take(sizeof(int) * (k + 1));
   w.perm = (int *)take(sizeof(int) * (nb > 0 ? nb : 1));
-  w.P = (double *)take(sizeof(double) * qmax * d);
-  w.S = (double *)take(sizeof(double) * (long)k * d);
+  w.P = (double *)take(sizeof(double) * qmax * (d + 1));
+  w.S = (double *)take(sizeof(double) * (long)k * (d + 1));
   w.cnt = (long long *)take(sizeof(long long) * k);
-  w.E = (double *)take(sizeof(double) * (km_cdiv(nb > 0 ? nb : 1, KM_CHUNK) + 1));
+  w.compj = (double *)take(sizeof(double) * (k + 1));
+  w.qb = (double *)take(sizeof(double) * (nb > 0 ? nb : 1));
   w.bytes = off;
   if (ws) *ws = w;
   return off;
@@ -625,8 +648,14 @@ static hipError_t scan_excl(hipStream_t s, const int *in, long L, int *tsum, int
   return hipGetLastError();
 }
 
+hipError_t launch_point_sq(hipStream_t s, long m, int d, const float *X, double *q) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_point_sq, dim3(km_cdiv(m, 256)), dim3(256), 0, s, m, d, X, q);
+  return hipGetLastError();
+}
+
 hipError_t launch_update_zero(hipStream_t s, int d, int k, double *S, long long *cnt, int *empty) {
-  const long zn = (long)k * d;
+  const long zn = (long)k * (d + 1);
   const long zmax = zn > k ? zn : k;
   hipLaunchKernelGGL(k_zero_sums, dim3(km_cdiv(zmax, 256)), dim3(256), 0, s, zn, S, k, cnt);
   if (empty) {
@@ -636,10 +665,11 @@ hipError_t launch_update_zero(hipStream_t s, int d, int k, double *S, long long 
   return hipGetLastError();
 }
 
-// Adds the canonical block sums of rows [0, n) of X (labels aligned with X)
-// into S (k x d FP64) and cnt; blocks start at row 0 of X.
+// Adds the canonical block sums of rows [0, n) of X (labels and q aligned
+// with X) into S (k x (d+1) FP64: d sums and the q column) and cnt; blocks
+// start at row 0 of X.  q == nullptr: q is computed per block into ws.qb.
 hipError_t launch_update_accumulate(hipStream_t s, long n, int d, int k, const float *X,
-                                    const int *labels, double *S, long long *cnt,
+                                    const int *labels, const double *q, double *S, long long *cnt,
                                     const UpdateWs &ws) {
   hipError_t e;
   const int SB = sub_block(k);
@@ -647,11 +677,16 @@ hipError_t launch_update_accumulate(hipStream_t s, long n, int d, int k, const f
   while ((1 << nbits) < k) ++nbits;
   int Dp = 1;
   while (Dp < d && Dp < 256) Dp <<= 1;
-  const int G = 256 / Dp;
+  const int G = 256 / Dp < 64 ? 256 / Dp : 64;  // must match k_chunk_sums
   for (long i0 = 0; i0 < n; i0 += KM_BLOCK) {
     const long nb = (n - i0) < KM_BLOCK ? (n - i0) : KM_BLOCK;
     const int nsb = (int)km_cdiv(nb, SB);
     const long L = (long)k * nsb;
+    const double *qb = q;  // indexed by row of X
+    if (!q) {
+      if ((e = launch_point_sq(s, nb, d, X + i0 * d, ws.qb)) != hipSuccess) return e;
+      qb = ws.qb - i0;  // perm holds rows of X; shift so qb[i] is row i
+    }
     hipLaunchKernelGGL(k_hist, dim3(nsb), dim3(256), sizeof(int) * k, s, labels + i0, nb, k, nsb,
                        SB, ws.hist);
     if ((e = scan_excl(s, ws.hist, L, ws.tsum, ws.offs)) != hipSuccess) return e;
@@ -660,10 +695,10 @@ hipError_t launch_update_accumulate(hipStream_t s, long n, int d, int k, const f
     hipLaunchKernelGGL(k_cluster_setup, dim3(1), dim3(256), 0, s, k, nsb, ws.offs, ws.cstart,
                        ws.ccount, ws.cq);
     const long qmax = nb / KM_CHUNK + k + 1;
-    hipLaunchKernelGGL(k_chunk_sums, dim3(km_cdiv(qmax, G)), dim3(256), 0, s, X, d, Dp, ws.perm, k,
-                       ws.cstart, ws.ccount, ws.cq, ws.P);
-    hipLaunchKernelGGL(k_block_finalize, dim3(k), dim3(d < 256 ? 64 * (int)km_cdiv(d, 64) : 256), 0,
-                       s, d, k, ws.ccount, ws.cq, ws.P, S, cnt);
+    hipLaunchKernelGGL(k_chunk_sums, dim3(km_cdiv(qmax, G)), dim3(320), 0, s, X, d, Dp, qb, ws.perm,
+                       k, ws.cstart, ws.ccount, ws.cq, ws.P);
+    hipLaunchKernelGGL(k_block_finalize, dim3(k), dim3(64 * (int)km_cdiv(d + 1 < 256 ? d + 1 : 256, 64)),
+                       0, s, d, k, ws.ccount, ws.cq, ws.P, S, cnt);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   return hipSuccess;
@@ -672,41 +707,42 @@ hipError_t launch_update_accumulate(hipStream_t s, long n, int d, int k, const f
 hipError_t launch_update_finalize(hipStream_t s, int d, int k, const double *S,
                                   const long long *cnt, const float *C_old, float *C_new,
                                   float *Ct_new, int ldc, float *nc_new, long long *counts,
-                                  int *empty) {
+                                  int *empty, double *compj) {
   hipLaunchKernelGGL(k_final_centres, dim3(k), dim3(d < 256 ? 64 * (int)km_cdiv(d, 64) : 256),
                      sizeof(float) * d, s, d, k, S, cnt, C_old, C_new, Ct_new, ldc, nc_new, counts,
-                     empty);
+                     empty, compj);
   return hipGetLastError();
 }
 
+// Full update; ws.compj receives the per-centre compactness terms against
+// the new centres (reduce with launch_comp_reduce).
 hipError_t launch_update(hipStream_t s, long n, int d, int k, const float *X, const int *labels,
-                         const float *C_old, float *C_new, float *Ct_new, int ldc, float *nc_new,
-                         long long *counts, int *empty, const UpdateWs &ws) {
+                         const double *q, const float *C_old, float *C_new, float *Ct_new, int ldc,
+                         float *nc_new, long long *counts, int *empty, const UpdateWs &ws) {
   hipError_t e;
   if ((e = launch_update_zero(s, d, k, ws.S, ws.cnt, empty)) != hipSuccess) return e;
-  if ((e = launch_update_accumulate(s, n, d, k, X, labels, ws.S, ws.cnt, ws)) != hipSuccess) return e;
+  if ((e = launch_update_accumulate(s, n, d, k, X, labels, q, ws.S, ws.cnt, ws)) != hipSuccess)
+    return e;
   return launch_update_finalize(s, d, k, ws.S, ws.cnt, C_old, C_new, Ct_new, ldc, nc_new, counts,
-                                empty);
+                                empty, ws.compj);
 }
 
-// Canonical compactness of rows [0, n); first != 0 overwrites *out, else adds.
-hipError_t launch_compactness_acc(hipStream_t s, long n, int d, const float *X, const int *labels,
-                                  const float *C, double *out, const UpdateWs &ws, int first) {
-  if (n <= 0) return first ? hipMemsetAsync(out, 0, sizeof(double), s) : hipSuccess;
-  for (long i0 = 0; i0 < n; i0 += KM_BLOCK) {
-    const long nb = (n - i0) < KM_BLOCK ? (n - i0) : KM_BLOCK;
-    const long nq = km_cdiv(nb, KM_CHUNK);
-    hipLaunchKernelGGL(k_compact_chunks, dim3(nq), dim3(KM_CHUNK), 0, s, i0, nb, d, X, labels, C,
-                       ws.E);
-    hipLaunchKernelGGL(k_compact_block, dim3(1), dim3(256), 0, s, ws.E, (int)nq, out, first);
-    first = 0;
-  }
+hipError_t launch_comp_reduce(hipStream_t s, int k, const double *compj, double *out) {
+  hipLaunchKernelGGL(k_comp_reduce, dim3(1), dim3(256), 0, s, compj, k, out);
   return hipGetLastError();
 }
 
-hipError_t launch_compactness(hipStream_t s, long n, int d, const float *X, const int *labels,
-                              const float *C, double *out, const UpdateWs &ws) {
-  return launch_compactness_acc(s, n, d, X, labels, C, out, ws, 1);
+// Compactness of labels against centres C (any centres): moments, then terms.
+hipError_t launch_compactness(hipStream_t s, long n, int d, int k, const float *X,
+                              const int *labels, const double *q, const float *C, double *out,
+                              const UpdateWs &ws) {
+  hipError_t e;
+  if ((e = launch_update_zero(s, d, k, ws.S, ws.cnt, nullptr)) != hipSuccess) return e;
+  if ((e = launch_update_accumulate(s, n, d, k, X, labels, q, ws.S, ws.cnt, ws)) != hipSuccess)
+    return e;
+  hipLaunchKernelGGL(k_comp_terms, dim3(km_cdiv(k, 256)), dim3(256), 0, s, d, k, ws.S, ws.cnt, C,
+                     ws.compj);
+  return launch_comp_reduce(s, k, ws.compj, out);
 }
 
 // nc[j] = fma chain over d of Cdk[d*ld + j]^2 (D x K layout, Normalize.cu:33-47).

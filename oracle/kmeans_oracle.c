@@ -27,8 +27,14 @@
  *   Points are cut into blocks of KM_BLOCK consecutive indices.
  *   Centre sum   S[j][d] = (((0 + B_0[j][d]) + B_1[j][d]) + ...) where
  *                B_b[j][d] = CSUM(x[i][d] : i in block b, label i == j, i asc).
- *   Compactness  comp = (((0 + E_0) + E_1) + ...), E_b = CSUM(e_i : i in b),
- *                e_i = fma chain over d of v*v with v = x[i][d] - c'[l_i][d].
+ *   Moments      Q[j] = (((0 + QB_0[j]) + QB_1[j]) + ...), QB_b[j] = CSUM(q_i : i in
+ *                block b, label i == j, i asc), q_i = left-to-right FP64 sum over
+ *                d of (double)x[i][d]^2 (each square exact in FP64).
+ *   Compactness  comp = CSUM over j of comp_j, comp_j = (Q[j] - 2 a_j) + n_j b_j,
+ *                a_j = left-to-right FP64 sum over d of (double)c'[j][d]*S[j][d],
+ *                b_j = left-to-right FP64 sum over d of (double)c'[j][d]^2; this is
+ *                sum_i |x_i - c'_{l_i}|^2 written with per-centre moments, so no
+ *                second pass over the points is needed.
  *   New centre   c'[j][d] = (float)(S[j][d] / (double)n_j) if n_j > 0, else
  *                the previous centre (the reference divides 0/0 -> NaN,
  *                ClusterCenters.cu:81; this build keeps the old centre).
@@ -220,11 +226,19 @@ double orc_csum(const double *v, long n) {
   return r;
 }
 
+/* q_i: left-to-right FP64 sum of squares (each (double)x*(double)x is exact). */
+static inline double point_sq(const float *x, int d) {
+  double q = 0.0;
+  for (int t = 0; t < d; ++t) q += (double)x[t] * (double)x[t];
+  return q;
+}
+
 /* Per-block canonical centre sums.  For block b and each centre j, collect
  * the members in ascending i (stable counting sort by label), then CSUM each
- * dimension.  Adds B_b[j][d] into S (FP64, K x d) and n_b into counts. */
+ * dimension and the squared norms.  Adds B_b[j][d] into S (FP64, K x d),
+ * QB_b[j] into Q and n_b into counts. */
 static void block_update(const float *X, long i0, long n, int d, const int *labels, int k,
-                         double *S, long long *counts) {
+                         double *S, double *Q, long long *counts) {
   long *cnt = (long *)calloc((size_t)k + 1, sizeof(long));
   for (long r = 0; r < n; ++r) cnt[labels[i0 + r] + 1]++;
   for (int j = 0; j < k; ++j) cnt[j + 1] += cnt[j];
@@ -243,25 +257,55 @@ static void block_update(const float *X, long i0, long n, int d, const int *labe
       if (m > vcap) { free(v); vcap = m; v = (double *)malloc(sizeof(double) * (size_t)m); }
       for (int q = 0; q < d; ++q) {
         for (long t = 0; t < m; ++t) v[t] = (double)X[perm[a + t] * d + q];
-        double bsum = orc_csum(v, m);
-        S[(long)j * d + q] += bsum;
+        S[(long)j * d + q] += orc_csum(v, m);
       }
+      for (long t = 0; t < m; ++t) v[t] = point_sq(X + perm[a + t] * d, d);
+      Q[j] += orc_csum(v, m);
     }
     free(v);
   }
   free(perm); free(off); free(cnt);
 }
 
-/* Update: new centres from labels (canonical order).  C_old/C_new are K x d
- * row-major; C_new may alias C_old.  counts (K) and S (K x d, FP64) optional. */
-int orc_update(const float *X, long n, int d, const int *labels, int k, const float *C_old,
-               float *C_new, long long *counts_out, double *S_out) {
-  double *S = (double *)calloc((size_t)k * d, sizeof(double));
-  long long *counts = (long long *)calloc((size_t)k, sizeof(long long));
+/* Canonical per-centre moments over all points: S (k x d), Q (k), counts. */
+void orc_moments(const float *X, long n, int d, const int *labels, int k, double *S, double *Q,
+                 long long *counts) {
+  memset(S, 0, sizeof(double) * (size_t)k * d);
+  memset(Q, 0, sizeof(double) * (size_t)k);
+  memset(counts, 0, sizeof(long long) * (size_t)k);
   for (long i0 = 0; i0 < n; i0 += KM_BLOCK) {
     long nb = (n - i0) < KM_BLOCK ? (n - i0) : KM_BLOCK;
-    block_update(X, i0, nb, d, labels, k, S, counts);
+    block_update(X, i0, nb, d, labels, k, S, Q, counts);
   }
+}
+
+/* Compactness from moments and centres C (k x d, any centres). */
+double orc_compactness_moments(const double *S, const double *Q, const long long *counts,
+                               const float *C, int k, int d) {
+  double *cj = (double *)malloc(sizeof(double) * (size_t)(k > 0 ? k : 1));
+  for (int j = 0; j < k; ++j) {
+    double a = 0.0, b = 0.0;
+    for (int q = 0; q < d; ++q) {
+      const double c = (double)C[(long)j * d + q];
+      a += c * S[(long)j * d + q];
+      b += c * c;
+    }
+    cj[j] = (Q[j] - 2.0 * a) + (double)counts[j] * b;
+  }
+  double r = orc_csum(cj, k);
+  free(cj);
+  return r;
+}
+
+/* Update: new centres from labels (canonical order).  C_old/C_new are K x d
+ * row-major; C_new may alias C_old.  counts (K), S (K x d) and Q (K) optional;
+ * comp (optional) receives the compactness against the new centres. */
+int orc_update(const float *X, long n, int d, const int *labels, int k, const float *C_old,
+               float *C_new, long long *counts_out, double *S_out, double *Q_out, double *comp) {
+  double *S = (double *)malloc(sizeof(double) * (size_t)k * d);
+  double *Q = (double *)malloc(sizeof(double) * (size_t)k);
+  long long *counts = (long long *)malloc(sizeof(long long) * (size_t)k);
+  orc_moments(X, n, d, labels, k, S, Q, counts);
   int empty = 0;
   for (int j = 0; j < k; ++j) {
     if (counts[j] > 0) {
@@ -273,40 +317,23 @@ int orc_update(const float *X, long n, int d, const int *labels, int k, const fl
         for (int q = 0; q < d; ++q) C_new[(long)j * d + q] = C_old[(long)j * d + q];
     }
   }
+  if (comp) *comp = orc_compactness_moments(S, Q, counts, C_new, k, d);
   if (counts_out) memcpy(counts_out, counts, sizeof(long long) * (size_t)k);
   if (S_out) memcpy(S_out, S, sizeof(double) * (size_t)k * d);
-  free(S); free(counts);
+  if (Q_out) memcpy(Q_out, Q, sizeof(double) * (size_t)k);
+  free(S); free(Q); free(counts);
   return empty;
 }
 
-/* Per-point compactness term e_i (FP32 fma chain of squared differences). */
-static inline float point_err(const float *x, const float *c, int d) {
-  float e = 0.0f;
-  for (int q = 0; q < d; ++q) {
-    float v = x[q] - c[q];
-    e = fmaf(v, v, e);
-  }
-  return e;
-}
-
-double orc_compactness(const float *X, long n, int d, const int *labels, const float *C, int k,
-                       float *e_out) {
-  (void)k;
-  double total = 0.0;
-  double *e = (double *)malloc(sizeof(double) * (size_t)(n < KM_BLOCK ? (n > 0 ? n : 1) : KM_BLOCK));
-  for (long i0 = 0; i0 < n; i0 += KM_BLOCK) {
-    long nb = (n - i0) < KM_BLOCK ? (n - i0) : KM_BLOCK;
-#pragma omp parallel for schedule(static)
-    for (long r = 0; r < nb; ++r) {
-      long i = i0 + r;
-      float ei = point_err(X + i * d, C + (long)labels[i] * d, d);
-      e[r] = (double)ei;
-      if (e_out) e_out[i] = ei;
-    }
-    total += orc_csum(e, nb);
-  }
-  free(e);
-  return total;
+/* Compactness of labels against arbitrary centres C (k x d). */
+double orc_compactness(const float *X, long n, int d, const int *labels, const float *C, int k) {
+  double *S = (double *)malloc(sizeof(double) * (size_t)k * d);
+  double *Q = (double *)malloc(sizeof(double) * (size_t)k);
+  long long *counts = (long long *)malloc(sizeof(long long) * (size_t)k);
+  orc_moments(X, n, d, labels, k, S, Q, counts);
+  double r = orc_compactness_moments(S, Q, counts, C, k, d);
+  free(S); free(Q); free(counts);
+  return r;
 }
 
 /* ------------------------------------------------------------------ */
@@ -323,8 +350,8 @@ int orc_lloyd(const float *X, long n, int d, int k, float *C, int maxIters, doub
   while (it < maxIters && relErr > criterion) {
     orc_col_norms(C, k, d, nc);
     orc_assign(X, n, d, C, k, nc, labels, NULL, NULL);
-    orc_update(X, n, d, labels, k, C, C, NULL, NULL);
-    double comp = orc_compactness(X, n, d, labels, C, k, NULL);
+    double comp = 0.0;
+    orc_update(X, n, d, labels, k, C, C, NULL, NULL, NULL, &comp);
     relErr = fabs(comp - prev) / fabs(comp);
     prev = comp;
     if (comp_hist) comp_hist[it] = comp;

@@ -42,8 +42,9 @@ def lib():
             "orc_assign": (None, [P, i64, i32, P, i32, P, P, P, P]),
             "orc_score": (f32, [P, P, i32, f32]),
             "orc_csum": (f64, [P, i64]),
-            "orc_update": (i32, [P, i64, i32, P, i32, P, P, P, P]),
-            "orc_compactness": (f64, [P, i64, i32, P, P, i32, P]),
+            "orc_update": (i32, [P, i64, i32, P, i32, P, P, P, P, P, P]),
+            "orc_compactness": (f64, [P, i64, i32, P, P, i32]),
+            "orc_moments": (None, [P, i64, i32, P, i32, P, P, P]),
             "orc_lloyd": (i32, [P, i64, i32, i32, P, i32, f64, P, P, P, P]),
             "orc_means_fp64": (None, [P, i64, i32, P, i32, P, P]),
             "orc_compactness_fp64": (f64, [P, i64, i32, P, P]),
@@ -124,23 +125,37 @@ def csum(v):
     return lib().orc_csum(_p(v), v.shape[0])
 
 
-def update(X, labels, C_old):
+def update(X, labels, C_old, with_comp=False):
+    """Canonical update.  Returns (C_new, counts, S, empty[, comp vs C_new])."""
     X, C_old, labels = _f32(X), _f32(C_old), _i32(labels)
     n, d = X.shape
     k = C_old.shape[0]
     C_new = np.empty_like(C_old)
     counts = np.empty(k, np.int64)
     S = np.empty((k, d), np.float64)
-    empty = lib().orc_update(_p(X), n, d, _p(labels), k, _p(C_old), _p(C_new), _p(counts), _p(S))
+    Q = np.empty(k, np.float64)
+    comp = ctypes.c_double(0)
+    empty = lib().orc_update(_p(X), n, d, _p(labels), k, _p(C_old), _p(C_new), _p(counts), _p(S),
+                             _p(Q), ctypes.byref(comp))
+    if with_comp:
+        return C_new, counts, S, empty, comp.value
     return C_new, counts, S, empty
 
 
-def compactness(X, labels, C, with_terms=False):
+def moments(X, labels, k):
+    X, labels = _f32(X), _i32(labels)
+    n, d = X.shape
+    S = np.empty((k, d), np.float64)
+    Q = np.empty(k, np.float64)
+    cnt = np.empty(k, np.int64)
+    lib().orc_moments(_p(X), n, d, _p(labels), k, _p(S), _p(Q), _p(cnt))
+    return S, Q, cnt
+
+
+def compactness(X, labels, C):
     X, C, labels = _f32(X), _f32(C), _i32(labels)
     n, d = X.shape
-    e = np.empty(n, np.float32) if with_terms else None
-    comp = lib().orc_compactness(_p(X), n, d, _p(labels), _p(C), C.shape[0], _p(e))
-    return (comp, e) if with_terms else comp
+    return lib().orc_compactness(_p(X), n, d, _p(labels), _p(C), C.shape[0])
 
 
 def lloyd(X, C0, max_iters, criterion=-1.0):

@@ -180,7 +180,7 @@ def test_compactness_fp64_and_identity(orc):
     C0 = X[:16].copy()
     lab = orc.assign(X, C0)
     C1, cnt, S, _ = orc.update(X, lab, C0)
-    comp, e = orc.compactness(X, lab, C1, with_terms=True)
+    comp = orc.compactness(X, lab, C1)
     ref = orc.compactness_fp64(X, lab, C1)
     assert abs(comp - ref) <= 1e-5 * ref
     # sum ||x - mu||^2 = sum ||x||^2 - sum_j n_j ||mu_j||^2 at the exact means
@@ -190,7 +190,14 @@ def test_compactness_fp64_and_identity(orc):
     rhs = (X64 ** 2).sum() - (cnt64[:, None] * M ** 2).sum()
     assert abs(lhs - rhs) < 1e-9 * lhs
     assert abs(comp - lhs) <= 1e-5 * lhs
-    assert np.all(e >= 0)
+    # the moments the formula uses
+    S, Q, cnt = orc.moments(X, lab, 16)
+    assert np.array_equal(cnt, cnt64)
+    assert np.allclose(Q, np.bincount(lab, (X64 ** 2).sum(1), minlength=16), rtol=1e-13)
+    assert np.allclose(S, np.stack([np.bincount(lab, X64[:, q], minlength=16) for q in range(20)], 1),
+                       rtol=1e-12, atol=1e-9)
+    _, _, _, _, comp2 = orc.update(X, lab, C0, with_comp=True)
+    assert comp2 == comp
 
 
 def test_compactness_uses_given_centres(orc):
