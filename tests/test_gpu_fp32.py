@@ -96,7 +96,7 @@ def gpu_update(X, lab, C):
             float(out.item()))
 
 
-@pytest.mark.parametrize("n,d,k", [(5000, 3, 7), (20000, 16, 64), (30000, 33, 100),
+@pytest.mark.parametrize("n,d,k", [(5000, 1, 4), (5000, 3, 7), (20000, 16, 64), (30000, 33, 100),
                                    (12345, 128, 1024), (4000, 300, 20)])
 def test_update_and_compactness_bit_exact(orc, n, d, k):
     print(f"update n={n} d={d} k={k}", flush=True)
