@@ -138,9 +138,11 @@ def compute_kmeans_f(data, k, criterion, max_iters, result=None, init_from_resul
 class KMeans:
     """The engine handle: kmeansCreate / SetData / SetCenters / Run / Get*."""
 
-    def __init__(self, m, n, k, precision="fp32", device=0, stream=None, timing=False):
+    def __init__(self, m, n, k, precision="fp32", device=0, stream=None, timing=False, tile_rows=0):
+        """tile_rows > 0 (a multiple of KM_BLOCK) with host data: stream the points
+        through HBM in tiles of that many rows instead of keeping them resident."""
         self.m, self.n, self.k = int(m), int(n), int(k)
-        opt = Options(PREC[precision], device, stream, 0, 1 if timing else 0)
+        opt = Options(PREC[precision], device, stream, int(tile_rows), 1 if timing else 0)
         h = ctypes.c_void_p()
         _check(lib().kmeansCreate(ctypes.byref(h), self.m, self.n, self.k, ctypes.byref(opt)),
                "kmeansCreate")
@@ -154,9 +156,14 @@ class KMeans:
     def __del__(self):
         self.close()
 
-    def set_data(self, data=None, device_ptr=None):
+    def set_data(self, data=None, device_ptr=None, host_ptr=None):
+        """data: NumPy array (copied to the GPU, or streamed when tile_rows is set);
+        device_ptr: device address (borrowed); host_ptr: host address (borrowed,
+        e.g. a pinned buffer, used in streamed mode)."""
         if device_ptr is not None:
             _check(lib().kmeansSetData(self._h, ctypes.c_void_p(device_ptr), 1), "kmeansSetData")
+        elif host_ptr is not None:
+            _check(lib().kmeansSetData(self._h, ctypes.c_void_p(host_ptr), 0), "kmeansSetData")
         else:
             self._host = np.ascontiguousarray(data, dtype=np.float32)
             assert self._host.shape == (self.m, self.n)

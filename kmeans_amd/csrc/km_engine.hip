@@ -35,6 +35,14 @@ Engine::~Engine() {
     if (e) hipEventDestroy(e);
   for (auto &e : iter_events)
     if (e) hipEventDestroy(e);
+  for (int b = 0; b < 2; ++b) {
+    hipFree(dXt[b]);
+    hipFree(dqt[b]);
+    hipHostFree(pin[b]);
+    if (ev_loaded[b]) hipEventDestroy(ev_loaded[b]);
+    if (ev_used[b]) hipEventDestroy(ev_used[b]);
+  }
+  if (copy_stream) hipStreamDestroy(copy_stream);
   if (own_stream && stream) hipStreamDestroy(stream);
 }
 
@@ -94,6 +102,7 @@ kmeansErrorStatus Engine::ensure_comp_slots(int nslots) {
 // kmeans.cpp:540-695 (buildData) without the row shuffle of :554-564.
 kmeansErrorStatus Engine::buildData(const float *data, int on_device) {
   if (!data) return KMEANS_ERROR_INVALID_ARGUMENT;
+  if (!on_device && opt.tile_rows > 0 && opt.tile_rows < m) return setup_streaming(data);
   if (on_device) {
     dX = data;
   } else {
@@ -140,7 +149,7 @@ kmeansErrorStatus Engine::transpose() {
 // draws rand()%m after srand(42); this build draws k distinct indices with
 // Floyd's algorithm from the counter-based generator (seed, stream 4).
 kmeansErrorStatus Engine::initialize(uint64_t seed) {
-  if (!dX) return KMEANS_ERROR_INITIALIZE;
+  if (!dX && !hX) return KMEANS_ERROR_INITIALIZE;
   std::vector<int> idx(k);
   long t = 0;
   for (long j = m - k; j < m; ++j) {
@@ -149,6 +158,12 @@ kmeansErrorStatus Engine::initialize(uint64_t seed) {
     for (long q = 0; q < t; ++q)
       if (idx[q] == r) { dup = true; break; }
     idx[t++] = dup ? (int)j : (int)r;
+  }
+  if (streaming) {  // gather the rows on the host
+    std::vector<float> c((size_t)k * n);
+    for (int j = 0; j < k; ++j)
+      memcpy(&c[(size_t)j * n], hX + (size_t)idx[j] * n, sizeof(float) * n);
+    return setCenters(c.data());
   }
   int *didx = nullptr;
   KM_HIP_CHECK(hipMalloc(&didx, sizeof(int) * k), KMEANS_ERROR_INITIALIZE);
@@ -207,7 +222,7 @@ kmeansErrorStatus Engine::constructMiniBatch(long index, long *r0, long *rows) {
 }
 
 kmeansErrorStatus Engine::compute() {
-  if (!dX || !centers_set) return KMEANS_ERROR_COMPUTE;
+  if ((!dX && !streaming) || !centers_set) return KMEANS_ERROR_COMPUTE;
   return mb_rows > 0 ? computeMiniBatch() : computeStandard();
 }
 
@@ -247,6 +262,29 @@ kmeansErrorStatus Engine::run_iterations(long r0, long rows, bool minibatch) {
     long b0 = r0, br = rows;
     if (minibatch) constructMiniBatch(mb_index++, &b0, &br);
     cur_slot = slot;
+    if (streaming) {
+      record(slot, 0);
+      record(slot, 1);
+      record(slot, 2);
+      if ((err = stream_iteration(slot)) != KMEANS_SUCCESS) return err;
+      record(slot, 3);
+      record(slot, 4);
+      ++iters;
+      ++slot;
+      if (sync_each) {
+        double comp = 0.0;
+        KM_HIP_CHECK(hipMemcpyAsync(&comp, dcomp + slot - 1, sizeof(double),
+                                    hipMemcpyDeviceToHost, stream),
+                     KMEANS_ERROR_COMPUTE);
+        KM_HIP_CHECK(hipStreamSynchronize(stream), KMEANS_ERROR_COMPUTE);
+        const double prev = compactnessScore;
+        compactnessScore = comp;
+        relErr = fabs(comp - prev) / fabs(comp);
+      } else {
+        relErr = 1.0 + fabs(criterion);
+      }
+      continue;
+    }
     record(slot, 0);
     if ((err = closestCenters(b0, br)) != KMEANS_SUCCESS) return err;
     record(slot, 1);
@@ -302,6 +340,99 @@ kmeansErrorStatus Engine::run_iterations(long r0, long rows, bool minibatch) {
   stats.compactness = compactnessScore;
   stats.rel_err = relErr;
   (void)it0;
+  return KMEANS_SUCCESS;
+}
+
+// Host-resident points streamed through HBM (kmeans.cpp:302-536 tiling and
+// :700-789 copyTileToDevice, here asynchronous and double-buffered).  Tiles
+// are multiples of KM_BLOCK rows, so the canonical sums equal the resident
+// path's; the compactness comes from the update's moments, so each
+// iteration reads the points exactly once.
+kmeansErrorStatus Engine::setup_streaming(const float *host) {
+  if (opt.tile_rows % KM_BLOCK != 0) {
+    fprintf(stderr, "kmeans_amd: tile_rows must be a multiple of KM_BLOCK (%ld)\n", (long)KM_BLOCK);
+    return KMEANS_ERROR_TILING;
+  }
+  streaming = true;
+  tile = opt.tile_rows;
+  hX = host;
+  hipPointerAttribute_t attr;
+  host_pinned = hipPointerGetAttributes(&attr, host) == hipSuccess &&
+                attr.type == hipMemoryTypeHost;
+  (void)hipGetLastError();  // a pageable pointer leaves an error code behind
+  for (int b = 0; b < 2; ++b) {
+    KM_HIP_CHECK(hipMalloc(&dXt[b], sizeof(float) * (size_t)tile * n), KMEANS_ERROR_TILING);
+    KM_HIP_CHECK(hipMalloc(&dqt[b], sizeof(double) * (size_t)tile), KMEANS_ERROR_TILING);
+    if (!host_pinned)
+      KM_HIP_CHECK(hipHostMalloc(&pin[b], sizeof(float) * (size_t)tile * n, hipHostMallocDefault),
+                   KMEANS_ERROR_TILING);
+    KM_HIP_CHECK(hipEventCreateWithFlags(&ev_loaded[b], hipEventDisableTiming), KMEANS_ERROR_TILING);
+    KM_HIP_CHECK(hipEventCreateWithFlags(&ev_used[b], hipEventDisableTiming), KMEANS_ERROR_TILING);
+  }
+  KM_HIP_CHECK(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking), KMEANS_ERROR_TILING);
+  // max |x| for the FP16 scale: one host pass
+  if (opt.precision == KM_PREC_FP16) {
+    float mx = 0.0f;
+    const size_t tot = (size_t)m * n;
+    for (size_t i = 0; i < tot; ++i) mx = fmaxf(mx, fabsf(host[i]));
+    x_absmax = mx;
+  }
+  return KMEANS_SUCCESS;
+}
+
+// Issue the copy of tile t into buffer b on the copy stream.
+kmeansErrorStatus Engine::load_tile(long t, int b) {
+  const long r0 = t * tile, rows = std::min(tile, m - r0);
+  KM_HIP_CHECK(hipStreamWaitEvent(copy_stream, ev_used[b], 0), KMEANS_ERROR_COPY_TILE);
+  const float *src = hX + r0 * n;
+  if (!host_pinned) {
+    // the previous copy out of this staging buffer must be finished
+    KM_HIP_CHECK(hipEventSynchronize(ev_loaded[b]), KMEANS_ERROR_COPY_TILE);
+    memcpy(pin[b], src, sizeof(float) * (size_t)rows * n);
+    src = pin[b];
+  }
+  KM_HIP_CHECK(hipMemcpyAsync(dXt[b], src, sizeof(float) * (size_t)rows * n, hipMemcpyHostToDevice,
+                              copy_stream),
+               KMEANS_ERROR_COPY_TILE);
+  KM_HIP_CHECK(hipEventRecord(ev_loaded[b], copy_stream), KMEANS_ERROR_COPY_TILE);
+  return KMEANS_SUCCESS;
+}
+
+kmeansErrorStatus Engine::stream_iteration(int slot) {
+  kmeansErrorStatus err;
+  const long nt = (m + tile - 1) / tile;
+  KM_HIP_CHECK(launch_update_zero(stream, n, k, ws.S, ws.cnt, dempty), KMEANS_ERROR_CLUSTER_CENTERS);
+  // the first tile may reuse buffer 0 only after the previous iteration used it
+  if ((err = load_tile(0, 0)) != KMEANS_SUCCESS) return err;
+  for (long t = 0; t < nt; ++t) {
+    const int b = (int)(t & 1);
+    const long r0 = t * tile, rows = std::min(tile, m - r0);
+    if (t + 1 < nt)
+      if ((err = load_tile(t + 1, b ^ 1)) != KMEANS_SUCCESS) return err;
+    KM_HIP_CHECK(hipStreamWaitEvent(stream, ev_loaded[b], 0), KMEANS_ERROR_COPY_TILE);
+    KM_HIP_CHECK(launch_point_sq(stream, rows, n, dXt[b], dqt[b]), KMEANS_ERROR_COPY_TILE);
+    if (opt.precision == KM_PREC_FP32) {
+      KM_HIP_CHECK(launch_assign_f32(stream, rows, nullptr, n, dXt[b], nullptr, k, dCt, ldc, dnc,
+                                     dlabels + r0, nullptr),
+                   KMEANS_ERROR_CLOSEST_CENTERS);
+    } else {
+      KM_HIP_CHECK(mfma_prepare_rows(this, dXt[b], rows), KMEANS_ERROR_CLOSEST_CENTERS);
+      KM_HIP_CHECK(mfma_assign_view(this, dXt[b], mfma_tile_xh(this), mfma_tile_xnorm(this),
+                                    dlabels + r0, rows),
+                   KMEANS_ERROR_CLOSEST_CENTERS);
+      KM_HIP_CHECK(mfma_reresolve_view(this, dXt[b], dlabels + r0, rows),
+                   KMEANS_ERROR_CLOSEST_CENTERS);
+    }
+    KM_HIP_CHECK(launch_update_accumulate(stream, rows, n, k, dXt[b], dlabels + r0, dqt[b], ws.S,
+                                          ws.cnt, ws),
+                 KMEANS_ERROR_CLUSTER_CENTERS);
+    KM_HIP_CHECK(hipEventRecord(ev_used[b], stream), KMEANS_ERROR_COPY_TILE);
+  }
+  KM_HIP_CHECK(launch_update_finalize(stream, n, k, ws.S, ws.cnt, dC, dC, dCt, ldc, dnc, dcounts,
+                                      dempty, ws.compj),
+               KMEANS_ERROR_CLUSTER_CENTERS);
+  if (opt.precision != KM_PREC_FP32) KM_HIP_CHECK(mfma_centers(this), KMEANS_ERROR_CLUSTER_CENTERS);
+  KM_HIP_CHECK(launch_comp_reduce(stream, k, ws.compj, dcomp + slot), KMEANS_ERROR_COMPACTNESS);
   return KMEANS_SUCCESS;
 }
 

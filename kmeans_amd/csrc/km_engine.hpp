@@ -50,6 +50,19 @@ struct Engine {
   long long last_reresolved = 0;
   kmeansStats stats{};
 
+  // streamed mode: points stay in host memory and pass through HBM in tiles
+  // of `tile` rows (a multiple of KM_BLOCK) on a copy stream, double-buffered
+  bool streaming = false;
+  long tile = 0;
+  const float *hX = nullptr;
+  bool host_pinned = false;
+  float x_absmax = 0.0f;          // max |x| over the streamed data (FP16 scale)
+  float *dXt[2] = {nullptr, nullptr};
+  double *dqt[2] = {nullptr, nullptr};
+  float *pin[2] = {nullptr, nullptr};  // staging when the host buffer is pageable
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t ev_loaded[2] = {nullptr, nullptr}, ev_used[2] = {nullptr, nullptr};
+
   // mini-batch (kmeans.cpp:1253-1312)
   long mb_rows = 0;   // 0: standard loop
   long mb_index = 0;
@@ -78,6 +91,9 @@ struct Engine {
   }
   kmeansErrorStatus ensure_comp_slots(int n);
   kmeansErrorStatus run_iterations(long r0, long rows, bool minibatch);
+  kmeansErrorStatus setup_streaming(const float *host);
+  kmeansErrorStatus load_tile(long t, int b);
+  kmeansErrorStatus stream_iteration(int slot);
   void record(int slot, int phase);
   void collect_timing(int nslots);
 };

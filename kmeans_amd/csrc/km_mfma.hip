@@ -475,13 +475,14 @@ static hipError_t ensure_state(Engine *e) {
   s->Dp = s->KS * 16;
   s->Kp = (int)km_cdiv(e->k, MF_TILE * MF_STAGE) * MF_TILE * MF_STAGE;
   hipError_t r;
-  if ((r = hipMalloc(&s->Xh, sizeof(uint16_t) * (size_t)e->m * s->Dp)) != hipSuccess) return r;
+  const long xrows = e->streaming ? e->tile : e->m;  // points held in HBM at once
+  if ((r = hipMalloc(&s->Xh, sizeof(uint16_t) * (size_t)xrows * s->Dp)) != hipSuccess) return r;
   if ((r = hipMalloc(&s->Ch, sizeof(uint16_t) * (size_t)s->Kp * s->Dp)) != hipSuccess) return r;
   if ((r = hipMalloc(&s->bias, sizeof(float) * s->Kp)) != hipSuccess) return r;
   if ((r = hipMalloc(&s->bnd, sizeof(float) * 4)) != hipSuccess) return r;
-  if ((r = hipMalloc(&s->xnorm, sizeof(float) * (size_t)e->m)) != hipSuccess) return r;
-  if ((r = hipMalloc(&s->flags, sizeof(int) * (size_t)e->m)) != hipSuccess) return r;
-  if ((r = hipMalloc(&s->cands, sizeof(int4) * (size_t)e->m)) != hipSuccess) return r;
+  if ((r = hipMalloc(&s->xnorm, sizeof(float) * (size_t)xrows)) != hipSuccess) return r;
+  if ((r = hipMalloc(&s->flags, sizeof(int) * (size_t)xrows)) != hipSuccess) return r;
+  if ((r = hipMalloc(&s->cands, sizeof(int4) * (size_t)xrows)) != hipSuccess) return r;
   s->nflag_slots = 64;
   if ((r = hipMalloc(&s->nflag, sizeof(int) * s->nflag_slots)) != hipSuccess) return r;
   if ((r = hipMalloc(&s->ncand, sizeof(int) * s->nflag_slots)) != hipSuccess) return r;
@@ -522,6 +523,8 @@ __global__ void k_absmax(long n, const float *__restrict__ v, unsigned *__restri
   atomicMax(out, __float_as_uint(m));
 }
 
+static hipError_t prepare_centres(Engine *e);
+
 static hipError_t prepare_x(Engine *e) {
   MfmaState *s = st(e);
   if (s->x_ready) return hipSuccess;
@@ -530,8 +533,10 @@ static hipError_t prepare_x(Engine *e) {
   if (s->fp16) {
     unsigned *dm = nullptr;
     if ((r = hipMalloc(&dm, sizeof(unsigned))) != hipSuccess) return r;
-    hipMemsetAsync(dm, 0, sizeof(unsigned), e->stream);
-    hipLaunchKernelGGL(k_absmax, dim3(1024), dim3(256), 0, e->stream, (long)e->m * e->n, e->dX, dm);
+    const unsigned hx = __builtin_bit_cast(unsigned, e->x_absmax);  // streamed data: host pass
+    hipMemcpyAsync(dm, &hx, sizeof(unsigned), hipMemcpyHostToDevice, e->stream);
+    if (!e->streaming)
+      hipLaunchKernelGGL(k_absmax, dim3(1024), dim3(256), 0, e->stream, (long)e->m * e->n, e->dX, dm);
     hipLaunchKernelGGL(k_absmax, dim3(64), dim3(256), 0, e->stream, (long)e->k * e->n, e->dC, dm);
     unsigned um = 0;
     hipMemcpyAsync(&um, dm, sizeof(unsigned), hipMemcpyDeviceToHost, e->stream);
@@ -543,14 +548,35 @@ static hipError_t prepare_x(Engine *e) {
     if (mx > 0.0f && isfinite(mx)) frexpf(mx, &ex);
     s->scale = ldexpf(1.0f, 14 - ex);
   }
-  const long tot = (long)e->m * s->Dp;
-  hipLaunchKernelGGL(k_convert_rows, dim3(km_cdiv(tot, 256)), dim3(256), 0, e->stream, e->m, e->n,
-                     s->Dp, e->dX, s->scale, (int)s->fp16, s->Xh);
-  hipLaunchKernelGGL(k_xnorm, dim3(km_cdiv(e->m, 256)), dim3(256), 0, e->stream, e->m, e->n, e->dX,
-                     s->xnorm);
-  if ((r = hipGetLastError()) != hipSuccess) return r;
+  if (!e->streaming) {
+    const long tot = (long)e->m * s->Dp;
+    hipLaunchKernelGGL(k_convert_rows, dim3(km_cdiv(tot, 256)), dim3(256), 0, e->stream, e->m,
+                       e->n, s->Dp, e->dX, s->scale, (int)s->fp16, s->Xh);
+    hipLaunchKernelGGL(k_xnorm, dim3(km_cdiv(e->m, 256)), dim3(256), 0, e->stream, e->m, e->n,
+                       e->dX, s->xnorm);
+    if ((r = hipGetLastError()) != hipSuccess) return r;
+  }
   s->x_ready = true;
   return hipSuccess;
+}
+
+// Streamed tiles: reduced-precision copy and norm bounds of rows [0, rows)
+// of a tile buffer into the state's tile-sized arrays.
+hipError_t mfma_prepare_rows(Engine *e, const float *X, long rows) {
+  hipError_t r;
+  if ((r = ensure_state(e)) != hipSuccess) return r;
+  MfmaState *s = st(e);
+  if (!s->x_ready)
+    if ((r = prepare_x(e)) != hipSuccess) return r;
+  if (!s->c_ready)
+    if ((r = prepare_centres(e)) != hipSuccess) return r;
+  if (rows <= 0) return hipSuccess;
+  const long tot = rows * s->Dp;
+  hipLaunchKernelGGL(k_convert_rows, dim3(km_cdiv(tot, 256)), dim3(256), 0, e->stream, rows, e->n,
+                     s->Dp, X, s->scale, (int)s->fp16, s->Xh);
+  hipLaunchKernelGGL(k_xnorm, dim3(km_cdiv(rows, 256)), dim3(256), 0, e->stream, rows, e->n, X,
+                     s->xnorm);
+  return hipGetLastError();
 }
 
 static hipError_t prepare_centres(Engine *e) {
@@ -607,6 +633,16 @@ hipError_t mfma_assign(Engine *e, long r0, long rows) {
     if ((r = prepare_x(e)) != hipSuccess) return r;
     s->c_ready = false;
   }
+  return mfma_assign_view(e, e->dX + r0 * e->n, s->Xh + r0 * s->Dp, s->xnorm + r0,
+                          e->dlabels + r0, rows);
+}
+
+// Assignment of `rows` points: FP32 rows X, their reduced-precision copy Xh
+// and norm bounds xn, labels written to lab (all indexed from 0).
+hipError_t mfma_assign_view(Engine *e, const float *X, const uint16_t *Xh, const float *xn,
+                            int *lab, long rows) {
+  hipError_t r;
+  MfmaState *s = st(e);
   if (!s->c_ready)
     if ((r = prepare_centres(e)) != hipSuccess) return r;
   if ((r = mfma_ensure_slots(e, e->cur_slot + 1)) != hipSuccess) return r;
@@ -619,16 +655,13 @@ hipError_t mfma_assign(Engine *e, long r0, long rows) {
   const float eta = s->fp16 ? 2.9802322e-8f / s->scale : 0.0f;  // half FP16 subnormal step
   const float inv_s2 = 1.0f / (s->scale * s->scale);
   dim3 g(km_cdiv(rows, MF_PTS));
-  const uint16_t *Xh = s->Xh + r0 * s->Dp;
-  const float *X = e->dX + r0 * e->n;
-  int *lab = e->dlabels + r0;
   switch (s->KS) {
-    case 1: launch_mfma_ks<1>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, s->xnorm + r0, inv_s2, ub, eta, lab, s->flags, nf, s->cands, nc2); break;
-    case 2: launch_mfma_ks<2>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, s->xnorm + r0, inv_s2, ub, eta, lab, s->flags, nf, s->cands, nc2); break;
-    case 4: launch_mfma_ks<4>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, s->xnorm + r0, inv_s2, ub, eta, lab, s->flags, nf, s->cands, nc2); break;
-    case 6: launch_mfma_ks<6>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, s->xnorm + r0, inv_s2, ub, eta, lab, s->flags, nf, s->cands, nc2); break;
-    case 8: launch_mfma_ks<8>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, s->xnorm + r0, inv_s2, ub, eta, lab, s->flags, nf, s->cands, nc2); break;
-    default: launch_mfma_ks<16>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, s->xnorm + r0, inv_s2, ub, eta, lab, s->flags, nf, s->cands, nc2); break;
+    case 1: launch_mfma_ks<1>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, ub, eta, lab, s->flags, nf, s->cands, nc2); break;
+    case 2: launch_mfma_ks<2>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, ub, eta, lab, s->flags, nf, s->cands, nc2); break;
+    case 4: launch_mfma_ks<4>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, ub, eta, lab, s->flags, nf, s->cands, nc2); break;
+    case 6: launch_mfma_ks<6>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, ub, eta, lab, s->flags, nf, s->cands, nc2); break;
+    case 8: launch_mfma_ks<8>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, ub, eta, lab, s->flags, nf, s->cands, nc2); break;
+    default: launch_mfma_ks<16>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, ub, eta, lab, s->flags, nf, s->cands, nc2); break;
   }
   if ((r = hipGetLastError()) != hipSuccess) return r;
   const long rg = km_cdiv(2 * rows, 256);
@@ -678,10 +711,17 @@ hipError_t mfma_prepare(Engine *e) {
 
 // FP32 full search for the uncertified points (device-side count).
 hipError_t mfma_reresolve(Engine *e, long r0, long rows) {
-  MfmaState *s = st(e);
-  return launch_assign_f32(e->stream, rows, s->nflag + e->cur_slot, e->n, e->dX + r0 * e->n, s->flags,
-                           e->k, e->dCt, e->ldc, e->dnc, e->dlabels + r0, nullptr);
+  return mfma_reresolve_view(e, e->dX + r0 * e->n, e->dlabels + r0, rows);
 }
+
+hipError_t mfma_reresolve_view(Engine *e, const float *X, int *lab, long rows) {
+  MfmaState *s = st(e);
+  return launch_assign_f32(e->stream, rows, s->nflag + e->cur_slot, e->n, X, s->flags, e->k,
+                           e->dCt, e->ldc, e->dnc, lab, nullptr);
+}
+
+const uint16_t *mfma_tile_xh(Engine *e) { return st(e)->Xh; }
+const float *mfma_tile_xnorm(Engine *e) { return st(e)->xnorm; }
 
 void mfma_collect(Engine *e, int nslots) {
   MfmaState *s = st(e);

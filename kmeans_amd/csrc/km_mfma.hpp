@@ -13,4 +13,10 @@ void mfma_collect(Engine *e, int nslots);
 void mfma_centers_changed(Engine *e, bool new_range);
 hipError_t mfma_ensure_slots(Engine *e, int n);
 hipError_t mfma_prepare(Engine *e);
+hipError_t mfma_prepare_rows(Engine *e, const float *X, long rows);
+hipError_t mfma_assign_view(Engine *e, const float *X, const uint16_t *Xh, const float *xn,
+                            int *lab, long rows);
+hipError_t mfma_reresolve_view(Engine *e, const float *X, int *lab, long rows);
+const uint16_t *mfma_tile_xh(Engine *e);
+const float *mfma_tile_xnorm(Engine *e);
 }  // namespace km

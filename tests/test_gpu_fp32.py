@@ -262,3 +262,40 @@ def test_generator_matches_cpu(orc):
     torch.cuda.synchronize()
     assert np.array_equal(zg.cpu().numpy(), z)
     assert np.allclose(Xg.cpu().numpy(), X, rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
+def test_streamed_tiles_equal_resident(orc, prec):
+    # points in host memory, two KM_BLOCK tiles plus a ragged one through HBM
+    n, d, k = 2 * km.KM_BLOCK + 12345, 4, 16
+    X, _, _, _ = orc.mixture(51, n, d, k, -20, 20, 1.0, 1.0)
+    C0 = X[:k].copy()
+    res = km.KMeans(n, d, k, precision=prec)
+    res.set_data(X)
+    res.set_centers(C0)
+    res.run(3)
+    st = km.KMeans(n, d, k, precision=prec, tile_rows=km.KM_BLOCK)
+    st.set_data(X)  # pageable NumPy memory: staged through pinned buffers
+    st.set_centers(C0)
+    st.run(3)
+    assert np.array_equal(st.labels(), res.labels())
+    assert np.array_equal(st.centers(), res.centers())
+    assert st.stats()["compactness"] == res.stats()["compactness"]
+    # pinned host memory: direct asynchronous copies
+    Xp = torch.from_numpy(X).pin_memory()
+    st2 = km.KMeans(n, d, k, precision=prec, tile_rows=km.KM_BLOCK)
+    st2.set_data(host_ptr=Xp.data_ptr())
+    st2.set_centers(C0)
+    st2.run(3)
+    assert np.array_equal(st2.centers(), res.centers())
+    if prec == "fp32":
+        ref = orc.lloyd(X, C0, 3)
+        assert np.array_equal(st.centers(), ref["centers"])
+        assert st.stats()["compactness"] == ref["comp_hist"][-1]
+
+
+def test_streamed_rejects_unaligned_tiles(orc):
+    X = np.zeros((1000, 2), np.float32)
+    e = km.KMeans(1000, 2, 4, tile_rows=500)
+    with pytest.raises(km.KmeansError):
+        e.set_data(X)
