@@ -215,6 +215,21 @@ kmeansCudaErrorStatus kmGenerateMixture(void *stream, uint64_t seed, long i0, lo
 kmeansCudaErrorStatus kmGatherRows(void *stream, long n, int d, const float *X, const int *idx,
                                    float *out);
 
+/* Candidate-group proposer MLP (din -> hidden -> hidden -> groups, ReLU,
+ * softmax cross-entropy; BF16 products, FP32 accumulation and parameters).
+ * No counterpart in the reference: it serves the learned pruning of the
+ * assignment.  Parameters are one flat FP32 vector: W1 (hidden x din), b1,
+ * W2 (hidden x hidden), b2, W3 (groups x hidden), b3.  din must be a
+ * multiple of 16.  Exposed so its gradients can be checked independently. */
+size_t kmMlpParamCount(int din, int hidden, int groups);
+size_t kmMlpWorkspaceSize(int din, int hidden, int groups, int batch);
+/* One forward + backward pass: X (batch x din BF16 bits), targets (batch),
+ * P (params), G (gradients out), loss (device float, batch mean). */
+kmeansCudaErrorStatus kmMlpForwardBackward(void *stream, int din, int hidden, int groups,
+                                           int batch, const uint16_t *X, const int *targets,
+                                           const float *P, float *G, float *loss,
+                                           void *workspace, size_t workspace_bytes);
+
 /* =========================================================================
  * 3. Legacy kernel names (KmeansCudaKernels.h:238-474), default stream.
  * Scratch arguments of the reference (Bpadded, C, Cindices, centers_large,

@@ -89,6 +89,9 @@ SIGNATURES = {
     "kmRowNorms": (_I, [_P, _L, _L, _I, _P, _P]),
     "kmGenerateMixture": (_I, [_P, _U64, _L, _L, _I, _I, _P, _P, _P, _P, _P]),
     "kmGatherRows": (_I, [_P, _L, _I, _P, _P, _P]),
+    "kmMlpParamCount": (_SZ, [_I, _I, _I]),
+    "kmMlpWorkspaceSize": (_SZ, [_I, _I, _I, _I]),
+    "kmMlpForwardBackward": (_I, [_P, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _SZ]),
     "ClosestCentersF": (_I, [_I, _I, _I, _P, _I, _P, _P, _P, _P, _I, _P, _P, _P]),
     "ClusterCentersF": (_I, [_I, _I, _I, _I, ctypes.c_bool, _P, _P, _P, _P, _P, _P]),
     "CompactnessF": (_I, [_I, _I, ctypes.c_bool, _I, _I, _P, _P, _P, _P]),
@@ -258,3 +261,18 @@ def generate_mixture(seed, i0, n, d, ncomp, means_ptr, sigmas_ptr, cdf_ptr, X_pt
 def gather_rows(n, d, X_ptr, idx_ptr, out_ptr, stream=None):
     _check(lib().kmGatherRows(_vp(stream), n, d, _vp(X_ptr), _vp(idx_ptr), _vp(out_ptr)),
            "kmGatherRows")
+
+
+def mlp_param_count(din, hidden, groups):
+    return lib().kmMlpParamCount(din, hidden, groups)
+
+
+def mlp_workspace_size(din, hidden, groups, batch):
+    return lib().kmMlpWorkspaceSize(din, hidden, groups, batch)
+
+
+def mlp_forward_backward(din, hidden, groups, batch, X_ptr, t_ptr, P_ptr, G_ptr, loss_ptr, ws_ptr,
+                         ws_bytes, stream=None):
+    _check(lib().kmMlpForwardBackward(_vp(stream), din, hidden, groups, batch, _vp(X_ptr), _vp(t_ptr),
+                                      _vp(P_ptr), _vp(G_ptr), _vp(loss_ptr), _vp(ws_ptr), ws_bytes),
+           "kmMlpForwardBackward")

@@ -54,9 +54,11 @@ int sub_block(int k);
 
 hipError_t launch_col_norms(hipStream_t s, int k, int d, const float *C, float *nc);
 hipError_t launch_transpose(hipStream_t s, int k, int d, const float *C, float *Ct, int ldc);
+// rows (nullable): point indices of the m rows; out_pos (nullable): output
+// slot of each row (default: rows[p], else p)
 hipError_t launch_assign_f32(hipStream_t s, long m, const int *m_dev, int d, const float *X,
                              const int *rows, int k, const float *Ct, int ldc, const float *nc,
-                             int *labels, float *best);
+                             int *labels, float *best, const int *out_pos = nullptr);
 hipError_t launch_update(hipStream_t s, long n, int d, int k, const float *X, const int *labels,
                          const double *q, const float *C_old, float *C_new, float *Ct_new, int ldc,
                          float *nc_new, long long *counts, int *empty, const UpdateWs &ws);

@@ -15,6 +15,7 @@
 
 #include "km_engine.hpp"
 #include "km_mfma.hpp"
+#include "km_mlp.hpp"
 
 namespace km {
 
@@ -714,6 +715,28 @@ kmeansCudaErrorStatus kmGatherRows(void *stream, long n, int d, const float *X, 
                                    float *out) {
   if (n < 0 || d <= 0 || !X || !idx || !out) return ERROR_INVALID_ARGUMENT;
   KM_K_CHECK(km::launch_gather_rows((hipStream_t)stream, n, d, X, idx, out), ERROR_INVALID_ARGUMENT);
+  return NO_ERROR;
+}
+
+size_t kmMlpParamCount(int din, int hidden, int groups) {
+  return km::mlp_param_count(km::MlpShape{din, hidden, groups});
+}
+
+size_t kmMlpWorkspaceSize(int din, int hidden, int groups, int batch) {
+  return km::mlp_ws_bytes(km::MlpShape{din, hidden, groups}, batch);
+}
+
+kmeansCudaErrorStatus kmMlpForwardBackward(void *stream, int din, int hidden, int groups,
+                                           int batch, const uint16_t *X, const int *targets,
+                                           const float *P, float *G, float *loss,
+                                           void *workspace, size_t workspace_bytes) {
+  if (din <= 0 || din % 16 || hidden <= 0 || hidden % 16 || groups <= 0 || batch <= 0 || !X ||
+      !targets || !P || !G || !workspace)
+    return ERROR_INVALID_ARGUMENT;
+  const km::MlpShape sh{din, hidden, groups};
+  if (km::mlp_ws_bytes(sh, batch) > workspace_bytes) return ERROR_INVALID_ARGUMENT;
+  KM_K_CHECK(km::mlp_fwd_bwd((hipStream_t)stream, sh, batch, X, targets, P, G, loss, workspace),
+             ERROR_MATMATMULT);
   return NO_ERROR;
 }
 

@@ -79,6 +79,7 @@ __global__ __launch_bounds__(256) void k_assign_f32(long M, const int *__restric
                                                     const int *__restrict__ rows, int K,
                                                     const float *__restrict__ Ct, int ldc,
                                                     const float *__restrict__ nc,
+                                                    const int *__restrict__ out_pos,
                                                     int *__restrict__ labels,
                                                     float *__restrict__ best) {
   __shared__ __attribute__((aligned(16))) float Xs[A_DS][A_BM + 4];
@@ -189,7 +190,7 @@ __global__ __launch_bounds__(256) void k_assign_f32(long M, const int *__restric
     for (int p = 0; p < 8; ++p) {
       const long pt = r0 + pg * 8 + p;
       if (cg == p && pt < M) {
-        const long o = rows ? (long)rows[pt] : pt;
+        const long o = out_pos ? (long)out_pos[pt] : rows ? (long)rows[pt] : pt;
         labels[o] = bi[p];
         if (best) best[o] = bv[p];
       }
@@ -598,12 +599,12 @@ hipError_t launch_transpose(hipStream_t s, int k, int d, const float *C, float *
 
 hipError_t launch_assign_f32(hipStream_t s, long m, const int *m_dev, int d, const float *X,
                              const int *rows, int k, const float *Ct, int ldc, const float *nc,
-                             int *labels, float *best) {
+                             int *labels, float *best, const int *out_pos) {
   long tiles = km_cdiv(m, A_BM);
   if (tiles <= 0) return hipSuccess;
   long grid = tiles < (long)num_cus() * 4 ? tiles : (long)num_cus() * 4;
   hipLaunchKernelGGL(k_assign_f32, dim3(grid), dim3(256), 0, s, m, m_dev, d, X, rows, k, Ct, ldc,
-                     nc, labels, best);
+                     nc, out_pos, labels, best);
   return hipGetLastError();
 }
 

@@ -404,3 +404,121 @@ double orc_compactness_fp64(const float *X, long n, int d, const int *labels, co
 
 int orc_num_threads(void) { return omp_get_max_threads(); }
 void orc_set_threads(int t) { omp_set_num_threads(t); }
+
+/* ------------------------------------------------------------------ */
+/* Candidate-group proposer MLP (no reference counterpart: the learned
+ * pruning of the assignment).  din -> h -> h -> g, ReLU, softmax
+ * cross-entropy, mean over the batch.  Parameters as one flat vector:
+ * W1 (h x din), b1 (h), W2 (h x h), b2 (h), W3 (g x h), b3 (g).  FP64
+ * arithmetic; with mirror != 0 the operands of every product are rounded to
+ * BF16 where the GPU rounds them (weights, hidden activations, the three
+ * back-propagated signals), so the comparison isolates accumulation error. */
+static double bf16r(double v) {
+  float f = (float)v;
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) != 0x7f800000u) {
+    u += 0x7fffu + ((u >> 16) & 1u);
+    u &= 0xffff0000u;
+    memcpy(&f, &u, 4);
+  }
+  return (double)f;
+}
+
+double orc_mlp_fwd_bwd(int din, int h, int g, int B, const float *X, const int *t, const double *P,
+                       double *G, int mirror) {
+  const double *W1 = P, *b1 = W1 + (long)h * din, *W2 = b1 + h, *b2 = W2 + (long)h * h;
+  const double *W3 = b2 + h, *b3 = W3 + (long)g * h;
+  double *gW1 = G, *gb1 = gW1 + (long)h * din, *gW2 = gb1 + h, *gb2 = gW2 + (long)h * h;
+  double *gW3 = gb2 + h, *gb3 = gW3 + (long)g * h;
+  const long P_n = (long)h * din + h + (long)h * h + h + (long)g * h + g;
+  memset(G, 0, sizeof(double) * (size_t)P_n);
+#define R(v) (mirror ? bf16r(v) : (v))
+  double *w1 = malloc(sizeof(double) * (size_t)h * din), *w2 = malloc(sizeof(double) * (size_t)h * h);
+  double *w3 = malloc(sizeof(double) * (size_t)g * h);
+  for (long i = 0; i < (long)h * din; ++i) w1[i] = R(W1[i]);
+  for (long i = 0; i < (long)h * h; ++i) w2[i] = R(W2[i]);
+  for (long i = 0; i < (long)g * h; ++i) w3[i] = R(W3[i]);
+  double *H1 = malloc(sizeof(double) * (size_t)B * h), *H2 = malloc(sizeof(double) * (size_t)B * h);
+  double *dZ = malloc(sizeof(double) * (size_t)B * g), *dH = malloc(sizeof(double) * (size_t)B * h);
+  double *dHr = malloc(sizeof(double) * (size_t)B * h), *dZr = malloc(sizeof(double) * (size_t)B * g);
+  double loss = 0.0;
+  double *z = calloc((size_t)g, sizeof(double));
+  for (int r = 0; r < B; ++r) {
+    const float *x = X + (long)r * din;
+    for (int o = 0; o < h; ++o) {
+      double a = 0.0;
+      for (int i = 0; i < din; ++i) a += w1[(long)o * din + i] * (double)x[i];
+      a += b1[o];
+      H1[(long)r * h + o] = R(a > 0.0 ? a : 0.0);
+    }
+    for (int o = 0; o < h; ++o) {
+      double a = 0.0;
+      for (int i = 0; i < h; ++i) a += w2[(long)o * h + i] * H1[(long)r * h + i];
+      a += b2[o];
+      H2[(long)r * h + o] = R(a > 0.0 ? a : 0.0);
+    }
+    double mx = -INFINITY, se = 0.0;
+    for (int o = 0; o < g; ++o) {
+      double a = 0.0;
+      for (int i = 0; i < h; ++i) a += w3[(long)o * h + i] * H2[(long)r * h + i];
+      z[o] = a + b3[o];
+      if (z[o] > mx) mx = z[o];
+    }
+    for (int o = 0; o < g; ++o) se += exp(z[o] - mx);
+    loss += log(se) + mx - z[t[r]];
+    for (int o = 0; o < g; ++o) {
+      const double d = (exp(z[o] - mx) / se - (o == t[r] ? 1.0 : 0.0)) / (double)B;
+      dZ[(long)r * g + o] = d;
+      dZr[(long)r * g + o] = R(d);
+    }
+  }
+  /* layer 3 */
+  for (int o = 0; o < g; ++o)
+    for (int i = 0; i < h; ++i) {
+      double a = 0.0;
+      for (int r = 0; r < B; ++r) a += dZr[(long)r * g + o] * H2[(long)r * h + i];
+      gW3[(long)o * h + i] = a;
+    }
+  for (int o = 0; o < g; ++o)
+    for (int r = 0; r < B; ++r) gb3[o] += dZ[(long)r * g + o];
+  for (int r = 0; r < B; ++r)
+    for (int i = 0; i < h; ++i) {
+      double a = 0.0;
+      for (int o = 0; o < g; ++o) a += dZr[(long)r * g + o] * w3[(long)o * h + i];
+      a = H2[(long)r * h + i] > 0.0 ? a : 0.0;
+      dH[(long)r * h + i] = a;
+      dHr[(long)r * h + i] = R(a);
+    }
+  /* layer 2 */
+  for (int o = 0; o < h; ++o)
+    for (int i = 0; i < h; ++i) {
+      double a = 0.0;
+      for (int r = 0; r < B; ++r) a += dHr[(long)r * h + o] * H1[(long)r * h + i];
+      gW2[(long)o * h + i] = a;
+    }
+  for (int o = 0; o < h; ++o)
+    for (int r = 0; r < B; ++r) gb2[o] += dH[(long)r * h + o];
+  double *dH1 = malloc(sizeof(double) * (size_t)B * h), *dH1r = malloc(sizeof(double) * (size_t)B * h);
+  for (int r = 0; r < B; ++r)
+    for (int i = 0; i < h; ++i) {
+      double a = 0.0;
+      for (int o = 0; o < h; ++o) a += dHr[(long)r * h + o] * w2[(long)o * h + i];
+      a = H1[(long)r * h + i] > 0.0 ? a : 0.0;
+      dH1[(long)r * h + i] = a;
+      dH1r[(long)r * h + i] = R(a);
+    }
+  /* layer 1 */
+  for (int o = 0; o < h; ++o)
+    for (int i = 0; i < din; ++i) {
+      double a = 0.0;
+      for (int r = 0; r < B; ++r) a += dH1r[(long)r * h + o] * (double)X[(long)r * din + i];
+      gW1[(long)o * din + i] = a;
+    }
+  for (int o = 0; o < h; ++o)
+    for (int r = 0; r < B; ++r) gb1[o] += dH1[(long)r * h + o];
+#undef R
+  free(w1); free(w2); free(w3); free(H1); free(H2); free(dZ); free(dH); free(dHr); free(dZr);
+  free(dH1); free(dH1r); free(z);
+  return loss / (double)B;
+}

@@ -48,6 +48,7 @@ def lib():
             "orc_lloyd": (i32, [P, i64, i32, i32, P, i32, f64, P, P, P, P]),
             "orc_means_fp64": (None, [P, i64, i32, P, i32, P, P]),
             "orc_compactness_fp64": (f64, [P, i64, i32, P, P]),
+            "orc_mlp_fwd_bwd": (f64, [i32, i32, i32, i32, P, P, P, P, i32]),
             "orc_num_threads": (i32, []),
             "orc_set_threads": (None, [i32]),
         }
@@ -201,3 +202,20 @@ def config1():
     """N=100,000, D=32, K=64 mixture: means U[-10,10], sigma 1, seed 0; C0 = first K points."""
     X, z, means, sig = mixture(0, 100_000, 32, 64, -10.0, 10.0, 1.0, 1.0)
     return X, X[:64].copy()
+
+
+def mlp_fwd_bwd(din, h, g, X, t, P, mirror=False):
+    """FP64 MLP loss and gradients (flat parameter layout of the GPU kernels)."""
+    X = _f32(X)
+    t = _i32(t)
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    G = np.empty_like(P)
+    loss = lib().orc_mlp_fwd_bwd(din, h, g, X.shape[0], _p(X), _p(t), _p(P), _p(G), int(mirror))
+    return loss, G
+
+
+def bf16_round(a):
+    """Round float32 values to the nearest BF16 (ties to even), kept as float32."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    return u.view(np.float32)

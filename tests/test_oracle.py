@@ -272,3 +272,35 @@ def test_sample_init_distinct(orc):
     idx = orc.sample_init(1, 10_000_000, 1024)
     assert len(set(idx.tolist())) == 1024 and idx.min() >= 0 and idx.max() < 10_000_000
     assert np.array_equal(idx, orc.sample_init(1, 10_000_000, 1024))
+
+
+# ------------------------------------------------------------------ MLP proposer
+def test_mlp_oracle_gradients_by_finite_differences(orc):
+    rng = np.random.default_rng(4)
+    din, h, g, B = 16, 24, 7, 40
+    n = h * din + h + h * h + h + g * h + g
+    P = rng.standard_normal(n) * 0.3
+    X = rng.standard_normal((B, din)).astype(np.float32)
+    t = rng.integers(0, g, B)
+    loss, G = orc.mlp_fwd_bwd(din, h, g, X, t, P)
+    assert np.isfinite(loss) and loss > 0
+    for i in rng.choice(n, 40, replace=False):
+        e = np.zeros(n)
+        e[i] = 1e-6
+        lp, _ = orc.mlp_fwd_bwd(din, h, g, X, t, P + e)
+        lm, _ = orc.mlp_fwd_bwd(din, h, g, X, t, P - e)
+        fd = (lp - lm) / 2e-6
+        assert abs(fd - G[i]) <= 1e-6 + 1e-4 * abs(G[i]), (i, fd, G[i])
+
+
+def test_mlp_oracle_loss_closed_form(orc):
+    # zero weights: uniform softmax, loss = log(g), db3 = mean(1/g - onehot)
+    din, h, g, B = 16, 8, 5, 10
+    n = h * din + h + h * h + h + g * h + g
+    X = np.ones((B, din), np.float32)
+    t = np.arange(B) % g
+    loss, G = orc.mlp_fwd_bwd(din, h, g, X, t, np.zeros(n))
+    assert abs(loss - np.log(g)) < 1e-12
+    gb3 = G[-g:]
+    expect = np.full(g, 1.0 / g) - np.bincount(t, minlength=g) / B
+    assert np.allclose(gb3, expect, atol=1e-15)
