@@ -219,8 +219,10 @@ kmeansCudaErrorStatus kmGatherRows(void *stream, long n, int d, const float *X, 
  * softmax cross-entropy; BF16 products, FP32 accumulation and parameters).
  * No counterpart in the reference: it serves the learned pruning of the
  * assignment.  Parameters are one flat FP32 vector: W1 (hidden x din), b1,
- * W2 (hidden x hidden), b2, W3 (groups x hidden), b3.  din must be a
- * multiple of 16.  Exposed so its gradients can be checked independently. */
+ * W2 (hidden x hidden), b2, W3 (groups x hidden), b3.  din, hidden and
+ * groups must be multiples of 16; targets must lie in [0, groups) (a row
+ * whose target does not gets a NaN loss and no one-hot term).  Exposed so
+ * its gradients can be checked independently. */
 size_t kmMlpParamCount(int din, int hidden, int groups);
 size_t kmMlpWorkspaceSize(int din, int hidden, int groups, int batch);
 /* One forward + backward pass: X (batch x din BF16 bits), targets (batch),

@@ -730,7 +730,8 @@ kmeansCudaErrorStatus kmMlpForwardBackward(void *stream, int din, int hidden, in
                                            int batch, const uint16_t *X, const int *targets,
                                            const float *P, float *G, float *loss,
                                            void *workspace, size_t workspace_bytes) {
-  if (din <= 0 || din % 16 || hidden <= 0 || hidden % 16 || groups <= 0 || batch <= 0 || !X ||
+  if (din <= 0 || din % 16 || hidden <= 0 || hidden % 16 || groups <= 0 || groups % 16 ||
+      batch <= 0 || !X ||
       !targets || !P || !G || !workspace)
     return ERROR_INVALID_ARGUMENT;
   const km::MlpShape sh{din, hidden, groups};

@@ -119,6 +119,7 @@ __global__ __launch_bounds__(64) void k_softmax_ce(int B, int G, const float *__
   for (int g = lane; g < G; g += 64) se += expf(z[g] - mx);
   for (int o = 32; o > 0; o >>= 1) se += __shfl_xor(se, o);
   const int tg = t[row];
+  const bool tok = tg >= 0 && tg < G;
   const float inv = 1.0f / (float)B;
   for (int g = lane; g < G; g += 64) {
     const float p = expf(z[g] - mx) / se;
@@ -126,7 +127,7 @@ __global__ __launch_bounds__(64) void k_softmax_ce(int B, int G, const float *__
     dZ[(long)row * G + g] = d;
     dZh[(long)row * G + g] = mlp_bf16(d);
   }
-  if (lane == 0) loss[row] = logf(se) + mx - z[tg];
+  if (lane == 0) loss[row] = tok ? logf(se) + mx - z[tg] : __builtin_nanf("");
 }
 
 // dH = dY where H > 0 (ReLU backward); FP32 and BF16 copies.

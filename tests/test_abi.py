@@ -55,6 +55,18 @@ def test_kernel_api_rejects_bad_arguments():
     assert L.kmCompactness(None, 10, 4, 2, None, None, None, None, None, 0) == 8
 
 
+def test_mlp_api_rejects_bad_arguments():
+    L = kmeans_amd.lib()
+    assert L.kmMlpParamCount(96, 256, 128) == 256 * 96 + 256 + 256 * 256 + 256 + 128 * 256 + 128
+    wsb = L.kmMlpWorkspaceSize(96, 128, 128, 8)
+    args = (1, 1, 1, 1, 1, 1)
+    assert L.kmMlpForwardBackward(None, 96, 100, 128, 8, *args, wsb) == 8  # hidden % 16
+    assert L.kmMlpForwardBackward(None, 96, 128, 120, 8, *args, wsb) == 8  # groups % 16
+    assert L.kmMlpForwardBackward(None, 90, 128, 128, 8, *args, wsb) == 8  # din % 16
+    assert L.kmMlpForwardBackward(None, 96, 128, 128, 8, *args, 16) == 8   # workspace too small
+    assert L.kmMlpForwardBackward(None, 96, 128, 128, 8, None, 1, 1, 1, 1, 1, wsb) == 8
+
+
 def test_workspace_size_and_layout():
     L = kmeans_amd.lib()
     assert L.kmCentersLd(1) == 256 and L.kmCentersLd(256) == 256 and L.kmCentersLd(257) == 512
