@@ -35,7 +35,15 @@ def main():
     ap.add_argument("--no-fp32-ref", action="store_true")
     ap.add_argument("--calib", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy (PMC byte-counter calibration)")
+    ap.add_argument("--proposer", default=None, choices=["none", "mlp", "centroid"],
+                    help="candidate proposer (default: mlp for c5, none otherwise)")
+    ap.add_argument("--groups", type=int, default=128)
+    ap.add_argument("--top-groups", type=int, default=4)
+    ap.add_argument("--train-iters", type=int, default=3)
+    ap.add_argument("--hidden", type=int, default=256)
     args = ap.parse_args()
+    if args.proposer is None:
+        args.proposer = "mlp" if args.workload == "c5" else "none"
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus > 1 is not supported "
                                    "(the multi-GPU sharding is a design note, not built)"}))
@@ -71,6 +79,10 @@ def main():
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         return e2, dt
+
+    if args.proposer != "none":
+        print(json.dumps(bench_proposer(args, X, C0, spec, km, np, torch)))
+        return
 
     eng, dt = run(args.precision, args.steps, args.warmup)
     st = eng.stats()
@@ -155,6 +167,92 @@ def main():
         torch.cuda.synchronize()
         out["calibration_copy_bytes"] = 1 << 30
     print(json.dumps(out))
+
+
+def bench_proposer(args, X, C0, spec, km, np, torch):
+    """Lloyd with the candidate proposer: train_iters plain iterations, grouping
+    and MLP training, then proposer iterations (exact FP32 scores within the
+    proposed groups, certified, full FP32 search for the rest).  Compared with
+    full-search runs (FP32 and the reduced-precision path) from the same
+    initial centres; labels must equal the FP32 run's."""
+    n, d, k = spec["n"], spec["d"], spec["k"]
+    steps, ti = args.steps, args.train_iters
+    assert steps > ti, "steps must exceed train_iters"
+    kw = dict(proposer=args.proposer, groups=args.groups, top_groups=args.top_groups,
+              train_iters=ti, hidden=args.hidden)
+
+    def fresh(precision, **extra):
+        e = km.KMeans(n, d, k, precision=precision, timing=True, **extra)
+        e.set_data(device_ptr=X.data_ptr())
+        e.set_centers(C0)
+        e.synchronize()
+        torch.cuda.synchronize()
+        return e
+
+    if args.warmup:
+        w = fresh(args.precision)
+        w.run(args.warmup)
+        w.synchronize()
+        w.close()
+    e = fresh(args.precision, **kw)
+    t0 = time.perf_counter()
+    e.run(ti)
+    e.synchronize()
+    t1 = time.perf_counter()
+    st_plain = e.stats()
+    e.run(steps - ti)
+    e.synchronize()
+    t2 = time.perf_counter()
+    st = e.stats()
+    labels = e.labels()
+    centers = e.centers()
+    e.close()
+    nl = steps - ti
+    learned_assign_ms = ((st["ms_assign"] - st_plain["ms_assign"]) +
+                         (st["ms_reresolve"] - st_plain["ms_reresolve"])) / nl
+    learned_iter_ms = (st["ms_total"] - st_plain["ms_total"]) / nl
+    pairs = float(n) * k
+    total = t2 - t0
+    out = {
+        "metric": "assign pts*ctrs/s per Lloyd iter",
+        "value": pairs * steps / total,
+        "unit": "pts*ctrs/s",
+        "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": total * 1e3 / steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (exact candidate scores) + bf16 MLP",
+        "data": "synthetic (Gaussian mixture generated on the GPU)",
+        "config": {"workload": f"{args.workload}: N={n} D={d} K={k} Lloyd, {args.proposer} proposer, "
+                               f"{args.groups} groups, top {args.top_groups}, trained on "
+                               f"{ti} plain iterations",
+                   "N": n, "D": d, "K": k, "precision": args.precision, "proposer": args.proposer},
+        "plain_iter_ms": st_plain["ms_total"] / ti,
+        "learned_iter_ms": learned_iter_ms,
+        "learned_assign_ms": learned_assign_ms,
+        "learned_assign_pairs_per_s": pairs / (learned_assign_ms * 1e-3),
+        "train_ms": st["ms_train"],
+        "train_loss": [st["train_loss_first"], st["train_loss_last"]],
+        "proposer_stats": {f: st[f] for f in ("proposed", "proposal_hits", "certified",
+                                              "certified_group", "fallback", "candidate_pairs")},
+        "wall_ms": {"plain_iters": (t1 - t0) * 1e3, "train_and_learned_iters": (t2 - t1) * 1e3},
+    }
+    out["proposer_stats"]["candidate_pairs_fraction_of_full"] = st["candidate_pairs"] / (pairs * nl)
+    for prec in (["fp32"] if args.no_fp32_ref else ["fp32", "bf16"]):
+        r = fresh(prec)
+        ta = time.perf_counter()
+        r.run(steps)
+        r.synchronize()
+        tb = time.perf_counter()
+        sr = r.stats()
+        out[f"full_{prec}"] = {"iter_ms": (tb - ta) * 1e3 / steps,
+                               "assign_ms": (sr["ms_assign"] + sr["ms_reresolve"]) / steps}
+        if prec == "fp32":
+            lab32 = r.labels()
+            out["label_mismatches_vs_fp32"] = int((lab32 != labels).sum())
+            out["centers_equal_fp32"] = bool(np.array_equal(r.centers(), centers))
+        r.close()
+    out["speedup_assign_vs_fp32"] = out["full_fp32"]["assign_ms"] / learned_assign_ms
+    return out
 
 
 if __name__ == "__main__":

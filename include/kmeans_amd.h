@@ -112,12 +112,33 @@ typedef enum kmeansPrecision {
   KM_PREC_FP16 = 2       /* FP16 MFMA scores + FP32 re-resolve of near-ties  */
 } kmeansPrecision;
 
+/* Candidate proposer of the assignment (no counterpart in the reference).
+ * The centres are split once into `groups` groups by a coarse K-means of the
+ * centres; after `train_iters` plain iterations each point is scored exactly
+ * only against the members of `top_groups` proposed groups, its label is
+ * certified by a triangle-inequality bound (centre separation, or the group
+ * bound ||x - c|| >= ||x - G_g|| - r_g), and uncertified points are searched
+ * in full, so labels equal the plain FP32 labels.  Resident data only (no
+ * streamed tiles, no mini-batch). */
+typedef enum kmeansProposer {
+  KM_PROPOSER_NONE = 0,
+  KM_PROPOSER_MLP = 1,       /* MLP n -> hidden -> hidden -> groups, trained on
+                                group(label) of rows sampled in the plain
+                                iterations (BF16 MFMA, FP32 masters, Adam)  */
+  KM_PROPOSER_CENTROID = 2   /* the top_groups nearest group centroids      */
+} kmeansProposer;
+
 typedef struct kmeansOptions {
   int precision;          /* kmeansPrecision                                  */
   int device;             /* HIP device ordinal                               */
   void *stream;           /* hipStream_t to run on; NULL: engine creates one  */
   long tile_rows;         /* points per streamed tile (0: all resident)       */
   int timing;             /* 1: record hipEvents per phase (no host syncs)    */
+  int proposer;           /* kmeansProposer                                   */
+  int groups;             /* centre groups (0: min(128, k); MLP: multiple of 16) */
+  int top_groups;         /* groups proposed per point (0: 4; at most 8)      */
+  int train_iters;        /* plain iterations before the proposer (0: 3)      */
+  int hidden;             /* MLP hidden width (0: 256; multiple of 16)        */
 } kmeansOptions;
 
 typedef struct kmeansStats {
@@ -132,6 +153,15 @@ typedef struct kmeansStats {
   double ms_update;
   double ms_compactness;
   double ms_total;
+  long long proposed;         /* point assignments made through the proposer  */
+  long long certified;        /* ... certified by the centre-separation bound */
+  long long certified_group;  /* ... certified by the group bound             */
+  long long fallback;         /* ... searched in full (FP32)                  */
+  long long candidate_pairs;  /* exact (point, centre) scores in named groups */
+  long long proposal_hits;    /* ... whose final label was in a named group   */
+  double ms_train;            /* grouping + MLP training (once, wall clock)   */
+  double train_loss_first;    /* MLP cross-entropy, first and last batch      */
+  double train_loss_last;
 } kmeansStats;
 
 typedef struct kmeansEngine *kmeansHandle;

@@ -25,6 +25,7 @@ KMEANS_SUCCESS = 0
 KMEANS_NONCONVERGENCE = 13
 KMEANS_ERROR_INVALID_ARGUMENT = 14
 PREC = {"fp32": 0, "bf16": 1, "fp16": 2}
+PROPOSER = {None: 0, "none": 0, "mlp": 1, "centroid": 2}
 KM_CHUNK = 256
 KM_BLOCK = 1 << 23
 
@@ -45,7 +46,9 @@ class KmeansError(RuntimeError):
 
 class Options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("device", ctypes.c_int), ("stream", ctypes.c_void_p),
-                ("tile_rows", ctypes.c_long), ("timing", ctypes.c_int)]
+                ("tile_rows", ctypes.c_long), ("timing", ctypes.c_int), ("proposer", ctypes.c_int),
+                ("groups", ctypes.c_int), ("top_groups", ctypes.c_int), ("train_iters", ctypes.c_int),
+                ("hidden", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -54,7 +57,12 @@ class Stats(ctypes.Structure):
                 ("reresolved", ctypes.c_longlong), ("rescored", ctypes.c_longlong),
                 ("ms_assign", ctypes.c_double),
                 ("ms_reresolve", ctypes.c_double), ("ms_update", ctypes.c_double),
-                ("ms_compactness", ctypes.c_double), ("ms_total", ctypes.c_double)]
+                ("ms_compactness", ctypes.c_double), ("ms_total", ctypes.c_double),
+                ("proposed", ctypes.c_longlong), ("certified", ctypes.c_longlong),
+                ("certified_group", ctypes.c_longlong), ("fallback", ctypes.c_longlong),
+                ("candidate_pairs", ctypes.c_longlong), ("proposal_hits", ctypes.c_longlong),
+                ("ms_train", ctypes.c_double), ("train_loss_first", ctypes.c_double),
+                ("train_loss_last", ctypes.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -141,11 +149,16 @@ def compute_kmeans_f(data, k, criterion, max_iters, result=None, init_from_resul
 class KMeans:
     """The engine handle: kmeansCreate / SetData / SetCenters / Run / Get*."""
 
-    def __init__(self, m, n, k, precision="fp32", device=0, stream=None, timing=False, tile_rows=0):
+    def __init__(self, m, n, k, precision="fp32", device=0, stream=None, timing=False, tile_rows=0,
+                 proposer=None, groups=0, top_groups=0, train_iters=0, hidden=0):
         """tile_rows > 0 (a multiple of KM_BLOCK) with host data: stream the points
-        through HBM in tiles of that many rows instead of keeping them resident."""
+        through HBM in tiles of that many rows instead of keeping them resident.
+        proposer "mlp" | "centroid": after train_iters plain iterations, score each
+        point only against the members of top_groups proposed centre groups and
+        certify the label (uncertified points are searched in full)."""
         self.m, self.n, self.k = int(m), int(n), int(k)
-        opt = Options(PREC[precision], device, stream, int(tile_rows), 1 if timing else 0)
+        opt = Options(PREC[precision], device, stream, int(tile_rows), 1 if timing else 0,
+                      PROPOSER[proposer], int(groups), int(top_groups), int(train_iters), int(hidden))
         h = ctypes.c_void_p()
         _check(lib().kmeansCreate(ctypes.byref(h), self.m, self.n, self.k, ctypes.byref(opt)),
                "kmeansCreate")

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "km_engine.hpp"
+#include "km_learned.hpp"
 #include "km_mfma.hpp"
 #include "km_mlp.hpp"
 
@@ -22,6 +23,7 @@ namespace km {
 Engine::~Engine() {
   if (stream) hipStreamSynchronize(stream);
   mfma_free(this);
+  learned_free(this);
   hipFree(dX_owned);
   hipFree(dC);
   hipFree(dCt);
@@ -56,6 +58,7 @@ kmeansErrorStatus Engine::create(long m_, int n_, int k_, const kmeansOptions *o
   if (o) opt = *o;
   if (opt.precision < KM_PREC_FP32 || opt.precision > KM_PREC_FP16)
     return KMEANS_ERROR_INVALID_ARGUMENT;
+  if (learned_configure(this) != KMEANS_SUCCESS) return KMEANS_ERROR_INVALID_ARGUMENT;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
     fprintf(stderr, "kmeans_amd: no HIP device available\n");
@@ -286,12 +289,25 @@ kmeansErrorStatus Engine::run_iterations(long r0, long rows, bool minibatch) {
       }
       continue;
     }
-    record(slot, 0);
-    if ((err = closestCenters(b0, br)) != KMEANS_SUCCESS) return err;
-    record(slot, 1);
-    if (opt.precision != KM_PREC_FP32)
-      KM_HIP_CHECK(mfma_reresolve(this, b0, br), KMEANS_ERROR_CLOSEST_CENTERS);
-    record(slot, 2);
+    if (learned && minibatch) return KMEANS_ERROR_INVALID_ARGUMENT;
+    if (learned_active(this)) {
+      // proposer iteration: the proposal and the grouped exact scores are the
+      // assign phase, the full search of uncertified points the re-resolve
+      if ((err = learned_prepare(this)) != KMEANS_SUCCESS) return err;
+      record(slot, 0);
+      KM_HIP_CHECK(learned_assign(this), KMEANS_ERROR_CLOSEST_CENTERS);
+      record(slot, 1);
+      KM_HIP_CHECK(learned_fallback(this), KMEANS_ERROR_CLOSEST_CENTERS);
+      record(slot, 2);
+    } else {
+      record(slot, 0);
+      if ((err = closestCenters(b0, br)) != KMEANS_SUCCESS) return err;
+      record(slot, 1);
+      if (opt.precision != KM_PREC_FP32)
+        KM_HIP_CHECK(mfma_reresolve(this, b0, br), KMEANS_ERROR_CLOSEST_CENTERS);
+      record(slot, 2);
+      if (learned) KM_HIP_CHECK(learned_collect(this, iters), KMEANS_ERROR_CLOSEST_CENTERS);
+    }
     if ((err = clusterCenters(b0, br)) != KMEANS_SUCCESS) return err;
     KM_HIP_CHECK(hipMemcpyAsync(dempty + 1, dempty, sizeof(int), hipMemcpyDeviceToDevice, stream),
                  KMEANS_ERROR_COMPUTE);
@@ -336,6 +352,7 @@ kmeansErrorStatus Engine::run_iterations(long r0, long rows, bool minibatch) {
     }
   }
   if (opt.precision != KM_PREC_FP32) mfma_collect(this, slot);
+  learned_stats(this);
   collect_timing(slot);
   stats.iterations = iters;
   stats.compactness = compactnessScore;

@@ -38,6 +38,8 @@ struct Engine {
   UpdateWs ws{};
   // precision path buffers (allocated on first use)
   void *mfma = nullptr;
+  // candidate proposer state (options.proposer != 0)
+  void *learned = nullptr;
 
   // loop state (kmeans.cpp:1194,1235-1237)
   double criterion = -1.0;

@@ -1,0 +1,919 @@
+// Learned candidate proposer (see km_learned.hpp for the method).
+//
+// Rounding model of the certificates.  The canonical score of centre j is
+// s_j = fl(-0.5*nc_j + sum_d x_d c_jd), one fma chain in ascending d, with
+// nc_j itself an fma chain of c_jd^2.  With sigma_j = x.c_j - 0.5*||c_j||^2
+// (exact), the standard forward-error bound of a length-n fma chain gives
+//   |s_j - sigma_j| <= gamma_n (0.5 nc_j + ||x|| ||c_j||) + 0.5 gamma_n ||c_j||^2
+//                   <= E(x) = gam * (||x|| cmax + ncmax),
+// gam = 1.01 * gamma_{n+2}, gamma_k = k u / (1 - k u), u = 2^-24 (Higham,
+// Accuracy and Stability of Numerical Algorithms, 2nd ed., sec. 3.1).  Since
+// ||x - c_j||^2 = ||x||^2 - 2 sigma_j, a label j* with score s* is the plain
+// FP32 argmax whenever every other centre is shown to have s_j < s*:
+//  * separation (Elkan, ICML 2003, lemma 1): with a >= ||x - c*|| and
+//    h <= 0.5 min_{j != *} ||c* - c_j||, every other centre has
+//    sigma* - sigma_j >= 2 h (h - a); certified when h > a and
+//    2 h (h - a) > 2 E;
+//  * group bound: for a group g not proposed, ||x - c_j|| >= t_g =
+//    max(0, ||x - G_g|| - r_g) for its members, so s_j <= 0.5 (||x||^2 - t_g^2)
+//    + E; certified when s* exceeds that for every such group.
+// Squared distances computed as FP32 chains of squared differences are
+// within a factor (1 +- gamma_{n+2}) of the exact ones; lower bounds divide by
+// (1 + gam), upper bounds multiply by (1 + 2 gam).  All bound arithmetic is
+// FP64 with relative slack far above its own rounding.
+#include <float.h>
+#include <math.h>
+
+#include <algorithm>
+#include <chrono>
+
+#include "km_common.hpp"
+#include "km_engine.hpp"
+#include "km_learned.hpp"
+#include "km_mlp.hpp"
+
+namespace km {
+
+struct Learned {
+  int G = 0, T = 0, H = 0, din = 0, train_iters = 0, proposer = 0;
+  long S = 0;            // training rows sampled per plain iteration
+  bool ready = false;
+  // training samples: row and label of sample i of iteration t at i*train_iters + t
+  int *samp_rows = nullptr, *samp_lab = nullptr, *samp_tg = nullptr;
+  uint16_t *samp_x = nullptr;
+  // groups (membership fixed once built)
+  int *gof = nullptr, *members = nullptr, *gstart = nullptr, *gsize = nullptr, *gcol = nullptr;
+  int *permP = nullptr;
+  int ldP = 0;
+  std::vector<int> gsize_h, gcol_h;
+  float *CtP = nullptr, *ncP = nullptr, *Gc = nullptr, *gnh = nullptr, *rad2 = nullptr;
+  unsigned *sep2 = nullptr;
+  float *ncmax = nullptr;
+  // MLP
+  MlpShape sh{};
+  size_t Pn = 0;
+  float *P = nullptr, *Gr = nullptr, *Am = nullptr, *Av = nullptr, *loss = nullptr;
+  uint16_t *Ph = nullptr;
+  void *mws = nullptr;
+  size_t mws_bytes = 0;
+  uint16_t *Xh = nullptr;  // m x din, scaled BF16 inputs
+  float xscale = 1.0f;
+  long chunk = 0;
+  uint16_t *H1 = nullptr, *H2 = nullptr;
+  float *Z = nullptr;
+  // per-iteration work
+  int *cand = nullptr;                      // m x T
+  int *gcnt = nullptr, *goff = nullptr, *gcur = nullptr;
+  int *ent_pt = nullptr, *ent_id = nullptr, *elab = nullptr;
+  float *escore = nullptr;
+  int *ul_pt = nullptr, *ul_best = nullptr, *ucnt = nullptr, *fb = nullptr, *fbcnt = nullptr;
+  float *ul_score = nullptr;
+  unsigned long long *cnt = nullptr;        // separation, group, fallback, proposed, hits
+  int *hcnt = nullptr;                      // pinned: gcnt, goff
+  double ms_train = 0.0, loss_first = 0.0, loss_last = 0.0;
+  long long pairs = 0;                      // exact candidate scores issued
+};
+
+static inline Learned *L_of(const Engine *e) { return (Learned *)e->learned; }
+static inline unsigned g1(long n) { return (unsigned)((n + 255) / 256); }
+
+static double gamma_n(int n) {
+  const double u = ldexp(1.0, -24);
+  return 1.01 * ((n + 2) * u) / (1.0 - (n + 2) * u);
+}
+
+// ------------------------------------------------------------------ kernels
+__global__ void k_fill_i32(long n, int *p, int v) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Wave-aggregated increment of a 64-bit counter (one atomic per wave).
+__device__ inline void wave_count(bool f, unsigned long long *c) {
+  const uint64_t mk = __ballot(f);
+  const int lane = threadIdx.x & 63;
+  if (mk && lane == __ffsll((long long)mk) - 1) atomicAdd(c, (unsigned long long)__popcll(mk));
+}
+
+// Wave-aggregated append: returns this lane's slot (valid where f).
+__device__ inline int wave_slot(bool f, int *counter) {
+  const uint64_t mk = __ballot(f);
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  if (mk) {
+    const int lead = __ffsll((long long)mk) - 1;
+    if (lane == lead) base = atomicAdd(counter, __popcll(mk));
+    base = __shfl(base, lead);
+  }
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  return base + __popcll(mk & below);
+}
+
+// out[r][d] = BF16(scale * X[idx ? idx[r] : r][d]) for d < n, 0 for n <= d < din.
+__global__ void k_x_bf16(long rows, int n, int din, const float *__restrict__ X,
+                         const int *__restrict__ idx, float scale, uint16_t *__restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * din) return;
+  const long r = i / din;
+  const int d = (int)(i - r * din);
+  const long src = idx ? (long)idx[r] : r;
+  out[i] = mlp_bf16(d < n ? X[src * n + d] * scale : 0.0f);
+}
+
+// Mean of q (FP64 squared norms) over sampled rows; one block.
+__global__ void k_mean_q(long ns, const int *__restrict__ rows, const double *__restrict__ q,
+                         double *__restrict__ out) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (long i = threadIdx.x; i < ns; i += 256) s += q[rows[i]];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0] / (double)(ns > 0 ? ns : 1);
+}
+
+__global__ void k_collect(long S, int nit, int it, const int *__restrict__ rows,
+                          const int *__restrict__ labels, int *__restrict__ lab) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < S) lab[i * nit + it] = labels[rows[i * nit + it]];
+}
+
+__global__ void k_targets(long ns, const int *__restrict__ lab, const int *__restrict__ gof,
+                          int *__restrict__ tg) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ns) tg[i] = gof[lab[i]];
+}
+
+// Top-T (T <= 8) logits per row; ties to the lower group index.
+__global__ void k_top_logits(long rows, int G, int T, const float *__restrict__ Z,
+                             int *__restrict__ cand) {
+  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  float bv[8];
+  int bi[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) { bv[t] = -INFINITY; bi[t] = -1; }
+  const float *z = Z + r * G;
+  for (int g = 0; g < G; ++g) {
+    float v = z[g];
+    int gi = g;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      if (t < T && v > bv[t]) {
+        const float tv = bv[t];
+        const int ti = bi[t];
+        bv[t] = v; bi[t] = gi; v = tv; gi = ti;
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 8; ++t)
+    if (t < T) cand[r * T + t] = bi[t];
+}
+
+// Nearest-centroid proposer: the T groups with the largest
+// x.G_g - 0.5 ||G_g||^2 (empty groups skipped).  One thread per point, the
+// point's row in registers (NV float4), centroids staged through LDS.
+template <int NV>
+__global__ __launch_bounds__(256) void k_group_top(long m, int n, const float *__restrict__ X,
+                                                   int G, const float *__restrict__ Gc,
+                                                   const float *__restrict__ gnh,
+                                                   const int *__restrict__ gsize, int T,
+                                                   int *__restrict__ cand) {
+  extern __shared__ float4 sg[];  // GT x NV
+  const int GT = (int)min((long)G, (long)(65536 / (16 * NV)));
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = p < m;
+  float4 x[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    float a[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int d = 4 * v + c;
+      a[c] = (live && d < n) ? X[p * n + d] : 0.0f;
+    }
+    x[v] = make_float4(a[0], a[1], a[2], a[3]);
+  }
+  float bv[8];
+  int bi[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) { bv[t] = -INFINITY; bi[t] = -1; }
+  for (int g0 = 0; g0 < G; g0 += GT) {
+    const int gt = min(GT, G - g0);
+    __syncthreads();
+    for (int i = threadIdx.x; i < gt * NV; i += blockDim.x) {
+      const int gg = i / NV, v = i - gg * NV;
+      float a[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int d = 4 * v + c;
+        a[c] = d < n ? Gc[(long)(g0 + gg) * n + d] : 0.0f;
+      }
+      sg[i] = make_float4(a[0], a[1], a[2], a[3]);
+    }
+    __syncthreads();
+    if (!live) continue;
+    for (int gg = 0; gg < gt; ++gg) {
+      const int g = g0 + gg;
+      if (gsize[g] == 0) continue;
+      float s = gnh[g];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const float4 c = sg[gg * NV + v];
+        s = __builtin_fmaf(x[v].x, c.x, s);
+        s = __builtin_fmaf(x[v].y, c.y, s);
+        s = __builtin_fmaf(x[v].z, c.z, s);
+        s = __builtin_fmaf(x[v].w, c.w, s);
+      }
+      int gi = g;
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        if (t < T && s > bv[t]) {
+          const float tv = bv[t];
+          const int ti = bi[t];
+          bv[t] = s; bi[t] = gi; s = tv; gi = ti;
+        }
+      }
+    }
+  }
+  if (live) {
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+      if (t < T) cand[p * T + t] = bi[t];
+  }
+}
+
+// Entries e = p*T + t sorted by group: histogram, scan, scatter.
+__global__ void k_ent_hist(long ne, const int *__restrict__ cand, int G, int *__restrict__ gcnt) {
+  extern __shared__ int h[];
+  for (int j = threadIdx.x; j < G; j += blockDim.x) h[j] = 0;
+  __syncthreads();
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < ne; i += (long)gridDim.x * blockDim.x) {
+    const int g = cand[i];
+    if (g >= 0 && g < G) atomicAdd(&h[g], 1);
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < G; j += blockDim.x)
+    if (h[j]) atomicAdd(&gcnt[j], h[j]);
+}
+
+__global__ void k_ent_scan(int G, const int *__restrict__ gcnt, int *__restrict__ goff,
+                           int *__restrict__ gcur) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int s = 0;
+  for (int g = 0; g < G; ++g) {
+    goff[g] = s;
+    gcur[g] = s;
+    s += gcnt[g];
+  }
+}
+
+#define ENT_CHUNK 8192
+__global__ void k_ent_scatter(long ne, int T, const int *__restrict__ cand, int G,
+                              int *__restrict__ gcur, int *__restrict__ ent_pt,
+                              int *__restrict__ ent_id) {
+  extern __shared__ int sh[];  // h[G], base[G]
+  int *h = sh, *base = sh + G;
+  const long a = (long)blockIdx.x * ENT_CHUNK, e = min(a + ENT_CHUNK, ne);
+  for (int j = threadIdx.x; j < G; j += blockDim.x) h[j] = 0;
+  __syncthreads();
+  for (long i = a + threadIdx.x; i < e; i += blockDim.x) {
+    const int g = cand[i];
+    if (g >= 0 && g < G) atomicAdd(&h[g], 1);
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < G; j += blockDim.x) {
+    base[j] = h[j] ? atomicAdd(&gcur[j], h[j]) : 0;
+    h[j] = 0;
+  }
+  __syncthreads();
+  for (long i = a + threadIdx.x; i < e; i += blockDim.x) {
+    const int g = cand[i];
+    if (g < 0 || g >= G) continue;
+    const int pos = base[g] + atomicAdd(&h[g], 1);
+    ent_pt[pos] = (int)(i / T);
+    ent_id[pos] = (int)i;
+  }
+}
+
+// Centre columns in group order: CtP[d][c] = C[permP[c]][d] (0 for padding).
+__global__ void k_gather_perm(int ldP, int n, const int *__restrict__ permP,
+                              const float *__restrict__ C, const float *__restrict__ nc,
+                              float *__restrict__ CtP, float *__restrict__ ncP) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)ldP * n) return;
+  const int d = (int)(i / ldP), c = (int)(i - (long)d * ldP);
+  const int j = permP[c];
+  CtP[i] = j >= 0 ? C[(long)j * n + d] : 0.0f;
+  if (d == 0) ncP[c] = j >= 0 ? nc[j] : 0.0f;
+}
+
+// Per group: centroid (FP64 mean of the members, stored FP32), -0.5||G||^2,
+// and rad2 = max over members of the FP32 squared distance to the centroid.
+__global__ __launch_bounds__(256) void k_group_geom(int n, const int *__restrict__ gstart,
+                                                    const int *__restrict__ members,
+                                                    const float *__restrict__ C,
+                                                    float *__restrict__ Gc,
+                                                    float *__restrict__ gnh,
+                                                    float *__restrict__ rad2) {
+  extern __shared__ float gs[];  // n
+  __shared__ float red[256];
+  const int g = blockIdx.x, a = gstart[g], b = gstart[g + 1], cnt = b - a;
+  for (int d = threadIdx.x; d < n; d += blockDim.x) {
+    double s = 0.0;
+    for (int i = a; i < b; ++i) s += (double)C[(long)members[i] * n + d];
+    const float v = cnt ? (float)(s / (double)cnt) : 0.0f;
+    gs[d] = v;
+    Gc[(long)g * n + d] = v;
+  }
+  __syncthreads();
+  float mx = 0.0f;
+  for (int i = a + threadIdx.x; i < b; i += blockDim.x) {
+    const float *c = C + (long)members[i] * n;
+    float dd = 0.0f;
+    for (int d = 0; d < n; ++d) {
+      const float df = c[d] - gs[d];
+      dd = __builtin_fmaf(df, df, dd);
+    }
+    mx = fmaxf(mx, dd);
+  }
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    rad2[g] = red[0];
+    float s = 0.0f;
+    for (int d = 0; d < n; ++d) s = __builtin_fmaf(gs[d], gs[d], s);
+    gnh[g] = -0.5f * s;
+  }
+}
+
+// sep2[i] = min over j != i of the FP32 squared distance ||c_i - c_j||^2
+// (chain of squared differences, ascending d), as float bits; sep2 must hold
+// +inf bits on entry.  64 x 64 centre pairs per block, 4 x 4 per thread.
+#define SEP_T 64
+#define SEP_DS 32
+__global__ __launch_bounds__(256) void k_center_sep(int k, int n, const float *__restrict__ C,
+                                                    unsigned *__restrict__ sep2) {
+  __shared__ float A[SEP_DS][SEP_T + 1], B[SEP_DS][SEP_T + 1];
+  const int i0 = blockIdx.y * SEP_T, j0 = blockIdx.x * SEP_T;
+  const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15;
+  float acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0f;
+  for (int d0 = 0; d0 < n; d0 += SEP_DS) {
+    __syncthreads();
+    for (int e = tid; e < SEP_T * SEP_DS; e += 256) {
+      const int r = e / SEP_DS, dd = e - r * SEP_DS;
+      const int d = d0 + dd;
+      A[dd][r] = (i0 + r < k && d < n) ? C[(long)(i0 + r) * n + d] : 0.0f;
+      B[dd][r] = (j0 + r < k && d < n) ? C[(long)(j0 + r) * n + d] : 0.0f;
+    }
+    __syncthreads();
+    const int dn = min(SEP_DS, n - d0);
+    for (int dd = 0; dd < dn; ++dd) {
+      float av[4], bv[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) av[a] = A[dd][ti * 4 + a];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) bv[b] = B[dd][tj + 16 * b];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const float df = av[a] - bv[b];
+          acc[a][b] = __builtin_fmaf(df, df, acc[a][b]);
+        }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int i = i0 + ti * 4 + a;
+    float mn = INFINITY;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int j = j0 + tj + 16 * b;
+      if (j < k && j != i) mn = fminf(mn, acc[a][b]);
+    }
+#pragma unroll
+    for (int o = 8; o >= 1; o >>= 1) mn = fminf(mn, __shfl_xor(mn, o));
+    if (tj == 0 && i < k && mn < INFINITY) atomicMin(&sep2[i], __float_as_uint(mn));
+  }
+}
+
+__global__ void k_ncmax(int k, const float *__restrict__ nc, float *__restrict__ out) {
+  __shared__ float red[256];
+  float mx = 0.0f;
+  for (int j = threadIdx.x; j < k; j += 256) mx = fmaxf(mx, nc[j]);
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+struct BoundParams {
+  double gam;  // 1.01 gamma_{n+2}
+};
+
+__device__ inline double score_err(double qu, double ncm, double gam) {
+  const double cmax = sqrt(ncm * (1.0 + gam)) * (1.0 + 1e-15);
+  return gam * (sqrt(qu) * cmax + ncm) * (1.0 + 1e-12) + 1e-300;
+}
+
+// Best candidate per point (largest canonical score, lowest original index on
+// ties) and the separation certificate; the rest go to the uncertified list.
+__global__ __launch_bounds__(256) void k_combine(
+    long m, int T, const int *__restrict__ cand, const int *__restrict__ gsize,
+    const int *__restrict__ gcol, const int *__restrict__ permP, const int *__restrict__ elab,
+    const float *__restrict__ escore, const double *__restrict__ q,
+    const unsigned *__restrict__ sep2, const float *__restrict__ ncmax, BoundParams bp,
+    int *__restrict__ labels, int *__restrict__ ul_pt, int *__restrict__ ul_best,
+    float *__restrict__ ul_score, int *__restrict__ ucnt, unsigned long long *__restrict__ cnt) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = p < m;
+  int bj = -1;
+  float bs = -INFINITY;
+  bool cert = false;
+  if (live) {
+    for (int t = 0; t < T; ++t) {
+      const long e = p * T + t;
+      const int g = cand[e];
+      if (g < 0 || gsize[g] == 0) continue;
+      const int j = permP[gcol[g] + elab[e]];
+      const float s = escore[e];
+      if (s > bs || (s == bs && j < bj)) { bs = s; bj = j; }
+    }
+    if (bj >= 0 && bs > -INFINITY) {
+      const double qu = q[p] * (1.0 + 1e-12);
+      const double E = score_err(qu, (double)ncmax[0], bp.gam);
+      const double a2 = qu - 2.0 * (double)bs + 2.0 * E;
+      const double a = sqrt(fmax(a2, 0.0)) * (1.0 + 1e-14);
+      const double s2 = (double)__uint_as_float(sep2[bj]);
+      const double h = 0.5 * sqrt(s2 / (1.0 + bp.gam)) * (1.0 - 1e-14);
+      cert = (h > a) && (2.0 * h * (h - a) > 2.0 * E * (1.0 + 1e-12));
+    }
+  }
+  if (cert) labels[p] = bj;
+  const bool unc = live && !cert;
+  const int slot = wave_slot(unc, ucnt);
+  if (unc) {
+    ul_pt[slot] = (int)p;
+    ul_best[slot] = bj;
+    ul_score[slot] = bs;
+  }
+  wave_count(cert, &cnt[0]);
+  wave_count(live, &cnt[3]);
+}
+
+// Group bound for the uncertified points; the rest go to the fallback list.
+__global__ __launch_bounds__(256) void k_group_bound(
+    const int *__restrict__ ucnt, int n, const float *__restrict__ X, int T,
+    const int *__restrict__ cand, int G, const float *__restrict__ Gc,
+    const float *__restrict__ rad2, const int *__restrict__ gsize, const double *__restrict__ q,
+    const float *__restrict__ ncmax, BoundParams bp, const int *__restrict__ ul_pt,
+    const int *__restrict__ ul_best, const float *__restrict__ ul_score, int *__restrict__ labels,
+    int *__restrict__ fb, int *__restrict__ fbcnt, unsigned long long *__restrict__ cnt) {
+  const long nu = *ucnt;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long b0 = (long)blockIdx.x * blockDim.x; b0 < nu; b0 += stride) {
+    const long i = b0 + threadIdx.x;
+    const bool live = i < nu;
+    bool cert = false;
+    int p = 0;
+    if (live) {
+      p = ul_pt[i];
+      const int bj = ul_best[i];
+      const double bs = (double)ul_score[i];
+      if (bj >= 0) {
+        const double qu = q[p] * (1.0 + 1e-12);
+        const double E = score_err(qu, (double)ncmax[0], bp.gam);
+        const float *x = X + (long)p * n;
+        cert = true;
+        for (int g = 0; g < G && cert; ++g) {
+          if (gsize[g] == 0) continue;
+          bool named = false;
+          for (int t = 0; t < T; ++t) named |= cand[(long)p * T + t] == g;
+          if (named) continue;
+          const float *c = Gc + (long)g * n;
+          float dd = 0.0f;
+          for (int d = 0; d < n; ++d) {
+            const float df = x[d] - c[d];
+            dd = __builtin_fmaf(df, df, dd);
+          }
+          const double dlow = sqrt((double)dd / (1.0 + bp.gam)) * (1.0 - 1e-14);
+          const double r = sqrt((double)rad2[g] * (1.0 + 2.0 * bp.gam)) * (1.0 + 1e-14);
+          const double t = fmax(dlow - r, 0.0);
+          const double ub = 0.5 * (qu - t * t);
+          cert = bs > ub + E * (1.0 + 1e-12);
+        }
+      }
+    }
+    if (cert) labels[p] = ul_best[i];
+    const bool f = live && !cert;
+    const int slot = wave_slot(f, fbcnt);
+    if (f) fb[slot] = p;
+    wave_count(cert, &cnt[1]);
+    wave_count(f, &cnt[2]);
+  }
+}
+
+// Proposal hits: points whose final label lies in one of their named groups.
+__global__ void k_hits(long m, int T, const int *__restrict__ cand, const int *__restrict__ gof,
+                       const int *__restrict__ labels, unsigned long long *__restrict__ cnt) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  bool hit = false;
+  if (p < m) {
+    const int g = gof[labels[p]];
+    for (int t = 0; t < T; ++t) hit |= cand[p * T + t] == g;
+  }
+  wave_count(hit, &cnt[4]);
+}
+
+// ------------------------------------------------------------------ host
+static hipError_t lmalloc(void **p, size_t bytes) { return hipMalloc(p, bytes ? bytes : 1); }
+
+#define LCHK(x)                          \
+  do {                                   \
+    hipError_t _e = (x);                 \
+    if (_e != hipSuccess) return _e;     \
+  } while (0)
+
+bool learned_active(const Engine *e) {
+  const Learned *L = L_of(e);
+  return L && e->iters >= L->train_iters;
+}
+
+kmeansErrorStatus learned_configure(Engine *e) {
+  kmeansOptions &o = e->opt;
+  if (o.proposer == KM_PROPOSER_NONE) return KMEANS_SUCCESS;
+  if (o.proposer != KM_PROPOSER_MLP && o.proposer != KM_PROPOSER_CENTROID)
+    return KMEANS_ERROR_INVALID_ARGUMENT;
+  if (o.groups <= 0) o.groups = std::min(128, e->k);
+  if (o.top_groups <= 0) o.top_groups = std::min(4, o.groups);
+  if (o.train_iters <= 0) o.train_iters = 3;
+  if (o.hidden <= 0) o.hidden = 256;
+  if (o.groups > e->k || o.top_groups > 8 || o.top_groups > o.groups || o.tile_rows > 0 ||
+      (double)e->m * o.top_groups > 2147483647.0)
+    return KMEANS_ERROR_INVALID_ARGUMENT;
+  if (o.proposer == KM_PROPOSER_MLP && (o.groups % 16 || o.hidden % 16))
+    return KMEANS_ERROR_INVALID_ARGUMENT;
+  if (o.proposer == KM_PROPOSER_CENTROID && e->n > 256) return KMEANS_ERROR_INVALID_ARGUMENT;
+  Learned *L = new Learned();
+  e->learned = L;
+  L->proposer = o.proposer;
+  L->G = o.groups;
+  L->T = o.top_groups;
+  L->H = o.hidden;
+  L->train_iters = o.train_iters;
+  L->din = (e->n + 15) / 16 * 16;
+  L->S = std::min<long>(e->m, 262144);
+  return KMEANS_SUCCESS;
+}
+
+void learned_free(Engine *e) {
+  Learned *L = L_of(e);
+  if (!L) return;
+  void *bufs[] = {L->samp_rows, L->samp_lab, L->samp_tg, L->samp_x, L->gof, L->members, L->gstart,
+                  L->gsize, L->gcol, L->permP, L->CtP, L->ncP, L->Gc, L->gnh, L->rad2, L->sep2,
+                  L->ncmax, L->P, L->Gr, L->Am, L->Av, L->loss, L->Ph, L->mws, L->Xh, L->H1,
+                  L->H2, L->Z, L->cand, L->gcnt, L->ent_pt, L->ent_id, L->elab, L->escore,
+                  L->ul_pt, L->ul_best, L->ucnt, L->fb, L->ul_score, L->cnt};  // goff, gcur, fbcnt
+                                                                               // live inside gcnt, ucnt
+  for (void *b : bufs) hipFree(b);
+  if (L->hcnt) hipHostFree(L->hcnt);
+  delete L;
+  e->learned = nullptr;
+}
+
+// Sample rows of plain iteration `it` (fixed per iteration, counter-based).
+hipError_t learned_collect(Engine *e, int it) {
+  Learned *L = L_of(e);
+  if (!L || it >= L->train_iters || L->ready) return hipSuccess;
+  const int nit = L->train_iters;
+  const long ns = L->S * nit;
+  if (!L->samp_rows) {
+    std::vector<int> rows((size_t)ns);
+    for (int t = 0; t < nit; ++t)
+      for (long i = 0; i < L->S; ++i)
+        rows[(size_t)(i * nit + t)] = (int)(rng_u64(0x1ea7ULL, 8 + (uint64_t)t, (uint64_t)i) % (uint64_t)e->m);
+    LCHK(lmalloc((void **)&L->samp_rows, sizeof(int) * ns));
+    LCHK(lmalloc((void **)&L->samp_lab, sizeof(int) * ns));
+    LCHK(hipMemcpy(L->samp_rows, rows.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
+    LCHK(hipMemset(L->samp_lab, 0, sizeof(int) * ns));
+  }
+  hipLaunchKernelGGL(k_collect, dim3(g1(L->S)), dim3(256), 0, e->stream, L->S, nit, it, L->samp_rows,
+                     e->dlabels, L->samp_lab);
+  return hipGetLastError();
+}
+
+// Group membership from a coarse K-means of the centres (the engine's own
+// plain FP32 loop, 20 iterations, sampled initial centres).
+static kmeansErrorStatus build_groups(Engine *e, Learned *L) {
+  const int k = e->k, n = e->n, G = L->G;
+  std::vector<int> gof((size_t)k);
+  {
+    float *Ccopy = nullptr;
+    KM_HIP_CHECK(hipMalloc(&Ccopy, sizeof(float) * (size_t)k * n), KMEANS_ERROR_COMPUTE);
+    KM_HIP_CHECK(hipMemcpyAsync(Ccopy, e->dC, sizeof(float) * (size_t)k * n,
+                                hipMemcpyDeviceToDevice, e->stream),
+                 KMEANS_ERROR_COMPUTE);
+    Engine sub;
+    kmeansOptions so{};
+    so.precision = KM_PREC_FP32;
+    so.device = e->opt.device;
+    so.stream = e->stream;
+    kmeansErrorStatus st = sub.create(k, n, G, &so);
+    if (st == KMEANS_SUCCESS) st = sub.buildData(Ccopy, 1);
+    if (st == KMEANS_SUCCESS) st = sub.initialize(0x9a0aULL);
+    if (st == KMEANS_SUCCESS) {
+      sub.maxIters = 20;
+      sub.criterion = -1.0;
+      sub.relErr = 1.0;
+      st = sub.computeStandard();
+    }
+    if (st == KMEANS_SUCCESS) st = sub.getLabels(gof.data());
+    hipStreamSynchronize(e->stream);
+    hipFree(Ccopy);
+    if (st != KMEANS_SUCCESS) return st;
+  }
+  // members sorted by group, ascending centre index within a group
+  std::vector<int> gsize(G, 0), gstart(G + 1, 0), members(k), gcol(G), pos(G);
+  for (int j = 0; j < k; ++j) gsize[gof[j]]++;
+  for (int g = 0; g < G; ++g) gstart[g + 1] = gstart[g] + gsize[g];
+  for (int g = 0; g < G; ++g) pos[g] = gstart[g];
+  for (int j = 0; j < k; ++j) members[pos[gof[j]]++] = j;
+  int col = 0;
+  for (int g = 0; g < G; ++g) {
+    gcol[g] = col;
+    col += (gsize[g] + 3) / 4 * 4;
+  }
+  L->ldP = col + 256;  // the FP32 kernel reads whole 256-column tiles
+  std::vector<int> permP((size_t)L->ldP, -1);
+  for (int g = 0; g < G; ++g)
+    for (int i = 0; i < gsize[g]; ++i) permP[(size_t)gcol[g] + i] = members[(size_t)gstart[g] + i];
+  L->gsize_h = gsize;
+  L->gcol_h = gcol;
+  auto up = [&](int **dst, const std::vector<int> &src) -> bool {
+    if (lmalloc((void **)dst, sizeof(int) * src.size()) != hipSuccess) return false;
+    return hipMemcpy(*dst, src.data(), sizeof(int) * src.size(), hipMemcpyHostToDevice) == hipSuccess;
+  };
+  if (!up(&L->gof, gof) || !up(&L->members, members) || !up(&L->gstart, gstart) ||
+      !up(&L->gsize, gsize) || !up(&L->gcol, gcol) || !up(&L->permP, permP))
+    return KMEANS_ERROR_COMPUTE;
+  const size_t ldP = (size_t)L->ldP;
+  KM_HIP_CHECK(lmalloc((void **)&L->CtP, sizeof(float) * ldP * n), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(lmalloc((void **)&L->ncP, sizeof(float) * ldP), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(lmalloc((void **)&L->Gc, sizeof(float) * (size_t)G * n), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(lmalloc((void **)&L->gnh, sizeof(float) * G), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(lmalloc((void **)&L->rad2, sizeof(float) * G), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(lmalloc((void **)&L->sep2, sizeof(unsigned) * k), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(lmalloc((void **)&L->ncmax, sizeof(float)), KMEANS_ERROR_COMPUTE);
+  return KMEANS_SUCCESS;
+}
+
+static void init_params(const MlpShape &sh, std::vector<float> &P) {
+  size_t off[6];
+  mlp_offsets(sh, off);
+  const int fan_in[6] = {sh.din, sh.din, sh.h, sh.h, sh.h, sh.h};
+  P.resize(mlp_param_count(sh));
+  for (int b = 0; b < 6; ++b) {
+    const size_t end = b < 5 ? off[b + 1] : P.size();
+    const double bound = 1.0 / sqrt((double)fan_in[b]);
+    for (size_t i = off[b]; i < end; ++i)
+      P[i] = (float)((2.0 * rng_unif(0x31a9ULL, 16 + (uint64_t)b, i) - 1.0) * bound);
+  }
+}
+
+// MLP training on the sampled rows (Adam, batches of 4096, 3 epochs).
+static kmeansErrorStatus train_mlp(Engine *e, Learned *L) {
+  const long ns = L->S * L->train_iters;
+  const int din = L->din;
+  hipStream_t s = e->stream;
+  L->sh = MlpShape{din, L->H, L->G};
+  L->Pn = mlp_param_count(L->sh);
+  const int B = (int)std::min<long>(4096, ns);
+  L->mws_bytes = std::max(mlp_ws_bytes(L->sh, B), (size_t)1);
+  KM_HIP_CHECK(lmalloc((void **)&L->samp_tg, sizeof(int) * ns), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(lmalloc((void **)&L->samp_x, sizeof(uint16_t) * ns * din), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(lmalloc((void **)&L->P, sizeof(float) * L->Pn), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(lmalloc((void **)&L->Gr, sizeof(float) * L->Pn), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(lmalloc((void **)&L->Am, sizeof(float) * L->Pn), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(lmalloc((void **)&L->Av, sizeof(float) * L->Pn), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(lmalloc((void **)&L->Ph, sizeof(uint16_t) * L->Pn), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(lmalloc((void **)&L->loss, sizeof(float) * 4), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(lmalloc(&L->mws, L->mws_bytes), KMEANS_ERROR_COMPUTE);
+  // input scale: 1 / rms coordinate over the sample
+  double *dmean = nullptr;
+  KM_HIP_CHECK(hipMalloc(&dmean, sizeof(double)), KMEANS_ERROR_COMPUTE);
+  hipLaunchKernelGGL(k_mean_q, dim3(1), dim3(256), 0, s, ns, L->samp_rows, e->dq, dmean);
+  double mq = 0.0;
+  KM_HIP_CHECK(hipMemcpyAsync(&mq, dmean, sizeof(double), hipMemcpyDeviceToHost, s), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(hipStreamSynchronize(s), KMEANS_ERROR_COMPUTE);
+  hipFree(dmean);
+  const double rms = sqrt(mq / (double)e->n);
+  L->xscale = rms > 0.0 && std::isfinite(rms) ? (float)(1.0 / rms) : 1.0f;
+  hipLaunchKernelGGL(k_x_bf16, dim3(g1(ns * din)), dim3(256), 0, s, ns, e->n, din, e->dX,
+                     L->samp_rows, L->xscale, L->samp_x);
+  hipLaunchKernelGGL(k_targets, dim3(g1(ns)), dim3(256), 0, s, ns, L->samp_lab, L->gof, L->samp_tg);
+  std::vector<float> P0;
+  init_params(L->sh, P0);
+  KM_HIP_CHECK(hipMemcpyAsync(L->P, P0.data(), sizeof(float) * L->Pn, hipMemcpyHostToDevice, s),
+               KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(hipMemsetAsync(L->Am, 0, sizeof(float) * L->Pn, s), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(hipMemsetAsync(L->Av, 0, sizeof(float) * L->Pn, s), KMEANS_ERROR_COMPUTE);
+  const long nb = ns / B;
+  const int epochs = 3;
+  const long steps = std::max<long>(1, nb * epochs);
+  for (long st = 0; st < steps; ++st) {
+    const long b = st % std::max<long>(nb, 1);
+    float *lp = st == 0 ? L->loss : (st == steps - 1 ? L->loss + 1 : nullptr);
+    const float lr = st < steps / 2 ? 2e-3f : (st < 5 * steps / 6 ? 6e-4f : 2e-4f);
+    KM_HIP_CHECK(mlp_fwd_bwd(s, L->sh, B, L->samp_x + (size_t)b * B * din, L->samp_tg + (size_t)b * B,
+                             L->P, L->Gr, lp, L->mws),
+                 KMEANS_ERROR_COMPUTE);
+    KM_HIP_CHECK(mlp_adam(s, L->Pn, L->P, L->Gr, L->Am, L->Av, lr, (int)st + 1, nullptr),
+                 KMEANS_ERROR_COMPUTE);
+  }
+  if (steps == 1) KM_HIP_CHECK(hipMemcpyAsync(L->loss + 1, L->loss, sizeof(float), hipMemcpyDeviceToDevice, s),
+                               KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(params_to_bf16(s, L->Pn, L->P, L->Ph), KMEANS_ERROR_COMPUTE);
+  float lh[2] = {0.f, 0.f};
+  KM_HIP_CHECK(hipMemcpyAsync(lh, L->loss, sizeof(float) * 2, hipMemcpyDeviceToHost, s), KMEANS_ERROR_COMPUTE);
+  // scaled BF16 copy of every point for inference (points do not change)
+  KM_HIP_CHECK(lmalloc((void **)&L->Xh, sizeof(uint16_t) * (size_t)e->m * din), KMEANS_ERROR_COMPUTE);
+  hipLaunchKernelGGL(k_x_bf16, dim3(g1(e->m * din)), dim3(256), 0, s, e->m, e->n, din, e->dX, nullptr,
+                     L->xscale, L->Xh);
+  L->chunk = std::min<long>(e->m, 1L << 20);
+  KM_HIP_CHECK(lmalloc((void **)&L->H1, sizeof(uint16_t) * L->chunk * L->H), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(lmalloc((void **)&L->H2, sizeof(uint16_t) * L->chunk * L->H), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(lmalloc((void **)&L->Z, sizeof(float) * L->chunk * L->G), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(hipStreamSynchronize(s), KMEANS_ERROR_COMPUTE);
+  L->loss_first = lh[0];
+  L->loss_last = lh[1];
+  return KMEANS_SUCCESS;
+}
+
+kmeansErrorStatus learned_prepare(Engine *e) {
+  Learned *L = L_of(e);
+  if (!L || L->ready) return KMEANS_SUCCESS;
+  KM_HIP_CHECK(hipStreamSynchronize(e->stream), KMEANS_ERROR_COMPUTE);
+  const auto t0 = std::chrono::steady_clock::now();
+  kmeansErrorStatus st = build_groups(e, L);
+  if (st != KMEANS_SUCCESS) return st;
+  if (L->proposer == KM_PROPOSER_MLP) {
+    if (!L->samp_rows) return KMEANS_ERROR_COMPUTE;
+    if ((st = train_mlp(e, L)) != KMEANS_SUCCESS) return st;
+  }
+  const long m = e->m, ne = m * L->T;
+  KM_HIP_CHECK(lmalloc((void **)&L->cand, sizeof(int) * ne), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(lmalloc((void **)&L->ent_pt, sizeof(int) * ne), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(lmalloc((void **)&L->ent_id, sizeof(int) * ne), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(lmalloc((void **)&L->elab, sizeof(int) * ne), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(lmalloc((void **)&L->escore, sizeof(float) * ne), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(lmalloc((void **)&L->ul_pt, sizeof(int) * m), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(lmalloc((void **)&L->ul_best, sizeof(int) * m), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(lmalloc((void **)&L->ul_score, sizeof(float) * m), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(lmalloc((void **)&L->fb, sizeof(int) * m), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(lmalloc((void **)&L->gcnt, sizeof(int) * L->G * 3), KMEANS_ERROR_COMPUTE);
+  L->goff = L->gcnt + L->G;
+  L->gcur = L->gcnt + 2 * L->G;
+  KM_HIP_CHECK(lmalloc((void **)&L->ucnt, sizeof(int) * 2), KMEANS_ERROR_COMPUTE);
+  L->fbcnt = L->ucnt + 1;
+  KM_HIP_CHECK(lmalloc((void **)&L->cnt, sizeof(unsigned long long) * 5), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(hipMemsetAsync(L->cnt, 0, sizeof(unsigned long long) * 5, e->stream), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(hipHostMalloc((void **)&L->hcnt, sizeof(int) * 2 * L->G), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(hipStreamSynchronize(e->stream), KMEANS_ERROR_COMPUTE);
+  L->ms_train = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  L->ready = true;
+  return KMEANS_SUCCESS;
+}
+
+// Per-iteration centre-side data: group-ordered columns, centroids, radii,
+// centre separations, max ||c||^2.
+static hipError_t refresh_centres(Engine *e, Learned *L) {
+  hipStream_t s = e->stream;
+  const int k = e->k, n = e->n;
+  hipLaunchKernelGGL(k_gather_perm, dim3(g1((long)L->ldP * n)), dim3(256), 0, s, L->ldP, n, L->permP,
+                     e->dC, e->dnc, L->CtP, L->ncP);
+  hipLaunchKernelGGL(k_group_geom, dim3(L->G), dim3(256), sizeof(float) * n, s, n, L->gstart,
+                     L->members, e->dC, L->Gc, L->gnh, L->rad2);
+  hipLaunchKernelGGL(k_fill_i32, dim3(g1(k)), dim3(256), 0, s, (long)k, (int *)L->sep2,
+                     (int)0x7f800000);
+  const int tiles = (k + SEP_T - 1) / SEP_T;
+  hipLaunchKernelGGL(k_center_sep, dim3(tiles, tiles), dim3(256), 0, s, k, n, e->dC, L->sep2);
+  hipLaunchKernelGGL(k_ncmax, dim3(1), dim3(256), 0, s, k, e->dnc, L->ncmax);
+  return hipGetLastError();
+}
+
+static hipError_t propose(Engine *e, Learned *L) {
+  hipStream_t s = e->stream;
+  const long m = e->m;
+  if (L->proposer == KM_PROPOSER_MLP) {
+    for (long r0 = 0; r0 < m; r0 += L->chunk) {
+      const long R = std::min(L->chunk, m - r0);
+      LCHK(mlp_forward(s, L->sh, (int)R, L->Xh + (size_t)r0 * L->din, L->din, L->Ph, L->P, L->H1,
+                       L->H2, L->Z));
+      hipLaunchKernelGGL(k_top_logits, dim3(g1(R)), dim3(256), 0, s, R, L->G, L->T, L->Z,
+                         L->cand + (size_t)r0 * L->T);
+    }
+    return hipGetLastError();
+  }
+  const int nv = (e->n + 3) / 4;
+  const int NVs[] = {4, 8, 16, 24, 32, 48, 64};
+  int NV = 64;
+  for (int v : NVs)
+    if (v >= nv) { NV = v; break; }
+  const int GT = std::min(L->G, 65536 / (16 * NV));
+  const size_t lds = sizeof(float) * 4 * (size_t)GT * NV;
+#define GT_CASE(V)                                                                             \
+  case V:                                                                                      \
+    hipFuncSetAttribute((const void *)k_group_top<V>,                                          \
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                 \
+    hipLaunchKernelGGL(k_group_top<V>, dim3(g1(m)), dim3(256), lds, s, m, e->n, e->dX, L->G,   \
+                       L->Gc, L->gnh, L->gsize, L->T, L->cand);                                \
+    break;
+  switch (NV) {
+    GT_CASE(4) GT_CASE(8) GT_CASE(16) GT_CASE(24) GT_CASE(32) GT_CASE(48) GT_CASE(64)
+    default: return hipErrorInvalidValue;
+  }
+#undef GT_CASE
+  return hipGetLastError();
+}
+
+hipError_t learned_assign(Engine *e) {
+  Learned *L = L_of(e);
+  hipStream_t s = e->stream;
+  const long m = e->m, ne = m * L->T;
+  const int G = L->G;
+  LCHK(refresh_centres(e, L));
+  LCHK(propose(e, L));
+  // entries sorted by group
+  hipLaunchKernelGGL(k_fill_i32, dim3(g1(G)), dim3(256), 0, s, (long)G, L->gcnt, 0);
+  hipLaunchKernelGGL(k_ent_hist, dim3((unsigned)std::min<long>(g1(ne), 4L * num_cus())), dim3(256),
+                     sizeof(int) * G, s, ne, L->cand, G, L->gcnt);
+  hipLaunchKernelGGL(k_ent_scan, dim3(1), dim3(64), 0, s, G, L->gcnt, L->goff, L->gcur);
+  hipLaunchKernelGGL(k_ent_scatter, dim3((unsigned)((ne + ENT_CHUNK - 1) / ENT_CHUNK)), dim3(256),
+                     sizeof(int) * 2 * G, s, ne, L->T, L->cand, G, L->gcur, L->ent_pt, L->ent_id);
+  LCHK(hipMemcpyAsync(L->hcnt, L->gcnt, sizeof(int) * 2 * G, hipMemcpyDeviceToHost, s));
+  LCHK(hipStreamSynchronize(s));
+  // exact canonical scores within each named group
+  for (int g = 0; g < G; ++g) {
+    const int c = L->hcnt[g], off = L->hcnt[G + g];
+    if (c == 0 || L->gsize_h[g] == 0) continue;
+    L->pairs += (long long)c * L->gsize_h[g];
+    LCHK(launch_assign_f32(s, c, nullptr, e->n, e->dX, L->ent_pt + off, L->gsize_h[g],
+                           L->CtP + L->gcol_h[g], L->ldP, L->ncP + L->gcol_h[g], L->elab,
+                           L->escore, L->ent_id + off));
+  }
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, 2L, L->ucnt, 0);
+  BoundParams bp{gamma_n(e->n)};
+  hipLaunchKernelGGL(k_combine, dim3(g1(m)), dim3(256), 0, s, m, L->T, L->cand, L->gsize, L->gcol,
+                     L->permP, L->elab, L->escore, e->dq, L->sep2, L->ncmax, bp, e->dlabels,
+                     L->ul_pt, L->ul_best, L->ul_score, L->ucnt, L->cnt);
+  hipLaunchKernelGGL(k_group_bound, dim3((unsigned)std::min<long>(g1(m), 4L * num_cus())), dim3(256),
+                     0, s, L->ucnt, e->n, e->dX, L->T, L->cand, G, L->Gc, L->rad2, L->gsize, e->dq,
+                     L->ncmax, bp, L->ul_pt, L->ul_best, L->ul_score, e->dlabels, L->fb, L->fbcnt,
+                     L->cnt);
+  return hipGetLastError();
+}
+
+hipError_t learned_fallback(Engine *e) {
+  Learned *L = L_of(e);
+  LCHK(launch_assign_f32(e->stream, e->m, L->fbcnt, e->n, e->dX, L->fb, e->k, e->dCt, e->ldc,
+                         e->dnc, e->dlabels, nullptr));
+  hipLaunchKernelGGL(k_hits, dim3(g1(e->m)), dim3(256), 0, e->stream, e->m, L->T, L->cand, L->gof,
+                     e->dlabels, L->cnt);
+  return hipGetLastError();
+}
+
+void learned_stats(Engine *e) {
+  Learned *L = L_of(e);
+  if (!L) return;
+  e->stats.ms_train = L->ms_train;
+  e->stats.train_loss_first = L->loss_first;
+  e->stats.train_loss_last = L->loss_last;
+  e->stats.candidate_pairs = L->pairs;
+  if (!L->cnt) return;
+  unsigned long long c[5] = {0, 0, 0, 0, 0};
+  if (hipMemcpyAsync(c, L->cnt, sizeof(c), hipMemcpyDeviceToHost, e->stream) != hipSuccess) return;
+  if (hipStreamSynchronize(e->stream) != hipSuccess) return;
+  e->stats.certified = (long long)c[0];
+  e->stats.certified_group = (long long)c[1];
+  e->stats.fallback = (long long)c[2];
+  e->stats.proposed = (long long)c[3];
+  e->stats.proposal_hits = (long long)c[4];
+}
+
+}  // namespace km

@@ -16,12 +16,6 @@ typedef __attribute__((ext_vector_type(8))) __bf16 mlp_bf16x8;
 typedef __attribute__((ext_vector_type(16))) float mlp_f32x16;
 typedef __attribute__((ext_vector_type(4))) unsigned int mlp_u32x4;
 
-__device__ inline uint16_t mlp_bf16(float f) {  // round to nearest even
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
 __device__ inline float mlp_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 __device__ inline int mlp_row(int r, int h) { return 8 * (r >> 2) + 4 * h + (r & 3); }
@@ -140,13 +134,24 @@ __global__ void k_relu_bwd(long n, const float *__restrict__ dY, const uint16_t 
   dHh[i] = mlp_bf16(v);
 }
 
-// db[c] = sum over rows of dY[r][c] (left to right).
-__global__ void k_colsum(int rows, int cols, const float *__restrict__ dY, float *__restrict__ db) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cols) return;
+// db[c] = sum over rows of dY[r][c]: 64 columns per block, 16 row phases
+// (rows r = phase mod 16, ascending) combined in phase order, so the result
+// does not depend on scheduling.
+__global__ __launch_bounds__(1024) void k_colsum(int rows, int cols, const float *__restrict__ dY,
+                                                 float *__restrict__ db) {
+  __shared__ float part[16][64];
+  const int lane = threadIdx.x & 63, ph = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + lane;
   float s = 0.0f;
-  for (int r = 0; r < rows; ++r) s += dY[(long)r * cols + c];
-  db[c] = s;
+  if (c < cols)
+    for (int r = ph; r < rows; r += 16) s += dY[(long)r * cols + c];
+  part[ph][lane] = s;
+  __syncthreads();
+  if (ph == 0 && c < cols) {
+    float t = 0.0f;
+    for (int q = 0; q < 16; ++q) t += part[q][lane];
+    db[c] = t;
+  }
 }
 
 // Adam (Kingma & Ba, ICLR 2015) on FP32 masters, refreshes the BF16 copy.
@@ -264,7 +269,7 @@ hipError_t mlp_fwd_bwd(hipStream_t s, const MlpShape &sh, int B, const uint16_t 
   if ((e = transpose_b16(s, B, sh.g, dZh, sh.g, Tz, Bp))) return e;
   if ((e = transpose_b16(s, B, sh.h, H2, sh.h, Ta, Bp))) return e;
   if ((e = gemm_nt(s, sh.g, sh.h, Bp, Tz, Bp, Ta, Bp, nullptr, false, gW3, sh.h, nullptr, 0))) return e;
-  hipLaunchKernelGGL(k_colsum, dim3(grid1(sh.g)), dim3(256), 0, s, B, sh.g, dZ, gb3);
+  hipLaunchKernelGGL(k_colsum, dim3((sh.g + 63) / 64), dim3(1024), 0, s, B, sh.g, dZ, gb3);
   if ((e = transpose_b16(s, sh.g, sh.h, W3h, sh.h, W3t, sh.g))) return e;
   if ((e = gemm_nt(s, B, sh.h, sh.g, dZh, sh.g, W3t, sh.g, nullptr, false, dH, sh.h, nullptr, 0))) return e;
   hipLaunchKernelGGL(k_relu_bwd, dim3(grid1((long)B * sh.h)), dim3(256), 0, s, (long)B * sh.h, dH,
@@ -273,7 +278,7 @@ hipError_t mlp_fwd_bwd(hipStream_t s, const MlpShape &sh, int B, const uint16_t 
   if ((e = transpose_b16(s, B, sh.h, dHh, sh.h, Tb, Bp))) return e;
   if ((e = transpose_b16(s, B, sh.h, H1, sh.h, Ta, Bp))) return e;
   if ((e = gemm_nt(s, sh.h, sh.h, Bp, Tb, Bp, Ta, Bp, nullptr, false, gW2, sh.h, nullptr, 0))) return e;
-  hipLaunchKernelGGL(k_colsum, dim3(grid1(sh.h)), dim3(256), 0, s, B, sh.h, dHm, gb2);
+  hipLaunchKernelGGL(k_colsum, dim3((sh.h + 63) / 64), dim3(1024), 0, s, B, sh.h, dHm, gb2);
   if ((e = transpose_b16(s, sh.h, sh.h, W2h, sh.h, W2t, sh.h))) return e;
   if ((e = gemm_nt(s, B, sh.h, sh.h, dHh, sh.h, W2t, sh.h, nullptr, false, dH, sh.h, nullptr, 0))) return e;
   hipLaunchKernelGGL(k_relu_bwd, dim3(grid1((long)B * sh.h)), dim3(256), 0, s, (long)B * sh.h, dH,
@@ -282,10 +287,10 @@ hipError_t mlp_fwd_bwd(hipStream_t s, const MlpShape &sh, int B, const uint16_t 
   if ((e = transpose_b16(s, B, sh.h, dHh, sh.h, Tb, Bp))) return e;
   if ((e = transpose_b16(s, B, sh.din, X, sh.din, Tx, Bp))) return e;
   if ((e = gemm_nt(s, sh.h, sh.din, Bp, Tb, Bp, Tx, Bp, nullptr, false, gW1, sh.din, nullptr, 0))) return e;
-  hipLaunchKernelGGL(k_colsum, dim3(grid1(sh.h)), dim3(256), 0, s, B, sh.h, dHm, gb1);
+  hipLaunchKernelGGL(k_colsum, dim3((sh.h + 63) / 64), dim3(1024), 0, s, B, sh.h, dHm, gb1);
   // mean loss (left to right over the batch rows)
   if (loss) {
-    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, s, B, 1, rl, loss);
+    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(1024), 0, s, B, 1, rl, loss);
     hipLaunchKernelGGL(k_scale, dim3(1), dim3(1), 0, s, loss, 1.0f / (float)B);
   }
   return hipGetLastError();

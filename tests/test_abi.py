@@ -80,3 +80,16 @@ def test_workspace_size_and_layout():
 def test_no_cpu_fallback_in_package():
     src = open(os.path.join(ROOT, "kmeans_amd", "__init__.py")).read()
     assert "oracle" not in src.split('"""', 2)[2]
+
+
+def test_create_rejects_bad_proposer_options_without_device():
+    L = kmeans_amd.lib()
+    h = ctypes.c_void_p()
+    bad = [dict(proposer=3), dict(proposer=1, groups=20), dict(proposer=1, hidden=100),
+           dict(proposer=2, groups=2000), dict(proposer=2, top_groups=9),
+           dict(proposer=2, groups=4, top_groups=5), dict(proposer=1, tile_rows=1 << 23)]
+    for kw in bad:
+        opt = kmeans_amd.Options()
+        for f, v in kw.items():
+            setattr(opt, f, v)
+        assert L.kmeansCreate(ctypes.byref(h), 100_000, 8, 1024, ctypes.byref(opt)) == 14, kw

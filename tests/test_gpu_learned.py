@@ -85,3 +85,98 @@ def test_mlp_gradients_match_fp64_oracle(din, h, g, B, seed):
         assert np.linalg.norm(gg - gx) <= 1e-1 * np.linalg.norm(gx), (
             name, np.linalg.norm(gg - gx) / np.linalg.norm(gx))
 
+
+
+# ---------------------------------------------------------------- engine level
+def _c5_like(n=120_000, d=32, k=1024, seed=3):
+    """Gaussian mixture shaped like the 20M x 96 x 16384 workload, scaled down:
+    k means uniform in [-6, 6]^d, sigma uniform in [0.3, 1.5]; seeded sampled
+    initial centres."""
+    X, _, _, _ = orc.mixture(seed, n, d, k, -6.0, 6.0, 0.3, 1.5)
+    C0 = X[orc.sample_init(seed, n, k)].copy()
+    return X, C0
+
+
+def _two_level(n=100_000, d=32, k=1024, seed=5):
+    """Centres in 16 well separated clusters of 64: the group bound's case."""
+    rng = np.random.default_rng(seed)
+    sup = rng.uniform(-60, 60, (16, d))
+    means = (np.repeat(sup, k // 16, axis=0) + rng.normal(0, 1.2, (k, d))).astype(np.float32)
+    z = rng.integers(0, k, n)
+    X = (means[z] + rng.normal(0, 0.6, (n, d))).astype(np.float32)
+    C0 = X[orc.sample_init(seed, n, k)].copy()
+    return X, C0
+
+
+_ORACLE_CACHE = {}
+
+
+def _oracle_run(key, X, C0, iters):
+    if key not in _ORACLE_CACHE:
+        _ORACLE_CACHE[key] = orc.lloyd(X, C0, iters)
+    return _ORACLE_CACHE[key]
+
+
+def _run(X, C0, iters, **kw):
+    e = km.KMeans(X.shape[0], X.shape[1], C0.shape[0], **kw)
+    e.set_data(X)
+    e.set_centers(C0)
+    e.run(iters)
+    out = e.labels(), e.centers(), e.stats()
+    e.close()
+    return out
+
+
+def _check_counts(st, m, iters, train_iters):
+    learned_iters = iters - train_iters
+    assert st["proposed"] == m * learned_iters
+    assert st["certified"] + st["certified_group"] + st["fallback"] == st["proposed"]
+
+
+@pytest.mark.parametrize("proposer,precision", [("mlp", "fp32"), ("centroid", "fp32"),
+                                                ("mlp", "bf16")])
+def test_learned_lloyd_equals_oracle(proposer, precision):
+    X, C0 = _c5_like()
+    iters = 7
+    ref = _oracle_run("c5like", X, C0, iters)
+    lab, C, st = _run(X, C0, iters, precision=precision, proposer=proposer, groups=32,
+                      top_groups=4, train_iters=3, hidden=128)
+    print(f"\n{proposer}/{precision}: {st}")
+    # every iteration's labels equal the plain FP32 labels, so the whole
+    # trajectory (labels and canonical centres) is bit-identical
+    assert np.array_equal(lab, ref["labels"]), int((lab != ref["labels"]).sum())
+    assert np.array_equal(C, ref["centers"])
+    _check_counts(st, X.shape[0], iters, 3)
+    assert st["certified"] > 0
+    # far fewer exact scores than a full search of the proposer iterations
+    assert st["candidate_pairs"] < 0.5 * st["proposed"] * C0.shape[0]
+    if proposer == "mlp":
+        assert st["train_loss_last"] < st["train_loss_first"]
+
+
+def test_learned_group_bound_certifies_separated_groups():
+    X, C0 = _two_level()
+    iters = 6
+    ref = _oracle_run("twolevel", X, C0, iters)
+    lab, C, st = _run(X, C0, iters, proposer="centroid", groups=16, top_groups=2, train_iters=2)
+    print(f"\ntwo-level: {st}")
+    assert np.array_equal(lab, ref["labels"])
+    assert np.array_equal(C, ref["centers"])
+    _check_counts(st, X.shape[0], iters, 2)
+    assert st["certified_group"] > 0
+
+
+def test_learned_mlp_gain_over_centroid_proposer():
+    """What the trained network gains over the method without it: fewer points
+    left for the full search than the nearest-centroid proposer, at the same
+    groups and the same number of proposed groups."""
+    X, C0 = _c5_like()
+    iters = 7
+    res = {}
+    for prop in ("mlp", "centroid"):
+        _, _, st = _run(X, C0, iters, proposer=prop, groups=32, top_groups=4, train_iters=3,
+                        hidden=128)
+        res[prop] = st
+    print("\ngain:", {p: (s["certified"], s["certified_group"], s["fallback"], s["candidate_pairs"])
+                      for p, s in res.items()})
+    assert res["mlp"]["fallback"] <= res["centroid"]["fallback"]
