@@ -237,7 +237,7 @@ def bench_proposer(args, X, C0, spec, km, np, torch):
         "wall_ms": {"plain_iters": (t1 - t0) * 1e3, "train_and_learned_iters": (t2 - t1) * 1e3},
     }
     out["proposer_stats"]["candidate_pairs_fraction_of_full"] = st["candidate_pairs"] / (pairs * nl)
-    for prec in (["fp32"] if args.no_fp32_ref else ["fp32", "bf16"]):
+    for prec in ([] if args.no_fp32_ref else ["fp32", "bf16"]):
         r = fresh(prec)
         ta = time.perf_counter()
         r.run(steps)
@@ -251,7 +251,8 @@ def bench_proposer(args, X, C0, spec, km, np, torch):
             out["label_mismatches_vs_fp32"] = int((lab32 != labels).sum())
             out["centers_equal_fp32"] = bool(np.array_equal(r.centers(), centers))
         r.close()
-    out["speedup_assign_vs_fp32"] = out["full_fp32"]["assign_ms"] / learned_assign_ms
+    if "full_fp32" in out:
+        out["speedup_assign_vs_fp32"] = out["full_fp32"]["assign_ms"] / learned_assign_ms
     return out
 
 

@@ -10,10 +10,11 @@
 // Accuracy and Stability of Numerical Algorithms, 2nd ed., sec. 3.1).  Since
 // ||x - c_j||^2 = ||x||^2 - 2 sigma_j, a label j* with score s* is the plain
 // FP32 argmax whenever every other centre is shown to have s_j < s*:
-//  * separation (Elkan, ICML 2003, lemma 1): with a >= ||x - c*|| and
-//    h <= 0.5 min_{j != *} ||c* - c_j||, every other centre has
+//  * separation (Elkan, ICML 2003, lemma 1, restricted to the centres not
+//    scored): with a >= ||x - c*|| and h <= 0.5 min ||c* - c_j|| over the
+//    members j of the groups not proposed, every such centre has
 //    sigma* - sigma_j >= 2 h (h - a); certified when h > a and
-//    2 h (h - a) > 2 E;
+//    2 h (h - a) > 2 E (the proposed groups were scored exactly);
 //  * group bound: for a group g not proposed, ||x - c_j|| >= t_g =
 //    max(0, ||x - G_g|| - r_g) for its members, so s_j <= 0.5 (||x||^2 - t_g^2)
 //    + E; certified when s* exceeds that for every such group.
@@ -30,6 +31,7 @@
 #include "km_common.hpp"
 #include "km_engine.hpp"
 #include "km_learned.hpp"
+#include "km_mfma.hpp"
 #include "km_mlp.hpp"
 
 namespace km {
@@ -47,7 +49,7 @@ struct Learned {
   int ldP = 0;
   std::vector<int> gsize_h, gcol_h;
   float *CtP = nullptr, *ncP = nullptr, *Gc = nullptr, *gnh = nullptr, *rad2 = nullptr;
-  unsigned *sep2 = nullptr;
+  unsigned *sep2 = nullptr;  // k x G: min squared distance from centre i to group g
   float *ncmax = nullptr;
   // MLP
   MlpShape sh{};
@@ -72,6 +74,11 @@ struct Learned {
   int *hcnt = nullptr;                      // pinned: gcnt, goff
   double ms_train = 0.0, loss_first = 0.0, loss_last = 0.0;
   long long pairs = 0;                      // exact candidate scores issued
+  // reduced-precision fallback: uncertified rows gathered contiguously
+  long fb_cap = 0;
+  float *fbX = nullptr, *fbN = nullptr;
+  uint16_t *fbXh = nullptr;
+  int *fbLab = nullptr;
 };
 
 static inline Learned *L_of(const Engine *e) { return (Learned *)e->learned; }
@@ -147,31 +154,52 @@ __global__ void k_targets(long ns, const int *__restrict__ lab, const int *__res
   if (i < ns) tg[i] = gof[lab[i]];
 }
 
-// Top-T (T <= 8) logits per row; ties to the lower group index.
-__global__ void k_top_logits(long rows, int G, int T, const float *__restrict__ Z,
-                             int *__restrict__ cand) {
-  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= rows) return;
+// Top-T (T <= 8) logits per row; ties to the lower group index.  Sixteen
+// lanes per row read the row in 64-byte pieces, keep their own top-T, then T
+// rounds of a 16-lane argmax pick the winners.
+__global__ __launch_bounds__(256) void k_top_logits(long rows, int G, int T,
+                                                    const float *__restrict__ Z,
+                                                    int *__restrict__ cand) {
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long r = gid >> 4;
+  const int l = (int)(gid & 15);
+  const bool live = r < rows;
   float bv[8];
   int bi[8];
 #pragma unroll
-  for (int t = 0; t < 8; ++t) { bv[t] = -INFINITY; bi[t] = -1; }
-  const float *z = Z + r * G;
-  for (int g = 0; g < G; ++g) {
-    float v = z[g];
-    int gi = g;
+  for (int t = 0; t < 8; ++t) { bv[t] = -INFINITY; bi[t] = 0x7fffffff; }
+  if (live) {
+    const float *z = Z + r * G;
+    for (int g = l; g < G; g += 16) {
+      float v = z[g];
+      int gi = g;
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      if (t < T && v > bv[t]) {
-        const float tv = bv[t];
-        const int ti = bi[t];
-        bv[t] = v; bi[t] = gi; v = tv; gi = ti;
+      for (int t = 0; t < 8; ++t) {
+        if (t < T && (v > bv[t] || (v == bv[t] && gi < bi[t]))) {
+          const float tv = bv[t];
+          const int ti = bi[t];
+          bv[t] = v; bi[t] = gi; v = tv; gi = ti;
+        }
       }
     }
   }
+  for (int t = 0; t < T; ++t) {
+    float v = bv[0];
+    int i = bi[0];
 #pragma unroll
-  for (int t = 0; t < 8; ++t)
-    if (t < T) cand[r * T + t] = bi[t];
+    for (int off = 8; off >= 1; off >>= 1) {
+      const float ov = __shfl_xor(v, off);
+      const int oi = __shfl_xor(i, off);
+      if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+    }
+    if (bi[0] == i) {  // this lane's head won: pop it
+#pragma unroll
+      for (int q = 0; q < 7; ++q) { bv[q] = bv[q + 1]; bi[q] = bi[q + 1]; }
+      bv[7] = -INFINITY;
+      bi[7] = 0x7fffffff;
+    }
+    if (live && l == 0) cand[r * T + t] = (v == -INFINITY || i == 0x7fffffff) ? -1 : i;
+  }
 }
 
 // Nearest-centroid proposer: the T groups with the largest
@@ -355,16 +383,40 @@ __global__ __launch_bounds__(256) void k_group_geom(int n, const int *__restrict
   }
 }
 
-// sep2[i] = min over j != i of the FP32 squared distance ||c_i - c_j||^2
-// (chain of squared differences, ascending d), as float bits; sep2 must hold
-// +inf bits on entry.  64 x 64 centre pairs per block, 4 x 4 per thread.
+// D[i][g] = min over members j != i of group g of the FP32 squared distance
+// ||c_i - c_j||^2 (chain of squared differences, ascending d), as float bits;
+// D must hold +inf bits on entry.  Rows i in centre order, columns in group
+// order (CtP), so a 64-column tile spans few groups: minima are first taken
+// per (row, group) in LDS, then one global atomicMin per (row, group).
 #define SEP_T 64
 #define SEP_DS 32
-__global__ __launch_bounds__(256) void k_center_sep(int k, int n, const float *__restrict__ C,
-                                                    unsigned *__restrict__ sep2) {
+__global__ __launch_bounds__(256) void k_center_sep(int k, int n, int G, const float *__restrict__ C,
+                                                    const float *__restrict__ CtP, int ldP,
+                                                    const int *__restrict__ permP,
+                                                    const int *__restrict__ gof,
+                                                    unsigned *__restrict__ D) {
   __shared__ float A[SEP_DS][SEP_T + 1], B[SEP_DS][SEP_T + 1];
-  const int i0 = blockIdx.y * SEP_T, j0 = blockIdx.x * SEP_T;
+  __shared__ unsigned mins[SEP_T][SEP_T];
+  __shared__ int colj[SEP_T], colg[SEP_T], colslot[SEP_T], slotg[SEP_T], nslot;
+  const int i0 = blockIdx.y * SEP_T, c0 = blockIdx.x * SEP_T;
   const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15;
+  if (tid < SEP_T) {
+    const int c = c0 + tid;
+    const int j = c < ldP ? permP[c] : -1;
+    colj[tid] = j;
+    colg[tid] = j >= 0 ? gof[j] : -1;
+  }
+  for (int e = tid; e < SEP_T * SEP_T; e += 256) (&mins[0][0])[e] = 0x7f800000u;
+  __syncthreads();
+  if (tid == 0) {  // columns are in group order: one slot per distinct group
+    int ns = 0, last = -1;
+    for (int q = 0; q < SEP_T; ++q) {
+      const int g = colg[q];
+      if (g >= 0 && g != last) { slotg[ns++] = g; last = g; }
+      colslot[q] = ns - 1;
+    }
+    nslot = ns;
+  }
   float acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
@@ -376,7 +428,11 @@ __global__ __launch_bounds__(256) void k_center_sep(int k, int n, const float *_
       const int r = e / SEP_DS, dd = e - r * SEP_DS;
       const int d = d0 + dd;
       A[dd][r] = (i0 + r < k && d < n) ? C[(long)(i0 + r) * n + d] : 0.0f;
-      B[dd][r] = (j0 + r < k && d < n) ? C[(long)(j0 + r) * n + d] : 0.0f;
+    }
+    for (int e = tid; e < SEP_T * SEP_DS; e += 256) {
+      const int dd = e / SEP_T, r = e - dd * SEP_T;
+      const int d = d0 + dd;
+      B[dd][r] = (c0 + r < ldP && d < n) ? CtP[(long)d * ldP + c0 + r] : 0.0f;
     }
     __syncthreads();
     const int dn = min(SEP_DS, n - d0);
@@ -398,15 +454,21 @@ __global__ __launch_bounds__(256) void k_center_sep(int k, int n, const float *_
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
     const int i = i0 + ti * 4 + a;
-    float mn = INFINITY;
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-      const int j = j0 + tj + 16 * b;
-      if (j < k && j != i) mn = fminf(mn, acc[a][b]);
+      const int q = tj + 16 * b;
+      const int j = colj[q];
+      if (i < k && j >= 0 && j != i)
+        atomicMin(&mins[ti * 4 + a][colslot[q]], __float_as_uint(acc[a][b]));
     }
-#pragma unroll
-    for (int o = 8; o >= 1; o >>= 1) mn = fminf(mn, __shfl_xor(mn, o));
-    if (tj == 0 && i < k && mn < INFINITY) atomicMin(&sep2[i], __float_as_uint(mn));
+  }
+  __syncthreads();
+  const int ns = nslot;
+  for (int e = tid; e < SEP_T * ns; e += 256) {
+    const int r = e / ns, sl = e - r * ns;
+    const int i = i0 + r;
+    const unsigned v = mins[r][sl];
+    if (i < k && v != 0x7f800000u) atomicMin(&D[(long)i * G + slotg[sl]], v);
   }
 }
 
@@ -437,8 +499,8 @@ __device__ inline double score_err(double qu, double ncm, double gam) {
 __global__ __launch_bounds__(256) void k_combine(
     long m, int T, const int *__restrict__ cand, const int *__restrict__ gsize,
     const int *__restrict__ gcol, const int *__restrict__ permP, const int *__restrict__ elab,
-    const float *__restrict__ escore, const double *__restrict__ q,
-    const unsigned *__restrict__ sep2, const float *__restrict__ ncmax, BoundParams bp,
+    const float *__restrict__ escore, const double *__restrict__ q, int G,
+    const unsigned *__restrict__ sepD, const float *__restrict__ ncmax, BoundParams bp,
     int *__restrict__ labels, int *__restrict__ ul_pt, int *__restrict__ ul_best,
     float *__restrict__ ul_score, int *__restrict__ ucnt, unsigned long long *__restrict__ cnt) {
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -460,8 +522,16 @@ __global__ __launch_bounds__(256) void k_combine(
       const double E = score_err(qu, (double)ncmax[0], bp.gam);
       const double a2 = qu - 2.0 * (double)bs + 2.0 * E;
       const double a = sqrt(fmax(a2, 0.0)) * (1.0 + 1e-14);
-      const double s2 = (double)__uint_as_float(sep2[bj]);
-      const double h = 0.5 * sqrt(s2 / (1.0 + bp.gam)) * (1.0 - 1e-14);
+      // nearest centre outside the named groups (their members were scored)
+      const unsigned *Dr = sepD + (long)bj * G;
+      float s2f = INFINITY;
+      for (int g = 0; g < G; ++g) {
+        bool named = false;
+        for (int t = 0; t < T; ++t) named |= cand[p * T + t] == g;
+        if (!named) s2f = fminf(s2f, __uint_as_float(Dr[g]));
+      }
+      const double s2 = (double)s2f;
+      const double h = s2f == INFINITY ? INFINITY : 0.5 * sqrt(s2 / (1.0 + bp.gam)) * (1.0 - 1e-14);
       cert = (h > a) && (2.0 * h * (h - a) > 2.0 * E * (1.0 + 1e-12));
     }
   }
@@ -541,6 +611,28 @@ __global__ void k_hits(long m, int T, const int *__restrict__ cand, const int *_
   wave_count(hit, &cnt[4]);
 }
 
+// Uncertified rows gathered for the reduced-precision full search: FP32
+// row, reduced-precision row (stride Dp) and norm bound.
+__global__ void k_gather_fb(long cnt, const int *__restrict__ fb, int n, int Dp,
+                            const float *__restrict__ X, const uint16_t *__restrict__ Xh,
+                            const float *__restrict__ xn, float *__restrict__ gX,
+                            uint16_t *__restrict__ gXh, float *__restrict__ gN) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long r = i / Dp;
+  const int d = (int)(i - r * Dp);
+  if (r >= cnt) return;
+  const long p = fb[r];
+  gXh[i] = Xh[p * Dp + d];
+  if (d < n) gX[r * n + d] = X[p * n + d];
+  if (d == 0) gN[r] = xn[p];
+}
+
+__global__ void k_scatter_fb(long cnt, const int *__restrict__ fb, const int *__restrict__ lab,
+                             int *__restrict__ labels) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < cnt) labels[fb[i]] = lab[i];
+}
+
 // ------------------------------------------------------------------ host
 static hipError_t lmalloc(void **p, size_t bytes) { return hipMalloc(p, bytes ? bytes : 1); }
 
@@ -589,7 +681,8 @@ void learned_free(Engine *e) {
                   L->gsize, L->gcol, L->permP, L->CtP, L->ncP, L->Gc, L->gnh, L->rad2, L->sep2,
                   L->ncmax, L->P, L->Gr, L->Am, L->Av, L->loss, L->Ph, L->mws, L->Xh, L->H1,
                   L->H2, L->Z, L->cand, L->gcnt, L->ent_pt, L->ent_id, L->elab, L->escore,
-                  L->ul_pt, L->ul_best, L->ucnt, L->fb, L->ul_score, L->cnt};  // goff, gcur, fbcnt
+                  L->ul_pt, L->ul_best, L->ucnt, L->fb, L->ul_score, L->cnt, L->fbX, L->fbN,
+                  L->fbXh, L->fbLab};  // goff, gcur, fbcnt
                                                                                // live inside gcnt, ucnt
   for (void *b : bufs) hipFree(b);
   if (L->hcnt) hipHostFree(L->hcnt);
@@ -678,7 +771,7 @@ static kmeansErrorStatus build_groups(Engine *e, Learned *L) {
   KM_HIP_CHECK(lmalloc((void **)&L->Gc, sizeof(float) * (size_t)G * n), KMEANS_ERROR_COMPUTE);
   KM_HIP_CHECK(lmalloc((void **)&L->gnh, sizeof(float) * G), KMEANS_ERROR_COMPUTE);
   KM_HIP_CHECK(lmalloc((void **)&L->rad2, sizeof(float) * G), KMEANS_ERROR_COMPUTE);
-  KM_HIP_CHECK(lmalloc((void **)&L->sep2, sizeof(unsigned) * k), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(lmalloc((void **)&L->sep2, sizeof(unsigned) * (size_t)k * G), KMEANS_ERROR_COMPUTE);
   KM_HIP_CHECK(lmalloc((void **)&L->ncmax, sizeof(float)), KMEANS_ERROR_COMPUTE);
   return KMEANS_SUCCESS;
 }
@@ -809,10 +902,11 @@ static hipError_t refresh_centres(Engine *e, Learned *L) {
                      e->dC, e->dnc, L->CtP, L->ncP);
   hipLaunchKernelGGL(k_group_geom, dim3(L->G), dim3(256), sizeof(float) * n, s, n, L->gstart,
                      L->members, e->dC, L->Gc, L->gnh, L->rad2);
-  hipLaunchKernelGGL(k_fill_i32, dim3(g1(k)), dim3(256), 0, s, (long)k, (int *)L->sep2,
-                     (int)0x7f800000);
-  const int tiles = (k + SEP_T - 1) / SEP_T;
-  hipLaunchKernelGGL(k_center_sep, dim3(tiles, tiles), dim3(256), 0, s, k, n, e->dC, L->sep2);
+  hipLaunchKernelGGL(k_fill_i32, dim3(g1((long)k * L->G)), dim3(256), 0, s, (long)k * L->G,
+                     (int *)L->sep2, (int)0x7f800000);
+  dim3 sg((L->ldP + SEP_T - 1) / SEP_T, (k + SEP_T - 1) / SEP_T);
+  hipLaunchKernelGGL(k_center_sep, sg, dim3(256), 0, s, k, n, L->G, e->dC, L->CtP, L->ldP, L->permP,
+                     L->gof, L->sep2);
   hipLaunchKernelGGL(k_ncmax, dim3(1), dim3(256), 0, s, k, e->dnc, L->ncmax);
   return hipGetLastError();
 }
@@ -825,7 +919,7 @@ static hipError_t propose(Engine *e, Learned *L) {
       const long R = std::min(L->chunk, m - r0);
       LCHK(mlp_forward(s, L->sh, (int)R, L->Xh + (size_t)r0 * L->din, L->din, L->Ph, L->P, L->H1,
                        L->H2, L->Z));
-      hipLaunchKernelGGL(k_top_logits, dim3(g1(R)), dim3(256), 0, s, R, L->G, L->T, L->Z,
+      hipLaunchKernelGGL(k_top_logits, dim3(g1(R * 16)), dim3(256), 0, s, R, L->G, L->T, L->Z,
                          L->cand + (size_t)r0 * L->T);
     }
     return hipGetLastError();
@@ -880,7 +974,7 @@ hipError_t learned_assign(Engine *e) {
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, 2L, L->ucnt, 0);
   BoundParams bp{gamma_n(e->n)};
   hipLaunchKernelGGL(k_combine, dim3(g1(m)), dim3(256), 0, s, m, L->T, L->cand, L->gsize, L->gcol,
-                     L->permP, L->elab, L->escore, e->dq, L->sep2, L->ncmax, bp, e->dlabels,
+                     L->permP, L->elab, L->escore, e->dq, G, L->sep2, L->ncmax, bp, e->dlabels,
                      L->ul_pt, L->ul_best, L->ul_score, L->ucnt, L->cnt);
   hipLaunchKernelGGL(k_group_bound, dim3((unsigned)std::min<long>(g1(m), 4L * num_cus())), dim3(256),
                      0, s, L->ucnt, e->n, e->dX, L->T, L->cand, G, L->Gc, L->rad2, L->gsize, e->dq,
@@ -891,8 +985,37 @@ hipError_t learned_assign(Engine *e) {
 
 hipError_t learned_fallback(Engine *e) {
   Learned *L = L_of(e);
-  LCHK(launch_assign_f32(e->stream, e->m, L->fbcnt, e->n, e->dX, L->fb, e->k, e->dCt, e->ldc,
-                         e->dnc, e->dlabels, nullptr));
+  if (e->opt.precision != KM_PREC_FP32 && mfma_ready(e)) {
+    // full search of the uncertified rows on the reduced-precision path
+    // (itself certified against the FP32 scores, near-ties re-resolved in FP32)
+    int cnt = 0;
+    LCHK(hipMemcpyAsync(&cnt, L->fbcnt, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    LCHK(hipStreamSynchronize(e->stream));
+    const int Dp = mfma_row_stride(e);
+    if (cnt > L->fb_cap) {
+      const long cap = std::min<long>(e->m, (long)cnt + cnt / 4 + 1024);
+      hipFree(L->fbX); hipFree(L->fbN); hipFree(L->fbXh); hipFree(L->fbLab);
+      L->fbX = nullptr; L->fbN = nullptr; L->fbXh = nullptr; L->fbLab = nullptr;
+      L->fb_cap = 0;
+      LCHK(lmalloc((void **)&L->fbX, sizeof(float) * cap * e->n));
+      LCHK(lmalloc((void **)&L->fbN, sizeof(float) * cap));
+      LCHK(lmalloc((void **)&L->fbXh, sizeof(uint16_t) * cap * Dp));
+      LCHK(lmalloc((void **)&L->fbLab, sizeof(int) * cap));
+      L->fb_cap = cap;
+    }
+    if (cnt > 0) {
+      hipLaunchKernelGGL(k_gather_fb, dim3(g1((long)cnt * Dp)), dim3(256), 0, e->stream, (long)cnt,
+                         L->fb, e->n, Dp, e->dX, mfma_tile_xh(e), mfma_tile_xnorm(e), L->fbX,
+                         L->fbXh, L->fbN);
+      LCHK(mfma_assign_view(e, L->fbX, L->fbXh, L->fbN, L->fbLab, cnt));
+      LCHK(mfma_reresolve_view(e, L->fbX, L->fbLab, cnt));
+      hipLaunchKernelGGL(k_scatter_fb, dim3(g1(cnt)), dim3(256), 0, e->stream, (long)cnt, L->fb,
+                         L->fbLab, e->dlabels);
+    }
+  } else {
+    LCHK(launch_assign_f32(e->stream, e->m, L->fbcnt, e->n, e->dX, L->fb, e->k, e->dCt, e->ldc,
+                           e->dnc, e->dlabels, nullptr));
+  }
   hipLaunchKernelGGL(k_hits, dim3(g1(e->m)), dim3(256), 0, e->stream, e->m, L->T, L->cand, L->gof,
                      e->dlabels, L->cnt);
   return hipGetLastError();

@@ -722,6 +722,8 @@ hipError_t mfma_reresolve_view(Engine *e, const float *X, int *lab, long rows) {
 
 const uint16_t *mfma_tile_xh(Engine *e) { return st(e)->Xh; }
 const float *mfma_tile_xnorm(Engine *e) { return st(e)->xnorm; }
+int mfma_row_stride(Engine *e) { return st(e)->Dp; }
+bool mfma_ready(Engine *e) { return st(e) && st(e)->x_ready && !e->streaming; }
 
 void mfma_collect(Engine *e, int nslots) {
   MfmaState *s = st(e);

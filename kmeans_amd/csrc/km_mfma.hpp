@@ -19,4 +19,6 @@ hipError_t mfma_assign_view(Engine *e, const float *X, const uint16_t *Xh, const
 hipError_t mfma_reresolve_view(Engine *e, const float *X, int *lab, long rows);
 const uint16_t *mfma_tile_xh(Engine *e);
 const float *mfma_tile_xnorm(Engine *e);
+int mfma_row_stride(Engine *e);  // padded width Dp of the reduced-precision rows
+bool mfma_ready(Engine *e);      // reduced-precision copy of every point present
 }  // namespace km
