@@ -56,6 +56,9 @@ def main():
     from kmeans_amd import datagen
 
     torch.cuda.init()
+    if args.workload == "c4":
+        print(json.dumps(bench_batch(args, km, datagen, np, torch)))
+        return
     X, C0, spec = datagen.generate_on_gpu(args.workload, n=args.n, torch=torch)
     n, d, k = spec["n"], spec["d"], spec["k"]
 
@@ -167,6 +170,95 @@ def main():
         torch.cuda.synchronize()
         out["calibration_copy_bytes"] = 1 << 30
     print(json.dumps(out))
+
+
+def bench_batch(args, km, datagen, np, torch):
+    """2048 independent clusterings (D=16, K=256, lognormal N, first-K initial
+    centres) as one batched engine: every iteration is one persistent assign
+    launch and one persistent update launch, the run one hipGraph.  Compared
+    with running the single-problem engine once per instance (timed on a
+    sample of instances, scaled by their share of the points)."""
+    spec = datagen.C4
+    d, k = spec["d"], spec["k"]
+    X, n, seeds = datagen.generate_c4_on_gpu(torch, nb=args.n)
+    nb = len(n)
+    total = int(n.sum())
+    steps = args.steps
+    b = km.KMeansBatch(n, d, k)
+    b.set_data(device_ptr=X.data_ptr())
+    b.init_first_k()
+    if args.warmup:
+        b.run(args.warmup)  # also autotunes the tile size
+        b.init_first_k()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.run(steps)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    st = b.stats()
+    pairs = float(total) * k
+    out = {
+        "metric": "assign pts*ctrs/s per Lloyd iter",
+        "value": pairs * steps / dt,
+        "unit": "pts*ctrs/s",
+        "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3 / steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (per-instance Gaussian mixtures generated on the GPU)",
+        "config": {"workload": f"c4: {nb} instances, D={d} K={k}, lognormal N (median 20000, "
+                               f"sigma 1, clipped [1000, 200000]), sum N={total}, first-K init, "
+                               f"one hipGraph per run",
+                   "instances": nb, "D": d, "K": k, "sum_N": total},
+        "graph_ms_per_step": st["ms_graph"] / steps,
+        "batch_stats": st,
+        "roofline": {"bound": "fp32", "achieved": 2.0 * d * pairs / (st["ms_graph"] / steps * 1e-3) / 1e12,
+                     "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
+                     "frac": 2.0 * d * pairs / (st["ms_graph"] / steps * 1e-3) / PEAK["fp32"],
+                     "traffic": None, "kernel": "whole iteration (assign + update)",
+                     "algorithmic": "2*D*sum(N)*K flops per iteration"},
+    }
+    labels = b.labels()
+    b.close()
+    # the method without batching: one single-problem engine per instance
+    sample = np.linspace(0, nb - 1, min(nb, 48)).astype(int)
+    off = np.concatenate([[0], np.cumsum(n)])
+    t_seq = 0.0
+    mism = 0
+    for i in sample:
+        Xi = X[off[i]:off[i + 1]]
+        e = km.KMeans(int(n[i]), d, k)
+        e.set_data(device_ptr=Xi.data_ptr())
+        e.set_centers(Xi[:k].cpu().numpy())
+        e.synchronize()
+        ta = time.perf_counter()
+        e.run(steps)
+        e.synchronize()
+        t_seq += time.perf_counter() - ta
+        mism += int((e.labels() != labels[off[i]:off[i + 1]]).sum())
+        e.close()
+    share = float(n[sample].sum()) / total
+    out["per_instance_engine"] = {
+        "sample_instances": len(sample), "sample_points": int(n[sample].sum()),
+        "ms_per_step_extrapolated": t_seq / share * 1e3 / steps,
+        "label_mismatches_vs_batch": mism,
+    }
+    out["speedup_vs_per_instance_engine"] = out["per_instance_engine"]["ms_per_step_extrapolated"] / out["ms_per_step"]
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        cs = sample[:8]
+        t_cpu = 0.0
+        for i in cs:
+            Xi = X[off[i]:off[i + 1]].cpu().numpy()
+            ta = time.perf_counter()
+            orc.lloyd(Xi, Xi[:k].copy(), 1)
+            t_cpu += time.perf_counter() - ta
+        pts = float(n[cs].sum())
+        out["cpu_baseline"] = {"value": pts * k / t_cpu, "unit": "pts*ctrs/s",
+                               "cores": orc.num_threads(), "kind": "port",
+                               "sample": f"one full Lloyd iteration of the CPU oracle on {len(cs)} "
+                                         f"instances ({int(pts)} points), first-K centres"}
+    return out
 
 
 def bench_proposer(args, X, C0, spec, km, np, torch):

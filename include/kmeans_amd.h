@@ -188,6 +188,54 @@ const int *kmeansDeviceLabels(kmeansHandle h);
 kmeansErrorStatus kmeansSynchronize(kmeansHandle h);
 
 /* =========================================================================
+ * 1b. Batched independent clusterings (no counterpart in the reference: its
+ * host class runs one problem per object).  For callers with thousands of
+ * small problems of uneven size: every instance has its own points (n[i] rows,
+ * concatenated in instance order), its own k centres and the same canonical
+ * semantics as kmeansRun (FP32 labels, canonical FP64 sums, compactness from
+ * FP64 moments).  Each phase of an iteration is one persistent launch over
+ * all instances, and a run of `iters` iterations is one hipGraph launch.
+ * Limits: d <= 64, k <= 1024, k * (d rounded up to 4, 8, 16, 32 or 64, + 1)
+ * * 4 bytes <= 64 KiB, k <= n[i] < KM_BLOCK, sum of n[i] < 2^31; FP32 only.
+ * ========================================================================= */
+typedef struct kmeansBatchStats {
+  int iterations;          /* total over runs                                 */
+  int last_run_iters;
+  int tile_rows;           /* points per assign work item (autotuned)         */
+  int work_items;
+  int graph_nodes;
+  int tuned_from_cache;    /* 1: tile_rows came from the tuning cache         */
+  long long empty_clusters;
+  double ms_graph;         /* device time of the last run's graph             */
+  double ms_capture;       /* host time to capture and instantiate it         */
+  float tune_ms[5];        /* assign time per candidate tile: 256 .. 4096     */
+} kmeansBatchStats;
+
+typedef struct kmeansBatchEngine *kmeansBatchHandle;
+
+kmeansErrorStatus kmeansBatchCreate(kmeansBatchHandle *h, int instances, const long *n, int d,
+                                    int k, const kmeansOptions *opt);
+void kmeansBatchDestroy(kmeansBatchHandle h);
+/* All points (sum n[i] x d), host (copied) or device (borrowed). */
+kmeansErrorStatus kmeansBatchSetData(kmeansBatchHandle h, const float *X, int on_device);
+/* Initial centres, instances x k x d, host or device. */
+kmeansErrorStatus kmeansBatchSetCenters(kmeansBatchHandle h, const float *C, int on_device);
+/* Initial centres = the first k points of each instance. */
+kmeansErrorStatus kmeansBatchInitFirstK(kmeansBatchHandle h);
+/* Points per assign work item (even); 0 = autotune on the next run. */
+kmeansErrorStatus kmeansBatchSetTileRows(kmeansBatchHandle h, int rows);
+/* `iters` Lloyd iterations of every instance (criterion < 0 semantics). */
+kmeansErrorStatus kmeansBatchRun(kmeansBatchHandle h, int iters);
+kmeansErrorStatus kmeansBatchGetCenters(kmeansBatchHandle h, float *C /* instances x k x d */);
+kmeansErrorStatus kmeansBatchGetLabels(kmeansBatchHandle h, int *labels /* sum n[i] */);
+/* Compactness of every instance after each iteration of the last run
+ * (last_run_iters x instances, FP64). */
+kmeansErrorStatus kmeansBatchGetCompactness(kmeansBatchHandle h, double *comp);
+kmeansErrorStatus kmeansBatchGetStats(kmeansBatchHandle h, kmeansBatchStats *stats);
+const float *kmeansBatchDeviceCenters(kmeansBatchHandle h);
+const int *kmeansBatchDeviceLabels(kmeansBatchHandle h);
+
+/* =========================================================================
  * 2. Kernel API (device pointers; stream is a hipStream_t or NULL)
  * ========================================================================= */
 

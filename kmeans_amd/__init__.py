@@ -87,6 +87,19 @@ SIGNATURES = {
     "kmeansDeviceCenters": (_P, [_P]),
     "kmeansDeviceLabels": (_P, [_P]),
     "kmeansSynchronize": (_I, [_P]),
+    "kmeansBatchCreate": (_I, [_P, _I, _P, _I, _I, _P]),
+    "kmeansBatchDestroy": (None, [_P]),
+    "kmeansBatchSetData": (_I, [_P, _P, _I]),
+    "kmeansBatchSetCenters": (_I, [_P, _P, _I]),
+    "kmeansBatchInitFirstK": (_I, [_P]),
+    "kmeansBatchSetTileRows": (_I, [_P, _I]),
+    "kmeansBatchRun": (_I, [_P, _I]),
+    "kmeansBatchGetCenters": (_I, [_P, _P]),
+    "kmeansBatchGetLabels": (_I, [_P, _P]),
+    "kmeansBatchGetCompactness": (_I, [_P, _P]),
+    "kmeansBatchGetStats": (_I, [_P, _P]),
+    "kmeansBatchDeviceCenters": (_P, [_P]),
+    "kmeansBatchDeviceLabels": (_P, [_P]),
     "kmColNorms": (_I, [_P, _I, _I, _P, _P]),
     "kmCentersLd": (_I, [_I]),
     "kmTransposeCenters": (_I, [_P, _I, _I, _P, _P]),
@@ -219,6 +232,89 @@ class KMeans:
 
     def device_centers_ptr(self):
         return lib().kmeansDeviceCenters(self._h)
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("last_run_iters", ctypes.c_int),
+                ("tile_rows", ctypes.c_int), ("work_items", ctypes.c_int),
+                ("graph_nodes", ctypes.c_int), ("tuned_from_cache", ctypes.c_int),
+                ("empty_clusters", ctypes.c_longlong), ("ms_graph", ctypes.c_double),
+                ("ms_capture", ctypes.c_double), ("tune_ms", ctypes.c_float * 5)]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_}
+        d["tune_ms"] = list(self.tune_ms)
+        return d
+
+
+class KMeansBatch:
+    """Many independent clusterings of uneven size (kmeansBatch*): instance i
+    owns n[i] consecutive rows of the data and k centres."""
+
+    def __init__(self, n, d, k, device=0, stream=None):
+        self.n = np.ascontiguousarray(n, dtype=np.int64)
+        self.nb, self.d, self.k = len(self.n), int(d), int(k)
+        self.total = int(self.n.sum())
+        opt = Options(0, device, stream, 0, 0)
+        h = ctypes.c_void_p()
+        _check(lib().kmeansBatchCreate(ctypes.byref(h), self.nb, _np_ptr(self.n), self.d, self.k,
+                                       ctypes.byref(opt)), "kmeansBatchCreate")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().kmeansBatchDestroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_data(self, X=None, device_ptr=None):
+        if device_ptr is not None:
+            _check(lib().kmeansBatchSetData(self._h, ctypes.c_void_p(device_ptr), 1), "kmeansBatchSetData")
+        else:
+            X = np.ascontiguousarray(X, dtype=np.float32)
+            assert X.shape == (self.total, self.d)
+            _check(lib().kmeansBatchSetData(self._h, _np_ptr(X), 0), "kmeansBatchSetData")
+
+    def set_centers(self, C=None, device_ptr=None):
+        if device_ptr is not None:
+            _check(lib().kmeansBatchSetCenters(self._h, ctypes.c_void_p(device_ptr), 1),
+                   "kmeansBatchSetCenters")
+        else:
+            C = np.ascontiguousarray(C, dtype=np.float32)
+            assert C.shape == (self.nb, self.k, self.d)
+            _check(lib().kmeansBatchSetCenters(self._h, _np_ptr(C), 0), "kmeansBatchSetCenters")
+
+    def init_first_k(self):
+        _check(lib().kmeansBatchInitFirstK(self._h), "kmeansBatchInitFirstK")
+
+    def set_tile_rows(self, rows):
+        _check(lib().kmeansBatchSetTileRows(self._h, int(rows)), "kmeansBatchSetTileRows")
+
+    def run(self, iters):
+        _check(lib().kmeansBatchRun(self._h, int(iters)), "kmeansBatchRun")
+
+    def centers(self):
+        out = np.empty((self.nb, self.k, self.d), np.float32)
+        _check(lib().kmeansBatchGetCenters(self._h, _np_ptr(out)), "kmeansBatchGetCenters")
+        return out
+
+    def labels(self):
+        out = np.empty(self.total, np.int32)
+        _check(lib().kmeansBatchGetLabels(self._h, _np_ptr(out)), "kmeansBatchGetLabels")
+        return out
+
+    def compactness(self):
+        it = self.stats()["last_run_iters"]
+        out = np.empty((it, self.nb), np.float64)
+        _check(lib().kmeansBatchGetCompactness(self._h, _np_ptr(out)), "kmeansBatchGetCompactness")
+        return out
+
+    def stats(self):
+        s = BatchStats()
+        _check(lib().kmeansBatchGetStats(self._h, ctypes.byref(s)), "kmeansBatchGetStats")
+        return s.as_dict()
 
 
 # ------------------------------------------------------------------ kernel API

@@ -1,0 +1,1021 @@
+// Batched independent clusterings: many small K-means problems of uneven N
+// in one persistent launch per phase, the whole fixed-count Lloyd loop
+// captured as one hipGraph.
+//
+// No counterpart in the reference (its host class runs one problem per
+// object, main/cpp/kmeans.hpp:174-658).  Every instance follows the same
+// canonical semantics as the single-problem engine: FP32 fma-chain scores
+// (ClosestCenters.cu:60,65-69) with the lowest index among maxima, canonical
+// FP64 centre sums (members in ascending point order, chunks of KM_CHUNK
+// summed left to right, chunk sums combined the same way), empty clusters
+// keep their centre, compactness from the per-centre FP64 moments.  Each
+// instance must have fewer than KM_BLOCK points, so its sums are one block.
+//
+// Layout in HBM: points of all instances concatenated (sum N x d FP32, C
+// order), instance i at row xoff[i]; centres instances x k x d; labels and
+// FP64 squared norms per point.  Work items of the assign phase are (instance,
+// first row) tiles of P points (P chosen by the autotuner); the update phase
+// takes one instance per workgroup, largest first.
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#include <algorithm>
+#include <chrono>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "km_common.hpp"
+
+namespace km {
+
+#define B_MAXK 1024
+#define B_MAXD 64
+
+struct Batch {
+  int nb = 0, d = 0, k = 0;
+  kmeansOptions opt{};
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::vector<long> n_h, xoff_h;
+  long total = 0;
+  // device
+  const float *X = nullptr;
+  float *X_owned = nullptr;
+  long *xoff = nullptr;
+  int *nvec = nullptr;
+  float *C = nullptr, *nc = nullptr;
+  int *labels = nullptr, *perm = nullptr;
+  double *q = nullptr, *part = nullptr, *comp = nullptr;
+  long *choff = nullptr;     // first chunk slot of each instance
+  long slots = 0;            // chunk slots in total
+  int4 *desc = nullptr;      // per slot: instance base row, member range
+  int *gcnt = nullptr, *goff = nullptr, *gcpre = nullptr;  // per (instance, cluster)
+  int *empty = nullptr;
+  int *order = nullptr;  // instances, largest first
+  int2 *items = nullptr;
+  int nitems = 0, P = 0;
+  int *counters = nullptr;  // assign, update
+  int comp_slots = 0;
+  bool data_set = false, centers_set = false;
+  // graph of the last run length
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_iters = 0;
+  int iters = 0;
+  kmeansBatchStats stats{};
+};
+
+static inline unsigned bg1(long n) { return (unsigned)((n + 255) / 256); }
+
+// ------------------------------------------------------------------ kernels
+__global__ void k_batch_fill(int *p, int n, int v) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = v;
+}
+
+// Assign, persistent: each workgroup takes (instance, first row) items from
+// a counter until none are left; centres of the current instance are staged
+// in LDS ([k][d], float4 when d % 4 == 0) with their -0.5*||c||^2.  Each
+// thread scores two points against every centre (canonical chains).
+template <int DT>
+__global__ __launch_bounds__(256) void k_batch_assign_f32(
+    int nitems, const int2 *__restrict__ items, int P, const long *__restrict__ xoff,
+    const int *__restrict__ nvec, int d, int k, const float *__restrict__ X,
+    const float *__restrict__ C, const float *__restrict__ nc, int *__restrict__ labels,
+    int *__restrict__ counter) {
+  extern __shared__ float sm[];  // k*DT centres, then k biases
+  float *Cs = sm, *bs = sm + (size_t)k * DT;
+  __shared__ int item_s;
+  int cur = -1;
+  for (;;) {
+    if (threadIdx.x == 0) item_s = atomicAdd(counter, 1);
+    __syncthreads();
+    const int it = item_s;
+    if (it >= nitems) break;
+    const int2 w = items[it];
+    const int inst = w.x;
+    if (inst != cur) {
+      const float *Ci = C + (size_t)inst * k * d;
+      for (int e = threadIdx.x; e < k * DT; e += blockDim.x) {
+        const int j = e / DT, q = e - j * DT;
+        Cs[e] = q < d ? Ci[(size_t)j * d + q] : 0.0f;
+      }
+      for (int j = threadIdx.x; j < k; j += blockDim.x) bs[j] = -0.5f * nc[(size_t)inst * k + j];
+      cur = inst;
+    }
+    __syncthreads();
+    const long n = nvec[inst];
+    const long rows = min((long)P, n - w.y);
+    const long base = xoff[inst] + w.y;
+    for (long p0 = 2 * threadIdx.x; p0 < rows; p0 += 2 * blockDim.x) {
+      const bool two = p0 + 1 < rows;
+      float x0[DT], x1[DT];
+#pragma unroll
+      for (int q = 0; q < DT; ++q) {
+        x0[q] = q < d ? X[(base + p0) * d + q] : 0.0f;
+        x1[q] = (q < d && two) ? X[(base + p0 + 1) * d + q] : 0.0f;
+      }
+      float b0 = -INFINITY, b1 = -INFINITY;
+      int i0 = 0, i1 = 0;
+      for (int j = 0; j < k; ++j) {
+        float s0 = bs[j], s1 = s0;
+        const float *c = Cs + (size_t)j * DT;
+#pragma unroll
+        for (int q = 0; q < DT; ++q) {
+          if (q < d) {
+            const float cv = c[q];
+            s0 = __builtin_fmaf(x0[q], cv, s0);
+            s1 = __builtin_fmaf(x1[q], cv, s1);
+          }
+        }
+        if (s0 > b0) { b0 = s0; i0 = j; }
+        if (s1 > b1) { b1 = s1; i1 = j; }
+      }
+      labels[base + p0] = i0;
+      if (two) labels[base + p0 + 1] = i1;
+    }
+    __syncthreads();  // item_s and the staged centres are reused
+  }
+}
+
+// Assign for d == DT (a multiple of 4): centres in LDS as float4 rows,
+// PPT points per thread (rows tid + 256*u of the item), every score one fma
+// chain in ascending d.
+template <int DT, int PPT>
+__global__ __launch_bounds__(256) void k_batch_assign_f32x(
+    int nitems, const int2 *__restrict__ items, int P, const long *__restrict__ xoff,
+    const int *__restrict__ nvec, int k, const float *__restrict__ X,
+    const float *__restrict__ C, const float *__restrict__ nc, int *__restrict__ labels,
+    int *__restrict__ counter) {
+  constexpr int V = DT / 4;
+  extern __shared__ float4 sm4[];  // k*V centre float4s, then k biases
+  float4 *Cs = sm4;
+  float *bs = reinterpret_cast<float *>(sm4 + (size_t)k * V);
+  __shared__ int item_s;
+  int cur = -1;
+  for (;;) {
+    if (threadIdx.x == 0) item_s = atomicAdd(counter, 1);
+    __syncthreads();
+    const int it = item_s;
+    if (it >= nitems) break;
+    const int2 w = items[it];
+    const int inst = w.x;
+    if (inst != cur) {
+      const float4 *Ci = reinterpret_cast<const float4 *>(C + (size_t)inst * k * DT);
+      for (int e = threadIdx.x; e < k * V; e += blockDim.x) Cs[e] = Ci[e];
+      for (int j = threadIdx.x; j < k; j += blockDim.x) bs[j] = -0.5f * nc[(size_t)inst * k + j];
+      cur = inst;
+    }
+    __syncthreads();
+    const long rows = min((long)P, (long)nvec[inst] - w.y);
+    const long base = xoff[inst] + w.y;
+    for (long r0 = 0; r0 < rows; r0 += 256 * PPT) {
+      float4 x[PPT][V];
+      bool ok[PPT];
+#pragma unroll
+      for (int u = 0; u < PPT; ++u) {
+        const long p = r0 + threadIdx.x + 256 * u;
+        ok[u] = p < rows;
+        const float4 *xr = reinterpret_cast<const float4 *>(X + (base + (ok[u] ? p : 0)) * DT);
+#pragma unroll
+        for (int v = 0; v < V; ++v) x[u][v] = xr[v];
+      }
+      float bv[PPT];
+      int bi[PPT];
+#pragma unroll
+      for (int u = 0; u < PPT; ++u) { bv[u] = -INFINITY; bi[u] = 0; }
+      for (int j = 0; j < k; ++j) {
+        float sc[PPT];
+        const float b0 = bs[j];
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) sc[u] = b0;
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          const float4 c = Cs[(size_t)j * V + v];
+#pragma unroll
+          for (int u = 0; u < PPT; ++u) {
+            sc[u] = __builtin_fmaf(x[u][v].x, c.x, sc[u]);
+            sc[u] = __builtin_fmaf(x[u][v].y, c.y, sc[u]);
+            sc[u] = __builtin_fmaf(x[u][v].z, c.z, sc[u]);
+            sc[u] = __builtin_fmaf(x[u][v].w, c.w, sc[u]);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < PPT; ++u)
+          if (sc[u] > bv[u]) { bv[u] = sc[u]; bi[u] = j; }
+      }
+#pragma unroll
+      for (int u = 0; u < PPT; ++u)
+        if (ok[u]) labels[base + r0 + threadIdx.x + 256 * u] = bi[u];
+    }
+    __syncthreads();
+  }
+}
+
+// Left-to-right sum of m FP64 values at stride st, in chunks of KM_CHUNK
+// whose sums are again added left to right (m <= KM_CHUNK^2).
+__device__ inline double csum2(const double *v, int m, int st) {
+  if (m <= KM_CHUNK) {
+    double s = 0.0;
+    for (int i = 0; i < m; ++i) s += v[(long)i * st];
+    return s;
+  }
+  double tot = 0.0;
+  for (int a = 0; a < m; a += KM_CHUNK) {
+    double s = 0.0;
+    const int e = min(m, a + KM_CHUNK);
+    for (int i = a; i < e; ++i) s += v[(long)i * st];
+    tot += s;
+  }
+  return tot;
+}
+
+// Exclusive scan of v[0..n) in LDS by the whole block (n <= 1024); tmp
+// holds blockDim.x + 1 ints.  Returns the total.
+__device__ int block_scan_any(int *v, int n, int *tmp) {
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int per = (n + nt - 1) / nt;
+  int s = 0;
+  for (int i = t * per; i < min(n, (t + 1) * per); ++i) s += v[i];
+  tmp[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    int a = 0;
+    for (int i = 0; i < nt; ++i) {
+      const int x = tmp[i];
+      tmp[i] = a;
+      a += x;
+    }
+    tmp[nt] = a;
+  }
+  __syncthreads();
+  int a = tmp[t];
+  for (int i = t * per; i < min(n, (t + 1) * per); ++i) {
+    const int x = v[i];
+    v[i] = a;
+    a += x;
+  }
+  __syncthreads();
+  const int total = tmp[nt];
+  __syncthreads();
+  return total;
+}
+
+// Update 1/3, persistent over instances (largest first), one 16-wave
+// workgroup per instance: stable counting sort of the instance's points by
+// label.  Wave w owns a contiguous segment of the points; per (wave, label)
+// counts give every wave its own cursors, so each wave scatters its segment
+// in order with no block barrier (ranks among equal labels from a match
+// over the label bits).  Writes perm (instance-local indices, label-major),
+// counts, offsets and the chunk prefix of every cluster.
+__global__ __launch_bounds__(1024) void k_batch_sort(
+    int nb, const int *__restrict__ order, const long *__restrict__ xoff,
+    const int *__restrict__ nvec, int k, const int *__restrict__ labels, int *__restrict__ perm,
+    int *__restrict__ gcnt, int *__restrict__ goff, int *__restrict__ gcpre,
+    const long *__restrict__ choff, int4 *__restrict__ desc, int *__restrict__ counter) {
+  extern __shared__ int sh[];  // cur[16][k], offs[k], cpre[k]
+  int *cur = sh, *offs = sh + 16 * k, *cpre = offs + k;
+  __shared__ int tmp[1025];
+  __shared__ int item_s;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  for (;;) {
+    if (t == 0) item_s = atomicAdd(counter, 1);
+    __syncthreads();
+    const int it = item_s;
+    if (it >= nb) break;
+    const int inst = order[it];
+    const long n = nvec[inst], base = xoff[inst];
+    const int *lab = labels + base;
+    for (int e = t; e < 16 * k; e += blockDim.x) cur[e] = 0;
+    __syncthreads();
+    const long S = ((n + 15) / 16 + 63) / 64 * 64;
+    const long a = min(n, (long)w * S), e = min(n, a + S);
+    for (long p = a + lane; p < e; p += 64) atomicAdd(&cur[w * k + lab[p]], 1);
+    __syncthreads();
+    for (int j = t; j < k; j += blockDim.x) {
+      int c = 0;
+      for (int ww = 0; ww < 16; ++ww) c += cur[ww * k + j];
+      offs[j] = c;
+      cpre[j] = (c + KM_CHUNK - 1) / KM_CHUNK;
+      gcnt[(size_t)inst * k + j] = c;
+    }
+    __syncthreads();
+    block_scan_any(offs, k, tmp);
+    const int nch = block_scan_any(cpre, k, tmp);
+    for (int j = t; j < k; j += blockDim.x) {
+      int c = offs[j];
+      for (int ww = 0; ww < 16; ++ww) {
+        const int h = cur[ww * k + j];
+        cur[ww * k + j] = c;
+        c += h;
+      }
+      goff[(size_t)inst * k + j] = offs[j];
+      gcpre[(size_t)inst * (k + 1) + j] = cpre[j];
+    }
+    if (t == 0) gcpre[(size_t)inst * (k + 1) + k] = nch;
+    __syncthreads();
+    // chunk descriptors (instance base, first and end member position); the
+    // instance's unused slots get empty ranges
+    {
+      int4 *dd = desc + choff[inst];
+      const int cap = (int)(choff[inst + 1] - choff[inst]);
+      for (int j = t; j < k; j += blockDim.x) {
+        const int o = offs[j], nj = gcnt[(size_t)inst * k + j];
+        for (int c = cpre[j], a0 = o; a0 < o + nj; ++c, a0 += KM_CHUNK)
+          dd[c] = make_int4((int)base, a0, min(o + nj, a0 + KM_CHUNK), 0);
+      }
+      for (int c = nch + t; c < cap; c += blockDim.x) dd[c] = make_int4(0, 0, 0, 0);
+    }
+    int *cw = cur + w * k;
+    for (long p0 = a; p0 < e; p0 += 64) {
+      const long p = p0 + lane;
+      const bool valid = p < e;
+      const int L = valid ? lab[p] : 0;
+      uint64_t m = __ballot(valid);
+      for (int bt = 0; bt < 10; ++bt) {
+        const uint64_t bb = __ballot(valid && ((L >> bt) & 1));
+        m &= ((L >> bt) & 1) ? bb : ~bb;
+      }
+      if (!valid) m = 0;
+      const int rank = __popcll(m & below);
+      int pos = 0;
+      if (valid) pos = cw[L] + rank;
+      __builtin_amdgcn_wave_barrier();
+      if (valid) perm[base + pos] = (int)p;
+      if (valid && rank == 0) cw[L] = pos + __popcll(m);
+      __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+  }
+}
+
+// Update 2/3: one thread per chunk slot of any instance: the FP64 sums of a
+// chunk of <= KM_CHUNK members (ascending point order) for every dimension
+// and the squared norms.
+template <int DT>
+__global__ __launch_bounds__(256) void k_batch_chunks(
+    int nb, long slots, const long *__restrict__ choff, const long *__restrict__ xoff, int d,
+    int k, const float *__restrict__ X, const double *__restrict__ q,
+    const int *__restrict__ perm, const int *__restrict__ gcnt, const int *__restrict__ goff,
+    const int *__restrict__ gcpre, double *__restrict__ part) {
+  const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= slots) return;
+  int lo = 0, hi = nb - 1;  // last instance with choff <= s
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (choff[mid] <= s) lo = mid; else hi = mid - 1;
+  }
+  const int inst = lo;
+  const int t = (int)(s - choff[inst]);
+  const int *cp = gcpre + (size_t)inst * (k + 1);
+  if (t >= cp[k]) return;
+  int jl = 0, jh = k - 1;  // last cluster with cp[j] <= t
+  while (jl < jh) {
+    const int mid = (jl + jh + 1) >> 1;
+    if (cp[mid] <= t) jl = mid; else jh = mid - 1;
+  }
+  const int j = jl, c = t - cp[j];
+  const int o = goff[(size_t)inst * k + j];
+  const int a = o + c * KM_CHUNK, e = min(o + gcnt[(size_t)inst * k + j], a + KM_CHUNK);
+  const long base = xoff[inst];
+  double acc[DT + 1];
+#pragma unroll
+  for (int qq = 0; qq <= DT; ++qq) acc[qq] = 0.0;
+  for (int r = a; r < e; ++r) {
+    const long idx = base + perm[base + r];
+    const float *x = X + idx * d;
+#pragma unroll
+    for (int qq = 0; qq < DT; ++qq)
+      if (qq < d) acc[qq] += (double)x[qq];
+    acc[DT] += q[idx];
+  }
+  double *out = part + s * (d + 1);
+#pragma unroll
+  for (int qq = 0; qq < DT; ++qq)
+    if (qq < d) out[qq] = acc[qq];
+  out[d] = acc[DT];
+}
+
+// Update 2/3 for d < 64: one wave per chunk slot.  The 64 lanes gather 64
+// members' rows (and squared norms) into LDS at once; then lane q < d adds
+// dimension q and lane d the squared norms, member by member in ascending
+// order, so the sums are the same sequential FP64 sums.
+template <int DT>
+__global__ __launch_bounds__(256) void k_batch_chunks_wave(
+    long slots, const int4 *__restrict__ desc, int d, const float *__restrict__ X,
+    const double *__restrict__ q, const int *__restrict__ perm, double *__restrict__ part) {
+  __shared__ float xs[4][64][DT + 1];
+  __shared__ double qs[4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const long s = (long)blockIdx.x * 4 + w;
+  if (s >= slots) return;
+  const int4 dsc = desc[s];
+  const long base = dsc.x;
+  const int a = dsc.y, e = dsc.z;
+  if (a >= e) return;
+  double acc = 0.0;
+  for (int r0 = a; r0 < e; r0 += 64) {
+    const int r = r0 + lane;
+    if (r < e) {
+      const long idx = base + perm[base + r];
+      if (d == DT && (DT & 3) == 0) {
+        const float4 *x4 = reinterpret_cast<const float4 *>(X + idx * DT);
+#pragma unroll
+        for (int v = 0; v < DT / 4; ++v) {
+          const float4 xv = x4[v];
+          xs[w][lane][4 * v] = xv.x;
+          xs[w][lane][4 * v + 1] = xv.y;
+          xs[w][lane][4 * v + 2] = xv.z;
+          xs[w][lane][4 * v + 3] = xv.w;
+        }
+      } else {
+        const float *x = X + idx * d;
+#pragma unroll
+        for (int qq = 0; qq < DT; ++qq)
+          if (qq < d) xs[w][lane][qq] = x[qq];
+      }
+      qs[w][lane] = q[idx];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int m = min(64, e - r0);
+    if (lane < d) {
+      for (int i = 0; i < m; ++i) acc += (double)xs[w][i][lane];
+    } else if (lane == d) {
+      for (int i = 0; i < m; ++i) acc += qs[w][i];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  if (lane <= d) part[s * (d + 1) + lane] = acc;
+}
+
+// Update 3/3: one workgroup per instance: canonical combination of the chunk
+// sums, new centres (empty clusters keep theirs), ||c||^2 chains,
+// compactness from the moments against the new centres.
+__global__ __launch_bounds__(256) void k_batch_finalize(
+    int d, int k, const long *__restrict__ choff, const int *__restrict__ gcnt,
+    const int *__restrict__ gcpre, const double *__restrict__ part, float *__restrict__ C,
+    float *__restrict__ nc, double *__restrict__ comp_out, int *__restrict__ empty_out) {
+  __shared__ double cj[B_MAXK];
+  __shared__ int nempty;
+  const int inst = blockIdx.x, t = threadIdx.x;
+  if (t == 0) nempty = 0;
+  __syncthreads();
+  const double *pi = part + choff[inst] * (d + 1);
+  const int *cp = gcpre + (size_t)inst * (k + 1);
+  float *Ci = C + (size_t)inst * k * d;
+  for (int j = t; j < k; j += blockDim.x) {
+    const int nj = gcnt[(size_t)inst * k + j], c0 = cp[j], ncj = cp[j + 1] - cp[j];
+    const double *pj = pi + (long)c0 * (d + 1);
+    double a = 0.0, b = 0.0;
+    float s2 = 0.0f;
+    for (int qq = 0; qq < d; ++qq) {
+      const double S = csum2(pj + qq, ncj, d + 1);
+      float cv = Ci[(size_t)j * d + qq];
+      if (nj > 0) cv = (float)(S / (double)nj);
+      Ci[(size_t)j * d + qq] = cv;
+      s2 = __builtin_fmaf(cv, cv, s2);
+      const double cd = (double)cv;
+      a += cd * S;
+      b += cd * cd;
+    }
+    const double Q = csum2(pj + d, ncj, d + 1);
+    nc[(size_t)inst * k + j] = s2;
+    cj[j] = (Q - 2.0 * a) + (double)nj * b;
+    if (nj == 0) atomicAdd(&nempty, 1);
+  }
+  __syncthreads();
+  if (t == 0) {
+    comp_out[inst] = csum2(cj, k, 1);
+    empty_out[inst] = nempty;
+  }
+}
+
+// ||c||^2 chains of every instance's centres.
+__global__ void k_batch_norms(long kk, int d, const float *__restrict__ C, float *__restrict__ nc) {
+  const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= kk) return;
+  float s = 0.0f;
+  for (int q = 0; q < d; ++q) s = __builtin_fmaf(C[j * d + q], C[j * d + q], s);
+  nc[j] = s;
+}
+
+// First k points of each instance as its centres.
+__global__ void k_batch_first_k(int nb, int k, int d, const long *__restrict__ xoff,
+                                const float *__restrict__ X, float *__restrict__ C) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)nb * k * d) return;
+  const long inst = e / ((long)k * d), r = e - inst * k * d;
+  C[e] = X[xoff[inst] * d + r];
+}
+
+// ------------------------------------------------------------------ host
+#define BCHK(x, err)                                                              \
+  do {                                                                            \
+    hipError_t _e = (x);                                                          \
+    if (_e != hipSuccess) {                                                       \
+      fprintf(stderr, "kmeans_amd: %s failed at %s:%d: %s\n", #x, __FILE__, __LINE__, \
+              hipGetErrorString(_e));                                             \
+      return err;                                                                 \
+    }                                                                             \
+  } while (0)
+
+static int pick_dt(int d) {
+  const int dts[] = {4, 8, 16, 32, 64};
+  for (int v : dts)
+    if (d <= v) return v;
+  return 0;
+}
+
+template <int DT>
+static hipError_t launch_assign_dt(Batch *b, int P, int nitems, const int2 *items) {
+  const size_t lds = sizeof(float) * ((size_t)b->k * DT + b->k);
+  static size_t attr = 64 * 1024;
+  if (lds > attr) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_batch_assign_f32<DT>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    attr = lds;
+  }
+  const int grid = num_cus() * 4;
+  hipLaunchKernelGGL(k_batch_assign_f32<DT>, dim3(grid), dim3(256), lds, b->stream, nitems, items,
+                     P, b->xoff, b->nvec, b->d, b->k, b->X, b->C, b->nc, b->labels, b->counters);
+  return hipGetLastError();
+}
+
+template <int DT>
+static hipError_t launch_assign_x(Batch *b, int P, int nitems, const int2 *items) {
+  const size_t lds = sizeof(float) * ((size_t)b->k * DT + b->k);
+  static size_t attr = 64 * 1024;
+  if (lds > attr) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_batch_assign_f32x<DT, 4>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    attr = lds;
+  }
+  const int grid = num_cus() * 4;
+  hipLaunchKernelGGL((k_batch_assign_f32x<DT, 4>), dim3(grid), dim3(256), lds, b->stream, nitems,
+                     items, P, b->xoff, b->nvec, b->k, b->X, b->C, b->nc, b->labels, b->counters);
+  return hipGetLastError();
+}
+
+static hipError_t launch_assign(Batch *b, int P, int nitems, const int2 *items) {
+  if (b->d == 16) return launch_assign_x<16>(b, P, nitems, items);
+  if (b->d == 8) return launch_assign_x<8>(b, P, nitems, items);
+  if (b->d == 32) return launch_assign_x<32>(b, P, nitems, items);
+  switch (pick_dt(b->d)) {
+    case 4: return launch_assign_dt<4>(b, P, nitems, items);
+    case 8: return launch_assign_dt<8>(b, P, nitems, items);
+    case 16: return launch_assign_dt<16>(b, P, nitems, items);
+    case 32: return launch_assign_dt<32>(b, P, nitems, items);
+    default: return launch_assign_dt<64>(b, P, nitems, items);
+  }
+}
+
+static hipError_t launch_update(Batch *b, int slot) {
+  double *co = b->comp + (size_t)slot * b->nb;
+  int *eo = b->empty + (size_t)slot * b->nb;
+  const size_t lds = sizeof(int) * 18 * (size_t)b->k;
+  static size_t attr = 64 * 1024;
+  if (lds > attr) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_batch_sort,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    attr = lds;
+  }
+  hipLaunchKernelGGL(k_batch_sort, dim3(std::min(b->nb, num_cus() * 2)), dim3(1024), lds, b->stream,
+                     b->nb, b->order, b->xoff, b->nvec, b->k, b->labels, b->perm, b->gcnt, b->goff,
+                     b->gcpre, b->choff, b->desc, b->counters + 1);
+#define CHK(DT)                                                                                  \
+  hipLaunchKernelGGL(k_batch_chunks<DT>, dim3(bg1(b->slots)), dim3(256), 0, b->stream, b->nb,    \
+                     b->slots, b->choff, b->xoff, b->d, b->k, b->X, b->q, b->perm, b->gcnt,      \
+                     b->goff, b->gcpre, b->part)
+#define CHW(DT)                                                                                  \
+  hipLaunchKernelGGL(k_batch_chunks_wave<DT>, dim3((unsigned)((b->slots + 3) / 4)), dim3(256), 0,  \
+                     b->stream, b->slots, b->desc, b->d, b->X, b->q, b->perm, b->part)
+  switch (pick_dt(b->d)) {
+    case 4: CHW(4); break;
+    case 8: CHW(8); break;
+    case 16: CHW(16); break;
+    case 32: CHW(32); break;
+    default:
+      if (b->d < 64) CHW(64); else CHK(64);
+      break;
+  }
+#undef CHK
+#undef CHW
+  hipLaunchKernelGGL(k_batch_finalize, dim3(b->nb), dim3(256), 0, b->stream, b->d, b->k, b->choff,
+                     b->gcnt, b->gcpre, b->part, b->C, b->nc, co, eo);
+  return hipGetLastError();
+}
+
+static void build_items(const Batch *b, int P, std::vector<int2> &items) {
+  items.clear();
+  // largest instances first so the tail of the queue is short items
+  std::vector<int> ord(b->nb);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int c) { return b->n_h[a] > b->n_h[c]; });
+  for (int i : ord)
+    for (long s = 0; s < b->n_h[i]; s += P) items.push_back(make_int2(i, (int)s));
+}
+
+static hipError_t upload_items(Batch *b, int P) {
+  std::vector<int2> items;
+  build_items(b, P, items);
+  hipFree(b->items);
+  b->items = nullptr;
+  hipError_t e = hipMalloc(&b->items, sizeof(int2) * std::max<size_t>(items.size(), 1));
+  if (e != hipSuccess) return e;
+  e = hipMemcpy(b->items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice);
+  b->nitems = (int)items.size();
+  b->P = P;
+  return e;
+}
+
+// ---- autotuner: points per work item of the assign phase, timed on the
+// caller's data and cached per shape (in process and in a text file named by
+// KMEANS_AMD_TUNE_CACHE, default $HOME/.cache/kmeans_amd_tune.txt).
+static std::mutex g_tune_mu;
+static std::map<std::string, int> g_tune;
+static bool g_tune_loaded = false;
+
+static std::string tune_path() {
+  const char *p = getenv("KMEANS_AMD_TUNE_CACHE");
+  if (p && *p) return p;
+  const char *h = getenv("HOME");
+  return std::string(h && *h ? h : "/tmp") + "/.cache/kmeans_amd_tune.txt";
+}
+
+static std::string tune_key(const Batch *b) {
+  int lg_n = 0, lg_b = 0;
+  while ((1L << lg_n) < b->total) ++lg_n;
+  while ((1 << lg_b) < b->nb) ++lg_b;
+  std::ostringstream s;
+  s << "batch_assign_f32 v2 d=" << b->d << " k=" << b->k << " lgN=" << lg_n << " lgB=" << lg_b;
+  return s.str();
+}
+
+static int autotune(Batch *b) {
+  const std::string key = tune_key(b);
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    if (!g_tune_loaded) {
+      std::ifstream f(tune_path());
+      std::string line;
+      while (std::getline(f, line)) {
+        const size_t eq = line.rfind('=');
+        const size_t sp = line.rfind(' ');
+        if (sp == std::string::npos || eq == std::string::npos || eq < sp) continue;
+        g_tune[line.substr(0, sp)] = atoi(line.c_str() + eq + 1);
+      }
+      g_tune_loaded = true;
+    }
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      b->stats.tuned_from_cache = 1;
+      return it->second;
+    }
+  }
+  const int cands[] = {256, 512, 1024, 2048, 4096};
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  int best = 1024;
+  float best_ms = 1e30f;
+  for (int ci = 0; ci < 5; ++ci) {
+    const int P = cands[ci];
+    if (upload_items(b, P) != hipSuccess) continue;
+    float tot = 0.0f;
+    for (int rep = 0; rep < 3; ++rep) {
+      hipLaunchKernelGGL(k_batch_fill, dim3(1), dim3(64), 0, b->stream, b->counters, 2, 0);
+      hipEventRecord(e0, b->stream);
+      launch_assign(b, P, b->nitems, b->items);
+      hipEventRecord(e1, b->stream);
+      hipEventSynchronize(e1);
+      float ms = 0.0f;
+      hipEventElapsedTime(&ms, e0, e1);
+      if (rep) tot += ms;  // first run warms up
+    }
+    b->stats.tune_ms[ci] = tot / 2.0f;
+    if (tot < best_ms) { best_ms = tot; best = P; }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune[key] = best;
+    const std::string path = tune_path();
+    const size_t sl = path.rfind('/');
+    if (sl != std::string::npos) {
+      std::string dir = path.substr(0, sl);
+      std::string cmd = "mkdir -p '" + dir + "' 2>/dev/null";
+      (void)system(cmd.c_str());
+    }
+    std::ofstream f(path, std::ios::app);
+    if (f) f << key << " P=" << best << "\n";
+  }
+  return best;
+}
+
+static void batch_free_graph(Batch *b) {
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->graph_iters = 0;
+}
+
+static void batch_free(Batch *b) {
+  if (b->stream) hipStreamSynchronize(b->stream);
+  batch_free_graph(b);
+  void *bufs[] = {b->X_owned, b->xoff, b->nvec, b->C, b->nc, b->labels, b->perm, b->q,
+                  b->part, b->comp, b->choff, b->empty, b->order, b->items, b->counters,
+                  b->gcnt, b->goff, b->gcpre, b->desc};
+  for (void *p : bufs) hipFree(p);
+  if (b->own_stream && b->stream) hipStreamDestroy(b->stream);
+}
+
+static kmeansErrorStatus batch_create(Batch *b, int nb, const long *n, int d, int k,
+                                      const kmeansOptions *o) {
+  if (nb <= 0 || !n || d <= 0 || d > B_MAXD || k <= 0 || k > B_MAXK)
+    return KMEANS_ERROR_INVALID_ARGUMENT;
+  if (o) b->opt = *o;
+  if (b->opt.precision != KM_PREC_FP32 || b->opt.proposer != KM_PROPOSER_NONE || b->opt.tile_rows)
+    return KMEANS_ERROR_INVALID_ARGUMENT;
+  const int dt = pick_dt(d);
+  if ((size_t)k * (dt + 1) * sizeof(float) > 64 * 1024) return KMEANS_ERROR_INVALID_ARGUMENT;
+  b->nb = nb;
+  b->d = d;
+  b->k = k;
+  b->n_h.assign(n, n + nb);
+  b->xoff_h.resize(nb);
+  long tot = 0, tch = 0;
+  std::vector<long> choff(nb + 1);
+  for (int i = 0; i < nb; ++i) {
+    if (n[i] < k || n[i] >= KM_BLOCK) return KMEANS_ERROR_INVALID_ARGUMENT;
+    b->xoff_h[i] = tot;
+    tot += n[i];
+    choff[i] = tch;
+    tch += (n[i] + KM_CHUNK - 1) / KM_CHUNK + k;
+  }
+  choff[nb] = tch;
+  if (tot > 0x7fffffffL) return KMEANS_ERROR_INVALID_ARGUMENT;
+  b->total = tot;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    fprintf(stderr, "kmeans_amd: no HIP device available\n");
+    return KMEANS_ERROR_BUILDDATA;
+  }
+  BCHK(hipSetDevice(b->opt.device), KMEANS_ERROR_BUILDDATA);
+  if (b->opt.stream) {
+    b->stream = (hipStream_t)b->opt.stream;
+  } else {
+    BCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), KMEANS_ERROR_BUILDDATA);
+    b->own_stream = true;
+  }
+  std::vector<int> nv(nb), ord(nb);
+  for (int i = 0; i < nb; ++i) nv[i] = (int)n[i];
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int c) { return n[a] > n[c]; });
+  BCHK(hipMalloc(&b->xoff, sizeof(long) * nb), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMalloc(&b->nvec, sizeof(int) * nb), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMalloc(&b->order, sizeof(int) * nb), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMalloc(&b->choff, sizeof(long) * (nb + 1)), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMalloc(&b->desc, sizeof(int4) * (size_t)tch), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMemcpy(b->xoff, b->xoff_h.data(), sizeof(long) * nb, hipMemcpyHostToDevice), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMemcpy(b->nvec, nv.data(), sizeof(int) * nb, hipMemcpyHostToDevice), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMemcpy(b->order, ord.data(), sizeof(int) * nb, hipMemcpyHostToDevice), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMemcpy(b->choff, choff.data(), sizeof(long) * (nb + 1), hipMemcpyHostToDevice), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMalloc(&b->C, sizeof(float) * (size_t)nb * k * d), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMalloc(&b->nc, sizeof(float) * (size_t)nb * k), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMalloc(&b->labels, sizeof(int) * (size_t)tot), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMemsetAsync(b->labels, 0, sizeof(int) * (size_t)tot, b->stream), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMalloc(&b->perm, sizeof(int) * (size_t)tot), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMalloc(&b->q, sizeof(double) * (size_t)tot), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMalloc(&b->part, sizeof(double) * (size_t)tch * (d + 1)), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMalloc(&b->gcnt, sizeof(int) * (size_t)nb * k), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMalloc(&b->goff, sizeof(int) * (size_t)nb * k), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMalloc(&b->gcpre, sizeof(int) * (size_t)nb * (k + 1)), KMEANS_ERROR_BUILDDATA);
+  b->slots = tch;
+  BCHK(hipMalloc(&b->counters, sizeof(int) * 2), KMEANS_ERROR_BUILDDATA);
+  return KMEANS_SUCCESS;
+}
+
+static kmeansErrorStatus ensure_slots(Batch *b, int iters) {
+  if (iters <= b->comp_slots) return KMEANS_SUCCESS;
+  hipFree(b->comp);
+  hipFree(b->empty);
+  b->comp = nullptr;
+  b->empty = nullptr;
+  b->comp_slots = 0;
+  BCHK(hipMalloc(&b->comp, sizeof(double) * (size_t)iters * b->nb), KMEANS_ERROR_COMPUTE);
+  BCHK(hipMalloc(&b->empty, sizeof(int) * (size_t)iters * b->nb), KMEANS_ERROR_COMPUTE);
+  b->comp_slots = iters;
+  batch_free_graph(b);  // slot addresses are baked into the graph
+  return KMEANS_SUCCESS;
+}
+
+static hipError_t enqueue_iterations(Batch *b, int iters) {
+  hipError_t e;
+  for (int it = 0; it < iters; ++it) {
+    hipLaunchKernelGGL(k_batch_fill, dim3(1), dim3(64), 0, b->stream, b->counters, 2, 0);
+    if ((e = launch_assign(b, b->P, b->nitems, b->items)) != hipSuccess) return e;
+    if ((e = launch_update(b, it)) != hipSuccess) return e;
+  }
+  return hipGetLastError();
+}
+
+static kmeansErrorStatus batch_run(Batch *b, int iters) {
+  if (!b->data_set || !b->centers_set) return KMEANS_ERROR_COMPUTE;
+  if (iters <= 0) return KMEANS_SUCCESS;
+  kmeansErrorStatus st;
+  if ((st = ensure_slots(b, iters)) != KMEANS_SUCCESS) return st;
+  if (!b->P) {
+    const int P = autotune(b);
+    BCHK(upload_items(b, P), KMEANS_ERROR_COMPUTE);
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  if (b->graph_iters != iters) {
+    batch_free_graph(b);
+    BCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), KMEANS_ERROR_COMPUTE);
+    hipError_t e = enqueue_iterations(b, iters);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+      if (g) hipGraphDestroy(g);
+      fprintf(stderr, "kmeans_amd: batch graph capture failed: %s\n",
+              hipGetErrorString(e != hipSuccess ? e : e2));
+      return KMEANS_ERROR_COMPUTE;
+    }
+    b->graph = g;
+    BCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), KMEANS_ERROR_COMPUTE);
+    b->graph_iters = iters;
+  }
+  const auto t1 = std::chrono::steady_clock::now();
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventRecord(e0, b->stream);
+  BCHK(hipGraphLaunch(b->exec, b->stream), KMEANS_ERROR_COMPUTE);
+  hipEventRecord(e1, b->stream);
+  BCHK(hipEventSynchronize(e1), KMEANS_ERROR_COMPUTE);
+  float ms = 0.0f;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  std::vector<int> emp((size_t)iters * b->nb);
+  BCHK(hipMemcpy(emp.data(), b->empty, sizeof(int) * emp.size(), hipMemcpyDeviceToHost), KMEANS_ERROR_COMPUTE);
+  long long es = 0;
+  for (int v : emp) es += v;
+  b->iters += iters;
+  b->stats.iterations = b->iters;
+  b->stats.ms_graph = ms;
+  b->stats.ms_capture = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  b->stats.empty_clusters += es;
+  b->stats.tile_rows = b->P;
+  b->stats.work_items = b->nitems;
+  b->stats.graph_nodes = 5 * iters;
+  b->stats.last_run_iters = iters;
+  return KMEANS_SUCCESS;
+}
+
+}  // namespace km
+
+struct kmeansBatchEngine {
+  km::Batch b;
+};
+
+using km::Batch;
+
+extern "C" {
+
+kmeansErrorStatus kmeansBatchCreate(kmeansBatchHandle *h, int instances, const long *n, int d,
+                                    int k, const kmeansOptions *opt) {
+  if (!h) return KMEANS_ERROR_INVALID_ARGUMENT;
+  *h = nullptr;
+  kmeansBatchEngine *p = new (std::nothrow) kmeansBatchEngine();
+  if (!p) return KMEANS_ERROR_BUILDDATA;
+  kmeansErrorStatus st = km::batch_create(&p->b, instances, n, d, k, opt);
+  if (st != KMEANS_SUCCESS) {
+    km::batch_free(&p->b);
+    delete p;
+    return st;
+  }
+  *h = p;
+  return KMEANS_SUCCESS;
+}
+
+void kmeansBatchDestroy(kmeansBatchHandle h) {
+  if (!h) return;
+  km::batch_free(&h->b);
+  delete h;
+}
+
+kmeansErrorStatus kmeansBatchSetData(kmeansBatchHandle h, const float *X, int on_device) {
+  if (!h || !X) return KMEANS_ERROR_INVALID_ARGUMENT;
+  Batch *b = &h->b;
+  const size_t bytes = sizeof(float) * (size_t)b->total * b->d;
+  if (on_device) {
+    b->X = X;
+  } else {
+    if (!b->X_owned) BCHK(hipMalloc(&b->X_owned, bytes), KMEANS_ERROR_BUILDDATA);
+    BCHK(hipMemcpy(b->X_owned, X, bytes, hipMemcpyHostToDevice), KMEANS_ERROR_BUILDDATA);
+    b->X = b->X_owned;
+  }
+  BCHK(km::launch_point_sq(b->stream, b->total, b->d, b->X, b->q), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipStreamSynchronize(b->stream), KMEANS_ERROR_BUILDDATA);
+  b->data_set = true;
+  km::batch_free_graph(b);
+  return KMEANS_SUCCESS;
+}
+
+kmeansErrorStatus kmeansBatchSetCenters(kmeansBatchHandle h, const float *C, int on_device) {
+  if (!h || !C) return KMEANS_ERROR_INVALID_ARGUMENT;
+  Batch *b = &h->b;
+  const size_t nkd = (size_t)b->nb * b->k * b->d;
+  BCHK(hipMemcpyAsync(b->C, C, sizeof(float) * nkd,
+                      on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, b->stream),
+       KMEANS_ERROR_INITIALIZE);
+  hipLaunchKernelGGL(km::k_batch_norms, dim3(km::bg1((long)b->nb * b->k)), dim3(256), 0, b->stream,
+                     (long)b->nb * b->k, b->d, b->C, b->nc);
+  BCHK(hipStreamSynchronize(b->stream), KMEANS_ERROR_INITIALIZE);
+  b->centers_set = true;
+  return KMEANS_SUCCESS;
+}
+
+kmeansErrorStatus kmeansBatchInitFirstK(kmeansBatchHandle h) {
+  if (!h || !h->b.data_set) return KMEANS_ERROR_INITIALIZE;
+  Batch *b = &h->b;
+  const long nkd = (long)b->nb * b->k * b->d;
+  hipLaunchKernelGGL(km::k_batch_first_k, dim3(km::bg1(nkd)), dim3(256), 0, b->stream, b->nb, b->k,
+                     b->d, b->xoff, b->X, b->C);
+  hipLaunchKernelGGL(km::k_batch_norms, dim3(km::bg1((long)b->nb * b->k)), dim3(256), 0, b->stream,
+                     (long)b->nb * b->k, b->d, b->C, b->nc);
+  BCHK(hipStreamSynchronize(b->stream), KMEANS_ERROR_INITIALIZE);
+  b->centers_set = true;
+  return KMEANS_SUCCESS;
+}
+
+kmeansErrorStatus kmeansBatchSetTileRows(kmeansBatchHandle h, int rows) {
+  if (!h || rows < 0 || (rows && rows % 2)) return KMEANS_ERROR_INVALID_ARGUMENT;
+  Batch *b = &h->b;
+  km::batch_free_graph(b);
+  if (rows == 0) {
+    b->P = 0;
+    return KMEANS_SUCCESS;
+  }
+  BCHK(km::upload_items(b, rows), KMEANS_ERROR_INVALID_ARGUMENT);
+  return KMEANS_SUCCESS;
+}
+
+kmeansErrorStatus kmeansBatchRun(kmeansBatchHandle h, int iters) {
+  if (!h || iters < 0) return KMEANS_ERROR_INVALID_ARGUMENT;
+  return km::batch_run(&h->b, iters);
+}
+
+kmeansErrorStatus kmeansBatchGetCenters(kmeansBatchHandle h, float *C) {
+  if (!h || !C) return KMEANS_ERROR_INVALID_ARGUMENT;
+  Batch *b = &h->b;
+  BCHK(hipMemcpy(C, b->C, sizeof(float) * (size_t)b->nb * b->k * b->d, hipMemcpyDeviceToHost),
+       KMEANS_ERROR_GET_CENTERS);
+  return KMEANS_SUCCESS;
+}
+
+kmeansErrorStatus kmeansBatchGetLabels(kmeansBatchHandle h, int *labels) {
+  if (!h || !labels) return KMEANS_ERROR_INVALID_ARGUMENT;
+  Batch *b = &h->b;
+  BCHK(hipMemcpy(labels, b->labels, sizeof(int) * (size_t)b->total, hipMemcpyDeviceToHost),
+       KMEANS_ERROR_GET_CENTERS);
+  return KMEANS_SUCCESS;
+}
+
+kmeansErrorStatus kmeansBatchGetCompactness(kmeansBatchHandle h, double *comp) {
+  if (!h || !comp) return KMEANS_ERROR_INVALID_ARGUMENT;
+  Batch *b = &h->b;
+  const int it = b->stats.last_run_iters;
+  if (it <= 0) return KMEANS_ERROR_COMPACTNESS;
+  BCHK(hipMemcpy(comp, b->comp, sizeof(double) * (size_t)it * b->nb, hipMemcpyDeviceToHost),
+       KMEANS_ERROR_COMPACTNESS);
+  return KMEANS_SUCCESS;
+}
+
+kmeansErrorStatus kmeansBatchGetStats(kmeansBatchHandle h, kmeansBatchStats *s) {
+  if (!h || !s) return KMEANS_ERROR_INVALID_ARGUMENT;
+  *s = h->b.stats;
+  return KMEANS_SUCCESS;
+}
+
+const float *kmeansBatchDeviceCenters(kmeansBatchHandle h) { return h ? h->b.C : nullptr; }
+const int *kmeansBatchDeviceLabels(kmeansBatchHandle h) { return h ? h->b.labels : nullptr; }
+
+}  // extern "C"

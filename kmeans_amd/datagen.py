@@ -99,3 +99,41 @@ def generate_on_gpu(name, n=None, torch=None, i0=0):
     torch.cuda.synchronize()
     spec["init_idx"] = idx
     return X, C0.cpu().numpy(), spec
+
+
+def c4_instances(nb=2048, sigma=1.0, seed=4, median=20000, lo=1000, hi=200000):
+    """Batched workload sizes and seeds: N_i = clip(round(median * exp(sigma * Z_i)), lo, hi)
+    with Z_i standard normal (Box-Muller on the counter-based generator, stream 10);
+    instance seed = generator output (stream 11) masked to 31 bits."""
+    i = np.arange(nb, dtype=np.uint64)
+    u1 = rng_unif(seed, 10, 2 * i)
+    u2 = rng_unif(seed, 10, 2 * i + 1)
+    z = np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+    n = np.clip(np.rint(median * np.exp(sigma * z)), lo, hi).astype(np.int64)
+    seeds = (rng_u64(seed, 11, i) & np.uint64(0x7FFFFFFF)).astype(np.int64)
+    return n, seeds
+
+
+C4 = dict(nb=2048, d=16, k=256, mlo=-4.0, mhi=4.0, sig=0.7, iters=15)
+
+
+def generate_c4_on_gpu(torch, nb=None, sigma=1.0):
+    """All instances of the batched workload concatenated on the GPU: each
+    instance is its own Gaussian mixture (k means U[-4,4]^16, sigma 0.7, equal
+    weights) from its own seed.  Returns (X device tensor, n per instance, seeds)."""
+    import kmeans_amd as km
+    nb = C4["nb"] if nb is None else nb
+    d, k = C4["d"], C4["k"]
+    n, seeds = c4_instances(nb, sigma)
+    X = torch.empty((int(n.sum()), d), dtype=torch.float32, device="cuda")
+    off = 0
+    for i in range(nb):
+        means, sig, cdf = mixture_tables(int(seeds[i]), d, k, C4["mlo"], C4["mhi"], C4["sig"], C4["sig"])
+        dm = torch.from_numpy(means).cuda()
+        ds = torch.from_numpy(sig).cuda()
+        dc = torch.from_numpy(cdf).cuda()
+        km.generate_mixture(int(seeds[i]), 0, int(n[i]), d, k, dm.data_ptr(), ds.data_ptr(),
+                            dc.data_ptr(), X[off:].data_ptr())
+        off += int(n[i])
+    torch.cuda.synchronize()
+    return X, n, seeds

@@ -93,3 +93,26 @@ def test_create_rejects_bad_proposer_options_without_device():
         for f, v in kw.items():
             setattr(opt, f, v)
         assert L.kmeansCreate(ctypes.byref(h), 100_000, 8, 1024, ctypes.byref(opt)) == 14, kw
+
+
+def test_batch_create_rejects_bad_shapes_without_device():
+    L = kmeans_amd.lib()
+    h = ctypes.c_void_p()
+    n = np.array([500, 800, 1200], np.int64)
+    p = n.ctypes.data_as(ctypes.c_void_p)
+    assert L.kmeansBatchCreate(ctypes.byref(h), 3, p, 16, 600, None) == 14   # k > n[0]
+    assert L.kmeansBatchCreate(ctypes.byref(h), 3, p, 65, 16, None) == 14    # d > 64
+    assert L.kmeansBatchCreate(ctypes.byref(h), 3, p, 16, 2000, None) == 14  # k > 1024
+    assert L.kmeansBatchCreate(ctypes.byref(h), 0, p, 16, 16, None) == 14
+    opt = kmeans_amd.Options(1, 0, None, 0, 0)  # reduced precision: not offered
+    assert L.kmeansBatchCreate(ctypes.byref(h), 3, p, 16, 16, ctypes.byref(opt)) == 14
+
+
+def test_c4_instance_sizes():
+    from kmeans_amd import datagen
+    n, seeds = datagen.c4_instances()
+    assert len(n) == 2048 and n.min() >= 1000 and n.max() <= 200000
+    # lognormal with median 20,000 and sigma 1: sum about 2048 * 20000 * exp(1/2)
+    assert 5.5e7 < n.sum() < 7.5e7
+    assert abs(np.median(n) / 20000 - 1) < 0.1
+    assert len(set(seeds.tolist())) == 2048

@@ -1,0 +1,98 @@
+"""Batched independent clusterings against the per-instance oracle.
+
+Every instance of a batch must follow exactly the single-problem canonical
+Lloyd trajectory: labels, centres and per-iteration compactness bit-identical
+to oracle.lloyd on that instance's points alone, whatever the work-item tile
+size, and across replays of the captured graph.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+torch = pytest.importorskip("torch")
+import kmeans_amd as km  # noqa: E402
+from kmeans_amd import datagen  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _instances(nb, d, k, median, lo, hi, seed=7):
+    n, seeds = datagen.c4_instances(nb, 1.0, seed, median, lo, hi)
+    Xs = []
+    for i in range(nb):
+        X, _, _, _ = orc.mixture(int(seeds[i]), int(n[i]), d, k, -4.0, 4.0, 0.7, 0.7)
+        Xs.append(X)
+    return n, Xs
+
+
+def _oracle(Xs, k, iters):
+    return [orc.lloyd(X, X[:k].copy(), iters) for X in Xs]
+
+
+def _check(b, n, Xs, ref, iters_last):
+    lab = b.labels()
+    C = b.centers()
+    comp = b.compactness()
+    off = np.concatenate([[0], np.cumsum(n)])
+    for i, r in enumerate(ref):
+        assert np.array_equal(lab[off[i]:off[i + 1]], r["labels"]), i
+        assert np.array_equal(C[i], r["centers"]), i
+        assert np.array_equal(comp[:, i], r["comp_hist"][-iters_last:]), i
+
+
+@pytest.mark.parametrize("d,k,median,lo,hi", [(16, 32, 3000, 300, 20000), (5, 7, 400, 40, 3000),
+                                              (20, 64, 1500, 200, 9000)])
+def test_batch_equals_per_instance_oracle(d, k, median, lo, hi):
+    nb, iters = 24, 6
+    n, Xs = _instances(nb, d, k, median, lo, hi)
+    ref = _oracle(Xs, k, iters)
+    b = km.KMeansBatch(n, d, k)
+    b.set_data(np.concatenate(Xs))
+    b.init_first_k()
+    b.run(iters)
+    st = b.stats()
+    print(f"\nd={d} k={k}: {st}")
+    assert st["tile_rows"] in (256, 512, 1024, 2048, 4096)
+    assert st["graph_nodes"] == 5 * iters
+    _check(b, n, Xs, ref, iters)
+    b.close()
+
+
+def test_batch_graph_replays_and_tile_sizes():
+    """Three replays of one captured 3-iteration graph equal 9 eager oracle
+    iterations; every explicit tile size gives the same result."""
+    nb, d, k = 16, 16, 32
+    n, Xs = _instances(nb, d, k, 2500, 300, 12000, seed=9)
+    ref = _oracle(Xs, k, 9)
+    for tile in (0, 256, 2048):
+        b = km.KMeansBatch(n, d, k)
+        b.set_data(np.concatenate(Xs))
+        b.init_first_k()
+        if tile:
+            b.set_tile_rows(tile)
+        for rep in range(3):
+            b.run(3)
+            torch.cuda.synchronize()  # host work between replays
+        st = b.stats()
+        assert st["iterations"] == 9 and st["last_run_iters"] == 3
+        if tile:
+            assert st["tile_rows"] == tile
+        _check(b, n, Xs, ref, 3)
+        b.close()
+
+
+def test_batch_explicit_centres_and_device_data():
+    nb, d, k = 8, 16, 16
+    n, Xs = _instances(nb, d, k, 1000, 200, 4000, seed=11)
+    rng = np.random.default_rng(0)
+    C0 = rng.uniform(-4, 4, (nb, k, d)).astype(np.float32)
+    ref = [orc.lloyd(X, C0[i], 4) for i, X in enumerate(Xs)]
+    Xd = torch.from_numpy(np.concatenate(Xs)).cuda()
+    Cd = torch.from_numpy(C0).cuda()
+    b = km.KMeansBatch(n, d, k)
+    b.set_data(device_ptr=Xd.data_ptr())
+    b.set_centers(device_ptr=Cd.data_ptr())
+    b.run(4)
+    _check(b, n, Xs, ref, 4)
+    b.close()
