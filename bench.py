@@ -113,11 +113,13 @@ def main():
                                ("assign", "reresolve", "update", "compactness")},
         "assign_pairs_per_s": pairs / (assign_ms * 1e-3),
         "reresolved_per_step": st["reresolved"] / args.steps,
-        "roofline": {"bound": "mfma" if args.precision != "fp32" else "mfma",
+        "roofline": {"bound": "mfma" if args.precision != "fp32" else "fp32",
                      "achieved": achieved / 1e12, "peak": PEAK[args.precision] / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK[args.precision],
-                     "traffic": None, "kernel": "assign",
-                     "algorithmic": "2*D*N*K flops per launch"},
+                     "traffic": measured_traffic(args.workload, args.precision, args.n),
+                     "kernel": "k_assign_mfma" if args.precision != "fp32" else "k_assign_f32",
+                     "algorithmic": "2*D*N*K flops per launch; achieved over the assign phase's "
+                                    "event-timed duration"},
     }
     labels = eng.labels()
     centers = eng.centers()
@@ -154,6 +156,22 @@ def main():
         parity = (bool(np.array_equal(lab_cpu, e1.labels())) and bool(np.array_equal(C1, e1.centers()))
                   and e1.stats()["compactness"] == comp_cpu)
         e1.close()
+        # error of the reduced-precision run's final centres and compactness
+        # against FP64 means and FP64 compactness of its own final labels
+        Xh = X.cpu().numpy()
+        M64, cnt64 = orc.means_fp64(Xh, labels, k)
+        nz = cnt64 > 0
+        rel_c = float(np.max(np.abs(centers[nz].astype(np.float64) - M64[nz]) /
+                             np.maximum(np.abs(M64[nz]), 1e-30)))
+        comp64 = orc.compactness_fp64(Xh, labels, centers)
+        out["error_vs_fp64"] = {
+            "centers_max_rel": rel_c,
+            "centers_max_abs_over_max_abs": float(np.max(np.abs(centers[nz] - M64[nz])) /
+                                                  np.max(np.abs(M64[nz]))),
+            "compactness_rel": abs(st["compactness"] - comp64) / abs(comp64),
+            "reference": "FP64 means and FP64 compactness (CPU oracle) of the final labels",
+        }
+        del Xh
         out["cpu_baseline"] = {
             "value": s * float(k) / t_cpu, "unit": "pts*ctrs/s", "cores": orc.num_threads(),
             "kind": "port",
@@ -170,6 +188,22 @@ def main():
         torch.cuda.synchronize()
         out["calibration_copy_bytes"] = 1 << 30
     print(json.dumps(out))
+
+
+def measured_traffic(workload, precision, n_override):
+    """HBM bytes per launch of the dominant kernel, from the committed PMC
+    summary (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE runs of this
+    workload, scaled by the calibration copy), or None when not measured."""
+    if n_override is not None:
+        return None
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        tab = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    key = f"{workload}:{precision}:" + ("k_assign_mfma" if precision != "fp32" else "k_assign_f32")
+    ent = tab.get(key)
+    return None if ent is None else ent["bytes_per_launch"]
 
 
 def bench_batch(args, km, datagen, np, torch):

@@ -438,7 +438,8 @@ kmeansErrorStatus Engine::stream_iteration(int slot) {
       KM_HIP_CHECK(mfma_assign_view(this, dXt[b], mfma_tile_xh(this), mfma_tile_xnorm(this),
                                     dlabels + r0, rows),
                    KMEANS_ERROR_CLOSEST_CENTERS);
-      KM_HIP_CHECK(mfma_reresolve_view(this, dXt[b], dlabels + r0, rows),
+      KM_HIP_CHECK(mfma_reresolve_view(this, dXt[b], mfma_tile_xh(this), mfma_tile_xnorm(this),
+                                       dlabels + r0, rows),
                    KMEANS_ERROR_CLOSEST_CENTERS);
     }
     KM_HIP_CHECK(launch_update_accumulate(stream, rows, n, k, dXt[b], dlabels + r0, dqt[b], ws.S,

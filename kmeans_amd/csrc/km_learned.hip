@@ -1008,7 +1008,7 @@ hipError_t learned_fallback(Engine *e) {
                          L->fb, e->n, Dp, e->dX, mfma_tile_xh(e), mfma_tile_xnorm(e), L->fbX,
                          L->fbXh, L->fbN);
       LCHK(mfma_assign_view(e, L->fbX, L->fbXh, L->fbN, L->fbLab, cnt));
-      LCHK(mfma_reresolve_view(e, L->fbX, L->fbLab, cnt));
+      LCHK(mfma_reresolve_view(e, L->fbX, L->fbXh, L->fbN, L->fbLab, cnt));
       hipLaunchKernelGGL(k_scatter_fb, dim3(g1(cnt)), dim3(256), 0, e->stream, (long)cnt, L->fb,
                          L->fbLab, e->dlabels);
     }

@@ -57,13 +57,16 @@ struct MfmaState {
   float scale = 1.f; // power of two applied to X and C before conversion (FP16)
   uint16_t *Xh = nullptr;   // m x Dp
   uint16_t *Ch = nullptr;   // Kp x Dp
+  uint16_t *Cl = nullptr;   // Kp x Dp: BF16 of (C - Ch), the low part (BF16 engines)
   float *bias = nullptr;    // Kp: -0.5*scale^2*nc (or -FLT_MAX past k)
   float *bnd = nullptr;     // [cmax, ncmax]
   float *xnorm = nullptr;   // m: upper bound of |x_i|
   int *flags = nullptr;     // m
-  int4 *cands = nullptr;    // m entries (point, k1, k2, threshold bits)
+  int4 *cands = nullptr;    // unused: certificate 2 runs inside the assign kernel
   int *ncand = nullptr;     // per iteration slot
   int *nflag = nullptr;     // per iteration slot
+  int *flags2 = nullptr;    // m: points left open by the deep pass
+  int *nflag2 = nullptr;    // per iteration slot
   int nflag_slots = 0;
   bool x_ready = false;
   bool c_ready = false;
@@ -87,20 +90,25 @@ __global__ __launch_bounds__(256) void k_prepare_centres(int k, int d, int Dp, i
                                                          const float *__restrict__ C,
                                                          const float *__restrict__ nc, float scale,
                                                          int fp16, uint16_t *__restrict__ Ch,
+                                                         uint16_t *__restrict__ Cl,
                                                          float *__restrict__ bias,
                                                          float *__restrict__ bnd) {
+  // block 0 reduces the bound; every block converts its share of the tiles
   __shared__ float red[256];
-  for (long e = threadIdx.x; e < (long)Kp * Dp; e += blockDim.x) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < (long)Kp * Dp; e += stride) {
     const long j = e / Dp;
     const int q = (int)(e - j * Dp);
     const float v = (j < k && q < d) ? C[j * d + q] * scale : 0.0f;
-    Ch[e] = fp16 ? f32_to_f16_rne(v) : f32_to_bf16_rne(v);
+    const uint16_t hi = fp16 ? f32_to_f16_rne(v) : f32_to_bf16_rne(v);
+    Ch[e] = hi;
+    if (!fp16) Cl[e] = f32_to_bf16_rne(v - __uint_as_float((uint32_t)hi << 16));
   }
-  float mx = 0.0f;
-  for (int j = threadIdx.x; j < Kp; j += blockDim.x) {
+  for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < Kp; j += stride)
     bias[j] = j < k ? -0.5f * nc[j] * (scale * scale) : -FLT_MAX;
-    if (j < k) mx = fmaxf(mx, nc[j]);
-  }
+  if (blockIdx.x != 0) return;
+  float mx = 0.0f;
+  for (int j = threadIdx.x; j < k; j += blockDim.x) mx = fmaxf(mx, nc[j]);
   red[threadIdx.x] = mx;
   __syncthreads();
   for (int off = 128; off > 0; off >>= 1) {
@@ -338,25 +346,55 @@ __global__ __launch_bounds__(256) void k_assign_mfma(
     const bool live = h == 0 && p < M;
     // certificate 1: the best approximate score beats every other by > 2E
     const bool quick = !tie && (a1 - a2 > 2.0f * E + slack) && k1 >= 0 && k1 < K;
-    // certificate 2 (separate pass): the exact FP32 chains of the two
-    // candidates must beat every other centre's upper bound a3 + E; with
-    // k <= 2 there is no other centre
+    // certificate 2: the exact FP32 chains (the plain path's arithmetic) of
+    // the two candidates, computed here by the two lane halves (k1 by half
+    // 0, k2 by half 1); the better one (lowest index on equal scores) must
+    // beat every other centre's upper bound a3 + E; with k <= 2 there is no
+    // other centre
     const bool has_third = m3 > -FLT_MAX * 0.5f;
     const float thr = has_third ? a3 + E + slack : -INFINITY;
-    const bool two = live && !quick && !tie && k1 >= 0 && k1 < K;
-    const bool fall = live && !quick && !two;
+    const bool need = p < M && !quick && !tie && k1 >= 0 && k1 < K;
+    float sx = -INFINITY;
+    if (need) {
+      const int cj = h == 0 ? k1 : k2;
+      if (cj >= 0 && cj < K) {
+        const float *xr = X + p * D;
+        const float *cr = C + (long)cj * D;
+        float sc = -0.5f * nc[cj];
+        if ((D & 3) == 0) {
+          const float4 *x4 = reinterpret_cast<const float4 *>(xr);
+          const float4 *c4 = reinterpret_cast<const float4 *>(cr);
+#pragma unroll 4
+          for (int q = 0; q < D / 4; ++q) {
+            const float4 xv = x4[q], cv = c4[q];
+            sc = __builtin_fmaf(xv.x, cv.x, sc);
+            sc = __builtin_fmaf(xv.y, cv.y, sc);
+            sc = __builtin_fmaf(xv.z, cv.z, sc);
+            sc = __builtin_fmaf(xv.w, cv.w, sc);
+          }
+        } else {
+          for (int q = 0; q < D; ++q) sc = __builtin_fmaf(xr[q], cr[q], sc);
+        }
+        sx = sc;
+      }
+    }
+    const float sy = __shfl_xor(sx, 32);  // the other half's chain
+    bool fall2 = false;
+    if (live && need) {
+      int lab = k1;
+      float sb = sx;
+      if (k2 >= 0 && k2 < K && (sy > sx || (sy == sx && k2 < k1))) { lab = k2; sb = sy; }
+      if (sb > thr && sb > -INFINITY) labels[p] = lab;
+      else fall2 = true;
+    }
+    const bool two = live && need;
+    const bool fall = (live && !quick && !need) || fall2;
     if (live && quick) labels[p] = k1;
-    // wave-aggregated appends (one atomic per wave and list)
+    // wave-aggregated counts and appends (one atomic per wave)
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     {
       const uint64_t mk = __ballot(two);
-      if (mk) {
-        const int lead = __ffsll((long long)mk) - 1;
-        int base = 0;
-        if (lane == lead) base = atomicAdd(ncand, __popcll(mk));
-        base = __shfl(base, lead);
-        if (two) cands[base + __popcll(mk & below)] = make_int4((int)p, k1, k2, __float_as_int(thr));
-      }
+      if (mk && lane == __ffsll((long long)mk) - 1) atomicAdd(ncand, __popcll(mk));
     }
     {
       const uint64_t mk = __ballot(fall);
@@ -373,64 +411,219 @@ __global__ __launch_bounds__(256) void k_assign_mfma(
   finish(1, S1);
 }
 
-// Certificate 2: exact FP32 chains (the plain path's arithmetic) of the two
-// candidates of each listed point, one lane per (point, candidate); the
-// better one (lowest index on equal scores) is the label if it exceeds the
-// point's threshold, otherwise the point goes to the full FP32 search.
-__global__ __launch_bounds__(256) void k_rescore2(const int *__restrict__ ncand,
-                                                  const int4 *__restrict__ cands, int D, int K,
-                                                  const float *__restrict__ X,
-                                                  const float *__restrict__ C,
-                                                  const float *__restrict__ nc,
-                                                  int *__restrict__ labels, int *__restrict__ flags,
-                                                  int *__restrict__ nflag) {
-  const long total = 2L * *ncand;
-  const int lane = threadIdx.x & 63;
-  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  for (long t0 = (long)blockIdx.x * blockDim.x; t0 < total; t0 += (long)gridDim.x * blockDim.x) {
-    const long t = t0 + threadIdx.x;
-    const bool ok = t < total;
-    int4 ce = make_int4(0, 0, -1, 0);
-    if (ok) ce = cands[t >> 1];
-    const int cand = (t & 1) ? ce.z : ce.y;
-    float s = -INFINITY;
-    if (ok && cand >= 0 && cand < K) {
-      const float *x = X + (long)ce.x * D;
-      const float *c = C + (long)cand * D;
-      s = -0.5f * nc[cand];
-      if ((D & 3) == 0) {
-        const float4 *x4 = reinterpret_cast<const float4 *>(x);
-        const float4 *c4 = reinterpret_cast<const float4 *>(c);
-#pragma unroll 8
-        for (int q = 0; q < D / 4; ++q) {
-          const float4 xv = x4[q], cv = c4[q];
-          s = __builtin_fmaf(xv.x, cv.x, s);
-          s = __builtin_fmaf(xv.y, cv.y, s);
-          s = __builtin_fmaf(xv.z, cv.z, s);
-          s = __builtin_fmaf(xv.w, cv.w, s);
-        }
-      } else {
-        for (int q = 0; q < D; ++q) s = __builtin_fmaf(x[q], c[q], s);
+// Deep re-resolve of the points certificate 2 left open (typically many
+// centres within the BF16 error bound of the best).  A second MFMA pass over
+// just those points in split BF16: x = xh + xl and c = ch + cl (the low parts
+// are the BF16 roundings of the residuals), scores xh.ch + xh.cl + xl.ch by
+// three MFMAs, whose operand error is at most u^2 (3 + 2u) sum|x_d c_d|
+// instead of (2u + u^2) sum|x_d c_d|.  The pass keeps the four largest
+// approximate scores, computes the exact FP32 chains of the first three (half
+// 0 of the wave two, half 1 one, each reading its point's row once), and
+// labels the point when the best of them (lowest index on equal scores)
+// exceeds the fourth + E; the rest go to the FP32 full search.  One wave
+// handles 32 listed points; centre fragments come from the L2-resident
+// arrays.  BF16 engines only.
+template <int KS>
+__global__ __launch_bounds__(256) void k_deep_mfma(
+    const int *__restrict__ nlist, const int *__restrict__ list, const uint16_t *__restrict__ Xh,
+    const float *__restrict__ X, int D, int K, int Kp, const uint16_t *__restrict__ Ch,
+    const uint16_t *__restrict__ Cl, const float *__restrict__ bias, const float *__restrict__ C,
+    const float *__restrict__ nc, const float *__restrict__ bnd, const float *__restrict__ xnorm,
+    int *__restrict__ labels, int *__restrict__ list2, int *__restrict__ nlist2) {
+  const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31;
+  const long total = *nlist;
+  const long wstride = (long)gridDim.x * 128;
+  const float cmax = bnd[0], ncmax = bnd[1];
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  constexpr int Dp = KS * 16;
+  constexpr int CH = MF_TILE * Dp / 8;  // 16-byte chunks of one tile (high or low part)
+  constexpr int PER = (2 * CH + 255) / 256;
+  // two LDS buffers of one centre tile: high chunks then low chunks; chunk q
+  // of row r at r*CPR + (q + r) % CPR (rotation: the 32 rows one fragment
+  // read touches fall in different banks)
+  constexpr int CPR = Dp / 8;
+  __shared__ u32x4_t ts[2][2 * CH];
+  const int nT = Kp / MF_TILE;
+  auto fetch = [&](int t, u32x4_t *reg) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int c = threadIdx.x + 256 * i;
+      if (c < 2 * CH) {
+        const int part = c >= CH, cc = part ? c - CH : c;
+        const uint16_t *src = part ? Cl : Ch;
+        reg[i] = *reinterpret_cast<const u32x4_t *>(src + (long)t * MF_TILE * Dp + (long)cc * 8);
       }
     }
-    const float so = __shfl_xor(s, 1);
+  };
+  // the loop over point groups is uniform across the workgroup (barriers)
+  for (long bb = (long)blockIdx.x * 128; bb < total; bb += wstride) {
+    const long b0 = bb + (threadIdx.x >> 6) * 32;
+    const long li = b0 + col;
+    const bool ok = li < total;
+    const long p = ok ? (long)list[li] : 0;
+    // point fragments (8 consecutive dimensions per k-step): high and low
+    // BF16 parts; v - high is exact in FP32
+    u32x4_t xh4[KS], xl4[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      xh4[ks] = u32x4_t{0u, 0u, 0u, 0u};
+      xl4[ks] = u32x4_t{0u, 0u, 0u, 0u};
+      if (ok) {
+        xh4[ks] = *reinterpret_cast<const u32x4_t *>(Xh + p * Dp + ks * 16 + 8 * h);
+        uint32_t w[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int d0 = ks * 16 + 8 * h + 2 * u;
+          const float v0 = d0 < D ? X[p * D + d0] : 0.0f;
+          const float v1 = d0 + 1 < D ? X[p * D + d0 + 1] : 0.0f;
+          const uint32_t hw = xh4[ks][u];
+          const float r0 = v0 - __uint_as_float(hw << 16);
+          const float r1 = v1 - __uint_as_float(hw & 0xffff0000u);
+          w[u] = (uint32_t)f32_to_bf16_rne(r0) | ((uint32_t)f32_to_bf16_rne(r1) << 16);
+        }
+        xl4[ks] = u32x4_t{w[0], w[1], w[2], w[3]};
+      }
+    }
+    float bv[4];
+    int bi[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) { bv[q] = -INFINITY; bi[q] = -1; }
+    u32x4_t pre[PER];
+    fetch(0, pre);
+    for (int t = 0; t < nT; ++t) {
+      const int buf = t & 1;
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int c = threadIdx.x + 256 * i;
+        if (c < 2 * CH) {
+          const int part = c >= CH, cc = part ? c - CH : c;
+          const int r = cc / CPR, q = cc - r * CPR;
+          ts[buf][part * CH + r * CPR + (q + r) % CPR] = pre[i];
+        }
+      }
+      __syncthreads();
+      if (t + 1 < nT) fetch(t + 1, pre);
+      f32x16_t acc;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[r] = bias[t * MF_TILE + mf_row(r, h)];
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const int q = col * CPR + (ks * 2 + h + col) % CPR;
+        const u32x4_t ah = ts[buf][q];
+        const u32x4_t al = ts[buf][CH + q];
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, ah),
+                                                       __builtin_bit_cast(bf16x8_t, xh4[ks]), acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, ah),
+                                                       __builtin_bit_cast(bf16x8_t, xl4[ks]), acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, al),
+                                                       __builtin_bit_cast(bf16x8_t, xh4[ks]), acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        float v = acc[r];
+        int vi = t * MF_TILE + mf_row(r, h);
+        if (v > bv[3]) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            if (v > bv[q]) {
+              const float tv = bv[q];
+              const int ti = bi[q];
+              bv[q] = v; bi[q] = vi; v = tv; vi = ti;
+            }
+          }
+        }
+      }
+    }
+    // merge the other half's four (disjoint centres); use half 0's list
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      float v = __shfl_xor(bv[q], 32);
+      int vi = __shfl_xor(bi[q], 32);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (v > bv[u]) {
+          const float tv = bv[u];
+          const int ti = bi[u];
+          bv[u] = v; bi[u] = vi; v = tv; vi = ti;
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      bv[q] = __shfl(bv[q], col);
+      bi[q] = __shfl(bi[q], col);
+    }
+    // exact chains, one pass over the row: half 0 -> entries 0, 1; half 1 -> 2
+    const int c0 = h ? bi[2] : bi[0];
+    const int c1 = h ? -1 : bi[1];
+    const bool l0 = ok && c0 >= 0 && c0 < K, l1 = ok && c1 >= 0 && c1 < K;
+    float s0 = l0 ? -0.5f * nc[c0] : -INFINITY, s1 = l1 ? -0.5f * nc[c1] : -INFINITY;
+    if (l0 || l1) {
+      const float *xr = X + p * D;
+      const float *r0 = C + (long)(l0 ? c0 : 0) * D, *r1 = C + (long)(l1 ? c1 : 0) * D;
+      if ((D & 3) == 0) {
+        const float4 *x4 = reinterpret_cast<const float4 *>(xr);
+        const float4 *a4 = reinterpret_cast<const float4 *>(r0);
+        const float4 *b4 = reinterpret_cast<const float4 *>(r1);
+#pragma unroll 4
+        for (int q = 0; q < D / 4; ++q) {
+          const float4 xv = x4[q], av = a4[q], bw = b4[q];
+          s0 = __builtin_fmaf(xv.x, av.x, s0);
+          s1 = __builtin_fmaf(xv.x, bw.x, s1);
+          s0 = __builtin_fmaf(xv.y, av.y, s0);
+          s1 = __builtin_fmaf(xv.y, bw.y, s1);
+          s0 = __builtin_fmaf(xv.z, av.z, s0);
+          s1 = __builtin_fmaf(xv.z, bw.z, s1);
+          s0 = __builtin_fmaf(xv.w, av.w, s0);
+          s1 = __builtin_fmaf(xv.w, bw.w, s1);
+        }
+      } else {
+        for (int q = 0; q < D; ++q) {
+          s0 = __builtin_fmaf(xr[q], r0[q], s0);
+          s1 = __builtin_fmaf(xr[q], r1[q], s1);
+        }
+      }
+      if (!l0) s0 = -INFINITY;
+      if (!l1) s1 = -INFINITY;
+    }
+    const float s2 = __shfl_xor(s0, 32);
     bool fall = false;
-    if (ok && (t & 1) == 0) {
-      const float s1 = s, s2 = so;
-      int lab = ce.y;
-      float sb = s1;
-      if (ce.z >= 0 && ce.z < K && (s2 > s1 || (s2 == s1 && ce.z < ce.y))) { lab = ce.z; sb = s2; }
-      if (sb > __int_as_float(ce.w) && sb > -INFINITY) labels[ce.x] = lab;
+    if (ok && h == 0) {
+      const float sv[3] = {s0, s1, s2};
+      float sb = -INFINITY;
+      int lab = -1;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int cj = bi[q];
+        if (cj < 0 || cj >= K) continue;
+        if (sv[q] > sb || (sv[q] == sb && cj < lab)) { sb = sv[q]; lab = cj; }
+      }
+      const float xn = xnorm[p];
+      const float xc = xn * cmax;
+      const float ub = 3.90625e-3f;  // 2^-8
+      const float Df = (float)D;
+      const float g32 = Df * 5.9604645e-8f * 1.0001f;                 // gamma_D, u = 2^-24
+      const float gmf = (3.0f * Df + 1.0f) * 2.3841858e-7f * 1.001f;  // gamma'_{3D+1}, u = 2^-22
+      const float split = ub * ub * (3.0f + 2.0f * ub) * xc;
+      const float mfe = gmf * (0.5f * ncmax + (1.0f + ub) * (1.0f + ub) * (1.0f + 2.0f * ub) * xc);
+      const float fpe = g32 * (0.5f * ncmax + xc);
+      const float E = (split + mfe + fpe) * 1.001f + 1e-30f * Df;
+      const bool has4 = bi[3] >= 0 && bi[3] < K && bv[3] > -FLT_MAX * 0.5f;
+      const float a4 = has4 ? bv[3] : -INFINITY;
+      const float slack = (fabsf(bv[0]) + (has4 ? fabsf(a4) : 0.0f)) * 1.6e-5f;
+      const float thr = has4 ? a4 + E + slack : -INFINITY;
+      if (lab >= 0 && sb > thr && sb > -INFINITY) labels[p] = lab;
       else fall = true;
     }
     const uint64_t mk = __ballot(fall);
     if (mk) {
       const int lead = __ffsll((long long)mk) - 1;
       int base = 0;
-      if (lane == lead) base = atomicAdd(nflag, __popcll(mk));
+      if (lane == lead) base = atomicAdd(nlist2, __popcll(mk));
       base = __shfl(base, lead);
-      if (fall) flags[base + __popcll(mk & below)] = ce.x;
+      if (fall) list2[base + __popcll(mk & below)] = (int)p;
     }
+    __syncthreads();  // the tile buffers are refilled by the next group
   }
 }
 
@@ -447,6 +640,9 @@ void mfma_free(Engine *e) {
   hipFree(s->cands);
   hipFree(s->ncand);
   hipFree(s->nflag);
+  hipFree(s->flags2);
+  hipFree(s->Cl);
+  hipFree(s->nflag2);
   delete s;
   e->mfma = nullptr;
 }
@@ -478,15 +674,19 @@ static hipError_t ensure_state(Engine *e) {
   const long xrows = e->streaming ? e->tile : e->m;  // points held in HBM at once
   if ((r = hipMalloc(&s->Xh, sizeof(uint16_t) * (size_t)xrows * s->Dp)) != hipSuccess) return r;
   if ((r = hipMalloc(&s->Ch, sizeof(uint16_t) * (size_t)s->Kp * s->Dp)) != hipSuccess) return r;
+  if ((r = hipMalloc(&s->Cl, sizeof(uint16_t) * (size_t)s->Kp * s->Dp)) != hipSuccess) return r;
   if ((r = hipMalloc(&s->bias, sizeof(float) * s->Kp)) != hipSuccess) return r;
   if ((r = hipMalloc(&s->bnd, sizeof(float) * 4)) != hipSuccess) return r;
   if ((r = hipMalloc(&s->xnorm, sizeof(float) * (size_t)xrows)) != hipSuccess) return r;
   if ((r = hipMalloc(&s->flags, sizeof(int) * (size_t)xrows)) != hipSuccess) return r;
-  if ((r = hipMalloc(&s->cands, sizeof(int4) * (size_t)xrows)) != hipSuccess) return r;
+  if ((r = hipMalloc(&s->flags2, sizeof(int) * (size_t)xrows)) != hipSuccess) return r;
   s->nflag_slots = 64;
   if ((r = hipMalloc(&s->nflag, sizeof(int) * s->nflag_slots)) != hipSuccess) return r;
   if ((r = hipMalloc(&s->ncand, sizeof(int) * s->nflag_slots)) != hipSuccess) return r;
+  if ((r = hipMalloc(&s->nflag2, sizeof(int) * s->nflag_slots)) != hipSuccess) return r;
   if ((r = hipMemsetAsync(s->ncand, 0, sizeof(int) * s->nflag_slots, e->stream)) != hipSuccess)
+    return r;
+  if ((r = hipMemsetAsync(s->nflag2, 0, sizeof(int) * s->nflag_slots, e->stream)) != hipSuccess)
     return r;
   return hipMemsetAsync(s->nflag, 0, sizeof(int) * s->nflag_slots, e->stream);
 }
@@ -581,8 +781,10 @@ hipError_t mfma_prepare_rows(Engine *e, const float *X, long rows) {
 
 static hipError_t prepare_centres(Engine *e) {
   MfmaState *s = st(e);
-  hipLaunchKernelGGL(k_prepare_centres, dim3(1), dim3(256), 0, e->stream, e->k, e->n, s->Dp, s->Kp,
-                     e->dC, e->dnc, s->scale, (int)s->fp16, s->Ch, s->bias, s->bnd);
+  const long pe = (long)s->Kp * s->Dp;
+  const unsigned pg = (unsigned)std::min<long>(km_cdiv(pe, 256), 1024);
+  hipLaunchKernelGGL(k_prepare_centres, dim3(pg), dim3(256), 0, e->stream, e->k, e->n, s->Dp, s->Kp,
+                     e->dC, e->dnc, s->scale, (int)s->fp16, s->Ch, s->Cl, s->bias, s->bnd);
   s->c_ready = true;
   return hipGetLastError();
 }
@@ -664,9 +866,6 @@ hipError_t mfma_assign_view(Engine *e, const float *X, const uint16_t *Xh, const
     default: launch_mfma_ks<16>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, ub, eta, lab, s->flags, nf, s->cands, nc2); break;
   }
   if ((r = hipGetLastError()) != hipSuccess) return r;
-  const long rg = km_cdiv(2 * rows, 256);
-  hipLaunchKernelGGL(k_rescore2, dim3(rg < num_cus() * 8 ? rg : num_cus() * 8), dim3(256), 0,
-                     e->stream, nc2, s->cands, e->n, e->k, X, e->dC, e->dnc, lab, s->flags, nf);
   return hipGetLastError();
 }
 
@@ -675,8 +874,8 @@ hipError_t mfma_ensure_slots(Engine *e, int n) {
   if ((r = ensure_state(e)) != hipSuccess) return r;
   MfmaState *s = st(e);
   if (n <= s->nflag_slots) return hipSuccess;
-  int *arrs[2] = {s->nflag, s->ncand};
-  for (int a = 0; a < 2; ++a) {
+  int *arrs[3] = {s->nflag, s->ncand, s->nflag2};
+  for (int a = 0; a < 3; ++a) {
     int *nb = nullptr;
     if ((r = hipMalloc(&nb, sizeof(int) * n)) != hipSuccess) return r;
     if ((r = hipMemsetAsync(nb, 0, sizeof(int) * n, e->stream)) != hipSuccess) return r;
@@ -689,6 +888,7 @@ hipError_t mfma_ensure_slots(Engine *e, int n) {
   }
   s->nflag = arrs[0];
   s->ncand = arrs[1];
+  s->nflag2 = arrs[2];
   s->nflag_slots = n;
   return hipSuccess;
 }
@@ -711,12 +911,39 @@ hipError_t mfma_prepare(Engine *e) {
 
 // FP32 full search for the uncertified points (device-side count).
 hipError_t mfma_reresolve(Engine *e, long r0, long rows) {
-  return mfma_reresolve_view(e, e->dX + r0 * e->n, e->dlabels + r0, rows);
+  MfmaState *s = st(e);
+  return mfma_reresolve_view(e, e->dX + r0 * e->n, s->Xh + r0 * s->Dp, s->xnorm + r0,
+                             e->dlabels + r0, rows);
 }
 
-hipError_t mfma_reresolve_view(Engine *e, const float *X, int *lab, long rows) {
+// Points left open by the assign kernel: deep MFMA pass, then the FP32 full
+// search of what remains (device-side counts; X, Xh, xn are the same rows
+// the assign pass saw, indexed from 0).
+hipError_t mfma_reresolve_view(Engine *e, const float *X, const uint16_t *Xh, const float *xn,
+                               int *lab, long rows) {
   MfmaState *s = st(e);
-  return launch_assign_f32(e->stream, rows, s->nflag + e->cur_slot, e->n, X, s->flags, e->k,
+  hipError_t r;
+  if ((r = hipMemsetAsync(s->nflag2 + e->cur_slot, 0, sizeof(int), e->stream)) != hipSuccess) return r;
+  if (rows <= 0) return hipSuccess;
+  if (s->fp16)  // FP16 engines: straight to the FP32 search
+    return launch_assign_f32(e->stream, rows, s->nflag + e->cur_slot, e->n, X, s->flags, e->k,
+                             e->dCt, e->ldc, e->dnc, lab, nullptr);
+  const long g = std::min<long>(km_cdiv(rows, 128), (long)num_cus() * 8);
+#define DEEP(KS)                                                                                 \
+  hipLaunchKernelGGL(k_deep_mfma<KS>, dim3(g), dim3(256), 0, e->stream, s->nflag + e->cur_slot,  \
+                     s->flags, Xh, X, e->n, e->k, s->Kp, s->Ch, s->Cl, s->bias, e->dC, e->dnc,    \
+                     s->bnd, xn, lab, s->flags2, s->nflag2 + e->cur_slot)
+  switch (s->KS) {
+    case 1: DEEP(1); break;
+    case 2: DEEP(2); break;
+    case 4: DEEP(4); break;
+    case 6: DEEP(6); break;
+    case 8: DEEP(8); break;
+    default: DEEP(16); break;
+  }
+#undef DEEP
+  if ((r = hipGetLastError()) != hipSuccess) return r;
+  return launch_assign_f32(e->stream, rows, s->nflag2 + e->cur_slot, e->n, X, s->flags2, e->k,
                            e->dCt, e->ldc, e->dnc, lab, nullptr);
 }
 
@@ -730,7 +957,7 @@ void mfma_collect(Engine *e, int nslots) {
   if (!s || nslots <= 0) return;
   std::vector<int> v(nslots);
   const int n = nslots < s->nflag_slots ? nslots : s->nflag_slots;
-  if (hipMemcpyAsync(v.data(), s->nflag, sizeof(int) * n, hipMemcpyDeviceToHost, e->stream) !=
+  if (hipMemcpyAsync(v.data(), s->fp16 ? s->nflag : s->nflag2, sizeof(int) * n, hipMemcpyDeviceToHost, e->stream) !=
       hipSuccess)
     return;
   if (hipStreamSynchronize(e->stream) != hipSuccess) return;

@@ -16,7 +16,8 @@ hipError_t mfma_prepare(Engine *e);
 hipError_t mfma_prepare_rows(Engine *e, const float *X, long rows);
 hipError_t mfma_assign_view(Engine *e, const float *X, const uint16_t *Xh, const float *xn,
                             int *lab, long rows);
-hipError_t mfma_reresolve_view(Engine *e, const float *X, int *lab, long rows);
+hipError_t mfma_reresolve_view(Engine *e, const float *X, const uint16_t *Xh, const float *xn,
+                               int *lab, long rows);
 const uint16_t *mfma_tile_xh(Engine *e);
 const float *mfma_tile_xnorm(Engine *e);
 int mfma_row_stride(Engine *e);  // padded width Dp of the reduced-precision rows
