@@ -34,7 +34,10 @@ typedef __attribute__((ext_vector_type(16))) float f32x16_t;
 typedef __attribute__((ext_vector_type(4))) unsigned int u32x4_t;
 
 #define MF_TILE 32     // centres per MFMA tile (rows of the 32x32 output)
-#define MF_PTS 256     // points per workgroup (4 waves x 2 groups x 32)
+#ifndef MF_WAVES
+#define MF_WAVES 4     // waves per workgroup; every wave reads the same centre stages
+#endif
+#define MF_PTS (64 * MF_WAVES)  // points per workgroup (2 groups of 32 per wave)
 #define MF_STAGE 4     // centre tiles per LDS stage (128 centres)
 
 // Round-to-nearest-even conversions written out so the rounding is known.
@@ -143,7 +146,7 @@ struct Top3 {
 __device__ inline int mf_row(int r, int h) { return 8 * (r >> 2) + 4 * h + (r & 3); }
 
 template <int KS, bool FP16>
-__global__ __launch_bounds__(256) void k_assign_mfma(
+__global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
     long M, const uint16_t *__restrict__ Xh, const float *__restrict__ X, int D, int K, int Kp,
     const uint16_t *__restrict__ Ch, const float *__restrict__ bias, const float *__restrict__ C,
     const float *__restrict__ nc, const float *__restrict__ bnd, const float *__restrict__ xnorm,
@@ -154,6 +157,7 @@ __global__ __launch_bounds__(256) void k_assign_mfma(
   constexpr int CPR = Dp / 8;               // 16-byte chunks per centre row
   constexpr int SROWS = MF_STAGE * MF_TILE; // centre rows per LDS stage
   constexpr int SCH = SROWS * CPR;          // chunks per stage
+  constexpr int ROTM = CPR >= 16 ? 15 : 7;   // row rotation mask of the LDS swizzle
   // LDS: two stages of centre rows (chunk q of row r stored at position
   // (q + r) % CPR, so the 32 rows one ds_read_b128 touches fall in distinct
   // banks) and two stages of biases; filled by global->LDS DMA.
@@ -183,31 +187,39 @@ __global__ __launch_bounds__(256) void k_assign_mfma(
 
   // asynchronous copy of centre stage st into LDS buffer b: each DMA
   // instruction writes 64 consecutive chunks (one per lane, in lane order).
-  // Chunk q of row r lives at position (q + rot(r)) % CPR with rot(r) = r & 7
+  // Chunk q of row r lives at position (q + rot(r)) % CPR with rot(r) = r & ROTM
+  // (15 for rows of 16 or more chunks, else 7)
   // (for CPR >= 8), so the 8 rows one ds_read_b128 phase touches hit
   // distinct banks.
   auto issue_stage = [&](int st, int b) {
     u32x4_t *dst = As + b * SCH;
     const uint16_t *cs = Ch + (long)st * SROWS * Dp;
+    constexpr int NI = SCH / 64;                            // DMA instructions per stage
+    constexpr int PERW = (NI + MF_WAVES - 1) / MF_WAVES;    // per wave (the last may idle)
     if constexpr (64 % CPR == 0) {
       constexpr int RPI = 64 / CPR;  // rows per DMA instruction
-      const int r0 = w * RPI + lane / CPR;
-      const int rot = CPR >= 8 ? (r0 & 7) : 0;  // same for every instruction below
-      const int q = (lane % CPR - rot + CPR) % CPR;
-      const uint16_t *src = cs + (long)r0 * Dp + q * 8;
 #pragma unroll
-      for (int i = 0; i < SCH / 256; ++i)
-        __builtin_amdgcn_global_load_lds(src + (long)i * 4 * RPI * Dp, dst + (i * 4 + w) * 64, 16,
-                                         0, 0);
+      for (int i = 0; i < PERW; ++i) {
+        const int n = i * MF_WAVES + w;
+        if (n < NI) {
+          const int r0 = n * RPI + lane / CPR;
+          const int rot = CPR >= 8 ? (r0 & ROTM) : 0;
+          const int q = (lane % CPR - rot + CPR) % CPR;
+          __builtin_amdgcn_global_load_lds(cs + (long)r0 * Dp + q * 8, dst + n * 64, 16, 0, 0);
+        }
+      }
     } else {
 #pragma unroll
-      for (int i = 0; i < SCH / 256; ++i) {
-        const int c0 = (i * 4 + w) * 64;
-        const int c = c0 + lane;
-        const int r = c / CPR, pos = c - r * CPR;
-        const int rot = CPR >= 8 ? (r & 7) : 0;
-        const int q = (pos - rot + CPR) % CPR;
-        __builtin_amdgcn_global_load_lds(cs + (long)r * Dp + q * 8, dst + c0, 16, 0, 0);
+      for (int i = 0; i < PERW; ++i) {
+        const int n = i * MF_WAVES + w;
+        if (n < NI) {
+          const int c0 = n * 64;
+          const int c = c0 + lane;
+          const int r = c / CPR, pos = c - r * CPR;
+          const int rot = CPR >= 8 ? (r & ROTM) : 0;
+          const int q = (pos - rot + CPR) % CPR;
+          __builtin_amdgcn_global_load_lds(cs + (long)r * Dp + q * 8, dst + c0, 16, 0, 0);
+        }
       }
     }
     if (tid < SROWS / 4)  // biases: 4 floats per lane
@@ -234,7 +246,7 @@ __global__ __launch_bounds__(256) void k_assign_mfma(
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
       const int q = ks * 2 + h;
-      const u32x4_t a = A[row * CPR + (q + (CPR >= 8 ? (row & 7) : 0)) % CPR];
+      const u32x4_t a = A[row * CPR + (q + (CPR >= 8 ? (row & ROTM) : 0)) % CPR];
       if constexpr (FP16) {
         a0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(vec8, a),
                                                     __builtin_bit_cast(vec8, bfrag[0][ks]), a0, 0,
@@ -819,10 +831,10 @@ static void launch_mfma_ks(hipStream_t strm, bool fp16, dim3 g, long M, const ui
     attr_set[fp16] = true;
   }
   if (fp16)
-    hipLaunchKernelGGL((k_assign_mfma<KS, true>), g, dim3(256), lds, strm, M, Xh, X, D, K, Kp, Ch,
+    hipLaunchKernelGGL((k_assign_mfma<KS, true>), g, dim3(64 * MF_WAVES), lds, strm, M, Xh, X, D, K, Kp, Ch,
                        bias, C, nc, bnd, xnorm, inv_s2, ub, eta, FLT_MAX, labels, flags, nflag, cands, ncand);
   else
-    hipLaunchKernelGGL((k_assign_mfma<KS, false>), g, dim3(256), lds, strm, M, Xh, X, D, K, Kp, Ch,
+    hipLaunchKernelGGL((k_assign_mfma<KS, false>), g, dim3(64 * MF_WAVES), lds, strm, M, Xh, X, D, K, Kp, Ch,
                        bias, C, nc, bnd, xnorm, inv_s2, ub, eta, FLT_MAX, labels, flags, nflag, cands, ncand);
 }
 
