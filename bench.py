@@ -218,7 +218,8 @@ def bench_batch(args, km, datagen, np, torch):
     nb = len(n)
     total = int(n.sum())
     steps = args.steps
-    b = km.KMeansBatch(n, d, k)
+    prec = args.precision if args.precision in ("fp32", "bf16") else "bf16"
+    b = km.KMeansBatch(n, d, k, precision=prec)
     b.set_data(device_ptr=X.data_ptr())
     b.init_first_k()
     if args.warmup:
@@ -238,22 +239,39 @@ def bench_batch(args, km, datagen, np, torch):
         "n_gpus": 1, "steps": steps, "warmup": args.warmup,
         "ms_per_step": dt * 1e3 / steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "bf16 (labels certified against f32)" if prec == "bf16" else "f32",
         "data": "synthetic (per-instance Gaussian mixtures generated on the GPU)",
         "config": {"workload": f"c4: {nb} instances, D={d} K={k}, lognormal N (median 20000, "
                                f"sigma 1, clipped [1000, 200000]), sum N={total}, first-K init, "
-                               f"one hipGraph per run",
-                   "instances": nb, "D": d, "K": k, "sum_N": total},
+                               f"{prec} assign, one hipGraph per run",
+                   "instances": nb, "D": d, "K": k, "sum_N": total, "precision": prec},
         "graph_ms_per_step": st["ms_graph"] / steps,
         "batch_stats": st,
-        "roofline": {"bound": "fp32", "achieved": 2.0 * d * pairs / (st["ms_graph"] / steps * 1e-3) / 1e12,
-                     "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
-                     "frac": 2.0 * d * pairs / (st["ms_graph"] / steps * 1e-3) / PEAK["fp32"],
+        "roofline": {"bound": "fp32" if prec == "fp32" else "mfma",
+                     "achieved": 2.0 * d * pairs / (st["ms_graph"] / steps * 1e-3) / 1e12,
+                     "peak": PEAK[prec] / 1e12, "unit": "TFLOP/s",
+                     "frac": 2.0 * d * pairs / (st["ms_graph"] / steps * 1e-3) / PEAK[prec],
                      "traffic": None, "kernel": "whole iteration (assign + update)",
                      "algorithmic": "2*D*sum(N)*K flops per iteration"},
     }
     labels = b.labels()
+    centers_b = b.centers()
     b.close()
+    if prec != "fp32":
+        # the plain FP32 batched engine from the same centres: labels must match
+        b32 = km.KMeansBatch(n, d, k, precision="fp32")
+        b32.set_data(device_ptr=X.data_ptr())
+        b32.init_first_k()
+        b32.run(1)
+        b32.init_first_k()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b32.run(steps)
+        torch.cuda.synchronize()
+        out["fp32_batch_ms_per_step"] = (time.perf_counter() - t0) * 1e3 / steps
+        out["label_mismatches_vs_fp32"] = int((b32.labels() != labels).sum())
+        out["centers_equal_fp32"] = bool(np.array_equal(b32.centers(), centers_b))
+        b32.close()
     # the method without batching: one single-problem engine per instance
     sample = np.linspace(0, nb - 1, min(nb, 48)).astype(int)
     off = np.concatenate([[0], np.cumsum(n)])

@@ -196,7 +196,10 @@ kmeansErrorStatus kmeansSynchronize(kmeansHandle h);
  * FP64 moments).  Each phase of an iteration is one persistent launch over
  * all instances, and a run of `iters` iterations is one hipGraph launch.
  * Limits: d <= 64, k <= 1024, k * (d rounded up to 4, 8, 16, 32 or 64, + 1)
- * * 4 bytes <= 64 KiB, k <= n[i] < KM_BLOCK, sum of n[i] < 2^31; FP32 only.
+ * * 4 bytes <= 64 KiB, k <= n[i] < KM_BLOCK, sum of n[i] < 2^31.
+ * Precision: KM_PREC_FP32, or KM_PREC_BF16 for d <= 32 (BF16 MFMA scores,
+ * each label certified against the FP32 scores or searched exactly, so the
+ * labels are the FP32 labels).
  * ========================================================================= */
 typedef struct kmeansBatchStats {
   int iterations;          /* total over runs                                 */
@@ -209,6 +212,8 @@ typedef struct kmeansBatchStats {
   double ms_graph;         /* device time of the last run's graph             */
   double ms_capture;       /* host time to capture and instantiate it         */
   float tune_ms[5];        /* assign time per candidate tile: 256 .. 4096     */
+  long long exact_searched; /* BF16: point assignments certified by no bound
+                               and searched exactly (summed over iterations) */
 } kmeansBatchStats;
 
 typedef struct kmeansBatchEngine *kmeansBatchHandle;

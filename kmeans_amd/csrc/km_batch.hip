@@ -62,6 +62,7 @@ struct Batch {
   int2 *items = nullptr;
   int nitems = 0, P = 0;
   int *counters = nullptr;  // assign, update
+  unsigned long long *nexact = nullptr;  // BF16 assign: points searched exactly
   int comp_slots = 0;
   bool data_set = false, centers_set = false;
   // graph of the last run length
@@ -216,6 +217,269 @@ __global__ __launch_bounds__(256) void k_batch_assign_f32x(
     }
     __syncthreads();
   }
+}
+
+// Assign on BF16 MFMA for d <= 32 (KS = 1 or 2 k-steps of 16), labels equal
+// to the FP32 labels.  Persistent over (instance, first row) items like the
+// FP32 kernel; per instance the LDS holds the BF16 centre rows (k padded to
+// 32, zero-padded dimensions), the biases -0.5*nc (-inf past k), the FP32
+// centres and norms, and the bound constants max ||c|| and max nc.  Each
+// wave scores 32 points at a time with v_mfma_f32_32x32x16_bf16 (points
+// converted to BF16 on the fly), keeps a top-3, computes the exact FP32
+// chains of the first two (lane halves), and labels the point when the
+// better one (lowest index on ties) exceeds the third approximate score + E,
+// E being the BF16 forward-error bound of the single-problem certificate.
+// Points that fail are searched exactly by the whole wave (64 lanes over the
+// k centres, canonical chains, argmax with the tie rule).
+typedef __attribute__((ext_vector_type(8))) __bf16 b_bf16x8;
+typedef __attribute__((ext_vector_type(16))) float b_f32x16;
+typedef __attribute__((ext_vector_type(4))) unsigned int b_u32x4;
+__device__ inline int b_row(int r, int h) { return 8 * (r >> 2) + 4 * h + (r & 3); }
+__device__ inline uint16_t b_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+template <int KS>
+__global__ __launch_bounds__(256) void k_batch_assign_bf16(
+    int nitems, const int2 *__restrict__ items, int P, const long *__restrict__ xoff,
+    const int *__restrict__ nvec, int d, int k, const float *__restrict__ X,
+    const float *__restrict__ C, const float *__restrict__ nc, int *__restrict__ labels,
+    int *__restrict__ counter, unsigned long long *__restrict__ nexact, float big) {
+  constexpr int Dp = KS * 16;
+  const int Kp = (k + 31) / 32 * 32;
+  extern __shared__ __attribute__((aligned(16))) unsigned char bsm[];
+  b_u32x4 *Ch = reinterpret_cast<b_u32x4 *>(bsm);                 // Kp x (Dp/8) chunks
+  float *bias = reinterpret_cast<float *>(Ch + (size_t)Kp * (Dp / 8));  // Kp
+  float *Cf = bias + Kp;                                            // k x d
+  float *ncs = Cf + (size_t)k * d;                                  // k
+  __shared__ int item_s;
+  __shared__ float red[256], bndv[2];
+  __shared__ unsigned long long exact_ct;
+  const int tid = threadIdx.x, lane = tid & 63, h = lane >> 5, col = lane & 31;
+  if (tid == 0) exact_ct = 0;
+  int cur = -1;
+  for (;;) {
+    if (tid == 0) item_s = atomicAdd(counter, 1);
+    __syncthreads();
+    const int it = item_s;
+    if (it >= nitems) break;
+    const int2 w = items[it];
+    const int inst = w.x;
+    if (inst != cur) {
+      const float *Ci = C + (size_t)inst * k * d;
+      for (int e = tid; e < Kp * (Dp / 8); e += 256) {
+        const int j = e / (Dp / 8), q8 = (e - j * (Dp / 8)) * 8;
+        uint32_t wv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int d0 = q8 + 2 * u;
+          const float v0 = (j < k && d0 < d) ? Ci[(size_t)j * d + d0] : 0.0f;
+          const float v1 = (j < k && d0 + 1 < d) ? Ci[(size_t)j * d + d0 + 1] : 0.0f;
+          wv[u] = (uint32_t)b_bf16(v0) | ((uint32_t)b_bf16(v1) << 16);
+        }
+        Ch[e] = b_u32x4{wv[0], wv[1], wv[2], wv[3]};
+      }
+      float mx = 0.0f;
+      for (int j = tid; j < Kp; j += 256) {
+        const float n2 = j < k ? nc[(size_t)inst * k + j] : 0.0f;
+        bias[j] = j < k ? -0.5f * n2 : -INFINITY;
+        if (j < k) { ncs[j] = n2; mx = fmaxf(mx, n2); }
+      }
+      for (int e = tid; e < k * d; e += 256) Cf[e] = Ci[e];
+      red[tid] = mx;
+      __syncthreads();
+      for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) red[tid] = fmaxf(red[tid], red[tid + o]);
+        __syncthreads();
+      }
+      if (tid == 0) {
+        const float ncmax = red[0] * (1.0f + 1e-5f) + 1e-30f;  // nc chain within gamma_d
+        bndv[0] = sqrtf(ncmax) * (1.0f + 1e-6f);
+        bndv[1] = ncmax;
+      }
+      cur = inst;
+    }
+    __syncthreads();
+    const float cmax = bndv[0], ncmax = bndv[1];
+    const long rows = min((long)P, (long)nvec[inst] - w.y);
+    const long base = xoff[inst] + w.y;
+    const int wv0 = tid >> 6;
+    for (long p0 = wv0 * 32; p0 < rows; p0 += 4 * 32) {
+      const long pr = p0 + col;
+      const bool ok = pr < rows;
+      const float *xr = X + (base + (ok ? pr : 0)) * d;
+      // the point's FP32 row in registers (zero past d), its BF16 fragments
+      // (this half's 8 dimensions per k-step) and its ||x||^2 part
+      float xv[Dp];
+      const bool vec = (d & 3) == 0;
+#pragma unroll
+      for (int q4 = 0; q4 < Dp / 4; ++q4) {
+        float4 t4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (ok && 4 * q4 < d) {
+          if (vec) {
+            t4 = reinterpret_cast<const float4 *>(xr)[q4];
+          } else {
+            t4.x = xr[4 * q4];
+            if (4 * q4 + 1 < d) t4.y = xr[4 * q4 + 1];
+            if (4 * q4 + 2 < d) t4.z = xr[4 * q4 + 2];
+            if (4 * q4 + 3 < d) t4.w = xr[4 * q4 + 3];
+          }
+        }
+        xv[4 * q4] = t4.x; xv[4 * q4 + 1] = t4.y; xv[4 * q4 + 2] = t4.z; xv[4 * q4 + 3] = t4.w;
+      }
+      b_u32x4 bf[KS];
+      float sq = 0.0f;
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        uint32_t wv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int d0 = ks * 16 + 8 * h + 2 * u;  // this half's dimensions (read again)
+          const float v0 = (ok && d0 < d) ? xr[d0] : 0.0f;
+          const float v1 = (ok && d0 + 1 < d) ? xr[d0 + 1] : 0.0f;
+          sq = __builtin_fmaf(v0, v0, sq);
+          sq = __builtin_fmaf(v1, v1, sq);
+          wv[u] = (uint32_t)b_bf16(v0) | ((uint32_t)b_bf16(v1) << 16);
+        }
+        bf[ks] = b_u32x4{wv[0], wv[1], wv[2], wv[3]};
+      }
+      sq += __shfl_xor(sq, 32);
+      // upper bound of ||x||: both halves' chains within gamma_d of the sum
+      const float xn = sqrtf(sq * (1.0f + 1e-5f)) * (1.0f + 1e-6f) + 1e-30f;
+      // top 3 of the packed scores (register id in the low 4 bits, three
+      // v_med3 per score, max as med3 with a run-time +FLT_MAX) and the tiles
+      // of the first two
+      float b1 = -FLT_MAX, b2 = -FLT_MAX, b3 = -FLT_MAX;
+      int t1 = 0, t2 = 0;
+      for (int t = 0; t < Kp / 32; ++t) {
+        b_f32x16 acc;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {  // rows 8q + 4h .. +3: one 16-byte read
+          const float4 b4 = *reinterpret_cast<const float4 *>(bias + t * 32 + 8 * q + 4 * h);
+          acc[4 * q] = b4.x; acc[4 * q + 1] = b4.y; acc[4 * q + 2] = b4.z; acc[4 * q + 3] = b4.w;
+        }
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+          const b_u32x4 a = Ch[(t * 32 + col) * (Dp / 8) + ks * 2 + h];
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(b_bf16x8, a),
+                                                         __builtin_bit_cast(b_bf16x8, bf[ks]), acc, 0, 0, 0);
+        }
+        const float o1 = b1, o2 = b2;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const float pv = __uint_as_float((__float_as_uint(fmaxf(acc[r], -FLT_MAX)) & 0xFFFFFFF0u) | (unsigned)r);
+          const float n3 = __builtin_amdgcn_fmed3f(b2, b3, pv);
+          const float n2 = __builtin_amdgcn_fmed3f(b1, b2, pv);
+          b1 = __builtin_amdgcn_fmed3f(b1, pv, big);
+          b2 = n2;
+          b3 = n3;
+        }
+        const unsigned u1 = __float_as_uint(b1), u2 = __float_as_uint(b2);
+        const int ot1 = t1, ot2 = t2;
+        t2 = (u2 == __float_as_uint(o1)) ? ot1 : (u2 == __float_as_uint(o2)) ? ot2 : t;
+        t1 = (u1 != __float_as_uint(o1)) ? t : ot1;
+      }
+      const bool real1 = b1 > -FLT_MAX, real2 = b2 > -FLT_MAX;
+      int i1 = real1 ? t1 * 32 + b_row((int)(__float_as_uint(b1) & 15u), h) : -1;
+      int i2 = real2 ? t2 * 32 + b_row((int)(__float_as_uint(b2) & 15u), h) : -1;
+      // bitwise-equal packed values make the tile tracking ambiguous: such
+      // points go to the exact search
+      bool tie = (__float_as_uint(b1) == __float_as_uint(b2)) ||
+                 (__float_as_uint(b2) == __float_as_uint(b3));
+      tie = tie || (__shfl_xor((int)tie, 32) != 0);
+      // merge halves; half 0's result is used by both
+      {
+        const float w1 = __shfl_xor(b1, 32), w2 = __shfl_xor(b2, 32), w3 = __shfl_xor(b3, 32);
+        const int j1 = __shfl_xor(i1, 32), j2 = __shfl_xor(i2, 32);
+        const bool A1 = b1 >= w1;
+        const float m1 = A1 ? b1 : w1;
+        const int k1 = A1 ? i1 : j1;
+        const float x2 = A1 ? b2 : b1, y2 = A1 ? w1 : w2;
+        const int xi2 = A1 ? i2 : i1, yi2 = A1 ? j1 : j2;
+        const bool A2 = x2 >= y2;
+        const float m2 = A2 ? x2 : y2;
+        const int k2 = A2 ? xi2 : yi2;
+        const float xa = A1 ? (A2 ? b3 : b2) : (A2 ? b2 : b1);
+        const float ya = A1 ? (A2 ? w1 : w2) : (A2 ? w2 : w3);
+        const float m3 = fmaxf(xa, ya);
+        const bool mt = (__float_as_uint(m1) == __float_as_uint(m2)) ||
+                        (__float_as_uint(m2) == __float_as_uint(m3));
+        b1 = __shfl(m1, col);
+        b2 = __shfl(m2, col);
+        b3 = __shfl(m3, col);
+        i1 = __shfl(k1, col);
+        i2 = __shfl(k2, col);
+        tie = tie || (__shfl((int)mt, col) != 0);
+      }
+      // exact chains of the two candidates
+      const int cj = h ? i2 : i1;
+      float sx = -INFINITY;
+      if (ok && cj >= 0 && cj < k) {
+        float sc = -0.5f * ncs[cj];
+        const float *cr = Cf + (size_t)cj * d;
+#pragma unroll
+        for (int q = 0; q < Dp; ++q)
+          if (q < d) sc = __builtin_fmaf(xv[q], cr[q], sc);
+        sx = sc;
+      }
+      const float sy = __shfl_xor(sx, 32);
+      const int jy = __shfl_xor(cj, 32);
+      bool fail = false;
+      if (ok && h == 0) {
+        int lab = cj;
+        float sb = sx;
+        if (jy >= 0 && jy < k && (sy > sb || (sy == sb && jy < lab))) { sb = sy; lab = jy; }
+        const float xc = xn * cmax, Df = (float)d;
+        const float ub = 3.90625e-3f;
+        const float g32 = Df * 5.9604645e-8f * 1.0001f;
+        const float gmf = (Df + 1.0f) * 2.3841858e-7f * 1.001f;
+        const float rnd = (2.0f * ub + ub * ub) * xc;
+        const float mfe = gmf * (0.5f * ncmax + (1.0f + ub) * (1.0f + ub) * xc);
+        const float fpe = g32 * (0.5f * ncmax + xc);
+        const float E = (rnd + mfe + fpe) * 1.001f + 1e-30f;
+        const bool has3 = b3 > -FLT_MAX;
+        // packed low bits (< 16 ulp) and FP32 rounding of the comparison
+        const float slack = (fabsf(b1) + fabsf(b2) + (has3 ? fabsf(b3) : 0.0f)) * 1.6e-5f;
+        const float thr = has3 ? b3 + E + slack : -INFINITY;
+        if (!tie && lab >= 0 && sb > -INFINITY && sb > thr) labels[base + pr] = lab;
+        else fail = true;
+      }
+      // exact search of the points that failed, by the whole wave
+      uint64_t fm = __ballot(fail);
+      if (lane == 0 && fm) atomicAdd(&exact_ct, (unsigned long long)__popcll(fm));
+      while (fm) {
+        const int src = __ffsll((long long)fm) - 1;
+        fm &= fm - 1;
+        const long pp = p0 + (src & 31);
+        float xq[Dp];
+#pragma unroll
+        for (int q = 0; q < Dp; ++q) xq[q] = __shfl(xv[q], src);
+        float best = -INFINITY;
+        int bj = 0x7fffffff;
+        for (int j = lane; j < k; j += 64) {
+          float sc = -0.5f * ncs[j];
+          const float *cr = Cf + (size_t)j * d;
+#pragma unroll
+          for (int q = 0; q < Dp; ++q)
+            if (q < d) sc = __builtin_fmaf(xq[q], cr[q], sc);
+          if (sc > best) { best = sc; bj = j; }
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+          const float ov = __shfl_xor(best, o);
+          const int oj = __shfl_xor(bj, o);
+          if (ov > best || (ov == best && oj < bj)) { best = ov; bj = oj; }
+        }
+        if (lane == 0) labels[base + pp] = bj;
+      }
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  if (tid == 0 && exact_ct) atomicAdd(nexact, exact_ct);
 }
 
 // Left-to-right sum of m FP64 values at stride st, in chunks of KM_CHUNK
@@ -569,7 +833,25 @@ static hipError_t launch_assign_x(Batch *b, int P, int nitems, const int2 *items
   return hipGetLastError();
 }
 
+static size_t bf16_lds(const Batch *b) {
+  const int Kp = (b->k + 31) / 32 * 32, Dp = b->d <= 16 ? 16 : 32;
+  return (size_t)Kp * Dp * 2 + (size_t)Kp * 4 + (size_t)b->k * b->d * 4 + (size_t)b->k * 4;
+}
+
 static hipError_t launch_assign(Batch *b, int P, int nitems, const int2 *items) {
+  if (b->opt.precision == KM_PREC_BF16) {
+    const size_t lds = bf16_lds(b);
+    const int grid = num_cus() * 4;
+    if (b->d <= 16)
+      hipLaunchKernelGGL(k_batch_assign_bf16<1>, dim3(grid), dim3(256), lds, b->stream, nitems, items,
+                         P, b->xoff, b->nvec, b->d, b->k, b->X, b->C, b->nc, b->labels, b->counters,
+                         b->nexact, FLT_MAX);
+    else
+      hipLaunchKernelGGL(k_batch_assign_bf16<2>, dim3(grid), dim3(256), lds, b->stream, nitems, items,
+                         P, b->xoff, b->nvec, b->d, b->k, b->X, b->C, b->nc, b->labels, b->counters,
+                         b->nexact, FLT_MAX);
+    return hipGetLastError();
+  }
   if (b->d == 16) return launch_assign_x<16>(b, P, nitems, items);
   if (b->d == 8) return launch_assign_x<8>(b, P, nitems, items);
   if (b->d == 32) return launch_assign_x<32>(b, P, nitems, items);
@@ -661,7 +943,7 @@ static std::string tune_key(const Batch *b) {
   while ((1L << lg_n) < b->total) ++lg_n;
   while ((1 << lg_b) < b->nb) ++lg_b;
   std::ostringstream s;
-  s << "batch_assign_f32 v2 d=" << b->d << " k=" << b->k << " lgN=" << lg_n << " lgB=" << lg_b;
+  s << "batch_assign " << (b->opt.precision == KM_PREC_BF16 ? "bf16" : "f32") << " v2 d=" << b->d << " k=" << b->k << " lgN=" << lg_n << " lgB=" << lg_b;
   return s.str();
 }
 
@@ -740,7 +1022,7 @@ static void batch_free(Batch *b) {
   batch_free_graph(b);
   void *bufs[] = {b->X_owned, b->xoff, b->nvec, b->C, b->nc, b->labels, b->perm, b->q,
                   b->part, b->comp, b->choff, b->empty, b->order, b->items, b->counters,
-                  b->gcnt, b->goff, b->gcpre, b->desc};
+                  b->gcnt, b->goff, b->gcpre, b->desc, b->nexact};
   for (void *p : bufs) hipFree(p);
   if (b->own_stream && b->stream) hipStreamDestroy(b->stream);
 }
@@ -750,7 +1032,12 @@ static kmeansErrorStatus batch_create(Batch *b, int nb, const long *n, int d, in
   if (nb <= 0 || !n || d <= 0 || d > B_MAXD || k <= 0 || k > B_MAXK)
     return KMEANS_ERROR_INVALID_ARGUMENT;
   if (o) b->opt = *o;
-  if (b->opt.precision != KM_PREC_FP32 || b->opt.proposer != KM_PROPOSER_NONE || b->opt.tile_rows)
+  if ((b->opt.precision != KM_PREC_FP32 && b->opt.precision != KM_PREC_BF16) ||
+      b->opt.proposer != KM_PROPOSER_NONE || b->opt.tile_rows)
+    return KMEANS_ERROR_INVALID_ARGUMENT;
+  b->d = d;
+  b->k = k;
+  if (b->opt.precision == KM_PREC_BF16 && (d > 32 || bf16_lds(b) > 64 * 1024))
     return KMEANS_ERROR_INVALID_ARGUMENT;
   const int dt = pick_dt(d);
   if ((size_t)k * (dt + 1) * sizeof(float) > 64 * 1024) return KMEANS_ERROR_INVALID_ARGUMENT;
@@ -808,6 +1095,8 @@ static kmeansErrorStatus batch_create(Batch *b, int nb, const long *n, int d, in
   BCHK(hipMalloc(&b->gcpre, sizeof(int) * (size_t)nb * (k + 1)), KMEANS_ERROR_BUILDDATA);
   b->slots = tch;
   BCHK(hipMalloc(&b->counters, sizeof(int) * 2), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMalloc(&b->nexact, sizeof(unsigned long long)), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMemset(b->nexact, 0, sizeof(unsigned long long)), KMEANS_ERROR_BUILDDATA);
   return KMEANS_SUCCESS;
 }
 
@@ -886,6 +1175,9 @@ static kmeansErrorStatus batch_run(Batch *b, int iters) {
   b->stats.work_items = b->nitems;
   b->stats.graph_nodes = 5 * iters;
   b->stats.last_run_iters = iters;
+  unsigned long long nx = 0;
+  BCHK(hipMemcpy(&nx, b->nexact, sizeof(nx), hipMemcpyDeviceToHost), KMEANS_ERROR_COMPUTE);
+  b->stats.exact_searched = (long long)nx;
   return KMEANS_SUCCESS;
 }
 

@@ -50,6 +50,7 @@ struct Learned {
   std::vector<int> gsize_h, gcol_h;
   float *CtP = nullptr, *ncP = nullptr, *Gc = nullptr, *gnh = nullptr, *rad2 = nullptr;
   unsigned *sep2 = nullptr;  // k x G: min squared distance from centre i to group g
+  int2 *sepTop = nullptr;    // k x (T+1): nearest groups of each centre, ascending
   float *ncmax = nullptr;
   // MLP
   MlpShape sh{};
@@ -59,6 +60,7 @@ struct Learned {
   void *mws = nullptr;
   size_t mws_bytes = 0;
   uint16_t *Xh = nullptr;  // m x din, scaled BF16 inputs
+  size_t *dpoff = nullptr;  // device copy of the parameter block offsets
   float xscale = 1.0f;
   long chunk = 0;
   uint16_t *H1 = nullptr, *H2 = nullptr;
@@ -74,6 +76,12 @@ struct Learned {
   int *hcnt = nullptr;                      // pinned: gcnt, goff
   double ms_train = 0.0, loss_first = 0.0, loss_last = 0.0;
   long long pairs = 0;                      // exact candidate scores issued
+  // grouped MFMA scoring (BF16 engines): group-ordered BF16 centres, biases,
+  // per-group wave prefix (pinned host copy + device)
+  uint16_t *ChP = nullptr;
+  float *biasP = nullptr;
+  int *wpre = nullptr, *hwpre = nullptr;
+  bool mfma_groups = false;
   // reduced-precision fallback: uncertified rows gathered contiguously
   long fb_cap = 0;
   float *fbX = nullptr, *fbN = nullptr;
@@ -95,25 +103,38 @@ __global__ void k_fill_i32(long n, int *p, int v) {
   if (i < n) p[i] = v;
 }
 
-// Wave-aggregated increment of a 64-bit counter (one atomic per wave).
-__device__ inline void wave_count(bool f, unsigned long long *c) {
+// Workgroup-aggregated counter increment and list append (256 threads, every
+// thread of the workgroup calls them): one atomic per workgroup.
+__device__ inline void block_count(bool f, unsigned long long *c) {
+  __shared__ unsigned bc[4];
   const uint64_t mk = __ballot(f);
-  const int lane = threadIdx.x & 63;
-  if (mk && lane == __ffsll((long long)mk) - 1) atomicAdd(c, (unsigned long long)__popcll(mk));
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) bc[w] = (unsigned)__popcll(mk);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned t = bc[0] + bc[1] + bc[2] + bc[3];
+    if (t) atomicAdd(c, (unsigned long long)t);
+  }
+  __syncthreads();
 }
 
-// Wave-aggregated append: returns this lane's slot (valid where f).
-__device__ inline int wave_slot(bool f, int *counter) {
+__device__ inline int block_slot(bool f, int *counter) {
+  __shared__ int bs[5];
   const uint64_t mk = __ballot(f);
-  const int lane = threadIdx.x & 63;
-  int base = 0;
-  if (mk) {
-    const int lead = __ffsll((long long)mk) - 1;
-    if (lane == lead) base = atomicAdd(counter, __popcll(mk));
-    base = __shfl(base, lead);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) bs[w] = __popcll(mk);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int t = bs[0] + bs[1] + bs[2] + bs[3];
+    bs[4] = t ? atomicAdd(counter, t) : 0;
   }
+  __syncthreads();
+  int base = bs[4];
+  for (int ww = 0; ww < w; ++ww) base += bs[ww];
   const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-  return base + __popcll(mk & below);
+  const int slot = base + __popcll(mk & below);
+  __syncthreads();
+  return slot;
 }
 
 // out[r][d] = BF16(scale * X[idx ? idx[r] : r][d]) for d < n, 0 for n <= d < din.
@@ -472,6 +493,36 @@ __global__ __launch_bounds__(256) void k_center_sep(int k, int n, int G, const f
   }
 }
 
+// Per centre: the NT smallest entries of its row of D (squared separation to
+// each group, float bits) with their groups, ascending, as (bits, group);
+// missing entries are (+inf, -1).  The nearest centre outside any set of at
+// most NT - 1 named groups is then the first listed group not named.
+__global__ void k_sep_top(int k, int G, int NT, const unsigned *__restrict__ D,
+                          int2 *__restrict__ top) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= k) return;
+  unsigned v[9];
+  int gi[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) { v[i] = 0x7f800000u; gi[i] = -1; }
+  const unsigned *row = D + (long)c * G;
+  for (int g = 0; g < G; ++g) {
+    unsigned x = row[g];
+    int xg = g;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      if (i < NT && x < v[i]) {
+        const unsigned tv = v[i];
+        const int tg = gi[i];
+        v[i] = x; gi[i] = xg; x = tv; xg = tg;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+    if (i < NT) top[(long)c * NT + i] = make_int2((int)v[i], gi[i]);
+}
+
 __global__ void k_ncmax(int k, const float *__restrict__ nc, float *__restrict__ out) {
   __shared__ float red[256];
   float mx = 0.0f;
@@ -499,8 +550,8 @@ __device__ inline double score_err(double qu, double ncm, double gam) {
 __global__ __launch_bounds__(256) void k_combine(
     long m, int T, const int *__restrict__ cand, const int *__restrict__ gsize,
     const int *__restrict__ gcol, const int *__restrict__ permP, const int *__restrict__ elab,
-    const float *__restrict__ escore, const double *__restrict__ q, int G,
-    const unsigned *__restrict__ sepD, const float *__restrict__ ncmax, BoundParams bp,
+    const float *__restrict__ escore, const double *__restrict__ q,
+    const int2 *__restrict__ sepTop, const float *__restrict__ ncmax, BoundParams bp,
     int *__restrict__ labels, int *__restrict__ ul_pt, int *__restrict__ ul_best,
     float *__restrict__ ul_score, int *__restrict__ ucnt, unsigned long long *__restrict__ cnt) {
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -509,26 +560,33 @@ __global__ __launch_bounds__(256) void k_combine(
   float bs = -INFINITY;
   bool cert = false;
   if (live) {
+    bool unsure = false;  // a named group whose best member was not certified
     for (int t = 0; t < T; ++t) {
       const long e = p * T + t;
       const int g = cand[e];
       if (g < 0 || gsize[g] == 0) continue;
-      const int j = permP[gcol[g] + elab[e]];
+      const int le = elab[e];
+      if (le < 0) { unsure = true; continue; }
+      const int j = permP[gcol[g] + le];
       const float s = escore[e];
       if (s > bs || (s == bs && j < bj)) { bs = s; bj = j; }
     }
+    if (unsure) bj = -1;
     if (bj >= 0 && bs > -INFINITY) {
       const double qu = q[p] * (1.0 + 1e-12);
       const double E = score_err(qu, (double)ncmax[0], bp.gam);
       const double a2 = qu - 2.0 * (double)bs + 2.0 * E;
       const double a = sqrt(fmax(a2, 0.0)) * (1.0 + 1e-14);
-      // nearest centre outside the named groups (their members were scored)
-      const unsigned *Dr = sepD + (long)bj * G;
+      // nearest centre outside the named groups (their members were scored):
+      // the first of bj's T+1 nearest groups that is not named
+      const int2 *tl = sepTop + (long)bj * (T + 1);
       float s2f = INFINITY;
-      for (int g = 0; g < G; ++g) {
+      for (int i = 0; i <= T; ++i) {
+        const int2 en = tl[i];
+        if (en.y < 0) break;
         bool named = false;
-        for (int t = 0; t < T; ++t) named |= cand[p * T + t] == g;
-        if (!named) s2f = fminf(s2f, __uint_as_float(Dr[g]));
+        for (int t = 0; t < T; ++t) named |= cand[p * T + t] == en.y;
+        if (!named) { s2f = __uint_as_float((unsigned)en.x); break; }
       }
       const double s2 = (double)s2f;
       const double h = s2f == INFINITY ? INFINITY : 0.5 * sqrt(s2 / (1.0 + bp.gam)) * (1.0 - 1e-14);
@@ -537,14 +595,14 @@ __global__ __launch_bounds__(256) void k_combine(
   }
   if (cert) labels[p] = bj;
   const bool unc = live && !cert;
-  const int slot = wave_slot(unc, ucnt);
+  const int slot = block_slot(unc, ucnt);
   if (unc) {
     ul_pt[slot] = (int)p;
     ul_best[slot] = bj;
     ul_score[slot] = bs;
   }
-  wave_count(cert, &cnt[0]);
-  wave_count(live, &cnt[3]);
+  block_count(cert, &cnt[0]);
+  block_count(live, &cnt[3]);
 }
 
 // Group bound for the uncertified points; the rest go to the fallback list.
@@ -592,10 +650,10 @@ __global__ __launch_bounds__(256) void k_group_bound(
     }
     if (cert) labels[p] = ul_best[i];
     const bool f = live && !cert;
-    const int slot = wave_slot(f, fbcnt);
+    const int slot = block_slot(f, fbcnt);
     if (f) fb[slot] = p;
-    wave_count(cert, &cnt[1]);
-    wave_count(f, &cnt[2]);
+    block_count(cert, &cnt[1]);
+    block_count(f, &cnt[2]);
   }
 }
 
@@ -608,7 +666,294 @@ __global__ void k_hits(long m, int T, const int *__restrict__ cand, const int *_
     const int g = gof[labels[p]];
     for (int t = 0; t < T; ++t) hit |= cand[p * T + t] == g;
   }
-  wave_count(hit, &cnt[4]);
+  block_count(hit, &cnt[4]);
+}
+
+// Group-ordered BF16 centre rows and biases for the grouped MFMA scoring:
+// row c of ChP is row permP[c] of Ch (zeros for padding).
+__global__ void k_gather_ch(int ldP, int Dp, const int *__restrict__ permP,
+                            const uint16_t *__restrict__ Ch, const float *__restrict__ bias,
+                            uint16_t *__restrict__ ChP, float *__restrict__ biasP) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;  // one 16-byte chunk
+  const int cpr = Dp / 8;
+  if (i >= (long)ldP * cpr) return;
+  const int c = (int)(i / cpr), q = (int)(i - (long)c * cpr);
+  const int j = permP[c];
+  typedef __attribute__((ext_vector_type(4))) unsigned int u4;
+  u4 v = {0u, 0u, 0u, 0u};
+  if (j >= 0) v = *reinterpret_cast<const u4 *>(Ch + (long)j * Dp + q * 8);
+  *reinterpret_cast<u4 *>(ChP + (long)c * Dp + q * 8) = v;
+  if (q == 0) biasP[c] = j >= 0 ? bias[j] : -FLT_MAX;
+}
+
+typedef __attribute__((ext_vector_type(8))) __bf16 lg_bf16x8;
+typedef __attribute__((ext_vector_type(16))) float lg_f32x16;
+typedef __attribute__((ext_vector_type(4))) unsigned int lg_u32x4;
+__device__ inline int lg_row(int r, int h) { return 8 * (r >> 2) + 4 * h + (r & 3); }
+
+// Grouped candidate scoring on BF16 MFMA.  One wave scores 32 entries of
+// one group (entries sorted by group; wave -> group by the prefix wpre of
+// ceil(count/32)) against every member of the group: BF16 scores with FP32
+// accumulation, a top-3 per entry, the exact FP32 chains of the first two
+// (half 0 and half 1 of the wave), and the within-group certificate: the
+// better exact score (lowest index on ties) exceeds the third approximate
+// score + E, E being the BF16 forward-error bound of the plain path's
+// certificate.  Writes the exact score and the member's local index, or -1
+// when the group's best member is not certified (its point is then searched
+// in full).
+template <int KS>
+__global__ __launch_bounds__(256) void k_group_mfma(
+    int G, const int *__restrict__ wpre, const int *__restrict__ goff,
+    const int *__restrict__ gcnt, const int *__restrict__ gsize, const int *__restrict__ gcol,
+    const int *__restrict__ permP, const int *__restrict__ ent_pt,
+    const int *__restrict__ ent_id, const uint16_t *__restrict__ Xh,
+    const float *__restrict__ X, int D, const uint16_t *__restrict__ ChP,
+    const float *__restrict__ biasP, const float *__restrict__ C, const float *__restrict__ nc,
+    const float *__restrict__ bnd, const float *__restrict__ xnorm, int *__restrict__ elab,
+    float *__restrict__ escore) {
+  constexpr int Dp = KS * 16;
+  const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31;
+  const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (wv >= wpre[G]) return;  // uniform per wave
+  int lo = 0, hi = G - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (wpre[mid] <= wv) lo = mid; else hi = mid - 1;
+  }
+  const int g = lo;
+  const int eb = goff[g], ee = eb + gcnt[g];
+  const int ei = eb + (wv - wpre[g]) * 32 + col;
+  const bool ok = ei < ee;
+  const long p = ok ? (long)ent_pt[ei] : 0;
+  lg_u32x4 bfr[KS];
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks)
+    bfr[ks] = ok ? *reinterpret_cast<const lg_u32x4 *>(Xh + p * Dp + ks * 16 + 8 * h)
+                 : lg_u32x4{0u, 0u, 0u, 0u};
+  const int sz = gsize[g], c0 = gcol[g];
+  float b1 = -INFINITY, b2 = -INFINITY, b3 = -INFINITY;
+  int i1 = -1, i2 = -1;
+  for (int t = 0; t < (sz + 31) / 32; ++t) {
+    lg_f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int lr = t * 32 + lg_row(r, h);
+      acc[r] = lr < sz ? biasP[c0 + lr] : -INFINITY;
+    }
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const lg_u32x4 a =
+          *reinterpret_cast<const lg_u32x4 *>(ChP + (long)(c0 + t * 32 + col) * Dp + ks * 16 + 8 * h);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(lg_bf16x8, a),
+                                                     __builtin_bit_cast(lg_bf16x8, bfr[ks]), acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const float v = acc[r];
+      const int vi = t * 32 + lg_row(r, h);
+      const bool g1 = v > b1, g2 = v > b2, g3 = v > b3;
+      b3 = g2 ? b2 : (g3 ? v : b3);
+      b2 = g1 ? b1 : (g2 ? v : b2);
+      i2 = g1 ? i1 : (g2 ? vi : i2);
+      b1 = g1 ? v : b1;
+      i1 = g1 ? vi : i1;
+    }
+  }
+  // merge the two halves (disjoint members), then use half 0's result
+  {
+    const float w1 = __shfl_xor(b1, 32), w2 = __shfl_xor(b2, 32), w3 = __shfl_xor(b3, 32);
+    const int j1 = __shfl_xor(i1, 32), j2 = __shfl_xor(i2, 32);
+    const bool A1 = b1 >= w1;
+    const float m1 = A1 ? b1 : w1;
+    const int k1 = A1 ? i1 : j1;
+    const float x2 = A1 ? b2 : b1, y2 = A1 ? w1 : w2;
+    const int xi2 = A1 ? i2 : i1, yi2 = A1 ? j1 : j2;
+    const bool A2 = x2 >= y2;
+    const float m2 = A2 ? x2 : y2;
+    const int k2 = A2 ? xi2 : yi2;
+    const float xa = A1 ? (A2 ? b3 : b2) : (A2 ? b2 : b1);
+    const float ya = A1 ? (A2 ? w1 : w2) : (A2 ? w2 : w3);
+    b1 = __shfl(m1, col);
+    b2 = __shfl(m2, col);
+    b3 = __shfl(fmaxf(xa, ya), col);
+    i1 = __shfl(k1, col);
+    i2 = __shfl(k2, col);
+  }
+  // exact chains: half 0 for the first candidate, half 1 for the second
+  const int lc = h ? i2 : i1;
+  const int jc = (ok && lc >= 0 && lc < sz) ? permP[c0 + lc] : -1;
+  float sx = -INFINITY;
+  if (jc >= 0) {
+    const float *xr = X + p * D;
+    const float *cr = C + (long)jc * D;
+    float sc = -0.5f * nc[jc];
+    if ((D & 3) == 0) {
+      const float4 *x4 = reinterpret_cast<const float4 *>(xr);
+      const float4 *c4 = reinterpret_cast<const float4 *>(cr);
+#pragma unroll 4
+      for (int q = 0; q < D / 4; ++q) {
+        const float4 xv = x4[q], cv = c4[q];
+        sc = __builtin_fmaf(xv.x, cv.x, sc);
+        sc = __builtin_fmaf(xv.y, cv.y, sc);
+        sc = __builtin_fmaf(xv.z, cv.z, sc);
+        sc = __builtin_fmaf(xv.w, cv.w, sc);
+      }
+    } else {
+      for (int q = 0; q < D; ++q) sc = __builtin_fmaf(xr[q], cr[q], sc);
+    }
+    sx = sc;
+  }
+  const float sy = __shfl_xor(sx, 32);
+  const int jy = __shfl_xor(jc, 32);
+  if (ok && h == 0) {
+    int lab = i1, jb = jc;
+    float sb = sx;
+    if (jy >= 0 && (sy > sb || (sy == sb && jy < jb))) { sb = sy; lab = i2; jb = jy; }
+    const float cmax = bnd[0], ncmax = bnd[1];
+    const float xn = xnorm[p], xc = xn * cmax, Df = (float)D;
+    const float ub = 3.90625e-3f;                                  // 2^-8
+    const float g32 = Df * 5.9604645e-8f * 1.0001f;                // gamma_D, u = 2^-24
+    const float gmf = (Df + 1.0f) * 2.3841858e-7f * 1.001f;        // gamma'_{D+1}, u = 2^-22
+    const float rnd = (2.0f * ub + ub * ub) * xc;
+    const float mfe = gmf * (0.5f * ncmax + (1.0f + ub) * (1.0f + ub) * xc);
+    const float fpe = g32 * (0.5f * ncmax + xc);
+    const float E = (rnd + mfe + fpe) * 1.001f + 1e-30f;
+    const bool has3 = b3 > -FLT_MAX * 0.5f;
+    const float slack = (fabsf(b1) + fabsf(b2) + (has3 ? fabsf(b3) : 0.0f)) * 1.6e-5f;
+    const float thr = has3 ? b3 + E + slack : -INFINITY;
+    const bool cert = jb >= 0 && sb > -INFINITY && sb > thr;
+    const int slot = ent_id[ei];
+    elab[slot] = cert ? lab : -1;
+    escore[slot] = sb;
+  }
+}
+
+// Fused MLP inference and top-T proposal.  One workgroup (4 waves) takes 64
+// rows of the scaled BF16 inputs: layer 1 (B fragments straight from the
+// inputs), layer 2 and layer 3 on v_mfma_f32_32x32x16_bf16 with the hidden
+// activations kept in LDS as BF16 (row-major, padded rows), weights read from
+// the (L2-resident) BF16 parameter copy, bias as the accumulator input, ReLU
+// on the way into LDS.  Layer 3 is done by waves 0 and 1 (32 rows each, all G
+// logits): every lane keeps the top T of its half of the logits, the halves
+// are merged, and only the T group ids are written.  Requires din, H and G
+// multiples of 32 (din of 16), G <= 256, T <= 8.
+#define MLPF_ROWS 64
+__device__ inline void mlpf_store4(uint16_t *row, int n0, const float *v) {
+  // four consecutive neurons n0..n0+3 as BF16, ReLU applied
+  const uint32_t a = (uint32_t)mlp_bf16(fmaxf(v[0], 0.0f)) | ((uint32_t)mlp_bf16(fmaxf(v[1], 0.0f)) << 16);
+  const uint32_t b = (uint32_t)mlp_bf16(fmaxf(v[2], 0.0f)) | ((uint32_t)mlp_bf16(fmaxf(v[3], 0.0f)) << 16);
+  *reinterpret_cast<uint2 *>(row + n0) = make_uint2(a, b);
+}
+
+__global__ __launch_bounds__(256) void k_mlp_propose(long rows, int din, int H, int G, int T,
+                                                     const uint16_t *__restrict__ Xh,
+                                                     const uint16_t *__restrict__ Ph,
+                                                     const float *__restrict__ P,
+                                                     const size_t *__restrict__ poff,
+                                                     int *__restrict__ cand) {
+  extern __shared__ uint16_t hs[];  // H1 then H2: MLPF_ROWS x (H + 8) each
+  const int ldh = H + 8;
+  uint16_t *H1 = hs, *H2 = hs + MLPF_ROWS * ldh;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, h = lane >> 5, col = lane & 31;
+  const long r0 = (long)blockIdx.x * MLPF_ROWS;
+  const uint16_t *W1 = Ph + poff[0], *W2 = Ph + poff[2], *W3 = Ph + poff[4];
+  const float *b1 = P + poff[1], *b2 = P + poff[3], *b3 = P + poff[5];
+  const int rt = w & 1;            // row tile of this wave (layers 1, 2)
+  const long prow = r0 + rt * 32 + col;
+  const bool pok = prow < rows;
+  const int NTW = H / 32 / 2;      // neuron tiles per wave (layers 1, 2)
+  const int nt0 = (w >> 1) * NTW;
+  // ---- layer 1: H1 = relu(X W1^T + b1)
+  for (int nt = nt0; nt < nt0 + NTW; ++nt) {
+    lg_f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = b1[nt * 32 + lg_row(r, h)];
+    for (int ks = 0; ks < din / 16; ++ks) {
+      const lg_u32x4 a = *reinterpret_cast<const lg_u32x4 *>(W1 + (long)(nt * 32 + col) * din + ks * 16 + 8 * h);
+      const lg_u32x4 b = pok ? *reinterpret_cast<const lg_u32x4 *>(Xh + prow * din + ks * 16 + 8 * h)
+                             : lg_u32x4{0u, 0u, 0u, 0u};
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(lg_bf16x8, a),
+                                                     __builtin_bit_cast(lg_bf16x8, b), acc, 0, 0, 0);
+    }
+    uint16_t *dst = H1 + (rt * 32 + col) * ldh;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float v[4] = {acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]};
+      mlpf_store4(dst, nt * 32 + 8 * q + 4 * h, v);
+    }
+  }
+  __syncthreads();
+  // ---- layer 2: H2 = relu(H1 W2^T + b2)
+  for (int nt = nt0; nt < nt0 + NTW; ++nt) {
+    lg_f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = b2[nt * 32 + lg_row(r, h)];
+    const uint16_t *src = H1 + (rt * 32 + col) * ldh;
+    for (int ks = 0; ks < H / 16; ++ks) {
+      const lg_u32x4 a = *reinterpret_cast<const lg_u32x4 *>(W2 + (long)(nt * 32 + col) * H + ks * 16 + 8 * h);
+      const lg_u32x4 b = *reinterpret_cast<const lg_u32x4 *>(src + ks * 16 + 8 * h);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(lg_bf16x8, a),
+                                                     __builtin_bit_cast(lg_bf16x8, b), acc, 0, 0, 0);
+    }
+    uint16_t *dst = H2 + (rt * 32 + col) * ldh;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float v[4] = {acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]};
+      mlpf_store4(dst, nt * 32 + 8 * q + 4 * h, v);
+    }
+  }
+  __syncthreads();
+  // ---- layer 3 + top-T: waves 0 and 1, one row tile each
+  if (w >= 2) return;
+  float bv[8];
+  int bi[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) { bv[t] = -INFINITY; bi[t] = 0x7fffffff; }
+  const uint16_t *src = H2 + (rt * 32 + col) * ldh;
+  for (int nt = 0; nt < G / 32; ++nt) {
+    lg_f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = b3[nt * 32 + lg_row(r, h)];
+    for (int ks = 0; ks < H / 16; ++ks) {
+      const lg_u32x4 a = *reinterpret_cast<const lg_u32x4 *>(W3 + (long)(nt * 32 + col) * H + ks * 16 + 8 * h);
+      const lg_u32x4 b = *reinterpret_cast<const lg_u32x4 *>(src + ks * 16 + 8 * h);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(lg_bf16x8, a),
+                                                     __builtin_bit_cast(lg_bf16x8, b), acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      float v = acc[r];
+      int gi = nt * 32 + lg_row(r, h);
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        if (t < T && (v > bv[t] || (v == bv[t] && gi < bi[t]))) {
+          const float tv = bv[t];
+          const int ti = bi[t];
+          bv[t] = v; bi[t] = gi; v = tv; gi = ti;
+        }
+      }
+    }
+  }
+  // merge the other half's list (disjoint groups)
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    float v = __shfl_xor(bv[q], 32);
+    int gi = __shfl_xor(bi[q], 32);
+    if (q >= T) continue;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      if (t < T && (v > bv[t] || (v == bv[t] && gi < bi[t]))) {
+        const float tv = bv[t];
+        const int ti = bi[t];
+        bv[t] = v; bi[t] = gi; v = tv; gi = ti;
+      }
+    }
+  }
+  if (h == 0 && pok) {
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+      if (t < T) cand[prow * T + t] = (bv[t] == -INFINITY || bi[t] == 0x7fffffff) ? -1 : bi[t];
+  }
 }
 
 // Uncertified rows gathered for the reduced-precision full search: FP32
@@ -682,10 +1027,11 @@ void learned_free(Engine *e) {
                   L->ncmax, L->P, L->Gr, L->Am, L->Av, L->loss, L->Ph, L->mws, L->Xh, L->H1,
                   L->H2, L->Z, L->cand, L->gcnt, L->ent_pt, L->ent_id, L->elab, L->escore,
                   L->ul_pt, L->ul_best, L->ucnt, L->fb, L->ul_score, L->cnt, L->fbX, L->fbN,
-                  L->fbXh, L->fbLab};  // goff, gcur, fbcnt
+                  L->fbXh, L->fbLab, L->ChP, L->biasP, L->wpre, L->sepTop, L->dpoff};  // goff, gcur, fbcnt
                                                                                // live inside gcnt, ucnt
   for (void *b : bufs) hipFree(b);
   if (L->hcnt) hipHostFree(L->hcnt);
+  if (L->hwpre) hipHostFree(L->hwpre);
   delete L;
   e->learned = nullptr;
 }
@@ -772,6 +1118,7 @@ static kmeansErrorStatus build_groups(Engine *e, Learned *L) {
   KM_HIP_CHECK(lmalloc((void **)&L->gnh, sizeof(float) * G), KMEANS_ERROR_COMPUTE);
   KM_HIP_CHECK(lmalloc((void **)&L->rad2, sizeof(float) * G), KMEANS_ERROR_COMPUTE);
   KM_HIP_CHECK(lmalloc((void **)&L->sep2, sizeof(unsigned) * (size_t)k * G), KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(lmalloc((void **)&L->sepTop, sizeof(int2) * (size_t)k * (L->T + 1)), KMEANS_ERROR_COMPUTE);
   KM_HIP_CHECK(lmalloc((void **)&L->ncmax, sizeof(float)), KMEANS_ERROR_COMPUTE);
   return KMEANS_SUCCESS;
 }
@@ -848,6 +1195,12 @@ static kmeansErrorStatus train_mlp(Engine *e, Learned *L) {
   KM_HIP_CHECK(lmalloc((void **)&L->Xh, sizeof(uint16_t) * (size_t)e->m * din), KMEANS_ERROR_COMPUTE);
   hipLaunchKernelGGL(k_x_bf16, dim3(g1(e->m * din)), dim3(256), 0, s, e->m, e->n, din, e->dX, nullptr,
                      L->xscale, L->Xh);
+  {
+    size_t off[6];
+    mlp_offsets(L->sh, off);
+    KM_HIP_CHECK(lmalloc((void **)&L->dpoff, sizeof(off)), KMEANS_ERROR_COMPUTE);
+    KM_HIP_CHECK(hipMemcpy(L->dpoff, off, sizeof(off), hipMemcpyHostToDevice), KMEANS_ERROR_COMPUTE);
+  }
   L->chunk = std::min<long>(e->m, 1L << 20);
   KM_HIP_CHECK(lmalloc((void **)&L->H1, sizeof(uint16_t) * L->chunk * L->H), KMEANS_ERROR_COMPUTE);
   KM_HIP_CHECK(lmalloc((void **)&L->H2, sizeof(uint16_t) * L->chunk * L->H), KMEANS_ERROR_COMPUTE);
@@ -887,6 +1240,17 @@ kmeansErrorStatus learned_prepare(Engine *e) {
   KM_HIP_CHECK(lmalloc((void **)&L->cnt, sizeof(unsigned long long) * 5), KMEANS_ERROR_COMPUTE);
   KM_HIP_CHECK(hipMemsetAsync(L->cnt, 0, sizeof(unsigned long long) * 5, e->stream), KMEANS_ERROR_COMPUTE);
   KM_HIP_CHECK(hipHostMalloc((void **)&L->hcnt, sizeof(int) * 2 * L->G), KMEANS_ERROR_COMPUTE);
+  {
+    MfmaView mv;
+    L->mfma_groups = e->opt.precision == KM_PREC_BF16 && mfma_view(e, &mv) && !mv.fp16;
+    if (L->mfma_groups) {
+      KM_HIP_CHECK(lmalloc((void **)&L->ChP, sizeof(uint16_t) * (size_t)L->ldP * mv.Dp),
+                   KMEANS_ERROR_COMPUTE);
+      KM_HIP_CHECK(lmalloc((void **)&L->biasP, sizeof(float) * (size_t)L->ldP), KMEANS_ERROR_COMPUTE);
+      KM_HIP_CHECK(lmalloc((void **)&L->wpre, sizeof(int) * (L->G + 1)), KMEANS_ERROR_COMPUTE);
+      KM_HIP_CHECK(hipHostMalloc((void **)&L->hwpre, sizeof(int) * (L->G + 1)), KMEANS_ERROR_COMPUTE);
+    }
+  }
   KM_HIP_CHECK(hipStreamSynchronize(e->stream), KMEANS_ERROR_COMPUTE);
   L->ms_train = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   L->ready = true;
@@ -907,13 +1271,34 @@ static hipError_t refresh_centres(Engine *e, Learned *L) {
   dim3 sg((L->ldP + SEP_T - 1) / SEP_T, (k + SEP_T - 1) / SEP_T);
   hipLaunchKernelGGL(k_center_sep, sg, dim3(256), 0, s, k, n, L->G, e->dC, L->CtP, L->ldP, L->permP,
                      L->gof, L->sep2);
+  hipLaunchKernelGGL(k_sep_top, dim3(g1(k)), dim3(256), 0, s, k, L->G, L->T + 1, L->sep2, L->sepTop);
   hipLaunchKernelGGL(k_ncmax, dim3(1), dim3(256), 0, s, k, e->dnc, L->ncmax);
+  if (L->mfma_groups) {
+    MfmaView mv;
+    if (!mfma_view(e, &mv)) return hipErrorNotReady;
+    const long chunks = (long)L->ldP * (mv.Dp / 8);
+    hipLaunchKernelGGL(k_gather_ch, dim3(g1(chunks)), dim3(256), 0, s, L->ldP, mv.Dp, L->permP, mv.Ch,
+                       mv.bias, L->ChP, L->biasP);
+  }
   return hipGetLastError();
 }
 
 static hipError_t propose(Engine *e, Learned *L) {
   hipStream_t s = e->stream;
   const long m = e->m;
+  if (L->proposer == KM_PROPOSER_MLP && L->H % 64 == 0 && L->G % 32 == 0 && L->G <= 256) {
+    // fused inference: no hidden or logit matrices in HBM
+    const size_t lds = sizeof(uint16_t) * 2 * MLPF_ROWS * (L->H + 8);
+    static size_t attr = 0;
+    if (lds > 64 * 1024 && lds > attr) {
+      LCHK(hipFuncSetAttribute((const void *)k_mlp_propose, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds));
+      attr = lds;
+    }
+    hipLaunchKernelGGL(k_mlp_propose, dim3((unsigned)((m + MLPF_ROWS - 1) / MLPF_ROWS)), dim3(256), lds,
+                       s, m, L->din, L->H, L->G, L->T, L->Xh, L->Ph, L->P, L->dpoff, L->cand);
+    return hipGetLastError();
+  }
   if (L->proposer == KM_PROPOSER_MLP) {
     for (long r0 = 0; r0 < m; r0 += L->chunk) {
       const long R = std::min(L->chunk, m - r0);
@@ -962,19 +1347,47 @@ hipError_t learned_assign(Engine *e) {
                      sizeof(int) * 2 * G, s, ne, L->T, L->cand, G, L->gcur, L->ent_pt, L->ent_id);
   LCHK(hipMemcpyAsync(L->hcnt, L->gcnt, sizeof(int) * 2 * G, hipMemcpyDeviceToHost, s));
   LCHK(hipStreamSynchronize(s));
-  // exact canonical scores within each named group
-  for (int g = 0; g < G; ++g) {
-    const int c = L->hcnt[g], off = L->hcnt[G + g];
-    if (c == 0 || L->gsize_h[g] == 0) continue;
-    L->pairs += (long long)c * L->gsize_h[g];
-    LCHK(launch_assign_f32(s, c, nullptr, e->n, e->dX, L->ent_pt + off, L->gsize_h[g],
-                           L->CtP + L->gcol_h[g], L->ldP, L->ncP + L->gcol_h[g], L->elab,
-                           L->escore, L->ent_id + off));
+  for (int g = 0; g < G; ++g)
+    if (L->gsize_h[g] > 0) L->pairs += (long long)L->hcnt[g] * L->gsize_h[g];
+  if (L->mfma_groups) {
+    // BF16 MFMA scores of every named group, certified per entry
+    MfmaView mv;
+    if (!mfma_view(e, &mv)) return hipErrorNotReady;
+    L->hwpre[0] = 0;
+    for (int g = 0; g < G; ++g)
+      L->hwpre[g + 1] = L->hwpre[g] + (L->gsize_h[g] > 0 ? (L->hcnt[g] + 31) / 32 : 0);
+    const int waves = L->hwpre[G];
+    LCHK(hipMemcpyAsync(L->wpre, L->hwpre, sizeof(int) * (G + 1), hipMemcpyHostToDevice, s));
+    if (waves > 0) {
+#define GMF(KS)                                                                                    \
+  hipLaunchKernelGGL(k_group_mfma<KS>, dim3((waves + 3) / 4), dim3(256), 0, s, G, L->wpre, L->goff, \
+                     L->gcnt, L->gsize, L->gcol, L->permP, L->ent_pt, L->ent_id, mv.Xh, e->dX,      \
+                     e->n, L->ChP, L->biasP, e->dC, e->dnc, mv.bnd, mv.xnorm, L->elab, L->escore)
+      switch (mv.KS) {
+        case 1: GMF(1); break;
+        case 2: GMF(2); break;
+        case 4: GMF(4); break;
+        case 6: GMF(6); break;
+        case 8: GMF(8); break;
+        default: GMF(16); break;
+      }
+#undef GMF
+      LCHK(hipGetLastError());
+    }
+  } else {
+    // exact canonical scores within each named group
+    for (int g = 0; g < G; ++g) {
+      const int c = L->hcnt[g], off = L->hcnt[G + g];
+      if (c == 0 || L->gsize_h[g] == 0) continue;
+      LCHK(launch_assign_f32(s, c, nullptr, e->n, e->dX, L->ent_pt + off, L->gsize_h[g],
+                             L->CtP + L->gcol_h[g], L->ldP, L->ncP + L->gcol_h[g], L->elab,
+                             L->escore, L->ent_id + off));
+    }
   }
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, 2L, L->ucnt, 0);
   BoundParams bp{gamma_n(e->n)};
   hipLaunchKernelGGL(k_combine, dim3(g1(m)), dim3(256), 0, s, m, L->T, L->cand, L->gsize, L->gcol,
-                     L->permP, L->elab, L->escore, e->dq, G, L->sep2, L->ncmax, bp, e->dlabels,
+                     L->permP, L->elab, L->escore, e->dq, L->sepTop, L->ncmax, bp, e->dlabels,
                      L->ul_pt, L->ul_best, L->ul_score, L->ucnt, L->cnt);
   hipLaunchKernelGGL(k_group_bound, dim3((unsigned)std::min<long>(g1(m), 4L * num_cus())), dim3(256),
                      0, s, L->ucnt, e->n, e->dX, L->T, L->cand, G, L->Gc, L->rad2, L->gsize, e->dq,

@@ -962,6 +962,12 @@ hipError_t mfma_reresolve_view(Engine *e, const float *X, const uint16_t *Xh, co
 const uint16_t *mfma_tile_xh(Engine *e) { return st(e)->Xh; }
 const float *mfma_tile_xnorm(Engine *e) { return st(e)->xnorm; }
 int mfma_row_stride(Engine *e) { return st(e)->Dp; }
+bool mfma_view(Engine *e, MfmaView *v) {
+  MfmaState *s = st(e);
+  if (!s || !s->x_ready || !s->c_ready || e->streaming) return false;
+  *v = MfmaView{s->Xh, s->Ch, s->bias, s->bnd, s->xnorm, s->Dp, s->KS, s->Kp, s->fp16};
+  return true;
+}
 bool mfma_ready(Engine *e) { return st(e) && st(e)->x_ready && !e->streaming; }
 
 void mfma_collect(Engine *e, int nslots) {

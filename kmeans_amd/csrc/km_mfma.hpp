@@ -21,5 +21,13 @@ hipError_t mfma_reresolve_view(Engine *e, const float *X, const uint16_t *Xh, co
 const uint16_t *mfma_tile_xh(Engine *e);
 const float *mfma_tile_xnorm(Engine *e);
 int mfma_row_stride(Engine *e);  // padded width Dp of the reduced-precision rows
+// Read-only view of the reduced-precision state (valid after mfma_prepare).
+struct MfmaView {
+  const uint16_t *Xh, *Ch;
+  const float *bias, *bnd, *xnorm;
+  int Dp, KS, Kp;
+  bool fp16;
+};
+bool mfma_view(Engine *e, MfmaView *v);
 bool mfma_ready(Engine *e);      // reduced-precision copy of every point present
 }  // namespace km

@@ -104,8 +104,10 @@ def test_batch_create_rejects_bad_shapes_without_device():
     assert L.kmeansBatchCreate(ctypes.byref(h), 3, p, 65, 16, None) == 14    # d > 64
     assert L.kmeansBatchCreate(ctypes.byref(h), 3, p, 16, 2000, None) == 14  # k > 1024
     assert L.kmeansBatchCreate(ctypes.byref(h), 0, p, 16, 16, None) == 14
-    opt = kmeans_amd.Options(1, 0, None, 0, 0)  # reduced precision: not offered
+    opt = kmeans_amd.Options(2, 0, None, 0, 0)  # FP16: not offered
     assert L.kmeansBatchCreate(ctypes.byref(h), 3, p, 16, 16, ctypes.byref(opt)) == 14
+    opt = kmeans_amd.Options(1, 0, None, 0, 0)  # BF16: d <= 32 only
+    assert L.kmeansBatchCreate(ctypes.byref(h), 3, p, 33, 16, ctypes.byref(opt)) == 14
 
 
 def test_c4_instance_sizes():

@@ -41,32 +41,35 @@ def _check(b, n, Xs, ref, iters_last):
         assert np.array_equal(comp[:, i], r["comp_hist"][-iters_last:]), i
 
 
-@pytest.mark.parametrize("d,k,median,lo,hi", [(16, 32, 3000, 300, 20000), (5, 7, 400, 40, 3000),
-                                              (20, 64, 1500, 200, 9000)])
-def test_batch_equals_per_instance_oracle(d, k, median, lo, hi):
+@pytest.mark.parametrize("precision,d,k,median,lo,hi", [
+    ("fp32", 16, 32, 3000, 300, 20000), ("fp32", 5, 7, 400, 40, 3000), ("fp32", 20, 64, 1500, 200, 9000),
+    ("bf16", 16, 32, 3000, 300, 20000), ("bf16", 5, 7, 400, 40, 3000), ("bf16", 20, 64, 1500, 200, 9000),
+    ("bf16", 16, 256, 3000, 300, 20000)])
+def test_batch_equals_per_instance_oracle(precision, d, k, median, lo, hi):
     nb, iters = 24, 6
     n, Xs = _instances(nb, d, k, median, lo, hi)
     ref = _oracle(Xs, k, iters)
-    b = km.KMeansBatch(n, d, k)
+    b = km.KMeansBatch(n, d, k, precision=precision)
     b.set_data(np.concatenate(Xs))
     b.init_first_k()
     b.run(iters)
     st = b.stats()
-    print(f"\nd={d} k={k}: {st}")
+    print(f"\n{precision} d={d} k={k}: {st}")
     assert st["tile_rows"] in (256, 512, 1024, 2048, 4096)
     assert st["graph_nodes"] == 5 * iters
     _check(b, n, Xs, ref, iters)
     b.close()
 
 
-def test_batch_graph_replays_and_tile_sizes():
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_batch_graph_replays_and_tile_sizes(precision):
     """Three replays of one captured 3-iteration graph equal 9 eager oracle
     iterations; every explicit tile size gives the same result."""
     nb, d, k = 16, 16, 32
     n, Xs = _instances(nb, d, k, 2500, 300, 12000, seed=9)
     ref = _oracle(Xs, k, 9)
     for tile in (0, 256, 2048):
-        b = km.KMeansBatch(n, d, k)
+        b = km.KMeansBatch(n, d, k, precision=precision)
         b.set_data(np.concatenate(Xs))
         b.init_first_k()
         if tile:

@@ -167,16 +167,26 @@ def test_learned_group_bound_certifies_separated_groups():
 
 
 def test_learned_mlp_gain_over_centroid_proposer():
-    """What the trained network gains over the method without it: fewer points
-    left for the full search than the nearest-centroid proposer, at the same
-    groups and the same number of proposed groups."""
-    X, C0 = _c5_like()
-    iters = 7
+    """What the trained network gains over the method without it.  Against
+    the nearest-centroid proposer (same groups, same number of proposed
+    groups): the MLP names the group of the final label for more points, more
+    points are certified and fewer are searched in full.  Against the full
+    search: a fraction of the exact scores, with the same labels (the parity
+    tests above).  Data: 200,000 points, D = 64, K = 2048 means uniform in
+    [-6, 6]^64, sigma uniform in [0.3, 1.5], 32 groups, top 4, BF16 engine."""
+    X, _, _, _ = orc.mixture(3, 200_000, 64, 2048, -6.0, 6.0, 0.3, 1.5)
+    C0 = X[orc.sample_init(3, X.shape[0], 2048)].copy()
     res = {}
     for prop in ("mlp", "centroid"):
-        _, _, st = _run(X, C0, iters, proposer=prop, groups=32, top_groups=4, train_iters=3,
-                        hidden=128)
+        _, _, st = _run(X, C0, 6, precision="bf16", proposer=prop, groups=32, top_groups=4,
+                        train_iters=3, hidden=128)
         res[prop] = st
-    print("\ngain:", {p: (s["certified"], s["certified_group"], s["fallback"], s["candidate_pairs"])
+    m, c = res["mlp"], res["centroid"]
+    n = m["proposed"]
+    print("\ngain:", {p: {f: s[f] / s["proposed"] for f in ("proposal_hits", "certified", "fallback")}
                       for p, s in res.items()})
-    assert res["mlp"]["fallback"] <= res["centroid"]["fallback"]
+    assert c["proposed"] == n
+    assert m["proposal_hits"] >= c["proposal_hits"] + 0.02 * n
+    assert m["certified"] > c["certified"]
+    assert m["fallback"] < c["fallback"]
+    assert m["candidate_pairs"] < 0.25 * n * 2048
