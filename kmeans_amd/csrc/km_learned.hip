@@ -710,19 +710,36 @@ __global__ __launch_bounds__(256) void k_group_mfma(
     const float *__restrict__ X, int D, const uint16_t *__restrict__ ChP,
     const float *__restrict__ biasP, const float *__restrict__ C, const float *__restrict__ nc,
     const float *__restrict__ bnd, const float *__restrict__ xnorm, int *__restrict__ elab,
-    float *__restrict__ escore) {
+    float *__restrict__ escore, float big) {
   constexpr int Dp = KS * 16;
-  const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31;
-  const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (wv >= wpre[G]) return;  // uniform per wave
-  int lo = 0, hi = G - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (wpre[mid] <= wv) lo = mid; else hi = mid - 1;
+  constexpr int CPR = Dp / 8;                    // 16-byte chunks per centre row
+  constexpr int ROTM = CPR >= 16 ? 15 : 7;       // LDS swizzle rotation mask
+  __shared__ lg_u32x4 ts[2][32 * CPR];           // two centre tiles (double buffer)
+  __shared__ float tb[2][32];
+  __shared__ int wg[4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, h = lane >> 5, col = lane & 31;
+  const int wv = blockIdx.x * 4 + w;
+  const int nw = wpre[G];
+  // group of this wave (-1 past the end)
+  int g = -1;
+  if (wv < nw) {
+    int lo = 0, hi = G - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (wpre[mid] <= wv) lo = mid; else hi = mid - 1;
+    }
+    g = lo;
   }
-  const int g = lo;
-  const int eb = goff[g], ee = eb + gcnt[g];
-  const int ei = eb + (wv - wpre[g]) * 32 + col;
+  if (lane == 0) wg[w] = g;
+  __syncthreads();
+  // the workgroup shares LDS tiles when all its live waves score one group
+  const int g0 = wg[0];
+  const bool shared = g0 >= 0 && (wg[1] == g0 || wg[1] < 0) && (wg[2] == g0 || wg[2] < 0) &&
+                      (wg[3] == g0 || wg[3] < 0);
+  if (g < 0 && !shared) return;  // idle wave of a mixed workgroup (no barriers follow)
+  const int gg = shared ? g0 : g;
+  const int eb = goff[gg], ee = eb + gcnt[gg];
+  const int ei = g >= 0 ? eb + (wv - wpre[gg]) * 32 + col : ee;
   const bool ok = ei < ee;
   const long p = ok ? (long)ent_pt[ei] : 0;
   lg_u32x4 bfr[KS];
@@ -730,36 +747,83 @@ __global__ __launch_bounds__(256) void k_group_mfma(
   for (int ks = 0; ks < KS; ++ks)
     bfr[ks] = ok ? *reinterpret_cast<const lg_u32x4 *>(Xh + p * Dp + ks * 16 + 8 * h)
                  : lg_u32x4{0u, 0u, 0u, 0u};
-  const int sz = gsize[g], c0 = gcol[g];
-  float b1 = -INFINITY, b2 = -INFINITY, b3 = -INFINITY;
-  int i1 = -1, i2 = -1;
-  for (int t = 0; t < (sz + 31) / 32; ++t) {
-    lg_f32x16 acc;
+  const int sz = gsize[gg], c0 = gcol[gg];
+  const int nt = (sz + 31) / 32;
+  // top 3 of packed scores (register id in the low 4 bits) and tiles of the first two
+  float b1 = -FLT_MAX, b2 = -FLT_MAX, b3 = -FLT_MAX;
+  int t1 = 0, t2 = 0;
+  auto epilogue = [&](int t, const lg_f32x16 &acc) {
+    const float o1 = b1, o2 = b2;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const float pv = __uint_as_float((__float_as_uint(fmaxf(acc[r], -FLT_MAX)) & 0xFFFFFFF0u) | (unsigned)r);
+      const float n3 = __builtin_amdgcn_fmed3f(b2, b3, pv);
+      const float n2 = __builtin_amdgcn_fmed3f(b1, b2, pv);
+      b1 = __builtin_amdgcn_fmed3f(b1, pv, big);
+      b2 = n2;
+      b3 = n3;
+    }
+    const unsigned u1 = __float_as_uint(b1), u2 = __float_as_uint(b2);
+    const int ot1 = t1, ot2 = t2;
+    t2 = (u2 == __float_as_uint(o1)) ? ot1 : (u2 == __float_as_uint(o2)) ? ot2 : t;
+    t1 = (u1 != __float_as_uint(o1)) ? t : ot1;
+  };
+  auto tile_bias = [&](int t, const float *bsrc, lg_f32x16 &acc) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int lr = t * 32 + lg_row(r, h);
-      acc[r] = lr < sz ? biasP[c0 + lr] : -INFINITY;
+      acc[r] = lr < sz ? bsrc[lg_row(r, h)] : -INFINITY;
     }
+  };
+  if (shared) {
+    // stage tile t into buffer t & 1 (rows past the group's end read the
+    // padding/next group's rows, masked by the bias)
+    auto stage = [&](int t) {
+      const int b = t & 1;
+      for (int c = threadIdx.x; c < 32 * CPR; c += 256) {
+        const int r = c / CPR, q = c - r * CPR;
+        ts[b][r * CPR + (q + (r & ROTM)) % CPR] =
+            *reinterpret_cast<const lg_u32x4 *>(ChP + (long)(c0 + t * 32 + r) * Dp + q * 8);
+      }
+      if (threadIdx.x < 32) tb[b][threadIdx.x] = biasP[c0 + t * 32 + threadIdx.x];
+    };
+    stage(0);
+    for (int t = 0; t < nt; ++t) {
+      __syncthreads();
+      if (t + 1 < nt) stage(t + 1);
+      const int b = t & 1;
+      lg_f32x16 acc;
+      tile_bias(t, tb[b], acc);
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks) {
-      const lg_u32x4 a =
-          *reinterpret_cast<const lg_u32x4 *>(ChP + (long)(c0 + t * 32 + col) * Dp + ks * 16 + 8 * h);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(lg_bf16x8, a),
-                                                     __builtin_bit_cast(lg_bf16x8, bfr[ks]), acc, 0, 0, 0);
+      for (int ks = 0; ks < KS; ++ks) {
+        const int q = ks * 2 + h;
+        const lg_u32x4 a = ts[b][col * CPR + (q + (col & ROTM)) % CPR];
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(lg_bf16x8, a),
+                                                       __builtin_bit_cast(lg_bf16x8, bfr[ks]), acc, 0, 0, 0);
+      }
+      epilogue(t, acc);
     }
+  } else {
+    for (int t = 0; t < nt; ++t) {
+      lg_f32x16 acc;
+      tile_bias(t, biasP + c0 + t * 32, acc);
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const float v = acc[r];
-      const int vi = t * 32 + lg_row(r, h);
-      const bool g1 = v > b1, g2 = v > b2, g3 = v > b3;
-      b3 = g2 ? b2 : (g3 ? v : b3);
-      b2 = g1 ? b1 : (g2 ? v : b2);
-      i2 = g1 ? i1 : (g2 ? vi : i2);
-      b1 = g1 ? v : b1;
-      i1 = g1 ? vi : i1;
+      for (int ks = 0; ks < KS; ++ks) {
+        const lg_u32x4 a =
+            *reinterpret_cast<const lg_u32x4 *>(ChP + (long)(c0 + t * 32 + col) * Dp + ks * 16 + 8 * h);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(lg_bf16x8, a),
+                                                       __builtin_bit_cast(lg_bf16x8, bfr[ks]), acc, 0, 0, 0);
+      }
+      epilogue(t, acc);
     }
   }
-  // merge the two halves (disjoint members), then use half 0's result
+  if (g < 0) return;  // idle wave of a shared workgroup (after the barriers)
+  // local indices of the first two; bitwise-equal packed values are ties
+  const bool real1 = b1 > -FLT_MAX, real2 = b2 > -FLT_MAX;
+  int i1 = real1 ? t1 * 32 + lg_row((int)(__float_as_uint(b1) & 15u), h) : -1;
+  int i2 = real2 ? t2 * 32 + lg_row((int)(__float_as_uint(b2) & 15u), h) : -1;
+  bool tie = (__float_as_uint(b1) == __float_as_uint(b2)) || (__float_as_uint(b2) == __float_as_uint(b3));
+  tie = tie || (__shfl_xor((int)tie, 32) != 0);
   {
     const float w1 = __shfl_xor(b1, 32), w2 = __shfl_xor(b2, 32), w3 = __shfl_xor(b3, 32);
     const int j1 = __shfl_xor(i1, 32), j2 = __shfl_xor(i2, 32);
@@ -773,11 +837,14 @@ __global__ __launch_bounds__(256) void k_group_mfma(
     const int k2 = A2 ? xi2 : yi2;
     const float xa = A1 ? (A2 ? b3 : b2) : (A2 ? b2 : b1);
     const float ya = A1 ? (A2 ? w1 : w2) : (A2 ? w2 : w3);
+    const float m3 = fmaxf(xa, ya);
+    const bool mt = (__float_as_uint(m1) == __float_as_uint(m2)) || (__float_as_uint(m2) == __float_as_uint(m3));
     b1 = __shfl(m1, col);
     b2 = __shfl(m2, col);
-    b3 = __shfl(fmaxf(xa, ya), col);
+    b3 = __shfl(m3, col);
     i1 = __shfl(k1, col);
     i2 = __shfl(k2, col);
+    tie = tie || (__shfl((int)mt, col) != 0);
   }
   // exact chains: half 0 for the first candidate, half 1 for the second
   const int lc = h ? i2 : i1;
@@ -818,10 +885,11 @@ __global__ __launch_bounds__(256) void k_group_mfma(
     const float mfe = gmf * (0.5f * ncmax + (1.0f + ub) * (1.0f + ub) * xc);
     const float fpe = g32 * (0.5f * ncmax + xc);
     const float E = (rnd + mfe + fpe) * 1.001f + 1e-30f;
-    const bool has3 = b3 > -FLT_MAX * 0.5f;
+    const bool has3 = b3 > -FLT_MAX;
+    // packed low bits (< 16 ulp) and the FP32 comparison
     const float slack = (fabsf(b1) + fabsf(b2) + (has3 ? fabsf(b3) : 0.0f)) * 1.6e-5f;
     const float thr = has3 ? b3 + E + slack : -INFINITY;
-    const bool cert = jb >= 0 && sb > -INFINITY && sb > thr;
+    const bool cert = !tie && jb >= 0 && sb > -INFINITY && sb > thr;
     const int slot = ent_id[ei];
     elab[slot] = cert ? lab : -1;
     escore[slot] = sb;
@@ -845,114 +913,122 @@ __device__ inline void mlpf_store4(uint16_t *row, int n0, const float *v) {
   *reinterpret_cast<uint2 *>(row + n0) = make_uint2(a, b);
 }
 
-__global__ __launch_bounds__(256) void k_mlp_propose(long rows, int din, int H, int G, int T,
+template <int KH>
+__global__ __launch_bounds__(256) void k_mlp_propose(long rows, int din, int G, int T,
                                                      const uint16_t *__restrict__ Xh,
                                                      const uint16_t *__restrict__ Ph,
                                                      const float *__restrict__ P,
                                                      const size_t *__restrict__ poff,
                                                      int *__restrict__ cand) {
-  extern __shared__ uint16_t hs[];  // H1 then H2: MLPF_ROWS x (H + 8) each
-  const int ldh = H + 8;
+  constexpr int H = KH * 16;       // hidden width
+  constexpr int NTW = H / 32 / 4;  // neuron tiles per wave (layers 1, 2)
+  extern __shared__ uint16_t hs[];  // H1 then H2: MLPF_ROWS x (H + 8) each; logits reuse H1
+  constexpr int ldh = H + 8;
   uint16_t *H1 = hs, *H2 = hs + MLPF_ROWS * ldh;
+  float *Zs = reinterpret_cast<float *>(hs);  // MLPF_ROWS x (G + 4), after layer 2
+  const int ldz = G + 4;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, h = lane >> 5, col = lane & 31;
   const long r0 = (long)blockIdx.x * MLPF_ROWS;
   const uint16_t *W1 = Ph + poff[0], *W2 = Ph + poff[2], *W3 = Ph + poff[4];
   const float *b1 = P + poff[1], *b2 = P + poff[3], *b3 = P + poff[5];
-  const int rt = w & 1;            // row tile of this wave (layers 1, 2)
-  const long prow = r0 + rt * 32 + col;
-  const bool pok = prow < rows;
-  const int NTW = H / 32 / 2;      // neuron tiles per wave (layers 1, 2)
-  const int nt0 = (w >> 1) * NTW;
-  // ---- layer 1: H1 = relu(X W1^T + b1)
-  for (int nt = nt0; nt < nt0 + NTW; ++nt) {
-    lg_f32x16 acc;
+  const long pr0 = r0 + col, pr1 = r0 + 32 + col;  // this lane's rows in the two row tiles
+  const bool ok0 = pr0 < rows, ok1 = pr1 < rows;
+  // one layer: out[rows][nt*32..] = relu(in[rows][:] W^T + b) for this wave's
+  // neuron tiles and both row tiles; B fragments from LDS (or the inputs)
+  auto store_tile = [&](uint16_t *dst_rows, int nt, int rt, const lg_f32x16 &acc) {
+    uint16_t *dst = dst_rows + (rt * 32 + col) * ldh;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] = b1[nt * 32 + lg_row(r, h)];
+    for (int q = 0; q < 4; ++q) {
+      const float v[4] = {acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]};
+      mlpf_store4(dst, nt * 32 + 8 * q + 4 * h, v);
+    }
+  };
+  // ---- layer 1 (K = din, runtime)
+#pragma unroll
+  for (int u = 0; u < NTW; ++u) {
+    const int nt = w * NTW + u;
+    lg_f32x16 a0, a1;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) a0[r] = a1[r] = b1[nt * 32 + lg_row(r, h)];
     for (int ks = 0; ks < din / 16; ++ks) {
       const lg_u32x4 a = *reinterpret_cast<const lg_u32x4 *>(W1 + (long)(nt * 32 + col) * din + ks * 16 + 8 * h);
-      const lg_u32x4 b = pok ? *reinterpret_cast<const lg_u32x4 *>(Xh + prow * din + ks * 16 + 8 * h)
-                             : lg_u32x4{0u, 0u, 0u, 0u};
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(lg_bf16x8, a),
-                                                     __builtin_bit_cast(lg_bf16x8, b), acc, 0, 0, 0);
+      const lg_u32x4 x0 = ok0 ? *reinterpret_cast<const lg_u32x4 *>(Xh + pr0 * din + ks * 16 + 8 * h)
+                              : lg_u32x4{0u, 0u, 0u, 0u};
+      const lg_u32x4 x1 = ok1 ? *reinterpret_cast<const lg_u32x4 *>(Xh + pr1 * din + ks * 16 + 8 * h)
+                              : lg_u32x4{0u, 0u, 0u, 0u};
+      a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(lg_bf16x8, a), __builtin_bit_cast(lg_bf16x8, x0), a0, 0, 0, 0);
+      a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(lg_bf16x8, a), __builtin_bit_cast(lg_bf16x8, x1), a1, 0, 0, 0);
     }
-    uint16_t *dst = H1 + (rt * 32 + col) * ldh;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const float v[4] = {acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]};
-      mlpf_store4(dst, nt * 32 + 8 * q + 4 * h, v);
-    }
+    store_tile(H1, nt, 0, a0);
+    store_tile(H1, nt, 1, a1);
   }
   __syncthreads();
-  // ---- layer 2: H2 = relu(H1 W2^T + b2)
-  for (int nt = nt0; nt < nt0 + NTW; ++nt) {
-    lg_f32x16 acc;
+  // ---- layer 2 (K = H, unrolled: the weight loads overlap)
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] = b2[nt * 32 + lg_row(r, h)];
-    const uint16_t *src = H1 + (rt * 32 + col) * ldh;
-    for (int ks = 0; ks < H / 16; ++ks) {
+  for (int u = 0; u < NTW; ++u) {
+    const int nt = w * NTW + u;
+    lg_f32x16 a0, a1;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) a0[r] = a1[r] = b2[nt * 32 + lg_row(r, h)];
+    const uint16_t *s0 = H1 + col * ldh, *s1 = H1 + (32 + col) * ldh;
+#pragma unroll
+    for (int ks = 0; ks < KH; ++ks) {
       const lg_u32x4 a = *reinterpret_cast<const lg_u32x4 *>(W2 + (long)(nt * 32 + col) * H + ks * 16 + 8 * h);
-      const lg_u32x4 b = *reinterpret_cast<const lg_u32x4 *>(src + ks * 16 + 8 * h);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(lg_bf16x8, a),
-                                                     __builtin_bit_cast(lg_bf16x8, b), acc, 0, 0, 0);
+      const lg_u32x4 x0 = *reinterpret_cast<const lg_u32x4 *>(s0 + ks * 16 + 8 * h);
+      const lg_u32x4 x1 = *reinterpret_cast<const lg_u32x4 *>(s1 + ks * 16 + 8 * h);
+      a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(lg_bf16x8, a), __builtin_bit_cast(lg_bf16x8, x0), a0, 0, 0, 0);
+      a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(lg_bf16x8, a), __builtin_bit_cast(lg_bf16x8, x1), a1, 0, 0, 0);
     }
-    uint16_t *dst = H2 + (rt * 32 + col) * ldh;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const float v[4] = {acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]};
-      mlpf_store4(dst, nt * 32 + 8 * q + 4 * h, v);
-    }
+    store_tile(H2, nt, 0, a0);
+    store_tile(H2, nt, 1, a1);
   }
   __syncthreads();
-  // ---- layer 3 + top-T: waves 0 and 1, one row tile each
-  if (w >= 2) return;
-  float bv[8];
-  int bi[8];
+  // ---- layer 3: logits of this wave's group tiles into LDS (over H1)
+  for (int nt = w; nt < G / 32; nt += 4) {
+    lg_f32x16 a0, a1;
 #pragma unroll
-  for (int t = 0; t < 8; ++t) { bv[t] = -INFINITY; bi[t] = 0x7fffffff; }
-  const uint16_t *src = H2 + (rt * 32 + col) * ldh;
-  for (int nt = 0; nt < G / 32; ++nt) {
-    lg_f32x16 acc;
+    for (int r = 0; r < 16; ++r) a0[r] = a1[r] = b3[nt * 32 + lg_row(r, h)];
+    const uint16_t *s0 = H2 + col * ldh, *s1 = H2 + (32 + col) * ldh;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] = b3[nt * 32 + lg_row(r, h)];
-    for (int ks = 0; ks < H / 16; ++ks) {
+    for (int ks = 0; ks < KH; ++ks) {
       const lg_u32x4 a = *reinterpret_cast<const lg_u32x4 *>(W3 + (long)(nt * 32 + col) * H + ks * 16 + 8 * h);
-      const lg_u32x4 b = *reinterpret_cast<const lg_u32x4 *>(src + ks * 16 + 8 * h);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(lg_bf16x8, a),
-                                                     __builtin_bit_cast(lg_bf16x8, b), acc, 0, 0, 0);
+      const lg_u32x4 x0 = *reinterpret_cast<const lg_u32x4 *>(s0 + ks * 16 + 8 * h);
+      const lg_u32x4 x1 = *reinterpret_cast<const lg_u32x4 *>(s1 + ks * 16 + 8 * h);
+      a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(lg_bf16x8, a), __builtin_bit_cast(lg_bf16x8, x0), a0, 0, 0, 0);
+      a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(lg_bf16x8, a), __builtin_bit_cast(lg_bf16x8, x1), a1, 0, 0, 0);
     }
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      float v = acc[r];
-      int gi = nt * 32 + lg_row(r, h);
+      Zs[col * ldz + nt * 32 + lg_row(r, h)] = a0[r];
+      Zs[(32 + col) * ldz + nt * 32 + lg_row(r, h)] = a1[r];
+    }
+  }
+  __syncthreads();
+  // ---- top T per row (ties to the lower group index)
+  if (threadIdx.x < MLPF_ROWS) {
+    const long pr = r0 + threadIdx.x;
+    if (pr < rows) {
+      float bv[8];
+      int bi[8];
 #pragma unroll
-      for (int t = 0; t < 8; ++t) {
-        if (t < T && (v > bv[t] || (v == bv[t] && gi < bi[t]))) {
-          const float tv = bv[t];
-          const int ti = bi[t];
-          bv[t] = v; bi[t] = gi; v = tv; gi = ti;
+      for (int t = 0; t < 8; ++t) { bv[t] = -INFINITY; bi[t] = 0x7fffffff; }
+      const float *z = Zs + threadIdx.x * ldz;
+      for (int g = 0; g < G; ++g) {
+        float v = z[g];
+        int gi = g;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          if (t < T && v > bv[t]) {
+            const float tv = bv[t];
+            const int ti = bi[t];
+            bv[t] = v; bi[t] = gi; v = tv; gi = ti;
+          }
         }
       }
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+        if (t < T) cand[pr * T + t] = (bv[t] == -INFINITY) ? -1 : bi[t];
     }
-  }
-  // merge the other half's list (disjoint groups)
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    float v = __shfl_xor(bv[q], 32);
-    int gi = __shfl_xor(bi[q], 32);
-    if (q >= T) continue;
-#pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      if (t < T && (v > bv[t] || (v == bv[t] && gi < bi[t]))) {
-        const float tv = bv[t];
-        const int ti = bi[t];
-        bv[t] = v; bi[t] = gi; v = tv; gi = ti;
-      }
-    }
-  }
-  if (h == 0 && pok) {
-#pragma unroll
-    for (int t = 0; t < 8; ++t)
-      if (t < T) cand[prow * T + t] = (bv[t] == -INFINITY || bi[t] == 0x7fffffff) ? -1 : bi[t];
   }
 }
 
@@ -1286,17 +1362,20 @@ static hipError_t refresh_centres(Engine *e, Learned *L) {
 static hipError_t propose(Engine *e, Learned *L) {
   hipStream_t s = e->stream;
   const long m = e->m;
-  if (L->proposer == KM_PROPOSER_MLP && L->H % 64 == 0 && L->G % 32 == 0 && L->G <= 256) {
+  if (L->proposer == KM_PROPOSER_MLP && (L->H == 128 || L->H == 256) && L->G % 32 == 0 &&
+      (size_t)MLPF_ROWS * (L->G + 4) * 4 <= (size_t)MLPF_ROWS * (L->H + 8) * 2) {
     // fused inference: no hidden or logit matrices in HBM
     const size_t lds = sizeof(uint16_t) * 2 * MLPF_ROWS * (L->H + 8);
-    static size_t attr = 0;
-    if (lds > 64 * 1024 && lds > attr) {
-      LCHK(hipFuncSetAttribute((const void *)k_mlp_propose, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)lds));
-      attr = lds;
+    const dim3 grid((unsigned)((m + MLPF_ROWS - 1) / MLPF_ROWS));
+    if (L->H == 256) {
+      LCHK(hipFuncSetAttribute((const void *)k_mlp_propose<16>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL(k_mlp_propose<16>, grid, dim3(256), lds, s, m, L->din, L->G, L->T, L->Xh, L->Ph,
+                         L->P, L->dpoff, L->cand);
+    } else {
+      hipLaunchKernelGGL(k_mlp_propose<8>, grid, dim3(256), lds, s, m, L->din, L->G, L->T, L->Xh, L->Ph,
+                         L->P, L->dpoff, L->cand);
     }
-    hipLaunchKernelGGL(k_mlp_propose, dim3((unsigned)((m + MLPF_ROWS - 1) / MLPF_ROWS)), dim3(256), lds,
-                       s, m, L->din, L->H, L->G, L->T, L->Xh, L->Ph, L->P, L->dpoff, L->cand);
     return hipGetLastError();
   }
   if (L->proposer == KM_PROPOSER_MLP) {
@@ -1362,7 +1441,8 @@ hipError_t learned_assign(Engine *e) {
 #define GMF(KS)                                                                                    \
   hipLaunchKernelGGL(k_group_mfma<KS>, dim3((waves + 3) / 4), dim3(256), 0, s, G, L->wpre, L->goff, \
                      L->gcnt, L->gsize, L->gcol, L->permP, L->ent_pt, L->ent_id, mv.Xh, e->dX,      \
-                     e->n, L->ChP, L->biasP, e->dC, e->dnc, mv.bnd, mv.xnorm, L->elab, L->escore)
+                     e->n, L->ChP, L->biasP, e->dC, e->dnc, mv.bnd, mv.xnorm, L->elab, L->escore, \
+                     FLT_MAX)
       switch (mv.KS) {
         case 1: GMF(1); break;
         case 2: GMF(2); break;
