@@ -113,6 +113,7 @@ def main():
                                ("assign", "reresolve", "update", "compactness")},
         "assign_pairs_per_s": pairs / (assign_ms * 1e-3),
         "reresolved_per_step": st["reresolved"] / args.steps,
+        "rescored_per_step": st["rescored"] / args.steps,
         "roofline": {"bound": "mfma" if args.precision != "fp32" else "fp32",
                      "achieved": achieved / 1e12, "peak": PEAK[args.precision] / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK[args.precision],
