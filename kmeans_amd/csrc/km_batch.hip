@@ -256,7 +256,7 @@ __global__ __launch_bounds__(256) void k_batch_assign_bf16(
   float *Cf = bias + Kp;                                            // k x d
   float *ncs = Cf + (size_t)k * d;                                  // k
   __shared__ int item_s;
-  __shared__ float red[256], bndv[2];
+  __shared__ float red[256], bndv[3];
   __shared__ unsigned long long exact_ct;
   const int tid = threadIdx.x, lane = tid & 63, h = lane >> 5, col = lane & 31;
   if (tid == 0) exact_ct = 0;
@@ -289,6 +289,18 @@ __global__ __launch_bounds__(256) void k_batch_assign_bf16(
         if (j < k) { ncs[j] = n2; mx = fmaxf(mx, n2); }
       }
       for (int e = tid; e < k * d; e += 256) Cf[e] = Ci[e];
+      // max over the centres of |c_j - bf16(c_j)| (c - bf16(c) is exact in
+      // FP32; the chain of squares is within gamma_d of the sum)
+      float dm = 0.0f;
+      for (int j = tid; j < k; j += 256) {
+        float t = 0.0f;
+        for (int q = 0; q < d; ++q) {
+          const float v = Ci[(size_t)j * d + q];
+          const float r = v - __uint_as_float((uint32_t)b_bf16(v) << 16);
+          t = __builtin_fmaf(r, r, t);
+        }
+        dm = fmaxf(dm, t);
+      }
       red[tid] = mx;
       __syncthreads();
       for (int o = 128; o > 0; o >>= 1) {
@@ -296,14 +308,22 @@ __global__ __launch_bounds__(256) void k_batch_assign_bf16(
         __syncthreads();
       }
       if (tid == 0) {
-        const float ncmax = red[0] * (1.0f + 1e-5f) + 1e-30f;  // nc chain within gamma_d
+        const float ncmax = red[0] * (1.0f + (float)d * 1.2e-7f + 1e-6f) + 1e-30f;  // nc chain within gamma_d
         bndv[0] = sqrtf(ncmax) * (1.0f + 1e-6f);
         bndv[1] = ncmax;
       }
+      __syncthreads();
+      red[tid] = dm;
+      __syncthreads();
+      for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) red[tid] = fmaxf(red[tid], red[tid + o]);
+        __syncthreads();
+      }
+      if (tid == 0) bndv[2] = sqrtf(red[0] * (1.0f + (float)d * 1.2e-7f + 1e-6f)) * (1.0f + 1e-6f) + 1e-30f;
       cur = inst;
     }
     __syncthreads();
-    const float cmax = bndv[0], ncmax = bndv[1];
+    const float cmax = bndv[0], ncmax = bndv[1], dcmax = bndv[2];
     const long rows = min((long)P, (long)nvec[inst] - w.y);
     const long base = xoff[inst] + w.y;
     const int wv0 = tid >> 6;
@@ -331,7 +351,7 @@ __global__ __launch_bounds__(256) void k_batch_assign_bf16(
         xv[4 * q4] = t4.x; xv[4 * q4 + 1] = t4.y; xv[4 * q4 + 2] = t4.z; xv[4 * q4 + 3] = t4.w;
       }
       b_u32x4 bf[KS];
-      float sq = 0.0f;
+      float sq = 0.0f, se = 0.0f;
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks) {
         uint32_t wv[4];
@@ -342,13 +362,22 @@ __global__ __launch_bounds__(256) void k_batch_assign_bf16(
           const float v1 = (ok && d0 + 1 < d) ? xr[d0 + 1] : 0.0f;
           sq = __builtin_fmaf(v0, v0, sq);
           sq = __builtin_fmaf(v1, v1, sq);
-          wv[u] = (uint32_t)b_bf16(v0) | ((uint32_t)b_bf16(v1) << 16);
+          const uint16_t h0 = b_bf16(v0), h1 = b_bf16(v1);
+          const float r0 = v0 - __uint_as_float((uint32_t)h0 << 16);
+          const float r1 = v1 - __uint_as_float((uint32_t)h1 << 16);
+          se = __builtin_fmaf(r0, r0, se);
+          se = __builtin_fmaf(r1, r1, se);
+          wv[u] = (uint32_t)h0 | ((uint32_t)h1 << 16);
         }
         bf[ks] = b_u32x4{wv[0], wv[1], wv[2], wv[3]};
       }
       sq += __shfl_xor(sq, 32);
-      // upper bound of ||x||: both halves' chains within gamma_d of the sum
-      const float xn = sqrtf(sq * (1.0f + 1e-5f)) * (1.0f + 1e-6f) + 1e-30f;
+      se += __shfl_xor(se, 32);
+      // upper bounds of |x| and |x - bf16(x)|: both halves' chains within
+      // gamma_d of the sums
+      const float inf = 1.0f + (float)d * 1.2e-7f + 1e-6f;
+      const float xn = sqrtf(sq * inf) * (1.0f + 1e-6f) + 1e-30f;
+      const float ex = sqrtf(se * inf) * (1.0f + 1e-6f) + 1e-30f;
       // top 3 of the packed scores (register id in the low 4 bits, three
       // v_med3 per score, max as med3 with a run-time +FLT_MAX) and the tiles
       // of the first two
@@ -432,13 +461,15 @@ __global__ __launch_bounds__(256) void k_batch_assign_bf16(
         int lab = cj;
         float sb = sx;
         if (jy >= 0 && jy < k && (sy > sb || (sy == sb && jy < lab))) { sb = sy; lab = jy; }
-        const float xc = xn * cmax, Df = (float)d;
-        const float ub = 3.90625e-3f;
+        // forward-error bound of the BF16 score (|x.c - xh.ch| <= |dx| |c| +
+        // |xh| |dc| by Cauchy-Schwarz, dx and dc the rounding errors), the
+        // matrix core's accumulation and the FP32 chain
+        const float Df = (float)d;
         const float g32 = Df * 5.9604645e-8f * 1.0001f;
         const float gmf = (Df + 1.0f) * 2.3841858e-7f * 1.001f;
-        const float rnd = (2.0f * ub + ub * ub) * xc;
-        const float mfe = gmf * (0.5f * ncmax + (1.0f + ub) * (1.0f + ub) * xc);
-        const float fpe = g32 * (0.5f * ncmax + xc);
+        const float rnd = ex * cmax + (xn + ex) * dcmax;
+        const float mfe = gmf * (0.5f * ncmax + (xn + ex) * (cmax + dcmax));
+        const float fpe = g32 * (0.5f * ncmax + xn * cmax);
         const float E = (rnd + mfe + fpe) * 1.001f + 1e-30f;
         const bool has3 = b3 > -FLT_MAX;
         // packed low bits (< 16 ulp) and FP32 rounding of the comparison

@@ -84,7 +84,8 @@ struct Learned {
   bool mfma_groups = false;
   // reduced-precision fallback: uncertified rows gathered contiguously
   long fb_cap = 0;
-  float *fbX = nullptr, *fbN = nullptr;
+  float *fbX = nullptr;
+  float2 *fbN = nullptr;
   uint16_t *fbXh = nullptr;
   int *fbLab = nullptr;
 };
@@ -709,7 +710,7 @@ __global__ __launch_bounds__(256) void k_group_mfma(
     const int *__restrict__ ent_id, const uint16_t *__restrict__ Xh,
     const float *__restrict__ X, int D, const uint16_t *__restrict__ ChP,
     const float *__restrict__ biasP, const float *__restrict__ C, const float *__restrict__ nc,
-    const float *__restrict__ bnd, const float *__restrict__ xnorm, int *__restrict__ elab,
+    const float *__restrict__ bnd, const float2 *__restrict__ xnb, int *__restrict__ elab,
     float *__restrict__ escore, float big) {
   constexpr int Dp = KS * 16;
   constexpr int CPR = Dp / 8;                    // 16-byte chunks per centre row
@@ -876,14 +877,16 @@ __global__ __launch_bounds__(256) void k_group_mfma(
     int lab = i1, jb = jc;
     float sb = sx;
     if (jy >= 0 && (sy > sb || (sy == sb && jy < jb))) { sb = sy; lab = i2; jb = jy; }
-    const float cmax = bnd[0], ncmax = bnd[1];
-    const float xn = xnorm[p], xc = xn * cmax, Df = (float)D;
-    const float ub = 3.90625e-3f;                                  // 2^-8
+    // the plain path's bound: |x| and |x - xh| per point, max |c_j|,
+    // |c_j|^2 and |c_j - ch_j| over the centres (BF16, no scale)
+    const float cmax = bnd[0], ncmax = bnd[1], dcmax = bnd[2];
+    const float2 nb = xnb[p];
+    const float xn = nb.x, ex = nb.y, Df = (float)D;
     const float g32 = Df * 5.9604645e-8f * 1.0001f;                // gamma_D, u = 2^-24
     const float gmf = (Df + 1.0f) * 2.3841858e-7f * 1.001f;        // gamma'_{D+1}, u = 2^-22
-    const float rnd = (2.0f * ub + ub * ub) * xc;
-    const float mfe = gmf * (0.5f * ncmax + (1.0f + ub) * (1.0f + ub) * xc);
-    const float fpe = g32 * (0.5f * ncmax + xc);
+    const float rnd = ex * cmax + (xn + ex) * dcmax;
+    const float mfe = gmf * (0.5f * ncmax + (xn + ex) * (cmax + dcmax));
+    const float fpe = g32 * (0.5f * ncmax + xn * cmax);
     const float E = (rnd + mfe + fpe) * 1.001f + 1e-30f;
     const bool has3 = b3 > -FLT_MAX;
     // packed low bits (< 16 ulp) and the FP32 comparison
@@ -1036,8 +1039,8 @@ __global__ __launch_bounds__(256) void k_mlp_propose(long rows, int din, int G, 
 // row, reduced-precision row (stride Dp) and norm bound.
 __global__ void k_gather_fb(long cnt, const int *__restrict__ fb, int n, int Dp,
                             const float *__restrict__ X, const uint16_t *__restrict__ Xh,
-                            const float *__restrict__ xn, float *__restrict__ gX,
-                            uint16_t *__restrict__ gXh, float *__restrict__ gN) {
+                            const float2 *__restrict__ xn, float *__restrict__ gX,
+                            uint16_t *__restrict__ gXh, float2 *__restrict__ gN) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long r = i / Dp;
   const int d = (int)(i - r * Dp);
@@ -1491,7 +1494,7 @@ hipError_t learned_fallback(Engine *e) {
       L->fbX = nullptr; L->fbN = nullptr; L->fbXh = nullptr; L->fbLab = nullptr;
       L->fb_cap = 0;
       LCHK(lmalloc((void **)&L->fbX, sizeof(float) * cap * e->n));
-      LCHK(lmalloc((void **)&L->fbN, sizeof(float) * cap));
+      LCHK(lmalloc((void **)&L->fbN, sizeof(float2) * cap));
       LCHK(lmalloc((void **)&L->fbXh, sizeof(uint16_t) * cap * Dp));
       LCHK(lmalloc((void **)&L->fbLab, sizeof(int) * cap));
       L->fb_cap = cap;

@@ -11,13 +11,19 @@
 // other centre can reach it.  Uncertified points go to the FP32 kernel.
 //
 // Error bound (Higham, Accuracy and Stability of Numerical Algorithms, 2nd
-// ed., sec. 3.1; u = 2^-24, gamma_n = n u / (1 - n u)):
-//   |fl-chain - exact| <= gamma_D (0.5 nc_j + sum|x c|)          (FP32 path)
-//   |xh ch - s^2 x.c|  <= (2 ub + ub^2) s^2 sum|x c| + eta terms  (rounding)
-//   |mfma - xh.ch|     <= gamma'_{D+1} (0.5 s^2 nc_j + sum|xh ch|) with a
-//                         pessimistic unit roundoff 2^-22 for the matrix core
-// and sum|x c| <= |x| |c| (Cauchy-Schwarz), maximised over j with the
-// largest centre norm.  ub = 2^-8 (BF16) or 2^-11 (FP16).
+// ed., sec. 3.1; u = 2^-24, gamma_n = n u / (1 - n u)), in FP32 score units
+// (the FP16 scale s divided out), with dx = x - xh/s and dc = c - ch/s the
+// representation errors of the reduced-precision operands:
+//   |fl-chain - exact| <= gamma_D (0.5 nc_j + |x| |c_j|)          (FP32 path)
+//   |xh.ch/s^2 - x.c|  =  |dx.c + (xh/s).dc|
+//                      <= |dx| |c_j| + (|x| + |dx|) |dc_j|        (rounding)
+//   |mfma - xh.ch|/s^2 <= gamma'_{D+1} (0.5 nc_j + (|x| + |dx|)(|c_j| + |dc_j|))
+//                         with a pessimistic unit roundoff 2^-22 for the
+//                         matrix core
+// (Cauchy-Schwarz), maximised over j with the largest |c_j|, nc_j and
+// |dc_j|.  |dx| is measured per point and |dc_j| per centre (norms of the
+// actual rounding errors, typically 0.3-0.4 ub |x| for ub = 2^-8 (BF16) or
+// 2^-11 (FP16), against the worst case ub |x| of a component-wise bound).
 #include <float.h>
 #include <math.h>
 
@@ -62,8 +68,9 @@ struct MfmaState {
   uint16_t *Ch = nullptr;   // Kp x Dp
   uint16_t *Cl = nullptr;   // Kp x Dp: BF16 of (C - Ch), the low part (BF16 engines)
   float *bias = nullptr;    // Kp: -0.5*scale^2*nc (or -FLT_MAX past k)
-  float *bnd = nullptr;     // [cmax, ncmax]
-  float *xnorm = nullptr;   // m: upper bound of |x_i|
+  float *bnd = nullptr;     // [cmax, ncmax, dcmax]: bounds over the centres
+  float *dcn = nullptr;     // Kp: upper bound of |c_j - ch_j/s|
+  float2 *xnb = nullptr;    // m: upper bounds of |x_i| and |x_i - xh_i/s|
   int *flags = nullptr;     // m
   int4 *cands = nullptr;    // unused: certificate 2 runs inside the assign kernel
   int *ncand = nullptr;     // per iteration slot
@@ -88,41 +95,65 @@ __global__ void k_convert_rows(long m, int d, int Dp, const float *__restrict__ 
   out[e] = fp16 ? f32_to_f16_rne(v) : f32_to_bf16_rne(v);
 }
 
-// Ch, bias and the bound constants [max |c_j|, max nc_j] (one block).
+// Ch (and for BF16 the low parts Cl), bias and the per-centre bound dcn_j
+// >= |c_j - ch_j/s|: one wave per centre row.  c*s - float(ch) is exact in
+// FP32; the FP32 sum of its squares is within gamma_Dp of the exact sum.
 __global__ __launch_bounds__(256) void k_prepare_centres(int k, int d, int Dp, int Kp,
                                                          const float *__restrict__ C,
                                                          const float *__restrict__ nc, float scale,
                                                          int fp16, uint16_t *__restrict__ Ch,
                                                          uint16_t *__restrict__ Cl,
                                                          float *__restrict__ bias,
-                                                         float *__restrict__ bnd) {
-  // block 0 reduces the bound; every block converts its share of the tiles
-  __shared__ float red[256];
-  const long stride = (long)gridDim.x * blockDim.x;
-  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < (long)Kp * Dp; e += stride) {
-    const long j = e / Dp;
-    const int q = (int)(e - j * Dp);
-    const float v = (j < k && q < d) ? C[j * d + q] * scale : 0.0f;
-    const uint16_t hi = fp16 ? f32_to_f16_rne(v) : f32_to_bf16_rne(v);
-    Ch[e] = hi;
-    if (!fp16) Cl[e] = f32_to_bf16_rne(v - __uint_as_float((uint32_t)hi << 16));
+                                                         float *__restrict__ dcn) {
+  const int lane = threadIdx.x & 63;
+  const long nw = (long)gridDim.x * (blockDim.x >> 6);
+  for (long j = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); j < Kp; j += nw) {
+    float sq = 0.0f;
+    for (int q = lane; q < Dp; q += 64) {
+      const float v = (j < k && q < d) ? C[j * d + q] * scale : 0.0f;
+      const uint16_t hi = fp16 ? f32_to_f16_rne(v) : f32_to_bf16_rne(v);
+      const float hv = fp16 ? (float)__builtin_bit_cast(_Float16, hi) : __uint_as_float((uint32_t)hi << 16);
+      const float r = v - hv;
+      Ch[j * Dp + q] = hi;
+      if (!fp16) Cl[j * Dp + q] = f32_to_bf16_rne(r);
+      sq = __builtin_fmaf(r, r, sq);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off);
+    if (lane == 0) {
+      bias[j] = j < k ? -0.5f * nc[j] * (scale * scale) : -FLT_MAX;
+      dcn[j] = sqrtf(sq * (1.0f + (float)Dp * 1.2e-7f + 1e-6f)) * (1.0f + 1e-6f) / scale + 1e-30f;
+    }
   }
-  for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < Kp; j += stride)
-    bias[j] = j < k ? -0.5f * nc[j] * (scale * scale) : -FLT_MAX;
-  if (blockIdx.x != 0) return;
-  float mx = 0.0f;
-  for (int j = threadIdx.x; j < k; j += blockDim.x) mx = fmaxf(mx, nc[j]);
-  red[threadIdx.x] = mx;
+}
+
+// bnd = [cmax, ncmax, dcmax]: upper bounds of max |c_j|, max |c_j|^2 and
+// max |c_j - ch_j/s| over the k centres (one block).  nc_j is the FP32 chain
+// of squares; the true |c_j|^2 is at most nc_j (1 + gamma_d).
+__global__ __launch_bounds__(256) void k_centre_bounds(int k, int d, const float *__restrict__ nc,
+                                                       const float *__restrict__ dcn,
+                                                       float *__restrict__ bnd) {
+  __shared__ float r1[256], r2[256];
+  float m1 = 0.0f, m2 = 0.0f;
+  for (int j = threadIdx.x; j < k; j += blockDim.x) {
+    m1 = fmaxf(m1, nc[j]);
+    m2 = fmaxf(m2, dcn[j]);
+  }
+  r1[threadIdx.x] = m1;
+  r2[threadIdx.x] = m2;
   __syncthreads();
   for (int off = 128; off > 0; off >>= 1) {
-    if (threadIdx.x < off) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + off]);
+    if (threadIdx.x < off) {
+      r1[threadIdx.x] = fmaxf(r1[threadIdx.x], r1[threadIdx.x + off]);
+      r2[threadIdx.x] = fmaxf(r2[threadIdx.x], r2[threadIdx.x + off]);
+    }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    // nc_j is the FP32 chain; the true |c_j|^2 is at most nc_j (1 + gamma_D)
-    const float ncmax = red[0] * (1.0f + 1e-5f) + 1e-30f;
+    const float ncmax = r1[0] * (1.0f + (float)d * 1.2e-7f + 1e-6f) + 1e-30f;
     bnd[0] = sqrtf(ncmax) * (1.0f + 1e-6f);
     bnd[1] = ncmax;
+    bnd[2] = r2[0];
   }
 }
 
@@ -149,8 +180,8 @@ template <int KS, bool FP16>
 __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
     long M, const uint16_t *__restrict__ Xh, const float *__restrict__ X, int D, int K, int Kp,
     const uint16_t *__restrict__ Ch, const float *__restrict__ bias, const float *__restrict__ C,
-    const float *__restrict__ nc, const float *__restrict__ bnd, const float *__restrict__ xnorm,
-    float inv_s2, float ub, float eta, float big, int *__restrict__ labels,
+    const float *__restrict__ nc, const float *__restrict__ bnd, const float2 *__restrict__ xnb,
+    float inv_s2, float eta, float big, int *__restrict__ labels,
     int *__restrict__ flags, int *__restrict__ nflag, int4 *__restrict__ cands,
     int *__restrict__ ncand) {
   constexpr int Dp = KS * 16;
@@ -307,7 +338,7 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
   }
 
   // merge the two lane halves (same points, different centre rows)
-  const float cmax = bnd[0], ncmax = bnd[1];
+  const float cmax = bnd[0], ncmax = bnd[1], dcmax = bnd[2];
   auto finish = [&](int g, const Top3 &T) {
     // candidate indices of this half
     const unsigned u1 = __float_as_uint(T.b1), u2 = __float_as_uint(T.b2);
@@ -341,15 +372,16 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
     }
     const long p = p0 + g * 32 + col;
     // forward-error bound E for every centre's approximate score (FP32 units)
-    const float xn = p < M ? xnorm[p] : 0.0f;
-    const float xc = xn * cmax;
+    const float2 nb = p < M ? xnb[p] : make_float2(0.0f, 0.0f);
+    const float xn = nb.x, ex = nb.y;  // |x|, |x - xh/s|
     const float Df = (float)D;
     const float g32 = Df * 5.9604645e-8f * 1.0001f;        // gamma_D, u = 2^-24
     const float gmf = (Df + 1.0f) * 2.3841858e-7f * 1.001f; // gamma'_{D+1}, u = 2^-22
-    const float rnd = (2.0f * ub + ub * ub) * xc;
+    const float rnd = ex * cmax + (xn + ex) * dcmax;
+    // FP16 engines: margin for subnormal operands the matrix core may flush
     const float etat = eta * sqrtf(Df) * (xn + cmax) + eta * eta * Df;
-    const float mfe = gmf * (0.5f * ncmax + (1.0f + ub) * (1.0f + ub) * xc + etat);
-    const float fpe = g32 * (0.5f * ncmax + xc);
+    const float mfe = gmf * (0.5f * ncmax + (xn + ex) * (cmax + dcmax) + etat);
+    const float fpe = g32 * (0.5f * ncmax + xn * cmax);
     const float E = (rnd + etat + mfe + fpe) * 1.001f + 1e-30f;
     const float a1 = m1 * inv_s2, a2 = m2 * inv_s2, a3 = m3 * inv_s2;  // FP32 score units
     // packed values carry 4 replaced low bits (< 16 ulp); FP32 subtraction
@@ -440,7 +472,7 @@ __global__ __launch_bounds__(256) void k_deep_mfma(
     const int *__restrict__ nlist, const int *__restrict__ list, const uint16_t *__restrict__ Xh,
     const float *__restrict__ X, int D, int K, int Kp, const uint16_t *__restrict__ Ch,
     const uint16_t *__restrict__ Cl, const float *__restrict__ bias, const float *__restrict__ C,
-    const float *__restrict__ nc, const float *__restrict__ bnd, const float *__restrict__ xnorm,
+    const float *__restrict__ nc, const float *__restrict__ bnd, const float2 *__restrict__ xnb,
     int *__restrict__ labels, int *__restrict__ list2, int *__restrict__ nlist2) {
   const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31;
   const long total = *nlist;
@@ -610,7 +642,7 @@ __global__ __launch_bounds__(256) void k_deep_mfma(
         if (cj < 0 || cj >= K) continue;
         if (sv[q] > sb || (sv[q] == sb && cj < lab)) { sb = sv[q]; lab = cj; }
       }
-      const float xn = xnorm[p];
+      const float xn = xnb[p].x;
       const float xc = xn * cmax;
       const float ub = 3.90625e-3f;  // 2^-8
       const float Df = (float)D;
@@ -647,7 +679,8 @@ void mfma_free(Engine *e) {
   hipFree(s->Ch);
   hipFree(s->bias);
   hipFree(s->bnd);
-  hipFree(s->xnorm);
+  hipFree(s->xnb);
+  hipFree(s->dcn);
   hipFree(s->flags);
   hipFree(s->cands);
   hipFree(s->ncand);
@@ -689,7 +722,8 @@ static hipError_t ensure_state(Engine *e) {
   if ((r = hipMalloc(&s->Cl, sizeof(uint16_t) * (size_t)s->Kp * s->Dp)) != hipSuccess) return r;
   if ((r = hipMalloc(&s->bias, sizeof(float) * s->Kp)) != hipSuccess) return r;
   if ((r = hipMalloc(&s->bnd, sizeof(float) * 4)) != hipSuccess) return r;
-  if ((r = hipMalloc(&s->xnorm, sizeof(float) * (size_t)xrows)) != hipSuccess) return r;
+  if ((r = hipMalloc(&s->xnb, sizeof(float2) * (size_t)xrows)) != hipSuccess) return r;
+  if ((r = hipMalloc(&s->dcn, sizeof(float) * s->Kp)) != hipSuccess) return r;
   if ((r = hipMalloc(&s->flags, sizeof(int) * (size_t)xrows)) != hipSuccess) return r;
   if ((r = hipMalloc(&s->flags2, sizeof(int) * (size_t)xrows)) != hipSuccess) return r;
   s->nflag_slots = 64;
@@ -703,14 +737,22 @@ static hipError_t ensure_state(Engine *e) {
   return hipMemsetAsync(s->nflag, 0, sizeof(int) * s->nflag_slots, e->stream);
 }
 
-// upper bound of |x_i|: the FP32 chain of squares is within gamma_D of the
-// exact sum, inflated by 1e-5 (covers D <= 160,000) and one more ulp for sqrt.
-// One point per lane, rows read in 16-byte pieces (whole cache lines).
-__global__ void k_xnorm(long m, int d, const float *__restrict__ X, float *__restrict__ out) {
+// Per point upper bounds of |x_i| and of |x_i - xh_i/s| (xh the BF16/FP16
+// rounding of s*x_i, exactly as k_convert_rows makes it; s*x - float(xh) is
+// exact in FP32).  The FP32 chains of squares are within gamma_d of the exact
+// sums (factor 1 + 1.2e-7 d covers it for d < 2^22), one more ulp each for
+// the product and the square root.  One point per lane, rows read in 16-byte
+// pieces (whole cache lines).
+__device__ inline float round_err(float v, int fp16) {
+  const uint16_t hi = fp16 ? f32_to_f16_rne(v) : f32_to_bf16_rne(v);
+  return v - (fp16 ? (float)__builtin_bit_cast(_Float16, hi) : __uint_as_float((uint32_t)hi << 16));
+}
+__global__ void k_xnorm(long m, int d, const float *__restrict__ X, float scale, int fp16,
+                        float2 *__restrict__ out) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
   const float *x = X + i * d;
-  float s = 0.0f;
+  float s = 0.0f, t = 0.0f;
   if ((d & 3) == 0) {
     const float4 *x4 = reinterpret_cast<const float4 *>(x);
 #pragma unroll 8
@@ -720,11 +762,23 @@ __global__ void k_xnorm(long m, int d, const float *__restrict__ X, float *__res
       s = __builtin_fmaf(v.y, v.y, s);
       s = __builtin_fmaf(v.z, v.z, s);
       s = __builtin_fmaf(v.w, v.w, s);
+      const float r0 = round_err(v.x * scale, fp16), r1 = round_err(v.y * scale, fp16);
+      const float r2 = round_err(v.z * scale, fp16), r3 = round_err(v.w * scale, fp16);
+      t = __builtin_fmaf(r0, r0, t);
+      t = __builtin_fmaf(r1, r1, t);
+      t = __builtin_fmaf(r2, r2, t);
+      t = __builtin_fmaf(r3, r3, t);
     }
   } else {
-    for (int q = 0; q < d; ++q) s = __builtin_fmaf(x[q], x[q], s);
+    for (int q = 0; q < d; ++q) {
+      s = __builtin_fmaf(x[q], x[q], s);
+      const float r = round_err(x[q] * scale, fp16);
+      t = __builtin_fmaf(r, r, t);
+    }
   }
-  out[i] = sqrtf(s * (1.0f + 1e-5f)) * (1.0f + 1e-6f) + 1e-30f;
+  const float inf = 1.0f + (float)d * 1.2e-7f + 1e-6f;
+  out[i] = make_float2(sqrtf(s * inf) * (1.0f + 1e-6f) + 1e-30f,
+                       sqrtf(t * inf) * (1.0f + 1e-6f) / scale + 1e-30f);
 }
 
 // max |v| over a device array (FP16 scaling only; one host read per run)
@@ -765,7 +819,7 @@ static hipError_t prepare_x(Engine *e) {
     hipLaunchKernelGGL(k_convert_rows, dim3(km_cdiv(tot, 256)), dim3(256), 0, e->stream, e->m,
                        e->n, s->Dp, e->dX, s->scale, (int)s->fp16, s->Xh);
     hipLaunchKernelGGL(k_xnorm, dim3(km_cdiv(e->m, 256)), dim3(256), 0, e->stream, e->m, e->n,
-                       e->dX, s->xnorm);
+                       e->dX, s->scale, (int)s->fp16, s->xnb);
     if ((r = hipGetLastError()) != hipSuccess) return r;
   }
   s->x_ready = true;
@@ -787,16 +841,17 @@ hipError_t mfma_prepare_rows(Engine *e, const float *X, long rows) {
   hipLaunchKernelGGL(k_convert_rows, dim3(km_cdiv(tot, 256)), dim3(256), 0, e->stream, rows, e->n,
                      s->Dp, X, s->scale, (int)s->fp16, s->Xh);
   hipLaunchKernelGGL(k_xnorm, dim3(km_cdiv(rows, 256)), dim3(256), 0, e->stream, rows, e->n, X,
-                     s->xnorm);
+                     s->scale, (int)s->fp16, s->xnb);
   return hipGetLastError();
 }
 
 static hipError_t prepare_centres(Engine *e) {
   MfmaState *s = st(e);
-  const long pe = (long)s->Kp * s->Dp;
-  const unsigned pg = (unsigned)std::min<long>(km_cdiv(pe, 256), 1024);
+  const unsigned pg = (unsigned)std::min<long>(km_cdiv(s->Kp, 4), 1024);  // a wave per row
   hipLaunchKernelGGL(k_prepare_centres, dim3(pg), dim3(256), 0, e->stream, e->k, e->n, s->Dp, s->Kp,
-                     e->dC, e->dnc, s->scale, (int)s->fp16, s->Ch, s->Cl, s->bias, s->bnd);
+                     e->dC, e->dnc, s->scale, (int)s->fp16, s->Ch, s->Cl, s->bias, s->dcn);
+  hipLaunchKernelGGL(k_centre_bounds, dim3(1), dim3(256), 0, e->stream, e->k, e->n, e->dnc, s->dcn,
+                     s->bnd);
   s->c_ready = true;
   return hipGetLastError();
 }
@@ -817,7 +872,7 @@ template <int KS>
 static void launch_mfma_ks(hipStream_t strm, bool fp16, dim3 g, long M, const uint16_t *Xh,
                            const float *X, int D, int K, int Kp, const uint16_t *Ch,
                            const float *bias, const float *C, const float *nc, const float *bnd,
-                           const float *xnorm, float inv_s2, float ub, float eta, int *labels,
+                           const float2 *xnb, float inv_s2, float eta, int *labels,
                            int *flags, int *nflag, int4 *cands, int *ncand) {
   const size_t lds = 2 * (size_t)MF_STAGE * MF_TILE * (KS * 2) * 16 + 2 * MF_STAGE * MF_TILE * 4;
   static bool attr_set[2] = {false, false};
@@ -832,10 +887,10 @@ static void launch_mfma_ks(hipStream_t strm, bool fp16, dim3 g, long M, const ui
   }
   if (fp16)
     hipLaunchKernelGGL((k_assign_mfma<KS, true>), g, dim3(64 * MF_WAVES), lds, strm, M, Xh, X, D, K, Kp, Ch,
-                       bias, C, nc, bnd, xnorm, inv_s2, ub, eta, FLT_MAX, labels, flags, nflag, cands, ncand);
+                       bias, C, nc, bnd, xnb, inv_s2, eta, FLT_MAX, labels, flags, nflag, cands, ncand);
   else
     hipLaunchKernelGGL((k_assign_mfma<KS, false>), g, dim3(64 * MF_WAVES), lds, strm, M, Xh, X, D, K, Kp, Ch,
-                       bias, C, nc, bnd, xnorm, inv_s2, ub, eta, FLT_MAX, labels, flags, nflag, cands, ncand);
+                       bias, C, nc, bnd, xnb, inv_s2, eta, FLT_MAX, labels, flags, nflag, cands, ncand);
 }
 
 hipError_t mfma_assign(Engine *e, long r0, long rows) {
@@ -847,13 +902,13 @@ hipError_t mfma_assign(Engine *e, long r0, long rows) {
     if ((r = prepare_x(e)) != hipSuccess) return r;
     s->c_ready = false;
   }
-  return mfma_assign_view(e, e->dX + r0 * e->n, s->Xh + r0 * s->Dp, s->xnorm + r0,
+  return mfma_assign_view(e, e->dX + r0 * e->n, s->Xh + r0 * s->Dp, s->xnb + r0,
                           e->dlabels + r0, rows);
 }
 
 // Assignment of `rows` points: FP32 rows X, their reduced-precision copy Xh
 // and norm bounds xn, labels written to lab (all indexed from 0).
-hipError_t mfma_assign_view(Engine *e, const float *X, const uint16_t *Xh, const float *xn,
+hipError_t mfma_assign_view(Engine *e, const float *X, const uint16_t *Xh, const float2 *xn,
                             int *lab, long rows) {
   hipError_t r;
   MfmaState *s = st(e);
@@ -865,17 +920,16 @@ hipError_t mfma_assign_view(Engine *e, const float *X, const uint16_t *Xh, const
   if ((r = hipMemsetAsync(nf, 0, sizeof(int), e->stream)) != hipSuccess) return r;
   if ((r = hipMemsetAsync(nc2, 0, sizeof(int), e->stream)) != hipSuccess) return r;
   if (rows <= 0) return hipSuccess;
-  const float ub = s->fp16 ? 4.8828125e-4f : 3.90625e-3f;       // 2^-11, 2^-8
   const float eta = s->fp16 ? 2.9802322e-8f / s->scale : 0.0f;  // half FP16 subnormal step
   const float inv_s2 = 1.0f / (s->scale * s->scale);
   dim3 g(km_cdiv(rows, MF_PTS));
   switch (s->KS) {
-    case 1: launch_mfma_ks<1>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, ub, eta, lab, s->flags, nf, s->cands, nc2); break;
-    case 2: launch_mfma_ks<2>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, ub, eta, lab, s->flags, nf, s->cands, nc2); break;
-    case 4: launch_mfma_ks<4>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, ub, eta, lab, s->flags, nf, s->cands, nc2); break;
-    case 6: launch_mfma_ks<6>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, ub, eta, lab, s->flags, nf, s->cands, nc2); break;
-    case 8: launch_mfma_ks<8>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, ub, eta, lab, s->flags, nf, s->cands, nc2); break;
-    default: launch_mfma_ks<16>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, ub, eta, lab, s->flags, nf, s->cands, nc2); break;
+    case 1: launch_mfma_ks<1>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, s->cands, nc2); break;
+    case 2: launch_mfma_ks<2>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, s->cands, nc2); break;
+    case 4: launch_mfma_ks<4>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, s->cands, nc2); break;
+    case 6: launch_mfma_ks<6>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, s->cands, nc2); break;
+    case 8: launch_mfma_ks<8>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, s->cands, nc2); break;
+    default: launch_mfma_ks<16>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, s->cands, nc2); break;
   }
   if ((r = hipGetLastError()) != hipSuccess) return r;
   return hipGetLastError();
@@ -924,14 +978,14 @@ hipError_t mfma_prepare(Engine *e) {
 // FP32 full search for the uncertified points (device-side count).
 hipError_t mfma_reresolve(Engine *e, long r0, long rows) {
   MfmaState *s = st(e);
-  return mfma_reresolve_view(e, e->dX + r0 * e->n, s->Xh + r0 * s->Dp, s->xnorm + r0,
+  return mfma_reresolve_view(e, e->dX + r0 * e->n, s->Xh + r0 * s->Dp, s->xnb + r0,
                              e->dlabels + r0, rows);
 }
 
 // Points left open by the assign kernel: deep MFMA pass, then the FP32 full
 // search of what remains (device-side counts; X, Xh, xn are the same rows
 // the assign pass saw, indexed from 0).
-hipError_t mfma_reresolve_view(Engine *e, const float *X, const uint16_t *Xh, const float *xn,
+hipError_t mfma_reresolve_view(Engine *e, const float *X, const uint16_t *Xh, const float2 *xn,
                                int *lab, long rows) {
   MfmaState *s = st(e);
   hipError_t r;
@@ -960,12 +1014,12 @@ hipError_t mfma_reresolve_view(Engine *e, const float *X, const uint16_t *Xh, co
 }
 
 const uint16_t *mfma_tile_xh(Engine *e) { return st(e)->Xh; }
-const float *mfma_tile_xnorm(Engine *e) { return st(e)->xnorm; }
+const float2 *mfma_tile_xnorm(Engine *e) { return st(e)->xnb; }
 int mfma_row_stride(Engine *e) { return st(e)->Dp; }
 bool mfma_view(Engine *e, MfmaView *v) {
   MfmaState *s = st(e);
   if (!s || !s->x_ready || !s->c_ready || e->streaming) return false;
-  *v = MfmaView{s->Xh, s->Ch, s->bias, s->bnd, s->xnorm, s->Dp, s->KS, s->Kp, s->fp16};
+  *v = MfmaView{s->Xh, s->Ch, s->bias, s->bnd, s->xnb, s->Dp, s->KS, s->Kp, s->fp16};
   return true;
 }
 bool mfma_ready(Engine *e) { return st(e) && st(e)->x_ready && !e->streaming; }

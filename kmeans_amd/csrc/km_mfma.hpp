@@ -14,17 +14,19 @@ void mfma_centers_changed(Engine *e, bool new_range);
 hipError_t mfma_ensure_slots(Engine *e, int n);
 hipError_t mfma_prepare(Engine *e);
 hipError_t mfma_prepare_rows(Engine *e, const float *X, long rows);
-hipError_t mfma_assign_view(Engine *e, const float *X, const uint16_t *Xh, const float *xn,
+// xn: per point upper bounds of |x| and |x - xh/s| (FP32 units)
+hipError_t mfma_assign_view(Engine *e, const float *X, const uint16_t *Xh, const float2 *xn,
                             int *lab, long rows);
-hipError_t mfma_reresolve_view(Engine *e, const float *X, const uint16_t *Xh, const float *xn,
+hipError_t mfma_reresolve_view(Engine *e, const float *X, const uint16_t *Xh, const float2 *xn,
                                int *lab, long rows);
 const uint16_t *mfma_tile_xh(Engine *e);
-const float *mfma_tile_xnorm(Engine *e);
+const float2 *mfma_tile_xnorm(Engine *e);
 int mfma_row_stride(Engine *e);  // padded width Dp of the reduced-precision rows
 // Read-only view of the reduced-precision state (valid after mfma_prepare).
 struct MfmaView {
   const uint16_t *Xh, *Ch;
-  const float *bias, *bnd, *xnorm;
+  const float *bias, *bnd;    // bnd = [max |c_j|, max |c_j|^2, max |c_j - ch_j/s|] bounds
+  const float2 *xnorm;        // per point bounds of |x| and |x - xh/s|
   int Dp, KS, Kp;
   bool fp16;
 };
