@@ -275,36 +275,54 @@ __global__ __launch_bounds__(256) void k_scan_down(const int *__restrict__ in, l
 }
 
 // Stable scatter: one wave per sub-block walks its points in order; lanes
-// with equal labels are ranked by a ballot match over the label bits.
+// with equal labels are ranked by a ballot match over the label bits.  The
+// per-label write positions of the sub-block (offs) are held in LDS; the
+// labels of 16 steps (1024 points) are read into registers at once and the
+// next 16 are in flight while these are ranked.
+#define SC_STEPS 16
 __global__ __launch_bounds__(64) void k_scatter(const int *__restrict__ labels, long i0, long nb,
                                                 int K, int nsb, int SB, int nbits,
                                                 const int *__restrict__ offs,
                                                 int *__restrict__ perm) {
   extern __shared__ int cnt[];
-  for (int j = threadIdx.x; j < K; j += 64) cnt[j] = 0;
+  for (int j = threadIdx.x; j < K; j += 64) cnt[j] = offs[(long)j * nsb + blockIdx.x];
   __syncthreads();
   const int s = blockIdx.x, lane = threadIdx.x;
   const long a = (long)s * SB, e = min(a + SB, nb);
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  for (long base = a; base < e; base += 64) {
-    const long i = base + lane;
-    const bool ok = i < e;
-    const int l = ok ? labels[i] : 0;
-    uint64_t peers = __ballot(ok);
-    for (int b = 0; b < nbits; ++b) {
-      const bool bit = (l >> b) & 1;
-      const uint64_t m = __ballot(bit);
-      peers &= bit ? m : ~m;
+  int cur[SC_STEPS], nxt[SC_STEPS];
+  auto fetch = [&](long b0, int (&v)[SC_STEPS]) {
+#pragma unroll
+    for (int u = 0; u < SC_STEPS; ++u) {
+      const long i = b0 + u * 64 + lane;
+      v[u] = i < e ? labels[i] : 0;
     }
-    int c = 0;
-    if (ok) {
-      c = cnt[l];
-      const int rank = __popcll(peers & below);
-      perm[offs[(long)l * nsb + s] + c + rank] = (int)(i0 + i);
+  };
+  fetch(a, cur);
+  for (long b0 = a; b0 < e; b0 += 64 * SC_STEPS) {
+    fetch(b0 + 64 * SC_STEPS, nxt);  // zeros past the sub-block
+#pragma unroll
+    for (int u = 0; u < SC_STEPS; ++u) {
+      const long i = b0 + u * 64 + lane;
+      const bool ok = i < e;
+      const int l = cur[u];
+      uint64_t peers = __ballot(ok);
+      for (int b = 0; b < nbits; ++b) {
+        const bool bit = (l >> b) & 1;
+        const uint64_t m = __ballot(bit);
+        peers &= bit ? m : ~m;
+      }
+      int c = 0;
+      if (ok) {
+        c = cnt[l];
+        perm[c + __popcll(peers & below)] = (int)(i0 + i);
+      }
+      __syncthreads();
+      if (ok && (peers >> lane) == 1ull) cnt[l] = c + __popcll(peers);
+      __syncthreads();
     }
-    __syncthreads();
-    if (ok && (peers >> lane) == 1ull) cnt[l] = c + __popcll(peers);
-    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < SC_STEPS; ++u) cur[u] = nxt[u];
   }
 }
 
@@ -406,6 +424,91 @@ __global__ __launch_bounds__(320) void k_chunk_sums(const float *__restrict__ X,
   }
 }
 
+// The same first-level sums with four dimensions per thread (D % 4 == 0,
+// rows read as float4): TPC threads per chunk (D/4 rounded up to a power of
+// two) and G = min(256/TPC, 64) chunks per block; every dimension still adds
+// its members in ascending point order, eight rows in flight per thread.
+// Thread 256 + g sums the q column of chunk g.
+__global__ __launch_bounds__(320) void k_chunk_sums4(const float *__restrict__ X, int D, int TPC,
+                                                     const double *__restrict__ q,
+                                                     const int *__restrict__ perm, int K,
+                                                     const int *__restrict__ cstart,
+                                                     const int *__restrict__ ccount,
+                                                     const int *__restrict__ cq,
+                                                     double *__restrict__ P) {
+  __shared__ int pidx[16][KM_CHUNK];
+  const int G = min(256 / TPC, 64);
+  const bool qwave = threadIdx.x >= 256;
+  const int sub = qwave ? (int)threadIdx.x - 256 : threadIdx.x / TPC;
+  const int t = qwave ? 0 : threadIdx.x - sub * TPC;
+  const int Q = cq[K];
+  const long c = (long)blockIdx.x * G + sub;
+  const int ld = D + 1;
+  int j = 0, cnt = 0, start = 0;
+  if (c < Q && sub < G) {
+    int lo = 0, hi = K - 1;  // last j with cq[j] <= c
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (cq[mid] <= c) lo = mid; else hi = mid - 1;
+    }
+    j = lo;
+    const int local = (int)(c - cq[j]);
+    start = cstart[j] + local * KM_CHUNK;
+    cnt = min(KM_CHUNK, ccount[j] - local * KM_CHUNK);
+  }
+  const bool staged = G <= 16;
+  if (!qwave && staged && c < Q && sub < G)
+    for (int m = t; m < cnt; m += TPC) pidx[sub][m] = perm[start + m];
+  __syncthreads();
+  if (c >= Q || sub >= G) return;
+  if (qwave) {
+    double acc = 0.0;
+    int m = 0;
+    for (; m + 8 <= cnt; m += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = q[staged ? pidx[sub][m + u] : perm[start + m + u]];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc += v[u];
+    }
+    for (; m < cnt; ++m) acc += q[staged ? pidx[sub][m] : perm[start + m]];
+    P[c * ld + D] = acc;
+    return;
+  }
+  if (4 * t >= D) return;
+  const float4 *__restrict__ X4 = reinterpret_cast<const float4 *>(X);
+  const long D4 = D / 4;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int m = 0;
+  for (; m + 8 <= cnt; m += 8) {
+    float4 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int pp = staged ? pidx[sub][m + u] : perm[start + m + u];
+      v[u] = X4[(long)pp * D4 + t];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      a0 += (double)v[u].x;
+      a1 += (double)v[u].y;
+      a2 += (double)v[u].z;
+      a3 += (double)v[u].w;
+    }
+  }
+  for (; m < cnt; ++m) {
+    const float4 v = X4[(long)(staged ? pidx[sub][m] : perm[start + m]) * D4 + t];
+    a0 += (double)v.x;
+    a1 += (double)v.y;
+    a2 += (double)v.z;
+    a3 += (double)v.w;
+  }
+  double *o = P + c * ld + 4 * t;
+  o[0] = a0;
+  o[1] = a1;
+  o[2] = a2;
+  o[3] = a3;
+}
+
 // Per-centre CSUM over its chunk partials (D sums and the q column),
 // added into the running block sums.
 __global__ __launch_bounds__(256) void k_block_finalize(int D, int K, const int *__restrict__ ccount,
@@ -419,7 +522,15 @@ __global__ __launch_bounds__(256) void k_block_finalize(int D, int K, const int 
   for (int d = threadIdx.x; d <= D; d += blockDim.x) {
     double b = 0.0;
     if (nq <= KM_CHUNK) {
-      for (int t = 0; t < nq; ++t) b += P[(long)(q0 + t) * ld + d];
+      int t = 0;
+      for (; t + 8 <= nq; t += 8) {  // eight loads in flight, added in order
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = P[(long)(q0 + t + u) * ld + d];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) b += v[u];
+      }
+      for (; t < nq; ++t) b += P[(long)(q0 + t) * ld + d];
     } else {  // nq <= KM_CHUNK^2 since KM_BLOCK <= KM_CHUNK^3
       for (int g = 0; g < nq; g += KM_CHUNK) {
         double gs = 0.0;
@@ -696,8 +807,16 @@ hipError_t launch_update_accumulate(hipStream_t s, long n, int d, int k, const f
     hipLaunchKernelGGL(k_cluster_setup, dim3(1), dim3(256), 0, s, k, nsb, ws.offs, ws.cstart,
                        ws.ccount, ws.cq);
     const long qmax = nb / KM_CHUNK + k + 1;
-    hipLaunchKernelGGL(k_chunk_sums, dim3(km_cdiv(qmax, G)), dim3(320), 0, s, X, d, Dp, qb, ws.perm,
-                       k, ws.cstart, ws.ccount, ws.cq, ws.P);
+    if (d % 4 == 0 && d >= 16 && d <= 256 && ((uintptr_t)X & 15) == 0) {
+      int TPC = 1;
+      while (TPC < d / 4) TPC <<= 1;
+      const int G4 = 256 / TPC < 64 ? 256 / TPC : 64;  // must match k_chunk_sums4
+      hipLaunchKernelGGL(k_chunk_sums4, dim3(km_cdiv(qmax, G4)), dim3(320), 0, s, X, d, TPC, qb,
+                         ws.perm, k, ws.cstart, ws.ccount, ws.cq, ws.P);
+    } else {
+      hipLaunchKernelGGL(k_chunk_sums, dim3(km_cdiv(qmax, G)), dim3(320), 0, s, X, d, Dp, qb, ws.perm,
+                         k, ws.cstart, ws.ccount, ws.cq, ws.P);
+    }
     hipLaunchKernelGGL(k_block_finalize, dim3(k), dim3(64 * (int)km_cdiv(d + 1 < 256 ? d + 1 : 256, 64)),
                        0, s, d, k, ws.ccount, ws.cq, ws.P, S, cnt);
     if ((e = hipGetLastError()) != hipSuccess) return e;
