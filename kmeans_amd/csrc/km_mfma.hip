@@ -72,7 +72,6 @@ struct MfmaState {
   float *dcn = nullptr;     // Kp: upper bound of |c_j - ch_j/s|
   float2 *xnb = nullptr;    // m: upper bounds of |x_i| and |x_i - xh_i/s|
   int *flags = nullptr;     // m
-  int4 *cands = nullptr;    // unused: certificate 2 runs inside the assign kernel
   int *ncand = nullptr;     // per iteration slot
   int *nflag = nullptr;     // per iteration slot
   int *flags2 = nullptr;    // m: points left open by the deep pass
@@ -182,8 +181,7 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
     const uint16_t *__restrict__ Ch, const float *__restrict__ bias, const float *__restrict__ C,
     const float *__restrict__ nc, const float *__restrict__ bnd, const float2 *__restrict__ xnb,
     float inv_s2, float eta, float big, int *__restrict__ labels,
-    int *__restrict__ flags, int *__restrict__ nflag, int4 *__restrict__ cands,
-    int *__restrict__ ncand) {
+    int *__restrict__ flags, int *__restrict__ nflag, int *__restrict__ ncand) {
   constexpr int Dp = KS * 16;
   constexpr int CPR = Dp / 8;               // 16-byte chunks per centre row
   constexpr int SROWS = MF_STAGE * MF_TILE; // centre rows per LDS stage
@@ -682,7 +680,6 @@ void mfma_free(Engine *e) {
   hipFree(s->xnb);
   hipFree(s->dcn);
   hipFree(s->flags);
-  hipFree(s->cands);
   hipFree(s->ncand);
   hipFree(s->nflag);
   hipFree(s->flags2);
@@ -873,7 +870,7 @@ static void launch_mfma_ks(hipStream_t strm, bool fp16, dim3 g, long M, const ui
                            const float *X, int D, int K, int Kp, const uint16_t *Ch,
                            const float *bias, const float *C, const float *nc, const float *bnd,
                            const float2 *xnb, float inv_s2, float eta, int *labels,
-                           int *flags, int *nflag, int4 *cands, int *ncand) {
+                           int *flags, int *nflag, int *ncand) {
   const size_t lds = 2 * (size_t)MF_STAGE * MF_TILE * (KS * 2) * 16 + 2 * MF_STAGE * MF_TILE * 4;
   static bool attr_set[2] = {false, false};
   if (!attr_set[fp16]) {
@@ -887,10 +884,10 @@ static void launch_mfma_ks(hipStream_t strm, bool fp16, dim3 g, long M, const ui
   }
   if (fp16)
     hipLaunchKernelGGL((k_assign_mfma<KS, true>), g, dim3(64 * MF_WAVES), lds, strm, M, Xh, X, D, K, Kp, Ch,
-                       bias, C, nc, bnd, xnb, inv_s2, eta, FLT_MAX, labels, flags, nflag, cands, ncand);
+                       bias, C, nc, bnd, xnb, inv_s2, eta, FLT_MAX, labels, flags, nflag, ncand);
   else
     hipLaunchKernelGGL((k_assign_mfma<KS, false>), g, dim3(64 * MF_WAVES), lds, strm, M, Xh, X, D, K, Kp, Ch,
-                       bias, C, nc, bnd, xnb, inv_s2, eta, FLT_MAX, labels, flags, nflag, cands, ncand);
+                       bias, C, nc, bnd, xnb, inv_s2, eta, FLT_MAX, labels, flags, nflag, ncand);
 }
 
 hipError_t mfma_assign(Engine *e, long r0, long rows) {
@@ -924,12 +921,12 @@ hipError_t mfma_assign_view(Engine *e, const float *X, const uint16_t *Xh, const
   const float inv_s2 = 1.0f / (s->scale * s->scale);
   dim3 g(km_cdiv(rows, MF_PTS));
   switch (s->KS) {
-    case 1: launch_mfma_ks<1>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, s->cands, nc2); break;
-    case 2: launch_mfma_ks<2>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, s->cands, nc2); break;
-    case 4: launch_mfma_ks<4>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, s->cands, nc2); break;
-    case 6: launch_mfma_ks<6>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, s->cands, nc2); break;
-    case 8: launch_mfma_ks<8>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, s->cands, nc2); break;
-    default: launch_mfma_ks<16>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, s->cands, nc2); break;
+    case 1: launch_mfma_ks<1>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2); break;
+    case 2: launch_mfma_ks<2>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2); break;
+    case 4: launch_mfma_ks<4>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2); break;
+    case 6: launch_mfma_ks<6>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2); break;
+    case 8: launch_mfma_ks<8>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2); break;
+    default: launch_mfma_ks<16>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2); break;
   }
   if ((r = hipGetLastError()) != hipSuccess) return r;
   return hipGetLastError();
