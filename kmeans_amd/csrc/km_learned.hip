@@ -1035,6 +1035,175 @@ __global__ __launch_bounds__(256) void k_mlp_propose(long rows, int din, int G, 
   }
 }
 
+// Weight-stationary fused MLP inference and top-T proposal (persistent: one
+// workgroup of H/32 waves per CU walks 64-row blocks).  Wave w owns neuron
+// tile w of layers 1 and 2: its W1 and W2 fragments and biases stay in
+// registers for the whole kernel; W3 is copied once into LDS.  Per block:
+// layer 1 from the rows' BF16 fragments (read one block ahead into
+// registers), layer 2 from H1 in LDS, layer 3 by (group tile, row tile)
+// pairs from H2 and W3 in LDS into FP32 logits, then the top T of each row
+// by H/32 lanes (strided scans, shuffle merge; ties to the lower group
+// index, as a left-to-right scan keeps them).  The products, accumulation
+// order and roundings are those of k_mlp_propose.  Requires din = 16*KI,
+// G % 32 == 0, G <= 128, T <= 8.
+#define MLPW_ROWS 64
+__host__ __device__ constexpr int mlpw_ldh(int H) { return H + 8; }
+__host__ __device__ inline size_t mlpw_zs_off(int H, int G) {
+  // logits over H1 when they fit there, else after H2 (in uint16 units)
+  return (size_t)MLPW_ROWS * (G + 4) * 4 <= (size_t)MLPW_ROWS * mlpw_ldh(H) * 2
+             ? (size_t)G * mlpw_ldh(H)
+             : (size_t)G * mlpw_ldh(H) + 2 * (size_t)MLPW_ROWS * mlpw_ldh(H);
+}
+__host__ __device__ inline size_t mlpw_lds_bytes(int H, int G) {
+  const size_t zs_end = mlpw_zs_off(H, G) * 2 + (size_t)MLPW_ROWS * (G + 4) * 4;
+  const size_t h_end = ((size_t)G * mlpw_ldh(H) + 2 * (size_t)MLPW_ROWS * mlpw_ldh(H)) * 2;
+  return zs_end > h_end ? zs_end : h_end;
+}
+template <int KH, int KI>
+__global__ __launch_bounds__(KH / 2 * 64) void k_mlp_propose_ws(long rows, int G, int T,
+                                                              const uint16_t *__restrict__ Xh,
+                                                              const uint16_t *__restrict__ Ph,
+                                                              const float *__restrict__ P,
+                                                              const size_t *__restrict__ poff,
+                                                              int *__restrict__ cand) {
+  constexpr int H = KH * 16, NW = H / 32, din = KI * 16, ldh = mlpw_ldh(H);
+  extern __shared__ __attribute__((aligned(16))) uint16_t wsm[];
+  uint16_t *W3s = wsm;                      // G x ldh
+  uint16_t *H1 = wsm + (size_t)G * ldh;     // rows x ldh
+  uint16_t *H2 = H1 + MLPW_ROWS * ldh;      // rows x ldh
+  float *Zs = reinterpret_cast<float *>(wsm + mlpw_zs_off(H, G));  // rows x (G + 4)
+  const int ldz = G + 4;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, h = lane >> 5, col = lane & 31;
+  const uint16_t *W1 = Ph + poff[0], *W2 = Ph + poff[2], *W3 = Ph + poff[4];
+  const float *b1 = P + poff[1], *b2 = P + poff[3], *b3 = P + poff[5];
+  // stationary operands of this wave's neuron tile
+  lg_u32x4 wf1[KI], wf2[KH];
+  lg_f32x16 bias1, bias2;
+#pragma unroll
+  for (int ks = 0; ks < KI; ++ks)
+    wf1[ks] = *reinterpret_cast<const lg_u32x4 *>(W1 + (long)(w * 32 + col) * din + ks * 16 + 8 * h);
+#pragma unroll
+  for (int ks = 0; ks < KH; ++ks)
+    wf2[ks] = *reinterpret_cast<const lg_u32x4 *>(W2 + (long)(w * 32 + col) * H + ks * 16 + 8 * h);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    bias1[r] = b1[w * 32 + lg_row(r, h)];
+    bias2[r] = b2[w * 32 + lg_row(r, h)];
+  }
+  for (int e = tid; e < G * (H / 8); e += NW * 64) {  // W3 into LDS, 16 bytes at a time
+    const int g = e / (H / 8), q = e - g * (H / 8);
+    *reinterpret_cast<lg_u32x4 *>(W3s + g * ldh + q * 8) =
+        *reinterpret_cast<const lg_u32x4 *>(W3 + (long)g * H + q * 8);
+  }
+  const long nblk = (rows + MLPW_ROWS - 1) / MLPW_ROWS;
+  lg_u32x4 xf[2][KI];
+  auto load_x = [&](long blk) {
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt) {
+      const long pr = blk * MLPW_ROWS + rt * 32 + col;
+#pragma unroll
+      for (int ks = 0; ks < KI; ++ks)
+        xf[rt][ks] = (blk < nblk && pr < rows)
+                         ? *reinterpret_cast<const lg_u32x4 *>(Xh + pr * din + ks * 16 + 8 * h)
+                         : lg_u32x4{0u, 0u, 0u, 0u};
+    }
+  };
+  auto store_relu = [&](uint16_t *dst_rows, int nt, int rt, const lg_f32x16 &acc) {
+    uint16_t *dst = dst_rows + (rt * 32 + col) * ldh;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float v[4] = {acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]};
+      mlpf_store4(dst, nt * 32 + 8 * q + 4 * h, v);
+    }
+  };
+  load_x(blockIdx.x);
+  __syncthreads();  // W3 in LDS
+  for (long blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    // layer 1
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt) {
+      lg_f32x16 a = bias1;
+#pragma unroll
+      for (int ks = 0; ks < KI; ++ks)
+        a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(lg_bf16x8, wf1[ks]),
+                                                    __builtin_bit_cast(lg_bf16x8, xf[rt][ks]), a, 0, 0, 0);
+      store_relu(H1, w, rt, a);
+    }
+    load_x(blk + gridDim.x);  // next block's rows, in flight through layers 2 and 3
+    __syncthreads();
+    // layer 2
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt) {
+      lg_f32x16 a = bias2;
+      const uint16_t *src = H1 + (rt * 32 + col) * ldh + 8 * h;
+#pragma unroll
+      for (int ks = 0; ks < KH; ++ks)
+        a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+            __builtin_bit_cast(lg_bf16x8, wf2[ks]),
+            __builtin_bit_cast(lg_bf16x8, *reinterpret_cast<const lg_u32x4 *>(src + ks * 16)), a, 0, 0, 0);
+      store_relu(H2, w, rt, a);
+    }
+    __syncthreads();
+    // layer 3: (group tile, row tile) pairs
+    for (int pr = w; pr < (G / 32) * 2; pr += NW) {
+      const int nt = pr >> 1, rt = pr & 1;
+      lg_f32x16 a;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) a[r] = b3[nt * 32 + lg_row(r, h)];
+      const uint16_t *wa = W3s + (nt * 32 + col) * ldh + 8 * h;
+      const uint16_t *src = H2 + (rt * 32 + col) * ldh + 8 * h;
+#pragma unroll
+      for (int ks = 0; ks < KH; ++ks)
+        a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+            __builtin_bit_cast(lg_bf16x8, *reinterpret_cast<const lg_u32x4 *>(wa + ks * 16)),
+            __builtin_bit_cast(lg_bf16x8, *reinterpret_cast<const lg_u32x4 *>(src + ks * 16)), a, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) Zs[(rt * 32 + col) * ldz + nt * 32 + lg_row(r, h)] = a[r];
+    }
+    __syncthreads();
+    // top T per row: NW lanes per row, lane sub scans groups sub, sub+NW, ...
+    {
+      const int row = tid / NW, sub = tid - row * NW;
+      const float *z = Zs + row * ldz;
+      float bv[8];
+      int bi[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) { bv[t] = -INFINITY; bi[t] = 0x7fffffff; }
+      auto insert = [&](float v, int gi) {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          if (t < T && (v > bv[t] || (v == bv[t] && gi < bi[t]))) {
+            const float tv = bv[t];
+            const int ti = bi[t];
+            bv[t] = v; bi[t] = gi; v = tv; gi = ti;
+          }
+        }
+      };
+      for (int g = sub; g < G; g += NW) insert(z[g], g);
+#pragma unroll
+      for (int off = 1; off < NW; off <<= 1) {
+        float pv[8];
+        int pi[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          pv[t] = __shfl_xor(bv[t], off);
+          pi[t] = __shfl_xor(bi[t], off);
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+          if (t < T) insert(pv[t], pi[t]);
+      }
+      const long prow = blk * MLPW_ROWS + row;
+      if (sub == 0 && prow < rows) {
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+          if (t < T) cand[prow * T + t] = (bv[t] == -INFINITY) ? -1 : bi[t];
+      }
+    }
+    __syncthreads();  // H1 (and the logits) are rewritten by the next block
+  }
+}
+
 // Uncertified rows gathered for the reduced-precision full search: FP32
 // row, reduced-precision row (stride Dp) and norm bound.
 __global__ void k_gather_fb(long cnt, const int *__restrict__ fb, int n, int Dp,
@@ -1365,6 +1534,32 @@ static hipError_t refresh_centres(Engine *e, Learned *L) {
 static hipError_t propose(Engine *e, Learned *L) {
   hipStream_t s = e->stream;
   const long m = e->m;
+  if (L->proposer == KM_PROPOSER_MLP && (L->H == 128 || L->H == 256) && L->G % 32 == 0 &&
+      L->G <= 128 && L->T <= 8 && L->din % 16 == 0 && L->din <= 128 && L->din / 16 != 7 &&
+      L->din / 16 != 5 && L->din / 16 != 3) {
+    // weight-stationary persistent inference (one workgroup per CU)
+    const size_t lds = mlpw_lds_bytes(L->H, L->G);
+    const dim3 grid((unsigned)std::min<long>((m + MLPW_ROWS - 1) / MLPW_ROWS, (long)num_cus()));
+#define MLPW(KH, KI)                                                                              \
+  do {                                                                                           \
+    LCHK(hipFuncSetAttribute((const void *)k_mlp_propose_ws<KH, KI>,                             \
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));             \
+    hipLaunchKernelGGL((k_mlp_propose_ws<KH, KI>), grid, dim3(KH / 2 * 64), lds, s, m, L->G, L->T, \
+                       L->Xh, L->Ph, L->P, L->dpoff, L->cand);                                   \
+  } while (0)
+#define MLPW_KI(KH)                           \
+  switch (L->din / 16) {                      \
+    case 1: MLPW(KH, 1); break;               \
+    case 2: MLPW(KH, 2); break;               \
+    case 4: MLPW(KH, 4); break;               \
+    case 6: MLPW(KH, 6); break;               \
+    default: MLPW(KH, 8); break;              \
+  }
+    if (L->H == 256) { MLPW_KI(16); } else { MLPW_KI(8); }
+#undef MLPW_KI
+#undef MLPW
+    return hipGetLastError();
+  }
   if (L->proposer == KM_PROPOSER_MLP && (L->H == 128 || L->H == 256) && L->G % 32 == 0 &&
       (size_t)MLPF_ROWS * (L->G + 4) * 4 <= (size_t)MLPF_ROWS * (L->H + 8) * 2) {
     // fused inference: no hidden or logit matrices in HBM
