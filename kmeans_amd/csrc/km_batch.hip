@@ -235,12 +235,7 @@ typedef __attribute__((ext_vector_type(8))) __bf16 b_bf16x8;
 typedef __attribute__((ext_vector_type(16))) float b_f32x16;
 typedef __attribute__((ext_vector_type(4))) unsigned int b_u32x4;
 __device__ inline int b_row(int r, int h) { return 8 * (r >> 2) + 4 * h + (r & 3); }
-__device__ inline uint16_t b_bf16(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
+__device__ inline uint16_t b_bf16(float f) { return bf16_rne(f); }
 
 template <int KS>
 __global__ __launch_bounds__(256) void k_batch_assign_bf16(

@@ -25,6 +25,17 @@ static inline long km_cdiv(long a, long b) { return (a + b - 1) / b; }
 
 namespace km {
 
+// FP32 -> BF16 bits, round to nearest even, by the gfx950 conversion
+// instruction (v_cvt_pk_bf16_f32; NaN stays NaN, infinities and subnormals
+// convert exactly as the RNE rule gives).
+__device__ inline uint16_t bf16_rne(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+// two values packed into one word (first in the low half), one instruction
+__device__ inline uint32_t bf16x2_rne(float a, float b) {
+  typedef __attribute__((ext_vector_type(2))) float f2_t;
+  typedef __attribute__((ext_vector_type(2))) __bf16 b2_t;
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector((f2_t){a, b}, b2_t));
+}
+
 // Counter-based generator shared with the oracle: SplitMix64 finaliser
 // (Steele, Lea, Flood, "Fast splittable pseudorandom number generators",
 // OOPSLA 2014) applied to (seed, stream, index).

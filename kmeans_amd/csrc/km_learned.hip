@@ -911,8 +911,8 @@ __global__ __launch_bounds__(256) void k_group_mfma(
 #define MLPF_ROWS 64
 __device__ inline void mlpf_store4(uint16_t *row, int n0, const float *v) {
   // four consecutive neurons n0..n0+3 as BF16, ReLU applied
-  const uint32_t a = (uint32_t)mlp_bf16(fmaxf(v[0], 0.0f)) | ((uint32_t)mlp_bf16(fmaxf(v[1], 0.0f)) << 16);
-  const uint32_t b = (uint32_t)mlp_bf16(fmaxf(v[2], 0.0f)) | ((uint32_t)mlp_bf16(fmaxf(v[3], 0.0f)) << 16);
+  const uint32_t a = bf16x2_rne(fmaxf(v[0], 0.0f), fmaxf(v[1], 0.0f));
+  const uint32_t b = bf16x2_rne(fmaxf(v[2], 0.0f), fmaxf(v[3], 0.0f));
   *reinterpret_cast<uint2 *>(row + n0) = make_uint2(a, b);
 }
 
@@ -1042,10 +1042,13 @@ __global__ __launch_bounds__(256) void k_mlp_propose(long rows, int din, int G, 
 // layer 1 from the rows' BF16 fragments (read one block ahead into
 // registers), layer 2 from H1 in LDS, layer 3 by (group tile, row tile)
 // pairs from H2 and W3 in LDS into FP32 logits, then the top T of each row
-// by H/32 lanes (strided scans, shuffle merge; ties to the lower group
-// index, as a left-to-right scan keeps them).  The products, accumulation
-// order and roundings are those of k_mlp_propose.  Requires din = 16*KI,
-// G % 32 == 0, G <= 128, T <= 8.
+// by H/32 lanes: each logit carries its group in the low 7 bits (G - 1 - g
+// for non-negative values, g for negative ones, so equal logits order by
+// the lower group index), the lanes keep a top-TT by a v_med3/v_max network
+// over strided groups and merge by shuffles.  The products, accumulation
+// order and roundings are those of k_mlp_propose; the packed bits move a
+// logit by less than 2^-16 of its value.  Requires din = 16*KI, G % 32 ==
+// 0, G <= 128, T <= TT <= 8.
 #define MLPW_ROWS 64
 __host__ __device__ constexpr int mlpw_ldh(int H) { return H + 8; }
 __host__ __device__ inline size_t mlpw_zs_off(int H, int G) {
@@ -1059,7 +1062,7 @@ __host__ __device__ inline size_t mlpw_lds_bytes(int H, int G) {
   const size_t h_end = ((size_t)G * mlpw_ldh(H) + 2 * (size_t)MLPW_ROWS * mlpw_ldh(H)) * 2;
   return zs_end > h_end ? zs_end : h_end;
 }
-template <int KH, int KI>
+template <int KH, int KI, int TT>
 __global__ __launch_bounds__(KH / 2 * 64) void k_mlp_propose_ws(long rows, int G, int T,
                                                               const uint16_t *__restrict__ Xh,
                                                               const uint16_t *__restrict__ Ph,
@@ -1165,39 +1168,38 @@ __global__ __launch_bounds__(KH / 2 * 64) void k_mlp_propose_ws(long rows, int G
     {
       const int row = tid / NW, sub = tid - row * NW;
       const float *z = Zs + row * ldz;
-      float bv[8];
-      int bi[8];
+      float bv[TT];
 #pragma unroll
-      for (int t = 0; t < 8; ++t) { bv[t] = -INFINITY; bi[t] = 0x7fffffff; }
-      auto insert = [&](float v, int gi) {
+      for (int t = 0; t < TT; ++t) bv[t] = -INFINITY;
+      auto insert = [&](float p) {  // sorted descending; packed values are distinct
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          if (t < T && (v > bv[t] || (v == bv[t] && gi < bi[t]))) {
-            const float tv = bv[t];
-            const int ti = bi[t];
-            bv[t] = v; bi[t] = gi; v = tv; gi = ti;
-          }
-        }
+        for (int t = TT - 1; t > 0; --t) bv[t] = __builtin_amdgcn_fmed3f(bv[t - 1], bv[t], p);
+        bv[0] = fmaxf(bv[0], p);
       };
-      for (int g = sub; g < G; g += NW) insert(z[g], g);
+      for (int g = sub; g < G; g += NW) {
+        float v = z[g];
+        v = v == v ? v : -FLT_MAX;  // NaN logits rank last
+        const unsigned u = __float_as_uint(v);
+        const unsigned id = (u >> 31) ? (unsigned)g : (unsigned)(127 - g);
+        insert(__uint_as_float((u & ~127u) | id));
+      }
 #pragma unroll
       for (int off = 1; off < NW; off <<= 1) {
-        float pv[8];
-        int pi[8];
+        float pv[TT];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          pv[t] = __shfl_xor(bv[t], off);
-          pi[t] = __shfl_xor(bi[t], off);
-        }
+        for (int t = 0; t < TT; ++t) pv[t] = __shfl_xor(bv[t], off);
 #pragma unroll
-        for (int t = 0; t < 8; ++t)
-          if (t < T) insert(pv[t], pi[t]);
+        for (int t = 0; t < TT; ++t) insert(pv[t]);
       }
       const long prow = blk * MLPW_ROWS + row;
       if (sub == 0 && prow < rows) {
 #pragma unroll
-        for (int t = 0; t < 8; ++t)
-          if (t < T) cand[prow * T + t] = (bv[t] == -INFINITY) ? -1 : bi[t];
+        for (int t = 0; t < TT; ++t) {
+          if (t < T) {
+            const unsigned u = __float_as_uint(bv[t]), id = u & 127u;
+            cand[prow * T + t] = bv[t] == -INFINITY ? -1 : (int)((u >> 31) ? id : 127u - id);
+          }
+        }
       }
     }
     __syncthreads();  // H1 (and the logits) are rewritten by the next block
@@ -1542,10 +1544,17 @@ static hipError_t propose(Engine *e, Learned *L) {
     const dim3 grid((unsigned)std::min<long>((m + MLPW_ROWS - 1) / MLPW_ROWS, (long)num_cus()));
 #define MLPW(KH, KI)                                                                              \
   do {                                                                                           \
-    LCHK(hipFuncSetAttribute((const void *)k_mlp_propose_ws<KH, KI>,                             \
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));             \
-    hipLaunchKernelGGL((k_mlp_propose_ws<KH, KI>), grid, dim3(KH / 2 * 64), lds, s, m, L->G, L->T, \
-                       L->Xh, L->Ph, L->P, L->dpoff, L->cand);                                   \
+    if (L->T <= 4) {                                                                             \
+      LCHK(hipFuncSetAttribute((const void *)k_mlp_propose_ws<KH, KI, 4>,                        \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));           \
+      hipLaunchKernelGGL((k_mlp_propose_ws<KH, KI, 4>), grid, dim3(KH / 2 * 64), lds, s, m, L->G, \
+                         L->T, L->Xh, L->Ph, L->P, L->dpoff, L->cand);                           \
+    } else {                                                                                     \
+      LCHK(hipFuncSetAttribute((const void *)k_mlp_propose_ws<KH, KI, 8>,                        \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));           \
+      hipLaunchKernelGGL((k_mlp_propose_ws<KH, KI, 8>), grid, dim3(KH / 2 * 64), lds, s, m, L->G, \
+                         L->T, L->Xh, L->Ph, L->P, L->dpoff, L->cand);                           \
+    }                                                                                            \
   } while (0)
 #define MLPW_KI(KH)                           \
   switch (L->din / 16) {                      \

@@ -46,13 +46,9 @@ typedef __attribute__((ext_vector_type(4))) unsigned int u32x4_t;
 #define MF_PTS (64 * MF_WAVES)  // points per workgroup (2 groups of 32 per wave)
 #define MF_STAGE 4     // centre tiles per LDS stage (128 centres)
 
-// Round-to-nearest-even conversions written out so the rounding is known.
-__device__ inline uint16_t f32_to_bf16_rne(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
+// Round-to-nearest-even conversions (the hardware instructions; the
+// certificate only needs the rounded values to be those the MFMAs see).
+__device__ inline uint16_t f32_to_bf16_rne(float f) { return bf16_rne(f); }
 __device__ inline uint16_t f32_to_f16_rne(float f) {
   _Float16 h = (_Float16)f;  // IEEE conversion, round to nearest even
   return __builtin_bit_cast(uint16_t, h);

@@ -4,12 +4,7 @@
 
 namespace km {
 // FP32 -> BF16 bits, round to nearest even (NaN stays NaN)
-__device__ inline uint16_t mlp_bf16(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
+__device__ inline uint16_t mlp_bf16(float f) { return bf16_rne(f); }
 
 struct MlpShape {
   int din, h, g;  // input width (multiple of 16), hidden width, outputs
