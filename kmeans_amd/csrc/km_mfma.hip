@@ -45,6 +45,11 @@ typedef __attribute__((ext_vector_type(4))) unsigned int u32x4_t;
 #endif
 #define MF_PTS (64 * MF_WAVES)  // points per workgroup (2 groups of 32 per wave)
 #define MF_STAGE 4     // centre tiles per LDS stage (128 centres)
+// rescored-point counter of one assign launch: 64 counters on separate
+// 128-byte lines (workgroup b adds into counter b % 64), summed on the host
+#define NC_SPREAD 64
+#define NC_STRIDE 32
+#define NC_SLOT (NC_SPREAD * NC_STRIDE)
 
 // Round-to-nearest-even conversions (the hardware instructions; the
 // certificate only needs the rounded values to be those the MFMAs see).
@@ -68,7 +73,7 @@ struct MfmaState {
   float *dcn = nullptr;     // Kp: upper bound of |c_j - ch_j/s|
   float2 *xnb = nullptr;    // m: upper bounds of |x_i| and |x_i - xh_i/s|
   int *flags = nullptr;     // m
-  int *ncand = nullptr;     // per iteration slot
+  int *ncand = nullptr;     // per iteration slot: NC_SPREAD counters, NC_STRIDE ints apart
   int *nflag = nullptr;     // per iteration slot
   int *flags2 = nullptr;    // m: points left open by the deep pass
   int *nflag2 = nullptr;    // per iteration slot
@@ -432,7 +437,8 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     {
       const uint64_t mk = __ballot(two);
-      if (mk && lane == __ffsll((long long)mk) - 1) atomicAdd(ncand, __popcll(mk));
+      if (mk && lane == __ffsll((long long)mk) - 1)
+        atomicAdd(ncand + (blockIdx.x & (NC_SPREAD - 1)) * NC_STRIDE, __popcll(mk));
     }
     {
       const uint64_t mk = __ballot(fall);
@@ -721,9 +727,9 @@ static hipError_t ensure_state(Engine *e) {
   if ((r = hipMalloc(&s->flags2, sizeof(int) * (size_t)xrows)) != hipSuccess) return r;
   s->nflag_slots = 64;
   if ((r = hipMalloc(&s->nflag, sizeof(int) * s->nflag_slots)) != hipSuccess) return r;
-  if ((r = hipMalloc(&s->ncand, sizeof(int) * s->nflag_slots)) != hipSuccess) return r;
+  if ((r = hipMalloc(&s->ncand, sizeof(int) * NC_SLOT * s->nflag_slots)) != hipSuccess) return r;
   if ((r = hipMalloc(&s->nflag2, sizeof(int) * s->nflag_slots)) != hipSuccess) return r;
-  if ((r = hipMemsetAsync(s->ncand, 0, sizeof(int) * s->nflag_slots, e->stream)) != hipSuccess)
+  if ((r = hipMemsetAsync(s->ncand, 0, sizeof(int) * NC_SLOT * s->nflag_slots, e->stream)) != hipSuccess)
     return r;
   if ((r = hipMemsetAsync(s->nflag2, 0, sizeof(int) * s->nflag_slots, e->stream)) != hipSuccess)
     return r;
@@ -909,9 +915,9 @@ hipError_t mfma_assign_view(Engine *e, const float *X, const uint16_t *Xh, const
     if ((r = prepare_centres(e)) != hipSuccess) return r;
   if ((r = mfma_ensure_slots(e, e->cur_slot + 1)) != hipSuccess) return r;
   int *nf = s->nflag + e->cur_slot;
-  int *nc2 = s->ncand + e->cur_slot;
+  int *nc2 = s->ncand + (size_t)e->cur_slot * NC_SLOT;
   if ((r = hipMemsetAsync(nf, 0, sizeof(int), e->stream)) != hipSuccess) return r;
-  if ((r = hipMemsetAsync(nc2, 0, sizeof(int), e->stream)) != hipSuccess) return r;
+  if ((r = hipMemsetAsync(nc2, 0, sizeof(int) * NC_SLOT, e->stream)) != hipSuccess) return r;
   if (rows <= 0) return hipSuccess;
   const float eta = s->fp16 ? 2.9802322e-8f / s->scale : 0.0f;  // half FP16 subnormal step
   const float inv_s2 = 1.0f / (s->scale * s->scale);
@@ -934,11 +940,12 @@ hipError_t mfma_ensure_slots(Engine *e, int n) {
   MfmaState *s = st(e);
   if (n <= s->nflag_slots) return hipSuccess;
   int *arrs[3] = {s->nflag, s->ncand, s->nflag2};
+  const size_t per[3] = {1, NC_SLOT, 1};  // ints per slot
   for (int a = 0; a < 3; ++a) {
     int *nb = nullptr;
-    if ((r = hipMalloc(&nb, sizeof(int) * n)) != hipSuccess) return r;
-    if ((r = hipMemsetAsync(nb, 0, sizeof(int) * n, e->stream)) != hipSuccess) return r;
-    if ((r = hipMemcpyAsync(nb, arrs[a], sizeof(int) * s->nflag_slots, hipMemcpyDeviceToDevice,
+    if ((r = hipMalloc(&nb, sizeof(int) * per[a] * n)) != hipSuccess) return r;
+    if ((r = hipMemsetAsync(nb, 0, sizeof(int) * per[a] * n, e->stream)) != hipSuccess) return r;
+    if ((r = hipMemcpyAsync(nb, arrs[a], sizeof(int) * per[a] * s->nflag_slots, hipMemcpyDeviceToDevice,
                             e->stream)) != hipSuccess)
       return r;
     if ((r = hipStreamSynchronize(e->stream)) != hipSuccess) return r;
@@ -1030,11 +1037,13 @@ void mfma_collect(Engine *e, int nslots) {
     e->stats.reresolved += v[i];
     e->last_reresolved = v[i];
   }
-  if (hipMemcpyAsync(v.data(), s->ncand, sizeof(int) * n, hipMemcpyDeviceToHost, e->stream) !=
+  std::vector<int> w((size_t)n * NC_SLOT);
+  if (hipMemcpyAsync(w.data(), s->ncand, sizeof(int) * w.size(), hipMemcpyDeviceToHost, e->stream) !=
       hipSuccess)
     return;
   if (hipStreamSynchronize(e->stream) != hipSuccess) return;
-  for (int i = 0; i < n; ++i) e->stats.rescored += v[i];
+  for (int i = 0; i < n; ++i)
+    for (int c = 0; c < NC_SPREAD; ++c) e->stats.rescored += w[(size_t)i * NC_SLOT + c * NC_STRIDE];
 }
 
 }  // namespace km
