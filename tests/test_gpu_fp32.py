@@ -76,10 +76,14 @@ def test_assign_rows_indirect(orc):
     assert np.all(out[mask] == -7)
 
 
-def gpu_update(X, lab, C):
+def gpu_update(X, lab, C, misalign=False):
     n, d = X.shape
     k = C.shape[0]
     Xd, ld, Cd = dev(X), dev(lab), dev(C)
+    if misalign:  # the same rows at an address 4 bytes past a 16-byte boundary
+        buf = torch.empty(n * d + 1, dtype=torch.float32, device="cuda")
+        buf[1:] = Xd.reshape(-1)
+        Xd = buf[1:]
     Cn = torch.empty_like(Cd)
     cnt = torch.empty(k, dtype=torch.int64, device="cuda")
     emp = torch.zeros(1, dtype=torch.int32, device="cuda")
@@ -97,7 +101,12 @@ def gpu_update(X, lab, C):
 
 
 @pytest.mark.parametrize("n,d,k", [(5000, 1, 4), (5000, 3, 7), (20000, 16, 64), (30000, 33, 100),
-                                   (12345, 128, 1024), (4000, 300, 20)])
+                                   (12345, 128, 1024), (4000, 300, 20),
+                                   # four dimensions per thread (d % 4 == 0, 16 <= d <= 256):
+                                   # unstaged member indices (d = 20), staged (96, 256)
+                                   (50000, 20, 37), (40000, 96, 300), (8000, 256, 16),
+                                   # k > 2048: the larger scatter sub-block
+                                   (60000, 8, 2500)])
 def test_update_and_compactness_bit_exact(orc, n, d, k):
     print(f"update n={n} d={d} k={k}", flush=True)
     X, _, _, _ = orc.mixture(7 + k, n, d, k, -6, 6, 0.5, 1.5)
@@ -123,6 +132,36 @@ def test_update_skewed_and_empty(orc):
     rC, rcnt, _, rempty = orc.update(X, lab, C0)
     assert emp == rempty == k - 11
     assert np.array_equal(C1, rC) and np.array_equal(cnt, rcnt)
+    assert comp == orc.compactness(X, lab, rC)
+
+
+def test_update_misaligned_rows(orc):
+    # float4 row reads need 16-byte aligned rows; a misaligned borrowed pointer
+    # takes the one-dimension-per-thread kernel and gives the same sums
+    X, _, _, _ = orc.mixture(3, 20000, 32, 50, -6, 6, 0.5, 1.5)
+    lab = orc.assign(X, X[:50].copy())
+    C0 = X[:50].copy()
+    a = gpu_update(X, lab, C0)
+    b = gpu_update(X, lab, C0, misalign=True)
+    rC, rcnt, _, _ = orc.update(X, lab, C0)
+    assert np.array_equal(a[0], rC) and np.array_equal(b[0], rC)
+    assert a[4] == b[4] == orc.compactness(X, lab, rC)
+
+
+@pytest.mark.parametrize("d", [16, 64])
+def test_update_skewed_wide(orc, d):
+    # a giant cluster (hundreds of full 256-member chunks plus a remainder) with
+    # the four-dimensions-per-thread chunk sums, and empty clusters
+    rng = np.random.default_rng(10 + d)
+    n, k = 200_003, 40
+    X = (rng.standard_normal((n, d)) * 3).astype(np.float32)
+    lab = np.where(rng.random(n) < 0.8, 7, rng.integers(20, 30, n)).astype(np.int32)
+    C0 = rng.standard_normal((k, d)).astype(np.float32)
+    C1, cnt, emp, nc, comp = gpu_update(X, lab, C0)
+    rC, rcnt, _, rempty = orc.update(X, lab, C0)
+    assert emp == rempty == k - 11
+    assert np.array_equal(C1, rC) and np.array_equal(cnt, rcnt)
+    assert np.array_equal(nc, orc.col_norms(rC))
     assert comp == orc.compactness(X, lab, rC)
 
 
