@@ -22,6 +22,13 @@
 #define KM_LAUNCH_CHECK(code) KM_HIP_CHECK(hipGetLastError(), code)
 
 static inline long km_cdiv(long a, long b) { return (a + b - 1) / b; }
+// Blocks for a grid-stride loop over `work` items: the dispatch packet holds
+// the grid size in work-items as 32 bits, so element-wise launches over
+// m x d items (which can exceed 2^32) are capped and loop instead.
+static inline unsigned km_grid(long work, int block) {
+  const long b = km_cdiv(work, block);
+  return (unsigned)(b < (1L << 22) ? b : (1L << 22));
+}
 
 namespace km {
 

@@ -683,11 +683,12 @@ __global__ void k_generate(uint64_t seed, long i0, long n, int d, int ncomp,
 
 __global__ void k_gather_rows(long n, int d, const float *__restrict__ X,
                               const int *__restrict__ idx, float *__restrict__ out) {
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n * d) return;
-  const long r = e / d;
-  const int q = (int)(e - r * d);
-  out[e] = X[(long)idx[r] * d + q];
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n * d;
+       e += (long)gridDim.x * blockDim.x) {
+    const long r = e / d;
+    const int q = (int)(e - r * d);
+    out[e] = X[(long)idx[r] * d + q];
+  }
 }
 
 // ---------------------------------------------------------------- launchers
@@ -895,7 +896,7 @@ hipError_t launch_generate(hipStream_t s, uint64_t seed, long i0, long n, int d,
 hipError_t launch_gather_rows(hipStream_t s, long n, int d, const float *X, const int *idx,
                               float *out) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_gather_rows, dim3(km_cdiv(n * d, 256)), dim3(256), 0, s, n, d, X, idx, out);
+  hipLaunchKernelGGL(k_gather_rows, dim3(km_grid(n * d, 256)), dim3(256), 0, s, n, d, X, idx, out);
   return hipGetLastError();
 }
 

@@ -141,12 +141,13 @@ __device__ inline int block_slot(bool f, int *counter) {
 // out[r][d] = BF16(scale * X[idx ? idx[r] : r][d]) for d < n, 0 for n <= d < din.
 __global__ void k_x_bf16(long rows, int n, int din, const float *__restrict__ X,
                          const int *__restrict__ idx, float scale, uint16_t *__restrict__ out) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= rows * din) return;
-  const long r = i / din;
-  const int d = (int)(i - r * din);
-  const long src = idx ? (long)idx[r] : r;
-  out[i] = mlp_bf16(d < n ? X[src * n + d] * scale : 0.0f);
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < rows * din;
+       i += (long)gridDim.x * blockDim.x) {
+    const long r = i / din;
+    const int d = (int)(i - r * din);
+    const long src = idx ? (long)idx[r] : r;
+    out[i] = mlp_bf16(d < n ? X[src * n + d] * scale : 0.0f);
+  }
 }
 
 // Mean of q (FP64 squared norms) over sampled rows; one block.
@@ -1212,14 +1213,15 @@ __global__ void k_gather_fb(long cnt, const int *__restrict__ fb, int n, int Dp,
                             const float *__restrict__ X, const uint16_t *__restrict__ Xh,
                             const float2 *__restrict__ xn, float *__restrict__ gX,
                             uint16_t *__restrict__ gXh, float2 *__restrict__ gN) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long r = i / Dp;
-  const int d = (int)(i - r * Dp);
-  if (r >= cnt) return;
-  const long p = fb[r];
-  gXh[i] = Xh[p * Dp + d];
-  if (d < n) gX[r * n + d] = X[p * n + d];
-  if (d == 0) gN[r] = xn[p];
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < cnt * Dp;
+       i += (long)gridDim.x * blockDim.x) {
+    const long r = i / Dp;
+    const int d = (int)(i - r * Dp);
+    const long p = fb[r];
+    gXh[i] = Xh[p * Dp + d];
+    if (d < n) gX[r * n + d] = X[p * n + d];
+    if (d == 0) gN[r] = xn[p];
+  }
 }
 
 __global__ void k_scatter_fb(long cnt, const int *__restrict__ fb, const int *__restrict__ lab,
@@ -1414,7 +1416,7 @@ static kmeansErrorStatus train_mlp(Engine *e, Learned *L) {
   hipFree(dmean);
   const double rms = sqrt(mq / (double)e->n);
   L->xscale = rms > 0.0 && std::isfinite(rms) ? (float)(1.0 / rms) : 1.0f;
-  hipLaunchKernelGGL(k_x_bf16, dim3(g1(ns * din)), dim3(256), 0, s, ns, e->n, din, e->dX,
+  hipLaunchKernelGGL(k_x_bf16, dim3(km_grid(ns * din, 256)), dim3(256), 0, s, ns, e->n, din, e->dX,
                      L->samp_rows, L->xscale, L->samp_x);
   hipLaunchKernelGGL(k_targets, dim3(g1(ns)), dim3(256), 0, s, ns, L->samp_lab, L->gof, L->samp_tg);
   std::vector<float> P0;
@@ -1443,7 +1445,7 @@ static kmeansErrorStatus train_mlp(Engine *e, Learned *L) {
   KM_HIP_CHECK(hipMemcpyAsync(lh, L->loss, sizeof(float) * 2, hipMemcpyDeviceToHost, s), KMEANS_ERROR_COMPUTE);
   // scaled BF16 copy of every point for inference (points do not change)
   KM_HIP_CHECK(lmalloc((void **)&L->Xh, sizeof(uint16_t) * (size_t)e->m * din), KMEANS_ERROR_COMPUTE);
-  hipLaunchKernelGGL(k_x_bf16, dim3(g1(e->m * din)), dim3(256), 0, s, e->m, e->n, din, e->dX, nullptr,
+  hipLaunchKernelGGL(k_x_bf16, dim3(km_grid(e->m * din, 256)), dim3(256), 0, s, e->m, e->n, din, e->dX, nullptr,
                      L->xscale, L->Xh);
   {
     size_t off[6];
@@ -1704,7 +1706,7 @@ hipError_t learned_fallback(Engine *e) {
       L->fb_cap = cap;
     }
     if (cnt > 0) {
-      hipLaunchKernelGGL(k_gather_fb, dim3(g1((long)cnt * Dp)), dim3(256), 0, e->stream, (long)cnt,
+      hipLaunchKernelGGL(k_gather_fb, dim3(km_grid((long)cnt * Dp, 256)), dim3(256), 0, e->stream, (long)cnt,
                          L->fb, e->n, Dp, e->dX, mfma_tile_xh(e), mfma_tile_xnorm(e), L->fbX,
                          L->fbXh, L->fbN);
       LCHK(mfma_assign_view(e, L->fbX, L->fbXh, L->fbN, L->fbLab, cnt));

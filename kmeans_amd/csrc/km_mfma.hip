@@ -87,12 +87,13 @@ static MfmaState *st(Engine *e) { return (MfmaState *)e->mfma; }
 // ------------------------------------------------------------------ kernels
 __global__ void k_convert_rows(long m, int d, int Dp, const float *__restrict__ X, float scale,
                                int fp16, uint16_t *__restrict__ out) {
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= m * Dp) return;
-  const long r = e / Dp;
-  const int q = (int)(e - r * Dp);
-  const float v = q < d ? X[r * d + q] * scale : 0.0f;
-  out[e] = fp16 ? f32_to_f16_rne(v) : f32_to_bf16_rne(v);
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < m * Dp;
+       e += (long)gridDim.x * blockDim.x) {
+    const long r = e / Dp;
+    const int q = (int)(e - r * Dp);
+    const float v = q < d ? X[r * d + q] * scale : 0.0f;
+    out[e] = fp16 ? f32_to_f16_rne(v) : f32_to_bf16_rne(v);
+  }
 }
 
 // Ch (and for BF16 the low parts Cl), bias and the per-centre bound dcn_j
@@ -815,7 +816,7 @@ static hipError_t prepare_x(Engine *e) {
   }
   if (!e->streaming) {
     const long tot = (long)e->m * s->Dp;
-    hipLaunchKernelGGL(k_convert_rows, dim3(km_cdiv(tot, 256)), dim3(256), 0, e->stream, e->m,
+    hipLaunchKernelGGL(k_convert_rows, dim3(km_grid(tot, 256)), dim3(256), 0, e->stream, e->m,
                        e->n, s->Dp, e->dX, s->scale, (int)s->fp16, s->Xh);
     hipLaunchKernelGGL(k_xnorm, dim3(km_cdiv(e->m, 256)), dim3(256), 0, e->stream, e->m, e->n,
                        e->dX, s->scale, (int)s->fp16, s->xnb);
@@ -837,7 +838,7 @@ hipError_t mfma_prepare_rows(Engine *e, const float *X, long rows) {
     if ((r = prepare_centres(e)) != hipSuccess) return r;
   if (rows <= 0) return hipSuccess;
   const long tot = rows * s->Dp;
-  hipLaunchKernelGGL(k_convert_rows, dim3(km_cdiv(tot, 256)), dim3(256), 0, e->stream, rows, e->n,
+  hipLaunchKernelGGL(k_convert_rows, dim3(km_grid(tot, 256)), dim3(256), 0, e->stream, rows, e->n,
                      s->Dp, X, s->scale, (int)s->fp16, s->Xh);
   hipLaunchKernelGGL(k_xnorm, dim3(km_cdiv(rows, 256)), dim3(256), 0, e->stream, rows, e->n, X,
                      s->scale, (int)s->fp16, s->xnb);

@@ -93,3 +93,29 @@ def test_mfma_tiny_magnitudes(orc):
         lab, C, st = run_engine(X, C0, 2, prec)
         ref = orc.lloyd(X, C0, 2)
         assert np.array_equal(lab, ref["labels"]) and np.array_equal(C, ref["centers"])
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_mfma_more_elements_than_a_dispatch_dimension(orc, prec):
+    """68M x 64 points: the reduced-precision copy has more than 2^32 elements,
+    more than one dispatch's 32-bit grid size can address; the rows past that
+    mark must be converted and labelled like the first ones."""
+    from kmeans_amd import datagen
+    n = 68_000_000
+    X, C0, spec = datagen.generate_on_gpu("c3", n=n, torch=torch)
+    d, k = spec["d"], spec["k"]
+    assert n * 64 > 2 ** 32
+    labs = {}
+    for p in ("fp32", prec):
+        e = km.KMeans(n, d, k, precision=p)
+        e.set_data(device_ptr=X.data_ptr())
+        e.set_centers(C0)
+        e.run(1)
+        labs[p] = e.labels()
+        e.close()
+    assert np.array_equal(labs["fp32"], labs[prec])
+    rows = np.sort(np.random.default_rng(1).choice(np.arange(2 ** 32 // 64, n), 2000, replace=False))
+    Xs = X[torch.from_numpy(rows).cuda()].cpu().numpy()
+    assert np.array_equal(labs[prec][rows], orc.assign(Xs, C0))
+    del X
+    torch.cuda.empty_cache()
