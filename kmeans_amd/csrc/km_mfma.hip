@@ -68,6 +68,10 @@ struct MfmaState {
   uint16_t *Xh = nullptr;   // m x Dp
   uint16_t *Ch = nullptr;   // Kp x Dp
   uint16_t *Cl = nullptr;   // Kp x Dp: BF16 of (C - Ch), the low part (BF16 engines)
+  // the deep pass's unscaled BF16 centre parts and biases: Ch, Cl and bias
+  // themselves for BF16 engines, own copies for FP16 engines
+  uint16_t *Chb = nullptr, *Clb = nullptr;
+  float *biasb = nullptr;
   float *bias = nullptr;    // Kp: -0.5*scale^2*nc (or -FLT_MAX past k)
   float *bnd = nullptr;     // [cmax, ncmax, dcmax]: bounds over the centres
   float *dcn = nullptr;     // Kp: upper bound of |c_j - ch_j/s|
@@ -105,7 +109,10 @@ __global__ __launch_bounds__(256) void k_prepare_centres(int k, int d, int Dp, i
                                                          int fp16, uint16_t *__restrict__ Ch,
                                                          uint16_t *__restrict__ Cl,
                                                          float *__restrict__ bias,
-                                                         float *__restrict__ dcn) {
+                                                         float *__restrict__ dcn,
+                                                         uint16_t *__restrict__ Chb,
+                                                         uint16_t *__restrict__ Clb,
+                                                         float *__restrict__ biasb) {
   const int lane = threadIdx.x & 63;
   const long nw = (long)gridDim.x * (blockDim.x >> 6);
   for (long j = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); j < Kp; j += nw) {
@@ -117,12 +124,19 @@ __global__ __launch_bounds__(256) void k_prepare_centres(int k, int d, int Dp, i
       const float r = v - hv;
       Ch[j * Dp + q] = hi;
       if (!fp16) Cl[j * Dp + q] = f32_to_bf16_rne(r);
+      if (fp16) {  // unscaled BF16 parts for the deep pass
+        const float c = (j < k && q < d) ? C[j * d + q] : 0.0f;
+        const uint16_t bh = f32_to_bf16_rne(c);
+        Chb[j * Dp + q] = bh;
+        Clb[j * Dp + q] = f32_to_bf16_rne(c - __uint_as_float((uint32_t)bh << 16));
+      }
       sq = __builtin_fmaf(r, r, sq);
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off);
     if (lane == 0) {
       bias[j] = j < k ? -0.5f * nc[j] * (scale * scale) : -FLT_MAX;
+      if (fp16) biasb[j] = j < k ? -0.5f * nc[j] : -FLT_MAX;
       dcn[j] = sqrtf(sq * (1.0f + (float)Dp * 1.2e-7f + 1e-6f)) * (1.0f + 1e-6f) / scale + 1e-30f;
     }
   }
@@ -467,10 +481,11 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
 // labels the point when the best of them (lowest index on equal scores)
 // exceeds the fourth + E; the rest go to the FP32 full search.  One wave
 // handles 32 listed points; centre fragments come from the L2-resident
-// arrays.  BF16 engines only.
+// arrays.  Both engines: the parts are unscaled BF16 (for FP16 engines the
+// BF16 centre copies Chb/Clb and biases biasb), computed from the FP32 rows.
 template <int KS>
 __global__ __launch_bounds__(256) void k_deep_mfma(
-    const int *__restrict__ nlist, const int *__restrict__ list, const uint16_t *__restrict__ Xh,
+    const int *__restrict__ nlist, const int *__restrict__ list,
     const float *__restrict__ X, int D, int K, int Kp, const uint16_t *__restrict__ Ch,
     const uint16_t *__restrict__ Cl, const float *__restrict__ bias, const float *__restrict__ C,
     const float *__restrict__ nc, const float *__restrict__ bnd, const float2 *__restrict__ xnb,
@@ -507,26 +522,27 @@ __global__ __launch_bounds__(256) void k_deep_mfma(
     const bool ok = li < total;
     const long p = ok ? (long)list[li] : 0;
     // point fragments (8 consecutive dimensions per k-step): high and low
-    // BF16 parts; v - high is exact in FP32
+    // BF16 parts of the FP32 row (v - high is exact in FP32)
     u32x4_t xh4[KS], xl4[KS];
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
       xh4[ks] = u32x4_t{0u, 0u, 0u, 0u};
       xl4[ks] = u32x4_t{0u, 0u, 0u, 0u};
       if (ok) {
-        xh4[ks] = *reinterpret_cast<const u32x4_t *>(Xh + p * Dp + ks * 16 + 8 * h);
-        uint32_t w[4];
+        uint32_t wh[4], wl[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const int d0 = ks * 16 + 8 * h + 2 * u;
           const float v0 = d0 < D ? X[p * D + d0] : 0.0f;
           const float v1 = d0 + 1 < D ? X[p * D + d0 + 1] : 0.0f;
-          const uint32_t hw = xh4[ks][u];
-          const float r0 = v0 - __uint_as_float(hw << 16);
-          const float r1 = v1 - __uint_as_float(hw & 0xffff0000u);
-          w[u] = (uint32_t)f32_to_bf16_rne(r0) | ((uint32_t)f32_to_bf16_rne(r1) << 16);
+          const uint16_t h0 = f32_to_bf16_rne(v0), h1 = f32_to_bf16_rne(v1);
+          const float r0 = v0 - __uint_as_float((uint32_t)h0 << 16);
+          const float r1 = v1 - __uint_as_float((uint32_t)h1 << 16);
+          wh[u] = (uint32_t)h0 | ((uint32_t)h1 << 16);
+          wl[u] = (uint32_t)f32_to_bf16_rne(r0) | ((uint32_t)f32_to_bf16_rne(r1) << 16);
         }
-        xl4[ks] = u32x4_t{w[0], w[1], w[2], w[3]};
+        xh4[ks] = u32x4_t{wh[0], wh[1], wh[2], wh[3]};
+        xl4[ks] = u32x4_t{wl[0], wl[1], wl[2], wl[3]};
       }
     }
     float bv[4];
@@ -687,6 +703,11 @@ void mfma_free(Engine *e) {
   hipFree(s->nflag);
   hipFree(s->flags2);
   hipFree(s->Cl);
+  if (s->fp16) {
+    hipFree(s->Chb);
+    hipFree(s->Clb);
+    hipFree(s->biasb);
+  }
   hipFree(s->nflag2);
   delete s;
   e->mfma = nullptr;
@@ -720,7 +741,16 @@ static hipError_t ensure_state(Engine *e) {
   if ((r = hipMalloc(&s->Xh, sizeof(uint16_t) * (size_t)xrows * s->Dp)) != hipSuccess) return r;
   if ((r = hipMalloc(&s->Ch, sizeof(uint16_t) * (size_t)s->Kp * s->Dp)) != hipSuccess) return r;
   if ((r = hipMalloc(&s->Cl, sizeof(uint16_t) * (size_t)s->Kp * s->Dp)) != hipSuccess) return r;
+  if (s->fp16) {
+    if ((r = hipMalloc(&s->Chb, sizeof(uint16_t) * (size_t)s->Kp * s->Dp)) != hipSuccess) return r;
+    if ((r = hipMalloc(&s->Clb, sizeof(uint16_t) * (size_t)s->Kp * s->Dp)) != hipSuccess) return r;
+    if ((r = hipMalloc(&s->biasb, sizeof(float) * s->Kp)) != hipSuccess) return r;
+  } else {
+    s->Chb = s->Ch;
+    s->Clb = s->Cl;
+  }
   if ((r = hipMalloc(&s->bias, sizeof(float) * s->Kp)) != hipSuccess) return r;
+  if (!s->fp16) s->biasb = s->bias;
   if ((r = hipMalloc(&s->bnd, sizeof(float) * 4)) != hipSuccess) return r;
   if ((r = hipMalloc(&s->xnb, sizeof(float2) * (size_t)xrows)) != hipSuccess) return r;
   if ((r = hipMalloc(&s->dcn, sizeof(float) * s->Kp)) != hipSuccess) return r;
@@ -849,7 +879,8 @@ static hipError_t prepare_centres(Engine *e) {
   MfmaState *s = st(e);
   const unsigned pg = (unsigned)std::min<long>(km_cdiv(s->Kp, 4), 1024);  // a wave per row
   hipLaunchKernelGGL(k_prepare_centres, dim3(pg), dim3(256), 0, e->stream, e->k, e->n, s->Dp, s->Kp,
-                     e->dC, e->dnc, s->scale, (int)s->fp16, s->Ch, s->Cl, s->bias, s->dcn);
+                     e->dC, e->dnc, s->scale, (int)s->fp16, s->Ch, s->Cl, s->bias, s->dcn, s->Chb,
+                     s->Clb, s->biasb);
   hipLaunchKernelGGL(k_centre_bounds, dim3(1), dim3(256), 0, e->stream, e->k, e->n, e->dnc, s->dcn,
                      s->bnd);
   s->c_ready = true;
@@ -992,13 +1023,11 @@ hipError_t mfma_reresolve_view(Engine *e, const float *X, const uint16_t *Xh, co
   hipError_t r;
   if ((r = hipMemsetAsync(s->nflag2 + e->cur_slot, 0, sizeof(int), e->stream)) != hipSuccess) return r;
   if (rows <= 0) return hipSuccess;
-  if (s->fp16)  // FP16 engines: straight to the FP32 search
-    return launch_assign_f32(e->stream, rows, s->nflag + e->cur_slot, e->n, X, s->flags, e->k,
-                             e->dCt, e->ldc, e->dnc, lab, nullptr);
+  (void)Xh;  // the deep pass splits the FP32 rows itself
   const long g = std::min<long>(km_cdiv(rows, 128), (long)num_cus() * 8);
 #define DEEP(KS)                                                                                 \
   hipLaunchKernelGGL(k_deep_mfma<KS>, dim3(g), dim3(256), 0, e->stream, s->nflag + e->cur_slot,  \
-                     s->flags, Xh, X, e->n, e->k, s->Kp, s->Ch, s->Cl, s->bias, e->dC, e->dnc,    \
+                     s->flags, X, e->n, e->k, s->Kp, s->Chb, s->Clb, s->biasb, e->dC, e->dnc,     \
                      s->bnd, xn, lab, s->flags2, s->nflag2 + e->cur_slot)
   switch (s->KS) {
     case 1: DEEP(1); break;
@@ -1030,7 +1059,7 @@ void mfma_collect(Engine *e, int nslots) {
   if (!s || nslots <= 0) return;
   std::vector<int> v(nslots);
   const int n = nslots < s->nflag_slots ? nslots : s->nflag_slots;
-  if (hipMemcpyAsync(v.data(), s->fp16 ? s->nflag : s->nflag2, sizeof(int) * n, hipMemcpyDeviceToHost, e->stream) !=
+  if (hipMemcpyAsync(v.data(), s->nflag2, sizeof(int) * n, hipMemcpyDeviceToHost, e->stream) !=
       hipSuccess)
     return;
   if (hipStreamSynchronize(e->stream) != hipSuccess) return;
