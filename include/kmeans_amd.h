@@ -98,8 +98,10 @@ kmeansErrorStatus computeKmeansF(const float *data, int m, int n, int k, float c
                                  float miniBatchFraction);
 
 /* Replaces computeKmeansD (main/cpp/kmeans.hpp:125-130, kmeans.cpp:1364-1409).
- * Same contract in double precision (the reference's fused FP64 path returns
- * label 0 for every row, reduction.hcu:8-11; this one computes real labels). */
+ * Same contract and deltas in double precision: scores are fma chains in
+ * double, centres the FP64 canonical sums divided by the counts (no rounding
+ * to float).  The reference's fused FP64 path gives label 0 to every row
+ * (reduction.hcu:8-11); this one computes real labels. */
 kmeansErrorStatus computeKmeansD(const double *data, int m, int n, int k, double criterion,
                                  int maxIters, int numRetries, int initAlgorithm,
                                  int doFullSGEMM, int matrixFormat, double *result,
@@ -317,24 +319,62 @@ kmeansCudaErrorStatus kmMlpForwardBackward(void *stream, int din, int hidden, in
 
 /* =========================================================================
  * 3. Legacy kernel names (KmeansCudaKernels.h:238-474), default stream.
- * Scratch arguments of the reference (Bpadded, C, Cindices, centers_large,
- * counts_large) are accepted and ignored: the fused kernels need none.
+ * Scratch arguments of the reference (Bpadded, C, Cindices) are accepted and
+ * ignored: the fused kernels need none.  ClusterCenters reuses
+ * centers_large as a k x (n+1) FP64 accumulator and counts_large as k int64
+ * counters (both caller-zeroed per iteration, as the reference requires);
+ * Compactness keeps an FP64 running total in the 8 bytes after
+ * compactness[0].  The ...D forms compute in double throughout (the
+ * reference's fused double argmax is an empty stub, reduction.hcu:8-11).
  * ========================================================================= */
+/* ClosestCentersF/D: KmeansCudaKernels.h:238-273 */
 kmeansCudaErrorStatus ClosestCentersF(const int m0, const int nRowsA, const int nColsA,
                                       const float *A, const int nColsB, const float *B,
                                       float *Bpadded, const float *normRowsOfA_squared,
                                       const float *normColsOfB_squared, const int nColsC,
                                       float *C, int *Cindices, int *CindicesFinal);
+kmeansCudaErrorStatus ClosestCentersD(const int m0, const int nRowsA, const int nColsA,
+                                      const double *A, const int nColsB, const double *B,
+                                      double *Bpadded, const double *normRowsOfA_squared,
+                                      const double *normColsOfB_squared, const int nColsC,
+                                      double *C, int *Cindices, int *CindicesFinal);
+/* rowTransformMinimumF/D: KmeansCudaKernels.h:290-313.  input = m x n
+ * product (n = number of centres); output[m0 + i] = lowest j minimising
+ * normRowsOfA_squared[i] + normColsOfB_squared[j] - 2 input[i][j] (rounded
+ * as RowTransformMinimum.cu:28 does), -1 if no value is below FLT_MAX.
+ * normRowsOfA_squared is indexed by the local row i, as in the reference. */
+kmeansCudaErrorStatus rowTransformMinimumF(const int m0, const int m, const int n,
+                                           const float *normRowsOfA_squared,
+                                           const float *normColsOfB_squared, const float *input,
+                                           int *output);
+kmeansCudaErrorStatus rowTransformMinimumD(const int m0, const int m, const int n,
+                                           const double *normRowsOfA_squared,
+                                           const double *normColsOfB_squared, const double *input,
+                                           int *output);
+/* ClusterCentersF/D: KmeansCudaKernels.h:337-369 */
 kmeansCudaErrorStatus ClusterCentersF(const int m0, const int m, const int n, const int k,
                                       const bool lastTile, const float *data, const int *indices,
                                       float *centers_large, int *counts_large, float *centers,
                                       int *counts);
+kmeansCudaErrorStatus ClusterCentersD(const int m0, const int m, const int n, const int k,
+                                      const bool lastTile, const double *data, const int *indices,
+                                      double *centers_large, int *counts_large, double *centers,
+                                      int *counts);
+/* CompactnessF/D: KmeansCudaKernels.h:389-414 */
 kmeansCudaErrorStatus CompactnessF(const int m0, const int m, const bool lastTile, const int n,
                                    const int k, const float *data, const int *indices,
                                    const float *centers, float *compactness);
+kmeansCudaErrorStatus CompactnessD(const int m0, const int m, const bool lastTile, const int n,
+                                   const int k, const double *data, const int *indices,
+                                   const double *centers, double *compactness);
+/* rowNormalizeF/D: KmeansCudaKernels.h:428-445; colNormalizeF/D: :458-474
+ * (in = m x n = D x K) */
 kmeansCudaErrorStatus rowNormalizeF(const int m0, const int m, const int n, const float *in,
                                     float *out);
+kmeansCudaErrorStatus rowNormalizeD(const int m0, const int m, const int n, const double *in,
+                                    double *out);
 kmeansCudaErrorStatus colNormalizeF(const int m, const int n, const float *in, float *out);
+kmeansCudaErrorStatus colNormalizeD(const int m, const int n, const double *in, double *out);
 
 #ifdef __cplusplus
 }

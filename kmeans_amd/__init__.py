@@ -6,6 +6,7 @@ binds it with ctypes so Python callers, the tests and bench.py reach the same
 entry points a C or C++ caller links against:
 
   compute_kmeans_f(...)        computeKmeansF (reference main/cpp/kmeans.hpp:102-107)
+  compute_kmeans_d(...)        computeKmeansD (kmeans.hpp:125-130), double precision
   KMeans(m, n, k, ...)         the engine handle (kmeans<T> of kmeans.hpp:174-658)
   col_norms / assign_f32 /     kernel API, device pointers (KmeansCudaKernels.h:
   update / compactness          238-474 replaced with stream + workspace arguments)
@@ -114,10 +115,17 @@ SIGNATURES = {
     "kmMlpWorkspaceSize": (_SZ, [_I, _I, _I, _I]),
     "kmMlpForwardBackward": (_I, [_P, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _SZ]),
     "ClosestCentersF": (_I, [_I, _I, _I, _P, _I, _P, _P, _P, _P, _I, _P, _P, _P]),
+    "ClosestCentersD": (_I, [_I, _I, _I, _P, _I, _P, _P, _P, _P, _I, _P, _P, _P]),
+    "rowTransformMinimumF": (_I, [_I, _I, _I, _P, _P, _P, _P]),
+    "rowTransformMinimumD": (_I, [_I, _I, _I, _P, _P, _P, _P]),
     "ClusterCentersF": (_I, [_I, _I, _I, _I, ctypes.c_bool, _P, _P, _P, _P, _P, _P]),
+    "ClusterCentersD": (_I, [_I, _I, _I, _I, ctypes.c_bool, _P, _P, _P, _P, _P, _P]),
     "CompactnessF": (_I, [_I, _I, ctypes.c_bool, _I, _I, _P, _P, _P, _P]),
+    "CompactnessD": (_I, [_I, _I, ctypes.c_bool, _I, _I, _P, _P, _P, _P]),
     "rowNormalizeF": (_I, [_I, _I, _I, _P, _P]),
+    "rowNormalizeD": (_I, [_I, _I, _I, _P, _P]),
     "colNormalizeF": (_I, [_I, _I, _P, _P]),
+    "colNormalizeD": (_I, [_I, _I, _P, _P]),
 }
 
 _lib = None
@@ -155,6 +163,17 @@ def compute_kmeans_f(data, k, criterion, max_iters, result=None, init_from_resul
     m, n = data.shape
     res = np.zeros((n, k), np.float32) if result is None else np.ascontiguousarray(result, np.float32)
     st = lib().computeKmeansF(_np_ptr(data), m, n, k, criterion, max_iters, 0,
+                              1 if init_from_result else 0, 0, 0, _np_ptr(res), mini_batch_fraction)
+    return st, res
+
+
+def compute_kmeans_d(data, k, criterion, max_iters, result=None, init_from_result=False,
+                     mini_batch_fraction=1.0):
+    """computeKmeansD: returns (status, centers as D x K, float64)."""
+    data = np.ascontiguousarray(data, dtype=np.float64)
+    m, n = data.shape
+    res = np.zeros((n, k), np.float64) if result is None else np.ascontiguousarray(result, np.float64)
+    st = lib().computeKmeansD(_np_ptr(data), m, n, k, criterion, max_iters, 0,
                               1 if init_from_result else 0, 0, 0, _np_ptr(res), mini_batch_fraction)
     return st, res
 

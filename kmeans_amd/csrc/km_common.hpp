@@ -60,6 +60,24 @@ __host__ __device__ inline double rng_unif(uint64_t seed, uint64_t stream, uint6
   return ((double)(rng_u64(seed, stream, i) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
 }
 
+// k distinct row indices in [0, m), in draw order, by Floyd's algorithm on
+// the counter-based generator (seed, stream 4).  Stands in for the
+// reference's srand(42) / rand() % m pick (kmeans.cpp:876-888).
+inline void sample_rows(uint64_t seed, long m, int k, int *idx) {
+  long t = 0;
+  for (long j = m - k; j < m; ++j) {
+    long r = (long)(rng_u64(seed, 4, (uint64_t)t) % (uint64_t)(j + 1));
+    bool dup = false;
+    for (long q = 0; q < t; ++q)
+      if (idx[q] == r) { dup = true; break; }
+    idx[t++] = dup ? (int)j : (int)r;
+  }
+}
+
+kmeansErrorStatus compute_d(const double *data, int m, int n, int k, double criterion,
+                            int maxIters, int initAlgorithm, double *result,
+                            double miniBatchFraction);
+
 // Internal launchers (implemented in km_kernels.hip); all stream-ordered.
 struct UpdateWs {
   int *hist, *offs, *tsum, *cstart, *ccount, *cq, *perm;
@@ -70,36 +88,45 @@ struct UpdateWs {
 size_t update_ws_layout(long n, int d, int k, char *base, UpdateWs *ws);
 int sub_block(int k);
 
-hipError_t launch_col_norms(hipStream_t s, int k, int d, const float *C, float *nc);
-hipError_t launch_transpose(hipStream_t s, int k, int d, const float *C, float *Ct, int ldc);
+// Launchers with a float and a double form (element type of the points and
+// centres; the sums are FP64 in both).
+#define KM_TYPED_LAUNCHERS(T)                                                                    \
+  hipError_t launch_col_norms(hipStream_t s, int k, int d, const T *C, T *nc);                   \
+  hipError_t launch_transpose(hipStream_t s, int k, int d, const T *C, T *Ct, int ldc);          \
+  hipError_t launch_update(hipStream_t s, long n, int d, int k, const T *X, const int *labels,   \
+                           const double *q, const T *C_old, T *C_new, T *Ct_new, int ldc,        \
+                           T *nc_new, long long *counts, int *empty, const UpdateWs &ws);        \
+  hipError_t launch_compactness(hipStream_t s, long n, int d, int k, const T *X,                 \
+                                const int *labels, const double *q, const T *C, double *out,     \
+                                const UpdateWs &ws);                                             \
+  hipError_t launch_point_sq(hipStream_t s, long m, int d, const T *X, double *q);               \
+  hipError_t launch_update_accumulate(hipStream_t s, long n, int d, int k, const T *X,           \
+                                      const int *labels, const double *q, double *S,             \
+                                      long long *cnt, const UpdateWs &ws);                       \
+  hipError_t launch_update_finalize(hipStream_t s, int d, int k, const double *S,                \
+                                    const long long *cnt, const T *C_old, T *C_new, T *Ct_new,   \
+                                    int ldc, T *nc_new, long long *counts, int *empty,           \
+                                    double *compj);                                              \
+  hipError_t launch_col_norms_dk(hipStream_t s, int k, int d, int ld, const T *C, T *nc);        \
+  hipError_t launch_row_norms(hipStream_t s, long m, int d, const T *X, T *out);                 \
+  hipError_t launch_row_transform_min(hipStream_t s, long m0, long m, int n, const T *rn,        \
+                                      const T *cn, const T *P, int *out);                        \
+  hipError_t launch_gather_rows(hipStream_t s, long n, int d, const T *X, const int *idx, T *out);
+KM_TYPED_LAUNCHERS(float)
+KM_TYPED_LAUNCHERS(double)
+#undef KM_TYPED_LAUNCHERS
 // rows (nullable): point indices of the m rows; out_pos (nullable): output
 // slot of each row (default: rows[p], else p)
 hipError_t launch_assign_f32(hipStream_t s, long m, const int *m_dev, int d, const float *X,
                              const int *rows, int k, const float *Ct, int ldc, const float *nc,
                              int *labels, float *best, const int *out_pos = nullptr);
-hipError_t launch_update(hipStream_t s, long n, int d, int k, const float *X, const int *labels,
-                         const double *q, const float *C_old, float *C_new, float *Ct_new, int ldc,
-                         float *nc_new, long long *counts, int *empty, const UpdateWs &ws);
-hipError_t launch_compactness(hipStream_t s, long n, int d, int k, const float *X,
-                              const int *labels, const double *q, const float *C, double *out,
-                              const UpdateWs &ws);
+hipError_t launch_assign_f64(hipStream_t s, long m, int d, const double *X, int k,
+                             const double *Ct, int ldc, const double *nc, int *labels);
 hipError_t launch_comp_reduce(hipStream_t s, int k, const double *compj, double *out);
-hipError_t launch_point_sq(hipStream_t s, long m, int d, const float *X, double *q);
 hipError_t launch_update_zero(hipStream_t s, int d, int k, double *S, long long *cnt, int *empty);
-hipError_t launch_update_accumulate(hipStream_t s, long n, int d, int k, const float *X,
-                                    const int *labels, const double *q, double *S, long long *cnt,
-                                    const UpdateWs &ws);
-hipError_t launch_update_finalize(hipStream_t s, int d, int k, const double *S,
-                                  const long long *cnt, const float *C_old, float *C_new,
-                                  float *Ct_new, int ldc, float *nc_new, long long *counts,
-                                  int *empty, double *compj);
-hipError_t launch_col_norms_dk(hipStream_t s, int k, int d, int ld, const float *C, float *nc);
-hipError_t launch_row_norms(hipStream_t s, long m, int d, const float *X, float *out);
 hipError_t launch_generate(hipStream_t s, uint64_t seed, long i0, long n, int d, int ncomp,
                            const float *means, const float *sigmas, const double *cdf, float *X,
                            int *z);
-hipError_t launch_gather_rows(hipStream_t s, long n, int d, const float *X, const int *idx,
-                              float *out);
 int centers_ld(int k);
 int num_cus();
 

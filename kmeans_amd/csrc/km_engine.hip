@@ -155,14 +155,7 @@ kmeansErrorStatus Engine::transpose() {
 kmeansErrorStatus Engine::initialize(uint64_t seed) {
   if (!dX && !hX) return KMEANS_ERROR_INITIALIZE;
   std::vector<int> idx(k);
-  long t = 0;
-  for (long j = m - k; j < m; ++j) {
-    long r = (long)(rng_u64(seed, 4, (uint64_t)t) % (uint64_t)(j + 1));
-    bool dup = false;
-    for (long q = 0; q < t; ++q)
-      if (idx[q] == r) { dup = true; break; }
-    idx[t++] = dup ? (int)j : (int)r;
-  }
+  sample_rows(seed, m, k, idx.data());
   if (streaming) {  // gather the rows on the host
     std::vector<float> c((size_t)k * n);
     for (int j = 0; j < k; ++j)
@@ -478,7 +471,7 @@ kmeansErrorStatus Engine::getLabels(int *out) {
 static std::mutex g_legacy_mu;
 static char *g_legacy_ws = nullptr;
 static size_t g_legacy_bytes = 0;
-static float *g_legacy_ct = nullptr;
+static void *g_legacy_ct = nullptr;
 static size_t g_legacy_ct_bytes = 0;
 
 static hipError_t legacy_ws(long n, int d, int k, UpdateWs *ws) {
@@ -638,11 +631,11 @@ kmeansErrorStatus computeKmeansF(const float *data, int m, int n, int k, float c
 kmeansErrorStatus computeKmeansD(const double *data, int m, int n, int k, double criterion,
                                  int maxIters, int numRetries, int initAlgorithm, int doFullSGEMM,
                                  int matrixFormat, double *result, double miniBatchFraction) {
-  (void)data; (void)m; (void)n; (void)k; (void)criterion; (void)maxIters; (void)numRetries;
-  (void)initAlgorithm; (void)doFullSGEMM; (void)matrixFormat; (void)result;
-  (void)miniBatchFraction;
-  fprintf(stderr, "kmeans_amd: computeKmeansD: the FP64 engine is not built in this version\n");
-  return KMEANS_ERROR_COMPUTE;
+  (void)numRetries;
+  (void)doFullSGEMM;
+  (void)matrixFormat;
+  return km::compute_d(data, m, n, k, criterion, maxIters, initAlgorithm, result,
+                       miniBatchFraction);
 }
 
 // ------------------------------------------------------------ kernel API
@@ -759,64 +752,69 @@ kmeansCudaErrorStatus kmMlpForwardBackward(void *stream, int din, int hidden, in
   return NO_ERROR;
 }
 
+}  // extern "C"
+
 // ------------------------------------------------------------ legacy names
-// ClosestCentersF (KmeansCudaKernels.h:238-244): A = nRowsA x nColsA tile,
-// B = nColsA x nColsB centres (D x K), labels to CindicesFinal[m0 + i].
-kmeansCudaErrorStatus ClosestCentersF(const int m0, const int nRowsA, const int nColsA,
-                                      const float *A, const int nColsB, const float *B,
-                                      float *Bpadded, const float *normRowsOfA_squared,
-                                      const float *normColsOfB_squared, const int nColsC,
-                                      float *C, int *Cindices, int *CindicesFinal) {
-  (void)Bpadded; (void)normRowsOfA_squared; (void)nColsC; (void)C; (void)Cindices;
-  if (nRowsA < 0 || nColsA <= 0 || nColsB <= 0 || !A || !B || !normColsOfB_squared ||
-      !CindicesFinal)
+// Shared bodies of the ...F and ...D kernel names (KmeansCudaKernels.h:
+// 238-474); the element type is that of the points and centres.
+namespace km {
+
+template <typename T>
+static kmeansCudaErrorStatus legacy_closest(int m0, int nRowsA, int nColsA, const T *A, int nColsB,
+                                            const T *B, const T *nc, int *CindicesFinal) {
+  if (nRowsA < 0 || nColsA <= 0 || nColsB <= 0 || !A || !B || !nc || !CindicesFinal)
     return ERROR_INVALID_ARGUMENT;
-  const int k = nColsB, d = nColsA, ld = km::centers_ld(k);
-  const float *Ct = B;
-  std::lock_guard<std::mutex> g(km::g_legacy_mu);
-  if (ld != k) {  // pad the D x K centres to the kernel's leading dimension
-    const size_t need = sizeof(float) * (size_t)ld * d;
-    if (need > km::g_legacy_ct_bytes) {
-      hipFree(km::g_legacy_ct);
-      km::g_legacy_ct = nullptr;
-      km::g_legacy_ct_bytes = 0;
-      KM_K_CHECK(hipMalloc(&km::g_legacy_ct, need), ERROR_CLOSESTCENTERS);
-      km::g_legacy_ct_bytes = need;
+  const int k = nColsB, d = nColsA, ld = centers_ld(k);
+  const T *Ct = B;
+  std::lock_guard<std::mutex> g(g_legacy_mu);
+  // the kernel reads the D x K centres four at a time from rows of ld
+  // (a multiple of 256) entries: pad them unless B already has that shape
+  if (ld != k || ((uintptr_t)B & (4 * sizeof(T) - 1)) != 0) {
+    const size_t need = sizeof(T) * (size_t)ld * d;
+    if (need > g_legacy_ct_bytes) {
+      hipFree(g_legacy_ct);
+      g_legacy_ct = nullptr;
+      g_legacy_ct_bytes = 0;
+      KM_K_CHECK(hipMalloc(&g_legacy_ct, need), ERROR_CLOSESTCENTERS);
+      g_legacy_ct_bytes = need;
     }
-    KM_K_CHECK(hipMemset(km::g_legacy_ct, 0, need), ERROR_CLOSESTCENTERS);
-    KM_K_CHECK(hipMemcpy2D(km::g_legacy_ct, sizeof(float) * ld, B, sizeof(float) * k,
-                           sizeof(float) * k, d, hipMemcpyDeviceToDevice),
+    KM_K_CHECK(hipMemset(g_legacy_ct, 0, need), ERROR_CLOSESTCENTERS);
+    KM_K_CHECK(hipMemcpy2D(g_legacy_ct, sizeof(T) * ld, B, sizeof(T) * k, sizeof(T) * k, d,
+                           hipMemcpyDeviceToDevice),
                ERROR_CLOSESTCENTERS);
-    Ct = km::g_legacy_ct;
+    Ct = reinterpret_cast<const T *>(g_legacy_ct);
   }
-  KM_K_CHECK(km::launch_assign_f32(nullptr, nRowsA, nullptr, d, A, nullptr, k, Ct, ld,
-                                   normColsOfB_squared, CindicesFinal + m0, nullptr),
-             ERROR_CLOSESTCENTERS);
+  hipError_t e;
+  if constexpr (sizeof(T) == 4)
+    e = launch_assign_f32(nullptr, nRowsA, nullptr, d, A, nullptr, k, Ct, ld, nc,
+                          CindicesFinal + m0, nullptr);
+  else
+    e = launch_assign_f64(nullptr, nRowsA, d, A, k, Ct, ld, nc, CindicesFinal + m0);
+  KM_K_CHECK(e, ERROR_CLOSESTCENTERS);
   return NO_ERROR;
 }
 
-// ClusterCentersF (KmeansCudaKernels.h:337-343): accumulates the tile into
-// centers_large (reused as a k x n FP64 accumulator; the caller zeroes it per
-// iteration as kmeans.cpp:1152-1156 does) and counts_large (k int64); on
-// lastTile writes centers (k x n) and counts.  Canonical blocks start at
-// each tile's first row.
-kmeansCudaErrorStatus ClusterCentersF(const int m0, const int m, const int n, const int k,
-                                      const bool lastTile, const float *data, const int *indices,
-                                      float *centers_large, int *counts_large, float *centers,
-                                      int *counts) {
+// centers_large is reused as a k x (n+1) FP64 accumulator and counts_large
+// as k int64 counters (the caller zeroes both per iteration, as
+// kmeans.cpp:1152-1156 does); on lastTile centers (k x n) and counts are
+// written.  Canonical blocks start at each tile's first row.
+template <typename T>
+static kmeansCudaErrorStatus legacy_cluster(int m0, int m, int n, int k, bool lastTile,
+                                            const T *data, const int *indices, T *centers_large,
+                                            int *counts_large, T *centers, int *counts) {
   if (m < 0 || n <= 0 || k <= 0 || !data || !indices || !centers_large || !counts_large ||
       !centers)
     return ERROR_INVALID_ARGUMENT;
-  std::lock_guard<std::mutex> g(km::g_legacy_mu);
-  km::UpdateWs ws;
-  KM_K_CHECK(km::legacy_ws(m, n, k, &ws), ERROR_CLUSTERCENTERS);
+  std::lock_guard<std::mutex> g(g_legacy_mu);
+  UpdateWs ws;
+  KM_K_CHECK(legacy_ws(m, n, k, &ws), ERROR_CLUSTERCENTERS);
   double *S = (double *)centers_large;
   long long *cnt = (long long *)counts_large;
-  KM_K_CHECK(km::launch_update_accumulate(nullptr, m, n, k, data, indices + m0, nullptr, S, cnt, ws),
+  KM_K_CHECK(launch_update_accumulate(nullptr, m, n, k, data, indices + m0, nullptr, S, cnt, ws),
              ERROR_CLUSTERCENTERS);
   if (lastTile) {
-    KM_K_CHECK(km::launch_update_finalize(nullptr, n, k, S, cnt, centers, centers, nullptr, 0,
-                                          nullptr, ws.cnt, nullptr, nullptr),
+    KM_K_CHECK(launch_update_finalize(nullptr, n, k, S, cnt, (const T *)centers, centers,
+                                      (T *)nullptr, 0, (T *)nullptr, ws.cnt, nullptr, nullptr),
                ERROR_CLUSTERCENTERS);
     if (counts) {
       std::vector<long long> c(k);
@@ -832,20 +830,21 @@ kmeansCudaErrorStatus ClusterCentersF(const int m0, const int m, const int n, co
   return NO_ERROR;
 }
 
-// CompactnessF (KmeansCudaKernels.h:389-394): compactness[2..3] (bytes 8-15)
-// hold the FP64 running total across tiles; on lastTile compactness[0] =
-// (float)total.  The caller zeroes the buffer per iteration.
-kmeansCudaErrorStatus CompactnessF(const int m0, const int m, const bool lastTile, const int n,
-                                   const int k, const float *data, const int *indices,
-                                   const float *centers, float *compactness) {
+// The 8 bytes after compactness[0] hold the FP64 running total across tiles
+// (compactness[2..3] for float, compactness[1] for double); on lastTile
+// compactness[0] = (T)total.  The caller zeroes the buffer per iteration.
+template <typename T>
+static kmeansCudaErrorStatus legacy_compactness(int m0, int m, bool lastTile, int n, int k,
+                                                const T *data, const int *indices,
+                                                const T *centers, T *compactness) {
   if (m < 0 || n <= 0 || k <= 0 || !data || !indices || !centers || !compactness)
     return ERROR_INVALID_ARGUMENT;
-  std::lock_guard<std::mutex> g(km::g_legacy_mu);
-  km::UpdateWs ws;
-  KM_K_CHECK(km::legacy_ws(m, n, k, &ws), ERROR_COMPACTNESS);
-  double *acc = (double *)(compactness + 2);
+  std::lock_guard<std::mutex> g(g_legacy_mu);
+  UpdateWs ws;
+  KM_K_CHECK(legacy_ws(m, n, k, &ws), ERROR_COMPACTNESS);
+  double *acc = (double *)(compactness + 8 / sizeof(T));
   double *tile = ws.compj + k;  // one spare slot past the per-centre terms
-  KM_K_CHECK(km::launch_compactness(nullptr, m, n, k, data, indices + m0, nullptr, centers, tile, ws),
+  KM_K_CHECK(launch_compactness(nullptr, m, n, k, data, indices + m0, nullptr, centers, tile, ws),
              ERROR_COMPACTNESS);
   double tv = 0.0, tot = 0.0;
   KM_K_CHECK(hipMemcpy(&tv, tile, sizeof(double), hipMemcpyDeviceToHost), ERROR_COMPACTNESS);
@@ -853,25 +852,125 @@ kmeansCudaErrorStatus CompactnessF(const int m0, const int m, const bool lastTil
   tot += tv;
   KM_K_CHECK(hipMemcpy(acc, &tot, sizeof(double), hipMemcpyHostToDevice), ERROR_COMPACTNESS);
   if (lastTile) {
-    const float f = (float)tot;
-    KM_K_CHECK(hipMemcpy(compactness, &f, sizeof(float), hipMemcpyHostToDevice), ERROR_COMPACTNESS);
+    const T f = (T)tot;
+    KM_K_CHECK(hipMemcpy(compactness, &f, sizeof(T), hipMemcpyHostToDevice), ERROR_COMPACTNESS);
   }
   KM_K_CHECK(hipDeviceSynchronize(), ERROR_COMPACTNESS);
   return NO_ERROR;
 }
 
-kmeansCudaErrorStatus rowNormalizeF(const int m0, const int m, const int n, const float *in,
-                                    float *out) {
+template <typename T>
+static kmeansCudaErrorStatus legacy_row_norms(int m0, int m, int n, const T *in, T *out) {
   if (m < 0 || n <= 0 || !in || !out) return ERROR_INVALID_ARGUMENT;
-  KM_K_CHECK(km::launch_row_norms(nullptr, m, n, in, out + m0), ERROR_ROWNORMALIZE);
+  KM_K_CHECK(launch_row_norms(nullptr, m, n, in, out + m0), ERROR_ROWNORMALIZE);
   return NO_ERROR;
 }
 
-// colNormalizeF (KmeansCudaKernels.h:458-463): in is m x n = D x K.
-kmeansCudaErrorStatus colNormalizeF(const int m, const int n, const float *in, float *out) {
+// in is m x n = D x K
+template <typename T>
+static kmeansCudaErrorStatus legacy_col_norms(int m, int n, const T *in, T *out) {
   if (m <= 0 || n <= 0 || !in || !out) return ERROR_INVALID_ARGUMENT;
-  KM_K_CHECK(km::launch_col_norms_dk(nullptr, n, m, n, in, out), ERROR_COLNORMALIZE);
+  KM_K_CHECK(launch_col_norms_dk(nullptr, n, m, n, in, out), ERROR_COLNORMALIZE);
   return NO_ERROR;
+}
+
+template <typename T>
+static kmeansCudaErrorStatus legacy_row_transform_min(int m0, int m, int n, const T *rn,
+                                                      const T *cn, const T *input, int *output) {
+  if (m0 < 0 || m < 0 || n <= 0 || !rn || !cn || !input || !output) return ERROR_INVALID_ARGUMENT;
+  KM_K_CHECK(launch_row_transform_min(nullptr, m0, m, n, rn, cn, input, output),
+             ERROR_ROWTRANSFORMMINIMUM);
+  return NO_ERROR;
+}
+
+}  // namespace km
+
+extern "C" {
+
+// ClosestCentersF/D (KmeansCudaKernels.h:238-273): A = nRowsA x nColsA tile,
+// B = nColsA x nColsB centres (D x K), labels to CindicesFinal[m0 + i].
+kmeansCudaErrorStatus ClosestCentersF(const int m0, const int nRowsA, const int nColsA,
+                                      const float *A, const int nColsB, const float *B,
+                                      float *Bpadded, const float *normRowsOfA_squared,
+                                      const float *normColsOfB_squared, const int nColsC,
+                                      float *C, int *Cindices, int *CindicesFinal) {
+  (void)Bpadded; (void)normRowsOfA_squared; (void)nColsC; (void)C; (void)Cindices;
+  return km::legacy_closest(m0, nRowsA, nColsA, A, nColsB, B, normColsOfB_squared, CindicesFinal);
+}
+
+kmeansCudaErrorStatus ClosestCentersD(const int m0, const int nRowsA, const int nColsA,
+                                      const double *A, const int nColsB, const double *B,
+                                      double *Bpadded, const double *normRowsOfA_squared,
+                                      const double *normColsOfB_squared, const int nColsC,
+                                      double *C, int *Cindices, int *CindicesFinal) {
+  (void)Bpadded; (void)normRowsOfA_squared; (void)nColsC; (void)C; (void)Cindices;
+  return km::legacy_closest(m0, nRowsA, nColsA, A, nColsB, B, normColsOfB_squared, CindicesFinal);
+}
+
+// rowTransformMinimumF/D (KmeansCudaKernels.h:290-313)
+kmeansCudaErrorStatus rowTransformMinimumF(const int m0, const int m, const int n,
+                                           const float *normRowsOfA_squared,
+                                           const float *normColsOfB_squared, const float *input,
+                                           int *output) {
+  return km::legacy_row_transform_min(m0, m, n, normRowsOfA_squared, normColsOfB_squared, input,
+                                      output);
+}
+
+kmeansCudaErrorStatus rowTransformMinimumD(const int m0, const int m, const int n,
+                                           const double *normRowsOfA_squared,
+                                           const double *normColsOfB_squared, const double *input,
+                                           int *output) {
+  return km::legacy_row_transform_min(m0, m, n, normRowsOfA_squared, normColsOfB_squared, input,
+                                      output);
+}
+
+// ClusterCentersF/D (KmeansCudaKernels.h:337-369)
+kmeansCudaErrorStatus ClusterCentersF(const int m0, const int m, const int n, const int k,
+                                      const bool lastTile, const float *data, const int *indices,
+                                      float *centers_large, int *counts_large, float *centers,
+                                      int *counts) {
+  return km::legacy_cluster(m0, m, n, k, lastTile, data, indices, centers_large, counts_large,
+                            centers, counts);
+}
+
+kmeansCudaErrorStatus ClusterCentersD(const int m0, const int m, const int n, const int k,
+                                      const bool lastTile, const double *data, const int *indices,
+                                      double *centers_large, int *counts_large, double *centers,
+                                      int *counts) {
+  return km::legacy_cluster(m0, m, n, k, lastTile, data, indices, centers_large, counts_large,
+                            centers, counts);
+}
+
+// CompactnessF/D (KmeansCudaKernels.h:389-414)
+kmeansCudaErrorStatus CompactnessF(const int m0, const int m, const bool lastTile, const int n,
+                                   const int k, const float *data, const int *indices,
+                                   const float *centers, float *compactness) {
+  return km::legacy_compactness(m0, m, lastTile, n, k, data, indices, centers, compactness);
+}
+
+kmeansCudaErrorStatus CompactnessD(const int m0, const int m, const bool lastTile, const int n,
+                                   const int k, const double *data, const int *indices,
+                                   const double *centers, double *compactness) {
+  return km::legacy_compactness(m0, m, lastTile, n, k, data, indices, centers, compactness);
+}
+
+// rowNormalizeF/D (KmeansCudaKernels.h:428-445), colNormalizeF/D (:458-474)
+kmeansCudaErrorStatus rowNormalizeF(const int m0, const int m, const int n, const float *in,
+                                    float *out) {
+  return km::legacy_row_norms(m0, m, n, in, out);
+}
+
+kmeansCudaErrorStatus rowNormalizeD(const int m0, const int m, const int n, const double *in,
+                                    double *out) {
+  return km::legacy_row_norms(m0, m, n, in, out);
+}
+
+kmeansCudaErrorStatus colNormalizeF(const int m, const int n, const float *in, float *out) {
+  return km::legacy_col_norms(m, n, in, out);
+}
+
+kmeansCudaErrorStatus colNormalizeD(const int m, const int n, const double *in, double *out) {
+  return km::legacy_col_norms(m, n, in, out);
 }
 
 }  // extern "C"

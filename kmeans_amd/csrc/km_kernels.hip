@@ -8,6 +8,7 @@
 //   compactness         main/cuda/Compactness.cu:3-96
 //   k_col_norms         main/cuda/Normalize.cu:33-47
 //   k_row_norms         main/cuda/Normalize.cu:3-31
+#include <float.h>
 #include <math.h>
 
 #include "km_common.hpp"
@@ -28,35 +29,46 @@ int num_cus() {
   return cus;
 }
 
+// Element types: float (the FP32 engine) and double (the FP64 engine,
+// computeKmeansD and the ...D kernel names).  Every chain below is written
+// with explicit fused multiply-adds in the element type.
+__device__ inline float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+template <typename T> struct Vec4;
+template <> struct Vec4<float> { typedef float4 type; };
+template <> struct Vec4<double> { typedef double4 type; };
+
 // ---------------------------------------------------------------- norms
 // nc[j] = fma chain over d of C[j][d]^2 (Normalize.cu:41-44, contraction explicit).
-__global__ void k_col_norms(int k, int d, const float *__restrict__ C, float *__restrict__ nc) {
+template <typename T>
+__global__ void k_col_norms(int k, int d, const T *__restrict__ C, T *__restrict__ nc) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= k) return;
-  const float *c = C + (long)j * d;
-  float s = 0.0f;
-  for (int q = 0; q < d; ++q) s = __builtin_fmaf(c[q], c[q], s);
+  const T *c = C + (long)j * d;
+  T s = 0;
+  for (int q = 0; q < d; ++q) s = fma_t(c[q], c[q], s);
   nc[j] = s;
 }
 
-__global__ void k_row_norms(long m, int d, const float *__restrict__ X, float *__restrict__ out) {
+template <typename T>
+__global__ void k_row_norms(long m, int d, const T *__restrict__ X, T *__restrict__ out) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
-  const float *x = X + i * d;
-  float s = 0.0f;
-  for (int q = 0; q < d; ++q) s = __builtin_fmaf(x[q], x[q], s);
+  const T *x = X + i * d;
+  T s = 0;
+  for (int q = 0; q < d; ++q) s = fma_t(x[q], x[q], s);
   out[i] = s;
 }
 
 // Ct[q][j] = C[j][q], zero for j >= k (Ct pre-cleared by the caller once).
-__global__ void k_transpose(int k, int d, const float *__restrict__ C, float *__restrict__ Ct,
-                            int ldc) {
-  __shared__ float t[32][33];
+template <typename T>
+__global__ void k_transpose(int k, int d, const T *__restrict__ C, T *__restrict__ Ct, int ldc) {
+  __shared__ T t[32][33];
   int j0 = blockIdx.x * 32, q0 = blockIdx.y * 32;
   int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
   for (int r = ty; r < 32; r += 8) {
     int j = j0 + r, q = q0 + tx;
-    t[r][tx] = (j < k && q < d) ? C[(long)j * d + q] : 0.0f;
+    t[r][tx] = (j < k && q < d) ? C[(long)j * d + q] : (T)0;
   }
   __syncthreads();
   for (int r = ty; r < 32; r += 8) {
@@ -66,24 +78,28 @@ __global__ void k_transpose(int k, int d, const float *__restrict__ C, float *__
 }
 
 // ---------------------------------------------------------------- K1 assign
-// One workgroup (4 waves) owns a tile of 64 points and sweeps all centres in
-// tiles of 256, d-slices of 16 staged through LDS with register prefetch.
-// Thread (pg, cg) accumulates 8 points x 8 centres; every score is a single
-// fma chain over d in ascending order starting from -0.5f*nc[j].
-#define A_BM 64
+// One workgroup (4 waves) owns a tile of 8*PP points and sweeps all centres
+// in tiles of 256, d-slices of 16 staged through LDS with register prefetch.
+// Thread (pg, cg) accumulates PP points x 8 centres; every score is a single
+// fma chain over d in ascending order starting from -0.5*nc[j].  FP32 uses
+// PP = 8 (64 points per tile), FP64 PP = 4 so that its accumulators fit the
+// register file at two waves per SIMD.
 #define A_BN 256
 #define A_DS 16
+#define A_BM_F32 64
+#define A_BM_F64 32
 
-__global__ __launch_bounds__(256) void k_assign_f32(long M, const int *__restrict__ Mdev, int D,
-                                                    const float *__restrict__ X,
-                                                    const int *__restrict__ rows, int K,
-                                                    const float *__restrict__ Ct, int ldc,
-                                                    const float *__restrict__ nc,
-                                                    const int *__restrict__ out_pos,
-                                                    int *__restrict__ labels,
-                                                    float *__restrict__ best) {
-  __shared__ __attribute__((aligned(16))) float Xs[A_DS][A_BM + 4];
-  __shared__ float4 Cs[A_DS][A_BN / 4];
+template <typename T, int PP>
+__device__ __forceinline__ void assign_tiles(long M, const int *__restrict__ Mdev, int D,
+                                             const T *__restrict__ X, const int *__restrict__ rows,
+                                             int K, const T *__restrict__ Ct, int ldc,
+                                             const T *__restrict__ nc,
+                                             const int *__restrict__ out_pos,
+                                             int *__restrict__ labels, T *__restrict__ best) {
+  typedef typename Vec4<T>::type T4;
+  constexpr int A_BM = 8 * PP, XQ = A_BM / 16;  // tile rows, rows staged per thread
+  __shared__ __attribute__((aligned(32))) T Xs[A_DS][A_BM + 4];
+  __shared__ T4 Cs[A_DS][A_BN / 4];
   if (Mdev) M = *Mdev;
   const long ntiles = (M + A_BM - 1) / A_BM;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -95,37 +111,39 @@ __global__ __launch_bounds__(256) void k_assign_f32(long M, const int *__restric
 
   for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const long r0 = tile * A_BM;
-    long xrow[4];
-    bool xok[4];
+    long xrow[XQ];
+    bool xok[XQ];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < XQ; ++q) {
       long p = r0 + (tid >> 4) + 16 * q;
       xok[q] = p < M;
       xrow[q] = xok[q] ? (rows ? (long)rows[p] : p) : 0;
     }
-    float bv[8];
-    int bi[8];
+    T bv[PP];
+    int bi[PP];
 #pragma unroll
-    for (int p = 0; p < 8; ++p) { bv[p] = -INFINITY; bi[p] = 0; }
+    for (int p = 0; p < PP; ++p) { bv[p] = (T)-INFINITY; bi[p] = 0; }
 
-    float xr[4];
-    float4 cr[4];
+    T xr[XQ];
+    T4 cr[4];
     auto load = [&](int it) {
       const int kt = it / nS, s = it - kt * nS;
       const int d0 = s * A_DS;
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-        xr[q] = (xok[q] && d0 + xdd < D) ? X[xrow[q] * D + d0 + xdd] : 0.0f;
+      for (int q = 0; q < XQ; ++q)
+        xr[q] = (xok[q] && d0 + xdd < D) ? X[xrow[q] * D + d0 + xdd] : (T)0;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int e = tid + 256 * q, dd = e >> 6, c4 = e & 63;
-        cr[q] = (d0 + dd < D)
-                    ? *reinterpret_cast<const float4 *>(Ct + (long)(d0 + dd) * ldc + kt * A_BN + c4 * 4)
-                    : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (d0 + dd < D) {
+          cr[q] = *reinterpret_cast<const T4 *>(Ct + (long)(d0 + dd) * ldc + kt * A_BN + c4 * 4);
+        } else {
+          cr[q].x = 0; cr[q].y = 0; cr[q].z = 0; cr[q].w = 0;
+        }
       }
     };
 
-    float acc[8][8];
+    T acc[PP][8];
     load(0);
     for (int it = 0; it < total; ++it) {
       const int kt = it / nS, s = it - kt * nS;
@@ -133,14 +151,14 @@ __global__ __launch_bounds__(256) void k_assign_f32(long M, const int *__restric
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
           const int j = kt * A_BN + (q < 4 ? cg * 4 + q : 128 + cg * 4 + q - 4);
-          const float v = j < K ? -0.5f * nc[j] : -INFINITY;
+          const T v = j < K ? (T)-0.5 * nc[j] : (T)-INFINITY;
 #pragma unroll
-          for (int p = 0; p < 8; ++p) acc[p][q] = v;
+          for (int p = 0; p < PP; ++p) acc[p][q] = v;
         }
       }
       __syncthreads();
 #pragma unroll
-      for (int q = 0; q < 4; ++q) Xs[xdd][(tid >> 4) + 16 * q] = xr[q];
+      for (int q = 0; q < XQ; ++q) Xs[xdd][(tid >> 4) + 16 * q] = xr[q];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int e = tid + 256 * q;
@@ -150,15 +168,18 @@ __global__ __launch_bounds__(256) void k_assign_f32(long M, const int *__restric
       if (it + 1 < total) load(it + 1);
       const int dn = min(A_DS, D - s * A_DS);
       auto step = [&](int dd) {
-        const float4 xa = *reinterpret_cast<const float4 *>(&Xs[dd][pg * 8]);
-        const float4 xb = *reinterpret_cast<const float4 *>(&Xs[dd][pg * 8 + 4]);
-        const float4 ca = Cs[dd][cg], cb = Cs[dd][32 + cg];
-        const float xv[8] = {xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w};
-        const float cv[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
+        T xv[PP];
 #pragma unroll
-        for (int p = 0; p < 8; ++p)
+        for (int h = 0; h < PP / 4; ++h) {
+          const T4 xa = *reinterpret_cast<const T4 *>(&Xs[dd][pg * PP + 4 * h]);
+          xv[4 * h] = xa.x; xv[4 * h + 1] = xa.y; xv[4 * h + 2] = xa.z; xv[4 * h + 3] = xa.w;
+        }
+        const T4 ca = Cs[dd][cg], cb = Cs[dd][32 + cg];
+        const T cv[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
 #pragma unroll
-          for (int q = 0; q < 8; ++q) acc[p][q] = __builtin_fmaf(xv[p], cv[q], acc[p][q]);
+        for (int p = 0; p < PP; ++p)
+#pragma unroll
+          for (int q = 0; q < 8; ++q) acc[p][q] = fma_t(xv[p], cv[q], acc[p][q]);
       };
       if (dn == A_DS) {
 #pragma unroll
@@ -168,7 +189,7 @@ __global__ __launch_bounds__(256) void k_assign_f32(long M, const int *__restric
       }
       if (s == nS - 1) {
 #pragma unroll
-        for (int p = 0; p < 8; ++p)
+        for (int p = 0; p < PP; ++p)
 #pragma unroll
           for (int q = 0; q < 8; ++q) {
             const int j = kt * A_BN + (q < 4 ? cg * 4 + q : 128 + cg * 4 + q - 4);
@@ -176,19 +197,19 @@ __global__ __launch_bounds__(256) void k_assign_f32(long M, const int *__restric
           }
       }
     }
-    // lanes of one half-wave share the 8 points; merge with the tie rule
+    // lanes of one half-wave share the PP points; merge with the tie rule
 #pragma unroll
-    for (int p = 0; p < 8; ++p) {
+    for (int p = 0; p < PP; ++p) {
 #pragma unroll
       for (int off = 16; off >= 1; off >>= 1) {
-        const float ov = __shfl_xor(bv[p], off);
+        const T ov = __shfl_xor(bv[p], off);
         const int oi = __shfl_xor(bi[p], off);
         if (ov > bv[p] || (ov == bv[p] && oi < bi[p])) { bv[p] = ov; bi[p] = oi; }
       }
     }
 #pragma unroll
-    for (int p = 0; p < 8; ++p) {
-      const long pt = r0 + pg * 8 + p;
+    for (int p = 0; p < PP; ++p) {
+      const long pt = r0 + pg * PP + p;
       if (cg == p && pt < M) {
         const long o = out_pos ? (long)out_pos[pt] : rows ? (long)rows[pt] : pt;
         labels[o] = bi[p];
@@ -196,6 +217,28 @@ __global__ __launch_bounds__(256) void k_assign_f32(long M, const int *__restric
       }
     }
   }
+}
+
+__global__ __launch_bounds__(256) void k_assign_f32(long M, const int *__restrict__ Mdev, int D,
+                                                    const float *__restrict__ X,
+                                                    const int *__restrict__ rows, int K,
+                                                    const float *__restrict__ Ct, int ldc,
+                                                    const float *__restrict__ nc,
+                                                    const int *__restrict__ out_pos,
+                                                    int *__restrict__ labels,
+                                                    float *__restrict__ best) {
+  assign_tiles<float, A_BM_F32 / 8>(M, Mdev, D, X, rows, K, Ct, ldc, nc, out_pos, labels, best);
+}
+
+__global__ __launch_bounds__(256) void k_assign_f64(long M, const int *__restrict__ Mdev, int D,
+                                                    const double *__restrict__ X,
+                                                    const int *__restrict__ rows, int K,
+                                                    const double *__restrict__ Ct, int ldc,
+                                                    const double *__restrict__ nc,
+                                                    const int *__restrict__ out_pos,
+                                                    int *__restrict__ labels,
+                                                    double *__restrict__ best) {
+  assign_tiles<double, A_BM_F64 / 8>(M, Mdev, D, X, rows, K, Ct, ldc, nc, out_pos, labels, best);
 }
 
 // ---------------------------------------------------------------- update
@@ -357,7 +400,8 @@ __global__ __launch_bounds__(256) void k_cluster_setup(int K, int nsb, const int
 // First-level sums: one group of threads per chunk of <= KM_CHUNK members,
 // one thread per dimension, members added in ascending point order.  Column
 // D of each partial holds the sum of the members' squared norms q_i.
-__global__ __launch_bounds__(320) void k_chunk_sums(const float *__restrict__ X, int D, int Dp,
+template <typename T>
+__global__ __launch_bounds__(320) void k_chunk_sums(const T *__restrict__ X, int D, int Dp,
                                                     const double *__restrict__ q,
                                                     const int *__restrict__ perm, int K,
                                                     const int *__restrict__ cstart,
@@ -412,8 +456,8 @@ __global__ __launch_bounds__(320) void k_chunk_sums(const float *__restrict__ X,
       const int p1 = staged ? pidx[sub][m + 1] : perm[start + m + 1];
       const int p2 = staged ? pidx[sub][m + 2] : perm[start + m + 2];
       const int p3 = staged ? pidx[sub][m + 3] : perm[start + m + 3];
-      const float v0 = X[(long)p0 * D + d], v1 = X[(long)p1 * D + d];
-      const float v2 = X[(long)p2 * D + d], v3 = X[(long)p3 * D + d];
+      const T v0 = X[(long)p0 * D + d], v1 = X[(long)p1 * D + d];
+      const T v2 = X[(long)p2 * D + d], v3 = X[(long)p3 * D + d];
       acc += (double)v0;
       acc += (double)v1;
       acc += (double)v2;
@@ -553,20 +597,22 @@ __global__ void k_zero_sums(long n, double *__restrict__ S, int K, long long *__
 // New centres, transposed copy and norms; empty centres keep the old value.
 // compj[j] = (Q_j - 2 sum_d c_jd S_jd) + n_j sum_d c_jd^2, the compactness
 // contribution of centre j against the new centre (sums left to right).
+template <typename T>
 __global__ __launch_bounds__(256) void k_final_centres(int D, int K, const double *__restrict__ S,
                                                        const long long *__restrict__ cnt,
-                                                       const float *C_old, float *C_new,
-                                                       float *__restrict__ Ct, int ldc,
-                                                       float *__restrict__ nc,
+                                                       const T *C_old, T *C_new,
+                                                       T *__restrict__ Ct, int ldc,
+                                                       T *__restrict__ nc,
                                                        long long *__restrict__ counts,
                                                        int *__restrict__ empty,
                                                        double *__restrict__ compj) {
-  extern __shared__ float cs[];
+  extern __shared__ __attribute__((aligned(8))) unsigned char fc_smem[];
+  T *cs = reinterpret_cast<T *>(fc_smem);
   const int j = blockIdx.x;
   const long long n = cnt[j];
   const int ld = D + 1;
   for (int d = threadIdx.x; d < D; d += blockDim.x) {
-    const float v = n > 0 ? (float)(S[(long)j * ld + d] / (double)n) : C_old[(long)j * D + d];
+    const T v = n > 0 ? (T)(S[(long)j * ld + d] / (double)n) : C_old[(long)j * D + d];
     cs[d] = v;
     C_new[(long)j * D + d] = v;
     if (Ct) Ct[(long)d * ldc + j] = v;
@@ -574,8 +620,8 @@ __global__ __launch_bounds__(256) void k_final_centres(int D, int K, const doubl
   __syncthreads();
   if (threadIdx.x == 0) {
     if (nc) {
-      float s = 0.0f;
-      for (int d = 0; d < D; ++d) s = __builtin_fmaf(cs[d], cs[d], s);
+      T s = 0;
+      for (int d = 0; d < D; ++d) s = fma_t(cs[d], cs[d], s);
       nc[j] = s;
     }
     if (compj) {
@@ -593,8 +639,9 @@ __global__ __launch_bounds__(256) void k_final_centres(int D, int K, const doubl
 }
 
 // comp_j terms for arbitrary centres C (k x D) from the moments.
+template <typename T>
 __global__ void k_comp_terms(int D, int K, const double *__restrict__ S,
-                             const long long *__restrict__ cnt, const float *__restrict__ C,
+                             const long long *__restrict__ cnt, const T *__restrict__ C,
                              double *__restrict__ compj) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= K) return;
@@ -657,6 +704,18 @@ __global__ void k_point_sq(long m, int d, const float *__restrict__ X, double *_
   q[i] = s;
 }
 
+// FP64 points: q_i = left-to-right sum over d of the rounded squares
+// x_id * x_id (no fused multiply-add, so the same value as the oracle's).
+__global__ void k_point_sq_f64(long m, int d, const double *__restrict__ X,
+                               double *__restrict__ q) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const double *x = X + i * d;
+  double s = 0.0;
+  for (int t = 0; t < d; ++t) s += x[t] * x[t];
+  q[i] = s;
+}
+
 // ---------------------------------------------------------------- generator
 __global__ void k_generate(uint64_t seed, long i0, long n, int d, int ncomp,
                            const float *__restrict__ means, const float *__restrict__ sigmas,
@@ -681,8 +740,9 @@ __global__ void k_generate(uint64_t seed, long i0, long n, int d, int ncomp,
   }
 }
 
-__global__ void k_gather_rows(long n, int d, const float *__restrict__ X,
-                              const int *__restrict__ idx, float *__restrict__ out) {
+template <typename T>
+__global__ void k_gather_rows(long n, int d, const T *__restrict__ X, const int *__restrict__ idx,
+                              T *__restrict__ out) {
   for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n * d;
        e += (long)gridDim.x * blockDim.x) {
     const long r = e / d;
@@ -692,31 +752,45 @@ __global__ void k_gather_rows(long n, int d, const float *__restrict__ X,
 }
 
 // ---------------------------------------------------------------- launchers
-hipError_t launch_col_norms(hipStream_t s, int k, int d, const float *C, float *nc) {
-  hipLaunchKernelGGL(k_col_norms, dim3(km_cdiv(k, 256)), dim3(256), 0, s, k, d, C, nc);
+template <typename T>
+static hipError_t launch_col_norms_t(hipStream_t s, int k, int d, const T *C, T *nc) {
+  hipLaunchKernelGGL(k_col_norms<T>, dim3(km_cdiv(k, 256)), dim3(256), 0, s, k, d, C, nc);
   return hipGetLastError();
 }
 
-hipError_t launch_row_norms(hipStream_t s, long m, int d, const float *X, float *out) {
+template <typename T>
+static hipError_t launch_row_norms_t(hipStream_t s, long m, int d, const T *X, T *out) {
   if (m <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_row_norms, dim3(km_cdiv(m, 256)), dim3(256), 0, s, m, d, X, out);
+  hipLaunchKernelGGL(k_row_norms<T>, dim3(km_cdiv(m, 256)), dim3(256), 0, s, m, d, X, out);
   return hipGetLastError();
 }
 
-hipError_t launch_transpose(hipStream_t s, int k, int d, const float *C, float *Ct, int ldc) {
+template <typename T>
+static hipError_t launch_transpose_t(hipStream_t s, int k, int d, const T *C, T *Ct, int ldc) {
   dim3 grid(km_cdiv(ldc, 32), km_cdiv(d, 32));
-  hipLaunchKernelGGL(k_transpose, grid, dim3(256), 0, s, k, d, C, Ct, ldc);
+  hipLaunchKernelGGL(k_transpose<T>, grid, dim3(256), 0, s, k, d, C, Ct, ldc);
   return hipGetLastError();
 }
 
 hipError_t launch_assign_f32(hipStream_t s, long m, const int *m_dev, int d, const float *X,
                              const int *rows, int k, const float *Ct, int ldc, const float *nc,
                              int *labels, float *best, const int *out_pos) {
-  long tiles = km_cdiv(m, A_BM);
+  long tiles = km_cdiv(m, A_BM_F32);
   if (tiles <= 0) return hipSuccess;
   long grid = tiles < (long)num_cus() * 4 ? tiles : (long)num_cus() * 4;
   hipLaunchKernelGGL(k_assign_f32, dim3(grid), dim3(256), 0, s, m, m_dev, d, X, rows, k, Ct, ldc,
                      nc, out_pos, labels, best);
+  return hipGetLastError();
+}
+
+hipError_t launch_assign_f64(hipStream_t s, long m, int d, const double *X, int k,
+                             const double *Ct, int ldc, const double *nc, int *labels) {
+  long tiles = km_cdiv(m, A_BM_F64);
+  if (tiles <= 0) return hipSuccess;
+  long grid = tiles < (long)num_cus() * 4 ? tiles : (long)num_cus() * 4;
+  hipLaunchKernelGGL(k_assign_f64, dim3(grid), dim3(256), 0, s, m, (const int *)nullptr, d, X,
+                     (const int *)nullptr, k, Ct, ldc, nc, (const int *)nullptr, labels,
+                     (double *)nullptr);
   return hipGetLastError();
 }
 
@@ -767,6 +841,12 @@ hipError_t launch_point_sq(hipStream_t s, long m, int d, const float *X, double 
   return hipGetLastError();
 }
 
+hipError_t launch_point_sq(hipStream_t s, long m, int d, const double *X, double *q) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_point_sq_f64, dim3(km_cdiv(m, 256)), dim3(256), 0, s, m, d, X, q);
+  return hipGetLastError();
+}
+
 hipError_t launch_update_zero(hipStream_t s, int d, int k, double *S, long long *cnt, int *empty) {
   const long zn = (long)k * (d + 1);
   const long zmax = zn > k ? zn : k;
@@ -781,7 +861,8 @@ hipError_t launch_update_zero(hipStream_t s, int d, int k, double *S, long long 
 // Adds the canonical block sums of rows [0, n) of X (labels and q aligned
 // with X) into S (k x (d+1) FP64: d sums and the q column) and cnt; blocks
 // start at row 0 of X.  q == nullptr: q is computed per block into ws.qb.
-hipError_t launch_update_accumulate(hipStream_t s, long n, int d, int k, const float *X,
+template <typename T>
+static hipError_t launch_update_accumulate_t(hipStream_t s, long n, int d, int k, const T *X,
                                     const int *labels, const double *q, double *S, long long *cnt,
                                     const UpdateWs &ws) {
   hipError_t e;
@@ -808,14 +889,15 @@ hipError_t launch_update_accumulate(hipStream_t s, long n, int d, int k, const f
     hipLaunchKernelGGL(k_cluster_setup, dim3(1), dim3(256), 0, s, k, nsb, ws.offs, ws.cstart,
                        ws.ccount, ws.cq);
     const long qmax = nb / KM_CHUNK + k + 1;
-    if (d % 4 == 0 && d >= 16 && d <= 256 && ((uintptr_t)X & 15) == 0) {
+    if (sizeof(T) == 4 && d % 4 == 0 && d >= 16 && d <= 256 && ((uintptr_t)X & 15) == 0) {
       int TPC = 1;
       while (TPC < d / 4) TPC <<= 1;
       const int G4 = 256 / TPC < 64 ? 256 / TPC : 64;  // must match k_chunk_sums4
-      hipLaunchKernelGGL(k_chunk_sums4, dim3(km_cdiv(qmax, G4)), dim3(320), 0, s, X, d, TPC, qb,
+      hipLaunchKernelGGL(k_chunk_sums4, dim3(km_cdiv(qmax, G4)), dim3(320), 0, s,
+                         reinterpret_cast<const float *>(X), d, TPC, qb,
                          ws.perm, k, ws.cstart, ws.ccount, ws.cq, ws.P);
     } else {
-      hipLaunchKernelGGL(k_chunk_sums, dim3(km_cdiv(qmax, G)), dim3(320), 0, s, X, d, Dp, qb, ws.perm,
+      hipLaunchKernelGGL(k_chunk_sums<T>, dim3(km_cdiv(qmax, G)), dim3(320), 0, s, X, d, Dp, qb, ws.perm,
                          k, ws.cstart, ws.ccount, ws.cq, ws.P);
     }
     hipLaunchKernelGGL(k_block_finalize, dim3(k), dim3(64 * (int)km_cdiv(d + 1 < 256 ? d + 1 : 256, 64)),
@@ -825,21 +907,23 @@ hipError_t launch_update_accumulate(hipStream_t s, long n, int d, int k, const f
   return hipSuccess;
 }
 
-hipError_t launch_update_finalize(hipStream_t s, int d, int k, const double *S,
-                                  const long long *cnt, const float *C_old, float *C_new,
-                                  float *Ct_new, int ldc, float *nc_new, long long *counts,
-                                  int *empty, double *compj) {
-  hipLaunchKernelGGL(k_final_centres, dim3(k), dim3(d < 256 ? 64 * (int)km_cdiv(d, 64) : 256),
-                     sizeof(float) * d, s, d, k, S, cnt, C_old, C_new, Ct_new, ldc, nc_new, counts,
+template <typename T>
+static hipError_t launch_update_finalize_t(hipStream_t s, int d, int k, const double *S,
+                                  const long long *cnt, const T *C_old, T *C_new, T *Ct_new,
+                                  int ldc, T *nc_new, long long *counts, int *empty,
+                                  double *compj) {
+  hipLaunchKernelGGL(k_final_centres<T>, dim3(k), dim3(d < 256 ? 64 * (int)km_cdiv(d, 64) : 256),
+                     sizeof(T) * d, s, d, k, S, cnt, C_old, C_new, Ct_new, ldc, nc_new, counts,
                      empty, compj);
   return hipGetLastError();
 }
 
 // Full update; ws.compj receives the per-centre compactness terms against
 // the new centres (reduce with launch_comp_reduce).
-hipError_t launch_update(hipStream_t s, long n, int d, int k, const float *X, const int *labels,
-                         const double *q, const float *C_old, float *C_new, float *Ct_new, int ldc,
-                         float *nc_new, long long *counts, int *empty, const UpdateWs &ws) {
+template <typename T>
+static hipError_t launch_update_t(hipStream_t s, long n, int d, int k, const T *X, const int *labels,
+                         const double *q, const T *C_old, T *C_new, T *Ct_new, int ldc,
+                         T *nc_new, long long *counts, int *empty, const UpdateWs &ws) {
   hipError_t e;
   if ((e = launch_update_zero(s, d, k, ws.S, ws.cnt, empty)) != hipSuccess) return e;
   if ((e = launch_update_accumulate(s, n, d, k, X, labels, q, ws.S, ws.cnt, ws)) != hipSuccess)
@@ -854,33 +938,80 @@ hipError_t launch_comp_reduce(hipStream_t s, int k, const double *compj, double 
 }
 
 // Compactness of labels against centres C (any centres): moments, then terms.
-hipError_t launch_compactness(hipStream_t s, long n, int d, int k, const float *X,
-                              const int *labels, const double *q, const float *C, double *out,
+template <typename T>
+static hipError_t launch_compactness_t(hipStream_t s, long n, int d, int k, const T *X,
+                              const int *labels, const double *q, const T *C, double *out,
                               const UpdateWs &ws) {
   hipError_t e;
   if ((e = launch_update_zero(s, d, k, ws.S, ws.cnt, nullptr)) != hipSuccess) return e;
   if ((e = launch_update_accumulate(s, n, d, k, X, labels, q, ws.S, ws.cnt, ws)) != hipSuccess)
     return e;
-  hipLaunchKernelGGL(k_comp_terms, dim3(km_cdiv(k, 256)), dim3(256), 0, s, d, k, ws.S, ws.cnt, C,
+  hipLaunchKernelGGL(k_comp_terms<T>, dim3(km_cdiv(k, 256)), dim3(256), 0, s, d, k, ws.S, ws.cnt, C,
                      ws.compj);
   return launch_comp_reduce(s, k, ws.compj, out);
 }
 
 // nc[j] = fma chain over d of Cdk[d*ld + j]^2 (D x K layout, Normalize.cu:33-47).
-__global__ void k_col_norms_dk(int k, int d, int ld, const float *__restrict__ C,
-                               float *__restrict__ nc) {
+template <typename T>
+__global__ void k_col_norms_dk(int k, int d, int ld, const T *__restrict__ C, T *__restrict__ nc) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= k) return;
-  float s = 0.0f;
+  T s = 0;
   for (int q = 0; q < d; ++q) {
-    const float v = C[(long)q * ld + j];
-    s = __builtin_fmaf(v, v, s);
+    const T v = C[(long)q * ld + j];
+    s = fma_t(v, v, s);
   }
   nc[j] = s;
 }
 
-hipError_t launch_col_norms_dk(hipStream_t s, int k, int d, int ld, const float *C, float *nc) {
-  hipLaunchKernelGGL(k_col_norms_dk, dim3(km_cdiv(k, 256)), dim3(256), 0, s, k, d, ld, C, nc);
+template <typename T>
+static hipError_t launch_col_norms_dk_t(hipStream_t s, int k, int d, int ld, const T *C, T *nc) {
+  hipLaunchKernelGGL(k_col_norms_dk<T>, dim3(km_cdiv(k, 256)), dim3(256), 0, s, k, d, ld, C, nc);
+  return hipGetLastError();
+}
+
+// rowTransformMinimum (RowTransformMinimum.cu:3-58): for each row i of the
+// m x n product P (row-major; n = number of centres), the lowest j among the
+// minima of v_ij = (T)((double)(rn[i] + cn[j]) - 2.0 * (double)P[i][j]); the
+// first sum is taken in T and the rest in double, then rounded back to T.
+// Only values below FLT_MAX count (the reference's start value for both
+// types, :17); a row with none gets -1.
+// rn is read at the row's local index i and out at m0 + i, as in the
+// reference kernel (:20, :57).  One wave per row, lanes stride over j and
+// the wave merges with the tie rule.
+template <typename T>
+__global__ __launch_bounds__(256) void k_row_transform_min(long m0, long m, int n,
+                                                           const T *__restrict__ rn,
+                                                           const T *__restrict__ cn,
+                                                           const T *__restrict__ P,
+                                                           int *__restrict__ out) {
+  const T big = (T)FLT_MAX;
+  const int lane = threadIdx.x & 63;
+  const long nw = (long)gridDim.x * (blockDim.x >> 6);
+  for (long i = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < m; i += nw) {
+    const T r = rn[i];
+    T bv = big;
+    int bi = -1;
+    for (int j = lane; j < n; j += 64) {
+      const T v = (T)((double)(r + cn[j]) - 2.0 * (double)P[i * n + j]);
+      if (v < bv) { bv = v; bi = j; }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      const T ov = __shfl_xor(bv, off);
+      const int oi = __shfl_xor(bi, off);
+      if (ov < bv || (ov == bv && (unsigned)oi < (unsigned)bi)) { bv = ov; bi = oi; }
+    }
+    if (lane == 0) out[m0 + i] = bi;
+  }
+}
+
+template <typename T>
+static hipError_t launch_row_transform_min_t(hipStream_t s, long m0, long m, int n, const T *rn,
+                                    const T *cn, const T *P, int *out) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_row_transform_min<T>, dim3(km_grid(m * 64, 256)), dim3(256), 0, s, m0, m, n,
+                     rn, cn, P, out);
   return hipGetLastError();
 }
 
@@ -893,11 +1024,61 @@ hipError_t launch_generate(hipStream_t s, uint64_t seed, long i0, long n, int d,
   return hipGetLastError();
 }
 
-hipError_t launch_gather_rows(hipStream_t s, long n, int d, const float *X, const int *idx,
-                              float *out) {
+template <typename T>
+static hipError_t launch_gather_rows_t(hipStream_t s, long n, int d, const T *X, const int *idx, T *out) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_gather_rows, dim3(km_grid(n * d, 256)), dim3(256), 0, s, n, d, X, idx, out);
+  hipLaunchKernelGGL(k_gather_rows<T>, dim3(km_grid(n * d, 256)), dim3(256), 0, s, n, d, X, idx,
+                     out);
   return hipGetLastError();
 }
+
+// The float and double entry points of every templated launcher.
+#define KM_OVERLOADS(T)                                                                          \
+  hipError_t launch_col_norms(hipStream_t s, int k, int d, const T *C, T *nc) {                  \
+    return launch_col_norms_t<T>(s, k, d, C, nc);                                                \
+  }                                                                                              \
+  hipError_t launch_row_norms(hipStream_t s, long m, int d, const T *X, T *out) {                \
+    return launch_row_norms_t<T>(s, m, d, X, out);                                               \
+  }                                                                                              \
+  hipError_t launch_transpose(hipStream_t s, int k, int d, const T *C, T *Ct, int ldc) {         \
+    return launch_transpose_t<T>(s, k, d, C, Ct, ldc);                                           \
+  }                                                                                              \
+  hipError_t launch_update_accumulate(hipStream_t s, long n, int d, int k, const T *X,           \
+                                      const int *labels, const double *q, double *S,             \
+                                      long long *cnt, const UpdateWs &ws) {                      \
+    return launch_update_accumulate_t<T>(s, n, d, k, X, labels, q, S, cnt, ws);                  \
+  }                                                                                              \
+  hipError_t launch_update_finalize(hipStream_t s, int d, int k, const double *S,                \
+                                    const long long *cnt, const T *C_old, T *C_new, T *Ct_new,   \
+                                    int ldc, T *nc_new, long long *counts, int *empty,           \
+                                    double *compj) {                                             \
+    return launch_update_finalize_t<T>(s, d, k, S, cnt, C_old, C_new, Ct_new, ldc, nc_new,       \
+                                       counts, empty, compj);                                    \
+  }                                                                                              \
+  hipError_t launch_update(hipStream_t s, long n, int d, int k, const T *X, const int *labels,   \
+                           const double *q, const T *C_old, T *C_new, T *Ct_new, int ldc,        \
+                           T *nc_new, long long *counts, int *empty, const UpdateWs &ws) {       \
+    return launch_update_t<T>(s, n, d, k, X, labels, q, C_old, C_new, Ct_new, ldc, nc_new,       \
+                              counts, empty, ws);                                                \
+  }                                                                                              \
+  hipError_t launch_compactness(hipStream_t s, long n, int d, int k, const T *X,                 \
+                                const int *labels, const double *q, const T *C, double *out,     \
+                                const UpdateWs &ws) {                                            \
+    return launch_compactness_t<T>(s, n, d, k, X, labels, q, C, out, ws);                        \
+  }                                                                                              \
+  hipError_t launch_col_norms_dk(hipStream_t s, int k, int d, int ld, const T *C, T *nc) {       \
+    return launch_col_norms_dk_t<T>(s, k, d, ld, C, nc);                                         \
+  }                                                                                              \
+  hipError_t launch_row_transform_min(hipStream_t s, long m0, long m, int n, const T *rn,        \
+                                      const T *cn, const T *P, int *out) {                       \
+    return launch_row_transform_min_t<T>(s, m0, m, n, rn, cn, P, out);                           \
+  }                                                                                              \
+  hipError_t launch_gather_rows(hipStream_t s, long n, int d, const T *X, const int *idx,        \
+                                T *out) {                                                        \
+    return launch_gather_rows_t<T>(s, n, d, X, idx, out);                                        \
+  }
+KM_OVERLOADS(float)
+KM_OVERLOADS(double)
+#undef KM_OVERLOADS
 
 }  // namespace km

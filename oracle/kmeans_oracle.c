@@ -19,6 +19,16 @@
  *   orc_compactness  main/cuda/Compactness.cu:3-59 and main/cpp/kmeans.cpp:
  *                    1045-1104 (sum of squared distances to the NEW centres).
  *   orc_lloyd        main/cpp/kmeans.cpp:1186-1247 (loop, relErr criterion).
+ *   orc_*_f64        the same in double precision (computeKmeansD,
+ *                    kmeans.cpp:1364-1409, and the ...D kernel names,
+ *                    KmeansCudaKernels.h:247-474): every float chain above
+ *                    becomes the double chain (fma in double, scores start
+ *                    at -0.5*nc_j), sums and moments are unchanged (they are
+ *                    FP64 already), centres are S/n without the cast to
+ *                    float.  q_i is the left-to-right sum of the rounded
+ *                    squares x*x, which for float data equals the FP32
+ *                    definition below (those squares are exact).
+ *   orc_row_transform_min{f,d}  main/cuda/RowTransformMinimum.cu:3-58.
  *
  * Canonical summation (the contract the GPU path reproduces bit for bit):
  *   CSUM(v[0..n)) = left-to-right FP64 sum from +0.0 if n <= KM_CHUNK,
@@ -226,60 +236,88 @@ double orc_csum(const double *v, long n) {
   return r;
 }
 
-/* q_i: left-to-right FP64 sum of squares (each (double)x*(double)x is exact). */
-static inline double point_sq(const float *x, int d) {
-  double q = 0.0;
-  for (int t = 0; t < d; ++t) q += (double)x[t] * (double)x[t];
-  return q;
-}
+/* q_i: left-to-right FP64 sum of squares (each (double)x*(double)x is exact
+ * for float x; for double x each square is rounded once, no fma). */
+#define DEF_POINT_SQ(NAME, T)                                                  \
+  static inline double NAME(const T *x, int d) {                               \
+    double q = 0.0;                                                            \
+    for (int t = 0; t < d; ++t) q += (double)x[t] * (double)x[t];              \
+    return q;                                                                  \
+  }
+DEF_POINT_SQ(point_sq, float)
+DEF_POINT_SQ(point_sq_f64, double)
 
 /* Per-block canonical centre sums.  For block b and each centre j, collect
  * the members in ascending i (stable counting sort by label), then CSUM each
  * dimension and the squared norms.  Adds B_b[j][d] into S (FP64, K x d),
  * QB_b[j] into Q and n_b into counts. */
-static void block_update(const float *X, long i0, long n, int d, const int *labels, int k,
-                         double *S, double *Q, long long *counts) {
-  long *cnt = (long *)calloc((size_t)k + 1, sizeof(long));
-  for (long r = 0; r < n; ++r) cnt[labels[i0 + r] + 1]++;
-  for (int j = 0; j < k; ++j) cnt[j + 1] += cnt[j];
-  long *off = (long *)malloc(sizeof(long) * ((size_t)k + 1));
-  memcpy(off, cnt, sizeof(long) * ((size_t)k + 1));
-  long *perm = (long *)malloc(sizeof(long) * (size_t)(n > 0 ? n : 1));
-  for (long r = 0; r < n; ++r) perm[off[labels[i0 + r]]++] = i0 + r;
-#pragma omp parallel
-  {
-    double *v = NULL;
-    long vcap = 0;
-#pragma omp for schedule(dynamic, 1)
-    for (int j = 0; j < k; ++j) {
-      long a = cnt[j], m = cnt[j + 1] - cnt[j];
-      counts[j] += m;
-      if (m > vcap) { free(v); vcap = m; v = (double *)malloc(sizeof(double) * (size_t)m); }
-      for (int q = 0; q < d; ++q) {
-        for (long t = 0; t < m; ++t) v[t] = (double)X[perm[a + t] * d + q];
-        S[(long)j * d + q] += orc_csum(v, m);
-      }
-      for (long t = 0; t < m; ++t) v[t] = point_sq(X + perm[a + t] * d, d);
-      Q[j] += orc_csum(v, m);
-    }
-    free(v);
+#define DEF_BLOCK_UPDATE(NAME, T, PSQ)                                                     \
+  static void NAME(const T *X, long i0, long n, int d, const int *labels, int k,           \
+                   double *S, double *Q, long long *counts) {                              \
+    long *cnt = (long *)calloc((size_t)k + 1, sizeof(long));                               \
+    for (long r = 0; r < n; ++r) cnt[labels[i0 + r] + 1]++;                                \
+    for (int j = 0; j < k; ++j) cnt[j + 1] += cnt[j];                                      \
+    long *off = (long *)malloc(sizeof(long) * ((size_t)k + 1));                            \
+    memcpy(off, cnt, sizeof(long) * ((size_t)k + 1));                                      \
+    long *perm = (long *)malloc(sizeof(long) * (size_t)(n > 0 ? n : 1));                   \
+    for (long r = 0; r < n; ++r) perm[off[labels[i0 + r]]++] = i0 + r;                     \
+    _Pragma("omp parallel")                                                                \
+    {                                                                                      \
+      double *v = NULL;                                                                    \
+      long vcap = 0;                                                                       \
+      _Pragma("omp for schedule(dynamic, 1)")                                              \
+      for (int j = 0; j < k; ++j) {                                                        \
+        long a = cnt[j], m = cnt[j + 1] - cnt[j];                                          \
+        counts[j] += m;                                                                    \
+        if (m > vcap) { free(v); vcap = m; v = (double *)malloc(sizeof(double) * (size_t)m); } \
+        for (int q = 0; q < d; ++q) {                                                      \
+          for (long t = 0; t < m; ++t) v[t] = (double)X[perm[a + t] * d + q];              \
+          S[(long)j * d + q] += orc_csum(v, m);                                            \
+        }                                                                                  \
+        for (long t = 0; t < m; ++t) v[t] = PSQ(X + perm[a + t] * d, d);                   \
+        Q[j] += orc_csum(v, m);                                                            \
+      }                                                                                    \
+      free(v);                                                                             \
+    }                                                                                      \
+    free(perm); free(off); free(cnt);                                                      \
   }
-  free(perm); free(off); free(cnt);
-}
+DEF_BLOCK_UPDATE(block_update, float, point_sq)
+DEF_BLOCK_UPDATE(block_update_f64, double, point_sq_f64)
 
 /* Canonical per-centre moments over all points: S (k x d), Q (k), counts. */
-void orc_moments(const float *X, long n, int d, const int *labels, int k, double *S, double *Q,
-                 long long *counts) {
-  memset(S, 0, sizeof(double) * (size_t)k * d);
-  memset(Q, 0, sizeof(double) * (size_t)k);
-  memset(counts, 0, sizeof(long long) * (size_t)k);
-  for (long i0 = 0; i0 < n; i0 += KM_BLOCK) {
-    long nb = (n - i0) < KM_BLOCK ? (n - i0) : KM_BLOCK;
-    block_update(X, i0, nb, d, labels, k, S, Q, counts);
+#define DEF_MOMENTS(NAME, T, BLK)                                                          \
+  void NAME(const T *X, long n, int d, const int *labels, int k, double *S, double *Q,     \
+            long long *counts) {                                                           \
+    memset(S, 0, sizeof(double) * (size_t)k * d);                                          \
+    memset(Q, 0, sizeof(double) * (size_t)k);                                              \
+    memset(counts, 0, sizeof(long long) * (size_t)k);                                      \
+    for (long i0 = 0; i0 < n; i0 += KM_BLOCK) {                                            \
+      long nb = (n - i0) < KM_BLOCK ? (n - i0) : KM_BLOCK;                                 \
+      BLK(X, i0, nb, d, labels, k, S, Q, counts);                                          \
+    }                                                                                      \
   }
-}
+DEF_MOMENTS(orc_moments, float, block_update)
+DEF_MOMENTS(orc_moments_f64, double, block_update_f64)
 
 /* Compactness from moments and centres C (k x d, any centres). */
+#define DEF_COMP_MOMENTS(NAME, T)                                                          \
+  double NAME(const double *S, const double *Q, const long long *counts, const T *C, int k,  \
+              int d) {                                                                     \
+    double *cj = (double *)malloc(sizeof(double) * (size_t)(k > 0 ? k : 1));               \
+    for (int j = 0; j < k; ++j) {                                                          \
+      double a = 0.0, b = 0.0;                                                             \
+      for (int q = 0; q < d; ++q) {                                                        \
+        const double c = (double)C[(long)j * d + q];                                       \
+        a += c * S[(long)j * d + q];                                                       \
+        b += c * c;                                                                        \
+      }                                                                                    \
+      cj[j] = (Q[j] - 2.0 * a) + (double)counts[j] * b;                                    \
+    }                                                                                      \
+    double r = orc_csum(cj, k);                                                            \
+    free(cj);                                                                              \
+    return r;                                                                              \
+  }
+DEF_COMP_MOMENTS(orc_compactness_moments_f64, double)
 double orc_compactness_moments(const double *S, const double *Q, const long long *counts,
                                const float *C, int k, int d) {
   double *cj = (double *)malloc(sizeof(double) * (size_t)(k > 0 ? k : 1));
@@ -361,6 +399,136 @@ int orc_lloyd(const float *X, long n, int d, int k, float *C, int maxIters, doub
   if (iters_out) *iters_out = it;
   if (relerr_out) *relerr_out = relErr;
   return relErr <= criterion ? 0 : 13;
+}
+
+/* ------------------------------------------------------------------ */
+/* Double-precision engine (computeKmeansD and the ...D kernel names). */
+void orc_col_norms_f64(const double *C, int k, int d, double *nc) {
+  for (int j = 0; j < k; ++j) {
+    double s = 0.0;
+    for (int q = 0; q < d; ++q) s = fma(C[(long)j * d + q], C[(long)j * d + q], s);
+    nc[j] = s;
+  }
+}
+
+/* rowNormalizeD (Normalize.cu:3-31): fma chain of the squares of each row. */
+void orc_row_norms_f64(const double *X, long n, int d, double *out) {
+  for (long i = 0; i < n; ++i) {
+    double s = 0.0;
+    for (int q = 0; q < d; ++q) s = fma(X[i * d + q], X[i * d + q], s);
+    out[i] = s;
+  }
+}
+
+double orc_score_f64(const double *x, const double *c, int d, double ncj) {
+  double s = -0.5 * ncj;
+  for (int q = 0; q < d; ++q) s = fma(x[q], c[q], s);
+  return s;
+}
+
+/* Labels in double: s_ij = fma chain from -0.5*nc_j, lowest index among the
+ * maxima.  One point per iteration, centres in order (no SIMD needed for the
+ * sizes the tests use). */
+void orc_assign_f64(const double *X, long n, int d, const double *C, int k, const double *nc,
+                    int *labels) {
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; ++i) {
+    double best = -INFINITY;
+    int lab = 0;
+    for (int j = 0; j < k; ++j) {
+      const double v = orc_score_f64(X + i * d, C + (long)j * d, d, nc[j]);
+      if (v > best) { best = v; lab = j; }
+    }
+    labels[i] = lab;
+  }
+}
+
+int orc_update_f64(const double *X, long n, int d, const int *labels, int k, const double *C_old,
+                   double *C_new, long long *counts_out, double *comp) {
+  double *S = (double *)malloc(sizeof(double) * (size_t)k * d);
+  double *Q = (double *)malloc(sizeof(double) * (size_t)k);
+  long long *counts = (long long *)malloc(sizeof(long long) * (size_t)k);
+  orc_moments_f64(X, n, d, labels, k, S, Q, counts);
+  int empty = 0;
+  for (int j = 0; j < k; ++j) {
+    if (counts[j] > 0) {
+      for (int q = 0; q < d; ++q) C_new[(long)j * d + q] = S[(long)j * d + q] / (double)counts[j];
+    } else {
+      ++empty;
+      if (C_new != C_old)
+        for (int q = 0; q < d; ++q) C_new[(long)j * d + q] = C_old[(long)j * d + q];
+    }
+  }
+  if (comp) *comp = orc_compactness_moments_f64(S, Q, counts, C_new, k, d);
+  if (counts_out) memcpy(counts_out, counts, sizeof(long long) * (size_t)k);
+  free(S); free(Q); free(counts);
+  return empty;
+}
+
+double orc_compactness_f64(const double *X, long n, int d, const int *labels, const double *C,
+                           int k) {
+  double *S = (double *)malloc(sizeof(double) * (size_t)k * d);
+  double *Q = (double *)malloc(sizeof(double) * (size_t)k);
+  long long *counts = (long long *)malloc(sizeof(long long) * (size_t)k);
+  orc_moments_f64(X, n, d, labels, k, S, Q, counts);
+  double r = orc_compactness_moments_f64(S, Q, counts, C, k, d);
+  free(S); free(Q); free(counts);
+  return r;
+}
+
+/* Lloyd loop in double (same control flow as orc_lloyd). */
+int orc_lloyd_f64(const double *X, long n, int d, int k, double *C, int maxIters,
+                  double criterion, int *labels, double *comp_hist, int *iters_out,
+                  double *relerr_out) {
+  double *nc = (double *)malloc(sizeof(double) * (size_t)k);
+  double prev = 0.0, relErr = 1.0;
+  int it = 0;
+  while (it < maxIters && relErr > criterion) {
+    orc_col_norms_f64(C, k, d, nc);
+    orc_assign_f64(X, n, d, C, k, nc, labels);
+    double comp = 0.0;
+    orc_update_f64(X, n, d, labels, k, C, C, NULL, &comp);
+    relErr = fabs(comp - prev) / fabs(comp);
+    prev = comp;
+    if (comp_hist) comp_hist[it] = comp;
+    ++it;
+  }
+  free(nc);
+  if (iters_out) *iters_out = it;
+  if (relerr_out) *relerr_out = relErr;
+  return relErr <= criterion ? 0 : 13;
+}
+
+/* rowTransformMinimum (RowTransformMinimum.cu:3-58): for row i of the m x n
+ * product P, v_ij = (T)((double)(rn[i] + cn[j]) - 2.0 * (double)P[i][j])
+ * (the first sum in T: TYPE + TYPE, then a double expression from the
+ * 2.0 literal, assigned back to TYPE); out[m0 + i] = the lowest j with the
+ * smallest v below FLT_MAX (the start value for both types, :17), or -1 if
+ * there is none.  rn is
+ * indexed by the local row i (:20). */
+void orc_row_transform_min_f(long m0, long m, int n, const float *rn, const float *cn,
+                             const float *P, int *out) {
+  for (long i = 0; i < m; ++i) {
+    float bv = 3.40282346638528859812e+38f;
+    int bi = -1;
+    for (int j = 0; j < n; ++j) {
+      const float v = (float)((double)(rn[i] + cn[j]) - 2.0 * (double)P[i * n + j]);
+      if (v < bv) { bv = v; bi = j; }
+    }
+    out[m0 + i] = bi;
+  }
+}
+void orc_row_transform_min_d(long m0, long m, int n, const double *rn, const double *cn,
+                             const double *P, int *out) {
+  for (long i = 0; i < m; ++i) {
+    double bv = 3.40282346638528859812e+38;  /* FLT_MAX for both types (:17) */
+    int bi = -1;
+    for (int j = 0; j < n; ++j) {
+      const double v = (rn[i] + cn[j]) - 2.0 * P[i * n + j];
+      if (v < bv) { bv = v; bi = j; }
+    }
+    out[m0 + i] = bi;
+  }
 }
 
 /* ------------------------------------------------------------------ */

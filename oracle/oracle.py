@@ -47,6 +47,16 @@ def lib():
             "orc_moments": (None, [P, i64, i32, P, i32, P, P, P]),
             "orc_lloyd": (i32, [P, i64, i32, i32, P, i32, f64, P, P, P, P]),
             "orc_means_fp64": (None, [P, i64, i32, P, i32, P, P]),
+            "orc_col_norms_f64": (None, [P, i32, i32, P]),
+            "orc_row_norms_f64": (None, [P, i64, i32, P]),
+            "orc_score_f64": (f64, [P, P, i32, f64]),
+            "orc_assign_f64": (None, [P, i64, i32, P, i32, P, P]),
+            "orc_moments_f64": (None, [P, i64, i32, P, i32, P, P, P]),
+            "orc_update_f64": (i32, [P, i64, i32, P, i32, P, P, P, P]),
+            "orc_compactness_f64": (f64, [P, i64, i32, P, P, i32]),
+            "orc_lloyd_f64": (i32, [P, i64, i32, i32, P, i32, f64, P, P, P, P]),
+            "orc_row_transform_min_f": (None, [i64, i64, i32, P, P, P, P]),
+            "orc_row_transform_min_d": (None, [i64, i64, i32, P, P, P, P]),
             "orc_compactness_fp64": (f64, [P, i64, i32, P, P]),
             "orc_mlp_fwd_bwd": (f64, [i32, i32, i32, i32, P, P, P, P, i32]),
             "orc_num_threads": (i32, []),
@@ -187,6 +197,95 @@ def means_fp64(X, labels, k):
 def compactness_fp64(X, labels, C):
     X, C, labels = _f32(X), _f32(C), _i32(labels)
     return lib().orc_compactness_fp64(_p(X), X.shape[0], X.shape[1], _p(labels), _p(C))
+
+
+# ---------------------------------------------------------------- double precision
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def col_norms64(C):
+    C = _f64(C)
+    nc = np.empty(C.shape[0], np.float64)
+    lib().orc_col_norms_f64(_p(C), C.shape[0], C.shape[1], _p(nc))
+    return nc
+
+
+def row_norms64(X):
+    X = _f64(X)
+    out = np.empty(X.shape[0], np.float64)
+    lib().orc_row_norms_f64(_p(X), X.shape[0], X.shape[1], _p(out))
+    return out
+
+
+def score64(x, c, ncj):
+    x, c = _f64(x), _f64(c)
+    return lib().orc_score_f64(_p(x), _p(c), x.shape[0], float(ncj))
+
+
+def assign64(X, C, nc=None):
+    X, C = _f64(X), _f64(C)
+    nc = col_norms64(C) if nc is None else _f64(nc)
+    lab = np.empty(X.shape[0], np.int32)
+    lib().orc_assign_f64(_p(X), X.shape[0], X.shape[1], _p(C), C.shape[0], _p(nc), _p(lab))
+    return lab
+
+
+def moments64(X, labels, k):
+    X, labels = _f64(X), _i32(labels)
+    n, d = X.shape
+    S = np.empty((k, d), np.float64)
+    Q = np.empty(k, np.float64)
+    cnt = np.empty(k, np.int64)
+    lib().orc_moments_f64(_p(X), n, d, _p(labels), k, _p(S), _p(Q), _p(cnt))
+    return S, Q, cnt
+
+
+def update64(X, labels, C_old):
+    """Double-precision update.  Returns (C_new, counts, empty, comp vs C_new)."""
+    X, C_old, labels = _f64(X), _f64(C_old), _i32(labels)
+    n, d = X.shape
+    k = C_old.shape[0]
+    C_new = np.empty_like(C_old)
+    counts = np.empty(k, np.int64)
+    comp = ctypes.c_double(0)
+    empty = lib().orc_update_f64(_p(X), n, d, _p(labels), k, _p(C_old), _p(C_new), _p(counts),
+                                 ctypes.byref(comp))
+    return C_new, counts, empty, comp.value
+
+
+def compactness64(X, labels, C):
+    X, C, labels = _f64(X), _f64(C), _i32(labels)
+    return lib().orc_compactness_f64(_p(X), X.shape[0], X.shape[1], _p(labels), _p(C), C.shape[0])
+
+
+def lloyd64(X, C0, max_iters, criterion=-1.0):
+    """Double-precision Lloyd loop; same result dict as lloyd()."""
+    X = _f64(X)
+    C = _f64(C0).copy()
+    n, d = X.shape
+    k = C.shape[0]
+    lab = np.empty(n, np.int32)
+    hist = np.zeros(max(max_iters, 1), np.float64)
+    it = ctypes.c_int(0)
+    rel = ctypes.c_double(0)
+    st = lib().orc_lloyd_f64(_p(X), n, d, k, _p(C), max_iters, float(criterion), _p(lab), _p(hist),
+                             ctypes.byref(it), ctypes.byref(rel))
+    return dict(labels=lab, centers=C, comp_hist=hist[: it.value], iters=it.value,
+                rel_err=rel.value, status=st)
+
+
+def row_transform_min(P, rn, cn, m0=0, out=None):
+    """rowTransformMinimum{F,D} on a host m x n product (dtype picks the variant)."""
+    f = P.dtype == np.float32
+    cv = _f32 if f else _f64
+    P, rn, cn = cv(P), cv(rn), cv(cn)
+    m, n = P.shape
+    if out is None:
+        out = np.full(m0 + m, -7, np.int32)
+    fn = lib().orc_row_transform_min_f if f else lib().orc_row_transform_min_d
+    fn(m0, m, n, _p(rn), _p(cn), _p(P), _p(out))
+    return out
 
 
 def num_threads():

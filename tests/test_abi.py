@@ -53,6 +53,18 @@ def test_kernel_api_rejects_bad_arguments():
     assert L.kmAssignF32(None, 10, None, 0, None, None, 4, None, None, None, None) == 8
     assert L.kmUpdate(None, 10, 4, 2, None, None, None, None, None, None, None, None, None, 0) == 8
     assert L.kmCompactness(None, 10, 4, 2, None, None, None, None, None, 0) == 8
+    assert L.rowTransformMinimumF(0, 10, 4, None, None, None, None) == 8
+    assert L.rowTransformMinimumD(-1, 10, 4, None, None, None, None) == 8
+    assert L.ClosestCentersD(0, 10, 0, None, 4, None, None, None, None, 0, None, None, None) == 8
+
+
+def test_compute_kmeans_d_rejects_bad_shapes_without_device():
+    X = np.zeros((10, 3))
+    for k in (11, 0):
+        st, _ = kmeans_amd.compute_kmeans_d(X, k, -1.0, 3) if k else (
+            kmeans_amd.lib().computeKmeansD(kmeans_amd._np_ptr(X), 10, 3, 0, -1.0, 3, 0, 0, 0, 0,
+                                            kmeans_amd._np_ptr(np.zeros(3)), 1.0), None)
+        assert st == kmeans_amd.KMEANS_ERROR_INVALID_ARGUMENT
 
 
 def test_mlp_api_rejects_bad_arguments():

@@ -304,3 +304,132 @@ def test_mlp_oracle_loss_closed_form(orc):
     gb3 = G[-g:]
     expect = np.full(g, 1.0 / g) - np.bincount(t, minlength=g) / B
     assert np.allclose(gb3, expect, atol=1e-15)
+
+
+# ------------------------------------------------------------------ double precision
+def exact_chain64(x, c, ncj):
+    """fma chain in double by exact rationals: float(Fraction) rounds to nearest even."""
+    s = float(Fraction(-0.5) * Fraction(float(ncj)))
+    for a, b in zip(x, c):
+        s = float(Fraction(float(a)) * Fraction(float(b)) + Fraction(s))
+    return s
+
+
+def test_f64_norm_and_score_are_single_rounding_fma_chains(orc):
+    rng = np.random.default_rng(8)
+    for d in (1, 3, 17, 40):
+        for _ in range(20):
+            x = rng.standard_normal(d) * 3
+            c = rng.standard_normal(d) * 3
+            nc = orc.col_norms64(c[None, :])[0]
+            ref = 0.0
+            for a in c:
+                ref = float(Fraction(float(a)) ** 2 + Fraction(ref))
+            assert nc == ref
+            assert orc.row_norms64(c[None, :])[0] == ref
+            assert orc.score64(x, c, nc) == exact_chain64(x, c, nc)
+
+
+def test_f64_fma_chain_differs_from_mul_add(orc):
+    # (1 + 2^-30)^2 = 1 + 2^-29 + 2^-60: the 2^-60 survives only without a
+    # rounding of the product (score = -1 + x*x with ncj = 2)
+    x = np.array([1.0 + 2.0 ** -30])
+    s = orc.score64(x, x, 2.0)
+    assert s == 2.0 ** -29 + 2.0 ** -60
+    assert (x[0] * x[0]) - 1.0 == 2.0 ** -29
+
+
+def test_f64_assign_matches_bruteforce_and_ties(orc):
+    rng = np.random.default_rng(9)
+    X = rng.standard_normal((2000, 13)) * 4
+    C = rng.standard_normal((37, 13)) * 4
+    lab = orc.assign64(X, C)
+    D2 = ((X[:, None, :] - C[None]) ** 2).sum(-1)
+    srt = np.sort(D2, 1)
+    clear = (srt[:, 1] - srt[:, 0]) > 1e-9
+    assert clear.all()
+    assert np.array_equal(lab, D2.argmin(1))
+    # duplicate centres: the lower index wins
+    C2 = np.concatenate([C[:5], C[:5]])
+    lab2 = orc.assign64(X, C2)
+    assert lab2.max() < 5
+
+
+def test_f64_moments_equal_fp32_moments_on_float_data(orc):
+    # on float-valued points the double engine's sums and squared norms are the
+    # FP32 engine's bit for bit (same canonical order; the squares are exact),
+    # and rounding its centres to float gives the FP32 centres
+    X, _, _, _ = orc.mixture(21, 20_000, 9, 12, -5, 5, 1.0, 1.0)
+    C = X[:12].copy()
+    lab = orc.assign(X, C)
+    S32, Q32, n32 = orc.moments(X, lab, 12)
+    S64, Q64, n64 = orc.moments64(X.astype(np.float64), lab, 12)
+    assert np.array_equal(S32, S64) and np.array_equal(Q32, Q64) and np.array_equal(n32, n64)
+    C32 = orc.update(X, lab, C)[0]
+    C64, cnt, empty, comp = orc.update64(X.astype(np.float64), lab, C.astype(np.float64))
+    assert empty == 0
+    assert np.array_equal(C64.astype(np.float32), C32)
+
+
+def test_f64_update_and_compactness_vs_numpy(orc):
+    rng = np.random.default_rng(10)
+    X = rng.standard_normal((30_000, 7)) * 2 + 1
+    lab = rng.integers(0, 9, 30_000).astype(np.int32)
+    lab[lab == 4] = 5  # centre 4 empty
+    C0 = rng.standard_normal((9, 7))
+    C, cnt, empty, comp = orc.update64(X, lab, C0)
+    assert empty == 1 and cnt[4] == 0
+    assert np.array_equal(C[4], C0[4])
+    for j in range(9):
+        if cnt[j]:
+            np.testing.assert_allclose(C[j], X[lab == j].mean(0), rtol=1e-13, atol=1e-14)
+    direct = ((X - C[lab]) ** 2).sum()
+    assert abs(comp - direct) <= 1e-10 * direct
+    assert orc.compactness64(X, lab, C) == comp
+
+
+def test_f64_lloyd_closed_form_and_criterion(orc):
+    # well separated, tiny spread: one step lands on the component means
+    rng = np.random.default_rng(11)
+    means = rng.uniform(-50, 50, (6, 5))
+    z = np.arange(6000) % 6
+    X = means[z] + rng.standard_normal((6000, 5)) * 1e-3
+    r = orc.lloyd64(X, X[:6], 3)
+    for j in range(6):
+        np.testing.assert_allclose(r["centers"][j], X[z == j].mean(0), rtol=0, atol=1e-12)
+    assert r["status"] == 13
+    r2 = orc.lloyd64(X, X[:6], 50, criterion=1e-9)
+    assert r2["status"] == 0 and r2["iters"] < 50
+
+
+def test_row_transform_min_known_answers(orc):
+    # one row, n = 4: values rn + cn_j - 2 P_j = [5, 3, 3, 7] -> index 1 (tie to the lower)
+    for dt in (np.float32, np.float64):
+        P = np.array([[0.0, 1.0, 0.5, -1.0]], dt)
+        rn = np.array([1.0], dt)
+        cn = np.array([4.0, 4.0, 3.0, 4.0], dt)
+        out = orc.row_transform_min(P, rn, cn, m0=2)
+        assert list(out) == [-7, -7, 1]
+        # nothing below FLT_MAX (for double too): -1 (P = -inf gives +inf, NaN never compares)
+        big = np.array([[-np.inf, np.nan]], dt)
+        assert orc.row_transform_min(big, np.zeros(1, dt), np.zeros(2, dt))[0] == -1
+        if dt == np.float64:
+            P2 = np.array([[0.0, 0.0]], dt)
+            assert orc.row_transform_min(P2, np.zeros(1, dt), np.array([1e39, 2e39]))[0] == -1
+
+
+def test_row_transform_min_matches_nearest_centre(orc):
+    # with P = X C^T and the squared norms, the minimiser is the nearest centre
+    X, z, means, _ = orc.mixture(22, 3000, 16, 30, -10, 10, 0.5, 0.5)
+    C = means + 0.01
+    X64, C64 = X.astype(np.float64), C.astype(np.float64)
+    D2 = ((X64[:, None, :] - C64[None]) ** 2).sum(-1)
+    for dt in (np.float32, np.float64):
+        P = (X.astype(dt) @ C.astype(dt).T).astype(dt)
+        rn = (X.astype(dt) ** 2).sum(1).astype(dt)
+        cn = (C.astype(dt) ** 2).sum(1).astype(dt)
+        out = orc.row_transform_min(P, rn, cn)
+        srt = np.sort(D2, 1)
+        clear = (srt[:, 1] - srt[:, 0]) > 1e-1
+        assert clear.mean() > 0.99
+        assert np.array_equal(out[clear], D2.argmin(1)[clear])
