@@ -141,6 +141,10 @@ typedef struct kmeansOptions {
   int top_groups;         /* groups proposed per point (0: 4; at most 8)      */
   int train_iters;        /* plain iterations before the proposer (0: 3)      */
   int hidden;             /* MLP hidden width (0: 256; multiple of 16)        */
+  int full_gemm;          /* 1: assign by a library GEMM (rocBLAS SGEMM) into
+                             an HBM product and a row argmin, the reference's
+                             doFullSGEMM = 1 path (FP32 precision only; labels
+                             follow the library's rounding)                   */
 } kmeansOptions;
 
 typedef struct kmeansStats {

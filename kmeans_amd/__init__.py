@@ -49,7 +49,7 @@ class Options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("device", ctypes.c_int), ("stream", ctypes.c_void_p),
                 ("tile_rows", ctypes.c_long), ("timing", ctypes.c_int), ("proposer", ctypes.c_int),
                 ("groups", ctypes.c_int), ("top_groups", ctypes.c_int), ("train_iters", ctypes.c_int),
-                ("hidden", ctypes.c_int)]
+                ("hidden", ctypes.c_int), ("full_gemm", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -157,24 +157,26 @@ def _check(code, where):
 
 # ------------------------------------------------------------------ host API
 def compute_kmeans_f(data, k, criterion, max_iters, result=None, init_from_result=False,
-                     mini_batch_fraction=1.0):
-    """computeKmeansF: returns (status, centers as D x K)."""
+                     mini_batch_fraction=1.0, full_gemm=False):
+    """computeKmeansF: returns (status, centers as D x K).  full_gemm: doFullSGEMM = 1."""
     data = np.ascontiguousarray(data, dtype=np.float32)
     m, n = data.shape
     res = np.zeros((n, k), np.float32) if result is None else np.ascontiguousarray(result, np.float32)
     st = lib().computeKmeansF(_np_ptr(data), m, n, k, criterion, max_iters, 0,
-                              1 if init_from_result else 0, 0, 0, _np_ptr(res), mini_batch_fraction)
+                              1 if init_from_result else 0, 1 if full_gemm else 0, 0, _np_ptr(res),
+                              mini_batch_fraction)
     return st, res
 
 
 def compute_kmeans_d(data, k, criterion, max_iters, result=None, init_from_result=False,
-                     mini_batch_fraction=1.0):
-    """computeKmeansD: returns (status, centers as D x K, float64)."""
+                     mini_batch_fraction=1.0, full_gemm=False):
+    """computeKmeansD: returns (status, centers as D x K, float64).  full_gemm: doFullSGEMM = 1."""
     data = np.ascontiguousarray(data, dtype=np.float64)
     m, n = data.shape
     res = np.zeros((n, k), np.float64) if result is None else np.ascontiguousarray(result, np.float64)
     st = lib().computeKmeansD(_np_ptr(data), m, n, k, criterion, max_iters, 0,
-                              1 if init_from_result else 0, 0, 0, _np_ptr(res), mini_batch_fraction)
+                              1 if init_from_result else 0, 1 if full_gemm else 0, 0, _np_ptr(res),
+                              mini_batch_fraction)
     return st, res
 
 
@@ -182,15 +184,18 @@ class KMeans:
     """The engine handle: kmeansCreate / SetData / SetCenters / Run / Get*."""
 
     def __init__(self, m, n, k, precision="fp32", device=0, stream=None, timing=False, tile_rows=0,
-                 proposer=None, groups=0, top_groups=0, train_iters=0, hidden=0):
+                 proposer=None, groups=0, top_groups=0, train_iters=0, hidden=0, full_gemm=False):
         """tile_rows > 0 (a multiple of KM_BLOCK) with host data: stream the points
         through HBM in tiles of that many rows instead of keeping them resident.
         proposer "mlp" | "centroid": after train_iters plain iterations, score each
         point only against the members of top_groups proposed centre groups and
-        certify the label (uncertified points are searched in full)."""
+        certify the label (uncertified points are searched in full).
+        full_gemm: assign through a library GEMM (rocBLAS SGEMM) and a row argmin,
+        the reference's doFullSGEMM = 1 path (fp32 only)."""
         self.m, self.n, self.k = int(m), int(n), int(k)
         opt = Options(PREC[precision], device, stream, int(tile_rows), 1 if timing else 0,
-                      PROPOSER[proposer], int(groups), int(top_groups), int(train_iters), int(hidden))
+                      PROPOSER[proposer], int(groups), int(top_groups), int(train_iters), int(hidden),
+                      1 if full_gemm else 0)
         h = ctypes.c_void_p()
         _check(lib().kmeansCreate(ctypes.byref(h), self.m, self.n, self.k, ctypes.byref(opt)),
                "kmeansCreate")

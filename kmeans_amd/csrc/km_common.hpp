@@ -75,7 +75,7 @@ inline void sample_rows(uint64_t seed, long m, int k, int *idx) {
 }
 
 kmeansErrorStatus compute_d(const double *data, int m, int n, int k, double criterion,
-                            int maxIters, int initAlgorithm, double *result,
+                            int maxIters, int initAlgorithm, int fullGemm, double *result,
                             double miniBatchFraction);
 
 // Internal launchers (implemented in km_kernels.hip); all stream-ordered.
@@ -110,7 +110,7 @@ int sub_block(int k);
   hipError_t launch_col_norms_dk(hipStream_t s, int k, int d, int ld, const T *C, T *nc);        \
   hipError_t launch_row_norms(hipStream_t s, long m, int d, const T *X, T *out);                 \
   hipError_t launch_row_transform_min(hipStream_t s, long m0, long m, int n, const T *rn,        \
-                                      const T *cn, const T *P, int *out);                        \
+                                      const T *cn, const T *P, int *out, int none);              \
   hipError_t launch_gather_rows(hipStream_t s, long n, int d, const T *X, const int *idx, T *out);
 KM_TYPED_LAUNCHERS(float)
 KM_TYPED_LAUNCHERS(double)

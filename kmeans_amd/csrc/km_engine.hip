@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "km_engine.hpp"
+#include "km_gemm.hpp"
 #include "km_learned.hpp"
 #include "km_mfma.hpp"
 #include "km_mlp.hpp"
@@ -24,6 +25,7 @@ Engine::~Engine() {
   if (stream) hipStreamSynchronize(stream);
   mfma_free(this);
   learned_free(this);
+  gemm_free(this);
   hipFree(dX_owned);
   hipFree(dC);
   hipFree(dCt);
@@ -57,6 +59,8 @@ kmeansErrorStatus Engine::create(long m_, int n_, int k_, const kmeansOptions *o
   k = k_;
   if (o) opt = *o;
   if (opt.precision < KM_PREC_FP32 || opt.precision > KM_PREC_FP16)
+    return KMEANS_ERROR_INVALID_ARGUMENT;
+  if (opt.full_gemm && (opt.precision != KM_PREC_FP32 || opt.proposer != KM_PROPOSER_NONE))
     return KMEANS_ERROR_INVALID_ARGUMENT;
   if (learned_configure(this) != KMEANS_SUCCESS) return KMEANS_ERROR_INVALID_ARGUMENT;
   int ndev = 0;
@@ -181,7 +185,9 @@ kmeansErrorStatus Engine::reset() { return KMEANS_SUCCESS; }
 
 kmeansErrorStatus Engine::closestCenters(long r0, long rows) {
   hipError_t e;
-  if (opt.precision == KM_PREC_FP32) {
+  if (opt.full_gemm) {
+    e = gemm_assign(this, dX + r0 * n, rows, dlabels + r0);
+  } else if (opt.precision == KM_PREC_FP32) {
     e = launch_assign_f32(stream, rows, nullptr, n, dX + r0 * n, nullptr, k, dCt, ldc, dnc,
                           dlabels + r0, nullptr);
   } else {
@@ -422,7 +428,9 @@ kmeansErrorStatus Engine::stream_iteration(int slot) {
       if ((err = load_tile(t + 1, b ^ 1)) != KMEANS_SUCCESS) return err;
     KM_HIP_CHECK(hipStreamWaitEvent(stream, ev_loaded[b], 0), KMEANS_ERROR_COPY_TILE);
     KM_HIP_CHECK(launch_point_sq(stream, rows, n, dXt[b], dqt[b]), KMEANS_ERROR_COPY_TILE);
-    if (opt.precision == KM_PREC_FP32) {
+    if (opt.full_gemm) {
+      KM_HIP_CHECK(gemm_assign(this, dXt[b], rows, dlabels + r0), KMEANS_ERROR_CLOSEST_CENTERS);
+    } else if (opt.precision == KM_PREC_FP32) {
       KM_HIP_CHECK(launch_assign_f32(stream, rows, nullptr, n, dXt[b], nullptr, k, dCt, ldc, dnc,
                                      dlabels + r0, nullptr),
                    KMEANS_ERROR_CLOSEST_CENTERS);
@@ -588,11 +596,13 @@ kmeansErrorStatus kmeansSetMiniBatch(kmeansHandle h, long rows) {
 }
 
 static kmeansErrorStatus compute_f(const float *data, int m, int n, int k, double criterion,
-                                   int maxIters, int initAlgorithm, float *result,
+                                   int maxIters, int initAlgorithm, int fullGemm, float *result,
                                    double miniBatchFraction) {
   if (!data || !result) return KMEANS_ERROR_INVALID_ARGUMENT;
   kmeansHandle h = nullptr;
-  kmeansErrorStatus st = kmeansCreate(&h, m, n, k, nullptr);
+  kmeansOptions opt = {};
+  opt.full_gemm = fullGemm ? 1 : 0;
+  kmeansErrorStatus st = kmeansCreate(&h, m, n, k, &opt);
   if (st != KMEANS_SUCCESS) return st;
   std::vector<float> c((size_t)k * n);
   if ((st = kmeansSetData(h, data, 0)) != KMEANS_SUCCESS) goto done;
@@ -623,18 +633,17 @@ kmeansErrorStatus computeKmeansF(const float *data, int m, int n, int k, float c
                                  int maxIters, int numRetries, int initAlgorithm, int doFullSGEMM,
                                  int matrixFormat, float *result, float miniBatchFraction) {
   (void)numRetries;
-  (void)doFullSGEMM;
   (void)matrixFormat;
-  return compute_f(data, m, n, k, criterion, maxIters, initAlgorithm, result, miniBatchFraction);
+  return compute_f(data, m, n, k, criterion, maxIters, initAlgorithm, doFullSGEMM, result,
+                   miniBatchFraction);
 }
 
 kmeansErrorStatus computeKmeansD(const double *data, int m, int n, int k, double criterion,
                                  int maxIters, int numRetries, int initAlgorithm, int doFullSGEMM,
                                  int matrixFormat, double *result, double miniBatchFraction) {
   (void)numRetries;
-  (void)doFullSGEMM;
   (void)matrixFormat;
-  return km::compute_d(data, m, n, k, criterion, maxIters, initAlgorithm, result,
+  return km::compute_d(data, m, n, k, criterion, maxIters, initAlgorithm, doFullSGEMM, result,
                        miniBatchFraction);
 }
 
@@ -878,7 +887,7 @@ template <typename T>
 static kmeansCudaErrorStatus legacy_row_transform_min(int m0, int m, int n, const T *rn,
                                                       const T *cn, const T *input, int *output) {
   if (m0 < 0 || m < 0 || n <= 0 || !rn || !cn || !input || !output) return ERROR_INVALID_ARGUMENT;
-  KM_K_CHECK(launch_row_transform_min(nullptr, m0, m, n, rn, cn, input, output),
+  KM_K_CHECK(launch_row_transform_min(nullptr, m0, m, n, rn, cn, input, output, -1),
              ERROR_ROWTRANSFORMMINIMUM);
   return NO_ERROR;
 }

@@ -40,6 +40,8 @@ struct Engine {
   void *mfma = nullptr;
   // candidate proposer state (options.proposer != 0)
   void *learned = nullptr;
+  // library-GEMM assignment state (options.full_gemm)
+  void *gemm = nullptr;
 
   // loop state (kmeans.cpp:1194,1235-1237)
   double criterion = -1.0;

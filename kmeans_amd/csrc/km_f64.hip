@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "km_common.hpp"
+#include "km_gemm.hpp"
 
 namespace km {
 
@@ -34,9 +35,11 @@ struct EngineD {
   long long *dcounts = nullptr;
   char *dws = nullptr;
   UpdateWs ws{};
+  GemmAssign *gemm = nullptr;  // doFullSGEMM = 1: rocBLAS DGEMM + row argmin
 
   ~EngineD() {
     if (stream) hipStreamSynchronize(stream);
+    gemm_destroy(gemm);
     hipFree(dX);
     hipFree(dC);
     hipFree(dCt);
@@ -55,7 +58,7 @@ struct EngineD {
 }  // namespace
 
 kmeansErrorStatus compute_d(const double *data, int m, int n, int k, double criterion,
-                            int maxIters, int initAlgorithm, double *result,
+                            int maxIters, int initAlgorithm, int fullGemm, double *result,
                             double miniBatchFraction) {
   if (!data || !result || m <= 0 || n <= 0 || k <= 0 || k > m || k > 16384)
     return KMEANS_ERROR_INVALID_ARGUMENT;
@@ -130,9 +133,14 @@ kmeansErrorStatus compute_d(const double *data, int m, int n, int k, double crit
       r0 = (mb_index++ % per_epoch) * mb;
       rows = std::min(mb, (long)m - r0);
     }
-    KM_HIP_CHECK(launch_assign_f64(s, rows, n, E.dX + r0 * n, k, E.dCt, E.ldc, E.dnc,
-                                   E.dlabels + r0),
-                 KMEANS_ERROR_CLOSEST_CENTERS);
+    if (fullGemm)
+      KM_HIP_CHECK(gemm_assign(&E.gemm, s, E.dX + r0 * n, rows, n, k, E.dCt, E.ldc, E.dnc,
+                               E.dlabels + r0),
+                   KMEANS_ERROR_CLOSEST_CENTERS);
+    else
+      KM_HIP_CHECK(launch_assign_f64(s, rows, n, E.dX + r0 * n, k, E.dCt, E.ldc, E.dnc,
+                                     E.dlabels + r0),
+                   KMEANS_ERROR_CLOSEST_CENTERS);
     KM_HIP_CHECK(launch_update(s, rows, n, k, E.dX + r0 * n, E.dlabels + r0, E.dq + r0, E.dC, E.dC,
                                E.dCt, E.ldc, E.dnc, E.dcounts, E.dempty, E.ws),
                  KMEANS_ERROR_CLUSTER_CENTERS);

@@ -975,7 +975,7 @@ static hipError_t launch_col_norms_dk_t(hipStream_t s, int k, int d, int ld, con
 // minima of v_ij = (T)((double)(rn[i] + cn[j]) - 2.0 * (double)P[i][j]); the
 // first sum is taken in T and the rest in double, then rounded back to T.
 // Only values below FLT_MAX count (the reference's start value for both
-// types, :17); a row with none gets -1.
+// types, :17); a row with none gets `none` (-1 in the reference).
 // rn is read at the row's local index i and out at m0 + i, as in the
 // reference kernel (:20, :57).  One wave per row, lanes stride over j and
 // the wave merges with the tie rule.
@@ -984,7 +984,7 @@ __global__ __launch_bounds__(256) void k_row_transform_min(long m0, long m, int 
                                                            const T *__restrict__ rn,
                                                            const T *__restrict__ cn,
                                                            const T *__restrict__ P,
-                                                           int *__restrict__ out) {
+                                                           int *__restrict__ out, int none) {
   const T big = (T)FLT_MAX;
   const int lane = threadIdx.x & 63;
   const long nw = (long)gridDim.x * (blockDim.x >> 6);
@@ -1002,16 +1002,16 @@ __global__ __launch_bounds__(256) void k_row_transform_min(long m0, long m, int 
       const int oi = __shfl_xor(bi, off);
       if (ov < bv || (ov == bv && (unsigned)oi < (unsigned)bi)) { bv = ov; bi = oi; }
     }
-    if (lane == 0) out[m0 + i] = bi;
+    if (lane == 0) out[m0 + i] = bi < 0 ? none : bi;
   }
 }
 
 template <typename T>
 static hipError_t launch_row_transform_min_t(hipStream_t s, long m0, long m, int n, const T *rn,
-                                    const T *cn, const T *P, int *out) {
+                                    const T *cn, const T *P, int *out, int none) {
   if (m <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_row_transform_min<T>, dim3(km_grid(m * 64, 256)), dim3(256), 0, s, m0, m, n,
-                     rn, cn, P, out);
+                     rn, cn, P, out, none);
   return hipGetLastError();
 }
 
@@ -1070,8 +1070,8 @@ static hipError_t launch_gather_rows_t(hipStream_t s, long n, int d, const T *X,
     return launch_col_norms_dk_t<T>(s, k, d, ld, C, nc);                                         \
   }                                                                                              \
   hipError_t launch_row_transform_min(hipStream_t s, long m0, long m, int n, const T *rn,        \
-                                      const T *cn, const T *P, int *out) {                       \
-    return launch_row_transform_min_t<T>(s, m0, m, n, rn, cn, P, out);                           \
+                                      const T *cn, const T *P, int *out, int none) {             \
+    return launch_row_transform_min_t<T>(s, m0, m, n, rn, cn, P, out, none);                     \
   }                                                                                              \
   hipError_t launch_gather_rows(hipStream_t s, long n, int d, const T *X, const int *idx,        \
                                 T *out) {                                                        \
