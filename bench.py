@@ -33,6 +33,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=0, help="points in the CPU baseline sample (0: all)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32-ref", action="store_true")
+    ap.add_argument("--full-gemm", action="store_true",
+                    help="fp32 only: assign through rocBLAS SGEMM + row argmin (the reference's "
+                         "doFullSGEMM = 1 algorithm), a comparison point for the fused kernels")
     ap.add_argument("--calib", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy (PMC byte-counter calibration)")
     ap.add_argument("--proposer", default=None, choices=["none", "mlp", "centroid"],
@@ -62,15 +65,19 @@ def main():
     X, C0, spec = datagen.generate_on_gpu(args.workload, n=args.n, torch=torch)
     n, d, k = spec["n"], spec["d"], spec["k"]
 
-    def run(precision, steps, warmup):
-        e = km.KMeans(n, d, k, precision=precision, timing=True)
+    if args.full_gemm and args.precision != "fp32":
+        ap.error("--full-gemm needs --precision fp32")
+
+    def run(precision, steps, warmup, full_gemm=False):
+        e = km.KMeans(n, d, k, precision=precision, timing=True, full_gemm=full_gemm)
         e.set_data(device_ptr=X.data_ptr())
         e.set_centers(C0)
         if warmup:
             e.run(warmup)
             e.synchronize()
             e.set_centers(C0)
-        e2 = km.KMeans(n, d, k, precision=precision, timing=True)  # fresh stats for the timed run
+        # fresh stats for the timed run
+        e2 = km.KMeans(n, d, k, precision=precision, timing=True, full_gemm=full_gemm)
         e.close()
         e2.set_data(device_ptr=X.data_ptr())
         e2.set_centers(C0)
@@ -87,7 +94,7 @@ def main():
         print(json.dumps(bench_proposer(args, X, C0, spec, km, np, torch)))
         return
 
-    eng, dt = run(args.precision, args.steps, args.warmup)
+    eng, dt = run(args.precision, args.steps, args.warmup, args.full_gemm)
     st = eng.stats()
     pairs = float(n) * k
     value = pairs * args.steps / dt
@@ -107,18 +114,21 @@ def main():
         "vs_baseline": None,
         "dtype": args.precision if args.precision != "fp32" else "f32",
         "data": "synthetic (Gaussian mixture generated on the GPU)",
-        "config": {"workload": f"{args.workload}: N={n} D={d} K={k} Lloyd, {args.precision} assign",
+        "config": {"workload": f"{args.workload}: N={n} D={d} K={k} Lloyd, {args.precision} assign"
+                               + (" by library GEMM + row argmin" if args.full_gemm else ""),
                    "N": n, "D": d, "K": k, "precision": args.precision},
         "phases_ms_per_step": {p: st[f"ms_{p}"] / args.steps for p in
                                ("assign", "reresolve", "update", "compactness")},
         "assign_pairs_per_s": pairs / (assign_ms * 1e-3),
         "reresolved_per_step": st["reresolved"] / args.steps,
         "rescored_per_step": st["rescored"] / args.steps,
-        "roofline": {"bound": "mfma" if args.precision != "fp32" else "fp32",
+        "roofline": {"bound": "mfma",
                      "achieved": achieved / 1e12, "peak": PEAK[args.precision] / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK[args.precision],
-                     "traffic": measured_traffic(args.workload, args.precision, args.n),
-                     "kernel": "k_assign_mfma" if args.precision != "fp32" else "k_assign_f32",
+                     "traffic": (None if args.full_gemm else
+                                 measured_traffic(args.workload, args.precision, args.n)),
+                     "kernel": ("rocBLAS SGEMM + k_row_transform_min" if args.full_gemm else
+                                "k_assign_mfma" if args.precision != "fp32" else "k_assign_f32_mfma"),
                      "algorithmic": "2*D*N*K flops per launch; achieved over the assign phase's "
                                     "event-timed duration"},
     }
@@ -202,7 +212,7 @@ def measured_traffic(workload, precision, n_override):
         tab = json.load(open(path))
     except (OSError, ValueError):
         return None
-    key = f"{workload}:{precision}:" + ("k_assign_mfma" if precision != "fp32" else "k_assign_f32")
+    key = f"{workload}:{precision}:" + ("k_assign_mfma" if precision != "fp32" else "k_assign_f32_mfma")
     ent = tab.get(key)
     return None if ent is None else ent["bytes_per_launch"]
 

@@ -2,7 +2,7 @@
 // compactness, norms, synthetic-data generator.
 //
 // Reference counterparts (paths relative to the reference repo):
-//   k_assign_f32        main/cuda/ClosestCenters.cu:46-123 (+ multiply.hcu,
+//   k_assign_f32_mfma   main/cuda/ClosestCenters.cu:46-123 (+ multiply.hcu,
 //                       dotproducts.hcu, fma.hcu, reduction.hcu, vectorMin.hcu)
 //   update kernels      main/cuda/ClusterCenters.cu:3-132
 //   compactness         main/cuda/Compactness.cu:3-96
@@ -50,14 +50,28 @@ __global__ void k_col_norms(int k, int d, const T *__restrict__ C, T *__restrict
   nc[j] = s;
 }
 
+// Row norms: the same chain per row (Normalize.cu:3-31).  256 rows per
+// block; slices of 16 dimensions are read row-contiguously into LDS (16
+// lanes per row) and each thread then runs its row's chain from LDS.
 template <typename T>
-__global__ void k_row_norms(long m, int d, const T *__restrict__ X, T *__restrict__ out) {
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  const T *x = X + i * d;
+__global__ __launch_bounds__(256) void k_row_norms(long m, int d, const T *__restrict__ X,
+                                                   T *__restrict__ out) {
+  __shared__ T tile[256][17];
+  const long r0 = (long)blockIdx.x * 256;
+  const int rows = (int)min(256L, m - r0);
   T s = 0;
-  for (int q = 0; q < d; ++q) s = fma_t(x[q], x[q], s);
-  out[i] = s;
+  for (int d0 = 0; d0 < d; d0 += 16) {
+    const int dn = min(16, d - d0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < rows * 16; e += 256) {
+      const int r = e >> 4, q = e & 15;
+      if (q < dn) tile[r][q] = X[(r0 + r) * d + d0 + q];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < rows)
+      for (int q = 0; q < dn; ++q) s = fma_t(tile[threadIdx.x][q], tile[threadIdx.x][q], s);
+  }
+  if ((int)threadIdx.x < rows) out[r0 + threadIdx.x] = s;
 }
 
 // Ct[q][j] = C[j][q], zero for j >= k (Ct pre-cleared by the caller once).
@@ -77,16 +91,15 @@ __global__ void k_transpose(int k, int d, const T *__restrict__ C, T *__restrict
   }
 }
 
-// ---------------------------------------------------------------- K1 assign
+// ---------------------------------------------------------------- K1 assign, VALU (double)
 // One workgroup (4 waves) owns a tile of 8*PP points and sweeps all centres
 // in tiles of 256, d-slices of 16 staged through LDS with register prefetch.
 // Thread (pg, cg) accumulates PP points x 8 centres; every score is a single
-// fma chain over d in ascending order starting from -0.5*nc[j].  FP32 uses
-// PP = 8 (64 points per tile), FP64 PP = 4 so that its accumulators fit the
-// register file at two waves per SIMD.
+// fma chain over d in ascending order starting from -0.5*nc[j].  Double uses
+// PP = 4 so that its accumulators fit the register file at two waves per
+// SIMD (PP = 8 measured 10% slower).
 #define A_BN 256
 #define A_DS 16
-#define A_BM_F32 64
 #define A_BM_F64 32
 
 template <typename T, int PP>
@@ -219,15 +232,163 @@ __device__ __forceinline__ void assign_tiles(long M, const int *__restrict__ Mde
   }
 }
 
-__global__ __launch_bounds__(256) void k_assign_f32(long M, const int *__restrict__ Mdev, int D,
-                                                    const float *__restrict__ X,
-                                                    const int *__restrict__ rows, int K,
-                                                    const float *__restrict__ Ct, int ldc,
-                                                    const float *__restrict__ nc,
-                                                    const int *__restrict__ out_pos,
-                                                    int *__restrict__ labels,
-                                                    float *__restrict__ best) {
-  assign_tiles<float, A_BM_F32 / 8>(M, Mdev, D, X, rows, K, Ct, ldc, nc, out_pos, labels, best);
+// ---------------------------------------------------------------- K1 assign, matrix cores
+// v_mfma_f32_16x16x4f32 computes D[i][j] = C[i][j] + sum_k A[i][k] B[k][j] as
+// four fused multiply-adds in k order, each rounded once: measured on gfx950
+// against the sequential fmaf chain, bit for bit, including subnormal,
+// infinite, NaN and signed-zero operands.  Chaining it over the dimensions
+// in steps of four, from C = -0.5f*nc[j], is therefore the canonical score
+// chain itself, at the matrix cores' FP32 rate (twice the unpacked VALU
+// rate).  Dimensions past D are padded with x = +0, c = -0: the product -0
+// leaves every running value unchanged (also -0).
+//
+// Layout of one 16x16x4 step (lane l, g = l / 16, r = l % 16): A = point r,
+// dimension g; B = dimension g, centre r; result register v = point 4g + v,
+// centre r.  A wave owns 32 points (two row tiles) with their dimensions in
+// registers (AM_DS per slice); the workgroup's four waves sweep the centres
+// in blocks of 64 (four column tiles), staged through LDS so that a lane's
+// four B operands of a 16-dimension group are one 16-byte read.
+#define AM_RT 2
+#define AM_CT 4
+#define AM_DS 128
+#define AM_BM (4 * 16 * AM_RT)  // points per workgroup
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_assign_f32_mfma(long M, const int *__restrict__ Mdev,
+                                                         int D, const float *__restrict__ X,
+                                                         const int *__restrict__ rows, int K,
+                                                         const float *__restrict__ Ct, int ldc,
+                                                         const float *__restrict__ nc,
+                                                         const int *__restrict__ out_pos,
+                                                         int *__restrict__ labels,
+                                                         float *__restrict__ best) {
+  // Bl[b][s][g][c] = the four dimensions 16 s + 4 t + g (t = 0..3) of column c,
+  // two buffers
+  __shared__ mfma_f4 Bl[2][AM_DS / 16][4][64];
+  if (Mdev) M = *Mdev;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int g = lane >> 4, r = lane & 15;
+  const long ntiles = (M + AM_BM - 1) / AM_BM;
+  const int nblk = (K + 63) / 64;
+  const int nsl = (D + AM_DS - 1) / AM_DS;
+  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const long p0 = tile * AM_BM + w * (16 * AM_RT);
+    long xrow[AM_RT];
+    bool xok[AM_RT];
+#pragma unroll
+    for (int rt = 0; rt < AM_RT; ++rt) {
+      const long p = p0 + rt * 16 + r;
+      xok[rt] = p < M;
+      xrow[rt] = xok[rt] ? (rows ? (long)rows[p] : p) : 0;
+    }
+    float bv[AM_RT][4];
+    int bi[AM_RT][4];
+#pragma unroll
+    for (int rt = 0; rt < AM_RT; ++rt)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) { bv[rt][v] = -INFINITY; bi[rt][v] = 0; }
+    float xa[AM_RT][AM_DS / 4];
+    // chunk i of a staged block for thread (w, c): dimensions 16 i + 4 t + w
+    // (t = 0..3) of column c, i.e. Bl[.][i][w][c]
+    auto fetch = [&](int i, int jbn, int sln) {
+      const int d0n = sln * AM_DS, dnn = min(AM_DS, D - d0n), c = threadIdx.x & 63;
+      const float *src = Ct + (long)(d0n + 16 * i + w) * ldc + jbn * 64 + c;
+      mfma_f4 v;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) v[t] = 16 * i + 4 * t + w < dnn ? src[(long)4 * t * ldc] : -0.0f;
+      return v;
+    };
+    int buf = 0;
+#pragma unroll
+    for (int i = 0; i < AM_DS / 16; ++i) Bl[0][i][w][threadIdx.x & 63] = fetch(i, 0, 0);
+    __syncthreads();
+    for (int jb = 0; jb < nblk; ++jb) {
+      mfma_f4 acc[AM_RT][AM_CT];
+#pragma unroll
+      for (int ct = 0; ct < AM_CT; ++ct) {
+        const int j = jb * 64 + ct * 16 + r;
+        const float v = j < K ? -0.5f * nc[j] : -INFINITY;
+#pragma unroll
+        for (int rt = 0; rt < AM_RT; ++rt) acc[rt][ct] = (mfma_f4){v, v, v, v};
+      }
+      for (int sl = 0; sl < nsl; ++sl) {
+        const int d0 = sl * AM_DS, dn = min(AM_DS, D - d0);
+        if (nsl > 1 || jb == 0) {
+#pragma unroll
+          for (int rt = 0; rt < AM_RT; ++rt)
+#pragma unroll
+            for (int q = 0; q < AM_DS / 4; ++q) {
+              const int d = d0 + 4 * q + g;
+              xa[rt][q] = (xok[rt] && 4 * q + g < dn) ? X[xrow[rt] * D + d] : 0.0f;
+            }
+        }
+        // the next block is fetched one chunk per 16 dimensions into the other
+        // buffer, so its loads are in flight during this block's MFMAs
+        const bool nxt = sl + 1 < nsl || jb + 1 < nblk;
+        const int jbn = sl + 1 < nsl ? jb : jb + 1, sln = sl + 1 < nsl ? sl + 1 : 0;
+        const int ns = (dn + 15) / 16;
+        mfma_f4 fl;
+#pragma unroll
+        for (int s16 = 0; s16 < AM_DS / 16; ++s16) {  // unrolled: xa stays in registers
+          if (nxt) {
+            if (s16 > 0) Bl[buf ^ 1][s16 - 1][w][threadIdx.x & 63] = fl;
+            fl = fetch(s16, jbn, sln);
+          }
+          if (s16 < ns) {
+            mfma_f4 bf[AM_CT];
+#pragma unroll
+            for (int ct = 0; ct < AM_CT; ++ct) bf[ct] = Bl[buf][s16][g][ct * 16 + r];
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+              for (int rt = 0; rt < AM_RT; ++rt)
+#pragma unroll
+                for (int ct = 0; ct < AM_CT; ++ct)
+                  acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                      xa[rt][4 * s16 + t], bf[ct][t], acc[rt][ct], 0, 0, 0);
+          }
+        }
+        if (nxt) Bl[buf ^ 1][AM_DS / 16 - 1][w][threadIdx.x & 63] = fl;
+        __syncthreads();  // next buffer written, this one no longer read
+        buf ^= 1;
+      }
+      // running best per (point, lane); centres visited in ascending order
+#pragma unroll
+      for (int ct = 0; ct < AM_CT; ++ct) {
+        const int j = jb * 64 + ct * 16 + r;
+#pragma unroll
+        for (int rt = 0; rt < AM_RT; ++rt)
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            if (acc[rt][ct][v] > bv[rt][v]) { bv[rt][v] = acc[rt][ct][v]; bi[rt][v] = j; }
+      }
+    }
+    // the 16 lanes of a row group hold the same points; merge with the tie rule
+#pragma unroll
+    for (int rt = 0; rt < AM_RT; ++rt)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int off = 8; off >= 1; off >>= 1) {
+          const float ov = __shfl_xor(bv[rt][v], off);
+          const int oi = __shfl_xor(bi[rt][v], off);
+          if (ov > bv[rt][v] || (ov == bv[rt][v] && oi < bi[rt][v])) { bv[rt][v] = ov; bi[rt][v] = oi; }
+        }
+    if (r == 0) {
+#pragma unroll
+      for (int rt = 0; rt < AM_RT; ++rt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const long pt = p0 + rt * 16 + g * 4 + v;
+          if (pt < M) {
+            const long o = out_pos ? (long)out_pos[pt] : rows ? (long)rows[pt] : pt;
+            labels[o] = bi[rt][v];
+            if (best) best[o] = bv[rt][v];
+          }
+        }
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void k_assign_f64(long M, const int *__restrict__ Mdev, int D,
@@ -775,11 +936,11 @@ static hipError_t launch_transpose_t(hipStream_t s, int k, int d, const T *C, T 
 hipError_t launch_assign_f32(hipStream_t s, long m, const int *m_dev, int d, const float *X,
                              const int *rows, int k, const float *Ct, int ldc, const float *nc,
                              int *labels, float *best, const int *out_pos) {
-  long tiles = km_cdiv(m, A_BM_F32);
+  long tiles = km_cdiv(m, AM_BM);
   if (tiles <= 0) return hipSuccess;
-  long grid = tiles < (long)num_cus() * 4 ? tiles : (long)num_cus() * 4;
-  hipLaunchKernelGGL(k_assign_f32, dim3(grid), dim3(256), 0, s, m, m_dev, d, X, rows, k, Ct, ldc,
-                     nc, out_pos, labels, best);
+  long grid = tiles < (long)num_cus() * 8 ? tiles : (long)num_cus() * 8;
+  hipLaunchKernelGGL(k_assign_f32_mfma, dim3(grid), dim3(256), 0, s, m, m_dev, d, X, rows, k, Ct,
+                     ldc, nc, out_pos, labels, best);
   return hipGetLastError();
 }
 
@@ -977,9 +1138,19 @@ static hipError_t launch_col_norms_dk_t(hipStream_t s, int k, int d, int ld, con
 // Only values below FLT_MAX count (the reference's start value for both
 // types, :17); a row with none gets `none` (-1 in the reference).
 // rn is read at the row's local index i and out at m0 + i, as in the
-// reference kernel (:20, :57).  One wave per row, lanes stride over j and
-// the wave merges with the tie rule.
-template <typename T>
+// reference kernel (:20, :57).  One wave per row, lanes stride over j (16
+// bytes at a time when the rows are aligned) and the wave merges with the
+// tie rule.
+__device__ inline void load16(const float *p, float *v) {
+  const float4 a = *reinterpret_cast<const float4 *>(p);
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+__device__ inline void load16(const double *p, double *v) {
+  const double2 a = *reinterpret_cast<const double2 *>(p);
+  v[0] = a.x; v[1] = a.y;
+}
+
+template <typename T, bool VEC>
 __global__ __launch_bounds__(256) void k_row_transform_min(long m0, long m, int n,
                                                            const T *__restrict__ rn,
                                                            const T *__restrict__ cn,
@@ -992,9 +1163,22 @@ __global__ __launch_bounds__(256) void k_row_transform_min(long m0, long m, int 
     const T r = rn[i];
     T bv = big;
     int bi = -1;
-    for (int j = lane; j < n; j += 64) {
-      const T v = (T)((double)(r + cn[j]) - 2.0 * (double)P[i * n + j]);
-      if (v < bv) { bv = v; bi = j; }
+    if (VEC) {  // 16-byte loads: a lane owns runs of 16/sizeof(T) consecutive j
+      constexpr int W = 16 / sizeof(T);
+      for (int j0 = lane * W; j0 < n; j0 += 64 * W) {
+        T pv[W];
+        load16(P + i * n + j0, pv);
+#pragma unroll
+        for (int u = 0; u < W; ++u) {
+          const T v = (T)((double)(r + cn[j0 + u]) - 2.0 * (double)pv[u]);
+          if (v < bv) { bv = v; bi = j0 + u; }
+        }
+      }
+    } else {
+      for (int j = lane; j < n; j += 64) {
+        const T v = (T)((double)(r + cn[j]) - 2.0 * (double)P[i * n + j]);
+        if (v < bv) { bv = v; bi = j; }
+      }
     }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
@@ -1010,8 +1194,13 @@ template <typename T>
 static hipError_t launch_row_transform_min_t(hipStream_t s, long m0, long m, int n, const T *rn,
                                     const T *cn, const T *P, int *out, int none) {
   if (m <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_row_transform_min<T>, dim3(km_grid(m * 64, 256)), dim3(256), 0, s, m0, m, n,
-                     rn, cn, P, out, none);
+  // vector loads when every row starts 16-byte aligned
+  if (n % (16 / sizeof(T)) == 0 && ((uintptr_t)P & 15) == 0)
+    hipLaunchKernelGGL((k_row_transform_min<T, true>), dim3(km_grid(m * 64, 256)), dim3(256), 0, s,
+                       m0, m, n, rn, cn, P, out, none);
+  else
+    hipLaunchKernelGGL((k_row_transform_min<T, false>), dim3(km_grid(m * 64, 256)), dim3(256), 0, s,
+                       m0, m, n, rn, cn, P, out, none);
   return hipGetLastError();
 }
 
