@@ -34,9 +34,6 @@ int num_cus() {
 // with explicit fused multiply-adds in the element type.
 __device__ inline float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ inline double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
-template <typename T> struct Vec4;
-template <> struct Vec4<float> { typedef float4 type; };
-template <> struct Vec4<double> { typedef double4 type; };
 
 // ---------------------------------------------------------------- norms
 // nc[j] = fma chain over d of C[j][d]^2 (Normalize.cu:41-44, contraction explicit).
@@ -91,236 +88,114 @@ __global__ void k_transpose(int k, int d, const T *__restrict__ C, T *__restrict
   }
 }
 
-// ---------------------------------------------------------------- K1 assign, VALU (double)
-// One workgroup (4 waves) owns a tile of 8*PP points and sweeps all centres
-// in tiles of 256, d-slices of 16 staged through LDS with register prefetch.
-// Thread (pg, cg) accumulates PP points x 8 centres; every score is a single
-// fma chain over d in ascending order starting from -0.5*nc[j].  Double uses
-// PP = 4 so that its accumulators fit the register file at two waves per
-// SIMD (PP = 8 measured 10% slower).
-#define A_BN 256
-#define A_DS 16
-#define A_BM_F64 32
-
-template <typename T, int PP>
-__device__ __forceinline__ void assign_tiles(long M, const int *__restrict__ Mdev, int D,
-                                             const T *__restrict__ X, const int *__restrict__ rows,
-                                             int K, const T *__restrict__ Ct, int ldc,
-                                             const T *__restrict__ nc,
-                                             const int *__restrict__ out_pos,
-                                             int *__restrict__ labels, T *__restrict__ best) {
-  typedef typename Vec4<T>::type T4;
-  constexpr int A_BM = 8 * PP, XQ = A_BM / 16;  // tile rows, rows staged per thread
-  __shared__ __attribute__((aligned(32))) T Xs[A_DS][A_BM + 4];
-  __shared__ T4 Cs[A_DS][A_BN / 4];
-  if (Mdev) M = *Mdev;
-  const long ntiles = (M + A_BM - 1) / A_BM;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int cg = lane & 31, pg = (lane >> 5) + 2 * w;
-  const int nS = (D + A_DS - 1) / A_DS;
-  const int nKT = (K + A_BN - 1) / A_BN;
-  const int total = nS * nKT;
-  const int xdd = tid & 15;
-
-  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const long r0 = tile * A_BM;
-    long xrow[XQ];
-    bool xok[XQ];
-#pragma unroll
-    for (int q = 0; q < XQ; ++q) {
-      long p = r0 + (tid >> 4) + 16 * q;
-      xok[q] = p < M;
-      xrow[q] = xok[q] ? (rows ? (long)rows[p] : p) : 0;
-    }
-    T bv[PP];
-    int bi[PP];
-#pragma unroll
-    for (int p = 0; p < PP; ++p) { bv[p] = (T)-INFINITY; bi[p] = 0; }
-
-    T xr[XQ];
-    T4 cr[4];
-    auto load = [&](int it) {
-      const int kt = it / nS, s = it - kt * nS;
-      const int d0 = s * A_DS;
-#pragma unroll
-      for (int q = 0; q < XQ; ++q)
-        xr[q] = (xok[q] && d0 + xdd < D) ? X[xrow[q] * D + d0 + xdd] : (T)0;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int e = tid + 256 * q, dd = e >> 6, c4 = e & 63;
-        if (d0 + dd < D) {
-          cr[q] = *reinterpret_cast<const T4 *>(Ct + (long)(d0 + dd) * ldc + kt * A_BN + c4 * 4);
-        } else {
-          cr[q].x = 0; cr[q].y = 0; cr[q].z = 0; cr[q].w = 0;
-        }
-      }
-    };
-
-    T acc[PP][8];
-    load(0);
-    for (int it = 0; it < total; ++it) {
-      const int kt = it / nS, s = it - kt * nS;
-      if (s == 0) {
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const int j = kt * A_BN + (q < 4 ? cg * 4 + q : 128 + cg * 4 + q - 4);
-          const T v = j < K ? (T)-0.5 * nc[j] : (T)-INFINITY;
-#pragma unroll
-          for (int p = 0; p < PP; ++p) acc[p][q] = v;
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int q = 0; q < XQ; ++q) Xs[xdd][(tid >> 4) + 16 * q] = xr[q];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int e = tid + 256 * q;
-        Cs[e >> 6][e & 63] = cr[q];
-      }
-      __syncthreads();
-      if (it + 1 < total) load(it + 1);
-      const int dn = min(A_DS, D - s * A_DS);
-      auto step = [&](int dd) {
-        T xv[PP];
-#pragma unroll
-        for (int h = 0; h < PP / 4; ++h) {
-          const T4 xa = *reinterpret_cast<const T4 *>(&Xs[dd][pg * PP + 4 * h]);
-          xv[4 * h] = xa.x; xv[4 * h + 1] = xa.y; xv[4 * h + 2] = xa.z; xv[4 * h + 3] = xa.w;
-        }
-        const T4 ca = Cs[dd][cg], cb = Cs[dd][32 + cg];
-        const T cv[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
-#pragma unroll
-        for (int p = 0; p < PP; ++p)
-#pragma unroll
-          for (int q = 0; q < 8; ++q) acc[p][q] = fma_t(xv[p], cv[q], acc[p][q]);
-      };
-      if (dn == A_DS) {
-#pragma unroll
-        for (int dd = 0; dd < A_DS; ++dd) step(dd);
-      } else {
-        for (int dd = 0; dd < dn; ++dd) step(dd);
-      }
-      if (s == nS - 1) {
-#pragma unroll
-        for (int p = 0; p < PP; ++p)
-#pragma unroll
-          for (int q = 0; q < 8; ++q) {
-            const int j = kt * A_BN + (q < 4 ? cg * 4 + q : 128 + cg * 4 + q - 4);
-            if (acc[p][q] > bv[p]) { bv[p] = acc[p][q]; bi[p] = j; }
-          }
-      }
-    }
-    // lanes of one half-wave share the PP points; merge with the tie rule
-#pragma unroll
-    for (int p = 0; p < PP; ++p) {
-#pragma unroll
-      for (int off = 16; off >= 1; off >>= 1) {
-        const T ov = __shfl_xor(bv[p], off);
-        const int oi = __shfl_xor(bi[p], off);
-        if (ov > bv[p] || (ov == bv[p] && oi < bi[p])) { bv[p] = ov; bi[p] = oi; }
-      }
-    }
-#pragma unroll
-    for (int p = 0; p < PP; ++p) {
-      const long pt = r0 + pg * PP + p;
-      if (cg == p && pt < M) {
-        const long o = out_pos ? (long)out_pos[pt] : rows ? (long)rows[pt] : pt;
-        labels[o] = bi[p];
-        if (best) best[o] = bv[p];
-      }
-    }
-  }
-}
-
 // ---------------------------------------------------------------- K1 assign, matrix cores
-// v_mfma_f32_16x16x4f32 computes D[i][j] = C[i][j] + sum_k A[i][k] B[k][j] as
-// four fused multiply-adds in k order, each rounded once: measured on gfx950
-// against the sequential fmaf chain, bit for bit, including subnormal,
-// infinite, NaN and signed-zero operands.  Chaining it over the dimensions
-// in steps of four, from C = -0.5f*nc[j], is therefore the canonical score
-// chain itself, at the matrix cores' FP32 rate (twice the unpacked VALU
-// rate).  Dimensions past D are padded with x = +0, c = -0: the product -0
-// leaves every running value unchanged (also -0).
+// v_mfma_f32_16x16x4f32 and v_mfma_f64_16x16x4f64 compute D[i][j] = C[i][j] +
+// sum_k A[i][k] B[k][j] as four fused multiply-adds in k order, each rounded
+// once: measured on gfx950 against the sequential fma chain, bit for bit,
+// including subnormal, infinite, NaN and signed-zero operands.  Chaining the
+// instruction over the dimensions in steps of four, from C = -0.5*nc[j], is
+// therefore the canonical score chain itself, at the matrix cores' rate (for
+// FP32 twice the unpacked VALU rate; for FP64 the same rate, without the
+// VALU kernel's LDS traffic).  Dimensions past D are padded with x = +0,
+// c = -0: the product -0 leaves every running value unchanged (also -0).
 //
 // Layout of one 16x16x4 step (lane l, g = l / 16, r = l % 16): A = point r,
-// dimension g; B = dimension g, centre r; result register v = point 4g + v,
-// centre r.  A wave owns 32 points (two row tiles) with their dimensions in
-// registers (AM_DS per slice); the workgroup's four waves sweep the centres
-// in blocks of 64 (four column tiles), staged through LDS so that a lane's
-// four B operands of a 16-dimension group are one 16-byte read.
-#define AM_RT 2
+// dimension g; B = dimension g, centre r; result register v = point 4g + v
+// (FP32) or 4v + g (FP64), centre r.  A wave owns 16*RT points with DS
+// dimensions of them in registers; the workgroup's four waves sweep the
+// centres in blocks of 64 (four column tiles), staged through a double
+// buffer in LDS so that a lane's four B operands of a 16-dimension group are
+// one read, and the next block is fetched one chunk per 16 dimensions while
+// the current block's MFMAs run.
+template <typename T> struct MfmaCfg;
+template <> struct MfmaCfg<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static constexpr int RT = 2, DS = 128;
+  static __device__ __forceinline__ v4 mfma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int g, int v) { return 4 * g + v; }
+};
+template <> struct MfmaCfg<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static constexpr int RT = 2, DS = 64;
+  static __device__ __forceinline__ v4 mfma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int g, int v) { return 4 * v + g; }
+};
 #define AM_CT 4
-#define AM_DS 128
-#define AM_BM (4 * 16 * AM_RT)  // points per workgroup
-typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+template <typename T> constexpr int am_bm() { return 4 * 16 * MfmaCfg<T>::RT; }  // points per workgroup
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-void k_assign_f32_mfma(long M, const int *__restrict__ Mdev,
-                                                         int D, const float *__restrict__ X,
-                                                         const int *__restrict__ rows, int K,
-                                                         const float *__restrict__ Ct, int ldc,
-                                                         const float *__restrict__ nc,
-                                                         const int *__restrict__ out_pos,
-                                                         int *__restrict__ labels,
-                                                         float *__restrict__ best) {
+template <typename T>
+__device__ __forceinline__ void assign_mfma_tiles(long M, const int *__restrict__ Mdev, int D,
+                                                  const T *__restrict__ X,
+                                                  const int *__restrict__ rows, int K,
+                                                  const T *__restrict__ Ct, int ldc,
+                                                  const T *__restrict__ nc,
+                                                  const int *__restrict__ out_pos,
+                                                  int *__restrict__ labels, T *__restrict__ best) {
+  typedef MfmaCfg<T> Cfg;
+  typedef typename Cfg::v4 v4;
+  constexpr int RT = Cfg::RT, DS = Cfg::DS, BM = am_bm<T>();
   // Bl[b][s][g][c] = the four dimensions 16 s + 4 t + g (t = 0..3) of column c,
   // two buffers
-  __shared__ mfma_f4 Bl[2][AM_DS / 16][4][64];
+  __shared__ v4 Bl[2][DS / 16][4][64];
   if (Mdev) M = *Mdev;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int g = lane >> 4, r = lane & 15;
-  const long ntiles = (M + AM_BM - 1) / AM_BM;
+  const long ntiles = (M + BM - 1) / BM;
   const int nblk = (K + 63) / 64;
-  const int nsl = (D + AM_DS - 1) / AM_DS;
+  const int nsl = (D + DS - 1) / DS;
   for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const long p0 = tile * AM_BM + w * (16 * AM_RT);
-    long xrow[AM_RT];
-    bool xok[AM_RT];
+    const long p0 = tile * BM + w * (16 * RT);
+    long xrow[RT];
+    bool xok[RT];
 #pragma unroll
-    for (int rt = 0; rt < AM_RT; ++rt) {
+    for (int rt = 0; rt < RT; ++rt) {
       const long p = p0 + rt * 16 + r;
       xok[rt] = p < M;
       xrow[rt] = xok[rt] ? (rows ? (long)rows[p] : p) : 0;
     }
-    float bv[AM_RT][4];
-    int bi[AM_RT][4];
+    T bv[RT][4];
+    int bi[RT][4];
 #pragma unroll
-    for (int rt = 0; rt < AM_RT; ++rt)
+    for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) { bv[rt][v] = -INFINITY; bi[rt][v] = 0; }
-    float xa[AM_RT][AM_DS / 4];
+      for (int v = 0; v < 4; ++v) { bv[rt][v] = (T)-INFINITY; bi[rt][v] = 0; }
+    T xa[RT][DS / 4];
     // chunk i of a staged block for thread (w, c): dimensions 16 i + 4 t + w
     // (t = 0..3) of column c, i.e. Bl[.][i][w][c]
     auto fetch = [&](int i, int jbn, int sln) {
-      const int d0n = sln * AM_DS, dnn = min(AM_DS, D - d0n), c = threadIdx.x & 63;
-      const float *src = Ct + (long)(d0n + 16 * i + w) * ldc + jbn * 64 + c;
-      mfma_f4 v;
+      const int d0n = sln * DS, dnn = min(DS, D - d0n), c = threadIdx.x & 63;
+      const T *src = Ct + (long)(d0n + 16 * i + w) * ldc + jbn * 64 + c;
+      v4 v;
 #pragma unroll
-      for (int t = 0; t < 4; ++t) v[t] = 16 * i + 4 * t + w < dnn ? src[(long)4 * t * ldc] : -0.0f;
+      for (int t = 0; t < 4; ++t) v[t] = 16 * i + 4 * t + w < dnn ? src[(long)4 * t * ldc] : (T)-0.0;
       return v;
     };
     int buf = 0;
 #pragma unroll
-    for (int i = 0; i < AM_DS / 16; ++i) Bl[0][i][w][threadIdx.x & 63] = fetch(i, 0, 0);
+    for (int i = 0; i < DS / 16; ++i) Bl[0][i][w][threadIdx.x & 63] = fetch(i, 0, 0);
     __syncthreads();
     for (int jb = 0; jb < nblk; ++jb) {
-      mfma_f4 acc[AM_RT][AM_CT];
+      v4 acc[RT][AM_CT];
 #pragma unroll
       for (int ct = 0; ct < AM_CT; ++ct) {
         const int j = jb * 64 + ct * 16 + r;
-        const float v = j < K ? -0.5f * nc[j] : -INFINITY;
+        const T v = j < K ? (T)-0.5 * nc[j] : (T)-INFINITY;
 #pragma unroll
-        for (int rt = 0; rt < AM_RT; ++rt) acc[rt][ct] = (mfma_f4){v, v, v, v};
+        for (int rt = 0; rt < RT; ++rt) acc[rt][ct] = (v4){v, v, v, v};
       }
       for (int sl = 0; sl < nsl; ++sl) {
-        const int d0 = sl * AM_DS, dn = min(AM_DS, D - d0);
+        const int d0 = sl * DS, dn = min(DS, D - d0);
         if (nsl > 1 || jb == 0) {
 #pragma unroll
-          for (int rt = 0; rt < AM_RT; ++rt)
+          for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-            for (int q = 0; q < AM_DS / 4; ++q) {
+            for (int q = 0; q < DS / 4; ++q) {
               const int d = d0 + 4 * q + g;
-              xa[rt][q] = (xok[rt] && 4 * q + g < dn) ? X[xrow[rt] * D + d] : 0.0f;
+              xa[rt][q] = (xok[rt] && 4 * q + g < dn) ? X[xrow[rt] * D + d] : (T)0;
             }
         }
         // the next block is fetched one chunk per 16 dimensions into the other
@@ -328,28 +203,27 @@ void k_assign_f32_mfma(long M, const int *__restrict__ Mdev,
         const bool nxt = sl + 1 < nsl || jb + 1 < nblk;
         const int jbn = sl + 1 < nsl ? jb : jb + 1, sln = sl + 1 < nsl ? sl + 1 : 0;
         const int ns = (dn + 15) / 16;
-        mfma_f4 fl;
+        v4 fl;
 #pragma unroll
-        for (int s16 = 0; s16 < AM_DS / 16; ++s16) {  // unrolled: xa stays in registers
+        for (int s16 = 0; s16 < DS / 16; ++s16) {  // unrolled: xa stays in registers
           if (nxt) {
             if (s16 > 0) Bl[buf ^ 1][s16 - 1][w][threadIdx.x & 63] = fl;
             fl = fetch(s16, jbn, sln);
           }
           if (s16 < ns) {
-            mfma_f4 bf[AM_CT];
+            v4 bf[AM_CT];
 #pragma unroll
             for (int ct = 0; ct < AM_CT; ++ct) bf[ct] = Bl[buf][s16][g][ct * 16 + r];
 #pragma unroll
             for (int t = 0; t < 4; ++t)
 #pragma unroll
-              for (int rt = 0; rt < AM_RT; ++rt)
+              for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
                 for (int ct = 0; ct < AM_CT; ++ct)
-                  acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-                      xa[rt][4 * s16 + t], bf[ct][t], acc[rt][ct], 0, 0, 0);
+                  acc[rt][ct] = Cfg::mfma(xa[rt][4 * s16 + t], bf[ct][t], acc[rt][ct]);
           }
         }
-        if (nxt) Bl[buf ^ 1][AM_DS / 16 - 1][w][threadIdx.x & 63] = fl;
+        if (nxt) Bl[buf ^ 1][DS / 16 - 1][w][threadIdx.x & 63] = fl;
         __syncthreads();  // next buffer written, this one no longer read
         buf ^= 1;
       }
@@ -358,7 +232,7 @@ void k_assign_f32_mfma(long M, const int *__restrict__ Mdev,
       for (int ct = 0; ct < AM_CT; ++ct) {
         const int j = jb * 64 + ct * 16 + r;
 #pragma unroll
-        for (int rt = 0; rt < AM_RT; ++rt)
+        for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
           for (int v = 0; v < 4; ++v)
             if (acc[rt][ct][v] > bv[rt][v]) { bv[rt][v] = acc[rt][ct][v]; bi[rt][v] = j; }
@@ -366,21 +240,21 @@ void k_assign_f32_mfma(long M, const int *__restrict__ Mdev,
     }
     // the 16 lanes of a row group hold the same points; merge with the tie rule
 #pragma unroll
-    for (int rt = 0; rt < AM_RT; ++rt)
+    for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
       for (int v = 0; v < 4; ++v)
 #pragma unroll
         for (int off = 8; off >= 1; off >>= 1) {
-          const float ov = __shfl_xor(bv[rt][v], off);
+          const T ov = __shfl_xor(bv[rt][v], off);
           const int oi = __shfl_xor(bi[rt][v], off);
           if (ov > bv[rt][v] || (ov == bv[rt][v] && oi < bi[rt][v])) { bv[rt][v] = ov; bi[rt][v] = oi; }
         }
     if (r == 0) {
 #pragma unroll
-      for (int rt = 0; rt < AM_RT; ++rt)
+      for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const long pt = p0 + rt * 16 + g * 4 + v;
+          const long pt = p0 + rt * 16 + Cfg::row(g, v);
           if (pt < M) {
             const long o = out_pos ? (long)out_pos[pt] : rows ? (long)rows[pt] : pt;
             labels[o] = bi[rt][v];
@@ -391,16 +265,22 @@ void k_assign_f32_mfma(long M, const int *__restrict__ Mdev,
   }
 }
 
-__global__ __launch_bounds__(256) void k_assign_f64(long M, const int *__restrict__ Mdev, int D,
-                                                    const double *__restrict__ X,
-                                                    const int *__restrict__ rows, int K,
-                                                    const double *__restrict__ Ct, int ldc,
-                                                    const double *__restrict__ nc,
-                                                    const int *__restrict__ out_pos,
-                                                    int *__restrict__ labels,
-                                                    double *__restrict__ best) {
-  assign_tiles<double, A_BM_F64 / 8>(M, Mdev, D, X, rows, K, Ct, ldc, nc, out_pos, labels, best);
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_assign_f32_mfma(long M, const int *__restrict__ Mdev, int D, const float *__restrict__ X,
+                       const int *__restrict__ rows, int K, const float *__restrict__ Ct, int ldc,
+                       const float *__restrict__ nc, const int *__restrict__ out_pos,
+                       int *__restrict__ labels, float *__restrict__ best) {
+  assign_mfma_tiles<float>(M, Mdev, D, X, rows, K, Ct, ldc, nc, out_pos, labels, best);
 }
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_assign_f64_mfma(long M, const int *__restrict__ Mdev, int D, const double *__restrict__ X,
+                       const int *__restrict__ rows, int K, const double *__restrict__ Ct, int ldc,
+                       const double *__restrict__ nc, const int *__restrict__ out_pos,
+                       int *__restrict__ labels, double *__restrict__ best) {
+  assign_mfma_tiles<double>(M, Mdev, D, X, rows, K, Ct, ldc, nc, out_pos, labels, best);
+}
+
 
 // ---------------------------------------------------------------- update
 // Sub-block label histogram, label-major: hist[j*nsb + s].
@@ -936,7 +816,7 @@ static hipError_t launch_transpose_t(hipStream_t s, int k, int d, const T *C, T 
 hipError_t launch_assign_f32(hipStream_t s, long m, const int *m_dev, int d, const float *X,
                              const int *rows, int k, const float *Ct, int ldc, const float *nc,
                              int *labels, float *best, const int *out_pos) {
-  long tiles = km_cdiv(m, AM_BM);
+  long tiles = km_cdiv(m, am_bm<float>());
   if (tiles <= 0) return hipSuccess;
   long grid = tiles < (long)num_cus() * 8 ? tiles : (long)num_cus() * 8;
   hipLaunchKernelGGL(k_assign_f32_mfma, dim3(grid), dim3(256), 0, s, m, m_dev, d, X, rows, k, Ct,
@@ -946,10 +826,10 @@ hipError_t launch_assign_f32(hipStream_t s, long m, const int *m_dev, int d, con
 
 hipError_t launch_assign_f64(hipStream_t s, long m, int d, const double *X, int k,
                              const double *Ct, int ldc, const double *nc, int *labels) {
-  long tiles = km_cdiv(m, A_BM_F64);
+  long tiles = km_cdiv(m, am_bm<double>());
   if (tiles <= 0) return hipSuccess;
-  long grid = tiles < (long)num_cus() * 4 ? tiles : (long)num_cus() * 4;
-  hipLaunchKernelGGL(k_assign_f64, dim3(grid), dim3(256), 0, s, m, (const int *)nullptr, d, X,
+  long grid = tiles < (long)num_cus() * 8 ? tiles : (long)num_cus() * 8;
+  hipLaunchKernelGGL(k_assign_f64_mfma, dim3(grid), dim3(256), 0, s, m, (const int *)nullptr, d, X,
                      (const int *)nullptr, k, Ct, ldc, nc, (const int *)nullptr, labels,
                      (double *)nullptr);
   return hipGetLastError();

@@ -44,6 +44,33 @@ def test_assign_bit_exact(orc, n, d, k):
     assert np.array_equal(best, rb)
 
 
+@pytest.mark.parametrize("case", ["wide", "subnormal", "cancel"])
+def test_assign_adversarial_rounding(orc, case):
+    """The matrix-core chain must round exactly like the fmaf chain where the
+    orders and roundings of alternative evaluations differ most: values over
+    ~100 binades, subnormal products and partial sums, and near-cancelling
+    terms.  Labels and best scores bit for bit against the oracle."""
+    rng = np.random.default_rng({"wide": 1, "subnormal": 2, "cancel": 3}[case])
+    n, d, k = 3000, 37, 70
+    if case == "wide":
+        X = rng.standard_normal((n, d)) * np.exp2(rng.integers(-50, 50, (n, d)))
+        C = rng.standard_normal((k, d)) * np.exp2(rng.integers(-50, 50, (k, d)))
+    elif case == "subnormal":
+        X = rng.standard_normal((n, d)) * 2.0 ** -70
+        C = rng.standard_normal((k, d)) * 2.0 ** -62
+    else:
+        X = rng.standard_normal((n, d)) * 1e3
+        C = rng.standard_normal((k, d))
+        C[:, 1::2] = C[:, 0::2][:, : d // 2] * -1.0000001  # pairs that nearly cancel
+        X[:, 1::2] = X[:, 0::2][:, : d // 2]
+    X, C = X.astype(np.float32), C.astype(np.float32)
+    lab, best, nc = gpu_assign(X, C)
+    rl, rb, _ = orc.assign(X, C, with_scores=True)
+    assert np.array_equal(nc, orc.col_norms(C))
+    assert np.array_equal(lab, rl)
+    assert np.array_equal(best.view(np.uint32), rb.view(np.uint32))
+
+
 def test_assign_ties_lowest_index(orc):
     rng = np.random.default_rng(5)
     X = rng.standard_normal((2048, 24)).astype(np.float32)
