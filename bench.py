@@ -28,7 +28,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2")
-    ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16", "fp16"])
+    ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16", "fp16", "fp64"])
     ap.add_argument("--n", type=int, default=None, help="override N (smaller runs)")
     ap.add_argument("--cpu-sample", type=int, default=0, help="points in the CPU baseline sample (0: all)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -65,8 +65,11 @@ def main():
     X, C0, spec = datagen.generate_on_gpu(args.workload, n=args.n, torch=torch)
     n, d, k = spec["n"], spec["d"], spec["k"]
 
-    if args.full_gemm and args.precision != "fp32":
-        ap.error("--full-gemm needs --precision fp32")
+    if args.full_gemm and args.precision not in ("fp32", "fp64"):
+        ap.error("--full-gemm needs --precision fp32 or fp64")
+    if args.precision == "fp64":
+        print(json.dumps(bench_fp64(args, X, C0, spec, km, np)))
+        return
 
     def run(precision, steps, warmup, full_gemm=False):
         e = km.KMeans(n, d, k, precision=precision, timing=True, full_gemm=full_gemm)
@@ -199,6 +202,37 @@ def main():
         torch.cuda.synchronize()
         out["calibration_copy_bytes"] = 1 << 30
     print(json.dumps(out))
+
+
+def bench_fp64(args, X, C0, spec, km, np):
+    """computeKmeansD on the same workload in double: the points are copied to
+    the host as double and the call's wall clock covers its own host-to-device
+    copy and set-up, so per-iteration kernel times come from a kernel trace
+    (two calls, `--warmup` and `--warmup + --steps` iterations, differenced)."""
+    n, d, k = spec["n"], spec["d"], spec["k"]
+    Xh = X.double().cpu().numpy()
+    C0d = np.ascontiguousarray(C0.astype(np.float64).T)
+
+    def call(iters):
+        t0 = time.perf_counter()
+        st, res = km.compute_kmeans_d(Xh, k, -1.0, iters, result=C0d.copy(), init_from_result=True,
+                                      full_gemm=args.full_gemm)
+        return time.perf_counter() - t0, st, res
+
+    t_w, _, _ = call(max(args.warmup, 1))
+    t_all, st, res = call(max(args.warmup, 1) + args.steps)
+    dt = (t_all - t_w) / args.steps
+    return {
+        "metric": "assign pts*ctrs/s per Lloyd iter", "value": float(n) * k / dt,
+        "unit": "pts*ctrs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (Gaussian mixture generated on the GPU, as double)",
+        "config": {"workload": f"{args.workload}: N={n} D={d} K={k} Lloyd in double (computeKmeansD"
+                               + (", doFullSGEMM = 1)" if args.full_gemm else ")"),
+                   "N": n, "D": d, "K": k, "precision": "fp64"},
+        "timing": "difference of two computeKmeansD calls (set-up and copies cancel)",
+        "status": int(st),
+    }
 
 
 def measured_traffic(workload, precision, n_override):

@@ -87,11 +87,13 @@ const char *kmeansGetErrorString(kmeansErrorStatus err);
 /* Replaces computeKmeansF (main/cpp/kmeans.hpp:102-107, kmeans.cpp:1316-1361).
  * data: m x n host, C-ordered.  result: n x k host (D x K).
  * Deliberate deltas: rows are not shuffled; initAlgorithm == 1 takes the
- * initial centres from result (D x K) instead of sampling; an empty cluster
- * keeps its previous centre instead of becoming NaN.  doFullSGEMM and
- * matrixFormat are accepted and select the same (fused) path; numRetries is
- * ignored as in the reference.  miniBatchFraction < 1 runs the mini-batch
- * loop (kmeans.cpp:1253-1312). */
+ * initial centres from result (D x K) and 2 seeds them by k-means++ (the
+ * reference ignores initAlgorithm; any other value samples k distinct rows);
+ * an empty cluster keeps its previous centre instead of becoming NaN.
+ * doFullSGEMM == 1 assigns through a library GEMM (rocBLAS) and a row argmin
+ * as kmeans.cpp:918-992 does; otherwise the fused kernels.  matrixFormat and
+ * numRetries are ignored (the reference ignores numRetries too).
+ * miniBatchFraction < 1 runs the mini-batch loop (kmeans.cpp:1253-1312). */
 kmeansErrorStatus computeKmeansF(const float *data, int m, int n, int k, float criterion,
                                  int maxIters, int numRetries, int initAlgorithm,
                                  int doFullSGEMM, int matrixFormat, float *result,
@@ -179,6 +181,11 @@ void kmeansDestroy(kmeansHandle h);
 kmeansErrorStatus kmeansSetData(kmeansHandle h, const float *data, int on_device);
 /* Initial centres, k x n host (C-ordered). */
 kmeansErrorStatus kmeansSetCenters(kmeansHandle h, const float *centers);
+/* Initial centres by k-means++ seeding (Arthur & Vassilvitskii 2007) on the
+ * GPU, resident data only.  No counterpart in the reference (its
+ * initAlgorithm is ignored, kmeans.cpp:876-888).  Deterministic for a seed:
+ * distances in FP64, the draw walks the canonical 256-way sum tree. */
+kmeansErrorStatus kmeansInitPlusPlus(kmeansHandle h, uint64_t seed);
 /* Lloyd loop: while (it < maxIters && relErr > criterion) -- kmeans.cpp:1194. */
 kmeansErrorStatus kmeansRun(kmeansHandle h, int maxIters, double criterion);
 kmeansErrorStatus kmeansGetCenters(kmeansHandle h, float *centers /* k x n host */);

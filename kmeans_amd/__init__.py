@@ -80,6 +80,7 @@ SIGNATURES = {
     "kmeansDestroy": (None, [_P]),
     "kmeansSetData": (_I, [_P, _P, _I]),
     "kmeansSetCenters": (_I, [_P, _P]),
+    "kmeansInitPlusPlus": (_I, [_P, _U64]),
     "kmeansRun": (_I, [_P, _I, _D]),
     "kmeansGetCenters": (_I, [_P, _P]),
     "kmeansGetLabels": (_I, [_P, _P]),
@@ -226,6 +227,10 @@ class KMeans:
         C = np.ascontiguousarray(C, dtype=np.float32)
         assert C.shape == (self.k, self.n)
         _check(lib().kmeansSetCenters(self._h, _np_ptr(C)), "kmeansSetCenters")
+
+    def init_plus_plus(self, seed=42):
+        """k-means++ seeding of the centres on the GPU (kmeansInitPlusPlus)."""
+        _check(lib().kmeansInitPlusPlus(self._h, int(seed)), "kmeansInitPlusPlus")
 
     def set_mini_batch(self, rows):
         _check(lib().kmeansSetMiniBatch(self._h, int(rows)), "kmeansSetMiniBatch")

@@ -74,6 +74,11 @@ inline void sample_rows(uint64_t seed, long m, int k, int *idx) {
   }
 }
 
+// k-means++ seeding (km_init.hip): the k picked rows of X into idx (device)
+hipError_t kmeanspp(hipStream_t s, long m, int d, int k, const float *X, uint64_t seed, int *idx);
+hipError_t kmeanspp(hipStream_t s, long m, int d, int k, const double *X, uint64_t seed,
+                    int *idx);
+
 kmeansErrorStatus compute_d(const double *data, int m, int n, int k, double criterion,
                             int maxIters, int initAlgorithm, int fullGemm, double *result,
                             double miniBatchFraction);
@@ -116,10 +121,12 @@ KM_TYPED_LAUNCHERS(float)
 KM_TYPED_LAUNCHERS(double)
 #undef KM_TYPED_LAUNCHERS
 // rows (nullable): point indices of the m rows; out_pos (nullable): output
-// slot of each row (default: rows[p], else p)
+// slot of each row (default: rows[p], else p); few_rows: the rows are a short
+// list (64-point workgroups instead of 128)
 hipError_t launch_assign_f32(hipStream_t s, long m, const int *m_dev, int d, const float *X,
                              const int *rows, int k, const float *Ct, int ldc, const float *nc,
-                             int *labels, float *best, const int *out_pos = nullptr);
+                             int *labels, float *best, const int *out_pos = nullptr,
+                             bool few_rows = false);
 hipError_t launch_assign_f64(hipStream_t s, long m, int d, const double *X, int k,
                              const double *Ct, int ldc, const double *nc, int *labels);
 hipError_t launch_comp_reduce(hipStream_t s, int k, const double *compj, double *out);

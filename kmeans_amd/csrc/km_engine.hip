@@ -179,6 +179,24 @@ kmeansErrorStatus Engine::initialize(uint64_t seed) {
   return transpose();
 }
 
+// k-means++ seeding of the resident points (no counterpart in the
+// reference, whose initAlgorithm is ignored): the picks on the GPU, then the
+// picked rows as the centres.
+kmeansErrorStatus Engine::initialize_pp(uint64_t seed) {
+  if (!dX) return KMEANS_ERROR_INITIALIZE;
+  int *didx = nullptr;
+  KM_HIP_CHECK(hipMalloc(&didx, sizeof(int) * k), KMEANS_ERROR_INITIALIZE);
+  hipError_t e = kmeanspp(stream, m, n, k, dX, seed, didx);
+  if (e == hipSuccess) e = launch_gather_rows(stream, k, n, dX, didx, dC);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  hipFree(didx);
+  KM_HIP_CHECK(e, KMEANS_ERROR_INITIALIZE);
+  centers_set = true;
+  relErr = 1.0;
+  compactnessScore = 0.0;
+  return transpose();
+}
+
 // kmeans.cpp:1143-1170: nothing to clear (no G x K x D slabs); the update
 // zeroes its own FP64 accumulators.
 kmeansErrorStatus Engine::reset() { return KMEANS_SUCCESS; }
@@ -553,6 +571,11 @@ kmeansErrorStatus kmeansSetCenters(kmeansHandle h, const float *centers) {
   return h->e.setCenters(centers);
 }
 
+kmeansErrorStatus kmeansInitPlusPlus(kmeansHandle h, uint64_t seed) {
+  if (!h) return KMEANS_ERROR_INVALID_ARGUMENT;
+  return h->e.initialize_pp(seed);
+}
+
 kmeansErrorStatus kmeansRun(kmeansHandle h, int maxIters, double criterion) {
   if (!h || maxIters < 0) return KMEANS_ERROR_INVALID_ARGUMENT;
   h->e.maxIters = h->e.iters + maxIters;
@@ -610,6 +633,8 @@ static kmeansErrorStatus compute_f(const float *data, int m, int n, int k, doubl
     for (int j = 0; j < k; ++j)
       for (int q = 0; q < n; ++q) c[(size_t)j * n + q] = result[(size_t)q * k + j];
     st = kmeansSetCenters(h, c.data());
+  } else if (initAlgorithm == 2) {
+    st = h->e.initialize_pp(42);
   } else {
     st = h->e.initialize(42);
   }

@@ -79,6 +79,7 @@ struct Engine {
   kmeansErrorStatus buildData(const float *data, int on_device);
   kmeansErrorStatus setCenters(const float *centers_kxn_host);
   kmeansErrorStatus initialize(uint64_t seed);  // k distinct sampled rows
+  kmeansErrorStatus initialize_pp(uint64_t seed);  // k-means++ seeding (resident data)
   kmeansErrorStatus reset();
   kmeansErrorStatus closestCenters(long r0, long rows);
   kmeansErrorStatus clusterCenters(long r0, long rows);

@@ -105,11 +105,16 @@ kmeansErrorStatus compute_d(const double *data, int m, int n, int k, double crit
                  KMEANS_ERROR_INITIALIZE);
     KM_HIP_CHECK(hipStreamSynchronize(s), KMEANS_ERROR_INITIALIZE);
   } else {
-    std::vector<int> idx(k);
-    sample_rows(42, m, k, idx.data());
     KM_HIP_CHECK(hipMalloc(&E.didx, sizeof(int) * k), KMEANS_ERROR_INITIALIZE);
-    KM_HIP_CHECK(hipMemcpyAsync(E.didx, idx.data(), sizeof(int) * k, hipMemcpyHostToDevice, s),
-                 KMEANS_ERROR_INITIALIZE);
+    if (initAlgorithm == 2) {  // k-means++ seeding
+      KM_HIP_CHECK(kmeanspp(s, m, n, k, E.dX, 42, E.didx), KMEANS_ERROR_INITIALIZE);
+    } else {
+      std::vector<int> idx(k);
+      sample_rows(42, m, k, idx.data());
+      KM_HIP_CHECK(hipMemcpyAsync(E.didx, idx.data(), sizeof(int) * k, hipMemcpyHostToDevice, s),
+                   KMEANS_ERROR_INITIALIZE);
+      KM_HIP_CHECK(hipStreamSynchronize(s), KMEANS_ERROR_INITIALIZE);
+    }
     KM_HIP_CHECK(launch_gather_rows(s, k, n, E.dX, E.didx, E.dC), KMEANS_ERROR_INITIALIZE);
     KM_HIP_CHECK(hipStreamSynchronize(s), KMEANS_ERROR_INITIALIZE);
   }

@@ -125,9 +125,10 @@ template <> struct MfmaCfg<double> {
   static __device__ __forceinline__ int row(int g, int v) { return 4 * v + g; }
 };
 #define AM_CT 4
-template <typename T> constexpr int am_bm() { return 4 * 16 * MfmaCfg<T>::RT; }  // points per workgroup
+// points per workgroup (RT row tiles of 16 per wave)
+template <typename T, int RT = MfmaCfg<T>::RT> constexpr int am_bm() { return 4 * 16 * RT; }
 
-template <typename T>
+template <typename T, int RT>
 __device__ __forceinline__ void assign_mfma_tiles(long M, const int *__restrict__ Mdev, int D,
                                                   const T *__restrict__ X,
                                                   const int *__restrict__ rows, int K,
@@ -137,7 +138,7 @@ __device__ __forceinline__ void assign_mfma_tiles(long M, const int *__restrict_
                                                   int *__restrict__ labels, T *__restrict__ best) {
   typedef MfmaCfg<T> Cfg;
   typedef typename Cfg::v4 v4;
-  constexpr int RT = Cfg::RT, DS = Cfg::DS, BM = am_bm<T>();
+  constexpr int DS = Cfg::DS, BM = am_bm<T, RT>();
   // Bl[b][s][g][c] = the four dimensions 16 s + 4 t + g (t = 0..3) of column c,
   // two buffers
   __shared__ v4 Bl[2][DS / 16][4][64];
@@ -265,12 +266,16 @@ __device__ __forceinline__ void assign_mfma_tiles(long M, const int *__restrict_
   }
 }
 
+// RT = 1 (64 points per workgroup) serves short lists of rows (the points
+// a reduced-precision pass leaves open), where 128-point tiles leave most
+// CUs idle
+template <int RT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void k_assign_f32_mfma(long M, const int *__restrict__ Mdev, int D, const float *__restrict__ X,
                        const int *__restrict__ rows, int K, const float *__restrict__ Ct, int ldc,
                        const float *__restrict__ nc, const int *__restrict__ out_pos,
                        int *__restrict__ labels, float *__restrict__ best) {
-  assign_mfma_tiles<float>(M, Mdev, D, X, rows, K, Ct, ldc, nc, out_pos, labels, best);
+  assign_mfma_tiles<float, RT>(M, Mdev, D, X, rows, K, Ct, ldc, nc, out_pos, labels, best);
 }
 
 // one wave per SIMD: the double operands of 128 dimensions stay in registers
@@ -281,7 +286,8 @@ void k_assign_f64_mfma(long M, const int *__restrict__ Mdev, int D, const double
                        const int *__restrict__ rows, int K, const double *__restrict__ Ct, int ldc,
                        const double *__restrict__ nc, const int *__restrict__ out_pos,
                        int *__restrict__ labels, double *__restrict__ best) {
-  assign_mfma_tiles<double>(M, Mdev, D, X, rows, K, Ct, ldc, nc, out_pos, labels, best);
+  assign_mfma_tiles<double, MfmaCfg<double>::RT>(M, Mdev, D, X, rows, K, Ct, ldc, nc, out_pos,
+                                                 labels, best);
 }
 
 
@@ -829,12 +835,17 @@ static hipError_t launch_transpose_t(hipStream_t s, int k, int d, const T *C, T 
 
 hipError_t launch_assign_f32(hipStream_t s, long m, const int *m_dev, int d, const float *X,
                              const int *rows, int k, const float *Ct, int ldc, const float *nc,
-                             int *labels, float *best, const int *out_pos) {
-  long tiles = km_cdiv(m, am_bm<float>());
+                             int *labels, float *best, const int *out_pos, bool few_rows) {
+  const int bm = few_rows ? am_bm<float, 1>() : am_bm<float, 2>();
+  long tiles = km_cdiv(m, bm);
   if (tiles <= 0) return hipSuccess;
   long grid = tiles < (long)num_cus() * 8 ? tiles : (long)num_cus() * 8;
-  hipLaunchKernelGGL(k_assign_f32_mfma, dim3(grid), dim3(256), 0, s, m, m_dev, d, X, rows, k, Ct,
-                     ldc, nc, out_pos, labels, best);
+  if (few_rows)
+    hipLaunchKernelGGL(k_assign_f32_mfma<1>, dim3(grid), dim3(256), 0, s, m, m_dev, d, X, rows, k,
+                       Ct, ldc, nc, out_pos, labels, best);
+  else
+    hipLaunchKernelGGL(k_assign_f32_mfma<2>, dim3(grid), dim3(256), 0, s, m, m_dev, d, X, rows, k,
+                       Ct, ldc, nc, out_pos, labels, best);
   return hipGetLastError();
 }
 

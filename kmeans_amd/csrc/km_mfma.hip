@@ -1040,7 +1040,7 @@ hipError_t mfma_reresolve_view(Engine *e, const float *X, const uint16_t *Xh, co
 #undef DEEP
   if ((r = hipGetLastError()) != hipSuccess) return r;
   return launch_assign_f32(e->stream, rows, s->nflag2 + e->cur_slot, e->n, X, s->flags2, e->k,
-                           e->dCt, e->ldc, e->dnc, lab, nullptr);
+                           e->dCt, e->ldc, e->dnc, lab, nullptr, nullptr, true);
 }
 
 const uint16_t *mfma_tile_xh(Engine *e) { return st(e)->Xh; }

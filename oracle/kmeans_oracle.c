@@ -29,6 +29,9 @@
  *                    squares x*x, which for float data equals the FP32
  *                    definition below (those squares are exact).
  *   orc_row_transform_min{f,d}  main/cuda/RowTransformMinimum.cu:3-58.
+ *   orc_kmeanspp     k-means++ seeding (Arthur & Vassilvitskii 2007) in this
+ *                    build's canonical form (see the function); the
+ *                    reference has none (initAlgorithm is ignored).
  *
  * Canonical summation (the contract the GPU path reproduces bit for bit):
  *   CSUM(v[0..n)) = left-to-right FP64 sum from +0.0 if n <= KM_CHUNK,
@@ -400,6 +403,117 @@ int orc_lloyd(const float *X, long n, int d, int k, float *C, int maxIters, doub
   if (relerr_out) *relerr_out = relErr;
   return relErr <= criterion ? 0 : 13;
 }
+
+/* ------------------------------------------------------------------ */
+/* k-means++ seeding (no counterpart in the reference, whose initAlgorithm is
+ * ignored, kmeans.cpp:876-888; the seeding of Arthur & Vassilvitskii,
+ * SODA 2007).  Canonical form:
+ *   dist(x, c) = left-to-right FP64 sum over d of ((double)x_d - (double)c_d)^2
+ *                (difference and square each rounded, no fma);
+ *   i_0 = rng_u64(seed, 6, 0) % m;  D_i = dist(x_i, x_{i_0});
+ *   step t = 1..k-1: W = the CSUM tree of D (level 0: left-to-right sums of
+ *   256 consecutive D_i, level l+1: the same over 256 consecutive level-l
+ *   sums, up to a level of at most 256 values); T = left-to-right sum of the
+ *   top level; if T == 0, i_t = rng_u64(seed, 6, t) % m; otherwise with
+ *   target = rng_unif(seed, 6, t) * T, descend the tree: at each level keep a
+ *   running sum r (the prefix before the current node, starting from 0),
+ *   add the node's children left to right and enter the first child that
+ *   takes r past target (if none does, the last child with a positive
+ *   value, r then being the running sum before it); at level 0 the child is
+ *   the point.  D_i = min(D_i, dist(x_i, x_{i_t})).                       */
+#define DEF_PP_DIST(NAME, T)                                                   \
+  static double NAME(const T *x, const T *c, int d) {                          \
+    double s = 0.0;                                                            \
+    for (int q = 0; q < d; ++q) {                                              \
+      const double v = (double)x[q] - (double)c[q];                            \
+      s += v * v;                                                              \
+    }                                                                          \
+    return s;                                                                  \
+  }
+DEF_PP_DIST(pp_dist, float)
+DEF_PP_DIST(pp_dist_f64, double)
+
+/* descend one level: children v[0..n), running sum *r; returns the child */
+static long pp_descend(const double *v, long n, double *r, double target) {
+  double run = *r, before = *r;
+  long last = -1;
+  double last_before = *r;
+  for (long i = 0; i < n; ++i) {
+    before = run;
+    run += v[i];
+    if (v[i] > 0.0) { last = i; last_before = before; }
+    if (run > target) { *r = before; return i; }
+  }
+  *r = last_before;
+  return last < 0 ? n - 1 : last;
+}
+
+#define DEF_KMEANSPP(NAME, ET, DIST)                                                   \
+int NAME(const ET *X, long m, int d, int k, uint64_t seed, int *idx) {                 \
+  if (k > m || k <= 0) return -1;                                                      \
+  double *D = (double *)malloc(sizeof(double) * (size_t)m);                            \
+  /* tree levels: lv[0] = sums of 256 points, ... */                                   \
+  double *lv[8];                                                                       \
+  long ln[8];                                                                          \
+  int L = 0;                                                                           \
+  long n = m;                                                                          \
+  while (n > KM_CHUNK) {                                                               \
+    ln[L] = (n + KM_CHUNK - 1) / KM_CHUNK;                                             \
+    lv[L] = (double *)malloc(sizeof(double) * (size_t)ln[L]);                          \
+    n = ln[L];                                                                         \
+    ++L;                                                                               \
+  }                                                                                    \
+  idx[0] = (int)(rng_u64(seed, 6, 0) % (uint64_t)m);                                   \
+  {                                                                                    \
+    const ET *c = X + (long)idx[0] * d;                                                \
+_Pragma("omp parallel for schedule(static)")                                           \
+    for (long i = 0; i < m; ++i) D[i] = DIST(X + i * d, c, d);                         \
+  }                                                                                    \
+  for (int t = 1; t < k; ++t) {                                                        \
+    /* the tree */                                                                     \
+    const double *below = D;                                                           \
+    long nb = m;                                                                       \
+    for (int l = 0; l < L; ++l) {                                                      \
+      for (long g = 0; g < ln[l]; ++g) {                                               \
+        const long a = g * KM_CHUNK, e = a + KM_CHUNK < nb ? a + KM_CHUNK : nb;        \
+        double s = 0.0;                                                                \
+        for (long i = a; i < e; ++i) s += below[i];                                    \
+        lv[l][g] = s;                                                                  \
+      }                                                                                \
+      below = lv[l];                                                                   \
+      nb = ln[l];                                                                      \
+    }                                                                                  \
+    double tot = 0.0;                                                                  \
+    for (long i = 0; i < nb; ++i) tot += below[i];                                     \
+    long pick;                                                                         \
+    if (tot == 0.0) {                                                                  \
+      pick = (long)(rng_u64(seed, 6, (uint64_t)t) % (uint64_t)m);                      \
+    } else {                                                                           \
+      const double target = rng_unif(seed, 6, (uint64_t)t) * tot;                      \
+      double r = 0.0;                                                                  \
+      long node = pp_descend(below, nb, &r, target);  /* top level */                  \
+      for (int l = L - 1; l >= 0; --l) {                                               \
+        const double *ch = l > 0 ? lv[l - 1] : D;                                      \
+        const long cn = l > 0 ? ln[l - 1] : m;                                         \
+        const long a = node * KM_CHUNK, e = a + KM_CHUNK < cn ? a + KM_CHUNK : cn;     \
+        node = a + pp_descend(ch + a, e - a, &r, target);                              \
+      }                                                                                \
+      pick = node;                                                                     \
+    }                                                                                  \
+    idx[t] = (int)pick;                                                                \
+    const ET *c = X + pick * d;                                                        \
+_Pragma("omp parallel for schedule(static)")                                           \
+    for (long i = 0; i < m; ++i) {                                                     \
+      const double v = DIST(X + i * d, c, d);                                          \
+      if (v < D[i]) D[i] = v;                                                          \
+    }                                                                                  \
+  }                                                                                    \
+  for (int l = 0; l < L; ++l) free(lv[l]);                                             \
+  free(D);                                                                             \
+  return 0;                                                                            \
+}
+DEF_KMEANSPP(orc_kmeanspp, float, pp_dist)
+DEF_KMEANSPP(orc_kmeanspp_f64, double, pp_dist_f64)
 
 /* ------------------------------------------------------------------ */
 /* Double-precision engine (computeKmeansD and the ...D kernel names). */

@@ -57,6 +57,8 @@ def lib():
             "orc_lloyd_f64": (i32, [P, i64, i32, i32, P, i32, f64, P, P, P, P]),
             "orc_row_transform_min_f": (None, [i64, i64, i32, P, P, P, P]),
             "orc_row_transform_min_d": (None, [i64, i64, i32, P, P, P, P]),
+            "orc_kmeanspp": (i32, [P, i64, i32, i32, u64, P]),
+            "orc_kmeanspp_f64": (i32, [P, i64, i32, i32, u64, P]),
             "orc_compactness_fp64": (f64, [P, i64, i32, P, P]),
             "orc_mlp_fwd_bwd": (f64, [i32, i32, i32, i32, P, P, P, P, i32]),
             "orc_num_threads": (i32, []),
@@ -286,6 +288,16 @@ def row_transform_min(P, rn, cn, m0=0, out=None):
     fn = lib().orc_row_transform_min_f if f else lib().orc_row_transform_min_d
     fn(m0, m, n, _p(rn), _p(cn), _p(P), _p(out))
     return out
+
+
+def kmeanspp(X, k, seed):
+    """k-means++ seeding: the k chosen row indices in pick order (dtype picks the variant)."""
+    f64 = np.asarray(X).dtype == np.float64
+    X = _f64(X) if f64 else _f32(X)
+    idx = np.empty(k, np.int32)
+    fn = lib().orc_kmeanspp_f64 if f64 else lib().orc_kmeanspp
+    assert fn(_p(X), X.shape[0], X.shape[1], k, seed, _p(idx)) == 0
+    return idx
 
 
 def num_threads():

@@ -433,3 +433,61 @@ def test_row_transform_min_matches_nearest_centre(orc):
         clear = (srt[:, 1] - srt[:, 0]) > 1e-1
         assert clear.mean() > 0.99
         assert np.array_equal(out[clear], D2.argmin(1)[clear])
+
+
+# ------------------------------------------------------------------ k-means++ seeding
+def test_kmeanspp_one_pick_per_separated_component(orc):
+    X, z, _, _ = orc.mixture(5, 50_000, 8, 20, -50, 50, 0.1, 0.1)
+    for X_ in (X, X.astype(np.float64)):
+        idx = orc.kmeanspp(X_, 20, 1)
+        assert len(set(idx.tolist())) == 20
+        assert len(set(z[idx].tolist())) == 20
+
+
+def test_kmeanspp_first_pick_seed_and_type(orc):
+    X, _, _, _ = orc.mixture(6, 3000, 5, 7, -5, 5, 1.0, 1.0)
+    for seed in (0, 1, 99):
+        idx = orc.kmeanspp(X, 7, seed)
+        assert idx[0] == orc.lib().orc_rng_u64(seed, 6, 0) % 3000
+        assert np.array_equal(idx, orc.kmeanspp(X, 7, seed))
+        # on float-valued data the double variant draws the same points
+        assert np.array_equal(idx, orc.kmeanspp(X.astype(np.float64), 7, seed))
+    assert not np.array_equal(orc.kmeanspp(X, 7, 0), orc.kmeanspp(X, 7, 1))
+
+
+def test_kmeanspp_draw_is_proportional_to_squared_distance(orc):
+    # 1-D points; over many seeds the second pick must follow D(x) / sum D for
+    # the first pick (not sqrt(D), not uniform): chi-square against expectation
+    X = np.array([[0.0], [1.0], [3.0], [3.5], [7.0], [10.0]], np.float32)
+    obs = np.zeros(6)
+    exp = np.zeros(6)
+    for seed in range(6000):
+        i0, i1 = orc.kmeanspp(X, 2, seed)
+        D = (X[:, 0].astype(np.float64) - X[i0, 0]) ** 2
+        exp += D / D.sum()
+        obs[i1] += 1
+    chi2 = ((obs - exp) ** 2 / exp).sum()
+    assert chi2 < 25.0  # 5 degrees of freedom: p ~ 1e-4
+    # the alternatives are far off
+    alt = np.zeros(6)
+    for seed in range(6000):
+        i0 = orc.kmeanspp(X, 1, seed)[0]
+        D = np.abs(X[:, 0].astype(np.float64) - X[i0, 0])
+        alt += D / D.sum()
+    assert ((obs - alt) ** 2 / alt).sum() > 100.0
+
+
+def test_kmeanspp_tree_descent_and_zero_total(orc):
+    # 70,000 points (a two-level tree) all at the origin but one: step 1 must
+    # find the only point with a positive distance
+    X = np.zeros((70_000, 3), np.float32)
+    X[61_234] = [5.0, -1.0, 2.0]
+    for seed in range(5):
+        idx = orc.kmeanspp(X, 2, seed)
+        if idx[0] != 61_234:
+            assert idx[1] == 61_234
+    # all points identical: total 0, the draw is rng_u64(seed, 6, t) % m
+    Y = np.ones((1000, 4), np.float32)
+    idx = orc.kmeanspp(Y, 3, 7)
+    assert idx[1] == orc.lib().orc_rng_u64(7, 6, 1) % 1000
+    assert idx[2] == orc.lib().orc_rng_u64(7, 6, 2) % 1000
