@@ -7,6 +7,7 @@ entry points a C or C++ caller links against:
 
   compute_kmeans_f(...)        computeKmeansF (reference main/cpp/kmeans.hpp:102-107)
   compute_kmeans_d(...)        computeKmeansD (kmeans.hpp:125-130), double precision
+  load_raw_f32(path, n)        the reference's headerless float32 input files, memory-mapped
   KMeans(m, n, k, ...)         the engine handle (kmeans<T> of kmeans.hpp:174-658)
   col_norms / assign_f32 /     kernel API, device pointers (KmeansCudaKernels.h:
   update / compactness          238-474 replaced with stream + workspace arguments)
@@ -154,6 +155,25 @@ def _np_ptr(a):
 def _check(code, where):
     if code != 0:
         raise KmeansError(code, where)
+
+
+# ------------------------------------------------------------------ data files
+def load_raw_f32(path, n, m=None):
+    """Map a headerless row-major float32 file of m x n points, the reference's
+    input format (test/cpp/KmeansTests.cpp:84-87 reads m*n floats from it),
+    without reading it into memory.  m=None takes every whole row in the file.
+    The returned array can go to KMeans.set_data directly: resident engines
+    copy it once, engines with tile_rows stream it through pinned staging
+    buffers tile by tile."""
+    size = os.path.getsize(path)
+    if size % 4:
+        raise ValueError(f"{path}: {size} bytes is not a whole number of float32 values")
+    rows = size // (4 * n)
+    if m is None:
+        m = rows
+    if m <= 0 or m > rows:
+        raise ValueError(f"{path}: holds {size // 4} floats, {m} x {n} requested")
+    return np.memmap(path, dtype=np.float32, mode="r", shape=(m, n))
 
 
 # ------------------------------------------------------------------ host API

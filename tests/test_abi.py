@@ -130,3 +130,18 @@ def test_c4_instance_sizes():
     assert 5.5e7 < n.sum() < 7.5e7
     assert abs(np.median(n) / 20000 - 1) < 0.1
     assert len(set(seeds.tolist())) == 2048
+
+
+def test_load_raw_f32_maps_reference_files(tmp_path):
+    X = np.arange(7 * 5, dtype=np.float32).reshape(7, 5)
+    path = tmp_path / "points.bin"
+    X.tofile(path)
+    M = kmeans_amd.load_raw_f32(str(path), 5)
+    assert isinstance(M, np.memmap) and M.shape == (7, 5)
+    assert np.array_equal(M, X)
+    assert np.array_equal(kmeans_amd.load_raw_f32(str(path), 5, m=3), X[:3])
+    with pytest.raises(ValueError):
+        kmeans_amd.load_raw_f32(str(path), 5, m=8)  # fewer floats than m*n, as fread would find
+    (tmp_path / "odd.bin").write_bytes(b"\0" * 6)
+    with pytest.raises(ValueError):
+        kmeans_amd.load_raw_f32(str(tmp_path / "odd.bin"), 1)

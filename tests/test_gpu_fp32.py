@@ -360,6 +360,24 @@ def test_streamed_tiles_equal_resident(orc, prec):
         assert st.stats()["compactness"] == ref["comp_hist"][-1]
 
 
+def test_streamed_from_a_mapped_file(orc, tmp_path):
+    # the reference's input format (headerless float32 rows) memory-mapped and
+    # streamed through HBM tile by tile; equal to the resident run
+    n, d, k = 2 * km.KM_BLOCK + 777, 3, 8
+    X, _, _, _ = orc.mixture(52, n, d, k, -20, 20, 1.0, 1.0)
+    path = tmp_path / "points.f32"
+    X.tofile(path)
+    M = km.load_raw_f32(str(path), d)
+    st = km.KMeans(n, d, k, tile_rows=km.KM_BLOCK)
+    st.set_data(M)
+    st.set_centers(X[:k])
+    st.run(2)
+    ref = orc.lloyd(X, X[:k].copy(), 2)
+    assert np.array_equal(st.centers(), ref["centers"])
+    assert np.array_equal(st.labels(), ref["labels"])
+    st.close()
+
+
 def test_streamed_rejects_unaligned_tiles(orc):
     X = np.zeros((1000, 2), np.float32)
     e = km.KMeans(1000, 2, 4, tile_rows=500)
