@@ -9,6 +9,7 @@
 //   k_col_norms         main/cuda/Normalize.cu:33-47
 //   k_row_norms         main/cuda/Normalize.cu:3-31
 #include <float.h>
+#include <limits.h>
 #include <math.h>
 
 #include "km_common.hpp"
@@ -135,7 +136,8 @@ __device__ __forceinline__ void assign_mfma_tiles(long M, const int *__restrict_
                                                   const T *__restrict__ Ct, int ldc,
                                                   const T *__restrict__ nc,
                                                   const int *__restrict__ out_pos,
-                                                  int *__restrict__ labels, T *__restrict__ best) {
+                                                  int *__restrict__ labels, T *__restrict__ best,
+                                                  long mlo, long mhi) {
   typedef MfmaCfg<T> Cfg;
   typedef typename Cfg::v4 v4;
   constexpr int DS = Cfg::DS, BM = am_bm<T, RT>();
@@ -143,6 +145,7 @@ __device__ __forceinline__ void assign_mfma_tiles(long M, const int *__restrict_
   // two buffers
   __shared__ v4 Bl[2][DS / 16][4][64];
   if (Mdev) M = *Mdev;
+  if (M < mlo || M > mhi) return;  // the row count is served by the other tile shape
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int g = lane >> 4, r = lane & 15;
   const long ntiles = (M + BM - 1) / BM;
@@ -268,14 +271,17 @@ __device__ __forceinline__ void assign_mfma_tiles(long M, const int *__restrict_
 
 // RT = 1 (64 points per workgroup) serves short lists of rows (the points
 // a reduced-precision pass leaves open), where 128-point tiles leave most
-// CUs idle
+// CUs idle.  A kernel only works when the row count lies in [mlo, mhi]: for a
+// count known only on the device both shapes are launched with complementary
+// ranges and one of them exits at once.
 template <int RT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void k_assign_f32_mfma(long M, const int *__restrict__ Mdev, int D, const float *__restrict__ X,
                        const int *__restrict__ rows, int K, const float *__restrict__ Ct, int ldc,
                        const float *__restrict__ nc, const int *__restrict__ out_pos,
-                       int *__restrict__ labels, float *__restrict__ best) {
-  assign_mfma_tiles<float, RT>(M, Mdev, D, X, rows, K, Ct, ldc, nc, out_pos, labels, best);
+                       int *__restrict__ labels, float *__restrict__ best, long mlo, long mhi) {
+  assign_mfma_tiles<float, RT>(M, Mdev, D, X, rows, K, Ct, ldc, nc, out_pos, labels, best, mlo,
+                               mhi);
 }
 
 // one wave per SIMD: the double operands of 128 dimensions stay in registers
@@ -287,7 +293,7 @@ void k_assign_f64_mfma(long M, const int *__restrict__ Mdev, int D, const double
                        const double *__restrict__ nc, const int *__restrict__ out_pos,
                        int *__restrict__ labels, double *__restrict__ best) {
   assign_mfma_tiles<double, MfmaCfg<double>::RT>(M, Mdev, D, X, rows, K, Ct, ldc, nc, out_pos,
-                                                 labels, best);
+                                                 labels, best, 0L, LONG_MAX);
 }
 
 
@@ -836,16 +842,19 @@ static hipError_t launch_transpose_t(hipStream_t s, int k, int d, const T *C, T 
 hipError_t launch_assign_f32(hipStream_t s, long m, const int *m_dev, int d, const float *X,
                              const int *rows, int k, const float *Ct, int ldc, const float *nc,
                              int *labels, float *best, const int *out_pos, bool few_rows) {
-  const int bm = few_rows ? am_bm<float, 1>() : am_bm<float, 2>();
-  long tiles = km_cdiv(m, bm);
-  if (tiles <= 0) return hipSuccess;
-  long grid = tiles < (long)num_cus() * 8 ? tiles : (long)num_cus() * 8;
-  if (few_rows)
-    hipLaunchKernelGGL(k_assign_f32_mfma<1>, dim3(grid), dim3(256), 0, s, m, m_dev, d, X, rows, k,
-                       Ct, ldc, nc, out_pos, labels, best);
-  else
-    hipLaunchKernelGGL(k_assign_f32_mfma<2>, dim3(grid), dim3(256), 0, s, m, m_dev, d, X, rows, k,
-                       Ct, ldc, nc, out_pos, labels, best);
+  const long cap = (long)am_bm<float, 1>() * 2 * num_cus();  // two 64-point tiles per CU
+  const long ncap = (long)num_cus() * 8;
+  if (m <= 0) return hipSuccess;
+  if (few_rows && (m_dev != nullptr || m <= cap)) {
+    const long t1 = km_cdiv(m < cap ? m : cap, am_bm<float, 1>());
+    hipLaunchKernelGGL(k_assign_f32_mfma<1>, dim3(t1 < ncap ? t1 : ncap), dim3(256), 0, s, m,
+                       m_dev, d, X, rows, k, Ct, ldc, nc, out_pos, labels, best, 0L, cap);
+    if (m_dev == nullptr && m <= cap) return hipGetLastError();
+  }
+  const long t2 = km_cdiv(m, am_bm<float, 2>());
+  hipLaunchKernelGGL(k_assign_f32_mfma<2>, dim3(t2 < ncap ? t2 : ncap), dim3(256), 0, s, m, m_dev,
+                     d, X, rows, k, Ct, ldc, nc, out_pos, labels, best, few_rows ? cap + 1 : 0L,
+                     LONG_MAX);
   return hipGetLastError();
 }
 

@@ -1716,7 +1716,7 @@ hipError_t learned_fallback(Engine *e) {
     }
   } else {
     LCHK(launch_assign_f32(e->stream, e->m, L->fbcnt, e->n, e->dX, L->fb, e->k, e->dCt, e->ldc,
-                           e->dnc, e->dlabels, nullptr));
+                           e->dnc, e->dlabels, nullptr, nullptr, true));
   }
   hipLaunchKernelGGL(k_hits, dim3(g1(e->m)), dim3(256), 0, e->stream, e->m, L->T, L->cand, L->gof,
                      e->dlabels, L->cnt);
