@@ -156,3 +156,14 @@ def test_row_transform_minimum(orc, dt, m, n):
     got = out.cpu().numpy()
     assert np.array_equal(got, ref)
     assert got[m0 + 1] == -1 and np.all(got[:m0] == -7)
+
+
+@pytest.mark.parametrize("n,d,k", [(500, 1, 3), (400, 6, 1), (64, 5, 64), (1, 3, 1)])
+def test_compute_kmeans_d_edge_shapes(orc, n, d, k):
+    # one dimension, one centre, every point its own centre, a single point
+    rng = np.random.default_rng(n + d + k)
+    X = rng.standard_normal((n, d)) * 3
+    C0 = X[:k].copy()
+    st, res = km.compute_kmeans_d(X, k, -1.0, 3, result=C0.T.copy(), init_from_result=True)
+    ref = orc.lloyd64(X, C0, 3)
+    assert np.array_equal(res, ref["centers"].T)

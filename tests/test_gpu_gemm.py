@@ -90,3 +90,21 @@ def test_compute_kmeans_d_full_gemm_equals_fused_on_separated_data(orc):
 def test_full_gemm_rejects_reduced_precision():
     with pytest.raises(km.KmeansError):
         km.KMeans(1000, 8, 4, precision="bf16", full_gemm=True)
+
+
+def test_gemm_assign_single_centre_and_dimension(orc):
+    # K = 1: every label 0; D = 1 with two centres: the argmin on a line
+    X = np.linspace(-5, 5, 1001, dtype=np.float32)[:, None]
+    e = km.KMeans(1001, 1, 1, full_gemm=True)
+    e.set_data(X)
+    e.set_centers(np.array([[0.5]], np.float32))
+    e.run(1)
+    assert np.all(e.labels() == 0)
+    e.close()
+    e = km.KMeans(1001, 1, 2, full_gemm=True)
+    e.set_data(X)
+    e.set_centers(np.array([[-2.0], [3.0]], np.float32))
+    e.run(1)
+    lab = e.labels()
+    e.close()
+    check_labels(orc, X, np.array([[-2.0], [3.0]], np.float32), lab)

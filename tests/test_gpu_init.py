@@ -73,3 +73,19 @@ def test_plus_plus_needs_resident_data():
     with pytest.raises(km.KmeansError):
         e.init_plus_plus(1)
     e.close()
+
+
+@pytest.mark.parametrize("n,d,k", [(64, 3, 64), (257, 1, 5), (5000, 2, 1)])
+def test_plus_plus_edge_shapes(orc, n, d, k):
+    # k == m (every point is picked once), one dimension with 257 points (a
+    # second chunk of one point), a single centre
+    rng = np.random.default_rng(n)
+    X = rng.standard_normal((n, d)).astype(np.float32)
+    idx = orc.kmeanspp(X, k, 3)
+    if k == n:
+        assert sorted(idx.tolist()) == list(range(n))
+    e = km.KMeans(n, d, k)
+    e.set_data(X)
+    e.init_plus_plus(3)
+    assert np.array_equal(e.centers(), X[idx])
+    e.close()
