@@ -111,9 +111,12 @@ kmeansErrorStatus computeKmeansD(const double *data, int m, int n, int k, double
 
 /* Assignment precision of the engine. */
 typedef enum kmeansPrecision {
-  KM_PREC_FP32 = 0,      /* plain path: fma-chain scores on VALU             */
-  KM_PREC_BF16 = 1,      /* BF16 MFMA scores + FP32 re-resolve of near-ties  */
-  KM_PREC_FP16 = 2       /* FP16 MFMA scores + FP32 re-resolve of near-ties  */
+  KM_PREC_FP32 = 0,      /* plain path: the canonical fma-chain scores (run on
+                            the FP32 matrix cores, bit-identical)            */
+  KM_PREC_BF16 = 1,      /* BF16 MFMA scores + FP32 re-resolve of near-ties
+                            (n <= 256)                                        */
+  KM_PREC_FP16 = 2       /* FP16 MFMA scores + FP32 re-resolve of near-ties
+                            (n <= 256)                                        */
 } kmeansPrecision;
 
 /* Candidate proposer of the assignment (no counterpart in the reference).
@@ -174,7 +177,8 @@ typedef struct kmeansStats {
 
 typedef struct kmeansEngine *kmeansHandle;
 
-/* Engine lifecycle.  m points of dimension n, k centres. */
+/* Engine lifecycle.  m points of dimension n, k centres: 1 <= k <= m,
+ * m < 2^31, k <= 16384; the BF16/FP16 precisions need n <= 256. */
 kmeansErrorStatus kmeansCreate(kmeansHandle *h, long m, int n, int k, const kmeansOptions *opt);
 void kmeansDestroy(kmeansHandle h);
 /* Points: host (on_device == 0, copied) or device (borrowed, must outlive h). */

@@ -62,6 +62,7 @@ kmeansErrorStatus Engine::create(long m_, int n_, int k_, const kmeansOptions *o
     return KMEANS_ERROR_INVALID_ARGUMENT;
   if (opt.full_gemm && (opt.precision != KM_PREC_FP32 || opt.proposer != KM_PROPOSER_NONE))
     return KMEANS_ERROR_INVALID_ARGUMENT;
+  if (opt.precision != KM_PREC_FP32 && n_ > 256) return KMEANS_ERROR_INVALID_ARGUMENT;
   if (learned_configure(this) != KMEANS_SUCCESS) return KMEANS_ERROR_INVALID_ARGUMENT;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {

@@ -145,3 +145,10 @@ def test_load_raw_f32_maps_reference_files(tmp_path):
     (tmp_path / "odd.bin").write_bytes(b"\0" * 6)
     with pytest.raises(ValueError):
         kmeans_amd.load_raw_f32(str(tmp_path / "odd.bin"), 1)
+
+
+def test_create_rejects_reduced_precision_above_256_dimensions():
+    L = kmeans_amd.lib()
+    h = ctypes.c_void_p()
+    opt = kmeans_amd.Options(1, 0, None, 0, 0)  # BF16
+    assert L.kmeansCreate(ctypes.byref(h), 1000, 257, 4, ctypes.byref(opt)) == 14
