@@ -195,6 +195,19 @@ kmeansErrorStatus kmeansRun(kmeansHandle h, int maxIters, double criterion);
 kmeansErrorStatus kmeansGetCenters(kmeansHandle h, float *centers /* k x n host */);
 kmeansErrorStatus kmeansGetLabels(kmeansHandle h, int *labels /* m host */);
 kmeansErrorStatus kmeansGetStats(kmeansHandle h, kmeansStats *stats);
+/* The reference class's steps, one call each (used by include/kmeans_amd.hpp):
+ * initialize (kmeans.cpp:822-907): initAlgorithm 1 takes centers_dxk (n x k,
+ * the reference's result layout), 2 seeds by k-means++, else k sampled rows;
+ * closestCenters (:911-1014): one assignment of every resident point;
+ * clusterCenters (:1019-1041): new centres from the current labels;
+ * compactness (:1045-1104): FP64 sum of squared distances to the current
+ * centres of the current labels.  Resident data only; each call returns when
+ * its work is done. */
+kmeansErrorStatus kmeansInitialize(kmeansHandle h, int initAlgorithm, uint64_t seed,
+                                   const float *centers_dxk);
+kmeansErrorStatus kmeansAssign(kmeansHandle h);
+kmeansErrorStatus kmeansUpdateCenters(kmeansHandle h);
+kmeansErrorStatus kmeansCompactness(kmeansHandle h, double *out);
 /* Mini-batch mode (kmeans.cpp:1253-1312): each iteration assigns and updates
  * from the next window of `rows` points (wrapping at m); 0 restores the
  * standard loop over all points. */

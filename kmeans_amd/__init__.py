@@ -82,6 +82,10 @@ SIGNATURES = {
     "kmeansSetData": (_I, [_P, _P, _I]),
     "kmeansSetCenters": (_I, [_P, _P]),
     "kmeansInitPlusPlus": (_I, [_P, _U64]),
+    "kmeansInitialize": (_I, [_P, _I, _U64, _P]),
+    "kmeansAssign": (_I, [_P]),
+    "kmeansUpdateCenters": (_I, [_P]),
+    "kmeansCompactness": (_I, [_P, _P]),
     "kmeansRun": (_I, [_P, _I, _D]),
     "kmeansGetCenters": (_I, [_P, _P]),
     "kmeansGetLabels": (_I, [_P, _P]),

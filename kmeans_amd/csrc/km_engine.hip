@@ -612,6 +612,62 @@ kmeansErrorStatus kmeansSynchronize(kmeansHandle h) {
   return KMEANS_SUCCESS;
 }
 
+// initialize (kmeans.cpp:822-907): 1 takes centres (n x k, the reference's
+// D x K result layout), 2 seeds by k-means++, anything else samples k rows.
+kmeansErrorStatus kmeansInitialize(kmeansHandle h, int initAlgorithm, uint64_t seed,
+                                   const float *centers_dxk) {
+  if (!h) return KMEANS_ERROR_INVALID_ARGUMENT;
+  Engine &e = h->e;
+  if (initAlgorithm == 1) {
+    if (!centers_dxk) return KMEANS_ERROR_INVALID_ARGUMENT;
+    std::vector<float> c((size_t)e.k * e.n);
+    for (int j = 0; j < e.k; ++j)
+      for (int q = 0; q < e.n; ++q) c[(size_t)j * e.n + q] = centers_dxk[(size_t)q * e.k + j];
+    return e.setCenters(c.data());
+  }
+  return initAlgorithm == 2 ? e.initialize_pp(seed) : e.initialize(seed);
+}
+
+// closestCenters (kmeans.cpp:911-1014): one assignment of every resident
+// point to the current centres (certified re-resolve included for BF16/FP16).
+kmeansErrorStatus kmeansAssign(kmeansHandle h) {
+  if (!h) return KMEANS_ERROR_INVALID_ARGUMENT;
+  Engine &e = h->e;
+  if (!e.dX || !e.centers_set) return KMEANS_ERROR_CLOSEST_CENTERS;
+  kmeansErrorStatus st = e.closestCenters(0, e.m);
+  if (st != KMEANS_SUCCESS) return st;
+  if (e.opt.precision != KM_PREC_FP32)
+    KM_HIP_CHECK(mfma_reresolve(&e, 0, e.m), KMEANS_ERROR_CLOSEST_CENTERS);
+  KM_HIP_CHECK(hipStreamSynchronize(e.stream), KMEANS_ERROR_CLOSEST_CENTERS);
+  return KMEANS_SUCCESS;
+}
+
+// clusterCenters (kmeans.cpp:1019-1041): new centres from the current labels.
+kmeansErrorStatus kmeansUpdateCenters(kmeansHandle h) {
+  if (!h) return KMEANS_ERROR_INVALID_ARGUMENT;
+  Engine &e = h->e;
+  if (!e.dX || !e.centers_set) return KMEANS_ERROR_CLUSTER_CENTERS;
+  kmeansErrorStatus st = e.clusterCenters(0, e.m);
+  if (st != KMEANS_SUCCESS) return st;
+  KM_HIP_CHECK(hipStreamSynchronize(e.stream), KMEANS_ERROR_CLUSTER_CENTERS);
+  return KMEANS_SUCCESS;
+}
+
+// compactness (kmeans.cpp:1045-1104): sum of squared distances of the points
+// to the centres of their current labels (FP64, canonical order).
+kmeansErrorStatus kmeansCompactness(kmeansHandle h, double *out) {
+  if (!h || !out) return KMEANS_ERROR_INVALID_ARGUMENT;
+  Engine &e = h->e;
+  if (!e.dX || !e.centers_set) return KMEANS_ERROR_COMPACTNESS;
+  double *dv = e.ws.compj + e.k;  // the spare slot past the per-centre terms
+  KM_HIP_CHECK(launch_compactness(e.stream, e.m, e.n, e.k, e.dX, e.dlabels, e.dq, e.dC, dv, e.ws),
+               KMEANS_ERROR_COMPACTNESS);
+  KM_HIP_CHECK(hipMemcpyAsync(out, dv, sizeof(double), hipMemcpyDeviceToHost, e.stream),
+               KMEANS_ERROR_COMPACTNESS);
+  KM_HIP_CHECK(hipStreamSynchronize(e.stream), KMEANS_ERROR_COMPACTNESS);
+  return KMEANS_SUCCESS;
+}
+
 kmeansErrorStatus kmeansSetMiniBatch(kmeansHandle h, long rows) {
   if (!h || rows < 0 || rows > h->e.m) return KMEANS_ERROR_INVALID_ARGUMENT;
   h->e.mb_rows = rows;

@@ -152,3 +152,13 @@ def test_create_rejects_reduced_precision_above_256_dimensions():
     h = ctypes.c_void_p()
     opt = kmeans_amd.Options(1, 0, None, 0, 0)  # BF16
     assert L.kmeansCreate(ctypes.byref(h), 1000, 257, 4, ctypes.byref(opt)) == 14
+
+
+def test_cpp_class_header_compiles():
+    src = '#include "kmeans_amd.hpp"\nint main() { kmeans_amd::kmeans<float> *a = nullptr; ' \
+          'kmeans_amd::kmeans<double> *b = nullptr; return a == nullptr && b == nullptr ? 0 : 1; }\n'
+    import subprocess
+    r = subprocess.run(["g++", "-std=c++14", "-Wall", "-Werror", "-fsyntax-only", "-I",
+                        os.path.join(ROOT, "include"), "-x", "c++", "-"], input=src, text=True,
+                       capture_output=True)
+    assert r.returncode == 0, r.stderr
