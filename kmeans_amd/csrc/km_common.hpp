@@ -91,6 +91,16 @@ struct UpdateWs {
   size_t bytes;
 };
 size_t update_ws_layout(long n, int d, int k, char *base, UpdateWs *ws);
+// Streams, events and the two per-block workspaces of the overlapped update
+// (launch_update_accumulate_2s); S, cnt and compj stay with the caller.
+struct UpdateOverlap {
+  hipStream_t side[2];
+  hipEvent_t start, done[2], fin[2];
+  const UpdateWs *ws[2];
+};
+hipError_t launch_update_accumulate_2s(hipStream_t s, const UpdateOverlap &ov, long n, int d,
+                                       int k, const float *X, const int *labels, const double *q,
+                                       double *S, long long *cnt);
 int sub_block(int k);
 
 // Launchers with a float and a double form (element type of the points and

@@ -36,6 +36,11 @@ Engine::~Engine() {
   hipFree(dcomp);
   hipFree(dq);
   hipFree(dws);
+  hipFree(dws2);
+  for (auto &s : upd_side)
+    if (s) hipStreamDestroy(s);
+  for (auto &e : upd_ev)
+    if (e) hipEventDestroy(e);
   for (auto &e : ev)
     if (e) hipEventDestroy(e);
   for (auto &e : iter_events)
@@ -89,6 +94,14 @@ kmeansErrorStatus Engine::create(long m_, int n_, int k_, const kmeansOptions *o
   const size_t wsb = update_ws_layout(m, n, k, nullptr, nullptr);
   KM_HIP_CHECK(hipMalloc(&dws, wsb), KMEANS_ERROR_BUILDDATA);
   update_ws_layout(m, n, k, dws, &ws);
+  if (m > KM_BLOCK && opt.tile_rows == 0) {  // resident points in several canonical blocks
+    KM_HIP_CHECK(hipMalloc(&dws2, wsb), KMEANS_ERROR_BUILDDATA);
+    update_ws_layout(m, n, k, dws2, &ws2);
+    for (auto &s : upd_side)
+      KM_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), KMEANS_ERROR_BUILDDATA);
+    for (auto &e : upd_ev)
+      KM_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming), KMEANS_ERROR_BUILDDATA);
+  }
   return ensure_comp_slots(64);
 }
 
@@ -217,9 +230,24 @@ kmeansErrorStatus Engine::closestCenters(long r0, long rows) {
 }
 
 kmeansErrorStatus Engine::clusterCenters(long r0, long rows) {
-  KM_HIP_CHECK(launch_update(stream, rows, n, k, dX + r0 * n, dlabels + r0, dq + r0, dC, dC, dCt,
-                             ldc, dnc, dcounts, dempty, ws),
-               KMEANS_ERROR_CLUSTER_CENTERS);
+  if (rows > KM_BLOCK && upd_side[0]) {
+    // several canonical blocks: sorts and chunk sums of alternate blocks on
+    // two side streams, the in-order additions on the engine's stream
+    const UpdateOverlap ov{{upd_side[0], upd_side[1]}, upd_ev[0], {upd_ev[1], upd_ev[2]},
+                           {upd_ev[3], upd_ev[4]}, {&ws, &ws2}};
+    KM_HIP_CHECK(launch_update_zero(stream, n, k, ws.S, ws.cnt, dempty),
+                 KMEANS_ERROR_CLUSTER_CENTERS);
+    KM_HIP_CHECK(launch_update_accumulate_2s(stream, ov, rows, n, k, dX + r0 * n, dlabels + r0,
+                                             dq + r0, ws.S, ws.cnt),
+                 KMEANS_ERROR_CLUSTER_CENTERS);
+    KM_HIP_CHECK(launch_update_finalize(stream, n, k, ws.S, ws.cnt, dC, dC, dCt, ldc, dnc, dcounts,
+                                        dempty, ws.compj),
+                 KMEANS_ERROR_CLUSTER_CENTERS);
+  } else {
+    KM_HIP_CHECK(launch_update(stream, rows, n, k, dX + r0 * n, dlabels + r0, dq + r0, dC, dC, dCt,
+                               ldc, dnc, dcounts, dempty, ws),
+                 KMEANS_ERROR_CLUSTER_CENTERS);
+  }
   if (opt.precision != KM_PREC_FP32) KM_HIP_CHECK(mfma_centers(this), KMEANS_ERROR_CLUSTER_CENTERS);
   return KMEANS_SUCCESS;
 }

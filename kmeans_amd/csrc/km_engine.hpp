@@ -36,6 +36,13 @@ struct Engine {
   double *dq = nullptr;     // m: FP64 squared norms of the points
   char *dws = nullptr;
   UpdateWs ws{};
+  // overlapped update when the points span more than one canonical block:
+  // two side streams (default priority), their events and a second
+  // per-block workspace
+  hipStream_t upd_side[2] = {nullptr, nullptr};
+  hipEvent_t upd_ev[5] = {};
+  char *dws2 = nullptr;
+  UpdateWs ws2{};
   // precision path buffers (allocated on first use)
   void *mfma = nullptr;
   // candidate proposer state (options.proposer != 0)

@@ -933,6 +933,57 @@ hipError_t launch_update_zero(hipStream_t s, int d, int k, double *S, long long 
   return hipGetLastError();
 }
 
+// First-level sums of one canonical block (rows [i0, i0 + nb) of X): the
+// stable counting sort of its labels and the FP64 chunk sums into ws.P.
+// q == nullptr: q is computed for the block into ws.qb.
+template <typename T>
+static hipError_t block_sums(hipStream_t s, long i0, long nb, int d, int k, const T *X,
+                             const int *labels, const double *q, const UpdateWs &ws) {
+  hipError_t e;
+  const int SB = sub_block(k);
+  int nbits = 0;
+  while ((1 << nbits) < k) ++nbits;
+  int Dp = 1;
+  while (Dp < d && Dp < 256) Dp <<= 1;
+  const int G = 256 / Dp < 64 ? 256 / Dp : 64;  // must match k_chunk_sums
+  const int nsb = (int)km_cdiv(nb, SB);
+  const long L = (long)k * nsb;
+  const double *qb = q;  // indexed by row of X
+  if (!q) {
+    if ((e = launch_point_sq(s, nb, d, X + i0 * d, ws.qb)) != hipSuccess) return e;
+    qb = ws.qb - i0;  // perm holds rows of X; shift so qb[i] is row i
+  }
+  hipLaunchKernelGGL(k_hist, dim3(nsb), dim3(256), sizeof(int) * k, s, labels + i0, nb, k, nsb,
+                     SB, ws.hist);
+  if ((e = scan_excl(s, ws.hist, L, ws.tsum, ws.offs)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_scatter, dim3(nsb), dim3(64), sizeof(int) * k, s, labels + i0, i0, nb, k,
+                     nsb, SB, nbits, ws.offs, ws.perm);
+  hipLaunchKernelGGL(k_cluster_setup, dim3(1), dim3(256), 0, s, k, nsb, ws.offs, ws.cstart,
+                     ws.ccount, ws.cq);
+  const long qmax = nb / KM_CHUNK + k + 1;
+  if (sizeof(T) == 4 && d % 4 == 0 && d >= 16 && d <= 256 && ((uintptr_t)X & 15) == 0) {
+    int TPC = 1;
+    while (TPC < d / 4) TPC <<= 1;
+    const int G4 = 256 / TPC < 64 ? 256 / TPC : 64;  // must match k_chunk_sums4
+    hipLaunchKernelGGL(k_chunk_sums4, dim3(km_cdiv(qmax, G4)), dim3(320), 0, s,
+                       reinterpret_cast<const float *>(X), d, TPC, qb,
+                       ws.perm, k, ws.cstart, ws.ccount, ws.cq, ws.P);
+  } else {
+    hipLaunchKernelGGL(k_chunk_sums<T>, dim3(km_cdiv(qmax, G)), dim3(320), 0, s, X, d, Dp, qb,
+                       ws.perm, k, ws.cstart, ws.ccount, ws.cq, ws.P);
+  }
+  return hipGetLastError();
+}
+
+// S[j] += the block's CSUM per centre (the order-dependent step: blocks are
+// added left to right), cnt[j] += its member count.
+static hipError_t block_add(hipStream_t s, int d, int k, double *S, long long *cnt,
+                            const UpdateWs &ws) {
+  hipLaunchKernelGGL(k_block_finalize, dim3(k), dim3(64 * (int)km_cdiv(d + 1 < 256 ? d + 1 : 256, 64)),
+                     0, s, d, k, ws.ccount, ws.cq, ws.P, S, cnt);
+  return hipGetLastError();
+}
+
 // Adds the canonical block sums of rows [0, n) of X (labels and q aligned
 // with X) into S (k x (d+1) FP64: d sums and the q column) and cnt; blocks
 // start at row 0 of X.  q == nullptr: q is computed per block into ws.qb.
@@ -941,43 +992,37 @@ static hipError_t launch_update_accumulate_t(hipStream_t s, long n, int d, int k
                                     const int *labels, const double *q, double *S, long long *cnt,
                                     const UpdateWs &ws) {
   hipError_t e;
-  const int SB = sub_block(k);
-  int nbits = 0;
-  while ((1 << nbits) < k) ++nbits;
-  int Dp = 1;
-  while (Dp < d && Dp < 256) Dp <<= 1;
-  const int G = 256 / Dp < 64 ? 256 / Dp : 64;  // must match k_chunk_sums
   for (long i0 = 0; i0 < n; i0 += KM_BLOCK) {
     const long nb = (n - i0) < KM_BLOCK ? (n - i0) : KM_BLOCK;
-    const int nsb = (int)km_cdiv(nb, SB);
-    const long L = (long)k * nsb;
-    const double *qb = q;  // indexed by row of X
-    if (!q) {
-      if ((e = launch_point_sq(s, nb, d, X + i0 * d, ws.qb)) != hipSuccess) return e;
-      qb = ws.qb - i0;  // perm holds rows of X; shift so qb[i] is row i
-    }
-    hipLaunchKernelGGL(k_hist, dim3(nsb), dim3(256), sizeof(int) * k, s, labels + i0, nb, k, nsb,
-                       SB, ws.hist);
-    if ((e = scan_excl(s, ws.hist, L, ws.tsum, ws.offs)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_scatter, dim3(nsb), dim3(64), sizeof(int) * k, s, labels + i0, i0, nb, k,
-                       nsb, SB, nbits, ws.offs, ws.perm);
-    hipLaunchKernelGGL(k_cluster_setup, dim3(1), dim3(256), 0, s, k, nsb, ws.offs, ws.cstart,
-                       ws.ccount, ws.cq);
-    const long qmax = nb / KM_CHUNK + k + 1;
-    if (sizeof(T) == 4 && d % 4 == 0 && d >= 16 && d <= 256 && ((uintptr_t)X & 15) == 0) {
-      int TPC = 1;
-      while (TPC < d / 4) TPC <<= 1;
-      const int G4 = 256 / TPC < 64 ? 256 / TPC : 64;  // must match k_chunk_sums4
-      hipLaunchKernelGGL(k_chunk_sums4, dim3(km_cdiv(qmax, G4)), dim3(320), 0, s,
-                         reinterpret_cast<const float *>(X), d, TPC, qb,
-                         ws.perm, k, ws.cstart, ws.ccount, ws.cq, ws.P);
-    } else {
-      hipLaunchKernelGGL(k_chunk_sums<T>, dim3(km_cdiv(qmax, G)), dim3(320), 0, s, X, d, Dp, qb, ws.perm,
-                         k, ws.cstart, ws.ccount, ws.cq, ws.P);
-    }
-    hipLaunchKernelGGL(k_block_finalize, dim3(k), dim3(64 * (int)km_cdiv(d + 1 < 256 ? d + 1 : 256, 64)),
-                       0, s, d, k, ws.ccount, ws.cq, ws.P, S, cnt);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = block_sums(s, i0, nb, d, k, X, labels, q, ws)) != hipSuccess) return e;
+    if ((e = block_add(s, d, k, S, cnt, ws)) != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// The same sums with the blocks' sorts and chunk sums on two side streams
+// (block b on side[b & 1] with workspace ws[b & 1]) and the additions into
+// S on s in block order, so one block's latency-bound sort overlaps the
+// other's bandwidth-bound chunk sums; the result is the single-stream one
+// bit for bit.  A side workspace is reused two blocks later, after s has
+// added the earlier block.
+hipError_t launch_update_accumulate_2s(hipStream_t s, const UpdateOverlap &ov, long n, int d,
+                                       int k, const float *X, const int *labels, const double *q,
+                                       double *S, long long *cnt) {
+  hipError_t e;
+  if ((e = hipEventRecord(ov.start, s)) != hipSuccess) return e;
+  for (int i = 0; i < 2; ++i)
+    if ((e = hipStreamWaitEvent(ov.side[i], ov.start, 0)) != hipSuccess) return e;
+  int b = 0;
+  for (long i0 = 0; i0 < n; i0 += KM_BLOCK, ++b) {
+    const long nb = (n - i0) < KM_BLOCK ? (n - i0) : KM_BLOCK;
+    const int i = b & 1;
+    if (b >= 2 && (e = hipStreamWaitEvent(ov.side[i], ov.fin[i], 0)) != hipSuccess) return e;
+    if ((e = block_sums(ov.side[i], i0, nb, d, k, X, labels, q, *ov.ws[i])) != hipSuccess) return e;
+    if ((e = hipEventRecord(ov.done[i], ov.side[i])) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(s, ov.done[i], 0)) != hipSuccess) return e;
+    if ((e = block_add(s, d, k, S, cnt, *ov.ws[i])) != hipSuccess) return e;
+    if ((e = hipEventRecord(ov.fin[i], s)) != hipSuccess) return e;
   }
   return hipSuccess;
 }

@@ -221,6 +221,24 @@ def test_engine_lloyd_config1_bit_exact(orc):
     e.close()
 
 
+def test_engine_update_overlaps_four_blocks(orc):
+    # resident points over four canonical blocks (the last one short): the
+    # blocks' sorts and chunk sums run on two side streams, the additions in
+    # block order; labels, centres and compactness bit for bit
+    n, d, k = 3 * km.KM_BLOCK + 4321, 4, 50
+    X, _, _, _ = orc.mixture(18, n, d, k, -10, 10, 1.0, 1.0)
+    C0 = X[:k].copy()
+    ref = orc.lloyd(X, C0, 2)
+    e = km.KMeans(n, d, k)
+    e.set_data(X)
+    e.set_centers(C0)
+    e.run(2)
+    assert np.array_equal(e.labels(), ref["labels"])
+    assert np.array_equal(e.centers(), ref["centers"])
+    assert e.stats()["compactness"] == ref["comp_hist"][-1]
+    e.close()
+
+
 def test_engine_criterion_stops_like_reference(orc):
     X, _, _, _ = orc.mixture(12, 30_000, 8, 10, -10, 10, 0.7, 0.7)
     C0 = X[:10].copy()
