@@ -125,6 +125,7 @@ def main():
         "assign_pairs_per_s": pairs / (assign_ms * 1e-3),
         "reresolved_per_step": st["reresolved"] / args.steps,
         "rescored_per_step": st["rescored"] / args.steps,
+        "deep_per_step": st["deep"] / args.steps,
         "roofline": {"bound": "mfma",
                      "achieved": achieved / 1e12, "peak": PEAK[args.precision] / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK[args.precision],

@@ -173,6 +173,7 @@ typedef struct kmeansStats {
   double ms_train;            /* grouping + MLP training (once, wall clock)   */
   double train_loss_first;    /* MLP cross-entropy, first and last batch      */
   double train_loss_last;
+  long long deep;             /* points given to the split-BF16 second pass   */
 } kmeansStats;
 
 typedef struct kmeansEngine *kmeansHandle;

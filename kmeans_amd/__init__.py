@@ -64,7 +64,7 @@ class Stats(ctypes.Structure):
                 ("certified_group", ctypes.c_longlong), ("fallback", ctypes.c_longlong),
                 ("candidate_pairs", ctypes.c_longlong), ("proposal_hits", ctypes.c_longlong),
                 ("ms_train", ctypes.c_double), ("train_loss_first", ctypes.c_double),
-                ("train_loss_last", ctypes.c_double)]
+                ("train_loss_last", ctypes.c_double), ("deep", ctypes.c_longlong)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

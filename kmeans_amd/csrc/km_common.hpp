@@ -132,11 +132,15 @@ KM_TYPED_LAUNCHERS(double)
 #undef KM_TYPED_LAUNCHERS
 // rows (nullable): point indices of the m rows; out_pos (nullable): output
 // slot of each row (default: rows[p], else p); few_rows: the rows are a short
-// list (64-point workgroups instead of 128)
+// list (64-point workgroups instead of 128); kbest (nullable, few_rows and
+// no best only): split_rows_cap() zeroed keys that let a list of at most that
+// many rows be searched with the centres split over workgroups (left zeroed
+// afterwards)
 hipError_t launch_assign_f32(hipStream_t s, long m, const int *m_dev, int d, const float *X,
                              const int *rows, int k, const float *Ct, int ldc, const float *nc,
                              int *labels, float *best, const int *out_pos = nullptr,
-                             bool few_rows = false);
+                             bool few_rows = false, unsigned long long *kbest = nullptr);
+long split_rows_cap();
 hipError_t launch_assign_f64(hipStream_t s, long m, int d, const double *X, int k,
                              const double *Ct, int ldc, const double *nc, int *labels);
 hipError_t launch_comp_reduce(hipStream_t s, int k, const double *compj, double *out);

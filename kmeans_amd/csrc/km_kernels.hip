@@ -129,7 +129,21 @@ template <> struct MfmaCfg<double> {
 // points per workgroup (RT row tiles of 16 per wave)
 template <typename T, int RT = MfmaCfg<T>::RT> constexpr int am_bm() { return 4 * 16 * RT; }
 
-template <typename T, int RT>
+// Order-preserving 64-bit key of (score, centre): a larger key is a larger
+// score, or the lower centre index on equal scores, so the maximum key over
+// any partition of the centres is the single pass's (score, label).  Key 0
+// lies below every key of a non-NaN score.
+__device__ __forceinline__ unsigned long long split_key(float v, int j) {
+  const unsigned u = __float_as_uint(v);
+  const unsigned o = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+  return ((unsigned long long)o << 32) | (unsigned long long)(0xFFFFFFFFu - (unsigned)j);
+}
+
+// SPLIT: the work items are (tile, slice of the centre blocks); each item's
+// best (score, centre) per point goes into kbest[point position] by a 64-bit
+// atomic maximum of split_key, and k_split_labels writes the labels.  For
+// short row lists, where whole tiles would leave most CUs idle.
+template <typename T, int RT, bool SPLIT = false>
 __device__ __forceinline__ void assign_mfma_tiles(long M, const int *__restrict__ Mdev, int D,
                                                   const T *__restrict__ X,
                                                   const int *__restrict__ rows, int K,
@@ -137,7 +151,8 @@ __device__ __forceinline__ void assign_mfma_tiles(long M, const int *__restrict_
                                                   const T *__restrict__ nc,
                                                   const int *__restrict__ out_pos,
                                                   int *__restrict__ labels, T *__restrict__ best,
-                                                  long mlo, long mhi) {
+                                                  long mlo, long mhi,
+                                                  unsigned long long *__restrict__ kbest = nullptr) {
   typedef MfmaCfg<T> Cfg;
   typedef typename Cfg::v4 v4;
   constexpr int DS = Cfg::DS, BM = am_bm<T, RT>();
@@ -151,7 +166,18 @@ __device__ __forceinline__ void assign_mfma_tiles(long M, const int *__restrict_
   const long ntiles = (M + BM - 1) / BM;
   const int nblk = (K + 63) / 64;
   const int nsl = (D + DS - 1) / DS;
-  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  // centre-block slices per tile: enough items for one item per workgroup
+  long nslice = 1;
+  if constexpr (SPLIT) {
+    nslice = ((long)gridDim.x + ntiles - 1) / ntiles;
+    if (nslice > nblk) nslice = nblk;
+    if (nslice < 1) nslice = 1;
+  }
+  const long nitems = ntiles * nslice;
+  for (long item = blockIdx.x; item < nitems; item += gridDim.x) {
+    const long tile = item / nslice, slice = item - tile * nslice;
+    const int jb0 = (int)((long)nblk * slice / nslice);
+    const int jb1 = (int)((long)nblk * (slice + 1) / nslice);
     const long p0 = tile * BM + w * (16 * RT);
     long xrow[RT];
     bool xok[RT];
@@ -180,9 +206,9 @@ __device__ __forceinline__ void assign_mfma_tiles(long M, const int *__restrict_
     };
     int buf = 0;
 #pragma unroll
-    for (int i = 0; i < DS / 16; ++i) Bl[0][i][w][threadIdx.x & 63] = fetch(i, 0, 0);
+    for (int i = 0; i < DS / 16; ++i) Bl[0][i][w][threadIdx.x & 63] = fetch(i, jb0, 0);
     __syncthreads();
-    for (int jb = 0; jb < nblk; ++jb) {
+    for (int jb = jb0; jb < jb1; ++jb) {
       v4 acc[RT][AM_CT];
 #pragma unroll
       for (int ct = 0; ct < AM_CT; ++ct) {
@@ -193,7 +219,7 @@ __device__ __forceinline__ void assign_mfma_tiles(long M, const int *__restrict_
       }
       for (int sl = 0; sl < nsl; ++sl) {
         const int d0 = sl * DS, dn = min(DS, D - d0);
-        if (nsl > 1 || jb == 0) {
+        if (nsl > 1 || jb == jb0) {
 #pragma unroll
           for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
@@ -204,7 +230,7 @@ __device__ __forceinline__ void assign_mfma_tiles(long M, const int *__restrict_
         }
         // the next block is fetched one chunk per 16 dimensions into the other
         // buffer, so its loads are in flight during this block's MFMAs
-        const bool nxt = sl + 1 < nsl || jb + 1 < nblk;
+        const bool nxt = sl + 1 < nsl || jb + 1 < jb1;
         const int jbn = sl + 1 < nsl ? jb : jb + 1, sln = sl + 1 < nsl ? sl + 1 : 0;
         const int ns = (dn + 15) / 16;
         v4 fl;
@@ -260,9 +286,13 @@ __device__ __forceinline__ void assign_mfma_tiles(long M, const int *__restrict_
         for (int v = 0; v < 4; ++v) {
           const long pt = p0 + rt * 16 + Cfg::row(g, v);
           if (pt < M) {
-            const long o = out_pos ? (long)out_pos[pt] : rows ? (long)rows[pt] : pt;
-            labels[o] = bi[rt][v];
-            if (best) best[o] = bv[rt][v];
+            if constexpr (SPLIT) {
+              atomicMax(kbest + pt, split_key((float)bv[rt][v], bi[rt][v]));
+            } else {
+              const long o = out_pos ? (long)out_pos[pt] : rows ? (long)rows[pt] : pt;
+              labels[o] = bi[rt][v];
+              if (best) best[o] = bv[rt][v];
+            }
           }
         }
     }
@@ -282,6 +312,32 @@ void k_assign_f32_mfma(long M, const int *__restrict__ Mdev, int D, const float 
                        int *__restrict__ labels, float *__restrict__ best, long mlo, long mhi) {
   assign_mfma_tiles<float, RT>(M, Mdev, D, X, rows, K, Ct, ldc, nc, out_pos, labels, best, mlo,
                                mhi);
+}
+
+// 64-point tiles with the centres split over workgroups (see SPLIT above)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_assign_f32_mfma_split(long M, const int *__restrict__ Mdev, int D,
+                             const float *__restrict__ X, const int *__restrict__ rows, int K,
+                             const float *__restrict__ Ct, int ldc, const float *__restrict__ nc,
+                             long mlo, long mhi, unsigned long long *__restrict__ kbest) {
+  assign_mfma_tiles<float, 1, true>(M, Mdev, D, X, rows, K, Ct, ldc, nc, nullptr, nullptr,
+                                    nullptr, mlo, mhi, kbest);
+}
+
+// Labels from the split search's keys; each key is reset to 0 for the next
+// search.
+__global__ void k_split_labels(long M, const int *__restrict__ Mdev, long mlo, long mhi,
+                               const int *__restrict__ rows, const int *__restrict__ out_pos,
+                               unsigned long long *__restrict__ kbest, int *__restrict__ labels) {
+  if (Mdev) M = *Mdev;
+  if (M < mlo || M > mhi) return;
+  for (long pt = (long)blockIdx.x * blockDim.x + threadIdx.x; pt < M;
+       pt += (long)gridDim.x * blockDim.x) {
+    const unsigned long long key = kbest[pt];
+    kbest[pt] = 0ull;
+    const long o = out_pos ? (long)out_pos[pt] : rows ? (long)rows[pt] : pt;
+    labels[o] = (int)(0xFFFFFFFFu - (unsigned)(key & 0xFFFFFFFFull));
+  }
 }
 
 // one wave per SIMD: the double operands of 128 dimensions stay in registers
@@ -839,16 +895,27 @@ static hipError_t launch_transpose_t(hipStream_t s, int k, int d, const T *C, T 
   return hipGetLastError();
 }
 
+long split_rows_cap() { return (long)am_bm<float, 1>() * 2 * num_cus(); }
+
 hipError_t launch_assign_f32(hipStream_t s, long m, const int *m_dev, int d, const float *X,
                              const int *rows, int k, const float *Ct, int ldc, const float *nc,
-                             int *labels, float *best, const int *out_pos, bool few_rows) {
-  const long cap = (long)am_bm<float, 1>() * 2 * num_cus();  // two 64-point tiles per CU
+                             int *labels, float *best, const int *out_pos, bool few_rows,
+                             unsigned long long *kbest) {
+  const long cap = split_rows_cap();  // two 64-point tiles per CU
   const long ncap = (long)num_cus() * 8;
   if (m <= 0) return hipSuccess;
   if (few_rows && (m_dev != nullptr || m <= cap)) {
     const long t1 = km_cdiv(m < cap ? m : cap, am_bm<float, 1>());
-    hipLaunchKernelGGL(k_assign_f32_mfma<1>, dim3(t1 < ncap ? t1 : ncap), dim3(256), 0, s, m,
-                       m_dev, d, X, rows, k, Ct, ldc, nc, out_pos, labels, best, 0L, cap);
+    if (kbest && !best) {  // one item per workgroup slot: tiles times centre slices
+      const long gs = 2L * num_cus();
+      hipLaunchKernelGGL(k_assign_f32_mfma_split, dim3(gs), dim3(256), 0, s, m, m_dev, d, X,
+                         rows, k, Ct, ldc, nc, 0L, cap, kbest);
+      hipLaunchKernelGGL(k_split_labels, dim3(km_cdiv(m < cap ? m : cap, 256)), dim3(256), 0, s,
+                         m, m_dev, 0L, cap, rows, out_pos, kbest, labels);
+    } else {
+      hipLaunchKernelGGL(k_assign_f32_mfma<1>, dim3(t1 < ncap ? t1 : ncap), dim3(256), 0, s, m,
+                         m_dev, d, X, rows, k, Ct, ldc, nc, out_pos, labels, best, 0L, cap);
+    }
     if (m_dev == nullptr && m <= cap) return hipGetLastError();
   }
   const long t2 = km_cdiv(m, am_bm<float, 2>());

@@ -81,6 +81,7 @@ struct MfmaState {
   int *nflag = nullptr;     // per iteration slot
   int *flags2 = nullptr;    // m: points left open by the deep pass
   int *nflag2 = nullptr;    // per iteration slot
+  unsigned long long *kbest = nullptr;  // split_rows_cap() keys of the FP32 search, zeroed
   int nflag_slots = 0;
   bool x_ready = false;
   bool c_ready = false;
@@ -709,6 +710,7 @@ void mfma_free(Engine *e) {
     hipFree(s->biasb);
   }
   hipFree(s->nflag2);
+  hipFree(s->kbest);
   delete s;
   e->mfma = nullptr;
 }
@@ -764,6 +766,9 @@ static hipError_t ensure_state(Engine *e) {
     return r;
   if ((r = hipMemsetAsync(s->nflag2, 0, sizeof(int) * s->nflag_slots, e->stream)) != hipSuccess)
     return r;
+  const size_t kb = sizeof(unsigned long long) * (size_t)split_rows_cap();
+  if ((r = hipMalloc(&s->kbest, kb)) != hipSuccess) return r;
+  if ((r = hipMemsetAsync(s->kbest, 0, kb, e->stream)) != hipSuccess) return r;
   return hipMemsetAsync(s->nflag, 0, sizeof(int) * s->nflag_slots, e->stream);
 }
 
@@ -1040,7 +1045,7 @@ hipError_t mfma_reresolve_view(Engine *e, const float *X, const uint16_t *Xh, co
 #undef DEEP
   if ((r = hipGetLastError()) != hipSuccess) return r;
   return launch_assign_f32(e->stream, rows, s->nflag2 + e->cur_slot, e->n, X, s->flags2, e->k,
-                           e->dCt, e->ldc, e->dnc, lab, nullptr, nullptr, true);
+                           e->dCt, e->ldc, e->dnc, lab, nullptr, nullptr, true, s->kbest);
 }
 
 const uint16_t *mfma_tile_xh(Engine *e) { return st(e)->Xh; }
@@ -1067,6 +1072,11 @@ void mfma_collect(Engine *e, int nslots) {
     e->stats.reresolved += v[i];
     e->last_reresolved = v[i];
   }
+  if (hipMemcpyAsync(v.data(), s->nflag, sizeof(int) * n, hipMemcpyDeviceToHost, e->stream) !=
+      hipSuccess)
+    return;
+  if (hipStreamSynchronize(e->stream) != hipSuccess) return;
+  for (int i = 0; i < n; ++i) e->stats.deep += v[i];
   std::vector<int> w((size_t)n * NC_SLOT);
   if (hipMemcpyAsync(w.data(), s->ncand, sizeof(int) * w.size(), hipMemcpyDeviceToHost, e->stream) !=
       hipSuccess)

@@ -75,6 +75,30 @@ def test_mfma_adversarial_near_ties(orc, prec):
     e.close()
 
 
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_mfma_fp32_search_split_over_centres_ties(orc, prec):
+    # every centre four times (indices j, j + 100, j + 200, j + 300): no
+    # reduced-precision certificate holds, so every point goes to the FP32
+    # search, whose centre slices each see a different copy; the label must
+    # be the lowest index of the best score, as in the single pass
+    print(f"split ties {prec}", flush=True)
+    rng = np.random.default_rng(11)
+    d, kb = 24, 100
+    base = rng.standard_normal((kb, d)).astype(np.float32) * 3
+    C0 = np.concatenate([base] * 4)
+    X = (base[rng.integers(0, kb, 5000)] + rng.standard_normal((5000, d)) * 0.5).astype(np.float32)
+    ref = orc.assign(X, C0)
+    assert ref.max() < kb
+    e = km.KMeans(X.shape[0], d, C0.shape[0], precision=prec)
+    e.set_data(X)
+    e.set_centers(C0)
+    e.run(1)
+    assert np.array_equal(e.labels(), ref)
+    st = e.stats()
+    assert st["reresolved"] == X.shape[0]
+    e.close()
+
+
 def test_mfma_large_magnitudes_fp16_scaling(orc):
     # values far outside FP16 range must be scaled, not overflow
     X, _, _, _ = orc.mixture(41, 4000, 24, 20, -5, 5, 1.0, 1.0)
