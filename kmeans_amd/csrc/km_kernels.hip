@@ -169,6 +169,7 @@ __device__ __forceinline__ void assign_mfma_tiles(long M, const int *__restrict_
   // centre-block slices per tile: enough items for one item per workgroup
   long nslice = 1;
   if constexpr (SPLIT) {
+    if (ntiles == 0) return;
     nslice = ((long)gridDim.x + ntiles - 1) / ntiles;
     if (nslice > nblk) nslice = nblk;
     if (nslice < 1) nslice = 1;
