@@ -35,6 +35,7 @@ Engine::~Engine() {
   hipFree(dempty);
   hipFree(dcomp);
   hipFree(dq);
+  hipFree(dxn);
   hipFree(dws);
   hipFree(dws2);
   for (auto &s : upd_side)
@@ -139,6 +140,11 @@ kmeansErrorStatus Engine::buildData(const float *data, int on_device) {
   // per-point FP64 squared norms, used by the compactness moments
   if (!dq) KM_HIP_CHECK(hipMalloc(&dq, sizeof(double) * (size_t)m), KMEANS_ERROR_BUILDDATA);
   KM_HIP_CHECK(launch_point_sq(stream, m, n, dX, dq), KMEANS_ERROR_BUILDDATA);
+  // the GEMM path's row norms, once per data set (k_row_norms, as per tile)
+  if (opt.full_gemm) {
+    if (!dxn) KM_HIP_CHECK(hipMalloc(&dxn, sizeof(float) * (size_t)m), KMEANS_ERROR_BUILDDATA);
+    KM_HIP_CHECK(launch_row_norms(stream, m, n, dX, dxn), KMEANS_ERROR_BUILDDATA);
+  }
   KM_HIP_CHECK(hipStreamSynchronize(stream), KMEANS_ERROR_BUILDDATA);
   if (mfma) mfma_invalidate(this);
   if (opt.precision != KM_PREC_FP32) KM_HIP_CHECK(mfma_prepare(this), KMEANS_ERROR_BUILDDATA);
@@ -218,7 +224,7 @@ kmeansErrorStatus Engine::reset() { return KMEANS_SUCCESS; }
 kmeansErrorStatus Engine::closestCenters(long r0, long rows) {
   hipError_t e;
   if (opt.full_gemm) {
-    e = gemm_assign(this, dX + r0 * n, rows, dlabels + r0);
+    e = gemm_assign(this, dX + r0 * n, dxn ? dxn + r0 : nullptr, rows, dlabels + r0);
   } else if (opt.precision == KM_PREC_FP32) {
     e = launch_assign_f32(stream, rows, nullptr, n, dX + r0 * n, nullptr, k, dCt, ldc, dnc,
                           dlabels + r0, nullptr);
@@ -476,7 +482,8 @@ kmeansErrorStatus Engine::stream_iteration(int slot) {
     KM_HIP_CHECK(hipStreamWaitEvent(stream, ev_loaded[b], 0), KMEANS_ERROR_COPY_TILE);
     KM_HIP_CHECK(launch_point_sq(stream, rows, n, dXt[b], dqt[b]), KMEANS_ERROR_COPY_TILE);
     if (opt.full_gemm) {
-      KM_HIP_CHECK(gemm_assign(this, dXt[b], rows, dlabels + r0), KMEANS_ERROR_CLOSEST_CENTERS);
+      KM_HIP_CHECK(gemm_assign(this, dXt[b], nullptr, rows, dlabels + r0),
+                   KMEANS_ERROR_CLOSEST_CENTERS);
     } else if (opt.precision == KM_PREC_FP32) {
       KM_HIP_CHECK(launch_assign_f32(stream, rows, nullptr, n, dXt[b], nullptr, k, dCt, ldc, dnc,
                                      dlabels + r0, nullptr),

@@ -34,6 +34,7 @@ struct Engine {
   int *dempty = nullptr;
   double *dcomp = nullptr;  // one per iteration slot
   double *dq = nullptr;     // m: FP64 squared norms of the points
+  float *dxn = nullptr;     // m: FP32 row norms for the library-GEMM path (resident data)
   char *dws = nullptr;
   UpdateWs ws{};
   // overlapped update when the points span more than one canonical block:

@@ -31,6 +31,7 @@ struct EngineD {
   hipStream_t stream = nullptr;
   double *dX = nullptr, *dC = nullptr, *dCt = nullptr, *dnc = nullptr, *dq = nullptr;
   double *dcomp = nullptr;
+  double *dxn = nullptr;  // row norms for the library-GEMM path
   int *dlabels = nullptr, *dempty = nullptr, *didx = nullptr;
   long long *dcounts = nullptr;
   char *dws = nullptr;
@@ -45,6 +46,7 @@ struct EngineD {
     hipFree(dCt);
     hipFree(dnc);
     hipFree(dq);
+    hipFree(dxn);
     hipFree(dcomp);
     hipFree(dlabels);
     hipFree(dempty);
@@ -127,6 +129,10 @@ kmeansErrorStatus compute_d(const double *data, int m, int n, int k, double crit
     mb = (long)(miniBatchFraction * m);
     if (mb < k) mb = std::min<long>(k, m);
   }
+  if (fullGemm) {  // the data's row norms, once (k_row_norms, as per tile)
+    KM_HIP_CHECK(hipMalloc(&E.dxn, sizeof(double) * (size_t)m), KMEANS_ERROR_BUILDDATA);
+    KM_HIP_CHECK(launch_row_norms(s, m, n, E.dX, E.dxn), KMEANS_ERROR_BUILDDATA);
+  }
   const bool sync_each = criterion >= 0.0;
   double prev = 0.0, relErr = 1.0;
   int it = 0;
@@ -139,8 +145,8 @@ kmeansErrorStatus compute_d(const double *data, int m, int n, int k, double crit
       rows = std::min(mb, (long)m - r0);
     }
     if (fullGemm)
-      KM_HIP_CHECK(gemm_assign(&E.gemm, s, E.dX + r0 * n, rows, n, k, E.dCt, E.ldc, E.dnc,
-                               E.dlabels + r0),
+      KM_HIP_CHECK(gemm_assign(&E.gemm, s, E.dX + r0 * n, E.dxn + r0, rows, n, k, E.dCt, E.ldc,
+                               E.dnc, E.dlabels + r0),
                    KMEANS_ERROR_CLOSEST_CENTERS);
     else
       KM_HIP_CHECK(launch_assign_f64(s, rows, n, E.dX + r0 * n, k, E.dCt, E.ldc, E.dnc,

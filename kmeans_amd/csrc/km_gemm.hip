@@ -23,7 +23,7 @@ namespace km {
 struct GemmAssign {
   rocblas_handle h = nullptr;
   void *P = nullptr;   // tile x k product
-  void *xn = nullptr;  // tile row norms
+  void *xn = nullptr;  // tile row norms (when the caller has none)
   long tile = 0;
 };
 
@@ -80,20 +80,22 @@ static rocblas_status gemm_call(rocblas_handle h, int k, long tr, int n, const d
 }
 
 template <typename T>
-static hipError_t gemm_assign_t(GemmAssign **pg, hipStream_t s, const T *X, long rows, int n,
-                                int k, const T *Ct, int ldc, const T *nc, int *labels) {
+static hipError_t gemm_assign_t(GemmAssign **pg, hipStream_t s, const T *X, const T *xn_rows,
+                                long rows, int n, int k, const T *Ct, int ldc, const T *nc,
+                                int *labels) {
   hipError_t err;
   if (rows <= 0) return hipSuccess;
   if ((err = gemm_setup(pg, s, rows, k, sizeof(T))) != hipSuccess) return err;
   GemmAssign *g = *pg;
-  T *P = (T *)g->P, *xn = (T *)g->xn;
+  T *P = (T *)g->P;
   for (long r = 0; r < rows; r += g->tile) {
     const long tr = std::min(g->tile, rows - r);
     const T *Xr = X + r * n;
     // P (k x tr, column-major) = Ct^T (k x n; Ct is n x ldc row-major) * X^T
     if (gemm_call(g->h, k, tr, n, Ct, ldc, Xr, P) != rocblas_status_success)
       return hipErrorLaunchFailure;
-    if ((err = launch_row_norms(s, tr, n, Xr, xn)) != hipSuccess) return err;
+    const T *xn = xn_rows ? xn_rows + r : (const T *)g->xn;
+    if (!xn_rows && (err = launch_row_norms(s, tr, n, Xr, (T *)g->xn)) != hipSuccess) return err;
     // rows with no value below FLT_MAX (NaN data) get label 0, a valid centre
     if ((err = launch_row_transform_min(s, 0, tr, k, xn, nc, P, labels + r, 0)) != hipSuccess)
       return err;
@@ -101,18 +103,19 @@ static hipError_t gemm_assign_t(GemmAssign **pg, hipStream_t s, const T *X, long
   return hipSuccess;
 }
 
-hipError_t gemm_assign(GemmAssign **g, hipStream_t s, const float *X, long rows, int n, int k,
-                       const float *Ct, int ldc, const float *nc, int *labels) {
-  return gemm_assign_t(g, s, X, rows, n, k, Ct, ldc, nc, labels);
+hipError_t gemm_assign(GemmAssign **g, hipStream_t s, const float *X, const float *xn, long rows,
+                       int n, int k, const float *Ct, int ldc, const float *nc, int *labels) {
+  return gemm_assign_t(g, s, X, xn, rows, n, k, Ct, ldc, nc, labels);
 }
-hipError_t gemm_assign(GemmAssign **g, hipStream_t s, const double *X, long rows, int n, int k,
-                       const double *Ct, int ldc, const double *nc, int *labels) {
-  return gemm_assign_t(g, s, X, rows, n, k, Ct, ldc, nc, labels);
+hipError_t gemm_assign(GemmAssign **g, hipStream_t s, const double *X, const double *xn,
+                       long rows, int n, int k, const double *Ct, int ldc, const double *nc,
+                       int *labels) {
+  return gemm_assign_t(g, s, X, xn, rows, n, k, Ct, ldc, nc, labels);
 }
 
-hipError_t gemm_assign(Engine *e, const float *X, long rows, int *labels) {
-  return gemm_assign_t((GemmAssign **)&e->gemm, e->stream, X, rows, e->n, e->k, e->dCt, e->ldc,
-                       e->dnc, labels);
+hipError_t gemm_assign(Engine *e, const float *X, const float *xn, long rows, int *labels) {
+  return gemm_assign_t((GemmAssign **)&e->gemm, e->stream, X, xn, rows, e->n, e->k, e->dCt,
+                       e->ldc, e->dnc, labels);
 }
 
 void gemm_free(Engine *e) {

@@ -65,6 +65,25 @@ def test_gemm_assign_many_product_tiles(orc):
     check_labels(orc, X[idx], C, lab[idx])
 
 
+def test_gemm_resident_row_norms_equal_streamed(orc):
+    # resident data: the row norms are computed once per data set; streamed
+    # tiles compute theirs per tile; same kernel, so identical results
+    # (streamed tiles are whole canonical blocks: two tiles here, one short)
+    n, d, k = km.KM_BLOCK + 50_000, 8, 64
+    X, _, _, _ = orc.mixture(61, n, d, k, -6, 6, 0.7, 1.5)
+    C0 = X[:k].copy()
+    out = {}
+    for tr in (0, km.KM_BLOCK):
+        e = km.KMeans(n, d, k, full_gemm=True, tile_rows=tr)
+        e.set_data(X)
+        e.set_centers(C0)
+        e.run(3)
+        out[tr] = (e.labels(), e.centers())
+        e.close()
+    assert np.array_equal(out[0][0], out[km.KM_BLOCK][0])
+    assert np.array_equal(out[0][1], out[km.KM_BLOCK][1])
+
+
 def test_compute_kmeans_f_full_gemm_equals_fused_on_separated_data(orc):
     # centres start near the component means: no near-ties in any iteration
     X, _, means, _ = orc.mixture(61, 40_000, 24, 16, -20, 20, 0.5, 0.5)
