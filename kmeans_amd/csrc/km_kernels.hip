@@ -166,11 +166,11 @@ __device__ __forceinline__ void assign_mfma_tiles(long M, const int *__restrict_
   const long ntiles = (M + BM - 1) / BM;
   const int nblk = (K + 63) / 64;
   const int nsl = (D + DS - 1) / DS;
-  // centre-block slices per tile: enough items for one item per workgroup
+  // centre-block slices per tile: as many items as workgroups, at most
   long nslice = 1;
   if constexpr (SPLIT) {
     if (ntiles == 0) return;
-    nslice = ((long)gridDim.x + ntiles - 1) / ntiles;
+    nslice = (long)gridDim.x / ntiles;
     if (nslice > nblk) nslice = nblk;
     if (nslice < 1) nslice = 1;
   }
