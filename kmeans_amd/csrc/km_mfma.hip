@@ -490,7 +490,7 @@ __global__ __launch_bounds__(256) void k_deep_mfma(
     const float *__restrict__ X, int D, int K, int Kp, const uint16_t *__restrict__ Ch,
     const uint16_t *__restrict__ Cl, const float *__restrict__ bias, const float *__restrict__ C,
     const float *__restrict__ nc, const float *__restrict__ bnd, const float2 *__restrict__ xnb,
-    int *__restrict__ labels, int *__restrict__ list2, int *__restrict__ nlist2) {
+    float big, int *__restrict__ labels, int *__restrict__ list2, int *__restrict__ nlist2) {
   const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31;
   const long total = *nlist;
   const long wstride = (long)gridDim.x * 128;
@@ -504,8 +504,9 @@ __global__ __launch_bounds__(256) void k_deep_mfma(
   // read touches fall in different banks)
   constexpr int CPR = Dp / 8;
   __shared__ u32x4_t ts[2][2 * CH];
+  __shared__ float tb[2][MF_TILE];  // the tile's biases
   const int nT = Kp / MF_TILE;
-  auto fetch = [&](int t, u32x4_t *reg) {
+  auto fetch = [&](int t, u32x4_t *reg, float &rb) {
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const int c = threadIdx.x + 256 * i;
@@ -515,6 +516,7 @@ __global__ __launch_bounds__(256) void k_deep_mfma(
         reg[i] = *reinterpret_cast<const u32x4_t *>(src + (long)t * MF_TILE * Dp + (long)cc * 8);
       }
     }
+    if (threadIdx.x < MF_TILE) rb = bias[t * MF_TILE + threadIdx.x];
   };
   // the loop over point groups is uniform across the workgroup (barriers)
   for (long bb = (long)blockIdx.x * 128; bb < total; bb += wstride) {
@@ -546,12 +548,13 @@ __global__ __launch_bounds__(256) void k_deep_mfma(
         xl4[ks] = u32x4_t{wl[0], wl[1], wl[2], wl[3]};
       }
     }
-    float bv[4];
-    int bi[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) { bv[q] = -INFINITY; bi[q] = -1; }
+    // per lane half: the four largest packed scores (register id in the low
+    // 4 bits, as in k_assign_mfma) and the tiles of the first three
+    float b1 = -FLT_MAX, b2 = -FLT_MAX, b3 = -FLT_MAX, b4 = -FLT_MAX;
+    int t1 = 0, t2 = 0, t3 = 0;
     u32x4_t pre[PER];
-    fetch(0, pre);
+    float preb = 0.0f;
+    fetch(0, pre, preb);
     for (int t = 0; t < nT; ++t) {
       const int buf = t & 1;
 #pragma unroll
@@ -563,11 +566,12 @@ __global__ __launch_bounds__(256) void k_deep_mfma(
           ts[buf][part * CH + r * CPR + (q + r) % CPR] = pre[i];
         }
       }
+      if (threadIdx.x < MF_TILE) tb[buf][threadIdx.x] = preb;
       __syncthreads();
-      if (t + 1 < nT) fetch(t + 1, pre);
+      if (t + 1 < nT) fetch(t + 1, pre, preb);
       f32x16_t acc;
 #pragma unroll
-      for (int r = 0; r < 16; ++r) acc[r] = bias[t * MF_TILE + mf_row(r, h)];
+      for (int r = 0; r < 16; ++r) acc[r] = tb[buf][mf_row(r, h)];
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks) {
         const int q = col * CPR + (ks * 2 + h + col) % CPR;
@@ -580,27 +584,46 @@ __global__ __launch_bounds__(256) void k_deep_mfma(
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, al),
                                                        __builtin_bit_cast(bf16x8_t, xh4[ks]), acc, 0, 0, 0);
       }
+      const unsigned o1 = __float_as_uint(b1), o2 = __float_as_uint(b2), o3 = __float_as_uint(b3);
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        float v = acc[r];
-        int vi = t * MF_TILE + mf_row(r, h);
-        if (v > bv[3]) {
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            if (v > bv[q]) {
-              const float tv = bv[q];
-              const int ti = bi[q];
-              bv[q] = v; bi[q] = vi; v = tv; vi = ti;
-            }
-          }
-        }
+        const float pv = __uint_as_float((__float_as_uint(acc[r]) & 0xFFFFFFF0u) | (unsigned)r);
+        const float n4 = __builtin_amdgcn_fmed3f(b3, b4, pv);
+        const float n3 = __builtin_amdgcn_fmed3f(b2, b3, pv);
+        const float n2 = __builtin_amdgcn_fmed3f(b1, b2, pv);
+        b1 = __builtin_amdgcn_fmed3f(b1, pv, big);
+        b2 = n2;
+        b3 = n3;
+        b4 = n4;
       }
+      // an entry is an earlier entry (entries only move down) or from tile t;
+      // bitwise-equal entries make this ambiguous and are flagged below
+      const unsigned u1 = __float_as_uint(b1), u2 = __float_as_uint(b2), u3 = __float_as_uint(b3);
+      const int n1 = u1 == o1 ? t1 : t;
+      const int n2 = u2 == o1 ? t1 : u2 == o2 ? t2 : t;
+      const int n3 = u3 == o1 ? t1 : u3 == o2 ? t2 : u3 == o3 ? t3 : t;
+      t1 = n1;
+      t2 = n2;
+      t3 = n3;
     }
-    // merge the other half's four (disjoint centres); use half 0's list
+    // unpacked lists: the first three with their centres, the fourth (value
+    // only) as the bound
+    float bv[4] = {b1, b2, b3, b4};
+    int bi[4] = {t1 * MF_TILE + mf_row((int)(__float_as_uint(b1) & 15u), h),
+                 t2 * MF_TILE + mf_row((int)(__float_as_uint(b2) & 15u), h),
+                 t3 * MF_TILE + mf_row((int)(__float_as_uint(b3) & 15u), h), -1};
+    // merge the other half's four (disjoint centres), read before any update
+    float ov[4];
+    int oi[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      float v = __shfl_xor(bv[q], 32);
-      int vi = __shfl_xor(bi[q], 32);
+      ov[q] = __shfl_xor(bv[q], 32);
+      oi[q] = __shfl_xor(bi[q], 32);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      float v = ov[q];
+      int vi = oi[q];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         if (v > bv[u]) {
@@ -610,6 +633,13 @@ __global__ __launch_bounds__(256) void k_deep_mfma(
         }
       }
     }
+    // two equal packed values among the four (their tiles cannot be told
+    // apart) or a repeated candidate send the point on
+    const float lo = -FLT_MAX * 0.5f;
+    const bool tie = (bv[1] > lo && __float_as_uint(bv[0]) == __float_as_uint(bv[1])) ||
+                     (bv[2] > lo && __float_as_uint(bv[1]) == __float_as_uint(bv[2])) ||
+                     (bv[3] > lo && __float_as_uint(bv[2]) == __float_as_uint(bv[3])) ||
+                     bi[0] == bi[1] || bi[1] == bi[2] || bi[0] == bi[2];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       bv[q] = __shfl(bv[q], col);
@@ -670,11 +700,11 @@ __global__ __launch_bounds__(256) void k_deep_mfma(
       const float mfe = gmf * (0.5f * ncmax + (1.0f + ub) * (1.0f + ub) * (1.0f + 2.0f * ub) * xc);
       const float fpe = g32 * (0.5f * ncmax + xc);
       const float E = (split + mfe + fpe) * 1.001f + 1e-30f * Df;
-      const bool has4 = bi[3] >= 0 && bi[3] < K && bv[3] > -FLT_MAX * 0.5f;
+      const bool has4 = bv[3] > -FLT_MAX * 0.5f;
       const float a4 = has4 ? bv[3] : -INFINITY;
       const float slack = (fabsf(bv[0]) + (has4 ? fabsf(a4) : 0.0f)) * 1.6e-5f;
       const float thr = has4 ? a4 + E + slack : -INFINITY;
-      if (lab >= 0 && sb > thr && sb > -INFINITY) labels[p] = lab;
+      if (!tie && lab >= 0 && sb > thr && sb > -INFINITY) labels[p] = lab;
       else fall = true;
     }
     const uint64_t mk = __ballot(fall);
@@ -1033,7 +1063,7 @@ hipError_t mfma_reresolve_view(Engine *e, const float *X, const uint16_t *Xh, co
 #define DEEP(KS)                                                                                 \
   hipLaunchKernelGGL(k_deep_mfma<KS>, dim3(g), dim3(256), 0, e->stream, s->nflag + e->cur_slot,  \
                      s->flags, X, e->n, e->k, s->Kp, s->Chb, s->Clb, s->biasb, e->dC, e->dnc,     \
-                     s->bnd, xn, lab, s->flags2, s->nflag2 + e->cur_slot)
+                     s->bnd, xn, FLT_MAX, lab, s->flags2, s->nflag2 + e->cur_slot)
   switch (s->KS) {
     case 1: DEEP(1); break;
     case 2: DEEP(2); break;
