@@ -476,13 +476,18 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
 // just those points in split BF16: x = xh + xl and c = ch + cl (the low parts
 // are the BF16 roundings of the residuals), scores xh.ch + xh.cl + xl.ch by
 // three MFMAs, whose operand error is at most u^2 (3 + 2u) sum|x_d c_d|
-// instead of (2u + u^2) sum|x_d c_d|.  The pass keeps the four largest
-// approximate scores, computes the exact FP32 chains of the first three (half
-// 0 of the wave two, half 1 one, each reading its point's row once), and
-// labels the point when the best of them (lowest index on equal scores)
-// exceeds the fourth + E; the rest go to the FP32 full search.  One wave
-// handles 32 listed points; centre fragments come from the L2-resident
-// arrays.  Both engines: the parts are unscaled BF16 (for FP16 engines the
+// instead of (2u + u^2) sum|x_d c_d|.  Each lane half keeps the four largest
+// approximate scores of its centre rows, packed with the register id in the
+// low 4 bits as in k_assign_mfma (four v_med3 per score), and the tiles of
+// the first three; the halves' lists are merged (each reads the other's
+// original list), and two equal packed values or a repeated candidate among
+// the merged four send the point on, since their tiles cannot be told apart.
+// The pass computes the exact FP32 chains of the first three (half 0 of the
+// wave two, half 1 one, each reading its point's row once) and labels the
+// point when the best of them (lowest index on equal scores) exceeds the
+// fourth + E + the packing slack; the rest go to the FP32 full search.  One
+// wave handles 32 listed points; centre tiles and their biases are staged
+// in LDS.  Both engines: the parts are unscaled BF16 (for FP16 engines the
 // BF16 centre copies Chb/Clb and biases biasb), computed from the FP32 rows.
 template <int KS>
 __global__ __launch_bounds__(256) void k_deep_mfma(
