@@ -99,6 +99,29 @@ def test_mfma_fp32_search_split_over_centres_ties(orc, prec):
     e.close()
 
 
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_mfma_deep_pass_twins_in_one_tile_row(orc, prec):
+    # centres j and j + 128 are near-twins in the same row of two different
+    # 32-centre tiles, so their reduced-precision scores (split BF16 in the
+    # deep pass) agree to within the packed low bits; the deep pass must
+    # either tell them apart by their exact FP32 chains or send the point on
+    print(f"deep twins {prec}", flush=True)
+    rng = np.random.default_rng(23)
+    d, kb = 64, 128
+    base = (rng.standard_normal((kb, d)) * 3).astype(np.float32)
+    twin = base + (rng.standard_normal((kb, d)) * 2e-5).astype(np.float32)
+    C0 = np.concatenate([base, twin]).astype(np.float32)
+    X = (base[rng.integers(0, kb, 20000)] + rng.standard_normal((20000, d)) * 0.4).astype(np.float32)
+    ref = orc.assign(X, C0)
+    e = km.KMeans(X.shape[0], d, C0.shape[0], precision=prec)
+    e.set_data(X)
+    e.set_centers(C0)
+    e.run(1)
+    assert np.array_equal(e.labels(), ref)
+    assert e.stats()["deep"] > 0
+    e.close()
+
+
 def test_mfma_large_magnitudes_fp16_scaling(orc):
     # values far outside FP16 range must be scaled, not overflow
     X, _, _, _ = orc.mixture(41, 4000, 24, 20, -5, 5, 1.0, 1.0)
