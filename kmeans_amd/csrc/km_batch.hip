@@ -326,25 +326,8 @@ __global__ __launch_bounds__(256) void k_batch_assign_bf16(
       const long pr = p0 + col;
       const bool ok = pr < rows;
       const float *xr = X + (base + (ok ? pr : 0)) * d;
-      // the point's FP32 row in registers (zero past d), its BF16 fragments
-      // (this half's 8 dimensions per k-step) and its ||x||^2 part
-      float xv[Dp];
-      const bool vec = (d & 3) == 0;
-#pragma unroll
-      for (int q4 = 0; q4 < Dp / 4; ++q4) {
-        float4 t4 = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (ok && 4 * q4 < d) {
-          if (vec) {
-            t4 = reinterpret_cast<const float4 *>(xr)[q4];
-          } else {
-            t4.x = xr[4 * q4];
-            if (4 * q4 + 1 < d) t4.y = xr[4 * q4 + 1];
-            if (4 * q4 + 2 < d) t4.z = xr[4 * q4 + 2];
-            if (4 * q4 + 3 < d) t4.w = xr[4 * q4 + 3];
-          }
-        }
-        xv[4 * q4] = t4.x; xv[4 * q4 + 1] = t4.y; xv[4 * q4 + 2] = t4.z; xv[4 * q4 + 3] = t4.w;
-      }
+      // the point's BF16 fragments (this half's 8 dimensions per k-step) and
+      // its ||x||^2 part
       b_u32x4 bf[KS];
       float sq = 0.0f, se = 0.0f;
 #pragma unroll
@@ -378,7 +361,7 @@ __global__ __launch_bounds__(256) void k_batch_assign_bf16(
       // of the first two
       float b1 = -FLT_MAX, b2 = -FLT_MAX, b3 = -FLT_MAX;
       int t1 = 0, t2 = 0;
-      for (int t = 0; t < Kp / 32; ++t) {
+      auto score_tile = [&](int t) {
         b_f32x16 acc;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {  // rows 8q + 4h .. +3: one 16-byte read
@@ -391,6 +374,9 @@ __global__ __launch_bounds__(256) void k_batch_assign_bf16(
           acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(b_bf16x8, a),
                                                          __builtin_bit_cast(b_bf16x8, bf[ks]), acc, 0, 0, 0);
         }
+        return acc;
+      };
+      auto epilogue = [&](int t, const b_f32x16 &acc) {
         const float o1 = b1, o2 = b2;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
@@ -405,6 +391,18 @@ __global__ __launch_bounds__(256) void k_batch_assign_bf16(
         const int ot1 = t1, ot2 = t2;
         t2 = (u2 == __float_as_uint(o1)) ? ot1 : (u2 == __float_as_uint(o2)) ? ot2 : t;
         t1 = (u1 != __float_as_uint(o1)) ? t : ot1;
+      };
+      // the matrix core scores tile t+1 while the VALU runs tile t's
+      // epilogue; tiles past the end are clamped to the last one and their
+      // scores are never read
+      const int nT = Kp / 32;
+      b_f32x16 A = score_tile(0), B;
+      for (int t = 0; t < nT; t += 2) {
+        B = score_tile(min(t + 1, nT - 1));
+        epilogue(t, A);
+        if (t + 1 >= nT) break;
+        A = score_tile(min(t + 2, nT - 1));
+        epilogue(t + 1, B);
       }
       const bool real1 = b1 > -FLT_MAX, real2 = b2 > -FLT_MAX;
       int i1 = real1 ? t1 * 32 + b_row((int)(__float_as_uint(b1) & 15u), h) : -1;
@@ -437,6 +435,25 @@ __global__ __launch_bounds__(256) void k_batch_assign_bf16(
         i1 = __shfl(k1, col);
         i2 = __shfl(k2, col);
         tie = tie || (__shfl((int)mt, col) != 0);
+      }
+      // the point's FP32 row in registers (zero past d), loaded only now so
+      // that it is not held across the tile loop
+      float xv[Dp];
+      const bool vec = (d & 3) == 0;
+#pragma unroll
+      for (int q4 = 0; q4 < Dp / 4; ++q4) {
+        float4 t4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (ok && 4 * q4 < d) {
+          if (vec) {
+            t4 = reinterpret_cast<const float4 *>(xr)[q4];
+          } else {
+            t4.x = xr[4 * q4];
+            if (4 * q4 + 1 < d) t4.y = xr[4 * q4 + 1];
+            if (4 * q4 + 2 < d) t4.z = xr[4 * q4 + 2];
+            if (4 * q4 + 3 < d) t4.w = xr[4 * q4 + 3];
+          }
+        }
+        xv[4 * q4] = t4.x; xv[4 * q4 + 1] = t4.y; xv[4 * q4 + 2] = t4.z; xv[4 * q4 + 3] = t4.w;
       }
       // exact chains of the two candidates
       const int cj = h ? i2 : i1;
