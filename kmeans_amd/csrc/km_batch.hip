@@ -51,6 +51,7 @@ struct Batch {
   long *xoff = nullptr;
   int *nvec = nullptr;
   float *C = nullptr, *nc = nullptr;
+  float2 *bnd = nullptr;  // per instance: max ||c||^2 and max ||c - bf16(c)||^2 (raw FP32 chains)
   int *labels = nullptr, *perm = nullptr;
   double *q = nullptr, *part = nullptr, *comp = nullptr;
   long *choff = nullptr;     // first chunk slot of each instance
@@ -241,8 +242,9 @@ template <int KS>
 __global__ __launch_bounds__(256) void k_batch_assign_bf16(
     int nitems, const int2 *__restrict__ items, int P, const long *__restrict__ xoff,
     const int *__restrict__ nvec, int d, int k, const float *__restrict__ X,
-    const float *__restrict__ C, const float *__restrict__ nc, int *__restrict__ labels,
-    int *__restrict__ counter, unsigned long long *__restrict__ nexact, float big) {
+    const float *__restrict__ C, const float *__restrict__ nc, const float2 *__restrict__ bnd,
+    int *__restrict__ labels, int *__restrict__ counter, unsigned long long *__restrict__ nexact,
+    float big) {
   constexpr int Dp = KS * 16;
   const int Kp = (k + 31) / 32 * 32;
   extern __shared__ __attribute__((aligned(16))) unsigned char bsm[];
@@ -251,7 +253,7 @@ __global__ __launch_bounds__(256) void k_batch_assign_bf16(
   float *Cf = bias + Kp;                                            // k x d
   float *ncs = Cf + (size_t)k * d;                                  // k
   __shared__ int item_s;
-  __shared__ float red[256], bndv[3];
+  __shared__ float bndv[3];
   __shared__ unsigned long long exact_ct;
   const int tid = threadIdx.x, lane = tid & 63, h = lane >> 5, col = lane & 31;
   if (tid == 0) exact_ct = 0;
@@ -277,44 +279,22 @@ __global__ __launch_bounds__(256) void k_batch_assign_bf16(
         }
         Ch[e] = b_u32x4{wv[0], wv[1], wv[2], wv[3]};
       }
-      float mx = 0.0f;
       for (int j = tid; j < Kp; j += 256) {
         const float n2 = j < k ? nc[(size_t)inst * k + j] : 0.0f;
         bias[j] = j < k ? -0.5f * n2 : -INFINITY;
-        if (j < k) { ncs[j] = n2; mx = fmaxf(mx, n2); }
+        if (j < k) ncs[j] = n2;
       }
       for (int e = tid; e < k * d; e += 256) Cf[e] = Ci[e];
-      // max over the centres of |c_j - bf16(c_j)| (c - bf16(c) is exact in
-      // FP32; the chain of squares is within gamma_d of the sum)
-      float dm = 0.0f;
-      for (int j = tid; j < k; j += 256) {
-        float t = 0.0f;
-        for (int q = 0; q < d; ++q) {
-          const float v = Ci[(size_t)j * d + q];
-          const float r = v - __uint_as_float((uint32_t)b_bf16(v) << 16);
-          t = __builtin_fmaf(r, r, t);
-        }
-        dm = fmaxf(dm, t);
-      }
-      red[tid] = mx;
-      __syncthreads();
-      for (int o = 128; o > 0; o >>= 1) {
-        if (tid < o) red[tid] = fmaxf(red[tid], red[tid + o]);
-        __syncthreads();
-      }
+      // the instance's bounds (k_batch_bounds / k_batch_finalize): the nc
+      // chains are within gamma_d of the sums, and so are the chains of the
+      // BF16 residuals c - bf16(c) (exact in FP32)
       if (tid == 0) {
-        const float ncmax = red[0] * (1.0f + (float)d * 1.2e-7f + 1e-6f) + 1e-30f;  // nc chain within gamma_d
+        const float2 bb = bnd[inst];
+        const float ncmax = bb.x * (1.0f + (float)d * 1.2e-7f + 1e-6f) + 1e-30f;
         bndv[0] = sqrtf(ncmax) * (1.0f + 1e-6f);
         bndv[1] = ncmax;
+        bndv[2] = sqrtf(bb.y * (1.0f + (float)d * 1.2e-7f + 1e-6f)) * (1.0f + 1e-6f) + 1e-30f;
       }
-      __syncthreads();
-      red[tid] = dm;
-      __syncthreads();
-      for (int o = 128; o > 0; o >>= 1) {
-        if (tid < o) red[tid] = fmaxf(red[tid], red[tid + o]);
-        __syncthreads();
-      }
-      if (tid == 0) bndv[2] = sqrtf(red[0] * (1.0f + (float)d * 1.2e-7f + 1e-6f)) * (1.0f + 1e-6f) + 1e-30f;
       cur = inst;
     }
     __syncthreads();
@@ -769,12 +749,41 @@ __global__ __launch_bounds__(256) void k_batch_chunks_wave(
 // Update 3/3: one workgroup per instance: canonical combination of the chunk
 // sums, new centres (empty clusters keep theirs), ||c||^2 chains,
 // compactness from the moments against the new centres.
+// FP32 chain over the dimensions of (c - bf16(c))^2 (each residual exact)
+__device__ inline float bf16_residual_sq(const float *c, int d) {
+  float t = 0.0f;
+  for (int q = 0; q < d; ++q) {
+    const float v = c[q];
+    const float r = v - __uint_as_float((uint32_t)b_bf16(v) << 16);
+    t = __builtin_fmaf(r, r, t);
+  }
+  return t;
+}
+
+// block-wide maxima of two values (256 threads), written by thread 0
+__device__ inline void block_max2(float a, float b, float *ra, float *rb, float2 *out) {
+  ra[threadIdx.x] = a;
+  rb[threadIdx.x] = b;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      ra[threadIdx.x] = fmaxf(ra[threadIdx.x], ra[threadIdx.x + o]);
+      rb[threadIdx.x] = fmaxf(rb[threadIdx.x], rb[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = make_float2(ra[0], rb[0]);
+}
+
 __global__ __launch_bounds__(256) void k_batch_finalize(
     int d, int k, const long *__restrict__ choff, const int *__restrict__ gcnt,
     const int *__restrict__ gcpre, const double *__restrict__ part, float *__restrict__ C,
-    float *__restrict__ nc, double *__restrict__ comp_out, int *__restrict__ empty_out) {
+    float *__restrict__ nc, float2 *__restrict__ bnd, double *__restrict__ comp_out,
+    int *__restrict__ empty_out) {
   __shared__ double cj[B_MAXK];
   __shared__ int nempty;
+  __shared__ float rx[256], ry[256];
+  float mx = 0.0f, dm = 0.0f;
   const int inst = blockIdx.x, t = threadIdx.x;
   if (t == 0) nempty = 0;
   __syncthreads();
@@ -798,6 +807,8 @@ __global__ __launch_bounds__(256) void k_batch_finalize(
     }
     const double Q = csum2(pj + d, ncj, d + 1);
     nc[(size_t)inst * k + j] = s2;
+    mx = fmaxf(mx, s2);
+    dm = fmaxf(dm, bf16_residual_sq(Ci + (size_t)j * d, d));
     cj[j] = (Q - 2.0 * a) + (double)nj * b;
     if (nj == 0) atomicAdd(&nempty, 1);
   }
@@ -806,6 +817,22 @@ __global__ __launch_bounds__(256) void k_batch_finalize(
     comp_out[inst] = csum2(cj, k, 1);
     empty_out[inst] = nempty;
   }
+  block_max2(mx, dm, rx, ry, bnd + inst);
+}
+
+// Per instance: the largest ||c_j||^2 (nc, already computed) and the largest
+// ||c_j - bf16(c_j)||^2, for centres set from outside the iteration.
+__global__ __launch_bounds__(256) void k_batch_bounds(int d, int k, const float *__restrict__ C,
+                                                      const float *__restrict__ nc,
+                                                      float2 *__restrict__ bnd) {
+  __shared__ float rx[256], ry[256];
+  const int inst = blockIdx.x;
+  float mx = 0.0f, dm = 0.0f;
+  for (int j = threadIdx.x; j < k; j += blockDim.x) {
+    mx = fmaxf(mx, nc[(size_t)inst * k + j]);
+    dm = fmaxf(dm, bf16_residual_sq(C + ((size_t)inst * k + j) * d, d));
+  }
+  block_max2(mx, dm, rx, ry, bnd + inst);
 }
 
 // ||c||^2 chains of every instance's centres.
@@ -887,12 +914,12 @@ static hipError_t launch_assign(Batch *b, int P, int nitems, const int2 *items) 
     const int grid = num_cus() * 4;
     if (b->d <= 16)
       hipLaunchKernelGGL(k_batch_assign_bf16<1>, dim3(grid), dim3(256), lds, b->stream, nitems, items,
-                         P, b->xoff, b->nvec, b->d, b->k, b->X, b->C, b->nc, b->labels, b->counters,
-                         b->nexact, FLT_MAX);
+                         P, b->xoff, b->nvec, b->d, b->k, b->X, b->C, b->nc, b->bnd, b->labels,
+                         b->counters, b->nexact, FLT_MAX);
     else
       hipLaunchKernelGGL(k_batch_assign_bf16<2>, dim3(grid), dim3(256), lds, b->stream, nitems, items,
-                         P, b->xoff, b->nvec, b->d, b->k, b->X, b->C, b->nc, b->labels, b->counters,
-                         b->nexact, FLT_MAX);
+                         P, b->xoff, b->nvec, b->d, b->k, b->X, b->C, b->nc, b->bnd, b->labels,
+                         b->counters, b->nexact, FLT_MAX);
     return hipGetLastError();
   }
   if (b->d == 16) return launch_assign_x<16>(b, P, nitems, items);
@@ -940,7 +967,7 @@ static hipError_t launch_update(Batch *b, int slot) {
 #undef CHK
 #undef CHW
   hipLaunchKernelGGL(k_batch_finalize, dim3(b->nb), dim3(256), 0, b->stream, b->d, b->k, b->choff,
-                     b->gcnt, b->gcpre, b->part, b->C, b->nc, co, eo);
+                     b->gcnt, b->gcpre, b->part, b->C, b->nc, b->bnd, co, eo);
   return hipGetLastError();
 }
 
@@ -1065,7 +1092,7 @@ static void batch_free(Batch *b) {
   batch_free_graph(b);
   void *bufs[] = {b->X_owned, b->xoff, b->nvec, b->C, b->nc, b->labels, b->perm, b->q,
                   b->part, b->comp, b->choff, b->empty, b->order, b->items, b->counters,
-                  b->gcnt, b->goff, b->gcpre, b->desc, b->nexact};
+                  b->gcnt, b->goff, b->gcpre, b->desc, b->nexact, b->bnd};
   for (void *p : bufs) hipFree(p);
   if (b->own_stream && b->stream) hipStreamDestroy(b->stream);
 }
@@ -1128,6 +1155,7 @@ static kmeansErrorStatus batch_create(Batch *b, int nb, const long *n, int d, in
   BCHK(hipMemcpy(b->choff, choff.data(), sizeof(long) * (nb + 1), hipMemcpyHostToDevice), KMEANS_ERROR_BUILDDATA);
   BCHK(hipMalloc(&b->C, sizeof(float) * (size_t)nb * k * d), KMEANS_ERROR_BUILDDATA);
   BCHK(hipMalloc(&b->nc, sizeof(float) * (size_t)nb * k), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMalloc(&b->bnd, sizeof(float2) * (size_t)nb), KMEANS_ERROR_BUILDDATA);
   BCHK(hipMalloc(&b->labels, sizeof(int) * (size_t)tot), KMEANS_ERROR_BUILDDATA);
   BCHK(hipMemsetAsync(b->labels, 0, sizeof(int) * (size_t)tot, b->stream), KMEANS_ERROR_BUILDDATA);
   BCHK(hipMalloc(&b->perm, sizeof(int) * (size_t)tot), KMEANS_ERROR_BUILDDATA);
@@ -1283,6 +1311,8 @@ kmeansErrorStatus kmeansBatchSetCenters(kmeansBatchHandle h, const float *C, int
        KMEANS_ERROR_INITIALIZE);
   hipLaunchKernelGGL(km::k_batch_norms, dim3(km::bg1((long)b->nb * b->k)), dim3(256), 0, b->stream,
                      (long)b->nb * b->k, b->d, b->C, b->nc);
+  hipLaunchKernelGGL(km::k_batch_bounds, dim3(b->nb), dim3(256), 0, b->stream, b->d, b->k, b->C,
+                     b->nc, b->bnd);
   BCHK(hipStreamSynchronize(b->stream), KMEANS_ERROR_INITIALIZE);
   b->centers_set = true;
   return KMEANS_SUCCESS;
@@ -1296,6 +1326,8 @@ kmeansErrorStatus kmeansBatchInitFirstK(kmeansBatchHandle h) {
                      b->d, b->xoff, b->X, b->C);
   hipLaunchKernelGGL(km::k_batch_norms, dim3(km::bg1((long)b->nb * b->k)), dim3(256), 0, b->stream,
                      (long)b->nb * b->k, b->d, b->C, b->nc);
+  hipLaunchKernelGGL(km::k_batch_bounds, dim3(b->nb), dim3(256), 0, b->stream, b->d, b->k, b->C,
+                     b->nc, b->bnd);
   BCHK(hipStreamSynchronize(b->stream), KMEANS_ERROR_INITIALIZE);
   b->centers_set = true;
   return KMEANS_SUCCESS;
