@@ -302,10 +302,30 @@ __global__ __launch_bounds__(256) void k_batch_assign_bf16(
     const long rows = min((long)P, (long)nvec[inst] - w.y);
     const long base = xoff[inst] + w.y;
     const int wv0 = tid >> 6;
+    // this half's 8 dimensions per k-step of the point in row pn (zero past
+    // d or past the rows); the next 32 points' values are loaded while the
+    // current ones are scored
+    auto load_half = [&](long pn, float *v) {
+      const bool okn = pn < rows;
+      const float *xn = X + (base + (okn ? pn : 0)) * d;
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int d0 = ks * 16 + 8 * h + u;
+          v[ks * 8 + u] = (okn && d0 < d) ? xn[d0] : 0.0f;
+        }
+    };
+    float nxt[KS * 8];
+    load_half(wv0 * 32 + col, nxt);
     for (long p0 = wv0 * 32; p0 < rows; p0 += 4 * 32) {
       const long pr = p0 + col;
       const bool ok = pr < rows;
       const float *xr = X + (base + (ok ? pr : 0)) * d;
+      float cur_x[KS * 8];
+#pragma unroll
+      for (int i = 0; i < KS * 8; ++i) cur_x[i] = nxt[i];
+      if (p0 + 4 * 32 < rows) load_half(p0 + 4 * 32 + col, nxt);
       // the point's BF16 fragments (this half's 8 dimensions per k-step) and
       // its ||x||^2 part
       b_u32x4 bf[KS];
@@ -315,9 +335,8 @@ __global__ __launch_bounds__(256) void k_batch_assign_bf16(
         uint32_t wv[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          const int d0 = ks * 16 + 8 * h + 2 * u;  // this half's dimensions (read again)
-          const float v0 = (ok && d0 < d) ? xr[d0] : 0.0f;
-          const float v1 = (ok && d0 + 1 < d) ? xr[d0 + 1] : 0.0f;
+          const float v0 = cur_x[ks * 8 + 2 * u];
+          const float v1 = cur_x[ks * 8 + 2 * u + 1];
           sq = __builtin_fmaf(v0, v0, sq);
           sq = __builtin_fmaf(v1, v1, sq);
           const uint16_t h0 = b_bf16(v0), h1 = b_bf16(v1);
