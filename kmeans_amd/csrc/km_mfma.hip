@@ -537,12 +537,25 @@ __global__ __launch_bounds__(256) void k_deep_mfma(
       xh4[ks] = u32x4_t{0u, 0u, 0u, 0u};
       xl4[ks] = u32x4_t{0u, 0u, 0u, 0u};
       if (ok) {
+        // this half's 8 dimensions of the k-step: two 16-byte loads when they
+        // all lie inside a row whose length is a multiple of 8 (16-byte
+        // aligned), else one guarded load each
+        const int db = ks * 16 + 8 * h;
+        float xs[8];
+        if ((D & 7) == 0 && db + 8 <= D) {
+          const float4 lo4 = *reinterpret_cast<const float4 *>(X + p * D + db);
+          const float4 hi4 = *reinterpret_cast<const float4 *>(X + p * D + db + 4);
+          xs[0] = lo4.x; xs[1] = lo4.y; xs[2] = lo4.z; xs[3] = lo4.w;
+          xs[4] = hi4.x; xs[5] = hi4.y; xs[6] = hi4.z; xs[7] = hi4.w;
+        } else {
+#pragma unroll
+          for (int u = 0; u < 8; ++u) xs[u] = db + u < D ? X[p * D + db + u] : 0.0f;
+        }
         uint32_t wh[4], wl[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          const int d0 = ks * 16 + 8 * h + 2 * u;
-          const float v0 = d0 < D ? X[p * D + d0] : 0.0f;
-          const float v1 = d0 + 1 < D ? X[p * D + d0 + 1] : 0.0f;
+          const float v0 = xs[2 * u];
+          const float v1 = xs[2 * u + 1];
           const uint16_t h0 = f32_to_bf16_rne(v0), h1 = f32_to_bf16_rne(v1);
           const float r0 = v0 - __uint_as_float((uint32_t)h0 << 16);
           const float r1 = v1 - __uint_as_float((uint32_t)h1 << 16);
