@@ -44,6 +44,14 @@ extern "C" {
 #define KM_CHUNK 256           /* members per first-level FP64 partial sum */
 #define KM_BLOCK (1L << 23)    /* points per canonical summation block     */
 
+/* C++ callers get the reference's default arguments (kmeans.hpp:102-107,
+ * 125-130); C callers pass every argument. */
+#ifdef __cplusplus
+#define KM_DEFAULT_ARG(v) = v
+#else
+#define KM_DEFAULT_ARG(v)
+#endif
+
 /* ---- status codes: values of main/cpp/kmeans.hpp:44-59 ------------------ */
 typedef enum kmeansErrorStatus {
   KMEANS_SUCCESS = 0,
@@ -97,7 +105,7 @@ const char *kmeansGetErrorString(kmeansErrorStatus err);
 kmeansErrorStatus computeKmeansF(const float *data, int m, int n, int k, float criterion,
                                  int maxIters, int numRetries, int initAlgorithm,
                                  int doFullSGEMM, int matrixFormat, float *result,
-                                 float miniBatchFraction);
+                                 float miniBatchFraction KM_DEFAULT_ARG(1.0f));
 
 /* Replaces computeKmeansD (main/cpp/kmeans.hpp:125-130, kmeans.cpp:1364-1409).
  * Same contract and deltas in double precision: scores are fma chains in
@@ -107,7 +115,7 @@ kmeansErrorStatus computeKmeansF(const float *data, int m, int n, int k, float c
 kmeansErrorStatus computeKmeansD(const double *data, int m, int n, int k, double criterion,
                                  int maxIters, int numRetries, int initAlgorithm,
                                  int doFullSGEMM, int matrixFormat, double *result,
-                                 double miniBatchFraction);
+                                 double miniBatchFraction KM_DEFAULT_ARG(1.0));
 
 /* Assignment precision of the engine. */
 typedef enum kmeansPrecision {
@@ -217,6 +225,29 @@ kmeansErrorStatus kmeansSetMiniBatch(kmeansHandle h, long rows);
 const float *kmeansDeviceCenters(kmeansHandle h);
 const int *kmeansDeviceLabels(kmeansHandle h);
 kmeansErrorStatus kmeansSynchronize(kmeansHandle h);
+
+/* Double-precision engine handle: the same steps as the float handle for
+ * the reference's kmeans<double> class (kmeans.hpp:182-312, instantiated in
+ * kmeans.cpp:1364-1409).  Scores are fma chains in double, centres the FP64
+ * canonical sums divided by the counts.  Options: device and full_gemm are
+ * used; precision must be KM_PREC_FP32 (meaning: no reduced precision); no
+ * streamed tiles and no proposer.  Stats fill iterations, compactness and
+ * rel_err. */
+typedef struct kmeansEngineD *kmeansHandleD;
+kmeansErrorStatus kmeansCreateD(kmeansHandleD *h, long m, int n, int k, const kmeansOptions *opt);
+void kmeansDestroyD(kmeansHandleD h);
+kmeansErrorStatus kmeansSetDataD(kmeansHandleD h, const double *data, int on_device);
+kmeansErrorStatus kmeansSetCentersD(kmeansHandleD h, const double *centers /* k x n host */);
+kmeansErrorStatus kmeansInitializeD(kmeansHandleD h, int initAlgorithm, uint64_t seed,
+                                    const double *centers_dxk);
+kmeansErrorStatus kmeansAssignD(kmeansHandleD h);
+kmeansErrorStatus kmeansUpdateCentersD(kmeansHandleD h);
+kmeansErrorStatus kmeansCompactnessD(kmeansHandleD h, double *out);
+kmeansErrorStatus kmeansRunD(kmeansHandleD h, int maxIters, double criterion);
+kmeansErrorStatus kmeansSetMiniBatchD(kmeansHandleD h, long rows);
+kmeansErrorStatus kmeansGetCentersD(kmeansHandleD h, double *centers /* k x n host */);
+kmeansErrorStatus kmeansGetLabelsD(kmeansHandleD h, int *labels /* m host */);
+kmeansErrorStatus kmeansGetStatsD(kmeansHandleD h, kmeansStats *stats);
 
 /* =========================================================================
  * 1b. Batched independent clusterings (no counterpart in the reference: its

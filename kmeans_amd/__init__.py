@@ -94,6 +94,19 @@ SIGNATURES = {
     "kmeansDeviceCenters": (_P, [_P]),
     "kmeansDeviceLabels": (_P, [_P]),
     "kmeansSynchronize": (_I, [_P]),
+    "kmeansCreateD": (_I, [_P, _L, _I, _I, _P]),
+    "kmeansDestroyD": (None, [_P]),
+    "kmeansSetDataD": (_I, [_P, _P, _I]),
+    "kmeansSetCentersD": (_I, [_P, _P]),
+    "kmeansInitializeD": (_I, [_P, _I, _U64, _P]),
+    "kmeansAssignD": (_I, [_P]),
+    "kmeansUpdateCentersD": (_I, [_P]),
+    "kmeansCompactnessD": (_I, [_P, _P]),
+    "kmeansRunD": (_I, [_P, _I, _D]),
+    "kmeansSetMiniBatchD": (_I, [_P, _L]),
+    "kmeansGetCentersD": (_I, [_P, _P]),
+    "kmeansGetLabelsD": (_I, [_P, _P]),
+    "kmeansGetStatsD": (_I, [_P, _P]),
     "kmeansBatchCreate": (_I, [_P, _I, _P, _I, _I, _P]),
     "kmeansBatchDestroy": (None, [_P]),
     "kmeansBatchSetData": (_I, [_P, _P, _I]),

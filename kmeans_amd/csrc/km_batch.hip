@@ -392,16 +392,23 @@ __global__ __launch_bounds__(256) void k_batch_assign_bf16(
         t1 = (u1 != __float_as_uint(o1)) ? t : ot1;
       };
       // the matrix core scores tile t+1 while the VALU runs tile t's
-      // epilogue; tiles past the end are clamped to the last one and their
-      // scores are never read
+      // epilogue; the last one or two tiles are peeled so that no tile past
+      // the end is scored
       const int nT = Kp / 32;
       b_f32x16 A = score_tile(0), B;
-      for (int t = 0; t < nT; t += 2) {
-        B = score_tile(min(t + 1, nT - 1));
+      int t = 0;
+      for (; t + 2 < nT; t += 2) {
+        B = score_tile(t + 1);
         epilogue(t, A);
-        if (t + 1 >= nT) break;
-        A = score_tile(min(t + 2, nT - 1));
+        A = score_tile(t + 2);
         epilogue(t + 1, B);
+      }
+      if (t + 1 < nT) {  // t = nT - 2
+        B = score_tile(t + 1);
+        epilogue(t, A);
+        epilogue(t + 1, B);
+      } else {  // t = nT - 1
+        epilogue(t, A);
       }
       const bool real1 = b1 > -FLT_MAX, real2 = b2 > -FLT_MAX;
       int i1 = real1 ? t1 * 32 + b_row((int)(__float_as_uint(b1) & 15u), h) : -1;

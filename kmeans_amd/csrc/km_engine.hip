@@ -33,6 +33,11 @@ Engine::~Engine() {
   hipFree(dlabels);
   hipFree(dcounts);
   hipFree(dempty);
+  hipFree(dempty_hist);
+  hipFree(dctl);
+  hipHostFree(hflag);
+  for (auto &e : ev_iter)
+    if (e) hipEventDestroy(e);
   hipFree(dcomp);
   hipFree(dq);
   hipFree(dxn);
@@ -106,19 +111,28 @@ kmeansErrorStatus Engine::create(long m_, int n_, int k_, const kmeansOptions *o
   return ensure_comp_slots(64);
 }
 
+// Per-iteration-slot arrays of a run (compactness, empty-cluster counts and
+// the host copies of the stop flag); their contents do not outlive a run.
 kmeansErrorStatus Engine::ensure_comp_slots(int nslots) {
   if (nslots <= comp_slots) return KMEANS_SUCCESS;
-  double *nb = nullptr;
-  KM_HIP_CHECK(hipMalloc(&nb, sizeof(double) * nslots), KMEANS_ERROR_BUILDDATA);
-  if (dcomp) {
-    KM_HIP_CHECK(hipMemcpyAsync(nb, dcomp, sizeof(double) * comp_slots, hipMemcpyDeviceToDevice,
-                                stream),
-                 KMEANS_ERROR_BUILDDATA);
-    KM_HIP_CHECK(hipStreamSynchronize(stream), KMEANS_ERROR_BUILDDATA);
-    hipFree(dcomp);
-  }
-  dcomp = nb;
+  KM_HIP_CHECK(hipStreamSynchronize(stream), KMEANS_ERROR_BUILDDATA);
+  hipFree(dcomp);
+  hipFree(dempty_hist);
+  hipHostFree(hflag);
+  dcomp = nullptr;
+  dempty_hist = nullptr;
+  hflag = nullptr;
+  comp_slots = 0;
+  KM_HIP_CHECK(hipMalloc(&dcomp, sizeof(double) * nslots), KMEANS_ERROR_BUILDDATA);
+  KM_HIP_CHECK(hipMalloc(&dempty_hist, sizeof(int) * nslots), KMEANS_ERROR_BUILDDATA);
+  KM_HIP_CHECK(hipHostMalloc(&hflag, sizeof(int) * nslots, hipHostMallocDefault),
+               KMEANS_ERROR_BUILDDATA);
   comp_slots = nslots;
+  if (!dctl) {
+    KM_HIP_CHECK(hipMalloc(&dctl, sizeof(int) * 2), KMEANS_ERROR_BUILDDATA);
+    for (auto &e : ev_iter)
+      KM_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming), KMEANS_ERROR_BUILDDATA);
+  }
   return KMEANS_SUCCESS;
 }
 
@@ -226,8 +240,10 @@ kmeansErrorStatus Engine::closestCenters(long r0, long rows) {
   if (opt.full_gemm) {
     e = gemm_assign(this, dX + r0 * n, dxn ? dxn + r0 : nullptr, rows, dlabels + r0);
   } else if (opt.precision == KM_PREC_FP32) {
-    e = launch_assign_f32(stream, rows, nullptr, n, dX + r0 * n, nullptr, k, dCt, ldc, dnc,
-                          dlabels + r0, nullptr);
+    // under a device-side convergence test the row count comes from ctl[1]
+    // (0 after the converged iteration)
+    e = launch_assign_f32(stream, rows, assign_stop ? assign_stop + 1 : nullptr, n, dX + r0 * n,
+                          nullptr, k, dCt, ldc, dnc, dlabels + r0, nullptr);
   } else {
     e = mfma_assign(this, r0, rows);
   }
@@ -235,7 +251,8 @@ kmeansErrorStatus Engine::closestCenters(long r0, long rows) {
   return KMEANS_SUCCESS;
 }
 
-kmeansErrorStatus Engine::clusterCenters(long r0, long rows) {
+kmeansErrorStatus Engine::clusterCenters(long r0, long rows, int *empty) {
+  int *dempty = empty ? empty : this->dempty;
   if (rows > KM_BLOCK && upd_side[0]) {
     // several canonical blocks: sorts and chunk sums of alternate blocks on
     // two side streams, the in-order additions on the engine's stream
@@ -308,12 +325,78 @@ void Engine::collect_timing(int nslots) {
   }
 }
 
+// Control words of a run with a device-side convergence test: no stop, all
+// rows assigned.
+__global__ void k_ctl_reset(int *ctl, int rows) {
+  ctl[0] = 0;
+  ctl[1] = rows;
+}
+
+// The reference's convergence test (kmeans.cpp:1194,1235-1237) after
+// iteration slot s: relErr = |comp - prev| / |comp| with prev the previous
+// slot's compactness (prev0 before slot 0); the loop continues only while
+// relErr > criterion.  Once it stops, ctl[1] = 0 gives the assign kernels of
+// the iterations already enqueued behind it no rows, so the labels stay
+// those of the converged iteration and the rest of such an iteration
+// recomputes the same centres and compactness from them.
+__global__ void k_converge(const double *comp, int s, double prev0, double criterion, int *ctl) {
+  if (ctl[0]) return;
+  const double c = comp[s], prev = s > 0 ? comp[s - 1] : prev0;
+  const double rel = fabs(c - prev) / fabs(c);
+  if (!(rel > criterion)) {
+    ctl[0] = 1;
+    ctl[1] = 0;
+  }
+}
+
+kmeansErrorStatus Engine::collect_empty(int nslots) {
+  if (nslots <= 0) return KMEANS_SUCCESS;
+  std::vector<int> emp(nslots);
+  KM_HIP_CHECK(hipMemcpyAsync(emp.data(), dempty_hist, sizeof(int) * nslots, hipMemcpyDeviceToHost,
+                              stream),
+               KMEANS_ERROR_COMPUTE);
+  KM_HIP_CHECK(hipStreamSynchronize(stream), KMEANS_ERROR_COMPUTE);
+  for (int s = 0; s < nslots; ++s) stats.empty_clusters += emp[s];
+  return KMEANS_SUCCESS;
+}
+
+// The loop of kmeans.cpp:1186-1247.  Three ways to run it:
+//  - criterion < 0 (a fixed iteration count): everything enqueued, the
+//    compactness history read once at the end;
+//  - criterion >= 0 over resident points in full passes: the convergence
+//    test runs on the device after each iteration (k_converge); the host
+//    keeps at most three iterations in flight and stops enqueuing once it
+//    sees the flag, then finds the converged iteration in the history;
+//  - otherwise (mini-batch windows, streamed tiles, the proposer, the
+//    library GEMM) one host round trip per iteration.
+// Statistics count iterations up to the converged one.
 kmeansErrorStatus Engine::run_iterations(long r0, long rows, bool minibatch) {
   kmeansErrorStatus err;
-  const bool sync_each = criterion >= 0.0;
+  const bool crit = criterion >= 0.0;
+  const bool devconv = crit && !streaming && !minibatch && !learned && !opt.full_gemm;
+  const bool sync_each = crit && !devconv;
   int it0 = iters, slot = 0;
   if ((err = ensure_comp_slots(std::max(maxIters, 1))) != KMEANS_SUCCESS) return err;
-  std::vector<double> hist;
+  const double prev0 = compactnessScore;
+  if (devconv) {
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, stream, dctl, (int)rows);
+    KM_LAUNCH_CHECK(KMEANS_ERROR_COMPUTE);
+    assign_stop = dctl;
+  }
+  // assign_stop is cleared however the loop is left
+  struct StopGuard {
+    const int **p;
+    ~StopGuard() { *p = nullptr; }
+  } guard{&assign_stop};
+  // one iteration's compactness and empty-cluster count back to the host
+  auto read_slot = [&](int s, double *comp, int *emp) -> kmeansErrorStatus {
+    KM_HIP_CHECK(hipMemcpyAsync(comp, dcomp + s, sizeof(double), hipMemcpyDeviceToHost, stream),
+                 KMEANS_ERROR_COMPUTE);
+    KM_HIP_CHECK(hipMemcpyAsync(emp, dempty_hist + s, sizeof(int), hipMemcpyDeviceToHost, stream),
+                 KMEANS_ERROR_COMPUTE);
+    KM_HIP_CHECK(hipStreamSynchronize(stream), KMEANS_ERROR_COMPUTE);
+    return KMEANS_SUCCESS;
+  };
   while (iters < maxIters && relErr > criterion) {
     long b0 = r0, br = rows;
     if (minibatch) constructMiniBatch(mb_index++, &b0, &br);
@@ -325,91 +408,88 @@ kmeansErrorStatus Engine::run_iterations(long r0, long rows, bool minibatch) {
       if ((err = stream_iteration(slot)) != KMEANS_SUCCESS) return err;
       record(slot, 3);
       record(slot, 4);
-      ++iters;
-      ++slot;
-      if (sync_each) {
-        double comp = 0.0;
-        KM_HIP_CHECK(hipMemcpyAsync(&comp, dcomp + slot - 1, sizeof(double),
-                                    hipMemcpyDeviceToHost, stream),
-                     KMEANS_ERROR_COMPUTE);
-        KM_HIP_CHECK(hipStreamSynchronize(stream), KMEANS_ERROR_COMPUTE);
-        const double prev = compactnessScore;
-        compactnessScore = comp;
-        relErr = fabs(comp - prev) / fabs(comp);
-      } else {
-        relErr = 1.0 + fabs(criterion);
-      }
-      continue;
-    }
-    if (learned && minibatch) return KMEANS_ERROR_INVALID_ARGUMENT;
-    if (learned_active(this)) {
-      // proposer iteration: the proposal and the grouped exact scores are the
-      // assign phase, the full search of uncertified points the re-resolve
-      if ((err = learned_prepare(this)) != KMEANS_SUCCESS) return err;
-      record(slot, 0);
-      KM_HIP_CHECK(learned_assign(this), KMEANS_ERROR_CLOSEST_CENTERS);
-      record(slot, 1);
-      KM_HIP_CHECK(learned_fallback(this), KMEANS_ERROR_CLOSEST_CENTERS);
-      record(slot, 2);
     } else {
-      record(slot, 0);
-      if ((err = closestCenters(b0, br)) != KMEANS_SUCCESS) return err;
-      record(slot, 1);
-      if (opt.precision != KM_PREC_FP32)
-        KM_HIP_CHECK(mfma_reresolve(this, b0, br), KMEANS_ERROR_CLOSEST_CENTERS);
-      record(slot, 2);
-      if (learned) KM_HIP_CHECK(learned_collect(this, iters), KMEANS_ERROR_CLOSEST_CENTERS);
+      if (learned && minibatch) return KMEANS_ERROR_INVALID_ARGUMENT;
+      if (learned_active(this)) {
+        // proposer iteration: the proposal and the grouped exact scores are
+        // the assign phase, the full search of uncertified points the
+        // re-resolve
+        if ((err = learned_prepare(this)) != KMEANS_SUCCESS) return err;
+        record(slot, 0);
+        KM_HIP_CHECK(learned_assign(this), KMEANS_ERROR_CLOSEST_CENTERS);
+        record(slot, 1);
+        KM_HIP_CHECK(learned_fallback(this), KMEANS_ERROR_CLOSEST_CENTERS);
+        record(slot, 2);
+      } else {
+        record(slot, 0);
+        if ((err = closestCenters(b0, br)) != KMEANS_SUCCESS) return err;
+        record(slot, 1);
+        if (opt.precision != KM_PREC_FP32)
+          KM_HIP_CHECK(mfma_reresolve(this, b0, br), KMEANS_ERROR_CLOSEST_CENTERS);
+        record(slot, 2);
+        if (learned) KM_HIP_CHECK(learned_collect(this, iters), KMEANS_ERROR_CLOSEST_CENTERS);
+      }
+      if ((err = clusterCenters(b0, br, dempty_hist + slot)) != KMEANS_SUCCESS) return err;
+      record(slot, 3);
+      if ((err = compactness(b0, br, slot)) != KMEANS_SUCCESS) return err;
+      record(slot, 4);
     }
-    if ((err = clusterCenters(b0, br)) != KMEANS_SUCCESS) return err;
-    KM_HIP_CHECK(hipMemcpyAsync(dempty + 1, dempty, sizeof(int), hipMemcpyDeviceToDevice, stream),
-                 KMEANS_ERROR_COMPUTE);
-    record(slot, 3);
-    if ((err = compactness(b0, br, slot)) != KMEANS_SUCCESS) return err;
-    record(slot, 4);
+    if (devconv) {
+      hipLaunchKernelGGL(k_converge, dim3(1), dim3(1), 0, stream, dcomp, slot, prev0, criterion,
+                         dctl);
+      KM_LAUNCH_CHECK(KMEANS_ERROR_COMPUTE);
+      KM_HIP_CHECK(hipMemcpyAsync(hflag + slot, dctl, sizeof(int), hipMemcpyDeviceToHost, stream),
+                   KMEANS_ERROR_COMPUTE);
+      KM_HIP_CHECK(hipEventRecord(ev_iter[slot % 3], stream), KMEANS_ERROR_COMPUTE);
+    }
     ++iters;
     ++slot;
     if (sync_each) {
       int emp = 0;
       double comp = 0.0;
-      KM_HIP_CHECK(hipMemcpyAsync(&comp, dcomp + slot - 1, sizeof(double), hipMemcpyDeviceToHost,
-                                  stream),
-                   KMEANS_ERROR_COMPUTE);
-      KM_HIP_CHECK(hipMemcpyAsync(&emp, dempty + 1, sizeof(int), hipMemcpyDeviceToHost, stream),
-                   KMEANS_ERROR_COMPUTE);
-      KM_HIP_CHECK(hipStreamSynchronize(stream), KMEANS_ERROR_COMPUTE);
+      if ((err = read_slot(slot - 1, &comp, &emp)) != KMEANS_SUCCESS) return err;
       stats.empty_clusters += emp;
       const double prev = compactnessScore;
       compactnessScore = comp;
       relErr = fabs(comp - prev) / fabs(comp);
       if (minibatch && relErr <= criterion) break;
     } else {
-      // fixed iteration count: no host round trip inside the loop
+      // no host round trip inside the loop; with the device-side test the
+      // host waits for the iteration two before the one just enqueued (two
+      // stay queued, so the GPU does not idle) and stops enqueuing once that
+      // one had converged
       relErr = 1.0 + fabs(criterion);
+      if (devconv && slot >= 3) {
+        KM_HIP_CHECK(hipEventSynchronize(ev_iter[(slot - 3) % 3]), KMEANS_ERROR_COMPUTE);
+        if (hflag[slot - 3]) break;
+      }
     }
   }
+  int used = slot;  // iterations that count (up to the converged one)
   if (!sync_each && slot > 0) {
-    hist.resize(slot);
+    std::vector<double> hist(slot);
     KM_HIP_CHECK(hipMemcpyAsync(hist.data(), dcomp, sizeof(double) * slot, hipMemcpyDeviceToHost,
                                 stream),
                  KMEANS_ERROR_COMPUTE);
-    int emp = 0;
-    KM_HIP_CHECK(hipMemcpyAsync(&emp, dempty + 1, sizeof(int), hipMemcpyDeviceToHost, stream),
-                 KMEANS_ERROR_COMPUTE);
     KM_HIP_CHECK(hipStreamSynchronize(stream), KMEANS_ERROR_COMPUTE);
-    stats.empty_clusters += emp;  // last iteration only in this mode
     for (int s = 0; s < slot; ++s) {
       const double prev = compactnessScore;
       compactnessScore = hist[s];
       relErr = fabs(hist[s] - prev) / fabs(hist[s]);
+      if (devconv && !(relErr > criterion)) {  // the test k_converge made
+        used = s + 1;
+        break;
+      }
     }
+    iters = it0 + used;
+    if ((err = collect_empty(used)) != KMEANS_SUCCESS) return err;
   }
-  if (opt.precision != KM_PREC_FP32) mfma_collect(this, slot);
+  if (opt.precision != KM_PREC_FP32) mfma_collect(this, slot);  // later slots add 0
   learned_stats(this);
-  collect_timing(slot);
+  collect_timing(used);
   stats.iterations = iters;
   stats.compactness = compactnessScore;
   stats.rel_err = relErr;
-  (void)it0;
   return KMEANS_SUCCESS;
 }
 
@@ -471,7 +551,8 @@ kmeansErrorStatus Engine::load_tile(long t, int b) {
 kmeansErrorStatus Engine::stream_iteration(int slot) {
   kmeansErrorStatus err;
   const long nt = (m + tile - 1) / tile;
-  KM_HIP_CHECK(launch_update_zero(stream, n, k, ws.S, ws.cnt, dempty), KMEANS_ERROR_CLUSTER_CENTERS);
+  int *empty = dempty_hist + slot;
+  KM_HIP_CHECK(launch_update_zero(stream, n, k, ws.S, ws.cnt, empty), KMEANS_ERROR_CLUSTER_CENTERS);
   // the first tile may reuse buffer 0 only after the previous iteration used it
   if ((err = load_tile(0, 0)) != KMEANS_SUCCESS) return err;
   for (long t = 0; t < nt; ++t) {
@@ -503,7 +584,7 @@ kmeansErrorStatus Engine::stream_iteration(int slot) {
     KM_HIP_CHECK(hipEventRecord(ev_used[b], stream), KMEANS_ERROR_COPY_TILE);
   }
   KM_HIP_CHECK(launch_update_finalize(stream, n, k, ws.S, ws.cnt, dC, dC, dCt, ldc, dnc, dcounts,
-                                      dempty, ws.compj),
+                                      empty, ws.compj),
                KMEANS_ERROR_CLUSTER_CENTERS);
   if (opt.precision != KM_PREC_FP32) KM_HIP_CHECK(mfma_centers(this), KMEANS_ERROR_CLUSTER_CENTERS);
   KM_HIP_CHECK(launch_comp_reduce(stream, k, ws.compj, dcomp + slot), KMEANS_ERROR_COMPACTNESS);

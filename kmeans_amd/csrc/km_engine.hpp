@@ -31,7 +31,8 @@ struct Engine {
   float *dnc = nullptr;     // k
   int *dlabels = nullptr;   // m
   long long *dcounts = nullptr;
-  int *dempty = nullptr;
+  int *dempty = nullptr;    // empty clusters of a single update call
+  int *dempty_hist = nullptr;  // empty clusters, one per iteration slot
   double *dcomp = nullptr;  // one per iteration slot
   double *dq = nullptr;     // m: FP64 squared norms of the points
   float *dxn = nullptr;     // m: FP32 row norms for the library-GEMM path (resident data)
@@ -59,6 +60,13 @@ struct Engine {
   bool centers_set = false;
   int comp_slots = 0;
   int cur_slot = 0;               // iteration slot of the running kmeansRun
+  // device-side convergence test (criterion >= 0, resident full passes):
+  // ctl[0] = stop flag, ctl[1] = rows the assign kernels take (0 once
+  // stopped); hflag[slot] = host copy of the stop flag after that slot
+  int *dctl = nullptr;
+  int *hflag = nullptr;
+  hipEvent_t ev_iter[3] = {nullptr, nullptr, nullptr};
+  const int *assign_stop = nullptr;  // dctl while such a run is enqueued
   long long last_reresolved = 0;
   kmeansStats stats{};
 
@@ -90,7 +98,8 @@ struct Engine {
   kmeansErrorStatus initialize_pp(uint64_t seed);  // k-means++ seeding (resident data)
   kmeansErrorStatus reset();
   kmeansErrorStatus closestCenters(long r0, long rows);
-  kmeansErrorStatus clusterCenters(long r0, long rows);
+  // empty (nullable): device counter of the empty clusters (default dempty)
+  kmeansErrorStatus clusterCenters(long r0, long rows, int *empty = nullptr);
   kmeansErrorStatus compactness(long r0, long rows, int slot);
   kmeansErrorStatus transpose();
   kmeansErrorStatus constructMiniBatch(long index, long *r0, long *rows);
@@ -107,6 +116,8 @@ struct Engine {
   kmeansErrorStatus setup_streaming(const float *host);
   kmeansErrorStatus load_tile(long t, int b);
   kmeansErrorStatus stream_iteration(int slot);
+  // adds the empty-cluster counts of slots [0, nslots) to the stats
+  kmeansErrorStatus collect_empty(int nslots);
   void record(int slot, int phase);
   void collect_timing(int nslots);
 };

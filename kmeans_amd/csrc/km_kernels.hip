@@ -132,9 +132,12 @@ template <typename T, int RT = MfmaCfg<T>::RT> constexpr int am_bm() { return 4 
 // Order-preserving 64-bit key of (score, centre): a larger key is a larger
 // score, or the lower centre index on equal scores, so the maximum key over
 // any partition of the centres is the single pass's (score, label).  Key 0
-// lies below every key of a non-NaN score.
+// lies below every key of a non-NaN score.  -0 and +0 compare equal in the
+// single pass (a later centre must be strictly greater), so both get the
+// bits of +0 here: otherwise a +0 in a later slice would beat a -0 at a lower
+// index.  NaN never reaches the key: each slice's best is taken with '>'.
 __device__ __forceinline__ unsigned long long split_key(float v, int j) {
-  const unsigned u = __float_as_uint(v);
+  const unsigned u = v == 0.0f ? 0u : __float_as_uint(v);
   const unsigned o = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
   return ((unsigned long long)o << 32) | (unsigned long long)(0xFFFFFFFFu - (unsigned)j);
 }
@@ -701,10 +704,14 @@ __global__ __launch_bounds__(256) void k_block_finalize(int D, int K, const int 
   if (threadIdx.x == 0) cnt[j] += ccount[j];
 }
 
-__global__ void k_zero_sums(long n, double *__restrict__ S, int K, long long *__restrict__ cnt) {
+// Clears the FP64 sums, the counts and (nullable) the empty-cluster counter
+// with stores, not a memset, so the launch can sit in a captured graph.
+__global__ void k_zero_sums(long n, double *__restrict__ S, int K, long long *__restrict__ cnt,
+                            int *__restrict__ empty) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) S[i] = 0.0;
   if (i < K) cnt[i] = 0;
+  if (i == 0 && empty) *empty = 0;
 }
 
 // New centres, transposed copy and norms; empty centres keep the old value.
@@ -993,11 +1000,7 @@ hipError_t launch_point_sq(hipStream_t s, long m, int d, const double *X, double
 hipError_t launch_update_zero(hipStream_t s, int d, int k, double *S, long long *cnt, int *empty) {
   const long zn = (long)k * (d + 1);
   const long zmax = zn > k ? zn : k;
-  hipLaunchKernelGGL(k_zero_sums, dim3(km_cdiv(zmax, 256)), dim3(256), 0, s, zn, S, k, cnt);
-  if (empty) {
-    hipError_t e = hipMemsetAsync(empty, 0, sizeof(int), s);
-    if (e != hipSuccess) return e;
-  }
+  hipLaunchKernelGGL(k_zero_sums, dim3(km_cdiv(zmax, 256)), dim3(256), 0, s, zn, S, k, cnt, empty);
   return hipGetLastError();
 }
 

@@ -198,7 +198,11 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
     const uint16_t *__restrict__ Ch, const float *__restrict__ bias, const float *__restrict__ C,
     const float *__restrict__ nc, const float *__restrict__ bnd, const float2 *__restrict__ xnb,
     float inv_s2, float eta, float big, int *__restrict__ labels,
-    int *__restrict__ flags, int *__restrict__ nflag, int *__restrict__ ncand) {
+    int *__restrict__ flags, int *__restrict__ nflag, int *__restrict__ ncand,
+    const int *__restrict__ stop) {
+  // stop (nullable): set by a device-side convergence test; the launch then
+  // assigns nothing and lists nothing
+  if (stop && *stop) return;
   constexpr int Dp = KS * 16;
   constexpr int CPR = Dp / 8;               // 16-byte chunks per centre row
   constexpr int SROWS = MF_STAGE * MF_TILE; // centre rows per LDS stage
@@ -332,25 +336,30 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
 
   // Software pipeline: while the VALU runs the epilogue of tile t, the
   // matrix core computes tile t+1 (same basic block); stage s+1 streams into
-  // the other LDS buffer while stage s is consumed.  Tiles past the end are
-  // clamped to the last one and their scores are never read.
+  // the other LDS buffer while stage s is consumed.  nKT is a multiple of
+  // MF_STAGE (at least 4), so the loop covers tiles [0, nKT - 2) and the
+  // last pair is peeled: no tile past the end is computed and no barrier is
+  // waited at after the last stage.
   issue_stage(0, 0);
   __builtin_amdgcn_s_waitcnt(0);  // point fragments and stage 0 have arrived
   __syncthreads();
   f32x16_t A0, A1, B0, B1;
   mfma_tile(0, A0, A1);
-  for (int kt = 0; kt < nKT; kt += 2) {
+  int kt = 0;
+  for (; kt + 2 < nKT; kt += 2) {
     if (kt % MF_STAGE == 0 && kt / MF_STAGE + 1 < nS) issue_stage(kt / MF_STAGE + 1, (kt / MF_STAGE + 1) & 1);
-    mfma_tile(min(kt + 1, nKT - 1), B0, B1);
+    mfma_tile(kt + 1, B0, B1);
     epi_tile(kt, A0, A1);
-    if (kt + 1 >= nKT) break;
     if ((kt + 2) % MF_STAGE == 0) {  // next tile opens a new stage
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
     }
-    mfma_tile(min(kt + 2, nKT - 1), A0, A1);
+    mfma_tile(kt + 2, A0, A1);
     epi_tile(kt + 1, B0, B1);
   }
+  mfma_tile(kt + 1, B0, B1);  // kt = nKT - 2: a pair within the last stage
+  epi_tile(kt, A0, A1);
+  epi_tile(kt + 1, B0, B1);
 
   // merge the two lane halves (same points, different centre rows)
   const float cmax = bnd[0], ncmax = bnd[1], dcmax = bnd[2];
@@ -957,7 +966,7 @@ static void launch_mfma_ks(hipStream_t strm, bool fp16, dim3 g, long M, const ui
                            const float *X, int D, int K, int Kp, const uint16_t *Ch,
                            const float *bias, const float *C, const float *nc, const float *bnd,
                            const float2 *xnb, float inv_s2, float eta, int *labels,
-                           int *flags, int *nflag, int *ncand) {
+                           int *flags, int *nflag, int *ncand, const int *stop) {
   const size_t lds = 2 * (size_t)MF_STAGE * MF_TILE * (KS * 2) * 16 + 2 * MF_STAGE * MF_TILE * 4;
   static bool attr_set[2] = {false, false};
   if (!attr_set[fp16]) {
@@ -971,10 +980,10 @@ static void launch_mfma_ks(hipStream_t strm, bool fp16, dim3 g, long M, const ui
   }
   if (fp16)
     hipLaunchKernelGGL((k_assign_mfma<KS, true>), g, dim3(64 * MF_WAVES), lds, strm, M, Xh, X, D, K, Kp, Ch,
-                       bias, C, nc, bnd, xnb, inv_s2, eta, FLT_MAX, labels, flags, nflag, ncand);
+                       bias, C, nc, bnd, xnb, inv_s2, eta, FLT_MAX, labels, flags, nflag, ncand, stop);
   else
     hipLaunchKernelGGL((k_assign_mfma<KS, false>), g, dim3(64 * MF_WAVES), lds, strm, M, Xh, X, D, K, Kp, Ch,
-                       bias, C, nc, bnd, xnb, inv_s2, eta, FLT_MAX, labels, flags, nflag, ncand);
+                       bias, C, nc, bnd, xnb, inv_s2, eta, FLT_MAX, labels, flags, nflag, ncand, stop);
 }
 
 hipError_t mfma_assign(Engine *e, long r0, long rows) {
@@ -1008,12 +1017,12 @@ hipError_t mfma_assign_view(Engine *e, const float *X, const uint16_t *Xh, const
   const float inv_s2 = 1.0f / (s->scale * s->scale);
   dim3 g(km_cdiv(rows, MF_PTS));
   switch (s->KS) {
-    case 1: launch_mfma_ks<1>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2); break;
-    case 2: launch_mfma_ks<2>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2); break;
-    case 4: launch_mfma_ks<4>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2); break;
-    case 6: launch_mfma_ks<6>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2); break;
-    case 8: launch_mfma_ks<8>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2); break;
-    default: launch_mfma_ks<16>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2); break;
+    case 1: launch_mfma_ks<1>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop); break;
+    case 2: launch_mfma_ks<2>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop); break;
+    case 4: launch_mfma_ks<4>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop); break;
+    case 6: launch_mfma_ks<6>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop); break;
+    case 8: launch_mfma_ks<8>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop); break;
+    default: launch_mfma_ks<16>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop); break;
   }
   if ((r = hipGetLastError()) != hipSuccess) return r;
   return hipGetLastError();

@@ -1,15 +1,64 @@
 // Drives the kmeans<TYPE> class of include/kmeans_amd.hpp the way a caller of
 // the reference's class (main/cpp/kmeans.hpp:174-312) would, for the GPU
-// tests: compute() against the step-by-step loop, and the double class.
+// tests: compute() against the step-by-step loop, in float and in double,
+// and computeKmeansF called with the reference's default miniBatchFraction.
 // usage: class_api_test <points.f32> m n k iters <out.bin>
 // out.bin: centres n x k float32 (compute()), labels m int32, compactness
-// double, centres n x k float64 (kmeans<double>), step-by-step match int32.
+// double, centres n x k float64 (kmeans<double>::compute()), labels m int32
+// and compactness double (kmeans<double> step by step), then int32 flags:
+// float steps match, float convergence flag, double steps match, double
+// convergence flag, computeKmeansF (default last argument) matches compute().
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
 
 #include "kmeans_amd.hpp"
+
+template <class T>
+struct Run {
+  std::vector<T> centers;  // n x k
+  std::vector<int> labels;
+  double comp = 0.0;
+  int conv = 0;
+};
+
+// the whole loop through compute()
+template <class T>
+static int whole(const std::vector<T> &X, const std::vector<T> &C0, int m, int n, int k, int iters,
+                 Run<T> *r) {
+  kmeans_amd::kmeans<T> a(m, n, k, (T)-1, iters, 1, 1, 0, 0);
+  r->centers = C0;
+  if (a.buildData(X.data()) || a.initialize(r->centers.data()) || a.compute()) return 1;
+  if (a.getClusterCenters(r->centers.data())) return 2;
+  r->labels.resize(m);
+  if (a.getLabels(r->labels.data())) return 3;
+  r->comp = a.getCompactness();
+  r->conv = a.getConvergenceFlag();
+  return 0;
+}
+
+// the same iterations step by step
+template <class T>
+static int steps(const std::vector<T> &X, const std::vector<T> &C0, int m, int n, int k, int iters,
+                 Run<T> *r) {
+  kmeans_amd::kmeans<T> b(m, n, k, (T)-1, iters, 1, 1, 0, 0);
+  r->centers = C0;
+  if (b.buildData(X.data()) || b.initialize(r->centers.data())) return 1;
+  for (int it = 0; it < iters; ++it)
+    if (b.reset() || b.closestCenters() || b.clusterCenters() || b.compactness() || b.transpose())
+      return 2;
+  r->comp = b.getCompactness();
+  if (b.getClusterCenters(r->centers.data())) return 3;
+  r->labels.resize(m);
+  if (b.getLabels(r->labels.data())) return 4;
+  return 0;
+}
+
+template <class T>
+static bool same(const Run<T> &a, const Run<T> &b) {
+  return a.centers == b.centers && a.labels == b.labels && a.comp == b.comp;
+}
 
 int main(int argc, char **argv) {
   if (argc != 7) {
@@ -26,48 +75,35 @@ int main(int argc, char **argv) {
   std::vector<float> C0((size_t)n * k);
   for (int j = 0; j < k; ++j)
     for (int q = 0; q < n; ++q) C0[(size_t)q * k + j] = X[(size_t)j * n + q];
+  std::vector<double> Xd(X.begin(), X.end()), C0d(C0.begin(), C0.end());
 
-  // 1. the whole loop
-  kmeans_amd::kmeans<float> a(m, n, k, -1.0f, iters, 1, 1, 0, 0);
-  std::vector<float> ra(C0);
-  if (a.buildData(X.data()) || a.initialize(ra.data()) || a.compute()) return 4;
-  if (a.getClusterCenters(ra.data())) return 5;
-  std::vector<int> la(m);
-  if (a.getLabels(la.data())) return 6;
-  const double ca = a.getCompactness();
-  const int conv = a.getConvergenceFlag();
+  Run<float> fa, fb;
+  Run<double> da, db;
+  if (whole(X, C0, m, n, k, iters, &fa)) return 4;
+  if (steps(X, C0, m, n, k, iters, &fb)) return 5;
+  if (whole(Xd, C0d, m, n, k, iters, &da)) return 6;
+  if (steps(Xd, C0d, m, n, k, iters, &db)) return 7;
+  const int fmatch = same(fa, fb), dmatch = same(da, db);
 
-  // 2. the same iterations step by step
-  kmeans_amd::kmeans<float> b(m, n, k, -1.0f, iters, 1, 1, 0, 0);
-  std::vector<float> rb(C0);
-  if (b.buildData(X.data()) || b.initialize(rb.data())) return 7;
-  double cb = 0.0;
-  for (int it = 0; it < iters; ++it) {
-    if (b.reset() || b.closestCenters() || b.clusterCenters() || b.compactness() || b.transpose())
-      return 8;
-  }
-  cb = b.getCompactness();
-  if (b.getClusterCenters(rb.data())) return 9;
-  std::vector<int> lb(m);
-  if (b.getLabels(lb.data())) return 10;
-  const int match = std::memcmp(ra.data(), rb.data(), sizeof(float) * ra.size()) == 0 &&
-                    std::memcmp(la.data(), lb.data(), sizeof(int) * la.size()) == 0 && ca == cb;
-
-  // 3. the double class on the same points
-  std::vector<double> Xd(X.begin(), X.end()), rd(C0.begin(), C0.end());
-  kmeans_amd::kmeans<double> d(m, n, k, -1.0, iters, 1, 1, 0, 0);
-  if (d.buildData(Xd.data()) || d.initialize(rd.data()) || d.compute()) return 11;
-  if (d.getClusterCenters(rd.data())) return 12;
+  // computeKmeansF as a C++ caller of the reference writes it: the last
+  // argument (miniBatchFraction) left to its default of 1
+  std::vector<float> rc(C0);
+  const kmeansErrorStatus cst =
+      computeKmeansF(X.data(), m, n, k, -1.0f, iters, 1, 1, 0, 0, rc.data());
+  const int cmatch = (cst == KMEANS_NONCONVERGENCE) && rc == fa.centers;
 
   FILE *o = std::fopen(argv[6], "wb");
   if (!o) return 13;
-  std::fwrite(ra.data(), sizeof(float), ra.size(), o);
-  std::fwrite(la.data(), sizeof(int), la.size(), o);
-  std::fwrite(&ca, sizeof(double), 1, o);
-  std::fwrite(rd.data(), sizeof(double), rd.size(), o);
-  std::fwrite(&match, sizeof(int), 1, o);
-  std::fwrite(&conv, sizeof(int), 1, o);
+  std::fwrite(fa.centers.data(), sizeof(float), fa.centers.size(), o);
+  std::fwrite(fa.labels.data(), sizeof(int), fa.labels.size(), o);
+  std::fwrite(&fa.comp, sizeof(double), 1, o);
+  std::fwrite(da.centers.data(), sizeof(double), da.centers.size(), o);
+  std::fwrite(db.labels.data(), sizeof(int), db.labels.size(), o);
+  std::fwrite(&db.comp, sizeof(double), 1, o);
+  const int flags[5] = {fmatch, fa.conv, dmatch, da.conv, cmatch};
+  std::fwrite(flags, sizeof(int), 5, o);
   std::fclose(o);
-  std::printf("class api: step-by-step %s compute()\n", match ? "matches" : "DIFFERS FROM");
+  std::printf("class api: float steps %s compute(), double steps %s compute()\n",
+              fmatch ? "match" : "DIFFER FROM", dmatch ? "match" : "DIFFER FROM");
   return 0;
 }

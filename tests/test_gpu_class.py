@@ -1,7 +1,7 @@
 """The reference-style C++ class (include/kmeans_amd.hpp) on the GPU: a
 compiled program (tests/cpp/class_api.cpp, built with the library) runs the
 loop through compute(), then step by step through closestCenters /
-clusterCenters / compactness, and the double class; results against the
+clusterCenters / compactness, in float and in double; results against the
 oracle bit for bit."""
 import os
 import subprocess
@@ -31,12 +31,19 @@ def test_class_api_matches_oracle(orc, tmp_path, n, d, k, iters):
     lab = np.frombuffer(raw, np.int32, n, off); off += 4 * n
     comp = np.frombuffer(raw, np.float64, 1, off)[0]; off += 8
     Cd = np.frombuffer(raw, np.float64, d * k, off).reshape(d, k); off += 8 * d * k
-    match, conv = np.frombuffer(raw, np.int32, 2, off)
+    labd = np.frombuffer(raw, np.int32, n, off); off += 4 * n
+    compd = np.frombuffer(raw, np.float64, 1, off)[0]; off += 8
+    fmatch, fconv, dmatch, dconv, cmatch = np.frombuffer(raw, np.int32, 5, off)
     ref = orc.lloyd(X, X[:k].copy(), iters)
     assert np.array_equal(C, ref["centers"].T)
     assert np.array_equal(lab, ref["labels"])
     assert comp == ref["comp_hist"][-1]
-    assert match == 1  # step-by-step equals compute()
-    assert conv == 13  # criterion < 0 is never met
+    assert fmatch == 1  # float step-by-step equals compute()
+    assert fconv == 13  # criterion < 0 is never met
+    assert cmatch == 1  # computeKmeansF with the default miniBatchFraction
     ref64 = orc.lloyd64(X.astype(np.float64), X[:k].astype(np.float64), iters)
     assert np.array_equal(Cd, ref64["centers"].T)
+    assert dmatch == 1  # double step-by-step equals compute()
+    assert dconv == 13
+    assert np.array_equal(labd, ref64["labels"])
+    assert compd == ref64["comp_hist"][-1]

@@ -99,6 +99,42 @@ def test_mfma_fp32_search_split_over_centres_ties(orc, prec):
     e.close()
 
 
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
+def test_split_search_signed_zero_tie(orc, prec):
+    # x = (-1, -0, -0, -0): the zero centre 0 scores -0 (every product is -0)
+    # and the six centres at distance |x| in the next centre slices score +0.
+    # -0 and +0 are equal scores, so the label is the lowest index, 0.  The
+    # reduced-precision certificates cannot separate seven equal scores, so
+    # these points reach the FP32 search split over 64-centre slices.
+    print(f"signed-zero tie {prec}", flush=True)
+    rng = np.random.default_rng(5)
+    d, k = 4, 256
+    C0 = rng.uniform(20, 60, (k, d)).astype(np.float32)
+    C0[0] = 0.0
+    ties = {64: (-2, 0, 0, 0), 65: (-1, 1, 0, 0), 128: (-1, -1, 0, 0), 129: (-1, 0, 1, 0),
+            192: (-1, 0, -1, 0), 193: (-1, 0, 0, 1)}
+    for j, c in ties.items():
+        C0[j] = c
+    x = np.array([-1.0, -0.0, -0.0, -0.0], np.float32)
+    far = rng.integers(1, 64, 448)
+    X = np.concatenate([np.tile(x, (64, 1)),
+                        C0[far] + rng.standard_normal((448, d)).astype(np.float32) * 0.1])
+    X = X.astype(np.float32)
+    ref, best, _ = orc.assign(X, C0, with_scores=True)
+    assert np.all(ref[:64] == 0) and np.all(best[:64] == 0.0)
+    assert np.signbit(best[0])  # the zero centre's chain really is -0
+    e = km.KMeans(X.shape[0], d, k, precision=prec)
+    e.set_data(X)
+    e.set_centers(C0)
+    e.run(1)
+    lab = e.labels()
+    st = e.stats()
+    e.close()
+    assert np.array_equal(lab, ref)
+    if prec != "fp32":
+        assert st["reresolved"] >= 64  # the tie points took the split FP32 search
+
+
 @pytest.mark.parametrize("prec", ["bf16", "fp16"])
 def test_mfma_deep_pass_twins_in_one_tile_row(orc, prec):
     # centres j and j + 128 are near-twins in the same row of two different
