@@ -94,10 +94,15 @@ const char *kmeansGetErrorString(kmeansErrorStatus err);
 
 /* Replaces computeKmeansF (main/cpp/kmeans.hpp:102-107, kmeans.cpp:1316-1361).
  * data: m x n host, C-ordered.  result: n x k host (D x K).
- * Deliberate deltas: rows are not shuffled; initAlgorithm == 1 takes the
- * initial centres from result (D x K) and 2 seeds them by k-means++ (the
- * reference ignores initAlgorithm; any other value samples k distinct rows);
- * an empty cluster keeps its previous centre instead of becoming NaN.
+ * initAlgorithm: the reference ignores it and always starts from the rows
+ * its own shuffle and srand(42)/rand() pick select (kmeans.cpp:553-556,
+ * 876-888); here 0 and every value other than 1, 2 and 3 start from those
+ * same rows (kmeansReferenceInitRows), 1 takes the initial centres from
+ * result (D x K), 2 seeds them by k-means++ and 3 samples k distinct rows
+ * (Floyd's algorithm, seed 42).
+ * Deliberate deltas: the rows are not reordered (labels come back in the
+ * caller's order and the canonical sums run over the caller's order); an
+ * empty cluster keeps its previous centre instead of becoming NaN.
  * doFullSGEMM == 1 assigns through a library GEMM (rocBLAS) and a row argmin
  * as kmeans.cpp:918-992 does; otherwise the fused kernels.  matrixFormat and
  * numRetries are ignored (the reference ignores numRetries too).
@@ -199,6 +204,15 @@ kmeansErrorStatus kmeansSetCenters(kmeansHandle h, const float *centers);
  * initAlgorithm is ignored, kmeans.cpp:876-888).  Deterministic for a seed:
  * distances in FP64, the draw walks the canonical 256-way sum tree. */
 kmeansErrorStatus kmeansInitPlusPlus(kmeansHandle h, uint64_t seed);
+/* The k rows the reference's initialize takes as initial centres, as indices
+ * into the caller's (unshuffled) rows: rows[i] = indices[points[i]], where
+ * indices is 0..m-1 shuffled by std::shuffle with std::default_random_engine(0)
+ * (kmeans.cpp:553-556) and points[i] = rand() % m after srand(42), redrawn
+ * while std::binary_search over the earlier points (an unsorted vector, as in
+ * the reference) finds it (kmeans.cpp:876-883).  Computed with this
+ * process's libstdc++ and glibc routines (rand's sequence from a private
+ * state: the caller's rand() is untouched).  Host only, no device. */
+kmeansErrorStatus kmeansReferenceInitRows(long m, int k, int *rows);
 /* Lloyd loop: while (it < maxIters && relErr > criterion) -- kmeans.cpp:1194. */
 kmeansErrorStatus kmeansRun(kmeansHandle h, int maxIters, double criterion);
 kmeansErrorStatus kmeansGetCenters(kmeansHandle h, float *centers /* k x n host */);
@@ -206,7 +220,8 @@ kmeansErrorStatus kmeansGetLabels(kmeansHandle h, int *labels /* m host */);
 kmeansErrorStatus kmeansGetStats(kmeansHandle h, kmeansStats *stats);
 /* The reference class's steps, one call each (used by include/kmeans_amd.hpp):
  * initialize (kmeans.cpp:822-907): initAlgorithm 1 takes centers_dxk (n x k,
- * the reference's result layout), 2 seeds by k-means++, else k sampled rows;
+ * the reference's result layout), 2 seeds by k-means++, 3 samples k distinct
+ * rows with the seed, any other value takes kmeansReferenceInitRows' rows;
  * closestCenters (:911-1014): one assignment of every resident point;
  * clusterCenters (:1019-1041): new centres from the current labels;
  * compactness (:1045-1104): FP64 sum of squared distances to the current

@@ -117,7 +117,9 @@ class kmeans {
     return h_ ? A::setData(h_, data) : status_;
   }
   /* initAlgorithm 1: result (n x k) holds the initial centres; 2: k-means++;
-   * otherwise k sampled rows.  The initial centres are written to result. */
+   * 3: k rows sampled with seed 42; otherwise the rows the reference's own
+   * initialize picks (kmeansReferenceInitRows).  The initial centres are
+   * written to result. */
   virtual kmeansErrorStatus initialize(TYPE *result) {
     if (!h_) return status_;
     kmeansErrorStatus st = A::initialize(h_, initAlgorithm_, 42, result);

@@ -91,6 +91,7 @@ SIGNATURES = {
     "kmeansGetLabels": (_I, [_P, _P]),
     "kmeansGetStats": (_I, [_P, _P]),
     "kmeansSetMiniBatch": (_I, [_P, _L]),
+    "kmeansReferenceInitRows": (_I, [_L, _I, _P]),
     "kmeansDeviceCenters": (_P, [_P]),
     "kmeansDeviceLabels": (_P, [_P]),
     "kmeansSynchronize": (_I, [_P]),
@@ -194,26 +195,45 @@ def load_raw_f32(path, n, m=None):
 
 
 # ------------------------------------------------------------------ host API
+def _init_alg(init_algorithm, init_from_result):
+    if init_algorithm is not None:
+        return int(init_algorithm)
+    return 1 if init_from_result else 0
+
+
+def reference_init_rows(m, k):
+    """The k rows the reference's initialize picks as initial centres
+    (kmeansReferenceInitRows), as indices into the unshuffled rows."""
+    out = np.empty(k, np.int32)
+    _check(lib().kmeansReferenceInitRows(int(m), int(k), _np_ptr(out)), "kmeansReferenceInitRows")
+    return out
+
+
 def compute_kmeans_f(data, k, criterion, max_iters, result=None, init_from_result=False,
-                     mini_batch_fraction=1.0, full_gemm=False):
-    """computeKmeansF: returns (status, centers as D x K).  full_gemm: doFullSGEMM = 1."""
+                     mini_batch_fraction=1.0, full_gemm=False, init_algorithm=None):
+    """computeKmeansF: returns (status, centers as D x K).  full_gemm: doFullSGEMM = 1.
+    init_algorithm (default: 1 with init_from_result, else 0): 0 the reference's
+    own initial rows, 1 the centres in result, 2 k-means++, 3 seeded sampling."""
     data = np.ascontiguousarray(data, dtype=np.float32)
     m, n = data.shape
     res = np.zeros((n, k), np.float32) if result is None else np.ascontiguousarray(result, np.float32)
     st = lib().computeKmeansF(_np_ptr(data), m, n, k, criterion, max_iters, 0,
-                              1 if init_from_result else 0, 1 if full_gemm else 0, 0, _np_ptr(res),
+                              _init_alg(init_algorithm, init_from_result), 1 if full_gemm else 0, 0,
+                              _np_ptr(res),
                               mini_batch_fraction)
     return st, res
 
 
 def compute_kmeans_d(data, k, criterion, max_iters, result=None, init_from_result=False,
-                     mini_batch_fraction=1.0, full_gemm=False):
-    """computeKmeansD: returns (status, centers as D x K, float64).  full_gemm: doFullSGEMM = 1."""
+                     mini_batch_fraction=1.0, full_gemm=False, init_algorithm=None):
+    """computeKmeansD: returns (status, centers as D x K, float64).  full_gemm: doFullSGEMM = 1.
+    init_algorithm as for compute_kmeans_f."""
     data = np.ascontiguousarray(data, dtype=np.float64)
     m, n = data.shape
     res = np.zeros((n, k), np.float64) if result is None else np.ascontiguousarray(result, np.float64)
     st = lib().computeKmeansD(_np_ptr(data), m, n, k, criterion, max_iters, 0,
-                              1 if init_from_result else 0, 1 if full_gemm else 0, 0, _np_ptr(res),
+                              _init_alg(init_algorithm, init_from_result), 1 if full_gemm else 0, 0,
+                              _np_ptr(res),
                               mini_batch_fraction)
     return st, res
 

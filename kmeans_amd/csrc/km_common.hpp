@@ -74,6 +74,11 @@ inline void sample_rows(uint64_t seed, long m, int k, int *idx) {
   }
 }
 
+// The rows the reference's initialize picks as initial centres
+// (kmeans.cpp:553-556 shuffle, :876-888 pick), as indices into the caller's
+// unshuffled rows (km_engine.hip).
+void reference_init_rows(long m, int k, int *rows);
+
 // k-means++ seeding (km_init.hip): the k picked rows of X into idx (device)
 hipError_t kmeanspp(hipStream_t s, long m, int d, int k, const float *X, uint64_t seed, int *idx);
 hipError_t kmeanspp(hipStream_t s, long m, int d, int k, const double *X, uint64_t seed,

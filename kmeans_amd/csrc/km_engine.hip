@@ -6,11 +6,13 @@
 // replaces the centres with the means of its members, early return when
 // relErr <= criterion), computeKmeansF follows kmeans.cpp:1316-1361.
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
 #include <mutex>
 #include <new>
+#include <random>
 #include <vector>
 
 #include "km_engine.hpp"
@@ -187,13 +189,51 @@ kmeansErrorStatus Engine::transpose() {
   return KMEANS_SUCCESS;
 }
 
-// kmeans.cpp:822-907: k distinct rows as initial centres.  The reference
-// draws rand()%m after srand(42); this build draws k distinct indices with
-// Floyd's algorithm from the counter-based generator (seed, stream 4).
-kmeansErrorStatus Engine::initialize(uint64_t seed) {
+// The reference's initial centres without reordering the data.  Its
+// buildData stores row indices[i] of the caller's data as row i, with
+// indices = 0..m-1 shuffled by std::shuffle with std::default_random_engine(0)
+// (kmeans.cpp:553-556); initialize then takes rows points[0..k) of that
+// shuffled copy, points[i] = rand() % m after srand(42), redrawn while
+// std::binary_search over the points so far (an unsorted vector, so some
+// duplicates get through) finds it (kmeans.cpp:876-883).  Centre i is
+// therefore caller row indices[points[i]].  The same library routines are
+// called here; glibc's rand() is reproduced with a private random_r state of
+// the same type (128 bytes, seeded 42), so the caller's rand() sequence is
+// not disturbed.
+void reference_init_rows(long m, int k, int *rows) {
+  std::vector<int> indices(m);
+  for (long i = 0; i < m; ++i) indices[i] = (int)i;
+  std::shuffle(indices.begin(), indices.end(), std::default_random_engine(0));
+  char state[128];
+  struct random_data rd;
+  memset(&rd, 0, sizeof(rd));
+  memset(state, 0, sizeof(state));
+  initstate_r(42, state, sizeof(state), &rd);
+  auto next = [&]() {
+    int32_t r = 0;
+    random_r(&rd, &r);
+    return (int)(r % (int)m);
+  };
+  std::vector<int> points;
+  points.reserve(k);
+  for (int i = 0; i < k; ++i) {
+    int index = next();
+    while (std::binary_search(points.begin(), points.end(), index)) index = next();
+    points.push_back(index);
+  }
+  for (int i = 0; i < k; ++i) rows[i] = indices[points[i]];
+}
+
+// kmeans.cpp:822-907: k rows as initial centres.  reference: the rows the
+// reference picks (reference_init_rows); otherwise k distinct indices drawn
+// with Floyd's algorithm from the counter-based generator (seed, stream 4).
+kmeansErrorStatus Engine::initialize(uint64_t seed, bool reference) {
   if (!dX && !hX) return KMEANS_ERROR_INITIALIZE;
   std::vector<int> idx(k);
-  sample_rows(seed, m, k, idx.data());
+  if (reference)
+    reference_init_rows(m, k, idx.data());
+  else
+    sample_rows(seed, m, k, idx.data());
   if (streaming) {  // gather the rows on the host
     std::vector<float> c((size_t)k * n);
     for (int j = 0; j < k; ++j)
@@ -729,7 +769,8 @@ kmeansErrorStatus kmeansSynchronize(kmeansHandle h) {
 }
 
 // initialize (kmeans.cpp:822-907): 1 takes centres (n x k, the reference's
-// D x K result layout), 2 seeds by k-means++, anything else samples k rows.
+// D x K result layout), 2 seeds by k-means++, 3 samples k distinct rows with
+// the seed, anything else takes the rows the reference picks.
 kmeansErrorStatus kmeansInitialize(kmeansHandle h, int initAlgorithm, uint64_t seed,
                                    const float *centers_dxk) {
   if (!h) return KMEANS_ERROR_INVALID_ARGUMENT;
@@ -741,7 +782,8 @@ kmeansErrorStatus kmeansInitialize(kmeansHandle h, int initAlgorithm, uint64_t s
       for (int q = 0; q < e.n; ++q) c[(size_t)j * e.n + q] = centers_dxk[(size_t)q * e.k + j];
     return e.setCenters(c.data());
   }
-  return initAlgorithm == 2 ? e.initialize_pp(seed) : e.initialize(seed);
+  if (initAlgorithm == 2) return e.initialize_pp(seed);
+  return e.initialize(seed, initAlgorithm != 3);
 }
 
 // closestCenters (kmeans.cpp:911-1014): one assignment of every resident
@@ -784,6 +826,12 @@ kmeansErrorStatus kmeansCompactness(kmeansHandle h, double *out) {
   return KMEANS_SUCCESS;
 }
 
+kmeansErrorStatus kmeansReferenceInitRows(long m, int k, int *rows) {
+  if (m <= 0 || m > 0x7fffffffL || k <= 0 || k > m || !rows) return KMEANS_ERROR_INVALID_ARGUMENT;
+  km::reference_init_rows(m, k, rows);
+  return KMEANS_SUCCESS;
+}
+
 kmeansErrorStatus kmeansSetMiniBatch(kmeansHandle h, long rows) {
   if (!h || rows < 0 || rows > h->e.m) return KMEANS_ERROR_INVALID_ARGUMENT;
   h->e.mb_rows = rows;
@@ -809,7 +857,7 @@ static kmeansErrorStatus compute_f(const float *data, int m, int n, int k, doubl
   } else if (initAlgorithm == 2) {
     st = h->e.initialize_pp(42);
   } else {
-    st = h->e.initialize(42);
+    st = h->e.initialize(42, initAlgorithm != 3);
   }
   if (st != KMEANS_SUCCESS) goto done;
   if (miniBatchFraction < 1.0) {

@@ -94,7 +94,8 @@ struct Engine {
   kmeansErrorStatus create(long m_, int n_, int k_, const kmeansOptions *o);
   kmeansErrorStatus buildData(const float *data, int on_device);
   kmeansErrorStatus setCenters(const float *centers_kxn_host);
-  kmeansErrorStatus initialize(uint64_t seed);  // k distinct sampled rows
+  // k sampled rows (Floyd), or with reference the rows the reference picks
+  kmeansErrorStatus initialize(uint64_t seed, bool reference = false);
   kmeansErrorStatus initialize_pp(uint64_t seed);  // k-means++ seeding (resident data)
   kmeansErrorStatus reset();
   kmeansErrorStatus closestCenters(long r0, long rows);

@@ -138,7 +138,8 @@ struct EngineD {
   }
 
   // initialize (kmeans.cpp:822-907): 1 takes centres_dxk (n x k, the
-  // reference's result layout), 2 seeds by k-means++, else k sampled rows
+  // reference's result layout), 2 seeds by k-means++, 3 samples k distinct
+  // rows with the seed, anything else takes the rows the reference picks
   kmeansErrorStatus initialize(int initAlgorithm, uint64_t seed, const double *centers_dxk) {
     if (!dXb) return KMEANS_ERROR_INITIALIZE;
     if (initAlgorithm == 1) {
@@ -153,7 +154,10 @@ struct EngineD {
       KM_HIP_CHECK(kmeanspp(stream, m, n, k, dXb, seed, didx), KMEANS_ERROR_INITIALIZE);
     } else {
       std::vector<int> idx(k);
-      sample_rows(seed, m, k, idx.data());
+      if (initAlgorithm == 3)
+        sample_rows(seed, m, k, idx.data());
+      else
+        reference_init_rows(m, k, idx.data());
       KM_HIP_CHECK(hipMemcpyAsync(didx, idx.data(), sizeof(int) * k, hipMemcpyHostToDevice,
                                   stream),
                    KMEANS_ERROR_INITIALIZE);
@@ -276,8 +280,7 @@ kmeansErrorStatus compute_d(const double *data, int m, int n, int k, double crit
   kmeansErrorStatus st;
   if ((st = E.create(m, n, k, &opt)) != KMEANS_SUCCESS) return st;
   if ((st = E.setData(data, 0)) != KMEANS_SUCCESS) return st;
-  if ((st = E.initialize(initAlgorithm == 1 || initAlgorithm == 2 ? initAlgorithm : 0, 42,
-                         result)) != KMEANS_SUCCESS)
+  if ((st = E.initialize(initAlgorithm, 42, result)) != KMEANS_SUCCESS)
     return st;
   if (miniBatchFraction < 1.0) {
     E.mb = (long)(miniBatchFraction * m);

@@ -151,6 +151,18 @@ int orc_sample_init(uint64_t seed, long n, int k, int *idx) {
 }
 
 /* ------------------------------------------------------------------ */
+/* rowNormalizeF (Normalize.cu:3-31, KmeansCudaKernels.h:428-432): out[i] =
+ * fma chain over d ascending of x_id*x_id, the same chain as the centre
+ * norms (the reference adds the squares in per-thread partial sums and a
+ * block tree; this build fixes one order, the sequential one). */
+void orc_row_norms(const float *X, long n, int d, float *out) {
+  for (long i = 0; i < n; ++i) {
+    float s = 0.0f;
+    for (int q = 0; q < d; ++q) s = fmaf(X[i * d + q], X[i * d + q], s);
+    out[i] = s;
+  }
+}
+
 /* Centre norms: nc_j = fma chain over d of c_jd*c_jd (Normalize.cu:41-44). */
 void orc_col_norms(const float *C, int k, int d, float *nc) {
   for (int j = 0; j < k; ++j) {

@@ -39,6 +39,7 @@ def lib():
             "orc_gen_points": (None, [u64, i64, i64, i32, i32, P, P, P, P, P]),
             "orc_sample_init": (i32, [u64, i64, i32, P]),
             "orc_col_norms": (None, [P, i32, i32, P]),
+            "orc_row_norms": (None, [P, i64, i32, P]),
             "orc_assign": (None, [P, i64, i32, P, i32, P, P, P, P]),
             "orc_score": (f32, [P, P, i32, f32]),
             "orc_csum": (f64, [P, i64]),
@@ -204,6 +205,14 @@ def compactness_fp64(X, labels, C):
 # ---------------------------------------------------------------- double precision
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def row_norms(X):
+    """rowNormalizeF: the fma chain of each row's squares (float32)."""
+    X = _f32(X)
+    out = np.empty(X.shape[0], np.float32)
+    lib().orc_row_norms(_p(X), X.shape[0], X.shape[1], _p(out))
+    return out
 
 
 def col_norms64(C):

@@ -162,3 +162,38 @@ def test_cpp_class_header_compiles():
                         os.path.join(ROOT, "include"), "-x", "c++", "-"], input=src, text=True,
                        capture_output=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_reference_init_rows_follow_the_reference_steps():
+    """kmeansReferenceInitRows against a program that runs the reference's
+    own steps (kmeans.cpp:553-556 shuffle, :876-883 pick with the global
+    srand/rand), for small and large m, k close to m (where the unsorted
+    binary_search lets duplicates through) and k = m."""
+    import subprocess
+    helper = os.path.join(ROOT, "tests", "cpp", "reference_init")
+    assert os.path.exists(helper), "build the library first (it builds the helper too)"
+    for m, k in [(10, 3), (1000, 8), (64, 60), (5, 5), (100_000, 1024)]:
+        out = subprocess.run([helper, str(m), str(k)], capture_output=True, text=True,
+                             check=True).stdout.split()
+        assert kmeans_amd.reference_init_rows(m, k).tolist() == [int(v) for v in out], (m, k)
+    # known index sequences (this image's libstdc++ and glibc)
+    assert kmeans_amd.reference_init_rows(10, 3).tolist() == [4, 2, 7]
+    assert kmeans_amd.reference_init_rows(1000, 8).tolist() == [946, 542, 598, 509, 471, 220,
+                                                                299, 710]
+    rows = kmeans_amd.reference_init_rows(64, 60)
+    assert len(set(rows.tolist())) < 60  # the reference's dedup misses some repeats
+
+
+def test_reference_init_rows_leave_the_callers_rand_alone():
+    libc = ctypes.CDLL("libc.so.6")
+    libc.srand(42)
+    first = [libc.rand() for _ in range(3)]
+    assert first == [71876166, 708592740, 1483128881]  # glibc's srand(42) stream
+    libc.srand(7)
+    a = libc.rand()
+    libc.srand(7)
+    kmeans_amd.reference_init_rows(1000, 50)
+    assert libc.rand() == a
+    L = kmeans_amd.lib()
+    out = np.zeros(4, np.int32)
+    assert L.kmeansReferenceInitRows(3, 4, out.ctypes.data_as(ctypes.c_void_p)) == 14

@@ -43,7 +43,7 @@ def test_compute_kmeans_d_criterion_and_sampled_init(orc):
     # 1e-5 stops after 5 iterations; 1e-7 is still unmet after 60
     for crit, status in ((1e-5, km.KMEANS_SUCCESS), (1e-7, km.KMEANS_NONCONVERGENCE)):
         ref = orc.lloyd64(X, X[idx], 60, criterion=crit)
-        st, res = km.compute_kmeans_d(X, 10, crit, 60)
+        st, res = km.compute_kmeans_d(X, 10, crit, 60, init_algorithm=3)
         assert st == status == (0 if ref["status"] == 0 else km.KMEANS_NONCONVERGENCE)
         assert np.array_equal(res, ref["centers"].T)
 

@@ -239,18 +239,55 @@ def test_engine_update_overlaps_four_blocks(orc):
     e.close()
 
 
-def test_engine_criterion_stops_like_reference(orc):
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_engine_criterion_stops_like_reference(orc, prec):
+    # the convergence test runs on the device: the iterations enqueued past
+    # the converged one must leave its labels, centres and compactness as
+    # they are, and the statistics must count iterations up to it
     X, _, _, _ = orc.mixture(12, 30_000, 8, 10, -10, 10, 0.7, 0.7)
     C0 = X[:10].copy()
     ref = orc.lloyd(X, C0, 50, criterion=1e-6)
-    e = km.KMeans(X.shape[0], 8, 10)
+    e = km.KMeans(X.shape[0], 8, 10, precision=prec, timing=True)
     e.set_data(X)
     e.set_centers(C0)
     e.run(50, 1e-6)
     st = e.stats()
     assert st["iterations"] == ref["iters"] < 50
     assert st["rel_err"] == ref["rel_err"]
+    assert st["compactness"] == ref["comp_hist"][-1]
     assert np.array_equal(e.centers(), ref["centers"])
+    assert np.array_equal(e.labels(), ref["labels"])
+    # a second run from the converged state: its first iteration compares
+    # against the converged compactness (the oracle's fresh run against 0,
+    # so it always continues); from the second iteration on both compare
+    # consecutive compactness values
+    e.run(50, 1e-6)
+    ref3 = orc.lloyd(X, ref["centers"], 50, criterion=1e-6)
+    c1 = ref3["comp_hist"][0]
+    first_stops = abs(c1 - ref["comp_hist"][-1]) / abs(c1) <= 1e-6
+    n2 = 1 if first_stops else ref3["iters"]
+    assert e.stats()["iterations"] == ref["iters"] + n2
+    if not first_stops:
+        assert np.array_equal(e.labels(), ref3["labels"])
+        assert np.array_equal(e.centers(), ref3["centers"])
+    e.close()
+
+
+@pytest.mark.parametrize("crit", [-1.0, 0.0])
+def test_engine_counts_empty_clusters_per_iteration(orc, crit):
+    # centre 4 attracts no point in any iteration: one empty cluster per
+    # iteration, with a fixed count and with the device-side convergence test
+    X, _, _, _ = orc.mixture(19, 8_000, 5, 4, -6, 6, 0.5, 0.5)
+    C0 = np.concatenate([X[:4], np.full((1, 5), 1e6, np.float32)])
+    e = km.KMeans(X.shape[0], 5, 5)
+    e.set_data(X)
+    e.set_centers(C0)
+    e.run(9, crit)
+    st = e.stats()
+    ref = orc.lloyd(X, C0, 9, criterion=crit)
+    assert st["iterations"] == ref["iters"]
+    assert st["empty_clusters"] == st["iterations"]
+    e.close()
 
 
 def test_engine_device_data_borrowed(orc):
@@ -273,6 +310,24 @@ def test_compute_kmeans_f_with_given_init(orc):
     ref = orc.lloyd(X, C0, 7)
     assert st == km.KMEANS_NONCONVERGENCE  # criterion < 0 can never be met
     assert np.array_equal(res, ref["centers"].T)
+
+
+@pytest.mark.parametrize("alg", [0, 7])
+def test_compute_kmeans_starts_from_the_reference_rows(orc, alg):
+    # initAlgorithm other than 1, 2, 3: the rows the reference's own shuffle
+    # and srand(42) pick select (the reference ignores initAlgorithm)
+    X, _, _, _ = orc.mixture(18, 20_000, 6, 12, -8, 8, 0.8, 0.8)
+    rows = km.reference_init_rows(X.shape[0], 12)
+    st, res = km.compute_kmeans_f(X, 12, -1.0, 0, init_algorithm=alg)
+    assert np.array_equal(res, X[rows].T)
+    ref = orc.lloyd(X, X[rows].copy(), 5)
+    st, res = km.compute_kmeans_f(X, 12, -1.0, 5, init_algorithm=alg)
+    assert st == km.KMEANS_NONCONVERGENCE
+    assert np.array_equal(res, ref["centers"].T)
+    X64 = X.astype(np.float64)
+    ref64 = orc.lloyd64(X64, X64[rows].copy(), 5)
+    st, res64 = km.compute_kmeans_d(X64, 12, -1.0, 5, init_algorithm=alg)
+    assert np.array_equal(res64, ref64["centers"].T)
 
 
 def test_compute_kmeans_f_sampled_init_converges(orc):
@@ -306,6 +361,11 @@ def test_legacy_kernel_names(orc):
     Xd, B = dev(X), dev(C.T.copy())  # D x K as the reference passes it
     ncg = torch.empty(k, dtype=torch.float32, device="cuda")
     assert L.colNormalizeF(d, k, km._vp(B.data_ptr()), km._vp(ncg.data_ptr())) == 0
+    # rowNormalizeF of two tiles into one output at offsets m0 (Normalize.cu:28)
+    rn = torch.full((n,), -1.0, dtype=torch.float32, device="cuda")
+    for m0, m in ((0, 3001), (3001, n - 3001)):
+        assert L.rowNormalizeF(m0, m, d, km._vp(Xd.data_ptr() + 4 * m0 * d),
+                               km._vp(rn.data_ptr())) == 0
     lab = torch.empty(n, dtype=torch.int32, device="cuda")
     for m0 in (0, 2500):  # two tiles
         m = 2500 if m0 == 0 else n - 2500
@@ -314,9 +374,10 @@ def test_legacy_kernel_names(orc):
                                  None, None, km._vp(lab.data_ptr())) == 0
     torch.cuda.synchronize()
     assert np.array_equal(ncg.cpu().numpy(), orc.col_norms(C))
+    assert np.array_equal(rn.cpu().numpy(), orc.row_norms(X))
     ref = orc.assign(X, C)
     assert np.array_equal(lab.cpu().numpy(), ref)
-    large = torch.zeros(4 * k * d, dtype=torch.float32, device="cuda")
+    large =torch.zeros(4 * k * d, dtype=torch.float32, device="cuda")
     clarge = torch.zeros(4 * k, dtype=torch.int32, device="cuda")
     cen = torch.empty((k, d), dtype=torch.float32, device="cuda")
     cnt = torch.empty(k, dtype=torch.int32, device="cuda")

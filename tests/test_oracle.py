@@ -50,7 +50,20 @@ def test_norm_and_score_are_single_rounding_fma_chains(orc):
             c = rng.standard_normal(d).astype(np.float32) * 3
             nc = orc.col_norms(c[None, :])[0]
             assert float(nc) == exact_norm(c)
+            assert float(orc.row_norms(c[None, :])[0]) == exact_norm(c)
             assert float(orc.score(x, c, nc)) == exact_chain(x, c, nc)
+
+
+def test_row_norms_fma_chain_known_answer(orc):
+    # each row against the chain computed in exact rationals with one
+    # float32 rounding per step: the same squares in both orders, an exact
+    # case (3-4-5) and squares below the smallest subnormal (they vanish)
+    rows = np.array([[1.0 + 2.0 ** -12, 1.0], [1.0, 1.0 + 2.0 ** -12], [3.0, 4.0],
+                     [2.0 ** -70, 2.0 ** -70]], np.float32)
+    got = orc.row_norms(rows)
+    for r, g in zip(rows, got):
+        assert float(g) == exact_norm(r)
+    assert float(got[2]) == 25.0
 
 
 def test_fma_chain_differs_from_mul_add(orc):
