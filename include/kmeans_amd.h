@@ -353,6 +353,23 @@ kmeansCudaErrorStatus kmUpdate(void *stream, long n, int d, int k, const float *
                                float *Ct_new, float *nc_new, long long *counts, int *empty,
                                void *workspace, size_t workspace_bytes);
 
+/* kmUpdate without a workspace argument.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  The workspace
+ * is a fixed device array of KM_NOALLOC_SCRATCH_BYTES inside the library
+ * (one per device, held until the process ends), so shapes whose
+ * kmUpdateWorkspaceSize(n, d, k) exceeds it are rejected with
+ * ERROR_INVALID_ARGUMENT (about n = 1.5M points at d = 128, k = 1024).  The
+ * call allocates nothing and clears its accumulators with a kernel.  All
+ * calls on one device share the array: calls that overlap on one device
+ * (other streams, other threads) race; calls on one stream are ordered.
+ * Results are those of kmUpdate bit for bit. */
+#define KM_NOALLOC_SCRATCH_BYTES (32L << 20)
+kmeansCudaErrorStatus kmUpdateNoAlloc(void *stream, long n, int d, int k, const float *X,
+                                      const int *labels, const float *C_old, float *C_new,
+                                      float *Ct_new, float *nc_new, long long *counts,
+                                      int *empty);
+
 /* Replaces CompactnessF (KmeansCudaKernels.h:389-394, Compactness.cu:3-96):
  * out[0] (device double) = canonical FP64 sum of ||x_i - C[l_i]||^2. */
 kmeansCudaErrorStatus kmCompactness(void *stream, long n, int d, int k, const float *X,

@@ -30,6 +30,7 @@ PREC = {"fp32": 0, "bf16": 1, "fp16": 2}
 PROPOSER = {None: 0, "none": 0, "mlp": 1, "centroid": 2}
 KM_CHUNK = 256
 KM_BLOCK = 1 << 23
+KM_NOALLOC_SCRATCH_BYTES = 32 << 20  # kmUpdateNoAlloc's per-device scratch array
 
 ERRORS = {
     1: "KMEANS_ERROR_BUILDDATA", 2: "KMEANS_ERROR_INITIALIZE", 3: "KMEANS_ERROR_CLOSEST_CENTERS",
@@ -127,6 +128,7 @@ SIGNATURES = {
     "kmAssignF32": (_I, [_P, _L, _P, _I, _P, _P, _I, _P, _P, _P, _P]),
     "kmUpdateWorkspaceSize": (_SZ, [_L, _I, _I]),
     "kmUpdate": (_I, [_P, _L, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _SZ]),
+    "kmUpdateNoAlloc": (_I, [_P, _L, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     "kmCompactness": (_I, [_P, _L, _I, _I, _P, _P, _P, _P, _P, _SZ]),
     "kmRowNorms": (_I, [_P, _L, _L, _I, _P, _P]),
     "kmGenerateMixture": (_I, [_P, _U64, _L, _L, _I, _I, _P, _P, _P, _P, _P]),
@@ -436,6 +438,15 @@ def update(n, d, k, X_ptr, labels_ptr, C_old_ptr, C_new_ptr, ws_ptr, ws_bytes, C
     _check(lib().kmUpdate(_vp(stream), n, d, k, _vp(X_ptr), _vp(labels_ptr), _vp(C_old_ptr),
                           _vp(C_new_ptr), _vp(Ct_ptr), _vp(nc_ptr), _vp(counts_ptr), _vp(empty_ptr),
                           _vp(ws_ptr), ws_bytes), "kmUpdate")
+
+
+def update_no_alloc(n, d, k, X_ptr, labels_ptr, C_old_ptr, C_new_ptr, Ct_ptr=None, nc_ptr=None,
+                    counts_ptr=None, empty_ptr=None, stream=None):
+    """kmUpdateNoAlloc: kmUpdate with its workspace in the library's fixed
+    per-device scratch array (no allocation; shapes beyond it are rejected)."""
+    _check(lib().kmUpdateNoAlloc(_vp(stream), n, d, k, _vp(X_ptr), _vp(labels_ptr),
+                                   _vp(C_old_ptr), _vp(C_new_ptr), _vp(Ct_ptr), _vp(nc_ptr),
+                                   _vp(counts_ptr), _vp(empty_ptr)), "kmUpdateNoAlloc")
 
 
 def compactness(n, d, k, X_ptr, labels_ptr, C_ptr, out_ptr, ws_ptr, ws_bytes, stream=None):
