@@ -298,7 +298,8 @@ typedef struct kmeansBatchEngine *kmeansBatchHandle;
 kmeansErrorStatus kmeansBatchCreate(kmeansBatchHandle *h, int instances, const long *n, int d,
                                     int k, const kmeansOptions *opt);
 void kmeansBatchDestroy(kmeansBatchHandle h);
-/* All points (sum n[i] x d), host (copied) or device (borrowed). */
+/* All points (sum n[i] x d), host (copied) or device (borrowed; 16-byte
+ * aligned when d is a multiple of 4, else KMEANS_ERROR_INVALID_ARGUMENT). */
 kmeansErrorStatus kmeansBatchSetData(kmeansBatchHandle h, const float *X, int on_device);
 /* Initial centres, instances x k x d, host or device. */
 kmeansErrorStatus kmeansBatchSetCenters(kmeansBatchHandle h, const float *C, int on_device);

@@ -1315,6 +1315,12 @@ kmeansErrorStatus kmeansBatchSetData(kmeansBatchHandle h, const float *X, int on
   Batch *b = &h->b;
   const size_t bytes = sizeof(float) * (size_t)b->total * b->d;
   if (on_device) {
+    // the kernels read rows as 16-byte pieces when d is a multiple of 4
+    if (b->d % 4 == 0 && ((uintptr_t)X & 15) != 0) {
+      fprintf(stderr, "kmeans_amd: kmeansBatchSetData: borrowed device points must be 16-byte "
+                      "aligned when d is a multiple of 4\n");
+      return KMEANS_ERROR_INVALID_ARGUMENT;
+    }
     b->X = X;
   } else {
     if (!b->X_owned) BCHK(hipMalloc(&b->X_owned, bytes), KMEANS_ERROR_BUILDDATA);

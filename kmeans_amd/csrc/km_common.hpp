@@ -94,6 +94,9 @@ struct UpdateWs {
   double *P, *S, *compj, *qb;  // S: k x (d+1), column d = sum of q_i
   long long *cnt;
   size_t bytes;
+  // nullable: an n-sized array that receives each block's stable sort by
+  // label at the block's own rows (instead of perm), kept for the next assign
+  int *perm_out;
 };
 size_t update_ws_layout(long n, int d, int k, char *base, UpdateWs *ws);
 // Streams, events and the two per-block workspaces of the overlapped update

@@ -45,6 +45,7 @@ Engine::~Engine() {
   hipFree(dxn);
   hipFree(dws);
   hipFree(dws2);
+  hipFree(dorder);
   for (auto &s : upd_side)
     if (s) hipStreamDestroy(s);
   for (auto &e : upd_ev)
@@ -110,6 +111,8 @@ kmeansErrorStatus Engine::create(long m_, int n_, int k_, const kmeansOptions *o
     for (auto &e : upd_ev)
       KM_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming), KMEANS_ERROR_BUILDDATA);
   }
+  if (opt.precision != KM_PREC_FP32 && opt.tile_rows == 0)
+    KM_HIP_CHECK(hipMalloc(&dorder, sizeof(int) * (size_t)m), KMEANS_ERROR_BUILDDATA);
   return ensure_comp_slots(64);
 }
 
@@ -293,6 +296,11 @@ kmeansErrorStatus Engine::closestCenters(long r0, long rows) {
 
 kmeansErrorStatus Engine::clusterCenters(long r0, long rows, int *empty) {
   int *dempty = empty ? empty : this->dempty;
+  // a full update over all resident points leaves its sort by label in
+  // dorder for the next assign
+  const bool keep_order = dorder && r0 == 0 && rows == m;
+  ws.perm_out = keep_order ? dorder : nullptr;
+  ws2.perm_out = ws.perm_out;
   if (rows > KM_BLOCK && upd_side[0]) {
     // several canonical blocks: sorts and chunk sums of alternate blocks on
     // two side streams, the in-order additions on the engine's stream
@@ -311,6 +319,7 @@ kmeansErrorStatus Engine::clusterCenters(long r0, long rows, int *empty) {
                                ldc, dnc, dcounts, dempty, ws),
                  KMEANS_ERROR_CLUSTER_CENTERS);
   }
+  if (keep_order) order_ok = true;
   if (opt.precision != KM_PREC_FP32) KM_HIP_CHECK(mfma_centers(this), KMEANS_ERROR_CLUSTER_CENTERS);
   return KMEANS_SUCCESS;
 }

@@ -45,6 +45,11 @@ struct Engine {
   hipEvent_t upd_ev[5] = {};
   char *dws2 = nullptr;
   UpdateWs ws2{};
+  // BF16/FP16 engines over resident points: the last full update's stable
+  // sort of the points by label (order_ok once written), the order in which
+  // the next assign takes them
+  int *dorder = nullptr;
+  bool order_ok = false;
   // precision path buffers (allocated on first use)
   void *mfma = nullptr;
   // candidate proposer state (options.proposer != 0)

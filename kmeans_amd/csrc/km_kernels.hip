@@ -973,6 +973,7 @@ size_t update_ws_layout(long n, int d, int k, char *base, UpdateWs *ws) {
   w.compj = (double *)take(sizeof(double) * (k + 1));
   w.qb = (double *)take(sizeof(double) * (nb > 0 ? nb : 1));
   w.bytes = off;
+  w.perm_out = nullptr;
   if (ws) *ws = w;
   return off;
 }
@@ -1027,8 +1028,9 @@ static hipError_t block_sums(hipStream_t s, long i0, long nb, int d, int k, cons
   hipLaunchKernelGGL(k_hist, dim3(nsb), dim3(256), sizeof(int) * k, s, labels + i0, nb, k, nsb,
                      SB, ws.hist);
   if ((e = scan_excl(s, ws.hist, L, ws.tsum, ws.offs)) != hipSuccess) return e;
+  int *perm = ws.perm_out ? ws.perm_out + i0 : ws.perm;
   hipLaunchKernelGGL(k_scatter, dim3(nsb), dim3(64), sizeof(int) * k, s, labels + i0, i0, nb, k,
-                     nsb, SB, nbits, ws.offs, ws.perm);
+                     nsb, SB, nbits, ws.offs, perm);
   hipLaunchKernelGGL(k_cluster_setup, dim3(1), dim3(256), 0, s, k, nsb, ws.offs, ws.cstart,
                      ws.ccount, ws.cq);
   const long qmax = nb / KM_CHUNK + k + 1;
@@ -1038,10 +1040,10 @@ static hipError_t block_sums(hipStream_t s, long i0, long nb, int d, int k, cons
     const int G4 = 256 / TPC < 64 ? 256 / TPC : 64;  // must match k_chunk_sums4
     hipLaunchKernelGGL(k_chunk_sums4, dim3(km_cdiv(qmax, G4)), dim3(320), 0, s,
                        reinterpret_cast<const float *>(X), d, TPC, qb,
-                       ws.perm, k, ws.cstart, ws.ccount, ws.cq, ws.P);
+                       perm, k, ws.cstart, ws.ccount, ws.cq, ws.P);
   } else {
     hipLaunchKernelGGL(k_chunk_sums<T>, dim3(km_cdiv(qmax, G)), dim3(320), 0, s, X, d, Dp, qb,
-                       ws.perm, k, ws.cstart, ws.ccount, ws.cq, ws.P);
+                       perm, k, ws.cstart, ws.ccount, ws.cq, ws.P);
   }
   return hipGetLastError();
 }

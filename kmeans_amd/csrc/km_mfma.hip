@@ -50,6 +50,9 @@ typedef __attribute__((ext_vector_type(4))) unsigned int u32x4_t;
 #define NC_SPREAD 64
 #define NC_STRIDE 32
 #define NC_SLOT (NC_SPREAD * NC_STRIDE)
+// largest k with the label-ordered assign (its tile table costs k^2 d flops
+// per centre update)
+#define MF_ORD_MAX_K 8192
 
 // Round-to-nearest-even conversions (the hardware instructions; the
 // certificate only needs the rounded values to be those the MFMAs see).
@@ -82,6 +85,13 @@ struct MfmaState {
   int *flags2 = nullptr;    // m: points left open by the deep pass
   int *nflag2 = nullptr;    // per iteration slot
   unsigned long long *kbest = nullptr;  // split_rows_cap() keys of the FP32 search, zeroed
+  // label-ordered assign: ttab[j][0..nKT) = the centre tiles by increasing
+  // distance from centre j (k <= MF_ORD_MAX_K), home[wg] = the previous label
+  // of a workgroup's first point
+  int *ttab = nullptr;
+  int *home = nullptr;
+  long home_cap = 0;
+  bool ttab_ok = false;
   int nflag_slots = 0;
   bool x_ready = false;
   bool c_ready = false;
@@ -173,6 +183,48 @@ __global__ __launch_bounds__(256) void k_centre_bounds(int k, int d, const float
   }
 }
 
+// ttab[j][rank] = the centre tiles (MF_TILE rows each) in increasing order of
+// the smallest squared distance from centre j to a member (ties to the lower
+// tile), one workgroup per centre.  FP32 distances from the norms and the
+// transposed centres; only an order of visit, so rounding does not matter.
+__global__ __launch_bounds__(256) void k_tile_order(int K, int D, int nKT,
+                                                    const float *__restrict__ Ct, int ldc,
+                                                    const float *__restrict__ nc,
+                                                    int *__restrict__ ttab) {
+  extern __shared__ float to_smem[];
+  float *cj = to_smem;                                              // [D]
+  unsigned *key = reinterpret_cast<unsigned *>(to_smem + D);       // [nKT]
+  const int j = blockIdx.x, tid = threadIdx.x;
+  for (int d = tid; d < D; d += 256) cj[d] = Ct[(long)d * ldc + j];
+  for (int t = tid; t < nKT; t += 256) key[t] = 0x7F800000u;  // +inf
+  __syncthreads();
+  const float ncj = nc[j];
+  for (int i = tid; i < K; i += 256) {
+    float dot = 0.0f;
+    for (int d = 0; d < D; ++d) dot = __builtin_fmaf(cj[d], Ct[(long)d * ldc + i], dot);
+    const float d2 = fmaxf(ncj + nc[i] - 2.0f * dot, 0.0f);
+    atomicMin(&key[i / MF_TILE], __float_as_uint(d2));  // non-negative: bits order as values
+  }
+  __syncthreads();
+  for (int t = tid; t < nKT; t += 256) {
+    const unsigned kt = key[t];
+    int r = 0;
+    for (int u = 0; u < nKT; ++u) r += (key[u] < kt) || (key[u] == kt && u < t);
+    ttab[(long)j * nKT + r] = t;
+  }
+}
+
+// home[w] = the label (from the previous assign) of the first point of
+// workgroup w's slice of the order, clamped into [0, K)
+__global__ void k_wg_home(long nwg, long M, int K, const int *__restrict__ order,
+                          const int *__restrict__ labels, int *__restrict__ home) {
+  const long w = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nwg) return;
+  const long q = w * MF_PTS < M ? w * MF_PTS : M - 1;
+  const int l = labels[order[q]];
+  home[w] = l < 0 ? 0 : (l >= K ? K - 1 : l);
+}
+
 // top-3 of packed values: b1 >= b2 >= b3
 // (max written as med3 with a run-time +FLT_MAX the compiler cannot fold:
 // one instruction, no NaN canonicalisation of the bit-packed input)
@@ -187,19 +239,62 @@ __device__ inline void top3_insert(float p, float &b1, float &b2, float &b3, flo
 struct Top3 {
   float b1 = -FLT_MAX, b2 = -FLT_MAX, b3 = -FLT_MAX;
   int t1 = 0, t2 = 0;
+  // label-ordered assign: U = the largest score of the tiles whose insertion
+  // was skipped, th = the skip threshold, mg = 2E in MFMA score units
+  float U = -INFINITY, th = -INFINITY, mg = 0.0f;
 };
+
+// Forward-error bound E (FP32 score units) of every centre's approximate
+// score of one point: xn >= |x|, ex >= |x - xh/s|, the centre bounds cmax,
+// ncmax, dcmax, eta the FP16 subnormal margin (0 for BF16).  The terms are
+// those of the certificate described at the top of this file.
+__host__ __device__ inline float mf_err_bound(float xn, float ex, float Df, float cmax,
+                                              float ncmax, float dcmax, float eta) {
+  const float g32 = Df * 5.9604645e-8f * 1.0001f;         // gamma_D, u = 2^-24
+  const float gmf = (Df + 1.0f) * 2.3841858e-7f * 1.001f;  // gamma'_{D+1}, u = 2^-22
+  const float rnd = ex * cmax + (xn + ex) * dcmax;
+  // FP16 engines: margin for subnormal operands the matrix core may flush
+  const float etat = eta * sqrtf(Df) * (xn + cmax) + eta * eta * Df;
+  const float mfe = gmf * (0.5f * ncmax + (xn + ex) * (cmax + dcmax) + etat);
+  const float fpe = g32 * (0.5f * ncmax + xn * cmax);
+  return (rnd + etat + mfe + fpe) * 1.001f + 1e-30f;
+}
 
 // row of output register r (0..15) of a 32x32 MFMA tile for lane half h
 __device__ inline int mf_row(int r, int h) { return 8 * (r >> 2) + 4 * h + (r & 3); }
 
-template <int KS, bool FP16>
+// Label-ordered assign helpers (ORD): the centre tile visited at step kt
+// (Ts = the workgroup's tile order in LDS) and the point at sorted position q.
+// (The DMA addresses in k_assign_mfma are written out inline: with a call in
+// a global_load_lds address argument, the host compilation of this file
+// drops the kernel instantiations without a diagnostic.)
+template <bool ORD>
+__host__ __device__ __forceinline__ int mf_tile(const int *Ts, int kt) {
+  return ORD ? Ts[kt] : kt;
+}
+template <bool ORD>
+__host__ __device__ __forceinline__ long mf_point(const int *order, long q) {
+  return ORD ? (long)order[q] : q;
+}
+
+// ORD: the workgroups take the points in the order `order` (the previous
+// update's stable sort by label, so a workgroup's points mostly share their
+// previous label) and visit the centre tiles in the order ttab[home[wg]]
+// (tiles nearest to the centre of that label first).  A point's top three
+// then settle within the first few tiles, and each later tile's insertion is
+// skipped when no lane of the wave holds a score that could enter its top
+// three (the test is exact, so the state is the one a full insertion leaves).
+// The certificates do not depend on the order in which centres are visited,
+// so the labels are those of the unordered kernel.
+template <int KS, bool FP16, bool ORD>
 __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
     long M, const uint16_t *__restrict__ Xh, const float *__restrict__ X, int D, int K, int Kp,
     const uint16_t *__restrict__ Ch, const float *__restrict__ bias, const float *__restrict__ C,
     const float *__restrict__ nc, const float *__restrict__ bnd, const float2 *__restrict__ xnb,
     float inv_s2, float eta, float big, int *__restrict__ labels,
     int *__restrict__ flags, int *__restrict__ nflag, int *__restrict__ ncand,
-    const int *__restrict__ stop) {
+    const int *__restrict__ stop, const int *__restrict__ order, const int *__restrict__ ttab,
+    const int *__restrict__ home) {
   // stop (nullable): set by a device-side convergence test; the launch then
   // assigns nothing and lists nothing
   if (stop && *stop) return;
@@ -214,24 +309,47 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   u32x4_t *As = reinterpret_cast<u32x4_t *>(smem);                         // [2][SCH]
   float *Bs = reinterpret_cast<float *>(smem + 2 * SCH * sizeof(u32x4_t));  // [2][SROWS]
+  int *Ts = reinterpret_cast<int *>(Bs + 2 * SROWS);                        // [nKT] (ORD)
   typedef typename std::conditional<FP16, f16x8_t, bf16x8_t>::type vec8;
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, h = lane >> 5, col = lane & 31;
-  const long p0 = (long)blockIdx.x * MF_PTS + w * 64;
+  const long p0 = (long)blockIdx.x * MF_PTS + w * 64;  // sorted positions of the wave
   const int nKT = Kp / MF_TILE;      // multiple of MF_STAGE
   const int nS = nKT / MF_STAGE;
+  if constexpr (ORD) {  // the workgroup's tile order, read once
+    const int *tr = ttab + (long)home[blockIdx.x] * nKT;
+    for (int t = tid; t < nKT; t += 64 * MF_WAVES) Ts[t] = tr[t];
+    __syncthreads();
+  }
 
-  // point fragments: lane holds point (p0 + g*32 + col), k = ks*16 + 8h + 0..7
+  // point fragments: lane holds the point at sorted position p0 + g*32 + col,
+  // k = ks*16 + 8h + 0..7
   u32x4_t bfrag[2][KS];
 #pragma unroll
   for (int g = 0; g < 2; ++g) {
-    const long p = p0 + g * 32 + col;
+    const long sp = p0 + g * 32 + col;
+    const long p = sp < M ? mf_point<ORD>(order, sp) : 0;
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
-      if (p < M)
+      if (sp < M)
         bfrag[g][ks] = *reinterpret_cast<const u32x4_t *>(Xh + p * Dp + ks * 16 + 8 * h);
       else
         bfrag[g][ks] = u32x4_t{0u, 0u, 0u, 0u};
+    }
+  }
+
+  // per point group: the three largest packed scores and the tiles of the
+  // first two (named structs, not arrays: selects between array elements
+  // would force the state into scratch memory)
+  Top3 S0, S1;
+  if constexpr (ORD) {  // 2E of each lane's point, in MFMA score units
+    const float cmx = bnd[0], ncx = bnd[1], dcx = bnd[2];
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      const long sp = p0 + g * 32 + col;
+      const float2 nb = sp < M ? xnb[mf_point<ORD>(order, sp)] : make_float2(0.0f, 0.0f);
+      const float e2 = 2.0f * mf_err_bound(nb.x, nb.y, (float)D, cmx, ncx, dcx, eta) / inv_s2;
+      (g == 0 ? S0 : S1).mg = e2;
     }
   }
 
@@ -243,7 +361,6 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
   // distinct banks.
   auto issue_stage = [&](int st, int b) {
     u32x4_t *dst = As + b * SCH;
-    const uint16_t *cs = Ch + (long)st * SROWS * Dp;
     constexpr int NI = SCH / 64;                            // DMA instructions per stage
     constexpr int PERW = (NI + MF_WAVES - 1) / MF_WAVES;    // per wave (the last may idle)
     if constexpr (64 % CPR == 0) {
@@ -255,7 +372,9 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
           const int r0 = n * RPI + lane / CPR;
           const int rot = CPR >= 8 ? (r0 & ROTM) : 0;
           const int q = (lane % CPR - rot + CPR) % CPR;
-          __builtin_amdgcn_global_load_lds(cs + (long)r0 * Dp + q * 8, dst + n * 64, 16, 0, 0);
+          const int tl = st * MF_STAGE + r0 / MF_TILE;
+          const long row = (long)(ORD ? Ts[tl] : tl) * MF_TILE + (r0 % MF_TILE);
+          __builtin_amdgcn_global_load_lds(Ch + row * Dp + q * 8, dst + n * 64, 16, 0, 0);
         }
       }
     } else {
@@ -268,18 +387,19 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
           const int r = c / CPR, pos = c - r * CPR;
           const int rot = CPR >= 8 ? (r & ROTM) : 0;
           const int q = (pos - rot + CPR) % CPR;
-          __builtin_amdgcn_global_load_lds(cs + (long)r * Dp + q * 8, dst + c0, 16, 0, 0);
+          const int tl = st * MF_STAGE + r / MF_TILE;
+          const long row = (long)(ORD ? Ts[tl] : tl) * MF_TILE + (r % MF_TILE);
+          __builtin_amdgcn_global_load_lds(Ch + row * Dp + q * 8, dst + c0, 16, 0, 0);
         }
       }
     }
     if (tid < SROWS / 4)  // biases: 4 floats per lane
-      __builtin_amdgcn_global_load_lds(bias + (long)st * SROWS + tid * 4, Bs + b * SROWS, 16, 0, 0);
+    {
+      const int tl = st * MF_STAGE + (tid * 4) / MF_TILE;
+      const long row = (long)(ORD ? Ts[tl] : tl) * MF_TILE + ((tid * 4) % MF_TILE);
+      __builtin_amdgcn_global_load_lds(bias + row, Bs + b * SROWS, 16, 0, 0);
+    }
   };
-
-  // per point group: the three largest packed scores and the tiles of the
-  // first two (named structs, not arrays: selects between array elements
-  // would force the state into scratch memory)
-  Top3 S0, S1;
 
   // scores of one centre tile for both point groups; bias is the
   // accumulator input (rows 8*(r/4) + 4h + r%4)
@@ -315,8 +435,23 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
     }
   };
   // pack the register id into the low 4 bits, keep the top 3 and the tiles
-  // of the first two
+  // of the first two.  ORD: a tile whose every score lies more than 2E (plus
+  // a slack) below the point's best so far (both lane halves) cannot hold a
+  // competitor of it; when that holds for every lane of the wave the
+  // insertion is skipped and only the tile's largest score is kept, in U.
+  // The top three are then exact over the tiles inserted and U bounds every
+  // other score, which is all the certificates use (see finish).
   auto epi_group = [&](int kt, const f32x16_t &acc, Top3 &T) {
+    if constexpr (ORD) {
+      float m = acc[0];
+#pragma unroll
+      for (int r = 1; r < 16; ++r) m = __builtin_fmaxf(m, acc[r]);
+      if (__ballot(!(m < T.th)) == 0) {
+        T.U = __builtin_fmaxf(T.U, m);
+        return;
+      }
+    }
+    kt = mf_tile<ORD>(Ts, kt);
     const float o1 = T.b1, o2 = T.b2;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
@@ -328,6 +463,10 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
     const int nt2 = (u2 == __float_as_uint(o1)) ? ot1 : (u2 == __float_as_uint(o2)) ? ot2 : kt;
     T.t1 = (u1 != __float_as_uint(o1)) ? kt : ot1;
     T.t2 = nt2;
+    if constexpr (ORD) {
+      const float bc = __builtin_fmaxf(T.b1, __shfl_xor(T.b1, 32));
+      T.th = bc - T.mg - __builtin_fabsf(bc) * 1e-4f;
+    }
   };
   auto epi_tile = [&](int kt, const f32x16_t &a0, const f32x16_t &a1) {
     epi_group(kt, a0, S0);
@@ -394,24 +533,22 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
       tie = tie || otie || (__float_as_uint(m1) == __float_as_uint(m2)) ||
             (__float_as_uint(m2) == __float_as_uint(m3));
     }
-    const long p = p0 + g * 32 + col;
+    const long sp = p0 + g * 32 + col;
+    const bool inr = sp < M;
+    const long p = inr ? mf_point<ORD>(order, sp) : 0;
+    if constexpr (ORD) {  // the skipped tiles' scores are at most Ub (both halves)
+      const float Ub = fmaxf(T.U, __shfl_xor(T.U, 32));
+      m2 = fmaxf(m2, Ub);
+      m3 = fmaxf(m3, Ub);
+    }
     // forward-error bound E for every centre's approximate score (FP32 units)
-    const float2 nb = p < M ? xnb[p] : make_float2(0.0f, 0.0f);
-    const float xn = nb.x, ex = nb.y;  // |x|, |x - xh/s|
-    const float Df = (float)D;
-    const float g32 = Df * 5.9604645e-8f * 1.0001f;        // gamma_D, u = 2^-24
-    const float gmf = (Df + 1.0f) * 2.3841858e-7f * 1.001f; // gamma'_{D+1}, u = 2^-22
-    const float rnd = ex * cmax + (xn + ex) * dcmax;
-    // FP16 engines: margin for subnormal operands the matrix core may flush
-    const float etat = eta * sqrtf(Df) * (xn + cmax) + eta * eta * Df;
-    const float mfe = gmf * (0.5f * ncmax + (xn + ex) * (cmax + dcmax) + etat);
-    const float fpe = g32 * (0.5f * ncmax + xn * cmax);
-    const float E = (rnd + etat + mfe + fpe) * 1.001f + 1e-30f;
+    const float2 nb = inr ? xnb[p] : make_float2(0.0f, 0.0f);
+    const float E = mf_err_bound(nb.x, nb.y, (float)D, cmax, ncmax, dcmax, eta);
     const float a1 = m1 * inv_s2, a2 = m2 * inv_s2, a3 = m3 * inv_s2;  // FP32 score units
     // packed values carry 4 replaced low bits (< 16 ulp); FP32 subtraction
     // and the scaling add a few more ulps
     const float slack = (fabsf(a1) + fabsf(a2) + fabsf(a3)) * 1.6e-5f;
-    const bool live = h == 0 && p < M;
+    const bool live = h == 0 && inr;
     // certificate 1: the best approximate score beats every other by > 2E
     const bool quick = !tie && (a1 - a2 > 2.0f * E + slack) && k1 >= 0 && k1 < K;
     // certificate 2: the exact FP32 chains (the plain path's arithmetic) of
@@ -421,7 +558,7 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
     // other centre
     const bool has_third = m3 > -FLT_MAX * 0.5f;
     const float thr = has_third ? a3 + E + slack : -INFINITY;
-    const bool need = p < M && !quick && !tie && k1 >= 0 && k1 < K;
+    const bool need = inr && !quick && !tie && k1 >= 0 && k1 < K;
     float sx = -INFINITY;
     if (need) {
       const int cj = h == 0 ? k1 : k2;
@@ -429,16 +566,28 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
         const float *xr = X + p * D;
         const float *cr = C + (long)cj * D;
         float sc = -0.5f * nc[cj];
-        if ((D & 3) == 0) {
+        if ((D & 3) == 0 && ((uintptr_t)X & 15) == 0 && ((uintptr_t)C & 15) == 0) {
+          // 16-byte pieces, 16 of each row in flight at a time (the FP32 row
+          // of the point comes from HBM), added in ascending d
           const float4 *x4 = reinterpret_cast<const float4 *>(xr);
           const float4 *c4 = reinterpret_cast<const float4 *>(cr);
-#pragma unroll 4
-          for (int q = 0; q < D / 4; ++q) {
-            const float4 xv = x4[q], cv = c4[q];
-            sc = __builtin_fmaf(xv.x, cv.x, sc);
-            sc = __builtin_fmaf(xv.y, cv.y, sc);
-            sc = __builtin_fmaf(xv.z, cv.z, sc);
-            sc = __builtin_fmaf(xv.w, cv.w, sc);
+          const int n4 = D / 4;
+          for (int b0 = 0; b0 < n4; b0 += 16) {
+            float4 xv[16], cv[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u)
+              if (b0 + u < n4) {
+                xv[u] = x4[b0 + u];
+                cv[u] = c4[b0 + u];
+              }
+#pragma unroll
+            for (int u = 0; u < 16; ++u)
+              if (b0 + u < n4) {
+                sc = __builtin_fmaf(xv[u].x, cv[u].x, sc);
+                sc = __builtin_fmaf(xv[u].y, cv[u].y, sc);
+                sc = __builtin_fmaf(xv[u].z, cv[u].z, sc);
+                sc = __builtin_fmaf(xv[u].w, cv[u].w, sc);
+              }
           }
         } else {
           for (int q = 0; q < D; ++q) sc = __builtin_fmaf(xr[q], cr[q], sc);
@@ -551,7 +700,7 @@ __global__ __launch_bounds__(256) void k_deep_mfma(
         // aligned), else one guarded load each
         const int db = ks * 16 + 8 * h;
         float xs[8];
-        if ((D & 7) == 0 && db + 8 <= D) {
+        if ((D & 7) == 0 && db + 8 <= D && ((uintptr_t)X & 15) == 0) {
           const float4 lo4 = *reinterpret_cast<const float4 *>(X + p * D + db);
           const float4 hi4 = *reinterpret_cast<const float4 *>(X + p * D + db + 4);
           xs[0] = lo4.x; xs[1] = lo4.y; xs[2] = lo4.z; xs[3] = lo4.w;
@@ -680,7 +829,7 @@ __global__ __launch_bounds__(256) void k_deep_mfma(
     if (l0 || l1) {
       const float *xr = X + p * D;
       const float *r0 = C + (long)(l0 ? c0 : 0) * D, *r1 = C + (long)(l1 ? c1 : 0) * D;
-      if ((D & 3) == 0) {
+      if ((D & 3) == 0 && (((uintptr_t)X | (uintptr_t)C) & 15) == 0) {
         const float4 *x4 = reinterpret_cast<const float4 *>(xr);
         const float4 *a4 = reinterpret_cast<const float4 *>(r0);
         const float4 *b4 = reinterpret_cast<const float4 *>(r1);
@@ -768,6 +917,8 @@ void mfma_free(Engine *e) {
   }
   hipFree(s->nflag2);
   hipFree(s->kbest);
+  hipFree(s->ttab);
+  hipFree(s->home);
   delete s;
   e->mfma = nullptr;
 }
@@ -845,7 +996,7 @@ __global__ void k_xnorm(long m, int d, const float *__restrict__ X, float scale,
   if (i >= m) return;
   const float *x = X + i * d;
   float s = 0.0f, t = 0.0f;
-  if ((d & 3) == 0) {
+  if ((d & 3) == 0 && ((uintptr_t)X & 15) == 0) {
     const float4 *x4 = reinterpret_cast<const float4 *>(x);
 #pragma unroll 8
     for (int q = 0; q < d / 4; ++q) {
@@ -946,6 +1097,16 @@ static hipError_t prepare_centres(Engine *e) {
   hipLaunchKernelGGL(k_centre_bounds, dim3(1), dim3(256), 0, e->stream, e->k, e->n, e->dnc, s->dcn,
                      s->bnd);
   s->c_ready = true;
+  if (e->dorder && e->k <= MF_ORD_MAX_K) {  // tile table of the label-ordered assign
+    hipError_t r;
+    const int nKT = s->Kp / MF_TILE;
+    if (!s->ttab &&
+        (r = hipMalloc(&s->ttab, sizeof(int) * (size_t)e->k * nKT)) != hipSuccess)
+      return r;
+    hipLaunchKernelGGL(k_tile_order, dim3(e->k), dim3(256), sizeof(float) * (e->n + nKT),
+                       e->stream, e->k, e->n, nKT, e->dCt, e->ldc, e->dnc, s->ttab);
+    s->ttab_ok = true;
+  }
   return hipGetLastError();
 }
 
@@ -961,29 +1122,39 @@ hipError_t mfma_centers(Engine *e) {
   return prepare_centres(e);
 }
 
+// order/ttab/home (all non-null, or all null): the label-ordered variant
 template <int KS>
 static void launch_mfma_ks(hipStream_t strm, bool fp16, dim3 g, long M, const uint16_t *Xh,
                            const float *X, int D, int K, int Kp, const uint16_t *Ch,
                            const float *bias, const float *C, const float *nc, const float *bnd,
                            const float2 *xnb, float inv_s2, float eta, int *labels,
-                           int *flags, int *nflag, int *ncand, const int *stop) {
-  const size_t lds = 2 * (size_t)MF_STAGE * MF_TILE * (KS * 2) * 16 + 2 * MF_STAGE * MF_TILE * 4;
-  static bool attr_set[2] = {false, false};
-  if (!attr_set[fp16]) {
-    if (fp16)
-      hipFuncSetAttribute((const void *)k_assign_mfma<KS, true>,
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    else
-      hipFuncSetAttribute((const void *)k_assign_mfma<KS, false>,
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    attr_set[fp16] = true;
+                           int *flags, int *nflag, int *ncand, const int *stop, const int *order,
+                           const int *ttab, const int *home) {
+  const bool ord = order != nullptr;
+  const size_t lds = 2 * (size_t)MF_STAGE * MF_TILE * (KS * 2) * 16 + 2 * MF_STAGE * MF_TILE * 4 +
+                     (ord ? sizeof(int) * (size_t)(Kp / MF_TILE) : 0);
+  const void *fn = fp16 ? (ord ? (const void *)k_assign_mfma<KS, true, true>
+                               : (const void *)k_assign_mfma<KS, true, false>)
+                        : (ord ? (const void *)k_assign_mfma<KS, false, true>
+                               : (const void *)k_assign_mfma<KS, false, false>);
+  static int attr_lds[4] = {-1, -1, -1, -1};
+  const int v = (fp16 ? 2 : 0) + (ord ? 1 : 0);
+  if ((int)lds > attr_lds[v]) {
+    hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr_lds[v] = (int)lds;
   }
-  if (fp16)
-    hipLaunchKernelGGL((k_assign_mfma<KS, true>), g, dim3(64 * MF_WAVES), lds, strm, M, Xh, X, D, K, Kp, Ch,
-                       bias, C, nc, bnd, xnb, inv_s2, eta, FLT_MAX, labels, flags, nflag, ncand, stop);
+#define KM_ARGS                                                                                 \
+  M, Xh, X, D, K, Kp, Ch, bias, C, nc, bnd, xnb, inv_s2, eta, FLT_MAX, labels, flags, nflag,    \
+      ncand, stop, order, ttab, home
+  if (fp16 && ord)
+    hipLaunchKernelGGL((k_assign_mfma<KS, true, true>), g, dim3(64 * MF_WAVES), lds, strm, KM_ARGS);
+  else if (fp16)
+    hipLaunchKernelGGL((k_assign_mfma<KS, true, false>), g, dim3(64 * MF_WAVES), lds, strm, KM_ARGS);
+  else if (ord)
+    hipLaunchKernelGGL((k_assign_mfma<KS, false, true>), g, dim3(64 * MF_WAVES), lds, strm, KM_ARGS);
   else
-    hipLaunchKernelGGL((k_assign_mfma<KS, false>), g, dim3(64 * MF_WAVES), lds, strm, M, Xh, X, D, K, Kp, Ch,
-                       bias, C, nc, bnd, xnb, inv_s2, eta, FLT_MAX, labels, flags, nflag, ncand, stop);
+    hipLaunchKernelGGL((k_assign_mfma<KS, false, false>), g, dim3(64 * MF_WAVES), lds, strm, KM_ARGS);
+#undef KM_ARGS
 }
 
 hipError_t mfma_assign(Engine *e, long r0, long rows) {
@@ -995,14 +1166,34 @@ hipError_t mfma_assign(Engine *e, long r0, long rows) {
     if ((r = prepare_x(e)) != hipSuccess) return r;
     s->c_ready = false;
   }
+  if (!s->c_ready)
+    if ((r = prepare_centres(e)) != hipSuccess) return r;
+  // all resident points, with the previous update's order and a tile table:
+  // the label-ordered variant
+  const bool ord = r0 == 0 && rows == e->m && e->order_ok && s->ttab_ok && e->k <= MF_ORD_MAX_K;
+  const int *home = nullptr;
+  if (ord) {
+    const long nwg = km_cdiv(rows, MF_PTS);
+    if (nwg > s->home_cap) {
+      hipFree(s->home);
+      s->home = nullptr;
+      s->home_cap = 0;
+      if ((r = hipMalloc(&s->home, sizeof(int) * (size_t)nwg)) != hipSuccess) return r;
+      s->home_cap = nwg;
+    }
+    hipLaunchKernelGGL(k_wg_home, dim3(km_cdiv(nwg, 256)), dim3(256), 0, e->stream, nwg, rows,
+                       e->k, e->dorder, e->dlabels, s->home);
+    home = s->home;
+  }
   return mfma_assign_view(e, e->dX + r0 * e->n, s->Xh + r0 * s->Dp, s->xnb + r0,
-                          e->dlabels + r0, rows);
+                          e->dlabels + r0, rows, ord ? e->dorder : nullptr, home);
 }
 
 // Assignment of `rows` points: FP32 rows X, their reduced-precision copy Xh
-// and norm bounds xn, labels written to lab (all indexed from 0).
+// and norm bounds xn, labels written to lab (all indexed from 0).  order and
+// home (nullable, both or neither): the label-ordered variant over all rows.
 hipError_t mfma_assign_view(Engine *e, const float *X, const uint16_t *Xh, const float2 *xn,
-                            int *lab, long rows) {
+                            int *lab, long rows, const int *order, const int *home) {
   hipError_t r;
   MfmaState *s = st(e);
   if (!s->c_ready)
@@ -1017,12 +1208,12 @@ hipError_t mfma_assign_view(Engine *e, const float *X, const uint16_t *Xh, const
   const float inv_s2 = 1.0f / (s->scale * s->scale);
   dim3 g(km_cdiv(rows, MF_PTS));
   switch (s->KS) {
-    case 1: launch_mfma_ks<1>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop); break;
-    case 2: launch_mfma_ks<2>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop); break;
-    case 4: launch_mfma_ks<4>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop); break;
-    case 6: launch_mfma_ks<6>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop); break;
-    case 8: launch_mfma_ks<8>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop); break;
-    default: launch_mfma_ks<16>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop); break;
+    case 1: launch_mfma_ks<1>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, order ? s->ttab : nullptr, home); break;
+    case 2: launch_mfma_ks<2>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, order ? s->ttab : nullptr, home); break;
+    case 4: launch_mfma_ks<4>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, order ? s->ttab : nullptr, home); break;
+    case 6: launch_mfma_ks<6>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, order ? s->ttab : nullptr, home); break;
+    case 8: launch_mfma_ks<8>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, order ? s->ttab : nullptr, home); break;
+    default: launch_mfma_ks<16>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, order ? s->ttab : nullptr, home); break;
   }
   if ((r = hipGetLastError()) != hipSuccess) return r;
   return hipGetLastError();

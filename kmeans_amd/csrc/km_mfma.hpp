@@ -15,8 +15,11 @@ hipError_t mfma_ensure_slots(Engine *e, int n);
 hipError_t mfma_prepare(Engine *e);
 hipError_t mfma_prepare_rows(Engine *e, const float *X, long rows);
 // xn: per point upper bounds of |x| and |x - xh/s| (FP32 units)
+// order, home (nullable, both or neither): the label-ordered variant over all
+// resident points (order = the previous update's sort by label)
 hipError_t mfma_assign_view(Engine *e, const float *X, const uint16_t *Xh, const float2 *xn,
-                            int *lab, long rows);
+                            int *lab, long rows, const int *order = nullptr,
+                            const int *home = nullptr);
 hipError_t mfma_reresolve_view(Engine *e, const float *X, const uint16_t *Xh, const float2 *xn,
                                int *lab, long rows);
 const uint16_t *mfma_tile_xh(Engine *e);

@@ -202,3 +202,35 @@ def test_mfma_more_elements_than_a_dispatch_dimension(orc, prec):
     assert np.array_equal(labs[prec][rows], orc.assign(Xs, C0))
     del X
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_mfma_misaligned_borrowed_rows(orc, prec):
+    # a borrowed device pointer 4 bytes past a 16-byte boundary: the kernels
+    # that read FP32 rows as 16-byte pieces must fall back to scalar reads
+    print(f"misaligned rows {prec}", flush=True)
+    X, _, _, _ = orc.mixture(61, 6000, 64, 40, -5, 5, 0.5, 2.0)
+    C0 = X[:40].copy()
+    buf = torch.empty(X.size + 1, dtype=torch.float32, device="cuda")
+    buf[1:] = torch.from_numpy(X.reshape(-1)).cuda()
+    Xd = buf[1:]
+    assert Xd.data_ptr() % 16 == 4
+    torch.cuda.synchronize()
+    e = km.KMeans(X.shape[0], 64, 40, precision=prec)
+    e.set_data(device_ptr=Xd.data_ptr())
+    e.set_centers(C0)
+    e.run(3)
+    ref = orc.lloyd(X, C0, 3)
+    assert np.array_equal(e.labels(), ref["labels"])
+    assert np.array_equal(e.centers(), ref["centers"])
+    e.close()
+
+
+def test_batch_rejects_misaligned_borrowed_rows():
+    n = np.array([1000, 1200])
+    X = torch.zeros(int(n.sum()) * 16 + 1, dtype=torch.float32, device="cuda")
+    b = km.KMeansBatch(n, 16, 8)
+    with pytest.raises(km.KmeansError) as err:
+        b.set_data(device_ptr=X[1:].data_ptr())
+    assert err.value.code == km.KMEANS_ERROR_INVALID_ARGUMENT
+    b.close()
