@@ -53,7 +53,7 @@ struct Batch {
   float *C = nullptr, *nc = nullptr;
   float2 *bnd = nullptr;  // per instance: max ||c||^2 and max ||c - bf16(c)||^2 (raw FP32 chains)
   int *labels = nullptr, *perm = nullptr;
-  double *q = nullptr, *part = nullptr, *comp = nullptr;
+  double *part = nullptr, *comp = nullptr;
   long *choff = nullptr;     // first chunk slot of each instance
   long slots = 0;            // chunk slots in total
   int4 *desc = nullptr;      // per slot: instance base row, member range
@@ -669,63 +669,17 @@ __global__ __launch_bounds__(1024) void k_batch_sort(
   }
 }
 
-// Update 2/3: one thread per chunk slot of any instance: the FP64 sums of a
-// chunk of <= KM_CHUNK members (ascending point order) for every dimension
-// and the squared norms.
-template <int DT>
-__global__ __launch_bounds__(256) void k_batch_chunks(
-    int nb, long slots, const long *__restrict__ choff, const long *__restrict__ xoff, int d,
-    int k, const float *__restrict__ X, const double *__restrict__ q,
-    const int *__restrict__ perm, const int *__restrict__ gcnt, const int *__restrict__ goff,
-    const int *__restrict__ gcpre, double *__restrict__ part) {
-  const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= slots) return;
-  int lo = 0, hi = nb - 1;  // last instance with choff <= s
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (choff[mid] <= s) lo = mid; else hi = mid - 1;
-  }
-  const int inst = lo;
-  const int t = (int)(s - choff[inst]);
-  const int *cp = gcpre + (size_t)inst * (k + 1);
-  if (t >= cp[k]) return;
-  int jl = 0, jh = k - 1;  // last cluster with cp[j] <= t
-  while (jl < jh) {
-    const int mid = (jl + jh + 1) >> 1;
-    if (cp[mid] <= t) jl = mid; else jh = mid - 1;
-  }
-  const int j = jl, c = t - cp[j];
-  const int o = goff[(size_t)inst * k + j];
-  const int a = o + c * KM_CHUNK, e = min(o + gcnt[(size_t)inst * k + j], a + KM_CHUNK);
-  const long base = xoff[inst];
-  double acc[DT + 1];
-#pragma unroll
-  for (int qq = 0; qq <= DT; ++qq) acc[qq] = 0.0;
-  for (int r = a; r < e; ++r) {
-    const long idx = base + perm[base + r];
-    const float *x = X + idx * d;
-#pragma unroll
-    for (int qq = 0; qq < DT; ++qq)
-      if (qq < d) acc[qq] += (double)x[qq];
-    acc[DT] += q[idx];
-  }
-  double *out = part + s * (d + 1);
-#pragma unroll
-  for (int qq = 0; qq < DT; ++qq)
-    if (qq < d) out[qq] = acc[qq];
-  out[d] = acc[DT];
-}
-
-// Update 2/3 for d < 64: one wave per chunk slot.  The 64 lanes gather 64
-// members' rows (and squared norms) into LDS at once; then lane q < d adds
-// dimension q and lane d the squared norms, member by member in ascending
-// order, so the sums are the same sequential FP64 sums.
+// Update 2/3: one wave per chunk slot (d <= 64).  The 64 lanes gather 64
+// members' rows into LDS at once; then lane q < d adds dimension q and its
+// square (exact in FP64), member by member in ascending order, so the sums
+// are the same sequential FP64 sums.  The squared-norm column is the
+// per-dimension square sums added in ascending dimension order.
 template <int DT>
 __global__ __launch_bounds__(256) void k_batch_chunks_wave(
     long slots, const int4 *__restrict__ desc, int d, const float *__restrict__ X,
-    const double *__restrict__ q, const int *__restrict__ perm, double *__restrict__ part) {
+    const int *__restrict__ perm, double *__restrict__ part) {
   __shared__ float xs[4][64][DT + 1];
-  __shared__ double qs[4][64];
+  __shared__ double ts[4][DT];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const long s = (long)blockIdx.x * 4 + w;
   if (s >= slots) return;
@@ -733,7 +687,7 @@ __global__ __launch_bounds__(256) void k_batch_chunks_wave(
   const long base = dsc.x;
   const int a = dsc.y, e = dsc.z;
   if (a >= e) return;
-  double acc = 0.0;
+  double acc = 0.0, sq = 0.0;
   for (int r0 = a; r0 < e; r0 += 64) {
     const int r = r0 + lane;
     if (r < e) {
@@ -754,22 +708,34 @@ __global__ __launch_bounds__(256) void k_batch_chunks_wave(
         for (int qq = 0; qq < DT; ++qq)
           if (qq < d) xs[w][lane][qq] = x[qq];
       }
-      qs[w][lane] = q[idx];
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const int m = min(64, e - r0);
     if (lane < d) {
-      for (int i = 0; i < m; ++i) acc += (double)xs[w][i][lane];
-    } else if (lane == d) {
-      for (int i = 0; i < m; ++i) acc += qs[w][i];
+      for (int i = 0; i < m; ++i) {
+        const double v = xs[w][i][lane];
+        acc += v;
+        sq = __builtin_fma(v, v, sq);  // the square of a float is exact in FP64
+      }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
-  if (lane <= d) part[s * (d + 1) + lane] = acc;
+  if (lane < d) {
+    part[s * (d + 1) + lane] = acc;
+    ts[w][lane] = sq;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (lane == 0) {
+    double qv = 0.0;
+    for (int u = 0; u < d; ++u) qv += ts[w][u];
+    part[s * (d + 1) + d] = qv;
+  }
 }
 
 // Update 3/3: one workgroup per instance: canonical combination of the chunk
@@ -974,23 +940,16 @@ static hipError_t launch_update(Batch *b, int slot) {
   hipLaunchKernelGGL(k_batch_sort, dim3(std::min(b->nb, num_cus() * 2)), dim3(1024), lds, b->stream,
                      b->nb, b->order, b->xoff, b->nvec, b->k, b->labels, b->perm, b->gcnt, b->goff,
                      b->gcpre, b->choff, b->desc, b->counters + 1);
-#define CHK(DT)                                                                                  \
-  hipLaunchKernelGGL(k_batch_chunks<DT>, dim3(bg1(b->slots)), dim3(256), 0, b->stream, b->nb,    \
-                     b->slots, b->choff, b->xoff, b->d, b->k, b->X, b->q, b->perm, b->gcnt,      \
-                     b->goff, b->gcpre, b->part)
 #define CHW(DT)                                                                                  \
   hipLaunchKernelGGL(k_batch_chunks_wave<DT>, dim3((unsigned)((b->slots + 3) / 4)), dim3(256), 0,  \
-                     b->stream, b->slots, b->desc, b->d, b->X, b->q, b->perm, b->part)
+                     b->stream, b->slots, b->desc, b->d, b->X, b->perm, b->part)
   switch (pick_dt(b->d)) {
     case 4: CHW(4); break;
     case 8: CHW(8); break;
     case 16: CHW(16); break;
     case 32: CHW(32); break;
-    default:
-      if (b->d < 64) CHW(64); else CHK(64);
-      break;
+    default: CHW(64); break;
   }
-#undef CHK
 #undef CHW
   hipLaunchKernelGGL(k_batch_finalize, dim3(b->nb), dim3(256), 0, b->stream, b->d, b->k, b->choff,
                      b->gcnt, b->gcpre, b->part, b->C, b->nc, b->bnd, co, eo);
@@ -1116,7 +1075,7 @@ static void batch_free_graph(Batch *b) {
 static void batch_free(Batch *b) {
   if (b->stream) hipStreamSynchronize(b->stream);
   batch_free_graph(b);
-  void *bufs[] = {b->X_owned, b->xoff, b->nvec, b->C, b->nc, b->labels, b->perm, b->q,
+  void *bufs[] = {b->X_owned, b->xoff, b->nvec, b->C, b->nc, b->labels, b->perm,
                   b->part, b->comp, b->choff, b->empty, b->order, b->items, b->counters,
                   b->gcnt, b->goff, b->gcpre, b->desc, b->nexact, b->bnd};
   for (void *p : bufs) hipFree(p);
@@ -1185,7 +1144,6 @@ static kmeansErrorStatus batch_create(Batch *b, int nb, const long *n, int d, in
   BCHK(hipMalloc(&b->labels, sizeof(int) * (size_t)tot), KMEANS_ERROR_BUILDDATA);
   BCHK(hipMemsetAsync(b->labels, 0, sizeof(int) * (size_t)tot, b->stream), KMEANS_ERROR_BUILDDATA);
   BCHK(hipMalloc(&b->perm, sizeof(int) * (size_t)tot), KMEANS_ERROR_BUILDDATA);
-  BCHK(hipMalloc(&b->q, sizeof(double) * (size_t)tot), KMEANS_ERROR_BUILDDATA);
   BCHK(hipMalloc(&b->part, sizeof(double) * (size_t)tch * (d + 1)), KMEANS_ERROR_BUILDDATA);
   BCHK(hipMalloc(&b->gcnt, sizeof(int) * (size_t)nb * k), KMEANS_ERROR_BUILDDATA);
   BCHK(hipMalloc(&b->goff, sizeof(int) * (size_t)nb * k), KMEANS_ERROR_BUILDDATA);
@@ -1327,7 +1285,6 @@ kmeansErrorStatus kmeansBatchSetData(kmeansBatchHandle h, const float *X, int on
     BCHK(hipMemcpy(b->X_owned, X, bytes, hipMemcpyHostToDevice), KMEANS_ERROR_BUILDDATA);
     b->X = b->X_owned;
   }
-  BCHK(km::launch_point_sq(b->stream, b->total, b->d, b->X, b->q), KMEANS_ERROR_BUILDDATA);
   BCHK(hipStreamSynchronize(b->stream), KMEANS_ERROR_BUILDDATA);
   b->data_set = true;
   km::batch_free_graph(b);

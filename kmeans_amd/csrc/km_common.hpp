@@ -6,8 +6,7 @@
 
 #include "../../include/kmeans_amd.h"
 
-#define KM_SB_SMALL 4096    // points per histogram/scatter sub-block (k <= 2048)
-#define KM_SB_LARGE 16384   // points per sub-block for larger k
+#define KM_SB 16384         // points per histogram/scatter sub-block
 
 #define KM_HIP_CHECK(expr, code)                                                     \
   do {                                                                               \
@@ -91,7 +90,7 @@ kmeansErrorStatus compute_d(const double *data, int m, int n, int k, double crit
 // Internal launchers (implemented in km_kernels.hip); all stream-ordered.
 struct UpdateWs {
   int *hist, *offs, *tsum, *cstart, *ccount, *cq, *perm;
-  double *P, *S, *compj, *qb;  // S: k x (d+1), column d = sum of q_i
+  double *P, *S, *compj;  // S: k x (d+1), column d = the squared-norm moment
   long long *cnt;
   size_t bytes;
   // nullable: an n-sized array that receives each block's stable sort by
@@ -103,12 +102,12 @@ size_t update_ws_layout(long n, int d, int k, char *base, UpdateWs *ws);
 // (launch_update_accumulate_2s); S, cnt and compj stay with the caller.
 struct UpdateOverlap {
   hipStream_t side[2];
-  hipEvent_t start, done[2], fin[2];
+  hipEvent_t start, done[2], fin[2], sorted[2];
   const UpdateWs *ws[2];
 };
 hipError_t launch_update_accumulate_2s(hipStream_t s, const UpdateOverlap &ov, long n, int d,
-                                       int k, const float *X, const int *labels, const double *q,
-                                       double *S, long long *cnt);
+                                       int k, const float *X, const int *labels, double *S,
+                                       long long *cnt);
 int sub_block(int k);
 
 // Launchers with a float and a double form (element type of the points and
@@ -117,15 +116,14 @@ int sub_block(int k);
   hipError_t launch_col_norms(hipStream_t s, int k, int d, const T *C, T *nc);                   \
   hipError_t launch_transpose(hipStream_t s, int k, int d, const T *C, T *Ct, int ldc);          \
   hipError_t launch_update(hipStream_t s, long n, int d, int k, const T *X, const int *labels,   \
-                           const double *q, const T *C_old, T *C_new, T *Ct_new, int ldc,        \
-                           T *nc_new, long long *counts, int *empty, const UpdateWs &ws);        \
+                           const T *C_old, T *C_new, T *Ct_new, int ldc, T *nc_new,              \
+                           long long *counts, int *empty, const UpdateWs &ws);                   \
   hipError_t launch_compactness(hipStream_t s, long n, int d, int k, const T *X,                 \
-                                const int *labels, const double *q, const T *C, double *out,     \
-                                const UpdateWs &ws);                                             \
+                                const int *labels, const T *C, double *out, const UpdateWs &ws); \
   hipError_t launch_point_sq(hipStream_t s, long m, int d, const T *X, double *q);               \
   hipError_t launch_update_accumulate(hipStream_t s, long n, int d, int k, const T *X,           \
-                                      const int *labels, const double *q, double *S,             \
-                                      long long *cnt, const UpdateWs &ws);                       \
+                                      const int *labels, double *S, long long *cnt,              \
+                                      const UpdateWs &ws);                                       \
   hipError_t launch_update_finalize(hipStream_t s, int d, int k, const double *S,                \
                                     const long long *cnt, const T *C_old, T *C_new, T *Ct_new,   \
                                     int ldc, T *nc_new, long long *counts, int *empty,           \

@@ -56,7 +56,6 @@ Engine::~Engine() {
     if (e) hipEventDestroy(e);
   for (int b = 0; b < 2; ++b) {
     hipFree(dXt[b]);
-    hipFree(dqt[b]);
     hipHostFree(pin[b]);
     if (ev_loaded[b]) hipEventDestroy(ev_loaded[b]);
     if (ev_used[b]) hipEventDestroy(ev_used[b]);
@@ -156,9 +155,12 @@ kmeansErrorStatus Engine::buildData(const float *data, int on_device) {
     KM_HIP_CHECK(hipStreamSynchronize(stream), KMEANS_ERROR_BUILDDATA);
     dX = dX_owned;
   }
-  // per-point FP64 squared norms, used by the compactness moments
-  if (!dq) KM_HIP_CHECK(hipMalloc(&dq, sizeof(double) * (size_t)m), KMEANS_ERROR_BUILDDATA);
-  KM_HIP_CHECK(launch_point_sq(stream, m, n, dX, dq), KMEANS_ERROR_BUILDDATA);
+  // per-point FP64 squared norms for the learned proposer's certificates
+  // (the update takes its squared-norm moments from the rows it reads)
+  if (learned) {
+    if (!dq) KM_HIP_CHECK(hipMalloc(&dq, sizeof(double) * (size_t)m), KMEANS_ERROR_BUILDDATA);
+    KM_HIP_CHECK(launch_point_sq(stream, m, n, dX, dq), KMEANS_ERROR_BUILDDATA);
+  }
   // the GEMM path's row norms, once per data set (k_row_norms, as per tile)
   if (opt.full_gemm) {
     if (!dxn) KM_HIP_CHECK(hipMalloc(&dxn, sizeof(float) * (size_t)m), KMEANS_ERROR_BUILDDATA);
@@ -305,18 +307,18 @@ kmeansErrorStatus Engine::clusterCenters(long r0, long rows, int *empty) {
     // several canonical blocks: sorts and chunk sums of alternate blocks on
     // two side streams, the in-order additions on the engine's stream
     const UpdateOverlap ov{{upd_side[0], upd_side[1]}, upd_ev[0], {upd_ev[1], upd_ev[2]},
-                           {upd_ev[3], upd_ev[4]}, {&ws, &ws2}};
+                           {upd_ev[3], upd_ev[4]}, {upd_ev[5], upd_ev[6]}, {&ws, &ws2}};
     KM_HIP_CHECK(launch_update_zero(stream, n, k, ws.S, ws.cnt, dempty),
                  KMEANS_ERROR_CLUSTER_CENTERS);
     KM_HIP_CHECK(launch_update_accumulate_2s(stream, ov, rows, n, k, dX + r0 * n, dlabels + r0,
-                                             dq + r0, ws.S, ws.cnt),
+                                             ws.S, ws.cnt),
                  KMEANS_ERROR_CLUSTER_CENTERS);
     KM_HIP_CHECK(launch_update_finalize(stream, n, k, ws.S, ws.cnt, dC, dC, dCt, ldc, dnc, dcounts,
                                         dempty, ws.compj),
                  KMEANS_ERROR_CLUSTER_CENTERS);
   } else {
-    KM_HIP_CHECK(launch_update(stream, rows, n, k, dX + r0 * n, dlabels + r0, dq + r0, dC, dC, dCt,
-                               ldc, dnc, dcounts, dempty, ws),
+    KM_HIP_CHECK(launch_update(stream, rows, n, k, dX + r0 * n, dlabels + r0, dC, dC, dCt, ldc,
+                               dnc, dcounts, dempty, ws),
                  KMEANS_ERROR_CLUSTER_CENTERS);
   }
   if (keep_order) order_ok = true;
@@ -561,7 +563,6 @@ kmeansErrorStatus Engine::setup_streaming(const float *host) {
   (void)hipGetLastError();  // a pageable pointer leaves an error code behind
   for (int b = 0; b < 2; ++b) {
     KM_HIP_CHECK(hipMalloc(&dXt[b], sizeof(float) * (size_t)tile * n), KMEANS_ERROR_TILING);
-    KM_HIP_CHECK(hipMalloc(&dqt[b], sizeof(double) * (size_t)tile), KMEANS_ERROR_TILING);
     if (!host_pinned)
       KM_HIP_CHECK(hipHostMalloc(&pin[b], sizeof(float) * (size_t)tile * n, hipHostMallocDefault),
                    KMEANS_ERROR_TILING);
@@ -610,7 +611,6 @@ kmeansErrorStatus Engine::stream_iteration(int slot) {
     if (t + 1 < nt)
       if ((err = load_tile(t + 1, b ^ 1)) != KMEANS_SUCCESS) return err;
     KM_HIP_CHECK(hipStreamWaitEvent(stream, ev_loaded[b], 0), KMEANS_ERROR_COPY_TILE);
-    KM_HIP_CHECK(launch_point_sq(stream, rows, n, dXt[b], dqt[b]), KMEANS_ERROR_COPY_TILE);
     if (opt.full_gemm) {
       KM_HIP_CHECK(gemm_assign(this, dXt[b], nullptr, rows, dlabels + r0),
                    KMEANS_ERROR_CLOSEST_CENTERS);
@@ -627,8 +627,8 @@ kmeansErrorStatus Engine::stream_iteration(int slot) {
                                        dlabels + r0, rows),
                    KMEANS_ERROR_CLOSEST_CENTERS);
     }
-    KM_HIP_CHECK(launch_update_accumulate(stream, rows, n, k, dXt[b], dlabels + r0, dqt[b], ws.S,
-                                          ws.cnt, ws),
+    KM_HIP_CHECK(launch_update_accumulate(stream, rows, n, k, dXt[b], dlabels + r0, ws.S, ws.cnt,
+                                          ws),
                  KMEANS_ERROR_CLUSTER_CENTERS);
     KM_HIP_CHECK(hipEventRecord(ev_used[b], stream), KMEANS_ERROR_COPY_TILE);
   }
@@ -827,7 +827,7 @@ kmeansErrorStatus kmeansCompactness(kmeansHandle h, double *out) {
   Engine &e = h->e;
   if (!e.dX || !e.centers_set) return KMEANS_ERROR_COMPACTNESS;
   double *dv = e.ws.compj + e.k;  // the spare slot past the per-centre terms
-  KM_HIP_CHECK(launch_compactness(e.stream, e.m, e.n, e.k, e.dX, e.dlabels, e.dq, e.dC, dv, e.ws),
+  KM_HIP_CHECK(launch_compactness(e.stream, e.m, e.n, e.k, e.dX, e.dlabels, e.dC, dv, e.ws),
                KMEANS_ERROR_COMPACTNESS);
   KM_HIP_CHECK(hipMemcpyAsync(out, dv, sizeof(double), hipMemcpyDeviceToHost, e.stream),
                KMEANS_ERROR_COMPACTNESS);
@@ -951,7 +951,7 @@ kmeansCudaErrorStatus kmUpdate(void *stream, long n, int d, int k, const float *
   km::UpdateWs ws;
   if (km::update_ws_layout(n, d, k, (char *)workspace, &ws) > workspace_bytes)
     return ERROR_INVALID_ARGUMENT;
-  KM_K_CHECK(km::launch_update((hipStream_t)stream, n, d, k, X, labels, nullptr, C_old, C_new,
+  KM_K_CHECK(km::launch_update((hipStream_t)stream, n, d, k, X, labels, C_old, C_new,
                                Ct_new, km::centers_ld(k), nc_new, counts, empty, ws),
              ERROR_CLUSTERCENTERS);
   return NO_ERROR;
@@ -965,7 +965,7 @@ kmeansCudaErrorStatus kmCompactness(void *stream, long n, int d, int k, const fl
   km::UpdateWs ws;
   if (km::update_ws_layout(n, d, k, (char *)workspace, &ws) > workspace_bytes)
     return ERROR_INVALID_ARGUMENT;
-  KM_K_CHECK(km::launch_compactness((hipStream_t)stream, n, d, k, X, labels, nullptr, C, out, ws),
+  KM_K_CHECK(km::launch_compactness((hipStream_t)stream, n, d, k, X, labels, C, out, ws),
              ERROR_COMPACTNESS);
   return NO_ERROR;
 }
@@ -1074,7 +1074,7 @@ static kmeansCudaErrorStatus legacy_cluster(int m0, int m, int n, int k, bool la
   KM_K_CHECK(legacy_ws(m, n, k, &ws), ERROR_CLUSTERCENTERS);
   double *S = (double *)centers_large;
   long long *cnt = (long long *)counts_large;
-  KM_K_CHECK(launch_update_accumulate(nullptr, m, n, k, data, indices + m0, nullptr, S, cnt, ws),
+  KM_K_CHECK(launch_update_accumulate(nullptr, m, n, k, data, indices + m0, S, cnt, ws),
              ERROR_CLUSTERCENTERS);
   if (lastTile) {
     KM_K_CHECK(launch_update_finalize(nullptr, n, k, S, cnt, (const T *)centers, centers,
@@ -1108,7 +1108,7 @@ static kmeansCudaErrorStatus legacy_compactness(int m0, int m, bool lastTile, in
   KM_K_CHECK(legacy_ws(m, n, k, &ws), ERROR_COMPACTNESS);
   double *acc = (double *)(compactness + 8 / sizeof(T));
   double *tile = ws.compj + k;  // one spare slot past the per-centre terms
-  KM_K_CHECK(launch_compactness(nullptr, m, n, k, data, indices + m0, nullptr, centers, tile, ws),
+  KM_K_CHECK(launch_compactness(nullptr, m, n, k, data, indices + m0, centers, tile, ws),
              ERROR_COMPACTNESS);
   double tv = 0.0, tot = 0.0;
   KM_K_CHECK(hipMemcpy(&tv, tile, sizeof(double), hipMemcpyDeviceToHost), ERROR_COMPACTNESS);

@@ -34,7 +34,7 @@ struct Engine {
   int *dempty = nullptr;    // empty clusters of a single update call
   int *dempty_hist = nullptr;  // empty clusters, one per iteration slot
   double *dcomp = nullptr;  // one per iteration slot
-  double *dq = nullptr;     // m: FP64 squared norms of the points
+  double *dq = nullptr;     // m: FP64 squared norms of the points (learned proposer only)
   float *dxn = nullptr;     // m: FP32 row norms for the library-GEMM path (resident data)
   char *dws = nullptr;
   UpdateWs ws{};
@@ -42,7 +42,7 @@ struct Engine {
   // two side streams (default priority), their events and a second
   // per-block workspace
   hipStream_t upd_side[2] = {nullptr, nullptr};
-  hipEvent_t upd_ev[5] = {};
+  hipEvent_t upd_ev[7] = {};
   char *dws2 = nullptr;
   UpdateWs ws2{};
   // BF16/FP16 engines over resident points: the last full update's stable
@@ -83,7 +83,6 @@ struct Engine {
   bool host_pinned = false;
   float x_absmax = 0.0f;          // max |x| over the streamed data (FP16 scale)
   float *dXt[2] = {nullptr, nullptr};
-  double *dqt[2] = {nullptr, nullptr};
   float *pin[2] = {nullptr, nullptr};  // staging when the host buffer is pageable
   hipStream_t copy_stream = nullptr;
   hipEvent_t ev_loaded[2] = {nullptr, nullptr}, ev_used[2] = {nullptr, nullptr};

@@ -29,7 +29,7 @@ struct EngineD {
   int n = 0, k = 0, ldc = 0;
   bool full_gemm = false;
   hipStream_t stream = nullptr;
-  double *dX = nullptr, *dC = nullptr, *dCt = nullptr, *dnc = nullptr, *dq = nullptr;
+  double *dX = nullptr, *dC = nullptr, *dCt = nullptr, *dnc = nullptr;
   const double *dXb = nullptr;  // the points (dX, or a borrowed device array)
   double *dcomp = nullptr;
   int comp_slots = 0;
@@ -52,7 +52,6 @@ struct EngineD {
     hipFree(dC);
     hipFree(dCt);
     hipFree(dnc);
-    hipFree(dq);
     hipFree(dxn);
     hipFree(dcomp);
     hipFree(dlabels);
@@ -84,7 +83,6 @@ struct EngineD {
     KM_HIP_CHECK(hipMemsetAsync(dCt, 0, sizeof(double) * (size_t)ldc * n, stream),
                  KMEANS_ERROR_BUILDDATA);
     KM_HIP_CHECK(hipMalloc(&dnc, sizeof(double) * k), KMEANS_ERROR_BUILDDATA);
-    KM_HIP_CHECK(hipMalloc(&dq, sizeof(double) * (size_t)m), KMEANS_ERROR_BUILDDATA);
     KM_HIP_CHECK(hipMalloc(&dlabels, sizeof(int) * (size_t)m), KMEANS_ERROR_BUILDDATA);
     KM_HIP_CHECK(hipMemsetAsync(dlabels, 0, sizeof(int) * (size_t)m, stream),
                  KMEANS_ERROR_BUILDDATA);
@@ -96,8 +94,7 @@ struct EngineD {
     return KMEANS_SUCCESS;
   }
 
-  // buildData (kmeans.cpp:540-695, without the row shuffle): the points and
-  // their FP64 squared norms for the compactness moments
+  // buildData (kmeans.cpp:540-695, without the row shuffle): the points
   kmeansErrorStatus setData(const double *data, int on_device) {
     if (!data) return KMEANS_ERROR_INVALID_ARGUMENT;
     if (on_device) {
@@ -109,7 +106,6 @@ struct EngineD {
                    KMEANS_ERROR_BUILDDATA);
       dXb = dX;
     }
-    KM_HIP_CHECK(launch_point_sq(stream, m, n, dXb, dq), KMEANS_ERROR_BUILDDATA);
     if (full_gemm) {  // the data's row norms, once (k_row_norms, as per tile)
       if (!dxn) KM_HIP_CHECK(hipMalloc(&dxn, sizeof(double) * (size_t)m), KMEANS_ERROR_BUILDDATA);
       KM_HIP_CHECK(launch_row_norms(stream, m, n, dXb, dxn), KMEANS_ERROR_BUILDDATA);
@@ -181,7 +177,7 @@ struct EngineD {
   }
 
   kmeansErrorStatus clusterCenters(long r0, long rows) {
-    KM_HIP_CHECK(launch_update(stream, rows, n, k, dXb + r0 * n, dlabels + r0, dq + r0, dC, dC,
+    KM_HIP_CHECK(launch_update(stream, rows, n, k, dXb + r0 * n, dlabels + r0, dC, dC,
                                dCt, ldc, dnc, dcounts, dempty, ws),
                  KMEANS_ERROR_CLUSTER_CENTERS);
     return KMEANS_SUCCESS;
@@ -356,7 +352,7 @@ kmeansErrorStatus kmeansCompactnessD(kmeansHandleD h, double *out) {
   km::EngineD &e = h->e;
   if (!e.dXb || !e.centers_set) return KMEANS_ERROR_COMPACTNESS;
   double *dv = e.ws.compj + e.k;  // the spare slot past the per-centre terms
-  KM_HIP_CHECK(km::launch_compactness(e.stream, e.m, e.n, e.k, e.dXb, e.dlabels, e.dq, e.dC, dv,
+  KM_HIP_CHECK(km::launch_compactness(e.stream, e.m, e.n, e.k, e.dXb, e.dlabels, e.dC, dv,
                                       e.ws),
                KMEANS_ERROR_COMPACTNESS);
   KM_HIP_CHECK(hipMemcpyAsync(out, dv, sizeof(double), hipMemcpyDeviceToHost, e.stream),

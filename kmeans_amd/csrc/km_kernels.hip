@@ -366,7 +366,18 @@ __global__ void k_hist(const int *__restrict__ labels, long nb, int K, int nsb, 
   __syncthreads();
   const int s = blockIdx.x;
   const long a = (long)s * SB, e = min(a + SB, nb);
-  for (long i = a + threadIdx.x; i < e; i += blockDim.x) atomicAdd(&h[labels[i]], 1);
+  // 16 labels per thread in flight before their counts are added
+  for (long b0 = a; b0 < e; b0 += 16L * blockDim.x) {
+    int v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const long i = b0 + (long)u * blockDim.x + threadIdx.x;
+      v[u] = i < e ? labels[i] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      if (v[u] >= 0) atomicAdd(&h[v[u]], 1);
+  }
   __syncthreads();
   for (int j = threadIdx.x; j < K; j += blockDim.x) hist[(long)j * nsb + s] = h[j];
 }
@@ -433,21 +444,47 @@ __global__ __launch_bounds__(256) void k_scan_down(const int *__restrict__ in, l
   if (base <= L - 1 && base + 16 >= L) out[L] = ex;  // total after the last element
 }
 
-// Stable scatter: one wave per sub-block walks its points in order; lanes
-// with equal labels are ranked by a ballot match over the label bits.  The
-// per-label write positions of the sub-block (offs) are held in LDS; the
-// labels of 16 steps (1024 points) are read into registers at once and the
-// next 16 are in flight while these are ranked.
+// Stable scatter of one sub-block of W * SBW points (blockDim = 64 W):
+// wave w walks points [a + w SBW, a + (w + 1) SBW) in order.  The waves
+// first count their labels into W x K LDS counters, which become each
+// wave's write positions: the sub-block's offsets (offs) plus the counts of
+// the waves before it.  Lanes with equal labels are then ranked by a ballot
+// match over the label bits.  The labels of 16 steps (1024 points) are read
+// into registers at once and the next 16 are in flight while these are
+// ranked.  A sub-block's W waves write one run per label, so neighbouring
+// waves fill the same cache lines.
 #define SC_STEPS 16
-__global__ __launch_bounds__(64) void k_scatter(const int *__restrict__ labels, long i0, long nb,
-                                                int K, int nsb, int SB, int nbits,
-                                                const int *__restrict__ offs,
-                                                int *__restrict__ perm) {
-  extern __shared__ int cnt[];
-  for (int j = threadIdx.x; j < K; j += 64) cnt[j] = offs[(long)j * nsb + blockIdx.x];
+__global__ __launch_bounds__(1024) void k_scatter(const int *__restrict__ labels, long i0, long nb,
+                                                  int K, int nsb, int SB, int SBW, int nbits,
+                                                  const int *__restrict__ offs,
+                                                  int *__restrict__ perm) {
+  extern __shared__ int hw[];
+  const int W = blockDim.x >> 6, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int t = threadIdx.x; t < W * K; t += blockDim.x) hw[t] = 0;
   __syncthreads();
-  const int s = blockIdx.x, lane = threadIdx.x;
-  const long a = (long)s * SB, e = min(a + SB, nb);
+  const long a = (long)blockIdx.x * SB + (long)w * SBW, e = min(a + SBW, nb);
+  for (long b0 = a; b0 < e; b0 += 64 * SC_STEPS) {  // SC_STEPS labels per lane in flight
+    int v[SC_STEPS];
+#pragma unroll
+    for (int u = 0; u < SC_STEPS; ++u) {
+      const long i = b0 + u * 64 + lane;
+      v[u] = i < e ? labels[i] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < SC_STEPS; ++u)
+      if (v[u] >= 0) atomicAdd(&hw[w * K + v[u]], 1);
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < K; j += blockDim.x) {
+    int run = offs[(long)j * nsb + blockIdx.x];
+    for (int u = 0; u < W; ++u) {
+      const int t = hw[u * K + j];
+      hw[u * K + j] = run;
+      run += t;
+    }
+  }
+  __syncthreads();
+  int *cnt = hw + w * K;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   int cur[SC_STEPS], nxt[SC_STEPS];
   auto fetch = [&](long b0, int (&v)[SC_STEPS]) {
@@ -459,7 +496,7 @@ __global__ __launch_bounds__(64) void k_scatter(const int *__restrict__ labels, 
   };
   fetch(a, cur);
   for (long b0 = a; b0 < e; b0 += 64 * SC_STEPS) {
-    fetch(b0 + 64 * SC_STEPS, nxt);  // zeros past the sub-block
+    fetch(b0 + 64 * SC_STEPS, nxt);  // zeros past the wave's range
 #pragma unroll
     for (int u = 0; u < SC_STEPS; ++u) {
       const long i = b0 + u * 64 + lane;
@@ -471,202 +508,241 @@ __global__ __launch_bounds__(64) void k_scatter(const int *__restrict__ labels, 
         const uint64_t m = __ballot(bit);
         peers &= bit ? m : ~m;
       }
+      // the counters are the wave's own: its LDS reads and writes run in
+      // program order, so ordering the compiler's schedule is enough, and
+      // the perm stores are never waited for
       int c = 0;
       if (ok) {
         c = cnt[l];
         perm[c + __popcll(peers & below)] = (int)(i0 + i);
       }
-      __syncthreads();
+      __builtin_amdgcn_wave_barrier();
       if (ok && (peers >> lane) == 1ull) cnt[l] = c + __popcll(peers);
-      __syncthreads();
+      __builtin_amdgcn_wave_barrier();
     }
 #pragma unroll
     for (int u = 0; u < SC_STEPS; ++u) cur[u] = nxt[u];
   }
 }
 
-// Per-centre member range in the sorted order and chunk prefix (one block).
-__global__ __launch_bounds__(256) void k_cluster_setup(int K, int nsb, const int *__restrict__ offs,
-                                                       int *__restrict__ cstart,
-                                                       int *__restrict__ ccount,
-                                                       int *__restrict__ cq) {
-  __shared__ int lds[256];
-  __shared__ int carry;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (int b = 0; b < K; b += 256) {
-    const int j = b + threadIdx.x;
-    int q = 0;
-    if (j < K) {
-      const int st = offs[(long)j * nsb], en = offs[(long)(j + 1) * nsb];
-      cstart[j] = st;
-      ccount[j] = en - st;
-      q = (en - st + KM_CHUNK - 1) / KM_CHUNK;
-    }
-    int tot;
-    int ex = block_excl_scan256(q, lds, &tot);
-    if (j < K) cq[j] = carry + ex;
-    __syncthreads();
-    if (threadIdx.x == 0) carry += tot;
-    __syncthreads();
+// Exclusive scan of one int per thread over a 1024-thread block.
+__device__ inline int block_excl_scan1024(int v, int *wsum, int *total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
   }
-  if (threadIdx.x == 0) cq[K] = carry;
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  int pre = 0, tot = 0;
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int t = wsum[u];
+    if (u < w) pre += t;
+    tot += t;
+  }
+  *total = tot;
+  return pre + x - v;
 }
 
-// First-level sums: one group of threads per chunk of <= KM_CHUNK members,
-// one thread per dimension, members added in ascending point order.  Column
-// D of each partial holds the sum of the members' squared norms q_i.
+// Per-centre member range in the sorted order and chunk prefix: one block
+// of 1024 threads, each owning ceil(K/1024) consecutive centres, one scan.
+__global__ __launch_bounds__(1024) void k_cluster_setup(int K, int nsb, const int *__restrict__ offs,
+                                                        int *__restrict__ cstart,
+                                                        int *__restrict__ ccount,
+                                                        int *__restrict__ cq) {
+  __shared__ int wsum[16];
+  const int R = (K + 1023) / 1024;
+  const int j0 = (int)threadIdx.x * R, j1 = min(K, j0 + R);
+  int q = 0;
+  for (int j = j0; j < j1; ++j) {
+    const int st = offs[(long)j * nsb], n = offs[(long)(j + 1) * nsb] - st;
+    cstart[j] = st;
+    ccount[j] = n;
+    q += (n + KM_CHUNK - 1) / KM_CHUNK;
+  }
+  int tot;
+  int ex = block_excl_scan1024(q, wsum, &tot);
+  for (int j = j0; j < j1; ++j) {
+    cq[j] = ex;
+    ex += (offs[(long)(j + 1) * nsb] - offs[(long)j * nsb] + KM_CHUNK - 1) / KM_CHUNK;
+  }
+  if (threadIdx.x == 0) cq[K] = tot;
+}
+
+// Read-once loads for the gathered rows (nontemporal: the update reads
+// each row once, so the caches keep the sorted indices instead).
+typedef float km_f4v __attribute__((ext_vector_type(4)));
+__device__ inline float4 load_once(const float4 *p) {
+  const km_f4v v = __builtin_nontemporal_load(reinterpret_cast<const km_f4v *>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
 template <typename T>
-__global__ __launch_bounds__(320) void k_chunk_sums(const T *__restrict__ X, int D, int Dp,
-                                                    const double *__restrict__ q,
+__device__ inline T load_once(const T *p) { return __builtin_nontemporal_load(p); }
+
+// s + v * v.  For a float widened to double the square is exact, so the
+// fused form, which rounds once, equals the unfused sum; a double square
+// is rounded on its own first, as the oracle does.
+template <typename T>
+__device__ inline double add_square(double s, double v) {
+  if (sizeof(T) == 4) return __builtin_fma(v, v, s);
+  return s + v * v;
+}
+
+// Finds the centre of chunk c (the last j with cq[j] <= c) and the chunk's
+// member range [start, start + cnt) in the sorted order.
+__device__ inline void chunk_range(long c, int K, const int *__restrict__ cstart,
+                                   const int *__restrict__ ccount, const int *__restrict__ cq,
+                                   int &start, int &cnt) {
+  int lo = 0, hi = K - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (cq[mid] <= c) lo = mid; else hi = mid - 1;
+  }
+  const int local = (int)(c - cq[lo]);
+  start = cstart[lo] + local * KM_CHUNK;
+  cnt = min(KM_CHUNK, ccount[lo] - local * KM_CHUNK);
+}
+
+// First-level sums: one group of Dp threads per chunk of <= KM_CHUNK members,
+// one dimension per thread (slices of Dp dimensions), members added in
+// ascending point order.  Each thread also sums the squares of its
+// dimension (exact products for float rows, rounded ones for double rows,
+// never fused); column D of the partial is those per-dimension sums added
+// in ascending dimension order, so the squared norms come from the rows
+// already read.  Every thread reaches every barrier: the slice loop is
+// uniform over the block.
+template <typename T>
+__global__ __launch_bounds__(256) void k_chunk_sums(const T *__restrict__ X, int D, int Dp,
                                                     const int *__restrict__ perm, int K,
                                                     const int *__restrict__ cstart,
                                                     const int *__restrict__ ccount,
                                                     const int *__restrict__ cq,
                                                     double *__restrict__ P) {
   __shared__ int pidx[4][KM_CHUNK];
-  const int G = min(256 / Dp, 64);  // chunks per block; threads >= 256 sum q (one each)
-  const bool qwave = threadIdx.x >= 256;
-  const int sub = qwave ? (int)threadIdx.x - 256 : threadIdx.x / Dp;
-  const int t = qwave ? 0 : threadIdx.x - sub * Dp;
-  const int Q = cq[K];
+  __shared__ double sqs[256];
+  const int G = min(256 / Dp, 64);  // chunks per block
+  const int sub = threadIdx.x / Dp;
+  const int t = threadIdx.x - sub * Dp;
   const long c = (long)blockIdx.x * G + sub;
+  const bool act = sub < G && c < cq[K];
   const int ld = D + 1;
-  int j = 0, cnt = 0, start = 0;
-  if (c < Q && sub < G) {
-    int lo = 0, hi = K - 1;  // last j with cq[j] <= c
-    while (lo < hi) {
-      int mid = (lo + hi + 1) >> 1;
-      if (cq[mid] <= c) lo = mid; else hi = mid - 1;
-    }
-    j = lo;
-    const int local = (int)(c - cq[j]);
-    start = cstart[j] + local * KM_CHUNK;
-    cnt = min(KM_CHUNK, ccount[j] - local * KM_CHUNK);
-  }
+  int cnt = 0, start = 0;
+  if (act) chunk_range(c, K, cstart, ccount, cq, start, cnt);
   // stage member indices (G <= 4 when Dp >= 64; otherwise read perm directly)
   const bool staged = G <= 4;
-  if (!qwave && staged && c < Q && sub < G)
+  if (act && staged)
     for (int m = t; m < cnt; m += Dp) pidx[sub][m] = perm[start + m];
   __syncthreads();
-  if (c >= Q || sub >= G) return;
-  if (qwave) {
-    double acc = 0.0;
-    int m = 0;
-    for (; m + 8 <= cnt; m += 8) {
-      double v[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) v[u] = q[staged ? pidx[sub][m + u] : perm[start + m + u]];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) acc += v[u];
+  double qs = 0.0;
+  for (int d0 = 0; d0 < D; d0 += Dp) {
+    const int d = d0 + t;
+    double acc = 0.0, sq = 0.0;
+    if (act && d < D) {
+      int m = 0;
+      for (; m + 4 <= cnt; m += 4) {
+        const int p0 = staged ? pidx[sub][m] : perm[start + m];
+        const int p1 = staged ? pidx[sub][m + 1] : perm[start + m + 1];
+        const int p2 = staged ? pidx[sub][m + 2] : perm[start + m + 2];
+        const int p3 = staged ? pidx[sub][m + 3] : perm[start + m + 3];
+        const double v0 = (double)load_once(X + (long)p0 * D + d);
+        const double v1 = (double)load_once(X + (long)p1 * D + d);
+        const double v2 = (double)load_once(X + (long)p2 * D + d);
+        const double v3 = (double)load_once(X + (long)p3 * D + d);
+        acc += v0; sq = add_square<T>(sq, v0);
+        acc += v1; sq = add_square<T>(sq, v1);
+        acc += v2; sq = add_square<T>(sq, v2);
+        acc += v3; sq = add_square<T>(sq, v3);
+      }
+      for (; m < cnt; ++m) {
+        const double v =
+            (double)load_once(X + (long)(staged ? pidx[sub][m] : perm[start + m]) * D + d);
+        acc += v;
+        sq = add_square<T>(sq, v);
+      }
+      P[c * ld + d] = acc;
     }
-    for (; m < cnt; ++m) acc += q[staged ? pidx[sub][m] : perm[start + m]];
-    P[c * ld + D] = acc;
-    return;
-  }
-  for (int d = t; d < D; d += Dp) {
-    double acc = 0.0;
-    int m = 0;
-    for (; m + 4 <= cnt; m += 4) {
-      const int p0 = staged ? pidx[sub][m] : perm[start + m];
-      const int p1 = staged ? pidx[sub][m + 1] : perm[start + m + 1];
-      const int p2 = staged ? pidx[sub][m + 2] : perm[start + m + 2];
-      const int p3 = staged ? pidx[sub][m + 3] : perm[start + m + 3];
-      const T v0 = X[(long)p0 * D + d], v1 = X[(long)p1 * D + d];
-      const T v2 = X[(long)p2 * D + d], v3 = X[(long)p3 * D + d];
-      acc += (double)v0;
-      acc += (double)v1;
-      acc += (double)v2;
-      acc += (double)v3;
+    sqs[threadIdx.x] = sq;
+    __syncthreads();
+    if (act && t == 0) {
+      const int dn = min(Dp, D - d0);
+      for (int u = 0; u < dn; ++u) qs += sqs[sub * Dp + u];
     }
-    for (; m < cnt; ++m) acc += (double)X[(long)(staged ? pidx[sub][m] : perm[start + m]) * D + d];
-    P[c * ld + d] = acc;
+    __syncthreads();
   }
+  if (act && t == 0) P[c * ld + D] = qs;
 }
 
 // The same first-level sums with four dimensions per thread (D % 4 == 0,
-// rows read as float4): TPC threads per chunk (D/4 rounded up to a power of
-// two) and G = min(256/TPC, 64) chunks per block; every dimension still adds
-// its members in ascending point order, eight rows in flight per thread.
-// Thread 256 + g sums the q column of chunk g.
-__global__ __launch_bounds__(320) void k_chunk_sums4(const float *__restrict__ X, int D, int TPC,
-                                                     const double *__restrict__ q,
+// 16 <= D <= 256, rows 16-byte aligned and read as float4): TPC threads per
+// chunk (D/4 rounded up to a power of two) and G = min(256/TPC, 64) chunks
+// per block (G * D <= 1024); every dimension still adds its members in
+// ascending point order, eight rows in flight per thread.  The square sums
+// go through LDS to the chunk's first thread, which adds them in dimension
+// order into column D.
+__global__ __launch_bounds__(256) void k_chunk_sums4(const float *__restrict__ X, int D, int TPC,
                                                      const int *__restrict__ perm, int K,
                                                      const int *__restrict__ cstart,
                                                      const int *__restrict__ ccount,
                                                      const int *__restrict__ cq,
                                                      double *__restrict__ P) {
   __shared__ int pidx[16][KM_CHUNK];
+  __shared__ double sqs[1024];
   const int G = min(256 / TPC, 64);
-  const bool qwave = threadIdx.x >= 256;
-  const int sub = qwave ? (int)threadIdx.x - 256 : threadIdx.x / TPC;
-  const int t = qwave ? 0 : threadIdx.x - sub * TPC;
-  const int Q = cq[K];
+  const int sub = threadIdx.x / TPC;
+  const int t = threadIdx.x - sub * TPC;
   const long c = (long)blockIdx.x * G + sub;
+  const bool act = sub < G && c < cq[K];
   const int ld = D + 1;
-  int j = 0, cnt = 0, start = 0;
-  if (c < Q && sub < G) {
-    int lo = 0, hi = K - 1;  // last j with cq[j] <= c
-    while (lo < hi) {
-      int mid = (lo + hi + 1) >> 1;
-      if (cq[mid] <= c) lo = mid; else hi = mid - 1;
-    }
-    j = lo;
-    const int local = (int)(c - cq[j]);
-    start = cstart[j] + local * KM_CHUNK;
-    cnt = min(KM_CHUNK, ccount[j] - local * KM_CHUNK);
-  }
+  int cnt = 0, start = 0;
+  if (act) chunk_range(c, K, cstart, ccount, cq, start, cnt);
   const bool staged = G <= 16;
-  if (!qwave && staged && c < Q && sub < G)
+  if (act && staged)
     for (int m = t; m < cnt; m += TPC) pidx[sub][m] = perm[start + m];
   __syncthreads();
-  if (c >= Q || sub >= G) return;
-  if (qwave) {
-    double acc = 0.0;
+  if (act && 4 * t < D) {
+    const float4 *__restrict__ X4 = reinterpret_cast<const float4 *>(X);
+    const long D4 = D / 4;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    auto add = [&](const float4 v) {
+      const double x = v.x, y = v.y, z = v.z, w = v.w;
+      a0 += x; s0 = add_square<float>(s0, x);
+      a1 += y; s1 = add_square<float>(s1, y);
+      a2 += z; s2 = add_square<float>(s2, z);
+      a3 += w; s3 = add_square<float>(s3, w);
+    };
     int m = 0;
     for (; m + 8 <= cnt; m += 8) {
-      double v[8];
+      float4 v[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) v[u] = q[staged ? pidx[sub][m + u] : perm[start + m + u]];
+      for (int u = 0; u < 8; ++u) {
+        const int pp = staged ? pidx[sub][m + u] : perm[start + m + u];
+        v[u] = load_once(X4 + (long)pp * D4 + t);
+      }
 #pragma unroll
-      for (int u = 0; u < 8; ++u) acc += v[u];
+      for (int u = 0; u < 8; ++u) add(v[u]);
     }
-    for (; m < cnt; ++m) acc += q[staged ? pidx[sub][m] : perm[start + m]];
-    P[c * ld + D] = acc;
-    return;
+    for (; m < cnt; ++m) add(load_once(X4 + (long)(staged ? pidx[sub][m] : perm[start + m]) * D4 + t));
+    double *o = P + c * ld + 4 * t;
+    o[0] = a0;
+    o[1] = a1;
+    o[2] = a2;
+    o[3] = a3;
+    double *sq = sqs + sub * D + 4 * t;
+    sq[0] = s0;
+    sq[1] = s1;
+    sq[2] = s2;
+    sq[3] = s3;
   }
-  if (4 * t >= D) return;
-  const float4 *__restrict__ X4 = reinterpret_cast<const float4 *>(X);
-  const long D4 = D / 4;
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-  int m = 0;
-  for (; m + 8 <= cnt; m += 8) {
-    float4 v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int pp = staged ? pidx[sub][m + u] : perm[start + m + u];
-      v[u] = X4[(long)pp * D4 + t];
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      a0 += (double)v[u].x;
-      a1 += (double)v[u].y;
-      a2 += (double)v[u].z;
-      a3 += (double)v[u].w;
-    }
+  __syncthreads();
+  if (act && t == 0) {
+    double qs = 0.0;
+    for (int u = 0; u < D; ++u) qs += sqs[sub * D + u];
+    P[c * ld + D] = qs;
   }
-  for (; m < cnt; ++m) {
-    const float4 v = X4[(long)(staged ? pidx[sub][m] : perm[start + m]) * D4 + t];
-    a0 += (double)v.x;
-    a1 += (double)v.y;
-    a2 += (double)v.z;
-    a3 += (double)v.w;
-  }
-  double *o = P + c * ld + 4 * t;
-  o[0] = a0;
-  o[1] = a1;
-  o[2] = a2;
-  o[3] = a3;
 }
 
 // Per-centre CSUM over its chunk partials (D sums and the q column),
@@ -725,15 +801,21 @@ __global__ __launch_bounds__(256) void k_final_centres(int D, int K, const doubl
                                                        T *__restrict__ nc,
                                                        long long *__restrict__ counts,
                                                        int *__restrict__ empty,
-                                                       double *__restrict__ compj) {
+                                                       double *__restrict__ compj,
+                                                       int stage_sums) {
+  // the new centre (T) and, when stage_sums, its sums (FP64) for the
+  // compactness terms
   extern __shared__ __attribute__((aligned(8))) unsigned char fc_smem[];
-  T *cs = reinterpret_cast<T *>(fc_smem);
+  double *ss = reinterpret_cast<double *>(fc_smem);
+  T *cs = reinterpret_cast<T *>(ss + (stage_sums ? D : 0));
   const int j = blockIdx.x;
   const long long n = cnt[j];
   const int ld = D + 1;
   for (int d = threadIdx.x; d < D; d += blockDim.x) {
-    const T v = n > 0 ? (T)(S[(long)j * ld + d] / (double)n) : C_old[(long)j * D + d];
+    const double sv = S[(long)j * ld + d];
+    const T v = n > 0 ? (T)(sv / (double)n) : C_old[(long)j * D + d];
     cs[d] = v;
+    if (stage_sums) ss[d] = sv;
     C_new[(long)j * D + d] = v;
     if (Ct) Ct[(long)d * ldc + j] = v;
   }
@@ -748,7 +830,7 @@ __global__ __launch_bounds__(256) void k_final_centres(int D, int K, const doubl
       double a = 0.0, b = 0.0;
       for (int d = 0; d < D; ++d) {
         const double c = (double)cs[d];
-        a += c * S[(long)j * ld + d];
+        a += c * (stage_sums ? ss[d] : S[(long)j * ld + d]);
         b += c * c;
       }
       compj[j] = (S[(long)j * ld + D] - 2.0 * a) + (double)n * b;
@@ -775,30 +857,35 @@ __global__ void k_comp_terms(int D, int K, const double *__restrict__ S,
   compj[j] = (S[(long)j * ld + D] - 2.0 * a) + (double)cnt[j] * b;
 }
 
-// out = CSUM of compj[0..K) (K <= KM_CHUNK^2).
+// out = CSUM of compj[0..K) (K <= KM_CHUNK^2).  The terms are staged in LDS
+// 4096 at a time (all 256 threads loading), then 16 threads add one
+// KM_CHUNK piece each in order, so the serial sums wait on LDS, not memory.
 __global__ __launch_bounds__(256) void k_comp_reduce(const double *__restrict__ compj, int K,
                                                      double *__restrict__ out) {
+  __shared__ double st[16 * KM_CHUNK];
   __shared__ double g[KM_CHUNK];
   const int ng = (K + KM_CHUNK - 1) / KM_CHUNK;
-  if (K <= KM_CHUNK) {
-    if (threadIdx.x == 0) {
-      double s = 0.0;
-      for (int t = 0; t < K; ++t) s += compj[t];
-      *out = s;
+  for (int r0 = 0; r0 < K; r0 += 16 * KM_CHUNK) {
+    const int rn = min(16 * KM_CHUNK, K - r0);
+    for (int t = threadIdx.x; t < rn; t += blockDim.x) st[t] = compj[r0 + t];
+    __syncthreads();
+    const int q = threadIdx.x;
+    if (q < 16 && q * KM_CHUNK < rn) {
+      const int e = min(rn, (q + 1) * KM_CHUNK);
+      double sum = 0.0;
+      for (int t = q * KM_CHUNK; t < e; ++t) sum += st[t];
+      g[r0 / KM_CHUNK + q] = sum;
     }
-    return;
+    __syncthreads();
   }
-  for (int q = threadIdx.x; q < ng; q += blockDim.x) {
-    double s = 0.0;
-    const int e = min(K, (q + 1) * KM_CHUNK);
-    for (int t = q * KM_CHUNK; t < e; ++t) s += compj[t];
-    g[q] = s;
-  }
-  __syncthreads();
   if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int q = 0; q < ng; ++q) s += g[q];
-    *out = s;
+    if (ng <= 1) {
+      *out = ng == 1 ? g[0] : 0.0;
+    } else {
+      double sum = 0.0;
+      for (int q = 0; q < ng; ++q) sum += g[q];
+      *out = sum;
+    }
   }
 }
 
@@ -944,7 +1031,12 @@ hipError_t launch_assign_f64(hipStream_t s, long m, int d, const double *X, int 
   return hipGetLastError();
 }
 
-int sub_block(int k) { return k <= 2048 ? KM_SB_SMALL : KM_SB_LARGE; }
+// Waves per scatter sub-block (a power of two dividing KM_SB): its W x K LDS
+// counters stay within 64 KB.
+static int scatter_waves(int k) { return k <= 1024 ? 16 : k <= 2048 ? 8 : 1; }
+// Points per sub-block (one histogram column).
+int sub_block(int) { return KM_SB; }
+
 
 size_t update_ws_layout(long n, int d, int k, char *base, UpdateWs *ws) {
   const long nb = n < KM_BLOCK ? n : KM_BLOCK;
@@ -971,7 +1063,6 @@ size_t update_ws_layout(long n, int d, int k, char *base, UpdateWs *ws) {
   w.S = (double *)take(sizeof(double) * (long)k * (d + 1));
   w.cnt = (long long *)take(sizeof(long long) * k);
   w.compj = (double *)take(sizeof(double) * (k + 1));
-  w.qb = (double *)take(sizeof(double) * (nb > 0 ? nb : 1));
   w.bytes = off;
   w.perm_out = nullptr;
   if (ws) *ws = w;
@@ -1005,47 +1096,57 @@ hipError_t launch_update_zero(hipStream_t s, int d, int k, double *S, long long 
   return hipGetLastError();
 }
 
-// First-level sums of one canonical block (rows [i0, i0 + nb) of X): the
-// stable counting sort of its labels and the FP64 chunk sums into ws.P.
-// q == nullptr: q is computed for the block into ws.qb.
-template <typename T>
-static hipError_t block_sums(hipStream_t s, long i0, long nb, int d, int k, const T *X,
-                             const int *labels, const double *q, const UpdateWs &ws) {
+// The stable counting sort of one canonical block's labels (rows
+// [i0, i0 + nb)): sub-block histograms, their exclusive scan, the scatter
+// into perm (or ws.perm_out at the block's rows) and the per-centre ranges.
+static hipError_t block_sort(hipStream_t s, long i0, long nb, int k, const int *labels,
+                             const UpdateWs &ws) {
   hipError_t e;
-  const int SB = sub_block(k);
+  const int SB = sub_block(k), W = scatter_waves(k);
   int nbits = 0;
   while ((1 << nbits) < k) ++nbits;
-  int Dp = 1;
-  while (Dp < d && Dp < 256) Dp <<= 1;
-  const int G = 256 / Dp < 64 ? 256 / Dp : 64;  // must match k_chunk_sums
   const int nsb = (int)km_cdiv(nb, SB);
   const long L = (long)k * nsb;
-  const double *qb = q;  // indexed by row of X
-  if (!q) {
-    if ((e = launch_point_sq(s, nb, d, X + i0 * d, ws.qb)) != hipSuccess) return e;
-    qb = ws.qb - i0;  // perm holds rows of X; shift so qb[i] is row i
-  }
-  hipLaunchKernelGGL(k_hist, dim3(nsb), dim3(256), sizeof(int) * k, s, labels + i0, nb, k, nsb,
-                     SB, ws.hist);
+  hipLaunchKernelGGL(k_hist, dim3(nsb), dim3(SB >= 32768 ? 1024 : 256), sizeof(int) * k, s,
+                     labels + i0, nb, k, nsb, SB, ws.hist);
   if ((e = scan_excl(s, ws.hist, L, ws.tsum, ws.offs)) != hipSuccess) return e;
   int *perm = ws.perm_out ? ws.perm_out + i0 : ws.perm;
-  hipLaunchKernelGGL(k_scatter, dim3(nsb), dim3(64), sizeof(int) * k, s, labels + i0, i0, nb, k,
-                     nsb, SB, nbits, ws.offs, perm);
-  hipLaunchKernelGGL(k_cluster_setup, dim3(1), dim3(256), 0, s, k, nsb, ws.offs, ws.cstart,
+  hipLaunchKernelGGL(k_scatter, dim3(nsb), dim3(64 * W), sizeof(int) * W * k, s, labels + i0, i0,
+                     nb, k, nsb, SB, SB / W, nbits, ws.offs, perm);
+  hipLaunchKernelGGL(k_cluster_setup, dim3(1), dim3(1024), 0, s, k, nsb, ws.offs, ws.cstart,
                      ws.ccount, ws.cq);
+  return hipGetLastError();
+}
+
+// The FP64 chunk sums of one sorted canonical block into ws.P.
+template <typename T>
+static hipError_t block_chunks(hipStream_t s, long i0, long nb, int d, int k, const T *X,
+                               const UpdateWs &ws) {
+  const int *perm = ws.perm_out ? ws.perm_out + i0 : ws.perm;
   const long qmax = nb / KM_CHUNK + k + 1;
   if (sizeof(T) == 4 && d % 4 == 0 && d >= 16 && d <= 256 && ((uintptr_t)X & 15) == 0) {
     int TPC = 1;
     while (TPC < d / 4) TPC <<= 1;
     const int G4 = 256 / TPC < 64 ? 256 / TPC : 64;  // must match k_chunk_sums4
-    hipLaunchKernelGGL(k_chunk_sums4, dim3(km_cdiv(qmax, G4)), dim3(320), 0, s,
-                       reinterpret_cast<const float *>(X), d, TPC, qb,
-                       perm, k, ws.cstart, ws.ccount, ws.cq, ws.P);
+    hipLaunchKernelGGL(k_chunk_sums4, dim3(km_cdiv(qmax, G4)), dim3(256), 0, s,
+                       reinterpret_cast<const float *>(X), d, TPC, perm, k, ws.cstart, ws.ccount,
+                       ws.cq, ws.P);
   } else {
-    hipLaunchKernelGGL(k_chunk_sums<T>, dim3(km_cdiv(qmax, G)), dim3(320), 0, s, X, d, Dp, qb,
-                       perm, k, ws.cstart, ws.ccount, ws.cq, ws.P);
+    int Dp = 1;
+    while (Dp < d && Dp < 256) Dp <<= 1;
+    const int G = 256 / Dp < 64 ? 256 / Dp : 64;  // must match k_chunk_sums
+    hipLaunchKernelGGL(k_chunk_sums<T>, dim3(km_cdiv(qmax, G)), dim3(256), 0, s, X, d, Dp, perm,
+                       k, ws.cstart, ws.ccount, ws.cq, ws.P);
   }
   return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t block_sums(hipStream_t s, long i0, long nb, int d, int k, const T *X,
+                             const int *labels, const UpdateWs &ws) {
+  hipError_t e;
+  if ((e = block_sort(s, i0, nb, k, labels, ws)) != hipSuccess) return e;
+  return block_chunks(s, i0, nb, d, k, X, ws);
 }
 
 // S[j] += the block's CSUM per centre (the order-dependent step: blocks are
@@ -1057,17 +1158,17 @@ static hipError_t block_add(hipStream_t s, int d, int k, double *S, long long *c
   return hipGetLastError();
 }
 
-// Adds the canonical block sums of rows [0, n) of X (labels and q aligned
-// with X) into S (k x (d+1) FP64: d sums and the q column) and cnt; blocks
-// start at row 0 of X.  q == nullptr: q is computed per block into ws.qb.
+// Adds the canonical block sums of rows [0, n) of X (labels aligned with X)
+// into S (k x (d+1) FP64: d sums and the squared-norm column) and cnt;
+// blocks start at row 0 of X.
 template <typename T>
 static hipError_t launch_update_accumulate_t(hipStream_t s, long n, int d, int k, const T *X,
-                                    const int *labels, const double *q, double *S, long long *cnt,
+                                    const int *labels, double *S, long long *cnt,
                                     const UpdateWs &ws) {
   hipError_t e;
   for (long i0 = 0; i0 < n; i0 += KM_BLOCK) {
     const long nb = (n - i0) < KM_BLOCK ? (n - i0) : KM_BLOCK;
-    if ((e = block_sums(s, i0, nb, d, k, X, labels, q, ws)) != hipSuccess) return e;
+    if ((e = block_sums(s, i0, nb, d, k, X, labels, ws)) != hipSuccess) return e;
     if ((e = block_add(s, d, k, S, cnt, ws)) != hipSuccess) return e;
   }
   return hipSuccess;
@@ -1075,27 +1176,42 @@ static hipError_t launch_update_accumulate_t(hipStream_t s, long n, int d, int k
 
 // The same sums with the blocks' sorts and chunk sums on two side streams
 // (block b on side[b & 1] with workspace ws[b & 1]) and the additions into
-// S on s in block order, so one block's latency-bound sort overlaps the
-// other's bandwidth-bound chunk sums; the result is the single-stream one
-// bit for bit.  A side workspace is reused two blocks later, after s has
-// added the earlier block.
+// S on s in block order; the result is the single-stream one bit for bit.
+// Blocks go in pairs: the two latency-bound sorts run side by side, and
+// neither pair member's bandwidth-bound chunk sums start before both sorts
+// are done, so they do not slow the sorts down.  A side workspace is reused
+// two blocks later, after s has added the earlier block.
 hipError_t launch_update_accumulate_2s(hipStream_t s, const UpdateOverlap &ov, long n, int d,
-                                       int k, const float *X, const int *labels, const double *q,
-                                       double *S, long long *cnt) {
+                                       int k, const float *X, const int *labels, double *S,
+                                       long long *cnt) {
   hipError_t e;
   if ((e = hipEventRecord(ov.start, s)) != hipSuccess) return e;
   for (int i = 0; i < 2; ++i)
     if ((e = hipStreamWaitEvent(ov.side[i], ov.start, 0)) != hipSuccess) return e;
-  int b = 0;
-  for (long i0 = 0; i0 < n; i0 += KM_BLOCK, ++b) {
-    const long nb = (n - i0) < KM_BLOCK ? (n - i0) : KM_BLOCK;
-    const int i = b & 1;
-    if (b >= 2 && (e = hipStreamWaitEvent(ov.side[i], ov.fin[i], 0)) != hipSuccess) return e;
-    if ((e = block_sums(ov.side[i], i0, nb, d, k, X, labels, q, *ov.ws[i])) != hipSuccess) return e;
-    if ((e = hipEventRecord(ov.done[i], ov.side[i])) != hipSuccess) return e;
-    if ((e = hipStreamWaitEvent(s, ov.done[i], 0)) != hipSuccess) return e;
-    if ((e = block_add(s, d, k, S, cnt, *ov.ws[i])) != hipSuccess) return e;
-    if ((e = hipEventRecord(ov.fin[i], s)) != hipSuccess) return e;
+  const long nblk = km_cdiv(n, KM_BLOCK);
+  for (long p = 0; p < nblk; p += 2) {
+    const bool pair = p + 1 < nblk;
+    for (long b = p; b < p + 2 && b < nblk; ++b) {
+      const int i = (int)(b & 1);
+      const long i0 = b * KM_BLOCK, nb = std::min<long>(KM_BLOCK, n - i0);
+      if (b >= 2 && (e = hipStreamWaitEvent(ov.side[i], ov.fin[i], 0)) != hipSuccess) return e;
+      if ((e = block_sort(ov.side[i], i0, nb, k, labels, *ov.ws[i])) != hipSuccess) return e;
+      if ((e = hipEventRecord(ov.sorted[i], ov.side[i])) != hipSuccess) return e;
+    }
+    for (long b = p; b < p + 2 && b < nblk; ++b) {
+      const int i = (int)(b & 1);
+      const long i0 = b * KM_BLOCK, nb = std::min<long>(KM_BLOCK, n - i0);
+      if (pair && (e = hipStreamWaitEvent(ov.side[i], ov.sorted[i ^ 1], 0)) != hipSuccess)
+        return e;
+      if ((e = block_chunks(ov.side[i], i0, nb, d, k, X, *ov.ws[i])) != hipSuccess) return e;
+      if ((e = hipEventRecord(ov.done[i], ov.side[i])) != hipSuccess) return e;
+    }
+    for (long b = p; b < p + 2 && b < nblk; ++b) {
+      const int i = (int)(b & 1);
+      if ((e = hipStreamWaitEvent(s, ov.done[i], 0)) != hipSuccess) return e;
+      if ((e = block_add(s, d, k, S, cnt, *ov.ws[i])) != hipSuccess) return e;
+      if ((e = hipEventRecord(ov.fin[i], s)) != hipSuccess) return e;
+    }
   }
   return hipSuccess;
 }
@@ -1105,9 +1221,11 @@ static hipError_t launch_update_finalize_t(hipStream_t s, int d, int k, const do
                                   const long long *cnt, const T *C_old, T *C_new, T *Ct_new,
                                   int ldc, T *nc_new, long long *counts, int *empty,
                                   double *compj) {
+  const bool stage = compj && (sizeof(T) + sizeof(double)) * (size_t)d <= 65536;
+  const size_t lds = sizeof(T) * d + (stage ? sizeof(double) * d : 0);
   hipLaunchKernelGGL(k_final_centres<T>, dim3(k), dim3(d < 256 ? 64 * (int)km_cdiv(d, 64) : 256),
-                     sizeof(T) * d, s, d, k, S, cnt, C_old, C_new, Ct_new, ldc, nc_new, counts,
-                     empty, compj);
+                     lds, s, d, k, S, cnt, C_old, C_new, Ct_new, ldc, nc_new, counts, empty,
+                     compj, (int)stage);
   return hipGetLastError();
 }
 
@@ -1115,11 +1233,11 @@ static hipError_t launch_update_finalize_t(hipStream_t s, int d, int k, const do
 // the new centres (reduce with launch_comp_reduce).
 template <typename T>
 static hipError_t launch_update_t(hipStream_t s, long n, int d, int k, const T *X, const int *labels,
-                         const double *q, const T *C_old, T *C_new, T *Ct_new, int ldc,
+                         const T *C_old, T *C_new, T *Ct_new, int ldc,
                          T *nc_new, long long *counts, int *empty, const UpdateWs &ws) {
   hipError_t e;
   if ((e = launch_update_zero(s, d, k, ws.S, ws.cnt, empty)) != hipSuccess) return e;
-  if ((e = launch_update_accumulate(s, n, d, k, X, labels, q, ws.S, ws.cnt, ws)) != hipSuccess)
+  if ((e = launch_update_accumulate(s, n, d, k, X, labels, ws.S, ws.cnt, ws)) != hipSuccess)
     return e;
   return launch_update_finalize(s, d, k, ws.S, ws.cnt, C_old, C_new, Ct_new, ldc, nc_new, counts,
                                 empty, ws.compj);
@@ -1133,11 +1251,10 @@ hipError_t launch_comp_reduce(hipStream_t s, int k, const double *compj, double 
 // Compactness of labels against centres C (any centres): moments, then terms.
 template <typename T>
 static hipError_t launch_compactness_t(hipStream_t s, long n, int d, int k, const T *X,
-                              const int *labels, const double *q, const T *C, double *out,
-                              const UpdateWs &ws) {
+                              const int *labels, const T *C, double *out, const UpdateWs &ws) {
   hipError_t e;
   if ((e = launch_update_zero(s, d, k, ws.S, ws.cnt, nullptr)) != hipSuccess) return e;
-  if ((e = launch_update_accumulate(s, n, d, k, X, labels, q, ws.S, ws.cnt, ws)) != hipSuccess)
+  if ((e = launch_update_accumulate(s, n, d, k, X, labels, ws.S, ws.cnt, ws)) != hipSuccess)
     return e;
   hipLaunchKernelGGL(k_comp_terms<T>, dim3(km_cdiv(k, 256)), dim3(256), 0, s, d, k, ws.S, ws.cnt, C,
                      ws.compj);
@@ -1265,9 +1382,9 @@ static hipError_t launch_gather_rows_t(hipStream_t s, long n, int d, const T *X,
     return launch_transpose_t<T>(s, k, d, C, Ct, ldc);                                           \
   }                                                                                              \
   hipError_t launch_update_accumulate(hipStream_t s, long n, int d, int k, const T *X,           \
-                                      const int *labels, const double *q, double *S,             \
-                                      long long *cnt, const UpdateWs &ws) {                      \
-    return launch_update_accumulate_t<T>(s, n, d, k, X, labels, q, S, cnt, ws);                  \
+                                      const int *labels, double *S, long long *cnt,              \
+                                      const UpdateWs &ws) {                                      \
+    return launch_update_accumulate_t<T>(s, n, d, k, X, labels, S, cnt, ws);                     \
   }                                                                                              \
   hipError_t launch_update_finalize(hipStream_t s, int d, int k, const double *S,                \
                                     const long long *cnt, const T *C_old, T *C_new, T *Ct_new,   \
@@ -1277,15 +1394,15 @@ static hipError_t launch_gather_rows_t(hipStream_t s, long n, int d, const T *X,
                                        counts, empty, compj);                                    \
   }                                                                                              \
   hipError_t launch_update(hipStream_t s, long n, int d, int k, const T *X, const int *labels,   \
-                           const double *q, const T *C_old, T *C_new, T *Ct_new, int ldc,        \
-                           T *nc_new, long long *counts, int *empty, const UpdateWs &ws) {       \
-    return launch_update_t<T>(s, n, d, k, X, labels, q, C_old, C_new, Ct_new, ldc, nc_new,       \
+                           const T *C_old, T *C_new, T *Ct_new, int ldc, T *nc_new,              \
+                           long long *counts, int *empty, const UpdateWs &ws) {                  \
+    return launch_update_t<T>(s, n, d, k, X, labels, C_old, C_new, Ct_new, ldc, nc_new,          \
                               counts, empty, ws);                                                \
   }                                                                                              \
   hipError_t launch_compactness(hipStream_t s, long n, int d, int k, const T *X,                 \
-                                const int *labels, const double *q, const T *C, double *out,     \
+                                const int *labels, const T *C, double *out,                      \
                                 const UpdateWs &ws) {                                            \
-    return launch_compactness_t<T>(s, n, d, k, X, labels, q, C, out, ws);                        \
+    return launch_compactness_t<T>(s, n, d, k, X, labels, C, out, ws);                           \
   }                                                                                              \
   hipError_t launch_col_norms_dk(hipStream_t s, int k, int d, int ld, const T *C, T *nc) {       \
     return launch_col_norms_dk_t<T>(s, k, d, ld, C, nc);                                         \

@@ -67,7 +67,7 @@ extern "C" kmeansCudaErrorStatus kmUpdateNoAlloc(void *stream, long n, int d, in
   }
   km::UpdateWs ws;
   km::update_ws_layout(n, d, k, (char *)base, &ws);
-  e = km::launch_update((hipStream_t)stream, n, d, k, X, labels, nullptr, C_old, C_new, Ct_new,
+  e = km::launch_update((hipStream_t)stream, n, d, k, X, labels, C_old, C_new, Ct_new,
                         km::centers_ld(k), nc_new, counts, empty, ws);
   if (e != hipSuccess) {
     fprintf(stderr, "kmeans_amd: kmUpdateNoAlloc: %s\n", hipGetErrorString(e));

@@ -25,9 +25,9 @@
  *                    becomes the double chain (fma in double, scores start
  *                    at -0.5*nc_j), sums and moments are unchanged (they are
  *                    FP64 already), centres are S/n without the cast to
- *                    float.  q_i is the left-to-right sum of the rounded
- *                    squares x*x, which for float data equals the FP32
- *                    definition below (those squares are exact).
+ *                    float.  The squares in T_c are the rounded x*x, which
+ *                    for float data equals the FP32 definition below (those
+ *                    squares are exact).
  *   orc_row_transform_min{f,d}  main/cuda/RowTransformMinimum.cu:3-58.
  *   orc_kmeanspp     k-means++ seeding (Arthur & Vassilvitskii 2007) in this
  *                    build's canonical form (see the function); the
@@ -40,9 +40,12 @@
  *   Points are cut into blocks of KM_BLOCK consecutive indices.
  *   Centre sum   S[j][d] = (((0 + B_0[j][d]) + B_1[j][d]) + ...) where
  *                B_b[j][d] = CSUM(x[i][d] : i in block b, label i == j, i asc).
- *   Moments      Q[j] = (((0 + QB_0[j]) + QB_1[j]) + ...), QB_b[j] = CSUM(q_i : i in
- *                block b, label i == j, i asc), q_i = left-to-right FP64 sum over
- *                d of (double)x[i][d]^2 (each square exact in FP64).
+ *   Moments      Q[j] = (((0 + QB_0[j]) + QB_1[j]) + ...).  The members of j in
+ *                block b (i asc) are cut into KM_CHUNK pieces as CSUM cuts them;
+ *                piece c gives q_c = left-to-right sum over d of T_c[d], T_c[d] =
+ *                left-to-right FP64 sum over the piece of (double)x[i][d]^2 (each
+ *                square exact in FP64); QB_b[j] = q_0 for one piece, else CSUM of
+ *                the q_c, as CSUM combines piece sums.
  *   Compactness  comp = CSUM over j of comp_j, comp_j = (Q[j] - 2 a_j) + n_j b_j,
  *                a_j = left-to-right FP64 sum over d of (double)c'[j][d]*S[j][d],
  *                b_j = left-to-right FP64 sum over d of (double)c'[j][d]^2; this is
@@ -251,21 +254,34 @@ double orc_csum(const double *v, long n) {
   return r;
 }
 
-/* q_i: left-to-right FP64 sum of squares (each (double)x*(double)x is exact
- * for float x; for double x each square is rounded once, no fma). */
-#define DEF_POINT_SQ(NAME, T)                                                  \
-  static inline double NAME(const T *x, int d) {                               \
-    double q = 0.0;                                                            \
-    for (int t = 0; t < d; ++t) q += (double)x[t] * (double)x[t];              \
-    return q;                                                                  \
+/* The squared-norm moment of one chunk of members (rows[0..m) of X, m <=
+ * KM_CHUNK, ascending point order): for each dimension q the left-to-right
+ * FP64 sum over the members of x_q * x_q (exact for float x; for double x
+ * each square is rounded once, no fma), then the left-to-right sum of those
+ * d values in ascending q.  The per-dimension sums are what the chunk-sum
+ * kernels accumulate beside the coordinate sums from the rows they read, so
+ * no per-point squared norm is gathered. */
+#define DEF_CHUNK_SQ(NAME, T)                                                              \
+  static double NAME(const T *X, const long *rows, long m, int d) {                        \
+    double tot = 0.0;                                                                      \
+    for (int q = 0; q < d; ++q) {                                                          \
+      double t = 0.0;                                                                      \
+      for (long r = 0; r < m; ++r) {                                                       \
+        const double v = (double)X[rows[r] * d + q];                                       \
+        t += v * v;                                                                        \
+      }                                                                                    \
+      tot += t;                                                                            \
+    }                                                                                      \
+    return tot;                                                                            \
   }
-DEF_POINT_SQ(point_sq, float)
-DEF_POINT_SQ(point_sq_f64, double)
+DEF_CHUNK_SQ(chunk_sq, float)
+DEF_CHUNK_SQ(chunk_sq_f64, double)
 
 /* Per-block canonical centre sums.  For block b and each centre j, collect
  * the members in ascending i (stable counting sort by label), then CSUM each
- * dimension and the squared norms.  Adds B_b[j][d] into S (FP64, K x d),
- * QB_b[j] into Q and n_b into counts. */
+ * dimension; the squared norms are summed per chunk of KM_CHUNK members
+ * (chunk_sq) and the chunk values combined as CSUM combines chunk partials.
+ * Adds B_b[j][d] into S (FP64, K x d), QB_b[j] into Q and n_b into counts. */
 #define DEF_BLOCK_UPDATE(NAME, T, PSQ)                                                     \
   static void NAME(const T *X, long i0, long n, int d, const int *labels, int k,           \
                    double *S, double *Q, long long *counts) {                              \
@@ -289,15 +305,21 @@ DEF_POINT_SQ(point_sq_f64, double)
           for (long t = 0; t < m; ++t) v[t] = (double)X[perm[a + t] * d + q];              \
           S[(long)j * d + q] += orc_csum(v, m);                                            \
         }                                                                                  \
-        for (long t = 0; t < m; ++t) v[t] = PSQ(X + perm[a + t] * d, d);                   \
-        Q[j] += orc_csum(v, m);                                                            \
+        {                                                                                  \
+          const long nq = (m + KM_CHUNK - 1) / KM_CHUNK;                                   \
+          for (long c = 0; c < nq; ++c) {                                                  \
+            const long ca = c * KM_CHUNK, cm = m - ca < KM_CHUNK ? m - ca : KM_CHUNK;      \
+            v[c] = PSQ(X, perm + a + ca, cm, d);                                           \
+          }                                                                                \
+          if (m > 0) Q[j] += nq == 1 ? v[0] : orc_csum(v, nq);                             \
+        }                                                                                  \
       }                                                                                    \
       free(v);                                                                             \
     }                                                                                      \
     free(perm); free(off); free(cnt);                                                      \
   }
-DEF_BLOCK_UPDATE(block_update, float, point_sq)
-DEF_BLOCK_UPDATE(block_update_f64, double, point_sq_f64)
+DEF_BLOCK_UPDATE(block_update, float, chunk_sq)
+DEF_BLOCK_UPDATE(block_update_f64, double, chunk_sq_f64)
 
 /* Canonical per-centre moments over all points: S (k x d), Q (k), counts. */
 #define DEF_MOMENTS(NAME, T, BLK)                                                          \

@@ -192,6 +192,32 @@ def test_update_skewed_wide(orc, d):
     assert comp == orc.compactness(X, lab, rC)
 
 
+@pytest.mark.parametrize("d", [2, 16])
+def test_update_squared_norm_order_known_answer(orc, d):
+    # the oracle's pinned moment order (per KM_CHUNK piece, dimension by
+    # dimension): against zero centres the compactness is the moment itself,
+    # 2^60 + 16896; d = 2 takes the one-dimension-per-thread chunk sums, d = 16
+    # the four-dimensions-per-thread ones
+    X = np.zeros((513, d), np.float32)
+    X[0, 0] = 2.0 ** 30
+    X[1:256, 1] = 8.0
+    X[256:512, 1] = 1.0
+    X[512, 1] = 12.0
+    lab = np.zeros(513, np.int32)
+    n, k = 513, 1
+    Xd, ld, Cz = dev(X), dev(lab), dev(np.zeros((1, d), np.float32))
+    wsb = km.update_workspace_size(n, d, k)
+    ws = torch.empty(wsb, dtype=torch.uint8, device="cuda")
+    out = torch.empty(1, dtype=torch.float64, device="cuda")
+    km.compactness(n, d, k, Xd.data_ptr(), ld.data_ptr(), Cz.data_ptr(), out.data_ptr(),
+                   ws.data_ptr(), wsb)
+    torch.cuda.synchronize()
+    assert float(out.item()) == 2.0 ** 60 + 16896.0
+    C1, cnt, emp, nc, comp = gpu_update(X, lab, np.zeros((1, d), np.float32))
+    rC, _, _, _ = orc.update(X, lab, np.zeros((1, d), np.float32))
+    assert np.array_equal(C1, rC) and comp == orc.compactness(X, lab, rC)
+
+
 def test_update_spans_two_blocks(orc):
     # the oracle's block known answer: 2^53 then ones over KM_BLOCK + 1000 rows
     n = km.KM_BLOCK + 1000

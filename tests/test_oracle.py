@@ -187,6 +187,40 @@ def test_update_block_structure_known_answer(orc):
     assert cnt[0] == n
 
 
+def q_order_rows(d=2):
+    """One centre, 513 members: row 0 = (2^30, 0, ...), 255 rows (0, 8, ...),
+    256 rows (0, 1, ...), one row (0, 12, ...)."""
+    X = np.zeros((513, d), np.float32)
+    X[0, 0] = 2.0 ** 30
+    X[1:256, 1] = 8.0
+    X[256:512, 1] = 1.0
+    X[512, 1] = 12.0
+    return X
+
+
+Q_ORDER_ANSWER = 2.0 ** 60 + 16896.0
+
+
+def test_moments_squared_norm_order_known_answer(orc):
+    # The squared-norm moment sums each KM_CHUNK piece of members dimension by
+    # dimension, then combines the pieces as CSUM does:
+    #   piece 0 = fl(2^60 + 255 * 64) = 2^60 + 16384 (ulp 256 at 2^60),
+    #   piece 1 = 256, piece 2 = 144,
+    #   Q = fl(fl(2^60 + 16384 + 256) + 144) = fl(2^60 + 16784) = 2^60 + 16896.
+    # Summing each point's squared norm first would give 2^60 + 512, and
+    # per-dimension totals over all members 2^60 + 16640: both are excluded.
+    for d in (2, 16):
+        X = q_order_rows(d)
+        lab = np.zeros(513, np.int32)
+        _, Q, cnt = orc.moments(X, lab, 1)
+        assert cnt[0] == 513
+        assert Q[0] == Q_ORDER_ANSWER
+        _, Q64, _ = orc.moments64(X.astype(np.float64), lab, 1)
+        assert Q64[0] == Q_ORDER_ANSWER
+        # against zero centres the compactness is the moment itself
+        assert orc.compactness(X, lab, np.zeros((1, d), np.float32)) == Q_ORDER_ANSWER
+
+
 # ------------------------------------------------------------------ compactness
 def test_compactness_fp64_and_identity(orc):
     X, z, means, _ = orc.mixture(9, 30_000, 20, 16, -6, 6, 0.8, 1.2)
