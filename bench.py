@@ -126,6 +126,7 @@ def main():
         "reresolved_per_step": st["reresolved"] / args.steps,
         "rescored_per_step": st["rescored"] / args.steps,
         "deep_per_step": st["deep"] / args.steps,
+        "assign_tiles_frac": (st["tiles"] / st["tiles_total"]) if st["tiles_total"] else None,
         "roofline": {"bound": "mfma",
                      "achieved": achieved / 1e12, "peak": PEAK[args.precision] / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK[args.precision],

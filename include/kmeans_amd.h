@@ -187,6 +187,10 @@ typedef struct kmeansStats {
   double train_loss_first;    /* MLP cross-entropy, first and last batch      */
   double train_loss_last;
   long long deep;             /* points given to the split-BF16 second pass   */
+  long long tiles;            /* (256-point, 32-centre) tiles the BF16/FP16
+                                 assign computed; the rest were cut by the
+                                 triangle bound                               */
+  long long tiles_total;      /* ... out of this many                         */
 } kmeansStats;
 
 typedef struct kmeansEngine *kmeansHandle;

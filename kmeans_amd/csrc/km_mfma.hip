@@ -44,7 +44,9 @@ typedef __attribute__((ext_vector_type(4))) unsigned int u32x4_t;
 #define MF_WAVES 4     // waves per workgroup; every wave reads the same centre stages
 #endif
 #define MF_PTS (64 * MF_WAVES)  // points per workgroup (2 groups of 32 per wave)
-#define MF_STAGE 4     // centre tiles per LDS stage (128 centres)
+#define MF_STAGE 2     // centre tiles per LDS stage (64 centres): 33 KB of LDS a
+                       // workgroup, so four fit on a CU
+#define MF_KPAD 128    // centres are padded to a multiple of this
 // rescored-point counter of one assign launch: 64 counters on separate
 // 128-byte lines (workgroup b adds into counter b % 64), summed on the host
 #define NC_SPREAD 64
@@ -65,7 +67,7 @@ __device__ inline uint16_t f32_to_f16_rne(float f) {
 struct MfmaState {
   int Dp = 0;        // padded dimension (multiple of 16)
   int KS = 0;        // k-steps of 16
-  int Kp = 0;        // centres padded to MF_TILE
+  int Kp = 0;        // centres padded to MF_KPAD
   bool fp16 = false;
   float scale = 1.f; // power of two applied to X and C before conversion (FP16)
   uint16_t *Xh = nullptr;   // m x Dp
@@ -89,6 +91,7 @@ struct MfmaState {
   // distance from centre j (k <= MF_ORD_MAX_K), home[wg] = the previous label
   // of a workgroup's first point
   int *ttab = nullptr;
+  float *tdist = nullptr;   // tdist[j][rank]: lower bound of |c_j - c_k| over tile ttab[j][rank]
   int *home = nullptr;
   long home_cap = 0;
   bool ttab_ok = false;
@@ -184,13 +187,19 @@ __global__ __launch_bounds__(256) void k_centre_bounds(int k, int d, const float
 }
 
 // ttab[j][rank] = the centre tiles (MF_TILE rows each) in increasing order of
-// the smallest squared distance from centre j to a member (ties to the lower
-// tile), one workgroup per centre.  FP32 distances from the norms and the
-// transposed centres; only an order of visit, so rounding does not matter.
+// a lower bound of the smallest squared distance from centre j to a member
+// (ties to the lower tile), one workgroup per centre, and tdist[j][rank] =
+// the square root of that bound (+inf for a tile of padding rows).  The
+// squared distance is nc_j + nc_i - 2 c_j.c_i in FP32 (fma chain for the dot
+// product); nc are FP32 chains of squares.  Each of the three is within
+// gamma_D of its exact value, relative to nc_j + nc_i (|c_j.c_i| is at most
+// half of it), and the two subtractions add at most 2 ulps of that size, so
+// subtracting (2 D + 8) 2^-24 (nc_j + nc_i) (with a 1% margin) gives a bound.
 __global__ __launch_bounds__(256) void k_tile_order(int K, int D, int nKT,
                                                     const float *__restrict__ Ct, int ldc,
                                                     const float *__restrict__ nc,
-                                                    int *__restrict__ ttab) {
+                                                    int *__restrict__ ttab,
+                                                    float *__restrict__ tdist) {
   extern __shared__ float to_smem[];
   float *cj = to_smem;                                              // [D]
   unsigned *key = reinterpret_cast<unsigned *>(to_smem + D);       // [nKT]
@@ -202,7 +211,8 @@ __global__ __launch_bounds__(256) void k_tile_order(int K, int D, int nKT,
   for (int i = tid; i < K; i += 256) {
     float dot = 0.0f;
     for (int d = 0; d < D; ++d) dot = __builtin_fmaf(cj[d], Ct[(long)d * ldc + i], dot);
-    const float d2 = fmaxf(ncj + nc[i] - 2.0f * dot, 0.0f);
+    const float err = (2.0f * (float)D + 8.0f) * 5.9604645e-8f * 1.01f * (ncj + nc[i]);
+    const float d2 = fmaxf((ncj + nc[i] - 2.0f * dot) - err, 0.0f);
     atomicMin(&key[i / MF_TILE], __float_as_uint(d2));  // non-negative: bits order as values
   }
   __syncthreads();
@@ -211,6 +221,7 @@ __global__ __launch_bounds__(256) void k_tile_order(int K, int D, int nKT,
     int r = 0;
     for (int u = 0; u < nKT; ++u) r += (key[u] < kt) || (key[u] == kt && u < t);
     ttab[(long)j * nKT + r] = t;
+    tdist[(long)j * nKT + r] = sqrtf(__uint_as_float(kt)) * (1.0f - 1e-6f);
   }
 }
 
@@ -240,8 +251,11 @@ struct Top3 {
   float b1 = -FLT_MAX, b2 = -FLT_MAX, b3 = -FLT_MAX;
   int t1 = 0, t2 = 0;
   // label-ordered assign: U = the largest score of the tiles whose insertion
-  // was skipped, th = the skip threshold, mg = 2E in MFMA score units
+  // was skipped, th = the skip threshold, mg = 2E in MFMA score units; hs =
+  // the point's approximate score of the workgroup's home centre (MFMA
+  // units), xn >= |x|, E the error bound (FP32 units)
   float U = -INFINITY, th = -INFINITY, mg = 0.0f;
+  float hs = -INFINITY, xn = 0.0f, E = 0.0f;
 };
 
 // Forward-error bound E (FP32 score units) of every centre's approximate
@@ -294,10 +308,7 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
     float inv_s2, float eta, float big, int *__restrict__ labels,
     int *__restrict__ flags, int *__restrict__ nflag, int *__restrict__ ncand,
     const int *__restrict__ stop, const int *__restrict__ order, const int *__restrict__ ttab,
-    const int *__restrict__ home) {
-  // stop (nullable): set by a device-side convergence test; the launch then
-  // assigns nothing and lists nothing
-  if (stop && *stop) return;
+    const float *__restrict__ tdist, const int *__restrict__ home) {
   constexpr int Dp = KS * 16;
   constexpr int CPR = Dp / 8;               // 16-byte chunks per centre row
   constexpr int SROWS = MF_STAGE * MF_TILE; // centre rows per LDS stage
@@ -310,20 +321,21 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
   u32x4_t *As = reinterpret_cast<u32x4_t *>(smem);                         // [2][SCH]
   float *Bs = reinterpret_cast<float *>(smem + 2 * SCH * sizeof(u32x4_t));  // [2][SROWS]
   int *Ts = reinterpret_cast<int *>(Bs + 2 * SROWS);                        // [nKT] (ORD)
+  float *Dt = reinterpret_cast<float *>(Ts + Kp / MF_TILE);                 // [nKT] (ORD)
+  __shared__ int cutv[2][MF_WAVES];                                         // (ORD)
   typedef typename std::conditional<FP16, f16x8_t, bf16x8_t>::type vec8;
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, h = lane >> 5, col = lane & 31;
   const long p0 = (long)blockIdx.x * MF_PTS + w * 64;  // sorted positions of the wave
-  const int nKT = Kp / MF_TILE;      // multiple of MF_STAGE
+  const int nKT = Kp / MF_TILE;      // multiple of MF_KPAD / MF_TILE = 4
   const int nS = nKT / MF_STAGE;
-  if constexpr (ORD) {  // the workgroup's tile order, read once
-    const int *tr = ttab + (long)home[blockIdx.x] * nKT;
-    for (int t = tid; t < nKT; t += 64 * MF_WAVES) Ts[t] = tr[t];
-    __syncthreads();
-  }
-
+  // ORD: the home centre, its tile and the register (hr) and lane half (hh)
+  // that hold its row of that tile's scores
+  const int hc = ORD ? home[blockIdx.x] : 0;
+  const int htile = hc / MF_TILE, hrow = hc % MF_TILE;
+  const int hh = (hrow >> 2) & 1, hr = ((hrow >> 3) << 2) | (hrow & 3);
   // point fragments: lane holds the point at sorted position p0 + g*32 + col,
-  // k = ks*16 + 8h + 0..7
+  // k = ks*16 + 8h + 0..7 (issued first: the loads below do not wait on them)
   u32x4_t bfrag[2][KS];
 #pragma unroll
   for (int g = 0; g < 2; ++g) {
@@ -337,6 +349,25 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
         bfrag[g][ks] = u32x4_t{0u, 0u, 0u, 0u};
     }
   }
+  int tsv = 0;  // ORD: this thread's entries of the tile order and bounds
+  float dsv = 0.0f;
+  if constexpr (ORD) {
+    if (tid < nKT) {
+      tsv = ttab[(long)hc * nKT + tid];
+      dsv = tdist[(long)hc * nKT + tid];
+    }
+  }
+  // stop (nullable): set by a device-side convergence test; the launch then
+  // assigns nothing and lists nothing (checked after the loads above are in
+  // flight, which are in bounds either way)
+  if (stop && *stop) return;
+  if constexpr (ORD) {  // the workgroup's tile order and distance bounds
+    for (int t = tid; t < nKT; t += 64 * MF_WAVES) {
+      Ts[t] = t == tid ? tsv : ttab[(long)hc * nKT + t];
+      Dt[t] = t == tid ? dsv : tdist[(long)hc * nKT + t];
+    }
+    __syncthreads();
+  }
 
   // per point group: the three largest packed scores and the tiles of the
   // first two (named structs, not arrays: selects between array elements
@@ -348,8 +379,11 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
     for (int g = 0; g < 2; ++g) {
       const long sp = p0 + g * 32 + col;
       const float2 nb = sp < M ? xnb[mf_point<ORD>(order, sp)] : make_float2(0.0f, 0.0f);
-      const float e2 = 2.0f * mf_err_bound(nb.x, nb.y, (float)D, cmx, ncx, dcx, eta) / inv_s2;
-      (g == 0 ? S0 : S1).mg = e2;
+      const float E = mf_err_bound(nb.x, nb.y, (float)D, cmx, ncx, dcx, eta);
+      Top3 &T = g == 0 ? S0 : S1;
+      T.mg = 2.0f * E / inv_s2;
+      T.E = E;
+      T.xn = nb.x;
     }
   }
 
@@ -443,6 +477,13 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
   // other score, which is all the certificates use (see finish).
   auto epi_group = [&](int kt, const f32x16_t &acc, Top3 &T) {
     if constexpr (ORD) {
+      if (mf_tile<ORD>(Ts, kt) == htile) {  // the home centre's score of each point
+        float v = acc[0];
+#pragma unroll
+        for (int r = 1; r < 16; ++r) v = r == hr ? acc[r] : v;
+        const float o = __shfl_xor(v, 32);
+        T.hs = h == hh ? v : o;
+      }
       float m = acc[0];
 #pragma unroll
       for (int r = 1; r < 16; ++r) m = __builtin_fmaxf(m, acc[r]);
@@ -476,29 +517,85 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
   // Software pipeline: while the VALU runs the epilogue of tile t, the
   // matrix core computes tile t+1 (same basic block); stage s+1 streams into
   // the other LDS buffer while stage s is consumed.  nKT is a multiple of
-  // MF_STAGE (at least 4), so the loop covers tiles [0, nKT - 2) and the
+  // 4 (and so of MF_STAGE), so the loop covers tiles [0, nKT - 2) and the
   // last pair is peeled: no tile past the end is computed and no barrier is
   // waited at after the last stage.
+  // ORD cut: every centre k of a tile at rank >= kn is at least Dt[kn] from
+  // the home centre c, so |x - c_k| >= Dt[kn] - |x - c| (triangle
+  // inequality) and its exact score x.c_k - 0.5|c_k|^2 = 0.5 (|x|^2 -
+  // |x - c_k|^2) is at most 0.5 (xn^2 - g^2) with g = Dt[kn] - hb, hb >=
+  // |x - c| from the home score (exact >= hs - E): hb^2 = xn^2 - 2 (hs - E).
+  // That plus E bounds the approximate scores of all those tiles.  When the
+  // bound is below the skip threshold for every point of the workgroup, the
+  // remaining tiles are not computed and the bound joins U, as for a
+  // skipped insertion.  (Roundings of the bound are covered by relative
+  // margins of 1e-6 and more; Dt is itself a lower bound.)
+  auto cut_bound = [&](const Top3 &T, float dn, float &ubm) -> bool {
+    const float a_h = T.hs * inv_s2;
+    const float x2 = T.xn * T.xn * (1.0f + 2.4e-7f);
+    const float h2 = (x2 - 2.0f * (a_h - T.E)) +
+                     3.6e-7f * (x2 + 2.0f * __builtin_fabsf(a_h) + 2.0f * T.E);
+    const float hb = __builtin_sqrtf(__builtin_fmaxf(h2, 0.0f)) * (1.0f + 1e-6f);
+    const float g = (dn - hb) * (1.0f - 1e-6f);
+    float ub = 0.5f * (x2 - g * g) + T.E;
+    ub += 1e-6f * (x2 + g * g + T.E);
+    ubm = ub / inv_s2;
+    ubm += 1e-6f * __builtin_fabsf(ubm);
+    return g > 0.0f && ubm < T.th;
+  };
   issue_stage(0, 0);
   __builtin_amdgcn_s_waitcnt(0);  // point fragments and stage 0 have arrived
   __syncthreads();
   f32x16_t A0, A1, B0, B1;
   mfma_tile(0, A0, A1);
   int kt = 0;
+  bool cut = false;
   for (; kt + 2 < nKT; kt += 2) {
     if (kt % MF_STAGE == 0 && kt / MF_STAGE + 1 < nS) issue_stage(kt / MF_STAGE + 1, (kt / MF_STAGE + 1) & 1);
     mfma_tile(kt + 1, B0, B1);
     epi_tile(kt, A0, A1);
     if ((kt + 2) % MF_STAGE == 0) {  // next tile opens a new stage
       __builtin_amdgcn_s_waitcnt(0);
-      __syncthreads();
+      if constexpr (ORD) {
+        // workgroup vote: each wave's verdict in LDS, double-buffered by
+        // stage parity (a wave can be at most one stage boundary ahead)
+        const float dn = Dt[kt + 2];
+        float u0 = -INFINITY, u1 = -INFINITY;
+        const bool ok0 = !(p0 + col < M) || cut_bound(S0, dn, u0);
+        const bool ok1 = !(p0 + 32 + col < M) || cut_bound(S1, dn, u1);
+        const int par = ((kt + 2) / MF_STAGE) & 1;
+        const bool wok = __ballot(!(ok0 && ok1)) == 0;
+        if (lane == 0) cutv[par][w] = wok ? 1 : 0;
+        __syncthreads();
+        int all = 1;
+#pragma unroll
+        for (int u = 0; u < MF_WAVES; ++u) all &= cutv[par][u];
+        if (all) {
+          epi_tile(kt + 1, B0, B1);
+          S0.U = __builtin_fmaxf(S0.U, u0);
+          S1.U = __builtin_fmaxf(S1.U, u1);
+          cut = true;
+          break;
+        }
+      } else {
+        __syncthreads();
+      }
     }
     mfma_tile(kt + 2, A0, A1);
     epi_tile(kt + 1, B0, B1);
   }
-  mfma_tile(kt + 1, B0, B1);  // kt = nKT - 2: a pair within the last stage
-  epi_tile(kt, A0, A1);
-  epi_tile(kt + 1, B0, B1);
+  if (!cut) {
+    mfma_tile(kt + 1, B0, B1);  // kt = nKT - 2: a pair within the last stage
+    epi_tile(kt, A0, A1);
+    epi_tile(kt + 1, B0, B1);
+  }
+  // tiles computed and tiles in all, beside the rescored count of the same
+  // counter line
+  if (tid == 0) {
+    int *cl = ncand + (blockIdx.x & (NC_SPREAD - 1)) * NC_STRIDE;
+    atomicAdd(cl + 1, cut ? kt + 2 : nKT);
+    atomicAdd(cl + 2, nKT);
+  }
 
   // merge the two lane halves (same points, different centre rows)
   const float cmax = bnd[0], ncmax = bnd[1], dcmax = bnd[2];
@@ -918,6 +1015,7 @@ void mfma_free(Engine *e) {
   hipFree(s->nflag2);
   hipFree(s->kbest);
   hipFree(s->ttab);
+  hipFree(s->tdist);
   hipFree(s->home);
   delete s;
   e->mfma = nullptr;
@@ -945,7 +1043,7 @@ static hipError_t ensure_state(Engine *e) {
   else if (s->KS > 4) s->KS = 6;
   else if (s->KS > 2) s->KS = 4;
   s->Dp = s->KS * 16;
-  s->Kp = (int)km_cdiv(e->k, MF_TILE * MF_STAGE) * MF_TILE * MF_STAGE;
+  s->Kp = (int)km_cdiv(e->k, MF_KPAD) * MF_KPAD;
   hipError_t r;
   const long xrows = e->streaming ? e->tile : e->m;  // points held in HBM at once
   if ((r = hipMalloc(&s->Xh, sizeof(uint16_t) * (size_t)xrows * s->Dp)) != hipSuccess) return r;
@@ -1103,8 +1201,11 @@ static hipError_t prepare_centres(Engine *e) {
     if (!s->ttab &&
         (r = hipMalloc(&s->ttab, sizeof(int) * (size_t)e->k * nKT)) != hipSuccess)
       return r;
+    if (!s->tdist &&
+        (r = hipMalloc(&s->tdist, sizeof(float) * (size_t)e->k * nKT)) != hipSuccess)
+      return r;
     hipLaunchKernelGGL(k_tile_order, dim3(e->k), dim3(256), sizeof(float) * (e->n + nKT),
-                       e->stream, e->k, e->n, nKT, e->dCt, e->ldc, e->dnc, s->ttab);
+                       e->stream, e->k, e->n, nKT, e->dCt, e->ldc, e->dnc, s->ttab, s->tdist);
     s->ttab_ok = true;
   }
   return hipGetLastError();
@@ -1129,10 +1230,10 @@ static void launch_mfma_ks(hipStream_t strm, bool fp16, dim3 g, long M, const ui
                            const float *bias, const float *C, const float *nc, const float *bnd,
                            const float2 *xnb, float inv_s2, float eta, int *labels,
                            int *flags, int *nflag, int *ncand, const int *stop, const int *order,
-                           const int *ttab, const int *home) {
+                           const int *ttab, const float *tdist, const int *home) {
   const bool ord = order != nullptr;
   const size_t lds = 2 * (size_t)MF_STAGE * MF_TILE * (KS * 2) * 16 + 2 * MF_STAGE * MF_TILE * 4 +
-                     (ord ? sizeof(int) * (size_t)(Kp / MF_TILE) : 0);
+                     (ord ? 2 * sizeof(int) * (size_t)(Kp / MF_TILE) : 0);
   const void *fn = fp16 ? (ord ? (const void *)k_assign_mfma<KS, true, true>
                                : (const void *)k_assign_mfma<KS, true, false>)
                         : (ord ? (const void *)k_assign_mfma<KS, false, true>
@@ -1145,7 +1246,7 @@ static void launch_mfma_ks(hipStream_t strm, bool fp16, dim3 g, long M, const ui
   }
 #define KM_ARGS                                                                                 \
   M, Xh, X, D, K, Kp, Ch, bias, C, nc, bnd, xnb, inv_s2, eta, FLT_MAX, labels, flags, nflag,    \
-      ncand, stop, order, ttab, home
+      ncand, stop, order, ttab, tdist, home
   if (fp16 && ord)
     hipLaunchKernelGGL((k_assign_mfma<KS, true, true>), g, dim3(64 * MF_WAVES), lds, strm, KM_ARGS);
   else if (fp16)
@@ -1208,12 +1309,12 @@ hipError_t mfma_assign_view(Engine *e, const float *X, const uint16_t *Xh, const
   const float inv_s2 = 1.0f / (s->scale * s->scale);
   dim3 g(km_cdiv(rows, MF_PTS));
   switch (s->KS) {
-    case 1: launch_mfma_ks<1>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, order ? s->ttab : nullptr, home); break;
-    case 2: launch_mfma_ks<2>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, order ? s->ttab : nullptr, home); break;
-    case 4: launch_mfma_ks<4>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, order ? s->ttab : nullptr, home); break;
-    case 6: launch_mfma_ks<6>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, order ? s->ttab : nullptr, home); break;
-    case 8: launch_mfma_ks<8>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, order ? s->ttab : nullptr, home); break;
-    default: launch_mfma_ks<16>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, order ? s->ttab : nullptr, home); break;
+    case 1: launch_mfma_ks<1>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, order ? s->ttab : nullptr, order ? s->tdist : nullptr, home); break;
+    case 2: launch_mfma_ks<2>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, order ? s->ttab : nullptr, order ? s->tdist : nullptr, home); break;
+    case 4: launch_mfma_ks<4>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, order ? s->ttab : nullptr, order ? s->tdist : nullptr, home); break;
+    case 6: launch_mfma_ks<6>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, order ? s->ttab : nullptr, order ? s->tdist : nullptr, home); break;
+    case 8: launch_mfma_ks<8>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, order ? s->ttab : nullptr, order ? s->tdist : nullptr, home); break;
+    default: launch_mfma_ks<16>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, order ? s->ttab : nullptr, order ? s->tdist : nullptr, home); break;
   }
   if ((r = hipGetLastError()) != hipSuccess) return r;
   return hipGetLastError();
@@ -1331,7 +1432,11 @@ void mfma_collect(Engine *e, int nslots) {
     return;
   if (hipStreamSynchronize(e->stream) != hipSuccess) return;
   for (int i = 0; i < n; ++i)
-    for (int c = 0; c < NC_SPREAD; ++c) e->stats.rescored += w[(size_t)i * NC_SLOT + c * NC_STRIDE];
+    for (int c = 0; c < NC_SPREAD; ++c) {
+      e->stats.rescored += w[(size_t)i * NC_SLOT + c * NC_STRIDE];
+      e->stats.tiles += w[(size_t)i * NC_SLOT + c * NC_STRIDE + 1];
+      e->stats.tiles_total += w[(size_t)i * NC_SLOT + c * NC_STRIDE + 2];
+    }
 }
 
 }  // namespace km

@@ -234,3 +234,34 @@ def test_batch_rejects_misaligned_borrowed_rows():
         b.set_data(device_ptr=X[1:].data_ptr())
     assert err.value.code == km.KMEANS_ERROR_INVALID_ARGUMENT
     b.close()
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+@pytest.mark.parametrize("sig_lo,sig_hi,init", [(0.2, 0.3, "means"), (0.3, 2.0, "means"),
+                                                (0.3, 2.0, "sample")])
+def test_mfma_ordered_assign_tile_cut(orc, prec, sig_lo, sig_hi, init):
+    # tight, well-separated clusters in label order: workgroups stop after a
+    # few centre tiles once the triangle bound clears every remaining tile;
+    # with a spread of widths some workgroups stop and others do not.  Labels
+    # and centres must stay those of the oracle on every iteration.
+    print(f"tile cut {prec} sigma [{sig_lo}, {sig_hi}] init {init}", flush=True)
+    n, d, k = 400_000, 64, 256
+    X, _, means, _ = orc.mixture(71, n, d, k, -5, 5, sig_lo, sig_hi)
+    C0 = means.copy() if init == "means" else X[orc.sample_init(71, n, k)].copy()
+    e = km.KMeans(n, d, k, precision=prec)
+    e.set_data(X)
+    e.set_centers(C0)
+    e.run(1)  # the first assign has no label order yet
+    s1 = e.stats()
+    e.run(4)
+    lab, C, st = e.labels(), e.centers(), e.stats()
+    e.close()
+    ref = orc.lloyd(X, C0, 5)
+    assert np.array_equal(lab, ref["labels"])
+    assert np.array_equal(C, ref["centers"])
+    assert st["compactness"] == ref["comp_hist"][-1]
+    t, tt = st["tiles"] - s1["tiles"], st["tiles_total"] - s1["tiles_total"]
+    print(f"ordered iterations: tiles computed {t} of {tt}")
+    assert s1["tiles"] == s1["tiles_total"] > 0 and 0 < t <= tt
+    if sig_hi <= 0.3:  # most workgroups hold one tight cluster: the cut must happen
+        assert t < 0.75 * tt
