@@ -191,6 +191,7 @@ typedef struct kmeansStats {
                                  assign computed; the rest were cut by the
                                  triangle bound                               */
   long long tiles_total;      /* ... out of this many                         */
+  long long bounded;          /* point assignments kept by the drift bounds   */
 } kmeansStats;
 
 typedef struct kmeansEngine *kmeansHandle;

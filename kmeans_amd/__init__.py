@@ -66,7 +66,8 @@ class Stats(ctypes.Structure):
                 ("candidate_pairs", ctypes.c_longlong), ("proposal_hits", ctypes.c_longlong),
                 ("ms_train", ctypes.c_double), ("train_loss_first", ctypes.c_double),
                 ("train_loss_last", ctypes.c_double), ("deep", ctypes.c_longlong),
-                ("tiles", ctypes.c_longlong), ("tiles_total", ctypes.c_longlong)]
+                ("tiles", ctypes.c_longlong), ("tiles_total", ctypes.c_longlong),
+                ("bounded", ctypes.c_longlong)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -109,6 +109,10 @@ hipError_t launch_update_accumulate_2s(hipStream_t s, const UpdateOverlap &ov, l
                                        int k, const float *X, const int *labels, double *S,
                                        long long *cnt);
 int sub_block(int k);
+// out[0..L] = exclusive scan of in[0..L) (out[L] the total); tsum holds
+// scan_tiles(L) + 1 ints
+long scan_tiles(long L);
+hipError_t launch_scan_excl(hipStream_t s, const int *in, long L, int *tsum, int *out);
 
 // Launchers with a float and a double form (element type of the points and
 // centres; the sums are FP64 in both).

@@ -466,6 +466,7 @@ kmeansErrorStatus Engine::run_iterations(long r0, long rows, bool minibatch) {
         // the assign phase, the full search of uncertified points the
         // re-resolve
         if ((err = learned_prepare(this)) != KMEANS_SUCCESS) return err;
+        if (mfma) mfma_bounds_invalidate(this);
         record(slot, 0);
         KM_HIP_CHECK(learned_assign(this), KMEANS_ERROR_CLOSEST_CENTERS);
         record(slot, 1);

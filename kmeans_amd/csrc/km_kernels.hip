@@ -1077,6 +1077,11 @@ static hipError_t scan_excl(hipStream_t s, const int *in, long L, int *tsum, int
   return hipGetLastError();
 }
 
+long scan_tiles(long L) { return km_cdiv(L + 1, SCAN_TILE); }
+hipError_t launch_scan_excl(hipStream_t s, const int *in, long L, int *tsum, int *out) {
+  return scan_excl(s, in, L, tsum, out);
+}
+
 hipError_t launch_point_sq(hipStream_t s, long m, int d, const float *X, double *q) {
   if (m <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_point_sq, dim3(km_cdiv(m, 256)), dim3(256), 0, s, m, d, X, q);

@@ -95,6 +95,15 @@ struct MfmaState {
   int *home = nullptr;
   long home_cap = 0;
   bool ttab_ok = false;
+  // drift bounds of the resident points: xb[i] = (u, l) with u >= |x_i -
+  // c_label| and l <= |x_i - c_j| for every other centre j, valid for the
+  // centres Cref; drift[j] >= |c_j - Cref_j| and drift[k] their maximum.
+  // ulist: the points the bounds leave open, bcnt/bofs/btsum their per-block
+  // counts and offsets.  bnd_valid: xb and Cref describe the last assign.
+  float2 *xb = nullptr;
+  float *Cref = nullptr, *drift = nullptr;
+  int *ulist = nullptr, *bcnt = nullptr, *bofs = nullptr, *btsum = nullptr;
+  bool bnd_valid = false;
   int nflag_slots = 0;
   bool x_ready = false;
   bool c_ready = false;
@@ -225,13 +234,137 @@ __global__ __launch_bounds__(256) void k_tile_order(int K, int D, int nKT,
   }
 }
 
+// ---------------------------------------------------------------- drift bounds
+// A point keeps its label without being scored when its bounds, moved by the
+// centres' drift since they were made, still separate the label from every
+// other centre (Hamerly, "Making k-means even faster", SDM 2010, with the
+// separation required to exceed the FP32 chains' error): with u' = u +
+// drift[a] >= |x - c_a| and l' = l - max drift <= |x - c_j| (triangle
+// inequality), the exact scores differ by 0.5 (|x - c_j|^2 - |x - c_a|^2) >=
+// 0.5 (l'^2 - u'^2), and l'^2 - u'^2 > 4 E32 (E32 the FP32 chain bound of
+// the single-problem certificate) makes the FP32 chain of a larger than
+// every other, strictly, so the label is the plain path's.
+
+// drift[j] = upper bound of |C_j - Cref_j| (the FP32 differences are within
+// an ulp, the sum of squares within gamma_D in any order), one wave per
+// centre; drift[K] (as bits) += the largest by atomic maximum over
+// non-negative floats, from 0 (k_ref_copy clears it for the next pass)
+__global__ __launch_bounds__(256) void k_drift(int K, int D, const float *__restrict__ C,
+                                               const float *__restrict__ Cref,
+                                               float *__restrict__ drift,
+                                               const int *__restrict__ stop) {
+  if (stop && *stop) return;
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (j >= K) return;
+  float q = 0.0f;
+  for (int d = lane; d < D; d += 64) {
+    const float t = C[(long)j * D + d] - Cref[(long)j * D + d];
+    q = __builtin_fmaf(t, t, q);
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) q += __shfl_xor(q, o);
+  if (lane == 0) {
+    const float v = q > 0.0f ? sqrtf(q * (1.0f + (float)D * 1.2e-7f + 4e-7f)) * (1.0f + 1e-6f) : 0.0f;
+    drift[j] = v;
+    atomicMax(reinterpret_cast<unsigned *>(drift + K), __float_as_uint(v));
+  }
+}
+
+// The test for the point at position q of `order` (or point q): whether its
+// moved bounds keep its label, and the moved bounds.  bnd = the centre
+// bounds [cmax, ncmax, ...] of the current centres.
+__device__ inline bool bound_keep(long q, const int *__restrict__ order,
+                                  const int *__restrict__ labels, const float2 *__restrict__ xb,
+                                  const float *__restrict__ drift, int K, const float *bnd,
+                                  const float2 *__restrict__ xnb, float Df, long &p, float2 &nb) {
+  p = order ? (long)order[q] : q;
+  const int a = labels[p];
+  const float2 b = xb[p];
+  const float u = (b.x + drift[a]) * (1.0f + 1e-6f);
+  const float l = fmaxf((b.y - drift[K]) * (1.0f - 1e-6f), 0.0f);
+  nb = make_float2(u, l);
+  const float e32 = Df * 5.9604645e-8f * 1.0001f * (0.5f * bnd[1] + xnb[p].x * bnd[0]);
+  return l > u && (l - u) * (l + u) * (1.0f - 1e-6f) > 4.0f * e32 * 1.0001f + 1e-30f;
+}
+
+// Pass 1: per block of 256 positions, the number of points left open.
+__global__ __launch_bounds__(256) void k_bound_count(long M, const int *__restrict__ order,
+                                                     const int *__restrict__ labels,
+                                                     const float2 *__restrict__ xb,
+                                                     const float *__restrict__ drift, int K,
+                                                     const float *__restrict__ bnd,
+                                                     const float2 *__restrict__ xnb, int D,
+                                                     int *__restrict__ bcnt,
+                                                     const int *__restrict__ stop) {
+  if (stop && *stop) return;
+  __shared__ int wc[4];
+  const long q = (long)blockIdx.x * 256 + threadIdx.x;
+  bool open = false;
+  if (q < M) {
+    long p;
+    float2 nb;
+    open = !bound_keep(q, order, labels, xb, drift, K, bnd, xnb, (float)D, p, nb);
+  }
+  const uint64_t mk = __ballot(open);
+  if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = __popcll(mk);
+  __syncthreads();
+  if (threadIdx.x == 0) bcnt[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
+}
+
+// Pass 2 (same test, same inputs): the open points in order into ulist at
+// the block's offset, the moved bounds of the kept points into xb.
+__global__ __launch_bounds__(256) void k_bound_compact(long M, const int *__restrict__ order,
+                                                       const int *__restrict__ labels,
+                                                       float2 *__restrict__ xb,
+                                                       const float *__restrict__ drift, int K,
+                                                       const float *__restrict__ bnd,
+                                                       const float2 *__restrict__ xnb, int D,
+                                                       const int *__restrict__ bofs,
+                                                       int *__restrict__ ulist,
+                                                       const int *__restrict__ stop) {
+  if (stop && *stop) return;
+  __shared__ int wc[4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const long q = (long)blockIdx.x * 256 + threadIdx.x;
+  bool open = false;
+  long p = 0;
+  float2 nb = make_float2(0.0f, 0.0f);
+  if (q < M) open = !bound_keep(q, order, labels, xb, drift, K, bnd, xnb, (float)D, p, nb);
+  const uint64_t mk = __ballot(open);
+  if (lane == 0) wc[w] = __popcll(mk);
+  __syncthreads();
+  int base = bofs[blockIdx.x];
+  for (int u = 0; u < w; ++u) base += wc[u];
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  if (q < M) {
+    if (open) ulist[base + __popcll(mk & below)] = (int)p;
+    else xb[p] = nb;
+  }
+}
+
+// Cref = C once the assign has made every point's bounds valid for C, and
+// the drift maximum drift[K] back to 0 for the next k_drift
+__global__ void k_ref_copy(long n, const float *__restrict__ C, float *__restrict__ Cref,
+                           float *__restrict__ dmax, const int *__restrict__ stop) {
+  if (stop && *stop) return;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) Cref[i] = C[i];
+  if (i == 0) *dmax = 0.0f;
+}
+
 // home[w] = the label (from the previous assign) of the first point of
 // workgroup w's slice of the order, clamped into [0, K)
-__global__ void k_wg_home(long nwg, long M, int K, const int *__restrict__ order,
-                          const int *__restrict__ labels, int *__restrict__ home) {
+__global__ void k_wg_home(long nwg, long M, const int *__restrict__ Mdev, int K,
+                          const int *__restrict__ order, const int *__restrict__ labels,
+                          int *__restrict__ home) {
   const long w = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= nwg) return;
-  const long q = w * MF_PTS < M ? w * MF_PTS : M - 1;
+  if (Mdev) M = min(M, (long)*Mdev);  // the open points only (drift bounds)
+  if (w * MF_PTS >= M) {  // a workgroup with no points (it exits at once)
+    home[w] = 0;
+    return;
+  }
+  const long q = w * MF_PTS;
   const int l = labels[order[q]];
   home[w] = l < 0 ? 0 : (l >= K ? K - 1 : l);
 }
@@ -300,7 +433,7 @@ __host__ __device__ __forceinline__ long mf_point(const int *order, long q) {
 // three (the test is exact, so the state is the one a full insertion leaves).
 // The certificates do not depend on the order in which centres are visited,
 // so the labels are those of the unordered kernel.
-template <int KS, bool FP16, bool ORD>
+template <int KS, bool FP16, bool ORD, bool LIST>
 __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
     long M, const uint16_t *__restrict__ Xh, const float *__restrict__ X, int D, int K, int Kp,
     const uint16_t *__restrict__ Ch, const float *__restrict__ bias, const float *__restrict__ C,
@@ -308,7 +441,21 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
     float inv_s2, float eta, float big, int *__restrict__ labels,
     int *__restrict__ flags, int *__restrict__ nflag, int *__restrict__ ncand,
     const int *__restrict__ stop, const int *__restrict__ order, const int *__restrict__ ttab,
-    const float *__restrict__ tdist, const int *__restrict__ home) {
+    const float *__restrict__ tdist, const int *__restrict__ home, const int *__restrict__ Mdev,
+    float2 *__restrict__ xbo) {
+  // stop (nullable): set by a device-side convergence test; the launch then
+  // assigns nothing and lists nothing
+  if (stop && *stop) return;
+  // Mdev (nullable): the number of positions of `order` to assign (the points
+  // the drift bounds left open; the others keep their labels), counted into
+  // slot 3 of the first counter line; xbo (nullable): fresh drift bounds of
+  // the assigned points
+  if (Mdev) {
+    const long Mh = M;
+    M = min(M, (long)*Mdev);
+    if (blockIdx.x == 0 && threadIdx.x == 0) ncand[3] = (int)(Mh - M);
+    if ((long)blockIdx.x * MF_PTS >= M) return;  // uniform over the workgroup
+  }
   constexpr int Dp = KS * 16;
   constexpr int CPR = Dp / 8;               // 16-byte chunks per centre row
   constexpr int SROWS = MF_STAGE * MF_TILE; // centre rows per LDS stage
@@ -340,7 +487,7 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
 #pragma unroll
   for (int g = 0; g < 2; ++g) {
     const long sp = p0 + g * 32 + col;
-    const long p = sp < M ? mf_point<ORD>(order, sp) : 0;
+    const long p = sp < M ? mf_point<ORD || LIST>(order, sp) : 0;
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
       if (sp < M)
@@ -357,10 +504,6 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
       dsv = tdist[(long)hc * nKT + tid];
     }
   }
-  // stop (nullable): set by a device-side convergence test; the launch then
-  // assigns nothing and lists nothing (checked after the loads above are in
-  // flight, which are in bounds either way)
-  if (stop && *stop) return;
   if constexpr (ORD) {  // the workgroup's tile order and distance bounds
     for (int t = tid; t < nKT; t += 64 * MF_WAVES) {
       Ts[t] = t == tid ? tsv : ttab[(long)hc * nKT + t];
@@ -632,7 +775,7 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
     }
     const long sp = p0 + g * 32 + col;
     const bool inr = sp < M;
-    const long p = inr ? mf_point<ORD>(order, sp) : 0;
+    const long p = inr ? mf_point<ORD || LIST>(order, sp) : 0;
     if constexpr (ORD) {  // the skipped tiles' scores are at most Ub (both halves)
       const float Ub = fmaxf(T.U, __shfl_xor(T.U, 32));
       m2 = fmaxf(m2, Ub);
@@ -694,16 +837,42 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
     }
     const float sy = __shfl_xor(sx, 32);  // the other half's chain
     bool fall2 = false;
+    float sb = sx, so = sy;  // the chains of the label and of the other candidate
     if (live && need) {
       int lab = k1;
-      float sb = sx;
-      if (k2 >= 0 && k2 < K && (sy > sx || (sy == sx && k2 < k1))) { lab = k2; sb = sy; }
+      if (k2 >= 0 && k2 < K && (sy > sx || (sy == sx && k2 < k1))) { lab = k2; sb = sy; so = sx; }
       if (sb > thr && sb > -INFINITY) labels[p] = lab;
       else fall2 = true;
     }
     const bool two = live && need;
     const bool fall = (live && !quick && !need) || fall2;
     if (live && quick) labels[p] = k1;
+    if (xbo && live) {
+      // fresh drift bounds: lo <= the exact score of the label (the
+      // approximate one minus E and the packing slack, or its FP32 chain
+      // minus E >= the chain's own bound), hi >= the exact score of every
+      // other centre; distances from |x - c|^2 = |x|^2 - 2 score with the
+      // upper and lower bounds of |x|^2 that xn gives.  A point sent on to
+      // the later passes gets bounds that never hold.
+      float lo = -INFINITY, hi = INFINITY;
+      if (quick) {
+        lo = a1 - E - slack;
+        hi = a2 + E + slack;
+      } else if (need && !fall2) {
+        lo = sb - E;
+        hi = fmaxf(a3 + E + slack, so + E);
+      }
+      float u = INFINITY, l = 0.0f;
+      if (lo > -INFINITY) {
+        const float x2u = nb.x * nb.x * (1.0f + 2.4e-7f);
+        const float x2l = nb.x * nb.x * (1.0f - (float)D * 1.8e-7f - 4e-6f);
+        const float uu = (x2u - 2.0f * lo) + 3.6e-7f * (x2u + 2.0f * fabsf(lo));
+        u = sqrtf(fmaxf(uu, 0.0f)) * (1.0f + 1e-6f) + 1e-30f;
+        const float ll = (x2l - 2.0f * hi) - 3.6e-7f * (x2l + 2.0f * fabsf(hi));
+        l = ll > 0.0f ? sqrtf(ll) * (1.0f - 1e-6f) : 0.0f;
+      }
+      xbo[p] = make_float2(u, l);
+    }
     // wave-aggregated counts and appends (one atomic per wave)
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     {
@@ -1016,19 +1185,30 @@ void mfma_free(Engine *e) {
   hipFree(s->kbest);
   hipFree(s->ttab);
   hipFree(s->tdist);
+  void *bb[] = {s->xb, s->Cref, s->drift, s->ulist, s->bcnt, s->bofs, s->btsum};
+  for (void *b : bb) hipFree(b);
   hipFree(s->home);
   delete s;
   e->mfma = nullptr;
 }
 
 void mfma_invalidate(Engine *e) {
-  if (st(e)) st(e)->x_ready = false;
+  if (st(e)) {
+    st(e)->x_ready = false;
+    st(e)->bnd_valid = false;
+  }
 }
 
+void mfma_bounds_invalidate(Engine *e) {
+  if (st(e)) st(e)->bnd_valid = false;
+}
+
+// centres set from outside the iteration: the drift bounds no longer apply
 void mfma_centers_changed(Engine *e, bool new_range) {
   MfmaState *s = st(e);
   if (!s) return;
   s->c_ready = false;
+  s->bnd_valid = false;
   if (new_range && s->fp16) s->x_ready = false;  // the FP16 scale may change
 }
 
@@ -1223,39 +1403,75 @@ hipError_t mfma_centers(Engine *e) {
   return prepare_centres(e);
 }
 
-// order/ttab/home (all non-null, or all null): the label-ordered variant
+// order/ttab/home all non-null: the label-ordered variant; order alone: the
+// points order[0..M) (with Mdev, the first *Mdev of them) in that order
+template <int KS, bool FP16, bool ORD, bool LIST>
+static void launch_mfma_mode(hipStream_t strm, dim3 g, size_t lds, long M, const uint16_t *Xh,
+                             const float *X, int D, int K, int Kp, const uint16_t *Ch,
+                             const float *bias, const float *C, const float *nc, const float *bnd,
+                             const float2 *xnb, float inv_s2, float eta, int *labels, int *flags,
+                             int *nflag, int *ncand, const int *stop, const int *order,
+                             const int *ttab, const float *tdist, const int *home, const int *Mdev,
+                             float2 *xbo) {
+  static int attr_lds = -1;
+  if ((int)lds > attr_lds) {
+    hipFuncSetAttribute((const void *)k_assign_mfma<KS, FP16, ORD, LIST>,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr_lds = (int)lds;
+  }
+  hipLaunchKernelGGL((k_assign_mfma<KS, FP16, ORD, LIST>), g, dim3(64 * MF_WAVES), lds, strm, M,
+                     Xh, X, D, K, Kp, Ch, bias, C, nc, bnd, xnb, inv_s2, eta, FLT_MAX, labels,
+                     flags, nflag, ncand, stop, order, ttab, tdist, home, Mdev, xbo);
+}
+
 template <int KS>
 static void launch_mfma_ks(hipStream_t strm, bool fp16, dim3 g, long M, const uint16_t *Xh,
                            const float *X, int D, int K, int Kp, const uint16_t *Ch,
                            const float *bias, const float *C, const float *nc, const float *bnd,
                            const float2 *xnb, float inv_s2, float eta, int *labels,
                            int *flags, int *nflag, int *ncand, const int *stop, const int *order,
-                           const int *ttab, const float *tdist, const int *home) {
-  const bool ord = order != nullptr;
+                           const int *ttab, const float *tdist, const int *home, const int *Mdev,
+                           float2 *xbo) {
+  const bool ord = order != nullptr && home != nullptr;
+  const bool list = order != nullptr && home == nullptr;
   const size_t lds = 2 * (size_t)MF_STAGE * MF_TILE * (KS * 2) * 16 + 2 * MF_STAGE * MF_TILE * 4 +
                      (ord ? 2 * sizeof(int) * (size_t)(Kp / MF_TILE) : 0);
-  const void *fn = fp16 ? (ord ? (const void *)k_assign_mfma<KS, true, true>
-                               : (const void *)k_assign_mfma<KS, true, false>)
-                        : (ord ? (const void *)k_assign_mfma<KS, false, true>
-                               : (const void *)k_assign_mfma<KS, false, false>);
-  static int attr_lds[4] = {-1, -1, -1, -1};
-  const int v = (fp16 ? 2 : 0) + (ord ? 1 : 0);
-  if ((int)lds > attr_lds[v]) {
-    hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    attr_lds[v] = (int)lds;
-  }
 #define KM_ARGS                                                                                 \
-  M, Xh, X, D, K, Kp, Ch, bias, C, nc, bnd, xnb, inv_s2, eta, FLT_MAX, labels, flags, nflag,    \
-      ncand, stop, order, ttab, tdist, home
-  if (fp16 && ord)
-    hipLaunchKernelGGL((k_assign_mfma<KS, true, true>), g, dim3(64 * MF_WAVES), lds, strm, KM_ARGS);
-  else if (fp16)
-    hipLaunchKernelGGL((k_assign_mfma<KS, true, false>), g, dim3(64 * MF_WAVES), lds, strm, KM_ARGS);
-  else if (ord)
-    hipLaunchKernelGGL((k_assign_mfma<KS, false, true>), g, dim3(64 * MF_WAVES), lds, strm, KM_ARGS);
-  else
-    hipLaunchKernelGGL((k_assign_mfma<KS, false, false>), g, dim3(64 * MF_WAVES), lds, strm, KM_ARGS);
+  strm, g, lds, M, Xh, X, D, K, Kp, Ch, bias, C, nc, bnd, xnb, inv_s2, eta, labels, flags, nflag, \
+      ncand, stop, order, ttab, tdist, home, Mdev, xbo
+  if (fp16) {
+    if (ord) launch_mfma_mode<KS, true, true, false>(KM_ARGS);
+    else if (list) launch_mfma_mode<KS, true, false, true>(KM_ARGS);
+    else launch_mfma_mode<KS, true, false, false>(KM_ARGS);
+  } else {
+    if (ord) launch_mfma_mode<KS, false, true, false>(KM_ARGS);
+    else if (list) launch_mfma_mode<KS, false, false, true>(KM_ARGS);
+    else launch_mfma_mode<KS, false, false, false>(KM_ARGS);
+  }
 #undef KM_ARGS
+}
+
+// Drift-bound buffers (once per engine).
+static hipError_t ensure_bounds(Engine *e) {
+  MfmaState *s = st(e);
+  if (s->xb) return hipSuccess;
+  hipError_t r;
+  const long nblk = km_cdiv(e->m, 256);
+  if ((r = hipMalloc(&s->xb, sizeof(float2) * (size_t)e->m)) != hipSuccess) return r;
+  if ((r = hipMalloc(&s->Cref, sizeof(float) * (size_t)e->k * e->n)) != hipSuccess) return r;
+  if ((r = hipMalloc(&s->drift, sizeof(float) * (size_t)(e->k + 1))) != hipSuccess) return r;
+  if ((r = hipMemsetAsync(s->drift, 0, sizeof(float) * (size_t)(e->k + 1), e->stream)) != hipSuccess)
+    return r;
+  if ((r = hipMalloc(&s->ulist, sizeof(int) * (size_t)e->m)) != hipSuccess) return r;
+  if ((r = hipMalloc(&s->bcnt, sizeof(int) * (size_t)nblk)) != hipSuccess) return r;
+  if ((r = hipMalloc(&s->bofs, sizeof(int) * (size_t)(nblk + 1))) != hipSuccess) return r;
+  if ((r = hipMalloc(&s->btsum, sizeof(int) * (size_t)(scan_tiles(nblk) + 1))) != hipSuccess)
+    return r;
+  // an open list of zero points until the first compaction (a launch whose
+  // compaction a convergence stop skipped reads these)
+  if ((r = hipMemsetAsync(s->ulist, 0, sizeof(int) * (size_t)e->m, e->stream)) != hipSuccess)
+    return r;
+  return hipMemsetAsync(s->bofs, 0, sizeof(int) * (size_t)(nblk + 1), e->stream);
 }
 
 hipError_t mfma_assign(Engine *e, long r0, long rows) {
@@ -1269,11 +1485,30 @@ hipError_t mfma_assign(Engine *e, long r0, long rows) {
   }
   if (!s->c_ready)
     if ((r = prepare_centres(e)) != hipSuccess) return r;
-  // all resident points, with the previous update's order and a tile table:
-  // the label-ordered variant
-  const bool ord = r0 == 0 && rows == e->m && e->order_ok && s->ttab_ok && e->k <= MF_ORD_MAX_K;
-  const int *home = nullptr;
-  if (ord) {
+  const bool all = r0 == 0 && rows == e->m;
+  if (all && (r = ensure_bounds(e)) != hipSuccess) return r;
+  const int *order = nullptr, *home = nullptr, *Mdev = nullptr;
+  if (all && s->bnd_valid) {
+    // the drift bounds of the last assign over all points: only the points
+    // they leave open are assigned, in point order (the filter reads every
+    // point's label and bounds once, coalesced)
+    const long nblk = km_cdiv(rows, 256);
+    hipLaunchKernelGGL(k_drift, dim3(km_cdiv(e->k, 4)), dim3(256), 0, e->stream, e->k, e->n,
+                       e->dC, s->Cref, s->drift, e->assign_stop);
+    hipLaunchKernelGGL(k_bound_count, dim3(nblk), dim3(256), 0, e->stream, rows,
+                       (const int *)nullptr, e->dlabels, s->xb, s->drift, e->k, s->bnd, s->xnb,
+                       e->n, s->bcnt, e->assign_stop);
+    if ((r = launch_scan_excl(e->stream, s->bcnt, nblk, s->btsum, s->bofs)) != hipSuccess)
+      return r;
+    hipLaunchKernelGGL(k_bound_compact, dim3(nblk), dim3(256), 0, e->stream, rows,
+                       (const int *)nullptr, e->dlabels, s->xb, s->drift, e->k, s->bnd, s->xnb,
+                       e->n, s->bofs, s->ulist, e->assign_stop);
+    if ((r = hipGetLastError()) != hipSuccess) return r;
+    order = s->ulist;
+    Mdev = s->bofs + nblk;
+  } else if (all && e->order_ok && s->ttab_ok && e->k <= MF_ORD_MAX_K) {
+    // all resident points, with the previous update's order and a tile
+    // table: the label-ordered variant
     const long nwg = km_cdiv(rows, MF_PTS);
     if (nwg > s->home_cap) {
       hipFree(s->home);
@@ -1283,18 +1518,35 @@ hipError_t mfma_assign(Engine *e, long r0, long rows) {
       s->home_cap = nwg;
     }
     hipLaunchKernelGGL(k_wg_home, dim3(km_cdiv(nwg, 256)), dim3(256), 0, e->stream, nwg, rows,
-                       e->k, e->dorder, e->dlabels, s->home);
+                       (const int *)nullptr, e->k, e->dorder, e->dlabels, s->home);
+    order = e->dorder;
     home = s->home;
   }
-  return mfma_assign_view(e, e->dX + r0 * e->n, s->Xh + r0 * s->Dp, s->xnb + r0,
-                          e->dlabels + r0, rows, ord ? e->dorder : nullptr, home);
+  if ((r = mfma_assign_view(e, e->dX + r0 * e->n, s->Xh + r0 * s->Dp, s->xnb + r0,
+                            e->dlabels + r0, rows, order, home, Mdev, all ? s->xb : nullptr)) !=
+      hipSuccess)
+    return r;
+  // every point's bounds now hold for the current centres (fresh ones for the
+  // assigned points, moved ones for the others); a partial assign leaves them
+  // unusable
+  if (all) {
+    const long nc = (long)e->k * e->n;
+    hipLaunchKernelGGL(k_ref_copy, dim3(km_cdiv(nc, 256)), dim3(256), 0, e->stream, nc, e->dC,
+                       s->Cref, s->drift + e->k, e->assign_stop);
+    if ((r = hipGetLastError()) != hipSuccess) return r;
+  }
+  s->bnd_valid = all;
+  return hipSuccess;
 }
 
 // Assignment of `rows` points: FP32 rows X, their reduced-precision copy Xh
 // and norm bounds xn, labels written to lab (all indexed from 0).  order and
-// home (nullable, both or neither): the label-ordered variant over all rows.
+// home (nullable, both or neither): the label-ordered variant over all rows;
+// Mdev (nullable, with order): the number of positions of order to assign;
+// xbo (nullable): fresh drift bounds of the assigned points.
 hipError_t mfma_assign_view(Engine *e, const float *X, const uint16_t *Xh, const float2 *xn,
-                            int *lab, long rows, const int *order, const int *home) {
+                            int *lab, long rows, const int *order, const int *home,
+                            const int *Mdev, float2 *xbo) {
   hipError_t r;
   MfmaState *s = st(e);
   if (!s->c_ready)
@@ -1309,12 +1561,12 @@ hipError_t mfma_assign_view(Engine *e, const float *X, const uint16_t *Xh, const
   const float inv_s2 = 1.0f / (s->scale * s->scale);
   dim3 g(km_cdiv(rows, MF_PTS));
   switch (s->KS) {
-    case 1: launch_mfma_ks<1>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, order ? s->ttab : nullptr, order ? s->tdist : nullptr, home); break;
-    case 2: launch_mfma_ks<2>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, order ? s->ttab : nullptr, order ? s->tdist : nullptr, home); break;
-    case 4: launch_mfma_ks<4>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, order ? s->ttab : nullptr, order ? s->tdist : nullptr, home); break;
-    case 6: launch_mfma_ks<6>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, order ? s->ttab : nullptr, order ? s->tdist : nullptr, home); break;
-    case 8: launch_mfma_ks<8>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, order ? s->ttab : nullptr, order ? s->tdist : nullptr, home); break;
-    default: launch_mfma_ks<16>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, order ? s->ttab : nullptr, order ? s->tdist : nullptr, home); break;
+    case 1: launch_mfma_ks<1>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, home ? s->ttab : nullptr, home ? s->tdist : nullptr, home, Mdev, xbo); break;
+    case 2: launch_mfma_ks<2>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, home ? s->ttab : nullptr, home ? s->tdist : nullptr, home, Mdev, xbo); break;
+    case 4: launch_mfma_ks<4>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, home ? s->ttab : nullptr, home ? s->tdist : nullptr, home, Mdev, xbo); break;
+    case 6: launch_mfma_ks<6>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, home ? s->ttab : nullptr, home ? s->tdist : nullptr, home, Mdev, xbo); break;
+    case 8: launch_mfma_ks<8>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, home ? s->ttab : nullptr, home ? s->tdist : nullptr, home, Mdev, xbo); break;
+    default: launch_mfma_ks<16>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, home ? s->ttab : nullptr, home ? s->tdist : nullptr, home, Mdev, xbo); break;
   }
   if ((r = hipGetLastError()) != hipSuccess) return r;
   return hipGetLastError();
@@ -1435,6 +1687,7 @@ void mfma_collect(Engine *e, int nslots) {
     for (int c = 0; c < NC_SPREAD; ++c) {
       e->stats.rescored += w[(size_t)i * NC_SLOT + c * NC_STRIDE];
       e->stats.tiles += w[(size_t)i * NC_SLOT + c * NC_STRIDE + 1];
+      e->stats.bounded += w[(size_t)i * NC_SLOT + c * NC_STRIDE + 3];
       e->stats.tiles_total += w[(size_t)i * NC_SLOT + c * NC_STRIDE + 2];
     }
 }

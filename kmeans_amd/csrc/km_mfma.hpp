@@ -6,6 +6,7 @@ namespace km {
 struct Engine;
 void mfma_free(Engine *e);
 void mfma_invalidate(Engine *e);
+void mfma_bounds_invalidate(Engine *e);  // labels changed outside mfma_assign
 hipError_t mfma_assign(Engine *e, long r0, long rows);
 hipError_t mfma_reresolve(Engine *e, long r0, long rows);
 hipError_t mfma_centers(Engine *e);
@@ -16,10 +17,12 @@ hipError_t mfma_prepare(Engine *e);
 hipError_t mfma_prepare_rows(Engine *e, const float *X, long rows);
 // xn: per point upper bounds of |x| and |x - xh/s| (FP32 units)
 // order, home (nullable, both or neither): the label-ordered variant over all
-// resident points (order = the previous update's sort by label)
+// resident points (order = the previous update's sort by label, or the points
+// the drift bounds left open, Mdev of them); xbo: fresh drift bounds
 hipError_t mfma_assign_view(Engine *e, const float *X, const uint16_t *Xh, const float2 *xn,
                             int *lab, long rows, const int *order = nullptr,
-                            const int *home = nullptr);
+                            const int *home = nullptr, const int *Mdev = nullptr,
+                            float2 *xbo = nullptr);
 hipError_t mfma_reresolve_view(Engine *e, const float *X, const uint16_t *Xh, const float2 *xn,
                                int *lab, long rows);
 const uint16_t *mfma_tile_xh(Engine *e);

@@ -253,7 +253,11 @@ def test_mfma_ordered_assign_tile_cut(orc, prec, sig_lo, sig_hi, init):
     e.set_centers(C0)
     e.run(1)  # the first assign has no label order yet
     s1 = e.stats()
-    e.run(4)
+    for _ in range(4):
+        # setting the (same) centres again drops the drift bounds, so every
+        # point is scored in the label order and the cut decides the tiles
+        e.set_centers(e.centers())
+        e.run(1)
     lab, C, st = e.labels(), e.centers(), e.stats()
     e.close()
     ref = orc.lloyd(X, C0, 5)
@@ -265,3 +269,57 @@ def test_mfma_ordered_assign_tile_cut(orc, prec, sig_lo, sig_hi, init):
     assert s1["tiles"] == s1["tiles_total"] > 0 and 0 < t <= tt
     if sig_hi <= 0.3:  # most workgroups hold one tight cluster: the cut must happen
         assert t < 0.75 * tt
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_mfma_drift_bounds_keep_labels(orc, prec):
+    # late iterations: centres barely move, so most points keep their labels
+    # on the drift bounds alone; every run length must match the oracle
+    print(f"drift bounds {prec}", flush=True)
+    n, d, k = 300_000, 48, 200
+    X, _, means, _ = orc.mixture(91, n, d, k, -5, 5, 0.4, 1.2)
+    C0 = X[orc.sample_init(91, n, k)].copy()
+    e = km.KMeans(n, d, k, precision=prec)
+    e.set_data(X)
+    e.set_centers(C0)
+    done = 0
+    for steps in (1, 3, 4, 4):
+        e.run(steps)
+        done += steps
+        ref = orc.lloyd(X, C0, done)
+        assert np.array_equal(e.labels(), ref["labels"]), done
+        assert np.array_equal(e.centers(), ref["centers"]), done
+    st = e.stats()
+    print(f"kept by bounds {st['bounded']} of {done * n}")
+    assert st["bounded"] > 2 * n
+    # centres set from outside: the bounds must not be used for them
+    C1 = means.copy()
+    e.set_centers(C1)
+    e.run(2)
+    ref = orc.lloyd(X, C1, 2)
+    assert np.array_equal(e.labels(), ref["labels"])
+    assert np.array_equal(e.centers(), ref["centers"])
+    e.close()
+
+
+def test_mfma_drift_bounds_across_a_converged_run(orc):
+    # a device-side convergence stop skips the later assigns and their bound
+    # updates; a run after it must still match the oracle's continuation
+    n, d, k = 200_000, 32, 100
+    X, _, _, _ = orc.mixture(93, n, d, k, -5, 5, 0.5, 1.0)
+    C0 = X[orc.sample_init(93, n, k)].copy()
+    e = km.KMeans(n, d, k, precision="bf16")
+    e.set_data(X)
+    e.set_centers(C0)
+    e.run(40, criterion=1e-4)
+    it = e.stats()["iterations"]
+    ref = orc.lloyd(X, C0, 40, criterion=1e-4)
+    assert it == ref["iters"]
+    assert np.array_equal(e.labels(), ref["labels"])
+    C_conv = e.centers()
+    assert np.array_equal(C_conv, ref["centers"])
+    e.run(3)
+    ref2 = orc.lloyd(X, C_conv, 3)
+    assert np.array_equal(e.labels(), ref2["labels"])
+    assert np.array_equal(e.centers(), ref2["centers"])
+    e.close()
