@@ -44,6 +44,10 @@ def main():
     ap.add_argument("--top-groups", type=int, default=4)
     ap.add_argument("--train-iters", type=int, default=3)
     ap.add_argument("--hidden", type=int, default=256)
+    ap.add_argument("--tile-rows", type=int, default=0,
+                    help="stream the points from pinned host memory through HBM in tiles of this "
+                         "many rows (a multiple of 2^23), the reference's mini-batch tiling; the "
+                         "timed iterations then include every host-to-device copy")
     args = ap.parse_args()
     if args.proposer is None:
         args.proposer = "mlp" if args.workload == "c5" else "none"
@@ -70,6 +74,11 @@ def main():
     if args.precision == "fp64":
         print(json.dumps(bench_fp64(args, X, C0, spec, km, np)))
         return
+    if args.tile_rows > 0:
+        print(json.dumps(bench_streamed(args, X, C0, spec, km, np, torch)))
+        return
+
+    prep_ms = {}
 
     def run(precision, steps, warmup, full_gemm=False):
         e = km.KMeans(n, d, k, precision=precision, timing=True, full_gemm=full_gemm)
@@ -79,13 +88,18 @@ def main():
             e.run(warmup)
             e.synchronize()
             e.set_centers(C0)
-        # fresh stats for the timed run
+        # fresh stats for the timed run; its per-data-set preparation (the
+        # reduced-precision copy of the points and their norm bounds) is timed
+        # on its own and is not part of the iterations
         e2 = km.KMeans(n, d, k, precision=precision, timing=True, full_gemm=full_gemm)
         e.close()
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
         e2.set_data(device_ptr=X.data_ptr())
         e2.set_centers(C0)
-        torch.cuda.synchronize()
         e2.synchronize()
+        prep_ms[precision] = (time.perf_counter() - tp) * 1e3
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
         e2.run(steps)
         e2.synchronize()
@@ -126,6 +140,12 @@ def main():
         "reresolved_per_step": st["reresolved"] / args.steps,
         "rescored_per_step": st["rescored"] / args.steps,
         "deep_per_step": st["deep"] / args.steps,
+        "bounded_per_step": st["bounded"] / args.steps,
+        "prep_ms_excluded": prep_ms.get(args.precision),
+        "timing_scope": "ms_per_step covers the Lloyd iterations only (assign, re-resolve, update, "
+                        "compactness) with the points resident in HBM; the per-data-set "
+                        "preparation (reduced-precision copy of the points and their norm bounds, "
+                        "prep_ms_excluded) runs once before the timed region",
         "assign_tiles_frac": (st["tiles"] / st["tiles_total"]) if st["tiles_total"] else None,
         "roofline": {"bound": "mfma",
                      "achieved": achieved / 1e12, "peak": PEAK[args.precision] / 1e12,
@@ -158,6 +178,7 @@ def main():
         # compactness) on the first `s` points, and the same iteration on the
         # GPU's plain FP32 path for a full-size bit-exact parity check.
         from oracle import oracle as orc
+        orc.use_native()
         s = min(args.cpu_sample or n, n)
         Xs = X[:s].cpu().numpy()
         t0 = time.perf_counter()
@@ -190,7 +211,7 @@ def main():
         del Xh
         out["cpu_baseline"] = {
             "value": s * float(k) / t_cpu, "unit": "pts*ctrs/s", "cores": orc.num_threads(),
-            "kind": "port",
+            "kind": "port", **cpu_info(orc),
             "sample": f"one full Lloyd iteration (assign + canonical update + compactness) of the "
                       f"CPU oracle on the first {s} points of the same workload, same initial centres",
             "seconds": t_cpu,
@@ -234,6 +255,73 @@ def bench_fp64(args, X, C0, spec, km, np):
                    "N": n, "D": d, "K": k, "precision": "fp64"},
         "timing": "difference of two computeKmeansD calls (set-up and copies cancel)",
         "status": int(st),
+    }
+
+
+def cpu_info(orc):
+    """The host the CPU baseline ran on: model, logical CPUs available to the
+    process, OpenMP threads used and the oracle build's -march."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count()
+    return {"cpu_model": model, "nproc": nproc, "omp_threads": orc.num_threads(),
+            "march": orc.march()}
+
+
+def bench_streamed(args, X, C0, spec, km, np, torch):
+    """The points in pinned host memory, streamed through HBM in tiles of
+    --tile-rows rows every iteration (copies on their own stream, double
+    buffered, overlapped with the previous tile's assign and sums).  The
+    timed iterations include every copy."""
+    n, d, k = spec["n"], spec["d"], spec["k"]
+    Xp = torch.empty((n, d), dtype=torch.float32, pin_memory=True)
+    Xp.copy_(X)
+    del X
+    torch.cuda.empty_cache()
+    e = km.KMeans(n, d, k, precision=args.precision, timing=True, tile_rows=args.tile_rows)
+    e.set_data(host_ptr=Xp.data_ptr())
+    e.set_centers(C0)
+    if args.warmup:
+        e.run(args.warmup)
+        e.synchronize()
+        e.set_centers(C0)
+    e.close()
+    e = km.KMeans(n, d, k, precision=args.precision, timing=True, tile_rows=args.tile_rows)
+    e.set_data(host_ptr=Xp.data_ptr())
+    e.set_centers(C0)
+    e.synchronize()
+    t0 = time.perf_counter()
+    e.run(args.steps)
+    e.synchronize()
+    dt = time.perf_counter() - t0
+    st = e.stats()
+    e.close()
+    pairs = float(n) * k
+    xbytes = float(n) * d * 4
+    return {
+        "metric": "assign pts*ctrs/s per Lloyd iter", "value": pairs * args.steps / dt,
+        "unit": "pts*ctrs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": args.precision,
+        "data": "synthetic (Gaussian mixture generated on the GPU, then held in pinned host memory)",
+        "config": {"workload": f"{args.workload}: N={n} D={d} K={k} Lloyd, {args.precision} assign, "
+                               f"points streamed from pinned host memory in tiles of "
+                               f"{args.tile_rows} rows",
+                   "N": n, "D": d, "K": k, "precision": args.precision, "tile_rows": args.tile_rows},
+        "host_to_device_GBps": xbytes * args.steps / dt / 1e9,
+        "phases_ms_per_step": {p: st[f"ms_{p}"] / args.steps for p in
+                               ("assign", "reresolve", "update", "compactness")},
+        "timing_scope": "wall clock of the iterations, every tile's host-to-device copy included; "
+                        "host_to_device_GBps = bytes of the points per iteration / iteration time",
     }
 
 
@@ -345,6 +433,7 @@ def bench_batch(args, km, datagen, np, torch):
     out["speedup_vs_per_instance_engine"] = out["per_instance_engine"]["ms_per_step_extrapolated"] / out["ms_per_step"]
     if not args.no_cpu:
         from oracle import oracle as orc
+        orc.use_native()
         cs = sample[:8]
         t_cpu = 0.0
         for i in cs:
@@ -354,7 +443,7 @@ def bench_batch(args, km, datagen, np, torch):
             t_cpu += time.perf_counter() - ta
         pts = float(n[cs].sum())
         out["cpu_baseline"] = {"value": pts * k / t_cpu, "unit": "pts*ctrs/s",
-                               "cores": orc.num_threads(), "kind": "port",
+                               "cores": orc.num_threads(), "kind": "port", **cpu_info(orc),
                                "sample": f"one full Lloyd iteration of the CPU oracle on {len(cs)} "
                                          f"instances ({int(pts)} points), first-K centres"}
     return out

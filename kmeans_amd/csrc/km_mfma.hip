@@ -919,7 +919,9 @@ __global__ __launch_bounds__(256) void k_deep_mfma(
     const float *__restrict__ X, int D, int K, int Kp, const uint16_t *__restrict__ Ch,
     const uint16_t *__restrict__ Cl, const float *__restrict__ bias, const float *__restrict__ C,
     const float *__restrict__ nc, const float *__restrict__ bnd, const float2 *__restrict__ xnb,
-    float big, int *__restrict__ labels, int *__restrict__ list2, int *__restrict__ nlist2) {
+    float big, int *__restrict__ labels, int *__restrict__ list2, int *__restrict__ nlist2,
+    float2 *__restrict__ xbo) {
+  // xbo (nullable): fresh drift bounds of the points this pass labels
   const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31;
   const long total = *nlist;
   const long wstride = (long)gridDim.x * 128;
@@ -1146,8 +1148,28 @@ __global__ __launch_bounds__(256) void k_deep_mfma(
       const float a4 = has4 ? bv[3] : -INFINITY;
       const float slack = (fabsf(bv[0]) + (has4 ? fabsf(a4) : 0.0f)) * 1.6e-5f;
       const float thr = has4 ? a4 + E + slack : -INFINITY;
-      if (!tie && lab >= 0 && sb > thr && sb > -INFINITY) labels[p] = lab;
-      else fall = true;
+      if (!tie && lab >= 0 && sb > thr && sb > -INFINITY) {
+        labels[p] = lab;
+        if (xbo) {
+          // drift bounds as in k_assign_mfma: the label's chain minus E bounds
+          // its exact score from below; the other candidates' chains plus E
+          // and the fourth approximate score plus E and the slack bound the
+          // rest from above
+          float hi = a4 + E + slack;
+#pragma unroll
+          for (int q = 0; q < 3; ++q)
+            if (bi[q] != lab && bi[q] >= 0 && bi[q] < K) hi = fmaxf(hi, sv[q] + E);
+          const float lo = sb - E;
+          const float x2u = xn * xn * (1.0f + 2.4e-7f);
+          const float x2l = xn * xn * (1.0f - Df * 1.8e-7f - 4e-6f);
+          const float uu = (x2u - 2.0f * lo) + 3.6e-7f * (x2u + 2.0f * fabsf(lo));
+          const float ll = (x2l - 2.0f * hi) - 3.6e-7f * (x2l + 2.0f * fabsf(hi));
+          xbo[p] = make_float2(sqrtf(fmaxf(uu, 0.0f)) * (1.0f + 1e-6f) + 1e-30f,
+                               ll > 0.0f ? sqrtf(ll) * (1.0f - 1e-6f) : 0.0f);
+        }
+      } else {
+        fall = true;
+      }
     }
     const uint64_t mk = __ballot(fall);
     if (mk) {
@@ -1616,15 +1638,16 @@ hipError_t mfma_prepare(Engine *e) {
 // FP32 full search for the uncertified points (device-side count).
 hipError_t mfma_reresolve(Engine *e, long r0, long rows) {
   MfmaState *s = st(e);
+  const bool all = r0 == 0 && rows == e->m;
   return mfma_reresolve_view(e, e->dX + r0 * e->n, s->Xh + r0 * s->Dp, s->xnb + r0,
-                             e->dlabels + r0, rows);
+                             e->dlabels + r0, rows, all ? s->xb : nullptr);
 }
 
 // Points left open by the assign kernel: deep MFMA pass, then the FP32 full
 // search of what remains (device-side counts; X, Xh, xn are the same rows
 // the assign pass saw, indexed from 0).
 hipError_t mfma_reresolve_view(Engine *e, const float *X, const uint16_t *Xh, const float2 *xn,
-                               int *lab, long rows) {
+                               int *lab, long rows, float2 *xbo) {
   MfmaState *s = st(e);
   hipError_t r;
   if ((r = hipMemsetAsync(s->nflag2 + e->cur_slot, 0, sizeof(int), e->stream)) != hipSuccess) return r;
@@ -1634,7 +1657,7 @@ hipError_t mfma_reresolve_view(Engine *e, const float *X, const uint16_t *Xh, co
 #define DEEP(KS)                                                                                 \
   hipLaunchKernelGGL(k_deep_mfma<KS>, dim3(g), dim3(256), 0, e->stream, s->nflag + e->cur_slot,  \
                      s->flags, X, e->n, e->k, s->Kp, s->Chb, s->Clb, s->biasb, e->dC, e->dnc,     \
-                     s->bnd, xn, FLT_MAX, lab, s->flags2, s->nflag2 + e->cur_slot)
+                     s->bnd, xn, FLT_MAX, lab, s->flags2, s->nflag2 + e->cur_slot, xbo)
   switch (s->KS) {
     case 1: DEEP(1); break;
     case 2: DEEP(2); break;

@@ -24,7 +24,7 @@ hipError_t mfma_assign_view(Engine *e, const float *X, const uint16_t *Xh, const
                             const int *home = nullptr, const int *Mdev = nullptr,
                             float2 *xbo = nullptr);
 hipError_t mfma_reresolve_view(Engine *e, const float *X, const uint16_t *Xh, const float2 *xn,
-                               int *lab, long rows);
+                               int *lab, long rows, float2 *xbo = nullptr);
 const uint16_t *mfma_tile_xh(Engine *e);
 const float2 *mfma_tile_xnorm(Engine *e);
 int mfma_row_stride(Engine *e);  // padded width Dp of the reduced-precision rows

@@ -23,6 +23,30 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE, "-B"], check=True)
 
 
+_march = "x86-64-v3"
+
+
+def use_native():
+    """Build and load the oracle with -march=native for this host (the CPU
+    baseline's build; the portable x86-64-v3 build stays for the tests).
+    Returns False, keeping the portable build, if compiling fails or the
+    library is already loaded."""
+    global _LIB, _march
+    if _lib is not None:
+        return False
+    native = os.path.join(_HERE, "libkmoracle_native.so")
+    r = subprocess.run(["make", "-s", "-C", _HERE, "-B", "libkmoracle_native.so"],
+                       capture_output=True)
+    if r.returncode != 0 or not os.path.exists(native):
+        return False
+    _LIB, _march = native, "native"
+    return True
+
+
+def march():
+    return _march
+
+
 def lib():
     global _lib
     if _lib is None:
