@@ -1397,7 +1397,10 @@ static hipError_t prepare_centres(Engine *e) {
   hipLaunchKernelGGL(k_centre_bounds, dim3(1), dim3(256), 0, e->stream, e->k, e->n, e->dnc, s->dcn,
                      s->bnd);
   s->c_ready = true;
-  if (e->dorder && e->k <= MF_ORD_MAX_K) {  // tile table of the label-ordered assign
+  // tile table of the label-ordered assign, which runs when there are no
+  // drift bounds to use (after centres set from outside)
+  s->ttab_ok = false;
+  if (e->dorder && e->k <= MF_ORD_MAX_K && !s->bnd_valid) {
     hipError_t r;
     const int nKT = s->Kp / MF_TILE;
     if (!s->ttab &&
