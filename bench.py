@@ -141,6 +141,7 @@ def main():
         "rescored_per_step": st["rescored"] / args.steps,
         "deep_per_step": st["deep"] / args.steps,
         "bounded_per_step": st["bounded"] / args.steps,
+        "paired_per_step": st["paired"] / args.steps,
         "prep_ms_excluded": prep_ms.get(args.precision),
         "timing_scope": "ms_per_step covers the Lloyd iterations only (assign, re-resolve, update, "
                         "compactness) with the points resident in HBM; the per-data-set "

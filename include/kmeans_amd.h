@@ -192,6 +192,9 @@ typedef struct kmeansStats {
                                  triangle bound                               */
   long long tiles_total;      /* ... out of this many                         */
   long long bounded;          /* point assignments kept by the drift bounds   */
+  long long paired;           /* ... decided between the label and the runner-up
+                                 by their two FP32 chains, the drift bounds
+                                 having separated every other centre          */
 } kmeansStats;
 
 typedef struct kmeansEngine *kmeansHandle;

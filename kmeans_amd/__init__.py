@@ -67,7 +67,7 @@ class Stats(ctypes.Structure):
                 ("ms_train", ctypes.c_double), ("train_loss_first", ctypes.c_double),
                 ("train_loss_last", ctypes.c_double), ("deep", ctypes.c_longlong),
                 ("tiles", ctypes.c_longlong), ("tiles_total", ctypes.c_longlong),
-                ("bounded", ctypes.c_longlong)]
+                ("bounded", ctypes.c_longlong), ("paired", ctypes.c_longlong)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

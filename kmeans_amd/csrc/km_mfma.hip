@@ -95,12 +95,14 @@ struct MfmaState {
   int *home = nullptr;
   long home_cap = 0;
   bool ttab_ok = false;
-  // drift bounds of the resident points: xb[i] = (u, l) with u >= |x_i -
-  // c_label| and l <= |x_i - c_j| for every other centre j, valid for the
-  // centres Cref; drift[j] >= |c_j - Cref_j| and drift[k] their maximum.
-  // ulist: the points the bounds leave open, bcnt/bofs/btsum their per-block
-  // counts and offsets.  bnd_valid: xb and Cref describe the last assign.
-  float2 *xb = nullptr;
+  // drift bounds of the resident points: xb[i] = (u, l, l3, r) with u >=
+  // |x_i - c_label| and l <= |x_i - c_j| for every other centre j, r (int
+  // bits, -1 for none) the runner-up centre and l3 <= |x_i - c_j| for every
+  // centre j other than the label and r, valid for the centres Cref;
+  // drift[j] >= |c_j - Cref_j| and drift[k] their maximum.  ulist: the
+  // points the bounds leave open, bcnt/bofs/btsum their per-block counts and
+  // offsets.  bnd_valid: xb and Cref describe the last assign.
+  float4 *xb = nullptr;
   float *Cref = nullptr, *drift = nullptr;
   int *ulist = nullptr, *bcnt = nullptr, *bofs = nullptr, *btsum = nullptr;
   bool bnd_valid = false;
@@ -270,76 +272,260 @@ __global__ __launch_bounds__(256) void k_drift(int K, int D, const float *__rest
   }
 }
 
-// The test for the point at position q of `order` (or point q): whether its
-// moved bounds keep its label, and the moved bounds.  bnd = the centre
-// bounds [cmax, ncmax, ...] of the current centres.
-__device__ inline bool bound_keep(long q, const int *__restrict__ order,
-                                  const int *__restrict__ labels, const float2 *__restrict__ xb,
-                                  const float *__restrict__ drift, int K, const float *bnd,
-                                  const float2 *__restrict__ xnb, float Df, long &p, float2 &nb) {
-  p = order ? (long)order[q] : q;
-  const int a = labels[p];
-  const float2 b = xb[p];
-  const float u = (b.x + drift[a]) * (1.0f + 1e-6f);
-  const float l = fmaxf((b.y - drift[K]) * (1.0f - 1e-6f), 0.0f);
-  nb = make_float2(u, l);
-  const float e32 = Df * 5.9604645e-8f * 1.0001f * (0.5f * bnd[1] + xnb[p].x * bnd[0]);
-  return l > u && (l - u) * (l + u) * (1.0f - 1e-6f) > 4.0f * e32 * 1.0001f + 1e-30f;
+// Distances from scores: |x - c|^2 = |x|^2 - 2 (x.c - 0.5 |c|^2), with xn >=
+// |x| the point's norm bound.  dist_upper: an upper bound of |x - c| given lo
+// <= the exact score of c; dist_lower: a lower bound given hi >= it.  (|x|^2
+// lies between xn^2 (1 - 1.8e-7 D - 4e-6) and xn^2 (1 + 2.4e-7); the relative
+// terms cover the roundings of the subtraction and of the root.)
+__device__ inline float dist_upper(float xn, float lo) {
+  const float x2u = xn * xn * (1.0f + 2.4e-7f);
+  const float uu = (x2u - 2.0f * lo) + 3.6e-7f * (x2u + 2.0f * fabsf(lo));
+  return sqrtf(fmaxf(uu, 0.0f)) * (1.0f + 1e-6f) + 1e-30f;
+}
+__device__ inline float dist_lower(float xn, float hi, float Df) {
+  const float x2l = xn * xn * (1.0f - Df * 1.8e-7f - 4e-6f);
+  const float ll = (x2l - 2.0f * hi) - 3.6e-7f * (x2l + 2.0f * fabsf(hi));
+  return ll > 0.0f ? sqrtf(ll) * (1.0f - 1e-6f) : 0.0f;
 }
 
-// Pass 1: per block of 256 positions, the number of points left open.
+// The plain path's score chains of one point against two centres (-0.5 nc_j,
+// then one fused multiply-add per dimension in ascending order), one pass
+// over the point's row.
+__device__ inline void chain2(const float *__restrict__ xr, const float *__restrict__ ra,
+                              const float *__restrict__ rb, int D, bool vec, float &sa,
+                              float &sb) {
+  if (vec) {  // D % 4 == 0 and 16-byte aligned rows
+    const float4 *x4 = reinterpret_cast<const float4 *>(xr);
+    const float4 *a4 = reinterpret_cast<const float4 *>(ra);
+    const float4 *b4 = reinterpret_cast<const float4 *>(rb);
+#pragma unroll 4
+    for (int q = 0; q < D / 4; ++q) {
+      const float4 xv = x4[q], av = a4[q], bv = b4[q];
+      sa = __builtin_fmaf(xv.x, av.x, sa);
+      sb = __builtin_fmaf(xv.x, bv.x, sb);
+      sa = __builtin_fmaf(xv.y, av.y, sa);
+      sb = __builtin_fmaf(xv.y, bv.y, sb);
+      sa = __builtin_fmaf(xv.z, av.z, sa);
+      sb = __builtin_fmaf(xv.z, bv.z, sb);
+      sa = __builtin_fmaf(xv.w, av.w, sa);
+      sb = __builtin_fmaf(xv.w, bv.w, sb);
+    }
+  } else {
+    for (int q = 0; q < D; ++q) {
+      sa = __builtin_fmaf(xr[q], ra[q], sa);
+      sb = __builtin_fmaf(xr[q], rb[q], sb);
+    }
+  }
+}
+
+// The test for the point at position q of `order` (or point q), its bounds
+// moved by the drift (u' = u + drift[a], l' = l - max drift, l3' = l3 - max
+// drift): 0 when they keep its label (l'^2 - u'^2 > 4 E32 as above), 1 when
+// they separate every centre but the label a and the runner-up r in the same
+// way (l3'^2 - u'^2 > 4 E32: the FP32 chain of a then exceeds every other
+// centre's but possibly r's, so the label is a or r, whichever chain is
+// larger, lowest index on equal chains), 2 when the point must be scored.
+// nb = the moved bounds, e32 = the FP32 chains' error bound for this point.
+// bnd = the centre bounds [cmax, ncmax, ...] of the current centres.
+__device__ inline int bound_class(long q, const int *__restrict__ order,
+                                  const int *__restrict__ labels, const float4 *__restrict__ xb,
+                                  const float *__restrict__ drift, int K, const float *bnd,
+                                  const float2 *__restrict__ xnb, float Df, long &p, int &a,
+                                  float4 &nb, float &e32) {
+  p = order ? (long)order[q] : q;
+  a = labels[p];
+  const float4 b = xb[p];
+  const float dm = drift[K];
+  const float u = (b.x + drift[a]) * (1.0f + 1e-6f);
+  const float l = fmaxf((b.y - dm) * (1.0f - 1e-6f), 0.0f);
+  const float l3 = fmaxf((b.z - dm) * (1.0f - 1e-6f), 0.0f);
+  nb = make_float4(u, l, l3, b.w);
+  e32 = Df * 5.9604645e-8f * 1.0001f * (0.5f * bnd[1] + xnb[p].x * bnd[0]);
+  const float t = 4.0f * e32 * 1.0001f + 1e-30f;
+  if (l > u && (l - u) * (l + u) * (1.0f - 1e-6f) > t) return 0;
+  const int r = __float_as_int(b.w);
+  if (r >= 0 && r < K && r != a && l3 > u && (l3 - u) * (l3 + u) * (1.0f - 1e-6f) > t) return 1;
+  return 2;
+}
+
+// Pass 1: per block b of 256 positions, the number of points left open
+// (bcnt[b]) and of those with two candidates (bcnt[nblk + b], nblk =
+// gridDim.x), so that one exclusive scan over bcnt places the open points
+// first and the two-candidate points after them.
 __global__ __launch_bounds__(256) void k_bound_count(long M, const int *__restrict__ order,
                                                      const int *__restrict__ labels,
-                                                     const float2 *__restrict__ xb,
+                                                     const float4 *__restrict__ xb,
                                                      const float *__restrict__ drift, int K,
                                                      const float *__restrict__ bnd,
                                                      const float2 *__restrict__ xnb, int D,
                                                      int *__restrict__ bcnt,
                                                      const int *__restrict__ stop) {
   if (stop && *stop) return;
-  __shared__ int wc[4];
+  __shared__ int wc[2][4];
   const long q = (long)blockIdx.x * 256 + threadIdx.x;
-  bool open = false;
+  int cls = 0;
   if (q < M) {
     long p;
-    float2 nb;
-    open = !bound_keep(q, order, labels, xb, drift, K, bnd, xnb, (float)D, p, nb);
+    int a;
+    float4 nb;
+    float e32;
+    cls = bound_class(q, order, labels, xb, drift, K, bnd, xnb, (float)D, p, a, nb, e32);
   }
-  const uint64_t mk = __ballot(open);
-  if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = __popcll(mk);
+  const uint64_t mo = __ballot(q < M && cls == 2), mp = __ballot(q < M && cls == 1);
+  if ((threadIdx.x & 63) == 0) {
+    wc[0][threadIdx.x >> 6] = __popcll(mo);
+    wc[1][threadIdx.x >> 6] = __popcll(mp);
+  }
   __syncthreads();
-  if (threadIdx.x == 0) bcnt[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
+  if (threadIdx.x < 2)
+    bcnt[(long)threadIdx.x * gridDim.x + blockIdx.x] =
+        wc[threadIdx.x][0] + wc[threadIdx.x][1] + wc[threadIdx.x][2] + wc[threadIdx.x][3];
 }
 
-// Pass 2 (same test, same inputs): the open points in order into ulist at
-// the block's offset, the moved bounds of the kept points into xb.
-__global__ __launch_bounds__(256) void k_bound_compact(long M, const int *__restrict__ order,
-                                                       const int *__restrict__ labels,
-                                                       float2 *__restrict__ xb,
-                                                       const float *__restrict__ drift, int K,
-                                                       const float *__restrict__ bnd,
-                                                       const float2 *__restrict__ xnb, int D,
-                                                       const int *__restrict__ bofs,
-                                                       int *__restrict__ ulist,
-                                                       const int *__restrict__ stop) {
+// Pass 2 (same test, same inputs; bofs = the exclusive scan of bcnt): the
+// open points in order into ulist[bofs[b] ...], the two-candidate points in
+// order into ulist[bofs[nblk + b] ...], and the moved bounds of every point
+// not left open into xb.
+__global__ __launch_bounds__(256) void k_bound_compact(
+    long M, const int *__restrict__ order, const int *__restrict__ labels, float4 *__restrict__ xb,
+    const float *__restrict__ drift, int K, const float *__restrict__ bnd,
+    const float2 *__restrict__ xnb, int D, const int *__restrict__ bofs, int *__restrict__ ulist,
+    const int *__restrict__ stop) {
   if (stop && *stop) return;
-  __shared__ int wc[4];
+  __shared__ int wc[2][4];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const long q = (long)blockIdx.x * 256 + threadIdx.x;
-  bool open = false;
+  int cls = 0, a = 0;
   long p = 0;
-  float2 nb = make_float2(0.0f, 0.0f);
-  if (q < M) open = !bound_keep(q, order, labels, xb, drift, K, bnd, xnb, (float)D, p, nb);
-  const uint64_t mk = __ballot(open);
-  if (lane == 0) wc[w] = __popcll(mk);
-  __syncthreads();
-  int base = bofs[blockIdx.x];
-  for (int u = 0; u < w; ++u) base += wc[u];
-  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-  if (q < M) {
-    if (open) ulist[base + __popcll(mk & below)] = (int)p;
-    else xb[p] = nb;
+  float4 nb = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float e32 = 0.0f;
+  if (q < M) cls = bound_class(q, order, labels, xb, drift, K, bnd, xnb, (float)D, p, a, nb, e32);
+  const bool open = q < M && cls == 2, pair = q < M && cls == 1;
+  const uint64_t mo = __ballot(open), mp = __ballot(pair);
+  if (lane == 0) {
+    wc[0][w] = __popcll(mo);
+    wc[1][w] = __popcll(mp);
   }
+  __syncthreads();
+  int bo = bofs[blockIdx.x], bp = bofs[gridDim.x + blockIdx.x];
+  for (int u = 0; u < w; ++u) {
+    bo += wc[0][u];
+    bp += wc[1][u];
+  }
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  if (open) ulist[bo + __popcll(mo & below)] = (int)p;
+  else if (q < M) xb[p] = nb;
+  if (pair) ulist[bp + __popcll(mp & below)] = (int)p;
+}
+
+// The two-candidate points (positions [rng[0], rng[nblk]) of ulist, after
+// the open ones; xb holds their moved bounds (u', l', l3', r)): the label
+// is the label a or the runner-up r, whichever FP32 chain is larger (lowest
+// index on equal chains; X, C, nc the rows, the current centres and their
+// chains of squares), since every other centre is separated by the bounds.
+// Fresh bounds from the two chains (each within e32 of its exact score),
+// l3' kept; the count goes to slot 4 of the counter lines ncl.
+// Each wave takes 64 points, one per lane.  With 16-byte rows (D % 4 == 0,
+// aligned) the wave stages its points' rows and both candidates' centre rows
+// in LDS 16 dimensions at a time with coalesced loads (four lanes per
+// 64-byte piece of a row) and each lane continues its two chains from there;
+// otherwise each lane reads its rows itself.
+#define PR_PD 16                 // dimensions per staged piece
+#define PR_PS (PR_PD / 4 + 1)    // float4 stride of a staged row (conflict-free reads)
+__global__ __launch_bounds__(256) void k_pair_decide(
+    const int *__restrict__ ulist, const int *__restrict__ rng, long nblk,
+    int *__restrict__ labels, float4 *__restrict__ xb, const float2 *__restrict__ xnb,
+    const float *__restrict__ bnd, int D, const float *__restrict__ X,
+    const float *__restrict__ C, const float *__restrict__ nc, int *__restrict__ ncl,
+    const int *__restrict__ stop) {
+  if (stop && *stop) return;
+  __shared__ float4 lds[4 * 3 * 64 * PR_PS];  // per wave: point rows, label rows, runner-up rows
+  const long i0 = rng[0], i1 = rng[nblk];
+  const float Df = (float)D;
+  const bool vec = (D & 3) == 0 && (((uintptr_t)X | (uintptr_t)C) & 15) == 0;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  float4 *lx = lds + w * 3 * 64 * PR_PS, *la = lx + 64 * PR_PS, *lr = la + 64 * PR_PS;
+  const long nw = (long)gridDim.x * 4;
+  int cnt = 0;
+  for (long g = (long)blockIdx.x * 4 + w; i0 + g * 64 < i1; g += nw) {
+    const long i = i0 + g * 64 + lane;
+    const bool ok = i < i1;
+    const long p = ok ? ulist[i] : 0;
+    const int a = ok ? labels[p] : 0;
+    const float4 b = ok ? xb[p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const int r = ok ? __float_as_int(b.w) : 0;
+    float sa = ok ? -0.5f * nc[a] : 0.0f, sr = ok ? -0.5f * nc[r] : 0.0f;
+    if (vec) {
+      const int np = (D + PR_PD - 1) / PR_PD;
+      for (int pc = 0; pc < np; ++pc) {
+        // stage: row t*16 + lane/4 of each array, 16-byte piece lane % 4
+        const int q = lane & 3, d0 = pc * PR_PD + q * 4;
+        float4 xv[4], av[4], bv[4];  // all twelve loads in flight before the stores
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int row = t * 16 + (lane >> 2);
+          const long pp = __shfl(p, row);
+          const int aa = __shfl(a, row), rr = __shfl(r, row);
+          const bool v = i0 + g * 64 + row < i1 && d0 < D;
+          xv[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          av[t] = xv[t];
+          bv[t] = xv[t];
+          if (v) {
+            xv[t] = *reinterpret_cast<const float4 *>(X + pp * D + d0);
+            av[t] = *reinterpret_cast<const float4 *>(C + (long)aa * D + d0);
+            bv[t] = *reinterpret_cast<const float4 *>(C + (long)rr * D + d0);
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int row = t * 16 + (lane >> 2);
+          lx[row * PR_PS + q] = xv[t];
+          la[row * PR_PS + q] = av[t];
+          lr[row * PR_PS + q] = bv[t];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // the chains continue over this piece's dimensions, in order
+#pragma unroll
+        for (int u = 0; u < PR_PD / 4; ++u) {
+          if (pc * PR_PD + u * 4 < D) {  // uniform
+            const float4 x4 = lx[lane * PR_PS + u], a4 = la[lane * PR_PS + u],
+                         b4 = lr[lane * PR_PS + u];
+            sa = __builtin_fmaf(x4.x, a4.x, sa);
+            sr = __builtin_fmaf(x4.x, b4.x, sr);
+            sa = __builtin_fmaf(x4.y, a4.y, sa);
+            sr = __builtin_fmaf(x4.y, b4.y, sr);
+            sa = __builtin_fmaf(x4.z, a4.z, sa);
+            sr = __builtin_fmaf(x4.z, b4.z, sr);
+            sa = __builtin_fmaf(x4.w, a4.w, sa);
+            sr = __builtin_fmaf(x4.w, b4.w, sr);
+          }
+        }
+        // the next piece overwrites the staging arrays
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+    } else if (ok) {
+      chain2(X + p * D, C + (long)a * D, C + (long)r * D, D, false, sa, sr);
+    }
+    if (ok) {
+      const float xn = xnb[p].x;
+      const float e32 = Df * 5.9604645e-8f * 1.0001f * (0.5f * bnd[1] + xn * bnd[0]);
+      const bool tr = sr > sa || (sr == sa && r < a);
+      const int L = tr ? r : a, O = tr ? a : r;
+      const float sL = tr ? sr : sa, sO = tr ? sa : sr;
+      labels[p] = L;
+      const float lO = dist_lower(xn, sO + e32, Df);
+      xb[p] = make_float4(dist_upper(xn, sL - e32), fminf(lO, b.z), b.z, __int_as_float(O));
+      ++cnt;
+    }
+  }
+  // per-wave sum, one atomic per wave with points
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) cnt += __shfl_xor(cnt, o);
+  if (lane == 0 && cnt) atomicAdd(ncl + (blockIdx.x & (NC_SPREAD - 1)) * NC_STRIDE + 4, cnt);
 }
 
 // Cref = C once the assign has made every point's bounds valid for C, and
@@ -442,7 +628,7 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
     int *__restrict__ flags, int *__restrict__ nflag, int *__restrict__ ncand,
     const int *__restrict__ stop, const int *__restrict__ order, const int *__restrict__ ttab,
     const float *__restrict__ tdist, const int *__restrict__ home, const int *__restrict__ Mdev,
-    float2 *__restrict__ xbo) {
+    float4 *__restrict__ xbo) {
   // stop (nullable): set by a device-side convergence test; the launch then
   // assigns nothing and lists nothing
   if (stop && *stop) return;
@@ -838,9 +1024,14 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
     const float sy = __shfl_xor(sx, 32);  // the other half's chain
     bool fall2 = false;
     float sb = sx, so = sy;  // the chains of the label and of the other candidate
+    int lab = k1, oth = k2;
     if (live && need) {
-      int lab = k1;
-      if (k2 >= 0 && k2 < K && (sy > sx || (sy == sx && k2 < k1))) { lab = k2; sb = sy; so = sx; }
+      if (k2 >= 0 && k2 < K && (sy > sx || (sy == sx && k2 < k1))) {
+        lab = k2;
+        oth = k1;
+        sb = sy;
+        so = sx;
+      }
       if (sb > thr && sb > -INFINITY) labels[p] = lab;
       else fall2 = true;
     }
@@ -851,27 +1042,32 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
       // fresh drift bounds: lo <= the exact score of the label (the
       // approximate one minus E and the packing slack, or its FP32 chain
       // minus E >= the chain's own bound), hi >= the exact score of every
-      // other centre; distances from |x - c|^2 = |x|^2 - 2 score with the
-      // upper and lower bounds of |x|^2 that xn gives.  A point sent on to
-      // the later passes gets bounds that never hold.
-      float lo = -INFINITY, hi = INFINITY;
+      // other centre, hi3 >= that of every centre but the label and the
+      // runner-up rr (the other candidate: every centre but the two has an
+      // approximate score at most a3); distances from |x - c|^2 = |x|^2 - 2
+      // score with the upper and lower bounds of |x|^2 that xn gives.  A
+      // point sent on to the later passes gets bounds that never hold.
+      float lo = -INFINITY, hi = INFINITY, hi3 = INFINITY;
+      int rr = -1, lb = k1;
       if (quick) {
         lo = a1 - E - slack;
         hi = a2 + E + slack;
+        rr = k2;
       } else if (need && !fall2) {
         lo = sb - E;
         hi = fmaxf(a3 + E + slack, so + E);
+        rr = oth;
+        lb = lab;
       }
-      float u = INFINITY, l = 0.0f;
+      if (lo > -INFINITY) hi3 = has_third ? a3 + E + slack : -INFINITY;
+      if (rr < 0 || rr >= K || rr == lb) rr = -1;
+      float u = INFINITY, l = 0.0f, l3 = 0.0f;
       if (lo > -INFINITY) {
-        const float x2u = nb.x * nb.x * (1.0f + 2.4e-7f);
-        const float x2l = nb.x * nb.x * (1.0f - (float)D * 1.8e-7f - 4e-6f);
-        const float uu = (x2u - 2.0f * lo) + 3.6e-7f * (x2u + 2.0f * fabsf(lo));
-        u = sqrtf(fmaxf(uu, 0.0f)) * (1.0f + 1e-6f) + 1e-30f;
-        const float ll = (x2l - 2.0f * hi) - 3.6e-7f * (x2l + 2.0f * fabsf(hi));
-        l = ll > 0.0f ? sqrtf(ll) * (1.0f - 1e-6f) : 0.0f;
+        u = dist_upper(nb.x, lo);
+        l = dist_lower(nb.x, hi, (float)D);
+        l3 = hi3 == -INFINITY ? INFINITY : dist_lower(nb.x, hi3, (float)D);
       }
-      xbo[p] = make_float2(u, l);
+      xbo[p] = make_float4(u, l, l3, __int_as_float(rr));
     }
     // wave-aggregated counts and appends (one atomic per wave)
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -920,7 +1116,7 @@ __global__ __launch_bounds__(256) void k_deep_mfma(
     const uint16_t *__restrict__ Cl, const float *__restrict__ bias, const float *__restrict__ C,
     const float *__restrict__ nc, const float *__restrict__ bnd, const float2 *__restrict__ xnb,
     float big, int *__restrict__ labels, int *__restrict__ list2, int *__restrict__ nlist2,
-    float2 *__restrict__ xbo) {
+    float4 *__restrict__ xbo) {
   // xbo (nullable): fresh drift bounds of the points this pass labels
   const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31;
   const long total = *nlist;
@@ -1154,18 +1350,25 @@ __global__ __launch_bounds__(256) void k_deep_mfma(
           // drift bounds as in k_assign_mfma: the label's chain minus E bounds
           // its exact score from below; the other candidates' chains plus E
           // and the fourth approximate score plus E and the slack bound the
-          // rest from above
-          float hi = a4 + E + slack;
+          // rest from above; the runner-up rr is the other candidate with the
+          // larger chain, and hi3 leaves it out
+          float hi = a4 + E + slack, hi3 = a4 + E + slack, sr = -INFINITY;
+          int rr = -1;
 #pragma unroll
           for (int q = 0; q < 3; ++q)
-            if (bi[q] != lab && bi[q] >= 0 && bi[q] < K) hi = fmaxf(hi, sv[q] + E);
-          const float lo = sb - E;
-          const float x2u = xn * xn * (1.0f + 2.4e-7f);
-          const float x2l = xn * xn * (1.0f - Df * 1.8e-7f - 4e-6f);
-          const float uu = (x2u - 2.0f * lo) + 3.6e-7f * (x2u + 2.0f * fabsf(lo));
-          const float ll = (x2l - 2.0f * hi) - 3.6e-7f * (x2l + 2.0f * fabsf(hi));
-          xbo[p] = make_float2(sqrtf(fmaxf(uu, 0.0f)) * (1.0f + 1e-6f) + 1e-30f,
-                               ll > 0.0f ? sqrtf(ll) * (1.0f - 1e-6f) : 0.0f);
+            if (bi[q] != lab && bi[q] >= 0 && bi[q] < K) {
+              hi = fmaxf(hi, sv[q] + E);
+              if (rr < 0 || sv[q] > sr) {
+                if (rr >= 0) hi3 = fmaxf(hi3, sr + E);
+                rr = bi[q];
+                sr = sv[q];
+              } else {
+                hi3 = fmaxf(hi3, sv[q] + E);
+              }
+            }
+          xbo[p] = make_float4(dist_upper(xn, sb - E), dist_lower(xn, hi, Df),
+                               hi3 == -INFINITY ? INFINITY : dist_lower(xn, hi3, Df),
+                               __int_as_float(rr));
         }
       } else {
         fall = true;
@@ -1437,7 +1640,7 @@ static void launch_mfma_mode(hipStream_t strm, dim3 g, size_t lds, long M, const
                              const float2 *xnb, float inv_s2, float eta, int *labels, int *flags,
                              int *nflag, int *ncand, const int *stop, const int *order,
                              const int *ttab, const float *tdist, const int *home, const int *Mdev,
-                             float2 *xbo) {
+                             float4 *xbo) {
   static int attr_lds = -1;
   if ((int)lds > attr_lds) {
     hipFuncSetAttribute((const void *)k_assign_mfma<KS, FP16, ORD, LIST>,
@@ -1456,7 +1659,7 @@ static void launch_mfma_ks(hipStream_t strm, bool fp16, dim3 g, long M, const ui
                            const float2 *xnb, float inv_s2, float eta, int *labels,
                            int *flags, int *nflag, int *ncand, const int *stop, const int *order,
                            const int *ttab, const float *tdist, const int *home, const int *Mdev,
-                           float2 *xbo) {
+                           float4 *xbo) {
   const bool ord = order != nullptr && home != nullptr;
   const bool list = order != nullptr && home == nullptr;
   const size_t lds = 2 * (size_t)MF_STAGE * MF_TILE * (KS * 2) * 16 + 2 * MF_STAGE * MF_TILE * 4 +
@@ -1482,21 +1685,22 @@ static hipError_t ensure_bounds(Engine *e) {
   if (s->xb) return hipSuccess;
   hipError_t r;
   const long nblk = km_cdiv(e->m, 256);
-  if ((r = hipMalloc(&s->xb, sizeof(float2) * (size_t)e->m)) != hipSuccess) return r;
+  if ((r = hipMalloc(&s->xb, sizeof(float4) * (size_t)e->m)) != hipSuccess) return r;
   if ((r = hipMalloc(&s->Cref, sizeof(float) * (size_t)e->k * e->n)) != hipSuccess) return r;
   if ((r = hipMalloc(&s->drift, sizeof(float) * (size_t)(e->k + 1))) != hipSuccess) return r;
   if ((r = hipMemsetAsync(s->drift, 0, sizeof(float) * (size_t)(e->k + 1), e->stream)) != hipSuccess)
     return r;
   if ((r = hipMalloc(&s->ulist, sizeof(int) * (size_t)e->m)) != hipSuccess) return r;
-  if ((r = hipMalloc(&s->bcnt, sizeof(int) * (size_t)nblk)) != hipSuccess) return r;
-  if ((r = hipMalloc(&s->bofs, sizeof(int) * (size_t)(nblk + 1))) != hipSuccess) return r;
-  if ((r = hipMalloc(&s->btsum, sizeof(int) * (size_t)(scan_tiles(nblk) + 1))) != hipSuccess)
+  // per block: open and two-candidate counts, their exclusive scan
+  if ((r = hipMalloc(&s->bcnt, sizeof(int) * (size_t)(2 * nblk))) != hipSuccess) return r;
+  if ((r = hipMalloc(&s->bofs, sizeof(int) * (size_t)(2 * nblk + 1))) != hipSuccess) return r;
+  if ((r = hipMalloc(&s->btsum, sizeof(int) * (size_t)(scan_tiles(2 * nblk) + 1))) != hipSuccess)
     return r;
   // an open list of zero points until the first compaction (a launch whose
   // compaction a convergence stop skipped reads these)
   if ((r = hipMemsetAsync(s->ulist, 0, sizeof(int) * (size_t)e->m, e->stream)) != hipSuccess)
     return r;
-  return hipMemsetAsync(s->bofs, 0, sizeof(int) * (size_t)(nblk + 1), e->stream);
+  return hipMemsetAsync(s->bofs, 0, sizeof(int) * (size_t)(2 * nblk + 1), e->stream);
 }
 
 hipError_t mfma_assign(Engine *e, long r0, long rows) {
@@ -1512,22 +1716,34 @@ hipError_t mfma_assign(Engine *e, long r0, long rows) {
     if ((r = prepare_centres(e)) != hipSuccess) return r;
   const bool all = r0 == 0 && rows == e->m;
   if (all && (r = ensure_bounds(e)) != hipSuccess) return r;
+  // this iteration's counters start at zero before the filter adds to them
+  if ((r = mfma_ensure_slots(e, e->cur_slot + 1)) != hipSuccess) return r;
+  int *nc2 = s->ncand + (size_t)e->cur_slot * NC_SLOT;
+  if ((r = hipMemsetAsync(s->nflag + e->cur_slot, 0, sizeof(int), e->stream)) != hipSuccess)
+    return r;
+  if ((r = hipMemsetAsync(nc2, 0, sizeof(int) * NC_SLOT, e->stream)) != hipSuccess) return r;
   const int *order = nullptr, *home = nullptr, *Mdev = nullptr;
   if (all && s->bnd_valid) {
     // the drift bounds of the last assign over all points: only the points
     // they leave open are assigned, in point order (the filter reads every
-    // point's label and bounds once, coalesced)
+    // point's label and bounds once, coalesced, and decides the points with
+    // two candidates itself)
     const long nblk = km_cdiv(rows, 256);
     hipLaunchKernelGGL(k_drift, dim3(km_cdiv(e->k, 4)), dim3(256), 0, e->stream, e->k, e->n,
                        e->dC, s->Cref, s->drift, e->assign_stop);
     hipLaunchKernelGGL(k_bound_count, dim3(nblk), dim3(256), 0, e->stream, rows,
                        (const int *)nullptr, e->dlabels, s->xb, s->drift, e->k, s->bnd, s->xnb,
                        e->n, s->bcnt, e->assign_stop);
-    if ((r = launch_scan_excl(e->stream, s->bcnt, nblk, s->btsum, s->bofs)) != hipSuccess)
+    if ((r = launch_scan_excl(e->stream, s->bcnt, 2 * nblk, s->btsum, s->bofs)) != hipSuccess)
       return r;
     hipLaunchKernelGGL(k_bound_compact, dim3(nblk), dim3(256), 0, e->stream, rows,
                        (const int *)nullptr, e->dlabels, s->xb, s->drift, e->k, s->bnd, s->xnb,
                        e->n, s->bofs, s->ulist, e->assign_stop);
+    const unsigned pg = (unsigned)std::min<long>(nblk, (long)num_cus() * 16);
+    hipLaunchKernelGGL(k_pair_decide, dim3(pg), dim3(256), 0, e->stream, s->ulist,
+                       s->bofs + nblk, nblk, e->dlabels, s->xb, s->xnb, s->bnd, e->n,
+                       (const float *)e->dX, (const float *)e->dC, (const float *)e->dnc, nc2,
+                       e->assign_stop);
     if ((r = hipGetLastError()) != hipSuccess) return r;
     order = s->ulist;
     Mdev = s->bofs + nblk;
@@ -1548,7 +1764,8 @@ hipError_t mfma_assign(Engine *e, long r0, long rows) {
     home = s->home;
   }
   if ((r = mfma_assign_view(e, e->dX + r0 * e->n, s->Xh + r0 * s->Dp, s->xnb + r0,
-                            e->dlabels + r0, rows, order, home, Mdev, all ? s->xb : nullptr)) !=
+                            e->dlabels + r0, rows, order, home, Mdev, all ? s->xb : nullptr,
+                            false)) !=
       hipSuccess)
     return r;
   // every point's bounds now hold for the current centres (fresh ones for the
@@ -1571,7 +1788,7 @@ hipError_t mfma_assign(Engine *e, long r0, long rows) {
 // xbo (nullable): fresh drift bounds of the assigned points.
 hipError_t mfma_assign_view(Engine *e, const float *X, const uint16_t *Xh, const float2 *xn,
                             int *lab, long rows, const int *order, const int *home,
-                            const int *Mdev, float2 *xbo) {
+                            const int *Mdev, float4 *xbo, bool zero_counters) {
   hipError_t r;
   MfmaState *s = st(e);
   if (!s->c_ready)
@@ -1579,8 +1796,10 @@ hipError_t mfma_assign_view(Engine *e, const float *X, const uint16_t *Xh, const
   if ((r = mfma_ensure_slots(e, e->cur_slot + 1)) != hipSuccess) return r;
   int *nf = s->nflag + e->cur_slot;
   int *nc2 = s->ncand + (size_t)e->cur_slot * NC_SLOT;
-  if ((r = hipMemsetAsync(nf, 0, sizeof(int), e->stream)) != hipSuccess) return r;
-  if ((r = hipMemsetAsync(nc2, 0, sizeof(int) * NC_SLOT, e->stream)) != hipSuccess) return r;
+  if (zero_counters) {
+    if ((r = hipMemsetAsync(nf, 0, sizeof(int), e->stream)) != hipSuccess) return r;
+    if ((r = hipMemsetAsync(nc2, 0, sizeof(int) * NC_SLOT, e->stream)) != hipSuccess) return r;
+  }
   if (rows <= 0) return hipSuccess;
   const float eta = s->fp16 ? 2.9802322e-8f / s->scale : 0.0f;  // half FP16 subnormal step
   const float inv_s2 = 1.0f / (s->scale * s->scale);
@@ -1650,7 +1869,7 @@ hipError_t mfma_reresolve(Engine *e, long r0, long rows) {
 // search of what remains (device-side counts; X, Xh, xn are the same rows
 // the assign pass saw, indexed from 0).
 hipError_t mfma_reresolve_view(Engine *e, const float *X, const uint16_t *Xh, const float2 *xn,
-                               int *lab, long rows, float2 *xbo) {
+                               int *lab, long rows, float4 *xbo) {
   MfmaState *s = st(e);
   hipError_t r;
   if ((r = hipMemsetAsync(s->nflag2 + e->cur_slot, 0, sizeof(int), e->stream)) != hipSuccess) return r;
@@ -1713,7 +1932,11 @@ void mfma_collect(Engine *e, int nslots) {
     for (int c = 0; c < NC_SPREAD; ++c) {
       e->stats.rescored += w[(size_t)i * NC_SLOT + c * NC_STRIDE];
       e->stats.tiles += w[(size_t)i * NC_SLOT + c * NC_STRIDE + 1];
-      e->stats.bounded += w[(size_t)i * NC_SLOT + c * NC_STRIDE + 3];
+      // slot 3: the points the filter did not leave open; slot 4: those of
+      // them it decided between two candidates
+      e->stats.bounded += w[(size_t)i * NC_SLOT + c * NC_STRIDE + 3] -
+                          w[(size_t)i * NC_SLOT + c * NC_STRIDE + 4];
+      e->stats.paired += w[(size_t)i * NC_SLOT + c * NC_STRIDE + 4];
       e->stats.tiles_total += w[(size_t)i * NC_SLOT + c * NC_STRIDE + 2];
     }
 }

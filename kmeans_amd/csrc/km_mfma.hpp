@@ -22,9 +22,9 @@ hipError_t mfma_prepare_rows(Engine *e, const float *X, long rows);
 hipError_t mfma_assign_view(Engine *e, const float *X, const uint16_t *Xh, const float2 *xn,
                             int *lab, long rows, const int *order = nullptr,
                             const int *home = nullptr, const int *Mdev = nullptr,
-                            float2 *xbo = nullptr);
+                            float4 *xbo = nullptr, bool zero_counters = true);
 hipError_t mfma_reresolve_view(Engine *e, const float *X, const uint16_t *Xh, const float2 *xn,
-                               int *lab, long rows, float2 *xbo = nullptr);
+                               int *lab, long rows, float4 *xbo = nullptr);
 const uint16_t *mfma_tile_xh(Engine *e);
 const float2 *mfma_tile_xnorm(Engine *e);
 int mfma_row_stride(Engine *e);  // padded width Dp of the reduced-precision rows

@@ -290,8 +290,9 @@ def test_mfma_drift_bounds_keep_labels(orc, prec):
         assert np.array_equal(e.labels(), ref["labels"]), done
         assert np.array_equal(e.centers(), ref["centers"]), done
     st = e.stats()
-    print(f"kept by bounds {st['bounded']} of {done * n}")
+    print(f"kept by bounds {st['bounded']}, decided between two {st['paired']} of {done * n}")
     assert st["bounded"] > 2 * n
+    assert st["paired"] > 0
     # centres set from outside: the bounds must not be used for them
     C1 = means.copy()
     e.set_centers(C1)
@@ -322,4 +323,63 @@ def test_mfma_drift_bounds_across_a_converged_run(orc):
     ref2 = orc.lloyd(X, C_conv, 3)
     assert np.array_equal(e.labels(), ref2["labels"])
     assert np.array_equal(e.centers(), ref2["centers"])
+    e.close()
+
+
+def pair_fixed_point_data(m=1 << 10, d=16, seed=7):
+    """Two mirrored pairs of clusters at a Lloyd fixed point.  In each pair,
+    cluster a: 3m points at x0 = -1, m at the origin (exact ties between a
+    and b, which go to a, the lower index) and m at x0 = -2^-12; cluster b:
+    4m points at x0 = +3/4 and m at x0 = +2^-12 (near-ties).  The other
+    coordinates are +-pairs (multiples of 1/64) within every group, so their
+    sums are exactly zero, and both means sit at x0 = -+3072.25 / 5120.  The
+    second pair is the first shifted by 8 in x1 (small enough that the FP32
+    chains still resolve the near-ties)."""
+    rng = np.random.default_rng(seed)
+    eps = 2.0 ** -12
+
+    def group(cnt, x0):
+        h = rng.integers(1, 64, size=(cnt // 2, d - 1)).astype(np.float32) / 64.0
+        G = np.zeros((cnt, d), np.float32)
+        G[:, 0] = x0
+        G[:, 1:] = np.concatenate([h, -h])
+        return G
+
+    def pair(base):
+        P = np.concatenate([group(3 * m, -1.0), group(m, 0.0), group(m, -eps),
+                            group(4 * m, 0.75), group(m, eps)])
+        P[:, 1] += base
+        return P
+
+    X = np.concatenate([pair(0.0), pair(8.0)])
+    X = np.ascontiguousarray(X[rng.permutation(X.shape[0])])
+    c = np.float32(3072.25 / 5120.0)
+    C0 = np.zeros((4, d), np.float32)
+    C0[:, 0] = [-c, c, -c, c]
+    C0[2:, 1] = 8.0
+    return X, C0
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_mfma_pair_decisions_at_a_fixed_point(orc, prec):
+    # The origin points are exact ties between a and b and the points at
+    # x0 = +-2^-12 near-ties; every other centre is >= 8 away, so once the
+    # bounds exist these points are decided by the two FP32 chains in the
+    # drift-bound filter, and the ties must go to the lower index.
+    X, C0 = pair_fixed_point_data()
+    n = X.shape[0]
+    ref = orc.lloyd(X, C0, 6)
+    assert np.array_equal(ref["centers"], C0)  # a fixed point
+    e = km.KMeans(n, X.shape[1], 4, precision=prec)
+    e.set_data(X)
+    e.set_centers(C0)
+    done = 0
+    for steps in (2, 2, 2):
+        e.run(steps)
+        done += steps
+        assert np.array_equal(e.labels(), ref["labels"]), done
+        assert np.array_equal(e.centers(), C0), done
+    st = e.stats()
+    print(f"{prec}: paired {st['paired']} bounded {st['bounded']} of {done * n}")
+    assert st["paired"] > 0
     e.close()
