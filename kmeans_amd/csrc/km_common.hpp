@@ -96,6 +96,16 @@ struct UpdateWs {
   // nullable: an n-sized array that receives each block's stable sort by
   // label at the block's own rows (instead of perm), kept for the next assign
   int *perm_out;
+  // Reuse of the previous update's chunk partials (fd null: none).  The
+  // previous update of the same rows left its sort in perm_prev (n-sized, at
+  // the block's rows), its per-centre ranges in cs_prev ([cstart K | ccount
+  // K | cq K+1]) and its partials in P_prev; fd (K ints) receives each
+  // centre's first member position at which the two sorts differ, and every
+  // chunk that lies wholly before it has the same members, so the same
+  // partials, bit for bit.  fd without perm_prev: everything is computed.
+  const int *perm_prev = nullptr, *cs_prev = nullptr;
+  const double *P_prev = nullptr;
+  int *fd = nullptr;
 };
 size_t update_ws_layout(long n, int d, int k, char *base, UpdateWs *ws);
 // Streams, events and the two per-block workspaces of the overlapped update
@@ -105,9 +115,11 @@ struct UpdateOverlap {
   hipEvent_t start, done[2], fin[2], sorted[2];
   const UpdateWs *ws[2];
 };
+// blk (nullable): one workspace view per canonical block (the side
+// workspace's sort scratch with the block's own persistent arrays)
 hipError_t launch_update_accumulate_2s(hipStream_t s, const UpdateOverlap &ov, long n, int d,
                                        int k, const float *X, const int *labels, double *S,
-                                       long long *cnt);
+                                       long long *cnt, const UpdateWs *blk = nullptr);
 int sub_block(int k);
 // out[0..L] = exclusive scan of in[0..L) (out[L] the total); tsum holds
 // scan_tiles(L) + 1 ints

@@ -45,7 +45,12 @@ Engine::~Engine() {
   hipFree(dxn);
   hipFree(dws);
   hipFree(dws2);
-  hipFree(dorder);
+  for (int p = 0; p < 2; ++p) {
+    hipFree(dperm[p]);
+    hipFree(dsetup[p]);
+    hipFree(dpart[p]);
+  }
+  hipFree(dfd);
   for (auto &s : upd_side)
     if (s) hipStreamDestroy(s);
   for (auto &e : upd_ev)
@@ -110,8 +115,18 @@ kmeansErrorStatus Engine::create(long m_, int n_, int k_, const kmeansOptions *o
     for (auto &e : upd_ev)
       KM_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming), KMEANS_ERROR_BUILDDATA);
   }
-  if (opt.precision != KM_PREC_FP32 && opt.tile_rows == 0)
-    KM_HIP_CHECK(hipMalloc(&dorder, sizeof(int) * (size_t)m), KMEANS_ERROR_BUILDDATA);
+  if (opt.tile_rows == 0) {  // resident points: the updates' reuse state
+    const long nblk = km_cdiv(m, KM_BLOCK);
+    reuse_qmax = std::min<long>(m, KM_BLOCK) / KM_CHUNK + k + 1;
+    for (int p = 0; p < 2; ++p) {
+      KM_HIP_CHECK(hipMalloc(&dperm[p], sizeof(int) * (size_t)m), KMEANS_ERROR_BUILDDATA);
+      KM_HIP_CHECK(hipMalloc(&dsetup[p], sizeof(int) * (size_t)nblk * (3 * k + 1)),
+                   KMEANS_ERROR_BUILDDATA);
+      KM_HIP_CHECK(hipMalloc(&dpart[p], sizeof(double) * (size_t)nblk * reuse_qmax * (n + 1)),
+                   KMEANS_ERROR_BUILDDATA);
+    }
+    KM_HIP_CHECK(hipMalloc(&dfd, sizeof(int) * (size_t)nblk * k), KMEANS_ERROR_BUILDDATA);
+  }
   return ensure_comp_slots(64);
 }
 
@@ -143,6 +158,7 @@ kmeansErrorStatus Engine::ensure_comp_slots(int nslots) {
 // kmeans.cpp:540-695 (buildData) without the row shuffle of :554-564.
 kmeansErrorStatus Engine::buildData(const float *data, int on_device) {
   if (!data) return KMEANS_ERROR_INVALID_ARGUMENT;
+  reuse_ok = false;  // new points: no chunk partials to reuse
   if (!on_device && opt.tile_rows > 0 && opt.tile_rows < m) return setup_streaming(data);
   if (on_device) {
     dX = data;
@@ -298,11 +314,35 @@ kmeansErrorStatus Engine::closestCenters(long r0, long rows) {
 
 kmeansErrorStatus Engine::clusterCenters(long r0, long rows, int *empty) {
   int *dempty = empty ? empty : this->dempty;
-  // a full update over all resident points leaves its sort by label in
-  // dorder for the next assign
-  const bool keep_order = dorder && r0 == 0 && rows == m;
-  ws.perm_out = keep_order ? dorder : nullptr;
-  ws2.perm_out = ws.perm_out;
+  ws.perm_out = nullptr;
+  ws2.perm_out = nullptr;
+  // a full update over all resident points sorts each canonical block into
+  // the other parity of the reuse state, reusing the chunk partials of the
+  // last full update wherever a centre's members did not change; its sort is
+  // the order of the next assign
+  const bool full = dperm[0] && r0 == 0 && rows == m;
+  std::vector<UpdateWs> views;
+  const int nw = reuse_cur ^ 1;
+  if (full) {
+    const long nblk = km_cdiv(m, KM_BLOCK);
+    const size_t ps = (size_t)reuse_qmax * (n + 1);
+    views.resize(nblk);
+    for (long b = 0; b < nblk; ++b) {
+      UpdateWs v = ((b & 1) && upd_side[0]) ? ws2 : ws;
+      v.perm_out = dperm[nw];
+      v.cstart = dsetup[nw] + (size_t)b * (3 * k + 1);
+      v.ccount = v.cstart + k;
+      v.cq = v.ccount + k;
+      v.P = dpart[nw] + b * ps;
+      v.fd = dfd + (size_t)b * k;
+      if (reuse_ok) {
+        v.perm_prev = dperm[reuse_cur];
+        v.cs_prev = dsetup[reuse_cur] + (size_t)b * (3 * k + 1);
+        v.P_prev = dpart[reuse_cur] + b * ps;
+      }
+      views[b] = v;
+    }
+  }
   if (rows > KM_BLOCK && upd_side[0]) {
     // several canonical blocks: sorts and chunk sums of alternate blocks on
     // two side streams, the in-order additions on the engine's stream
@@ -311,17 +351,24 @@ kmeansErrorStatus Engine::clusterCenters(long r0, long rows, int *empty) {
     KM_HIP_CHECK(launch_update_zero(stream, n, k, ws.S, ws.cnt, dempty),
                  KMEANS_ERROR_CLUSTER_CENTERS);
     KM_HIP_CHECK(launch_update_accumulate_2s(stream, ov, rows, n, k, dX + r0 * n, dlabels + r0,
-                                             ws.S, ws.cnt),
+                                             ws.S, ws.cnt, full ? views.data() : nullptr),
                  KMEANS_ERROR_CLUSTER_CENTERS);
     KM_HIP_CHECK(launch_update_finalize(stream, n, k, ws.S, ws.cnt, dC, dC, dCt, ldc, dnc, dcounts,
                                         dempty, ws.compj),
                  KMEANS_ERROR_CLUSTER_CENTERS);
   } else {
     KM_HIP_CHECK(launch_update(stream, rows, n, k, dX + r0 * n, dlabels + r0, dC, dC, dCt, ldc,
-                               dnc, dcounts, dempty, ws),
+                               dnc, dcounts, dempty, full ? views[0] : ws),
                  KMEANS_ERROR_CLUSTER_CENTERS);
   }
-  if (keep_order) order_ok = true;
+  if (full) {
+    reuse_cur = nw;
+    reuse_ok = true;
+    dorder = dperm[nw];
+    order_ok = true;
+  } else {
+    reuse_ok = false;
+  }
   if (opt.precision != KM_PREC_FP32) KM_HIP_CHECK(mfma_centers(this), KMEANS_ERROR_CLUSTER_CENTERS);
   return KMEANS_SUCCESS;
 }

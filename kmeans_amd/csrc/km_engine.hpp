@@ -45,9 +45,22 @@ struct Engine {
   hipEvent_t upd_ev[7] = {};
   char *dws2 = nullptr;
   UpdateWs ws2{};
-  // BF16/FP16 engines over resident points: the last full update's stable
-  // sort of the points by label (order_ok once written), the order in which
-  // the next assign takes them
+  // Resident points: the state of the last two full updates, so that an
+  // update can reuse the chunk partials of the members that did not move:
+  // dperm[p] (m ints: each canonical block's stable sort by label at its
+  // rows), dsetup[p] (per block [cstart k | ccount k | cq k+1]), dpart[p]
+  // (per block reuse_qmax x (n+1) chunk partials), dfd (per block k first
+  // differing positions).  Parity reuse_cur is the latest; reuse_ok when it
+  // describes a full update of the current points.
+  int *dperm[2] = {nullptr, nullptr};
+  int *dsetup[2] = {nullptr, nullptr};
+  double *dpart[2] = {nullptr, nullptr};
+  int *dfd = nullptr;
+  long reuse_qmax = 0;
+  int reuse_cur = 0;
+  bool reuse_ok = false;
+  // the last full update's sort (one of dperm; order_ok once written), the
+  // order in which the next BF16/FP16 assign takes the points
   int *dorder = nullptr;
   bool order_ok = false;
   // precision path buffers (allocated on first use)

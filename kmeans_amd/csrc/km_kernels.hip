@@ -595,15 +595,29 @@ __device__ inline double add_square(double s, double v) {
 // member range [start, start + cnt) in the sorted order.
 __device__ inline void chunk_range(long c, int K, const int *__restrict__ cstart,
                                    const int *__restrict__ ccount, const int *__restrict__ cq,
-                                   int &start, int &cnt) {
+                                   int &start, int &cnt, int &j, int &local) {
   int lo = 0, hi = K - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (cq[mid] <= c) lo = mid; else hi = mid - 1;
   }
-  const int local = (int)(c - cq[lo]);
+  j = lo;
+  local = (int)(c - cq[lo]);
   start = cstart[lo] + local * KM_CHUNK;
   cnt = min(KM_CHUNK, ccount[lo] - local * KM_CHUNK);
+}
+
+// Whether chunk `local` of centre j has the same members as the previous
+// sort's chunk `local` of j (fd: j's first differing member position, cs_prev
+// the previous [cstart | ccount | cq]), and if so that chunk's index there.
+__device__ inline long chunk_reuse(int j, int local, int cnt_new, int K, const int *__restrict__ fd,
+                                   const int *__restrict__ cs_prev) {
+  if (!fd) return -1;
+  const int q0 = local * KM_CHUNK;
+  const int end_new = q0 + cnt_new;
+  const int end_old = min(q0 + KM_CHUNK, cs_prev[K + j]);
+  if (end_new != end_old || end_new > fd[j] || end_new <= q0) return -1;
+  return (long)cs_prev[2 * K + j] + local;
 }
 
 // First-level sums: one group of Dp threads per chunk of <= KM_CHUNK members,
@@ -614,13 +628,18 @@ __device__ inline void chunk_range(long c, int K, const int *__restrict__ cstart
 // in ascending dimension order, so the squared norms come from the rows
 // already read.  Every thread reaches every barrier: the slice loop is
 // uniform over the block.
+// With fd (nullable), chunks whose members the previous sort had in the same
+// chunk copy that chunk's partials from P_prev instead (chunk_reuse).
 template <typename T>
 __global__ __launch_bounds__(256) void k_chunk_sums(const T *__restrict__ X, int D, int Dp,
                                                     const int *__restrict__ perm, int K,
                                                     const int *__restrict__ cstart,
                                                     const int *__restrict__ ccount,
                                                     const int *__restrict__ cq,
-                                                    double *__restrict__ P) {
+                                                    double *__restrict__ P,
+                                                    const int *__restrict__ fd,
+                                                    const int *__restrict__ cs_prev,
+                                                    const double *__restrict__ P_prev) {
   __shared__ int pidx[4][KM_CHUNK];
   __shared__ double sqs[256];
   const int G = min(256 / Dp, 64);  // chunks per block
@@ -629,18 +648,24 @@ __global__ __launch_bounds__(256) void k_chunk_sums(const T *__restrict__ X, int
   const long c = (long)blockIdx.x * G + sub;
   const bool act = sub < G && c < cq[K];
   const int ld = D + 1;
-  int cnt = 0, start = 0;
-  if (act) chunk_range(c, K, cstart, ccount, cq, start, cnt);
+  int cnt = 0, start = 0, j = 0, local = 0;
+  long src = -1;
+  if (act) {
+    chunk_range(c, K, cstart, ccount, cq, start, cnt, j, local);
+    src = chunk_reuse(j, local, cnt, K, fd, cs_prev);
+  }
+  const bool cp = src >= 0;  // copied, not computed
   // stage member indices (G <= 4 when Dp >= 64; otherwise read perm directly)
   const bool staged = G <= 4;
-  if (act && staged)
+  if (act && !cp && staged)
     for (int m = t; m < cnt; m += Dp) pidx[sub][m] = perm[start + m];
   __syncthreads();
   double qs = 0.0;
   for (int d0 = 0; d0 < D; d0 += Dp) {
     const int d = d0 + t;
     double acc = 0.0, sq = 0.0;
-    if (act && d < D) {
+    if (act && cp && d < D) P[c * ld + d] = P_prev[src * ld + d];
+    if (act && !cp && d < D) {
       int m = 0;
       for (; m + 4 <= cnt; m += 4) {
         const int p0 = staged ? pidx[sub][m] : perm[start + m];
@@ -666,13 +691,13 @@ __global__ __launch_bounds__(256) void k_chunk_sums(const T *__restrict__ X, int
     }
     sqs[threadIdx.x] = sq;
     __syncthreads();
-    if (act && t == 0) {
+    if (act && !cp && t == 0) {
       const int dn = min(Dp, D - d0);
       for (int u = 0; u < dn; ++u) qs += sqs[sub * Dp + u];
     }
     __syncthreads();
   }
-  if (act && t == 0) P[c * ld + D] = qs;
+  if (act && t == 0) P[c * ld + D] = cp ? P_prev[src * ld + D] : qs;
 }
 
 // The same first-level sums with four dimensions per thread (D % 4 == 0,
@@ -687,7 +712,10 @@ __global__ __launch_bounds__(256) void k_chunk_sums4(const float *__restrict__ X
                                                      const int *__restrict__ cstart,
                                                      const int *__restrict__ ccount,
                                                      const int *__restrict__ cq,
-                                                     double *__restrict__ P) {
+                                                     double *__restrict__ P,
+                                                     const int *__restrict__ fd,
+                                                     const int *__restrict__ cs_prev,
+                                                     const double *__restrict__ P_prev) {
   __shared__ int pidx[16][KM_CHUNK];
   __shared__ double sqs[1024];
   const int G = min(256 / TPC, 64);
@@ -696,13 +724,20 @@ __global__ __launch_bounds__(256) void k_chunk_sums4(const float *__restrict__ X
   const long c = (long)blockIdx.x * G + sub;
   const bool act = sub < G && c < cq[K];
   const int ld = D + 1;
-  int cnt = 0, start = 0;
-  if (act) chunk_range(c, K, cstart, ccount, cq, start, cnt);
+  int cnt = 0, start = 0, j = 0, local = 0;
+  long src = -1;
+  if (act) {
+    chunk_range(c, K, cstart, ccount, cq, start, cnt, j, local);
+    src = chunk_reuse(j, local, cnt, K, fd, cs_prev);
+  }
+  const bool cp = src >= 0;  // copied, not computed
   const bool staged = G <= 16;
-  if (act && staged)
+  if (act && !cp && staged)
     for (int m = t; m < cnt; m += TPC) pidx[sub][m] = perm[start + m];
   __syncthreads();
-  if (act && 4 * t < D) {
+  if (act && cp) {
+    for (int d = t; d <= D; d += TPC) P[c * ld + d] = P_prev[src * ld + d];
+  } else if (act && 4 * t < D) {
     const float4 *__restrict__ X4 = reinterpret_cast<const float4 *>(X);
     const long D4 = D / 4;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
@@ -738,10 +773,52 @@ __global__ __launch_bounds__(256) void k_chunk_sums4(const float *__restrict__ X
     sq[3] = s3;
   }
   __syncthreads();
-  if (act && t == 0) {
+  if (act && !cp && t == 0) {
     double qs = 0.0;
     for (int u = 0; u < D; ++u) qs += sqs[sub * D + u];
     P[c * ld + D] = qs;
+  }
+}
+
+// Reuse: fd[j] = the shorter of centre j's member counts in the two sorts
+// (0 without a previous sort), lowered by k_first_diff.
+__global__ void k_fd_init(int K, const int *__restrict__ ccount, const int *__restrict__ cs_prev,
+                          int *__restrict__ fd) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < K) fd[j] = cs_prev ? min(ccount[j], cs_prev[K + j]) : 0;
+}
+
+// fd[j] = min over the positions m of centre j's new member list whose
+// member differs from position m of the previous list (one thread per
+// position s of the block's new sort; its centre is the last j with
+// cstart[j] <= s).  Lanes of a wave with the same centre hold increasing
+// positions, so the first of them carries the minimum: one atomic per
+// centre per wave.
+__global__ __launch_bounds__(256) void k_first_diff(long nb, int K, const int *__restrict__ perm,
+                                                    const int *__restrict__ cstart,
+                                                    const int *__restrict__ perm_prev,
+                                                    const int *__restrict__ cs_prev,
+                                                    int *__restrict__ fd) {
+  const long s = (long)blockIdx.x * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  int j = 0, m = 0;
+  bool diff = false;
+  if (s < nb) {
+    int lo = 0, hi = K - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (cstart[mid] <= s) lo = mid; else hi = mid - 1;
+    }
+    j = lo;
+    m = (int)(s - cstart[j]);
+    if (m < cs_prev[K + j]) diff = perm_prev[cs_prev[j] + m] != perm[s];
+  }
+  uint64_t mk = __ballot(diff);
+  while (mk) {
+    const int lead = __ffsll((long long)mk) - 1;
+    const int jl = __shfl(j, lead);
+    if (lane == lead) atomicMin(fd + jl, m);
+    mk &= ~__ballot(diff && j == jl);
   }
 }
 
@@ -1129,19 +1206,27 @@ static hipError_t block_chunks(hipStream_t s, long i0, long nb, int d, int k, co
                                const UpdateWs &ws) {
   const int *perm = ws.perm_out ? ws.perm_out + i0 : ws.perm;
   const long qmax = nb / KM_CHUNK + k + 1;
+  const bool reuse = ws.fd && ws.perm_prev;
+  if (reuse) {  // each centre's first member position that differs from the previous sort
+    hipLaunchKernelGGL(k_fd_init, dim3(km_cdiv(k, 256)), dim3(256), 0, s, k, ws.ccount, ws.cs_prev,
+                       ws.fd);
+    hipLaunchKernelGGL(k_first_diff, dim3(km_cdiv(nb, 256)), dim3(256), 0, s, nb, k, perm,
+                       ws.cstart, ws.perm_prev + i0, ws.cs_prev, ws.fd);
+  }
+  const int *fd = reuse ? ws.fd : nullptr;
   if (sizeof(T) == 4 && d % 4 == 0 && d >= 16 && d <= 256 && ((uintptr_t)X & 15) == 0) {
     int TPC = 1;
     while (TPC < d / 4) TPC <<= 1;
     const int G4 = 256 / TPC < 64 ? 256 / TPC : 64;  // must match k_chunk_sums4
     hipLaunchKernelGGL(k_chunk_sums4, dim3(km_cdiv(qmax, G4)), dim3(256), 0, s,
                        reinterpret_cast<const float *>(X), d, TPC, perm, k, ws.cstart, ws.ccount,
-                       ws.cq, ws.P);
+                       ws.cq, ws.P, fd, ws.cs_prev, ws.P_prev);
   } else {
     int Dp = 1;
     while (Dp < d && Dp < 256) Dp <<= 1;
     const int G = 256 / Dp < 64 ? 256 / Dp : 64;  // must match k_chunk_sums
     hipLaunchKernelGGL(k_chunk_sums<T>, dim3(km_cdiv(qmax, G)), dim3(256), 0, s, X, d, Dp, perm,
-                       k, ws.cstart, ws.ccount, ws.cq, ws.P);
+                       k, ws.cstart, ws.ccount, ws.cq, ws.P, fd, ws.cs_prev, ws.P_prev);
   }
   return hipGetLastError();
 }
@@ -1188,7 +1273,7 @@ static hipError_t launch_update_accumulate_t(hipStream_t s, long n, int d, int k
 // two blocks later, after s has added the earlier block.
 hipError_t launch_update_accumulate_2s(hipStream_t s, const UpdateOverlap &ov, long n, int d,
                                        int k, const float *X, const int *labels, double *S,
-                                       long long *cnt) {
+                                       long long *cnt, const UpdateWs *blk) {
   hipError_t e;
   if ((e = hipEventRecord(ov.start, s)) != hipSuccess) return e;
   for (int i = 0; i < 2; ++i)
@@ -1200,7 +1285,8 @@ hipError_t launch_update_accumulate_2s(hipStream_t s, const UpdateOverlap &ov, l
       const int i = (int)(b & 1);
       const long i0 = b * KM_BLOCK, nb = std::min<long>(KM_BLOCK, n - i0);
       if (b >= 2 && (e = hipStreamWaitEvent(ov.side[i], ov.fin[i], 0)) != hipSuccess) return e;
-      if ((e = block_sort(ov.side[i], i0, nb, k, labels, *ov.ws[i])) != hipSuccess) return e;
+      if ((e = block_sort(ov.side[i], i0, nb, k, labels, blk ? blk[b] : *ov.ws[i])) != hipSuccess)
+        return e;
       if ((e = hipEventRecord(ov.sorted[i], ov.side[i])) != hipSuccess) return e;
     }
     for (long b = p; b < p + 2 && b < nblk; ++b) {
@@ -1208,13 +1294,14 @@ hipError_t launch_update_accumulate_2s(hipStream_t s, const UpdateOverlap &ov, l
       const long i0 = b * KM_BLOCK, nb = std::min<long>(KM_BLOCK, n - i0);
       if (pair && (e = hipStreamWaitEvent(ov.side[i], ov.sorted[i ^ 1], 0)) != hipSuccess)
         return e;
-      if ((e = block_chunks(ov.side[i], i0, nb, d, k, X, *ov.ws[i])) != hipSuccess) return e;
+      if ((e = block_chunks(ov.side[i], i0, nb, d, k, X, blk ? blk[b] : *ov.ws[i])) != hipSuccess)
+        return e;
       if ((e = hipEventRecord(ov.done[i], ov.side[i])) != hipSuccess) return e;
     }
     for (long b = p; b < p + 2 && b < nblk; ++b) {
       const int i = (int)(b & 1);
       if ((e = hipStreamWaitEvent(s, ov.done[i], 0)) != hipSuccess) return e;
-      if ((e = block_add(s, d, k, S, cnt, *ov.ws[i])) != hipSuccess) return e;
+      if ((e = block_add(s, d, k, S, cnt, blk ? blk[b] : *ov.ws[i])) != hipSuccess) return e;
       if ((e = hipEventRecord(ov.fin[i], s)) != hipSuccess) return e;
     }
   }

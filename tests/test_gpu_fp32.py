@@ -266,6 +266,76 @@ def test_engine_update_overlaps_four_blocks(orc):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_update_reuses_unchanged_chunks(orc, prec):
+    # every full update after the first reuses the chunk partials of the
+    # centres' unchanged member prefixes (three canonical blocks, the last
+    # short); labels, centres and compactness after every run length
+    n, d, k = 2 * km.KM_BLOCK + 12_345, 8, 40
+    X, _, _, _ = orc.mixture(21, n, d, k, -10, 10, 1.0, 1.0)
+    C0 = X[:k].copy()
+    e = km.KMeans(n, d, k, precision=prec)
+    e.set_data(X)
+    e.set_centers(C0)
+    done = 0
+    for steps in (1, 1, 2, 3):
+        e.run(steps)
+        done += steps
+        ref = orc.lloyd(X, C0, done)
+        assert np.array_equal(e.labels(), ref["labels"]), done
+        assert np.array_equal(e.centers(), ref["centers"]), done
+        assert e.stats()["compactness"] == ref["comp_hist"][-1], done
+    e.close()
+
+
+def test_update_reuse_not_across_data_sets(orc):
+    # new points of the same shape: nothing of the previous data's partials
+    # may be reused (one and two canonical blocks)
+    for n in (150_000, km.KM_BLOCK + 1000):
+        d, k = 8, 30
+        X1, _, _, _ = orc.mixture(22, n, d, k, -10, 10, 1.0, 1.0)
+        X2, _, _, _ = orc.mixture(23, n, d, k, -10, 10, 1.0, 1.0)
+        e = km.KMeans(n, d, k)
+        e.set_data(X1)
+        e.set_centers(X1[:k].copy())
+        e.run(3)
+        e.set_data(X2)
+        e.set_centers(X2[:k].copy())
+        e.run(3)
+        ref = orc.lloyd(X2, X2[:k].copy(), 3)
+        assert np.array_equal(e.labels(), ref["labels"]), n
+        assert np.array_equal(e.centers(), ref["centers"]), n
+        e.close()
+
+
+def test_update_reuse_after_mini_batch_windows(orc):
+    # mini-batch windows update partial row ranges; the full updates after
+    # them must not reuse partials from before them: the result equals a fresh
+    # engine started from the centres the windows left
+    n, d, k = 300_000, 8, 30
+    X, _, _, _ = orc.mixture(24, n, d, k, -10, 10, 1.0, 1.0)
+    C0 = X[:k].copy()
+    e = km.KMeans(n, d, k)
+    e.set_data(X)
+    e.set_centers(C0)
+    e.run(2)
+    e.set_mini_batch(100_000)
+    e.run(2)
+    C_mb = e.centers()
+    e.set_mini_batch(0)
+    e.run(2)
+    f = km.KMeans(n, d, k)
+    f.set_data(X)
+    f.set_centers(C_mb)
+    f.run(2)
+    assert np.array_equal(e.labels(), f.labels())
+    assert np.array_equal(e.centers(), f.centers())
+    ref = orc.lloyd(X, C_mb, 2)
+    assert np.array_equal(e.centers(), ref["centers"])
+    e.close()
+    f.close()
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
 def test_engine_criterion_stops_like_reference(orc, prec):
     # the convergence test runs on the device: the iterations enqueued past
     # the converged one must leave its labels, centres and compactness as
