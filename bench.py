@@ -117,7 +117,17 @@ def main():
     value = pairs * args.steps / dt
     assign_ms = st["ms_assign"] / args.steps
     flops = 2.0 * d * pairs
-    achieved = flops / (assign_ms * 1e-3)
+    if args.precision != "fp32" and st["assign_launches"]:
+        # the reduced-precision assign kernel itself: flops of the points it
+        # scored (each against all K centres) over its own event-timed
+        # launches; the drift bounds and the pair test leave the rest unscored
+        kernel_ms = st["ms_assign_kernel"] / st["assign_launches"]
+        kernel_flops = 2.0 * d * k * st["scored"] / st["assign_launches"]
+    else:
+        # the FP32 assign phase is the one kernel, over every point
+        kernel_ms, kernel_flops = assign_ms, flops
+    achieved = kernel_flops / (kernel_ms * 1e-3)
+    prof = measured_profile(args.workload, args.precision, args.n)
     out = {
         "metric": "assign pts*ctrs/s per Lloyd iter",
         "value": value,
@@ -151,12 +161,24 @@ def main():
         "roofline": {"bound": "mfma",
                      "achieved": achieved / 1e12, "peak": PEAK[args.precision] / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK[args.precision],
-                     "traffic": (None if args.full_gemm else
-                                 measured_traffic(args.workload, args.precision, args.n)),
+                     "traffic": None if args.full_gemm or prof is None else prof["bytes_per_launch"],
+                     "mfma_busy": None if args.full_gemm or prof is None else prof.get("mfma_util"),
                      "kernel": ("rocBLAS SGEMM + k_row_transform_min" if args.full_gemm else
                                 "k_assign_mfma" if args.precision != "fp32" else "k_assign_f32_mfma"),
-                     "algorithmic": "2*D*N*K flops per launch; achieved over the assign phase's "
-                                    "event-timed duration"},
+                     "launches": st["assign_launches"] if args.precision != "fp32" else args.steps,
+                     "kernel_ms_per_launch": kernel_ms,
+                     "flops_per_launch": kernel_flops,
+                     "algorithmic": ("2*D*K flops per point the kernel scores (against all K "
+                                     "centres), averaged over its launches in the timed "
+                                     "iterations, over its own HIP-event-timed durations on "
+                                     "the engine's stream" if args.precision != "fp32" else
+                                     "2*D*N*K flops per launch over the assign phase's "
+                                     "event-timed duration (the phase is this one kernel)"),
+                     "effective_dense": {
+                         "achieved": flops / (assign_ms * 1e-3) / 1e12, "unit": "TFLOP/s",
+                         "frac": flops / (assign_ms * 1e-3) / PEAK[args.precision],
+                         "note": "dense equivalent: 2*D*N*K per iteration over the whole assign "
+                                 "phase (drift-bound filter, pair decisions and the kernel)"}},
     }
     labels = eng.labels()
     centers = eng.centers()
@@ -326,10 +348,13 @@ def bench_streamed(args, X, C0, spec, km, np, torch):
     }
 
 
-def measured_traffic(workload, precision, n_override):
-    """HBM bytes per launch of the dominant kernel, from the committed PMC
-    summary (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE runs of this
-    workload, scaled by the calibration copy), or None when not measured."""
+def measured_profile(workload, precision, n_override):
+    """Counters of the dominant kernel from the committed PMC summary
+    (profiles/traffic.json): HBM bytes per launch (FETCH_SIZE and WRITE_SIZE
+    runs of this workload, scaled by the calibration copy) and the MFMA
+    utilisation (SQ_VALU_MFMA_BUSY_CYCLES over the SIMD-cycles of the launch),
+    averaged over the launches of the timed iterations; None when not
+    measured."""
     if n_override is not None:
         return None
     path = os.path.join(ROOT, "profiles", "traffic.json")
@@ -338,8 +363,7 @@ def measured_traffic(workload, precision, n_override):
     except (OSError, ValueError):
         return None
     key = f"{workload}:{precision}:" + ("k_assign_mfma" if precision != "fp32" else "k_assign_f32_mfma")
-    ent = tab.get(key)
-    return None if ent is None else ent["bytes_per_launch"]
+    return tab.get(key)
 
 
 def bench_batch(args, km, datagen, np, torch):

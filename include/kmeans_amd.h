@@ -195,6 +195,10 @@ typedef struct kmeansStats {
   long long paired;           /* ... decided between the label and the runner-up
                                  by their two FP32 chains, the drift bounds
                                  having separated every other centre          */
+  long long scored;           /* points the BF16/FP16 assign kernel scored
+                                 against every centre                         */
+  long long assign_launches;  /* launches of that kernel (timing == 1)        */
+  double ms_assign_kernel;    /* their summed event-timed durations           */
 } kmeansStats;
 
 typedef struct kmeansEngine *kmeansHandle;

@@ -67,7 +67,9 @@ class Stats(ctypes.Structure):
                 ("ms_train", ctypes.c_double), ("train_loss_first", ctypes.c_double),
                 ("train_loss_last", ctypes.c_double), ("deep", ctypes.c_longlong),
                 ("tiles", ctypes.c_longlong), ("tiles_total", ctypes.c_longlong),
-                ("bounded", ctypes.c_longlong), ("paired", ctypes.c_longlong)]
+                ("bounded", ctypes.c_longlong), ("paired", ctypes.c_longlong),
+                ("scored", ctypes.c_longlong), ("assign_launches", ctypes.c_longlong),
+                ("ms_assign_kernel", ctypes.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

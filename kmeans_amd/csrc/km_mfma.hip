@@ -109,9 +109,18 @@ struct MfmaState {
   int nflag_slots = 0;
   bool x_ready = false;
   bool c_ready = false;
+  // timing runs: start/stop events of each assign launch not yet collected
+  std::vector<hipEvent_t> kev;
+  size_t kev_used = 0;
 };
 
 static MfmaState *st(Engine *e) { return (MfmaState *)e->mfma; }
+static void collect_kernel_times(Engine *e, MfmaState *s);
+static hipError_t timing_begin(Engine *e, MfmaState *s, hipEvent_t *k1);
+static hipError_t timing_end(Engine *e, hipEvent_t k1);
+static hipError_t launch_assign(Engine *e, const float *X, const uint16_t *Xh, const float2 *xn,
+                                int *lab, long rows, const int *order, const int *home,
+                                const int *Mdev, float4 *xbo);
 
 // ------------------------------------------------------------------ kernels
 __global__ void k_convert_rows(long m, int d, int Dp, const float *__restrict__ X, float scale,
@@ -250,7 +259,10 @@ __global__ __launch_bounds__(256) void k_tile_order(int K, int D, int nKT,
 // drift[j] = upper bound of |C_j - Cref_j| (the FP32 differences are within
 // an ulp, the sum of squares within gamma_D in any order), one wave per
 // centre; drift[K] (as bits) += the largest by atomic maximum over
-// non-negative floats, from 0 (k_ref_copy clears it for the next pass)
+// non-negative floats, from 0 (k_ref_copy clears it for the next pass).
+// drift[j] is 0 exactly when every coordinate difference is 0.  When the
+// largest difference is below 2^-40, squares may underflow, and
+// sqrt(D) * max |difference| bounds the drift instead.
 __global__ __launch_bounds__(256) void k_drift(int K, int D, const float *__restrict__ C,
                                                const float *__restrict__ Cref,
                                                float *__restrict__ drift,
@@ -258,15 +270,25 @@ __global__ __launch_bounds__(256) void k_drift(int K, int D, const float *__rest
   if (stop && *stop) return;
   const int j = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (j >= K) return;
-  float q = 0.0f;
+  float q = 0.0f, ma = 0.0f;
   for (int d = lane; d < D; d += 64) {
     const float t = C[(long)j * D + d] - Cref[(long)j * D + d];
     q = __builtin_fmaf(t, t, q);
+    const float at = fabsf(t);
+    if (!(at <= ma)) ma = at;  // NaN propagates
   }
 #pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) q += __shfl_xor(q, o);
+  for (int o = 32; o >= 1; o >>= 1) {
+    q += __shfl_xor(q, o);
+    const float om = __shfl_xor(ma, o);
+    if (!(om <= ma)) ma = om;
+  }
   if (lane == 0) {
-    const float v = q > 0.0f ? sqrtf(q * (1.0f + (float)D * 1.2e-7f + 4e-7f)) * (1.0f + 1e-6f) : 0.0f;
+    float v = 0.0f;
+    if (!(ma == 0.0f))
+      v = ma >= 9.094947e-13f  // 2^-40
+              ? sqrtf(q * (1.0f + (float)D * 1.2e-7f + 4e-7f)) * (1.0f + 1e-6f)
+              : sqrtf((float)D) * ma * (1.0f + 1e-6f) + 1e-45f;
     drift[j] = v;
     atomicMax(reinterpret_cast<unsigned *>(drift + K), __float_as_uint(v));
   }
@@ -325,8 +347,11 @@ __device__ inline void chain2(const float *__restrict__ xr, const float *__restr
 // way (l3'^2 - u'^2 > 4 E32: the FP32 chain of a then exceeds every other
 // centre's but possibly r's, so the label is a or r, whichever chain is
 // larger, lowest index on equal chains), 2 when the point must be scored.
-// nb = the moved bounds, e32 = the FP32 chains' error bound for this point.
-// bnd = the centre bounds [cmax, ncmax, ...] of the current centres.
+// When a and r have not moved at all (drift 0: every coordinate equal, so
+// both chains are the ones the last assign compared, which put a first) the
+// second case keeps the label too.  nb = the moved bounds, e32 = the FP32
+// chains' error bound for this point.  bnd = the centre bounds [cmax, ncmax,
+// ...] of the current centres.
 __device__ inline int bound_class(long q, const int *__restrict__ order,
                                   const int *__restrict__ labels, const float4 *__restrict__ xb,
                                   const float *__restrict__ drift, int K, const float *bnd,
@@ -344,7 +369,8 @@ __device__ inline int bound_class(long q, const int *__restrict__ order,
   const float t = 4.0f * e32 * 1.0001f + 1e-30f;
   if (l > u && (l - u) * (l + u) * (1.0f - 1e-6f) > t) return 0;
   const int r = __float_as_int(b.w);
-  if (r >= 0 && r < K && r != a && l3 > u && (l3 - u) * (l3 + u) * (1.0f - 1e-6f) > t) return 1;
+  if (r >= 0 && r < K && r != a && l3 > u && (l3 - u) * (l3 + u) * (1.0f - 1e-6f) > t)
+    return drift[a] == 0.0f && drift[r] == 0.0f ? 0 : 1;
   return 2;
 }
 
@@ -640,8 +666,11 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_assign_mfma(
     const long Mh = M;
     M = min(M, (long)*Mdev);
     if (blockIdx.x == 0 && threadIdx.x == 0) ncand[3] = (int)(Mh - M);
-    if ((long)blockIdx.x * MF_PTS >= M) return;  // uniform over the workgroup
   }
+  // the points this launch scores, into slot 5 (one launch per tile of
+  // streamed points adds into the same iteration's counters)
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(ncand + 5, (int)M);
+  if (Mdev && (long)blockIdx.x * MF_PTS >= M) return;  // uniform over the workgroup
   constexpr int Dp = KS * 16;
   constexpr int CPR = Dp / 8;               // 16-byte chunks per centre row
   constexpr int SROWS = MF_STAGE * MF_TILE; // centre rows per LDS stage
@@ -1413,6 +1442,7 @@ void mfma_free(Engine *e) {
   void *bb[] = {s->xb, s->Cref, s->drift, s->ulist, s->bcnt, s->bofs, s->btsum};
   for (void *b : bb) hipFree(b);
   hipFree(s->home);
+  for (hipEvent_t ev : s->kev) hipEventDestroy(ev);
   delete s;
   e->mfma = nullptr;
 }
@@ -1703,6 +1733,18 @@ static hipError_t ensure_bounds(Engine *e) {
   return hipMemsetAsync(s->bofs, 0, sizeof(int) * (size_t)(2 * nblk + 1), e->stream);
 }
 
+// After an assign over all resident points every point's bounds hold for the
+// current centres (fresh ones for the assigned points, moved ones for the
+// others): they become the reference centres.
+static hipError_t ref_copy(Engine *e, MfmaState *s, long rows) {
+  (void)rows;
+  const long nc = (long)e->k * e->n;
+  hipLaunchKernelGGL(k_ref_copy, dim3(km_cdiv(nc, 256)), dim3(256), 0, e->stream, nc, e->dC,
+                     s->Cref, s->drift + e->k, e->assign_stop);
+  s->bnd_valid = true;
+  return hipGetLastError();
+}
+
 hipError_t mfma_assign(Engine *e, long r0, long rows) {
   hipError_t r;
   if ((r = ensure_state(e)) != hipSuccess) return r;
@@ -1768,17 +1810,61 @@ hipError_t mfma_assign(Engine *e, long r0, long rows) {
                             false)) !=
       hipSuccess)
     return r;
-  // every point's bounds now hold for the current centres (fresh ones for the
-  // assigned points, moved ones for the others); a partial assign leaves them
-  // unusable
-  if (all) {
-    const long nc = (long)e->k * e->n;
-    hipLaunchKernelGGL(k_ref_copy, dim3(km_cdiv(nc, 256)), dim3(256), 0, e->stream, nc, e->dC,
-                       s->Cref, s->drift + e->k, e->assign_stop);
-    if ((r = hipGetLastError()) != hipSuccess) return r;
+  if (!all) {
+    s->bnd_valid = false;  // a partial assign leaves the bounds unusable
+    return hipSuccess;
   }
-  s->bnd_valid = all;
-  return hipSuccess;
+  return ref_copy(e, s, rows);
+}
+
+// Timing runs: an event before the assign kernel(s) of one call, and the
+// matching one after them, on the stream they run on (collected by
+// collect_kernel_times).
+static hipError_t timing_begin(Engine *e, MfmaState *s, hipEvent_t *k1) {
+  *k1 = nullptr;
+  if (!e->opt.timing) return hipSuccess;
+  if (s->kev_used >= 4096) collect_kernel_times(e, s);  // step calls outside a run
+  hipError_t r;
+  while (s->kev.size() < s->kev_used + 2) {
+    hipEvent_t ev = nullptr;
+    if ((r = hipEventCreate(&ev)) != hipSuccess) return r;
+    s->kev.push_back(ev);
+  }
+  *k1 = s->kev[s->kev_used + 1];
+  r = hipEventRecord(s->kev[s->kev_used], e->stream);
+  s->kev_used += 2;
+  return r;
+}
+static hipError_t timing_end(Engine *e, hipEvent_t k1) {
+  return k1 ? hipEventRecord(k1, e->stream) : hipSuccess;
+}
+
+// One launch of the assign kernel (see mfma_assign_view).
+static hipError_t launch_assign(Engine *e, const float *X, const uint16_t *Xh, const float2 *xn,
+                                int *lab, long rows, const int *order, const int *home,
+                                const int *Mdev, float4 *xbo) {
+  MfmaState *s = st(e);
+  int *nf = s->nflag + e->cur_slot;
+  int *nc2 = s->ncand + (size_t)e->cur_slot * NC_SLOT;
+  const float eta = s->fp16 ? 2.9802322e-8f / s->scale : 0.0f;  // half FP16 subnormal step
+  const float inv_s2 = 1.0f / (s->scale * s->scale);
+  dim3 g(km_cdiv(rows, MF_PTS));
+  const int *tt = home ? s->ttab : nullptr;
+  const float *td = home ? s->tdist : nullptr;
+#define KM_LAUNCH(KS)                                                                              \
+  launch_mfma_ks<KS>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, \
+                     e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop,      \
+                     order, tt, td, home, Mdev, xbo)
+  switch (s->KS) {
+    case 1: KM_LAUNCH(1); break;
+    case 2: KM_LAUNCH(2); break;
+    case 4: KM_LAUNCH(4); break;
+    case 6: KM_LAUNCH(6); break;
+    case 8: KM_LAUNCH(8); break;
+    default: KM_LAUNCH(16); break;
+  }
+#undef KM_LAUNCH
+  return hipGetLastError();
 }
 
 // Assignment of `rows` points: FP32 rows X, their reduced-precision copy Xh
@@ -1794,26 +1880,19 @@ hipError_t mfma_assign_view(Engine *e, const float *X, const uint16_t *Xh, const
   if (!s->c_ready)
     if ((r = prepare_centres(e)) != hipSuccess) return r;
   if ((r = mfma_ensure_slots(e, e->cur_slot + 1)) != hipSuccess) return r;
-  int *nf = s->nflag + e->cur_slot;
-  int *nc2 = s->ncand + (size_t)e->cur_slot * NC_SLOT;
   if (zero_counters) {
-    if ((r = hipMemsetAsync(nf, 0, sizeof(int), e->stream)) != hipSuccess) return r;
-    if ((r = hipMemsetAsync(nc2, 0, sizeof(int) * NC_SLOT, e->stream)) != hipSuccess) return r;
+    if ((r = hipMemsetAsync(s->nflag + e->cur_slot, 0, sizeof(int), e->stream)) != hipSuccess)
+      return r;
+    if ((r = hipMemsetAsync(s->ncand + (size_t)e->cur_slot * NC_SLOT, 0, sizeof(int) * NC_SLOT,
+                            e->stream)) != hipSuccess)
+      return r;
   }
   if (rows <= 0) return hipSuccess;
-  const float eta = s->fp16 ? 2.9802322e-8f / s->scale : 0.0f;  // half FP16 subnormal step
-  const float inv_s2 = 1.0f / (s->scale * s->scale);
-  dim3 g(km_cdiv(rows, MF_PTS));
-  switch (s->KS) {
-    case 1: launch_mfma_ks<1>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, home ? s->ttab : nullptr, home ? s->tdist : nullptr, home, Mdev, xbo); break;
-    case 2: launch_mfma_ks<2>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, home ? s->ttab : nullptr, home ? s->tdist : nullptr, home, Mdev, xbo); break;
-    case 4: launch_mfma_ks<4>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, home ? s->ttab : nullptr, home ? s->tdist : nullptr, home, Mdev, xbo); break;
-    case 6: launch_mfma_ks<6>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, home ? s->ttab : nullptr, home ? s->tdist : nullptr, home, Mdev, xbo); break;
-    case 8: launch_mfma_ks<8>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, home ? s->ttab : nullptr, home ? s->tdist : nullptr, home, Mdev, xbo); break;
-    default: launch_mfma_ks<16>(e->stream, s->fp16, g, rows, Xh, X, e->n, e->k, s->Kp, s->Ch, s->bias, e->dC, e->dnc, s->bnd, xn, inv_s2, eta, lab, s->flags, nf, nc2, e->assign_stop, order, home ? s->ttab : nullptr, home ? s->tdist : nullptr, home, Mdev, xbo); break;
-  }
-  if ((r = hipGetLastError()) != hipSuccess) return r;
-  return hipGetLastError();
+  hipEvent_t k1;
+  if ((r = timing_begin(e, s, &k1)) != hipSuccess) return r;
+  if ((r = launch_assign(e, X, Xh, xn, lab, rows, order, home, Mdev, xbo)) != hipSuccess)
+    return r;
+  return timing_end(e, k1);
 }
 
 hipError_t mfma_ensure_slots(Engine *e, int n) {
@@ -1905,9 +1984,25 @@ bool mfma_view(Engine *e, MfmaView *v) {
 }
 bool mfma_ready(Engine *e) { return st(e) && st(e)->x_ready && !e->streaming; }
 
+// Assign kernel durations recorded so far (timing) into the stats.
+static void collect_kernel_times(Engine *e, MfmaState *s) {
+  if (!s->kev_used) return;
+  if (hipStreamSynchronize(e->stream) == hipSuccess)
+    for (size_t i = 0; i + 1 < s->kev_used; i += 2) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, s->kev[i], s->kev[i + 1]) == hipSuccess) {
+        e->stats.ms_assign_kernel += ms;
+        e->stats.assign_launches += 1;
+      }
+    }
+  s->kev_used = 0;
+}
+
 void mfma_collect(Engine *e, int nslots) {
   MfmaState *s = st(e);
-  if (!s || nslots <= 0) return;
+  if (!s) return;
+  collect_kernel_times(e, s);
+  if (nslots <= 0) return;
   std::vector<int> v(nslots);
   const int n = nslots < s->nflag_slots ? nslots : s->nflag_slots;
   if (hipMemcpyAsync(v.data(), s->nflag2, sizeof(int) * n, hipMemcpyDeviceToHost, e->stream) !=
@@ -1937,6 +2032,7 @@ void mfma_collect(Engine *e, int nslots) {
       e->stats.bounded += w[(size_t)i * NC_SLOT + c * NC_STRIDE + 3] -
                           w[(size_t)i * NC_SLOT + c * NC_STRIDE + 4];
       e->stats.paired += w[(size_t)i * NC_SLOT + c * NC_STRIDE + 4];
+      e->stats.scored += w[(size_t)i * NC_SLOT + c * NC_STRIDE + 5];
       e->stats.tiles_total += w[(size_t)i * NC_SLOT + c * NC_STRIDE + 2];
     }
 }

@@ -363,18 +363,24 @@ def pair_fixed_point_data(m=1 << 10, d=16, seed=7):
 @pytest.mark.parametrize("prec", ["bf16", "fp16"])
 def test_mfma_pair_decisions_at_a_fixed_point(orc, prec):
     # The origin points are exact ties between a and b and the points at
-    # x0 = +-2^-12 near-ties; every other centre is >= 8 away, so once the
-    # bounds exist these points are decided by the two FP32 chains in the
-    # drift-bound filter, and the ties must go to the lower index.
+    # x0 = +-2^-12 near-ties; every other centre is >= 8 away.  The run starts
+    # with both centres of each pair shifted by 1/4 in x1, which keeps every
+    # tie and every label, so the first update moves them onto the fixed
+    # point: the second assign decides these points by the two FP32 chains in
+    # the drift-bound filter (the ties to the lower index), and later ones,
+    # with the centres unmoved, keep them.
     X, C0 = pair_fixed_point_data()
     n = X.shape[0]
-    ref = orc.lloyd(X, C0, 6)
-    assert np.array_equal(ref["centers"], C0)  # a fixed point
+    Cs = C0.copy()
+    Cs[:, 1] += 0.25
+    ref = orc.lloyd(X, Cs, 6)
+    assert np.array_equal(ref["centers"], C0)  # one update reaches the fixed point
+    assert np.array_equal(orc.lloyd(X, Cs, 1)["centers"], C0)
     e = km.KMeans(n, X.shape[1], 4, precision=prec)
     e.set_data(X)
-    e.set_centers(C0)
+    e.set_centers(Cs)
     done = 0
-    for steps in (2, 2, 2):
+    for steps in (1, 1, 2, 2):
         e.run(steps)
         done += steps
         assert np.array_equal(e.labels(), ref["labels"]), done
@@ -382,4 +388,5 @@ def test_mfma_pair_decisions_at_a_fixed_point(orc, prec):
     st = e.stats()
     print(f"{prec}: paired {st['paired']} bounded {st['bounded']} of {done * n}")
     assert st["paired"] > 0
+    assert st["bounded"] > 0
     e.close()
