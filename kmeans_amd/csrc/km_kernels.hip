@@ -792,8 +792,9 @@ __global__ void k_fd_init(int K, const int *__restrict__ ccount, const int *__re
 // member differs from position m of the previous list (one thread per
 // position s of the block's new sort; its centre is the last j with
 // cstart[j] <= s).  Lanes of a wave with the same centre hold increasing
-// positions, so the first of them carries the minimum: one atomic per
-// centre per wave.
+// positions, so the first of them carries the minimum: at most one atomic
+// per centre per wave, and none when fd[j] is already at or below it (every
+// position after a centre's first change differs, so most waves skip).
 __global__ __launch_bounds__(256) void k_first_diff(long nb, int K, const int *__restrict__ perm,
                                                     const int *__restrict__ cstart,
                                                     const int *__restrict__ perm_prev,
@@ -817,7 +818,7 @@ __global__ __launch_bounds__(256) void k_first_diff(long nb, int K, const int *_
   while (mk) {
     const int lead = __ffsll((long long)mk) - 1;
     const int jl = __shfl(j, lead);
-    if (lane == lead) atomicMin(fd + jl, m);
+    if (lane == lead && m < __builtin_nontemporal_load(fd + jl)) atomicMin(fd + jl, m);
     mk &= ~__ballot(diff && j == jl);
   }
 }
