@@ -385,12 +385,28 @@ def bench_batch(args, km, datagen, np, torch):
     if args.warmup:
         b.run(args.warmup)  # also autotunes the tile size
         b.init_first_k()
+    st0 = b.stats()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     b.run(steps)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     st = b.stats()
+    for f in ("kept", "paired", "scored", "exact_searched"):  # the timed run's own counts
+        st[f] -= st0[f]
+    # the same trajectory once more on an engine with timing on (enqueued
+    # eagerly, HIP events around every assign and update launch) for the
+    # kernels' own durations
+    bt = km.KMeansBatch(n, d, k, precision=prec, timing=True)
+    bt.set_data(device_ptr=X.data_ptr())
+    bt.set_tile_rows(st["tile_rows"])
+    bt.init_first_k()
+    bt.run(1)  # first touch of the engine's buffers, as the warm-up did for the graph run
+    bt.init_first_k()
+    bt.run(steps)
+    stt = bt.stats()
+    bt.close()
+    st["ms_assign"], st["ms_update"] = stt["ms_assign"], stt["ms_update"]
     pairs = float(total) * k
     out = {
         "metric": "assign pts*ctrs/s per Lloyd iter",
@@ -406,14 +422,36 @@ def bench_batch(args, km, datagen, np, torch):
                                f"{prec} assign, one hipGraph per run",
                    "instances": nb, "D": d, "K": k, "sum_N": total, "precision": prec},
         "graph_ms_per_step": st["ms_graph"] / steps,
+        "phases_ms_per_step": {"assign": st["ms_assign"] / steps, "update": st["ms_update"] / steps,
+                               "note": "kernel durations from a second, eagerly enqueued run of "
+                                       "the same iterations with HIP events"},
         "batch_stats": st,
-        "roofline": {"bound": "fp32" if prec == "fp32" else "mfma",
-                     "achieved": 2.0 * d * pairs / (st["ms_graph"] / steps * 1e-3) / 1e12,
-                     "peak": PEAK[prec] / 1e12, "unit": "TFLOP/s",
-                     "frac": 2.0 * d * pairs / (st["ms_graph"] / steps * 1e-3) / PEAK[prec],
-                     "traffic": None, "kernel": "whole iteration (assign + update)",
-                     "algorithmic": "2*D*sum(N)*K flops per iteration"},
     }
+    # the dominant kernel, the batched assign, on the HBM roofline (D=16 puts
+    # its intensity far below the BF16 ridge): the bytes it must move per
+    # launch -- the FP32 rows of the points it scores or decides between two
+    # candidates, and per point its label and drift bounds (read, and written
+    # back) -- over its event-timed launches in the graph
+    b0 = total * (4 + 16 + 16)
+    rows_read = st["scored"] + st["paired"]
+    if prec == "fp32":
+        kbytes = (4.0 * d + 4.0) * total * steps
+    else:
+        kbytes = float(b0) * steps + 4.0 * d * rows_read
+    out["roofline"] = {
+        "bound": "hbm", "achieved": kbytes / (st["ms_assign"] * 1e-3) / 1e9, "peak": 8000.0,
+        "unit": "GB/s", "frac": kbytes / (st["ms_assign"] * 1e-3) / 8e12, "traffic": None,
+        "kernel": "k_batch_assign_bf16" if prec != "fp32" else "k_batch_assign_f32x",
+        "kernel_ms_per_launch": st["ms_assign"] / steps,
+        "bytes_per_launch": kbytes / steps,
+        "algorithmic": ("per point 36 B (label, drift bounds read and written) plus 4*D B of FP32 "
+                        "row for each point scored or decided between two candidates (counts from "
+                        "the stats of the timed run); over the assign launches' durations (HIP "
+                        "events around each launch in an eagerly enqueued rerun of the same "
+                        "iterations)" if prec != "fp32" else
+                        "(4*D + 4) B per point per launch (rows read, labels written)"),
+        "mfma_flops_per_s": 2.0 * d * k * (st["scored"] if prec != "fp32" else total * steps) /
+                            (st["ms_assign"] * 1e-3)}
     labels = b.labels()
     centers_b = b.centers()
     b.close()

@@ -303,6 +303,14 @@ typedef struct kmeansBatchStats {
   float tune_ms[5];        /* assign time per candidate tile: 256 .. 4096     */
   long long exact_searched; /* BF16: point assignments certified by no bound
                                and searched exactly (summed over iterations) */
+  long long kept;          /* BF16: point assignments kept by the drift bounds */
+  long long paired;        /* BF16: ... decided between the label and the
+                              runner-up by their two FP32 chains             */
+  long long scored;        /* BF16: ... scored against every centre          */
+  double ms_assign;        /* options.timing == 1 (the run is then enqueued
+                              eagerly, not as a graph): the last run's assign
+                              launches, summed over iterations (HIP events)  */
+  double ms_update;        /* ... and its update launches                     */
 } kmeansBatchStats;
 
 typedef struct kmeansBatchEngine *kmeansBatchHandle;

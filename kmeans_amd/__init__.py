@@ -332,7 +332,9 @@ class BatchStats(ctypes.Structure):
                 ("graph_nodes", ctypes.c_int), ("tuned_from_cache", ctypes.c_int),
                 ("empty_clusters", ctypes.c_longlong), ("ms_graph", ctypes.c_double),
                 ("ms_capture", ctypes.c_double), ("tune_ms", ctypes.c_float * 5),
-                ("exact_searched", ctypes.c_longlong)]
+                ("exact_searched", ctypes.c_longlong), ("kept", ctypes.c_longlong),
+                ("paired", ctypes.c_longlong), ("scored", ctypes.c_longlong),
+                ("ms_assign", ctypes.c_double), ("ms_update", ctypes.c_double)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_}
@@ -344,11 +346,11 @@ class KMeansBatch:
     """Many independent clusterings of uneven size (kmeansBatch*): instance i
     owns n[i] consecutive rows of the data and k centres."""
 
-    def __init__(self, n, d, k, precision="fp32", device=0, stream=None):
+    def __init__(self, n, d, k, precision="fp32", device=0, stream=None, timing=False):
         self.n = np.ascontiguousarray(n, dtype=np.int64)
         self.nb, self.d, self.k = len(self.n), int(d), int(k)
         self.total = int(self.n.sum())
-        opt = Options(PREC[precision], device, stream, 0, 0)
+        opt = Options(PREC[precision], device, stream, 0, 1 if timing else 0)
         h = ctypes.c_void_p()
         _check(lib().kmeansBatchCreate(ctypes.byref(h), self.nb, _np_ptr(self.n), self.d, self.k,
                                        ctypes.byref(opt)), "kmeansBatchCreate")

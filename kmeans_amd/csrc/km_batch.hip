@@ -37,6 +37,7 @@ namespace km {
 
 #define B_MAXK 1024
 #define B_MAXD 64
+#define B_MAXP 4096  // largest points per assign work item
 
 struct Batch {
   int nb = 0, d = 0, k = 0;
@@ -52,24 +53,45 @@ struct Batch {
   int *nvec = nullptr;
   float *C = nullptr, *nc = nullptr;
   float2 *bnd = nullptr;  // per instance: max ||c||^2 and max ||c - bf16(c)||^2 (raw FP32 chains)
-  int *labels = nullptr, *perm = nullptr;
-  double *part = nullptr, *comp = nullptr;
+  int *labels = nullptr;
+  // the update's per-instance sort (perm: instance-local indices,
+  // label-major), chunk partials and per-cluster counts, offsets and chunk
+  // prefixes, in two parities: iteration t writes parity t % 2 and reuses the
+  // other's partials where a cluster's members did not change; rstate =
+  // [the other parity holds a full update of these points, its parity]
+  int *perm[2] = {nullptr, nullptr};
+  double *part[2] = {nullptr, nullptr};
+  int *gcnt[2] = {nullptr, nullptr}, *goff[2] = {nullptr, nullptr};
+  int *gcpre[2] = {nullptr, nullptr};
+  int *fd = nullptr, *rstate = nullptr;
+  int *imin = nullptr;  // per (instance, cluster): smallest row whose label changed since the last sort
+  double *comp = nullptr;
   long *choff = nullptr;     // first chunk slot of each instance
   long slots = 0;            // chunk slots in total
-  int4 *desc = nullptr;      // per slot: instance base row, member range
-  int *gcnt = nullptr, *goff = nullptr, *gcpre = nullptr;  // per (instance, cluster)
+  int4 *desc = nullptr;      // per slot: instance base row, member range, (local chunk, cluster)
   int *empty = nullptr;
   int *order = nullptr;  // instances, largest first
   int2 *items = nullptr;
   int nitems = 0, P = 0;
   int *counters = nullptr;  // assign, update
   unsigned long long *nexact = nullptr;  // BF16 assign: points searched exactly
+  // drift bounds of the BF16 assign (null when its LDS cannot also hold the
+  // open list): per point (u, l, l3, r), per centre drift, per instance the
+  // largest drift and largest |x|, the valid flag, the kept / paired / scored
+  // counts
+  float4 *xb = nullptr;
+  float *drift = nullptr, *dmaxv = nullptr, *xmax = nullptr;
+  int *bvalid = nullptr;
+  unsigned long long *nkp = nullptr;
   int comp_slots = 0;
   bool data_set = false, centers_set = false;
   // graph of the last run length
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graph_iters = 0;
+  // events recorded in the graph: before the assign, between assign and
+  // update, after the update, per iteration
+  std::vector<hipEvent_t> gev;
   int iters = 0;
   kmeansBatchStats stats{};
 };
@@ -81,6 +103,20 @@ __global__ void k_batch_fill(int *p, int n, int v) {
   if ((int)threadIdx.x < n) p[threadIdx.x] = v;
 }
 
+// Writes a point's label; when it changes, the point's row (within its
+// instance) lowers the smallest changed row of both clusters (im: the
+// instance's k entries), from which the update finds where each cluster's
+// member list first differs from the previous sort.
+__device__ inline void set_label(int *__restrict__ labels, long gi, int lab, int *__restrict__ im,
+                                 int row) {
+  const int old = labels[gi];
+  if (old != lab) {
+    atomicMin(im + old, row);
+    atomicMin(im + lab, row);
+  }
+  labels[gi] = lab;
+}
+
 // Assign, persistent: each workgroup takes (instance, first row) items from
 // a counter until none are left; centres of the current instance are staged
 // in LDS ([k][d], float4 when d % 4 == 0) with their -0.5*||c||^2.  Each
@@ -90,7 +126,7 @@ __global__ __launch_bounds__(256) void k_batch_assign_f32(
     int nitems, const int2 *__restrict__ items, int P, const long *__restrict__ xoff,
     const int *__restrict__ nvec, int d, int k, const float *__restrict__ X,
     const float *__restrict__ C, const float *__restrict__ nc, int *__restrict__ labels,
-    int *__restrict__ counter) {
+    int *__restrict__ counter, int *__restrict__ imin) {
   extern __shared__ float sm[];  // k*DT centres, then k biases
   float *Cs = sm, *bs = sm + (size_t)k * DT;
   __shared__ int item_s;
@@ -139,8 +175,9 @@ __global__ __launch_bounds__(256) void k_batch_assign_f32(
         if (s0 > b0) { b0 = s0; i0 = j; }
         if (s1 > b1) { b1 = s1; i1 = j; }
       }
-      labels[base + p0] = i0;
-      if (two) labels[base + p0 + 1] = i1;
+      int *im = imin + (size_t)inst * k;
+      set_label(labels, base + p0, i0, im, (int)(w.y + p0));
+      if (two) set_label(labels, base + p0 + 1, i1, im, (int)(w.y + p0 + 1));
     }
     __syncthreads();  // item_s and the staged centres are reused
   }
@@ -154,7 +191,7 @@ __global__ __launch_bounds__(256) void k_batch_assign_f32x(
     int nitems, const int2 *__restrict__ items, int P, const long *__restrict__ xoff,
     const int *__restrict__ nvec, int k, const float *__restrict__ X,
     const float *__restrict__ C, const float *__restrict__ nc, int *__restrict__ labels,
-    int *__restrict__ counter) {
+    int *__restrict__ counter, int *__restrict__ imin) {
   constexpr int V = DT / 4;
   extern __shared__ float4 sm4[];  // k*V centre float4s, then k biases
   float4 *Cs = sm4;
@@ -212,9 +249,12 @@ __global__ __launch_bounds__(256) void k_batch_assign_f32x(
         for (int u = 0; u < PPT; ++u)
           if (sc[u] > bv[u]) { bv[u] = sc[u]; bi[u] = j; }
       }
+      int *im = imin + (size_t)inst * k;
 #pragma unroll
       for (int u = 0; u < PPT; ++u)
-        if (ok[u]) labels[base + r0 + threadIdx.x + 256 * u] = bi[u];
+        if (ok[u])
+          set_label(labels, base + r0 + threadIdx.x + 256 * u, bi[u], im,
+                    (int)(w.y + r0 + threadIdx.x + 256 * u));
     }
     __syncthreads();
   }
@@ -238,13 +278,31 @@ typedef __attribute__((ext_vector_type(4))) unsigned int b_u32x4;
 __device__ inline int b_row(int r, int h) { return 8 * (r >> 2) + 4 * h + (r & 3); }
 __device__ inline uint16_t b_bf16(float f) { return bf16_rne(f); }
 
+//
+// Drift bounds across iterations (xb non-null, used when *bvalid): per point
+// (u, l, l3, r) as in the single-problem engine (u >= |x - c_label|, l <= the
+// distance to every other centre, r the runner-up, l3 <= the distance to
+// every centre but the label and r), valid for the centres of the last
+// assign; k_batch_finalize leaves each centre's drift and each instance's
+// largest.  Each work item first sorts its points into kept (the moved
+// bounds separate every other centre by more than the FP32 chains' error),
+// two-candidate (every centre but the label and r separated: the two FP32
+// chains decide, lowest index on equal chains; kept without them when
+// neither centre moved) and open; the open ones go into an LDS list, and only
+// they are scored.  E32 uses the instance's largest |x| (xmax).  Counts of
+// kept, paired and scored points go to nkp[0..2].
+// (four waves per SIMD for one k-step: 128 VGPRs, a few spilled outside the
+// tile loop; two k-steps keep the compiler's choice)
 template <int KS>
-__global__ __launch_bounds__(256) void k_batch_assign_bf16(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KS == 1 ? 4 : 1)))
+void k_batch_assign_bf16(
     int nitems, const int2 *__restrict__ items, int P, const long *__restrict__ xoff,
     const int *__restrict__ nvec, int d, int k, const float *__restrict__ X,
     const float *__restrict__ C, const float *__restrict__ nc, const float2 *__restrict__ bnd,
     int *__restrict__ labels, int *__restrict__ counter, unsigned long long *__restrict__ nexact,
-    float big) {
+    float big, float4 *__restrict__ xb, const int *__restrict__ bvalid,
+    const float *__restrict__ drift, const float *__restrict__ dmaxv,
+    const float *__restrict__ xmax, unsigned long long *__restrict__ nkp, int *__restrict__ imin) {
   constexpr int Dp = KS * 16;
   const int Kp = (k + 31) / 32 * 32;
   extern __shared__ __attribute__((aligned(16))) unsigned char bsm[];
@@ -252,11 +310,17 @@ __global__ __launch_bounds__(256) void k_batch_assign_bf16(
   float *bias = reinterpret_cast<float *>(Ch + (size_t)Kp * (Dp / 8));  // Kp
   float *Cf = bias + Kp;                                            // k x d
   float *ncs = Cf + (size_t)k * d;                                  // k
-  __shared__ int item_s;
+  uint16_t *olist = reinterpret_cast<uint16_t *>(ncs + k);          // P < 2^16 (drift bounds)
+  __shared__ int item_s, ocnt;
   __shared__ float bndv[3];
-  __shared__ unsigned long long exact_ct;
+  __shared__ unsigned long long exact_ct, kp_ct[3];
   const int tid = threadIdx.x, lane = tid & 63, h = lane >> 5, col = lane & 31;
-  if (tid == 0) exact_ct = 0;
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const bool bv = xb && *bvalid;
+  if (tid == 0) {
+    exact_ct = 0;
+    kp_ct[0] = kp_ct[1] = kp_ct[2] = 0;
+  }
   int cur = -1;
   for (;;) {
     if (tid == 0) item_s = atomicAdd(counter, 1);
@@ -302,12 +366,84 @@ __global__ __launch_bounds__(256) void k_batch_assign_bf16(
     const long rows = min((long)P, (long)nvec[inst] - w.y);
     const long base = xoff[inst] + w.y;
     const int wv0 = tid >> 6;
+    const float Df = (float)d;
+    long nopen = rows;  // the item's positions to score (through olist with bounds)
+    if (bv) {
+      if (tid == 0) ocnt = 0;
+      __syncthreads();
+      const float *dr = drift + (size_t)inst * k;
+      const float dm = dmaxv[inst];
+      const float e32i = Df * 5.9604645e-8f * 1.0001f * (0.5f * ncmax + xmax[inst] * cmax);
+      const float t4 = 4.0f * e32i * 1.0001f + 1e-30f;
+      unsigned nk = 0, np = 0;
+      for (long q0 = 0; q0 < rows; q0 += 256) {
+        const long q = q0 + tid;
+        int cls = 2;
+        if (q < rows) {
+          const long gp = base + q;
+          const int a = labels[gp];
+          const float4 b4 = xb[gp];
+          const int r = __float_as_int(b4.w);
+          const float u = (b4.x + dr[a]) * (1.0f + 1e-6f);
+          const float l = fmaxf((b4.y - dm) * (1.0f - 1e-6f), 0.0f);
+          const float l3 = fmaxf((b4.z - dm) * (1.0f - 1e-6f), 0.0f);
+          if (l > u && (l - u) * (l + u) * (1.0f - 1e-6f) > t4)
+            cls = 0;
+          else if (r >= 0 && r < k && r != a && l3 > u && (l3 - u) * (l3 + u) * (1.0f - 1e-6f) > t4)
+            cls = (dr[a] == 0.0f && dr[r] == 0.0f) ? 0 : 1;
+          if (cls == 0) {
+            xb[gp] = make_float4(u, l, l3, b4.w);
+            ++nk;
+          } else if (cls == 1) {
+            // the plain path's chains of a and r, from the LDS centres, and
+            // the point's own norm bound from the same row
+            const float *xr = X + gp * d;
+            const float *ca = Cf + (size_t)a * d, *cb = Cf + (size_t)r * d;
+            float sa = -0.5f * ncs[a], sr = -0.5f * ncs[r], sq = 0.0f;
+            for (int q2 = 0; q2 < d; ++q2) {
+              const float xv = xr[q2];
+              sa = __builtin_fmaf(xv, ca[q2], sa);
+              sr = __builtin_fmaf(xv, cb[q2], sr);
+              sq = __builtin_fmaf(xv, xv, sq);
+            }
+            const bool tr = sr > sa || (sr == sa && r < a);
+            const int L = tr ? r : a, O = tr ? a : r;
+            const float sL = tr ? sr : sa, sO = tr ? sa : sr;
+            if (L != a) {
+              atomicMin(imin + (size_t)inst * k + a, (int)(w.y + q));
+              atomicMin(imin + (size_t)inst * k + L, (int)(w.y + q));
+            }
+            labels[gp] = L;
+            const float xn = sqrtf(sq * (1.0f + Df * 1.2e-7f + 1e-6f)) * (1.0f + 1e-6f) + 1e-30f;
+            xb[gp] = make_float4(dist_upper(xn, sL - e32i),
+                                 fminf(dist_lower(xn, sO + e32i, Df), l3), l3,
+                                 __int_as_float(O));
+            ++np;
+          }
+        }
+        const bool open = q < rows && cls == 2;
+        const uint64_t mk = __ballot(open);
+        if (mk) {
+          const int lead = __ffsll((long long)mk) - 1;
+          int ob = 0;
+          if (lane == lead) ob = atomicAdd(&ocnt, __popcll(mk));
+          ob = __shfl(ob, lead);
+          if (open) olist[ob + __popcll(mk & below)] = (uint16_t)q;
+        }
+      }
+      if (nk) atomicAdd(&kp_ct[0], (unsigned long long)nk);
+      if (np) atomicAdd(&kp_ct[1], (unsigned long long)np);
+      __syncthreads();
+      nopen = ocnt;
+    }
+    // item position -> row within the item
+    auto row_of = [&](long pos) -> long { return bv ? (long)olist[pos] : pos; };
     // this half's 8 dimensions per k-step of the point in row pn (zero past
     // d or past the rows); the next 32 points' values are loaded while the
     // current ones are scored
     auto load_half = [&](long pn, float *v) {
-      const bool okn = pn < rows;
-      const float *xn = X + (base + (okn ? pn : 0)) * d;
+      const bool okn = pn < nopen;
+      const float *xn = X + (base + (okn ? row_of(pn) : 0)) * d;
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
@@ -318,14 +454,15 @@ __global__ __launch_bounds__(256) void k_batch_assign_bf16(
     };
     float nxt[KS * 8];
     load_half(wv0 * 32 + col, nxt);
-    for (long p0 = wv0 * 32; p0 < rows; p0 += 4 * 32) {
-      const long pr = p0 + col;
-      const bool ok = pr < rows;
-      const float *xr = X + (base + (ok ? pr : 0)) * d;
+    for (long p0 = wv0 * 32; p0 < nopen; p0 += 4 * 32) {
+      const long pos = p0 + col;
+      const bool ok = pos < nopen;
+      const long pr = ok ? row_of(pos) : 0;  // the point's row within the item
+      const float *xr = X + (base + pr) * d;
       float cur_x[KS * 8];
 #pragma unroll
       for (int i = 0; i < KS * 8; ++i) cur_x[i] = nxt[i];
-      if (p0 + 4 * 32 < rows) load_half(p0 + 4 * 32 + col, nxt);
+      if (p0 + 4 * 32 < nopen) load_half(p0 + 4 * 32 + col, nxt);
       // the point's BF16 fragments (this half's 8 dimensions per k-step) and
       // its ||x||^2 part
       b_u32x4 bf[KS];
@@ -476,13 +613,17 @@ __global__ __launch_bounds__(256) void k_batch_assign_bf16(
       const int jy = __shfl_xor(cj, 32);
       bool fail = false;
       if (ok && h == 0) {
-        int lab = cj;
-        float sb = sx;
-        if (jy >= 0 && jy < k && (sy > sb || (sy == sb && jy < lab))) { sb = sy; lab = jy; }
+        int lab = cj, oth = jy;
+        float sb = sx, so = sy;
+        if (jy >= 0 && jy < k && (sy > sb || (sy == sb && jy < lab))) {
+          sb = sy;
+          lab = jy;
+          oth = cj;
+          so = sx;
+        }
         // forward-error bound of the BF16 score (|x.c - xh.ch| <= |dx| |c| +
         // |xh| |dc| by Cauchy-Schwarz, dx and dc the rounding errors), the
         // matrix core's accumulation and the FP32 chain
-        const float Df = (float)d;
         const float g32 = Df * 5.9604645e-8f * 1.0001f;
         const float gmf = (Df + 1.0f) * 2.3841858e-7f * 1.001f;
         const float rnd = ex * cmax + (xn + ex) * dcmax;
@@ -493,8 +634,30 @@ __global__ __launch_bounds__(256) void k_batch_assign_bf16(
         // packed low bits (< 16 ulp) and FP32 rounding of the comparison
         const float slack = (fabsf(b1) + fabsf(b2) + (has3 ? fabsf(b3) : 0.0f)) * 1.6e-5f;
         const float thr = has3 ? b3 + E + slack : -INFINITY;
-        if (!tie && lab >= 0 && sb > -INFINITY && sb > thr) labels[base + pr] = lab;
-        else fail = true;
+        if (!tie && lab >= 0 && sb > -INFINITY && sb > thr)
+          set_label(labels, base + pr, lab, imin + (size_t)inst * k, (int)(w.y + pr));
+        else
+          fail = true;
+        if (xb) {
+          // fresh bounds as in the single-problem kernel: the label's chain
+          // minus E, the other candidate's chain and the third approximate
+          // score plus E (and the slack); a point searched exactly below gets
+          // bounds that never hold
+          float4 nb4 = make_float4(INFINITY, 0.0f, 0.0f, __int_as_float(-1));
+          if (!fail) {
+            const float hi3 = has3 ? b3 + E + slack : -INFINITY;
+            const float hi = fmaxf(hi3, so + E);
+            const int rr = (oth >= 0 && oth < k && oth != lab) ? oth : -1;
+            nb4 = make_float4(dist_upper(xn, sb - E), dist_lower(xn, hi, Df),
+                              hi3 == -INFINITY ? INFINITY : dist_lower(xn, hi3, Df),
+                              __int_as_float(rr));
+          }
+          xb[base + pr] = nb4;
+        }
+      }
+      if (bv || xb) {  // scored points, one count per wave
+        const uint64_t sm = __ballot(ok && h == 0);
+        if (lane == 0 && sm) atomicAdd(&kp_ct[2], (unsigned long long)__popcll(sm));
       }
       // exact search of the points that failed, by the whole wave
       uint64_t fm = __ballot(fail);
@@ -502,7 +665,7 @@ __global__ __launch_bounds__(256) void k_batch_assign_bf16(
       while (fm) {
         const int src = __ffsll((long long)fm) - 1;
         fm &= fm - 1;
-        const long pp = p0 + (src & 31);
+        const long pp = row_of(p0 + (src & 31));
         float xq[Dp];
 #pragma unroll
         for (int q = 0; q < Dp; ++q) xq[q] = __shfl(xv[q], src);
@@ -522,13 +685,16 @@ __global__ __launch_bounds__(256) void k_batch_assign_bf16(
           const int oj = __shfl_xor(bj, o);
           if (ov > best || (ov == best && oj < bj)) { best = ov; bj = oj; }
         }
-        if (lane == 0) labels[base + pp] = bj;
+        if (lane == 0) set_label(labels, base + pp, bj, imin + (size_t)inst * k, (int)(w.y + pp));
       }
     }
     __syncthreads();
   }
   __syncthreads();
   if (tid == 0 && exact_ct) atomicAdd(nexact, exact_ct);
+  if (tid == 0 && nkp)
+    for (int c = 0; c < 3; ++c)
+      if (kp_ct[c]) atomicAdd(nkp + c, kp_ct[c]);
 }
 
 // Left-to-right sum of m FP64 values at stride st, in chunks of KM_CHUNK
@@ -587,13 +753,27 @@ __device__ int block_scan_any(int *v, int n, int *tmp) {
 // in order with no block barrier (ranks among equal labels from a match
 // over the label bits).  Writes perm (instance-local indices, label-major),
 // counts, offsets and the chunk prefix of every cluster.
+// With reuse (rstate = [the other parity holds the previous update of these
+// points, its parity]): each cluster's first member position at which this
+// sort differs from the previous one goes to fd, for k_batch_chunks_wave.
+// Parity of an update: the one the previous update did not write (rstate[1]
+// once rstate[0] says it wrote one); the arrays come in pairs.
+__device__ inline int upd_parity(const int *rstate) { return rstate[0] ? rstate[1] ^ 1 : 0; }
+
 __global__ __launch_bounds__(1024) void k_batch_sort(
     int nb, const int *__restrict__ order, const long *__restrict__ xoff,
-    const int *__restrict__ nvec, int k, const int *__restrict__ labels, int *__restrict__ perm,
-    int *__restrict__ gcnt, int *__restrict__ goff, int *__restrict__ gcpre,
-    const long *__restrict__ choff, int4 *__restrict__ desc, int *__restrict__ counter) {
+    const int *__restrict__ nvec, int k, const int *__restrict__ labels, int *perm0, int *perm1,
+    int *gcnt0, int *gcnt1, int *goff0, int *goff1, int *gcpre0, int *gcpre1,
+    const long *__restrict__ choff, int4 *__restrict__ desc, int *__restrict__ counter,
+    const int *__restrict__ rstate, int *__restrict__ fd, int *__restrict__ imin) {
   extern __shared__ int sh[];  // cur[16][k], offs[k], cpre[k]
   int *cur = sh, *offs = sh + 16 * k, *cpre = offs + k;
+  const bool reuse = rstate[0] != 0;
+  const int par = upd_parity(rstate);
+  int *__restrict__ perm = par ? perm1 : perm0;
+  int *__restrict__ gcnt = par ? gcnt1 : gcnt0;
+  int *__restrict__ goff = par ? goff1 : goff0;
+  int *__restrict__ gcpre = par ? gcpre1 : gcpre0;
   __shared__ int tmp[1025];
   __shared__ int item_s;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -641,8 +821,9 @@ __global__ __launch_bounds__(1024) void k_batch_sort(
       const int cap = (int)(choff[inst + 1] - choff[inst]);
       for (int j = t; j < k; j += blockDim.x) {
         const int o = offs[j], nj = gcnt[(size_t)inst * k + j];
-        for (int c = cpre[j], a0 = o; a0 < o + nj; ++c, a0 += KM_CHUNK)
-          dd[c] = make_int4((int)base, a0, min(o + nj, a0 + KM_CHUNK), 0);
+        int q = 0;
+        for (int c = cpre[j], a0 = o; a0 < o + nj; ++c, a0 += KM_CHUNK, ++q)
+          dd[c] = make_int4((int)base, a0, min(o + nj, a0 + KM_CHUNK), q * 1024 + j);
       }
       for (int c = nch + t; c < cap; c += blockDim.x) dd[c] = make_int4(0, 0, 0, 0);
     }
@@ -665,6 +846,26 @@ __global__ __launch_bounds__(1024) void k_batch_sort(
       if (valid && rank == 0) cw[L] = pos + __popcll(m);
       __builtin_amdgcn_wave_barrier();
     }
+    // First differences: the members of cluster j below its smallest changed
+    // row (imin, recorded by the assign kernels since the previous sort) are
+    // the previous sort's first members of j, in the same positions; their
+    // count, found by a binary search of j's ascending member list, is where
+    // the lists first differ.  imin is then cleared for the next iteration.
+    __threadfence_block();  // the block's perm stores, visible after the barrier
+    __syncthreads();
+    for (int j = t; j < k; j += blockDim.x) {
+      int *imj = imin + (size_t)inst * k + j;
+      if (reuse) {
+        const int im = *imj, o = offs[j];
+        int lo = 0, hi = gcnt[(size_t)inst * k + j];
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (perm[base + o + mid] < im) lo = mid + 1; else hi = mid;
+        }
+        fd[(size_t)inst * k + j] = lo;
+      }
+      *imj = 0x7fffffff;
+    }
     __syncthreads();
   }
 }
@@ -674,19 +875,44 @@ __global__ __launch_bounds__(1024) void k_batch_sort(
 // square (exact in FP64), member by member in ascending order, so the sums
 // are the same sequential FP64 sums.  The squared-norm column is the
 // per-dimension square sums added in ascending dimension order.
+// With reuse (as k_batch_sort), a chunk whose members all lie before its
+// cluster's first changed position copies the previous update's partials.
 template <int DT>
 __global__ __launch_bounds__(256) void k_batch_chunks_wave(
     long slots, const int4 *__restrict__ desc, int d, const float *__restrict__ X,
-    const int *__restrict__ perm, double *__restrict__ part) {
+    const int *perm0, const int *perm1, double *part0, double *part1, int nb, int k,
+    const long *__restrict__ choff, const int *__restrict__ fd, const int *gcnt0,
+    const int *gcnt1, const int *gcpre0, const int *gcpre1, const int *__restrict__ rstate) {
   __shared__ float xs[4][64][DT + 1];
   __shared__ double ts[4][DT];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const long s = (long)blockIdx.x * 4 + w;
   if (s >= slots) return;
+  const int par = upd_parity(rstate);
+  const int *__restrict__ perm = par ? perm1 : perm0;
+  double *__restrict__ part = par ? part1 : part0;
+  const double *__restrict__ part_prev = par ? part0 : part1;
+  const int *__restrict__ gcnt_prev = par ? gcnt0 : gcnt1;
+  const int *__restrict__ gcpre_prev = par ? gcpre0 : gcpre1;
   const int4 dsc = desc[s];
   const long base = dsc.x;
   const int a = dsc.y, e = dsc.z;
   if (a >= e) return;
+  if (rstate[0]) {
+    int lo = 0, hi = nb - 1;  // the slot's instance
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (choff[mid] <= s) lo = mid; else hi = mid - 1;
+    }
+    const int j = dsc.w & 1023, q = dsc.w >> 10;
+    const int q0 = q * KM_CHUNK, end_new = q0 + (e - a);
+    const int end_old = min(q0 + KM_CHUNK, gcnt_prev[(size_t)lo * k + j]);
+    if (end_new == end_old && end_new <= fd[(size_t)lo * k + j]) {
+      const long src = choff[lo] + gcpre_prev[(size_t)lo * (k + 1) + j] + q;
+      if (lane <= d) part[s * (d + 1) + lane] = part_prev[src * (d + 1) + lane];
+      return;
+    }
+  }
   double acc = 0.0, sq = 0.0;
   for (int r0 = a; r0 < e; r0 += 64) {
     const int r = r0 + lane;
@@ -767,15 +993,22 @@ __device__ inline void block_max2(float a, float b, float *ra, float *rb, float2
   if (threadIdx.x == 0) *out = make_float2(ra[0], rb[0]);
 }
 
+// Also each centre's drift (an upper bound of how far it moved, 0 exactly for
+// an unmoved centre) and the instance's largest drift.
 __global__ __launch_bounds__(256) void k_batch_finalize(
-    int d, int k, const long *__restrict__ choff, const int *__restrict__ gcnt,
-    const int *__restrict__ gcpre, const double *__restrict__ part, float *__restrict__ C,
-    float *__restrict__ nc, float2 *__restrict__ bnd, double *__restrict__ comp_out,
-    int *__restrict__ empty_out) {
+    int d, int k, const long *__restrict__ choff, const int *gcnt0, const int *gcnt1,
+    const int *gcpre0, const int *gcpre1, const double *part0, const double *part1,
+    float *__restrict__ C, float *__restrict__ nc, float2 *__restrict__ bnd,
+    double *__restrict__ comp_out, int *__restrict__ empty_out, float *__restrict__ drift,
+    float *__restrict__ dmaxv, const int *__restrict__ rstate) {
   __shared__ double cj[B_MAXK];
   __shared__ int nempty;
   __shared__ float rx[256], ry[256];
-  float mx = 0.0f, dm = 0.0f;
+  float mx = 0.0f, dm = 0.0f, md = 0.0f;
+  const int par = upd_parity(rstate);
+  const int *__restrict__ gcnt = par ? gcnt1 : gcnt0;
+  const int *__restrict__ gcpre = par ? gcpre1 : gcpre0;
+  const double *__restrict__ part = par ? part1 : part0;
   const int inst = blockIdx.x, t = threadIdx.x;
   if (t == 0) nempty = 0;
   __syncthreads();
@@ -786,12 +1019,17 @@ __global__ __launch_bounds__(256) void k_batch_finalize(
     const int nj = gcnt[(size_t)inst * k + j], c0 = cp[j], ncj = cp[j + 1] - cp[j];
     const double *pj = pi + (long)c0 * (d + 1);
     double a = 0.0, b = 0.0;
-    float s2 = 0.0f;
+    float s2 = 0.0f, dq = 0.0f, dmx = 0.0f;
     for (int qq = 0; qq < d; ++qq) {
       const double S = csum2(pj + qq, ncj, d + 1);
-      float cv = Ci[(size_t)j * d + qq];
+      const float old = Ci[(size_t)j * d + qq];
+      float cv = old;
       if (nj > 0) cv = (float)(S / (double)nj);
       Ci[(size_t)j * d + qq] = cv;
+      const float tq = cv - old;
+      dq = __builtin_fmaf(tq, tq, dq);
+      const float at = fabsf(tq);
+      if (!(at <= dmx)) dmx = at;  // NaN propagates
       s2 = __builtin_fmaf(cv, cv, s2);
       const double cd = (double)cv;
       a += cd * S;
@@ -803,6 +1041,9 @@ __global__ __launch_bounds__(256) void k_batch_finalize(
     dm = fmaxf(dm, bf16_residual_sq(Ci + (size_t)j * d, d));
     cj[j] = (Q - 2.0 * a) + (double)nj * b;
     if (nj == 0) atomicAdd(&nempty, 1);
+    const float dv = drift_bound(dq, dmx, d);
+    drift[(size_t)inst * k + j] = dv;
+    if (!(dv <= md)) md = dv;
   }
   __syncthreads();
   if (t == 0) {
@@ -810,6 +1051,48 @@ __global__ __launch_bounds__(256) void k_batch_finalize(
     empty_out[inst] = nempty;
   }
   block_max2(mx, dm, rx, ry, bnd + inst);
+  __syncthreads();
+  rx[t] = md;  // the instance's largest drift (NaN propagates)
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o && !(rx[t + o] <= rx[t])) rx[t] = rx[t + o];
+    __syncthreads();
+  }
+  if (t == 0) dmaxv[inst] = rx[0];
+}
+
+// After an update: the drift bounds hold (for the BF16 assign) and the
+// parity it wrote becomes the previous one (a launch of its own, since every
+// block of the update's kernels reads the parity).
+__global__ void k_batch_flip(int *__restrict__ rstate, int *__restrict__ bvalid) {
+  rstate[1] = upd_parity(rstate);
+  rstate[0] = 1;
+  *bvalid = 1;
+}
+
+// Per instance: an upper bound of the largest |x| of its points (one block
+// per instance), for the drift bounds' FP32 chain error.
+__global__ __launch_bounds__(256) void k_batch_xmax(int d, const long *__restrict__ xoff,
+                                                    const int *__restrict__ nvec,
+                                                    const float *__restrict__ X,
+                                                    float *__restrict__ xmax) {
+  __shared__ float rx[256];
+  const int inst = blockIdx.x, t = threadIdx.x;
+  const float *Xi = X + xoff[inst] * d;
+  float m = 0.0f;
+  for (long p = t; p < nvec[inst]; p += 256) {
+    float sq = 0.0f;
+    for (int q = 0; q < d; ++q) sq = __builtin_fmaf(Xi[p * d + q], Xi[p * d + q], sq);
+    const float xn = sqrtf(sq * (1.0f + (float)d * 1.2e-7f + 1e-6f)) * (1.0f + 1e-6f) + 1e-30f;
+    if (!(xn <= m)) m = xn;
+  }
+  rx[t] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o && !(rx[t + o] <= rx[t])) rx[t] = rx[t + o];
+    __syncthreads();
+  }
+  if (t == 0) xmax[inst] = rx[0];
 }
 
 // Per instance: the largest ||c_j||^2 (nc, already computed) and the largest
@@ -875,7 +1158,8 @@ static hipError_t launch_assign_dt(Batch *b, int P, int nitems, const int2 *item
   }
   const int grid = num_cus() * 4;
   hipLaunchKernelGGL(k_batch_assign_f32<DT>, dim3(grid), dim3(256), lds, b->stream, nitems, items,
-                     P, b->xoff, b->nvec, b->d, b->k, b->X, b->C, b->nc, b->labels, b->counters);
+                     P, b->xoff, b->nvec, b->d, b->k, b->X, b->C, b->nc, b->labels, b->counters,
+                     b->imin);
   return hipGetLastError();
 }
 
@@ -891,7 +1175,8 @@ static hipError_t launch_assign_x(Batch *b, int P, int nitems, const int2 *items
   }
   const int grid = num_cus() * 4;
   hipLaunchKernelGGL((k_batch_assign_f32x<DT, 4>), dim3(grid), dim3(256), lds, b->stream, nitems,
-                     items, P, b->xoff, b->nvec, b->k, b->X, b->C, b->nc, b->labels, b->counters);
+                     items, P, b->xoff, b->nvec, b->k, b->X, b->C, b->nc, b->labels, b->counters,
+                     b->imin);
   return hipGetLastError();
 }
 
@@ -902,16 +1187,18 @@ static size_t bf16_lds(const Batch *b) {
 
 static hipError_t launch_assign(Batch *b, int P, int nitems, const int2 *items) {
   if (b->opt.precision == KM_PREC_BF16) {
-    const size_t lds = bf16_lds(b);
+    const size_t lds = bf16_lds(b) + (b->xb ? sizeof(uint16_t) * (size_t)P : 0);
     const int grid = num_cus() * 4;
-    if (b->d <= 16)
-      hipLaunchKernelGGL(k_batch_assign_bf16<1>, dim3(grid), dim3(256), lds, b->stream, nitems, items,
-                         P, b->xoff, b->nvec, b->d, b->k, b->X, b->C, b->nc, b->bnd, b->labels,
-                         b->counters, b->nexact, FLT_MAX);
-    else
-      hipLaunchKernelGGL(k_batch_assign_bf16<2>, dim3(grid), dim3(256), lds, b->stream, nitems, items,
-                         P, b->xoff, b->nvec, b->d, b->k, b->X, b->C, b->nc, b->bnd, b->labels,
-                         b->counters, b->nexact, FLT_MAX);
+    static size_t attr = 64 * 1024;
+    if (lds > attr) return hipErrorInvalidValue;  // batch_create keeps it within 64 KB
+#define BA(KS)                                                                                   \
+  hipLaunchKernelGGL(k_batch_assign_bf16<KS>, dim3(grid), dim3(256), lds, b->stream, nitems, items, \
+                     P, b->xoff, b->nvec, b->d, b->k, b->X, b->C, b->nc, b->bnd, b->labels,        \
+                     b->counters, b->nexact, FLT_MAX, b->xb, b->bvalid, b->drift, b->dmaxv,        \
+                     b->xmax, b->nkp, b->imin)
+    if (b->d <= 16) BA(1);
+    else BA(2);
+#undef BA
     return hipGetLastError();
   }
   if (b->d == 16) return launch_assign_x<16>(b, P, nitems, items);
@@ -938,11 +1225,14 @@ static hipError_t launch_update(Batch *b, int slot) {
     attr = lds;
   }
   hipLaunchKernelGGL(k_batch_sort, dim3(std::min(b->nb, num_cus() * 2)), dim3(1024), lds, b->stream,
-                     b->nb, b->order, b->xoff, b->nvec, b->k, b->labels, b->perm, b->gcnt, b->goff,
-                     b->gcpre, b->choff, b->desc, b->counters + 1);
+                     b->nb, b->order, b->xoff, b->nvec, b->k, b->labels, b->perm[0], b->perm[1],
+                     b->gcnt[0], b->gcnt[1], b->goff[0], b->goff[1], b->gcpre[0], b->gcpre[1],
+                     b->choff, b->desc, b->counters + 1, b->rstate, b->fd, b->imin);
 #define CHW(DT)                                                                                  \
   hipLaunchKernelGGL(k_batch_chunks_wave<DT>, dim3((unsigned)((b->slots + 3) / 4)), dim3(256), 0,  \
-                     b->stream, b->slots, b->desc, b->d, b->X, b->perm, b->part)
+                     b->stream, b->slots, b->desc, b->d, b->X, b->perm[0], b->perm[1], b->part[0], \
+                     b->part[1], b->nb, b->k, b->choff, b->fd, b->gcnt[0], b->gcnt[1], b->gcpre[0], \
+                     b->gcpre[1], b->rstate)
   switch (pick_dt(b->d)) {
     case 4: CHW(4); break;
     case 8: CHW(8); break;
@@ -952,7 +1242,9 @@ static hipError_t launch_update(Batch *b, int slot) {
   }
 #undef CHW
   hipLaunchKernelGGL(k_batch_finalize, dim3(b->nb), dim3(256), 0, b->stream, b->d, b->k, b->choff,
-                     b->gcnt, b->gcpre, b->part, b->C, b->nc, b->bnd, co, eo);
+                     b->gcnt[0], b->gcnt[1], b->gcpre[0], b->gcpre[1], b->part[0], b->part[1], b->C,
+                     b->nc, b->bnd, co, eo, b->drift, b->dmaxv, b->rstate);
+  hipLaunchKernelGGL(k_batch_flip, dim3(1), dim3(1), 0, b->stream, b->rstate, b->bvalid);
   return hipGetLastError();
 }
 
@@ -1075,10 +1367,21 @@ static void batch_free_graph(Batch *b) {
 static void batch_free(Batch *b) {
   if (b->stream) hipStreamSynchronize(b->stream);
   batch_free_graph(b);
-  void *bufs[] = {b->X_owned, b->xoff, b->nvec, b->C, b->nc, b->labels, b->perm,
-                  b->part, b->comp, b->choff, b->empty, b->order, b->items, b->counters,
-                  b->gcnt, b->goff, b->gcpre, b->desc, b->nexact, b->bnd};
+  for (hipEvent_t ev : b->gev) hipEventDestroy(ev);
+  b->gev.clear();
+  void *bufs[] = {b->X_owned, b->xoff, b->nvec, b->C, b->nc, b->labels,
+                  b->comp, b->choff, b->empty, b->order, b->items, b->counters,
+                  b->desc, b->nexact, b->bnd,
+                  b->xb, b->drift, b->dmaxv, b->xmax, b->bvalid, b->nkp, b->fd, b->rstate,
+                  b->imin};
   for (void *p : bufs) hipFree(p);
+  for (int q = 0; q < 2; ++q) {
+    hipFree(b->perm[q]);
+    hipFree(b->part[q]);
+    hipFree(b->gcnt[q]);
+    hipFree(b->goff[q]);
+    hipFree(b->gcpre[q]);
+  }
   if (b->own_stream && b->stream) hipStreamDestroy(b->stream);
 }
 
@@ -1143,15 +1446,33 @@ static kmeansErrorStatus batch_create(Batch *b, int nb, const long *n, int d, in
   BCHK(hipMalloc(&b->bnd, sizeof(float2) * (size_t)nb), KMEANS_ERROR_BUILDDATA);
   BCHK(hipMalloc(&b->labels, sizeof(int) * (size_t)tot), KMEANS_ERROR_BUILDDATA);
   BCHK(hipMemsetAsync(b->labels, 0, sizeof(int) * (size_t)tot, b->stream), KMEANS_ERROR_BUILDDATA);
-  BCHK(hipMalloc(&b->perm, sizeof(int) * (size_t)tot), KMEANS_ERROR_BUILDDATA);
-  BCHK(hipMalloc(&b->part, sizeof(double) * (size_t)tch * (d + 1)), KMEANS_ERROR_BUILDDATA);
-  BCHK(hipMalloc(&b->gcnt, sizeof(int) * (size_t)nb * k), KMEANS_ERROR_BUILDDATA);
-  BCHK(hipMalloc(&b->goff, sizeof(int) * (size_t)nb * k), KMEANS_ERROR_BUILDDATA);
-  BCHK(hipMalloc(&b->gcpre, sizeof(int) * (size_t)nb * (k + 1)), KMEANS_ERROR_BUILDDATA);
+  for (int q = 0; q < 2; ++q) {
+    BCHK(hipMalloc(&b->perm[q], sizeof(int) * (size_t)tot), KMEANS_ERROR_BUILDDATA);
+    BCHK(hipMalloc(&b->part[q], sizeof(double) * (size_t)tch * (d + 1)), KMEANS_ERROR_BUILDDATA);
+    BCHK(hipMalloc(&b->gcnt[q], sizeof(int) * (size_t)nb * k), KMEANS_ERROR_BUILDDATA);
+    BCHK(hipMalloc(&b->goff[q], sizeof(int) * (size_t)nb * k), KMEANS_ERROR_BUILDDATA);
+    BCHK(hipMalloc(&b->gcpre[q], sizeof(int) * (size_t)nb * (k + 1)), KMEANS_ERROR_BUILDDATA);
+  }
+  BCHK(hipMalloc(&b->fd, sizeof(int) * (size_t)nb * k), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMalloc(&b->rstate, sizeof(int) * 2), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMemset(b->rstate, 0, sizeof(int) * 2), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMalloc(&b->imin, sizeof(int) * (size_t)nb * k), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMemset(b->imin, 0x7f, sizeof(int) * (size_t)nb * k), KMEANS_ERROR_BUILDDATA);
   b->slots = tch;
   BCHK(hipMalloc(&b->counters, sizeof(int) * 2), KMEANS_ERROR_BUILDDATA);
   BCHK(hipMalloc(&b->nexact, sizeof(unsigned long long)), KMEANS_ERROR_BUILDDATA);
   BCHK(hipMemset(b->nexact, 0, sizeof(unsigned long long)), KMEANS_ERROR_BUILDDATA);
+  // drift arrays (every engine's finalize writes them) and, for BF16 engines
+  // whose LDS also holds an open list of the largest work item, the bounds
+  BCHK(hipMalloc(&b->drift, sizeof(float) * (size_t)nb * k), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMalloc(&b->dmaxv, sizeof(float) * (size_t)nb), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMalloc(&b->xmax, sizeof(float) * (size_t)nb), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMalloc(&b->bvalid, sizeof(int)), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMemset(b->bvalid, 0, sizeof(int)), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMalloc(&b->nkp, sizeof(unsigned long long) * 3), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMemset(b->nkp, 0, sizeof(unsigned long long) * 3), KMEANS_ERROR_BUILDDATA);
+  if (b->opt.precision == KM_PREC_BF16 && bf16_lds(b) + sizeof(uint16_t) * B_MAXP <= 64 * 1024)
+    BCHK(hipMalloc(&b->xb, sizeof(float4) * (size_t)tot), KMEANS_ERROR_BUILDDATA);
   return KMEANS_SUCCESS;
 }
 
@@ -1169,12 +1490,22 @@ static kmeansErrorStatus ensure_slots(Batch *b, int iters) {
   return KMEANS_SUCCESS;
 }
 
-static hipError_t enqueue_iterations(Batch *b, int iters) {
+// timed (kmeansOptions.timing): events before the assign, between assign and
+// update and after the update of every iteration (an eager run, no graph)
+static hipError_t enqueue_iterations(Batch *b, int iters, bool timed) {
   hipError_t e;
+  while (timed && b->gev.size() < (size_t)3 * iters) {
+    hipEvent_t ev = nullptr;
+    if ((e = hipEventCreate(&ev)) != hipSuccess) return e;
+    b->gev.push_back(ev);
+  }
   for (int it = 0; it < iters; ++it) {
     hipLaunchKernelGGL(k_batch_fill, dim3(1), dim3(64), 0, b->stream, b->counters, 2, 0);
+    if (timed && (e = hipEventRecord(b->gev[3 * it], b->stream)) != hipSuccess) return e;
     if ((e = launch_assign(b, b->P, b->nitems, b->items)) != hipSuccess) return e;
+    if (timed && (e = hipEventRecord(b->gev[3 * it + 1], b->stream)) != hipSuccess) return e;
     if ((e = launch_update(b, it)) != hipSuccess) return e;
+    if (timed && (e = hipEventRecord(b->gev[3 * it + 2], b->stream)) != hipSuccess) return e;
   }
   return hipGetLastError();
 }
@@ -1189,10 +1520,11 @@ static kmeansErrorStatus batch_run(Batch *b, int iters) {
     BCHK(upload_items(b, P), KMEANS_ERROR_COMPUTE);
   }
   const auto t0 = std::chrono::steady_clock::now();
-  if (b->graph_iters != iters) {
+  const bool timed = b->opt.timing != 0;
+  if (!timed && b->graph_iters != iters) {
     batch_free_graph(b);
     BCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), KMEANS_ERROR_COMPUTE);
-    hipError_t e = enqueue_iterations(b, iters);
+    hipError_t e = enqueue_iterations(b, iters, false);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(b->stream, &g);
     if (e != hipSuccess || e2 != hipSuccess) {
@@ -1210,7 +1542,10 @@ static kmeansErrorStatus batch_run(Batch *b, int iters) {
   hipEventCreate(&e0);
   hipEventCreate(&e1);
   hipEventRecord(e0, b->stream);
-  BCHK(hipGraphLaunch(b->exec, b->stream), KMEANS_ERROR_COMPUTE);
+  if (timed)
+    BCHK(enqueue_iterations(b, iters, true), KMEANS_ERROR_COMPUTE);
+  else
+    BCHK(hipGraphLaunch(b->exec, b->stream), KMEANS_ERROR_COMPUTE);
   hipEventRecord(e1, b->stream);
   BCHK(hipEventSynchronize(e1), KMEANS_ERROR_COMPUTE);
   float ms = 0.0f;
@@ -1228,11 +1563,26 @@ static kmeansErrorStatus batch_run(Batch *b, int iters) {
   b->stats.empty_clusters += es;
   b->stats.tile_rows = b->P;
   b->stats.work_items = b->nitems;
-  b->stats.graph_nodes = 5 * iters;
+  size_t nnodes = 0;
+  b->stats.graph_nodes = 0;
+  if (!timed && hipGraphGetNodes(b->graph, nullptr, &nnodes) == hipSuccess)
+    b->stats.graph_nodes = (int)nnodes;
+  b->stats.ms_assign = b->stats.ms_update = 0.0;
+  for (int it = 0; timed && it < iters; ++it) {
+    float ta = 0.0f, tu = 0.0f;
+    if (hipEventElapsedTime(&ta, b->gev[3 * it], b->gev[3 * it + 1]) == hipSuccess)
+      b->stats.ms_assign += ta;
+    if (hipEventElapsedTime(&tu, b->gev[3 * it + 1], b->gev[3 * it + 2]) == hipSuccess)
+      b->stats.ms_update += tu;
+  }
   b->stats.last_run_iters = iters;
-  unsigned long long nx = 0;
+  unsigned long long nx = 0, kp[3] = {0, 0, 0};
   BCHK(hipMemcpy(&nx, b->nexact, sizeof(nx), hipMemcpyDeviceToHost), KMEANS_ERROR_COMPUTE);
+  BCHK(hipMemcpy(kp, b->nkp, sizeof(kp), hipMemcpyDeviceToHost), KMEANS_ERROR_COMPUTE);
   b->stats.exact_searched = (long long)nx;
+  b->stats.kept = (long long)kp[0];
+  b->stats.paired = (long long)kp[1];
+  b->stats.scored = (long long)kp[2];
   return KMEANS_SUCCESS;
 }
 
@@ -1285,6 +1635,12 @@ kmeansErrorStatus kmeansBatchSetData(kmeansBatchHandle h, const float *X, int on
     BCHK(hipMemcpy(b->X_owned, X, bytes, hipMemcpyHostToDevice), KMEANS_ERROR_BUILDDATA);
     b->X = b->X_owned;
   }
+  // new points: no drift bounds and no partials to reuse; their instances'
+  // largest |x|
+  BCHK(hipMemsetAsync(b->bvalid, 0, sizeof(int), b->stream), KMEANS_ERROR_BUILDDATA);
+  BCHK(hipMemsetAsync(b->rstate, 0, sizeof(int) * 2, b->stream), KMEANS_ERROR_BUILDDATA);
+  hipLaunchKernelGGL(km::k_batch_xmax, dim3(b->nb), dim3(256), 0, b->stream, b->d, b->xoff, b->nvec,
+                     b->X, b->xmax);
   BCHK(hipStreamSynchronize(b->stream), KMEANS_ERROR_BUILDDATA);
   b->data_set = true;
   km::batch_free_graph(b);
@@ -1302,6 +1658,8 @@ kmeansErrorStatus kmeansBatchSetCenters(kmeansBatchHandle h, const float *C, int
                      (long)b->nb * b->k, b->d, b->C, b->nc);
   hipLaunchKernelGGL(km::k_batch_bounds, dim3(b->nb), dim3(256), 0, b->stream, b->d, b->k, b->C,
                      b->nc, b->bnd);
+  // centres set from outside: the drift bounds no longer hold
+  BCHK(hipMemsetAsync(b->bvalid, 0, sizeof(int), b->stream), KMEANS_ERROR_INITIALIZE);
   BCHK(hipStreamSynchronize(b->stream), KMEANS_ERROR_INITIALIZE);
   b->centers_set = true;
   return KMEANS_SUCCESS;
@@ -1317,13 +1675,15 @@ kmeansErrorStatus kmeansBatchInitFirstK(kmeansBatchHandle h) {
                      (long)b->nb * b->k, b->d, b->C, b->nc);
   hipLaunchKernelGGL(km::k_batch_bounds, dim3(b->nb), dim3(256), 0, b->stream, b->d, b->k, b->C,
                      b->nc, b->bnd);
+  // centres set from outside: the drift bounds no longer hold
+  BCHK(hipMemsetAsync(b->bvalid, 0, sizeof(int), b->stream), KMEANS_ERROR_INITIALIZE);
   BCHK(hipStreamSynchronize(b->stream), KMEANS_ERROR_INITIALIZE);
   b->centers_set = true;
   return KMEANS_SUCCESS;
 }
 
 kmeansErrorStatus kmeansBatchSetTileRows(kmeansBatchHandle h, int rows) {
-  if (!h || rows < 0 || (rows && rows % 2)) return KMEANS_ERROR_INVALID_ARGUMENT;
+  if (!h || rows < 0 || (rows && rows % 2) || rows > B_MAXP) return KMEANS_ERROR_INVALID_ARGUMENT;
   Batch *b = &h->b;
   km::batch_free_graph(b);
   if (rows == 0) {

@@ -173,4 +173,32 @@ hipError_t launch_generate(hipStream_t s, uint64_t seed, long i0, long n, int d,
 int centers_ld(int k);
 int num_cus();
 
+// Distances from scores: |x - c|^2 = |x|^2 - 2 (x.c - 0.5 |c|^2), with xn >=
+// |x| the point's norm bound.  dist_upper: an upper bound of |x - c| given lo
+// <= the exact score of c; dist_lower: a lower bound given hi >= it.  (|x|^2
+// lies between xn^2 (1 - 1.8e-7 D - 4e-6) and xn^2 (1 + 2.4e-7); the relative
+// terms cover the roundings of the subtraction and of the root.)
+__device__ inline float dist_upper(float xn, float lo) {
+  const float x2u = xn * xn * (1.0f + 2.4e-7f);
+  const float uu = (x2u - 2.0f * lo) + 3.6e-7f * (x2u + 2.0f * fabsf(lo));
+  return sqrtf(fmaxf(uu, 0.0f)) * (1.0f + 1e-6f) + 1e-30f;
+}
+__device__ inline float dist_lower(float xn, float hi, float Df) {
+  const float x2l = xn * xn * (1.0f - Df * 1.8e-7f - 4e-6f);
+  const float ll = (x2l - 2.0f * hi) - 3.6e-7f * (x2l + 2.0f * fabsf(hi));
+  return ll > 0.0f ? sqrtf(ll) * (1.0f - 1e-6f) : 0.0f;
+}
+
+// Upper bound of the distance a centre moved, from its FP32 coordinate
+// differences t_q: the chain of their squares with its rounding covered
+// (gamma_D in any order) when the largest |t_q| is at least 2^-40, else
+// sqrt(D) max |t_q| (squares of smaller differences may underflow); 0
+// exactly when every difference is 0, NaN when one is.
+__device__ inline float drift_bound(float q, float ma, int D) {
+  if (ma == 0.0f) return 0.0f;
+  return ma >= 9.094947e-13f  // 2^-40
+             ? sqrtf(q * (1.0f + (float)D * 1.2e-7f + 4e-7f)) * (1.0f + 1e-6f)
+             : sqrtf((float)D) * ma * (1.0f + 1e-6f) + 1e-45f;
+}
+
 }  // namespace km

@@ -259,10 +259,8 @@ __global__ __launch_bounds__(256) void k_tile_order(int K, int D, int nKT,
 // drift[j] = upper bound of |C_j - Cref_j| (the FP32 differences are within
 // an ulp, the sum of squares within gamma_D in any order), one wave per
 // centre; drift[K] (as bits) += the largest by atomic maximum over
-// non-negative floats, from 0 (k_ref_copy clears it for the next pass).
-// drift[j] is 0 exactly when every coordinate difference is 0.  When the
-// largest difference is below 2^-40, squares may underflow, and
-// sqrt(D) * max |difference| bounds the drift instead.
+// non-negative floats, from 0 (k_ref_copy clears it for the next pass);
+// drift_bound: 0 exactly when every coordinate difference is 0.
 __global__ __launch_bounds__(256) void k_drift(int K, int D, const float *__restrict__ C,
                                                const float *__restrict__ Cref,
                                                float *__restrict__ drift,
@@ -284,30 +282,10 @@ __global__ __launch_bounds__(256) void k_drift(int K, int D, const float *__rest
     if (!(om <= ma)) ma = om;
   }
   if (lane == 0) {
-    float v = 0.0f;
-    if (!(ma == 0.0f))
-      v = ma >= 9.094947e-13f  // 2^-40
-              ? sqrtf(q * (1.0f + (float)D * 1.2e-7f + 4e-7f)) * (1.0f + 1e-6f)
-              : sqrtf((float)D) * ma * (1.0f + 1e-6f) + 1e-45f;
+    const float v = drift_bound(q, ma, D);
     drift[j] = v;
     atomicMax(reinterpret_cast<unsigned *>(drift + K), __float_as_uint(v));
   }
-}
-
-// Distances from scores: |x - c|^2 = |x|^2 - 2 (x.c - 0.5 |c|^2), with xn >=
-// |x| the point's norm bound.  dist_upper: an upper bound of |x - c| given lo
-// <= the exact score of c; dist_lower: a lower bound given hi >= it.  (|x|^2
-// lies between xn^2 (1 - 1.8e-7 D - 4e-6) and xn^2 (1 + 2.4e-7); the relative
-// terms cover the roundings of the subtraction and of the root.)
-__device__ inline float dist_upper(float xn, float lo) {
-  const float x2u = xn * xn * (1.0f + 2.4e-7f);
-  const float uu = (x2u - 2.0f * lo) + 3.6e-7f * (x2u + 2.0f * fabsf(lo));
-  return sqrtf(fmaxf(uu, 0.0f)) * (1.0f + 1e-6f) + 1e-30f;
-}
-__device__ inline float dist_lower(float xn, float hi, float Df) {
-  const float x2l = xn * xn * (1.0f - Df * 1.8e-7f - 4e-6f);
-  const float ll = (x2l - 2.0f * hi) - 3.6e-7f * (x2l + 2.0f * fabsf(hi));
-  return ll > 0.0f ? sqrtf(ll) * (1.0f - 1e-6f) : 0.0f;
 }
 
 // The plain path's score chains of one point against two centres (-0.5 nc_j,

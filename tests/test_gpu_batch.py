@@ -56,7 +56,7 @@ def test_batch_equals_per_instance_oracle(precision, d, k, median, lo, hi):
     st = b.stats()
     print(f"\n{precision} d={d} k={k}: {st}")
     assert st["tile_rows"] in (256, 512, 1024, 2048, 4096)
-    assert st["graph_nodes"] == 5 * iters
+    assert st["graph_nodes"] == 6 * iters
     _check(b, n, Xs, ref, iters)
     b.close()
 
@@ -98,4 +98,54 @@ def test_batch_explicit_centres_and_device_data():
     b.set_centers(device_ptr=Cd.data_ptr())
     b.run(4)
     _check(b, n, Xs, ref, 4)
+    b.close()
+
+
+def test_batch_drift_bounds_keep_labels():
+    """BF16 batches keep the points whose drift bounds separate their label
+    (or decide them between two candidates) without scoring them; every run
+    length matches the per-instance oracle, and centres set from outside
+    afterwards drop the bounds."""
+    nb, d, k = 24, 16, 64
+    n, Xs = _instances(nb, d, k, 4000, 500, 20000, seed=13)
+    b = km.KMeansBatch(n, d, k, precision="bf16")
+    b.set_data(np.concatenate(Xs))
+    b.init_first_k()
+    b.set_tile_rows(1024)  # no tuning launches in the counts
+    done = 0
+    for steps in (2, 3, 5):
+        b.run(steps)
+        done += steps
+        _check(b, n, Xs, _oracle(Xs, k, done), steps)
+    st = b.stats()
+    print(f"kept {st['kept']} paired {st['paired']} scored {st['scored']} of {done * int(n.sum())}")
+    assert st["kept"] > 2 * int(n.sum())
+    assert st["scored"] < (done - 1) * int(n.sum())
+    C1 = b.centers()
+    C1[:, 0, :] += 0.5  # from outside: the bounds no longer hold
+    b.set_centers(C1)
+    b.run(3)
+    ref = [orc.lloyd(X, C1[i], 3) for i, X in enumerate(Xs)]
+    _check(b, n, Xs, ref, 3)
+    b.close()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_batch_new_data_drops_reuse(precision):
+    """The update reuses chunk partials of unchanged clusters across
+    iterations and graph replays; new points of the same shape must start
+    afresh (their sums and labels equal the oracle's on the new points)."""
+    nb, d, k = 12, 16, 32
+    n, Xs1 = _instances(nb, d, k, 2000, 300, 8000, seed=17)
+    Xs2 = [orc.mixture(500 + i, int(n[i]), d, k, -4.0, 4.0, 0.7, 0.7)[0] for i in range(nb)]
+    b = km.KMeansBatch(n, d, k, precision=precision)
+    b.set_data(np.concatenate(Xs1))
+    b.init_first_k()
+    b.run(4)
+    _check(b, n, Xs1, _oracle(Xs1, k, 4), 4)
+    b.set_data(np.concatenate(Xs2))
+    b.init_first_k()
+    b.run(3)
+    b.run(2)
+    _check(b, n, Xs2, _oracle(Xs2, k, 5), 2)
     b.close()
