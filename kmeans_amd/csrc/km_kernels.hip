@@ -780,47 +780,43 @@ __global__ __launch_bounds__(256) void k_chunk_sums4(const float *__restrict__ X
   }
 }
 
-// Reuse: fd[j] = the shorter of centre j's member counts in the two sorts
-// (0 without a previous sort), lowered by k_first_diff.
-__global__ void k_fd_init(int K, const int *__restrict__ ccount, const int *__restrict__ cs_prev,
-                          int *__restrict__ fd) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < K) fd[j] = cs_prev ? min(ccount[j], cs_prev[K + j]) : 0;
-}
-
-// fd[j] = min over the positions m of centre j's new member list whose
-// member differs from position m of the previous list (one thread per
-// position s of the block's new sort; its centre is the last j with
-// cstart[j] <= s).  Lanes of a wave with the same centre hold increasing
-// positions, so the first of them carries the minimum: at most one atomic
-// per centre per wave, and none when fd[j] is already at or below it (every
-// position after a centre's first change differs, so most waves skip).
-__global__ __launch_bounds__(256) void k_first_diff(long nb, int K, const int *__restrict__ perm,
+// Reuse: fd[j] = the first position at which centre j's member list in this
+// sort differs from the previous sort's (the shorter count when one is a
+// prefix of the other).  One workgroup per centre walks both lists 2048
+// positions at a time and stops at the first step with a difference, so a
+// centre with an early change costs little; lists read coalesced.
+__global__ __launch_bounds__(256) void k_first_diff(int K, const int *__restrict__ perm,
                                                     const int *__restrict__ cstart,
+                                                    const int *__restrict__ ccount,
                                                     const int *__restrict__ perm_prev,
                                                     const int *__restrict__ cs_prev,
                                                     int *__restrict__ fd) {
-  const long s = (long)blockIdx.x * 256 + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  int j = 0, m = 0;
-  bool diff = false;
-  if (s < nb) {
-    int lo = 0, hi = K - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (cstart[mid] <= s) lo = mid; else hi = mid - 1;
+  __shared__ int first;
+  const int j = blockIdx.x, t = threadIdx.x;
+  if (j >= K) return;
+  const int lim = min(ccount[j], cs_prev[K + j]);
+  const int *a = perm + cstart[j], *b = perm_prev + cs_prev[j];
+  if (t == 0) first = lim;
+  __syncthreads();
+  for (int m0 = 0; m0 < lim; m0 += 2048) {  // 8 positions per thread per step, loads issued together
+    int va[8], vb[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int m = m0 + u * 256 + t;
+      va[u] = m < lim ? a[m] : 0;
+      vb[u] = m < lim ? b[m] : 0;
     }
-    j = lo;
-    m = (int)(s - cstart[j]);
-    if (m < cs_prev[K + j]) diff = perm_prev[cs_prev[j] + m] != perm[s];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (va[u] != vb[u]) {
+        atomicMin(&first, m0 + u * 256 + t);
+        break;
+      }
+    __syncthreads();
+    if (first < lim) break;  // uniform: every thread reads it after the barrier
+    __syncthreads();         // the next step's atomics only after every thread's read
   }
-  uint64_t mk = __ballot(diff);
-  while (mk) {
-    const int lead = __ffsll((long long)mk) - 1;
-    const int jl = __shfl(j, lead);
-    if (lane == lead && m < __builtin_nontemporal_load(fd + jl)) atomicMin(fd + jl, m);
-    mk &= ~__ballot(diff && j == jl);
-  }
+  if (t == 0) fd[j] = first;
 }
 
 // Per-centre CSUM over its chunk partials (D sums and the q column),
@@ -1208,12 +1204,9 @@ static hipError_t block_chunks(hipStream_t s, long i0, long nb, int d, int k, co
   const int *perm = ws.perm_out ? ws.perm_out + i0 : ws.perm;
   const long qmax = nb / KM_CHUNK + k + 1;
   const bool reuse = ws.fd && ws.perm_prev;
-  if (reuse) {  // each centre's first member position that differs from the previous sort
-    hipLaunchKernelGGL(k_fd_init, dim3(km_cdiv(k, 256)), dim3(256), 0, s, k, ws.ccount, ws.cs_prev,
-                       ws.fd);
-    hipLaunchKernelGGL(k_first_diff, dim3(km_cdiv(nb, 256)), dim3(256), 0, s, nb, k, perm,
-                       ws.cstart, ws.perm_prev + i0, ws.cs_prev, ws.fd);
-  }
+  if (reuse)  // each centre's first member position that differs from the previous sort
+    hipLaunchKernelGGL(k_first_diff, dim3(k), dim3(256), 0, s, k, perm, ws.cstart, ws.ccount,
+                       ws.perm_prev + i0, ws.cs_prev, ws.fd);
   const int *fd = reuse ? ws.fd : nullptr;
   if (sizeof(T) == 4 && d % 4 == 0 && d >= 16 && d <= 256 && ((uintptr_t)X & 15) == 0) {
     int TPC = 1;
