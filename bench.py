@@ -80,7 +80,7 @@ def main():
 
     prep_ms = {}
 
-    def run(precision, steps, warmup, full_gemm=False):
+    def run(precision, steps, warmup, full_gemm=False, timing=True):
         e = km.KMeans(n, d, k, precision=precision, timing=True, full_gemm=full_gemm)
         e.set_data(device_ptr=X.data_ptr())
         e.set_centers(C0)
@@ -91,7 +91,7 @@ def main():
         # fresh stats for the timed run; its per-data-set preparation (the
         # reduced-precision copy of the points and their norm bounds) is timed
         # on its own and is not part of the iterations
-        e2 = km.KMeans(n, d, k, precision=precision, timing=True, full_gemm=full_gemm)
+        e2 = km.KMeans(n, d, k, precision=precision, timing=timing, full_gemm=full_gemm)
         e.close()
         torch.cuda.synchronize()
         tp = time.perf_counter()
@@ -99,6 +99,13 @@ def main():
         e2.set_centers(C0)
         e2.synchronize()
         prep_ms[precision] = (time.perf_counter() - tp) * 1e3
+        if not timing and warmup:
+            # graph launches: the engine's iteration graphs (one per update
+            # parity and reuse state) are captured and instantiated by a
+            # warm-up on the timed engine itself, as a caller's first run would
+            e2.run(max(warmup, 3))
+            e2.set_centers(C0)
+            e2.synchronize()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         e2.run(steps)
@@ -111,7 +118,21 @@ def main():
         print(json.dumps(bench_proposer(args, X, C0, spec, km, np, torch)))
         return
 
-    eng, dt = run(args.precision, args.steps, args.warmup, args.full_gemm)
+    # FP32 over one canonical block: the timed iterations are graph launches
+    # (timing off); the per-phase times come from an eagerly launched rerun
+    graphs = args.precision == "fp32" and not args.full_gemm and n <= km.KM_BLOCK
+    eng, dt = run(args.precision, args.steps, args.warmup, args.full_gemm, timing=not graphs)
+    graph_info = None
+    if graphs:
+        gl = eng.stats()["graph_launches"]
+        eng.close()
+        eng, dt_eager = run(args.precision, args.steps, 0)
+        graph_info = {"graph_launches_in_timed_run": gl - max(args.warmup, 3) if args.warmup else gl,
+                      "eager_ms_per_step": dt_eager * 1e3 / args.steps,
+                      "note": "ms_per_step: each iteration one hipGraph launch (assign, update, "
+                              "compactness) plus the slot commit, graphs instantiated in the "
+                              "warm-up; phases_ms_per_step and the roofline from an eagerly "
+                              "launched rerun of the same iterations with HIP events"}
     st = eng.stats()
     pairs = float(n) * k
     value = pairs * args.steps / dt
@@ -153,6 +174,7 @@ def main():
         "bounded_per_step": st["bounded"] / args.steps,
         "paired_per_step": st["paired"] / args.steps,
         "prep_ms_excluded": prep_ms.get(args.precision),
+        **({"graphs": graph_info} if graph_info else {}),
         "timing_scope": "ms_per_step covers the Lloyd iterations only (assign, re-resolve, update, "
                         "compactness) with the points resident in HBM; the per-data-set "
                         "preparation (reduced-precision copy of the points and their norm bounds, "

@@ -199,6 +199,8 @@ typedef struct kmeansStats {
                                  against every centre                         */
   long long assign_launches;  /* launches of that kernel (timing == 1)        */
   double ms_assign_kernel;    /* their summed event-timed durations           */
+  long long graph_launches;   /* iterations run as one hipGraph launch (FP32,
+                                 resident single-block data, timing == 0)     */
 } kmeansStats;
 
 typedef struct kmeansEngine *kmeansHandle;

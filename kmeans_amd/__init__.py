@@ -69,7 +69,7 @@ class Stats(ctypes.Structure):
                 ("tiles", ctypes.c_longlong), ("tiles_total", ctypes.c_longlong),
                 ("bounded", ctypes.c_longlong), ("paired", ctypes.c_longlong),
                 ("scored", ctypes.c_longlong), ("assign_launches", ctypes.c_longlong),
-                ("ms_assign_kernel", ctypes.c_double)]
+                ("ms_assign_kernel", ctypes.c_double), ("graph_launches", ctypes.c_longlong)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -25,6 +25,8 @@ namespace km {
 
 Engine::~Engine() {
   if (stream) hipStreamSynchronize(stream);
+  graphs_drop();
+  hipFree(dstage);
   mfma_free(this);
   learned_free(this);
   gemm_free(this);
@@ -126,6 +128,7 @@ kmeansErrorStatus Engine::create(long m_, int n_, int k_, const kmeansOptions *o
                    KMEANS_ERROR_BUILDDATA);
     }
     KM_HIP_CHECK(hipMalloc(&dfd, sizeof(int) * (size_t)nblk * k), KMEANS_ERROR_BUILDDATA);
+    KM_HIP_CHECK(hipMalloc(&dstage, sizeof(double) * 2), KMEANS_ERROR_BUILDDATA);
   }
   return ensure_comp_slots(64);
 }
@@ -159,6 +162,7 @@ kmeansErrorStatus Engine::ensure_comp_slots(int nslots) {
 kmeansErrorStatus Engine::buildData(const float *data, int on_device) {
   if (!data) return KMEANS_ERROR_INVALID_ARGUMENT;
   reuse_ok = false;  // new points: no chunk partials to reuse
+  graphs_drop();     // the graphs hold the old points' address
   if (!on_device && opt.tile_rows > 0 && opt.tile_rows < m) return setup_streaming(data);
   if (on_device) {
     dX = data;
@@ -447,6 +451,91 @@ __global__ void k_converge(const double *comp, int s, double prev0, double crite
   }
 }
 
+// After an iteration run as a graph: its compactness and empty-cluster count
+// (stage[0], and stage[1] read as an int) into slot s, then, with ctl, the
+// convergence test of k_converge for that slot.
+__global__ void k_slot_commit(const double *stage, double *comp, int *empty_hist, int s,
+                              double prev0, double criterion, int *ctl) {
+  comp[s] = stage[0];
+  empty_hist[s] = *reinterpret_cast<const int *>(stage + 1);
+  if (!ctl || ctl[0]) return;
+  const double c = comp[s], prev = s > 0 ? comp[s - 1] : prev0;
+  const double rel = fabs(c - prev) / fabs(c);
+  if (!(rel > criterion)) {
+    ctl[0] = 1;
+    ctl[1] = 0;
+  }
+}
+
+void Engine::graphs_drop() {
+  bool any = false;
+  for (auto &g : gexec) any |= g != nullptr;
+  if (!any) return;
+  hipStreamSynchronize(stream);  // no launch of them still in flight
+  for (auto &g : gexec) {
+    if (g) hipGraphExecDestroy(g);
+    g = nullptr;
+  }
+}
+
+bool Engine::graph_eligible(bool minibatch) const {
+  return graph_ok && !opt.timing && opt.precision == KM_PREC_FP32 && !opt.full_gemm && !learned &&
+         !streaming && !minibatch && dperm[0] && dstage && m <= KM_BLOCK;
+}
+
+// One full-pass iteration (assign, update, compactness into dstage) as a
+// graph launch; the graph of this state is captured from the engine's own
+// launch sequence on first use.  *ran false: no graph (capture or
+// instantiation failed; graph_ok is cleared) and nothing was enqueued.
+kmeansErrorStatus Engine::graph_iteration(bool devconv, bool *ran) {
+  *ran = false;
+  const int nw = reuse_cur ^ 1;
+  const int v = (reuse_ok ? 1 : 0) | (nw << 1) | (devconv ? 4 : 0);
+  if (!gexec[v]) {
+    if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      (void)hipGetLastError();
+      graph_ok = false;
+      return KMEANS_SUCCESS;
+    }
+    // clusterCenters advances the reuse state as it enqueues; the launch
+    // below advances it again, so it is put back after the capture
+    const int rc = reuse_cur;
+    const bool ro = reuse_ok, ook = order_ok;
+    int *dord = dorder;
+    kmeansErrorStatus err = closestCenters(0, m);
+    if (err == KMEANS_SUCCESS)
+      err = clusterCenters(0, m, reinterpret_cast<int *>(dstage + 1));
+    if (err == KMEANS_SUCCESS && launch_comp_reduce(stream, k, ws.compj, dstage) != hipSuccess)
+      err = KMEANS_ERROR_COMPACTNESS;
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(stream, &g);
+    reuse_cur = rc;
+    reuse_ok = ro;
+    order_ok = ook;
+    dorder = dord;
+    hipGraphExec_t x = nullptr;
+    const bool ok = err == KMEANS_SUCCESS && ce == hipSuccess && g &&
+                    hipGraphInstantiate(&x, g, nullptr, nullptr, 0) == hipSuccess;
+    if (g) hipGraphDestroy(g);
+    if (!ok) {
+      (void)hipGetLastError();
+      if (x) hipGraphExecDestroy(x);
+      graph_ok = false;
+      return KMEANS_SUCCESS;
+    }
+    gexec[v] = x;
+  }
+  KM_HIP_CHECK(hipGraphLaunch(gexec[v], stream), KMEANS_ERROR_COMPUTE);
+  // the host side of a full update (clusterCenters)
+  reuse_cur = nw;
+  reuse_ok = true;
+  dorder = dperm[nw];
+  order_ok = true;
+  ++stats.graph_launches;
+  *ran = true;
+  return KMEANS_SUCCESS;
+}
+
 kmeansErrorStatus Engine::collect_empty(int nslots) {
   if (nslots <= 0) return KMEANS_SUCCESS;
   std::vector<int> emp(nslots);
@@ -467,6 +556,10 @@ kmeansErrorStatus Engine::collect_empty(int nslots) {
 //    sees the flag, then finds the converged iteration in the history;
 //  - otherwise (mini-batch windows, streamed tiles, the proposer, the
 //    library GEMM) one host round trip per iteration.
+// FP32 full passes over resident points in one canonical block, timing off,
+// run each iteration as one graph launch plus the slot commit
+// (graph_iteration); the labels, centres and compactness are those of the
+// launches the graph was captured from.
 // Statistics count iterations up to the converged one.
 kmeansErrorStatus Engine::run_iterations(long r0, long rows, bool minibatch) {
   kmeansErrorStatus err;
@@ -495,7 +588,9 @@ kmeansErrorStatus Engine::run_iterations(long r0, long rows, bool minibatch) {
     KM_HIP_CHECK(hipStreamSynchronize(stream), KMEANS_ERROR_COMPUTE);
     return KMEANS_SUCCESS;
   };
+  const bool graphs = graph_eligible(minibatch);
   while (iters < maxIters && relErr > criterion) {
+    bool graph_ran = false;
     long b0 = r0, br = rows;
     if (minibatch) constructMiniBatch(mb_index++, &b0, &br);
     cur_slot = slot;
@@ -520,23 +615,36 @@ kmeansErrorStatus Engine::run_iterations(long r0, long rows, bool minibatch) {
         KM_HIP_CHECK(learned_fallback(this), KMEANS_ERROR_CLOSEST_CENTERS);
         record(slot, 2);
       } else {
-        record(slot, 0);
-        if ((err = closestCenters(b0, br)) != KMEANS_SUCCESS) return err;
-        record(slot, 1);
-        if (opt.precision != KM_PREC_FP32)
-          KM_HIP_CHECK(mfma_reresolve(this, b0, br), KMEANS_ERROR_CLOSEST_CENTERS);
-        record(slot, 2);
-        if (learned) KM_HIP_CHECK(learned_collect(this, iters), KMEANS_ERROR_CLOSEST_CENTERS);
+        if (graphs && b0 == 0 && br == m &&
+            (err = graph_iteration(devconv, &graph_ran)) != KMEANS_SUCCESS)
+          return err;
+        if (graph_ran) {
+          hipLaunchKernelGGL(k_slot_commit, dim3(1), dim3(1), 0, stream, dstage, dcomp, dempty_hist,
+                             slot, prev0, criterion, devconv ? dctl : nullptr);
+          KM_LAUNCH_CHECK(KMEANS_ERROR_COMPUTE);
+        } else {
+          record(slot, 0);
+          if ((err = closestCenters(b0, br)) != KMEANS_SUCCESS) return err;
+          record(slot, 1);
+          if (opt.precision != KM_PREC_FP32)
+            KM_HIP_CHECK(mfma_reresolve(this, b0, br), KMEANS_ERROR_CLOSEST_CENTERS);
+          record(slot, 2);
+          if (learned) KM_HIP_CHECK(learned_collect(this, iters), KMEANS_ERROR_CLOSEST_CENTERS);
+        }
       }
-      if ((err = clusterCenters(b0, br, dempty_hist + slot)) != KMEANS_SUCCESS) return err;
-      record(slot, 3);
-      if ((err = compactness(b0, br, slot)) != KMEANS_SUCCESS) return err;
-      record(slot, 4);
+      if (!graph_ran) {
+        if ((err = clusterCenters(b0, br, dempty_hist + slot)) != KMEANS_SUCCESS) return err;
+        record(slot, 3);
+        if ((err = compactness(b0, br, slot)) != KMEANS_SUCCESS) return err;
+        record(slot, 4);
+      }
     }
     if (devconv) {
-      hipLaunchKernelGGL(k_converge, dim3(1), dim3(1), 0, stream, dcomp, slot, prev0, criterion,
-                         dctl);
-      KM_LAUNCH_CHECK(KMEANS_ERROR_COMPUTE);
+      if (!graph_ran) {  // (the graph iteration's commit made the test)
+        hipLaunchKernelGGL(k_converge, dim3(1), dim3(1), 0, stream, dcomp, slot, prev0, criterion,
+                           dctl);
+        KM_LAUNCH_CHECK(KMEANS_ERROR_COMPUTE);
+      }
       KM_HIP_CHECK(hipMemcpyAsync(hflag + slot, dctl, sizeof(int), hipMemcpyDeviceToHost, stream),
                    KMEANS_ERROR_COMPUTE);
       KM_HIP_CHECK(hipEventRecord(ev_iter[slot % 3], stream), KMEANS_ERROR_COMPUTE);

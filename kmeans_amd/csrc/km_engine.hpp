@@ -88,6 +88,20 @@ struct Engine {
   long long last_reresolved = 0;
   kmeansStats stats{};
 
+  // FP32 iterations over resident points in one canonical block with timing
+  // off run as executable hipGraphs: the assign, the update and the
+  // compactness reduction of one iteration, one graph per (reuse state,
+  // parity, device-side stop); the graph writes the compactness and the
+  // empty-cluster count to dstage, and an ordinary launch moves them into
+  // the iteration's slot.  Dropped when the points change; graph_ok false
+  // after a failed capture (the loop then launches the kernels directly).
+  hipGraphExec_t gexec[8] = {};
+  bool graph_ok = true;
+  double *dstage = nullptr;  // [compactness, empty count (as int)]
+  void graphs_drop();
+  bool graph_eligible(bool minibatch) const;
+  kmeansErrorStatus graph_iteration(bool devconv, bool *ran);
+
   // streamed mode: points stay in host memory and pass through HBM in tiles
   // of `tile` rows (a multiple of KM_BLOCK) on a copy stream, double-buffered
   bool streaming = false;

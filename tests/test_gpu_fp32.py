@@ -369,6 +369,76 @@ def test_engine_criterion_stops_like_reference(orc, prec):
     e.close()
 
 
+@pytest.mark.parametrize("n,d,k", [(100_000, 32, 64), (3001, 7, 5), (40_000, 128, 300)])
+def test_graph_iterations_equal_eager(orc, n, d, k):
+    # FP32 iterations over one canonical block with timing off run as graph
+    # launches (one graph per reuse state and parity): every run length, a
+    # device-side convergence run and new centres between runs give the
+    # oracle's results, and the eagerly launched engine (timing on) the same
+    print(f"graph iterations n={n} d={d} k={k}", flush=True)
+    X, _, _, _ = orc.mixture(30 + d, n, d, k, -8, 8, 1.0, 1.5)
+    C0 = X[:: n // k][:k].copy()
+    g = km.KMeans(n, d, k)
+    e = km.KMeans(n, d, k, timing=True)
+    for eng in (g, e):
+        eng.set_data(X)
+        eng.set_centers(C0)
+    done = 0
+    for steps in (1, 2, 3, 1):
+        for eng in (g, e):
+            eng.run(steps)
+        done += steps
+        ref = orc.lloyd(X, C0, done)
+        for eng in (g, e):
+            assert np.array_equal(eng.labels(), ref["labels"]), done
+            assert np.array_equal(eng.centers(), ref["centers"]), done
+            assert eng.stats()["compactness"] == ref["comp_hist"][-1], done
+    assert g.stats()["graph_launches"] == done
+    assert e.stats()["graph_launches"] == 0
+    # new centres, then a run with the device-side convergence test
+    C1 = X[1 :: n // k][:k].copy()
+    ref = orc.lloyd(X, C1, 60, criterion=1e-5)
+    for eng in (g, e):
+        eng.set_centers(C1)
+        eng.run(60, 1e-5)
+        st = eng.stats()
+        assert st["iterations"] == done + ref["iters"]
+        assert st["compactness"] == ref["comp_hist"][-1]
+        assert st["rel_err"] == ref["rel_err"]
+        assert np.array_equal(eng.labels(), ref["labels"])
+        assert np.array_equal(eng.centers(), ref["centers"])
+    assert g.stats()["graph_launches"] > done
+    g.close()
+    e.close()
+
+
+def test_graph_iterations_after_steps_and_new_points(orc):
+    # the step calls (assign, update) between graph runs and new points of
+    # the same shape: the graph runs continue from the state they left
+    n, d, k = 20_000, 16, 24
+    X, _, _, _ = orc.mixture(41, n, d, k, -8, 8, 1.0, 1.0)
+    X2, _, _, _ = orc.mixture(42, n, d, k, -8, 8, 1.0, 1.0)
+    C0 = X[:k].copy()
+    g = km.KMeans(n, d, k)
+    g.set_data(X)
+    g.set_centers(C0)
+    g.run(2)
+    assert km.lib().kmeansAssign(g._h) == 0
+    assert km.lib().kmeansUpdateCenters(g._h) == 0
+    g.run(2)
+    ref = orc.lloyd(X, C0, 5)
+    assert np.array_equal(g.labels(), ref["labels"])
+    assert np.array_equal(g.centers(), ref["centers"])
+    g.set_data(X2)
+    g.set_centers(X2[:k].copy())
+    g.run(3)
+    ref = orc.lloyd(X2, X2[:k].copy(), 3)
+    assert np.array_equal(g.labels(), ref["labels"])
+    assert np.array_equal(g.centers(), ref["centers"])
+    assert g.stats()["graph_launches"] == 7
+    g.close()
+
+
 @pytest.mark.parametrize("crit", [-1.0, 0.0])
 def test_engine_counts_empty_clusters_per_iteration(orc, crit):
     # centre 4 attracts no point in any iteration: one empty cluster per
