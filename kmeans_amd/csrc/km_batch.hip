@@ -68,7 +68,7 @@ struct Batch {
   double *comp = nullptr;
   long *choff = nullptr;     // first chunk slot of each instance
   long slots = 0;            // chunk slots in total
-  int4 *desc = nullptr;      // per slot: instance base row, member range, (local chunk, cluster)
+  int4 *desc = nullptr;      // per slot: instance, member range, (local chunk, cluster)
   int *empty = nullptr;
   int *order = nullptr;  // instances, largest first
   int2 *items = nullptr;
@@ -715,33 +715,44 @@ __device__ inline double csum2(const double *v, int m, int st) {
   return tot;
 }
 
-// Exclusive scan of v[0..n) in LDS by the whole block (n <= 1024); tmp
-// holds blockDim.x + 1 ints.  Returns the total.
+// Exclusive scan of v[0..n) in LDS by the whole block (a multiple of 64
+// threads, at most 1024); tmp holds at least 65 ints.  Each thread scans a
+// run of ceil(n / threads) entries; the runs' totals are scanned by wave
+// shuffles, then the waves' totals.  Returns the total (integer sums: any
+// order gives the same values).
 __device__ int block_scan_any(int *v, int n, int *tmp) {
-  const int t = threadIdx.x, nt = blockDim.x;
-  const int per = (n + nt - 1) / nt;
+  const int t = threadIdx.x, nt = blockDim.x, lane = t & 63, w = t >> 6, nw = nt >> 6;
+  const int per = (n + nt - 1) / nt, i0 = min(n, t * per), i1 = min(n, i0 + per);
   int s = 0;
-  for (int i = t * per; i < min(n, (t + 1) * per); ++i) s += v[i];
-  tmp[t] = s;
+  for (int i = i0; i < i1; ++i) s += v[i];
+  int x = s;  // inclusive scan over the wave
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) tmp[w] = x;
   __syncthreads();
-  if (t == 0) {
-    int a = 0;
-    for (int i = 0; i < nt; ++i) {
-      const int x = tmp[i];
-      tmp[i] = a;
-      a += x;
+  if (t < 64) {
+    const int wt = t < nw ? tmp[t] : 0;
+    int z = wt;
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+      const int y = __shfl_up(z, o);
+      if (t >= o) z += y;
     }
-    tmp[nt] = a;
+    if (t < nw) tmp[16 + t] = z - wt;
+    if (t == nw - 1) tmp[64] = z;
   }
   __syncthreads();
-  int a = tmp[t];
-  for (int i = t * per; i < min(n, (t + 1) * per); ++i) {
-    const int x = v[i];
-    v[i] = a;
-    a += x;
+  int acc = tmp[16 + w] + x - s;
+  for (int i = i0; i < i1; ++i) {
+    const int y = v[i];
+    v[i] = acc;
+    acc += y;
   }
   __syncthreads();
-  const int total = tmp[nt];
+  const int total = tmp[64];
   __syncthreads();
   return total;
 }
@@ -756,6 +767,8 @@ __device__ int block_scan_any(int *v, int n, int *tmp) {
 // With reuse (rstate = [the other parity holds the previous update of these
 // points, its parity]): each cluster's first member position at which this
 // sort differs from the previous one goes to fd, for k_batch_chunks_wave.
+// Labels are read eight per lane ahead of their use, so each wave waits for
+// memory once per 512 points.
 // Parity of an update: the one the previous update did not write (rstate[1]
 // once rstate[0] says it wrote one); the arrays come in pairs.
 __device__ inline int upd_parity(const int *rstate) { return rstate[0] ? rstate[1] ^ 1 : 0; }
@@ -766,18 +779,20 @@ __global__ __launch_bounds__(1024) void k_batch_sort(
     int *gcnt0, int *gcnt1, int *goff0, int *goff1, int *gcpre0, int *gcpre1,
     const long *__restrict__ choff, int4 *__restrict__ desc, int *__restrict__ counter,
     const int *__restrict__ rstate, int *__restrict__ fd, int *__restrict__ imin) {
-  extern __shared__ int sh[];  // cur[16][k], offs[k], cpre[k]
-  int *cur = sh, *offs = sh + 16 * k, *cpre = offs + k;
+  extern __shared__ int sh[];  // cur[16][k], offs[k], cpre[k], cnt[k], fdc[k], ims[k]
+  int *cur = sh, *offs = sh + 16 * k, *cpre = offs + k, *cnts = cpre + k, *fdc = cnts + k,
+      *ims = fdc + k;
   const bool reuse = rstate[0] != 0;
   const int par = upd_parity(rstate);
   int *__restrict__ perm = par ? perm1 : perm0;
   int *__restrict__ gcnt = par ? gcnt1 : gcnt0;
   int *__restrict__ goff = par ? goff1 : goff0;
   int *__restrict__ gcpre = par ? gcpre1 : gcpre0;
-  __shared__ int tmp[1025];
+  __shared__ int tmp[65];
   __shared__ int item_s;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const int nbits = k > 1 ? 32 - __clz(k - 1) : 0;
   for (;;) {
     if (t == 0) item_s = atomicAdd(counter, 1);
     __syncthreads();
@@ -787,15 +802,36 @@ __global__ __launch_bounds__(1024) void k_batch_sort(
     const long n = nvec[inst], base = xoff[inst];
     const int *lab = labels + base;
     for (int e = t; e < 16 * k; e += blockDim.x) cur[e] = 0;
+    for (int j = t; j < k; j += blockDim.x) {
+      fdc[j] = 0;
+      ims[j] = reuse ? imin[(size_t)inst * k + j] : 0;
+    }
     __syncthreads();
     const long S = ((n + 15) / 16 + 63) / 64 * 64;
     const long a = min(n, (long)w * S), e = min(n, a + S);
-    for (long p = a + lane; p < e; p += 64) atomicAdd(&cur[w * k + lab[p]], 1);
+    // per-(wave, label) counts; fdc[j] = members of j below its smallest
+    // changed row, i.e. the length of j's unchanged member prefix (the list
+    // is in ascending row order)
+    for (long p0 = a; p0 < e; p0 += 512) {
+      int Lv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const long p = p0 + u * 64 + lane;
+        Lv[u] = p < e ? lab[p] : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (Lv[u] >= 0) {
+          atomicAdd(&cur[w * k + Lv[u]], 1);
+          if (p0 + u * 64 + lane < ims[Lv[u]]) atomicAdd(&fdc[Lv[u]], 1);
+        }
+    }
     __syncthreads();
     for (int j = t; j < k; j += blockDim.x) {
       int c = 0;
       for (int ww = 0; ww < 16; ++ww) c += cur[ww * k + j];
       offs[j] = c;
+      cnts[j] = c;
       cpre[j] = (c + KM_CHUNK - 1) / KM_CHUNK;
       gcnt[(size_t)inst * k + j] = c;
     }
@@ -811,60 +847,52 @@ __global__ __launch_bounds__(1024) void k_batch_sort(
       }
       goff[(size_t)inst * k + j] = offs[j];
       gcpre[(size_t)inst * (k + 1) + j] = cpre[j];
+      if (reuse) fd[(size_t)inst * k + j] = fdc[j];
+      imin[(size_t)inst * k + j] = 0x7fffffff;  // cleared for the next iteration
     }
     if (t == 0) gcpre[(size_t)inst * (k + 1) + k] = nch;
     __syncthreads();
-    // chunk descriptors (instance base, first and end member position); the
-    // instance's unused slots get empty ranges
+    // chunk descriptors (instance, first and end member position, chunk and
+    // cluster); the instance's unused slots get empty ranges
     {
       int4 *dd = desc + choff[inst];
       const int cap = (int)(choff[inst + 1] - choff[inst]);
       for (int j = t; j < k; j += blockDim.x) {
-        const int o = offs[j], nj = gcnt[(size_t)inst * k + j];
+        const int o = offs[j], nj = cnts[j];
         int q = 0;
         for (int c = cpre[j], a0 = o; a0 < o + nj; ++c, a0 += KM_CHUNK, ++q)
-          dd[c] = make_int4((int)base, a0, min(o + nj, a0 + KM_CHUNK), q * 1024 + j);
+          dd[c] = make_int4(inst, a0, min(o + nj, a0 + KM_CHUNK), q * 1024 + j);
       }
       for (int c = nch + t; c < cap; c += blockDim.x) dd[c] = make_int4(0, 0, 0, 0);
     }
     int *cw = cur + w * k;
-    for (long p0 = a; p0 < e; p0 += 64) {
-      const long p = p0 + lane;
-      const bool valid = p < e;
-      const int L = valid ? lab[p] : 0;
-      uint64_t m = __ballot(valid);
-      for (int bt = 0; bt < 10; ++bt) {
-        const uint64_t bb = __ballot(valid && ((L >> bt) & 1));
-        m &= ((L >> bt) & 1) ? bb : ~bb;
+    for (long p0 = a; p0 < e; p0 += 512) {
+      int Lv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const long p = p0 + u * 64 + lane;
+        Lv[u] = p < e ? lab[p] : 0;
       }
-      if (!valid) m = 0;
-      const int rank = __popcll(m & below);
-      int pos = 0;
-      if (valid) pos = cw[L] + rank;
-      __builtin_amdgcn_wave_barrier();
-      if (valid) perm[base + pos] = (int)p;
-      if (valid && rank == 0) cw[L] = pos + __popcll(m);
-      __builtin_amdgcn_wave_barrier();
-    }
-    // First differences: the members of cluster j below its smallest changed
-    // row (imin, recorded by the assign kernels since the previous sort) are
-    // the previous sort's first members of j, in the same positions; their
-    // count, found by a binary search of j's ascending member list, is where
-    // the lists first differ.  imin is then cleared for the next iteration.
-    __threadfence_block();  // the block's perm stores, visible after the barrier
-    __syncthreads();
-    for (int j = t; j < k; j += blockDim.x) {
-      int *imj = imin + (size_t)inst * k + j;
-      if (reuse) {
-        const int im = *imj, o = offs[j];
-        int lo = 0, hi = gcnt[(size_t)inst * k + j];
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (perm[base + o + mid] < im) lo = mid + 1; else hi = mid;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (p0 + u * 64 >= e) break;  // uniform over the wave
+        const long p = p0 + u * 64 + lane;
+        const bool valid = p < e;
+        const int L = Lv[u];
+        uint64_t m = __ballot(valid);
+        for (int bt = 0; bt < nbits; ++bt) {
+          const uint64_t bb = __ballot(valid && ((L >> bt) & 1));
+          m &= ((L >> bt) & 1) ? bb : ~bb;
         }
-        fd[(size_t)inst * k + j] = lo;
+        if (!valid) m = 0;
+        const int rank = __popcll(m & below);
+        int pos = 0;
+        if (valid) pos = cw[L] + rank;
+        __builtin_amdgcn_wave_barrier();
+        if (valid) perm[base + pos] = (int)p;
+        if (valid && rank == 0) cw[L] = pos + __popcll(m);
+        __builtin_amdgcn_wave_barrier();
       }
-      *imj = 0x7fffffff;
     }
     __syncthreads();
   }
@@ -879,8 +907,9 @@ __global__ __launch_bounds__(1024) void k_batch_sort(
 // cluster's first changed position copies the previous update's partials.
 template <int DT>
 __global__ __launch_bounds__(256) void k_batch_chunks_wave(
-    long slots, const int4 *__restrict__ desc, int d, const float *__restrict__ X,
-    const int *perm0, const int *perm1, double *part0, double *part1, int nb, int k,
+    long slots, const int4 *__restrict__ desc, const long *__restrict__ xoff, int d,
+    const float *__restrict__ X, const int *perm0, const int *perm1, double *part0, double *part1,
+    int nb, int k,
     const long *__restrict__ choff, const int *__restrict__ fd, const int *gcnt0,
     const int *gcnt1, const int *gcpre0, const int *gcpre1, const int *__restrict__ rstate) {
   __shared__ float xs[4][64][DT + 1];
@@ -895,15 +924,11 @@ __global__ __launch_bounds__(256) void k_batch_chunks_wave(
   const int *__restrict__ gcnt_prev = par ? gcnt0 : gcnt1;
   const int *__restrict__ gcpre_prev = par ? gcpre0 : gcpre1;
   const int4 dsc = desc[s];
-  const long base = dsc.x;
   const int a = dsc.y, e = dsc.z;
   if (a >= e) return;
+  const int lo = dsc.x;  // the slot's instance
+  const long base = xoff[lo];
   if (rstate[0]) {
-    int lo = 0, hi = nb - 1;  // the slot's instance
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (choff[mid] <= s) lo = mid; else hi = mid - 1;
-    }
     const int j = dsc.w & 1023, q = dsc.w >> 10;
     const int q0 = q * KM_CHUNK, end_new = q0 + (e - a);
     const int end_old = min(q0 + KM_CHUNK, gcnt_prev[(size_t)lo * k + j]);
@@ -995,12 +1020,20 @@ __device__ inline void block_max2(float a, float b, float *ra, float *rb, float2
 
 // Also each centre's drift (an upper bound of how far it moved, 0 exactly for
 // an unmoved centre) and the instance's largest drift.
+// The clusters go in slabs whose sums fit in LDS: first every (cluster,
+// column) sum of the slab and the old centre values, one per thread, so the
+// loads of the whole slab are in flight together; then one thread per
+// cluster finishes it from LDS (the same sums and chains as before, in the
+// same orders).
+#define B_FIN_LDS 4352  // FP64 sums per slab (256 clusters at d = 16)
 __global__ __launch_bounds__(256) void k_batch_finalize(
     int d, int k, const long *__restrict__ choff, const int *gcnt0, const int *gcnt1,
     const int *gcpre0, const int *gcpre1, const double *part0, const double *part1,
     float *__restrict__ C, float *__restrict__ nc, float2 *__restrict__ bnd,
     double *__restrict__ comp_out, int *__restrict__ empty_out, float *__restrict__ drift,
     float *__restrict__ dmaxv, const int *__restrict__ rstate) {
+  __shared__ double Ss[B_FIN_LDS];
+  __shared__ float Co[B_FIN_LDS];
   __shared__ double cj[B_MAXK];
   __shared__ int nempty;
   __shared__ float rx[256], ry[256];
@@ -1009,43 +1042,57 @@ __global__ __launch_bounds__(256) void k_batch_finalize(
   const int *__restrict__ gcnt = par ? gcnt1 : gcnt0;
   const int *__restrict__ gcpre = par ? gcpre1 : gcpre0;
   const double *__restrict__ part = par ? part1 : part0;
-  const int inst = blockIdx.x, t = threadIdx.x;
+  const int inst = blockIdx.x, t = threadIdx.x, ld = d + 1;
   if (t == 0) nempty = 0;
-  __syncthreads();
-  const double *pi = part + choff[inst] * (d + 1);
+  const double *pi = part + choff[inst] * ld;
   const int *cp = gcpre + (size_t)inst * (k + 1);
   float *Ci = C + (size_t)inst * k * d;
-  for (int j = t; j < k; j += blockDim.x) {
-    const int nj = gcnt[(size_t)inst * k + j], c0 = cp[j], ncj = cp[j + 1] - cp[j];
-    const double *pj = pi + (long)c0 * (d + 1);
-    double a = 0.0, b = 0.0;
-    float s2 = 0.0f, dq = 0.0f, dmx = 0.0f;
-    for (int qq = 0; qq < d; ++qq) {
-      const double S = csum2(pj + qq, ncj, d + 1);
-      const float old = Ci[(size_t)j * d + qq];
-      float cv = old;
-      if (nj > 0) cv = (float)(S / (double)nj);
-      Ci[(size_t)j * d + qq] = cv;
-      const float tq = cv - old;
-      dq = __builtin_fmaf(tq, tq, dq);
-      const float at = fabsf(tq);
-      if (!(at <= dmx)) dmx = at;  // NaN propagates
-      s2 = __builtin_fmaf(cv, cv, s2);
-      const double cd = (double)cv;
-      a += cd * S;
-      b += cd * cd;
+  const int JS = min(k, B_FIN_LDS / ld);
+  for (int j0 = 0; j0 < k; j0 += JS) {
+    const int jn = min(JS, k - j0);
+    for (int e = t; e < jn * ld; e += blockDim.x) {
+      const int jj = e / ld, qq = e - jj * ld, j = j0 + jj;
+      const int c0 = cp[j];
+      Ss[e] = csum2(pi + (long)c0 * ld + qq, cp[j + 1] - c0, ld);
+      if (qq < d) Co[jj * d + qq] = Ci[(size_t)j * d + qq];
     }
-    const double Q = csum2(pj + d, ncj, d + 1);
-    nc[(size_t)inst * k + j] = s2;
-    mx = fmaxf(mx, s2);
-    dm = fmaxf(dm, bf16_residual_sq(Ci + (size_t)j * d, d));
-    cj[j] = (Q - 2.0 * a) + (double)nj * b;
-    if (nj == 0) atomicAdd(&nempty, 1);
-    const float dv = drift_bound(dq, dmx, d);
-    drift[(size_t)inst * k + j] = dv;
-    if (!(dv <= md)) md = dv;
+    __syncthreads();
+    for (int j = j0 + t; j < j0 + jn; j += blockDim.x) {
+      const double *Sj = Ss + (j - j0) * ld;
+      const float *Cold = Co + (j - j0) * d;
+      const int nj = gcnt[(size_t)inst * k + j];
+      double a = 0.0, b = 0.0;
+      float s2 = 0.0f, dq = 0.0f, dmx = 0.0f, res = 0.0f;
+      for (int qq = 0; qq < d; ++qq) {
+        const double S = Sj[qq];
+        const float old = Cold[qq];
+        float cv = old;
+        if (nj > 0) cv = (float)(S / (double)nj);
+        Ci[(size_t)j * d + qq] = cv;
+        const float tq = cv - old;
+        dq = __builtin_fmaf(tq, tq, dq);
+        const float at = fabsf(tq);
+        if (!(at <= dmx)) dmx = at;  // NaN propagates
+        s2 = __builtin_fmaf(cv, cv, s2);
+        // FP32 chain of (c - bf16(c))^2 over the dimensions (each residual exact)
+        const float r = cv - __uint_as_float((uint32_t)b_bf16(cv) << 16);
+        res = __builtin_fmaf(r, r, res);
+        const double cd = (double)cv;
+        a += cd * S;
+        b += cd * cd;
+      }
+      const double Q = Sj[d];
+      nc[(size_t)inst * k + j] = s2;
+      mx = fmaxf(mx, s2);
+      dm = fmaxf(dm, res);
+      cj[j] = (Q - 2.0 * a) + (double)nj * b;
+      if (nj == 0) atomicAdd(&nempty, 1);
+      const float dv = drift_bound(dq, dmx, d);
+      drift[(size_t)inst * k + j] = dv;
+      if (!(dv <= md)) md = dv;
+    }
+    __syncthreads();
   }
-  __syncthreads();
   if (t == 0) {
     comp_out[inst] = csum2(cj, k, 1);
     empty_out[inst] = nempty;
@@ -1216,7 +1263,7 @@ static hipError_t launch_assign(Batch *b, int P, int nitems, const int2 *items) 
 static hipError_t launch_update(Batch *b, int slot) {
   double *co = b->comp + (size_t)slot * b->nb;
   int *eo = b->empty + (size_t)slot * b->nb;
-  const size_t lds = sizeof(int) * 18 * (size_t)b->k;
+  const size_t lds = sizeof(int) * 21 * (size_t)b->k;
   static size_t attr = 64 * 1024;
   if (lds > attr) {
     hipError_t e = hipFuncSetAttribute((const void *)k_batch_sort,
@@ -1230,7 +1277,8 @@ static hipError_t launch_update(Batch *b, int slot) {
                      b->choff, b->desc, b->counters + 1, b->rstate, b->fd, b->imin);
 #define CHW(DT)                                                                                  \
   hipLaunchKernelGGL(k_batch_chunks_wave<DT>, dim3((unsigned)((b->slots + 3) / 4)), dim3(256), 0,  \
-                     b->stream, b->slots, b->desc, b->d, b->X, b->perm[0], b->perm[1], b->part[0], \
+                     b->stream, b->slots, b->desc, b->xoff, b->d, b->X, b->perm[0], b->perm[1],      \
+                     b->part[0],                                                                   \
                      b->part[1], b->nb, b->k, b->choff, b->fd, b->gcnt[0], b->gcnt[1], b->gcpre[0], \
                      b->gcpre[1], b->rstate)
   switch (pick_dt(b->d)) {
