@@ -306,10 +306,15 @@ void k_batch_assign_bf16(
   constexpr int Dp = KS * 16;
   const int Kp = (k + 31) / 32 * 32;
   extern __shared__ __attribute__((aligned(16))) unsigned char bsm[];
-  b_u32x4 *Ch = reinterpret_cast<b_u32x4 *>(bsm);                 // Kp x (Dp/8) chunks
+  // LDS rows are laid out so that the lanes of a wave read distinct banks:
+  // the BF16 chunks as Dp/8 planes of Kp 16-byte entries (lane col of half h
+  // reads plane 2 ks + h, entry t * 32 + col), the FP32 centres at an odd
+  // row stride cs (lanes reading different centres' dimension q)
+  const int cs = d | 1;
+  b_u32x4 *Ch = reinterpret_cast<b_u32x4 *>(bsm);                 // (Dp/8) x Kp chunks
   float *bias = reinterpret_cast<float *>(Ch + (size_t)Kp * (Dp / 8));  // Kp
-  float *Cf = bias + Kp;                                            // k x d
-  float *ncs = Cf + (size_t)k * d;                                  // k
+  float *Cf = bias + Kp;                                            // k x cs
+  float *ncs = Cf + (size_t)k * cs;                                 // k
   uint16_t *olist = reinterpret_cast<uint16_t *>(ncs + k);          // P < 2^16 (drift bounds)
   __shared__ int item_s, ocnt;
   __shared__ float bndv[3];
@@ -341,14 +346,17 @@ void k_batch_assign_bf16(
           const float v1 = (j < k && d0 + 1 < d) ? Ci[(size_t)j * d + d0 + 1] : 0.0f;
           wv[u] = (uint32_t)b_bf16(v0) | ((uint32_t)b_bf16(v1) << 16);
         }
-        Ch[e] = b_u32x4{wv[0], wv[1], wv[2], wv[3]};
+        Ch[(size_t)(q8 / 8) * Kp + j] = b_u32x4{wv[0], wv[1], wv[2], wv[3]};
       }
       for (int j = tid; j < Kp; j += 256) {
         const float n2 = j < k ? nc[(size_t)inst * k + j] : 0.0f;
         bias[j] = j < k ? -0.5f * n2 : -INFINITY;
         if (j < k) ncs[j] = n2;
       }
-      for (int e = tid; e < k * d; e += 256) Cf[e] = Ci[e];
+      for (int e = tid; e < k * d; e += 256) {
+        const int j = e / d;
+        Cf[(size_t)j * cs + (e - j * d)] = Ci[e];
+      }
       // the instance's bounds (k_batch_bounds / k_batch_finalize): the nc
       // chains are within gamma_d of the sums, and so are the chains of the
       // BF16 residuals c - bf16(c) (exact in FP32)
@@ -398,7 +406,7 @@ void k_batch_assign_bf16(
             // the plain path's chains of a and r, from the LDS centres, and
             // the point's own norm bound from the same row
             const float *xr = X + gp * d;
-            const float *ca = Cf + (size_t)a * d, *cb = Cf + (size_t)r * d;
+            const float *ca = Cf + (size_t)a * cs, *cb = Cf + (size_t)r * cs;
             float sa = -0.5f * ncs[a], sr = -0.5f * ncs[r], sq = 0.0f;
             for (int q2 = 0; q2 < d; ++q2) {
               const float xv = xr[q2];
@@ -506,7 +514,7 @@ void k_batch_assign_bf16(
         }
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
-          const b_u32x4 a = Ch[(t * 32 + col) * (Dp / 8) + ks * 2 + h];
+          const b_u32x4 a = Ch[(size_t)(ks * 2 + h) * Kp + t * 32 + col];
           acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(b_bf16x8, a),
                                                          __builtin_bit_cast(b_bf16x8, bf[ks]), acc, 0, 0, 0);
         }
@@ -603,7 +611,7 @@ void k_batch_assign_bf16(
       float sx = -INFINITY;
       if (ok && cj >= 0 && cj < k) {
         float sc = -0.5f * ncs[cj];
-        const float *cr = Cf + (size_t)cj * d;
+        const float *cr = Cf + (size_t)cj * cs;
 #pragma unroll
         for (int q = 0; q < Dp; ++q)
           if (q < d) sc = __builtin_fmaf(xv[q], cr[q], sc);
@@ -673,7 +681,7 @@ void k_batch_assign_bf16(
         int bj = 0x7fffffff;
         for (int j = lane; j < k; j += 64) {
           float sc = -0.5f * ncs[j];
-          const float *cr = Cf + (size_t)j * d;
+          const float *cr = Cf + (size_t)j * cs;
 #pragma unroll
           for (int q = 0; q < Dp; ++q)
             if (q < d) sc = __builtin_fmaf(xq[q], cr[q], sc);
@@ -1229,7 +1237,7 @@ static hipError_t launch_assign_x(Batch *b, int P, int nitems, const int2 *items
 
 static size_t bf16_lds(const Batch *b) {
   const int Kp = (b->k + 31) / 32 * 32, Dp = b->d <= 16 ? 16 : 32;
-  return (size_t)Kp * Dp * 2 + (size_t)Kp * 4 + (size_t)b->k * b->d * 4 + (size_t)b->k * 4;
+  return (size_t)Kp * Dp * 2 + (size_t)Kp * 4 + (size_t)b->k * (b->d | 1) * 4 + (size_t)b->k * 4;
 }
 
 static hipError_t launch_assign(Batch *b, int P, int nitems, const int2 *items) {
