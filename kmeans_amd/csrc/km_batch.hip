@@ -75,11 +75,13 @@ struct Batch {
   int nitems = 0, P = 0;
   int *counters = nullptr;  // assign, update
   unsigned long long *nexact = nullptr;  // BF16 assign: points searched exactly
-  // drift bounds of the BF16 assign (null when its LDS cannot also hold the
-  // open list): per point (u, l, l3, r), per centre drift, per instance the
-  // largest drift and largest |x|, the valid flag, the kept / paired / scored
-  // counts
+  // drift bounds of the BF16 assign: per point (u, l, l3, r), the work
+  // items' open lists, per centre drift, per instance the largest drift and
+  // largest |x|, the valid flag, the kept / paired / scored counts
   float4 *xb = nullptr;
+  uint16_t *olist = nullptr;  // per work item: its open positions (P entries at item * P)
+  int *ocnt = nullptr;        // per work item: their count
+  size_t ocnt_cap = 0;
   float *drift = nullptr, *dmaxv = nullptr, *xmax = nullptr;
   int *bvalid = nullptr;
   unsigned long long *nkp = nullptr;
@@ -278,19 +280,181 @@ typedef __attribute__((ext_vector_type(4))) unsigned int b_u32x4;
 __device__ inline int b_row(int r, int h) { return 8 * (r >> 2) + 4 * h + (r & 3); }
 __device__ inline uint16_t b_bf16(float f) { return bf16_rne(f); }
 
+// An instance's certificate constants from its raw FP32 chains (bnd = max
+// ||c||^2, max ||c - bf16(c)||^2): max ||c||^2 and the bounds of ||c|| and
+// ||c - bf16(c)||, each chain within gamma_d of its sum.
+__device__ inline void inst_bounds(float2 bb, int d, float *cmax, float *ncmax, float *dcmax) {
+  const float nm = bb.x * (1.0f + (float)d * 1.2e-7f + 1e-6f) + 1e-30f;
+  *ncmax = nm;
+  *cmax = sqrtf(nm) * (1.0f + 1e-6f);
+  *dcmax = sqrtf(bb.y * (1.0f + (float)d * 1.2e-7f + 1e-6f)) * (1.0f + 1e-6f) + 1e-30f;
+}
+
+// Drift-bound classification of one work item's points, one workgroup per
+// item ahead of the BF16 assign (a kernel of its own, at full occupancy, so
+// that its gathers overlap across many items): kept points get their moved
+// bounds; two-candidate points are decided by the FP32 chains of their label
+// and runner-up (the centres and norms read from global memory, the values
+// the assign stages in LDS), lowest index on equal chains, and kept without
+// them when neither centre moved; the open ones go to the item's list
+// (olist + item * P, count ocnt[item]), which the assign then scores.
+// Nothing to do before the first assign of the current data and centres
+// (*bvalid == 0): the assign then scores every point.
+__global__ __launch_bounds__(256) void k_batch_classify(
+    int nitems, const int2 *__restrict__ items, int P, const long *__restrict__ xoff,
+    const int *__restrict__ nvec, int d, int k, const float *__restrict__ X,
+    const float *__restrict__ C, const float *__restrict__ nc, const float2 *__restrict__ bnd,
+    int *__restrict__ labels, float4 *__restrict__ xb, const int *__restrict__ bvalid,
+    const float *__restrict__ drift, const float *__restrict__ dmaxv,
+    const float *__restrict__ xmax, unsigned long long *__restrict__ nkp, int *__restrict__ imin,
+    uint16_t *__restrict__ olist, int *__restrict__ ocnt) {
+  extern __shared__ float csm[];  // the instance's centres (k x cs), norms and drifts (k each)
+  __shared__ int cnt_s;
+  __shared__ unsigned kp_s[2];
+  const int it = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
+  if (it >= nitems || !*bvalid) return;
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  if (tid == 0) {
+    cnt_s = 0;
+    kp_s[0] = kp_s[1] = 0;
+  }
+  __syncthreads();
+  const int2 w = items[it];
+  const int inst = w.x;
+  const long rows = min((long)P, (long)nvec[inst] - w.y);
+  const long base = xoff[inst] + w.y;
+  const float Df = (float)d;
+  float cmax, ncmax, dcmax;
+  inst_bounds(bnd[inst], d, &cmax, &ncmax, &dcmax);
+  // centres in LDS at an odd row stride (the two-candidate chains read rows
+  // of different centres across the lanes), their norms and drifts
+  const int cs = d | 1;
+  float *Cs = csm, *ncs = csm + (size_t)k * cs, *dr = ncs + k;
+  {
+    const float *Ci = C + (size_t)inst * k * d;
+    for (int e = tid; e < k * d; e += blockDim.x) {
+      const int j = e / d;
+      Cs[(size_t)j * cs + (e - j * d)] = Ci[e];
+    }
+    for (int j = tid; j < k; j += blockDim.x) {
+      ncs[j] = nc[(size_t)inst * k + j];
+      dr[j] = drift[(size_t)inst * k + j];
+    }
+  }
+  __syncthreads();
+  const float dm = dmaxv[inst];
+  const float e32i = Df * 5.9604645e-8f * 1.0001f * (0.5f * ncmax + xmax[inst] * cmax);
+  const float t4 = 4.0f * e32i * 1.0001f + 1e-30f;
+  uint16_t *ol = olist + (size_t)it * P;
+  unsigned nk = 0, np = 0;
+  // four points per thread per pass: their labels and bounds are loaded
+  // together
+  for (long q0 = 0; q0 < rows; q0 += 1024) {
+    int la[4];
+    float4 bb[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long q = q0 + 256 * u + tid;
+      la[u] = 0;
+      bb[u] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+      if (q < rows) {
+        la[u] = labels[base + q];
+        bb[u] = xb[base + q];
+      }
+    }
+    float da[4], drr[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int r = __float_as_int(bb[u].w);
+      da[u] = dr[la[u]];
+      drr[u] = (r >= 0 && r < k) ? dr[r] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long q = q0 + 256 * u + tid;
+      int cls = 2;
+      if (q < rows) {
+        const long gp = base + q;
+        const int a = la[u];
+        const float4 b4 = bb[u];
+        const int r = __float_as_int(b4.w);
+        const float uu = (b4.x + da[u]) * (1.0f + 1e-6f);
+        const float l = fmaxf((b4.y - dm) * (1.0f - 1e-6f), 0.0f);
+        const float l3 = fmaxf((b4.z - dm) * (1.0f - 1e-6f), 0.0f);
+        if (l > uu && (l - uu) * (l + uu) * (1.0f - 1e-6f) > t4)
+          cls = 0;
+        else if (r >= 0 && r < k && r != a && l3 > uu &&
+                 (l3 - uu) * (l3 + uu) * (1.0f - 1e-6f) > t4)
+          cls = (da[u] == 0.0f && drr[u] == 0.0f) ? 0 : 1;
+        if (cls == 0) {
+          xb[gp] = make_float4(uu, l, l3, b4.w);
+          ++nk;
+        } else if (cls == 1) {
+          // the plain path's chains of a and r, and the point's own norm
+          // bound from the same row (read 16 dimensions at a time)
+          const float *xr = X + gp * d;
+          const float *ca = Cs + (size_t)a * cs, *cb = Cs + (size_t)r * cs;
+          float sa = -0.5f * ncs[a], sr = -0.5f * ncs[r], sq = 0.0f;
+          for (int q0 = 0; q0 < d; q0 += 16) {
+            float xv[16];
+#pragma unroll
+            for (int v = 0; v < 16; ++v) xv[v] = q0 + v < d ? xr[q0 + v] : 0.0f;
+#pragma unroll
+            for (int v = 0; v < 16; ++v)
+              if (q0 + v < d) {
+                sa = __builtin_fmaf(xv[v], ca[q0 + v], sa);
+                sr = __builtin_fmaf(xv[v], cb[q0 + v], sr);
+                sq = __builtin_fmaf(xv[v], xv[v], sq);
+              }
+          }
+          const bool tr = sr > sa || (sr == sa && r < a);
+          const int L = tr ? r : a, O = tr ? a : r;
+          const float sL = tr ? sr : sa, sO = tr ? sa : sr;
+          if (L != a) {
+            atomicMin(imin + (size_t)inst * k + a, (int)(w.y + q));
+            atomicMin(imin + (size_t)inst * k + L, (int)(w.y + q));
+          }
+          labels[gp] = L;
+          const float xn = sqrtf(sq * (1.0f + Df * 1.2e-7f + 1e-6f)) * (1.0f + 1e-6f) + 1e-30f;
+          xb[gp] = make_float4(dist_upper(xn, sL - e32i),
+                               fminf(dist_lower(xn, sO + e32i, Df), l3), l3,
+                               __int_as_float(O));
+          ++np;
+        }
+      }
+      const bool open = q < rows && cls == 2;
+      const uint64_t mk = __ballot(open);
+      if (mk) {
+        const int lead = __ffsll((long long)mk) - 1;
+        int ob = 0;
+        if (lane == lead) ob = atomicAdd(&cnt_s, __popcll(mk));
+        ob = __shfl(ob, lead);
+        if (open) ol[ob + __popcll(mk & below)] = (uint16_t)q;
+      }
+    }
+  }
+  if (nk) atomicAdd(&kp_s[0], nk);
+  if (np) atomicAdd(&kp_s[1], np);
+  __syncthreads();
+  if (tid == 0) {
+    ocnt[it] = cnt_s;
+    if (kp_s[0]) atomicAdd(nkp, (unsigned long long)kp_s[0]);
+    if (kp_s[1]) atomicAdd(nkp + 1, (unsigned long long)kp_s[1]);
+  }
+}
+
 //
 // Drift bounds across iterations (xb non-null, used when *bvalid): per point
 // (u, l, l3, r) as in the single-problem engine (u >= |x - c_label|, l <= the
 // distance to every other centre, r the runner-up, l3 <= the distance to
 // every centre but the label and r), valid for the centres of the last
 // assign; k_batch_finalize leaves each centre's drift and each instance's
-// largest.  Each work item first sorts its points into kept (the moved
-// bounds separate every other centre by more than the FP32 chains' error),
-// two-candidate (every centre but the label and r separated: the two FP32
-// chains decide, lowest index on equal chains; kept without them when
-// neither centre moved) and open; the open ones go into an LDS list, and only
-// they are scored.  E32 uses the instance's largest |x| (xmax).  Counts of
-// kept, paired and scored points go to nkp[0..2].
+// largest.  k_batch_classify has sorted each work item's points into kept
+// (the moved bounds separate every other centre by more than the FP32
+// chains' error), two-candidate (decided there) and open, whose list
+// (olist + item * P, ocnt) is all this kernel scores.  E32 uses the
+// instance's largest |x| (xmax).  Counts of kept and paired points go to
+// nkp[0..1] (from the classification), of scored points to nkp[2].
 // (four waves per SIMD for one k-step: 128 VGPRs, a few spilled outside the
 // tile loop; two k-steps keep the compiler's choice)
 template <int KS>
@@ -302,7 +466,8 @@ void k_batch_assign_bf16(
     int *__restrict__ labels, int *__restrict__ counter, unsigned long long *__restrict__ nexact,
     float big, float4 *__restrict__ xb, const int *__restrict__ bvalid,
     const float *__restrict__ drift, const float *__restrict__ dmaxv,
-    const float *__restrict__ xmax, unsigned long long *__restrict__ nkp, int *__restrict__ imin) {
+    const float *__restrict__ xmax, unsigned long long *__restrict__ nkp, int *__restrict__ imin,
+    const uint16_t *__restrict__ olist, const int *__restrict__ ocnt) {
   constexpr int Dp = KS * 16;
   const int Kp = (k + 31) / 32 * 32;
   extern __shared__ __attribute__((aligned(16))) unsigned char bsm[];
@@ -315,16 +480,14 @@ void k_batch_assign_bf16(
   float *bias = reinterpret_cast<float *>(Ch + (size_t)Kp * (Dp / 8));  // Kp
   float *Cf = bias + Kp;                                            // k x cs
   float *ncs = Cf + (size_t)k * cs;                                 // k
-  uint16_t *olist = reinterpret_cast<uint16_t *>(ncs + k);          // P < 2^16 (drift bounds)
-  __shared__ int item_s, ocnt;
+  __shared__ int item_s;
   __shared__ float bndv[3];
-  __shared__ unsigned long long exact_ct, kp_ct[3];
+  __shared__ unsigned long long exact_ct, kp_ct;
   const int tid = threadIdx.x, lane = tid & 63, h = lane >> 5, col = lane & 31;
-  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
   const bool bv = xb && *bvalid;
   if (tid == 0) {
     exact_ct = 0;
-    kp_ct[0] = kp_ct[1] = kp_ct[2] = 0;
+    kp_ct = 0;
   }
   int cur = -1;
   for (;;) {
@@ -360,13 +523,7 @@ void k_batch_assign_bf16(
       // the instance's bounds (k_batch_bounds / k_batch_finalize): the nc
       // chains are within gamma_d of the sums, and so are the chains of the
       // BF16 residuals c - bf16(c) (exact in FP32)
-      if (tid == 0) {
-        const float2 bb = bnd[inst];
-        const float ncmax = bb.x * (1.0f + (float)d * 1.2e-7f + 1e-6f) + 1e-30f;
-        bndv[0] = sqrtf(ncmax) * (1.0f + 1e-6f);
-        bndv[1] = ncmax;
-        bndv[2] = sqrtf(bb.y * (1.0f + (float)d * 1.2e-7f + 1e-6f)) * (1.0f + 1e-6f) + 1e-30f;
-      }
+      if (tid == 0) inst_bounds(bnd[inst], d, &bndv[0], &bndv[1], &bndv[2]);
       cur = inst;
     }
     __syncthreads();
@@ -375,77 +532,11 @@ void k_batch_assign_bf16(
     const long base = xoff[inst] + w.y;
     const int wv0 = tid >> 6;
     const float Df = (float)d;
-    long nopen = rows;  // the item's positions to score (through olist with bounds)
-    if (bv) {
-      if (tid == 0) ocnt = 0;
-      __syncthreads();
-      const float *dr = drift + (size_t)inst * k;
-      const float dm = dmaxv[inst];
-      const float e32i = Df * 5.9604645e-8f * 1.0001f * (0.5f * ncmax + xmax[inst] * cmax);
-      const float t4 = 4.0f * e32i * 1.0001f + 1e-30f;
-      unsigned nk = 0, np = 0;
-      for (long q0 = 0; q0 < rows; q0 += 256) {
-        const long q = q0 + tid;
-        int cls = 2;
-        if (q < rows) {
-          const long gp = base + q;
-          const int a = labels[gp];
-          const float4 b4 = xb[gp];
-          const int r = __float_as_int(b4.w);
-          const float u = (b4.x + dr[a]) * (1.0f + 1e-6f);
-          const float l = fmaxf((b4.y - dm) * (1.0f - 1e-6f), 0.0f);
-          const float l3 = fmaxf((b4.z - dm) * (1.0f - 1e-6f), 0.0f);
-          if (l > u && (l - u) * (l + u) * (1.0f - 1e-6f) > t4)
-            cls = 0;
-          else if (r >= 0 && r < k && r != a && l3 > u && (l3 - u) * (l3 + u) * (1.0f - 1e-6f) > t4)
-            cls = (dr[a] == 0.0f && dr[r] == 0.0f) ? 0 : 1;
-          if (cls == 0) {
-            xb[gp] = make_float4(u, l, l3, b4.w);
-            ++nk;
-          } else if (cls == 1) {
-            // the plain path's chains of a and r, from the LDS centres, and
-            // the point's own norm bound from the same row
-            const float *xr = X + gp * d;
-            const float *ca = Cf + (size_t)a * cs, *cb = Cf + (size_t)r * cs;
-            float sa = -0.5f * ncs[a], sr = -0.5f * ncs[r], sq = 0.0f;
-            for (int q2 = 0; q2 < d; ++q2) {
-              const float xv = xr[q2];
-              sa = __builtin_fmaf(xv, ca[q2], sa);
-              sr = __builtin_fmaf(xv, cb[q2], sr);
-              sq = __builtin_fmaf(xv, xv, sq);
-            }
-            const bool tr = sr > sa || (sr == sa && r < a);
-            const int L = tr ? r : a, O = tr ? a : r;
-            const float sL = tr ? sr : sa, sO = tr ? sa : sr;
-            if (L != a) {
-              atomicMin(imin + (size_t)inst * k + a, (int)(w.y + q));
-              atomicMin(imin + (size_t)inst * k + L, (int)(w.y + q));
-            }
-            labels[gp] = L;
-            const float xn = sqrtf(sq * (1.0f + Df * 1.2e-7f + 1e-6f)) * (1.0f + 1e-6f) + 1e-30f;
-            xb[gp] = make_float4(dist_upper(xn, sL - e32i),
-                                 fminf(dist_lower(xn, sO + e32i, Df), l3), l3,
-                                 __int_as_float(O));
-            ++np;
-          }
-        }
-        const bool open = q < rows && cls == 2;
-        const uint64_t mk = __ballot(open);
-        if (mk) {
-          const int lead = __ffsll((long long)mk) - 1;
-          int ob = 0;
-          if (lane == lead) ob = atomicAdd(&ocnt, __popcll(mk));
-          ob = __shfl(ob, lead);
-          if (open) olist[ob + __popcll(mk & below)] = (uint16_t)q;
-        }
-      }
-      if (nk) atomicAdd(&kp_ct[0], (unsigned long long)nk);
-      if (np) atomicAdd(&kp_ct[1], (unsigned long long)np);
-      __syncthreads();
-      nopen = ocnt;
-    }
+    // the item's positions to score: the open list when the bounds hold
+    const long nopen = bv ? (long)ocnt[it] : rows;
+    const uint16_t *ol = olist + (size_t)it * P;
     // item position -> row within the item
-    auto row_of = [&](long pos) -> long { return bv ? (long)olist[pos] : pos; };
+    auto row_of = [&](long pos) -> long { return bv ? (long)ol[pos] : pos; };
     // this half's 8 dimensions per k-step of the point in row pn (zero past
     // d or past the rows); the next 32 points' values are loaded while the
     // current ones are scored
@@ -665,7 +756,7 @@ void k_batch_assign_bf16(
       }
       if (bv || xb) {  // scored points, one count per wave
         const uint64_t sm = __ballot(ok && h == 0);
-        if (lane == 0 && sm) atomicAdd(&kp_ct[2], (unsigned long long)__popcll(sm));
+        if (lane == 0 && sm) atomicAdd(&kp_ct, (unsigned long long)__popcll(sm));
       }
       // exact search of the points that failed, by the whole wave
       uint64_t fm = __ballot(fail);
@@ -700,9 +791,7 @@ void k_batch_assign_bf16(
   }
   __syncthreads();
   if (tid == 0 && exact_ct) atomicAdd(nexact, exact_ct);
-  if (tid == 0 && nkp)
-    for (int c = 0; c < 3; ++c)
-      if (kp_ct[c]) atomicAdd(nkp + c, kp_ct[c]);
+  if (tid == 0 && nkp && kp_ct) atomicAdd(nkp + 2, kp_ct);
 }
 
 // Left-to-right sum of m FP64 values at stride st, in chunks of KM_CHUNK
@@ -1242,15 +1331,20 @@ static size_t bf16_lds(const Batch *b) {
 
 static hipError_t launch_assign(Batch *b, int P, int nitems, const int2 *items) {
   if (b->opt.precision == KM_PREC_BF16) {
-    const size_t lds = bf16_lds(b) + (b->xb ? sizeof(uint16_t) * (size_t)P : 0);
+    const size_t lds = bf16_lds(b);
     const int grid = num_cus() * 4;
-    static size_t attr = 64 * 1024;
-    if (lds > attr) return hipErrorInvalidValue;  // batch_create keeps it within 64 KB
+    if (lds > 64 * 1024) return hipErrorInvalidValue;  // batch_create keeps it within 64 KB
+    if (b->xb)
+      hipLaunchKernelGGL(k_batch_classify, dim3(nitems), dim3(256),
+                         sizeof(float) * (size_t)b->k * ((b->d | 1) + 2), b->stream, nitems, items, P,
+                         b->xoff, b->nvec, b->d, b->k, b->X, b->C, b->nc, b->bnd, b->labels, b->xb,
+                         b->bvalid, b->drift, b->dmaxv, b->xmax, b->nkp, b->imin, b->olist,
+                         b->ocnt);
 #define BA(KS)                                                                                   \
   hipLaunchKernelGGL(k_batch_assign_bf16<KS>, dim3(grid), dim3(256), lds, b->stream, nitems, items, \
                      P, b->xoff, b->nvec, b->d, b->k, b->X, b->C, b->nc, b->bnd, b->labels,        \
                      b->counters, b->nexact, FLT_MAX, b->xb, b->bvalid, b->drift, b->dmaxv,        \
-                     b->xmax, b->nkp, b->imin)
+                     b->xmax, b->nkp, b->imin, b->olist, b->ocnt)
     if (b->d <= 16) BA(1);
     else BA(2);
 #undef BA
@@ -1324,6 +1418,13 @@ static hipError_t upload_items(Batch *b, int P) {
   e = hipMemcpy(b->items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice);
   b->nitems = (int)items.size();
   b->P = P;
+  if (e == hipSuccess && b->xb && items.size() > b->ocnt_cap) {  // one open count per item
+    hipFree(b->ocnt);
+    b->ocnt = nullptr;
+    b->ocnt_cap = 0;
+    if ((e = hipMalloc(&b->ocnt, sizeof(int) * items.size())) == hipSuccess)
+      b->ocnt_cap = items.size();
+  }
   return e;
 }
 
@@ -1429,7 +1530,7 @@ static void batch_free(Batch *b) {
                   b->comp, b->choff, b->empty, b->order, b->items, b->counters,
                   b->desc, b->nexact, b->bnd,
                   b->xb, b->drift, b->dmaxv, b->xmax, b->bvalid, b->nkp, b->fd, b->rstate,
-                  b->imin};
+                  b->imin, b->olist, b->ocnt};
   for (void *p : bufs) hipFree(p);
   for (int q = 0; q < 2; ++q) {
     hipFree(b->perm[q]);
@@ -1519,7 +1620,7 @@ static kmeansErrorStatus batch_create(Batch *b, int nb, const long *n, int d, in
   BCHK(hipMalloc(&b->nexact, sizeof(unsigned long long)), KMEANS_ERROR_BUILDDATA);
   BCHK(hipMemset(b->nexact, 0, sizeof(unsigned long long)), KMEANS_ERROR_BUILDDATA);
   // drift arrays (every engine's finalize writes them) and, for BF16 engines
-  // whose LDS also holds an open list of the largest work item, the bounds
+  // whose centres fit the assign's LDS, the bounds and the open lists
   BCHK(hipMalloc(&b->drift, sizeof(float) * (size_t)nb * k), KMEANS_ERROR_BUILDDATA);
   BCHK(hipMalloc(&b->dmaxv, sizeof(float) * (size_t)nb), KMEANS_ERROR_BUILDDATA);
   BCHK(hipMalloc(&b->xmax, sizeof(float) * (size_t)nb), KMEANS_ERROR_BUILDDATA);
@@ -1527,8 +1628,13 @@ static kmeansErrorStatus batch_create(Batch *b, int nb, const long *n, int d, in
   BCHK(hipMemset(b->bvalid, 0, sizeof(int)), KMEANS_ERROR_BUILDDATA);
   BCHK(hipMalloc(&b->nkp, sizeof(unsigned long long) * 3), KMEANS_ERROR_BUILDDATA);
   BCHK(hipMemset(b->nkp, 0, sizeof(unsigned long long) * 3), KMEANS_ERROR_BUILDDATA);
-  if (b->opt.precision == KM_PREC_BF16 && bf16_lds(b) + sizeof(uint16_t) * B_MAXP <= 64 * 1024)
+  if (b->opt.precision == KM_PREC_BF16 && bf16_lds(b) <= 64 * 1024) {
+    // the open lists of work items of any tile size P <= B_MAXP fill at most
+    // tot + nb * P positions (their counts: upload_items)
     BCHK(hipMalloc(&b->xb, sizeof(float4) * (size_t)tot), KMEANS_ERROR_BUILDDATA);
+    BCHK(hipMalloc(&b->olist, sizeof(uint16_t) * ((size_t)tot + (size_t)nb * B_MAXP)),
+         KMEANS_ERROR_BUILDDATA);
+  }
   return KMEANS_SUCCESS;
 }
 
