@@ -130,9 +130,9 @@ def main():
         graph_info = {"graph_launches_in_timed_run": gl - max(args.warmup, 3) if args.warmup else gl,
                       "eager_ms_per_step": dt_eager * 1e3 / args.steps,
                       "note": "ms_per_step: each iteration one hipGraph launch (assign, update, "
-                              "compactness) plus the slot commit, graphs instantiated in the "
-                              "warm-up; phases_ms_per_step and the roofline from an eagerly "
-                              "launched rerun of the same iterations with HIP events"}
+                              "compactness, slot commit), graphs instantiated in the warm-up; "
+                              "phases_ms_per_step and the roofline from an eagerly launched "
+                              "rerun of the same iterations with HIP events"}
     st = eng.stats()
     pairs = float(n) * k
     value = pairs * args.steps / dt

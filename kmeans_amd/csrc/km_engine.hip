@@ -128,7 +128,7 @@ kmeansErrorStatus Engine::create(long m_, int n_, int k_, const kmeansOptions *o
                    KMEANS_ERROR_BUILDDATA);
     }
     KM_HIP_CHECK(hipMalloc(&dfd, sizeof(int) * (size_t)nblk * k), KMEANS_ERROR_BUILDDATA);
-    KM_HIP_CHECK(hipMalloc(&dstage, sizeof(double) * 2), KMEANS_ERROR_BUILDDATA);
+    KM_HIP_CHECK(hipMalloc(&dstage, sizeof(double) * 4), KMEANS_ERROR_BUILDDATA);
   }
   return ensure_comp_slots(64);
 }
@@ -138,6 +138,7 @@ kmeansErrorStatus Engine::create(long m_, int n_, int k_, const kmeansOptions *o
 kmeansErrorStatus Engine::ensure_comp_slots(int nslots) {
   if (nslots <= comp_slots) return KMEANS_SUCCESS;
   KM_HIP_CHECK(hipStreamSynchronize(stream), KMEANS_ERROR_BUILDDATA);
+  graphs_drop();  // the graphs hold the slot arrays' addresses
   hipFree(dcomp);
   hipFree(dempty_hist);
   hipHostFree(hflag);
@@ -151,7 +152,7 @@ kmeansErrorStatus Engine::ensure_comp_slots(int nslots) {
                KMEANS_ERROR_BUILDDATA);
   comp_slots = nslots;
   if (!dctl) {
-    KM_HIP_CHECK(hipMalloc(&dctl, sizeof(int) * 2), KMEANS_ERROR_BUILDDATA);
+    KM_HIP_CHECK(hipMalloc(&dctl, sizeof(int) * 4), KMEANS_ERROR_BUILDDATA);
     for (auto &e : ev_iter)
       KM_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming), KMEANS_ERROR_BUILDDATA);
   }
@@ -427,11 +428,19 @@ void Engine::collect_timing(int nslots) {
   }
 }
 
-// Control words of a run with a device-side convergence test: no stop, all
-// rows assigned.
-__global__ void k_ctl_reset(int *ctl, int rows) {
+// Control words of a run: no stop, all rows assigned (ctl[1], read by the
+// assign kernels under a device-side convergence test), slot 0 next
+// (ctl[2], read by the graph iterations' commit); with stage, the run's
+// convergence parameters for that commit (stage[2] = the compactness before
+// the run, stage[3] = the criterion).
+__global__ void k_ctl_reset(int *ctl, int rows, double *stage, double prev0, double criterion) {
   ctl[0] = 0;
   ctl[1] = rows;
+  ctl[2] = 0;
+  if (stage) {
+    stage[2] = prev0;
+    stage[3] = criterion;
+  }
 }
 
 // The reference's convergence test (kmeans.cpp:1194,1235-1237) after
@@ -451,17 +460,20 @@ __global__ void k_converge(const double *comp, int s, double prev0, double crite
   }
 }
 
-// After an iteration run as a graph: its compactness and empty-cluster count
-// (stage[0], and stage[1] read as an int) into slot s, then, with ctl, the
-// convergence test of k_converge for that slot.
-__global__ void k_slot_commit(const double *stage, double *comp, int *empty_hist, int s,
-                              double prev0, double criterion, int *ctl) {
+// The last node of a graph iteration: its compactness and empty-cluster
+// count (stage[0], and stage[1] read as an int) into slot s = ctl[2], the
+// next slot in ctl[2]; with test, the convergence test of k_converge for
+// that slot against stage[2] (before slot 0) and criterion stage[3].
+__global__ void k_slot_commit(const double *stage, double *comp, int *empty_hist, int *ctl,
+                              int test) {
+  const int s = ctl[2];
+  ctl[2] = s + 1;
   comp[s] = stage[0];
   empty_hist[s] = *reinterpret_cast<const int *>(stage + 1);
-  if (!ctl || ctl[0]) return;
-  const double c = comp[s], prev = s > 0 ? comp[s - 1] : prev0;
+  if (!test || ctl[0]) return;
+  const double c = comp[s], prev = s > 0 ? comp[s - 1] : stage[2];
   const double rel = fabs(c - prev) / fabs(c);
-  if (!(rel > criterion)) {
+  if (!(rel > stage[3])) {
     ctl[0] = 1;
     ctl[1] = 0;
   }
@@ -483,12 +495,16 @@ bool Engine::graph_eligible(bool minibatch) const {
          !streaming && !minibatch && dperm[0] && dstage && m <= KM_BLOCK;
 }
 
-// One full-pass iteration (assign, update, compactness into dstage) as a
-// graph launch; the graph of this state is captured from the engine's own
-// launch sequence on first use.  *ran false: no graph (capture or
-// instantiation failed; graph_ok is cleared) and nothing was enqueued.
+// One full-pass iteration (assign, update, compactness into dstage and the
+// slot commit) as a graph launch; the graph of this state is captured from
+// the engine's own launch sequence on first use.  *ran false: no graph
+// (capture or instantiation failed; graph_ok is cleared) and nothing was
+// enqueued.
 kmeansErrorStatus Engine::graph_iteration(bool devconv, bool *ran) {
   *ran = false;
+  // after a failed capture no graph runs again: the commit's device-side
+  // slot counter only advances in graph iterations
+  if (!graph_ok) return KMEANS_SUCCESS;
   const int nw = reuse_cur ^ 1;
   const int v = (reuse_ok ? 1 : 0) | (nw << 1) | (devconv ? 4 : 0);
   if (!gexec[v]) {
@@ -507,6 +523,11 @@ kmeansErrorStatus Engine::graph_iteration(bool devconv, bool *ran) {
       err = clusterCenters(0, m, reinterpret_cast<int *>(dstage + 1));
     if (err == KMEANS_SUCCESS && launch_comp_reduce(stream, k, ws.compj, dstage) != hipSuccess)
       err = KMEANS_ERROR_COMPACTNESS;
+    if (err == KMEANS_SUCCESS) {
+      hipLaunchKernelGGL(k_slot_commit, dim3(1), dim3(1), 0, stream, dstage, dcomp, dempty_hist,
+                         dctl, devconv ? 1 : 0);
+      if (hipGetLastError() != hipSuccess) err = KMEANS_ERROR_COMPUTE;
+    }
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(stream, &g);
     reuse_cur = rc;
@@ -557,9 +578,9 @@ kmeansErrorStatus Engine::collect_empty(int nslots) {
 //  - otherwise (mini-batch windows, streamed tiles, the proposer, the
 //    library GEMM) one host round trip per iteration.
 // FP32 full passes over resident points in one canonical block, timing off,
-// run each iteration as one graph launch plus the slot commit
-// (graph_iteration); the labels, centres and compactness are those of the
-// launches the graph was captured from.
+// run each iteration as one graph launch (graph_iteration); the labels,
+// centres and compactness are those of the launches the graph was captured
+// from.
 // Statistics count iterations up to the converged one.
 kmeansErrorStatus Engine::run_iterations(long r0, long rows, bool minibatch) {
   kmeansErrorStatus err;
@@ -569,11 +590,13 @@ kmeansErrorStatus Engine::run_iterations(long r0, long rows, bool minibatch) {
   int it0 = iters, slot = 0;
   if ((err = ensure_comp_slots(std::max(maxIters, 1))) != KMEANS_SUCCESS) return err;
   const double prev0 = compactnessScore;
-  if (devconv) {
-    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, stream, dctl, (int)rows);
+  const bool graphs = graph_eligible(minibatch);
+  if (devconv || graphs) {
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, stream, dctl, (int)rows,
+                       graphs ? dstage : nullptr, prev0, criterion);
     KM_LAUNCH_CHECK(KMEANS_ERROR_COMPUTE);
-    assign_stop = dctl;
   }
+  if (devconv) assign_stop = dctl;
   // assign_stop is cleared however the loop is left
   struct StopGuard {
     const int **p;
@@ -588,7 +611,6 @@ kmeansErrorStatus Engine::run_iterations(long r0, long rows, bool minibatch) {
     KM_HIP_CHECK(hipStreamSynchronize(stream), KMEANS_ERROR_COMPUTE);
     return KMEANS_SUCCESS;
   };
-  const bool graphs = graph_eligible(minibatch);
   while (iters < maxIters && relErr > criterion) {
     bool graph_ran = false;
     long b0 = r0, br = rows;
@@ -618,11 +640,7 @@ kmeansErrorStatus Engine::run_iterations(long r0, long rows, bool minibatch) {
         if (graphs && b0 == 0 && br == m &&
             (err = graph_iteration(devconv, &graph_ran)) != KMEANS_SUCCESS)
           return err;
-        if (graph_ran) {
-          hipLaunchKernelGGL(k_slot_commit, dim3(1), dim3(1), 0, stream, dstage, dcomp, dempty_hist,
-                             slot, prev0, criterion, devconv ? dctl : nullptr);
-          KM_LAUNCH_CHECK(KMEANS_ERROR_COMPUTE);
-        } else {
+        if (!graph_ran) {
           record(slot, 0);
           if ((err = closestCenters(b0, br)) != KMEANS_SUCCESS) return err;
           record(slot, 1);

@@ -80,7 +80,8 @@ struct Engine {
   int cur_slot = 0;               // iteration slot of the running kmeansRun
   // device-side convergence test (criterion >= 0, resident full passes):
   // ctl[0] = stop flag, ctl[1] = rows the assign kernels take (0 once
-  // stopped); hflag[slot] = host copy of the stop flag after that slot
+  // stopped), ctl[2] = the next slot of a graph iteration; hflag[slot] =
+  // host copy of the stop flag after that slot
   int *dctl = nullptr;
   int *hflag = nullptr;
   hipEvent_t ev_iter[3] = {nullptr, nullptr, nullptr};
@@ -90,14 +91,14 @@ struct Engine {
 
   // FP32 iterations over resident points in one canonical block with timing
   // off run as executable hipGraphs: the assign, the update and the
-  // compactness reduction of one iteration, one graph per (reuse state,
-  // parity, device-side stop); the graph writes the compactness and the
-  // empty-cluster count to dstage, and an ordinary launch moves them into
-  // the iteration's slot.  Dropped when the points change; graph_ok false
-  // after a failed capture (the loop then launches the kernels directly).
+  // compactness reduction of one iteration and the commit of its compactness
+  // and empty-cluster count into the slot that ctl[2] names, one graph per
+  // (reuse state, parity, device-side stop).  Dropped when the points or the
+  // slot arrays change; graph_ok false after a failed capture (the loop then
+  // launches the kernels directly).
   hipGraphExec_t gexec[8] = {};
   bool graph_ok = true;
-  double *dstage = nullptr;  // [compactness, empty count (as int)]
+  double *dstage = nullptr;  // [compactness, empty count (as int), run's prev0, criterion]
   void graphs_drop();
   bool graph_eligible(bool minibatch) const;
   kmeansErrorStatus graph_iteration(bool devconv, bool *ran);

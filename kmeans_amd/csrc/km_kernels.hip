@@ -750,15 +750,29 @@ __global__ __launch_bounds__(256) void k_chunk_sums4(const float *__restrict__ X
       a3 += w; s3 = add_square<float>(s3, w);
     };
     int m = 0;
-    for (; m + 8 <= cnt; m += 8) {
-      float4 v[8];
+    if (staged) {
+      for (; m + 8 <= cnt; m += 8) {
+        float4 v[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int pp = staged ? pidx[sub][m + u] : perm[start + m + u];
-        v[u] = load_once(X4 + (long)pp * D4 + t);
+        for (int u = 0; u < 8; ++u) v[u] = load_once(X4 + (long)pidx[sub][m + u] * D4 + t);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) add(v[u]);
       }
+    } else {
+      // member indices from HBM: the next eight are loaded while this
+      // batch's rows are in flight, so each batch waits for memory once
+      int pn[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) add(v[u]);
+      for (int u = 0; u < 8; ++u) pn[u] = u < cnt ? perm[start + u] : 0;
+      for (; m + 8 <= cnt; m += 8) {
+        float4 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = load_once(X4 + (long)pn[u] * D4 + t);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) pn[u] = m + 8 + u < cnt ? perm[start + m + 8 + u] : 0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) add(v[u]);
+      }
     }
     for (; m < cnt; ++m) add(load_once(X4 + (long)(staged ? pidx[sub][m] : perm[start + m]) * D4 + t));
     double *o = P + c * ld + 4 * t;
