@@ -200,6 +200,19 @@ def load_raw_f32(path, n, m=None):
     return np.memmap(path, dtype=np.float32, mode="r", shape=(m, n))
 
 
+def load_npy(path):
+    """Map a NumPy .npy file of m x n float32 points (C order) without
+    reading it into memory, like load_raw_f32; other dtypes, ranks or Fortran
+    order are refused rather than converted (a conversion would read and copy
+    the whole file).  Object arrays are never unpickled."""
+    a = np.load(path, mmap_mode="r", allow_pickle=False)
+    if a.dtype != np.float32 or a.ndim != 2:
+        raise ValueError(f"{path}: {a.dtype} array of rank {a.ndim}, need 2-D float32")
+    if not a.flags.c_contiguous:
+        raise ValueError(f"{path}: Fortran-ordered array, need row-major points")
+    return a
+
+
 # ------------------------------------------------------------------ host API
 def _init_alg(init_algorithm, init_from_result):
     if init_algorithm is not None:

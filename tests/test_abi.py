@@ -197,3 +197,19 @@ def test_reference_init_rows_leave_the_callers_rand_alone():
     L = kmeans_amd.lib()
     out = np.zeros(4, np.int32)
     assert L.kmeansReferenceInitRows(3, 4, out.ctypes.data_as(ctypes.c_void_p)) == 14
+
+
+def test_load_npy_maps_float32_points(tmp_path):
+    X = np.arange(24, dtype=np.float32).reshape(6, 4)
+    np.save(tmp_path / "x.npy", X)
+    M = kmeans_amd.load_npy(str(tmp_path / "x.npy"))
+    assert isinstance(M, np.memmap) and np.array_equal(M, X)
+    np.save(tmp_path / "d.npy", X.astype(np.float64))
+    with pytest.raises(ValueError):
+        kmeans_amd.load_npy(str(tmp_path / "d.npy"))
+    np.save(tmp_path / "f.npy", np.asfortranarray(X))
+    with pytest.raises(ValueError):
+        kmeans_amd.load_npy(str(tmp_path / "f.npy"))
+    np.save(tmp_path / "o.npy", np.array([{"a": 1}], dtype=object), allow_pickle=True)
+    with pytest.raises(ValueError):
+        kmeans_amd.load_npy(str(tmp_path / "o.npy"))
