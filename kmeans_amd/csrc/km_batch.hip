@@ -870,15 +870,20 @@ __device__ int block_scan_any(int *v, int n, int *tmp) {
 // once rstate[0] says it wrote one); the arrays come in pairs.
 __device__ inline int upd_parity(const int *rstate) { return rstate[0] ? rstate[1] ^ 1 : 0; }
 
+template <bool PEER>
 __global__ __launch_bounds__(1024) void k_batch_sort(
     int nb, const int *__restrict__ order, const long *__restrict__ xoff,
     const int *__restrict__ nvec, int k, const int *__restrict__ labels, int *perm0, int *perm1,
     int *gcnt0, int *gcnt1, int *goff0, int *goff1, int *gcpre0, int *gcpre1,
     const long *__restrict__ choff, int4 *__restrict__ desc, int *__restrict__ counter,
     const int *__restrict__ rstate, int *__restrict__ fd, int *__restrict__ imin) {
-  extern __shared__ int sh[];  // cur[16][k], offs[k], cpre[k], cnt[k], fdc[k], ims[k]
+  // cur[16][k], offs[k], cpre[k], cnt[k], fdc[k], ims[k]; PEER: per wave
+  // and label the mask of the wave's lanes holding that label in the current
+  // step (all zero between steps)
+  extern __shared__ __attribute__((aligned(8))) int sh[];
   int *cur = sh, *offs = sh + 16 * k, *cpre = offs + k, *cnts = cpre + k, *fdc = cnts + k,
       *ims = fdc + k;
+  unsigned long long *pm = reinterpret_cast<unsigned long long *>(sh + ((21 * k + 1) & ~1));
   const bool reuse = rstate[0] != 0;
   const int par = upd_parity(rstate);
   int *__restrict__ perm = par ? perm1 : perm0;
@@ -896,32 +901,38 @@ __global__ __launch_bounds__(1024) void k_batch_sort(
     const int it = item_s;
     if (it >= nb) break;
     const int inst = order[it];
-    const long n = nvec[inst], base = xoff[inst];
+    const int n = nvec[inst];  // instance sizes stay below 2^31
+    const long base = xoff[inst];
     const int *lab = labels + base;
+    int *pb = perm + base;
     for (int e = t; e < 16 * k; e += blockDim.x) cur[e] = 0;
     for (int j = t; j < k; j += blockDim.x) {
       fdc[j] = 0;
       ims[j] = reuse ? imin[(size_t)inst * k + j] : 0;
     }
+    if (PEER)
+      for (int e = t; e < 16 * k; e += blockDim.x) pm[e] = 0ull;
     __syncthreads();
-    const long S = ((n + 15) / 16 + 63) / 64 * 64;
-    const long a = min(n, (long)w * S), e = min(n, a + S);
+    const int S = ((n + 15) / 16 + 63) / 64 * 64;
+    const int a = min(n, w * S), e = min(n, a + S);
     // per-(wave, label) counts; fdc[j] = members of j below its smallest
     // changed row, i.e. the length of j's unchanged member prefix (the list
-    // is in ascending row order)
-    for (long p0 = a; p0 < e; p0 += 512) {
-      int Lv[8];
+    // is in ascending row order).  Loads are clamped into the segment, not
+    // branched around.
+    for (int p0 = a; p0 < e; p0 += 512) {
+      int Lv[8], iv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) Lv[u] = lab[min(p0 + u * 64 + lane, e - 1)];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) iv[u] = ims[Lv[u]];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        const long p = p0 + u * 64 + lane;
-        Lv[u] = p < e ? lab[p] : -1;
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u)
-        if (Lv[u] >= 0) {
+        const int p = p0 + u * 64 + lane;
+        if (p < e) {
           atomicAdd(&cur[w * k + Lv[u]], 1);
-          if (p0 + u * 64 + lane < ims[Lv[u]]) atomicAdd(&fdc[Lv[u]], 1);
+          if (p < iv[u]) atomicAdd(&fdc[Lv[u]], 1);
         }
+      }
     }
     __syncthreads();
     for (int j = t; j < k; j += blockDim.x) {
@@ -963,31 +974,46 @@ __global__ __launch_bounds__(1024) void k_batch_sort(
       for (int c = nch + t; c < cap; c += blockDim.x) dd[c] = make_int4(0, 0, 0, 0);
     }
     int *cw = cur + w * k;
-    for (long p0 = a; p0 < e; p0 += 512) {
+    unsigned long long *pw = pm + w * k;
+    for (int p0 = a; p0 < e; p0 += 512) {
       int Lv[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const long p = p0 + u * 64 + lane;
-        Lv[u] = p < e ? lab[p] : 0;
-      }
+      for (int u = 0; u < 8; ++u) Lv[u] = lab[min(p0 + u * 64 + lane, e - 1)];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         if (p0 + u * 64 >= e) break;  // uniform over the wave
-        const long p = p0 + u * 64 + lane;
+        const int p = p0 + u * 64 + lane;
         const bool valid = p < e;
         const int L = Lv[u];
-        uint64_t m = __ballot(valid);
-        for (int bt = 0; bt < nbits; ++bt) {
-          const uint64_t bb = __ballot(valid && ((L >> bt) & 1));
-          m &= ((L >> bt) & 1) ? bb : ~bb;
+        // the lanes of this step holding label L: an LDS mask per label
+        // (PEER), else a match over the label bits by ballots
+        uint64_t m;
+        int cwL;  // the label's cursor, read with the mask
+        if (PEER) {
+          if (valid) atomicOr(&pw[L], 1ull << lane);
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          cwL = cw[L];
+          m = pw[L];
+          if (!valid) m = 0ull;
+        } else {
+          m = __ballot(valid);
+          for (int bt = 0; bt < nbits; ++bt) {
+            const uint64_t bb = __ballot(valid && ((L >> bt) & 1));
+            m &= ((L >> bt) & 1) ? bb : ~bb;
+          }
+          if (!valid) m = 0;
+          cwL = cw[L];
         }
-        if (!valid) m = 0;
         const int rank = __popcll(m & below);
-        int pos = 0;
-        if (valid) pos = cw[L] + rank;
+        const int pos = cwL + rank;
         __builtin_amdgcn_wave_barrier();
-        if (valid) perm[base + pos] = (int)p;
-        if (valid && rank == 0) cw[L] = pos + __popcll(m);
+        if (valid) pb[pos] = p;
+        if (valid && rank == 0) {
+          cw[L] = pos + __popcll(m);
+          if (PEER) pw[L] = 0ull;
+        }
         __builtin_amdgcn_wave_barrier();
       }
     }
@@ -1365,16 +1391,21 @@ static hipError_t launch_assign(Batch *b, int P, int nitems, const int2 *items) 
 static hipError_t launch_update(Batch *b, int slot) {
   double *co = b->comp + (size_t)slot * b->nb;
   int *eo = b->empty + (size_t)slot * b->nb;
-  const size_t lds = sizeof(int) * 21 * (size_t)b->k;
+  // the label masks of the peer match take 16 x k 8-byte words; used while
+  // everything stays within 64 KB
+  const size_t lds0 = sizeof(int) * (size_t)((21 * b->k + 1) & ~1);
+  const bool peer = lds0 + sizeof(unsigned long long) * 16 * (size_t)b->k <= 64 * 1024;
+  const size_t lds = peer ? lds0 + sizeof(unsigned long long) * 16 * (size_t)b->k : lds0;
   static size_t attr = 64 * 1024;
-  if (lds > attr) {
-    hipError_t e = hipFuncSetAttribute((const void *)k_batch_sort,
+  if (!peer && lds > attr) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_batch_sort<false>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     attr = lds;
   }
-  hipLaunchKernelGGL(k_batch_sort, dim3(std::min(b->nb, num_cus() * 2)), dim3(1024), lds, b->stream,
-                     b->nb, b->order, b->xoff, b->nvec, b->k, b->labels, b->perm[0], b->perm[1],
+  hipLaunchKernelGGL(peer ? k_batch_sort<true> : k_batch_sort<false>,
+                     dim3(std::min(b->nb, num_cus() * 2)), dim3(1024), lds, b->stream, b->nb,
+                     b->order, b->xoff, b->nvec, b->k, b->labels, b->perm[0], b->perm[1],
                      b->gcnt[0], b->gcnt[1], b->goff[0], b->goff[1], b->gcpre[0], b->gcpre[1],
                      b->choff, b->desc, b->counters + 1, b->rstate, b->fd, b->imin);
 #define CHW(DT)                                                                                  \

@@ -44,7 +44,7 @@ def _check(b, n, Xs, ref, iters_last):
 @pytest.mark.parametrize("precision,d,k,median,lo,hi", [
     ("fp32", 16, 32, 3000, 300, 20000), ("fp32", 5, 7, 400, 40, 3000), ("fp32", 20, 64, 1500, 200, 9000),
     ("bf16", 16, 32, 3000, 300, 20000), ("bf16", 5, 7, 400, 40, 3000), ("bf16", 20, 64, 1500, 200, 9000),
-    ("bf16", 16, 256, 3000, 300, 20000)])
+    ("bf16", 16, 256, 3000, 300, 20000), ("fp32", 8, 400, 3000, 1000, 9000)])
 def test_batch_equals_per_instance_oracle(precision, d, k, median, lo, hi):
     nb, iters = 24, 6
     n, Xs = _instances(nb, d, k, median, lo, hi)
