@@ -132,6 +132,25 @@ def test_batch_drift_bounds_keep_labels():
     b.close()
 
 
+def test_batch_drift_bounds_across_tile_sizes():
+    """The open lists are per work item: changing the tile size between runs
+    (more, smaller items: the per-item counts are reallocated) keeps every
+    instance on the oracle's trajectory, with the bounds carried over."""
+    nb, d, k = 16, 16, 48
+    n, Xs = _instances(nb, d, k, 3000, 400, 12000, seed=17)
+    b = km.KMeansBatch(n, d, k, precision="bf16")
+    b.set_data(np.concatenate(Xs))
+    b.init_first_k()
+    done = 0
+    for rows, steps in ((4096, 3), (256, 3), (1024, 2), (2, 1)):
+        b.set_tile_rows(rows)
+        b.run(steps)
+        done += steps
+        _check(b, n, Xs, _oracle(Xs, k, done), steps)
+    assert b.stats()["kept"] > 0
+    b.close()
+
+
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
 def test_batch_new_data_drops_reuse(precision):
     """The update reuses chunk partials of unchanged clusters across
