@@ -431,8 +431,8 @@ __global__ __launch_bounds__(256) void k_bound_compact(
 // l3' kept; the count goes to slot 4 of the counter lines ncl.
 // Each wave takes 64 points, one per lane.  With 16-byte rows (D % 4 == 0,
 // aligned) the wave stages its points' rows and both candidates' centre rows
-// in LDS 16 dimensions at a time with coalesced loads (four lanes per
-// 64-byte piece of a row) and each lane continues its two chains from there;
+// in LDS PR_PD dimensions at a time with coalesced loads (PR_PD / 4 lanes
+// per piece of a row) and each lane continues its two chains from there;
 // otherwise each lane reads its rows itself.
 #define PR_PD 16                 // dimensions per staged piece
 #define PR_PS (PR_PD / 4 + 1)    // float4 stride of a staged row (conflict-free reads)
@@ -460,14 +460,18 @@ __global__ __launch_bounds__(256) void k_pair_decide(
     const int r = ok ? __float_as_int(b.w) : 0;
     float sa = ok ? -0.5f * nc[a] : 0.0f, sr = ok ? -0.5f * nc[r] : 0.0f;
     if (vec) {
+      constexpr int LPR = PR_PD / 4, RPI = 64 / LPR;  // lanes per row piece, rows per load
       const int np = (D + PR_PD - 1) / PR_PD;
-      for (int pc = 0; pc < np; ++pc) {
-        // stage: row t*16 + lane/4 of each array, 16-byte piece lane % 4
-        const int q = lane & 3, d0 = pc * PR_PD + q * 4;
-        float4 xv[4], av[4], bv[4];  // all twelve loads in flight before the stores
+      const int q = lane % LPR;
+      // row t*RPI + lane/LPR of each array, 16-byte piece lane % LPR: all
+      // loads of a piece in flight together, and the next piece's loads
+      // issued before this piece's chains run
+      float4 xv[LPR], av[LPR], bv[LPR];
+      auto load_piece = [&](int pc) {
+        const int d0 = pc * PR_PD + q * 4;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          const int row = t * 16 + (lane >> 2);
+        for (int t = 0; t < LPR; ++t) {
+          const int row = t * RPI + lane / LPR;
           const long pp = __shfl(p, row);
           const int aa = __shfl(a, row), rr = __shfl(r, row);
           const bool v = i0 + g * 64 + row < i1 && d0 < D;
@@ -480,9 +484,12 @@ __global__ __launch_bounds__(256) void k_pair_decide(
             bv[t] = *reinterpret_cast<const float4 *>(C + (long)rr * D + d0);
           }
         }
+      };
+      load_piece(0);
+      for (int pc = 0; pc < np; ++pc) {
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          const int row = t * 16 + (lane >> 2);
+        for (int t = 0; t < LPR; ++t) {
+          const int row = t * RPI + lane / LPR;
           lx[row * PR_PS + q] = xv[t];
           la[row * PR_PS + q] = av[t];
           lr[row * PR_PS + q] = bv[t];
@@ -490,6 +497,7 @@ __global__ __launch_bounds__(256) void k_pair_decide(
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (pc + 1 < np) load_piece(pc + 1);  // uniform
         // the chains continue over this piece's dimensions, in order
 #pragma unroll
         for (int u = 0; u < PR_PD / 4; ++u) {
