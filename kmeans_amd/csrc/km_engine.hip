@@ -162,6 +162,8 @@ kmeansErrorStatus Engine::ensure_comp_slots(int nslots) {
 // kmeans.cpp:540-695 (buildData) without the row shuffle of :554-564.
 kmeansErrorStatus Engine::buildData(const float *data, int on_device) {
   if (!data) return KMEANS_ERROR_INVALID_ARGUMENT;
+  // the BF16/FP16 assign's side-stream work may still read the old points
+  if (mfma) KM_HIP_CHECK(mfma_pair_join(this), KMEANS_ERROR_BUILDDATA);
   reuse_ok = false;  // new points: no chunk partials to reuse
   graphs_drop();     // the graphs hold the old points' address
   if (!on_device && opt.tile_rows > 0 && opt.tile_rows < m) return setup_streaming(data);
@@ -319,6 +321,9 @@ kmeansErrorStatus Engine::closestCenters(long r0, long rows) {
 
 kmeansErrorStatus Engine::clusterCenters(long r0, long rows, int *empty) {
   int *dempty = empty ? empty : this->dempty;
+  // every label final: the BF16/FP16 assign's two-candidate decisions may
+  // still run on its side stream
+  if (mfma) KM_HIP_CHECK(mfma_pair_join(this), KMEANS_ERROR_CLUSTER_CENTERS);
   ws.perm_out = nullptr;
   ws2.perm_out = nullptr;
   // a full update over all resident points sorts each canonical block into
@@ -825,6 +830,7 @@ kmeansErrorStatus Engine::getClusterCenters(float *out) {
 }
 
 kmeansErrorStatus Engine::getLabels(int *out) {
+  if (mfma) KM_HIP_CHECK(mfma_pair_join(this), KMEANS_ERROR_COMPUTE);
   KM_HIP_CHECK(hipMemcpyAsync(out, dlabels, sizeof(int) * (size_t)m, hipMemcpyDeviceToHost, stream),
                KMEANS_ERROR_GET_CENTERS);
   KM_HIP_CHECK(hipStreamSynchronize(stream), KMEANS_ERROR_GET_CENTERS);
@@ -947,6 +953,7 @@ const int *kmeansDeviceLabels(kmeansHandle h) { return h ? h->e.dlabels : nullpt
 
 kmeansErrorStatus kmeansSynchronize(kmeansHandle h) {
   if (!h) return KMEANS_ERROR_INVALID_ARGUMENT;
+  if (h->e.mfma) KM_HIP_CHECK(mfma_pair_join(&h->e), KMEANS_ERROR_COMPUTE);
   KM_HIP_CHECK(hipStreamSynchronize(h->e.stream), KMEANS_ERROR_COMPUTE);
   return KMEANS_SUCCESS;
 }

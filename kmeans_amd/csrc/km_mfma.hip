@@ -106,6 +106,13 @@ struct MfmaState {
   float *Cref = nullptr, *drift = nullptr;
   int *ulist = nullptr, *bcnt = nullptr, *bofs = nullptr, *btsum = nullptr;
   bool bnd_valid = false;
+  // the two-candidate decisions run on a side stream (default priority)
+  // after the assign kernel, beside the re-resolve: they touch only points
+  // the assign and the re-resolve skip; pair_pending until the engine's
+  // stream has waited for them (mfma_pair_join)
+  hipStream_t pair_stream = nullptr;
+  hipEvent_t pair_fork = nullptr, pair_join = nullptr;
+  bool pair_pending = false;
   int nflag_slots = 0;
   bool x_ready = false;
   bool c_ready = false;
@@ -1429,12 +1436,19 @@ void mfma_free(Engine *e) {
   for (void *b : bb) hipFree(b);
   hipFree(s->home);
   for (hipEvent_t ev : s->kev) hipEventDestroy(ev);
+  if (s->pair_stream) {
+    hipStreamSynchronize(s->pair_stream);
+    hipStreamDestroy(s->pair_stream);
+  }
+  if (s->pair_fork) hipEventDestroy(s->pair_fork);
+  if (s->pair_join) hipEventDestroy(s->pair_join);
   delete s;
   e->mfma = nullptr;
 }
 
 void mfma_invalidate(Engine *e) {
   if (st(e)) {
+    mfma_pair_join(e);  // the data about to be replaced is still being read
     st(e)->x_ready = false;
     st(e)->bnd_valid = false;
   }
@@ -1712,6 +1726,9 @@ static hipError_t ensure_bounds(Engine *e) {
   if ((r = hipMalloc(&s->bofs, sizeof(int) * (size_t)(2 * nblk + 1))) != hipSuccess) return r;
   if ((r = hipMalloc(&s->btsum, sizeof(int) * (size_t)(scan_tiles(2 * nblk) + 1))) != hipSuccess)
     return r;
+  if ((r = hipStreamCreateWithFlags(&s->pair_stream, hipStreamNonBlocking)) != hipSuccess) return r;
+  if ((r = hipEventCreateWithFlags(&s->pair_fork, hipEventDisableTiming)) != hipSuccess) return r;
+  if ((r = hipEventCreateWithFlags(&s->pair_join, hipEventDisableTiming)) != hipSuccess) return r;
   // an open list of zero points until the first compaction (a launch whose
   // compaction a convergence stop skipped reads these)
   if ((r = hipMemsetAsync(s->ulist, 0, sizeof(int) * (size_t)e->m, e->stream)) != hipSuccess)
@@ -1735,6 +1752,9 @@ hipError_t mfma_assign(Engine *e, long r0, long rows) {
   hipError_t r;
   if ((r = ensure_state(e)) != hipSuccess) return r;
   MfmaState *s = st(e);
+  // the previous call's pair decisions (if nothing joined them yet) read the
+  // buffers prepared and counted below
+  if ((r = mfma_pair_join(e)) != hipSuccess) return r;
   if (e->n > 256) return hipErrorNotSupported;
   if (!s->x_ready) {
     if ((r = prepare_x(e)) != hipSuccess) return r;
@@ -1751,6 +1771,7 @@ hipError_t mfma_assign(Engine *e, long r0, long rows) {
     return r;
   if ((r = hipMemsetAsync(nc2, 0, sizeof(int) * NC_SLOT, e->stream)) != hipSuccess) return r;
   const int *order = nullptr, *home = nullptr, *Mdev = nullptr;
+  bool pairs = false;  // two-candidate points to decide after the assign
   if (all && s->bnd_valid) {
     // the drift bounds of the last assign over all points: only the points
     // they leave open are assigned, in point order (the filter reads every
@@ -1767,12 +1788,7 @@ hipError_t mfma_assign(Engine *e, long r0, long rows) {
     hipLaunchKernelGGL(k_bound_compact, dim3(nblk), dim3(256), 0, e->stream, rows,
                        (const int *)nullptr, e->dlabels, s->xb, s->drift, e->k, s->bnd, s->xnb,
                        e->n, s->bofs, s->ulist, e->assign_stop);
-    const unsigned pg = (unsigned)std::min<long>(nblk, (long)num_cus() * 16);
-    hipLaunchKernelGGL(k_pair_decide, dim3(pg), dim3(256), 0, e->stream, s->ulist,
-                       s->bofs + nblk, nblk, e->dlabels, s->xb, s->xnb, s->bnd, e->n,
-                       (const float *)e->dX, (const float *)e->dC, (const float *)e->dnc, nc2,
-                       e->assign_stop);
-    if ((r = hipGetLastError()) != hipSuccess) return r;
+    pairs = true;
     order = s->ulist;
     Mdev = s->bofs + nblk;
   } else if (all && e->order_ok && s->ttab_ok && e->k <= MF_ORD_MAX_K) {
@@ -1793,9 +1809,25 @@ hipError_t mfma_assign(Engine *e, long r0, long rows) {
   }
   if ((r = mfma_assign_view(e, e->dX + r0 * e->n, s->Xh + r0 * s->Dp, s->xnb + r0,
                             e->dlabels + r0, rows, order, home, Mdev, all ? s->xb : nullptr,
-                            false)) !=
-      hipSuccess)
+                            false)) != hipSuccess)
     return r;
+  if (pairs) {
+    // the two-candidate points (disjoint from the open ones the assign
+    // scored and the re-resolve revisits; the counters are atomics) are
+    // decided on the side stream while the re-resolve runs; mfma_reresolve
+    // (or mfma_pair_join) makes the engine's stream wait for them
+    const long nblk = km_cdiv(rows, 256);
+    if ((r = hipEventRecord(s->pair_fork, e->stream)) != hipSuccess) return r;
+    if ((r = hipStreamWaitEvent(s->pair_stream, s->pair_fork, 0)) != hipSuccess) return r;
+    const unsigned pg = (unsigned)std::min<long>(nblk, (long)num_cus() * 16);
+    hipLaunchKernelGGL(k_pair_decide, dim3(pg), dim3(256), 0, s->pair_stream, s->ulist,
+                       s->bofs + nblk, nblk, e->dlabels, s->xb, s->xnb, s->bnd, e->n,
+                       (const float *)e->dX, (const float *)e->dC, (const float *)e->dnc, nc2,
+                       e->assign_stop);
+    if ((r = hipGetLastError()) != hipSuccess) return r;
+    if ((r = hipEventRecord(s->pair_join, s->pair_stream)) != hipSuccess) return r;
+    s->pair_pending = true;
+  }
   if (!all) {
     s->bnd_valid = false;  // a partial assign leaves the bounds unusable
     return hipSuccess;
@@ -1926,8 +1958,17 @@ hipError_t mfma_prepare(Engine *e) {
 hipError_t mfma_reresolve(Engine *e, long r0, long rows) {
   MfmaState *s = st(e);
   const bool all = r0 == 0 && rows == e->m;
-  return mfma_reresolve_view(e, e->dX + r0 * e->n, s->Xh + r0 * s->Dp, s->xnb + r0,
-                             e->dlabels + r0, rows, all ? s->xb : nullptr);
+  const hipError_t r = mfma_reresolve_view(e, e->dX + r0 * e->n, s->Xh + r0 * s->Dp, s->xnb + r0,
+                                           e->dlabels + r0, rows, all ? s->xb : nullptr);
+  const hipError_t rj = mfma_pair_join(e);  // whatever the re-resolve's status
+  return r != hipSuccess ? r : rj;
+}
+
+hipError_t mfma_pair_join(Engine *e) {
+  MfmaState *s = st(e);
+  if (!s || !s->pair_pending) return hipSuccess;
+  s->pair_pending = false;
+  return hipStreamWaitEvent(e->stream, s->pair_join, 0);
 }
 
 // Points left open by the assign kernel: deep MFMA pass, then the FP32 full

@@ -8,7 +8,11 @@ void mfma_free(Engine *e);
 void mfma_invalidate(Engine *e);
 void mfma_bounds_invalidate(Engine *e);  // labels changed outside mfma_assign
 hipError_t mfma_assign(Engine *e, long r0, long rows);
+// also makes the engine's stream wait for the two-candidate decisions that
+// mfma_assign left running on its side stream
 hipError_t mfma_reresolve(Engine *e, long r0, long rows);
+// the engine's stream waits for those decisions (no-op when none are pending)
+hipError_t mfma_pair_join(Engine *e);
 hipError_t mfma_centers(Engine *e);
 void mfma_collect(Engine *e, int nslots);
 void mfma_centers_changed(Engine *e, bool new_range);
