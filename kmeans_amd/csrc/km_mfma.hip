@@ -100,11 +100,12 @@ struct MfmaState {
   // bits, -1 for none) the runner-up centre and l3 <= |x_i - c_j| for every
   // centre j other than the label and r, valid for the centres Cref;
   // drift[j] >= |c_j - Cref_j| and drift[k] their maximum.  ulist: the
-  // points the bounds leave open, bcnt/bofs/btsum their per-block counts and
-  // offsets.  bnd_valid: xb and Cref describe the last assign.
+  // points the bounds leave open (from the front) and the two-candidate
+  // points (from the back), fcnt their counts.  bnd_valid: xb and Cref
+  // describe the last assign.
   float4 *xb = nullptr;
   float *Cref = nullptr, *drift = nullptr;
-  int *ulist = nullptr, *bcnt = nullptr, *bofs = nullptr, *btsum = nullptr;
+  int *ulist = nullptr, *fcnt = nullptr;
   bool bnd_valid = false;
   // the two-candidate decisions run on a side stream (default priority)
   // after the assign kernel, beside the re-resolve: they touch only points
@@ -270,8 +271,9 @@ __global__ __launch_bounds__(256) void k_tile_order(int K, int D, int nKT,
 // drift_bound: 0 exactly when every coordinate difference is 0.
 __global__ __launch_bounds__(256) void k_drift(int K, int D, const float *__restrict__ C,
                                                const float *__restrict__ Cref,
-                                               float *__restrict__ drift,
+                                               float *__restrict__ drift, int *__restrict__ fcnt,
                                                const int *__restrict__ stop) {
+  if (blockIdx.x == 0 && threadIdx.x < 2) fcnt[threadIdx.x] = 0;  // the filter's counts
   if (stop && *stop) return;
   const int j = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (j >= K) return;
@@ -325,112 +327,118 @@ __device__ inline void chain2(const float *__restrict__ xr, const float *__restr
   }
 }
 
-// The test for the point at position q of `order` (or point q), its bounds
-// moved by the drift (u' = u + drift[a], l' = l - max drift, l3' = l3 - max
-// drift): 0 when they keep its label (l'^2 - u'^2 > 4 E32 as above), 1 when
+// The test for a point with label a, bounds b = (u, l, l3, r), drift da of
+// its label's centre and dr of r's, row norm bound xn and largest drift dm;
+// its bounds moved by the drift (u' = u + da, l' = l - dm, l3' = l3 - dm):
+// 0 when they keep its label (l'^2 - u'^2 > 4 E32 as above), 1 when
 // they separate every centre but the label a and the runner-up r in the same
 // way (l3'^2 - u'^2 > 4 E32: the FP32 chain of a then exceeds every other
 // centre's but possibly r's, so the label is a or r, whichever chain is
 // larger, lowest index on equal chains), 2 when the point must be scored.
 // When a and r have not moved at all (drift 0: every coordinate equal, so
 // both chains are the ones the last assign compared, which put a first) the
-// second case keeps the label too.  nb = the moved bounds, e32 = the FP32
-// chains' error bound for this point.  bnd = the centre bounds [cmax, ncmax,
-// ...] of the current centres.
-__device__ inline int bound_class(long q, const int *__restrict__ order,
-                                  const int *__restrict__ labels, const float4 *__restrict__ xb,
-                                  const float *__restrict__ drift, int K, const float *bnd,
-                                  const float2 *__restrict__ xnb, float Df, long &p, int &a,
-                                  float4 &nb, float &e32) {
-  p = order ? (long)order[q] : q;
-  a = labels[p];
-  const float4 b = xb[p];
-  const float dm = drift[K];
-  const float u = (b.x + drift[a]) * (1.0f + 1e-6f);
+// second case keeps the label too.  nb = the moved bounds; the FP32 chains'
+// error bound E32 comes from bnd = the centre bounds [cmax, ncmax, ...] of
+// the current centres.
+__device__ inline int bound_class(int a, float4 b, float da, float dr, float xn, float dm, int K,
+                                  const float *bnd, float Df, float4 &nb) {
+  const float u = (b.x + da) * (1.0f + 1e-6f);
   const float l = fmaxf((b.y - dm) * (1.0f - 1e-6f), 0.0f);
   const float l3 = fmaxf((b.z - dm) * (1.0f - 1e-6f), 0.0f);
   nb = make_float4(u, l, l3, b.w);
-  e32 = Df * 5.9604645e-8f * 1.0001f * (0.5f * bnd[1] + xnb[p].x * bnd[0]);
+  const float e32 = Df * 5.9604645e-8f * 1.0001f * (0.5f * bnd[1] + xn * bnd[0]);
   const float t = 4.0f * e32 * 1.0001f + 1e-30f;
   if (l > u && (l - u) * (l + u) * (1.0f - 1e-6f) > t) return 0;
   const int r = __float_as_int(b.w);
   if (r >= 0 && r < K && r != a && l3 > u && (l3 - u) * (l3 + u) * (1.0f - 1e-6f) > t)
-    return drift[a] == 0.0f && drift[r] == 0.0f ? 0 : 1;
+    return da == 0.0f && dr == 0.0f ? 0 : 1;
   return 2;
 }
 
-// Pass 1: per block b of 256 positions, the number of points left open
-// (bcnt[b]) and of those with two candidates (bcnt[nblk + b], nblk =
-// gridDim.x), so that one exclusive scan over bcnt places the open points
-// first and the two-candidate points after them.
-__global__ __launch_bounds__(256) void k_bound_count(long M, const int *__restrict__ order,
-                                                     const int *__restrict__ labels,
-                                                     const float4 *__restrict__ xb,
-                                                     const float *__restrict__ drift, int K,
-                                                     const float *__restrict__ bnd,
-                                                     const float2 *__restrict__ xnb, int D,
-                                                     int *__restrict__ bcnt,
-                                                     const int *__restrict__ stop) {
-  if (stop && *stop) return;
-  __shared__ int wc[2][4];
-  const long q = (long)blockIdx.x * 256 + threadIdx.x;
-  int cls = 0;
-  if (q < M) {
-    long p;
-    int a;
-    float4 nb;
-    float e32;
-    cls = bound_class(q, order, labels, xb, drift, K, bnd, xnb, (float)D, p, a, nb, e32);
-  }
-  const uint64_t mo = __ballot(q < M && cls == 2), mp = __ballot(q < M && cls == 1);
-  if ((threadIdx.x & 63) == 0) {
-    wc[0][threadIdx.x >> 6] = __popcll(mo);
-    wc[1][threadIdx.x >> 6] = __popcll(mp);
-  }
-  __syncthreads();
-  if (threadIdx.x < 2)
-    bcnt[(long)threadIdx.x * gridDim.x + blockIdx.x] =
-        wc[threadIdx.x][0] + wc[threadIdx.x][1] + wc[threadIdx.x][2] + wc[threadIdx.x][3];
-}
-
-// Pass 2 (same test, same inputs; bofs = the exclusive scan of bcnt): the
-// open points in order into ulist[bofs[b] ...], the two-candidate points in
-// order into ulist[bofs[nblk + b] ...], and the moved bounds of every point
-// not left open into xb.
-__global__ __launch_bounds__(256) void k_bound_compact(
+// One pass over the positions, BF_PPT per thread: the open points go to the
+// front of ulist and the two-candidate points to its back (ulist[m - 1 - i]
+// for the i-th), each workgroup reserving its runs with one atomic per list
+// on fcnt = [open count, two-candidate count] (zeroed by k_drift); the moved
+// bounds of every point not left open go to xb.  The lists' order among
+// workgroups is the order of the reservations; every point is handled on
+// its own, so no result depends on it.
+#define BF_PPT 8
+__global__ __launch_bounds__(256) void k_bound_filter(
     long M, const int *__restrict__ order, const int *__restrict__ labels, float4 *__restrict__ xb,
     const float *__restrict__ drift, int K, const float *__restrict__ bnd,
-    const float2 *__restrict__ xnb, int D, const int *__restrict__ bofs, int *__restrict__ ulist,
-    const int *__restrict__ stop) {
+    const float2 *__restrict__ xnb, int D, long m, int *__restrict__ ulist,
+    int *__restrict__ fcnt, const int *__restrict__ stop) {
   if (stop && *stop) return;
-  __shared__ int wc[2][4];
+  __shared__ int wc[2][BF_PPT][4];
+  __shared__ int base_s[2];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const long q = (long)blockIdx.x * 256 + threadIdx.x;
-  int cls = 0, a = 0;
-  long p = 0;
-  float4 nb = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  float e32 = 0.0f;
-  if (q < M) cls = bound_class(q, order, labels, xb, drift, K, bnd, xnb, (float)D, p, a, nb, e32);
-  const bool open = q < M && cls == 2, pair = q < M && cls == 1;
-  const uint64_t mo = __ballot(open), mp = __ballot(pair);
-  if (lane == 0) {
-    wc[0][w] = __popcll(mo);
-    wc[1][w] = __popcll(mp);
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const long q0 = (long)blockIdx.x * 256 * BF_PPT;
+  int cls[BF_PPT], pv[BF_PPT], la[BF_PPT];
+  float4 bv[BF_PPT];
+  float xv[BF_PPT], da[BF_PPT], dr[BF_PPT];
+  uint64_t mo[BF_PPT], mp[BF_PPT];
+  // every load of the thread's points first, then the drifts of their
+  // centres, so each thread waits for memory twice
+#pragma unroll
+  for (int u = 0; u < BF_PPT; ++u) {
+    const long q = q0 + u * 256 + threadIdx.x;
+    const long p = q < M ? (order ? (long)order[q] : q) : 0;
+    pv[u] = (int)p;
+    la[u] = q < M ? labels[p] : 0;
+    bv[u] = q < M ? xb[p] : make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
+    xv[u] = q < M ? xnb[p].x : 0.0f;
+  }
+  const float dm = drift[K];
+#pragma unroll
+  for (int u = 0; u < BF_PPT; ++u) {
+    const int r = __float_as_int(bv[u].w);
+    da[u] = drift[la[u]];
+    dr[u] = (r >= 0 && r < K) ? drift[r] : 0.0f;
+  }
+#pragma unroll
+  for (int u = 0; u < BF_PPT; ++u) {
+    const long q = q0 + u * 256 + threadIdx.x;
+    cls[u] = 0;
+    if (q < M) {
+      float4 nb;
+      cls[u] = bound_class(la[u], bv[u], da[u], dr[u], xv[u], dm, K, bnd, (float)D, nb);
+      if (cls[u] != 2) xb[pv[u]] = nb;
+    }
+    mo[u] = __ballot(q < M && cls[u] == 2);
+    mp[u] = __ballot(q < M && cls[u] == 1);
+    if (lane == 0) {
+      wc[0][u][w] = __popcll(mo[u]);
+      wc[1][u][w] = __popcll(mp[u]);
+    }
   }
   __syncthreads();
-  int bo = bofs[blockIdx.x], bp = bofs[gridDim.x + blockIdx.x];
-  for (int u = 0; u < w; ++u) {
-    bo += wc[0][u];
-    bp += wc[1][u];
+  if (threadIdx.x < 2) {  // the workgroup's runs in the two lists
+    int t = 0;
+    for (int u = 0; u < BF_PPT; ++u)
+      for (int v = 0; v < 4; ++v) t += wc[threadIdx.x][u][v];
+    base_s[threadIdx.x] = t ? atomicAdd(fcnt + threadIdx.x, t) : 0;
   }
-  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-  if (open) ulist[bo + __popcll(mo & below)] = (int)p;
-  else if (q < M) xb[p] = nb;
-  if (pair) ulist[bp + __popcll(mp & below)] = (int)p;
+  __syncthreads();
+  int bo = base_s[0], bp = base_s[1];
+#pragma unroll
+  for (int u = 0; u < BF_PPT; ++u) {
+    int so = bo, sp = bp;  // this wave's offsets in round u
+    for (int v = 0; v < w; ++v) {
+      so += wc[0][u][v];
+      sp += wc[1][u][v];
+    }
+    if (cls[u] == 2 && ((mo[u] >> lane) & 1)) ulist[so + __popcll(mo[u] & below)] = pv[u];
+    if (cls[u] == 1 && ((mp[u] >> lane) & 1)) ulist[m - 1 - (sp + __popcll(mp[u] & below))] = pv[u];
+    for (int v = 0; v < 4; ++v) {
+      bo += wc[0][u][v];
+      bp += wc[1][u][v];
+    }
+  }
 }
 
-// The two-candidate points (positions [rng[0], rng[nblk]) of ulist, after
-// the open ones; xb holds their moved bounds (u', l', l3', r)): the label
+// The two-candidate points (the last *pcnt of the m entries of ulist; xb
+// holds their moved bounds (u', l', l3', r)): the label
 // is the label a or the runner-up r, whichever FP32 chain is larger (lowest
 // index on equal chains; X, C, nc the rows, the current centres and their
 // chains of squares), since every other centre is separated by the bounds.
@@ -444,14 +452,14 @@ __global__ __launch_bounds__(256) void k_bound_compact(
 #define PR_PD 16                 // dimensions per staged piece
 #define PR_PS (PR_PD / 4 + 1)    // float4 stride of a staged row (conflict-free reads)
 __global__ __launch_bounds__(256) void k_pair_decide(
-    const int *__restrict__ ulist, const int *__restrict__ rng, long nblk,
+    const int *__restrict__ ulist, const int *__restrict__ pcnt, long m,
     int *__restrict__ labels, float4 *__restrict__ xb, const float2 *__restrict__ xnb,
     const float *__restrict__ bnd, int D, const float *__restrict__ X,
     const float *__restrict__ C, const float *__restrict__ nc, int *__restrict__ ncl,
     const int *__restrict__ stop) {
   if (stop && *stop) return;
   __shared__ float4 lds[4 * 3 * 64 * PR_PS];  // per wave: point rows, label rows, runner-up rows
-  const long i0 = rng[0], i1 = rng[nblk];
+  const long i0 = m - *pcnt, i1 = m;
   const float Df = (float)D;
   const bool vec = (D & 3) == 0 && (((uintptr_t)X | (uintptr_t)C) & 15) == 0;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1432,7 +1440,7 @@ void mfma_free(Engine *e) {
   hipFree(s->kbest);
   hipFree(s->ttab);
   hipFree(s->tdist);
-  void *bb[] = {s->xb, s->Cref, s->drift, s->ulist, s->bcnt, s->bofs, s->btsum};
+  void *bb[] = {s->xb, s->Cref, s->drift, s->ulist, s->fcnt};
   for (void *b : bb) hipFree(b);
   hipFree(s->home);
   for (hipEvent_t ev : s->kev) hipEventDestroy(ev);
@@ -1714,26 +1722,19 @@ static hipError_t ensure_bounds(Engine *e) {
   MfmaState *s = st(e);
   if (s->xb) return hipSuccess;
   hipError_t r;
-  const long nblk = km_cdiv(e->m, 256);
   if ((r = hipMalloc(&s->xb, sizeof(float4) * (size_t)e->m)) != hipSuccess) return r;
   if ((r = hipMalloc(&s->Cref, sizeof(float) * (size_t)e->k * e->n)) != hipSuccess) return r;
   if ((r = hipMalloc(&s->drift, sizeof(float) * (size_t)(e->k + 1))) != hipSuccess) return r;
   if ((r = hipMemsetAsync(s->drift, 0, sizeof(float) * (size_t)(e->k + 1), e->stream)) != hipSuccess)
     return r;
   if ((r = hipMalloc(&s->ulist, sizeof(int) * (size_t)e->m)) != hipSuccess) return r;
-  // per block: open and two-candidate counts, their exclusive scan
-  if ((r = hipMalloc(&s->bcnt, sizeof(int) * (size_t)(2 * nblk))) != hipSuccess) return r;
-  if ((r = hipMalloc(&s->bofs, sizeof(int) * (size_t)(2 * nblk + 1))) != hipSuccess) return r;
-  if ((r = hipMalloc(&s->btsum, sizeof(int) * (size_t)(scan_tiles(2 * nblk) + 1))) != hipSuccess)
-    return r;
+  if ((r = hipMalloc(&s->fcnt, sizeof(int) * 2)) != hipSuccess) return r;
   if ((r = hipStreamCreateWithFlags(&s->pair_stream, hipStreamNonBlocking)) != hipSuccess) return r;
   if ((r = hipEventCreateWithFlags(&s->pair_fork, hipEventDisableTiming)) != hipSuccess) return r;
   if ((r = hipEventCreateWithFlags(&s->pair_join, hipEventDisableTiming)) != hipSuccess) return r;
-  // an open list of zero points until the first compaction (a launch whose
-  // compaction a convergence stop skipped reads these)
-  if ((r = hipMemsetAsync(s->ulist, 0, sizeof(int) * (size_t)e->m, e->stream)) != hipSuccess)
-    return r;
-  return hipMemsetAsync(s->bofs, 0, sizeof(int) * (size_t)(2 * nblk + 1), e->stream);
+  // lists of zero points until the first filter (a launch whose filter a
+  // convergence stop skipped reads these)
+  return hipMemsetAsync(s->fcnt, 0, sizeof(int) * 2, e->stream);
 }
 
 // After an assign over all resident points every point's bounds hold for the
@@ -1774,23 +1775,18 @@ hipError_t mfma_assign(Engine *e, long r0, long rows) {
   bool pairs = false;  // two-candidate points to decide after the assign
   if (all && s->bnd_valid) {
     // the drift bounds of the last assign over all points: only the points
-    // they leave open are assigned, in point order (the filter reads every
-    // point's label and bounds once, coalesced, and decides the points with
-    // two candidates itself)
-    const long nblk = km_cdiv(rows, 256);
+    // they leave open are assigned, in the order of the filter's runs (the
+    // filter reads every point's label and bounds once, coalesced, and lists
+    // the points with two candidates for k_pair_decide)
     hipLaunchKernelGGL(k_drift, dim3(km_cdiv(e->k, 4)), dim3(256), 0, e->stream, e->k, e->n,
-                       e->dC, s->Cref, s->drift, e->assign_stop);
-    hipLaunchKernelGGL(k_bound_count, dim3(nblk), dim3(256), 0, e->stream, rows,
-                       (const int *)nullptr, e->dlabels, s->xb, s->drift, e->k, s->bnd, s->xnb,
-                       e->n, s->bcnt, e->assign_stop);
-    if ((r = launch_scan_excl(e->stream, s->bcnt, 2 * nblk, s->btsum, s->bofs)) != hipSuccess)
-      return r;
-    hipLaunchKernelGGL(k_bound_compact, dim3(nblk), dim3(256), 0, e->stream, rows,
-                       (const int *)nullptr, e->dlabels, s->xb, s->drift, e->k, s->bnd, s->xnb,
-                       e->n, s->bofs, s->ulist, e->assign_stop);
+                       e->dC, s->Cref, s->drift, s->fcnt, e->assign_stop);
+    hipLaunchKernelGGL(k_bound_filter, dim3(km_cdiv(rows, 256 * BF_PPT)), dim3(256), 0, e->stream,
+                       rows, (const int *)nullptr, e->dlabels, s->xb, s->drift, e->k, s->bnd,
+                       s->xnb, e->n, rows, s->ulist, s->fcnt, e->assign_stop);
+    if ((r = hipGetLastError()) != hipSuccess) return r;
     pairs = true;
     order = s->ulist;
-    Mdev = s->bofs + nblk;
+    Mdev = s->fcnt;
   } else if (all && e->order_ok && s->ttab_ok && e->k <= MF_ORD_MAX_K) {
     // all resident points, with the previous update's order and a tile
     // table: the label-ordered variant
@@ -1816,12 +1812,11 @@ hipError_t mfma_assign(Engine *e, long r0, long rows) {
     // scored and the re-resolve revisits; the counters are atomics) are
     // decided on the side stream while the re-resolve runs; mfma_reresolve
     // (or mfma_pair_join) makes the engine's stream wait for them
-    const long nblk = km_cdiv(rows, 256);
     if ((r = hipEventRecord(s->pair_fork, e->stream)) != hipSuccess) return r;
     if ((r = hipStreamWaitEvent(s->pair_stream, s->pair_fork, 0)) != hipSuccess) return r;
-    const unsigned pg = (unsigned)std::min<long>(nblk, (long)num_cus() * 16);
+    const unsigned pg = (unsigned)std::min<long>(km_cdiv(rows, 256), (long)num_cus() * 16);
     hipLaunchKernelGGL(k_pair_decide, dim3(pg), dim3(256), 0, s->pair_stream, s->ulist,
-                       s->bofs + nblk, nblk, e->dlabels, s->xb, s->xnb, s->bnd, e->n,
+                       s->fcnt + 1, rows, e->dlabels, s->xb, s->xnb, s->bnd, e->n,
                        (const float *)e->dX, (const float *)e->dC, (const float *)e->dnc, nc2,
                        e->assign_stop);
     if ((r = hipGetLastError()) != hipSuccess) return r;
