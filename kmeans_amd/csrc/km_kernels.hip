@@ -448,19 +448,24 @@ __global__ __launch_bounds__(256) void k_scan_down(const int *__restrict__ in, l
 // wave w walks points [a + w SBW, a + (w + 1) SBW) in order.  The waves
 // first count their labels into W x K LDS counters, which become each
 // wave's write positions: the sub-block's offsets (offs) plus the counts of
-// the waves before it.  Lanes with equal labels are then ranked by a ballot
-// match over the label bits.  The labels of 16 steps (1024 points) are read
+// the waves before it.  Lanes with equal labels are then ranked by a match:
+// with PEER, a per-wave LDS mask per label class (label mod 64, one 64-bit
+// LDS OR per step) narrowed by a ballot per remaining label bit; otherwise a
+// ballot per label bit.  The labels of 16 steps (1024 points) are read
 // into registers at once and the next 16 are in flight while these are
 // ranked.  A sub-block's W waves write one run per label, so neighbouring
 // waves fill the same cache lines.
 #define SC_STEPS 16
+template <bool PEER>
 __global__ __launch_bounds__(1024) void k_scatter(const int *__restrict__ labels, long i0, long nb,
                                                   int K, int nsb, int SB, int SBW, int nbits,
                                                   const int *__restrict__ offs,
                                                   int *__restrict__ perm) {
-  extern __shared__ int hw[];
+  extern __shared__ __attribute__((aligned(8))) int hw[];  // W x K counters, then W x 64 masks
   const int W = blockDim.x >> 6, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  unsigned long long *pm = reinterpret_cast<unsigned long long *>(hw + ((W * K + 1) & ~1)) + w * 64;
   for (int t = threadIdx.x; t < W * K; t += blockDim.x) hw[t] = 0;
+  if (PEER) pm[lane] = 0ull;
   __syncthreads();
   const long a = (long)blockIdx.x * SB + (long)w * SBW, e = min(a + SBW, nb);
   for (long b0 = a; b0 < e; b0 += 64 * SC_STEPS) {  // SC_STEPS labels per lane in flight
@@ -502,11 +507,26 @@ __global__ __launch_bounds__(1024) void k_scatter(const int *__restrict__ labels
       const long i = b0 + u * 64 + lane;
       const bool ok = i < e;
       const int l = cur[u];
-      uint64_t peers = __ballot(ok);
-      for (int b = 0; b < nbits; ++b) {
-        const bool bit = (l >> b) & 1;
-        const uint64_t m = __ballot(bit);
-        peers &= bit ? m : ~m;
+      uint64_t peers, cls = 0ull;
+      if (PEER) {  // lanes of the same label class, then the higher bits
+        if (ok) atomicOr(&pm[l & 63], 1ull << lane);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        cls = ok ? pm[l & 63] : 0ull;
+        peers = cls;
+        for (int b = 6; b < nbits; ++b) {
+          const bool bit = (l >> b) & 1;
+          const uint64_t m = __ballot(bit);
+          peers &= bit ? m : ~m;
+        }
+      } else {
+        peers = __ballot(ok);
+        for (int b = 0; b < nbits; ++b) {
+          const bool bit = (l >> b) & 1;
+          const uint64_t m = __ballot(bit);
+          peers &= bit ? m : ~m;
+        }
       }
       // the counters are the wave's own: its LDS reads and writes run in
       // program order, so ordering the compiler's schedule is enough, and
@@ -518,6 +538,7 @@ __global__ __launch_bounds__(1024) void k_scatter(const int *__restrict__ labels
       }
       __builtin_amdgcn_wave_barrier();
       if (ok && (peers >> lane) == 1ull) cnt[l] = c + __popcll(peers);
+      if (PEER && ok && (cls >> lane) == 1ull) pm[l & 63] = 0ull;  // the class's last lane
       __builtin_amdgcn_wave_barrier();
     }
 #pragma unroll
@@ -1204,8 +1225,16 @@ static hipError_t block_sort(hipStream_t s, long i0, long nb, int k, const int *
                      labels + i0, nb, k, nsb, SB, ws.hist);
   if ((e = scan_excl(s, ws.hist, L, ws.tsum, ws.offs)) != hipSuccess) return e;
   int *perm = ws.perm_out ? ws.perm_out + i0 : ws.perm;
-  hipLaunchKernelGGL(k_scatter, dim3(nsb), dim3(64 * W), sizeof(int) * W * k, s, labels + i0, i0,
-                     nb, k, nsb, SB, SB / W, nbits, ws.offs, perm);
+  // the peer match when the counters and the masks fit 64 KB of LDS with
+  // the same waves (fewer waves per sub-block cost more than the ballots
+  // the masks save)
+  const size_t lds_peer = sizeof(int) * ((size_t)W * k + 1) + sizeof(unsigned long long) * 64 * W;
+  if (lds_peer <= 65536)
+    hipLaunchKernelGGL(k_scatter<true>, dim3(nsb), dim3(64 * W), lds_peer, s, labels + i0, i0, nb,
+                       k, nsb, SB, SB / W, nbits, ws.offs, perm);
+  else
+    hipLaunchKernelGGL(k_scatter<false>, dim3(nsb), dim3(64 * W), sizeof(int) * W * k, s,
+                       labels + i0, i0, nb, k, nsb, SB, SB / W, nbits, ws.offs, perm);
   hipLaunchKernelGGL(k_cluster_setup, dim3(1), dim3(1024), 0, s, k, nsb, ws.offs, ws.cstart,
                      ws.ccount, ws.cq);
   return hipGetLastError();
