@@ -300,6 +300,7 @@ __device__ inline void inst_bounds(float2 bb, int d, float *cmax, float *ncmax, 
 // (olist + item * P, count ocnt[item]), which the assign then scores.
 // Nothing to do before the first assign of the current data and centres
 // (*bvalid == 0): the assign then scores every point.
+#define CL_PPT 4
 __global__ __launch_bounds__(256) void k_batch_classify(
     int nitems, const int2 *__restrict__ items, int P, const long *__restrict__ xoff,
     const int *__restrict__ nvec, int d, int k, const float *__restrict__ X,
@@ -347,13 +348,13 @@ __global__ __launch_bounds__(256) void k_batch_classify(
   const float t4 = 4.0f * e32i * 1.0001f + 1e-30f;
   uint16_t *ol = olist + (size_t)it * P;
   unsigned nk = 0, np = 0;
-  // four points per thread per pass: their labels and bounds are loaded
+  // CL_PPT points per thread per pass: their labels and bounds are loaded
   // together
-  for (long q0 = 0; q0 < rows; q0 += 1024) {
-    int la[4];
-    float4 bb[4];
+  for (long q0 = 0; q0 < rows; q0 += 256 * CL_PPT) {
+    int la[CL_PPT];
+    float4 bb[CL_PPT];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < CL_PPT; ++u) {
       const long q = q0 + 256 * u + tid;
       la[u] = 0;
       bb[u] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
@@ -362,15 +363,15 @@ __global__ __launch_bounds__(256) void k_batch_classify(
         bb[u] = xb[base + q];
       }
     }
-    float da[4], drr[4];
+    float da[CL_PPT], drr[CL_PPT];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < CL_PPT; ++u) {
       const int r = __float_as_int(bb[u].w);
       da[u] = dr[la[u]];
       drr[u] = (r >= 0 && r < k) ? dr[r] : 0.0f;
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < CL_PPT; ++u) {
       const long q = q0 + 256 * u + tid;
       int cls = 2;
       if (q < rows) {
