@@ -514,7 +514,7 @@ void k_batch_assign_bf16(
       }
       for (int j = tid; j < Kp; j += 256) {
         const float n2 = j < k ? nc[(size_t)inst * k + j] : 0.0f;
-        bias[j] = j < k ? -0.5f * n2 : -INFINITY;
+        bias[j] = j < k ? -0.5f * n2 : -FLT_MAX;
         if (j < k) ncs[j] = n2;
       }
       for (int e = tid; e < k * d; e += 256) {
@@ -541,9 +541,20 @@ void k_batch_assign_bf16(
     // this half's 8 dimensions per k-step of the point in row pn (zero past
     // d or past the rows); the next 32 points' values are loaded while the
     // current ones are scored
+    const bool vec = (d & 3) == 0;  // rows 16-byte aligned: float4 loads
     auto load_half = [&](long pn, float *v) {
       const bool okn = pn < nopen;
       const float *xn = X + (base + (okn ? row_of(pn) : 0)) * d;
+      if (vec) {
+#pragma unroll
+        for (int g = 0; g < 2 * KS; ++g) {
+          const int d0 = (g >> 1) * 16 + 8 * h + 4 * (g & 1);
+          const float4 t4 = (okn && d0 < d) ? *reinterpret_cast<const float4 *>(xn + d0)
+                                            : make_float4(0.f, 0.f, 0.f, 0.f);
+          v[4 * g] = t4.x; v[4 * g + 1] = t4.y; v[4 * g + 2] = t4.z; v[4 * g + 3] = t4.w;
+        }
+        return;
+      }
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
@@ -594,7 +605,9 @@ void k_batch_assign_bf16(
       const float ex = sqrtf(se * inf) * (1.0f + 1e-6f) + 1e-30f;
       // top 3 of the packed scores (register id in the low 4 bits, three
       // v_med3 per score, max as med3 with a run-time +FLT_MAX) and the tiles
-      // of the first two
+      // of the first two.  Centres past k have bias -FLT_MAX and zero BF16
+      // rows, so their scores are finite (-FLT_MAX) and pack without a clamp,
+      // below every real score
       float b1 = -FLT_MAX, b2 = -FLT_MAX, b3 = -FLT_MAX;
       int t1 = 0, t2 = 0;
       auto score_tile = [&](int t) {
@@ -616,7 +629,7 @@ void k_batch_assign_bf16(
         const float o1 = b1, o2 = b2;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-          const float pv = __uint_as_float((__float_as_uint(fmaxf(acc[r], -FLT_MAX)) & 0xFFFFFFF0u) | (unsigned)r);
+          const float pv = __uint_as_float((__float_as_uint(acc[r]) & 0xFFFFFFF0u) | (unsigned)r);
           const float n3 = __builtin_amdgcn_fmed3f(b2, b3, pv);
           const float n2 = __builtin_amdgcn_fmed3f(b1, b2, pv);
           b1 = __builtin_amdgcn_fmed3f(b1, pv, big);
@@ -647,7 +660,7 @@ void k_batch_assign_bf16(
       } else {  // t = nT - 1
         epilogue(t, A);
       }
-      const bool real1 = b1 > -FLT_MAX, real2 = b2 > -FLT_MAX;
+      const bool real1 = b1 > -FLT_MAX * 0.5f, real2 = b2 > -FLT_MAX * 0.5f;
       int i1 = real1 ? t1 * 32 + b_row((int)(__float_as_uint(b1) & 15u), h) : -1;
       int i2 = real2 ? t2 * 32 + b_row((int)(__float_as_uint(b2) & 15u), h) : -1;
       // bitwise-equal packed values make the tile tracking ambiguous: such
@@ -682,7 +695,6 @@ void k_batch_assign_bf16(
       // the point's FP32 row in registers (zero past d), loaded only now so
       // that it is not held across the tile loop
       float xv[Dp];
-      const bool vec = (d & 3) == 0;
 #pragma unroll
       for (int q4 = 0; q4 < Dp / 4; ++q4) {
         float4 t4 = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -704,9 +716,14 @@ void k_batch_assign_bf16(
       if (ok && cj >= 0 && cj < k) {
         float sc = -0.5f * ncs[cj];
         const float *cr = Cf + (size_t)cj * cs;
+        if (d == Dp) {
 #pragma unroll
-        for (int q = 0; q < Dp; ++q)
-          if (q < d) sc = __builtin_fmaf(xv[q], cr[q], sc);
+          for (int q = 0; q < Dp; ++q) sc = __builtin_fmaf(xv[q], cr[q], sc);
+        } else {
+#pragma unroll
+          for (int q = 0; q < Dp; ++q)
+            if (q < d) sc = __builtin_fmaf(xv[q], cr[q], sc);
+        }
         sx = sc;
       }
       const float sy = __shfl_xor(sx, 32);
@@ -730,7 +747,7 @@ void k_batch_assign_bf16(
         const float mfe = gmf * (0.5f * ncmax + (xn + ex) * (cmax + dcmax));
         const float fpe = g32 * (0.5f * ncmax + xn * cmax);
         const float E = (rnd + mfe + fpe) * 1.001f + 1e-30f;
-        const bool has3 = b3 > -FLT_MAX;
+        const bool has3 = b3 > -FLT_MAX * 0.5f;
         // packed low bits (< 16 ulp) and FP32 rounding of the comparison
         const float slack = (fabsf(b1) + fabsf(b2) + (has3 ? fabsf(b3) : 0.0f)) * 1.6e-5f;
         const float thr = has3 ? b3 + E + slack : -INFINITY;
