@@ -294,10 +294,13 @@ __device__ inline void inst_bounds(float2 bb, int d, float *cmax, float *ncmax, 
 // item ahead of the BF16 assign (a kernel of its own, at full occupancy, so
 // that its gathers overlap across many items): kept points get their moved
 // bounds; two-candidate points are decided by the FP32 chains of their label
-// and runner-up (the centres and norms read from global memory, the values
-// the assign stages in LDS), lowest index on equal chains, and kept without
-// them when neither centre moved; the open ones go to the item's list
-// (olist + item * P, count ocnt[item]), which the assign then scores.
+// and runner-up (centres and norms staged in LDS, the values the assign
+// uses), lowest index on equal chains, and kept without them when neither
+// centre moved; the open ones go to the item's list (olist + item * P, count
+// ocnt[item]), which the assign then scores.  The two-candidate points of a
+// pass are listed in LDS and decided after it by consecutive threads, so
+// their row loads are issued together rather than one divergent step at a
+// time.
 // Nothing to do before the first assign of the current data and centres
 // (*bvalid == 0): the assign then scores every point.
 #define CL_PPT 4
@@ -310,13 +313,15 @@ __global__ __launch_bounds__(256) void k_batch_classify(
     const float *__restrict__ xmax, unsigned long long *__restrict__ nkp, int *__restrict__ imin,
     uint16_t *__restrict__ olist, int *__restrict__ ocnt) {
   extern __shared__ float csm[];  // the instance's centres (k x cs), norms and drifts (k each)
-  __shared__ int cnt_s;
+  __shared__ int cnt_s, pc_s[2];
+  __shared__ int pl[256 * CL_PPT];  // the pass's two-candidate points
   __shared__ unsigned kp_s[2];
   const int it = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
   if (it >= nitems || !*bvalid) return;
   const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
   if (tid == 0) {
     cnt_s = 0;
+    pc_s[0] = pc_s[1] = 0;
     kp_s[0] = kp_s[1] = 0;
   }
   __syncthreads();
@@ -350,7 +355,9 @@ __global__ __launch_bounds__(256) void k_batch_classify(
   unsigned nk = 0, np = 0;
   // CL_PPT points per thread per pass: their labels and bounds are loaded
   // together
-  for (long q0 = 0; q0 < rows; q0 += 256 * CL_PPT) {
+  const bool vec = (d & 3) == 0;
+  int pp = 0;  // pass parity: the pair count of this pass is pc_s[pp]
+  for (long q0 = 0; q0 < rows; q0 += 256 * CL_PPT, pp ^= 1) {
     int la[CL_PPT];
     float4 bb[CL_PPT];
 #pragma unroll
@@ -390,37 +397,6 @@ __global__ __launch_bounds__(256) void k_batch_classify(
         if (cls == 0) {
           xb[gp] = make_float4(uu, l, l3, b4.w);
           ++nk;
-        } else if (cls == 1) {
-          // the plain path's chains of a and r, and the point's own norm
-          // bound from the same row (read 16 dimensions at a time)
-          const float *xr = X + gp * d;
-          const float *ca = Cs + (size_t)a * cs, *cb = Cs + (size_t)r * cs;
-          float sa = -0.5f * ncs[a], sr = -0.5f * ncs[r], sq = 0.0f;
-          for (int q0 = 0; q0 < d; q0 += 16) {
-            float xv[16];
-#pragma unroll
-            for (int v = 0; v < 16; ++v) xv[v] = q0 + v < d ? xr[q0 + v] : 0.0f;
-#pragma unroll
-            for (int v = 0; v < 16; ++v)
-              if (q0 + v < d) {
-                sa = __builtin_fmaf(xv[v], ca[q0 + v], sa);
-                sr = __builtin_fmaf(xv[v], cb[q0 + v], sr);
-                sq = __builtin_fmaf(xv[v], xv[v], sq);
-              }
-          }
-          const bool tr = sr > sa || (sr == sa && r < a);
-          const int L = tr ? r : a, O = tr ? a : r;
-          const float sL = tr ? sr : sa, sO = tr ? sa : sr;
-          if (L != a) {
-            atomicMin(imin + (size_t)inst * k + a, (int)(w.y + q));
-            atomicMin(imin + (size_t)inst * k + L, (int)(w.y + q));
-          }
-          labels[gp] = L;
-          const float xn = sqrtf(sq * (1.0f + Df * 1.2e-7f + 1e-6f)) * (1.0f + 1e-6f) + 1e-30f;
-          xb[gp] = make_float4(dist_upper(xn, sL - e32i),
-                               fminf(dist_lower(xn, sO + e32i, Df), l3), l3,
-                               __int_as_float(O));
-          ++np;
         }
       }
       const bool open = q < rows && cls == 2;
@@ -432,7 +408,76 @@ __global__ __launch_bounds__(256) void k_batch_classify(
         ob = __shfl(ob, lead);
         if (open) ol[ob + __popcll(mk & below)] = (uint16_t)q;
       }
+      const bool two = q < rows && cls == 1;
+      const uint64_t pk = __ballot(two);
+      if (pk) {
+        const int lead = __ffsll((long long)pk) - 1;
+        int ob = 0;
+        if (lane == lead) ob = atomicAdd(&pc_s[pp], __popcll(pk));
+        ob = __shfl(ob, lead);
+        if (two) pl[ob + __popcll(pk & below)] = (int)q;
+      }
     }
+    __syncthreads();
+    // the pass's two-candidate points: the plain path's chains of the label
+    // a and the runner-up r, and the point's own norm bound from the same
+    // row (label and bounds read again; nothing above wrote them)
+    const int npr = pc_s[pp];
+    if (tid == 0) pc_s[pp ^ 1] = 0;  // last read in the previous pass
+    for (int e = tid; e < npr; e += 256) {
+      const int q = pl[e];
+      const long gp = base + q;
+      const int a = labels[gp];
+      const float4 b4 = xb[gp];
+      const int r = __float_as_int(b4.w);
+      const float l3 = fmaxf((b4.z - dm) * (1.0f - 1e-6f), 0.0f);
+      const float *xr = X + gp * d;
+      const float *ca = Cs + (size_t)a * cs, *cb = Cs + (size_t)r * cs;
+      float sa = -0.5f * ncs[a], sr = -0.5f * ncs[r], sq = 0.0f;
+      for (int dq = 0; dq < d; dq += 16) {
+        float xv[16];
+        if (vec) {
+#pragma unroll
+          for (int g = 0; g < 4; ++g) {
+            const float4 t4v = dq + 4 * g < d ? *reinterpret_cast<const float4 *>(xr + dq + 4 * g)
+                                              : make_float4(0.f, 0.f, 0.f, 0.f);
+            xv[4 * g] = t4v.x; xv[4 * g + 1] = t4v.y; xv[4 * g + 2] = t4v.z; xv[4 * g + 3] = t4v.w;
+          }
+        } else {
+#pragma unroll
+          for (int v = 0; v < 16; ++v) xv[v] = dq + v < d ? xr[dq + v] : 0.0f;
+        }
+        if (dq + 16 <= d) {
+#pragma unroll
+          for (int v = 0; v < 16; ++v) {
+            sa = __builtin_fmaf(xv[v], ca[dq + v], sa);
+            sr = __builtin_fmaf(xv[v], cb[dq + v], sr);
+            sq = __builtin_fmaf(xv[v], xv[v], sq);
+          }
+        } else {
+#pragma unroll
+          for (int v = 0; v < 16; ++v)
+            if (dq + v < d) {
+              sa = __builtin_fmaf(xv[v], ca[dq + v], sa);
+              sr = __builtin_fmaf(xv[v], cb[dq + v], sr);
+              sq = __builtin_fmaf(xv[v], xv[v], sq);
+            }
+        }
+      }
+      const bool tr = sr > sa || (sr == sa && r < a);
+      const int L = tr ? r : a, O = tr ? a : r;
+      const float sL = tr ? sr : sa, sO = tr ? sa : sr;
+      if (L != a) {
+        atomicMin(imin + (size_t)inst * k + a, (int)(w.y + q));
+        atomicMin(imin + (size_t)inst * k + L, (int)(w.y + q));
+      }
+      labels[gp] = L;
+      const float xn = sqrtf(sq * (1.0f + Df * 1.2e-7f + 1e-6f)) * (1.0f + 1e-6f) + 1e-30f;
+      xb[gp] = make_float4(dist_upper(xn, sL - e32i), fminf(dist_lower(xn, sO + e32i, Df), l3),
+                           l3, __int_as_float(O));
+      ++np;
+    }
+    __syncthreads();
   }
   if (nk) atomicAdd(&kp_s[0], nk);
   if (np) atomicAdd(&kp_s[1], np);
@@ -499,27 +544,36 @@ void k_batch_assign_bf16(
     const int2 w = items[it];
     const int inst = w.x;
     if (inst != cur) {
+      // each centre value is read once (two 16-byte loads per 8-value chunk
+      // when d is a multiple of 8) for both its BF16 chunk and its FP32 row
       const float *Ci = C + (size_t)inst * k * d;
+      const bool v8 = (d & 7) == 0;
       for (int e = tid; e < Kp * (Dp / 8); e += 256) {
         const int j = e / (Dp / 8), q8 = (e - j * (Dp / 8)) * 8;
-        uint32_t wv[4];
+        float v[8];
+        if (v8) {
+          float4 lo = make_float4(0.f, 0.f, 0.f, 0.f), hi = lo;
+          if (j < k && q8 < d) {
+            lo = *reinterpret_cast<const float4 *>(Ci + (size_t)j * d + q8);
+            hi = *reinterpret_cast<const float4 *>(Ci + (size_t)j * d + q8 + 4);
+          }
+          v[0] = lo.x; v[1] = lo.y; v[2] = lo.z; v[3] = lo.w;
+          v[4] = hi.x; v[5] = hi.y; v[6] = hi.z; v[7] = hi.w;
+        } else {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int d0 = q8 + 2 * u;
-          const float v0 = (j < k && d0 < d) ? Ci[(size_t)j * d + d0] : 0.0f;
-          const float v1 = (j < k && d0 + 1 < d) ? Ci[(size_t)j * d + d0 + 1] : 0.0f;
-          wv[u] = (uint32_t)b_bf16(v0) | ((uint32_t)b_bf16(v1) << 16);
+          for (int u = 0; u < 8; ++u) v[u] = (j < k && q8 + u < d) ? Ci[(size_t)j * d + q8 + u] : 0.0f;
         }
-        Ch[(size_t)(q8 / 8) * Kp + j] = b_u32x4{wv[0], wv[1], wv[2], wv[3]};
+        Ch[(size_t)(q8 / 8) * Kp + j] = b_u32x4{bf16x2_rne(v[0], v[1]), bf16x2_rne(v[2], v[3]),
+                                                bf16x2_rne(v[4], v[5]), bf16x2_rne(v[6], v[7])};
+        if (j < k)
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (q8 + u < d) Cf[(size_t)j * cs + q8 + u] = v[u];
       }
       for (int j = tid; j < Kp; j += 256) {
         const float n2 = j < k ? nc[(size_t)inst * k + j] : 0.0f;
         bias[j] = j < k ? -0.5f * n2 : -FLT_MAX;
         if (j < k) ncs[j] = n2;
-      }
-      for (int e = tid; e < k * d; e += 256) {
-        const int j = e / d;
-        Cf[(size_t)j * cs + (e - j * d)] = Ci[e];
       }
       // the instance's bounds (k_batch_bounds / k_batch_finalize): the nc
       // chains are within gamma_d of the sums, and so are the chains of the
@@ -587,12 +641,12 @@ void k_batch_assign_bf16(
           const float v1 = cur_x[ks * 8 + 2 * u + 1];
           sq = __builtin_fmaf(v0, v0, sq);
           sq = __builtin_fmaf(v1, v1, sq);
-          const uint16_t h0 = b_bf16(v0), h1 = b_bf16(v1);
-          const float r0 = v0 - __uint_as_float((uint32_t)h0 << 16);
-          const float r1 = v1 - __uint_as_float((uint32_t)h1 << 16);
+          const uint32_t hw = bf16x2_rne(v0, v1);  // one conversion for both
+          const float r0 = v0 - __uint_as_float(hw << 16);
+          const float r1 = v1 - __uint_as_float(hw & 0xFFFF0000u);
           se = __builtin_fmaf(r0, r0, se);
           se = __builtin_fmaf(r1, r1, se);
-          wv[u] = (uint32_t)h0 | ((uint32_t)h1 << 16);
+          wv[u] = hw;
         }
         bf[ks] = b_u32x4{wv[0], wv[1], wv[2], wv[3]};
       }
@@ -874,37 +928,55 @@ __device__ int block_scan_any(int *v, int n, int *tmp) {
 
 // Update 1/3, persistent over instances (largest first), one 16-wave
 // workgroup per instance: stable counting sort of the instance's points by
-// label.  Wave w owns a contiguous segment of the points; per (wave, label)
-// counts give every wave its own cursors, so each wave scatters its segment
-// in order with no block barrier (ranks among equal labels from a match
-// over the label bits).  Writes perm (instance-local indices, label-major),
-// counts, offsets and the chunk prefix of every cluster.
+// label.  A first pass counts the labels (per wave and label, then per
+// label).  STAGED (k up to ~480): the points are then placed in tiles of
+// BS_TILE consecutive rows: each wave ranks its 256 rows of the tile among
+// equal labels (LDS lane masks per label class, label mod 64, plus ballots on
+// the remaining bits), the per-(wave, label) counts are scanned over waves and
+// labels, every row goes to its label-major slot of the tile in LDS, and the
+// block writes the tile out in slot order: a label's members of one tile are
+// consecutive in perm, so the writes come in runs of ~BS_TILE / k entries
+// instead of one scattered 4-byte write per point.  Otherwise wave w owns a
+// contiguous segment of the points; per (wave, label) counts give every wave
+// its own cursors, and each wave scatters its segment in order with no block
+// barrier (ranks among equal labels from ballots over the label bits).
+// Writes perm (instance-local indices, label-major), counts, offsets and the
+// chunk prefix of every cluster.
 // With reuse (rstate = [the other parity holds the previous update of these
 // points, its parity]): each cluster's first member position at which this
 // sort differs from the previous one goes to fd, for k_batch_chunks_wave.
-// Labels are read eight per lane ahead of their use, so each wave waits for
-// memory once per 512 points.
+// Labels are read ahead of their use (eight per lane in the count pass, the
+// next tile's four in the staged pass).  The staged pass issues exactly
+// BS_TILE / 1024 stores per thread and tile (rows past the end write a dummy
+// area past the perm arrays, dummy[0..1024)), so the loads of the next tile,
+// issued before them, are waited for without waiting for the stores.
 // Parity of an update: the one the previous update did not write (rstate[1]
 // once rstate[0] says it wrote one); the arrays come in pairs.
 __device__ inline int upd_parity(const int *rstate) { return rstate[0] ? rstate[1] ^ 1 : 0; }
 
-template <bool PEER>
+constexpr int BS_STEPS = 4;                   // 64-row steps per wave and tile
+constexpr int BS_TILE = 16 * 64 * BS_STEPS;   // rows per tile (12 bits)
+
+template <bool STAGED>
 __global__ __launch_bounds__(1024) void k_batch_sort(
     int nb, const int *__restrict__ order, const long *__restrict__ xoff,
     const int *__restrict__ nvec, int k, const int *__restrict__ labels, int *perm0, int *perm1,
-    int *gcnt0, int *gcnt1, int *goff0, int *goff1, int *gcpre0, int *gcpre1,
+    long tot, int *gcnt0, int *gcnt1, int *goff0, int *goff1, int *gcpre0, int *gcpre1,
     const long *__restrict__ choff, int4 *__restrict__ desc, int *__restrict__ counter,
     const int *__restrict__ rstate, int *__restrict__ fd, int *__restrict__ imin) {
-  // cur[16][k], offs[k], cpre[k], cnt[k], fdc[k], ims[k]; PEER: per wave
-  // and label the mask of the wave's lanes holding that label in the current
-  // step (all zero between steps)
+  // cur[16][k], offs[k], cpre[k], cnt[k], fdc[k], ims[k]; STAGED: then the
+  // tile's slots stage[BS_TILE] (local row | label << 12) and per wave the
+  // masks of the lanes holding each label class in the current step (all
+  // zero between steps)
   extern __shared__ __attribute__((aligned(8))) int sh[];
   int *cur = sh, *offs = sh + 16 * k, *cpre = offs + k, *cnts = cpre + k, *fdc = cnts + k,
       *ims = fdc + k;
-  unsigned long long *pm = reinterpret_cast<unsigned long long *>(sh + ((21 * k + 1) & ~1));
+  int *stage = sh + ((21 * k + 1) & ~1);
+  unsigned long long *pm = reinterpret_cast<unsigned long long *>(stage + BS_TILE);
   const bool reuse = rstate[0] != 0;
   const int par = upd_parity(rstate);
   int *__restrict__ perm = par ? perm1 : perm0;
+  int *__restrict__ dummy = perm + tot;  // 1024 ints past the points
   int *__restrict__ gcnt = par ? gcnt1 : gcnt0;
   int *__restrict__ goff = par ? goff1 : goff0;
   int *__restrict__ gcpre = par ? gcpre1 : gcpre0;
@@ -928,8 +1000,8 @@ __global__ __launch_bounds__(1024) void k_batch_sort(
       fdc[j] = 0;
       ims[j] = reuse ? imin[(size_t)inst * k + j] : 0;
     }
-    if (PEER)
-      for (int e = t; e < 16 * k; e += blockDim.x) pm[e] = 0ull;
+    if (STAGED)
+      for (int e = t; e < 16 * 64; e += blockDim.x) pm[e] = 0ull;
     __syncthreads();
     const int S = ((n + 15) / 16 + 63) / 64 * 64;
     const int a = min(n, w * S), e = min(n, a + S);
@@ -966,9 +1038,9 @@ __global__ __launch_bounds__(1024) void k_batch_sort(
     const int nch = block_scan_any(cpre, k, tmp);
     for (int j = t; j < k; j += blockDim.x) {
       int c = offs[j];
-      for (int ww = 0; ww < 16; ++ww) {
+      for (int ww = 0; ww < 16; ++ww) {  // STAGED: counts of the first tile
         const int h = cur[ww * k + j];
-        cur[ww * k + j] = c;
+        cur[ww * k + j] = STAGED ? 0 : c;
         c += h;
       }
       goff[(size_t)inst * k + j] = offs[j];
@@ -991,8 +1063,119 @@ __global__ __launch_bounds__(1024) void k_batch_sort(
       }
       for (int c = nch + t; c < cap; c += blockDim.x) dd[c] = make_int4(0, 0, 0, 0);
     }
+    if (STAGED) {
+      // the label cursors (instance-local perm positions) of the current and
+      // the next tile alternate between offs and cpre; cnts holds the tile's
+      // label offsets (the descriptors above were the last use of both)
+      int *toff = cnts;
+      int *cw = cur + w * k;
+      unsigned long long *pw = pm + w * 64;
+      const int wr = w * (64 * BS_STEPS);  // the wave's first row in a tile
+      int Lc[BS_STEPS], Ln[BS_STEPS];
+      auto fetch = [&](int t0, int (&v)[BS_STEPS]) {  // clamped into the instance
+#pragma unroll
+        for (int u = 0; u < BS_STEPS; ++u) v[u] = lab[min(t0 + wr + u * 64 + lane, n - 1)];
+      };
+      // labels are clamped into [0, k) where they arrive in registers: this
+      // also waits for the first tile's loads here, outside the tile loop
+      if (n > 0) {
+        fetch(0, Lc);
+#pragma unroll
+        for (int u = 0; u < BS_STEPS; ++u) Lc[u] = min(max(Lc[u], 0), k - 1);
+      }
+      int tp = 0;
+      for (int t0 = 0; t0 < n; t0 += BS_TILE, tp ^= 1) {
+        fetch(t0 + BS_TILE, Ln);  // unconditional: row n - 1 past the end
+        // ranks of the wave's rows among its rows of equal label (row order)
+        int lr[BS_STEPS];
+#pragma unroll
+        for (int u = 0; u < BS_STEPS; ++u) {
+          const int p = t0 + wr + u * 64 + lane;
+          const bool ok = p < n;
+          const int L = Lc[u];
+          if (ok) atomicOr(&pw[L & 63], 1ull << lane);
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          const uint64_t cls = ok ? pw[L & 63] : 0ull;
+          uint64_t m = cls;
+          for (int bt = 6; bt < nbits; ++bt) {
+            const bool bit = (L >> bt) & 1;
+            const uint64_t bb = __ballot(bit);
+            m &= bit ? bb : ~bb;
+          }
+          const int c = ok ? cw[L] : 0;
+          lr[u] = c + __popcll(m & below);
+          __builtin_amdgcn_wave_barrier();
+          if (ok && (m >> lane) == 1ull) cw[L] = c + __popcll(m);  // the label's last lane
+          if (ok && (cls >> lane) == 1ull) pw[L & 63] = 0ull;       // the class's last lane
+          __builtin_amdgcn_wave_barrier();
+        }
+        __syncthreads();
+        // per label: exclusive prefix of the counts over waves (in place),
+        // the next tile's cursor, and the exclusive prefix of the tile's
+        // label totals over labels (k <= 1024: one label per thread)
+        int s = 0;
+        if (t < k) {
+          for (int ww = 0; ww < 16; ++ww) {
+            const int c = cur[ww * k + t];
+            cur[ww * k + t] = s;
+            s += c;
+          }
+          (tp ? offs : cpre)[t] = (tp ? cpre : offs)[t] + s;
+        }
+        int x = s;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const int y = __shfl_up(x, o);
+          if (lane >= o) x += y;
+        }
+        if (lane == 63) tmp[w] = x;
+        __syncthreads();
+        if (t < 64) {
+          const int wt = t < 16 ? tmp[t] : 0;
+          int z = wt;
+#pragma unroll
+          for (int o = 1; o < 16; o <<= 1) {
+            const int y = __shfl_up(z, o);
+            if (t >= o) z += y;
+          }
+          if (t < 16) tmp[16 + t] = z - wt;
+        }
+        __syncthreads();
+        if (t < k) toff[t] = tmp[16 + w] + x - s;
+        __syncthreads();
+        // every row to its label-major slot of the tile
+#pragma unroll
+        for (int u = 0; u < BS_STEPS; ++u) {
+          const int L = Lc[u];
+          const int r = wr + u * 64 + lane;
+          if (t0 + r < n) stage[toff[L] + cw[L] + lr[u]] = r | (L << 12);
+        }
+        __syncthreads();
+        // the wave's counters back to zero for the next tile (only this
+        // wave writes them before the next tile's first barrier)
+        for (int j = lane; j < k; j += 64) cw[j] = 0;
+        // the tile in slot order: runs of one label go to consecutive
+        // positions from the label's cursor
+        const int rows = min(BS_TILE, n - t0);
+        const int *gcc = tp ? cpre : offs;
+#pragma unroll
+        for (int q = 0; q < BS_TILE / 1024; ++q) {
+          const int i = t + q * 1024;
+          const bool ok = i < rows;
+          const int v = stage[ok ? i : 0];
+          const int L = v >> 12;
+          int *dst = ok ? pb + gcc[L] + (i - toff[L]) : dummy + t;
+          *dst = t0 + (v & (BS_TILE - 1));
+        }
+#pragma unroll
+        for (int u = 0; u < BS_STEPS; ++u) Lc[u] = min(max(Ln[u], 0), k - 1);
+      }
+      __syncthreads();
+      continue;
+    }
     int *cw = cur + w * k;
-    unsigned long long *pw = pm + w * k;
     for (int p0 = a; p0 < e; p0 += 512) {
       int Lv[8];
 #pragma unroll
@@ -1003,35 +1186,19 @@ __global__ __launch_bounds__(1024) void k_batch_sort(
         const int p = p0 + u * 64 + lane;
         const bool valid = p < e;
         const int L = Lv[u];
-        // the lanes of this step holding label L: an LDS mask per label
-        // (PEER), else a match over the label bits by ballots
-        uint64_t m;
-        int cwL;  // the label's cursor, read with the mask
-        if (PEER) {
-          if (valid) atomicOr(&pw[L], 1ull << lane);
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-          cwL = cw[L];
-          m = pw[L];
-          if (!valid) m = 0ull;
-        } else {
-          m = __ballot(valid);
-          for (int bt = 0; bt < nbits; ++bt) {
-            const uint64_t bb = __ballot(valid && ((L >> bt) & 1));
-            m &= ((L >> bt) & 1) ? bb : ~bb;
-          }
-          if (!valid) m = 0;
-          cwL = cw[L];
+        // the lanes of this step holding label L: a match over the label bits
+        uint64_t m = __ballot(valid);
+        for (int bt = 0; bt < nbits; ++bt) {
+          const uint64_t bb = __ballot(valid && ((L >> bt) & 1));
+          m &= ((L >> bt) & 1) ? bb : ~bb;
         }
+        if (!valid) m = 0;
+        const int cwL = cw[L];
         const int rank = __popcll(m & below);
         const int pos = cwL + rank;
         __builtin_amdgcn_wave_barrier();
         if (valid) pb[pos] = p;
-        if (valid && rank == 0) {
-          cw[L] = pos + __popcll(m);
-          if (PEER) pw[L] = 0ull;
-        }
+        if (valid && rank == 0) cw[L] = pos + __popcll(m);
         __builtin_amdgcn_wave_barrier();
       }
     }
@@ -1409,22 +1576,23 @@ static hipError_t launch_assign(Batch *b, int P, int nitems, const int2 *items) 
 static hipError_t launch_update(Batch *b, int slot) {
   double *co = b->comp + (size_t)slot * b->nb;
   int *eo = b->empty + (size_t)slot * b->nb;
-  // the label masks of the peer match take 16 x k 8-byte words; used while
-  // everything stays within 64 KB
+  // the staged pass adds the tile's slots and 16 x 64 class masks; used
+  // while everything stays within 64 KB (k up to ~480)
   const size_t lds0 = sizeof(int) * (size_t)((21 * b->k + 1) & ~1);
-  const bool peer = lds0 + sizeof(unsigned long long) * 16 * (size_t)b->k <= 64 * 1024;
-  const size_t lds = peer ? lds0 + sizeof(unsigned long long) * 16 * (size_t)b->k : lds0;
+  const size_t lds1 = lds0 + sizeof(int) * BS_TILE + sizeof(unsigned long long) * 16 * 64;
+  const bool staged = lds1 <= 64 * 1024 && b->k <= 1024;
+  const size_t lds = staged ? lds1 : lds0;
   static size_t attr = 64 * 1024;
-  if (!peer && lds > attr) {
+  if (!staged && lds > attr) {
     hipError_t e = hipFuncSetAttribute((const void *)k_batch_sort<false>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     attr = lds;
   }
-  hipLaunchKernelGGL(peer ? k_batch_sort<true> : k_batch_sort<false>,
+  hipLaunchKernelGGL(staged ? k_batch_sort<true> : k_batch_sort<false>,
                      dim3(std::min(b->nb, num_cus() * 2)), dim3(1024), lds, b->stream, b->nb,
                      b->order, b->xoff, b->nvec, b->k, b->labels, b->perm[0], b->perm[1],
-                     b->gcnt[0], b->gcnt[1], b->goff[0], b->goff[1], b->gcpre[0], b->gcpre[1],
+                     b->total, b->gcnt[0], b->gcnt[1], b->goff[0], b->goff[1], b->gcpre[0], b->gcpre[1],
                      b->choff, b->desc, b->counters + 1, b->rstate, b->fd, b->imin);
 #define CHW(DT)                                                                                  \
   hipLaunchKernelGGL(k_batch_chunks_wave<DT>, dim3((unsigned)((b->slots + 3) / 4)), dim3(256), 0,  \
@@ -1653,7 +1821,9 @@ static kmeansErrorStatus batch_create(Batch *b, int nb, const long *n, int d, in
   BCHK(hipMalloc(&b->labels, sizeof(int) * (size_t)tot), KMEANS_ERROR_BUILDDATA);
   BCHK(hipMemsetAsync(b->labels, 0, sizeof(int) * (size_t)tot, b->stream), KMEANS_ERROR_BUILDDATA);
   for (int q = 0; q < 2; ++q) {
-    BCHK(hipMalloc(&b->perm[q], sizeof(int) * (size_t)tot), KMEANS_ERROR_BUILDDATA);
+    // 1024 ints past the points: the staged sort's stores of rows past an
+    // instance's end
+    BCHK(hipMalloc(&b->perm[q], sizeof(int) * ((size_t)tot + 1024)), KMEANS_ERROR_BUILDDATA);
     BCHK(hipMalloc(&b->part[q], sizeof(double) * (size_t)tch * (d + 1)), KMEANS_ERROR_BUILDDATA);
     BCHK(hipMalloc(&b->gcnt[q], sizeof(int) * (size_t)nb * k), KMEANS_ERROR_BUILDDATA);
     BCHK(hipMalloc(&b->goff[q], sizeof(int) * (size_t)nb * k), KMEANS_ERROR_BUILDDATA);
