@@ -929,13 +929,14 @@ __device__ int block_scan_any(int *v, int n, int *tmp) {
 // Update 1/3, persistent over instances (largest first), one 16-wave
 // workgroup per instance: stable counting sort of the instance's points by
 // label.  A first pass counts the labels (per wave and label, then per
-// label).  STAGED (k up to ~480): the points are then placed in tiles of
-// BS_TILE consecutive rows: each wave ranks its 256 rows of the tile among
-// equal labels (LDS lane masks per label class, label mod 64, plus ballots on
-// the remaining bits), the per-(wave, label) counts are scanned over waves and
+// label).  Staged (STEPS > 0, k up to ~480): the points are then placed in
+// tiles of 1024 * STEPS consecutive rows: each wave ranks its 64 * STEPS rows
+// of the tile among equal labels (LDS lane masks per label class, label mod
+// 64, plus ballots on the remaining bits), the per-(wave, label) counts are
+// scanned over waves and
 // labels, every row goes to its label-major slot of the tile in LDS, and the
 // block writes the tile out in slot order: a label's members of one tile are
-// consecutive in perm, so the writes come in runs of ~BS_TILE / k entries
+// consecutive in perm, so the writes come in runs of ~1024 * STEPS / k entries
 // instead of one scattered 4-byte write per point.  Otherwise wave w owns a
 // contiguous segment of the points; per (wave, label) counts give every wave
 // its own cursors, and each wave scatters its segment in order with no block
@@ -946,18 +947,15 @@ __device__ int block_scan_any(int *v, int n, int *tmp) {
 // points, its parity]): each cluster's first member position at which this
 // sort differs from the previous one goes to fd, for k_batch_chunks_wave.
 // Labels are read ahead of their use (eight per lane in the count pass, the
-// next tile's four in the staged pass).  The staged pass issues exactly
-// BS_TILE / 1024 stores per thread and tile (rows past the end write a dummy
+// next tile's STEPS in the staged pass).  The staged pass issues exactly
+// STEPS stores per thread and tile (rows past the end write a dummy
 // area past the perm arrays, dummy[0..1024)), so the loads of the next tile,
 // issued before them, are waited for without waiting for the stores.
 // Parity of an update: the one the previous update did not write (rstate[1]
 // once rstate[0] says it wrote one); the arrays come in pairs.
 __device__ inline int upd_parity(const int *rstate) { return rstate[0] ? rstate[1] ^ 1 : 0; }
 
-constexpr int BS_STEPS = 4;                   // 64-row steps per wave and tile
-constexpr int BS_TILE = 16 * 64 * BS_STEPS;   // rows per tile (12 bits)
-
-template <bool STAGED>
+template <int STEPS>  // staged pass: 64-row steps per wave and tile (4 or 8); 0: unstaged
 __global__ __launch_bounds__(1024) void k_batch_sort(
     int nb, const int *__restrict__ order, const long *__restrict__ xoff,
     const int *__restrict__ nvec, int k, const int *__restrict__ labels, int *perm0, int *perm1,
@@ -965,14 +963,18 @@ __global__ __launch_bounds__(1024) void k_batch_sort(
     const long *__restrict__ choff, int4 *__restrict__ desc, int *__restrict__ counter,
     const int *__restrict__ rstate, int *__restrict__ fd, int *__restrict__ imin) {
   // cur[16][k], offs[k], cpre[k], cnt[k], fdc[k], ims[k]; STAGED: then the
-  // tile's slots stage[BS_TILE] (local row | label << 12) and per wave the
+  // tile's slots stage[TILE] (local row | label << SH) and per wave the
   // masks of the lanes holding each label class in the current step (all
   // zero between steps)
+  constexpr bool STAGED = STEPS > 0;
+  constexpr int NS = STAGED ? STEPS : 1;  // register arrays of the staged pass
+  constexpr int TILE = 16 * 64 * (STAGED ? STEPS : 1);  // rows per tile
+  constexpr int SH = STEPS == 8 ? 13 : 12;              // bits of a row within a tile
   extern __shared__ __attribute__((aligned(8))) int sh[];
   int *cur = sh, *offs = sh + 16 * k, *cpre = offs + k, *cnts = cpre + k, *fdc = cnts + k,
       *ims = fdc + k;
   int *stage = sh + ((21 * k + 1) & ~1);
-  unsigned long long *pm = reinterpret_cast<unsigned long long *>(stage + BS_TILE);
+  unsigned long long *pm = reinterpret_cast<unsigned long long *>(stage + TILE);
   const bool reuse = rstate[0] != 0;
   const int par = upd_parity(rstate);
   int *__restrict__ perm = par ? perm1 : perm0;
@@ -1070,26 +1072,26 @@ __global__ __launch_bounds__(1024) void k_batch_sort(
       int *toff = cnts;
       int *cw = cur + w * k;
       unsigned long long *pw = pm + w * 64;
-      const int wr = w * (64 * BS_STEPS);  // the wave's first row in a tile
-      int Lc[BS_STEPS], Ln[BS_STEPS];
-      auto fetch = [&](int t0, int (&v)[BS_STEPS]) {  // clamped into the instance
+      const int wr = w * (64 * STEPS);  // the wave's first row in a tile
+      int Lc[NS], Ln[NS];
+      auto fetch = [&](int t0, int (&v)[NS]) {  // clamped into the instance
 #pragma unroll
-        for (int u = 0; u < BS_STEPS; ++u) v[u] = lab[min(t0 + wr + u * 64 + lane, n - 1)];
+        for (int u = 0; u < STEPS; ++u) v[u] = lab[min(t0 + wr + u * 64 + lane, n - 1)];
       };
       // labels are clamped into [0, k) where they arrive in registers: this
       // also waits for the first tile's loads here, outside the tile loop
       if (n > 0) {
         fetch(0, Lc);
 #pragma unroll
-        for (int u = 0; u < BS_STEPS; ++u) Lc[u] = min(max(Lc[u], 0), k - 1);
+        for (int u = 0; u < STEPS; ++u) Lc[u] = min(max(Lc[u], 0), k - 1);
       }
       int tp = 0;
-      for (int t0 = 0; t0 < n; t0 += BS_TILE, tp ^= 1) {
-        fetch(t0 + BS_TILE, Ln);  // unconditional: row n - 1 past the end
+      for (int t0 = 0; t0 < n; t0 += TILE, tp ^= 1) {
+        fetch(t0 + TILE, Ln);  // unconditional: row n - 1 past the end
         // ranks of the wave's rows among its rows of equal label (row order)
-        int lr[BS_STEPS];
+        int lr[NS];
 #pragma unroll
-        for (int u = 0; u < BS_STEPS; ++u) {
+        for (int u = 0; u < STEPS; ++u) {
           const int p = t0 + wr + u * 64 + lane;
           const bool ok = p < n;
           const int L = Lc[u];
@@ -1147,10 +1149,10 @@ __global__ __launch_bounds__(1024) void k_batch_sort(
         __syncthreads();
         // every row to its label-major slot of the tile
 #pragma unroll
-        for (int u = 0; u < BS_STEPS; ++u) {
+        for (int u = 0; u < STEPS; ++u) {
           const int L = Lc[u];
           const int r = wr + u * 64 + lane;
-          if (t0 + r < n) stage[toff[L] + cw[L] + lr[u]] = r | (L << 12);
+          if (t0 + r < n) stage[toff[L] + cw[L] + lr[u]] = r | (L << SH);
         }
         __syncthreads();
         // the wave's counters back to zero for the next tile (only this
@@ -1158,19 +1160,19 @@ __global__ __launch_bounds__(1024) void k_batch_sort(
         for (int j = lane; j < k; j += 64) cw[j] = 0;
         // the tile in slot order: runs of one label go to consecutive
         // positions from the label's cursor
-        const int rows = min(BS_TILE, n - t0);
+        const int rows = min(TILE, n - t0);
         const int *gcc = tp ? cpre : offs;
 #pragma unroll
-        for (int q = 0; q < BS_TILE / 1024; ++q) {
+        for (int q = 0; q < TILE / 1024; ++q) {
           const int i = t + q * 1024;
           const bool ok = i < rows;
           const int v = stage[ok ? i : 0];
-          const int L = v >> 12;
+          const int L = v >> SH;
           int *dst = ok ? pb + gcc[L] + (i - toff[L]) : dummy + t;
-          *dst = t0 + (v & (BS_TILE - 1));
+          *dst = t0 + (v & (TILE - 1));
         }
 #pragma unroll
-        for (int u = 0; u < BS_STEPS; ++u) Lc[u] = min(max(Ln[u], 0), k - 1);
+        for (int u = 0; u < STEPS; ++u) Lc[u] = min(max(Ln[u], 0), k - 1);
       }
       __syncthreads();
       continue;
@@ -1576,20 +1578,22 @@ static hipError_t launch_assign(Batch *b, int P, int nitems, const int2 *items) 
 static hipError_t launch_update(Batch *b, int slot) {
   double *co = b->comp + (size_t)slot * b->nb;
   int *eo = b->empty + (size_t)slot * b->nb;
-  // the staged pass adds the tile's slots and 16 x 64 class masks; used
-  // while everything stays within 64 KB (k up to ~480)
+  // the staged pass adds the tile's slots and 16 x 64 class masks; 8192-row
+  // tiles while everything stays within 64 KB (k up to ~290), else 4096-row
+  // tiles (k up to ~480), else the unstaged pass
   const size_t lds0 = sizeof(int) * (size_t)((21 * b->k + 1) & ~1);
-  const size_t lds1 = lds0 + sizeof(int) * BS_TILE + sizeof(unsigned long long) * 16 * 64;
-  const bool staged = lds1 <= 64 * 1024 && b->k <= 1024;
-  const size_t lds = staged ? lds1 : lds0;
+  const size_t masks = sizeof(unsigned long long) * 16 * 64;
+  const size_t lds8 = lds0 + sizeof(int) * 8192 + masks, lds4 = lds0 + sizeof(int) * 4096 + masks;
+  const int steps = lds8 <= 64 * 1024 ? 8 : lds4 <= 64 * 1024 ? 4 : 0;
+  const size_t lds = steps == 8 ? lds8 : steps == 4 ? lds4 : lds0;
   static size_t attr = 64 * 1024;
-  if (!staged && lds > attr) {
-    hipError_t e = hipFuncSetAttribute((const void *)k_batch_sort<false>,
+  if (steps == 0 && lds > attr) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_batch_sort<0>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     attr = lds;
   }
-  hipLaunchKernelGGL(staged ? k_batch_sort<true> : k_batch_sort<false>,
+  hipLaunchKernelGGL(steps == 8 ? k_batch_sort<8> : steps == 4 ? k_batch_sort<4> : k_batch_sort<0>,
                      dim3(std::min(b->nb, num_cus() * 2)), dim3(1024), lds, b->stream, b->nb,
                      b->order, b->xoff, b->nvec, b->k, b->labels, b->perm[0], b->perm[1],
                      b->total, b->gcnt[0], b->gcnt[1], b->goff[0], b->goff[1], b->gcpre[0], b->gcpre[1],
