@@ -90,6 +90,7 @@ kmeansErrorStatus compute_d(const double *data, int m, int n, int k, double crit
 // Internal launchers (implemented in km_kernels.hip); all stream-ordered.
 struct UpdateWs {
   int *hist, *offs, *tsum, *cstart, *ccount, *cq, *perm;
+  int *dummy;  // SCS_THREADS ints that the staged scatter's stores past a sub-block's end write
   double *P, *S, *compj;  // S: k x (d+1), column d = the squared-norm moment
   long long *cnt;
   size_t bytes;

@@ -456,6 +456,12 @@ __global__ __launch_bounds__(256) void k_scan_down(const int *__restrict__ in, l
 // ranked.  A sub-block's W waves write one run per label, so neighbouring
 // waves fill the same cache lines.
 #define SC_STEPS 16
+// the staged scatter: 8 waves, 16 steps of 64 rows per wave and tile
+#define SCS_THREADS 512
+#define SCS_STEPS 16
+#define SCS_TILE (SCS_THREADS * SCS_STEPS)  // 8192 rows
+#define SCS_SH 13                          // bits of a row within a tile
+#define SCS_MAXK 1024
 template <bool PEER>
 __global__ __launch_bounds__(1024) void k_scatter(const int *__restrict__ labels, long i0, long nb,
                                                   int K, int nsb, int SB, int SBW, int nbits,
@@ -543,6 +549,156 @@ __global__ __launch_bounds__(1024) void k_scatter(const int *__restrict__ labels
     }
 #pragma unroll
     for (int u = 0; u < SC_STEPS; ++u) cur[u] = nxt[u];
+  }
+}
+
+// The same scatter with its writes staged in LDS (k <= SCS_MAXK): one
+// SCS_THREADS-thread workgroup per sub-block, its rows in tiles of SCS_TILE
+// consecutive rows.  Each wave ranks its SCS_STEPS x 64 rows of a tile among
+// equal labels (per-wave LDS lane masks per label class, label mod 64, plus
+// ballots on the higher bits; 16-bit per-(wave, label) counts), the counts
+// are scanned over waves and labels, every row goes to its label-major slot
+// of the tile in LDS, and the block writes the tile in slot order: a label's
+// members of one tile are consecutive in perm, so the writes come in runs of
+// ~SCS_TILE / k entries instead of one scattered 4-byte write per point.
+// Each thread issues exactly SCS_TILE / SCS_THREADS stores per tile (slots
+// past the sub-block's end write dummy[thread]), so the next tile's labels,
+// loaded before them, are waited for without waiting for the stores.
+__global__ __launch_bounds__(SCS_THREADS) void k_scatter_staged(
+    const int *__restrict__ labels, long i0, long nb, int K, int nsb, int SB, int nbits,
+    const int *__restrict__ offs, int *__restrict__ perm, int *__restrict__ dummy) {
+  constexpr int NW = SCS_THREADS / 64;
+  extern __shared__ __attribute__((aligned(16))) int ss[];
+  int *stage = ss;                        // SCS_TILE: local row | label << SCS_SH
+  // int arrays padded to even lengths: the 64-bit masks (LDS atomics) need
+  // 8-byte alignment
+  int *gc = stage + SCS_TILE;             // K: each label's next perm position
+  int *toff = gc + ((K + 1) & ~1);        // K + 1: the tile's label offsets
+  unsigned long long *pm =
+      reinterpret_cast<unsigned long long *>(toff + ((K + 2) & ~1));  // NW x 64 class masks
+  uint16_t *cnt = reinterpret_cast<uint16_t *>(pm + NW * 64);        // NW x K counts
+  __shared__ int tmp[2 * NW];
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const long a0 = (long)blockIdx.x * SB;
+  const int rows_sb = (int)min((long)SB, nb - a0);  // > 0
+  const int *lab = labels + a0;
+  for (int j = t; j < K; j += SCS_THREADS) gc[j] = offs[(long)j * nsb + blockIdx.x];
+  for (int e = t; e < NW * K; e += SCS_THREADS) cnt[e] = 0;
+  for (int e = t; e < NW * 64; e += SCS_THREADS) pm[e] = 0ull;
+  __syncthreads();
+  uint16_t *cw = cnt + w * K;
+  unsigned long long *pw = pm + w * 64;
+  const int wr = w * (64 * SCS_STEPS);  // the wave's first row in a tile
+  int Lc[SCS_STEPS], Ln[SCS_STEPS];
+  auto fetch = [&](int t0, int (&v)[SCS_STEPS]) {  // clamped into the sub-block
+#pragma unroll
+    for (int u = 0; u < SCS_STEPS; ++u) v[u] = lab[min(t0 + wr + u * 64 + lane, rows_sb - 1)];
+  };
+  // labels clamped into [0, K) where they arrive (this also waits for the
+  // first tile's loads here, outside the tile loop)
+  fetch(0, Lc);
+#pragma unroll
+  for (int u = 0; u < SCS_STEPS; ++u) Lc[u] = min(max(Lc[u], 0), K - 1);
+  for (int t0 = 0; t0 < rows_sb; t0 += SCS_TILE) {
+    fetch(t0 + SCS_TILE, Ln);  // unconditional: the last row past the end
+    // ranks of the wave's rows among its rows of equal label (row order)
+    int lr[SCS_STEPS];
+#pragma unroll
+    for (int u = 0; u < SCS_STEPS; ++u) {
+      const bool ok = t0 + wr + u * 64 + lane < rows_sb;
+      const int L = Lc[u];
+      if (ok) atomicOr(&pw[L & 63], 1ull << lane);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const uint64_t cls = ok ? pw[L & 63] : 0ull;
+      uint64_t m = cls;
+      for (int bt = 6; bt < nbits; ++bt) {
+        const bool bit = (L >> bt) & 1;
+        const uint64_t bb = __ballot(bit);
+        m &= bit ? bb : ~bb;
+      }
+      const int c = ok ? (int)cw[L] : 0;
+      lr[u] = c + __popcll(m & below);
+      __builtin_amdgcn_wave_barrier();
+      if (ok && (m >> lane) == 1ull) cw[L] = (uint16_t)(c + __popcll(m));  // the label's last lane
+      if (ok && (cls >> lane) == 1ull) pw[L & 63] = 0ull;                  // the class's last lane
+      __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    // per label: exclusive prefix of the counts over waves (in place) and
+    // the tile's label offsets (exclusive prefix over labels; each thread
+    // owns labels 2t and 2t + 1)
+    int s0 = 0, s1 = 0;
+    if (2 * t < K) {
+      for (int ww = 0; ww < NW; ++ww) {
+        const int c = cnt[ww * K + 2 * t];
+        cnt[ww * K + 2 * t] = (uint16_t)s0;
+        s0 += c;
+      }
+    }
+    if (2 * t + 1 < K) {
+      for (int ww = 0; ww < NW; ++ww) {
+        const int c = cnt[ww * K + 2 * t + 1];
+        cnt[ww * K + 2 * t + 1] = (uint16_t)s1;
+        s1 += c;
+      }
+    }
+    const int s = s0 + s1;
+    int x = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) tmp[w] = x;
+    __syncthreads();
+    if (t < 64) {
+      const int wt = t < NW ? tmp[t] : 0;
+      int z = wt;
+#pragma unroll
+      for (int o = 1; o < NW; o <<= 1) {
+        const int y = __shfl_up(z, o);
+        if (t >= o) z += y;
+      }
+      if (t < NW) tmp[NW + t] = z - wt;
+    }
+    __syncthreads();
+    {
+      const int base = tmp[NW + w] + x - s;
+      if (2 * t < K) toff[2 * t] = base;
+      if (2 * t + 1 < K) toff[2 * t + 1] = base + s0;
+      if (t == SCS_THREADS - 1) toff[K] = base + s;  // the tile's rows
+    }
+    __syncthreads();
+    // every row to its label-major slot of the tile
+#pragma unroll
+    for (int u = 0; u < SCS_STEPS; ++u) {
+      const int r = wr + u * 64 + lane;
+      const int L = Lc[u];
+      if (t0 + r < rows_sb) stage[toff[L] + cw[L] + lr[u]] = r | (L << SCS_SH);
+    }
+    __syncthreads();
+    // the wave's counts back to zero for the next tile (only this wave
+    // writes them before the next tile's first barrier)
+    for (int j = lane; j < K; j += 64) cw[j] = 0;
+    // the tile in slot order: runs of one label go to consecutive positions
+    const int rows = min(SCS_TILE, rows_sb - t0);
+    const int rb = (int)(i0 + a0) + t0;  // the block's rows are below 2^31
+#pragma unroll
+    for (int q = 0; q < SCS_TILE / SCS_THREADS; ++q) {
+      const int i = t + q * SCS_THREADS;
+      const bool ok = i < rows;
+      const int v = stage[ok ? i : 0];
+      const int L = v >> SCS_SH;
+      int *dst = ok ? perm + gc[L] + (i - toff[L]) : dummy + t;
+      *dst = rb + (v & (SCS_TILE - 1));
+    }
+    __syncthreads();
+    for (int j = t; j < K; j += SCS_THREADS) gc[j] += toff[j + 1] - toff[j];
+#pragma unroll
+    for (int u = 0; u < SCS_STEPS; ++u) Lc[u] = min(max(Ln[u], 0), K - 1);
   }
 }
 
@@ -1168,6 +1324,7 @@ size_t update_ws_layout(long n, int d, int k, char *base, UpdateWs *ws) {
   w.ccount = (int *)take(sizeof(int) * k);
   w.cq = (int *)take(sizeof(int) * (k + 1));
   w.perm = (int *)take(sizeof(int) * (nb > 0 ? nb : 1));
+  w.dummy = (int *)take(sizeof(int) * SCS_THREADS);
   w.P = (double *)take(sizeof(double) * qmax * (d + 1));
   w.S = (double *)take(sizeof(double) * (long)k * (d + 1));
   w.cnt = (long long *)take(sizeof(long long) * k);
@@ -1225,11 +1382,18 @@ static hipError_t block_sort(hipStream_t s, long i0, long nb, int k, const int *
                      labels + i0, nb, k, nsb, SB, ws.hist);
   if ((e = scan_excl(s, ws.hist, L, ws.tsum, ws.offs)) != hipSuccess) return e;
   int *perm = ws.perm_out ? ws.perm_out + i0 : ws.perm;
-  // the peer match when the counters and the masks fit 64 KB of LDS with
-  // the same waves (fewer waves per sub-block cost more than the ballots
-  // the masks save)
+  // the staged scatter while its LDS fits (k <= SCS_MAXK: 61 KB at 1024);
+  // else the peer match when the counters and the masks fit 64 KB of LDS
+  // with the same waves (fewer waves per sub-block cost more than the
+  // ballots the masks save)
   const size_t lds_peer = sizeof(int) * ((size_t)W * k + 1) + sizeof(unsigned long long) * 64 * W;
-  if (lds_peer <= 65536)
+  if (k <= SCS_MAXK) {
+    const size_t lds = sizeof(int) * (SCS_TILE + (size_t)((k + 1) & ~1) + ((k + 2) & ~1)) +
+                       sizeof(unsigned long long) * (SCS_THREADS / 64) * 64 +
+                       sizeof(uint16_t) * (SCS_THREADS / 64) * (size_t)k;
+    hipLaunchKernelGGL(k_scatter_staged, dim3(nsb), dim3(SCS_THREADS), lds, s, labels + i0, i0, nb,
+                       k, nsb, SB, nbits, ws.offs, perm, ws.dummy);
+  } else if (lds_peer <= 65536)
     hipLaunchKernelGGL(k_scatter<true>, dim3(nsb), dim3(64 * W), lds_peer, s, labels + i0, i0, nb,
                        k, nsb, SB, SB / W, nbits, ws.offs, perm);
   else
