@@ -462,6 +462,7 @@ __global__ __launch_bounds__(256) void k_scan_down(const int *__restrict__ in, l
 #define SCS_TILE (SCS_THREADS * SCS_STEPS)  // 8192 rows
 #define SCS_SH 13                          // bits of a row within a tile
 #define SCS_MAXK 1024
+#define SCS_MIN_SUBBLOCKS 32  // below this the 16-wave scatter is faster
 template <bool PEER>
 __global__ __launch_bounds__(1024) void k_scatter(const int *__restrict__ labels, long i0, long nb,
                                                   int K, int nsb, int SB, int SBW, int nbits,
@@ -1382,12 +1383,14 @@ static hipError_t block_sort(hipStream_t s, long i0, long nb, int k, const int *
                      labels + i0, nb, k, nsb, SB, ws.hist);
   if ((e = scan_excl(s, ws.hist, L, ws.tsum, ws.offs)) != hipSuccess) return e;
   int *perm = ws.perm_out ? ws.perm_out + i0 : ws.perm;
-  // the staged scatter while its LDS fits (k <= SCS_MAXK: 61 KB at 1024);
-  // else the peer match when the counters and the masks fit 64 KB of LDS
-  // with the same waves (fewer waves per sub-block cost more than the
+  // the staged scatter while its LDS fits (k <= SCS_MAXK: 61 KB at 1024)
+  // and the block has enough sub-blocks for its 8-wave workgroups to fill
+  // the chip (measured: 98 sub-blocks 54 -> 29 us, 7 sub-blocks 13.6 ->
+  // 16.9 us); else the peer match when the counters and the masks fit 64 KB
+  // of LDS with the same waves (fewer waves per sub-block cost more than the
   // ballots the masks save)
   const size_t lds_peer = sizeof(int) * ((size_t)W * k + 1) + sizeof(unsigned long long) * 64 * W;
-  if (k <= SCS_MAXK) {
+  if (k <= SCS_MAXK && nsb >= SCS_MIN_SUBBLOCKS) {
     const size_t lds = sizeof(int) * (SCS_TILE + (size_t)((k + 1) & ~1) + ((k + 2) & ~1)) +
                        sizeof(unsigned long long) * (SCS_THREADS / 64) * 64 +
                        sizeof(uint16_t) * (SCS_THREADS / 64) * (size_t)k;

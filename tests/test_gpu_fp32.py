@@ -232,6 +232,24 @@ def test_update_spans_two_blocks(orc):
     assert comp == orc.compactness(X, lab, rC)
 
 
+@pytest.mark.parametrize("k", [101, 1024])
+def test_update_staged_scatter_odd_and_full_k(orc, k):
+    # a block of 37+ sub-blocks takes the LDS-staged scatter (k <= 1024):
+    # odd k (the int arrays before its 64-bit masks are padded) and the
+    # largest k; labels skewed so that some labels fill whole tiles
+    n, d = 600_000, 5
+    rng = np.random.default_rng(29)
+    X = rng.standard_normal((n, d)).astype(np.float32)
+    lab = np.where(rng.random(n) < 0.3, 0, rng.integers(0, k, n)).astype(np.int32)
+    lab[: 20_000] = k - 1
+    C0 = rng.standard_normal((k, d)).astype(np.float32)
+    C1, cnt, emp, nc, comp = gpu_update(X, lab, C0)
+    rC, rcnt, _, _ = orc.update(X, lab, C0)
+    assert np.array_equal(C1, rC)
+    assert np.array_equal(cnt, rcnt)
+    assert comp == orc.compactness(X, lab, rC)
+
+
 def test_engine_lloyd_config1_bit_exact(orc):
     X, C0 = orc.config1()
     ref = orc.lloyd(X, C0, 20)
