@@ -379,27 +379,26 @@ __global__ __launch_bounds__(256) void k_batch_classify(
     }
 #pragma unroll
     for (int u = 0; u < CL_PPT; ++u) {
+      // the class computed for every lane (rows past the item carry r = -1
+      // and zero bounds) and masked by the row's validity, without branches
       const long q = q0 + 256 * u + tid;
-      int cls = 2;
-      if (q < rows) {
-        const long gp = base + q;
-        const int a = la[u];
-        const float4 b4 = bb[u];
-        const int r = __float_as_int(b4.w);
-        const float uu = (b4.x + da[u]) * (1.0f + 1e-6f);
-        const float l = fmaxf((b4.y - dm) * (1.0f - 1e-6f), 0.0f);
-        const float l3 = fmaxf((b4.z - dm) * (1.0f - 1e-6f), 0.0f);
-        if (l > uu && (l - uu) * (l + uu) * (1.0f - 1e-6f) > t4)
-          cls = 0;
-        else if (r >= 0 && r < k && r != a && l3 > uu &&
-                 (l3 - uu) * (l3 + uu) * (1.0f - 1e-6f) > t4)
-          cls = (da[u] == 0.0f && drr[u] == 0.0f) ? 0 : 1;
-        if (cls == 0) {
-          xb[gp] = make_float4(uu, l, l3, b4.w);
-          ++nk;
-        }
+      const bool valid = q < rows;
+      const int a = la[u];
+      const float4 b4 = bb[u];
+      const int r = __float_as_int(b4.w);
+      const float uu = (b4.x + da[u]) * (1.0f + 1e-6f);
+      const float l = fmaxf((b4.y - dm) * (1.0f - 1e-6f), 0.0f);
+      const float l3 = fmaxf((b4.z - dm) * (1.0f - 1e-6f), 0.0f);
+      const bool keep1 = l > uu && (l - uu) * (l + uu) * (1.0f - 1e-6f) > t4;
+      const bool two = r >= 0 && r < k && r != a && l3 > uu &&
+                       (l3 - uu) * (l3 + uu) * (1.0f - 1e-6f) > t4;
+      const bool still = da[u] == 0.0f && drr[u] == 0.0f;
+      const int cls = !valid ? 3 : keep1 ? 0 : two ? (still ? 0 : 1) : 2;
+      if (cls == 0) {
+        xb[base + q] = make_float4(uu, l, l3, b4.w);
+        ++nk;
       }
-      const bool open = q < rows && cls == 2;
+      const bool open = cls == 2;
       const uint64_t mk = __ballot(open);
       if (mk) {
         const int lead = __ffsll((long long)mk) - 1;
@@ -408,14 +407,14 @@ __global__ __launch_bounds__(256) void k_batch_classify(
         ob = __shfl(ob, lead);
         if (open) ol[ob + __popcll(mk & below)] = (uint16_t)q;
       }
-      const bool two = q < rows && cls == 1;
-      const uint64_t pk = __ballot(two);
+      const bool pair = cls == 1;
+      const uint64_t pk = __ballot(pair);
       if (pk) {
         const int lead = __ffsll((long long)pk) - 1;
         int ob = 0;
         if (lane == lead) ob = atomicAdd(&pc_s[pp], __popcll(pk));
         ob = __shfl(ob, lead);
-        if (two) pl[ob + __popcll(pk & below)] = (int)q;
+        if (pair) pl[ob + __popcll(pk & below)] = (int)q;
       }
     }
     __syncthreads();
